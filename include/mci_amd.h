@@ -1,0 +1,194 @@
+/*
+ * mci_amd.h — C-ABI of the MI355X-native Metropolis integrator.
+ *
+ * This is the drop-in boundary for the Metropolis sampling path behind
+ * mci::MCI::integrate (reference: include/mci/MCIntegrator.hpp:26-226,
+ * src/MCIntegrator.cpp:43-649).  Every entry point below names the reference
+ * interface it replaces.  The C++ facade in include/mci_amd/MCIntegrator.hpp
+ * re-exposes the same calls with the reference's class, method names and
+ * exception types; Python binds the same symbols through ctypes.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; the caller owns every output array.
+ *  - Every function returns an mci_status; on failure mci_last_error() holds a
+ *    thread-local message.  Status codes map 1:1 onto the reference's exception
+ *    types (std::invalid_argument, std::domain_error, std::runtime_error).
+ *  - A context is one "MCI object" extended to W walkers.  Each walker is one
+ *    Metropolis chain (one lane on the GPU).  With W == 1 a context behaves
+ *    like one reference MCI; with W > 1 the walkers behave either like W
+ *    independent MCI objects (MCI_COUPLING_INDEPENDENT, the USE_MPI=0 build of
+ *    the reference run W times) or like W MPI ranks under MPIMCI
+ *    (MCI_COUPLING_SHARED: shared step calibration and decorrelation,
+ *    src/MCIntegrator.cpp:107,131-138,21-34; results combined as in
+ *    src/MPIMCI.cpp:85-92).
+ *  - Each context owns one HIP stream; calls on one context are serialised.
+ */
+#ifndef MCI_AMD_H
+#define MCI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference exception types) ---- */
+typedef int mci_status;
+#define MCI_OK                    0
+#define MCI_E_INVALID_ARGUMENT    1  /* std::invalid_argument */
+#define MCI_E_DOMAIN              2  /* std::domain_error     */
+#define MCI_E_RUNTIME             3  /* std::runtime_error    */
+#define MCI_E_DEVICE              4  /* HIP runtime failure (facade throws std::runtime_error) */
+
+/* ---- enums that mirror include/mci/Factories.hpp ---- */
+/* EstimatorType, Factories.hpp:52-59 (same order) */
+#define MCI_ESTIM_NOOP            0
+#define MCI_ESTIM_UNCORRELATED    1
+#define MCI_ESTIM_CORRELATED      2  /* MJBlocker if n is a power of 2, else FCBlocker */
+#define MCI_ESTIM_FCBLOCKER       3
+#define MCI_ESTIM_MJBLOCKER       4
+
+/* MoveType, Factories.hpp:108-113 */
+#define MCI_MOVE_ALL              0
+#define MCI_MOVE_VEC              1
+
+/* SRRDType, Factories.hpp:119-133 (same order).  Uniform is implemented. */
+#define MCI_SRRD_UNIFORM          0
+#define MCI_SRRD_GAUSSIAN         1
+
+/* ---- device functor catalogue (sampling functions) ----
+ * The reference's SamplingFunctionInterface subclasses become device functors
+ * selected by kind + parameter blob (SamplingFunctionInterface.hpp:38-105).
+ * The arithmetic of each kind is fixed below so that a reference subclass
+ * written with the same expressions gives bit-identical acceptance ratios.  */
+#define MCI_PDF_GAUSS_ISO   1 /* nproto 1: p = x0*x0 + x1*x1 + ... ; ratio exp(-pnew + pold)
+                                 (test/common/TestMCIFunctions.hpp:123-148 ThreeDimGaussianPDF form). params: none */
+#define MCI_PDF_GAUSS_DIAG  2 /* nproto ndim: p_i = (x_i*x_i)*a_i ; full ratio exp(sum pold - sum pnew),
+                                 single-index ratio exp(-(sum_changed pnew_i - pold_i))
+                                 (TestMCIFunctions.hpp:151-190 Gauss form with a_i = 1).
+                                 params: a[ndim] (optional; default all 1) */
+#define MCI_PDF_EXP_DIAG    3 /* nproto ndim: p_i = fabs(x_i)*a_i (TestMCIFunctions.hpp:219-256 ExpNDPDF form).
+                                 params: a[ndim] (optional) */
+#define MCI_PDF_GMM_ISO     4 /* isotropic Gaussian mixture, log-density via log-sum-exp.
+                                 params: [K, w_0..w_{K-1}, s_0..s_{K-1}, mu_0[ndim] .. mu_{K-1}[ndim]] */
+
+/* ---- device functor catalogue (observables), ObservableFunctionInterface.hpp:29-64 ---- */
+#define MCI_OBS_X2          1 /* nobs ndim: out_i = x_i*x_i (X2 / XYZSquared) */
+#define MCI_OBS_X2_SUM      2 /* nobs 1: out = (0 + x0*x0 + x1*x1 + ...)/c  (X2Sum, XSquared with c=3). params: [c] (default 1) */
+#define MCI_OBS_X           3 /* nobs ndim: out_i = x_i (XND / X1D) */
+#define MCI_OBS_X_SUM       4 /* nobs 1: out = 0 + x0 + x1 + ... (Polynom) */
+#define MCI_OBS_CONST       5 /* nobs 1: out = c (Constval). params: [c] */
+#define MCI_OBS_EXP_X2      6 /* nobs 1: out = exp(-((0 + x0*x0 + ...)/c)). params: [c] (default ndim) */
+#define MCI_OBS_INDICATOR   7 /* nobs 1: out = x_j > 0 ? 1 : 0. params: [j] (default 0) */
+
+/* ---- random-number modes ---- */
+#define MCI_RNG_MT19937_64  0 /* reference streams: one std::mt19937_64 per walker, libstdc++ distributions
+                                 (MCIntegrator.hpp:32-34, TrialMoveInterface.hpp:120-124, SRRDVecMove.hpp:27) */
+#define MCI_RNG_PHILOX      1 /* native: Philox4x32-10, key = seed, counter = (word block, walker, step) */
+
+/* ---- arithmetic of the chain ---- */
+#define MCI_PREC_FP64       0
+#define MCI_PREC_FP32       1 /* FP32 state/proposal/exp, Kahan FP32 block sums, FP64 estimators */
+
+/* ---- walker coupling ---- */
+#define MCI_COUPLING_INDEPENDENT 0 /* W independent MCI objects (USE_MPI=0 semantics per walker) */
+#define MCI_COUPLING_SHARED      1 /* W MPI ranks (USE_MPI=1 semantics, shared calibration/decorrelation) */
+
+typedef struct mci_ctx mci_ctx;
+
+/* thread-local message of the last failing call (never NULL) */
+const char *mci_last_error(void);
+/* library version string and the HIP device it runs on */
+const char *mci_version(void);
+mci_status mci_device_count(int *count);
+
+/* ---- lifetime: MCI::MCI(int ndim) src/MCIntegrator.cpp:624-649 ---- */
+mci_status mci_create(int ndim, int64_t nwalkers, mci_ctx **out);
+mci_status mci_destroy(mci_ctx *ctx);
+
+/* ---- settings ---- */
+/* MCI::setSeed src/MCIntegrator.cpp:546-549.  Walker w gets seed_w = seed for w == 0 and a
+ * splitmix64-derived seed otherwise (MT mode); Philox uses seed as the key. */
+mci_status mci_set_seed(mci_ctx *ctx, uint64_t seed);
+/* per-walker seeds for MT mode (the role of MPIMCI::setSeed src/MPIMCI.cpp:38-65) */
+mci_status mci_set_walker_seeds(mci_ctx *ctx, const uint64_t *seeds, int64_t nseeds);
+mci_status mci_set_rng(mci_ctx *ctx, int rng_kind);
+mci_status mci_set_precision(mci_ctx *ctx, int precision);
+mci_status mci_set_coupling(mci_ctx *ctx, int coupling);
+/* MCI::setTargetAcceptanceRate src/MCIntegrator.cpp:551-554 */
+mci_status mci_set_target_acceptance_rate(mci_ctx *ctx, double rate);
+/* MCI::setNfindMRT2Iterations / setNdecorrelationSteps MCIntegrator.hpp:115-123 (sign convention kept) */
+mci_status mci_set_nfind_mrt2_iterations(mci_ctx *ctx, int niterations);
+mci_status mci_set_ndecorrelation_steps(mci_ctx *ctx, int64_t nsteps);
+mci_status mci_get_settings(mci_ctx *ctx, double *target_rate, int *nfind, int64_t *ndecorr);
+
+/* ---- walker positions: MCI::setX/getX/centerX src/MCIntegrator.cpp:594-604, MCIntegrator.hpp:108,202-203 ----
+ * x has layout [nwalkers][ndim]; n == ndim broadcasts one position to all walkers. */
+mci_status mci_set_x(mci_ctx *ctx, const double *x, int64_t n);
+mci_status mci_get_x(mci_ctx *ctx, double *x, int64_t n);
+mci_status mci_center_x(mci_ctx *ctx);
+/* MCI::moveX src/MCIntegrator.cpp:606-612 and MCI::newRandomX :614-619 (draw from each walker's stream) */
+mci_status mci_move_x(mci_ctx *ctx);
+mci_status mci_new_random_x(mci_ctx *ctx);
+
+/* ---- step sizes: MCI::setMRT2Step (3 overloads) / getMRT2Step src/MCIntegrator.cpp:556-585 ----
+ * idx < 0 sets all step sizes; per_walker values have layout [nwalkers][nsteps]. */
+mci_status mci_set_mrt2step(mci_ctx *ctx, int idx, double step);
+mci_status mci_set_mrt2steps(mci_ctx *ctx, const double *steps, int nsteps);
+mci_status mci_get_mrt2step(mci_ctx *ctx, int idx, double *step);            /* walker 0 */
+mci_status mci_get_walker_mrt2steps(mci_ctx *ctx, double *steps, int64_t n); /* [nwalkers][nsteps] */
+mci_status mci_get_n_step_sizes(mci_ctx *ctx, int *n);
+
+/* ---- domain: MCI::setDomain/resetDomain/setIRange src/MCIntegrator.cpp:381-415;
+ *      OrthoPeriodicDomain src/OrthoPeriodicDomain.cpp:38-61; UnboundDomain.hpp ---- */
+mci_status mci_reset_domain(mci_ctx *ctx);
+mci_status mci_set_irange(mci_ctx *ctx, const double *lbounds, const double *ubounds); /* ndim each */
+mci_status mci_domain_is_finite(mci_ctx *ctx, int *finite);
+
+/* ---- trial move: MCI::setTrialMove(SRRDType, veclen, ntypes, typeEnds) src/MCIntegrator.cpp:437-453,
+ *      setTrialMove(MoveType) :429-435.  veclen 0 = all-move (SRRDAllMove.hpp:67-80),
+ *      veclen > 0 = single-vector move (SRRDVecMove.hpp:75-96). typeEnds may be NULL if ntypes == 1. */
+mci_status mci_set_trial_move(mci_ctx *ctx, int srrd, int veclen, int ntypes, const int *type_ends);
+mci_status mci_set_trial_move_type(mci_ctx *ctx, int move_type);
+mci_status mci_get_change_rate(mci_ctx *ctx, double *rate); /* TrialMoveInterface::getChangeRate */
+
+/* ---- sampling functions: MCI::addSamplingFunction src/MCIntegrator.cpp:486-492,
+ *      popSamplingFunction / clearSamplingFunctions MCIntegrator.hpp:174-175 ---- */
+mci_status mci_add_sampling_function(mci_ctx *ctx, int kind, int ndim, const double *params, int nparams);
+mci_status mci_pop_sampling_function(mci_ctx *ctx);
+mci_status mci_clear_sampling_functions(mci_ctx *ctx);
+mci_status mci_get_npdf(mci_ctx *ctx, int *npdf);
+
+/* ---- observables: MCI::addObservable(obs, blocksize, nskip, flag_equil, EstimatorType)
+ *      src/MCIntegrator.cpp:455-469; popObservable :481-484; clearObservables MCIntegrator.hpp:168 ---- */
+mci_status mci_add_observable(mci_ctx *ctx, int kind, int ndim, const double *params, int nparams,
+                              int blocksize, int nskip, int flag_equil, int estimator);
+mci_status mci_pop_observable(mci_ctx *ctx);
+mci_status mci_clear_observables(mci_ctx *ctx);
+mci_status mci_get_nobs(mci_ctx *ctx, int *nobs);
+mci_status mci_get_nobsdim(mci_ctx *ctx, int *nobsdim);
+
+/* ---- integrate: MCI::integrate src/MCIntegrator.cpp:43-82 (combine over walkers:
+ *      src/MPIMCI.cpp:85-92).  avg/err have nobsdim entries. ---- */
+mci_status mci_integrate(mci_ctx *ctx, int64_t nmc, double *average, double *error,
+                         int do_find_mrt2step, int do_decorrelation);
+/* per-walker estimates of the last integrate, layout [nwalkers][nobsdim] */
+mci_status mci_get_walker_results(mci_ctx *ctx, double *average, double *error, int64_t n);
+/* MCI::getAcceptanceRate src/MCIntegrator.cpp:587-592: counters of the last sample() call,
+ * summed over walkers; rate = mean over walkers of acc/(acc+rej) */
+mci_status mci_get_acceptance(mci_ctx *ctx, int64_t *accepted, int64_t *rejected, double *rate);
+mci_status mci_get_walker_acceptance(mci_ctx *ctx, int64_t *accepted, int64_t *rejected, int64_t n);
+
+/* ---- timing of the last integrate (HIP events on the context's stream, milliseconds) ----
+ * phases: [0] calibration, [1] decorrelation, [2] sampling kernel, [3] estimation+combine, [4] total */
+mci_status mci_get_timings(mci_ctx *ctx, double *ms, int n);
+
+/* ---- standalone estimators on host arrays (x has layout [n][ndim] like the reference),
+ *      run on the device: Estimators.cpp:125-155,191-246,271-286, MJBlocker.cpp:127-154 ---- */
+mci_status mci_estimate(int estimator, int64_t n, int ndim, const double *x, double *average, double *error);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCI_AMD_H */

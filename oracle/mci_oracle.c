@@ -1,0 +1,1773 @@
+/*
+ * mci_oracle.c — CPU oracle of MCIntegrator++'s Metropolis sampling path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used as the checker by tests/, the smoke check and
+ * the cpu_baseline leg of bench.py.  Nothing in mcintegratorplusplus_amd/
+ * links, loads or calls this file.
+ *
+ * A plain-C restatement of the reference algorithm.  Each function names the
+ * reference file:line it restates (paths relative to the reference root).
+ * Build with -O2 -ffp-contract=off (oracle/Makefile) so every floating-point
+ * operation rounds exactly as in the reference built with the same flags.
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *  - mt19937_64 / generate_canonical / uniform_int: the C++ standard's
+ *    known answer and a g++-compiled libstdc++ probe (oracle/stdrng_probe.cpp).
+ *  - Philox4x32-10: Random123 known-answer vectors.
+ *  - chain, accumulators, estimators, calibration, decorrelation, integrate:
+ *    golden vectors written by the reference itself (oracle/_ref, built from
+ *    the reference sources by oracle/Makefile) and closed forms.
+ */
+#include "mci_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* errors                                                                     */
+/* ------------------------------------------------------------------------- */
+static _Thread_local char g_err[512] = "";
+
+const char *orc_last_error(void) { return g_err; }
+
+static mci_status fail(mci_status code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define TRY(expr) do { mci_status _s = (expr); if (_s != MCI_OK) return _s; } while (0)
+
+/* ------------------------------------------------------------------------- */
+/* mt19937_64 — libstdc++ mersenne_twister_engine (bits/random.tcc) with the  */
+/* parameters of std::mt19937_64 ([rand.predef]); used as in                 */
+/* include/mci/MCIntegrator.hpp:32-34 and src/MCIntegrator.cpp:627.           */
+/* ------------------------------------------------------------------------- */
+#define MT_N 312
+#define MT_M 156
+#define MT_A 0xb5026f5aa96619e9ULL
+#define MT_UPPER 0xffffffff80000000ULL
+#define MT_LOWER 0x000000007fffffffULL
+
+typedef struct { uint64_t x[MT_N]; int p; } mt64;
+
+static void mt64_seed(mt64 *g, uint64_t s)
+{
+    g->x[0] = s;
+    for (int i = 1; i < MT_N; ++i) {
+        uint64_t v = g->x[i - 1];
+        v ^= v >> 62;
+        v *= 6364136223846793005ULL;
+        v += (uint64_t)i;
+        g->x[i] = v;
+    }
+    g->p = MT_N;
+}
+
+static void mt64_twist(mt64 *g)
+{
+    int k;
+    for (k = 0; k < MT_N - MT_M; ++k) {
+        uint64_t y = (g->x[k] & MT_UPPER) | (g->x[k + 1] & MT_LOWER);
+        g->x[k] = g->x[k + MT_M] ^ (y >> 1) ^ ((y & 1ULL) ? MT_A : 0ULL);
+    }
+    for (; k < MT_N - 1; ++k) {
+        uint64_t y = (g->x[k] & MT_UPPER) | (g->x[k + 1] & MT_LOWER);
+        g->x[k] = g->x[k + (MT_M - MT_N)] ^ (y >> 1) ^ ((y & 1ULL) ? MT_A : 0ULL);
+    }
+    uint64_t y = (g->x[MT_N - 1] & MT_UPPER) | (g->x[0] & MT_LOWER);
+    g->x[MT_N - 1] = g->x[MT_M - 1] ^ (y >> 1) ^ ((y & 1ULL) ? MT_A : 0ULL);
+    g->p = 0;
+}
+
+static uint64_t mt64_next(mt64 *g)
+{
+    if (g->p >= MT_N) mt64_twist(g);
+    uint64_t z = g->x[g->p++];
+    z ^= (z >> 29) & 0x5555555555555555ULL;
+    z ^= (z << 17) & 0x71d67fffeda60000ULL;
+    z ^= (z << 37) & 0xfff7eee000000000ULL;
+    z ^= (z >> 43);
+    return z;
+}
+
+uint64_t orc_mt64_nth(uint64_t seed, int64_t n)
+{
+    mt64 g;
+    mt64_seed(&g, seed);
+    uint64_t v = 0;
+    for (int64_t i = 0; i < n; ++i) v = mt64_next(&g);
+    return v;
+}
+
+void orc_mt64_words(uint64_t seed, int64_t n, uint64_t *out)
+{
+    mt64 g;
+    mt64_seed(&g, seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = mt64_next(&g);
+}
+
+/* generate_canonical<double, 53>(mt19937_64): one engine word,
+ * ret = double(word) / 2^64, clamped to nextafter(1, 0) (bits/random.tcc generate_canonical). */
+double orc_canonical53(uint64_t w)
+{
+    double r = (double)w / 18446744073709551616.0;
+    if (r >= 1.0) r = nextafter(1.0, 0.0);
+    return r;
+}
+
+/* uniform_int_distribution<int>(0, range-1) on a 64-bit engine: Lemire's nearly
+ * divisionless method with a 128-bit product (bits/uniform_int_dist.h _S_nd). */
+uint64_t orc_lemire64(const uint64_t *words, int64_t nwords, uint64_t range, int64_t *used)
+{
+    int64_t k = 0;
+    unsigned __int128 prod = (unsigned __int128)words[k++] * range;
+    uint64_t low = (uint64_t)prod;
+    if (low < range) {
+        uint64_t thr = (0 - range) % range;
+        while (low < thr && k < nwords) {
+            prod = (unsigned __int128)words[k++] * range;
+            low = (uint64_t)prod;
+        }
+    }
+    if (used) *used = k;
+    return (uint64_t)(prod >> 64);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon et al., SC'11; Random123 philox.h round/bumpkey).    */
+/* Native random mode of this project; no counterpart in the reference.       */
+/* ------------------------------------------------------------------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* ------------------------------------------------------------------------- */
+/* deterministic cross-walker sum: chunks of 256 values (zero padded), pairwise */
+/* tree inside a chunk (stride 128, 64, ..., 1), chunk sums added in order.    */
+/* The device reductions use exactly this order.                              */
+/* ------------------------------------------------------------------------- */
+double orc_det_sum(const double *v, int64_t n)
+{
+    double total = 0.0;
+    int first = 1;
+    for (int64_t c0 = 0; c0 < n; c0 += 256) {
+        double buf[256];
+        for (int i = 0; i < 256; ++i) buf[i] = (c0 + i < n) ? v[c0 + i] : 0.0;
+        for (int s = 128; s > 0; s >>= 1)
+            for (int i = 0; i < s; ++i) buf[i] = buf[i] + buf[i + s];
+        if (first) { total = buf[0]; first = 0; }
+        else total = total + buf[0];
+    }
+    return total;
+}
+
+/* ------------------------------------------------------------------------- */
+/* functor descriptors                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int kind, ndim, nparams, nproto;
+    double *params;
+    /* derived (GMM_ISO) */
+    int K;
+    double *gw, *gs, *gmu, *gc, *gb; /* weights, widths, means[K][ndim], log-weight terms, 1/(2 s^2) */
+} pdf_desc;
+
+typedef struct {
+    int kind, ndim, nparams, nobs, updateable;
+    double *params;
+    int blocksize, nskip, flag_equil, estim;
+} obs_desc;
+
+/* accumulator types, Factories.hpp:29-43 */
+enum { ACC_SIMPLE = 0, ACC_FULL = 1, ACC_BLOCK = 2 };
+
+/* AccumulatorInterface state (AccumulatorInterface.hpp:20-60) + child storage */
+typedef struct {
+    int type, nobs, xndim, nskip, blocksize, updateable;
+    double *obs_values;
+    uint8_t *flags_xchanged;
+    int nchanged;
+    int64_t nsteps, stepidx;
+    int skipidx, flag_final;
+    double *data;
+    int64_t nstore;   /* Simple: 1 row; Full: NAccu; Block: nblocks */
+    int bidx;
+    int64_t storeidx;
+} accu;
+
+typedef struct {
+    double *xold, *xnew;
+    int nchanged;
+    int *changedIdx;
+    int accepted;
+    double *pold, *pnew;   /* concatenated proto values of all pdfs */
+    double *steps;         /* one per type */
+    int64_t acc, rej;
+    mt64 mt;
+    uint64_t t;            /* Philox: steps taken */
+    uint32_t k, kx;        /* Philox: next word within the step / extra words */
+    uint32_t wid;
+} walker;
+
+#define MAX_PDF 16
+#define MAX_OBS 32
+
+struct orc_ctx {
+    int ndim;
+    int64_t W;
+    int rng, prec, coupling;
+    uint64_t seed;
+    uint64_t *wseeds;
+    double target;
+    int nfind;
+    int64_t ndecorr;
+    /* domain */
+    int periodic;
+    double *lb, *ub;
+    /* trial move */
+    int movetype, srrd, veclen, nvecs, ntypes;
+    int *typeEnds;
+    /* functors */
+    int npdf;
+    pdf_desc pdf[MAX_PDF];
+    int nproto_total;
+    int nobs;
+    obs_desc obs[MAX_OBS];
+    /* walkers */
+    walker *w;
+    /* per-walker results of last integrate */
+    double *wavg, *werr;
+    int nobsdim_last;
+    int64_t nwarn;
+    /* accumulators of the last main sampling, [walker][obs] */
+    accu *acc_last;
+    int acc_last_nobs;
+    /* trace */
+    double *trace_x;
+    uint8_t *trace_acc;
+    int64_t trace_cap, trace_n;
+    int tracing;
+};
+
+/* ------------------------------------------------------------------------- */
+/* per-walker random streams                                                  */
+/* ------------------------------------------------------------------------- */
+static uint64_t splitmix64(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* seed of walker w when only MCI::setSeed(seed) was called: w == 0 keeps the seed
+ * (so one walker reproduces one reference MCI), others get splitmix64 draws. */
+static uint64_t walker_seed(uint64_t seed, int64_t w)
+{
+    if (w == 0) return seed;
+    uint64_t s = seed ^ 0x6A09E667F3BCC909ULL;
+    uint64_t z = 0;
+    for (int64_t i = 0; i < 1; ++i) { s += (uint64_t)w * 0x9E3779B97F4A7C15ULL; z = splitmix64(&s); }
+    return z;
+}
+
+static uint32_t ph_word(const struct orc_ctx *c, walker *w, uint32_t k)
+{
+    uint32_t ctr[4] = { k >> 2, w->wid, (uint32_t)w->t, (uint32_t)(w->t >> 32) };
+    uint32_t key[2] = { (uint32_t)c->seed, (uint32_t)(c->seed >> 32) };
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    return out[k & 3u];
+}
+
+static uint32_t ph_next(const struct orc_ctx *c, walker *w) { return ph_word(c, w, w->k++); }
+static uint32_t ph_extra(const struct orc_ctx *c, walker *w) { return ph_word(c, w, 0x80000000u + (w->kx++)); }
+
+static void step_begin(const struct orc_ctx *c, walker *w)
+{
+    (void)c;
+    w->k = 0;
+    w->kx = 0;
+}
+
+static void step_end(const struct orc_ctx *c, walker *w)
+{
+    if (c->rng == MCI_RNG_PHILOX) ++w->t;
+}
+
+/* symmetric uniform in (-1,1): MT mode = uniform_real_distribution<double>(-1,1)
+ * (TrialMoveInterface.hpp:120-124): (u*(b-a)) + a; native = (2w+1-2^32) * 2^-32 */
+static double draw_sym(const struct orc_ctx *c, walker *w)
+{
+    if (c->rng == MCI_RNG_MT19937_64) {
+        double u = orc_canonical53(mt64_next(&w->mt));
+        return (u * 2.0) + (-1.0);
+    }
+    uint32_t v = ph_next(c, w);
+    return (double)(2 * (int64_t)v + 1 - 4294967296LL) * 0x1p-32;
+}
+
+/* uniform in [0,1): MT mode = uniform_real_distribution<double>(0,1) (MCIntegrator.cpp:628,343);
+ * native = 53 bits from two words */
+static double draw_unit(const struct orc_ctx *c, walker *w)
+{
+    if (c->rng == MCI_RNG_MT19937_64) {
+        double u = orc_canonical53(mt64_next(&w->mt));
+        return (u * 1.0) + 0.0;
+    }
+    uint64_t a = ph_next(c, w), b = ph_next(c, w);
+    uint64_t v = (a << 32) | b;
+    return (double)(v >> 11) * 0x1p-53;
+}
+
+/* uniform integer in [0, n): MT mode = uniform_int_distribution<int>(0, n-1) (SRRDVecMove.hpp:27,78);
+ * native = Lemire on 32-bit words with rejection from the extra-word stream */
+static int draw_index(const struct orc_ctx *c, walker *w, int n)
+{
+    if (c->rng == MCI_RNG_MT19937_64) {
+        uint64_t range = (uint64_t)n;
+        unsigned __int128 prod = (unsigned __int128)mt64_next(&w->mt) * range;
+        uint64_t low = (uint64_t)prod;
+        if (low < range) {
+            uint64_t thr = (0 - range) % range;
+            while (low < thr) {
+                prod = (unsigned __int128)mt64_next(&w->mt) * range;
+                low = (uint64_t)prod;
+            }
+        }
+        return (int)(prod >> 64);
+    }
+    uint32_t range = (uint32_t)n;
+    uint64_t prod = (uint64_t)ph_next(c, w) * range;
+    uint32_t low = (uint32_t)prod;
+    if (low < range) {
+        uint32_t thr = (0u - range) % range;
+        while (low < thr) {
+            prod = (uint64_t)ph_extra(c, w) * range;
+            low = (uint32_t)prod;
+        }
+    }
+    return (int)(prod >> 32);
+}
+
+/* ------------------------------------------------------------------------- */
+/* sampling functions (SamplingFunctionInterface.hpp:49-57,92-97)             */
+/* ------------------------------------------------------------------------- */
+static double pdf_a(const pdf_desc *p, int i) { return (p->nparams >= p->ndim) ? p->params[i] : 1.0; }
+
+/* log-density of the isotropic mixture: LSE_k (c_k - b_k r2_k), r2_k = sum_i (x_i - mu_ki)^2 */
+static double gmm_iso_logp(const pdf_desc *p, const double *r2)
+{
+    double m = -INFINITY;
+    for (int k = 0; k < p->K; ++k) {
+        double t = p->gc[k] - p->gb[k] * r2[k];
+        if (t > m) m = t;
+    }
+    double s = 0.0;
+    for (int k = 0; k < p->K; ++k) s = s + exp((p->gc[k] - p->gb[k] * r2[k]) - m);
+    return m + log(s);
+}
+
+/* protoFunction: fills proto[0..nproto) from x */
+static void pdf_proto(const pdf_desc *p, const double *x, double *proto)
+{
+    switch (p->kind) {
+    case MCI_PDF_GAUSS_ISO: { /* TestMCIFunctions.hpp:135-138 */
+        double s = x[0] * x[0];
+        for (int i = 1; i < p->ndim; ++i) s = s + x[i] * x[i];
+        proto[0] = s;
+        break;
+    }
+    case MCI_PDF_GAUSS_DIAG: /* TestMCIFunctions.hpp:159-164 */
+        for (int i = 0; i < p->ndim; ++i) proto[i] = (x[i] * x[i]) * pdf_a(p, i);
+        break;
+    case MCI_PDF_EXP_DIAG: /* TestMCIFunctions.hpp:227-230 */
+        for (int i = 0; i < p->ndim; ++i) proto[i] = fabs(x[i]) * pdf_a(p, i);
+        break;
+    case MCI_PDF_GMM_ISO: {
+        for (int k = 0; k < p->K; ++k) {
+            const double *mu = p->gmu + (size_t)k * p->ndim;
+            double s = 0.0;
+            for (int i = 0; i < p->ndim; ++i) { double d = x[i] - mu[i]; s = s + d * d; }
+            proto[k] = s;
+        }
+        proto[p->K] = gmm_iso_logp(p, proto);
+        break;
+    }
+    }
+}
+
+/* acceptanceFunction(protoold, protonew) */
+static double pdf_acceptance(const pdf_desc *p, const double *po, const double *pn)
+{
+    switch (p->kind) {
+    case MCI_PDF_GAUSS_ISO: /* TestMCIFunctions.hpp:145-148 */
+        return exp(-pn[0] + po[0]);
+    case MCI_PDF_GAUSS_DIAG:
+    case MCI_PDF_EXP_DIAG: { /* TestMCIFunctions.hpp:171-176 (std::accumulate from 0.) */
+        double e = 0.0;
+        for (int i = 0; i < p->nproto; ++i) e = e + po[i];
+        double s = 0.0;
+        for (int i = 0; i < p->nproto; ++i) s = s + pn[i];
+        e = e - s;
+        return exp(e);
+    }
+    case MCI_PDF_GMM_ISO:
+        return exp(pn[p->K] - po[p->K]);
+    }
+    return 0.0;
+}
+
+/* computeAcceptance (SamplingFunctionInterface.hpp:49-57) incl. updatedAcceptance overrides */
+static double pdf_compute_acceptance(const pdf_desc *p, const walker *w, const double *po, double *pn)
+{
+    if (w->nchanged < p->ndim) {
+        switch (p->kind) {
+        case MCI_PDF_GAUSS_DIAG: { /* TestMCIFunctions.hpp:178-187 */
+            double e = 0.0;
+            for (int i = 0; i < w->nchanged; ++i) {
+                int j = w->changedIdx[i];
+                pn[j] = (w->xnew[j] * w->xnew[j]) * pdf_a(p, j);
+                e = e + (pn[j] - po[j]);
+            }
+            return exp(-e);
+        }
+        case MCI_PDF_EXP_DIAG: { /* TestMCIFunctions.hpp:244-253 */
+            double e = 0.0;
+            for (int i = 0; i < w->nchanged; ++i) {
+                int j = w->changedIdx[i];
+                pn[j] = fabs(w->xnew[j]) * pdf_a(p, j);
+                e = e + (pn[j] - po[j]);
+            }
+            return exp(-e);
+        }
+        case MCI_PDF_GMM_ISO: { /* incremental r2 update over the changed indices */
+            for (int k = 0; k < p->K; ++k) {
+                const double *mu = p->gmu + (size_t)k * p->ndim;
+                double dr = 0.0;
+                for (int i = 0; i < w->nchanged; ++i) {
+                    int j = w->changedIdx[i];
+                    double dn = w->xnew[j] - mu[j], dd = w->xold[j] - mu[j];
+                    dr = dr + (dn * dn - dd * dd);
+                }
+                pn[k] = po[k] + dr;
+            }
+            pn[p->K] = gmm_iso_logp(p, pn);
+            return exp(pn[p->K] - po[p->K]);
+        }
+        default: /* default updatedAcceptance = full recomputation (:92-97) */
+            break;
+        }
+    }
+    pdf_proto(p, w->xnew, pn);
+    return pdf_acceptance(p, po, pn);
+}
+
+/* ------------------------------------------------------------------------- */
+/* observables (ObservableFunctionInterface.hpp:48,61)                        */
+/* ------------------------------------------------------------------------- */
+static double obs_param(const obs_desc *o, int i, double dflt) { return (o->nparams > i) ? o->params[i] : dflt; }
+
+static void obs_eval(const obs_desc *o, const double *x, double *out)
+{
+    int n = o->ndim;
+    switch (o->kind) {
+    case MCI_OBS_X2: /* TestMCIFunctions.hpp:446-451 */
+        for (int i = 0; i < n; ++i) out[i] = x[i] * x[i];
+        break;
+    case MCI_OBS_X2_SUM: { /* TestMCIFunctions.hpp:422-427 (X2Sum), :270-273 (XSquared, c = 3) */
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s = s + x[i] * x[i];
+        out[0] = s / obs_param(o, 0, 1.0);
+        break;
+    }
+    case MCI_OBS_X: /* TestMCIFunctions.hpp:345-348 */
+        for (int i = 0; i < n; ++i) out[i] = x[i];
+        break;
+    case MCI_OBS_X_SUM: { /* TestMCIFunctions.hpp:398-404 */
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s = s + x[i];
+        out[0] = s;
+        break;
+    }
+    case MCI_OBS_CONST: /* TestMCIFunctions.hpp:378-381 */
+        out[0] = obs_param(o, 0, 1.0);
+        break;
+    case MCI_OBS_EXP_X2: {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s = s + x[i] * x[i];
+        out[0] = exp(-(s / obs_param(o, 0, (double)n)));
+        break;
+    }
+    case MCI_OBS_INDICATOR: {
+        int j = (int)obs_param(o, 0, 0.0);
+        out[0] = (x[j] > 0.0) ? 1.0 : 0.0;
+        break;
+    }
+    }
+}
+
+/* updatedObservable of the updateable kinds (X2, TestMCIFunctions.hpp:453-460) */
+static void obs_update(const obs_desc *o, const double *x, const uint8_t *flags, double *out)
+{
+    if (o->kind == MCI_OBS_X2) {
+        for (int i = 0; i < o->ndim; ++i)
+            if (flags[i]) out[i] = x[i] * x[i];
+        return;
+    }
+    obs_eval(o, x, out);
+}
+
+static int obs_nobs(int kind, int ndim)
+{
+    switch (kind) {
+    case MCI_OBS_X2: case MCI_OBS_X: return ndim;
+    case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR: return 1;
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* accumulators: AccumulatorInterface.cpp:5-137, Simple/Block/FullAccumulator */
+/* ------------------------------------------------------------------------- */
+static int64_t accu_naccu(const accu *a) { return (a->nsteps > 0) ? 1 + (a->nsteps - 1) / a->nskip : 0; }
+
+static void accu_init_base(accu *a) /* AccumulatorInterface.cpp:24-32 */
+{
+    a->stepidx = 0;
+    a->skipidx = a->nskip - 1;
+    a->flag_final = 0;
+    a->nchanged = a->xndim;
+    if (a->updateable) memset(a->flags_xchanged, 1, (size_t)a->xndim);
+}
+
+static void accu_reset(accu *a) /* AccumulatorInterface.cpp:123-127 + child _reset */
+{
+    a->bidx = 0;
+    a->storeidx = 0;
+    if (a->data) memset(a->data, 0, sizeof(double) * (size_t)(a->nstore * a->nobs));
+    accu_init_base(a);
+}
+
+static void accu_free(accu *a)
+{
+    free(a->obs_values); free(a->flags_xchanged); free(a->data);
+    memset(a, 0, sizeof *a);
+}
+
+static mci_status accu_make(accu *a, const obs_desc *o, int xndim)
+{
+    memset(a, 0, sizeof *a);
+    a->blocksize = o->blocksize;
+    a->type = (o->blocksize == 0) ? ACC_SIMPLE : (o->blocksize == 1 ? ACC_FULL : ACC_BLOCK);
+    a->nobs = o->nobs;
+    a->xndim = xndim;
+    a->nskip = o->nskip;
+    a->updateable = o->updateable;
+    a->obs_values = calloc((size_t)a->nobs, sizeof(double));
+    a->flags_xchanged = calloc((size_t)xndim, 1);
+    accu_init_base(a);
+    return MCI_OK;
+}
+
+/* allocate (AccumulatorInterface.cpp:88-96 + BlockAccumulator.cpp:6-17, FullAccumulator.cpp:6-11,
+ * SimpleAccumulator.cpp:6-11) */
+static mci_status accu_allocate(accu *a, int64_t nsteps)
+{
+    free(a->data);
+    a->data = NULL;
+    a->nstore = 0;
+    if (nsteps < 1) return fail(MCI_E_INVALID_ARGUMENT, "[AccumulatorInterface::allocate] Provided number of MC steps was < 1 .");
+    a->nsteps = nsteps;
+    int64_t naccu = accu_naccu(a);
+    if (a->type == ACC_SIMPLE) a->nstore = 1;
+    else if (a->type == ACC_FULL) a->nstore = naccu;
+    else {
+        if (naccu < a->blocksize)
+            return fail(MCI_E_INVALID_ARGUMENT, "[BlockAccumulator::allocate] Requested number of accumulations is smaller than the requested block size.");
+        if (naccu % a->blocksize != 0)
+            return fail(MCI_E_INVALID_ARGUMENT, "[BlockAccumulator::allocate] Requested number of accumulations is not a multiple of the requested block size.");
+        a->nstore = naccu / a->blocksize;
+    }
+    a->data = calloc((size_t)(a->nstore * a->nobs), sizeof(double));
+    accu_reset(a);
+    return MCI_OK;
+}
+
+static void accu_store(accu *a) /* child _accumulate */
+{
+    switch (a->type) {
+    case ACC_SIMPLE: /* SimpleAccumulator.cpp:14-19 */
+        for (int i = 0; i < a->nobs; ++i) a->data[i] = a->data[i] + a->obs_values[i];
+        break;
+    case ACC_BLOCK: /* BlockAccumulator.cpp:20-30 */
+        for (int i = 0; i < a->nobs; ++i) a->data[a->storeidx + i] = a->data[a->storeidx + i] + a->obs_values[i];
+        if (++a->bidx == a->blocksize) { a->bidx = 0; a->storeidx += a->nobs; }
+        break;
+    case ACC_FULL: /* FullAccumulator.cpp:14-18 */
+        memcpy(a->data + a->storeidx, a->obs_values, sizeof(double) * (size_t)a->nobs);
+        a->storeidx += a->nobs;
+        break;
+    }
+}
+
+/* accumulate (AccumulatorInterface.cpp:99-114, _processOld/_processFull/_processSelective :34-85) */
+static void accu_accumulate(accu *a, const obs_desc *o, const walker *w)
+{
+    if (w->accepted || a->nchanged > 0) {
+        if (a->updateable) {
+            if (a->nchanged < a->xndim && w->accepted) {
+                if (w->nchanged < a->xndim) {
+                    for (int i = 0; i < w->nchanged; ++i) {
+                        int j = w->changedIdx[i];
+                        if (!a->flags_xchanged[j]) { a->flags_xchanged[j] = 1; ++a->nchanged; }
+                    }
+                } else {
+                    a->nchanged = a->xndim;
+                }
+            }
+            if (++a->skipidx == a->nskip) {
+                a->skipidx = 0;
+                if (a->nchanged < a->xndim) obs_update(o, w->xnew, a->flags_xchanged, a->obs_values);
+                else obs_eval(o, w->xnew, a->obs_values);
+                memset(a->flags_xchanged, 0, (size_t)a->xndim);
+                a->nchanged = 0;
+                accu_store(a);
+            }
+        } else {
+            a->nchanged = a->xndim;
+            if (++a->skipidx == a->nskip) {
+                a->skipidx = 0;
+                obs_eval(o, w->xnew, a->obs_values);
+                a->nchanged = 0;
+                accu_store(a);
+            }
+        }
+    } else {
+        if (++a->skipidx == a->nskip) { a->skipidx = 0; accu_store(a); }
+    }
+    ++a->stepidx;
+}
+
+/* finalize (AccumulatorInterface.cpp:117-121 + BlockAccumulator.cpp:33-41, SimpleAccumulator.cpp:22-28) */
+static mci_status accu_finalize(accu *a)
+{
+    if (a->stepidx != a->nsteps)
+        return fail(MCI_E_RUNTIME, "[AccumulatorInterface::finalize] Finalize was called, but number of accumulated steps do not match the planned amount.");
+    if (!a->flag_final) {
+        if (a->type == ACC_SIMPLE) {
+            double normf = 1. / (double)accu_naccu(a);
+            for (int i = 0; i < a->nobs; ++i) a->data[i] = a->data[i] * normf;
+        } else if (a->type == ACC_BLOCK) {
+            double normf = 1. / a->blocksize;
+            for (int64_t i = 0; i < a->nstore * a->nobs; ++i) a->data[i] = a->data[i] * normf;
+        }
+    }
+    a->flag_final = 1;
+    return MCI_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* estimators: src/Estimators.cpp, src/MJBlocker.cpp                          */
+/* ------------------------------------------------------------------------- */
+static const double chi2_q95[64] = {
+    /* 0.95 quantiles of the chi-squared distribution, df = 1..64, to 6 decimals
+     * (MJBlocker.cpp:6-13; regenerated and checked with scipy.stats.chi2.ppf in tests) */
+    3.841459, 5.991465, 7.814728, 9.487729, 11.070498, 12.591587, 14.067140, 15.507313,
+    16.918978, 18.307038, 19.675138, 21.026070, 22.362032, 23.684791, 24.995790, 26.296228,
+    27.587112, 28.869299, 30.143527, 31.410433, 32.670573, 33.924438, 35.172462, 36.415029,
+    37.652484, 38.885139, 40.113272, 41.337138, 42.556968, 43.772972, 44.985343, 46.194260,
+    47.399884, 48.602367, 49.801850, 50.998460, 52.192320, 53.383541, 54.572228, 55.758479,
+    56.942387, 58.124038, 59.303512, 60.480887, 61.656233, 62.829620, 64.001112, 65.170769,
+    66.338649, 67.504807, 68.669294, 69.832160, 70.993453, 72.153216, 73.311493, 74.468324,
+    75.623748, 76.777803, 77.930524, 79.081944, 80.232098, 81.381015, 82.528727, 83.675261};
+
+static int is_pow2(int64_t n) { return n > 0 && (((uint64_t)n & ((uint64_t)n - 1)) == 0); }
+
+/* OneDimUncorrelatedEstimator, Estimators.cpp:37-57 */
+static mci_status est_uncorr_1d(int64_t n, const double *x, double *avg, double *err)
+{
+    if (n < 2) return fail(MCI_E_INVALID_ARGUMENT, "[OneDimUncorrelatedEstimator] n must be larger than 1");
+    double a = 0.0, e = 0.0;
+    for (int64_t i = 0; i < n; ++i) a = a + x[i];
+    for (int64_t i = 0; i < n; ++i) e = e + x[i] * x[i];
+    const double norm = 1. / (double)n;
+    a = a * norm;
+    e = e * norm - a * a;
+    if (e > 1.e-300) e = sqrt(e / ((double)n - 1.));
+    else e = 0.;
+    *avg = a;
+    *err = e;
+    return MCI_OK;
+}
+
+/* MultiDimUncorrelatedEstimator, Estimators.cpp:125-155 (stride = row length of x) */
+static mci_status est_uncorr_nd(int64_t n, int ndim, const double *x, double *avg, double *err)
+{
+    if (n < 2) return fail(MCI_E_INVALID_ARGUMENT, "[MultiDimUncorrelatedEstimator] n must be larger than 1");
+    for (int j = 0; j < ndim; ++j) { avg[j] = 0.; err[j] = 0.; }
+    for (int64_t i = 0; i < n; ++i)
+        for (int j = 0; j < ndim; ++j) {
+            avg[j] = avg[j] + x[i * ndim + j];
+            err[j] = err[j] + x[i * ndim + j] * x[i * ndim + j];
+        }
+    const double norm = 1. / (double)n;
+    const double norm2 = 1. / ((double)n - 1.);
+    for (int j = 0; j < ndim; ++j) {
+        avg[j] = avg[j] * norm;
+        err[j] = err[j] * norm - avg[j] * avg[j];
+        if (err[j] > 1.e-300) err[j] = sqrt(err[j] * norm2);
+        else err[j] = 0.;
+    }
+    return MCI_OK;
+}
+
+/* MultiDimBlockEstimator, Estimators.cpp:158-185 */
+static mci_status est_block_nd(int64_t n, int ndim, const double *x, int64_t nblocks, double *avg, double *err)
+{
+    if (n < nblocks) return fail(MCI_E_INVALID_ARGUMENT, "MCI error MultiDimBlockEstimator() : n must be >= nblocks");
+    const int64_t nper = n / nblocks;
+    const double norm = 1. / (double)nper;
+    double *av = calloc((size_t)(nblocks * ndim), sizeof(double));
+    for (int64_t b = 0; b < nblocks; ++b)
+        for (int64_t i = b * nper; i < (b + 1) * nper; ++i)
+            for (int j = 0; j < ndim; ++j) av[b * ndim + j] = av[b * ndim + j] + x[i * ndim + j];
+    for (int64_t i = 0; i < nblocks * ndim; ++i) av[i] = av[i] * norm;
+    mci_status s = est_uncorr_nd(nblocks, ndim, av, avg, err);
+    free(av);
+    return s;
+}
+
+/* OneDimBlockEstimator, Estimators.cpp:60-77 */
+static mci_status est_block_1d(int64_t n, const double *x, int64_t nblocks, double *avg, double *err)
+{
+    if (n < nblocks) return fail(MCI_E_INVALID_ARGUMENT, "[OneDimBlockEstimator] n must be >= nblocks");
+    const int64_t nper = n / nblocks;
+    const double norm = 1. / (double)nper;
+    double *av = malloc(sizeof(double) * (size_t)nblocks);
+    for (int64_t b = 0; b < nblocks; ++b) {
+        double s = 0.0;
+        for (int64_t i = b * nper; i < (b + 1) * nper; ++i) s = s + x[i];
+        av[b] = s * norm;
+    }
+    mci_status st = est_uncorr_1d(nblocks, av, avg, err);
+    free(av);
+    return st;
+}
+
+/* calcErrDelta, Estimators.cpp:9-31 (central finite differences of order 2*mode) */
+static double calc_err_delta(int mode, const double *e)
+{
+    const int im = 4;
+    switch (mode) {
+    case 1: return (-0.5 * e[im - 1] + 0.5 * e[im + 1]);
+    case 2: return ((1. / 12.) * e[im - 2] - (2. / 3.) * e[im - 1] + (2. / 3.) * e[im + 1] - (1. / 12.) * e[im + 2]);
+    case 3: return (-(1. / 60.) * e[im - 3] + (3. / 20.) * e[im - 2] - 0.75 * e[im - 1] + 0.75 * e[im + 1]
+                    - (3. / 20.) * e[im + 2] + (1. / 60.) * e[im + 3]);
+    case 4: return ((1. / 280.) * e[im - 4] - (4. / 105.) * e[im - 3] + 0.2 * e[im - 2] - 0.8 * e[im - 1]
+                    + 0.8 * e[im + 1] - 0.2 * e[im + 2] + (4. / 105.) * e[im + 3] - (1. / 280.) * e[im + 4]);
+    }
+    return 0.;
+}
+
+#define FC_MIN 6
+#define FC_MAX 50
+#define FC_PLAT 4
+#define FC_NAV (FC_MAX - FC_MIN + 1)
+#define FC_NACCD (FC_NAV - 2 * FC_PLAT)
+
+/* OneDimFCBlockerEstimator, Estimators.cpp:83-122 */
+static mci_status est_fc_1d(int64_t n, const double *x, double *avg, double *err)
+{
+    if (n < FC_MAX) return fail(MCI_E_INVALID_ARGUMENT, "[OneDimFCBlockerEstimator] n must be >= 50");
+    double av[FC_NAV], er[FC_NAV], accd[FC_NACCD];
+    for (int i = 0; i < FC_NAV; ++i) TRY(est_block_1d(n, x, i + FC_MIN, &av[i], &er[i]));
+    for (int i = 0; i < FC_NACCD; ++i) accd[i] = 0.;
+    for (int i2 = FC_PLAT; i2 < FC_NACCD + FC_PLAT; ++i2)
+        for (int i1 = 1; i1 <= FC_PLAT; ++i1) accd[i2 - FC_PLAT] = accd[i2 - FC_PLAT] + calc_err_delta(i1, er + i2 - 4);
+    int imin = 0;
+    for (int i2 = 1; i2 < FC_NACCD; ++i2)
+        if (fabs(accd[i2]) < fabs(accd[imin])) imin = i2;
+    imin += FC_PLAT;
+    *avg = 0.2 * (av[imin - 2] + av[imin - 1] + av[imin] + av[imin + 1] + av[imin + 2]);
+    *err = 0.2 * (er[imin - 2] + er[imin - 1] + er[imin] + er[imin + 1] + er[imin + 2]);
+    return MCI_OK;
+}
+
+/* MultiDimFCBlockerEstimator, Estimators.cpp:191-246 */
+static mci_status est_fc_nd(int64_t n, int ndim, const double *x, double *avg, double *err)
+{
+    if (n < FC_MAX) return fail(MCI_E_INVALID_ARGUMENT, "MCI error MultiDimFCBlockerEstimator() : n must be >= 50");
+    double *av = malloc(sizeof(double) * FC_NAV * (size_t)ndim);
+    double *er = malloc(sizeof(double) * FC_NAV * (size_t)ndim);
+    double *accd = calloc((size_t)FC_NACCD * (size_t)ndim, sizeof(double));
+    int *imin = calloc((size_t)ndim, sizeof(int));
+    mci_status st = MCI_OK;
+    for (int i1 = 0; i1 < FC_NAV && st == MCI_OK; ++i1) st = est_block_nd(n, ndim, x, i1 + FC_MIN, av + i1 * ndim, er + i1 * ndim);
+    if (st == MCI_OK) {
+        double eh[9];
+        for (int i2 = FC_PLAT; i2 < FC_NACCD + FC_PLAT; ++i2)
+            for (int j = 0; j < ndim; ++j) {
+                for (int s = -4; s < 5; ++s) eh[s + 4] = er[(i2 + s) * ndim + j];
+                for (int i1 = 1; i1 <= FC_PLAT; ++i1)
+                    accd[(i2 - FC_PLAT) * ndim + j] = accd[(i2 - FC_PLAT) * ndim + j] + calc_err_delta(i1, eh);
+            }
+        for (int i2 = 1; i2 < FC_NACCD; ++i2)
+            for (int j = 0; j < ndim; ++j)
+                if (fabs(accd[i2 * ndim + j]) < fabs(accd[imin[j] * ndim + j])) imin[j] = i2;
+        for (int j = 0; j < ndim; ++j) {
+            int ix = imin[j] + FC_PLAT;
+            avg[j] = 0.2 * (av[(ix - 2) * ndim + j] + av[(ix - 1) * ndim + j] + av[ix * ndim + j] + av[(ix + 1) * ndim + j] + av[(ix + 2) * ndim + j]);
+            err[j] = 0.2 * (er[(ix - 2) * ndim + j] + er[(ix - 1) * ndim + j] + er[ix * ndim + j] + er[(ix + 1) * ndim + j] + er[(ix + 2) * ndim + j]);
+        }
+    }
+    free(imin); free(accd); free(er); free(av);
+    return st;
+}
+
+/* MJBlocker::estimate, MJBlocker.cpp:127-154 (Jonsson, PRE 98, 043304 (2018)).
+ * Defined behaviour where the reference reads out of bounds (a level with zero
+ * variance gives 0/0 in its test statistic): such a level contributes M = 0. */
+static mci_status est_mj(int64_t n, int ndim, const double *xin, double *avg, double *err)
+{
+    if (n <= 0) return fail(MCI_E_INVALID_ARGUMENT, "[MJBlocker] ndata must be a natural number.");
+    if (!is_pow2(n)) return fail(MCI_E_INVALID_ARGUMENT, "[MJBlocker] ndata must be a power of two.");
+    int npow = 0;
+    while (((int64_t)1 << npow) < n) ++npow;
+    double *x = malloc(sizeof(double) * (size_t)(n * ndim));
+    double *X = malloc(sizeof(double) * (size_t)(n * ndim));
+    double *var = calloc((size_t)(npow * ndim) + 1, sizeof(double));
+    double *gam = calloc((size_t)(npow * ndim) + 1, sizeof(double));
+    double *M = calloc((size_t)(npow * ndim) + 1, sizeof(double));
+    double *Ms = calloc((size_t)(npow * ndim) + 1, sizeof(double));
+    memcpy(x, xin, sizeof(double) * (size_t)(n * ndim));
+    /* _computeMean :34-44 */
+    for (int j = 0; j < ndim; ++j) avg[j] = 0.;
+    for (int64_t i = 0; i < n; ++i)
+        for (int j = 0; j < ndim; ++j) avg[j] = avg[j] + x[i * ndim + j];
+    for (int j = 0; j < ndim; ++j) avg[j] = avg[j] / (double)n;
+    /* _initX :47-54 */
+    for (int64_t i = 0; i < n * ndim; ++i) X[i] = x[i] - avg[i % ndim];
+    int64_t nred = n;
+    for (int k = 0; k < npow; ++k) {
+        double *v = var + k * ndim, *g = gam + k * ndim;
+        /* _gamma0 :59-68 */
+        for (int j = 0; j < ndim; ++j) v[j] = 0.;
+        for (int64_t i = 0; i < nred; ++i)
+            for (int j = 0; j < ndim; ++j) v[j] = v[j] + X[i * ndim + j] * X[i * ndim + j];
+        for (int j = 0; j < ndim; ++j) v[j] = v[j] / (double)nred;
+        /* _gamma1 :71-80 */
+        for (int j = 0; j < ndim; ++j) g[j] = 0.;
+        for (int64_t i = 0; i < nred - 1; ++i)
+            for (int j = 0; j < ndim; ++j) g[j] = g[j] + X[i * ndim + j] * X[(i + 1) * ndim + j];
+        for (int j = 0; j < ndim; ++j) g[j] = g[j] / (double)nred;
+        /* _transform :83-92 */
+        for (int64_t i = 0; i < nred / 2; ++i)
+            for (int j = 0; j < ndim; ++j) {
+                x[i * ndim + j] = 0.5 * (x[(2 * i) * ndim + j] + x[(2 * i + 1) * ndim + j]);
+                X[i * ndim + j] = x[i * ndim + j] - avg[j];
+            }
+        nred = nred / 2;
+    }
+    /* _generateM :106-123 */
+    for (int i = 0; i < npow; ++i)
+        for (int k = 0; k < ndim; ++k) {
+            double g = gam[i * ndim + k], v = var[i * ndim + k];
+            double m = (v == 0.0) ? 0.0 : pow(g / v, 2) * pow(2, npow - i);
+            M[(npow - i - 1) * ndim + k] = m;
+        }
+    for (int i = 0; i < npow; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int k = 0; k < ndim; ++k) Ms[i * ndim + k] = Ms[i * ndim + k] + M[j * ndim + k];
+    /* chi^2 search and error :140-153 */
+    for (int l = 0; l < ndim; ++l) {
+        int k;
+        for (k = npow - 1; k >= 0; --k)
+            if (Ms[k * ndim + l] < chi2_q95[k]) break;
+        k = npow - (k + 1);
+        if (k >= npow) k = npow - 1; /* unreachable with the M = 0 rule; kept as a guard */
+        err[l] = sqrt(var[k * ndim + l] / pow(2, npow - k));
+    }
+    free(Ms); free(M); free(gam); free(var); free(X); free(x);
+    return MCI_OK;
+}
+
+/* estimator dispatch: Factories.hpp:73-95, Estimators.cpp:250-293 */
+static mci_status estimate_any(int estim, int64_t n, int ndim, const double *x, double *avg, double *err)
+{
+    switch (estim) {
+    case MCI_ESTIM_NOOP: /* Estimators.cpp:289-293 */
+        for (int j = 0; j < ndim; ++j) { avg[j] = x[j]; err[j] = 0.; }
+        return MCI_OK;
+    case MCI_ESTIM_UNCORRELATED:
+        return (ndim > 1) ? est_uncorr_nd(n, ndim, x, avg, err) : est_uncorr_1d(n, x, avg, err);
+    case MCI_ESTIM_FCBLOCKER:
+        return (ndim > 1) ? est_fc_nd(n, ndim, x, avg, err) : est_fc_1d(n, x, avg, err);
+    case MCI_ESTIM_MJBLOCKER:
+        return est_mj(n, ndim, x, avg, err);
+    case MCI_ESTIM_CORRELATED:
+        if (!is_pow2(n)) return (ndim > 1) ? est_fc_nd(n, ndim, x, avg, err) : est_fc_1d(n, x, avg, err);
+        return est_mj(n, ndim, x, avg, err);
+    }
+    return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator.");
+}
+
+mci_status orc_estimate(int estimator, int64_t n, int ndim, const double *x, double *average, double *error)
+{
+    return estimate_any(estimator, n, ndim, x, average, error);
+}
+
+mci_status orc_block_estimate(int64_t n, int ndim, const double *x, int64_t nblocks, double *average, double *error)
+{
+    return (ndim > 1) ? est_block_nd(n, ndim, x, nblocks, average, error) : est_block_1d(n, x, nblocks, average, error);
+}
+
+/* ------------------------------------------------------------------------- */
+/* context                                                                    */
+/* ------------------------------------------------------------------------- */
+static void walker_alloc(orc_ctx *c, walker *w)
+{
+    w->xold = calloc((size_t)c->ndim, sizeof(double));
+    w->xnew = calloc((size_t)c->ndim, sizeof(double));
+    w->changedIdx = calloc((size_t)c->ndim, sizeof(int));
+    for (int i = 0; i < c->ndim; ++i) w->changedIdx[i] = i;
+    w->nchanged = c->ndim;
+    w->accepted = 1;
+    w->steps = calloc(64, sizeof(double));
+    w->pold = NULL;
+    w->pnew = NULL;
+}
+
+static void rebuild_protos(orc_ctx *c)
+{
+    int np = 0;
+    for (int i = 0; i < c->npdf; ++i) np += c->pdf[i].nproto;
+    c->nproto_total = np;
+    for (int64_t k = 0; k < c->W; ++k) {
+        walker *w = &c->w[k];
+        free(w->pold); free(w->pnew);
+        w->pold = calloc((size_t)np + 1, sizeof(double));
+        w->pnew = calloc((size_t)np + 1, sizeof(double));
+    }
+}
+
+static void seed_walkers(orc_ctx *c)
+{
+    for (int64_t k = 0; k < c->W; ++k) {
+        uint64_t s = c->wseeds ? c->wseeds[k] : walker_seed(c->seed, k);
+        mt64_seed(&c->w[k].mt, s);
+        c->w[k].t = 0;
+        c->w[k].wid = (uint32_t)k;
+    }
+}
+
+/* trial move construction: createSRRDAllMove / createSRRDVecMove (Factories.hpp:160-227),
+ * TypedMoveInterface ctor (TypedMoveInterface.hpp:27-39), SRRDVecMove ctor checks (SRRDVecMove.hpp:39-50) */
+static mci_status set_move(orc_ctx *c, int srrd, int veclen, int ntypes, const int *typeEnds, double init)
+{
+    if (srrd != MCI_SRRD_UNIFORM) return fail(MCI_E_DOMAIN, "[oracle] only the uniform SRRD move is restated");
+    if (ntypes < 1) ntypes = 1;
+    if (ntypes > 1 && typeEnds == NULL) return fail(MCI_E_INVALID_ARGUMENT, "[checkTrialMoveSanity] ntypes>1 requires passed typeEnds array.");
+    if (veclen > 0 && c->ndim % veclen != 0)
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::setTrialMove] MCI's number of walkers must be a multiple of passed veclen.");
+    if (ntypes > 64) return fail(MCI_E_INVALID_ARGUMENT, "[oracle] at most 64 step-size types");
+    if (veclen > 0 && ntypes > 1)
+        for (int i = 0; i < ntypes; ++i)
+            if (typeEnds[i] % veclen != 0)
+                return fail(MCI_E_INVALID_ARGUMENT, "[SRRDVecMove] All type end indices must be multiples of vector length.");
+    free(c->typeEnds);
+    c->typeEnds = malloc(sizeof(int) * (size_t)ntypes);
+    if (ntypes > 1) memcpy(c->typeEnds, typeEnds, sizeof(int) * (size_t)ntypes);
+    else c->typeEnds[0] = c->ndim;
+    c->ntypes = ntypes;
+    c->srrd = srrd;
+    c->movetype = (veclen > 0) ? MCI_MOVE_VEC : MCI_MOVE_ALL;
+    c->veclen = (veclen > 0) ? veclen : 0;
+    c->nvecs = (veclen > 0) ? c->ndim / veclen : 0;
+    for (int64_t k = 0; k < c->W; ++k)
+        for (int i = 0; i < ntypes; ++i) c->w[k].steps[i] = init;
+    return MCI_OK;
+}
+
+mci_status orc_create(int ndim, int64_t nwalkers, orc_ctx **out)
+{
+    if (ndim < 1) return fail(MCI_E_INVALID_ARGUMENT, "[ProtoFunctionInterface] Number of dimensions must be at least 1.");
+    if (nwalkers < 1) return fail(MCI_E_INVALID_ARGUMENT, "number of walkers must be at least 1");
+    orc_ctx *c = calloc(1, sizeof *c);
+    c->ndim = ndim;
+    c->W = nwalkers;
+    c->rng = MCI_RNG_MT19937_64;
+    c->prec = MCI_PREC_FP64;
+    c->coupling = MCI_COUPLING_INDEPENDENT;
+    c->seed = 5489u; /* a fixed default instead of std::random_device (MCIntegrator.cpp:627) */
+    c->target = 0.5;
+    c->nfind = -50;
+    c->ndecorr = -10000;
+    c->lb = calloc((size_t)ndim, sizeof(double));
+    c->ub = calloc((size_t)ndim, sizeof(double));
+    c->w = calloc((size_t)nwalkers, sizeof(walker));
+    for (int64_t k = 0; k < nwalkers; ++k) walker_alloc(c, &c->w[k]);
+    seed_walkers(c);
+    set_move(c, MCI_SRRD_UNIFORM, 0, 1, NULL, 0.05); /* MCI ctor default: MoveType::All, DEFAULT_MRT2STEP 0.05 */
+    rebuild_protos(c);
+    *out = c;
+    return MCI_OK;
+}
+
+static void free_last_accus(orc_ctx *c)
+{
+    if (c->acc_last) {
+        for (int64_t i = 0; i < c->W * c->acc_last_nobs; ++i) accu_free(&c->acc_last[i]);
+        free(c->acc_last);
+        c->acc_last = NULL;
+    }
+}
+
+mci_status orc_destroy(orc_ctx *c)
+{
+    if (!c) return MCI_OK;
+    free_last_accus(c);
+    for (int64_t k = 0; k < c->W; ++k) {
+        walker *w = &c->w[k];
+        free(w->xold); free(w->xnew); free(w->changedIdx); free(w->steps); free(w->pold); free(w->pnew);
+    }
+    for (int i = 0; i < c->npdf; ++i) {
+        free(c->pdf[i].params); free(c->pdf[i].gw); free(c->pdf[i].gs); free(c->pdf[i].gmu);
+        free(c->pdf[i].gc); free(c->pdf[i].gb);
+    }
+    for (int i = 0; i < c->nobs; ++i) free(c->obs[i].params);
+    free(c->w); free(c->lb); free(c->ub); free(c->typeEnds); free(c->wseeds); free(c->wavg); free(c->werr);
+    free(c);
+    return MCI_OK;
+}
+
+mci_status orc_set_seed(orc_ctx *c, uint64_t seed)
+{
+    c->seed = seed;
+    free(c->wseeds);
+    c->wseeds = NULL;
+    seed_walkers(c);
+    return MCI_OK;
+}
+
+mci_status orc_set_walker_seeds(orc_ctx *c, const uint64_t *seeds, int64_t n)
+{
+    if (n != c->W) return fail(MCI_E_INVALID_ARGUMENT, "need one seed per walker");
+    free(c->wseeds);
+    c->wseeds = malloc(sizeof(uint64_t) * (size_t)n);
+    memcpy(c->wseeds, seeds, sizeof(uint64_t) * (size_t)n);
+    seed_walkers(c);
+    return MCI_OK;
+}
+
+mci_status orc_set_rng(orc_ctx *c, int k)
+{
+    if (k != MCI_RNG_MT19937_64 && k != MCI_RNG_PHILOX) return fail(MCI_E_DOMAIN, "unknown rng kind");
+    c->rng = k;
+    seed_walkers(c);
+    return MCI_OK;
+}
+
+mci_status orc_set_precision(orc_ctx *c, int p)
+{
+    if (p != MCI_PREC_FP64) return fail(MCI_E_INVALID_ARGUMENT, "the oracle restates the FP64 chain only");
+    c->prec = p;
+    return MCI_OK;
+}
+
+mci_status orc_set_coupling(orc_ctx *c, int m)
+{
+    if (m != MCI_COUPLING_INDEPENDENT && m != MCI_COUPLING_SHARED) return fail(MCI_E_DOMAIN, "unknown coupling");
+    c->coupling = m;
+    return MCI_OK;
+}
+
+mci_status orc_set_target_acceptance_rate(orc_ctx *c, double r) { c->target = r; return MCI_OK; }
+mci_status orc_set_nfind_mrt2_iterations(orc_ctx *c, int n) { c->nfind = n; return MCI_OK; }
+mci_status orc_set_ndecorrelation_steps(orc_ctx *c, int64_t n) { c->ndecorr = n; return MCI_OK; }
+
+/* ------------------------------------------------------------------------- */
+/* domain: DomainInterface.hpp:12-17, UnboundDomain.hpp:25-35,                 */
+/* OrthoPeriodicDomain.cpp:38-83                                              */
+/* ------------------------------------------------------------------------- */
+static const double DC_INF = (double)FLT_MAX;
+static const double DC_INFX2 = (double)FLT_MAX + (double)FLT_MAX;
+
+static void dom_apply_all(const orc_ctx *c, double *x)
+{
+    if (!c->periodic) return;
+    for (int i = 0; i < c->ndim; ++i) {
+        while (x[i] < c->lb[i]) x[i] = x[i] + (c->ub[i] - c->lb[i]);
+        while (x[i] > c->ub[i]) x[i] = x[i] - (c->ub[i] - c->lb[i]);
+    }
+}
+
+static void dom_apply_wlk(const orc_ctx *c, walker *w)
+{
+    if (!c->periodic) return;
+    for (int i = 0; i < w->nchanged; ++i) {
+        int j = w->changedIdx[i];
+        while (w->xnew[j] < c->lb[j]) w->xnew[j] = w->xnew[j] + (c->ub[j] - c->lb[j]);
+        while (w->xnew[j] > c->ub[j]) w->xnew[j] = w->xnew[j] - (c->ub[j] - c->lb[j]);
+    }
+}
+
+static void dom_scale(const orc_ctx *c, double *x)
+{
+    for (int i = 0; i < c->ndim; ++i) {
+        if (c->periodic) x[i] = c->lb[i] + x[i] * (c->ub[i] - c->lb[i]);
+        else x[i] = -DC_INF + x[i] * DC_INFX2;
+    }
+}
+
+static void dom_sizes(const orc_ctx *c, double *s)
+{
+    for (int i = 0; i < c->ndim; ++i) s[i] = c->periodic ? (c->ub[i] - c->lb[i]) : DC_INFX2;
+}
+
+static double dom_volume(const orc_ctx *c)
+{
+    if (!c->periodic) return 0.;
+    double v = 1.;
+    for (int i = 0; i < c->ndim; ++i) v = v * (c->ub[i] - c->lb[i]);
+    return v;
+}
+
+mci_status orc_reset_domain(orc_ctx *c)
+{
+    c->periodic = 0;
+    return MCI_OK;
+}
+
+mci_status orc_set_irange(orc_ctx *c, const double *lb, const double *ub)
+{
+    for (int i = 0; i < c->ndim; ++i)
+        if (ub[i] <= lb[i])
+            return fail(MCI_E_INVALID_ARGUMENT, "[OrthoPeriodicDomain::checkBounds] All upper bounds must be truly greater than their corresponding lower bounds.");
+    memcpy(c->lb, lb, sizeof(double) * (size_t)c->ndim);
+    memcpy(c->ub, ub, sizeof(double) * (size_t)c->ndim);
+    c->periodic = 1;
+    for (int64_t k = 0; k < c->W; ++k) dom_apply_all(c, c->w[k].xold);
+    return MCI_OK;
+}
+
+mci_status orc_domain_is_finite(orc_ctx *c, int *f) { *f = dom_volume(c) != 0.; return MCI_OK; }
+
+/* ------------------------------------------------------------------------- */
+/* positions and step sizes (MCIntegrator.cpp:556-619)                        */
+/* ------------------------------------------------------------------------- */
+mci_status orc_set_x(orc_ctx *c, const double *x, int64_t n)
+{
+    if (n != c->ndim && n != c->ndim * c->W) return fail(MCI_E_INVALID_ARGUMENT, "x must have ndim or nwalkers*ndim entries");
+    for (int64_t k = 0; k < c->W; ++k) {
+        memcpy(c->w[k].xold, (n == c->ndim) ? x : x + k * c->ndim, sizeof(double) * (size_t)c->ndim);
+        dom_apply_all(c, c->w[k].xold);
+    }
+    return MCI_OK;
+}
+
+mci_status orc_get_x(orc_ctx *c, double *x, int64_t n)
+{
+    if (n < c->ndim * c->W) return fail(MCI_E_INVALID_ARGUMENT, "output too small");
+    for (int64_t k = 0; k < c->W; ++k) memcpy(x + k * c->ndim, c->w[k].xold, sizeof(double) * (size_t)c->ndim);
+    return MCI_OK;
+}
+
+mci_status orc_center_x(orc_ctx *c) /* DomainInterface::getCenter DomainInterface.hpp:39-43 */
+{
+    for (int64_t k = 0; k < c->W; ++k) {
+        for (int i = 0; i < c->ndim; ++i) c->w[k].xold[i] = 0.5;
+        dom_scale(c, c->w[k].xold);
+    }
+    return MCI_OK;
+}
+
+static double trial_move(const orc_ctx *c, walker *w);
+
+mci_status orc_move_x(orc_ctx *c) /* MCI::moveX MCIntegrator.cpp:606-612 */
+{
+    for (int64_t k = 0; k < c->W; ++k) {
+        walker *w = &c->w[k];
+        memcpy(w->xnew, w->xold, sizeof(double) * (size_t)c->ndim);
+        step_begin(c, w);
+        trial_move(c, w);
+        step_end(c, w);
+        if (w->nchanged < c->ndim) dom_apply_wlk(c, w);
+        else dom_apply_all(c, w->xnew);
+        memcpy(w->xold, w->xnew, sizeof(double) * (size_t)c->ndim);
+    }
+    return MCI_OK;
+}
+
+mci_status orc_new_random_x(orc_ctx *c) /* MCI::newRandomX MCIntegrator.cpp:614-619 */
+{
+    for (int64_t k = 0; k < c->W; ++k) {
+        walker *w = &c->w[k];
+        step_begin(c, w);
+        for (int i = 0; i < c->ndim; ++i) w->xnew[i] = draw_unit(c, w);
+        step_end(c, w);
+        dom_scale(c, w->xnew);
+        memcpy(w->xold, w->xnew, sizeof(double) * (size_t)c->ndim);
+    }
+    return MCI_OK;
+}
+
+mci_status orc_set_mrt2step(orc_ctx *c, int idx, double step)
+{
+    for (int64_t k = 0; k < c->W; ++k) {
+        if (idx < 0) for (int i = 0; i < c->ntypes; ++i) c->w[k].steps[i] = step;
+        else if (idx < c->ntypes) c->w[k].steps[idx] = step;
+    }
+    return MCI_OK;
+}
+
+mci_status orc_set_mrt2steps(orc_ctx *c, const double *steps, int n)
+{
+    if (n != c->ntypes) return fail(MCI_E_INVALID_ARGUMENT, "need one step per step-size type");
+    for (int64_t k = 0; k < c->W; ++k) memcpy(c->w[k].steps, steps, sizeof(double) * (size_t)n);
+    return MCI_OK;
+}
+
+mci_status orc_get_mrt2step(orc_ctx *c, int idx, double *step)
+{
+    *step = (idx >= 0 && idx < c->ntypes) ? c->w[0].steps[idx] : 0.;
+    return MCI_OK;
+}
+
+mci_status orc_get_walker_mrt2steps(orc_ctx *c, double *steps, int64_t n)
+{
+    if (n < c->W * c->ntypes) return fail(MCI_E_INVALID_ARGUMENT, "output too small");
+    for (int64_t k = 0; k < c->W; ++k) memcpy(steps + k * c->ntypes, c->w[k].steps, sizeof(double) * (size_t)c->ntypes);
+    return MCI_OK;
+}
+
+mci_status orc_get_n_step_sizes(orc_ctx *c, int *n) { *n = c->ntypes; return MCI_OK; }
+
+mci_status orc_set_trial_move(orc_ctx *c, int srrd, int veclen, int ntypes, const int *te)
+{
+    return set_move(c, srrd, veclen, ntypes, te, 0.05);
+}
+
+mci_status orc_set_trial_move_type(orc_ctx *c, int mt)
+{
+    if (mt == MCI_MOVE_ALL) return set_move(c, MCI_SRRD_UNIFORM, 0, 1, NULL, 0.05);
+    if (mt == MCI_MOVE_VEC) return set_move(c, MCI_SRRD_UNIFORM, 1, 1, NULL, 0.05);
+    return fail(MCI_E_DOMAIN, "[createMoveDefault] Unhandled MoveType enumerator.");
+}
+
+mci_status orc_get_change_rate(orc_ctx *c, double *r)
+{
+    *r = (c->movetype == MCI_MOVE_ALL) ? 1. : 1. / c->nvecs;
+    return MCI_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* functor registration                                                       */
+/* ------------------------------------------------------------------------- */
+mci_status orc_add_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams)
+{
+    if (ndim != c->ndim)
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
+    if (c->npdf >= MAX_PDF) return fail(MCI_E_INVALID_ARGUMENT, "too many sampling functions");
+    pdf_desc p;
+    memset(&p, 0, sizeof p);
+    p.kind = kind;
+    p.ndim = ndim;
+    p.nparams = nparams;
+    if (nparams > 0) {
+        p.params = malloc(sizeof(double) * (size_t)nparams);
+        memcpy(p.params, params, sizeof(double) * (size_t)nparams);
+    }
+    switch (kind) {
+    case MCI_PDF_GAUSS_ISO: p.nproto = 1; break;
+    case MCI_PDF_GAUSS_DIAG: case MCI_PDF_EXP_DIAG:
+        if (nparams != 0 && nparams != ndim) { free(p.params); return fail(MCI_E_INVALID_ARGUMENT, "params must be empty or have ndim entries"); }
+        p.nproto = ndim;
+        break;
+    case MCI_PDF_GMM_ISO: {
+        if (nparams < 1) { free(p.params); return fail(MCI_E_INVALID_ARGUMENT, "GMM needs K"); }
+        int K = (int)params[0];
+        if (K < 1 || nparams != 1 + 2 * K + K * ndim) { free(p.params); return fail(MCI_E_INVALID_ARGUMENT, "GMM params must be [K, w[K], s[K], mu[K][ndim]]"); }
+        p.K = K;
+        p.nproto = K + 1;
+        p.gw = malloc(sizeof(double) * (size_t)K);
+        p.gs = malloc(sizeof(double) * (size_t)K);
+        p.gc = malloc(sizeof(double) * (size_t)K);
+        p.gb = malloc(sizeof(double) * (size_t)K);
+        p.gmu = malloc(sizeof(double) * (size_t)(K * ndim));
+        for (int k = 0; k < K; ++k) {
+            p.gw[k] = params[1 + k];
+            p.gs[k] = params[1 + K + k];
+            p.gc[k] = log(p.gw[k]) - (double)ndim * log(p.gs[k]);
+            p.gb[k] = 0.5 / (p.gs[k] * p.gs[k]);
+        }
+        memcpy(p.gmu, params + 1 + 2 * K, sizeof(double) * (size_t)(K * ndim));
+        break;
+    }
+    default:
+        free(p.params);
+        return fail(MCI_E_DOMAIN, "unknown sampling-function kind %d", kind);
+    }
+    c->pdf[c->npdf++] = p;
+    rebuild_protos(c);
+    return MCI_OK;
+}
+
+static void pdf_free(pdf_desc *p)
+{
+    free(p->params); free(p->gw); free(p->gs); free(p->gmu); free(p->gc); free(p->gb);
+}
+
+mci_status orc_pop_sampling_function(orc_ctx *c)
+{
+    if (c->npdf == 0) return fail(MCI_E_RUNTIME, "no sampling function to pop");
+    pdf_free(&c->pdf[--c->npdf]);
+    rebuild_protos(c);
+    return MCI_OK;
+}
+
+mci_status orc_clear_sampling_functions(orc_ctx *c)
+{
+    while (c->npdf > 0) pdf_free(&c->pdf[--c->npdf]);
+    rebuild_protos(c);
+    return MCI_OK;
+}
+
+mci_status orc_get_npdf(orc_ctx *c, int *n) { *n = c->npdf; return MCI_OK; }
+
+/* MCI::addObservable MCIntegrator.cpp:455-469 */
+mci_status orc_add_observable(orc_ctx *c, int kind, int ndim, const double *params, int nparams,
+                              int blocksize, int nskip, int flag_equil, int estim)
+{
+    if (blocksize < 0) blocksize = 0;
+    if (nskip < 1) nskip = 1;
+    if (ndim != c->ndim)
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Passed observable function's number of inputs is not equal to MCI's number of walkers.");
+    if (flag_equil && estim == MCI_ESTIM_NOOP)
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Requested automatic observable equilibration requires estimator with error calculation.");
+    if (estim < MCI_ESTIM_NOOP || estim > MCI_ESTIM_MJBLOCKER) return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator.");
+    int nobs = obs_nobs(kind, ndim);
+    if (nobs < 0) return fail(MCI_E_DOMAIN, "unknown observable kind %d", kind);
+    if (c->nobs >= MAX_OBS) return fail(MCI_E_INVALID_ARGUMENT, "too many observables");
+    obs_desc o;
+    memset(&o, 0, sizeof o);
+    o.kind = kind; o.ndim = ndim; o.nparams = nparams; o.nobs = nobs;
+    o.updateable = (kind == MCI_OBS_X2);
+    if (nparams > 0) { o.params = malloc(sizeof(double) * (size_t)nparams); memcpy(o.params, params, sizeof(double) * (size_t)nparams); }
+    o.blocksize = blocksize; o.nskip = nskip; o.flag_equil = flag_equil ? 1 : 0; o.estim = estim;
+    c->obs[c->nobs++] = o;
+    return MCI_OK;
+}
+
+mci_status orc_pop_observable(orc_ctx *c)
+{
+    if (c->nobs == 0) return fail(MCI_E_RUNTIME, "no observable to pop");
+    free(c->obs[--c->nobs].params);
+    return MCI_OK;
+}
+
+mci_status orc_clear_observables(orc_ctx *c)
+{
+    while (c->nobs > 0) free(c->obs[--c->nobs].params);
+    return MCI_OK;
+}
+
+mci_status orc_get_nobs(orc_ctx *c, int *n) { *n = c->nobs; return MCI_OK; }
+
+mci_status orc_get_nobsdim(orc_ctx *c, int *n)
+{
+    int s = 0;
+    for (int i = 0; i < c->nobs; ++i) s += c->obs[i].nobs;
+    *n = s;
+    return MCI_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Metropolis step                                                            */
+/* ------------------------------------------------------------------------- */
+/* SRRDAllMove::trialMove SRRDAllMove.hpp:67-80, SRRDVecMove::trialMove SRRDVecMove.hpp:75-96 */
+static double trial_move(const orc_ctx *c, walker *w)
+{
+    if (c->movetype == MCI_MOVE_ALL) {
+        int xidx = 0;
+        for (int t = 0; t < c->ntypes; ++t)
+            while (xidx < c->typeEnds[t]) {
+                w->xnew[xidx] = w->xnew[xidx] + w->steps[t] * draw_sym(c, w);
+                ++xidx;
+            }
+        w->nchanged = c->ndim;
+        return 1.;
+    }
+    const int vidx = draw_index(c, w, c->nvecs);
+    const int xidx = vidx * c->veclen;
+    int t = 0;
+    while (t < c->ntypes) {
+        if (xidx < c->typeEnds[t]) break;
+        ++t;
+    }
+    for (int i = 0; i < c->veclen; ++i) {
+        w->xnew[xidx + i] = w->xnew[xidx + i] + w->steps[t] * draw_sym(c, w);
+        w->changedIdx[i] = xidx + i;
+    }
+    w->nchanged = c->veclen;
+    return 1.;
+}
+
+/* MCI::initializeSampling MCIntegrator.cpp:252-271 (+ WalkerState::initialize WalkerState.hpp:41-49,
+ * ProtoFunctionInterface::initializeProtoValues ProtoFunctionInterface.cpp:40-44) */
+static void init_sampling(orc_ctx *c, walker *w)
+{
+    w->acc = 0;
+    w->rej = 0;
+    memcpy(w->xnew, w->xold, sizeof(double) * (size_t)c->ndim);
+    w->nchanged = c->ndim;
+    for (int i = 0; i < c->ndim; ++i) w->changedIdx[i] = i;
+    w->accepted = 1;
+    int off = 0;
+    for (int p = 0; p < c->npdf; ++p) {
+        pdf_proto(&c->pdf[p], w->xold, w->pnew + off);
+        memcpy(w->pold + off, w->pnew + off, sizeof(double) * (size_t)c->pdf[p].nproto);
+        off += c->pdf[p].nproto;
+    }
+}
+
+/* MCI::doStepMRT2 MCIntegrator.cpp:326-360 (+ SamplingFunctionContainer::computeAcceptance
+ * SamplingFunctionContainer.cpp:41-48) */
+static void do_step_mrt2(orc_ctx *c, walker *w)
+{
+    step_begin(c, w);
+    const double moveAcc = trial_move(c, w);
+    if (w->nchanged < c->ndim) dom_apply_wlk(c, w);
+    else dom_apply_all(c, w->xnew);
+    double pdfAcc = 1.;
+    int off = 0;
+    for (int p = 0; p < c->npdf; ++p) {
+        pdfAcc = pdfAcc * pdf_compute_acceptance(&c->pdf[p], w, w->pold + off, w->pnew + off);
+        off += c->pdf[p].nproto;
+    }
+    const double u = draw_unit(c, w);
+    w->accepted = (u <= pdfAcc * moveAcc);
+    if (w->accepted) ++w->acc;
+    else ++w->rej;
+    step_end(c, w);
+    size_t nb = sizeof(double) * (size_t)c->ndim, pb = sizeof(double) * (size_t)c->nproto_total;
+    if (w->accepted) { memcpy(w->pold, w->pnew, pb); memcpy(w->xold, w->xnew, nb); }
+    else { memcpy(w->pnew, w->pold, pb); memcpy(w->xnew, w->xold, nb); }
+}
+
+/* MCI::doStepRandom MCIntegrator.cpp:362-376 (no sampling function) */
+static void do_step_random(orc_ctx *c, walker *w)
+{
+    step_begin(c, w);
+    for (int i = 0; i < c->ndim; ++i) w->xnew[i] = draw_unit(c, w);
+    step_end(c, w);
+    dom_scale(c, w->xnew);
+    w->nchanged = c->ndim;
+    w->accepted = 1;
+    ++w->acc;
+    memcpy(w->xold, w->xnew, sizeof(double) * (size_t)c->ndim);
+}
+
+/* MCI::sample(npoints) MCIntegrator.cpp:273-288 */
+static void sample_plain(orc_ctx *c, walker *w, int64_t n)
+{
+    init_sampling(c, w);
+    for (int64_t i = 0; i < n; ++i) {
+        if (c->npdf > 0) do_step_mrt2(c, w);
+        else do_step_random(c, w);
+    }
+}
+
+/* MCI::sample(npoints, container, flagMC) MCIntegrator.cpp:290-321 (PDF-observation bookkeeping is
+ * out of scope: no catalogue observable depends on the PDF) */
+static mci_status sample_obs(orc_ctx *c, walker *w, int64_t n, accu *accs, const obs_desc *obs, int nobs, int record)
+{
+    init_sampling(c, w);
+    for (int i = 0; i < nobs; ++i) accu_reset(&accs[i]);
+    for (int64_t s = 0; s < n; ++s) {
+        if (c->npdf > 0) do_step_mrt2(c, w);
+        else do_step_random(c, w);
+        for (int i = 0; i < nobs; ++i) accu_accumulate(&accs[i], &obs[i], w);
+        if (record && c->tracing && c->trace_n < c->trace_cap) {
+            memcpy(c->trace_x + c->trace_n * c->ndim, w->xold, sizeof(double) * (size_t)c->ndim);
+            c->trace_acc[c->trace_n] = (uint8_t)w->accepted;
+            ++c->trace_n;
+        }
+    }
+    for (int i = 0; i < nobs; ++i) TRY(accu_finalize(&accs[i]));
+    return MCI_OK;
+}
+
+static double walker_rate(const walker *w) /* MCI::getAcceptanceRate MCIntegrator.cpp:587-592 */
+{
+    return (w->acc > 0) ? (double)w->acc / ((double)w->acc + (double)w->rej) : 0.;
+}
+
+/* ------------------------------------------------------------------------- */
+/* calibration: MCI::findMRT2Step MCIntegrator.cpp:87-168                    */
+/* walkers [w0, w1) share one loop when nranks > 1 (USE_MPI semantics)        */
+/* ------------------------------------------------------------------------- */
+static void find_mrt2(orc_ctx *c, int64_t w0, int64_t w1, int64_t nranks)
+{
+    const int ndim = c->ndim;
+    const int64_t MIN_STAT = (int64_t)fmax(100., sqrt(40000. * ndim) / (double)nranks);
+    const int MIN_CONS = 5;
+    const double TOL = 0.05;
+    const double SMALLEST = (double)FLT_MIN;
+    double *sizes = malloc(sizeof(double) * (size_t)ndim);
+    int *sidx = malloc(sizeof(int) * (size_t)ndim);
+    double *rates = malloc(sizeof(double) * (size_t)(w1 - w0));
+    dom_sizes(c, sizes);
+    for (int i = 0; i < ndim; ++i) {
+        int t = 0;
+        while (t < c->ntypes && !(i < c->typeEnds[t])) ++t;
+        sidx[i] = t;
+    }
+    int cons = 0, counter = 0;
+    const int N = c->nfind;
+    while ((N < 0 && cons < MIN_CONS) || counter < N) {
+        for (int64_t k = w0; k < w1; ++k) {
+            sample_plain(c, &c->w[k], MIN_STAT);
+            rates[k - w0] = walker_rate(&c->w[k]);
+        }
+        double rate = (nranks > 1) ? orc_det_sum(rates, w1 - w0) / (double)nranks : rates[0];
+        if (fabs(rate - c->target) < TOL) ++cons;
+        else cons = 0;
+        const double fact = fmin(2., fmax(0.5, rate / c->target));
+        for (int64_t k = w0; k < w1; ++k) {
+            double *st = c->w[k].steps;
+            for (int t = 0; t < c->ntypes; ++t) st[t] = st[t] * fact;
+            for (int i = 0; i < ndim; ++i)
+                if (st[sidx[i]] > 0.5 * sizes[i]) st[sidx[i]] = 0.5 * sizes[i];
+            for (int t = 0; t < c->ntypes; ++t)
+                if (st[t] < SMALLEST) st[t] = SMALLEST;
+        }
+        ++counter;
+        if (N < 0 && counter >= abs(N)) break;
+    }
+    free(rates); free(sidx); free(sizes);
+}
+
+/* MPIReduceAvgErr MCIntegrator.cpp:21-34 with the deterministic sum */
+static void reduce_avg_err(int nd, int64_t R, const double *wa, const double *we, double *avg, double *err)
+{
+    double *tmp = malloc(sizeof(double) * (size_t)R);
+    for (int j = 0; j < nd; ++j) {
+        for (int64_t r = 0; r < R; ++r) tmp[r] = wa[r * nd + j];
+        avg[j] = orc_det_sum(tmp, R) / (double)R;
+        for (int64_t r = 0; r < R; ++r) tmp[r] = we[r * nd + j] * we[r * nd + j];
+        err[j] = sqrt(orc_det_sum(tmp, R)) / (double)R;
+    }
+    free(tmp);
+}
+
+/* MCI::initialDecorrelation MCIntegrator.cpp:171-247 */
+static mci_status decorrelate(orc_ctx *c, int64_t w0, int64_t w1, int64_t nranks)
+{
+    if (c->ndecorr > 0) {
+        for (int64_t k = w0; k < w1; ++k) sample_plain(c, &c->w[k], c->ndecorr);
+        return MCI_OK;
+    }
+    if (c->ndecorr == 0) return MCI_OK;
+    /* equilibration container: clones of flag_equil observables, Full accumulator, Correlated estimator */
+    obs_desc eo[MAX_OBS];
+    int neo = 0, nd = 0;
+    for (int i = 0; i < c->nobs; ++i)
+        if (c->obs[i].flag_equil) {
+            eo[neo] = c->obs[i];
+            eo[neo].blocksize = 1; eo[neo].nskip = 1; eo[neo].estim = MCI_ESTIM_CORRELATED;
+            nd += eo[neo].nobs;
+            ++neo;
+        }
+    const int64_t MIN_NMC = (int64_t)fmax(100., sqrt(40000. * c->ndim) / (double)nranks);
+    const int64_t R = w1 - w0;
+    accu *acc = calloc((size_t)(R * (neo > 0 ? neo : 1)), sizeof(accu));
+    for (int64_t k = 0; k < R; ++k)
+        for (int i = 0; i < neo; ++i) {
+            accu_make(&acc[k * neo + i], &eo[i], c->ndim);
+            TRY(accu_allocate(&acc[k * neo + i], MIN_NMC));
+        }
+    size_t nb = (size_t)(nd > 0 ? nd : 1);
+    double *wa = calloc(nb * (size_t)R, sizeof(double)), *we = calloc(nb * (size_t)R, sizeof(double));
+    double *oa = calloc(nb, sizeof(double)), *oe = calloc(nb, sizeof(double));
+    double *na = calloc(nb, sizeof(double)), *ne = calloc(nb, sizeof(double));
+    mci_status st = MCI_OK;
+
+#define EQ_SAMPLE_ESTIMATE(outa, oute) \
+    for (int64_t k = 0; k < R && st == MCI_OK; ++k) { \
+        st = sample_obs(c, &c->w[w0 + k], MIN_NMC, &acc[k * neo], eo, neo, 0); \
+        int off = 0; \
+        for (int i = 0; i < neo && st == MCI_OK; ++i) { \
+            st = estimate_any(MCI_ESTIM_CORRELATED, acc[k * neo + i].nstore, acc[k * neo + i].nobs, acc[k * neo + i].data, \
+                              wa + k * nd + off, we + k * nd + off); \
+            off += eo[i].nobs; \
+        } \
+    } \
+    if (st == MCI_OK) { \
+        if (nranks > 1) reduce_avg_err(nd, R, wa, we, outa, oute); \
+        else { memcpy(outa, wa, sizeof(double) * (size_t)nd); memcpy(oute, we, sizeof(double) * (size_t)nd); } \
+    }
+
+    EQ_SAMPLE_ESTIMATE(oa, oe)
+    int flag_loop = 1;
+    int64_t count = 0;
+    while (flag_loop && st == MCI_OK) {
+        flag_loop = 0;
+        for (int64_t k = 0; k < R; ++k) st = sample_obs(c, &c->w[w0 + k], MIN_NMC, &acc[k * neo], eo, neo, 0);
+        if (st != MCI_OK) break;
+        count += MIN_NMC;
+        if (count >= llabs(c->ndecorr)) { ++c->nwarn; break; }
+        /* estimate the samples just taken (the accumulators hold them) */
+        for (int64_t k = 0; k < R && st == MCI_OK; ++k) {
+            int off = 0;
+            for (int i = 0; i < neo && st == MCI_OK; ++i) {
+                st = estimate_any(MCI_ESTIM_CORRELATED, acc[k * neo + i].nstore, acc[k * neo + i].nobs, acc[k * neo + i].data,
+                                  wa + k * nd + off, we + k * nd + off);
+                off += eo[i].nobs;
+            }
+        }
+        if (st != MCI_OK) break;
+        if (nranks > 1) reduce_avg_err(nd, R, wa, we, na, ne);
+        else { memcpy(na, wa, sizeof(double) * (size_t)nd); memcpy(ne, we, sizeof(double) * (size_t)nd); }
+        for (int i = 0; i < nd; ++i)
+            if (fabs(oa[i] - na[i]) > 2 * sqrt(oe[i] * oe[i] + ne[i] * ne[i])) { flag_loop = 1; break; }
+        memcpy(oa, na, sizeof(double) * (size_t)nd);
+        memcpy(oe, ne, sizeof(double) * (size_t)nd);
+    }
+#undef EQ_SAMPLE_ESTIMATE
+    for (int64_t i = 0; i < R * neo; ++i) accu_free(&acc[i]);
+    free(acc); free(wa); free(we); free(oa); free(oe); free(na); free(ne);
+    return st;
+}
+
+mci_status orc_trace_enable(orc_ctx *c, double *xbuf, uint8_t *accbuf, int64_t cap)
+{
+    c->trace_x = xbuf;
+    c->trace_acc = accbuf;
+    c->trace_cap = cap;
+    c->trace_n = 0;
+    c->tracing = (xbuf != NULL && accbuf != NULL && cap > 0);
+    return MCI_OK;
+}
+
+/* MCI::integrate MCIntegrator.cpp:43-82 + MPIMCI::integrate MPIMCI.cpp:68-93 */
+mci_status orc_integrate(orc_ctx *c, int64_t nmc, double *average, double *error, int do_find, int do_decorr)
+{
+    int finite = dom_volume(c) != 0.;
+    if (c->npdf == 0 && !finite)
+        return fail(MCI_E_DOMAIN, "[MCI::integrate] Integrating over an infinite domain requires a sampling function.");
+    c->nwarn = 0;
+    const int64_t W = c->W;
+    const int shared = (c->coupling == MCI_COUPLING_SHARED);
+    if (c->npdf > 0) {
+        if (shared) {
+            if (do_find && c->ntypes > 0) find_mrt2(c, 0, W, W);
+            if (do_decorr) TRY(decorrelate(c, 0, W, W));
+        } else {
+            for (int64_t k = 0; k < W; ++k) {
+                if (do_find && c->ntypes > 0) find_mrt2(c, k, k + 1, 1);
+                if (do_decorr) TRY(decorrelate(c, k, k + 1, 1));
+            }
+        }
+    }
+    if (nmc <= 0) return MCI_OK;
+    int nd;
+    orc_get_nobsdim(c, &nd);
+    free_last_accus(c);
+    c->acc_last = calloc((size_t)(W * (c->nobs > 0 ? c->nobs : 1)), sizeof(accu));
+    c->acc_last_nobs = c->nobs;
+    for (int64_t k = 0; k < W; ++k)
+        for (int i = 0; i < c->nobs; ++i) {
+            accu_make(&c->acc_last[k * c->nobs + i], &c->obs[i], c->ndim);
+            TRY(accu_allocate(&c->acc_last[k * c->nobs + i], nmc));
+        }
+    free(c->wavg); free(c->werr);
+    c->wavg = calloc((size_t)(W * (nd > 0 ? nd : 1)), sizeof(double));
+    c->werr = calloc((size_t)(W * (nd > 0 ? nd : 1)), sizeof(double));
+    c->nobsdim_last = nd;
+    c->trace_n = 0;
+    for (int64_t k = 0; k < W; ++k)
+        TRY(sample_obs(c, &c->w[k], nmc, &c->acc_last[k * c->nobs], c->obs, c->nobs, k == 0));
+    /* ObservableContainer::estimate ObservableContainer.cpp:104-111 */
+    for (int64_t k = 0; k < W; ++k) {
+        int off = 0;
+        for (int i = 0; i < c->nobs; ++i) {
+            accu *a = &c->acc_last[k * c->nobs + i];
+            TRY(estimate_any(c->obs[i].estim, a->nstore, a->nobs, a->data, c->wavg + k * nd + off, c->werr + k * nd + off));
+            off += a->nobs;
+        }
+        if (c->npdf == 0) { /* volume scaling MCIntegrator.cpp:71-77 */
+            const double vol = dom_volume(c);
+            for (int j = 0; j < nd; ++j) { c->wavg[k * nd + j] = c->wavg[k * nd + j] * vol; c->werr[k * nd + j] = c->werr[k * nd + j] * vol; }
+        }
+    }
+    if (W == 1) {
+        memcpy(average, c->wavg, sizeof(double) * (size_t)nd);
+        memcpy(error, c->werr, sizeof(double) * (size_t)nd);
+    } else {
+        reduce_avg_err(nd, W, c->wavg, c->werr, average, error);
+    }
+    return MCI_OK;
+}
+
+mci_status orc_get_walker_results(orc_ctx *c, double *a, double *e, int64_t n)
+{
+    if (n < c->W * c->nobsdim_last) return fail(MCI_E_INVALID_ARGUMENT, "output too small");
+    memcpy(a, c->wavg, sizeof(double) * (size_t)(c->W * c->nobsdim_last));
+    memcpy(e, c->werr, sizeof(double) * (size_t)(c->W * c->nobsdim_last));
+    return MCI_OK;
+}
+
+mci_status orc_get_acceptance(orc_ctx *c, int64_t *acc, int64_t *rej, double *rate)
+{
+    int64_t a = 0, r = 0;
+    double *rates = malloc(sizeof(double) * (size_t)c->W);
+    for (int64_t k = 0; k < c->W; ++k) { a += c->w[k].acc; r += c->w[k].rej; rates[k] = walker_rate(&c->w[k]); }
+    *acc = a;
+    *rej = r;
+    *rate = (c->W == 1) ? rates[0] : orc_det_sum(rates, c->W) / (double)c->W;
+    free(rates);
+    return MCI_OK;
+}
+
+mci_status orc_get_walker_acceptance(orc_ctx *c, int64_t *acc, int64_t *rej, int64_t n)
+{
+    if (n < c->W) return fail(MCI_E_INVALID_ARGUMENT, "output too small");
+    for (int64_t k = 0; k < c->W; ++k) { acc[k] = c->w[k].acc; rej[k] = c->w[k].rej; }
+    return MCI_OK;
+}
+
+mci_status orc_get_warnings(orc_ctx *c, int64_t *n) { *n = c->nwarn; return MCI_OK; }
+
+mci_status orc_get_accu_data(orc_ctx *c, int obs, int64_t walker, double *data, int64_t n, int64_t *nstore)
+{
+    if (!c->acc_last || obs < 0 || obs >= c->acc_last_nobs || walker < 0 || walker >= c->W)
+        return fail(MCI_E_INVALID_ARGUMENT, "no such accumulator");
+    accu *a = &c->acc_last[walker * c->acc_last_nobs + obs];
+    *nstore = a->nstore;
+    int64_t m = a->nstore * a->nobs;
+    if (n < m) return fail(MCI_E_INVALID_ARGUMENT, "output too small");
+    memcpy(data, a->data, sizeof(double) * (size_t)m);
+    return MCI_OK;
+}

@@ -1,0 +1,472 @@
+// ref_harness.cpp — drives the REFERENCE library (built from its own sources by
+// oracle/Makefile into oracle/_ref/) to write golden vectors and to time the
+// reference CPU path.  TEST INFRASTRUCTURE ONLY; never part of the product.
+//
+// It uses only the reference's public plugin API (mci::MCI, SamplingFunctionInterface,
+// ObservableFunctionInterface, estimator functions, accumulator classes).  The
+// sampling functions and observables below are this project's functor catalogue
+// (include/mci_amd.h) written as reference plugins with the same expressions, so
+// the reference computes exactly what the device functors compute.
+//
+// Input: a line-oriented script on stdin.  Output: one JSON object per result line;
+// doubles are printed as C99 hex-floats ("%a") so they round-trip bit-exactly.
+#include "mci/MCIntegrator.hpp"
+#include "mci/Estimators.hpp"
+#include "mci/BlockAccumulator.hpp"
+#include "mci/FullAccumulator.hpp"
+#include "mci/SimpleAccumulator.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace
+{
+// functor kinds, as in include/mci_amd.h
+enum { PDF_GAUSS_ISO = 1, PDF_GAUSS_DIAG = 2, PDF_EXP_DIAG = 3, PDF_GMM_ISO = 4 };
+enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7 };
+
+std::string hx(double v)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "\"%a\"", v);
+    return b;
+}
+
+std::string hxv(const double *v, int n)
+{
+    std::string s = "[";
+    for (int i = 0; i < n; ++i) { if (i) s += ","; s += hx(v[i]); }
+    return s + "]";
+}
+
+// ---- sampling functions ----
+class GaussIso final: public mci::SamplingFunctionInterface
+{
+protected:
+    mci::SamplingFunctionInterface *_clone() const final { return new GaussIso(_ndim); }
+public:
+    explicit GaussIso(int nd): mci::SamplingFunctionInterface(nd, 1) {}
+    void protoFunction(const double in[], double pv[]) final
+    {
+        double s = in[0]*in[0];
+        for (int i = 1; i < _ndim; ++i) { s = s + in[i]*in[i]; }
+        pv[0] = s;
+    }
+    double samplingFunction(const double pv[]) const final { return exp(-pv[0]); }
+    double acceptanceFunction(const double po[], const double pn[]) const final { return exp(-pn[0] + po[0]); }
+};
+
+class DiagPDF final: public mci::SamplingFunctionInterface
+{
+    bool _abs;
+    std::vector<double> _a;
+protected:
+    mci::SamplingFunctionInterface *_clone() const final { return new DiagPDF(_ndim, _abs, _a); }
+public:
+    DiagPDF(int nd, bool absform, std::vector<double> a): mci::SamplingFunctionInterface(nd, nd), _abs(absform), _a(std::move(a))
+    {
+        if (_a.empty()) { _a.assign(nd, 1.); }
+    }
+    double f(double x, int i) const { return _abs ? fabs(x)*_a[i] : (x*x)*_a[i]; }
+    void protoFunction(const double in[], double pv[]) final
+    {
+        for (int i = 0; i < _ndim; ++i) { pv[i] = f(in[i], i); }
+    }
+    double samplingFunction(const double pv[]) const final { return exp(-std::accumulate(pv, pv + _nproto, 0.)); }
+    double acceptanceFunction(const double po[], const double pn[]) const final
+    {
+        double e = std::accumulate(po, po + _nproto, 0.);
+        e -= std::accumulate(pn, pn + _nproto, 0.);
+        return exp(e);
+    }
+    double updatedAcceptance(const mci::WalkerState &wlk, const double po[], double pn[]) final
+    {
+        double e = 0.;
+        for (int i = 0; i < wlk.nchanged; ++i) {
+            const int j = wlk.changedIdx[i];
+            pn[j] = f(wlk.xnew[j], j);
+            e += pn[j] - po[j];
+        }
+        return exp(-e);
+    }
+};
+
+class GmmIso final: public mci::SamplingFunctionInterface
+{
+    int _K;
+    std::vector<double> _w, _s, _mu, _c, _b;
+protected:
+    mci::SamplingFunctionInterface *_clone() const final { return new GmmIso(_ndim, _K, _w, _s, _mu); }
+public:
+    GmmIso(int nd, int K, std::vector<double> w, std::vector<double> s, std::vector<double> mu):
+            mci::SamplingFunctionInterface(nd, K + 1), _K(K), _w(std::move(w)), _s(std::move(s)), _mu(std::move(mu))
+    {
+        for (int k = 0; k < _K; ++k) {
+            _c.push_back(log(_w[k]) - static_cast<double>(_ndim)*log(_s[k]));
+            _b.push_back(0.5/(_s[k]*_s[k]));
+        }
+    }
+    double logp(const double r2[]) const
+    {
+        double m = -std::numeric_limits<double>::infinity();
+        for (int k = 0; k < _K; ++k) { const double t = _c[k] - _b[k]*r2[k]; if (t > m) { m = t; } }
+        double s = 0.;
+        for (int k = 0; k < _K; ++k) { s = s + exp((_c[k] - _b[k]*r2[k]) - m); }
+        return m + log(s);
+    }
+    void protoFunction(const double in[], double pv[]) final
+    {
+        for (int k = 0; k < _K; ++k) {
+            double s = 0.;
+            for (int i = 0; i < _ndim; ++i) { const double d = in[i] - _mu[k*_ndim + i]; s = s + d*d; }
+            pv[k] = s;
+        }
+        pv[_K] = logp(pv);
+    }
+    double samplingFunction(const double pv[]) const final { return exp(pv[_K]); }
+    double acceptanceFunction(const double po[], const double pn[]) const final { return exp(pn[_K] - po[_K]); }
+    double updatedAcceptance(const mci::WalkerState &wlk, const double po[], double pn[]) final
+    {
+        for (int k = 0; k < _K; ++k) {
+            double dr = 0.;
+            for (int i = 0; i < wlk.nchanged; ++i) {
+                const int j = wlk.changedIdx[i];
+                const double dn = wlk.xnew[j] - _mu[k*_ndim + j], dd = wlk.xold[j] - _mu[k*_ndim + j];
+                dr = dr + (dn*dn - dd*dd);
+            }
+            pn[k] = po[k] + dr;
+        }
+        pn[_K] = logp(pn);
+        return exp(pn[_K] - po[_K]);
+    }
+};
+
+// ---- observables ----
+class CatObs final: public mci::ObservableFunctionInterface
+{
+    int _kind;
+    std::vector<double> _p;
+    static int nobsOf(int kind, int nd) { return (kind == OBS_X2 || kind == OBS_X) ? nd : 1; }
+protected:
+    mci::ObservableFunctionInterface *_clone() const final { return new CatObs(_ndim, _kind, _p); }
+public:
+    CatObs(int nd, int kind, std::vector<double> p):
+            mci::ObservableFunctionInterface(nd, nobsOf(kind, nd), kind == OBS_X2), _kind(kind), _p(std::move(p)) {}
+    double par(size_t i, double d) const { return _p.size() > i ? _p[i] : d; }
+    void observableFunction(const double in[], double out[]) final
+    {
+        switch (_kind) {
+        case OBS_X2: for (int i = 0; i < _ndim; ++i) { out[i] = in[i]*in[i]; } break;
+        case OBS_X2_SUM: { double s = 0.; for (int i = 0; i < _ndim; ++i) { s = s + in[i]*in[i]; } out[0] = s/par(0, 1.); break; }
+        case OBS_X: for (int i = 0; i < _ndim; ++i) { out[i] = in[i]; } break;
+        case OBS_X_SUM: { double s = 0.; for (int i = 0; i < _ndim; ++i) { s = s + in[i]; } out[0] = s; break; }
+        case OBS_CONST: out[0] = par(0, 1.); break;
+        case OBS_EXP_X2: { double s = 0.; for (int i = 0; i < _ndim; ++i) { s = s + in[i]*in[i]; } out[0] = exp(-(s/par(0, _ndim))); break; }
+        case OBS_INDICATOR: out[0] = (in[static_cast<int>(par(0, 0.))] > 0.) ? 1. : 0.; break;
+        default: throw std::invalid_argument("unknown observable kind");
+        }
+    }
+    void updatedObservable(const double in[], const int /*nchanged*/, const bool flags[], double out[]) final
+    {
+        if (_kind == OBS_X2) {
+            for (int i = 0; i < _ndim; ++i) { if (flags[i]) { out[i] = in[i]*in[i]; } }
+        }
+        else { observableFunction(in, out); }
+    }
+};
+
+std::unique_ptr<mci::SamplingFunctionInterface> makePDF(int nd, int kind, const std::vector<double> &p)
+{
+    switch (kind) {
+    case PDF_GAUSS_ISO: return std::unique_ptr<mci::SamplingFunctionInterface>(new GaussIso(nd));
+    case PDF_GAUSS_DIAG: return std::unique_ptr<mci::SamplingFunctionInterface>(new DiagPDF(nd, false, p));
+    case PDF_EXP_DIAG: return std::unique_ptr<mci::SamplingFunctionInterface>(new DiagPDF(nd, true, p));
+    case PDF_GMM_ISO: {
+        const int K = static_cast<int>(p[0]);
+        std::vector<double> w(p.begin() + 1, p.begin() + 1 + K), s(p.begin() + 1 + K, p.begin() + 1 + 2*K);
+        std::vector<double> mu(p.begin() + 1 + 2*K, p.end());
+        return std::unique_ptr<mci::SamplingFunctionInterface>(new GmmIso(nd, K, w, s, mu));
+    }
+    default: throw std::invalid_argument("unknown pdf kind");
+    }
+}
+
+mci::EstimatorType estimOf(int e)
+{
+    switch (e) {
+    case 0: return mci::EstimatorType::Noop;
+    case 1: return mci::EstimatorType::Uncorrelated;
+    case 2: return mci::EstimatorType::Correlated;
+    case 3: return mci::EstimatorType::FCBlocker;
+    default: return mci::EstimatorType::MJBlocker;
+    }
+}
+
+std::vector<double> readDoubles(std::istringstream &is, int n)
+{
+    std::vector<double> v(n);
+    for (int i = 0; i < n; ++i) {
+        std::string t;
+        is >> t;
+        v[i] = std::strtod(t.c_str(), nullptr); // accepts decimal and hex-float
+    }
+    return v;
+}
+
+// glibc rand()-driven Metropolis walk with the acceptance rule of a 1-D Gaussian
+// orbital product (the kind of data test/ut1/main.cpp feeds to the estimators).
+void genWalk(int nmc, int nd, double step, double changeProb, std::vector<double> &x, std::vector<int> &acc,
+             std::vector<int> &nch, std::vector<int> &cidx)
+{
+    auto wf = [nd](const double *p) { double s = 0.; for (int i = 0; i < nd; ++i) { s += p[i]*p[i]; } return exp(-s); };
+    x.assign(static_cast<size_t>(nmc)*nd, 0.);
+    acc.assign(nmc, 0);
+    nch.assign(nmc, 0);
+    cidx.assign(static_cast<size_t>(nmc)*nd, 0);
+    for (int j = 0; j < nd; ++j) { x[j] = rand()*(1.0/RAND_MAX) - 0.5; }
+    acc[0] = 1;
+    nch[0] = nd;
+    for (int j = 0; j < nd; ++j) { cidx[j] = j; }
+    for (int i = 1; i < nmc; ++i) {
+        const double *po = &x[static_cast<size_t>(i - 1)*nd];
+        double *pn = &x[static_cast<size_t>(i)*nd];
+        const double ov = wf(po);
+        int n = 0;
+        for (int j = 0; j < nd; ++j) {
+            bool mv = true;
+            if (changeProb < 1.) { mv = (rand()*(1.0/RAND_MAX) <= changeProb); }
+            if (mv) { pn[j] = po[j] + 2.*step*(rand()*(1.0/RAND_MAX) - 0.5); cidx[static_cast<size_t>(i)*nd + n] = j; ++n; }
+            else { pn[j] = po[j]; }
+        }
+        nch[i] = n;
+        const double nv = wf(pn);
+        bool a;
+        if (ov == 0) { a = nv != 0; }
+        else {
+            const double thr = (nv*nv)/(ov*ov);
+            a = (thr >= 1.) ? true : (rand()*(1.0/RAND_MAX) <= thr);
+        }
+        acc[i] = a ? 1 : 0;
+        if (!a) { std::copy(po, po + nd, pn); }
+    }
+}
+
+struct Trace
+{
+    bool on = false;
+    int64_t ncalls = 0;
+    std::vector<double> x;
+    std::vector<double> rate;
+};
+} // namespace
+
+int main()
+{
+    std::unique_ptr<mci::MCI> mci;
+    int ndim = 0;
+    Trace tr;
+    std::string line;
+    while (std::getline(std::cin, line)) {
+        std::istringstream is(line);
+        std::string cmd;
+        if (!(is >> cmd) || cmd[0] == '#') { continue; }
+        try {
+            if (cmd == "create") {
+                is >> ndim;
+                mci.reset(new mci::MCI(ndim));
+            }
+            else if (cmd == "seed") { uint64_t s; is >> s; mci->setSeed(s); }
+            else if (cmd == "target") { double r; is >> r; mci->setTargetAcceptanceRate(r); }
+            else if (cmd == "nfind") { int n; is >> n; mci->setNfindMRT2Iterations(n); }
+            else if (cmd == "ndecorr") { int64_t n; is >> n; mci->setNdecorrelationSteps(n); }
+            else if (cmd == "setx") { auto v = readDoubles(is, ndim); mci->setX(v.data()); }
+            else if (cmd == "step") { double s; is >> s; mci->setMRT2Step(s); }
+            else if (cmd == "stepi") { int i; double s; is >> i >> s; mci->setMRT2Step(i, s); }
+            else if (cmd == "irange") { auto lb = readDoubles(is, ndim); auto ub = readDoubles(is, ndim); mci->setIRange(lb.data(), ub.data()); }
+            else if (cmd == "resetdomain") { mci->resetDomain(); }
+            else if (cmd == "move") {
+                int srrd, veclen, ntypes;
+                is >> srrd >> veclen >> ntypes;
+                std::vector<int> te(ntypes > 0 ? ntypes : 1);
+                for (int i = 0; i < ntypes; ++i) { is >> te[i]; }
+                mci->setTrialMove(static_cast<mci::SRRDType>(srrd), veclen, ntypes, ntypes > 1 ? te.data() : nullptr);
+            }
+            else if (cmd == "movetype") { int t; is >> t; mci->setTrialMove(static_cast<mci::MoveType>(t)); }
+            else if (cmd == "pdf") { int kind, np; is >> kind >> np; auto p = readDoubles(is, np); mci->addSamplingFunction(makePDF(ndim, kind, p)); }
+            else if (cmd == "poppdf") { mci->popSamplingFunction(); }
+            else if (cmd == "obs") {
+                int kind, np, bs, nskip, eq, est;
+                is >> kind >> np;
+                auto p = readDoubles(is, np);
+                is >> bs >> nskip >> eq >> est;
+                mci->addObservable(std::unique_ptr<mci::ObservableFunctionInterface>(new CatObs(ndim, kind, p)), bs, nskip, eq != 0, estimOf(est));
+            }
+            else if (cmd == "popobs") { mci->popObservable(); }
+            else if (cmd == "clearobs") { mci->clearObservables(); }
+            else if (cmd == "movex") { mci->moveX(); }
+            else if (cmd == "newrandomx") { mci->newRandomX(); }
+            else if (cmd == "centerx") { mci->centerX(); }
+            else if (cmd == "trace") {
+                // record x and the running acceptance rate at every callback (MCIntegrator.cpp:265,347)
+                tr = Trace();
+                tr.on = true;
+                mci->setCallback([&tr](const mci::MCI &m) {
+                    ++tr.ncalls;
+                    for (int i = 0; i < m.getNDim(); ++i) { tr.x.push_back(m.getX(i)); }
+                    tr.rate.push_back(m.getAcceptanceRate());
+                });
+            }
+            else if (cmd == "untrace") { mci->clearCallback(); tr.on = false; }
+            else if (cmd == "integrate") {
+                int64_t nmc; int df, dd;
+                is >> nmc >> df >> dd;
+                const int nd = mci->getNObsDim();
+                std::vector<double> avg(nd > 0 ? nd : 1), err(nd > 0 ? nd : 1);
+                tr.ncalls = 0; tr.x.clear(); tr.rate.clear();
+                auto t0 = std::chrono::steady_clock::now();
+                mci->integrate(nmc, avg.data(), err.data(), df != 0, dd != 0);
+                auto t1 = std::chrono::steady_clock::now();
+                std::vector<double> steps(mci->getTrialMove().getNStepSizes());
+                for (size_t i = 0; i < steps.size(); ++i) { steps[i] = mci->getMRT2Step(static_cast<int>(i)); }
+                std::cout << "{\"cmd\":\"integrate\",\"avg\":" << hxv(avg.data(), nd) << ",\"err\":" << hxv(err.data(), nd)
+                          << ",\"rate\":" << hx(mci->getAcceptanceRate()) << ",\"steps\":" << hxv(steps.data(), static_cast<int>(steps.size()))
+                          << ",\"x\":" << hxv(mci->getX(), ndim)
+                          << ",\"seconds\":" << std::chrono::duration<double>(t1 - t0).count();
+                if (tr.on) {
+                    std::cout << ",\"trace_x\":" << hxv(tr.x.data(), static_cast<int>(tr.x.size()))
+                              << ",\"trace_rate\":" << hxv(tr.rate.data(), static_cast<int>(tr.rate.size()));
+                }
+                std::cout << "}" << std::endl;
+            }
+            else if (cmd == "getx") { std::cout << "{\"cmd\":\"getx\",\"x\":" << hxv(mci->getX(), ndim) << "}" << std::endl; }
+            else if (cmd == "estimate") {
+                // estimate EST N ND v0 v1 ... (N*ND values)
+                int est; int64_t n; int nd;
+                is >> est >> n >> nd;
+                auto x = readDoubles(is, static_cast<int>(n*nd));
+                std::vector<double> a(nd), e(nd);
+                auto fn = mci::createEstimator(estimOf(est));
+                fn(n, nd, x.data(), a.data(), e.data());
+                std::cout << "{\"cmd\":\"estimate\",\"avg\":" << hxv(a.data(), nd) << ",\"err\":" << hxv(e.data(), nd) << "}" << std::endl;
+            }
+            else if (cmd == "walkdata") {
+                // walkdata SEED NMC ND STEP CHANGEPROB : emit a rand()-driven walk and the reference's
+                // estimator and accumulator outputs on it (the checks of test/ut1/main.cpp)
+                unsigned seed; int nmc, nd; double step, cp;
+                is >> seed >> nmc >> nd >> step >> cp;
+                srand(seed);
+                std::vector<double> x; std::vector<int> acc, nch, cidx;
+                genWalk(nmc, nd, step, cp, x, acc, nch, cidx);
+                std::cout << "{\"cmd\":\"walkdata\",\"x\":" << hxv(x.data(), static_cast<int>(x.size())) << ",\"acc\":[";
+                for (int i = 0; i < nmc; ++i) { std::cout << (i ? "," : "") << acc[i]; }
+                std::cout << "],\"nch\":[";
+                for (int i = 0; i < nmc; ++i) { std::cout << (i ? "," : "") << nch[i]; }
+                std::cout << "],\"cidx\":[";
+                for (size_t i = 0; i < cidx.size(); ++i) { std::cout << (i ? "," : "") << cidx[i]; }
+                std::cout << "]";
+                std::vector<double> a(nd), e(nd);
+                const char *names[] = {"noop", "uncorr", "corr", "fc", "mj"};
+                for (int est = 1; est <= 4; ++est) {
+                    mci::createEstimator(estimOf(est))(nmc, nd, x.data(), a.data(), e.data());
+                    std::cout << ",\"" << names[est] << "_nd\":{\"avg\":" << hxv(a.data(), nd) << ",\"err\":" << hxv(e.data(), nd) << "}";
+                }
+                std::vector<double> col(nmc);
+                for (int j = 0; j < nd; ++j) {
+                    for (int i = 0; i < nmc; ++i) { col[i] = x[static_cast<size_t>(i)*nd + j]; }
+                    double a1, e1;
+                    mci::OneDimUncorrelatedEstimator(nmc, col.data(), a1, e1);
+                    std::cout << ",\"uncorr_1d_" << j << "\":[" << hx(a1) << "," << hx(e1) << "]";
+                    mci::OneDimBlockEstimator(nmc, col.data(), 2048, a1, e1);
+                    std::cout << ",\"block_1d_" << j << "\":[" << hx(a1) << "," << hx(e1) << "]";
+                    mci::OneDimFCBlockerEstimator(nmc, col.data(), a1, e1);
+                    std::cout << ",\"fc_1d_" << j << "\":[" << hx(a1) << "," << hx(e1) << "]";
+                    mci::MJBlockerEstimator(nmc, 1, col.data(), &a1, &e1);
+                    std::cout << ",\"mj_1d_" << j << "\":[" << hx(a1) << "," << hx(e1) << "]";
+                }
+                mci::MultiDimBlockEstimator(nmc, nd, x.data(), 2048, a.data(), e.data());
+                std::cout << ",\"block_nd\":{\"avg\":" << hxv(a.data(), nd) << ",\"err\":" << hxv(e.data(), nd) << "}";
+                // accumulators fed with the walk (test/ut1/main.cpp:108-128 feeding scheme)
+                CatObs xobs(nd, OBS_X, {});
+                CatObs x2obs(nd, OBS_X2, {});
+                struct A { std::string name; std::unique_ptr<mci::AccumulatorInterface> acc; };
+                std::vector<A> accs;
+                accs.push_back({"simple1", std::unique_ptr<mci::AccumulatorInterface>(new mci::SimpleAccumulator(xobs, 1))});
+                accs.push_back({"simple2", std::unique_ptr<mci::AccumulatorInterface>(new mci::SimpleAccumulator(xobs, 2))});
+                accs.push_back({"block16", std::unique_ptr<mci::AccumulatorInterface>(new mci::BlockAccumulator(xobs, 1, 16))});
+                accs.push_back({"block8s2", std::unique_ptr<mci::AccumulatorInterface>(new mci::BlockAccumulator(xobs, 2, 8))});
+                accs.push_back({"full1", std::unique_ptr<mci::AccumulatorInterface>(new mci::FullAccumulator(xobs, 1))});
+                accs.push_back({"full2", std::unique_ptr<mci::AccumulatorInterface>(new mci::FullAccumulator(xobs, 2))});
+                accs.push_back({"x2full3", std::unique_ptr<mci::AccumulatorInterface>(new mci::FullAccumulator(x2obs, 3))});
+                accs.push_back({"x2block4s3", std::unique_ptr<mci::AccumulatorInterface>(new mci::BlockAccumulator(x2obs, 3, 4))});
+                for (auto &ac : accs) {
+                    const int64_t steps = (ac.name == "x2block4s3") ? static_cast<int64_t>(nmc) - 4 : nmc; // NAccu divisible by 4
+                    ac.acc->allocate(steps);
+                    mci::WalkerState wlk(nd, true);
+                    for (int64_t i = 0; i < steps; ++i) {
+                        std::copy(&x[static_cast<size_t>(i)*nd], &x[static_cast<size_t>(i + 1)*nd], wlk.xnew);
+                        wlk.nchanged = nch[i];
+                        std::copy(&cidx[static_cast<size_t>(i)*nd], &cidx[static_cast<size_t>(i + 1)*nd], wlk.changedIdx);
+                        wlk.accepted = acc[i] != 0;
+                        ac.acc->accumulate(wlk);
+                    }
+                    ac.acc->finalize();
+                    std::cout << ",\"accu_" << ac.name << "\":" << hxv(ac.acc->getData(), static_cast<int>(ac.acc->getNData()));
+                }
+                std::cout << "}" << std::endl;
+            }
+            else if (cmd == "bench") {
+                // bench W NMC : W independent walkers (one MCI each, seeds seed0+w), full integrate each,
+                // timed; prints walker-steps/s of the sampling phase and the total
+                int64_t W, nmc; uint64_t seed0;
+                is >> W >> nmc >> seed0;
+                // the configuration lines after "bench" are replayed for every walker
+                std::vector<std::string> setup;
+                std::string l2;
+                while (std::getline(std::cin, l2) && l2 != "endbench") { setup.push_back(l2); }
+                double tot = 0., avgsum = 0.;
+                int64_t steps_total = 0;
+                for (int64_t w = 0; w < W; ++w) {
+                    std::unique_ptr<mci::MCI> m;
+                    int nd0 = 0;
+                    for (auto &sl : setup) {
+                        std::istringstream ss(sl);
+                        std::string c2; ss >> c2;
+                        if (c2 == "create") { ss >> nd0; m.reset(new mci::MCI(nd0)); m->setSeed(seed0 + static_cast<uint64_t>(w)); }
+                        else if (c2 == "pdf") { int kind, np; ss >> kind >> np; auto p = readDoubles(ss, np); m->addSamplingFunction(makePDF(nd0, kind, p)); }
+                        else if (c2 == "obs") {
+                            int kind, np, bs, nskip, eq, est; ss >> kind >> np; auto p = readDoubles(ss, np); ss >> bs >> nskip >> eq >> est;
+                            m->addObservable(std::unique_ptr<mci::ObservableFunctionInterface>(new CatObs(nd0, kind, p)), bs, nskip, eq != 0, estimOf(est));
+                        }
+                        else if (c2 == "move") { int srrd, vl, nt; ss >> srrd >> vl >> nt; m->setTrialMove(static_cast<mci::SRRDType>(srrd), vl, nt, nullptr); }
+                        else if (c2 == "nfind") { int n; ss >> n; m->setNfindMRT2Iterations(n); }
+                        else if (c2 == "ndecorr") { int64_t n; ss >> n; m->setNdecorrelationSteps(n); }
+                    }
+                    std::vector<double> a(m->getNObsDim()), e(m->getNObsDim());
+                    auto t0 = std::chrono::steady_clock::now();
+                    m->integrate(nmc, a.data(), e.data(), true, true);
+                    auto t1 = std::chrono::steady_clock::now();
+                    tot += std::chrono::duration<double>(t1 - t0).count();
+                    steps_total += nmc;
+                    avgsum += a.empty() ? 0. : a[0];
+                }
+                std::cout << "{\"cmd\":\"bench\",\"walkers\":" << W << ",\"seconds\":" << tot << ",\"walker_steps\":" << steps_total
+                          << ",\"avg0_mean\":" << avgsum/W << "}" << std::endl;
+            }
+            else { std::cout << "{\"cmd\":\"error\",\"what\":\"unknown command " << cmd << "\"}" << std::endl; }
+        }
+        catch (std::invalid_argument &ex) { std::cout << "{\"cmd\":\"exception\",\"type\":\"invalid_argument\",\"what\":\"" << ex.what() << "\"}" << std::endl; }
+        catch (std::domain_error &ex) { std::cout << "{\"cmd\":\"exception\",\"type\":\"domain_error\",\"what\":\"" << ex.what() << "\"}" << std::endl; }
+        catch (std::runtime_error &ex) { std::cout << "{\"cmd\":\"exception\",\"type\":\"runtime_error\",\"what\":\"" << ex.what() << "\"}" << std::endl; }
+    }
+    return 0;
+}
