@@ -286,12 +286,12 @@ int main()
                 mci.reset(new mci::MCI(ndim));
             }
             else if (cmd == "seed") { uint64_t s; is >> s; mci->setSeed(s); }
-            else if (cmd == "target") { double r; is >> r; mci->setTargetAcceptanceRate(r); }
+            else if (cmd == "target") { mci->setTargetAcceptanceRate(readDoubles(is, 1)[0]); }
             else if (cmd == "nfind") { int n; is >> n; mci->setNfindMRT2Iterations(n); }
             else if (cmd == "ndecorr") { int64_t n; is >> n; mci->setNdecorrelationSteps(n); }
             else if (cmd == "setx") { auto v = readDoubles(is, ndim); mci->setX(v.data()); }
-            else if (cmd == "step") { double s; is >> s; mci->setMRT2Step(s); }
-            else if (cmd == "stepi") { int i; double s; is >> i >> s; mci->setMRT2Step(i, s); }
+            else if (cmd == "step") { mci->setMRT2Step(readDoubles(is, 1)[0]); }
+            else if (cmd == "stepi") { int i; is >> i; mci->setMRT2Step(i, readDoubles(is, 1)[0]); }
             else if (cmd == "irange") { auto lb = readDoubles(is, ndim); auto ub = readDoubles(is, ndim); mci->setIRange(lb.data(), ub.data()); }
             else if (cmd == "resetdomain") { mci->resetDomain(); }
             else if (cmd == "move") {

@@ -1,0 +1,213 @@
+"""Writes the golden vectors under tests/golden/ by running the REFERENCE itself.
+
+The reference is built from its own sources by oracle/Makefile into
+oracle/_ref/mci_ref (-O2 -ffp-contract=off).  This script feeds it scenario
+scripts (oracle/ref_harness.cpp) and stores inputs and outputs as JSON with
+hex-float doubles.  Run from the repository root:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Nothing here is imported by the tests; they read the JSON files only.
+"""
+import base64
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from mcitest import (ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, OBS_CONST, OBS_EXP_X2,  # noqa: E402
+                     OBS_INDICATOR, OBS_X, OBS_X2, OBS_X2_SUM, OBS_X_SUM, PDF_EXP_DIAG, PDF_GAUSS_DIAG,
+                     PDF_GAUSS_ISO, PDF_GMM_ISO, REF_BIN, ROOT, fmt, ref_run, script_setup)
+
+FLAGS = "g++ 11.4 -std=c++14 -O2 -ffp-contract=off -include stdexcept (oracle/Makefile target _ref/mci_ref)"
+
+
+def b64(a):
+    """little-endian float64 bytes, base64 (bit-exact and compact)"""
+    return base64.b64encode(np.ascontiguousarray(a, dtype="<f8").tobytes()).decode()
+
+
+def probe(cmd):
+    out = subprocess.run([os.path.join(ROOT, "oracle", "build", "stdrng_probe")], input=cmd + "\n",
+                         capture_output=True, text=True, check=True).stdout.split()
+    return out[:-1]
+
+
+def sigmas(n):
+    """sigma_i ~ U(0.5, 2): std::uniform_real_distribution<double>(0.5, 2) on std::mt19937_64(1337)."""
+    return [float.fromhex(v) for v in probe(f"real 1337 {n} 0.5 2.0")]
+
+
+def cfg2_a(n):
+    return [0.5 / (s * s) for s in sigmas(n)]
+
+
+def scenarios():
+    S = []
+    # configuration 1 of the benchmark set: d=3 Gaussian, XYZSquared blocks of 62500, MJBlocker, seed 1337
+    S.append(dict(name="cfg1", cfg=dict(ndim=3, seed=1337, pdfs=[(PDF_GAUSS_ISO, [])],
+                                        obs=[(OBS_X2, [], 62500, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=1000000, find=1, decorr=1)]))
+    # the sequence of test/ut2/main.cpp: bad start without/with calibration, continuation, PBC range
+    S.append(dict(name="ut2_sequence", cfg=dict(ndim=3, seed=5649871, pdfs=[(PDF_GAUSS_ISO, [])],
+                                                obs=[(OBS_X2_SUM, [3.0], 1, 1, 1, ESTIM_CORR)]),
+                  calls=[dict(nmc=10000, find=0, decorr=0, setx=[5., -5., 10.]),
+                         dict(nmc=10000, find=1, decorr=1, setx=[5., -5., 10.]),
+                         dict(nmc=10000, find=0, decorr=0),
+                         dict(nmc=10000, find=0, decorr=0, irange=([-5.] * 3, [5.] * 3))]))
+    # test/ut3/main.cpp: a 1-d and a 3-d observable together
+    S.append(dict(name="ut3_sequence", cfg=dict(ndim=3, seed=5649871, pdfs=[(PDF_GAUSS_ISO, [])],
+                                                obs=[(OBS_X2_SUM, [3.0], 1, 1, 1, ESTIM_CORR), (OBS_X2, [], 1, 1, 1, ESTIM_CORR)]),
+                  calls=[dict(nmc=10000, find=0, decorr=0, setx=[5., -5., 10.]),
+                         dict(nmc=10000, find=1, decorr=1, setx=[5., -5., 10.]),
+                         dict(nmc=10000, find=0, decorr=0)]))
+    # test/ut4/main.cpp: fixed calibration/decorrelation, power-of-two NMC, blocks and nskip
+    S.append(dict(name="ut4_fixed", cfg=dict(ndim=3, seed=1337, nfind=20, ndecorr=2000, pdfs=[(PDF_GAUSS_ISO, [])],
+                                             obs=[(OBS_X2_SUM, [3.0], 1, 1, 1, ESTIM_CORR),
+                                                  (OBS_X2_SUM, [3.0], 16, 1, 1, ESTIM_UNCORR),
+                                                  (OBS_X2_SUM, [3.0], 8, 2, 0, ESTIM_CORR)], x0=[5., -5., 10.]),
+                  calls=[dict(nmc=16384, find=1, decorr=1), dict(nmc=16384, find=0, decorr=0)]))
+    # test/ut5/main.cpp style: vector moves with types and nskip
+    for veclen, ntypes, te in [(1, 1, None), (1, 2, [1, 3]), (3, 1, None), (0, 2, [1, 3])]:
+        nskip = 3 if veclen == 1 else 1
+        S.append(dict(name=f"ut5_vec{veclen}_types{ntypes}",
+                      cfg=dict(ndim=3, seed=1337, pdfs=[(PDF_GAUSS_DIAG, [])], move=(0, veclen, ntypes, te),
+                               obs=[(OBS_X2_SUM, [3.0], 1, nskip, 1, ESTIM_CORR), (OBS_X2, [], 1, nskip, 1, ESTIM_CORR)]),
+                      calls=[dict(nmc=8192 * nskip, find=1, decorr=0)]))
+    # configuration 2 shape, one walker: d=16 anisotropic Gaussian, single-coordinate moves,
+    # 1000 decorrelation steps, 64 blocks of 160, MJBlocker
+    S.append(dict(name="cfg2_walker", cfg=dict(ndim=16, seed=1337, ndecorr=1000, move=(0, 1, 1, None),
+                                               pdfs=[(PDF_GAUSS_DIAG, cfg2_a(16))],
+                                               obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_X2_SUM, [], 160, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=10240, find=1, decorr=1)]))
+    # configuration 3 shape, one walker (FP64): d=64, x_i^2 and exp(-|x|^2/64)
+    S.append(dict(name="cfg3_walker", cfg=dict(ndim=64, seed=4242, ndecorr=1000, move=(0, 1, 1, None),
+                                               pdfs=[(PDF_GAUSS_DIAG, cfg2_a(64))],
+                                               obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_EXP_X2, [64.0], 160, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=10240, find=1, decorr=1)]))
+    # remaining catalogue kinds, periodic domain, mixture target, Simple/Noop, FCBlocker, Uncorrelated
+    S.append(dict(name="catalogue_exp", cfg=dict(ndim=4, seed=7, nfind=20, ndecorr=2000, move=(0, 2, 2, [2, 4]),
+                                                 pdfs=[(PDF_EXP_DIAG, [1., 2., 0.5, 1.])],
+                                                 obs=[(OBS_X, [], 16, 2, 0, ESTIM_UNCORR), (OBS_X_SUM, [], 0, 1, 0, ESTIM_NOOP),
+                                                      (OBS_EXP_X2, [], 8, 1, 1, ESTIM_CORR), (OBS_CONST, [1.3], 1, 1, 0, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=4096, find=1, decorr=1)]))
+    S.append(dict(name="catalogue_gmm_pbc", cfg=dict(ndim=2, seed=99, irange=([-1., -2.], [1., 3.]), move=(0, 1, 1, None),
+                                                     pdfs=[(PDF_GMM_ISO, [2, 0.3, 0.7, 0.5, 1.2, 0.2, 0.1, -0.5, 1.0])],
+                                                     obs=[(OBS_X2, [], 64, 1, 1, ESTIM_MJ), (OBS_INDICATOR, [1], 1, 1, 1, ESTIM_FC)]),
+                  calls=[dict(nmc=64 * 128, find=1, decorr=1)]))
+    S.append(dict(name="two_pdfs_allmove", cfg=dict(ndim=3, seed=31337, pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0]), (PDF_EXP_DIAG, [])],
+                                                    obs=[(OBS_X2, [], 32, 1, 1, ESTIM_CORR)]),
+                  calls=[dict(nmc=32 * 256, find=1, decorr=1)]))
+    # no sampling function: uniform random sampling in a periodic box with volume scaling
+    S.append(dict(name="nopdf_volume", cfg=dict(ndim=3, seed=5649871, irange=([-5.] * 3, [5.] * 3),
+                                                obs=[(OBS_X2_SUM, [3.0], 1, 1, 0, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=10000, find=0, decorr=0)]))
+    return S
+
+
+def run_scenario(s):
+    L = script_setup(s["cfg"])
+    for call in s["calls"]:
+        if "irange" in call:
+            lb, ub = call["irange"]
+            L.append("irange " + " ".join(fmt(v) for v in lb) + " " + " ".join(fmt(v) for v in ub))
+        if "setx" in call:
+            L.append("setx " + " ".join(fmt(v) for v in call["setx"]))
+        L.append(f"integrate {call['nmc']} {call['find']} {call['decorr']}")
+    recs = ref_run(L)
+    for r in recs:
+        r.pop("seconds", None)
+    return recs
+
+
+def trace_scenarios():
+    """Step-by-step traces of the main sampling (no calibration, no decorrelation)."""
+    T = []
+    for name, cfg, nmc in [
+        ("trace_all3", dict(ndim=3, seed=1337, pdfs=[(PDF_GAUSS_ISO, [])], obs=[(OBS_X2, [], 1, 1, 0, ESTIM_UNCORR)],
+                            x0=[0.3, -0.2, 0.1], step=1.1), 256),
+        ("trace_vec4", dict(ndim=4, seed=2024, pdfs=[(PDF_GAUSS_DIAG, [0.5, 1., 2., 4.])], move=(0, 1, 1, None),
+                            obs=[(OBS_X2, [], 1, 1, 0, ESTIM_UNCORR)], x0=[1., -1., 0.5, 0.], step=0.9), 256),
+        ("trace_types5", dict(ndim=6, seed=77, pdfs=[(PDF_EXP_DIAG, [])], move=(0, 2, 3, [2, 4, 6]),
+                              obs=[(OBS_X, [], 1, 1, 0, ESTIM_UNCORR)], step=0.7), 256),
+    ]:
+        L = script_setup(cfg) + ["trace", f"integrate {nmc} 0 0"]
+        r = ref_run(L)[-1]
+        r.pop("seconds", None)
+        T.append(dict(name=name, cfg=cfg, nmc=nmc, result=r))
+    return T
+
+
+def walker_scenarios():
+    """Several independent reference MCIs with their own seeds (one per GPU walker)."""
+    seeds = [1337, 11, 12345, 987654321, 2 ** 40 + 3, 5649871, 42, 7]
+    cfg = dict(ndim=16, ndecorr=1000, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, cfg2_a(16))],
+               obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_X2_SUM, [], 160, 1, 1, ESTIM_MJ)])
+    out = []
+    for s in seeds:
+        c = dict(cfg, seed=s)
+        r = run_scenario(dict(cfg=c, calls=[dict(nmc=10240, find=1, decorr=1)]))[-1]
+        out.append(r)
+    return dict(cfg=cfg, seeds=seeds, nmc=10240, results=out)
+
+
+def estimator_sets():
+    rng = np.random.default_rng(20240607)
+    sets = []
+    for n, nd, kind in [(2, 1, "iid"), (4, 2, "iid"), (64, 1, "ar"), (64, 17, "ar"), (128, 3, "ar"), (1000, 2, "ar"),
+                        (1024, 1, "ar"), (2048, 3, "ar"), (50, 1, "iid"), (51, 3, "ar"), (16, 3, "iid")]:
+        if kind == "iid":
+            x = rng.normal(size=(n, nd))
+        else:
+            x = np.zeros((n, nd))
+            e = rng.normal(size=(n, nd))
+            for i in range(1, n):
+                x[i] = 0.9 * x[i - 1] + e[i]
+            x += 2.0
+        ests = [ESTIM_UNCORR, ESTIM_NOOP]
+        if n >= 50:
+            ests.append(ESTIM_FC)
+        if n & (n - 1) == 0:
+            ests.append(ESTIM_MJ)
+        ests.append(ESTIM_CORR if (n >= 50 or n & (n - 1) == 0) else ESTIM_UNCORR)
+        res = {}
+        for est in ests:
+            L = [f"estimate {est} {n} {nd} " + " ".join(fmt(v) for v in x.ravel())]
+            r = ref_run(L)[-1]
+            res[str(est)] = dict(avg=r["avg"], err=r["err"])
+        sets.append(dict(n=n, ndim=nd, x_b64=b64(x.ravel()), results=res))
+    return sets
+
+
+def main():
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -C oracle")
+    meta = dict(flags=FLAGS, generator="tests/golden/make_golden.py")
+    chains = [dict(name=s["name"], cfg=s["cfg"], calls=s["calls"], results=run_scenario(s)) for s in scenarios()]
+    with open(os.path.join(HERE, "chains.json"), "w") as f:
+        json.dump(dict(meta=meta, scenarios=chains), f, indent=0)
+    with open(os.path.join(HERE, "traces.json"), "w") as f:
+        json.dump(dict(meta=meta, traces=trace_scenarios()), f, indent=0)
+    with open(os.path.join(HERE, "walkers.json"), "w") as f:
+        json.dump(dict(meta=meta, **walker_scenarios()), f, indent=0)
+    with open(os.path.join(HERE, "estimators.json"), "w") as f:
+        json.dump(dict(meta=meta, sets=estimator_sets()), f, indent=0)
+    walk = ref_run(["walkdata 1337 4096 2 2.0 0.5"])[-1]
+    walk["x_b64"] = b64([float.fromhex(v) for v in walk.pop("x")])
+    for k in [k for k in walk if k.startswith("accu_")]:
+        walk[k + "_b64"] = b64([float.fromhex(v) for v in walk.pop(k)])
+    with open(os.path.join(HERE, "walkdata.json"), "w") as f:
+        json.dump(dict(meta=dict(meta, args="seed 1337, nmc 4096, ndim 2, step 2.0, changeProb 0.5"), walk=walk), f)
+    with open(os.path.join(HERE, "sigmas.json"), "w") as f:
+        json.dump(dict(meta=dict(meta, draw="std::uniform_real_distribution<double>(0.5,2) on std::mt19937_64(1337)"),
+                       sigma16=[fmt(v) for v in sigmas(16)], sigma64=[fmt(v) for v in sigmas(64)]), f, indent=0)
+    for fn in sorted(os.listdir(HERE)):
+        print(fn, os.path.getsize(os.path.join(HERE, fn)))
+
+
+if __name__ == "__main__":
+    main()
