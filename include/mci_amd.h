@@ -174,6 +174,11 @@ mci_status mci_integrate(mci_ctx *ctx, int64_t nmc, double *average, double *err
                          int do_find_mrt2step, int do_decorrelation);
 /* per-walker estimates of the last integrate, layout [nwalkers][nobsdim] */
 mci_status mci_get_walker_results(mci_ctx *ctx, double *average, double *error, int64_t n);
+/* finalised accumulator data of observable `obs` after the last integrate, layout
+ * [nwalkers][nstore][nobs] (AccumulatorInterface::getData AccumulatorInterface.hpp:81; the
+ * device-side counterpart of MCI::storeObservablesOnFile MCIntegrator.cpp:495-509 is reading the
+ * Full accumulator's trace).  data == NULL only reports nstore. */
+mci_status mci_get_observable_data(mci_ctx *ctx, int obs, double *data, int64_t n, int64_t *nstore);
 /* MCI::getAcceptanceRate src/MCIntegrator.cpp:587-592: counters of the last sample() call,
  * summed over walkers; rate = mean over walkers of acc/(acc+rej) */
 mci_status mci_get_acceptance(mci_ctx *ctx, int64_t *accepted, int64_t *rejected, double *rate);

@@ -1,0 +1,14 @@
+"""Constants of include/mci_amd.h (enums mirror include/mci/Factories.hpp of the reference)."""
+OK, E_INVALID_ARGUMENT, E_DOMAIN, E_RUNTIME, E_DEVICE = 0, 1, 2, 3, 4
+
+# EstimatorType (Factories.hpp:52-59)
+ESTIM_NOOP, ESTIM_UNCORRELATED, ESTIM_CORRELATED, ESTIM_FCBLOCKER, ESTIM_MJBLOCKER = 0, 1, 2, 3, 4
+# MoveType (Factories.hpp:108-113) and SRRDType (Factories.hpp:119-133)
+MOVE_ALL, MOVE_VEC = 0, 1
+SRRD_UNIFORM = 0
+# device functor catalogue
+PDF_GAUSS_ISO, PDF_GAUSS_DIAG, PDF_EXP_DIAG, PDF_GMM_ISO = 1, 2, 3, 4
+OBS_X2, OBS_X2_SUM, OBS_X, OBS_X_SUM, OBS_CONST, OBS_EXP_X2, OBS_INDICATOR = 1, 2, 3, 4, 5, 6, 7
+RNG_MT19937_64, RNG_PHILOX = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
+COUPLING_INDEPENDENT, COUPLING_SHARED = 0, 1

@@ -1,0 +1,1167 @@
+// mci_api.cpp — C-ABI of the MI355X Metropolis integrator (include/mci_amd.h) and the host
+// orchestration of MCI::integrate (src/MCIntegrator.cpp:43-82): calibration, decorrelation,
+// sampling, estimation and the cross-walker combine run as HIP kernels on the context's
+// stream; the host only steers the loops whose exit depends on a statistic.
+#include "mci_kernels.hpp"
+#include "mci_fast.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace mcid;
+
+namespace
+{
+thread_local std::string g_err;
+
+mci_status fail(mci_status code, const char *fmt, ...)
+{
+    char buf[600];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) { return fail(MCI_E_DEVICE, "%s: %s", #expr, hipGetErrorString(_e)); }     \
+    } while (0)
+#define TRY(expr)                                                                                        \
+    do {                                                                                                 \
+        mci_status _s = (expr);                                                                          \
+        if (_s != MCI_OK) { return _s; }                                                                 \
+    } while (0)
+#define CHECKCTX(c)                                                                                      \
+    do {                                                                                                 \
+        if (!(c)) { return fail(MCI_E_INVALID_ARGUMENT, "null context"); }                               \
+    } while (0)
+
+// device buffer that only grows
+template <class T>
+struct DBuf
+{
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    ~DBuf() { if (p) { (void)hipFree(p); } }
+    hipError_t ensure(size_t m)
+    {
+        if (m <= n && p) { return hipSuccess; }
+        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(m, 1) * sizeof(T));
+        if (e == hipSuccess) { n = m; }
+        return e;
+    }
+};
+
+uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// seed of walker w when only MCI::setSeed(seed) was given: walker 0 keeps it (so one walker
+// reproduces one reference MCI seeded the same way), the others get splitmix64 draws
+uint64_t walker_seed(uint64_t seed, int64_t w)
+{
+    if (w == 0) { return seed; }
+    uint64_t s = (seed ^ 0x6A09E667F3BCC909ULL) + (uint64_t)w * 0x9E3779B97F4A7C15ULL;
+    return splitmix64(s);
+}
+
+// the deterministic walker sum used on the device (chunks of 256, pairwise tree, chunks in order)
+double det_sum(const double *v, int64_t n)
+{
+    double total = 0.0;
+    bool first = true;
+    for (int64_t c0 = 0; c0 < n; c0 += 256) {
+        double buf[256];
+        for (int i = 0; i < 256; ++i) { buf[i] = (c0 + i < n) ? v[c0 + i] : 0.0; }
+        for (int s = 128; s > 0; s >>= 1) {
+            for (int i = 0; i < s; ++i) { buf[i] = buf[i] + buf[i + s]; }
+        }
+        if (first) { total = buf[0]; first = false; }
+        else { total = total + buf[0]; }
+    }
+    return total;
+}
+
+int obs_nobs(int kind, int ndim)
+{
+    switch (kind) {
+    case MCI_OBS_X2: case MCI_OBS_X: return ndim;
+    case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR: return 1;
+    }
+    return -1;
+}
+
+bool is_pow2(int64_t n) { return n > 0 && (((uint64_t)n & ((uint64_t)n - 1)) == 0); }
+
+struct PdfHost
+{
+    int kind = 0, nproto = 0, K = 0;
+    std::vector<double> params;
+    DBuf<double> dev; // a[ndim] | gc[K], gb[K], gmu[K*ndim]
+};
+
+struct ObsHost
+{
+    int kind = 0, nobs = 0, blocksize = 1, nskip = 1, equil = 0, estim = 0;
+    std::vector<double> params;
+};
+
+const char *kVersion = "mcintegratorplusplus_amd 0.1 (gfx950)";
+} // namespace
+
+struct mci_ctx
+{
+    int ndim = 0;
+    int64_t W = 0;
+    int rng = MCI_RNG_MT19937_64, prec = MCI_PREC_FP64, coupling = MCI_COUPLING_INDEPENDENT;
+    uint64_t seed = 5489u;
+    std::vector<uint64_t> wseeds;
+    bool reseed = true;
+    double target = 0.5;
+    int nfind = -50;
+    int64_t ndecorr = -10000;
+    bool periodic = false;
+    std::vector<double> lb, ub;
+    int movetype = MCI_MOVE_ALL, srrd = MCI_SRRD_UNIFORM, veclen = 0, nvecs = 0, ntypes = 1;
+    std::vector<int> typeEnds;
+    std::vector<PdfHost> pdfs;
+    std::vector<ObsHost> obs;
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+
+    DBuf<double> x, xn, pold, pnew, steps, lbd, ubd, obsval, dimsizes, wavg, werr, chunks_a, chunks_e, res, rates, fact;
+    DBuf<double> oa, oe;  // decorrelation: previous estimates (independent walkers)
+    DBuf<uint64_t> mt, ctr, seeds;
+    DBuf<int32_t> mtp;
+    DBuf<int64_t> acc, rej;
+    DBuf<uint8_t> active;
+    std::vector<DBuf<double>> accdata, scratch;
+    std::vector<double> h_wavg, h_werr;
+    int nobsdim_last = 0;
+    std::vector<int64_t> nstore_last; // rows of each observable's data after the last integrate
+    double timings[5] = {0, 0, 0, 0, 0};
+    FastState fast;
+};
+
+namespace
+{
+int nproto_total(const mci_ctx *c)
+{
+    int n = 0;
+    for (auto &p : c->pdfs) { n += p.nproto; }
+    return n;
+}
+
+int nobsdim(const mci_ctx *c)
+{
+    int n = 0;
+    for (auto &o : c->obs) { n += o.nobs; }
+    return n;
+}
+
+mci_status set_current(mci_ctx *c)
+{
+    HIPCHK(hipSetDevice(c->device));
+    return MCI_OK;
+}
+
+mci_status fill_rows(mci_ctx *c, double *dev, int row0, int nrows, double v)
+{
+    std::vector<double> h((size_t)c->W * nrows, v);
+    HIPCHK(hipMemcpyAsync(dev + (size_t)row0 * c->W, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status ensure_protos(mci_ctx *c)
+{
+    const size_t n = (size_t)std::max(nproto_total(c), 1) * c->W;
+    HIPCHK(c->pold.ensure(n));
+    HIPCHK(c->pnew.ensure(n));
+    return MCI_OK;
+}
+
+mci_status ensure_rng(mci_ctx *c)
+{
+    if (!c->reseed) { return MCI_OK; }
+    if (c->rng == MCI_RNG_MT19937_64) {
+        std::vector<uint64_t> s(c->W);
+        for (int64_t w = 0; w < c->W; ++w) { s[w] = c->wseeds.empty() ? walker_seed(c->seed, w) : c->wseeds[w]; }
+        HIPCHK(c->seeds.ensure(c->W));
+        HIPCHK(c->mt.ensure((size_t)MT_N * c->W));
+        HIPCHK(c->mtp.ensure(c->W));
+        HIPCHK(hipMemcpyAsync(c->seeds.p, s.data(), s.size() * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_seed_mt(c->mt.p, c->mtp.p, c->seeds.p, c->W, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    else {
+        HIPCHK(c->ctr.ensure(c->W));
+        HIPCHK(hipMemsetAsync(c->ctr.p, 0, c->W * sizeof(uint64_t), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->reseed = false;
+    return MCI_OK;
+}
+
+// Chain descriptor of the context for the given observable list (main or equilibration)
+Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
+{
+    Chain ch;
+    std::memset(&ch, 0, sizeof ch);
+    ch.ndim = c->ndim;
+    ch.W = c->W;
+    ch.rng = c->rng;
+    ch.key0 = (uint32_t)c->seed;
+    ch.key1 = (uint32_t)(c->seed >> 32);
+    ch.mt = c->mt.p;
+    ch.mtp = c->mtp.p;
+    ch.ctr = c->ctr.p;
+    ch.xold = c->x.p;
+    ch.xnew = c->xn.p;
+    ch.pold = c->pold.p;
+    ch.pnew = c->pnew.p;
+    ch.nproto = nproto_total(c);
+    ch.steps = c->steps.p;
+    ch.ntypes = c->ntypes;
+    for (int t = 0; t < c->ntypes && t < MAX_TYPES; ++t) { ch.typeEnds[t] = c->typeEnds[t]; }
+    ch.movetype = c->movetype;
+    ch.veclen = c->veclen;
+    ch.nvecs = c->nvecs;
+    ch.periodic = c->periodic ? 1 : 0;
+    ch.lb = c->lbd.p;
+    ch.ub = c->ubd.p;
+    ch.npdf = (int)c->pdfs.size();
+    int off = 0;
+    for (size_t i = 0; i < c->pdfs.size(); ++i) {
+        const PdfHost &p = c->pdfs[i];
+        PdfDev &d = ch.pdf[i];
+        d.kind = p.kind;
+        d.nproto = p.nproto;
+        d.off = off;
+        d.K = p.K;
+        off += p.nproto;
+        if (p.kind == MCI_PDF_GMM_ISO) {
+            d.a = nullptr;
+            d.gc = p.dev.p;
+            d.gb = p.dev.p + p.K;
+            d.gmu = p.dev.p + 2 * p.K;
+        }
+        else { d.a = p.params.empty() ? nullptr : p.dev.p; }
+    }
+    ch.obsval = c->obsval.p;
+    ch.acc = c->acc.p;
+    ch.rej = c->rej.p;
+    if (obsd) {
+        ch.nobs = (int)obsd->size();
+        for (size_t i = 0; i < obsd->size(); ++i) { ch.obs[i] = (*obsd)[i]; }
+    }
+    return ch;
+}
+
+double obs_p0(const ObsHost &o, int ndim)
+{
+    switch (o.kind) {
+    case MCI_OBS_X2_SUM: return o.params.empty() ? 1.0 : o.params[0];
+    case MCI_OBS_CONST: return o.params.empty() ? 1.0 : o.params[0];
+    case MCI_OBS_EXP_X2: return o.params.empty() ? (double)ndim : o.params[0];
+    case MCI_OBS_INDICATOR: return o.params.empty() ? 0.0 : o.params[0];
+    }
+    return 0.0;
+}
+
+// the estimator's own size requirements (Estimators.cpp:39,85,127,197; MJBlocker.cpp:25-30),
+// checked before sampling
+mci_status check_estimator(int estim, int64_t n)
+{
+    switch (estim) {
+    case MCI_ESTIM_UNCORRELATED:
+        if (n < 2) { return fail(MCI_E_INVALID_ARGUMENT, "[UncorrelatedEstimator] n must be larger than 1"); }
+        break;
+    case MCI_ESTIM_FCBLOCKER:
+        if (n < 50) { return fail(MCI_E_INVALID_ARGUMENT, "[FCBlockerEstimator] n must be >= 50"); }
+        break;
+    case MCI_ESTIM_MJBLOCKER:
+        if (!is_pow2(n)) { return fail(MCI_E_INVALID_ARGUMENT, "[MJBlocker] ndata must be a power of two."); }
+        break;
+    case MCI_ESTIM_CORRELATED:
+        if (!is_pow2(n) && n < 50) { return fail(MCI_E_INVALID_ARGUMENT, "[FCBlockerEstimator] n must be >= 50"); }
+        break;
+    }
+    return MCI_OK;
+}
+
+int64_t nstore_of(int blocksize, int nskip, int64_t nmc)
+{
+    const int64_t naccu = 1 + (nmc - 1) / nskip;
+    if (blocksize == 0) { return 1; }
+    if (blocksize == 1) { return naccu; }
+    return naccu / blocksize;
+}
+
+bool needs_scratch(int estim, int64_t n) { return estim == MCI_ESTIM_MJBLOCKER || (estim == MCI_ESTIM_CORRELATED && is_pow2(n)); }
+
+float elapsed(mci_ctx *c, int a, int b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) { return 0.f; }
+    return ms;
+}
+
+// MCI::findMRT2Step for walkers that share the rank-averaged rate (MCIntegrator.cpp:87-168, USE_MPI)
+mci_status find_shared(mci_ctx *c)
+{
+    Chain ch = make_chain(c, nullptr);
+    const int64_t min_stat = (int64_t)std::fmax(100., std::sqrt(40000. * c->ndim) / (double)c->W);
+    const int64_t nch = (c->W + 255) / 256;
+    HIPCHK(c->rates.ensure(c->W));
+    HIPCHK(c->chunks_a.ensure(nch));
+    HIPCHK(c->fact.ensure(2));
+    int cons = 0, counter = 0;
+    const int N = c->nfind;
+    while ((N < 0 && cons < 5) || counter < N) {
+        HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream));
+        HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
+        HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
+        HIPCHK(launch_mean_of_chunks(c->chunks_a.p, c->W, c->fact.p + 1, c->stream));
+        double rate = 0.;
+        HIPCHK(hipMemcpyAsync(&rate, c->fact.p + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (std::fabs(rate - c->target) < 0.05) { ++cons; }
+        else { cons = 0; }
+        const double f = std::fmin(2., std::fmax(0.5, rate / c->target));
+        HIPCHK(hipMemcpyAsync(c->fact.p, &f, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_scale_steps(ch, c->fact.p, c->dimsizes.p, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        ++counter;
+        if (N < 0 && counter >= std::abs(N)) { break; }
+    }
+    return MCI_OK;
+}
+
+// MCI::initialDecorrelation, automatic branch (MCIntegrator.cpp:173-243): equilibration
+// observables with Full accumulators and the Correlated estimator; walkers either stop
+// individually (independent) or together on the rank-combined estimates (shared)
+mci_status decorrelate_auto(mci_ctx *c)
+{
+    const bool shared = (c->coupling == MCI_COUPLING_SHARED) && c->W > 1;
+    const int64_t nranks = (c->coupling == MCI_COUPLING_SHARED) ? c->W : 1;
+    const int64_t min_nmc = (int64_t)std::fmax(100., std::sqrt(40000. * c->ndim) / (double)nranks);
+    const int64_t W = c->W;
+    std::vector<ObsDev> od;
+    std::vector<DBuf<double>> data, scr;
+    int nd = 0;
+    for (auto &o : c->obs) {
+        if (!o.equil) { continue; }
+        ObsDev d;
+        std::memset(&d, 0, sizeof d);
+        d.kind = o.kind;
+        d.nobs = o.nobs;
+        d.voff = nd;
+        d.nskip = 1;
+        d.acc = ACC_FULL;
+        d.blocksize = 1;
+        d.p0 = obs_p0(o, c->ndim);
+        d.nstore = min_nmc;
+        data.emplace_back();
+        HIPCHK(data.back().ensure((size_t)min_nmc * o.nobs * W));
+        d.data = data.back().p;
+        scr.emplace_back();
+        if (is_pow2(min_nmc)) { HIPCHK(scr.back().ensure((size_t)min_nmc * o.nobs * W)); }
+        od.push_back(d);
+        nd += o.nobs;
+    }
+    HIPCHK(c->obsval.ensure((size_t)std::max(nd, std::max(nobsdim(c), 1)) * W));
+    const size_t ndw = (size_t)std::max(nd, 1) * W;
+    HIPCHK(c->wavg.ensure(ndw));
+    HIPCHK(c->werr.ensure(ndw));
+    HIPCHK(c->oa.ensure(ndw));
+    HIPCHK(c->oe.ensure(ndw));
+    HIPCHK(c->active.ensure(W));
+    const int64_t nch = (W + 255) / 256;
+    HIPCHK(c->chunks_a.ensure((size_t)nch * std::max(nd, 1)));
+    HIPCHK(c->chunks_e.ensure((size_t)nch * std::max(nd, 1)));
+    HIPCHK(c->res.ensure(2 * (size_t)std::max(nd, 1)));
+    HIPCHK(hipMemsetAsync(c->active.p, 1, W, c->stream));
+    Chain ch = make_chain(c, &od);
+
+    auto sample_estimate = [&](const uint8_t *act) -> mci_status {
+        HIPCHK(launch_sample(ch, min_nmc, 1, act, c->stream));
+        for (size_t i = 0; i < od.size(); ++i) {
+            EstimJob j;
+            j.data = od[i].data;
+            j.scratch = scr[i].p;
+            j.n = min_nmc;
+            j.W = W;
+            j.nobs = od[i].nobs;
+            j.off = od[i].voff;
+            j.estim = MCI_ESTIM_CORRELATED;
+            j.scale = 1.0;
+            j.active = act;
+            j.avg = c->wavg.p;
+            j.err = c->werr.p;
+            HIPCHK(launch_estimate(j, c->stream));
+        }
+        return MCI_OK;
+    };
+    auto combine = [&](std::vector<double> &ha, std::vector<double> &he) -> mci_status {
+        HIPCHK(launch_chunk_sum(c->wavg.p, W, nd, 0, c->chunks_a.p, c->stream));
+        HIPCHK(launch_chunk_sum(c->werr.p, W, nd, 1, c->chunks_e.p, c->stream));
+        HIPCHK(launch_combine(c->chunks_a.p, c->chunks_e.p, W, nd, c->res.p, c->res.p + nd, c->stream));
+        ha.resize(nd);
+        he.resize(nd);
+        HIPCHK(hipMemcpyAsync(ha.data(), c->res.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(he.data(), c->res.p + nd, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return MCI_OK;
+    };
+
+    std::vector<double> oa, oe, na, ne;
+    TRY(sample_estimate(nullptr));
+    if (shared) { TRY(combine(oa, oe)); }
+    else {
+        HIPCHK(hipMemcpyAsync(c->oa.p, c->wavg.p, ndw * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->oe.p, c->werr.p, ndw * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    int64_t count = 0;
+    const int64_t cap = std::llabs(c->ndecorr);
+    std::vector<uint8_t> hact(W);
+    while (true) {
+        const uint8_t *act = shared ? nullptr : c->active.p;
+        HIPCHK(launch_sample(ch, min_nmc, 1, act, c->stream));
+        count += min_nmc;
+        if (count >= cap) {
+            std::printf("Warning [MCI::initialDecorrelation]: Max number of MC steps reached without equilibration.\n");
+            std::fflush(stdout);
+            break;
+        }
+        for (size_t i = 0; i < od.size(); ++i) {
+            EstimJob j;
+            j.data = od[i].data;
+            j.scratch = scr[i].p;
+            j.n = min_nmc;
+            j.W = W;
+            j.nobs = od[i].nobs;
+            j.off = od[i].voff;
+            j.estim = MCI_ESTIM_CORRELATED;
+            j.scale = 1.0;
+            j.active = act;
+            j.avg = c->wavg.p;
+            j.err = c->werr.p;
+            HIPCHK(launch_estimate(j, c->stream));
+        }
+        if (shared) {
+            TRY(combine(na, ne));
+            bool again = false;
+            for (int i = 0; i < nd; ++i) {
+                if (std::fabs(oa[i] - na[i]) > 2 * std::sqrt(oe[i] * oe[i] + ne[i] * ne[i])) { again = true; break; }
+            }
+            oa = na;
+            oe = ne;
+            if (!again) { break; }
+        }
+        else {
+            HIPCHK(launch_decorr_compare(c->oa.p, c->oe.p, c->wavg.p, c->werr.p, nd, W, c->active.p, c->stream));
+            HIPCHK(hipMemcpyAsync(hact.data(), c->active.p, W, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            int64_t nact = 0;
+            for (int64_t w = 0; w < W; ++w) { nact += hact[w]; }
+            if (nact == 0) { break; }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+} // namespace
+
+// ============================================================================
+// C-ABI
+// ============================================================================
+extern "C" {
+
+const char *mci_last_error(void) { return g_err.c_str(); }
+
+const char *mci_version(void) { return kVersion; }
+
+mci_status mci_device_count(int *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { n = 0; }
+    *count = n;
+    return MCI_OK;
+}
+
+mci_status mci_create(int ndim, int64_t nwalkers, mci_ctx **out)
+{
+    if (!out) { return fail(MCI_E_INVALID_ARGUMENT, "null output pointer"); }
+    *out = nullptr;
+    if (ndim < 1) { return fail(MCI_E_INVALID_ARGUMENT, "[ProtoFunctionInterface] Number of dimensions must be at least 1."); }
+    if (nwalkers < 1) { return fail(MCI_E_INVALID_ARGUMENT, "number of walkers must be at least 1"); }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { return fail(MCI_E_DEVICE, "no HIP device available"); }
+    mci_ctx *c = new mci_ctx();
+    c->ndim = ndim;
+    c->W = nwalkers;
+    HIPCHK(hipGetDevice(&c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) { HIPCHK(hipEventCreate(&e)); }
+    c->lb.assign(ndim, 0.);
+    c->ub.assign(ndim, 0.);
+    HIPCHK(c->x.ensure((size_t)ndim * nwalkers));
+    HIPCHK(c->xn.ensure((size_t)ndim * nwalkers));
+    HIPCHK(c->acc.ensure(nwalkers));
+    HIPCHK(c->rej.ensure(nwalkers));
+    HIPCHK(c->lbd.ensure(ndim));
+    HIPCHK(c->ubd.ensure(ndim));
+    HIPCHK(c->dimsizes.ensure(ndim));
+    HIPCHK(hipMemsetAsync(c->x.p, 0, sizeof(double) * ndim * nwalkers, c->stream)); // WalkerState.hpp:30 x = 0
+    HIPCHK(hipMemsetAsync(c->acc.p, 0, sizeof(int64_t) * nwalkers, c->stream));
+    HIPCHK(hipMemsetAsync(c->rej.p, 0, sizeof(int64_t) * nwalkers, c->stream));
+    std::vector<double> sz(ndim, (double)FLT_MAX + (double)FLT_MAX);
+    HIPCHK(hipMemcpyAsync(c->dimsizes.p, sz.data(), sizeof(double) * ndim, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    TRY(ensure_protos(c));
+    mci_status st = mci_set_trial_move(c, MCI_SRRD_UNIFORM, 0, 1, nullptr); // MCI ctor default (MoveType::All)
+    if (st != MCI_OK) { mci_destroy(c); return st; }
+    *out = c;
+    return MCI_OK;
+}
+
+mci_status mci_destroy(mci_ctx *c)
+{
+    if (!c) { return MCI_OK; }
+    (void)hipSetDevice(c->device);
+    if (c->stream) { (void)hipStreamSynchronize(c->stream); }
+    for (auto &e : c->ev) { if (e) { (void)hipEventDestroy(e); } }
+    fast_release(c->fast);
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) { (void)hipStreamDestroy(s); }
+    return MCI_OK;
+}
+
+mci_status mci_set_seed(mci_ctx *c, uint64_t seed)
+{
+    CHECKCTX(c);
+    c->seed = seed;
+    c->wseeds.clear();
+    c->reseed = true;
+    return MCI_OK;
+}
+
+mci_status mci_set_walker_seeds(mci_ctx *c, const uint64_t *seeds, int64_t n)
+{
+    CHECKCTX(c);
+    if (n != c->W || !seeds) { return fail(MCI_E_INVALID_ARGUMENT, "need one seed per walker"); }
+    c->wseeds.assign(seeds, seeds + n);
+    c->reseed = true;
+    return MCI_OK;
+}
+
+mci_status mci_set_rng(mci_ctx *c, int k)
+{
+    CHECKCTX(c);
+    if (k != MCI_RNG_MT19937_64 && k != MCI_RNG_PHILOX) { return fail(MCI_E_DOMAIN, "unknown random-number mode %d", k); }
+    c->rng = k;
+    c->reseed = true;
+    return MCI_OK;
+}
+
+mci_status mci_set_precision(mci_ctx *c, int p)
+{
+    CHECKCTX(c);
+    if (p != MCI_PREC_FP64 && p != MCI_PREC_FP32) { return fail(MCI_E_DOMAIN, "unknown precision %d", p); }
+    c->prec = p;
+    return MCI_OK;
+}
+
+mci_status mci_set_coupling(mci_ctx *c, int m)
+{
+    CHECKCTX(c);
+    if (m != MCI_COUPLING_INDEPENDENT && m != MCI_COUPLING_SHARED) { return fail(MCI_E_DOMAIN, "unknown coupling %d", m); }
+    c->coupling = m;
+    return MCI_OK;
+}
+
+mci_status mci_set_target_acceptance_rate(mci_ctx *c, double r) { CHECKCTX(c); c->target = r; return MCI_OK; }
+mci_status mci_set_nfind_mrt2_iterations(mci_ctx *c, int n) { CHECKCTX(c); c->nfind = n; return MCI_OK; }
+mci_status mci_set_ndecorrelation_steps(mci_ctx *c, int64_t n) { CHECKCTX(c); c->ndecorr = n; return MCI_OK; }
+
+mci_status mci_get_settings(mci_ctx *c, double *t, int *nf, int64_t *nd)
+{
+    CHECKCTX(c);
+    if (t) { *t = c->target; }
+    if (nf) { *nf = c->nfind; }
+    if (nd) { *nd = c->ndecorr; }
+    return MCI_OK;
+}
+
+mci_status mci_set_x(mci_ctx *c, const double *x, int64_t n)
+{
+    CHECKCTX(c);
+    if (!x || (n != c->ndim && n != c->ndim * c->W)) { return fail(MCI_E_INVALID_ARGUMENT, "x must have ndim or nwalkers*ndim entries"); }
+    TRY(set_current(c));
+    std::vector<double> h((size_t)c->ndim * c->W);
+    for (int64_t w = 0; w < c->W; ++w) {
+        for (int i = 0; i < c->ndim; ++i) { h[(size_t)i * c->W + w] = (n == c->ndim) ? x[i] : x[w * c->ndim + i]; }
+    }
+    HIPCHK(hipMemcpyAsync(c->x.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->periodic) { HIPCHK(launch_apply_domain(make_chain(c, nullptr), c->stream)); }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_get_x(mci_ctx *c, double *x, int64_t n)
+{
+    CHECKCTX(c);
+    if (!x || n < c->ndim * c->W) { return fail(MCI_E_INVALID_ARGUMENT, "output too small"); }
+    TRY(set_current(c));
+    std::vector<double> h((size_t)c->ndim * c->W);
+    HIPCHK(hipMemcpyAsync(h.data(), c->x.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t w = 0; w < c->W; ++w) {
+        for (int i = 0; i < c->ndim; ++i) { x[w * c->ndim + i] = h[(size_t)i * c->W + w]; }
+    }
+    return MCI_OK;
+}
+
+mci_status mci_center_x(mci_ctx *c) // DomainInterface::getCenter (DomainInterface.hpp:39-43)
+{
+    CHECKCTX(c);
+    std::vector<double> ctr(c->ndim);
+    for (int i = 0; i < c->ndim; ++i) {
+        ctr[i] = c->periodic ? c->lb[i] + 0.5 * (c->ub[i] - c->lb[i])
+                             : -(double)FLT_MAX + 0.5 * ((double)FLT_MAX + (double)FLT_MAX);
+    }
+    return mci_set_x(c, ctr.data(), c->ndim);
+}
+
+mci_status mci_move_x(mci_ctx *c)
+{
+    CHECKCTX(c);
+    TRY(set_current(c));
+    TRY(ensure_protos(c));
+    TRY(ensure_rng(c));
+    HIPCHK(launch_move_x(make_chain(c, nullptr), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_new_random_x(mci_ctx *c)
+{
+    CHECKCTX(c);
+    TRY(set_current(c));
+    TRY(ensure_rng(c));
+    HIPCHK(launch_new_random_x(make_chain(c, nullptr), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_set_mrt2step(mci_ctx *c, int idx, double step)
+{
+    CHECKCTX(c);
+    TRY(set_current(c));
+    if (idx < 0) { return fill_rows(c, c->steps.p, 0, c->ntypes, step); }
+    if (idx < c->ntypes) { return fill_rows(c, c->steps.p, idx, 1, step); }
+    std::printf("[MCI::setMRT2Step] Warning: Tried to set non-existing MRT2step index.\n");
+    return MCI_OK;
+}
+
+mci_status mci_set_mrt2steps(mci_ctx *c, const double *steps, int n)
+{
+    CHECKCTX(c);
+    if (!steps || n != c->ntypes) { return fail(MCI_E_INVALID_ARGUMENT, "need one step per step-size type"); }
+    for (int t = 0; t < n; ++t) { TRY(mci_set_mrt2step(c, t, steps[t])); }
+    return MCI_OK;
+}
+
+mci_status mci_get_mrt2step(mci_ctx *c, int idx, double *step)
+{
+    CHECKCTX(c);
+    if (!step) { return fail(MCI_E_INVALID_ARGUMENT, "null output"); }
+    if (idx < 0 || idx >= c->ntypes) { *step = 0.; return MCI_OK; }
+    TRY(set_current(c));
+    HIPCHK(hipMemcpyAsync(step, c->steps.p + (size_t)idx * c->W, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_get_walker_mrt2steps(mci_ctx *c, double *steps, int64_t n)
+{
+    CHECKCTX(c);
+    if (!steps || n < c->W * c->ntypes) { return fail(MCI_E_INVALID_ARGUMENT, "output too small"); }
+    TRY(set_current(c));
+    std::vector<double> h((size_t)c->ntypes * c->W);
+    HIPCHK(hipMemcpyAsync(h.data(), c->steps.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t w = 0; w < c->W; ++w) {
+        for (int t = 0; t < c->ntypes; ++t) { steps[w * c->ntypes + t] = h[(size_t)t * c->W + w]; }
+    }
+    return MCI_OK;
+}
+
+mci_status mci_get_n_step_sizes(mci_ctx *c, int *n) { CHECKCTX(c); *n = c->ntypes; return MCI_OK; }
+
+mci_status mci_reset_domain(mci_ctx *c)
+{
+    CHECKCTX(c);
+    TRY(set_current(c));
+    c->periodic = false;
+    std::vector<double> sz(c->ndim, (double)FLT_MAX + (double)FLT_MAX);
+    HIPCHK(hipMemcpyAsync(c->dimsizes.p, sz.data(), sizeof(double) * c->ndim, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_set_irange(mci_ctx *c, const double *lb, const double *ub)
+{
+    CHECKCTX(c);
+    if (!lb || !ub) { return fail(MCI_E_INVALID_ARGUMENT, "null bounds"); }
+    for (int i = 0; i < c->ndim; ++i) {
+        if (ub[i] <= lb[i]) {
+            return fail(MCI_E_INVALID_ARGUMENT,
+                        "[OrthoPeriodicDomain::checkBounds] All upper bounds must be truly greater than their corresponding lower bounds.");
+        }
+    }
+    TRY(set_current(c));
+    c->lb.assign(lb, lb + c->ndim);
+    c->ub.assign(ub, ub + c->ndim);
+    c->periodic = true;
+    std::vector<double> sz(c->ndim);
+    for (int i = 0; i < c->ndim; ++i) { sz[i] = ub[i] - lb[i]; }
+    HIPCHK(hipMemcpyAsync(c->lbd.p, lb, sizeof(double) * c->ndim, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ubd.p, ub, sizeof(double) * c->ndim, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dimsizes.p, sz.data(), sizeof(double) * c->ndim, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_apply_domain(make_chain(c, nullptr), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_domain_is_finite(mci_ctx *c, int *f) { CHECKCTX(c); *f = c->periodic ? 1 : 0; return MCI_OK; }
+
+mci_status mci_set_trial_move(mci_ctx *c, int srrd, int veclen, int ntypes, const int *te)
+{
+    CHECKCTX(c);
+    if (srrd < MCI_SRRD_UNIFORM || srrd > 9) { return fail(MCI_E_DOMAIN, "[createSRRDAllMove] Unhandled SRRDType enumerator."); }
+    if (srrd != MCI_SRRD_UNIFORM) { return fail(MCI_E_DOMAIN, "only the uniform SRRD move is implemented on the device"); }
+    if (ntypes < 1) { ntypes = 1; }
+    if (ntypes > MAX_TYPES) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d step-size types", MAX_TYPES); }
+    if (ntypes > 1 && !te) { return fail(MCI_E_INVALID_ARGUMENT, "[checkTrialMoveSanity] ntypes>1 requires passed typeEnds array."); }
+    if (veclen > 0 && c->ndim % veclen != 0) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::setTrialMove] MCI's number of walkers must be a multiple of passed veclen.");
+    }
+    if (veclen > 0 && ntypes > 1) {
+        for (int i = 0; i < ntypes; ++i) {
+            if (te[i] % veclen != 0) { return fail(MCI_E_INVALID_ARGUMENT, "[SRRDVecMove] All type end indices must be multiples of vector length."); }
+        }
+    }
+    if (ntypes > 1) {
+        for (int i = 0; i < ntypes; ++i) {
+            if (te[i] < 1 || te[i] > c->ndim || (i > 0 && te[i] <= te[i - 1])) { return fail(MCI_E_INVALID_ARGUMENT, "typeEnds must increase within [1, ndim]"); }
+        }
+        if (te[ntypes - 1] != c->ndim) { return fail(MCI_E_INVALID_ARGUMENT, "the last type end must equal ndim"); }
+    }
+    TRY(set_current(c));
+    c->typeEnds.assign(ntypes, c->ndim);
+    if (ntypes > 1) { c->typeEnds.assign(te, te + ntypes); }
+    c->ntypes = ntypes;
+    c->srrd = srrd;
+    c->movetype = (veclen > 0) ? MCI_MOVE_VEC : MCI_MOVE_ALL;
+    c->veclen = veclen > 0 ? veclen : 0;
+    c->nvecs = veclen > 0 ? c->ndim / veclen : 0;
+    HIPCHK(c->steps.ensure((size_t)ntypes * c->W));
+    return fill_rows(c, c->steps.p, 0, ntypes, 0.05); // DEFAULT_MRT2STEP (Factories.hpp:145)
+}
+
+mci_status mci_set_trial_move_type(mci_ctx *c, int t)
+{
+    CHECKCTX(c);
+    if (t == MCI_MOVE_ALL) { return mci_set_trial_move(c, MCI_SRRD_UNIFORM, 0, 1, nullptr); }
+    if (t == MCI_MOVE_VEC) { return mci_set_trial_move(c, MCI_SRRD_UNIFORM, 1, 1, nullptr); }
+    return fail(MCI_E_DOMAIN, "[createMoveDefault] Unhandled MoveType enumerator.");
+}
+
+mci_status mci_get_change_rate(mci_ctx *c, double *r)
+{
+    CHECKCTX(c);
+    *r = (c->movetype == MCI_MOVE_ALL) ? 1. : 1. / c->nvecs;
+    return MCI_OK;
+}
+
+mci_status mci_add_sampling_function(mci_ctx *c, int kind, int ndim, const double *params, int nparams)
+{
+    CHECKCTX(c);
+    if (ndim != c->ndim) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
+    }
+    if ((int)c->pdfs.size() >= MAX_PDF) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d sampling functions", MAX_PDF); }
+    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
+    TRY(set_current(c));
+    PdfHost p;
+    p.kind = kind;
+    p.params.assign(params, params + nparams);
+    std::vector<double> blob;
+    switch (kind) {
+    case MCI_PDF_GAUSS_ISO:
+        p.nproto = 1;
+        break;
+    case MCI_PDF_GAUSS_DIAG:
+    case MCI_PDF_EXP_DIAG:
+        if (nparams != 0 && nparams != ndim) { return fail(MCI_E_INVALID_ARGUMENT, "params must be empty or have ndim entries"); }
+        p.nproto = ndim;
+        blob = p.params;
+        break;
+    case MCI_PDF_GMM_ISO: {
+        if (nparams < 1) { return fail(MCI_E_INVALID_ARGUMENT, "GMM needs K"); }
+        const int K = (int)params[0];
+        if (K < 1 || nparams != 1 + 2 * K + K * ndim) { return fail(MCI_E_INVALID_ARGUMENT, "GMM params must be [K, w[K], s[K], mu[K][ndim]]"); }
+        p.K = K;
+        p.nproto = K + 1;
+        blob.resize(2 * K + (size_t)K * ndim);
+        for (int k = 0; k < K; ++k) {
+            const double w = params[1 + k], s = params[1 + K + k];
+            if (!(w > 0.) || !(s > 0.)) { return fail(MCI_E_INVALID_ARGUMENT, "GMM weights and widths must be positive"); }
+            blob[k] = std::log(w) - (double)ndim * std::log(s);
+            blob[K + k] = 0.5 / (s * s);
+        }
+        std::copy(params + 1 + 2 * K, params + nparams, blob.begin() + 2 * K);
+        break;
+    }
+    default:
+        return fail(MCI_E_DOMAIN, "unknown sampling-function kind %d", kind);
+    }
+    if (!blob.empty()) {
+        HIPCHK(p.dev.ensure(blob.size()));
+        HIPCHK(hipMemcpy(p.dev.p, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->pdfs.push_back(std::move(p));
+    return ensure_protos(c);
+}
+
+mci_status mci_pop_sampling_function(mci_ctx *c)
+{
+    CHECKCTX(c);
+    if (c->pdfs.empty()) { return fail(MCI_E_RUNTIME, "no sampling function to pop"); }
+    c->pdfs.pop_back();
+    return MCI_OK;
+}
+
+mci_status mci_clear_sampling_functions(mci_ctx *c) { CHECKCTX(c); c->pdfs.clear(); return MCI_OK; }
+mci_status mci_get_npdf(mci_ctx *c, int *n) { CHECKCTX(c); *n = (int)c->pdfs.size(); return MCI_OK; }
+
+mci_status mci_add_observable(mci_ctx *c, int kind, int ndim, const double *params, int nparams, int blocksize, int nskip,
+                              int flag_equil, int estim)
+{
+    CHECKCTX(c);
+    blocksize = std::max(0, blocksize);
+    nskip = std::max(1, nskip);
+    if (ndim != c->ndim) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Passed observable function's number of inputs is not equal to MCI's number of walkers.");
+    }
+    if (flag_equil && estim == MCI_ESTIM_NOOP) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Requested automatic observable equilibration requires estimator with error calculation.");
+    }
+    if (estim < MCI_ESTIM_NOOP || estim > MCI_ESTIM_MJBLOCKER) { return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator."); }
+    const int nobs = obs_nobs(kind, ndim);
+    if (nobs < 0) { return fail(MCI_E_DOMAIN, "unknown observable kind %d", kind); }
+    if ((int)c->obs.size() >= MAX_OBS) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d observables", MAX_OBS); }
+    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
+    if (kind == MCI_OBS_INDICATOR && nparams > 0 && (params[0] < 0 || params[0] >= ndim)) {
+        return fail(MCI_E_INVALID_ARGUMENT, "indicator index out of range");
+    }
+    ObsHost o;
+    o.kind = kind;
+    o.nobs = nobs;
+    o.blocksize = blocksize;
+    o.nskip = nskip;
+    o.equil = flag_equil ? 1 : 0;
+    o.estim = estim;
+    o.params.assign(params, params + nparams);
+    c->obs.push_back(o);
+    return MCI_OK;
+}
+
+mci_status mci_pop_observable(mci_ctx *c)
+{
+    CHECKCTX(c);
+    if (c->obs.empty()) { return fail(MCI_E_RUNTIME, "no observable to pop"); }
+    c->obs.pop_back();
+    return MCI_OK;
+}
+
+mci_status mci_clear_observables(mci_ctx *c) { CHECKCTX(c); c->obs.clear(); return MCI_OK; }
+mci_status mci_get_nobs(mci_ctx *c, int *n) { CHECKCTX(c); *n = (int)c->obs.size(); return MCI_OK; }
+mci_status mci_get_nobsdim(mci_ctx *c, int *n) { CHECKCTX(c); *n = nobsdim(c); return MCI_OK; }
+
+mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error, int do_find, int do_decorr)
+{
+    CHECKCTX(c);
+    if (c->pdfs.empty() && !c->periodic) {
+        return fail(MCI_E_DOMAIN, "[MCI::integrate] Integrating over an infinite domain requires a sampling function.");
+    }
+    const int nd = nobsdim(c);
+    if (nmc > 0 && nd > 0 && (!average || !error)) { return fail(MCI_E_INVALID_ARGUMENT, "null output arrays"); }
+    // accumulator and estimator requirements (BlockAccumulator.cpp:8-13 and the estimators)
+    if (nmc > 0) {
+        for (auto &o : c->obs) {
+            const int64_t naccu = 1 + (nmc - 1) / o.nskip;
+            if (o.blocksize > 1) {
+                if (naccu < o.blocksize) {
+                    return fail(MCI_E_INVALID_ARGUMENT, "[BlockAccumulator::allocate] Requested number of accumulations is smaller than the requested block size.");
+                }
+                if (naccu % o.blocksize != 0) {
+                    return fail(MCI_E_INVALID_ARGUMENT, "[BlockAccumulator::allocate] Requested number of accumulations is not a multiple of the requested block size.");
+                }
+            }
+            TRY(check_estimator(o.estim, nstore_of(o.blocksize, o.nskip, nmc)));
+        }
+    }
+    TRY(set_current(c));
+    TRY(ensure_protos(c));
+    TRY(ensure_rng(c));
+    const int64_t W = c->W;
+    HIPCHK(c->obsval.ensure((size_t)std::max(nd, 1) * W));
+    for (double &t : c->timings) { t = 0.; }
+    const bool fast = fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
+                                    c->pdfs.empty() ? 0 : c->pdfs[0].kind);
+    if (c->prec == MCI_PREC_FP32 && !fast) {
+        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports single-coordinate uniform moves on one diagonal Gaussian in an unbounded domain");
+    }
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    // ---- calibration (MCIntegrator.cpp:49-51) ----
+    if (!c->pdfs.empty() && do_find && c->ntypes > 0) {
+        if (c->coupling == MCI_COUPLING_SHARED && W > 1) { TRY(find_shared(c)); }
+        else {
+            const int64_t min_stat = (int64_t)std::fmax(100., std::sqrt(40000. * c->ndim));
+            HIPCHK(launch_calibrate_indep(make_chain(c, nullptr), c->nfind, c->target, min_stat, c->dimsizes.p, c->stream));
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    // ---- initial decorrelation (MCIntegrator.cpp:171-247) ----
+    if (!c->pdfs.empty() && do_decorr) {
+        if (c->ndecorr > 0) {
+            if (fast) { TRY(fast_sample(c->fast, make_chain(c, nullptr), c->prec, c->ndecorr, nullptr, 0, c->stream)); }
+            else { HIPCHK(launch_sample(make_chain(c, nullptr), c->ndecorr, 0, nullptr, c->stream)); }
+        }
+        else if (c->ndecorr < 0) { TRY(decorrelate_auto(c)); }
+    }
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    if (nmc <= 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->timings[0] = elapsed(c, 0, 1);
+        c->timings[1] = elapsed(c, 1, 2);
+        c->timings[4] = elapsed(c, 0, 2);
+        return MCI_OK;
+    }
+    // ---- allocate accumulators and sample (MCIntegrator.cpp:54-62) ----
+    if (c->accdata.size() < c->obs.size()) { c->accdata.resize(c->obs.size()); c->scratch.resize(c->obs.size()); }
+    std::vector<ObsDev> od;
+    int voff = 0;
+    for (size_t i = 0; i < c->obs.size(); ++i) {
+        const ObsHost &o = c->obs[i];
+        ObsDev d;
+        std::memset(&d, 0, sizeof d);
+        d.kind = o.kind;
+        d.nobs = o.nobs;
+        d.voff = voff;
+        d.nskip = o.nskip;
+        d.acc = (o.blocksize == 0) ? ACC_SIMPLE : (o.blocksize == 1 ? ACC_FULL : ACC_BLOCK);
+        d.blocksize = std::max(o.blocksize, 1);
+        d.p0 = obs_p0(o, c->ndim);
+        d.nstore = nstore_of(o.blocksize, o.nskip, nmc);
+        HIPCHK(c->accdata[i].ensure((size_t)d.nstore * o.nobs * W));
+        d.data = c->accdata[i].p;
+        if (needs_scratch(o.estim, d.nstore)) { HIPCHK(c->scratch[i].ensure((size_t)d.nstore * o.nobs * W)); }
+        od.push_back(d);
+        voff += o.nobs;
+    }
+    c->nstore_last.clear();
+    for (auto &d : od) { c->nstore_last.push_back(d.nstore); }
+    Chain ch = make_chain(c, &od);
+    if (fast) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
+    else { HIPCHK(launch_sample(ch, nmc, 1, nullptr, c->stream)); }
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    // ---- estimate (ObservableContainer.cpp:104-111), volume scaling (:71-77), combine ----
+    double vol = 1.0;
+    if (c->pdfs.empty()) {
+        for (int i = 0; i < c->ndim; ++i) { vol = vol * (c->ub[i] - c->lb[i]); }
+    }
+    const size_t ndw = (size_t)std::max(nd, 1) * W;
+    HIPCHK(c->wavg.ensure(ndw));
+    HIPCHK(c->werr.ensure(ndw));
+    for (size_t i = 0; i < od.size(); ++i) {
+        EstimJob j;
+        j.data = od[i].data;
+        j.scratch = c->scratch[i].p;
+        j.n = od[i].nstore;
+        j.W = W;
+        j.nobs = od[i].nobs;
+        j.off = od[i].voff;
+        j.estim = c->obs[i].estim;
+        j.scale = vol;
+        j.active = nullptr;
+        j.avg = c->wavg.p;
+        j.err = c->werr.p;
+        HIPCHK(launch_estimate(j, c->stream));
+    }
+    c->h_wavg.assign(ndw, 0.);
+    c->h_werr.assign(ndw, 0.);
+    if (nd > 0) {
+        if (W > 1) {
+            const int64_t nch = (W + 255) / 256;
+            HIPCHK(c->chunks_a.ensure((size_t)nch * nd));
+            HIPCHK(c->chunks_e.ensure((size_t)nch * nd));
+            HIPCHK(c->res.ensure(2 * (size_t)nd));
+            HIPCHK(launch_chunk_sum(c->wavg.p, W, nd, 0, c->chunks_a.p, c->stream));
+            HIPCHK(launch_chunk_sum(c->werr.p, W, nd, 1, c->chunks_e.p, c->stream));
+            HIPCHK(launch_combine(c->chunks_a.p, c->chunks_e.p, W, nd, c->res.p, c->res.p + nd, c->stream));
+            HIPCHK(hipMemcpyAsync(average, c->res.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(error, c->res.p + nd, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_wavg.data(), c->wavg.p, (size_t)nd * W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_werr.data(), c->werr.p, (size_t)nd * W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    else { HIPCHK(hipEventRecord(c->ev[4], c->stream)); }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (W == 1) {
+        for (int i = 0; i < nd; ++i) { average[i] = c->h_wavg[i]; error[i] = c->h_werr[i]; }
+    }
+    c->nobsdim_last = nd;
+    c->timings[0] = elapsed(c, 0, 1);
+    c->timings[1] = elapsed(c, 1, 2);
+    c->timings[2] = elapsed(c, 2, 3);
+    c->timings[3] = elapsed(c, 3, 4);
+    c->timings[4] = elapsed(c, 0, 4);
+    return MCI_OK;
+}
+
+mci_status mci_get_walker_results(mci_ctx *c, double *a, double *e, int64_t n)
+{
+    CHECKCTX(c);
+    const int nd = c->nobsdim_last;
+    if (!a || !e || n < c->W * nd) { return fail(MCI_E_INVALID_ARGUMENT, "output too small"); }
+    for (int64_t w = 0; w < c->W; ++w) {
+        for (int i = 0; i < nd; ++i) {
+            a[w * nd + i] = c->h_wavg[(size_t)i * c->W + w];
+            e[w * nd + i] = c->h_werr[(size_t)i * c->W + w];
+        }
+    }
+    return MCI_OK;
+}
+
+mci_status mci_get_observable_data(mci_ctx *c, int obs, double *data, int64_t n, int64_t *nstore)
+{
+    CHECKCTX(c);
+    if (obs < 0 || obs >= (int)c->nstore_last.size() || obs >= (int)c->obs.size()) {
+        return fail(MCI_E_INVALID_ARGUMENT, "no data for observable %d", obs);
+    }
+    const int64_t ns = c->nstore_last[obs];
+    const int nobs = c->obs[obs].nobs;
+    if (nstore) { *nstore = ns; }
+    if (!data) { return MCI_OK; }
+    if (n < ns * nobs * c->W) { return fail(MCI_E_INVALID_ARGUMENT, "output too small"); }
+    TRY(set_current(c));
+    std::vector<double> h((size_t)ns * nobs * c->W);
+    HIPCHK(hipMemcpyAsync(h.data(), c->accdata[obs].p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t w = 0; w < c->W; ++w) {
+        for (int64_t r = 0; r < ns * nobs; ++r) { data[(w * ns * nobs) + r] = h[(size_t)r * c->W + w]; }
+    }
+    return MCI_OK;
+}
+
+mci_status mci_get_acceptance(mci_ctx *c, int64_t *acc, int64_t *rej, double *rate)
+{
+    CHECKCTX(c);
+    TRY(set_current(c));
+    std::vector<int64_t> a(c->W), r(c->W);
+    HIPCHK(hipMemcpyAsync(a.data(), c->acc.p, c->W * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(r.data(), c->rej.p, c->W * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int64_t sa = 0, sr = 0;
+    std::vector<double> rates(c->W);
+    for (int64_t w = 0; w < c->W; ++w) {
+        sa += a[w];
+        sr += r[w];
+        rates[w] = (a[w] > 0) ? (double)a[w] / ((double)a[w] + (double)r[w]) : 0.;
+    }
+    if (acc) { *acc = sa; }
+    if (rej) { *rej = sr; }
+    if (rate) { *rate = (c->W == 1) ? rates[0] : det_sum(rates.data(), c->W) / (double)c->W; }
+    return MCI_OK;
+}
+
+mci_status mci_get_walker_acceptance(mci_ctx *c, int64_t *acc, int64_t *rej, int64_t n)
+{
+    CHECKCTX(c);
+    if (!acc || !rej || n < c->W) { return fail(MCI_E_INVALID_ARGUMENT, "output too small"); }
+    TRY(set_current(c));
+    HIPCHK(hipMemcpyAsync(acc, c->acc.p, c->W * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(rej, c->rej.p, c->W * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_get_timings(mci_ctx *c, double *ms, int n)
+{
+    CHECKCTX(c);
+    for (int i = 0; i < n && i < 5; ++i) { ms[i] = c->timings[i]; }
+    return MCI_OK;
+}
+
+mci_status mci_estimate(int estim, int64_t n, int ndim, const double *x, double *average, double *error)
+{
+    if (!x || !average || !error || ndim < 1) { return fail(MCI_E_INVALID_ARGUMENT, "bad arguments"); }
+    if (estim < MCI_ESTIM_NOOP || estim > MCI_ESTIM_MJBLOCKER) { return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator."); }
+    if (n < 1) { return fail(MCI_E_INVALID_ARGUMENT, "n must be at least 1"); }
+    TRY(check_estimator(estim, n));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { return fail(MCI_E_DEVICE, "no HIP device available"); }
+    DBuf<double> d, s, a, e;
+    HIPCHK(d.ensure((size_t)n * ndim));
+    HIPCHK(s.ensure((size_t)n * ndim));
+    HIPCHK(a.ensure(ndim));
+    HIPCHK(e.ensure(ndim));
+    HIPCHK(hipMemcpy(d.p, x, sizeof(double) * n * ndim, hipMemcpyHostToDevice));
+    EstimJob j;
+    j.data = d.p;
+    j.scratch = s.p;
+    j.n = n;
+    j.W = 1;
+    j.nobs = ndim;
+    j.off = 0;
+    j.estim = estim;
+    j.scale = 1.0;
+    j.active = nullptr;
+    j.avg = a.p;
+    j.err = e.p;
+    HIPCHK(launch_estimate(j, nullptr));
+    HIPCHK(hipMemcpy(average, a.p, sizeof(double) * ndim, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(error, e.p, sizeof(double) * ndim, hipMemcpyDeviceToHost));
+    return MCI_OK;
+}
+
+} // extern "C"

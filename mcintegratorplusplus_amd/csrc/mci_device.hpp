@@ -1,0 +1,550 @@
+// mci_device.hpp — device-side building blocks of the Metropolis chain on gfx950.
+//
+// One walker per lane.  Walker state lives in HBM as structure-of-arrays with the
+// walker index fastest ([dim][walker]), so every per-dimension access of a wave is
+// one coalesced 512-byte transaction.  All FP64 arithmetic here is written in the
+// reference's expression order and compiled with -ffp-contract=off, so a walker fed
+// the reference's random stream reproduces the reference chain bit for bit.
+//
+// Reference map: doStepMRT2 src/MCIntegrator.cpp:326-360; trial moves
+// include/mci/SRRDAllMove.hpp:67-80, SRRDVecMove.hpp:75-96; acceptance
+// SamplingFunctionInterface.hpp:49-57; accumulation AccumulatorInterface.cpp:99-114.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mci_amd.h"
+
+namespace mcid
+{
+constexpr int MAX_PDF = 4;
+constexpr int MAX_OBS = 16;
+constexpr int MAX_TYPES = 16;
+constexpr int MT_N = 312;
+constexpr int MT_M = 156;
+
+// accumulator storage types (Factories.hpp:29-43: blocksize 0 / 1 / >1)
+enum { ACC_SIMPLE = 0, ACC_FULL = 1, ACC_BLOCK = 2 };
+
+struct PdfDev
+{
+    int kind, nproto, off, K;
+    const double *a;   // per-dimension weights (GAUSS_DIAG, EXP_DIAG) or nullptr (= 1)
+    const double *gc;  // GMM_ISO: log w_k - ndim log s_k
+    const double *gb;  // GMM_ISO: 1/(2 s_k^2)
+    const double *gmu; // GMM_ISO: means [K][ndim]
+};
+
+struct ObsDev
+{
+    int kind, nobs, voff, nskip, acc, blocksize;
+    double p0;        // first parameter (divisor, constant or index)
+    int64_t nstore;   // rows of data: 1 (simple), NAccu (full), nblocks (block)
+    double *data;     // [nstore][nobs][W]
+};
+
+// Everything a kernel needs to advance W walkers; passed by value.
+struct Chain
+{
+    int ndim;
+    int64_t W;
+    int rng;
+    uint32_t key0, key1;        // Philox key = seed
+    uint64_t *mt;               // [MT_N][W] mt19937_64 state (reference streams)
+    int32_t *mtp;               // [W] next state index
+    uint64_t *ctr;              // [W] Philox step counter
+    double *xold, *xnew;        // [ndim][W]
+    double *pold, *pnew;        // [nproto][W]
+    int nproto;
+    double *steps;              // [ntypes][W]
+    int ntypes;
+    int typeEnds[MAX_TYPES];
+    int movetype, veclen, nvecs;
+    int periodic;
+    const double *lb, *ub;      // [ndim]
+    int npdf;
+    PdfDev pdf[MAX_PDF];
+    int nobs;
+    ObsDev obs[MAX_OBS];
+    double *obsval;             // [nobsdim][W] last observable values
+    int64_t *acc, *rej;         // [W] counters of the last sample() call
+};
+
+__device__ __forceinline__ double &at(double *p, int64_t W, int i, int64_t w) { return p[(int64_t)i * W + w]; }
+
+// --------------------------------------------------------------------------
+// random streams
+// --------------------------------------------------------------------------
+struct Lane
+{
+    int64_t w;
+    int32_t mtp;       // MT: next index
+    uint64_t t;        // Philox: step counter
+    uint32_t k, kx;    // Philox: word index within the step, extra-word index
+    int nchanged;      // changed coordinates [cfirst, cfirst + nchanged) (== ndim: all)
+    int cfirst;
+    bool accepted;
+    int64_t acc, rej;
+};
+
+// mt19937_64 with the twist done lazily, one element per output: element k of a round
+// uses x[k+1] (not yet twisted this round) and x[(k+156) % 312] (twisted iff k >= 156),
+// exactly the values the batch twist of libstdc++ (bits/random.tcc) uses, so the
+// output stream is identical.  State index 0 after seeding = batch index 312.
+__device__ __forceinline__ uint64_t mt_next(const Chain &c, Lane &L)
+{
+    const int64_t W = c.W;
+    const int k = L.mtp;
+    const int k1 = (k + 1 == MT_N) ? 0 : k + 1;
+    const int km = (k < MT_N - MT_M) ? k + MT_M : k + MT_M - MT_N;
+    const uint64_t xk = c.mt[(int64_t)k * W + L.w];
+    const uint64_t xk1 = c.mt[(int64_t)k1 * W + L.w];
+    const uint64_t xkm = c.mt[(int64_t)km * W + L.w];
+    const uint64_t y = (xk & 0xffffffff80000000ULL) | (xk1 & 0x7fffffffULL);
+    const uint64_t nv = xkm ^ (y >> 1) ^ ((y & 1ULL) ? 0xb5026f5aa96619e9ULL : 0ULL);
+    c.mt[(int64_t)k * W + L.w] = nv;
+    L.mtp = k1;
+    uint64_t z = nv;
+    z ^= (z >> 29) & 0x5555555555555555ULL;
+    z ^= (z << 17) & 0x71d67fffeda60000ULL;
+    z ^= (z << 37) & 0xfff7eee000000000ULL;
+    z ^= (z >> 43);
+    return z;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11)
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                              uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// word k of the current step: counter = (k/4, walker, step_lo, step_hi), key = seed
+__device__ __forceinline__ uint32_t ph_word(const Chain &c, const Lane &L, uint32_t k)
+{
+    uint32_t o[4];
+    philox4x32_10(k >> 2, (uint32_t)L.w, (uint32_t)L.t, (uint32_t)(L.t >> 32), c.key0, c.key1, o);
+    return o[k & 3u];
+}
+
+__device__ __forceinline__ void step_begin(Lane &L) { L.k = 0; L.kx = 0; }
+__device__ __forceinline__ void step_end(const Chain &c, Lane &L) { if (c.rng == MCI_RNG_PHILOX) { ++L.t; } }
+
+// generate_canonical<double,53>(mt19937_64): double(word)/2^64, clamped below 1
+__device__ __forceinline__ double canonical53(uint64_t w)
+{
+    double r = (double)w / 18446744073709551616.0;
+    if (r >= 1.0) { r = 0x1.fffffffffffffp-1; }
+    return r;
+}
+
+// symmetric uniform on (-1,1) (TrialMoveInterface.hpp:120-124): (u*(b-a)) + a
+__device__ __forceinline__ double draw_sym(const Chain &c, Lane &L)
+{
+    if (c.rng == MCI_RNG_MT19937_64) {
+        const double u = canonical53(mt_next(c, L));
+        return (u * 2.0) + (-1.0);
+    }
+    const uint32_t v = ph_word(c, L, L.k++);
+    return (double)(2 * (int64_t)v + 1 - 4294967296LL) * 0x1p-32;
+}
+
+// uniform on [0,1) (MCIntegrator.cpp:343 acceptance draw; :365 random moves)
+__device__ __forceinline__ double draw_unit(const Chain &c, Lane &L)
+{
+    if (c.rng == MCI_RNG_MT19937_64) {
+        const double u = canonical53(mt_next(c, L));
+        return (u * 1.0) + 0.0;
+    }
+    const uint64_t a = ph_word(c, L, L.k++);
+    const uint64_t b = ph_word(c, L, L.k++);
+    return (double)(((a << 32) | b) >> 11) * 0x1p-53;
+}
+
+// uniform integer on [0, n) (SRRDVecMove.hpp:78): Lemire's method as in libstdc++
+// (64-bit engine, 128-bit product) or on 32-bit Philox words
+__device__ __forceinline__ int draw_index(const Chain &c, Lane &L, int n)
+{
+    if (c.rng == MCI_RNG_MT19937_64) {
+        const uint64_t range = (uint64_t)n;
+        uint64_t wd = mt_next(c, L);
+        uint64_t low = wd * range;
+        uint64_t high = __umul64hi(wd, range);
+        if (low < range) {
+            const uint64_t thr = (0 - range) % range;
+            while (low < thr) {
+                wd = mt_next(c, L);
+                low = wd * range;
+                high = __umul64hi(wd, range);
+            }
+        }
+        return (int)high;
+    }
+    const uint32_t range = (uint32_t)n;
+    uint64_t prod = (uint64_t)ph_word(c, L, L.k++) * range;
+    uint32_t low = (uint32_t)prod;
+    if (low < range) {
+        const uint32_t thr = (0u - range) % range;
+        while (low < thr) {
+            prod = (uint64_t)ph_word(c, L, 0x80000000u + (L.kx++)) * range;
+            low = (uint32_t)prod;
+        }
+    }
+    return (int)(prod >> 32);
+}
+
+// --------------------------------------------------------------------------
+// sampling functions (device functors; arithmetic of include/mci_amd.h)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double pdf_a(const PdfDev &p, int i) { return p.a ? p.a[i] : 1.0; }
+
+__device__ inline double gmm_logp(const Chain &c, const PdfDev &p, const double *pv, int64_t W, int64_t w)
+{
+    double m = -INFINITY;
+    for (int k = 0; k < p.K; ++k) {
+        const double t = p.gc[k] - p.gb[k] * pv[(int64_t)k * W + w];
+        if (t > m) { m = t; }
+    }
+    double s = 0.0;
+    for (int k = 0; k < p.K; ++k) { s = s + exp((p.gc[k] - p.gb[k] * pv[(int64_t)k * W + w]) - m); }
+    return m + log(s);
+}
+
+// protoFunction(x) -> proto values (written to pv + p.off rows)
+__device__ inline void pdf_proto(const Chain &c, const PdfDev &p, const double *x, double *pv, int64_t w)
+{
+    const int64_t W = c.W;
+    double *q = pv + (int64_t)p.off * W;
+    switch (p.kind) {
+    case MCI_PDF_GAUSS_ISO: {
+        double s = x[w] * x[w];
+        for (int i = 1; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; s = s + v * v; }
+        q[w] = s;
+        break;
+    }
+    case MCI_PDF_GAUSS_DIAG:
+        for (int i = 0; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; q[(int64_t)i * W + w] = (v * v) * pdf_a(p, i); }
+        break;
+    case MCI_PDF_EXP_DIAG:
+        for (int i = 0; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; q[(int64_t)i * W + w] = fabs(v) * pdf_a(p, i); }
+        break;
+    case MCI_PDF_GMM_ISO:
+        for (int k = 0; k < p.K; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < c.ndim; ++i) { const double d = x[(int64_t)i * W + w] - p.gmu[k * c.ndim + i]; s = s + d * d; }
+            q[(int64_t)k * W + w] = s;
+        }
+        q[(int64_t)p.K * W + w] = gmm_logp(c, p, q, W, w);
+        break;
+    }
+}
+
+__device__ inline double pdf_acceptance(const Chain &c, const PdfDev &p, int64_t w)
+{
+    const int64_t W = c.W;
+    const double *po = c.pold + (int64_t)p.off * W;
+    const double *pn = c.pnew + (int64_t)p.off * W;
+    switch (p.kind) {
+    case MCI_PDF_GAUSS_ISO:
+        return exp(-pn[w] + po[w]);
+    case MCI_PDF_GAUSS_DIAG:
+    case MCI_PDF_EXP_DIAG: {
+        double e = 0.0;
+        for (int i = 0; i < p.nproto; ++i) { e = e + po[(int64_t)i * W + w]; }
+        double s = 0.0;
+        for (int i = 0; i < p.nproto; ++i) { s = s + pn[(int64_t)i * W + w]; }
+        e = e - s;
+        return exp(e);
+    }
+    case MCI_PDF_GMM_ISO:
+        return exp(pn[(int64_t)p.K * W + w] - po[(int64_t)p.K * W + w]);
+    }
+    return 0.0;
+}
+
+// SamplingFunctionInterface::computeAcceptance (+ the updatedAcceptance of each kind)
+__device__ inline double pdf_compute_acceptance(const Chain &c, const PdfDev &p, const Lane &L)
+{
+    const int64_t W = c.W, w = L.w;
+    if (L.nchanged < c.ndim) {
+        double *po = c.pold + (int64_t)p.off * W;
+        double *pn = c.pnew + (int64_t)p.off * W;
+        if (p.kind == MCI_PDF_GAUSS_DIAG || p.kind == MCI_PDF_EXP_DIAG) {
+            double e = 0.0;
+            for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) {
+                const double v = c.xnew[(int64_t)j * W + w];
+                const double f = (p.kind == MCI_PDF_GAUSS_DIAG) ? (v * v) * pdf_a(p, j) : fabs(v) * pdf_a(p, j);
+                pn[(int64_t)j * W + w] = f;
+                e = e + (f - po[(int64_t)j * W + w]);
+            }
+            return exp(-e);
+        }
+        if (p.kind == MCI_PDF_GMM_ISO) {
+            for (int k = 0; k < p.K; ++k) {
+                double dr = 0.0;
+                for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) {
+                    const double mu = p.gmu[k * c.ndim + j];
+                    const double dn = c.xnew[(int64_t)j * W + w] - mu, dd = c.xold[(int64_t)j * W + w] - mu;
+                    dr = dr + (dn * dn - dd * dd);
+                }
+                pn[(int64_t)k * W + w] = po[(int64_t)k * W + w] + dr;
+            }
+            pn[(int64_t)p.K * W + w] = gmm_logp(c, p, pn, W, w);
+            return exp(pn[(int64_t)p.K * W + w] - po[(int64_t)p.K * W + w]);
+        }
+    }
+    pdf_proto(c, p, c.xnew, c.pnew, w);
+    return pdf_acceptance(c, p, w);
+}
+
+// --------------------------------------------------------------------------
+// domain (OrthoPeriodicDomain.cpp:38-61; UnboundDomain is a no-op)
+// --------------------------------------------------------------------------
+// The reference wraps by repeated addition/subtraction of the box length; that is kept
+// (bit-identical) for coordinates within 64 box lengths.  Farther coordinates are first
+// reduced with fmod so every loop is bounded; non-finite coordinates are left unchanged.
+__device__ __forceinline__ void wrap(const Chain &c, int i, double &x)
+{
+    const double lo = c.lb[i], hi = c.ub[i], len = hi - lo;
+    if (!isfinite(x)) { return; }
+    if (x < lo - 64.0 * len || x > hi + 64.0 * len) {
+        double r = fmod(x - lo, len);
+        if (r < 0.0) { r = r + len; }
+        x = lo + r;
+    }
+    int it = 0;
+    while (x < lo && it < 130) { x = x + len; ++it; }
+    while (x > hi && it < 260) { x = x - len; ++it; }
+}
+
+// --------------------------------------------------------------------------
+// the Metropolis step
+// --------------------------------------------------------------------------
+// SRRDAllMove / SRRDVecMove trial moves (uniform SRRD)
+__device__ inline double trial_move(const Chain &c, Lane &L)
+{
+    const int64_t W = c.W, w = L.w;
+    if (c.movetype == MCI_MOVE_ALL) {
+        int xidx = 0;
+        for (int t = 0; t < c.ntypes; ++t) {
+            const double st = c.steps[(int64_t)t * W + w];
+            while (xidx < c.typeEnds[t]) {
+                double &xn = c.xnew[(int64_t)xidx * W + w];
+                xn = xn + st * draw_sym(c, L);
+                ++xidx;
+            }
+        }
+        L.nchanged = c.ndim;
+        L.cfirst = 0;
+        return 1.;
+    }
+    const int vidx = draw_index(c, L, c.nvecs);
+    const int xidx = vidx * c.veclen;
+    int t = 0;
+    while (t < c.ntypes) {
+        if (xidx < c.typeEnds[t]) { break; }
+        ++t;
+    }
+    const double st = c.steps[(int64_t)t * W + w];
+    for (int i = 0; i < c.veclen; ++i) {
+        double &xn = c.xnew[(int64_t)(xidx + i) * W + w];
+        xn = xn + st * draw_sym(c, L);
+    }
+    L.nchanged = c.veclen;
+    L.cfirst = xidx;
+    return 1.;
+}
+
+// MCI::initializeSampling (MCIntegrator.cpp:252-271)
+__device__ inline void init_sampling(const Chain &c, Lane &L)
+{
+    const int64_t W = c.W, w = L.w;
+    L.acc = 0;
+    L.rej = 0;
+    for (int i = 0; i < c.ndim; ++i) { c.xnew[(int64_t)i * W + w] = c.xold[(int64_t)i * W + w]; }
+    L.nchanged = c.ndim;
+    L.cfirst = 0;
+    L.accepted = true;
+    for (int p = 0; p < c.npdf; ++p) { pdf_proto(c, c.pdf[p], c.xold, c.pold, w); }
+    for (int i = 0; i < c.nproto; ++i) { c.pnew[(int64_t)i * W + w] = c.pold[(int64_t)i * W + w]; }
+}
+
+// MCI::doStepMRT2 (MCIntegrator.cpp:326-360)
+__device__ inline void step_mrt2(const Chain &c, Lane &L)
+{
+    const int64_t W = c.W, w = L.w;
+    step_begin(L);
+    const double moveAcc = trial_move(c, L);
+    if (c.periodic) {
+        for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) { wrap(c, j, c.xnew[(int64_t)j * W + w]); }
+    }
+    double pdfAcc = 1.;
+    for (int p = 0; p < c.npdf; ++p) { pdfAcc = pdfAcc * pdf_compute_acceptance(c, c.pdf[p], L); }
+    const double u = draw_unit(c, L);
+    L.accepted = (u <= pdfAcc * moveAcc);
+    if (L.accepted) { ++L.acc; } else { ++L.rej; }
+    step_end(c, L);
+    // commit or roll back the coordinates and proto values that can differ
+    if (L.accepted) {
+        for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) { c.xold[(int64_t)j * W + w] = c.xnew[(int64_t)j * W + w]; }
+        for (int i = 0; i < c.nproto; ++i) { c.pold[(int64_t)i * W + w] = c.pnew[(int64_t)i * W + w]; }
+    }
+    else {
+        for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) { c.xnew[(int64_t)j * W + w] = c.xold[(int64_t)j * W + w]; }
+        for (int i = 0; i < c.nproto; ++i) { c.pnew[(int64_t)i * W + w] = c.pold[(int64_t)i * W + w]; }
+    }
+}
+
+// MCI::doStepRandom (MCIntegrator.cpp:362-376): no sampling function
+__device__ inline void step_random(const Chain &c, Lane &L)
+{
+    const int64_t W = c.W, w = L.w;
+    step_begin(L);
+    for (int i = 0; i < c.ndim; ++i) { c.xnew[(int64_t)i * W + w] = draw_unit(c, L); }
+    step_end(c, L);
+    for (int i = 0; i < c.ndim; ++i) {
+        double &x = c.xnew[(int64_t)i * W + w];
+        if (c.periodic) { x = c.lb[i] + x * (c.ub[i] - c.lb[i]); }
+        else { x = -(double)3.402823466e+38f + x * ((double)3.402823466e+38f + (double)3.402823466e+38f); }
+        c.xold[(int64_t)i * W + w] = x;
+    }
+    L.nchanged = c.ndim;
+    L.cfirst = 0;
+    L.accepted = true;
+    ++L.acc;
+}
+
+// --------------------------------------------------------------------------
+// observables (ObservableFunctionInterface::observableFunction of each kind)
+// --------------------------------------------------------------------------
+__device__ inline void obs_eval(const Chain &c, const ObsDev &o, int64_t w)
+{
+    const int64_t W = c.W;
+    const double *x = c.xold;
+    double *out = c.obsval + (int64_t)o.voff * W;
+    switch (o.kind) {
+    case MCI_OBS_X2:
+        for (int i = 0; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; out[(int64_t)i * W + w] = v * v; }
+        break;
+    case MCI_OBS_X2_SUM: {
+        double s = 0.0;
+        for (int i = 0; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; s = s + v * v; }
+        out[w] = s / o.p0;
+        break;
+    }
+    case MCI_OBS_X:
+        for (int i = 0; i < c.ndim; ++i) { out[(int64_t)i * W + w] = x[(int64_t)i * W + w]; }
+        break;
+    case MCI_OBS_X_SUM: {
+        double s = 0.0;
+        for (int i = 0; i < c.ndim; ++i) { s = s + x[(int64_t)i * W + w]; }
+        out[w] = s;
+        break;
+    }
+    case MCI_OBS_CONST:
+        out[w] = o.p0;
+        break;
+    case MCI_OBS_EXP_X2: {
+        double s = 0.0;
+        for (int i = 0; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; s = s + v * v; }
+        out[w] = exp(-(s / o.p0));
+        break;
+    }
+    case MCI_OBS_INDICATOR:
+        out[w] = (x[(int64_t)(int)o.p0 * W + w] > 0.0) ? 1.0 : 0.0;
+        break;
+    }
+}
+
+// One accumulation step of every observable (AccumulatorInterface::accumulate and the
+// storage of Simple/Block/FullAccumulator).  Observable values are a function of the
+// current position, so they are re-evaluated exactly when the position changed since the
+// last evaluation (the reference's _nchanged bookkeeping) — the stored numbers are equal.
+__device__ inline void accumulate(const Chain &c, const Lane &L, int64_t s, uint32_t &dirty)
+{
+    const int64_t W = c.W, w = L.w;
+    if (L.accepted) { dirty = 0xffffffffu; }
+    for (int i = 0; i < c.nobs; ++i) {
+        const ObsDev &o = c.obs[i];
+        if (s % o.nskip != 0) { continue; }
+        if (dirty & (1u << i)) {
+            obs_eval(c, o, w);
+            dirty &= ~(1u << i);
+        }
+        const int64_t a = s / o.nskip;
+        const double *v = c.obsval + (int64_t)o.voff * W;
+        if (o.acc == ACC_FULL) {
+            for (int j = 0; j < o.nobs; ++j) { o.data[(a * o.nobs + j) * W + w] = v[(int64_t)j * W + w]; }
+        }
+        else {
+            const int64_t b = (o.acc == ACC_BLOCK) ? a / o.blocksize : 0;
+            for (int j = 0; j < o.nobs; ++j) {
+                double &d = o.data[(b * o.nobs + j) * W + w];
+                d = d + v[(int64_t)j * W + w];
+            }
+        }
+    }
+}
+
+// AccumulatorInterface::finalize of Simple (x 1/NAccu) and Block (x 1/blocksize)
+__device__ inline void finalize_accu(const Chain &c, int64_t w, int64_t nsteps)
+{
+    const int64_t W = c.W;
+    for (int i = 0; i < c.nobs; ++i) {
+        const ObsDev &o = c.obs[i];
+        if (o.acc == ACC_FULL) { continue; }
+        double normf;
+        if (o.acc == ACC_SIMPLE) { normf = 1. / (double)(1 + (nsteps - 1) / o.nskip); }
+        else { normf = 1. / o.blocksize; }
+        for (int64_t r = 0; r < o.nstore * o.nobs; ++r) {
+            double &d = o.data[r * W + w];
+            d = d * normf;
+        }
+    }
+}
+
+__device__ inline void zero_accu(const Chain &c, int64_t w)
+{
+    const int64_t W = c.W;
+    for (int i = 0; i < c.nobs; ++i) {
+        const ObsDev &o = c.obs[i];
+        if (o.acc == ACC_FULL) { continue; }
+        for (int64_t r = 0; r < o.nstore * o.nobs; ++r) { o.data[r * W + w] = 0.0; }
+    }
+}
+
+__device__ inline void load_lane(const Chain &c, Lane &L, int64_t w)
+{
+    L.w = w;
+    L.mtp = (c.rng == MCI_RNG_MT19937_64) ? c.mtp[w] : 0;
+    L.t = (c.rng == MCI_RNG_PHILOX) ? c.ctr[w] : 0;
+    L.k = 0;
+    L.kx = 0;
+    L.nchanged = c.ndim;
+    L.cfirst = 0;
+    L.accepted = true;
+    L.acc = 0;
+    L.rej = 0;
+}
+
+__device__ inline void store_lane(const Chain &c, const Lane &L)
+{
+    if (c.rng == MCI_RNG_MT19937_64) { c.mtp[L.w] = L.mtp; }
+    else { c.ctr[L.w] = L.t; }
+    c.acc[L.w] = L.acc;
+    c.rej[L.w] = L.rej;
+}
+} // namespace mcid

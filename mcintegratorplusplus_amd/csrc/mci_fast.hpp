@@ -1,0 +1,22 @@
+// mci_fast.hpp — specialised sampling kernels for the hot configuration family:
+// single-coordinate uniform moves on one diagonal Gaussian (MCI_PDF_GAUSS_DIAG), unbounded
+// domain, observables from {X2, X2_SUM, EXP_X2} with Block/Simple accumulators.
+// Walker coordinates live in LDS, block sums are accumulated by dwell time, and the
+// FP64 acceptance test is pre-screened with the FP32 exponential.
+#pragma once
+
+#include "mci_device.hpp"
+
+namespace mcid
+{
+struct FastState
+{
+    float *x32 = nullptr;   // FP32 walker coordinates [ndim][W]
+    size_t nx32 = 0;
+};
+
+bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind);
+// nsteps Metropolis steps; obs == nullptr: no accumulation (decorrelation)
+int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s);
+void fast_release(FastState &fs);
+} // namespace mcid

@@ -1,0 +1,460 @@
+// mci_kernels.hip — HIP kernels of the Metropolis path (gfx950).
+//
+//  k_seed_mt          per-walker std::mt19937_64 seeding (libstdc++ seed sequence)
+//  k_sample           MCI::sample: init + nsteps Metropolis steps (+ accumulation)   MCIntegrator.cpp:252-321
+//  k_calibrate_indep  MCI::findMRT2Step, one independent loop per walker              MCIntegrator.cpp:87-168
+//  k_scale_steps      the step update of findMRT2Step for a shared (rank-averaged) rate
+//  k_estimate         estimators per (walker, observable component)                  Estimators.cpp, MJBlocker.cpp
+//  k_chunk_sum / k_combine   deterministic cross-walker reduction, MPIMCI combine     MPIMCI.cpp:85-92
+//  k_move_x / k_new_random_x  MCI::moveX / MCI::newRandomX                          MCIntegrator.cpp:606-619
+#include "mci_kernels.hpp"
+
+#include <float.h>
+
+namespace mcid
+{
+// std::mt19937_64::seed(s) (libstdc++ bits/random.tcc): x0 = s, x_i = f*(x_{i-1} ^ (x_{i-1} >> 62)) + i
+__global__ void k_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) { return; }
+    uint64_t v = seeds[w];
+    mt[w] = v;
+    for (int i = 1; i < MT_N; ++i) {
+        v ^= v >> 62;
+        v = v * 6364136223846793005ULL + (uint64_t)i;
+        mt[(int64_t)i * W + w] = v;
+    }
+    mtp[w] = 0;
+}
+
+__global__ void k_sample(Chain c, int64_t nsteps, int accumulate_obs, const uint8_t *active)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) { return; }
+    if (active && !active[w]) { return; }
+    Lane L;
+    load_lane(c, L, w);
+    init_sampling(c, L);
+    uint32_t dirty = 0xffffffffu;
+    if (accumulate_obs) { zero_accu(c, w); }
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if (c.npdf > 0) { step_mrt2(c, L); }
+        else { step_random(c, L); }
+        if (accumulate_obs) { accumulate(c, L, s, dirty); }
+    }
+    if (accumulate_obs) { finalize_accu(c, w, nsteps); }
+    store_lane(c, L);
+}
+
+// step clamps of findMRT2Step (MCIntegrator.cpp:147-161)
+__device__ inline void scale_and_clamp(const Chain &c, int64_t w, double fact, const double *dimSizes)
+{
+    const int64_t W = c.W;
+    for (int t = 0; t < c.ntypes; ++t) { double &s = c.steps[(int64_t)t * W + w]; s = s * fact; }
+    for (int i = 0; i < c.ndim; ++i) {
+        int t = 0;
+        while (t < c.ntypes && !(i < c.typeEnds[t])) { ++t; }
+        double &s = c.steps[(int64_t)t * W + w];
+        if (s > 0.5 * dimSizes[i]) { s = 0.5 * dimSizes[i]; }
+    }
+    for (int t = 0; t < c.ntypes; ++t) {
+        double &s = c.steps[(int64_t)t * W + w];
+        if (s < (double)FLT_MIN) { s = (double)FLT_MIN; }
+    }
+}
+
+__global__ void k_calibrate_indep(Chain c, int nfind, double target, int64_t min_stat, const double *dimSizes)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) { return; }
+    Lane L;
+    load_lane(c, L, w);
+    int cons = 0, counter = 0;
+    while ((nfind < 0 && cons < 5) || counter < nfind) {
+        init_sampling(c, L);
+        for (int64_t s = 0; s < min_stat; ++s) { step_mrt2(c, L); }
+        const double rate = (L.acc > 0) ? (double)L.acc / ((double)L.acc + (double)L.rej) : 0.;
+        if (fabs(rate - target) < 0.05) { ++cons; }
+        else { cons = 0; }
+        const double fact = fmin(2., fmax(0.5, rate / target));
+        scale_and_clamp(c, w, fact, dimSizes);
+        ++counter;
+        if (nfind < 0 && counter >= -nfind) { break; }
+    }
+    store_lane(c, L);
+}
+
+__global__ void k_scale_steps(Chain c, const double *fact, const double *dimSizes)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) { return; }
+    scale_and_clamp(c, w, *fact, dimSizes);
+}
+
+__global__ void k_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) { return; }
+    rate[w] = (acc[w] > 0) ? (double)acc[w] / ((double)acc[w] + (double)rej[w]) : 0.;
+}
+
+// ---- estimators (one thread per walker and observable component) ----
+struct Col
+{
+    const double *p;
+    int64_t stride;
+    __device__ double operator[](int64_t i) const { return p[i * stride]; }
+};
+
+// OneDim/MultiDimUncorrelatedEstimator (Estimators.cpp:37-57 / 125-155): identical sums,
+// the 1-D form divides by (n-1.), the N-D form multiplies by 1./(n-1.)
+template <class X>
+__device__ inline void uncorr(int64_t n, const X &x, bool onedim, double &avg, double &err)
+{
+    double a = 0.0, e = 0.0;
+    for (int64_t i = 0; i < n; ++i) { const double v = x[i]; a = a + v; e = e + v * v; }
+    const double norm = 1. / (double)n;
+    a = a * norm;
+    e = e * norm - a * a;
+    if (e > 1.e-300) { e = onedim ? sqrt(e / ((double)n - 1.)) : sqrt(e * (1. / ((double)n - 1.))); }
+    else { e = 0.; }
+    avg = a;
+    err = e;
+}
+
+struct Arr
+{
+    const double *p;
+    __device__ double operator[](int64_t i) const { return p[i]; }
+};
+
+// OneDim/MultiDimBlockEstimator (Estimators.cpp:60-77 / 158-185) into av[nblocks]
+__device__ inline void block_est(int64_t n, const Col &x, int64_t nblocks, bool onedim, double *av, double &avg, double &err)
+{
+    const int64_t nper = n / nblocks;
+    const double norm = 1. / (double)nper;
+    for (int64_t b = 0; b < nblocks; ++b) {
+        double s = 0.0;
+        for (int64_t i = b * nper; i < (b + 1) * nper; ++i) { s = s + x[i]; }
+        av[b] = s * norm;
+    }
+    uncorr(nblocks, Arr{av}, onedim, avg, err);
+}
+
+__device__ inline double err_delta(int mode, const double *e)
+{
+    const int im = 4;
+    switch (mode) {
+    case 1: return (-0.5 * e[im - 1] + 0.5 * e[im + 1]);
+    case 2: return ((1. / 12.) * e[im - 2] - (2. / 3.) * e[im - 1] + (2. / 3.) * e[im + 1] - (1. / 12.) * e[im + 2]);
+    case 3: return (-(1. / 60.) * e[im - 3] + (3. / 20.) * e[im - 2] - 0.75 * e[im - 1] + 0.75 * e[im + 1]
+                    - (3. / 20.) * e[im + 2] + (1. / 60.) * e[im + 3]);
+    case 4: return ((1. / 280.) * e[im - 4] - (4. / 105.) * e[im - 3] + 0.2 * e[im - 2] - 0.8 * e[im - 1]
+                    + 0.8 * e[im + 1] - 0.2 * e[im + 2] + (4. / 105.) * e[im + 3] - (1. / 280.) * e[im + 4]);
+    }
+    return 0.;
+}
+
+// FCBlocker (Estimators.cpp:83-122, 191-246): 45 block estimates (6..50 blocks), plateau by
+// the smallest summed central difference, 5-point average around it
+__device__ inline void fcblocker(int64_t n, const Col &x, bool onedim, double &avg, double &err)
+{
+    double av[45], er[45], tmp[50], acd[37];
+    for (int i = 0; i < 45; ++i) { block_est(n, x, i + 6, onedim, tmp, av[i], er[i]); }
+    for (int i = 0; i < 37; ++i) { acd[i] = 0.; }
+    for (int i2 = 4; i2 < 41; ++i2) {
+        for (int i1 = 1; i1 <= 4; ++i1) { acd[i2 - 4] = acd[i2 - 4] + err_delta(i1, er + i2 - 4); }
+    }
+    int im = 0;
+    for (int i2 = 1; i2 < 37; ++i2) {
+        if (fabs(acd[i2]) < fabs(acd[im])) { im = i2; }
+    }
+    im += 4;
+    avg = 0.2 * (av[im - 2] + av[im - 1] + av[im] + av[im + 1] + av[im + 2]);
+    err = 0.2 * (er[im - 2] + er[im - 1] + er[im] + er[im + 1] + er[im + 2]);
+}
+
+__constant__ double c_chi2_q95[64] = {
+    3.841459, 5.991465, 7.814728, 9.487729, 11.070498, 12.591587, 14.067140, 15.507313,
+    16.918978, 18.307038, 19.675138, 21.026070, 22.362032, 23.684791, 24.995790, 26.296228,
+    27.587112, 28.869299, 30.143527, 31.410433, 32.670573, 33.924438, 35.172462, 36.415029,
+    37.652484, 38.885139, 40.113272, 41.337138, 42.556968, 43.772972, 44.985343, 46.194260,
+    47.399884, 48.602367, 49.801850, 50.998460, 52.192320, 53.383541, 54.572228, 55.758479,
+    56.942387, 58.124038, 59.303512, 60.480887, 61.656233, 62.829620, 64.001112, 65.170769,
+    66.338649, 67.504807, 68.669294, 69.832160, 70.993453, 72.153216, 73.311493, 74.468324,
+    75.623748, 76.777803, 77.930524, 79.081944, 80.232098, 81.381015, 82.528727, 83.675261};
+
+// Automatic blocking of Jonsson (PRE 98, 043304 (2018)) as organised in MJBlocker.cpp:127-154.
+// The blocking transform runs in place on this thread's column of the scratch copy `y`.
+// A level whose variance is exactly zero contributes M = 0 (the reference reads out of
+// bounds there).
+__device__ inline void mjblocker(int64_t n, const Col &x, double *y, int64_t ystride, double &avg, double &err)
+{
+    int npow = 0;
+    while (((int64_t)1 << npow) < n) { ++npow; }
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) { const double v = x[i]; y[i * ystride] = v; mean = mean + v; }
+    mean = mean / (double)n;
+    double var[64], gam[64];
+    int64_t nred = n;
+    for (int k = 0; k < npow; ++k) {
+        double g0 = 0.0, g1 = 0.0;
+        double prev = y[0] - mean;
+        g0 = g0 + prev * prev;
+        for (int64_t i = 1; i < nred; ++i) {
+            const double cur = y[i * ystride] - mean;
+            g0 = g0 + cur * cur;
+            g1 = g1 + prev * cur;
+            prev = cur;
+        }
+        var[k] = g0 / (double)nred;
+        gam[k] = g1 / (double)nred;
+        for (int64_t i = 0; i < nred / 2; ++i) {
+            y[i * ystride] = 0.5 * (y[(2 * i) * ystride] + y[(2 * i + 1) * ystride]);
+        }
+        nred = nred / 2;
+    }
+    // M (reversed) and its cumulative sum; largest k with Msum_k < chi^2_{0.95}(k+1)
+    double M[64];
+    for (int i = 0; i < npow; ++i) {
+        const double v = var[i], g = gam[i];
+        const double r = g / v;
+        M[npow - i - 1] = (v == 0.0) ? 0.0 : (r * r) * ldexp(1.0, npow - i);
+    }
+    int kk = -1;
+    double msum = 0.0;
+    double ms[64];
+    for (int i = 0; i < npow; ++i) { msum = msum + M[i]; ms[i] = msum; }
+    for (kk = npow - 1; kk >= 0; --kk) {
+        if (ms[kk] < c_chi2_q95[kk]) { break; }
+    }
+    int k = npow - (kk + 1);
+    if (npow == 0) { avg = mean; err = 0.0; return; }
+    if (k >= npow) { k = npow - 1; }
+    avg = mean;
+    err = sqrt(var[k] / ldexp(1.0, npow - k));
+}
+
+__global__ void k_estimate(EstimJob j)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= j.W * j.nobs) { return; }
+    const int64_t w = tid % j.W;
+    const int comp = (int)(tid / j.W);
+    if (j.active && !j.active[w]) { return; }
+    const Col x{j.data + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W};
+    const bool onedim = (j.nobs == 1);
+    double a = 0., e = 0.;
+    switch (j.estim) {
+    case MCI_ESTIM_NOOP: a = x[0]; e = 0.; break;
+    case MCI_ESTIM_UNCORRELATED: uncorr(j.n, x, onedim, a, e); break;
+    case MCI_ESTIM_FCBLOCKER: fcblocker(j.n, x, onedim, a, e); break;
+    case MCI_ESTIM_MJBLOCKER:
+        mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e);
+        break;
+    case MCI_ESTIM_CORRELATED:
+        if ((j.n & (j.n - 1)) != 0) { fcblocker(j.n, x, onedim, a, e); }
+        else { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
+        break;
+    }
+    j.avg[(int64_t)(j.off + comp) * j.W + w] = a * j.scale;
+    j.err[(int64_t)(j.off + comp) * j.W + w] = e * j.scale;
+}
+
+// ---- deterministic cross-walker sums: chunks of 256 (pairwise tree), chunks in order ----
+__global__ void k_chunk_sum(const double *v, int64_t W, int ncomp, int square, double *chunks, int64_t nchunks)
+{
+    __shared__ double buf[256];
+    const int64_t chunk = blockIdx.x;
+    const int comp = blockIdx.y;
+    const int64_t i = chunk * 256 + threadIdx.x;
+    double x = (i < W) ? v[(int64_t)comp * W + i] : 0.0;
+    if (square) { x = x * x; }
+    buf[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { buf[threadIdx.x] = buf[threadIdx.x] + buf[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { chunks[(int64_t)comp * nchunks + chunk] = buf[0]; }
+}
+
+// MPIMCI::integrate combine (MPIMCI.cpp:85-92): avg = sum/W, err = sqrt(sum err^2)/W
+__global__ void k_combine(const double *csum_avg, const double *csum_err2, int64_t nchunks, int ncomp, int64_t W,
+                          double *avg, double *err)
+{
+    const int comp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (comp >= ncomp) { return; }
+    double sa = csum_avg[(int64_t)comp * nchunks], se = csum_err2[(int64_t)comp * nchunks];
+    for (int64_t k = 1; k < nchunks; ++k) {
+        sa = sa + csum_avg[(int64_t)comp * nchunks + k];
+        se = se + csum_err2[(int64_t)comp * nchunks + k];
+    }
+    avg[comp] = sa / (double)W;
+    err[comp] = sqrt(se) / (double)W;
+}
+
+// plain ordered sum of the chunk sums (rates), result / W
+__global__ void k_mean_of_chunks(const double *chunks, int64_t nchunks, int64_t W, double *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) { return; }
+    double s = chunks[0];
+    for (int64_t k = 1; k < nchunks; ++k) { s = s + chunks[k]; }
+    *out = s / (double)W;
+}
+
+// convergence test of the automatic decorrelation per independent walker
+// (MCIntegrator.cpp:229-240): keep sampling while any |old-new| > 2 sqrt(olderr^2 + newerr^2)
+__global__ void k_decorr_compare(double *oa, double *oe, const double *na, const double *ne, int nd, int64_t W, uint8_t *active)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W || !active[w]) { return; }
+    bool again = false;
+    for (int i = 0; i < nd; ++i) {
+        const double o = oa[(int64_t)i * W + w], n = na[(int64_t)i * W + w];
+        const double eo = oe[(int64_t)i * W + w], en = ne[(int64_t)i * W + w];
+        if (fabs(o - n) > 2 * sqrt(eo * eo + en * en)) { again = true; break; }
+    }
+    for (int i = 0; i < nd; ++i) {
+        oa[(int64_t)i * W + w] = na[(int64_t)i * W + w];
+        oe[(int64_t)i * W + w] = ne[(int64_t)i * W + w];
+    }
+    active[w] = again ? 1 : 0;
+}
+
+// MCI::moveX (MCIntegrator.cpp:606-612)
+__global__ void k_move_x(Chain c)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) { return; }
+    Lane L;
+    load_lane(c, L, w);
+    const int64_t W = c.W;
+    for (int i = 0; i < c.ndim; ++i) { c.xnew[(int64_t)i * W + w] = c.xold[(int64_t)i * W + w]; }
+    step_begin(L);
+    trial_move(c, L);
+    step_end(c, L);
+    for (int j = 0; j < c.ndim; ++j) {
+        double x = c.xnew[(int64_t)j * W + w];
+        if (c.periodic && j >= L.cfirst && j < L.cfirst + L.nchanged) { wrap(c, j, x); }
+        c.xold[(int64_t)j * W + w] = x;
+    }
+    if (c.rng == MCI_RNG_MT19937_64) { c.mtp[w] = L.mtp; } else { c.ctr[w] = L.t; }
+}
+
+// MCI::newRandomX (MCIntegrator.cpp:614-619)
+__global__ void k_new_random_x(Chain c)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) { return; }
+    Lane L;
+    load_lane(c, L, w);
+    const int64_t W = c.W;
+    step_begin(L);
+    for (int i = 0; i < c.ndim; ++i) { c.xnew[(int64_t)i * W + w] = draw_unit(c, L); }
+    step_end(c, L);
+    for (int i = 0; i < c.ndim; ++i) {
+        double x = c.xnew[(int64_t)i * W + w];
+        if (c.periodic) { x = c.lb[i] + x * (c.ub[i] - c.lb[i]); }
+        else { x = -(double)FLT_MAX + x * ((double)FLT_MAX + (double)FLT_MAX); }
+        c.xold[(int64_t)i * W + w] = x;
+    }
+    if (c.rng == MCI_RNG_MT19937_64) { c.mtp[w] = L.mtp; } else { c.ctr[w] = L.t; }
+}
+
+// periodic wrap of every coordinate of xold (MCI::setX/setIRange apply the domain)
+__global__ void k_apply_domain(Chain c)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W || !c.periodic) { return; }
+    for (int i = 0; i < c.ndim; ++i) { wrap(c, i, c.xold[(int64_t)i * c.W + w]); }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host)
+// ---------------------------------------------------------------------------
+static inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seed_mt, grid1(W, 256), dim3(256), 0, s, mt, mtp, seeds, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sample, grid1(c.W, 64), dim3(64), 0, s, c, nsteps, acc, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int64_t min_stat, const double *dimSizes, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_calibrate_indep, grid1(c.W, 64), dim3(64), 0, s, c, nfind, target, min_stat, dimSizes);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_steps(const Chain &c, const double *fact, const double *dimSizes, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale_steps, grid1(c.W, 256), dim3(256), 0, s, c, fact, dimSizes);
+    return hipGetLastError();
+}
+
+hipError_t launch_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_walker_rates, grid1(W, 256), dim3(256), 0, s, acc, rej, rate, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_estimate, grid1(j.W * j.nobs, 64), dim3(64), 0, s, j);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_sum(const double *v, int64_t W, int ncomp, int square, double *chunks, hipStream_t s)
+{
+    const int64_t nch = (W + 255) / 256;
+    hipLaunchKernelGGL(k_chunk_sum, dim3((unsigned)nch, (unsigned)ncomp), dim3(256), 0, s, v, W, ncomp, square, chunks, nch);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const double *ca, const double *ce, int64_t W, int ncomp, double *avg, double *err, hipStream_t s)
+{
+    const int64_t nch = (W + 255) / 256;
+    hipLaunchKernelGGL(k_combine, grid1(ncomp, 64), dim3(64), 0, s, ca, ce, nch, ncomp, W, avg, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_mean_of_chunks(const double *chunks, int64_t W, double *out, hipStream_t s)
+{
+    const int64_t nch = (W + 255) / 256;
+    hipLaunchKernelGGL(k_mean_of_chunks, dim3(1), dim3(1), 0, s, chunks, nch, W, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_decorr_compare(double *oa, double *oe, const double *na, const double *ne, int nd, int64_t W, uint8_t *active,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_decorr_compare, grid1(W, 256), dim3(256), 0, s, oa, oe, na, ne, nd, W, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_x(const Chain &c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_move_x, grid1(c.W, 64), dim3(64), 0, s, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_new_random_x(const Chain &c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_new_random_x, grid1(c.W, 64), dim3(64), 0, s, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_domain(const Chain &c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_apply_domain, grid1(c.W, 256), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
+} // namespace mcid

@@ -1,0 +1,35 @@
+// mci_kernels.hpp — launch interface between the host orchestration (mci_api.cpp) and the
+// HIP kernels (mci_kernels.hip, mci_fast.hip).
+#pragma once
+
+#include "mci_device.hpp"
+
+namespace mcid
+{
+// one estimator pass over one observable's accumulator data [n][nobs][W]
+struct EstimJob
+{
+    const double *data;
+    double *scratch;      // same layout as data (MJBlocker's in-place blocking)
+    int64_t n, W;
+    int nobs, off, estim;
+    double scale;         // volume factor (no sampling function), else 1
+    const uint8_t *active;
+    double *avg, *err;    // [nobsdim][W]
+};
+
+hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s);
+hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s);
+hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int64_t min_stat, const double *dimSizes, hipStream_t s);
+hipError_t launch_scale_steps(const Chain &c, const double *fact, const double *dimSizes, hipStream_t s);
+hipError_t launch_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W, hipStream_t s);
+hipError_t launch_estimate(const EstimJob &j, hipStream_t s);
+hipError_t launch_chunk_sum(const double *v, int64_t W, int ncomp, int square, double *chunks, hipStream_t s);
+hipError_t launch_combine(const double *ca, const double *ce, int64_t W, int ncomp, double *avg, double *err, hipStream_t s);
+hipError_t launch_mean_of_chunks(const double *chunks, int64_t W, double *out, hipStream_t s);
+hipError_t launch_decorr_compare(double *oa, double *oe, const double *na, const double *ne, int nd, int64_t W, uint8_t *active,
+                                 hipStream_t s);
+hipError_t launch_move_x(const Chain &c, hipStream_t s);
+hipError_t launch_new_random_x(const Chain &c, hipStream_t s);
+hipError_t launch_apply_domain(const Chain &c, hipStream_t s);
+} // namespace mcid

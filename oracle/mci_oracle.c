@@ -286,10 +286,8 @@ static uint64_t splitmix64(uint64_t *s)
 static uint64_t walker_seed(uint64_t seed, int64_t w)
 {
     if (w == 0) return seed;
-    uint64_t s = seed ^ 0x6A09E667F3BCC909ULL;
-    uint64_t z = 0;
-    for (int64_t i = 0; i < 1; ++i) { s += (uint64_t)w * 0x9E3779B97F4A7C15ULL; z = splitmix64(&s); }
-    return z;
+    uint64_t s = (seed ^ 0x6A09E667F3BCC909ULL) + (uint64_t)w * 0x9E3779B97F4A7C15ULL;
+    return splitmix64(&s);
 }
 
 static uint32_t ph_word(const struct orc_ctx *c, walker *w, uint32_t k)
@@ -1114,13 +1112,27 @@ mci_status orc_set_ndecorrelation_steps(orc_ctx *c, int64_t n) { c->ndecorr = n;
 static const double DC_INF = (double)FLT_MAX;
 static const double DC_INFX2 = (double)FLT_MAX + (double)FLT_MAX;
 
+/* The reference wraps with unbounded while loops (OrthoPeriodicDomain.cpp:41-46); the rule
+ * below is identical for coordinates within 64 box lengths and bounded beyond (fmod first),
+ * non-finite coordinates are left unchanged.  The device uses the same rule. */
+static void wrap1(double *x, double lo, double hi)
+{
+    const double len = hi - lo;
+    if (!isfinite(*x)) return;
+    if (*x < lo - 64.0 * len || *x > hi + 64.0 * len) {
+        double r = fmod(*x - lo, len);
+        if (r < 0.0) r = r + len;
+        *x = lo + r;
+    }
+    int it = 0;
+    while (*x < lo && it < 130) { *x = *x + len; ++it; }
+    while (*x > hi && it < 260) { *x = *x - len; ++it; }
+}
+
 static void dom_apply_all(const orc_ctx *c, double *x)
 {
     if (!c->periodic) return;
-    for (int i = 0; i < c->ndim; ++i) {
-        while (x[i] < c->lb[i]) x[i] = x[i] + (c->ub[i] - c->lb[i]);
-        while (x[i] > c->ub[i]) x[i] = x[i] - (c->ub[i] - c->lb[i]);
-    }
+    for (int i = 0; i < c->ndim; ++i) wrap1(&x[i], c->lb[i], c->ub[i]);
 }
 
 static void dom_apply_wlk(const orc_ctx *c, walker *w)
@@ -1128,8 +1140,7 @@ static void dom_apply_wlk(const orc_ctx *c, walker *w)
     if (!c->periodic) return;
     for (int i = 0; i < w->nchanged; ++i) {
         int j = w->changedIdx[i];
-        while (w->xnew[j] < c->lb[j]) w->xnew[j] = w->xnew[j] + (c->ub[j] - c->lb[j]);
-        while (w->xnew[j] > c->ub[j]) w->xnew[j] = w->xnew[j] - (c->ub[j] - c->lb[j]);
+        wrap1(&w->xnew[j], c->lb[j], c->ub[j]);
     }
 }
 
@@ -1703,8 +1714,8 @@ mci_status orc_integrate(orc_ctx *c, int64_t nmc, double *average, double *error
             TRY(accu_allocate(&c->acc_last[k * c->nobs + i], nmc));
         }
     free(c->wavg); free(c->werr);
-    c->wavg = calloc((size_t)(W * (nd > 0 ? nd : 1)), sizeof(double));
-    c->werr = calloc((size_t)(W * (nd > 0 ? nd : 1)), sizeof(double));
+    c->wavg = calloc((size_t)W * (size_t)(nd > 0 ? nd : 1), sizeof(double));
+    c->werr = calloc((size_t)W * (size_t)(nd > 0 ? nd : 1), sizeof(double));
     c->nobsdim_last = nd;
     c->trace_n = 0;
     for (int64_t k = 0; k < W; ++k)

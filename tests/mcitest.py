@@ -87,6 +87,7 @@ ORACLE_ONLY = {
 }
 MCI_ONLY = {
     "get_timings": [_DP, _I],
+    "get_observable_data": [_I, _DP, _I64, _I64P],
 }
 
 
@@ -270,6 +271,15 @@ class Chain:
         acc, rej, rate = _I64(), _I64(), _D()
         self.call("get_acceptance", C.byref(acc), C.byref(rej), C.byref(rate))
         return acc.value, rej.value, rate.value
+
+    def observableData(self, i, nobs):
+        """finalised accumulator data of observable i (nobs components), shape [W, nstore, nobs]
+        (HIP library only)"""
+        ns = _I64()
+        self.call("get_observable_data", i, None, 0, C.byref(ns))
+        buf = np.zeros(self.W * ns.value * nobs)
+        self.call("get_observable_data", i, dptr(buf), buf.size, C.byref(ns))
+        return buf.reshape(self.W, ns.value, nobs)
 
     def walkerAcceptance(self):
         a = np.zeros(self.W, dtype=np.int64)

@@ -1,0 +1,160 @@
+"""Parity of the HIP path (through the C-ABI) with the reference and the oracle.
+
+- reference streams (MT19937_64 mode): golden vectors written by the reference binary must be
+  reproduced bit for bit — averages, MJBlocker/FCBlocker errors, calibrated step sizes, final
+  positions, acceptance rates and every step of the traced chains;
+- native streams (Philox): the oracle restates the same draws; walkers and combined results
+  must match it bit for bit (FP64), at full configuration size through a walker subset;
+- physics: estimates within 4 sigma of the analytic values.
+"""
+import numpy as np
+import pytest
+
+import golden_io as G
+from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_MJ, OBS_EXP_X2, OBS_X, OBS_X2, OBS_X2_SUM,
+                     PDF_GAUSS_DIAG, RNG_MT, RNG_PHILOX, Chain, chain_setup, estimate)
+
+pytestmark = pytest.mark.gpu
+
+CHAINS = G.load("chains.json")["scenarios"]
+
+
+@pytest.mark.parametrize("sc", CHAINS, ids=[s["name"] for s in CHAINS])
+def test_reference_chains_bit_exact(sc):
+    print("scenario", sc["name"], flush=True)
+    c = chain_setup("mci", G.cfg_of(sc["cfg"]))
+    for call, r in zip(sc["calls"], sc["results"]):
+        if "irange" in call:
+            c.setIRange(*call["irange"])
+        if "setx" in call:
+            c.setX(call["setx"])
+        a, e = c.integrate(call["nmc"], call["find"], call["decorr"])
+        nd = c.getNObsDim()
+        assert np.array_equal(a[:nd], G.hexv(r["avg"])), (a[:nd], G.hexv(r["avg"]))
+        assert np.array_equal(e[:nd], G.hexv(r["err"])), (e[:nd], G.hexv(r["err"]))
+        assert np.array_equal(c.getWalkerSteps()[0], G.hexv(r["steps"]))
+        assert np.array_equal(c.getX()[0], G.hexv(r["x"]))
+        assert c.acceptance()[2] == float.fromhex(r["rate"])
+
+
+TRACES = G.load("traces.json")["traces"]
+
+
+@pytest.mark.parametrize("tr", TRACES, ids=[t["name"] for t in TRACES])
+def test_reference_step_traces_bit_exact(tr):
+    """Every position of the chain: a Full accumulator of x holds the trace (FullAccumulator.cpp:14-18)."""
+    cfg = G.cfg_of(tr["cfg"])
+    nd, n = cfg["ndim"], tr["nmc"]
+    cfg["obs"] = [(OBS_X, [], 1, 1, 0, 1)]
+    c = chain_setup("mci", cfg)
+    c.integrate(n, False, False)
+    xs = c.observableData(0, nd)[0]
+    ref_x = G.hexv(tr["result"]["trace_x"]).reshape(-1, nd)
+    assert np.array_equal(xs[:-1], ref_x[2:])
+    assert np.array_equal(xs[-1], G.hexv(tr["result"]["x"]))
+
+
+def test_reference_walkers_bit_exact():
+    d = G.load("walkers.json")
+    W = len(d["seeds"])
+    c = chain_setup("mci", G.cfg_of(d["cfg"]), nwalkers=W)
+    c.setWalkerSeeds(d["seeds"])
+    c.integrate(d["nmc"])
+    wa, we = c.walkerResults()
+    steps = c.getWalkerSteps()
+    for k, r in enumerate(d["results"]):
+        assert np.array_equal(wa[k], G.hexv(r["avg"])), k
+        assert np.array_equal(we[k], G.hexv(r["err"])), k
+        assert np.array_equal(steps[k], G.hexv(r["steps"])), k
+
+
+EST = G.load("estimators.json")["sets"]
+
+
+@pytest.mark.parametrize("s", EST, ids=[f"n{s['n']}_d{s['ndim']}" for s in EST])
+def test_device_estimators_bit_exact(s):
+    x = G.b64(s["x_b64"]).reshape(s["n"], s["ndim"])
+    for est, r in s["results"].items():
+        a, e = estimate("mci", int(est), x)
+        assert np.array_equal(a, G.hexv(r["avg"])), (est, a)
+        assert np.array_equal(e, G.hexv(r["err"])), (est, e)
+
+
+def sigma_a(n):
+    sig = G.hexv(G.load("sigmas.json")["sigma16" if n == 16 else "sigma64"])
+    return sig, 0.5 / (sig * sig)
+
+
+def cfg2(nmc_blocks=64, blocksize=160, ndecorr=1000):
+    sig, a = sigma_a(16)
+    return dict(ndim=16, seed=1337, ndecorr=ndecorr, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, a)],
+                obs=[(OBS_X2, [], blocksize, 1, 1, ESTIM_MJ), (OBS_X2_SUM, [], blocksize, 1, 1, ESTIM_MJ)])
+
+
+@pytest.mark.parametrize("rng", [RNG_MT, RNG_PHILOX])
+@pytest.mark.parametrize("coupling", [COUPLING_INDEPENDENT, COUPLING_SHARED])
+def test_oracle_parity_many_walkers(rng, coupling):
+    W, nmc = 200, 64 * 20
+    res = {}
+    for prefix in ("orc", "mci"):
+        c = chain_setup(prefix, cfg2(blocksize=20), nwalkers=W)
+        c.setRng(rng)
+        c.setCoupling(coupling)
+        a, e = c.integrate(nmc)
+        res[prefix] = (a, e, *c.walkerResults(), c.getWalkerSteps(), c.getX(), c.walkerAcceptance())
+    o, g = res["orc"], res["mci"]
+    for k in range(6):
+        assert np.array_equal(o[k], g[k]), k
+    assert np.array_equal(o[6][0], g[6][0]) and np.array_equal(o[6][1], g[6][1])
+
+
+def test_auto_decorrelation_and_calibration_parity():
+    """Default settings (auto calibration, auto decorrelation with FCBlocker) for many walkers."""
+    for coupling in (COUPLING_INDEPENDENT, COUPLING_SHARED):
+        res = {}
+        for prefix in ("orc", "mci"):
+            cfg = dict(ndim=3, seed=77, pdfs=[(1, [])], obs=[(OBS_X2, [], 1, 1, 1, 2)], x0=[2.0, -1.0, 3.0])
+            c = chain_setup(prefix, cfg, nwalkers=96)
+            c.setRng(RNG_PHILOX)
+            c.setCoupling(coupling)
+            a, e = c.integrate(4096)
+            res[prefix] = (a, e, *c.walkerResults(), c.getWalkerSteps(), c.getX())
+        for k in range(6):
+            assert np.array_equal(res["orc"][k], res["mci"][k]), (coupling, k)
+
+
+def test_full_size_configuration2_walker_subset():
+    """65,536 walkers x 10,240 steps (d=16): independent walkers 0..63 equal an oracle run of 64
+    walkers bit for bit (walker w's Philox stream depends only on (seed, w)); the combined
+    estimate agrees with sigma_i^2 and sum sigma_i^2 within 4 error bars."""
+    W, nmc = 65536, 10240
+    g = chain_setup("mci", cfg2(), nwalkers=W)
+    g.setRng(RNG_PHILOX)
+    a, e = g.integrate(nmc)
+    wa, we = g.walkerResults()
+    o = chain_setup("orc", cfg2(), nwalkers=64)
+    o.setRng(RNG_PHILOX)
+    o.integrate(nmc)
+    oa, oe = o.walkerResults()
+    assert np.array_equal(wa[:64], oa) and np.array_equal(we[:64], oe)
+    sig, _ = sigma_a(16)
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
+    # MPIMCI combine (MPIMCI.cpp:85-92) restated on the host
+    assert np.allclose(a, wa.mean(axis=0), rtol=1e-13)
+    assert np.allclose(e, np.sqrt((we ** 2).sum(axis=0)) / W, rtol=1e-13)
+
+
+def test_full_size_shared_coupling_physics():
+    W, nmc = 65536, 10240
+    g = chain_setup("mci", cfg2(), nwalkers=W)
+    g.setRng(RNG_PHILOX)
+    g.setCoupling(COUPLING_SHARED)
+    a, e = g.integrate(nmc)
+    steps = g.getWalkerSteps()
+    assert np.all(steps == steps[0, 0])  # one shared calibrated step (MCIntegrator.cpp:131-138)
+    acc, rej, rate = g.acceptance()
+    assert acc + rej == W * nmc and abs(rate - 0.5) < 0.06
+    sig, _ = sigma_a(16)
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
