@@ -1,0 +1,188 @@
+"""Benchmark of the Metropolis path on one MI355X.
+
+Workload (the GPU configuration of the benchmark set, FP64): 65,536 walkers x 10,240 steps,
+d = 16 anisotropic Gaussian p(x) ~ exp(-sum x_i^2 / (2 sigma_i^2)) with sigma_i ~ U(0.5, 2) from
+seed 1337 (tests/golden/sigmas.json), single-coordinate uniform moves with the step calibrated
+to 50 % acceptance (walkers coupled like MPI ranks: shared step, MIN_STAT 100), 1,000
+decorrelation steps, observables x_i^2 (16) and |x|^2 in 64 blocks of 160 steps, MJBlocker
+errors per walker, combined over walkers as MPIMCI does.
+
+One "step" of this benchmark = one complete MCI::integrate of that workload (calibration +
+decorrelation + sampling + estimation + combine), inputs resident on the GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  `value` is walker-steps/s of the production chain (65,536 x 10,240 per
+integrate) over the whole integrate; ESS/s, the roofline of the sampling kernel and the
+reference CPU path timed on this host's cores are reported beside it.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W, NMC, D, BLOCK, NDECORR = 65536, 10240, 16, 160, 1000
+# algorithmic work per walker-step of the sampling kernel (FP64): 4 (move) + 3 (delta log p)
+# + 2 (|x|^2 update) + 17 (block accumulation of 17 observable components) flops
+FLOPS_PER_STEP = 26
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector
+SEED = 1337
+
+
+def sigmas():
+    with open(os.path.join(ROOT, "tests", "golden", "sigmas.json")) as f:
+        return np.array([float.fromhex(v) for v in json.load(f)["sigma16"]])
+
+
+def make_mci(walkers, precision=0):
+    from mcintegratorplusplus_amd import MCI, constants as K
+    sig = sigmas()
+    m = MCI(D, walkers)
+    m.setSeed(SEED)
+    m.setRandomMode(K.RNG_PHILOX)
+    m.setPrecision(precision)
+    m.setWalkerCoupling(K.COUPLING_SHARED)
+    m.setTrialMove(K.SRRD_UNIFORM, 1, 1)
+    m.setNdecorrelationSteps(NDECORR)
+    m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+    m.addObservable(K.OBS_X2, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+    m.addObservable(K.OBS_X2_SUM, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+    return m
+
+
+def ess_per_obs(err, walkers_steps):
+    """ESS_j = s_j^2 / err_j^2 with the per-step variance s^2 of x_i^2 (2 sigma_i^4) and of |x|^2
+    (sum 2 sigma_i^4) under the Gaussian target."""
+    sig = sigmas()
+    s2 = np.concatenate([2 * sig ** 4, [np.sum(2 * sig ** 4)]])
+    return s2 / err ** 2
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference CPU path (oracle/_ref/mci_ref_fast: the reference library built from its
+    own sources with -O3 -flto) timed on this host: P processes, each integrating its share of
+    independent walkers of the same workload (per-process calibration, no MPI)."""
+    binary = os.path.join(ROOT, "oracle", "_ref", "mci_ref_fast")
+    sig = sigmas()
+    a = " ".join(float(v).hex() for v in 0.5 / sig ** 2)
+    script = (f"create {D}\nmove 0 1 1\nndecorr {NDECORR}\npdf 2 {D} {a}\n"
+              f"obs 1 0 {BLOCK} 1 1 4\nobs 2 0 {BLOCK} 1 1 4\nendbench\n")
+    procs = max(1, min(16, os.cpu_count() or 1))
+    if not os.path.exists(binary):
+        return None
+    # calibrate the per-walker cost with one process, then size the sample to the budget
+    t0 = time.time()
+    r = subprocess.run([binary], input=f"bench 8 {NMC} 9001\n" + script, capture_output=True, text=True, check=True)
+    per_walker = max((time.time() - t0) / 8, 1e-4)
+    per_proc = int(min(100000, max(8, budget_s / (procs * per_walker))))  # ~budget_s of CPU work in total
+    t0 = time.time()
+    ps = [subprocess.Popen([binary], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+          for _ in range(procs)]
+    for i, p in enumerate(ps):
+        p.stdin.write(f"bench {per_proc} {NMC} {100000 + i * per_proc}\n" + script)
+        p.stdin.close()
+    outs = [json.loads([l for l in p.stdout.read().splitlines() if l.startswith("{")][-1]) for p in ps]
+    for p in ps:
+        p.wait()
+    wall = time.time() - t0
+    nw = sum(o["walkers"] for o in outs)
+    steps = sum(o["walker_steps"] for o in outs)
+    err2 = np.sum([[float.fromhex(v) for v in o["err2_sum"]] for o in outs], axis=0)
+    err = np.sqrt(err2) / nw
+    ess = ess_per_obs(err, steps)
+    return dict(value=steps / wall, unit="walker-steps/s", cores=procs, kind="reference",
+                ess_per_s=float(np.min(ess)) / wall,
+                sample=f"{nw} walkers x {NMC} steps (+calibration, {NDECORR} decorrelation steps each) in {procs} "
+                       f"processes, {wall:.1f} s wall; reference built from its sources with -O3 -flto",
+                seconds=wall)
+
+
+def traffic_from_profile():
+    p = os.path.join(ROOT, "profiles", "traffic_cfg2_sample.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; the walkers of one GPU are the unit of "
+                                   "work (multi-GPU sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    m = make_mci(W, args.precision)
+    if args.generic:
+        m.setKernelPath(1)
+    for _ in range(args.warmup):
+        m.integrate(NMC)
+    times, ktimes, results = [], [], None
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        avg, err = m.integrate(NMC)
+        times.append(time.perf_counter() - t0)
+        ktimes.append(m.getTimings())
+        results = (avg, err)
+    total = time.perf_counter() - t_all
+    ms_per_step = 1000 * total / args.steps
+    kt = np.mean(ktimes, axis=0)  # calibration, decorrelation, sampling kernel, estimation, total (ms)
+    value = W * NMC / (ms_per_step / 1000)
+    avg, err = results
+    ess = ess_per_obs(err, W * NMC)
+    sig = sigmas()
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    achieved = FLOPS_PER_STEP * W * NMC / (kt[2] / 1000) / 1e12
+    line = {
+        "metric": "MC steps/s and ESS/s per GPU at matched error bar vs plain FP64 Metropolis",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{W} walkers x {NMC} steps, d={D} anisotropic Gaussian (sigma_i~U(0.5,2), seed 1337), "
+                               f"single-coordinate uniform moves, shared step calibration, {NDECORR} decorrelation "
+                               f"steps, x_i^2 and |x|^2 in 64 blocks of {BLOCK}, MJBlocker, MPIMCI combine",
+                   "walkers": W, "nmc": NMC, "ndim": D, "rng": "philox4x32-10",
+                   "parallelism": "one walker per lane, 1 GPU"},
+        "ess_per_s": float(np.min(ess)) / (ms_per_step / 1000),
+        "phase_ms": {"calibration": kt[0], "decorrelation": kt[1], "sampling_kernel": kt[2], "estimation": kt[3]},
+        "max_abs_z_vs_analytic": float(np.max(np.abs(avg - exact) / err)),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": traffic_from_profile(),
+                     "kernel": "sampling kernel (calibrated chain, 10240 steps x 65536 walkers)",
+                     "kernel_ms": kt[2]},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        if cb is not None:
+            line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            line["cpu_baseline"]["ess_per_s"] = cb["ess_per_s"]
+            line["speedup_vs_cpu_steps"] = value / cb["value"]
+            line["speedup_vs_cpu_ess"] = line["ess_per_s"] / cb["ess_per_s"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

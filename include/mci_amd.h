@@ -94,6 +94,10 @@ typedef int mci_status;
 #define MCI_COUPLING_INDEPENDENT 0 /* W independent MCI objects (USE_MPI=0 semantics per walker) */
 #define MCI_COUPLING_SHARED      1 /* W MPI ranks (USE_MPI=1 semantics, shared calibration/decorrelation) */
 
+/* ---- kernel selection (no counterpart in the reference: a tuning knob) ---- */
+#define MCI_KERNELS_AUTO     0 /* specialised kernels where the configuration allows, else generic */
+#define MCI_KERNELS_GENERIC  1 /* always the generic kernels (reference-order accumulation) */
+
 typedef struct mci_ctx mci_ctx;
 
 /* thread-local message of the last failing call (never NULL) */
@@ -115,6 +119,9 @@ mci_status mci_set_walker_seeds(mci_ctx *ctx, const uint64_t *seeds, int64_t nse
 mci_status mci_set_rng(mci_ctx *ctx, int rng_kind);
 mci_status mci_set_precision(mci_ctx *ctx, int precision);
 mci_status mci_set_coupling(mci_ctx *ctx, int coupling);
+/* MCI_KERNELS_AUTO / MCI_KERNELS_GENERIC.  Both give identical accept/reject decisions and
+ * positions; block means differ only by rounding. */
+mci_status mci_set_kernel_path(mci_ctx *ctx, int path);
 /* MCI::setTargetAcceptanceRate src/MCIntegrator.cpp:551-554 */
 mci_status mci_set_target_acceptance_rate(mci_ctx *ctx, double rate);
 /* MCI::setNfindMRT2Iterations / setNdecorrelationSteps MCIntegrator.hpp:115-123 (sign convention kept) */

@@ -135,6 +135,7 @@ struct mci_ctx
     int ndim = 0;
     int64_t W = 0;
     int rng = MCI_RNG_MT19937_64, prec = MCI_PREC_FP64, coupling = MCI_COUPLING_INDEPENDENT;
+    int kernel_path = MCI_KERNELS_AUTO;
     uint64_t seed = 5489u;
     std::vector<uint64_t> wseeds;
     bool reseed = true;
@@ -330,6 +331,14 @@ float elapsed(mci_ctx *c, int a, int b)
     return ms;
 }
 
+// the specialised kernels apply to this chain (observables are checked separately)
+bool fast_chain(const mci_ctx *c)
+{
+    return c->kernel_path == MCI_KERNELS_AUTO && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
+                         c->pdfs.empty() ? 0 : c->pdfs[0].kind) &&
+           !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
+}
+
 // MCI::findMRT2Step for walkers that share the rank-averaged rate (MCIntegrator.cpp:87-168, USE_MPI)
 mci_status find_shared(mci_ctx *c)
 {
@@ -341,8 +350,11 @@ mci_status find_shared(mci_ctx *c)
     HIPCHK(c->fact.ensure(2));
     int cons = 0, counter = 0;
     const int N = c->nfind;
+    const bool fast = fast_chain(c);
     while ((N < 0 && cons < 5) || counter < N) {
-        HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream));
+        // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
+        if (fast) { TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream)); }
+        else { HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream)); }
         HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
         HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
         HIPCHK(launch_mean_of_chunks(c->chunks_a.p, c->W, c->fact.p + 1, c->stream));
@@ -602,6 +614,14 @@ mci_status mci_set_coupling(mci_ctx *c, int m)
     CHECKCTX(c);
     if (m != MCI_COUPLING_INDEPENDENT && m != MCI_COUPLING_SHARED) { return fail(MCI_E_DOMAIN, "unknown coupling %d", m); }
     c->coupling = m;
+    return MCI_OK;
+}
+
+mci_status mci_set_kernel_path(mci_ctx *c, int path)
+{
+    CHECKCTX(c);
+    if (path != MCI_KERNELS_AUTO && path != MCI_KERNELS_GENERIC) { return fail(MCI_E_DOMAIN, "unknown kernel path %d", path); }
+    c->kernel_path = path;
     return MCI_OK;
 }
 
@@ -944,10 +964,10 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
     const int64_t W = c->W;
     HIPCHK(c->obsval.ensure((size_t)std::max(nd, 1) * W));
     for (double &t : c->timings) { t = 0.; }
-    const bool fast = fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
-                                    c->pdfs.empty() ? 0 : c->pdfs[0].kind);
+    const bool fast = fast_chain(c);
     if (c->prec == MCI_PREC_FP32 && !fast) {
-        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports single-coordinate uniform moves on one diagonal Gaussian in an unbounded domain");
+        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports Philox streams with single-coordinate uniform moves on one diagonal "
+                                            "Gaussian (explicit weights) in an unbounded domain");
     }
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // ---- calibration (MCIntegrator.cpp:49-51) ----
@@ -1000,7 +1020,11 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
     c->nstore_last.clear();
     for (auto &d : od) { c->nstore_last.push_back(d.nstore); }
     Chain ch = make_chain(c, &od);
-    if (fast) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
+    const bool fast_main = fast && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec);
+    if (c->prec == MCI_PREC_FP32 && !fast_main) {
+        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
+    }
+    if (fast_main) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
     else { HIPCHK(launch_sample(ch, nmc, 1, nullptr, c->stream)); }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     // ---- estimate (ObservableContainer.cpp:104-111), volume scaling (:71-77), combine ----

@@ -14,7 +14,7 @@ _DP, _IP, _I64P, _U64P = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.
 _SIGS = {
     "mci_create": [_I, _I64, C.POINTER(_P)], "mci_destroy": [_P], "mci_set_seed": [_P, _U64],
     "mci_set_walker_seeds": [_P, _U64P, _I64], "mci_set_rng": [_P, _I], "mci_set_precision": [_P, _I],
-    "mci_set_coupling": [_P, _I], "mci_set_target_acceptance_rate": [_P, _D],
+    "mci_set_coupling": [_P, _I], "mci_set_kernel_path": [_P, _I], "mci_set_target_acceptance_rate": [_P, _D],
     "mci_set_nfind_mrt2_iterations": [_P, _I], "mci_set_ndecorrelation_steps": [_P, _I64],
     "mci_get_settings": [_P, _DP, _IP, _I64P],
     "mci_set_x": [_P, _DP, _I64], "mci_get_x": [_P, _DP, _I64], "mci_center_x": [_P], "mci_move_x": [_P],
@@ -108,6 +108,7 @@ class MCI:
     def setRandomMode(self, mode): self._c("mci_set_rng", mode)
     def setPrecision(self, prec): self._c("mci_set_precision", prec)
     def setWalkerCoupling(self, mode): self._c("mci_set_coupling", mode)
+    def setKernelPath(self, path): self._c("mci_set_kernel_path", path)
     def setTargetAcceptanceRate(self, r): self._c("mci_set_target_acceptance_rate", r)
     def setNfindMRT2Iterations(self, n): self._c("mci_set_nfind_mrt2_iterations", n)
     def setNdecorrelationSteps(self, n): self._c("mci_set_ndecorrelation_steps", n)

@@ -435,6 +435,7 @@ int main()
                 while (std::getline(std::cin, l2) && l2 != "endbench") { setup.push_back(l2); }
                 double tot = 0., avgsum = 0.;
                 int64_t steps_total = 0;
+                std::vector<double> asum, e2sum;
                 for (int64_t w = 0; w < W; ++w) {
                     std::unique_ptr<mci::MCI> m;
                     int nd0 = 0;
@@ -458,9 +459,13 @@ int main()
                     tot += std::chrono::duration<double>(t1 - t0).count();
                     steps_total += nmc;
                     avgsum += a.empty() ? 0. : a[0];
+                    asum.resize(a.size(), 0.);
+                    e2sum.resize(e.size(), 0.);
+                    for (size_t i = 0; i < a.size(); ++i) { asum[i] += a[i]; e2sum[i] += e[i]*e[i]; }
                 }
                 std::cout << "{\"cmd\":\"bench\",\"walkers\":" << W << ",\"seconds\":" << tot << ",\"walker_steps\":" << steps_total
-                          << ",\"avg0_mean\":" << avgsum/W << "}" << std::endl;
+                          << ",\"avg0_mean\":" << avgsum/W << ",\"avg_sum\":" << hxv(asum.data(), static_cast<int>(asum.size()))
+                          << ",\"err2_sum\":" << hxv(e2sum.data(), static_cast<int>(e2sum.size())) << "}" << std::endl;
             }
             else { std::cout << "{\"cmd\":\"error\",\"what\":\"unknown command " << cmd << "\"}" << std::endl; }
         }

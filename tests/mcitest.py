@@ -28,6 +28,7 @@ OBS_X2, OBS_X2_SUM, OBS_X, OBS_X_SUM, OBS_CONST, OBS_EXP_X2, OBS_INDICATOR = 1, 
 RNG_MT, RNG_PHILOX = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 COUPLING_INDEPENDENT, COUPLING_SHARED = 0, 1
+KERNELS_AUTO, KERNELS_GENERIC = 0, 1
 
 _P = C.c_void_p
 _I = C.c_int
@@ -86,6 +87,7 @@ ORACLE_ONLY = {
     "get_accu_data": [_I, _I64, _DP, _I64, _I64P],
 }
 MCI_ONLY = {
+    "set_kernel_path": [_I],
     "get_timings": [_DP, _I],
     "get_observable_data": [_I, _DP, _I64, _I64P],
 }
@@ -180,6 +182,7 @@ class Chain:
     def setRng(self, k): self.call("set_rng", k)
     def setPrecision(self, p): self.call("set_precision", p)
     def setCoupling(self, m): self.call("set_coupling", m)
+    def setKernelPath(self, p): self.call("set_kernel_path", p)
     def setTargetAcceptanceRate(self, r): self.call("set_target_acceptance_rate", r)
     def setNfindMRT2Iterations(self, n): self.call("set_nfind_mrt2_iterations", n)
     def setNdecorrelationSteps(self, n): self.call("set_ndecorrelation_steps", n)

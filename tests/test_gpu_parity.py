@@ -11,8 +11,9 @@ import numpy as np
 import pytest
 
 import golden_io as G
-from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_MJ, OBS_EXP_X2, OBS_X, OBS_X2, OBS_X2_SUM,
-                     PDF_GAUSS_DIAG, RNG_MT, RNG_PHILOX, Chain, chain_setup, estimate)
+from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_MJ, ESTIM_NOOP, KERNELS_AUTO, KERNELS_GENERIC,
+                     OBS_EXP_X2, OBS_X, OBS_X2, OBS_X2_SUM, PDF_GAUSS_DIAG, PREC_FP32, RNG_MT, RNG_PHILOX, Chain,
+                     chain_setup, estimate)
 
 pytestmark = pytest.mark.gpu
 
@@ -119,6 +120,8 @@ def test_oracle_parity_many_walkers(rng, coupling):
         c = chain_setup(prefix, cfg2(blocksize=20), nwalkers=W)
         c.setRng(rng)
         c.setCoupling(coupling)
+        if prefix == "mci":
+            c.setKernelPath(KERNELS_GENERIC)
         a, e = c.integrate(nmc)
         res[prefix] = (a, e, *c.walkerResults(), c.getWalkerSteps(), c.getX(), c.walkerAcceptance())
     o, g = res["orc"], res["mci"]
@@ -142,10 +145,40 @@ def test_auto_decorrelation_and_calibration_parity():
             assert np.array_equal(res["orc"][k], res["mci"][k]), (coupling, k)
 
 
+RTOL_BLOCKS = 1e-12  # dwell-time block sums vs step-by-step sums (same FP64 values, other rounding)
+
+
+@pytest.mark.parametrize("blocksize,coupling", [(20, COUPLING_INDEPENDENT), (20, COUPLING_SHARED), (0, COUPLING_INDEPENDENT)])
+def test_specialised_kernel_matches_generic_and_oracle(blocksize, coupling):
+    """The LDS/dwell-time kernel makes the same decisions as the generic kernel and the oracle
+    (positions, step sizes and acceptance counters bit-identical); block means and estimates
+    agree to 1e-12 relative."""
+    W, nmc = 256, 64 * 20
+    cfg = cfg2(blocksize=blocksize)
+    if blocksize == 0:
+        cfg["obs"] = [(OBS_X2, [], 0, 1, 0, ESTIM_NOOP), (OBS_X2_SUM, [], 0, 1, 0, ESTIM_NOOP), (OBS_EXP_X2, [16.0], 0, 1, 0, ESTIM_NOOP)]
+    out = {}
+    for name in ("fast", "generic", "orc"):
+        c = chain_setup("orc" if name == "orc" else "mci", cfg, nwalkers=W)
+        c.setRng(RNG_PHILOX)
+        c.setCoupling(coupling)
+        if name == "generic":
+            c.setKernelPath(KERNELS_GENERIC)
+        a, e = c.integrate(nmc)
+        out[name] = (a, e, *c.walkerResults(), c.getWalkerSteps(), c.getX(), *c.walkerAcceptance())
+    f, g, o = out["fast"], out["generic"], out["orc"]
+    for k in (4, 5, 6, 7):
+        assert np.array_equal(f[k], g[k]) and np.array_equal(g[k], o[k]), k
+    for k in (0, 1, 2, 3):
+        assert np.array_equal(g[k], o[k]), k
+        np.testing.assert_allclose(f[k], g[k], rtol=RTOL_BLOCKS, atol=1e-300)
+
+
 def test_full_size_configuration2_walker_subset():
     """65,536 walkers x 10,240 steps (d=16): independent walkers 0..63 equal an oracle run of 64
-    walkers bit for bit (walker w's Philox stream depends only on (seed, w)); the combined
-    estimate agrees with sigma_i^2 and sum sigma_i^2 within 4 error bars."""
+    walkers (walker w's Philox stream depends only on (seed, w)): positions and counters bit for
+    bit, estimates to 1e-12; the combined estimate agrees with sigma_i^2 and sum sigma_i^2
+    within 4 error bars."""
     W, nmc = 65536, 10240
     g = chain_setup("mci", cfg2(), nwalkers=W)
     g.setRng(RNG_PHILOX)
@@ -155,7 +188,10 @@ def test_full_size_configuration2_walker_subset():
     o.setRng(RNG_PHILOX)
     o.integrate(nmc)
     oa, oe = o.walkerResults()
-    assert np.array_equal(wa[:64], oa) and np.array_equal(we[:64], oe)
+    assert np.array_equal(g.getX()[:64], o.getX())
+    assert np.array_equal(g.walkerAcceptance()[0][:64], o.walkerAcceptance()[0])
+    np.testing.assert_allclose(wa[:64], oa, rtol=RTOL_BLOCKS)
+    np.testing.assert_allclose(we[:64], oe, rtol=1e-10)
     sig, _ = sigma_a(16)
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
     assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
@@ -177,3 +213,34 @@ def test_full_size_shared_coupling_physics():
     sig, _ = sigma_a(16)
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
     assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
+
+
+def cfg3(prec_walkers=262144):
+    sig, a = sigma_a(64)
+    return dict(ndim=64, seed=4242, ndecorr=1000, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, a)],
+                obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_EXP_X2, [64.0], 160, 1, 1, ESTIM_MJ)])
+
+
+def test_fp32_precision_form_full_size():
+    """Configuration 3: d=64, 262,144 walkers x 10,240 steps, FP32 state/proposal/exp with Kahan
+    FP32 block sums, FP64 calibration, estimators and combine.  The FP32 chain and the FP64
+    chain driven by the same Philox words stay coupled; their estimates differ by far less than
+    the error bar, and both agree with the analytic values."""
+    W, nmc = 262144, 10240
+    sig, _ = sigma_a(64)
+    exact = np.concatenate([sig ** 2, [np.prod((1 + 2 * sig ** 2 / 64) ** -0.5)]])
+    res = {}
+    for prec in (0, 1):
+        c = chain_setup("mci", cfg3(), nwalkers=W)
+        c.setRng(RNG_PHILOX)
+        c.setCoupling(COUPLING_SHARED)
+        c.setPrecision(prec)
+        res[prec] = c.integrate(nmc)
+    a64, e64 = res[0]
+    a32, e32 = res[1]
+    for a, e in ((a64, e64), (a32, e32)):
+        assert np.all(np.abs(a - exact) < 4 * e), np.max(np.abs(a - exact) / e)
+    rel = np.abs(a32 - a64) / np.abs(a64)
+    print("FP32 vs FP64 (coupled chains): max relative difference", rel.max(), "median", np.median(rel),
+          "; max |diff|/err", np.max(np.abs(a32 - a64) / e64), flush=True)
+    assert np.all(np.abs(a32 - a64) < 0.25 * e64)
