@@ -169,8 +169,13 @@ def test_specialised_kernel_matches_generic_and_oracle(blocksize, coupling):
     f, g, o = out["fast"], out["generic"], out["orc"]
     for k in (4, 5, 6, 7):
         assert np.array_equal(f[k], g[k]) and np.array_equal(g[k], o[k]), k
+    exact = exact_components(cfg)
+    for k in (0, 1):
+        check_components(g[k], o[k], exact)
+    for k in (2, 3):
+        for wk in range(W):
+            check_components(g[k][wk], o[k][wk], exact)
     for k in (0, 1, 2, 3):
-        assert np.array_equal(g[k], o[k]), k
         np.testing.assert_allclose(f[k], g[k], rtol=RTOL_BLOCKS, atol=1e-300)
 
 
