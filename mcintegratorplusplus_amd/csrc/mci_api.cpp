@@ -155,6 +155,8 @@ struct mci_ctx
 
     DBuf<double> x, xn, pold, pnew, steps, lbd, ubd, obsval, dimsizes, wavg, werr, chunks_a, chunks_e, res, rates, fact;
     DBuf<double> oa, oe;  // decorrelation: previous estimates (independent walkers)
+    DBuf<CalibCtrl> ctrl;
+    bool walker_results_host = false; // h_wavg/h_werr hold the device's per-walker results
     DBuf<uint64_t> mt, ctr, seeds;
     DBuf<int32_t> mtp;
     DBuf<int64_t> acc, rej;
@@ -339,36 +341,39 @@ bool fast_chain(const mci_ctx *c)
            !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
 }
 
-// MCI::findMRT2Step for walkers that share the rank-averaged rate (MCIntegrator.cpp:87-168, USE_MPI)
+// MCI::findMRT2Step for walkers that share the rank-averaged rate (MCIntegrator.cpp:87-168, USE_MPI).
+// The loop's bookkeeping runs on the device (k_calib_control); the host enqueues iterations in
+// batches and reads the loop flag once per batch; iterations after the exit are no-ops.
 mci_status find_shared(mci_ctx *c)
 {
+    const int N = c->nfind;
+    if (N == 0) { return MCI_OK; }
     Chain ch = make_chain(c, nullptr);
     const int64_t min_stat = (int64_t)std::fmax(100., std::sqrt(40000. * c->ndim) / (double)c->W);
     const int64_t nch = (c->W + 255) / 256;
     HIPCHK(c->rates.ensure(c->W));
     HIPCHK(c->chunks_a.ensure(nch));
-    HIPCHK(c->fact.ensure(2));
-    int cons = 0, counter = 0;
-    const int N = c->nfind;
+    HIPCHK(c->ctrl.ensure(1));
+    CalibCtrl init;
+    std::memset(&init, 0, sizeof init);
+    HIPCHK(hipMemcpyAsync(c->ctrl.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
     const bool fast = fast_chain(c);
-    while ((N < 0 && cons < 5) || counter < N) {
-        // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
-        if (fast) { TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream)); }
-        else { HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream)); }
-        HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
-        HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
-        HIPCHK(launch_mean_of_chunks(c->chunks_a.p, c->W, c->fact.p + 1, c->stream));
-        double rate = 0.;
-        HIPCHK(hipMemcpyAsync(&rate, c->fact.p + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const int maxit = std::abs(N);
+    const int batch = 4;
+    for (int it = 0; it < maxit;) {
+        for (int b = 0; b < batch && it < maxit; ++b, ++it) {
+            // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
+            if (fast) { TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, &c->ctrl.p->done)); }
+            else { HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream, c->ctrl.p)); }
+            HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
+            HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
+            HIPCHK(launch_calib_control(c->chunks_a.p, c->W, c->target, N, c->ctrl.p, c->stream));
+            HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream));
+        }
+        int done = 0;
+        HIPCHK(hipMemcpyAsync(&done, &c->ctrl.p->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (std::fabs(rate - c->target) < 0.05) { ++cons; }
-        else { cons = 0; }
-        const double f = std::fmin(2., std::fmax(0.5, rate / c->target));
-        HIPCHK(hipMemcpyAsync(c->fact.p, &f, sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(launch_scale_steps(ch, c->fact.p, c->dimsizes.p, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        ++counter;
-        if (N < 0 && counter >= std::abs(N)) { break; }
+        if (done) { break; }
     }
     return MCI_OK;
 }
@@ -1050,8 +1055,7 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
         j.err = c->werr.p;
         HIPCHK(launch_estimate(j, c->stream));
     }
-    c->h_wavg.assign(ndw, 0.);
-    c->h_werr.assign(ndw, 0.);
+    c->walker_results_host = false;
     if (nd > 0) {
         if (W > 1) {
             const int64_t nch = (W + 255) / 256;
@@ -1064,15 +1068,13 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
             HIPCHK(hipMemcpyAsync(average, c->res.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipMemcpyAsync(error, c->res.p + nd, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         }
-        HIPCHK(hipEventRecord(c->ev[4], c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_wavg.data(), c->wavg.p, (size_t)nd * W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_werr.data(), c->werr.p, (size_t)nd * W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        else {
+            HIPCHK(hipMemcpyAsync(average, c->wavg.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(error, c->werr.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        }
     }
-    else { HIPCHK(hipEventRecord(c->ev[4], c->stream)); }
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (W == 1) {
-        for (int i = 0; i < nd; ++i) { average[i] = c->h_wavg[i]; error[i] = c->h_werr[i]; }
-    }
     c->nobsdim_last = nd;
     c->timings[0] = elapsed(c, 0, 1);
     c->timings[1] = elapsed(c, 1, 2);
@@ -1087,6 +1089,15 @@ mci_status mci_get_walker_results(mci_ctx *c, double *a, double *e, int64_t n)
     CHECKCTX(c);
     const int nd = c->nobsdim_last;
     if (!a || !e || n < c->W * nd) { return fail(MCI_E_INVALID_ARGUMENT, "output too small"); }
+    if (!c->walker_results_host && nd > 0) {
+        TRY(set_current(c));
+        c->h_wavg.assign((size_t)nd * c->W, 0.);
+        c->h_werr.assign((size_t)nd * c->W, 0.);
+        HIPCHK(hipMemcpyAsync(c->h_wavg.data(), c->wavg.p, (size_t)nd * c->W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_werr.data(), c->werr.p, (size_t)nd * c->W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->walker_results_host = true;
+    }
     for (int64_t w = 0; w < c->W; ++w) {
         for (int i = 0; i < nd; ++i) {
             a[w * nd + i] = c->h_wavg[(size_t)i * c->W + w];

@@ -51,6 +51,7 @@ struct FastArgs
     double x2sum_div;      // X2_SUM divisor
     double exp_div;        // EXP_X2 divisor
     int nvecs;
+    const int *skip;       // nonzero *skip: return at once (finished calibration loop)
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
+    if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
     double *xs = lds;                 // [d][64]
     double *Ss = lds + d * WAVE;      // [d][64] block sums of x_i^2 (X2)
@@ -140,16 +142,19 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         Sr = (double)B * r;
         if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
     }
+    // software pipeline: the words of step t+1 are computed while step t's LDS reads are in flight
+    uint32_t w0, w1, w2, w3;
+    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
     for (int64_t s = 0; s < A.nsteps; ++s) {
-        uint32_t w0, w1, w2, w3;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-        const double rr = (double)(2 * (int64_t)w1 + 1 - 4294967296LL) * 0x1p-32;
-        const double u = (double)((((uint64_t)w2 << 32) | w3) >> 11) * 0x1p-53;
-        ++t;
         const double xo = xs[v * WAVE + lane];
-        const double xn = xo + step * rr;
         const double av = as[v];
+        // (2 w1 + 1 - 2^32) 2^-32 and the top 53 bits of (w2:w3) 2^-53, both exact
+        const double rr = ((double)(int32_t)(w1 - 0x80000000u) + 0.5) * 0x1p-31;
+        const double u = (double)w2 * 0x1p-32 + (double)(w3 >> 11) * 0x1p-53;
+        ++t;
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+        const double xn = xo + step * rr;
         const double pn = (xn * xn) * av, po = (xo * xo) * av;
         const double ex = pn - po;
         if (accept64(u, -ex)) {
@@ -207,6 +212,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
+    if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
     float *xs = lds32;                  // [d][64]
     float *Ss = lds32 + d * WAVE;       // [d][64] Kahan sums of x_i^2
@@ -242,18 +248,19 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
         Sr = (float)B * r;
         if (hexp) { e = __expf(-r * inv_ediv); Se = (float)B * e; }
     }
+    uint32_t w0, w1, w2, w3;
+    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
     for (int64_t s = 0; s < A.nsteps; ++s) {
-        uint32_t w0, w1, w2, w3;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-        (void)w3;
         const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
+        const float xo = xs[v * WAVE + lane];
+        const float asv = as[v];
         const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
         const float u = (float)(w2 >> 8) * 0x1p-24f;
         ++t;
-        const float xo = xs[v * WAVE + lane];
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const float xn = fmaf(step, rr, xo);
         const float dx2 = (xn - xo) * (xn + xo);
-        const float ex = as[v] * dx2;
+        const float ex = asv * dx2;
         if (u <= __expf(-ex)) {
             ++nacc;
             xs[v * WAVE + lane] = xn;
@@ -344,10 +351,11 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
     return lds_bytes(prec, ndim, nobs > 0) <= lds_limit();
 }
 
-int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s)
+int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s, const int *skip)
 {
     (void)fs;
     FastArgs A;
+    A.skip = skip;
     A.W = c.W;
     A.ndim = c.ndim;
     A.nsteps = nsteps;

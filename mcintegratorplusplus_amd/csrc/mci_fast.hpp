@@ -19,6 +19,7 @@ bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int 
 // the observables of the main sampling fit the specialised kernel (and its LDS)
 bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec);
 // nsteps Metropolis steps; obs == nullptr: no accumulation (decorrelation)
-int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s);
+int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
+                const int *skip = nullptr);
 void fast_release(FastState &fs);
 } // namespace mcid

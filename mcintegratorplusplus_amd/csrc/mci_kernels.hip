@@ -28,10 +28,11 @@ __global__ void k_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int
     mtp[w] = 0;
 }
 
-__global__ void k_sample(Chain c, int64_t nsteps, int accumulate_obs, const uint8_t *active)
+__global__ void k_sample(Chain c, int64_t nsteps, int accumulate_obs, const uint8_t *active, const CalibCtrl *skip)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= c.W) { return; }
+    if (skip && skip->done) { return; }
     if (active && !active[w]) { return; }
     Lane L;
     load_lane(c, L, w);
@@ -85,11 +86,56 @@ __global__ void k_calibrate_indep(Chain c, int nfind, double target, int64_t min
     store_lane(c, L);
 }
 
-__global__ void k_scale_steps(Chain c, const double *fact, const double *dimSizes)
+__global__ void k_scale_steps(Chain c, const CalibCtrl *ctrl, const double *dimSizes)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= c.W) { return; }
-    scale_and_clamp(c, w, *fact, dimSizes);
+    if (w >= c.W || !ctrl->apply) { return; }
+    scale_and_clamp(c, w, ctrl->fact, dimSizes);
+}
+
+// ordered sum of the chunk sums (the last stage of the deterministic walker sum): the chunk
+// sums are staged through LDS by one wave, then added in order by lane 0
+__device__ inline double ordered_chunk_sum(const double *chunks, int64_t nch, double *sh)
+{
+    double s = 0.0;
+    bool first = true;
+    for (int64_t base = 0; base < nch; base += 1024) {
+        const int64_t m = (nch - base < 1024) ? nch - base : 1024;
+        for (int64_t i = threadIdx.x; i < m; i += blockDim.x) { sh[i] = chunks[base + i]; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int64_t i = 0; i < m; ++i) {
+                if (first) { s = sh[i]; first = false; }
+                else { s = s + sh[i]; }
+            }
+        }
+        __syncthreads();
+    }
+    return s;
+}
+
+// one findMRT2Step iteration's bookkeeping (MCIntegrator.cpp:124-167) on the device for walkers
+// sharing the rank-averaged rate: rate = (sum of walker rates)/W, consecutive-hit counter, the
+// factor clamp(rate/target, 0.5, 2) and the loop condition
+__global__ void k_calib_control(const double *chunks, int64_t nch, int64_t W, double target, int nfind, CalibCtrl *ctrl)
+{
+    __shared__ double sh[1024];
+    if (ctrl->done) {
+        if (threadIdx.x == 0) { ctrl->apply = 0; }
+        return;
+    }
+    const double sum = ordered_chunk_sum(chunks, nch, sh);
+    if (threadIdx.x != 0) { return; }
+    const double rate = sum / (double)W;
+    ctrl->rate = rate;
+    if (fabs(rate - target) < 0.05) { ++ctrl->cons; }
+    else { ctrl->cons = 0; }
+    ctrl->fact = fmin(2., fmax(0.5, rate / target));
+    ctrl->apply = 1;
+    ++ctrl->counter;
+    const bool stop_cap = (nfind < 0 && ctrl->counter >= -nfind);
+    const bool again = (nfind < 0 && ctrl->cons < 5) || ctrl->counter < nfind;
+    ctrl->done = (stop_cap || !again) ? 1 : 0;
 }
 
 __global__ void k_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W)
@@ -236,8 +282,17 @@ __device__ inline void mjblocker(int64_t n, const Col &x, double *y, int64_t yst
     err = sqrt(var[k] / ldexp(1.0, npow - k));
 }
 
+// MJBlocker's working copy: this thread's LDS column when the series is short (block means),
+// else its column of the scratch copy in HBM
+__device__ inline void mj_any(const EstimJob &j, const Col &x, int comp, int64_t w, double *sh, double &a, double &e)
+{
+    if (j.n <= MJ_LDS_MAX) { mjblocker(j.n, x, sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
+    else { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
+}
+
 __global__ void k_estimate(EstimJob j)
 {
+    extern __shared__ double est_sh[];
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= j.W * j.nobs) { return; }
     const int64_t w = tid % j.W;
@@ -250,12 +305,10 @@ __global__ void k_estimate(EstimJob j)
     case MCI_ESTIM_NOOP: a = x[0]; e = 0.; break;
     case MCI_ESTIM_UNCORRELATED: uncorr(j.n, x, onedim, a, e); break;
     case MCI_ESTIM_FCBLOCKER: fcblocker(j.n, x, onedim, a, e); break;
-    case MCI_ESTIM_MJBLOCKER:
-        mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e);
-        break;
+    case MCI_ESTIM_MJBLOCKER: mj_any(j, x, comp, w, est_sh, a, e); break;
     case MCI_ESTIM_CORRELATED:
         if ((j.n & (j.n - 1)) != 0) { fcblocker(j.n, x, onedim, a, e); }
-        else { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
+        else { mj_any(j, x, comp, w, est_sh, a, e); }
         break;
     }
     j.avg[(int64_t)(j.off + comp) * j.W + w] = a * j.scale;
@@ -298,10 +351,9 @@ __global__ void k_combine(const double *csum_avg, const double *csum_err2, int64
 // plain ordered sum of the chunk sums (rates), result / W
 __global__ void k_mean_of_chunks(const double *chunks, int64_t nchunks, int64_t W, double *out)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) { return; }
-    double s = chunks[0];
-    for (int64_t k = 1; k < nchunks; ++k) { s = s + chunks[k]; }
-    *out = s / (double)W;
+    __shared__ double sh[1024];
+    const double s = ordered_chunk_sum(chunks, nchunks, sh);
+    if (threadIdx.x == 0) { *out = s / (double)W; }
 }
 
 // convergence test of the automatic decorrelation per independent walker
@@ -382,9 +434,9 @@ hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int
     return hipGetLastError();
 }
 
-hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s)
+hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s, const CalibCtrl *skip)
 {
-    hipLaunchKernelGGL(k_sample, grid1(c.W, 64), dim3(64), 0, s, c, nsteps, acc, active);
+    hipLaunchKernelGGL(k_sample, grid1(c.W, 64), dim3(64), 0, s, c, nsteps, acc, active, skip);
     return hipGetLastError();
 }
 
@@ -394,9 +446,16 @@ hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int6
     return hipGetLastError();
 }
 
-hipError_t launch_scale_steps(const Chain &c, const double *fact, const double *dimSizes, hipStream_t s)
+hipError_t launch_scale_steps(const Chain &c, const CalibCtrl *ctrl, const double *dimSizes, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scale_steps, grid1(c.W, 256), dim3(256), 0, s, c, fact, dimSizes);
+    hipLaunchKernelGGL(k_scale_steps, grid1(c.W, 256), dim3(256), 0, s, c, ctrl, dimSizes);
+    return hipGetLastError();
+}
+
+hipError_t launch_calib_control(const double *chunks, int64_t W, double target, int nfind, CalibCtrl *ctrl, hipStream_t s)
+{
+    const int64_t nch = (W + 255) / 256;
+    hipLaunchKernelGGL(k_calib_control, dim3(1), dim3(256), 0, s, chunks, nch, W, target, nfind, ctrl);
     return hipGetLastError();
 }
 
@@ -408,7 +467,9 @@ hipError_t launch_walker_rates(const int64_t *acc, const int64_t *rej, double *r
 
 hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_estimate, grid1(j.W * j.nobs, 64), dim3(64), 0, s, j);
+    const bool mj = j.estim == MCI_ESTIM_MJBLOCKER || (j.estim == MCI_ESTIM_CORRELATED && (j.n & (j.n - 1)) == 0);
+    const size_t sh = (mj && j.n <= MJ_LDS_MAX) ? sizeof(double) * 64 * (size_t)j.n : 0;
+    hipLaunchKernelGGL(k_estimate, grid1(j.W * j.nobs, 64), dim3(64), sh, s, j);
     return hipGetLastError();
 }
 
@@ -429,7 +490,7 @@ hipError_t launch_combine(const double *ca, const double *ce, int64_t W, int nco
 hipError_t launch_mean_of_chunks(const double *chunks, int64_t W, double *out, hipStream_t s)
 {
     const int64_t nch = (W + 255) / 256;
-    hipLaunchKernelGGL(k_mean_of_chunks, dim3(1), dim3(1), 0, s, chunks, nch, W, out);
+    hipLaunchKernelGGL(k_mean_of_chunks, dim3(1), dim3(256), 0, s, chunks, nch, W, out);
     return hipGetLastError();
 }
 

@@ -6,6 +6,16 @@
 
 namespace mcid
 {
+// series up to this length run MJBlocker in LDS (64 x MJ_LDS_MAX doubles per workgroup)
+constexpr int64_t MJ_LDS_MAX = 64;
+
+// device-side state of the shared (rank-averaged) step calibration loop
+struct CalibCtrl
+{
+    double rate, fact;
+    int cons, counter, done, apply;
+};
+
 // one estimator pass over one observable's accumulator data [n][nobs][W]
 struct EstimJob
 {
@@ -19,9 +29,11 @@ struct EstimJob
 };
 
 hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s);
-hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s);
+hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s,
+                         const CalibCtrl *skip = nullptr);
 hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int64_t min_stat, const double *dimSizes, hipStream_t s);
-hipError_t launch_scale_steps(const Chain &c, const double *fact, const double *dimSizes, hipStream_t s);
+hipError_t launch_scale_steps(const Chain &c, const CalibCtrl *ctrl, const double *dimSizes, hipStream_t s);
+hipError_t launch_calib_control(const double *chunks, int64_t W, double target, int nfind, CalibCtrl *ctrl, hipStream_t s);
 hipError_t launch_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W, hipStream_t s);
 hipError_t launch_estimate(const EstimJob &j, hipStream_t s);
 hipError_t launch_chunk_sum(const double *v, int64_t W, int ncomp, int square, double *chunks, hipStream_t s);
