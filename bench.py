@@ -120,7 +120,19 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 1 << 27  # 2^27 doubles = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the walkers of one GPU are the unit of "
                                    "work (multi-GPU sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
