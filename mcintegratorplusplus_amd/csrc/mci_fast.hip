@@ -54,14 +54,6 @@ struct FastArgs
     const int *skip;       // nonzero *skip: return at once (finished calibration loop)
 };
 
-// a ^ b ^ k in one VALU instruction (LLVM emits two v_xor_b32 when k is a scalar register)
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t k)
-{
-    uint32_t r;
-    asm("v_xor3_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(k));
-    return r;
-}
-
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                                        uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3)
 {
@@ -70,8 +62,8 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -98,17 +90,6 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, int n, uint32_t wid, uint6
         }
     }
     return (int)(prod >> 32);
-}
-
-// u <= exp(x) exactly as the FP64 test, screened with the FP32 exponential
-__device__ __forceinline__ bool accept64(double u, double x)
-{
-    if (x >= 0.0) { return true; }
-    if (x < -80.0) { return u <= exp(x); }
-    const double f = (double)__expf((float)x);
-    if (u < f * (1.0 - 4e-5)) { return true; }
-    if (u > f * (1.0 + 4e-5)) { return false; }
-    return u <= exp(x);
 }
 
 template <bool ACC>
@@ -150,13 +131,19 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         Sr = (double)B * r;
         if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
     }
-    // software pipeline: the words of step t+1 are computed while step t's LDS reads are in flight
+    // Software pipeline: the words of step t+1 are computed while step t's LDS reads are in
+    // flight.  The common path has no divergent branch: the decision is formed with selects
+    // and the LDS state is always written back (old value on rejection; adding +0.0 to a sum
+    // is exact, so every stored number equals the branched form).
     uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+    double mult = (double)B;  // B - j: steps of the block the new value counts for
     for (int64_t s = 0; s < A.nsteps; ++s) {
         const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-        const double xo = xs[v * WAVE + lane];
+        const int slot = v * WAVE + lane;
+        const double xo = xs[slot];
         const double av = as[v];
+        const double So = hx2 ? Ss[slot] : 0.0;
         // (2 w1 + 1 - 2^32) 2^-32 and the top 53 bits of (w2:w3) 2^-53, both exact
         const double rr = ((double)(int32_t)(w1 - 0x80000000u) + 0.5) * 0x1p-31;
         const double u = (double)w2 * 0x1p-32 + (double)(w3 >> 11) * 0x1p-53;
@@ -165,37 +152,43 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         const double xn = xo + step * rr;
         const double pn = (xn * xn) * av, po = (xo * xo) * av;
         const double ex = pn - po;
-        if (accept64(u, -ex)) {
-            ++nacc;
-            xs[v * WAVE + lane] = xn;
-            if (ACC) {
-                const double dx2 = xn * xn - xo * xo;
-                const double mult = (double)(B - j);
-                if (hx2) { Ss[v * WAVE + lane] = Ss[v * WAVE + lane] + dx2 * mult; }
-                r = r + dx2;
-                Sr = Sr + dx2 * mult;
-                if (hexp) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
-            }
+        // u <= exp(-ex): FP32 exp outside a 4e-5 band, FP64 exp inside it
+        const double f = (double)__expf((float)(-ex));
+        bool acc = (ex <= 0.0) || (u < f * (1.0 - 4e-5));
+        const bool band = !acc && !(u > f * (1.0 + 4e-5));
+        if (__builtin_expect(__any(band), 0)) {
+            if (band) { acc = u <= exp(-ex); }
         }
-        if (ACC && ++j == B) {
-            const double norm = 1. / B;
-            double rx = 0.0;
-            for (int i = 0; i < d; ++i) {
-                const double v2 = xs[i * WAVE + lane];
-                const double q = v2 * v2;
-                rx = rx + q;
-                if (hx2) {
-                    A.d_x2[(b * d + i) * A.W + w] = Ss[i * WAVE + lane] * norm;
-                    Ss[i * WAVE + lane] = (double)B * q;
+        nacc += acc ? 1 : 0;
+        xs[slot] = acc ? xn : xo;
+        if (ACC) {
+            const double dx2 = acc ? (xn * xn - xo * xo) : 0.0;
+            if (hx2) { Ss[slot] = So + dx2 * mult; }
+            r = r + dx2;
+            Sr = Sr + dx2 * mult;
+            if (hexp && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
+            mult = mult - 1.0;
+            if (++j == B) {
+                const double norm = 1. / B;
+                double rx = 0.0;
+                for (int i = 0; i < d; ++i) {
+                    const double v2 = xs[i * WAVE + lane];
+                    const double q = v2 * v2;
+                    rx = rx + q;
+                    if (hx2) {
+                        A.d_x2[(b * d + i) * A.W + w] = Ss[i * WAVE + lane] * norm;
+                        Ss[i * WAVE + lane] = (double)B * q;
+                    }
                 }
+                if (hsum) { A.d_x2sum[b * A.W + w] = (Sr * norm) / A.x2sum_div; }
+                if (hexp) { A.d_exp[b * A.W + w] = Se * norm; }
+                r = rx;  // refresh |x|^2 exactly at every block boundary
+                Sr = (double)B * r;
+                if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+                j = 0;
+                mult = (double)B;
+                ++b;
             }
-            if (hsum) { A.d_x2sum[b * A.W + w] = (Sr * norm) / A.x2sum_div; }
-            if (hexp) { A.d_exp[b * A.W + w] = Se * norm; }
-            r = rx;  // refresh |x|^2 exactly at every block boundary
-            Sr = (double)B * r;
-            if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-            j = 0;
-            ++b;
         }
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = xs[i * WAVE + lane]; }
@@ -258,10 +251,14 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     }
     uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+    float mult = (float)B;
     for (int64_t s = 0; s < A.nsteps; ++s) {
         const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-        const float xo = xs[v * WAVE + lane];
+        const int slot = v * WAVE + lane;
+        const float xo = xs[slot];
         const float asv = as[v];
+        float So = 0.f, Co = 0.f;
+        if (hx2) { So = Ss[slot]; Co = Cs[slot]; }
         const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
         const float u = (float)(w2 >> 8) * 0x1p-24f;
         ++t;
@@ -269,44 +266,46 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
         const float xn = fmaf(step, rr, xo);
         const float dx2 = (xn - xo) * (xn + xo);
         const float ex = asv * dx2;
-        if (u <= __expf(-ex)) {
-            ++nacc;
-            xs[v * WAVE + lane] = xn;
-            if (ACC) {
-                const float mult = (float)(B - j);
-                if (hx2) {
-                    float S = Ss[v * WAVE + lane], Cc = Cs[v * WAVE + lane];
-                    kahan(S, Cc, dx2 * mult);
-                    Ss[v * WAVE + lane] = S;
-                    Cs[v * WAVE + lane] = Cc;
-                }
-                kahan(r, rc, dx2);
-                kahan(Sr, Cr, dx2 * mult);
+        const bool acc = u <= __expf(-ex);
+        nacc += acc ? 1 : 0;
+        xs[slot] = acc ? xn : xo;
+        if (ACC) {
+            const float da = acc ? dx2 : 0.f;
+            if (hx2) {
+                if (acc) { kahan(So, Co, da * mult); }
+                Ss[slot] = So;
+                Cs[slot] = Co;
+            }
+            if (acc) {
+                kahan(r, rc, da);
+                kahan(Sr, Cr, da * mult);
                 if (hexp) { const float en = __expf(-r * inv_ediv); kahan(Se, Ce, (en - e) * mult); e = en; }
             }
-        }
-        if (ACC && ++j == B) {
-            const double norm = 1. / B;
-            float rx = 0.f, rxc = 0.f;
-            for (int i = 0; i < d; ++i) {
-                const float v2 = xs[i * WAVE + lane];
-                const float q = v2 * v2;
-                kahan(rx, rxc, q);
-                if (hx2) {
-                    A.d_x2[(b * d + i) * A.W + w] = ((double)Ss[i * WAVE + lane] - (double)Cs[i * WAVE + lane]) * norm;
-                    Ss[i * WAVE + lane] = (float)B * q;
-                    Cs[i * WAVE + lane] = 0.f;
+            mult = mult - 1.f;
+            if (++j == B) {
+                const double norm = 1. / B;
+                float rx = 0.f, rxc = 0.f;
+                for (int i = 0; i < d; ++i) {
+                    const float v2 = xs[i * WAVE + lane];
+                    const float q = v2 * v2;
+                    kahan(rx, rxc, q);
+                    if (hx2) {
+                        A.d_x2[(b * d + i) * A.W + w] = ((double)Ss[i * WAVE + lane] - (double)Cs[i * WAVE + lane]) * norm;
+                        Ss[i * WAVE + lane] = (float)B * q;
+                        Cs[i * WAVE + lane] = 0.f;
+                    }
                 }
+                if (hsum) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
+                if (hexp) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
+                r = rx;
+                rc = rxc;
+                Sr = (float)B * r;
+                Cr = 0.f;
+                if (hexp) { e = __expf(-r * inv_ediv); Se = (float)B * e; Ce = 0.f; }
+                j = 0;
+                mult = (float)B;
+                ++b;
             }
-            if (hsum) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
-            if (hexp) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
-            r = rx;
-            rc = rxc;
-            Sr = (float)B * r;
-            Cr = 0.f;
-            if (hexp) { e = __expf(-r * inv_ediv); Se = (float)B * e; Ce = 0.f; }
-            j = 0;
-            ++b;
         }
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)xs[i * WAVE + lane]; }
