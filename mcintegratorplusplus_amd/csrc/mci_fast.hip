@@ -131,27 +131,24 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         Sr = (double)B * r;
         if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
     }
-    // Two-stage software pipeline.  Iteration s finishes step s (decision, write-back, block
-    // flush), then computes step s+1's coordinate index and issues its LDS reads, and then
-    // generates the Philox words of step s+2 while those reads are in flight; step s+1's
-    // arithmetic starts on data that has arrived.  LDS accesses of one wave complete in order,
-    // so a read issued after a write to the same coordinate returns the written value.
-    // The common path has no divergent branch: the decision is formed with selects and the
-    // LDS state is always written back (old value on rejection; adding +0.0 to a sum is
-    // exact, so every stored number equals the branched form).
-    uint32_t w0, w1, w2, w3, n0, n1, n2, n3;
+    // Software pipeline: the words of step t+1 are computed while step t's LDS reads are in
+    // flight.  The common path has no divergent branch: the decision is formed with selects
+    // and the LDS state is always written back (old value on rejection; adding +0.0 to a sum
+    // is exact, so every stored number equals the branched form).
+    uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-    int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-    int slot = v * WAVE + lane;
-    double xo = xs[slot];
-    double av = as[v];
-    double So = hx2 ? Ss[slot] : 0.0;
-    philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, n0, n1, n2, n3);
     double mult = (double)B;  // B - j: steps of the block the new value counts for
     for (int64_t s = 0; s < A.nsteps; ++s) {
+        const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
+        const int slot = v * WAVE + lane;
+        const double xo = xs[slot];
+        const double av = as[v];
+        const double So = hx2 ? Ss[slot] : 0.0;
         // (2 w1 + 1 - 2^32) 2^-32 and the top 53 bits of (w2:w3) 2^-53, both exact
         const double rr = ((double)(int32_t)(w1 - 0x80000000u) + 0.5) * 0x1p-31;
         const double u = (double)w2 * 0x1p-32 + (double)(w3 >> 11) * 0x1p-53;
+        ++t;
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const double xn = xo + step * rr;
         const double pn = (xn * xn) * av, po = (xo * xo) * av;
         const double ex = pn - po;
@@ -193,15 +190,6 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 ++b;
             }
         }
-        // stage 2: step s+1's index and reads, then step s+2's words
-        ++t;
-        w0 = n0; w1 = n1; w2 = n2; w3 = n3;
-        v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-        slot = v * WAVE + lane;
-        xo = xs[slot];
-        av = as[v];
-        So = hx2 ? Ss[slot] : 0.0;
-        philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, n0, n1, n2, n3);
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = xs[i * WAVE + lane]; }
     A.ctr[w] = t;
@@ -261,20 +249,20 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
         Sr = (float)B * r;
         if (hexp) { e = __expf(-r * inv_ediv); Se = (float)B * e; }
     }
-    // the two-stage pipeline of k_fast64
-    uint32_t w0, w1, w2, w3, n0, n1, n2, n3;
+    uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-    int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-    int slot = v * WAVE + lane;
-    float xo = xs[slot];
-    float asv = as[v];
-    float So = 0.f, Co = 0.f;
-    if (hx2) { So = Ss[slot]; Co = Cs[slot]; }
-    philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, n0, n1, n2, n3);
     float mult = (float)B;
     for (int64_t s = 0; s < A.nsteps; ++s) {
+        const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
+        const int slot = v * WAVE + lane;
+        const float xo = xs[slot];
+        const float asv = as[v];
+        float So = 0.f, Co = 0.f;
+        if (hx2) { So = Ss[slot]; Co = Cs[slot]; }
         const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
         const float u = (float)(w2 >> 8) * 0x1p-24f;
+        ++t;
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const float xn = fmaf(step, rr, xo);
         const float dx2 = (xn - xo) * (xn + xo);
         const float ex = asv * dx2;
@@ -319,14 +307,6 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
                 ++b;
             }
         }
-        ++t;
-        w0 = n0; w1 = n1; w2 = n2; w3 = n3;
-        v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-        slot = v * WAVE + lane;
-        xo = xs[slot];
-        asv = as[v];
-        if (hx2) { So = Ss[slot]; Co = Cs[slot]; }
-        philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, n0, n1, n2, n3);
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)xs[i * WAVE + lane]; }
     A.ctr[w] = t;
