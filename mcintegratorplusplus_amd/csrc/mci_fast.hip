@@ -131,24 +131,22 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         Sr = (double)B * r;
         if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
     }
-    // Software pipeline: the words of step t+1 are computed while step t's LDS reads are in
-    // flight.  The common path has no divergent branch: the decision is formed with selects
-    // and the LDS state is always written back (old value on rejection; adding +0.0 to a sum
-    // is exact, so every stored number equals the branched form).
-    uint32_t w0, w1, w2, w3;
-    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+    // Two steps per iteration.  Both steps' LDS operands are read before either write-back;
+    // when the second step moves the same coordinate as the first, the first step's new
+    // values are forwarded by select (what an LDS round trip would return).  The Philox words
+    // of the next two steps are generated meanwhile: two independent calls give the wave
+    // enough independent instructions to issue while the decision chains wait.
+    // The common path has no divergent branch: the decision is formed with selects and the
+    // LDS state is always written back (old value on rejection; adding +0.0 to a sum is
+    // exact, so every stored number equals the branched form).
     double mult = (double)B;  // B - j: steps of the block the new value counts for
-    for (int64_t s = 0; s < A.nsteps; ++s) {
-        const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
-        const int slot = v * WAVE + lane;
-        const double xo = xs[slot];
-        const double av = as[v];
-        const double So = hx2 ? Ss[slot] : 0.0;
+
+    // decision and write-back of one step; returns whether a block was flushed after it
+    auto step_core = [&](int slot, double xo, double av, double So, uint32_t w1, uint32_t w2, uint32_t w3,
+                         double &xout, double &Sout) -> bool {
         // (2 w1 + 1 - 2^32) 2^-32 and the top 53 bits of (w2:w3) 2^-53, both exact
         const double rr = ((double)(int32_t)(w1 - 0x80000000u) + 0.5) * 0x1p-31;
         const double u = (double)w2 * 0x1p-32 + (double)(w3 >> 11) * 0x1p-53;
-        ++t;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const double xn = xo + step * rr;
         const double pn = (xn * xn) * av, po = (xo * xo) * av;
         const double ex = pn - po;
@@ -160,10 +158,12 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             if (band) { acc = u <= exp(-ex); }
         }
         nacc += acc ? 1 : 0;
-        xs[slot] = acc ? xn : xo;
+        xout = acc ? xn : xo;
+        xs[slot] = xout;
+        Sout = So;
         if (ACC) {
             const double dx2 = acc ? (xn * xn - xo * xo) : 0.0;
-            if (hx2) { Ss[slot] = So + dx2 * mult; }
+            if (hx2) { Sout = So + dx2 * mult; Ss[slot] = Sout; }
             r = r + dx2;
             Sr = Sr + dx2 * mult;
             if (hexp && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
@@ -188,8 +188,43 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 j = 0;
                 mult = (double)B;
                 ++b;
+                return true;
             }
         }
+        return false;
+    };
+
+    uint32_t a0, a1, a2, a3, b0, b1, b2, b3;
+    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, a0, a1, a2, a3);
+    philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, b0, b1, b2, b3);
+    int64_t s = 0;
+    for (; s + 1 < A.nsteps; s += 2) {
+        const int va = draw_vidx(a0, A.nvecs, wid, t, A.key0, A.key1);
+        const int vb = draw_vidx(b0, A.nvecs, wid, t + 1, A.key0, A.key1);
+        const int sa = va * WAVE + lane, sb = vb * WAVE + lane;
+        const double xoa = xs[sa], ava = as[va];
+        const double xob_l = xs[sb], avb = as[vb];
+        const double Soa = hx2 ? Ss[sa] : 0.0;
+        const double Sob_l = hx2 ? Ss[sb] : 0.0;
+        const uint32_t pa1 = a1, pa2 = a2, pa3 = a3, pb1 = b1, pb2 = b2, pb3 = b3;
+        t += 2;
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, a0, a1, a2, a3);
+        philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, b0, b1, b2, b3);
+        double xa, Sa;
+        const bool flushed = step_core(sa, xoa, ava, Soa, pa1, pa2, pa3, xa, Sa);
+        const bool same = (vb == va);
+        const double xob = same ? xa : xob_l;
+        double Sob = same ? Sa : Sob_l;
+        if (hx2 && flushed) { Sob = Ss[sb]; }  // a block ended between the two steps
+        double xb, Sb;
+        step_core(sb, xob, avb, Sob, pb1, pb2, pb3, xb, Sb);
+    }
+    if (s < A.nsteps) {  // odd step count: the last step alone (its words are in a0..a3)
+        const int va = draw_vidx(a0, A.nvecs, wid, t, A.key0, A.key1);
+        const int sa = va * WAVE + lane;
+        double xa, Sa;
+        step_core(sa, xs[sa], as[va], hx2 ? Ss[sa] : 0.0, a1, a2, a3, xa, Sa);
+        ++t;
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = xs[i * WAVE + lane]; }
     A.ctr[w] = t;
