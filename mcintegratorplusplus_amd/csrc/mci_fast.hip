@@ -51,6 +51,7 @@ struct FastArgs
     double x2sum_div;      // X2_SUM divisor
     double exp_div;        // EXP_X2 divisor
     int nvecs;
+    uint32_t lemire_thr;   // 2^32 mod nvecs
     const int *skip;       // nonzero *skip: return at once (finished calibration loop)
 };
 
@@ -72,14 +73,15 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
-// Lemire index on [0, n) from word w0 with rejection words from the extra stream
-__device__ __forceinline__ int draw_vidx(uint32_t w0, int n, uint32_t wid, uint64_t t, uint32_t k0, uint32_t k1)
+// Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution):
+// low = (w0 * range) mod 2^32 < thr = 2^32 mod range rejects and draws from the extra-word stream.
+// thr is computed on the host (0 for powers of two); a wave vote skips the rare path.
+__device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t thr, uint32_t wid, uint64_t t, uint32_t k0,
+                                         uint32_t k1)
 {
-    const uint32_t range = (uint32_t)n;
     uint64_t prod = (uint64_t)w0 * range;
-    uint32_t low = (uint32_t)prod;
-    if (low < range) {
-        const uint32_t thr = (0u - range) % range;
+    if (__builtin_expect(__any((uint32_t)prod < thr), 0)) {
+        uint32_t low = (uint32_t)prod;
         uint32_t kx = 0;
         while (low < thr) {
             const uint32_t k = 0x80000000u + (kx++);
@@ -92,20 +94,29 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, int n, uint32_t wid, uint6
     return (int)(prod >> 32);
 }
 
-template <bool ACC>
+// observable set of one launch (template flags: no per-step branches on them)
+enum { F_X2 = 1, F_SUM = 2, F_EXP = 4 };
+
+// FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
+// 1,024 waves), so a step costs about 4 cycles per instruction of any kind: the step is
+// written to issue few instructions — no per-step branch on the observable set (template),
+// a select-formed decision screened in FP32 with the high 32 bits of u, and the exact FP64
+// test (53-bit u, FP64 exp) only for lanes inside the screening band.
+template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
+    constexpr bool ACC = FL != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
     extern __shared__ double lds[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
     if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
-    double *xs = lds;                 // [d][64]
-    double *Ss = lds + d * WAVE;      // [d][64] block sums of x_i^2 (X2)
-    double *as = lds + (ACC ? 2 : 1) * d * WAVE;  // [d]
+    double *xs = lds;                             // [d][64]
+    double *Ss = lds + d * WAVE;                  // [d][64] block sums of x_i^2 (X2)
+    double *as = lds + (HX2 ? 2 : 1) * d * WAVE;  // [d]
     for (int i = lane; i < d; i += WAVE) { as[i] = A.a[i]; }
-    double r = 0.0;                   // |x|^2
+    double r = 0.0;                               // |x|^2
     if (live) {
         for (int i = 0; i < d; ++i) {
             const double v = A.x64[(int64_t)i * A.W + w];
@@ -118,55 +129,57 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const double step = A.steps[w];
     uint64_t t = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
+    const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
     int64_t nacc = 0;
-    const bool hx2 = ACC && A.d_x2;
-    const bool hsum = ACC && A.d_x2sum;
-    const bool hexp = ACC && A.d_exp;
     const int B = A.blocksize;
     double Sr = 0.0, Se = 0.0, e = 0.0;
     int j = 0;
     int64_t b = 0;
-    if (ACC) {
-        if (hx2) { for (int i = 0; i < d; ++i) { const double v = xs[i * WAVE + lane]; Ss[i * WAVE + lane] = (double)B * (v * v); } }
-        Sr = (double)B * r;
-        if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-    }
-    // Two steps per iteration.  Both steps' LDS operands are read before either write-back;
-    // when the second step moves the same coordinate as the first, the first step's new
-    // values are forwarded by select (what an LDS round trip would return).  The Philox words
-    // of the next two steps are generated meanwhile: two independent calls give the wave
-    // enough independent instructions to issue while the decision chains wait.
-    // The common path has no divergent branch: the decision is formed with selects and the
-    // LDS state is always written back (old value on rejection; adding +0.0 to a sum is
-    // exact, so every stored number equals the branched form).
+    if (HX2) { for (int i = 0; i < d; ++i) { const double v = xs[i * WAVE + lane]; Ss[i * WAVE + lane] = (double)B * (v * v); } }
+    if (ACC) { Sr = (double)B * r; }
+    if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+    // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
+    // path has no divergent branch: the decision is a select and the LDS state is always
+    // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
+    // number equals the branched form).
+    uint32_t w0, w1, w2, w3;
+    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
     double mult = (double)B;  // B - j: steps of the block the new value counts for
-
-    // decision and write-back of one step; returns whether a block was flushed after it
-    auto step_core = [&](int slot, double xo, double av, double So, uint32_t w1, uint32_t w2, uint32_t w3,
-                         double &xout, double &Sout) -> bool {
-        // (2 w1 + 1 - 2^32) 2^-32 and the top 53 bits of (w2:w3) 2^-53, both exact
-        const double rr = ((double)(int32_t)(w1 - 0x80000000u) + 0.5) * 0x1p-31;
-        const double u = (double)w2 * 0x1p-32 + (double)(w3 >> 11) * 0x1p-53;
+    for (int64_t s = 0; s < A.nsteps; ++s) {
+        const int v = draw_vidx(w0, range, thr, wid, t, A.key0, A.key1);
+        const int slot = v * WAVE + lane;
+        const double xo = xs[slot];
+        const double av = as[v];
+        const double So = HX2 ? Ss[slot] : 0.0;
+        // proposal (2 w1 + 1 - 2^32) 2^-32, exact
+        const double rr = ((double)(int32_t)(w1 ^ 0x80000000u) + 0.5) * 0x1p-31;
+        const uint32_t u_hi = w2, u_lo = w3;
+        ++t;
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const double xn = xo + step * rr;
-        const double pn = (xn * xn) * av, po = (xo * xo) * av;
-        const double ex = pn - po;
-        // u <= exp(-ex): FP32 exp outside a 4e-5 band, FP64 exp inside it
-        const double f = (double)__expf((float)(-ex));
-        bool acc = (ex <= 0.0) || (u < f * (1.0 - 4e-5));
-        const bool band = !acc && !(u > f * (1.0 + 4e-5));
+        const double xn2 = xn * xn, xo2 = xo * xo;
+        const double ex = xn2 * av - xo2 * av;
+        // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  With the
+        // FP32 exp f (relative error far below 4e-5 for 0 < ex < 88), u <= exp(-ex) is certain when
+        // uf (1 + 4e-5) + 2^-32 < f (1 - 4e-5) and false when uf (1 - 4e-5) > f (1 + 4e-5).
+        const float f = __expf(-(float)ex);
+        const float uf = (float)u_hi * 0x1p-32f;
+        bool acc = (ex <= 0.0) || (fmaf(uf, 1.00004f, 0x1p-32f) < f * 0.99996f);
+        const bool band = !acc && !(uf * 0.99996f > f * 1.00004f);
         if (__builtin_expect(__any(band), 0)) {
-            if (band) { acc = u <= exp(-ex); }
+            if (band) {
+                const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
+                acc = u <= exp(-ex);
+            }
         }
         nacc += acc ? 1 : 0;
-        xout = acc ? xn : xo;
-        xs[slot] = xout;
-        Sout = So;
+        xs[slot] = acc ? xn : xo;
         if (ACC) {
-            const double dx2 = acc ? (xn * xn - xo * xo) : 0.0;
-            if (hx2) { Sout = So + dx2 * mult; Ss[slot] = Sout; }
+            const double dx2 = acc ? (xn2 - xo2) : 0.0;
+            if (HX2) { Ss[slot] = So + dx2 * mult; }
             r = r + dx2;
             Sr = Sr + dx2 * mult;
-            if (hexp && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
+            if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
             mult = mult - 1.0;
             if (++j == B) {
                 const double norm = 1. / B;
@@ -175,56 +188,21 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                     const double v2 = xs[i * WAVE + lane];
                     const double q = v2 * v2;
                     rx = rx + q;
-                    if (hx2) {
+                    if (HX2) {
                         A.d_x2[(b * d + i) * A.W + w] = Ss[i * WAVE + lane] * norm;
                         Ss[i * WAVE + lane] = (double)B * q;
                     }
                 }
-                if (hsum) { A.d_x2sum[b * A.W + w] = (Sr * norm) / A.x2sum_div; }
-                if (hexp) { A.d_exp[b * A.W + w] = Se * norm; }
+                if (HSUM) { A.d_x2sum[b * A.W + w] = (Sr * norm) / A.x2sum_div; }
+                if (HEXP) { A.d_exp[b * A.W + w] = Se * norm; }
                 r = rx;  // refresh |x|^2 exactly at every block boundary
                 Sr = (double)B * r;
-                if (hexp) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+                if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
                 j = 0;
                 mult = (double)B;
                 ++b;
-                return true;
             }
         }
-        return false;
-    };
-
-    uint32_t a0, a1, a2, a3, b0, b1, b2, b3;
-    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, a0, a1, a2, a3);
-    philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, b0, b1, b2, b3);
-    int64_t s = 0;
-    for (; s + 1 < A.nsteps; s += 2) {
-        const int va = draw_vidx(a0, A.nvecs, wid, t, A.key0, A.key1);
-        const int vb = draw_vidx(b0, A.nvecs, wid, t + 1, A.key0, A.key1);
-        const int sa = va * WAVE + lane, sb = vb * WAVE + lane;
-        const double xoa = xs[sa], ava = as[va];
-        const double xob_l = xs[sb], avb = as[vb];
-        const double Soa = hx2 ? Ss[sa] : 0.0;
-        const double Sob_l = hx2 ? Ss[sb] : 0.0;
-        const uint32_t pa1 = a1, pa2 = a2, pa3 = a3, pb1 = b1, pb2 = b2, pb3 = b3;
-        t += 2;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, a0, a1, a2, a3);
-        philox(0u, wid, (uint32_t)(t + 1), (uint32_t)((t + 1) >> 32), A.key0, A.key1, b0, b1, b2, b3);
-        double xa, Sa;
-        const bool flushed = step_core(sa, xoa, ava, Soa, pa1, pa2, pa3, xa, Sa);
-        const bool same = (vb == va);
-        const double xob = same ? xa : xob_l;
-        double Sob = same ? Sa : Sob_l;
-        if (hx2 && flushed) { Sob = Ss[sb]; }  // a block ended between the two steps
-        double xb, Sb;
-        step_core(sb, xob, avb, Sob, pb1, pb2, pb3, xb, Sb);
-    }
-    if (s < A.nsteps) {  // odd step count: the last step alone (its words are in a0..a3)
-        const int va = draw_vidx(a0, A.nvecs, wid, t, A.key0, A.key1);
-        const int sa = va * WAVE + lane;
-        double xa, Sa;
-        step_core(sa, xs[sa], as[va], hx2 ? Ss[sa] : 0.0, a1, a2, a3, xa, Sa);
-        ++t;
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = xs[i * WAVE + lane]; }
     A.ctr[w] = t;
@@ -241,21 +219,34 @@ __device__ __forceinline__ void kahan(float &s, float &c, float y)
     s = tt;
 }
 
-template <bool ACC>
+// Kahan step applied only where `on` holds (selects keep the common path branch-free; a Kahan
+// step with y = 0 is not an identity when the compensation is nonzero)
+__device__ __forceinline__ void kahan_if(float &s, float &c, float y, bool on)
+{
+    const float yy = y - c;
+    const float tt = s + yy;
+    const float cc = (tt - s) - yy;
+    s = on ? tt : s;
+    c = on ? cc : c;
+}
+
+// FP32 chain (same structure)
+template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 {
+    constexpr bool ACC = FL != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
     extern __shared__ float lds32[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
     if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
-    float *xs = lds32;                  // [d][64]
-    float *Ss = lds32 + d * WAVE;       // [d][64] Kahan sums of x_i^2
-    float *Cs = lds32 + 2 * d * WAVE;   // [d][64] their compensations
-    float *as = lds32 + (ACC ? 3 : 1) * d * WAVE;   // [d]
+    float *xs = lds32;                              // [d][64]
+    float *Ss = lds32 + d * WAVE;                   // [d][64] Kahan sums of x_i^2
+    float *Cs = lds32 + 2 * d * WAVE;               // [d][64] their compensations
+    float *as = lds32 + (HX2 ? 3 : 1) * d * WAVE;   // [d]
     for (int i = lane; i < d; i += WAVE) { as[i] = (float)A.a[i]; }
-    float r = 0.f, rc = 0.f;            // |x|^2, Kahan-compensated
+    float r = 0.f, rc = 0.f;                        // |x|^2, Kahan-compensated
     if (live) {
         for (int i = 0; i < d; ++i) {
             const float v = (float)A.x64[(int64_t)i * A.W + w];
@@ -268,32 +259,29 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const float step = (float)A.steps[w];
     uint64_t t = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
+    const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
     int64_t nacc = 0;
-    const bool hx2 = ACC && A.d_x2;
-    const bool hsum = ACC && A.d_x2sum;
-    const bool hexp = ACC && A.d_exp;
     const int B = A.blocksize;
     const float inv_ediv = (float)(1.0 / A.exp_div);
     float Sr = 0.f, Cr = 0.f, Se = 0.f, Ce = 0.f, e = 0.f;
     int j = 0;
     int64_t b = 0;
-    if (ACC) {
-        if (hx2) {
-            for (int i = 0; i < d; ++i) { const float v = xs[i * WAVE + lane]; Ss[i * WAVE + lane] = (float)B * (v * v); Cs[i * WAVE + lane] = 0.f; }
-        }
-        Sr = (float)B * r;
-        if (hexp) { e = __expf(-r * inv_ediv); Se = (float)B * e; }
+    if (HX2) {
+        for (int i = 0; i < d; ++i) { const float v = xs[i * WAVE + lane]; Ss[i * WAVE + lane] = (float)B * (v * v); Cs[i * WAVE + lane] = 0.f; }
     }
+    if (ACC) { Sr = (float)B * r; }
+    if (HEXP) { e = __expf(-r * inv_ediv); Se = (float)B * e; }
     uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
     float mult = (float)B;
     for (int64_t s = 0; s < A.nsteps; ++s) {
-        const int v = draw_vidx(w0, A.nvecs, wid, t, A.key0, A.key1);
+        const int v = draw_vidx(w0, range, thr, wid, t, A.key0, A.key1);
         const int slot = v * WAVE + lane;
         const float xo = xs[slot];
         const float asv = as[v];
         float So = 0.f, Co = 0.f;
-        if (hx2) { So = Ss[slot]; Co = Cs[slot]; }
+        if (HX2) { So = Ss[slot]; Co = Cs[slot]; }
+        // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24
         const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
         const float u = (float)(w2 >> 8) * 0x1p-24f;
         ++t;
@@ -305,17 +293,14 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
         nacc += acc ? 1 : 0;
         xs[slot] = acc ? xn : xo;
         if (ACC) {
-            const float da = acc ? dx2 : 0.f;
-            if (hx2) {
-                if (acc) { kahan(So, Co, da * mult); }
+            if (HX2) {
+                kahan_if(So, Co, dx2 * mult, acc);
                 Ss[slot] = So;
                 Cs[slot] = Co;
             }
-            if (acc) {
-                kahan(r, rc, da);
-                kahan(Sr, Cr, da * mult);
-                if (hexp) { const float en = __expf(-r * inv_ediv); kahan(Se, Ce, (en - e) * mult); e = en; }
-            }
+            kahan_if(r, rc, dx2, acc);
+            kahan_if(Sr, Cr, dx2 * mult, acc);
+            if (HEXP && acc) { const float en = __expf(-r * inv_ediv); kahan(Se, Ce, (en - e) * mult); e = en; }
             mult = mult - 1.f;
             if (++j == B) {
                 const double norm = 1. / B;
@@ -324,19 +309,19 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
                     const float v2 = xs[i * WAVE + lane];
                     const float q = v2 * v2;
                     kahan(rx, rxc, q);
-                    if (hx2) {
+                    if (HX2) {
                         A.d_x2[(b * d + i) * A.W + w] = ((double)Ss[i * WAVE + lane] - (double)Cs[i * WAVE + lane]) * norm;
                         Ss[i * WAVE + lane] = (float)B * q;
                         Cs[i * WAVE + lane] = 0.f;
                     }
                 }
-                if (hsum) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
-                if (hexp) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
+                if (HSUM) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
+                if (HEXP) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
                 r = rx;
                 rc = rxc;
                 Sr = (float)B * r;
                 Cr = 0.f;
-                if (hexp) { e = __expf(-r * inv_ediv); Se = (float)B * e; Ce = 0.f; }
+                if (HEXP) { e = __expf(-r * inv_ediv); Se = (float)B * e; Ce = 0.f; }
                 j = 0;
                 mult = (float)B;
                 ++b;
@@ -349,10 +334,10 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     A.rej[w] = A.nsteps - nacc;
 }
 
-size_t lds_bytes(int prec, int ndim, bool acc)
+size_t lds_bytes(int prec, int ndim, bool x2)
 {
-    if (prec == MCI_PREC_FP64) { return sizeof(double) * ((size_t)(acc ? 2 : 1) * ndim * WAVE + ndim); }
-    return sizeof(float) * ((size_t)(acc ? 3 : 1) * ndim * WAVE + ndim);
+    if (prec == MCI_PREC_FP64) { return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + ndim); }
+    return sizeof(float) * ((size_t)(x2 ? 3 : 1) * ndim * WAVE + ndim);
 }
 
 size_t lds_limit()
@@ -365,6 +350,13 @@ size_t lds_limit()
         if (lim == 0) { lim = 65536; }
     }
     return lim;
+}
+
+template <int FL>
+void launch_fl(int prec, dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
+{
+    if (prec == MCI_PREC_FP64) { hipLaunchKernelGGL(k_fast64<FL>, grid, dim3(WAVE), sh, s, A); }
+    else { hipLaunchKernelGGL(k_fast32<FL>, grid, dim3(WAVE), sh, s, A); }
 }
 } // namespace
 
@@ -379,6 +371,7 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
 {
     int B = -1;
     int seen = 0;
+    bool x2 = false;
     for (int i = 0; i < nobs; ++i) {
         const ObsDev &o = obs[i];
         if (o.nskip != 1 || o.acc == ACC_FULL) { return false; }
@@ -386,11 +379,12 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
         const int bit = 1 << o.kind;
         if (seen & bit) { return false; }
         seen |= bit;
+        x2 = x2 || o.kind == MCI_OBS_X2;
         const int bs = (int)(o.acc == ACC_SIMPLE ? -2 : o.blocksize);
         if (B == -1) { B = bs; }
         else if (B != bs) { return false; }
     }
-    return lds_bytes(prec, ndim, nobs > 0) <= lds_limit();
+    return lds_bytes(prec, ndim, x2) <= lds_limit();
 }
 
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s, const int *skip)
@@ -410,30 +404,33 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     A.acc = c.acc;
     A.rej = c.rej;
     A.nvecs = c.nvecs;
+    A.lemire_thr = (0u - (uint32_t)c.nvecs) % (uint32_t)c.nvecs;
     A.blocksize = 1;
     A.nblocks = 0;
     A.d_x2 = A.d_x2sum = A.d_exp = nullptr;
     A.x2sum_div = 1.0;
     A.exp_div = (double)c.ndim;
-    const bool acc = obs != nullptr && nobs > 0;
+    int fl = 0;
     for (int i = 0; i < nobs && obs; ++i) {
         const ObsDev &o = obs[i];
         A.blocksize = (o.acc == ACC_SIMPLE) ? (int)nsteps : o.blocksize;
         A.nblocks = o.nstore;
-        if (o.kind == MCI_OBS_X2) { A.d_x2 = o.data; }
-        if (o.kind == MCI_OBS_X2_SUM) { A.d_x2sum = o.data; A.x2sum_div = o.p0; }
-        if (o.kind == MCI_OBS_EXP_X2) { A.d_exp = o.data; A.exp_div = o.p0; }
+        if (o.kind == MCI_OBS_X2) { A.d_x2 = o.data; fl |= F_X2; }
+        if (o.kind == MCI_OBS_X2_SUM) { A.d_x2sum = o.data; A.x2sum_div = o.p0; fl |= F_SUM; }
+        if (o.kind == MCI_OBS_EXP_X2) { A.d_exp = o.data; A.exp_div = o.p0; fl |= F_EXP; }
     }
     if (!A.a) { return MCI_E_INVALID_ARGUMENT; } // the fast path needs explicit weights
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
-    const size_t sh = lds_bytes(prec, c.ndim, acc);
-    if (prec == MCI_PREC_FP64) {
-        if (acc) { hipLaunchKernelGGL(k_fast64<true>, grid, dim3(WAVE), sh, s, A); }
-        else { hipLaunchKernelGGL(k_fast64<false>, grid, dim3(WAVE), sh, s, A); }
-    }
-    else {
-        if (acc) { hipLaunchKernelGGL(k_fast32<true>, grid, dim3(WAVE), sh, s, A); }
-        else { hipLaunchKernelGGL(k_fast32<false>, grid, dim3(WAVE), sh, s, A); }
+    const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0);
+    switch (fl) {
+    case 0: launch_fl<0>(prec, grid, sh, s, A); break;
+    case 1: launch_fl<1>(prec, grid, sh, s, A); break;
+    case 2: launch_fl<2>(prec, grid, sh, s, A); break;
+    case 3: launch_fl<3>(prec, grid, sh, s, A); break;
+    case 4: launch_fl<4>(prec, grid, sh, s, A); break;
+    case 5: launch_fl<5>(prec, grid, sh, s, A); break;
+    case 6: launch_fl<6>(prec, grid, sh, s, A); break;
+    default: launch_fl<7>(prec, grid, sh, s, A); break;
     }
     return hipGetLastError() == hipSuccess ? MCI_OK : MCI_E_DEVICE;
 }
