@@ -104,12 +104,14 @@ def cpu_baseline(budget_s=20.0):
                 seconds=wall)
 
 
-def traffic_from_profile():
+def profile_summary():
+    """HBM bytes per launch of the sampling kernel (FETCH_SIZE + WRITE_SIZE, each scaled by a
+    1 GiB copy probe) and its VALU-busy fraction, from the committed rocprofv3 summary."""
     p = os.path.join(ROOT, "profiles", "traffic_cfg2_sample.json")
     if not os.path.exists(p):
-        return None
+        return {}
     with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+        return json.load(f)
 
 
 def main():
@@ -160,6 +162,7 @@ def main():
     sig = sigmas()
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
     achieved = FLOPS_PER_STEP * W * NMC / (kt[2] / 1000) / 1e12
+    prof = profile_summary() if args.precision == 0 else {}
     line = {
         "metric": "MC steps/s and ESS/s per GPU at matched error bar vs plain FP64 Metropolis",
         "value": value,
@@ -182,9 +185,12 @@ def main():
         "phase_ms": {"calibration": kt[0], "decorrelation": kt[1], "sampling_kernel": kt[2], "estimation": kt[3]},
         "max_abs_z_vs_analytic": float(np.max(np.abs(avg - exact) / err)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": traffic_from_profile(),
-                     "kernel": "sampling kernel (calibrated chain, 10240 steps x 65536 walkers)",
-                     "kernel_ms": kt[2]},
+                     "frac": achieved / PEAK_FP64, "traffic": prof.get("bytes_per_launch"),
+                     "kernel": "k_fast64 sampling kernel (10240 steps x 65536 walkers)",
+                     "kernel_ms": kt[2], "flops_per_walker_step": FLOPS_PER_STEP,
+                     "valu_busy_rocprof": prof.get("valu_busy"),
+                     "instructions_per_wave_step_rocprof": (prof.get("instructions_per_walker_step") or {}).get("SQ_INSTS"),
+                     "hbm_gbps": (prof["bytes_per_launch"] / (kt[2] / 1000) / 1e9) if "bytes_per_launch" in prof else None},
     }
     if not args.no_cpu:
         cb = cpu_baseline()
