@@ -80,6 +80,8 @@ typedef int mci_status;
 #define MCI_OBS_CONST       5 /* nobs 1: out = c (Constval). params: [c] */
 #define MCI_OBS_EXP_X2      6 /* nobs 1: out = exp(-((0 + x0*x0 + ...)/c)). params: [c] (default ndim) */
 #define MCI_OBS_INDICATOR   7 /* nobs 1: out = x_j > 0 ? 1 : 0. params: [j] (default 0) */
+#define MCI_OBS_GAUSS_X2    8 /* nobs 1: out = exp(-s)*s*c with s = x0*x0 + x1*x1 + ... (GaussXSquared,
+                                 TestMCIFunctions.hpp:278-295). params: [c] (default 1./sqrt(pi*pi*pi)/3.) */
 
 /* ---- random-number modes ---- */
 #define MCI_RNG_MT19937_64  0 /* reference streams: one std::mt19937_64 per walker, libstdc++ distributions

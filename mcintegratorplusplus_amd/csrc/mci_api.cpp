@@ -107,7 +107,8 @@ int obs_nobs(int kind, int ndim)
 {
     switch (kind) {
     case MCI_OBS_X2: case MCI_OBS_X: return ndim;
-    case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR: return 1;
+    case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR:
+    case MCI_OBS_GAUSS_X2: return 1;
     }
     return -1;
 }
@@ -291,6 +292,7 @@ double obs_p0(const ObsHost &o, int ndim)
     case MCI_OBS_CONST: return o.params.empty() ? 1.0 : o.params[0];
     case MCI_OBS_EXP_X2: return o.params.empty() ? (double)ndim : o.params[0];
     case MCI_OBS_INDICATOR: return o.params.empty() ? 0.0 : o.params[0];
+    case MCI_OBS_GAUSS_X2: return o.params.empty() ? 1. / sqrt(M_PI * M_PI * M_PI) / 3. : o.params[0];
     }
     return 0.0;
 }

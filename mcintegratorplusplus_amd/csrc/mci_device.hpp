@@ -466,6 +466,12 @@ __device__ inline void obs_eval(const Chain &c, const ObsDev &o, int64_t w)
     case MCI_OBS_INDICATOR:
         out[w] = (x[(int64_t)(int)o.p0 * W + w] > 0.0) ? 1.0 : 0.0;
         break;
+    case MCI_OBS_GAUSS_X2: {
+        double s = 0.0;
+        for (int i = 0; i < c.ndim; ++i) { const double v = x[(int64_t)i * W + w]; s = s + v * v; }
+        out[w] = exp(-s) * s * o.p0;
+        break;
+    }
     }
 }
 

@@ -27,6 +27,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
 /* ------------------------------------------------------------------------- */
 /* errors                                                                     */
 /* ------------------------------------------------------------------------- */
@@ -523,6 +527,12 @@ static void obs_eval(const obs_desc *o, const double *x, double *out)
         out[0] = (x[j] > 0.0) ? 1.0 : 0.0;
         break;
     }
+    case MCI_OBS_GAUSS_X2: { /* TestMCIFunctions.hpp:285-293 */
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s = s + x[i] * x[i];
+        out[0] = exp(-s) * s * obs_param(o, 0, 1. / sqrt(M_PI * M_PI * M_PI) / 3.);
+        break;
+    }
     }
 }
 
@@ -541,7 +551,8 @@ static int obs_nobs(int kind, int ndim)
 {
     switch (kind) {
     case MCI_OBS_X2: case MCI_OBS_X: return ndim;
-    case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR: return 1;
+    case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR:
+    case MCI_OBS_GAUSS_X2: return 1;
     }
     return -1;
 }

@@ -33,7 +33,7 @@ namespace
 {
 // functor kinds, as in include/mci_amd.h
 enum { PDF_GAUSS_ISO = 1, PDF_GAUSS_DIAG = 2, PDF_EXP_DIAG = 3, PDF_GMM_ISO = 4 };
-enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7 };
+enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7, OBS_GAUSS_X2 = 8 };
 
 std::string hx(double v)
 {
@@ -173,6 +173,7 @@ public:
         case OBS_CONST: out[0] = par(0, 1.); break;
         case OBS_EXP_X2: { double s = 0.; for (int i = 0; i < _ndim; ++i) { s = s + in[i]*in[i]; } out[0] = exp(-(s/par(0, _ndim))); break; }
         case OBS_INDICATOR: out[0] = (in[static_cast<int>(par(0, 0.))] > 0.) ? 1. : 0.; break;
+        case OBS_GAUSS_X2: { double s = 0.; for (int i = 0; i < _ndim; ++i) { s = s + in[i]*in[i]; } out[0] = exp(-s)*s*par(0, 1./sqrt(M_PI*M_PI*M_PI)/3.); break; }
         default: throw std::invalid_argument("unknown observable kind");
         }
     }

@@ -1,0 +1,227 @@
+// The checks of the reference's unit tests test/ut2, ut3 and ut4 (main.cpp of each), run
+// through the C++ facade include/mci_amd/MCIntegrator.hpp on the GPU library.  With the
+// default random mode (per-walker std::mt19937_64 streams) the chains are the reference's
+// own, so the seeded assertions hold exactly as they do there.  Also checks the facade's
+// exception mapping and a many-walker context.
+//
+// Built by tests/cpp/Makefile; run by tests/test_cpp_facade.py.  Prints one line per section
+// and exits non-zero on the first failed check.
+#include "mci_amd/MCIntegrator.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <vector>
+
+using namespace mci;
+
+static int g_checks = 0;
+#define CHECK(cond)                                                                          \
+    do {                                                                                     \
+        ++g_checks;                                                                          \
+        if (!(cond)) {                                                                       \
+            std::fprintf(stderr, "CHECK failed at %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            std::exit(1);                                                                    \
+        }                                                                                    \
+    } while (0)
+
+template <class E, class F>
+static bool throws(F f)
+{
+    try {
+        f();
+    }
+    catch (const E &) {
+        return true;
+    }
+    catch (...) {
+        return false;
+    }
+    return false;
+}
+
+static void ut2()
+{
+    MCI mci(3);
+    for (int i = 0; i < 3; ++i) { CHECK(mci.getX(i) == 0.); }
+    mci.moveX();
+    bool same = true;
+    for (int i = 0; i < 3; ++i) { same = same && mci.getX(i) == 0.; }
+    CHECK(!same);
+
+    mci.setIRange(-5., 5.); // periodic box, as OrthoPeriodicDomain(3, -5., 5.)
+    mci.centerX();
+    for (int i = 0; i < 3; ++i) { CHECK(mci.getX(i) == 0.); }
+    mci.newRandomX();
+    for (int i = 0; i < 3; ++i) { CHECK(mci.getX(i) != 0.); }
+    mci.resetDomain();
+
+    const int NMC = 10000;
+    const double CORRECT = 0.5;
+    mci.addSamplingFunction(ThreeDimGaussianPDF());
+    mci.addObservable(XSquared());
+    double x[3] = {5., -5., 10.};
+    mci.setSeed(5649871);
+    double average, error;
+
+    mci.setX(x);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) > 2. * error);
+
+    mci.setX(x);
+    mci.integrate(NMC, &average, &error);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.setIRange(-5., 5.);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.popSamplingFunction();
+    mci.popObservable();
+    mci.addObservable(GaussXSquared(), 1, 1, false, false);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+    std::printf("ut2 ok: GaussXSquared without sampling function %.6f +- %.6f\n", average, error);
+}
+
+static void ut3()
+{
+    const int NMC = 10000;
+    const double CORRECT = 0.5;
+    MCI mci(3);
+    mci.setSeed(5649871);
+    mci.addSamplingFunction(ThreeDimGaussianPDF());
+    mci.addObservable(XSquared());
+    mci.addObservable(XYZSquared());
+    double x[3] = {5., -5., 10.};
+    double average[4], error[4];
+
+    mci.setX(x);
+    mci.integrate(NMC, average, error, false, false);
+    for (int i = 0; i < mci.getNObsDim(); ++i) { CHECK(std::fabs(average[i] - CORRECT) > 2. * error[i]); }
+
+    mci.setX(x);
+    mci.integrate(NMC, average, error);
+    for (int i = 0; i < mci.getNObsDim(); ++i) { CHECK(std::fabs(average[i] - CORRECT) < 2. * error[i]); }
+
+    mci.integrate(NMC, average, error, false, false);
+    for (int i = 0; i < mci.getNObsDim(); ++i) { CHECK(std::fabs(average[i] - CORRECT) < 2. * error[i]); }
+    std::printf("ut3 ok: nobsdim %d\n", mci.getNObsDim());
+}
+
+static void ut4()
+{
+    const int NMC = 16384;
+    const double CORRECT = 0.5;
+    MCI mci(3);
+    mci.setSeed(1337);
+    mci.addSamplingFunction(ThreeDimGaussianPDF());
+    mci.addObservable(XSquared());
+    double x[3] = {5., -5., 10.};
+    double average, error;
+
+    mci.setNfindMRT2Iterations(20);
+    mci.setNdecorrelationSteps(2000);
+    CHECK(mci.getNfindMRT2Iterations() == 20 && mci.getNdecorrelationSteps() == 2000);
+
+    mci.setX(x);
+    mci.integrate(NMC, &average, &error, true, true);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.clearObservables();
+    mci.addObservable(XSquared(), 16);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.clearObservables();
+    mci.addObservable(XSquared(), 8, 2);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.clearObservables();
+    mci.addObservable(XSquared(), 8, 2, false, true);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+
+    mci.clearObservables();
+    mci.addObservable(XSquared(), 8, 2, false, EstimatorType::Correlated);
+    mci.integrate(NMC, &average, &error, false, false);
+    CHECK(std::fabs(average - CORRECT) < 2. * error);
+    std::printf("ut4 ok: acceptance %.4f\n", mci.getAcceptanceRate());
+}
+
+static void exceptions()
+{
+    // the reference's exception types for misuse (MCIntegrator.cpp:46,457-460, BlockAccumulator.cpp:12, MJBlocker.cpp:29, Factories.hpp:65)
+    CHECK(throws<std::invalid_argument>([] { MCI m(0); }));
+    MCI mci(3);
+    CHECK(throws<std::invalid_argument>([&] { mci.addObservable(X2(2)); }));
+    CHECK(throws<std::invalid_argument>([&] { mci.addObservable(XSquared(), 0, 1, true, EstimatorType::Noop); }));
+    CHECK(throws<std::invalid_argument>([] { selectEstimatorType(true, false); }));
+    mci.addObservable(XSquared());
+    double a, e;
+    CHECK(throws<std::domain_error>([&] { mci.integrate(1000, &a, &e); })); // infinite domain, no sampling function
+    mci.clearObservables();
+    mci.addSamplingFunction(ThreeDimGaussianPDF());
+    mci.addObservable(XSquared(), 16);
+    CHECK(throws<std::invalid_argument>([&] { mci.integrate(1000, &a, &e); })); // 1000 % 16 != 0
+    CHECK(throws<std::invalid_argument>([] {
+        std::vector<double> x(3, 1.);
+        double av[3], er[3];
+        MJBlockerEstimator(3, 1, x.data(), av, er); // MJBlocker needs a power of 2
+    }));
+    std::printf("exceptions ok\n");
+}
+
+static void walkers()
+{
+    // 256 independent chains on one context, combined as the reference's MPIMCI combines ranks
+    const int64_t W = 256;
+    MCI mci(3, W);
+    mci.setSeed(2024);
+    mci.setRandomMode(RandomMode::Philox);
+    mci.addSamplingFunction(Gauss(3));
+    mci.addObservable(X2(3), 64, 1, true, EstimatorType::MJBlocker);
+    double average[3], error[3];
+    mci.integrate(64 * 128, average, error);
+    std::vector<double> wa, we;
+    mci.getWalkerResults(wa, we);
+    CHECK((int64_t)wa.size() == 3 * W);
+    for (int i = 0; i < 3; ++i) {
+        double s = 0., s2 = 0.;
+        for (int64_t w = 0; w < W; ++w) {
+            s += wa[w * 3 + i];
+            s2 += we[w * 3 + i] * we[w * 3 + i];
+        }
+        CHECK(std::fabs(average[i] - s / W) < 1e-12);
+        CHECK(std::fabs(error[i] - std::sqrt(s2) / W) < 1e-12 * error[i] + 1e-300);
+        CHECK(std::fabs(average[i] - 0.5) < 4. * error[i]); // <x_i^2> = 1/2 under exp(-sum x^2)
+    }
+    std::printf("walkers ok: <x0^2> = %.6f +- %.6f over %lld walkers\n", average[0], error[0], (long long)W);
+}
+
+int main()
+{
+    int ndev = 0;
+    check(mci_device_count(&ndev));
+    if (ndev < 1) {
+        std::fprintf(stderr, "no GPU visible\n");
+        return 2;
+    }
+    std::printf("%s\n", mci_version());
+    std::fflush(stdout);
+    ut2();
+    ut3();
+    ut4();
+    exceptions();
+    walkers();
+    std::printf("all %d checks passed\n", g_checks);
+    return 0;
+}

@@ -19,7 +19,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from mcitest import (ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, OBS_CONST, OBS_EXP_X2,  # noqa: E402
+from mcitest import (ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, OBS_CONST, OBS_EXP_X2, OBS_GAUSS_X2,  # noqa: E402
                      OBS_INDICATOR, OBS_X, OBS_X2, OBS_X2_SUM, OBS_X_SUM, PDF_EXP_DIAG, PDF_GAUSS_DIAG,
                      PDF_GAUSS_ISO, PDF_GMM_ISO, REF_BIN, ROOT, fmt, ref_run, script_setup)
 
@@ -105,6 +105,10 @@ def scenarios():
     # no sampling function: uniform random sampling in a periodic box with volume scaling
     S.append(dict(name="nopdf_volume", cfg=dict(ndim=3, seed=5649871, irange=([-5.] * 3, [5.] * 3),
                                                 obs=[(OBS_X2_SUM, [3.0], 1, 1, 0, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=10000, find=0, decorr=0)]))
+    # the last integral of test/ut2/main.cpp: GaussXSquared without sampling function in [-5, 5]^3
+    S.append(dict(name="ut2_nopdf_gauss_x2", cfg=dict(ndim=3, seed=5649871, irange=([-5.] * 3, [5.] * 3),
+                                                      obs=[(OBS_GAUSS_X2, [], 1, 1, 0, ESTIM_UNCORR)]),
                   calls=[dict(nmc=10000, find=0, decorr=0)]))
     return S
 

@@ -5,7 +5,7 @@ import pytest
 from scipy import stats
 
 from mcitest import (Chain, MciError, E_INVALID, E_DOMAIN, ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP,
-                     ESTIM_UNCORR, OBS_X, OBS_X2, OBS_X2_SUM, PDF_GAUSS_DIAG, PDF_GAUSS_ISO, chain_setup, estimate)
+                     ESTIM_UNCORR, OBS_GAUSS_X2, OBS_X, OBS_X2, OBS_X2_SUM, PDF_GAUSS_DIAG, PDF_GAUSS_ISO, chain_setup, estimate)
 
 
 def test_chi2_table_is_the_095_quantile():
@@ -96,6 +96,15 @@ def test_gaussian_physics_within_error_bars():
     c = chain_setup("orc", cfg, nwalkers=16)
     avg, err = c.integrate(64 * 256)
     assert np.all(np.abs(avg - 0.5 / a_i) < 4 * err)
+
+
+def test_uniform_sampling_volume_integral():
+    # no sampling function: integrate returns <f> * V over the box (MCIntegrator.cpp:190-198).
+    # int exp(-r^2) r^2 d^3x / (3 pi^1.5) = 1/2 over R^3; [-5, 5]^3 misses < 1e-9 of it
+    c = chain_setup("orc", dict(ndim=3, seed=17, irange=([-5.] * 3, [5.] * 3),
+                                obs=[(OBS_GAUSS_X2, [], 1, 1, 0, ESTIM_UNCORR)]), nwalkers=16)
+    avg, err = c.integrate(20000, False, False)
+    assert abs(avg[0] - 0.5) < 4 * err[0] and err[0] < 0.01
 
 
 def test_errors_map_to_reference_exception_types():
