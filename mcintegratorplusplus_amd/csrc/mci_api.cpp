@@ -976,6 +976,37 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
         return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports Philox streams with single-coordinate uniform moves on one diagonal "
                                             "Gaussian (explicit weights) in an unbounded domain");
     }
+    std::vector<ObsDev> od;
+    c->nstore_last.clear();
+    bool fast_main = false;
+    if (nmc > 0) {
+        // ---- accumulators (MCIntegrator.cpp:54-57), allocated before any work so that a
+        //      configuration the kernels cannot run fails before calibration ----
+        if (c->accdata.size() < c->obs.size()) { c->accdata.resize(c->obs.size()); c->scratch.resize(c->obs.size()); }
+        int voff = 0;
+        for (size_t i = 0; i < c->obs.size(); ++i) {
+            const ObsHost &o = c->obs[i];
+            ObsDev d;
+            std::memset(&d, 0, sizeof d);
+            d.kind = o.kind;
+            d.nobs = o.nobs;
+            d.voff = voff;
+            d.nskip = o.nskip;
+            d.acc = (o.blocksize == 0) ? ACC_SIMPLE : (o.blocksize == 1 ? ACC_FULL : ACC_BLOCK);
+            d.blocksize = std::max(o.blocksize, 1);
+            d.p0 = obs_p0(o, c->ndim);
+            d.nstore = nstore_of(o.blocksize, o.nskip, nmc);
+            HIPCHK(c->accdata[i].ensure((size_t)d.nstore * o.nobs * W));
+            d.data = c->accdata[i].p;
+            if (needs_scratch(o.estim, d.nstore)) { HIPCHK(c->scratch[i].ensure((size_t)d.nstore * o.nobs * W)); }
+            od.push_back(d);
+            voff += o.nobs;
+        }
+        fast_main = fast && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec);
+        if (c->prec == MCI_PREC_FP32 && !fast_main) {
+            return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
+        }
+    }
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // ---- calibration (MCIntegrator.cpp:49-51) ----
     if (!c->pdfs.empty() && do_find && c->ntypes > 0) {
@@ -1002,35 +1033,9 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
         c->timings[4] = elapsed(c, 0, 2);
         return MCI_OK;
     }
-    // ---- allocate accumulators and sample (MCIntegrator.cpp:54-62) ----
-    if (c->accdata.size() < c->obs.size()) { c->accdata.resize(c->obs.size()); c->scratch.resize(c->obs.size()); }
-    std::vector<ObsDev> od;
-    int voff = 0;
-    for (size_t i = 0; i < c->obs.size(); ++i) {
-        const ObsHost &o = c->obs[i];
-        ObsDev d;
-        std::memset(&d, 0, sizeof d);
-        d.kind = o.kind;
-        d.nobs = o.nobs;
-        d.voff = voff;
-        d.nskip = o.nskip;
-        d.acc = (o.blocksize == 0) ? ACC_SIMPLE : (o.blocksize == 1 ? ACC_FULL : ACC_BLOCK);
-        d.blocksize = std::max(o.blocksize, 1);
-        d.p0 = obs_p0(o, c->ndim);
-        d.nstore = nstore_of(o.blocksize, o.nskip, nmc);
-        HIPCHK(c->accdata[i].ensure((size_t)d.nstore * o.nobs * W));
-        d.data = c->accdata[i].p;
-        if (needs_scratch(o.estim, d.nstore)) { HIPCHK(c->scratch[i].ensure((size_t)d.nstore * o.nobs * W)); }
-        od.push_back(d);
-        voff += o.nobs;
-    }
-    c->nstore_last.clear();
+    // ---- sample (MCIntegrator.cpp:58-62) ----
     for (auto &d : od) { c->nstore_last.push_back(d.nstore); }
     Chain ch = make_chain(c, &od);
-    const bool fast_main = fast && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec);
-    if (c->prec == MCI_PREC_FP32 && !fast_main) {
-        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
-    }
     if (fast_main) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
     else { HIPCHK(launch_sample(ch, nmc, 1, nullptr, c->stream)); }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
