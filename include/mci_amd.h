@@ -201,6 +201,70 @@ mci_status mci_get_timings(mci_ctx *ctx, double *ms, int n);
  *      run on the device: Estimators.cpp:125-155,191-246,271-286, MJBlocker.cpp:127-154 ---- */
 mci_status mci_estimate(int estimator, int64_t n, int ndim, const double *x, double *average, double *error);
 
+/* ---- batched integrals ----
+ * Many independent integrals in one submission.  Each integral i is what one reference MCI
+ * object computes with an isotropic Gaussian-mixture sampling function
+ * (weights w_k, widths s_k, means mu_k; MCI_PDF_GMM_ISO's density), the |x|^2 observable
+ * (MCI_OBS_X2_SUM, c = 1), single-coordinate uniform moves (SRRDVecMove, veclen 1) and step
+ * calibration (MCI::findMRT2Step src/MCIntegrator.cpp:87-168, with one wave of 64 pooled lanes
+ * as 64 MPI ranks) — i.e. it replaces a loop of MCI::integrate calls (src/MCIntegrator.cpp:43-82),
+ * one per integral, whose results MPIMCI would combine (src/MPIMCI.cpp:85-92).
+ *
+ * Work per integral is split over R_i walkers of T_i steps each (R_i * T_i >= nmc_i); each
+ * walker starts from an exact draw of its mixture (so it is stationary from step 0), keeps FP32
+ * coordinates with Kahan-compensated FP32 block sums flushed to FP64, and stores nblocks block
+ * means.  Walker means give avg_i (FP64, fixed summation order); err_i is the standard error of
+ * the R_i independent walker means (MCI_BATCH_ERR_SPREAD, the default) or the MPIMCI combine of
+ * per-walker MJBlocker errors (MCI_BATCH_ERR_COMBINE).  R_i and T_i (hence the block length
+ * T_i / nblocks) come from the autotuner unless set.  All walkers run in one persistent launch
+ * that takes 64-walker work items from a device queue.
+ *
+ * Parameter arrays are concatenated over integrals in index order:
+ *   ncomp[i] in 1..4, weights and widths [sum ncomp], means [sum ncomp_i * ndim_i]
+ *   (component-major within an integral).  ndim[i] in 1..MCI_BATCH_MAX_DIM. */
+#define MCI_BATCH_MAX_DIM      256
+#define MCI_BATCH_MAX_COMP     4
+#define MCI_BATCH_ERR_SPREAD   0
+#define MCI_BATCH_ERR_COMBINE  1
+typedef struct mci_batch mci_batch;
+mci_status mci_batch_create(int nint, const int *ndim, const int64_t *nmc, const int *ncomp, const double *weights,
+                            const double *widths, const double *means, mci_batch **out);
+mci_status mci_batch_destroy(mci_batch *b);
+mci_status mci_batch_set_seed(mci_batch *b, uint64_t seed);
+/* blocks per walker: a power of two in 4..64 (default 16) */
+mci_status mci_batch_set_blocks(mci_batch *b, int nblocks);
+/* fewest walkers per integral the autotuner aims for (default 16; 1..65536) */
+mci_status mci_batch_set_min_walkers(mci_batch *b, int rmin);
+mci_status mci_batch_set_error_mode(mci_batch *b, int mode);
+/* calibration: MCI::setNfindMRT2Iterations / setTargetAcceptanceRate semantics (defaults -50, 0.5);
+ * nfind == 0 keeps the initial step 2 * sum_k w_k s_k */
+mci_status mci_batch_set_calibration(mci_batch *b, int nfind, double target_rate);
+/* decorrelation steps per walker before accumulation (default 0: walkers start stationary) */
+mci_status mci_batch_set_ndecorrelation_steps(mci_batch *b, int64_t nsteps);
+/* explicit plan: steps per walker T_i (a power of two times nblocks); NULL returns to autotuning */
+mci_status mci_batch_set_plan(mci_batch *b, const int64_t *steps_per_walker);
+/* autotune the plan: measure == 0 uses the cost model only (no device work), measure > 0 also
+ * times that many candidate plans on the device and keeps the fastest */
+mci_status mci_batch_autotune(mci_batch *b, int measure);
+/* the plan in force: walkers R_i and steps per walker T_i, [nint] each; the walkers of integral i
+ * are walker_offset[i] .. walker_offset[i] + R_i - 1 in the walker order of the per-walker getters */
+mci_status mci_batch_get_plan(mci_batch *b, int64_t *walkers, int64_t *steps_per_walker, int64_t *walker_offset);
+mci_status mci_batch_get_nwalkers(mci_batch *b, int64_t *nwalkers);
+/* calibrate, sample, estimate: avg/err [nint] in integral order */
+mci_status mci_batch_run(mci_batch *b, double *average, double *error);
+/* capture calibration, sampling and estimation into one hipGraph and replay it on later runs
+ * (default on) */
+mci_status mci_batch_set_graph(mci_batch *b, int on);
+/* results of the last run: calibrated step and its last pooled acceptance rate [nint];
+ * acceptance rate of the sampling walkers [nint]; walker means/errors [nwalkers];
+ * block means [nblocks][nwalkers] */
+mci_status mci_batch_get_steps(mci_batch *b, double *steps, double *calib_rate);
+mci_status mci_batch_get_acceptance(mci_batch *b, double *rate);
+mci_status mci_batch_get_walker_results(mci_batch *b, double *average, double *error, int64_t n);
+mci_status mci_batch_get_block_means(mci_batch *b, double *data, int64_t n);
+/* [0] calibration, [1] sampling (persistent launch), [2] estimation+combine, [3] total (ms) */
+mci_status mci_batch_get_timings(mci_batch *b, double *ms, int n);
+
 #ifdef __cplusplus
 }
 #endif

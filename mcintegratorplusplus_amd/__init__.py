@@ -6,6 +6,7 @@ the built library raises ImportError, and every call runs on the GPU.
 """
 from ._lib import library_path, load  # noqa: F401
 from .mci import MCI, MciError  # noqa: F401
+from .batch import BatchIntegrals, mixture_workload  # noqa: F401
 from . import constants  # noqa: F401
 
-__all__ = ["MCI", "MciError", "constants", "library_path", "load"]
+__all__ = ["MCI", "MciError", "BatchIntegrals", "mixture_workload", "constants", "library_path", "load"]

@@ -4,7 +4,8 @@
 
 Every HIP source is compiled by hipcc for gfx950 only, with FP contraction off (the FP64
 chain is bit-exact with the reference; fused operations are written explicitly where
-wanted) and packed-FP32 instructions disabled.
+wanted), IEEE correctly rounded FP32 divide and sqrt (the batched FP32 chain is
+reproduced bit for bit by the CPU oracle) and packed-FP32 instructions disabled.
 """
 import concurrent.futures as cf
 import os
@@ -15,10 +16,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT, "libmci_amd.so")
-SOURCES = ["mci_kernels.hip", "mci_fast.hip", "mci_api.cpp"]
+SOURCES = ["mci_kernels.hip", "mci_fast.hip", "mci_batch.hip", "mci_api.cpp", "mci_batch_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCI_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+         "-fhip-fp32-correctly-rounded-divide-sqrt",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 

@@ -13,3 +13,6 @@ RNG_MT19937_64, RNG_PHILOX = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 COUPLING_INDEPENDENT, COUPLING_SHARED = 0, 1
 KERNELS_AUTO, KERNELS_GENERIC = 0, 1
+
+# batched integrals: error of each integral
+BATCH_ERR_SPREAD, BATCH_ERR_COMBINE = 0, 1

@@ -4,6 +4,7 @@
 // stream; the host only steers the loops whose exit depends on a statistic.
 #include "mci_kernels.hpp"
 #include "mci_fast.hpp"
+#include "mci_host.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -18,56 +19,17 @@
 
 using namespace mcid;
 
+using mcih::DBuf;
+using mcih::det_sum;
+using mcih::fail;
+using mcih::g_err;
+
 namespace
 {
-thread_local std::string g_err;
-
-mci_status fail(mci_status code, const char *fmt, ...)
-{
-    char buf[600];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                                     \
-    do {                                                                                                 \
-        hipError_t _e = (expr);                                                                          \
-        if (_e != hipSuccess) { return fail(MCI_E_DEVICE, "%s: %s", #expr, hipGetErrorString(_e)); }     \
-    } while (0)
-#define TRY(expr)                                                                                        \
-    do {                                                                                                 \
-        mci_status _s = (expr);                                                                          \
-        if (_s != MCI_OK) { return _s; }                                                                 \
-    } while (0)
 #define CHECKCTX(c)                                                                                      \
     do {                                                                                                 \
         if (!(c)) { return fail(MCI_E_INVALID_ARGUMENT, "null context"); }                               \
     } while (0)
-
-// device buffer that only grows
-template <class T>
-struct DBuf
-{
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    ~DBuf() { if (p) { (void)hipFree(p); } }
-    hipError_t ensure(size_t m)
-    {
-        if (m <= n && p) { return hipSuccess; }
-        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
-        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(m, 1) * sizeof(T));
-        if (e == hipSuccess) { n = m; }
-        return e;
-    }
-};
 
 uint64_t splitmix64(uint64_t &s)
 {
@@ -84,23 +46,6 @@ uint64_t walker_seed(uint64_t seed, int64_t w)
     if (w == 0) { return seed; }
     uint64_t s = (seed ^ 0x6A09E667F3BCC909ULL) + (uint64_t)w * 0x9E3779B97F4A7C15ULL;
     return splitmix64(s);
-}
-
-// the deterministic walker sum used on the device (chunks of 256, pairwise tree, chunks in order)
-double det_sum(const double *v, int64_t n)
-{
-    double total = 0.0;
-    bool first = true;
-    for (int64_t c0 = 0; c0 < n; c0 += 256) {
-        double buf[256];
-        for (int i = 0; i < 256; ++i) { buf[i] = (c0 + i < n) ? v[c0 + i] : 0.0; }
-        for (int s = 128; s > 0; s >>= 1) {
-            for (int i = 0; i < s; ++i) { buf[i] = buf[i] + buf[i + s]; }
-        }
-        if (first) { total = buf[0]; first = false; }
-        else { total = total + buf[0]; }
-    }
-    return total;
 }
 
 int obs_nobs(int kind, int ndim)

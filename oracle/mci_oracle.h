@@ -98,6 +98,20 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 /* deterministic reduction used for every cross-walker sum (chunks of 256, tree inside, chunks in order) */
 double orc_det_sum(const double *v, int64_t n);
 
+/* batched Gaussian-mixture integrals (mci_batch_oracle.c): FP32 exp/log of the batched chain,
+ * the walker order of a plan, and one complete run of a given plan (T = steps per walker).
+ * Output arrays may be NULL except avg/err; walker arrays have the plan's walker count,
+ * bmeans / bmeans_fp64 are [nblocks][nwalkers] (bmeans_fp64: the same samples summed in FP64). */
+float orc_bexp(float x);
+float orc_blog(float x);
+mci_status orc_batch_walker_order(int nint, const int *ndim, const int *ncomp, const int64_t *nmc, const int64_t *T,
+                                  int64_t *walkers, int64_t *woff, int64_t *nwalkers);
+mci_status orc_batch_run(int nint, const int *ndim, const int64_t *nmc, const int *ncomp, const double *weights,
+                         const double *widths, const double *means, uint64_t seed, int nblocks, int nfind, double target,
+                         int64_t ndecorr, int mode, const int64_t *T, double *avg, double *err, double *steps,
+                         double *calib_rate, double *accrate, double *wavg, double *werr, double *bmeans,
+                         double *bmeans_fp64);
+
 #ifdef __cplusplus
 }
 #endif
