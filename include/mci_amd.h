@@ -233,7 +233,7 @@ mci_status mci_batch_destroy(mci_batch *b);
 mci_status mci_batch_set_seed(mci_batch *b, uint64_t seed);
 /* blocks per walker: a power of two in 4..64 (default 16) */
 mci_status mci_batch_set_blocks(mci_batch *b, int nblocks);
-/* fewest walkers per integral the autotuner aims for (default 16; 1..65536) */
+/* fewest walkers per integral the autotuner aims for (default 64; 1..65536) */
 mci_status mci_batch_set_min_walkers(mci_batch *b, int rmin);
 mci_status mci_batch_set_error_mode(mci_batch *b, int mode);
 /* calibration: MCI::setNfindMRT2Iterations / setTargetAcceptanceRate semantics (defaults -50, 0.5);

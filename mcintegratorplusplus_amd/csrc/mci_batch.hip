@@ -25,7 +25,7 @@
 //                        Marsaglia's polar method two attempts per call
 // Sampling walkers use id = walker index; calibration lanes use id = 2^31 | (64 i + lane).
 #include "mci_batch.hpp"
-#include "mci_device.hpp"
+#include "../../include/mci_amd.h"
 
 #include <float.h>
 
@@ -33,17 +33,51 @@ namespace mcib
 {
 namespace
 {
-using mcid::philox4x32_10;
+// Philox4x32-10 with the round keys precomputed (they are the same for every lane and step)
+struct Keys
+{
+    uint32_t k0[10], k1[10];
+};
+
+__device__ __forceinline__ Keys make_keys(uint32_t k0, uint32_t k1)
+{
+    Keys K;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        K.k0[r] = k0 + (uint32_t)r * 0x9E3779B9u;
+        K.k1[r] = k1 + (uint32_t)r * 0xBB67AE85u;
+    }
+    return K;
+}
+
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, const Keys &K, uint32_t o[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ K.k0[r];
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ K.k1[r];
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = c2;
+    o[3] = c3;
+}
 
 // e^x: Cody–Waite reduction by ln 2 (two-part constant, fused multiply-add), the degree-7
 // Taylor polynomial of e^r on |r| <= ln2/2 (Horner, fused), exact scaling by 2^n.
-// 0 below -87, +inf above 88; at most 0.92 ulp from e^x on [-87, 88].
+// 0 below -87 (and for NaN), +inf above 88; at most 0.92 ulp from e^x on [-87, 88].
+// Branch-free: the range cases are selected at the end.
 __device__ __forceinline__ float bexp(float x)
 {
-    if (!(x >= -87.0f)) { return 0.0f; }
-    if (x > 88.0f) { return __builtin_huge_valf(); }
-    const float n = rintf(x * 0x1.715476p+0f);
-    float r = fmaf(-n, 0x1.62e43p-1f, x);
+    const float xc = fminf(fmaxf(x, -87.0f), 88.0f);
+    const float n = rintf(xc * 0x1.715476p+0f);
+    float r = fmaf(-n, 0x1.62e43p-1f, xc);
     r = fmaf(-n, -0x1.05c61p-29f, r);
     float p = 0x1.a01a02p-13f;
     p = fmaf(p, r, 0x1.6c16c2p-10f);
@@ -53,7 +87,9 @@ __device__ __forceinline__ float bexp(float x)
     p = fmaf(p, r, 0x1p-1f);
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
-    return p * __uint_as_float((uint32_t)((int)n + 127) << 23);
+    float res = p * __uint_as_float((uint32_t)((int)n + 127) << 23);
+    res = (x >= -87.0f) ? res : 0.0f;
+    return (x > 88.0f) ? __builtin_huge_valf() : res;
 }
 
 // log x for positive normal x: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s)
@@ -79,14 +115,18 @@ __device__ __forceinline__ float blog(float x)
 // symmetric grid (2m + 1 - 2^24) 2^-24 in (-1, 1) from the high 24 bits of a word; never 0
 __device__ __forceinline__ float sym24(uint32_t w) { return (float)(2 * (int32_t)(w >> 8) + 1 - 16777216) * 0x1p-24f; }
 
-__device__ __forceinline__ void kahan(float &s, float &c, float y)
+// Kahan-compensated add of y when on (selects, no branch)
+__device__ __forceinline__ void kahan_if(float &s, float &c, float y, bool on)
 {
     const float yy = y - c;
     const float tt = s + yy;
-    c = (tt - s) - yy;
-    s = tt;
+    const float cc = (tt - s) - yy;
+    s = on ? tt : s;
+    c = on ? cc : c;
 }
 
+// walker state.  Components k >= K are inert: lw = -inf and h = 0 make e_k = -inf and
+// exp(e_k - m) = 0 exactly, so a wave can run every lane with the wave's largest K.
 struct BW
 {
     int d, K;
@@ -107,8 +147,8 @@ __device__ __forceinline__ void bw_load(BW &L, const BatchParams &P, int i, uint
     L.thr = (uint32_t)(0u - (uint32_t)L.d) % (uint32_t)L.d;
 #pragma unroll
     for (int k = 0; k < MAXK; ++k) {
-        L.lw[k] = P.lw[i * MAXK + k];
-        L.h[k] = P.h[i * MAXK + k];
+        L.lw[k] = (k < L.K) ? P.lw[i * MAXK + k] : -__builtin_huge_valf();
+        L.h[k] = (k < L.K) ? P.h[i * MAXK + k] : 0.f;
         L.r[k] = 0.f;
     }
     L.mu = P.mu + P.mu_off[i];
@@ -120,10 +160,10 @@ __device__ __forceinline__ void bw_load(BW &L, const BatchParams &P, int i, uint
 }
 
 // exact draw from the mixture into this lane's LDS column
-__device__ void bw_draw(const BW &L, float *xs, int lane, const BatchParams &P, int i)
+__device__ void bw_draw(const BW &L, float *xs, int lane, const BatchParams &P, const Keys &K, int i)
 {
     uint32_t o[4];
-    philox4x32_10(0x40000000u, L.id, 0u, 0u, P.key0, P.key1, o);
+    philox(0x40000000u, L.id, 0u, 0u, K, o);
     const float uc = (float)(o[0] >> 8) * 0x1p-24f;
     int kc = 0;
 #pragma unroll
@@ -133,7 +173,7 @@ __device__ void bw_draw(const BW &L, float *xs, int lane, const BatchParams &P, 
     uint32_t call = 1;
     int j = 0;
     while (j < L.d) {
-        philox4x32_10(0x40000000u + call, L.id, 0u, 0u, P.key0, P.key1, o);
+        philox(0x40000000u + call, L.id, 0u, 0u, K, o);
         ++call;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -189,13 +229,23 @@ __device__ void bw_refresh(BW &L, const float *xs, int lane)
     L.S = S;
 }
 
-// one Metropolis step with a single-coordinate uniform move (SRRDVecMove, veclen 1); every lane
-// of the wave calls it together (wave vote in the index draw)
-__device__ __forceinline__ bool bw_step(BW &L, float *xs, int lane, uint32_t k0, uint32_t k1)
+// the random words and mixture means one step needs, fetched one step ahead so that the global
+// loads of mu_kj are in flight while the previous step computes
+struct Pre
+{
+    uint32_t w1, w2;   // proposal word, acceptance word
+    int j;             // coordinate
+    float mu[MAXK];    // mu_kj
+};
+
+// words of step t (Philox, Lemire index with the rejection words on a wave vote) and the loads
+// of mu_kj for the first KW components; every lane of the wave calls it together
+template <int KW>
+__device__ __forceinline__ void bw_fetch(const BW &L, uint64_t t, Pre &P, const Keys &K)
 {
     uint32_t o[4];
-    const uint32_t tl = (uint32_t)L.t, th = (uint32_t)(L.t >> 32);
-    philox4x32_10(0u, L.id, tl, th, k0, k1, o);
+    const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
+    philox(0u, L.id, tl, th, K, o);
     uint64_t prod = (uint64_t)o[0] * (uint32_t)L.d;
     if (__builtin_expect(__any((uint32_t)prod < L.thr), 0)) {
         uint32_t low = (uint32_t)prod;
@@ -203,39 +253,51 @@ __device__ __forceinline__ bool bw_step(BW &L, float *xs, int lane, uint32_t k0,
         while (low < L.thr) {
             const uint32_t k = 0x80000000u + (kx++);
             uint32_t e[4];
-            philox4x32_10(k >> 2, L.id, tl, th, k0, k1, e);
+            philox(k >> 2, L.id, tl, th, K, e);
             prod = (uint64_t)e[k & 3u] * (uint32_t)L.d;
             low = (uint32_t)prod;
         }
     }
-    const int j = (int)(prod >> 32);
+    P.j = (int)(prod >> 32);
+    P.w1 = o[1];
+    P.w2 = o[2];
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+        const int kk = k < L.K ? k : L.K - 1; // inert components read a valid row
+        P.mu[k] = L.mu[(int64_t)kk * L.d + P.j];
+    }
+}
+
+// one Metropolis step with a single-coordinate uniform move (SRRDVecMove, veclen 1) from the
+// prefetched words of step L.t; fetches step L.t + 1 into cur.  Every lane of the wave calls it.
+template <int KW>
+__device__ __forceinline__ bool bw_step(BW &L, float *xs, int lane, Pre &cur, const Keys &K)
+{
+    const int j = cur.j;
     const float xo = xs[j * WAVE + lane];
-    const float xn = fmaf(L.step, sym24(o[1]), xo);
-    const float ua = (float)(o[2] >> 8) * 0x1p-24f;
+    const float xn = fmaf(L.step, sym24(cur.w1), xo);
+    const float ua = (float)(cur.w2 >> 8) * 0x1p-24f;
+    float mk[KW];
+#pragma unroll
+    for (int k = 0; k < KW; ++k) { mk[k] = cur.mu[k]; }
+    bw_fetch<KW>(L, L.t + 1, cur, K);
     const float dx = xn - xo, sx = xn + xo;
-    float rn[MAXK], en[MAXK];
+    float rn[KW], en[KW];
     float mn = -__builtin_huge_valf();
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) {
-        rn[k] = L.r[k];
-        en[k] = 0.f;
-        if (k < L.K) {
-            const float mk = L.mu[(int64_t)k * L.d + j];
-            rn[k] = fmaf(dx, sx - (mk + mk), L.r[k]);
-            en[k] = fmaf(-L.h[k], rn[k], L.lw[k]);
-            mn = fmaxf(mn, en[k]);
-        }
+    for (int k = 0; k < KW; ++k) {
+        rn[k] = fmaf(dx, sx - (mk[k] + mk[k]), L.r[k]);
+        en[k] = fmaf(-L.h[k], rn[k], L.lw[k]);
+        mn = fmaxf(mn, en[k]);
     }
     float Sn = 0.f;
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) {
-        if (k < L.K) { Sn = Sn + bexp(en[k] - mn); }
-    }
+    for (int k = 0; k < KW; ++k) { Sn = Sn + bexp(en[k] - mn); }
     const float E = bexp(mn - L.m);
     const bool acc = ua * L.S < E * Sn;
     if (acc) { xs[j * WAVE + lane] = xn; }
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) { L.r[k] = acc ? rn[k] : L.r[k]; }
+    for (int k = 0; k < KW; ++k) { L.r[k] = acc ? rn[k] : L.r[k]; }
     L.m = acc ? mn : L.m;
     L.S = acc ? Sn : L.S;
     // x_j'^2 - x_j^2 of FP32 values, exact products in FP64, one rounding
@@ -246,42 +308,68 @@ __device__ __forceinline__ bool bw_step(BW &L, float *xs, int lane, uint32_t k0,
     return acc;
 }
 
-__global__ void k_batch_zero(unsigned int *q) { *q = 0u; }
+__device__ __forceinline__ int wave_max_int(int v)
+{
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const int u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+    return v;
+}
 
-// MCI::findMRT2Step (src/MCIntegrator.cpp:87-168) for integral blockIdx.x with its 64 lanes as
-// 64 ranks: MIN_STAT = max(100, sqrt(40000 d)/64), the rank-averaged acceptance rate (tree sum
+__global__ void k_batch_zero(unsigned int *q, int n)
+{
+    if ((int)threadIdx.x < n) { q[threadIdx.x] = 0u; }
+}
+
+// MCI::findMRT2Step (src/MCIntegrator.cpp:87-168) for one integral with its 64 lanes as 64
+// ranks: MIN_STAT = max(100, sqrt(40000 d)/64), the rank-averaged acceptance rate (tree sum
 // over lanes), MIN_CONS 5, TOL 0.05, factor in [0.5, 2], step in [FLT_MIN, FLT_MAX]
+template <int KW>
+__device__ void calibrate_loop(const CalibArgs &A, BW &L, float *xs, int lane, const Keys &K, double &step, double &rate)
+{
+    const int N = A.nfind;
+    const int64_t min_stat = (int64_t)fmax(100., sqrt(40000. * L.d) / (double)WAVE);
+    int cons = 0, counter = 0;
+    Pre cur;
+    bw_fetch<KW>(L, L.t, cur, K);
+    while ((N < 0 && cons < 5) || counter < N) {
+        L.step = (float)step;
+        int64_t acc = 0;
+        for (int64_t s = 0; s < min_stat; ++s) { acc += bw_step<KW>(L, xs, lane, cur, K) ? 1 : 0; }
+        double v = (acc > 0) ? (double)acc / ((double)acc + (double)(min_stat - acc)) : 0.;
+        for (int o = WAVE / 2; o > 0; o >>= 1) { v = v + __shfl_down(v, o); }
+        v = __shfl(v, 0);
+        rate = v / (double)WAVE;
+        if (fabs(rate - A.target) < 0.05) { ++cons; }
+        else { cons = 0; }
+        const double fact = fmin(2., fmax(0.5, rate / A.target));
+        step = step * fact;
+        if (step > (double)FLT_MAX) { step = (double)FLT_MAX; }
+        if (step < (double)FLT_MIN) { step = (double)FLT_MIN; }
+        ++counter;
+        if (N < 0 && counter >= -N) { break; }
+    }
+}
+
 __global__ __launch_bounds__(WAVE) void k_batch_calibrate(CalibArgs A)
 {
     extern __shared__ float cal_xs[];
-    const int i = blockIdx.x;
+    const int i = A.ilist[blockIdx.x];
     const int lane = threadIdx.x;
     double step = A.step0[i];
     double rate = 0.;
-    const int N = A.nfind;
-    if (N != 0) {
+    if (A.nfind != 0) {
+        const Keys K = make_keys(A.P.key0, A.P.key1);
         BW L;
         bw_load(L, A.P, i, 0x80000000u | (uint32_t)(i * WAVE + lane));
-        bw_draw(L, cal_xs, lane, A.P, i);
+        bw_draw(L, cal_xs, lane, A.P, K, i);
         bw_refresh(L, cal_xs, lane);
-        const int64_t min_stat = (int64_t)fmax(100., sqrt(40000. * L.d) / (double)WAVE);
-        int cons = 0, counter = 0;
-        while ((N < 0 && cons < 5) || counter < N) {
-            L.step = (float)step;
-            int64_t acc = 0;
-            for (int64_t s = 0; s < min_stat; ++s) { acc += bw_step(L, cal_xs, lane, A.P.key0, A.P.key1) ? 1 : 0; }
-            double v = (acc > 0) ? (double)acc / ((double)acc + (double)(min_stat - acc)) : 0.;
-            for (int o = WAVE / 2; o > 0; o >>= 1) { v = v + __shfl_down(v, o); }
-            v = __shfl(v, 0);
-            rate = v / (double)WAVE;
-            if (fabs(rate - A.target) < 0.05) { ++cons; }
-            else { cons = 0; }
-            const double fact = fmin(2., fmax(0.5, rate / A.target));
-            step = step * fact;
-            if (step > (double)FLT_MAX) { step = (double)FLT_MAX; }
-            if (step < (double)FLT_MIN) { step = (double)FLT_MIN; }
-            ++counter;
-            if (N < 0 && counter >= -N) { break; }
+        switch (L.K) {
+        case 1: calibrate_loop<1>(A, L, cal_xs, lane, K, step, rate); break;
+        case 2: calibrate_loop<2>(A, L, cal_xs, lane, K, step, rate); break;
+        case 3: calibrate_loop<3>(A, L, cal_xs, lane, K, step, rate); break;
+        default: calibrate_loop<4>(A, L, cal_xs, lane, K, step, rate); break;
         }
     }
     if (lane == 0) {
@@ -290,23 +378,55 @@ __global__ __launch_bounds__(WAVE) void k_batch_calibrate(CalibArgs A)
     }
 }
 
-// persistent sampling: each wave takes 64-walker items from the queue until it is empty
+// one item: ndecorr steps, then T steps with Kahan block sums of |x|^2 flushed as FP64 means
+template <int KW>
+__device__ void sample_item(const SampleArgs &A, BW &L, float *xs, int lane, const Keys &K, int64_t g, bool live, int64_t T,
+                            int64_t Tmax)
+{
+    const int64_t B = live ? T / A.nblocks : 1;
+    Pre cur;
+    bw_fetch<KW>(L, L.t, cur, K);
+    for (int64_t s = 0; s < A.ndecorr; ++s) { (void)bw_step<KW>(L, xs, lane, cur, K); }
+    float Sb = 0.f, Cb = 0.f;
+    int64_t jb = 0, b = 0;
+    int32_t nacc = 0;
+    for (int64_t s = 0; s < Tmax; ++s) {
+        const bool acc = bw_step<KW>(L, xs, lane, cur, K);
+        const bool on = s < T;
+        nacc += (on && acc) ? 1 : 0;
+        kahan_if(Sb, Cb, (float)L.q, on);
+        jb += on ? 1 : 0;
+        if (on && jb == B) {
+            A.bmeans[b * A.nwalkers + g] = ((double)Sb - (double)Cb) * (1. / (double)B);
+            Sb = 0.f;
+            Cb = 0.f;
+            jb = 0;
+            ++b;
+            bw_refresh(L, xs, lane);
+        }
+    }
+    if (live) { A.acc[g] = nacc; }
+}
+
+// persistent sampling: each wave takes 64-walker items from its launch's item list until the list
+// is exhausted (launches are split by dimension so that each sizes its LDS to its own items)
 __global__ __launch_bounds__(WAVE) void k_batch_sample(SampleArgs A)
 {
     extern __shared__ float smp_xs[];
     const int lane = threadIdx.x;
+    const Keys K = make_keys(A.P.key0, A.P.key1);
     for (;;) {
         unsigned int item = 0u;
         if (lane == 0) { item = atomicAdd(A.queue, 1u); }
         item = __shfl(item, 0);
         if ((int64_t)item >= A.nitems) { break; }
-        const int64_t g = (int64_t)item * WAVE + lane;
+        const int64_t g = (int64_t)A.items[item] * WAVE + lane;
         const bool live = g < A.nwalkers;
         const int i = A.wint[live ? g : A.nwalkers - 1];
         BW L;
         bw_load(L, A.P, i, (uint32_t)g);
         L.step = (float)A.step[i];
-        bw_draw(L, smp_xs, lane, A.P, i);
+        bw_draw(L, smp_xs, lane, A.P, K, i);
         bw_refresh(L, smp_xs, lane);
         const int64_t T = live ? A.wsteps[i] : 0;
         int64_t Tmax = T;
@@ -314,27 +434,12 @@ __global__ __launch_bounds__(WAVE) void k_batch_sample(SampleArgs A)
             const int64_t v = __shfl_xor(Tmax, o);
             Tmax = v > Tmax ? v : Tmax;
         }
-        const int64_t B = T / A.nblocks;
-        for (int64_t s = 0; s < A.ndecorr; ++s) { (void)bw_step(L, smp_xs, lane, A.P.key0, A.P.key1); }
-        float Sb = 0.f, Cb = 0.f;
-        int64_t jb = 0, b = 0;
-        int32_t nacc = 0;
-        for (int64_t s = 0; s < Tmax; ++s) {
-            const bool acc = bw_step(L, smp_xs, lane, A.P.key0, A.P.key1);
-            if (s < T) {
-                nacc += acc ? 1 : 0;
-                kahan(Sb, Cb, (float)L.q);
-                if (++jb == B) {
-                    A.bmeans[b * A.nwalkers + g] = ((double)Sb - (double)Cb) * (1. / (double)B);
-                    Sb = 0.f;
-                    Cb = 0.f;
-                    jb = 0;
-                    ++b;
-                    bw_refresh(L, smp_xs, lane);
-                }
-            }
+        switch (wave_max_int(L.K)) {
+        case 1: sample_item<1>(A, L, smp_xs, lane, K, g, live, T, Tmax); break;
+        case 2: sample_item<2>(A, L, smp_xs, lane, K, g, live, T, Tmax); break;
+        case 3: sample_item<3>(A, L, smp_xs, lane, K, g, live, T, Tmax); break;
+        default: sample_item<4>(A, L, smp_xs, lane, K, g, live, T, Tmax); break;
         }
-        if (live) { A.acc[g] = nacc; }
     }
 }
 
@@ -401,15 +506,15 @@ int batch_sample_waves_per_cu(int dmax)
     return n;
 }
 
-hipError_t launch_batch_zero(unsigned int *q, hipStream_t s)
+hipError_t launch_batch_zero(unsigned int *q, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(1), 0, s, q);
+    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(WAVE), 0, s, q, n);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_calibrate(const CalibArgs &a, int dmax, hipStream_t s)
+hipError_t launch_batch_calibrate(const CalibArgs &a, int nlist, int dmax, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_calibrate, dim3((unsigned)a.P.nint), dim3(WAVE), batch_lds_bytes(dmax), s, a);
+    hipLaunchKernelGGL(k_batch_calibrate, dim3((unsigned)nlist), dim3(WAVE), batch_lds_bytes(dmax), s, a);
     return hipGetLastError();
 }
 

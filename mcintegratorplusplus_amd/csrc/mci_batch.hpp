@@ -29,6 +29,7 @@ struct BatchParams
 struct CalibArgs
 {
     BatchParams P;
+    const int *ilist;         // integrals of this launch (one workgroup each)
     int nfind;
     double target;
     const double *step0;      // [nint] initial step
@@ -45,6 +46,7 @@ struct SampleArgs
     int64_t nwalkers;
     int64_t ndecorr;
     int nblocks;
+    const int *items;         // 64-walker items of this launch (item q = walkers 64q .. 64q+63)
     int64_t nitems;
     unsigned int *queue;      // work-queue head (zeroed by k_batch_zero before the launch)
     double *bmeans;           // [nblocks][nwalkers]
@@ -64,8 +66,8 @@ struct CombineArgs
 };
 
 size_t batch_lds_bytes(int dmax);
-hipError_t launch_batch_zero(unsigned int *q, hipStream_t s);
-hipError_t launch_batch_calibrate(const CalibArgs &a, int dmax, hipStream_t s);
+hipError_t launch_batch_zero(unsigned int *q, int n, hipStream_t s);
+hipError_t launch_batch_calibrate(const CalibArgs &a, int nlist, int dmax, hipStream_t s);
 hipError_t launch_batch_sample(const SampleArgs &a, int dmax, int nwaves, hipStream_t s);
 hipError_t launch_batch_combine(const CombineArgs &a, hipStream_t s);
 // resident 64-lane workgroups of k_batch_sample per CU at the given LDS size

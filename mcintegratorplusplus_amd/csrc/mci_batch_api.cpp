@@ -18,13 +18,24 @@ using mcih::fail;
 namespace
 {
 constexpr int MAXK = mcib::MAXK;
+// launches are split by dimension: a wave holds 64 d floats of LDS, so small-d items can keep
+// more waves resident per CU (the limits match 16, 8 and 5+ waves per CU of 160 KiB)
+constexpr int NCLASS = 3;
+constexpr int CLASS_DMAX[NCLASS] = {40, 80, MCI_BATCH_MAX_DIM};
+int dim_class(int d)
+{
+    int c = 0;
+    while (c < NCLASS - 1 && d > CLASS_DMAX[c]) { ++c; }
+    return c;
+}
 
-// cost model of one wave-step of k_batch_sample (instructions issued), from rocprof counts of the
-// kernel: a fixed part (Philox, index draw, proposal, test, Kahan add) and a part per component
-// (load, two fused ops, exp); the block-end refresh adds about 3 (K + 1) d per block
+// cost model of k_batch_sample in instructions issued per wave: per step a fixed part (Philox,
+// index draw, proposal, test, Kahan add) and a part per component (two fused ops, exp); the
+// block-end refresh adds about 3 (K + 1) d per block; a walker's exact start draw costs about
+// 60 d (polar method) plus one refresh
 struct CostModel
 {
-    double base = 110., per_comp = 36., refresh = 3.;
+    double base = 110., per_comp = 36., refresh = 3., init_per_dim = 60.;
     double cycles_per_instr_alone = 5.6; // one wave on its SIMD: dependent issue
     double cycles_per_instr_busy = 1.25; // several waves per SIMD: issue-bound
     double clock_hz = 2.4e9;
@@ -46,7 +57,7 @@ struct mci_batch
     std::vector<int64_t> nmc, coff, moff;
     std::vector<double> w, s, mu;
     uint64_t seed = 1;
-    int nblk = 16, rmin = 16, mode = MCI_BATCH_ERR_SPREAD, nfind = -50;
+    int nblk = 16, rmin = 64, mode = MCI_BATCH_ERR_SPREAD, nfind = -50;
     double target = 0.5;
     int64_t ndecorr = 0;
     bool user_plan = false, planned = false;
@@ -69,7 +80,9 @@ struct mci_batch
     DBuf<double> d_step0, d_step, d_rate, d_avg, d_err, d_accr, d_bm, d_wavg, d_werr;
     DBuf<int32_t> d_acc;
     DBuf<unsigned int> d_queue;
-    int nwaves = 0;
+    DBuf<int> d_items, d_ilist;
+    int item_off[NCLASS + 1] = {}, int_off[NCLASS + 1] = {};
+    int item_dmax[NCLASS] = {}, int_dmax[NCLASS] = {}, nwaves[NCLASS] = {};
 
     bool graph_on = true;
     hipGraph_t graph = nullptr;
@@ -187,14 +200,15 @@ double model_seconds(const mci_batch *b, const std::vector<int64_t> &T, const De
     double total = 0., longest = 0.;
     int64_t lanes = 0;
     int64_t steps = 0;
-    double instr = 0.;
+    double instr = 0., init = 0.;
     auto close_item = [&]() {
-        const double cyc = (double)steps * instr;
+        const double cyc = (double)steps * instr + init;
         total += cyc;
         longest = std::max(longest, cyc);
         lanes = 0;
         steps = 0;
         instr = 0.;
+        init = 0.;
     };
     for (int i : ord) {
         const int64_t R = (b->nmc[i] + T[i] - 1) / T[i];
@@ -204,6 +218,7 @@ double model_seconds(const mci_batch *b, const std::vector<int64_t> &T, const De
             const int64_t take = std::min<int64_t>(R - r, 64 - lanes);
             steps = std::max<int64_t>(steps, T[i] + b->ndecorr);
             instr = std::max(instr, ins);
+            init = std::max(init, c.init_per_dim * b->d[i] + c.refresh * (b->K[i] + 1) * b->d[i]);
             lanes += take;
             r += take;
             if (lanes == 64) { close_item(); }
@@ -252,11 +267,13 @@ mci_status ensure_device(mci_batch *b)
     return MCI_OK;
 }
 
+// setup copies are synchronous: the host vectors are temporaries
 template <class T>
 mci_status upload(DBuf<T> &dst, const std::vector<T> &src, hipStream_t s)
 {
     HIPCHK(dst.ensure(src.size()));
-    HIPCHK(hipMemcpyAsync(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     return MCI_OK;
 }
 
@@ -279,7 +296,21 @@ mci_status ensure_uploaded(mci_batch *b)
         HIPCHK(b->d_avg.ensure(b->nint));
         HIPCHK(b->d_err.ensure(b->nint));
         HIPCHK(b->d_accr.ensure(b->nint));
-        HIPCHK(b->d_queue.ensure(1));
+        HIPCHK(b->d_queue.ensure(NCLASS));
+        // integrals by dimension class for the calibration launches
+        std::vector<int> il;
+        for (int c = 0; c < NCLASS; ++c) {
+            b->int_off[c] = (int)il.size();
+            b->int_dmax[c] = 1;
+            for (int i = 0; i < b->nint; ++i) {
+                if (dim_class(b->d[i]) == c) {
+                    il.push_back(i);
+                    b->int_dmax[c] = std::max(b->int_dmax[c], b->d[i]);
+                }
+            }
+        }
+        b->int_off[NCLASS] = (int)il.size();
+        TRY(upload(b->d_ilist, il, b->stream));
         b->params_up = true;
     }
     if (!b->planned) {
@@ -299,9 +330,31 @@ mci_status ensure_uploaded(mci_batch *b)
         HIPCHK(b->d_wavg.ensure(b->nwalkers));
         HIPCHK(b->d_werr.ensure(b->nwalkers));
         HIPCHK(b->d_acc.ensure(b->nwalkers));
-        const DeviceShape sh = device_shape(b);
+        // items (64 consecutive walkers) by the dimension class of their largest d
         const int64_t nitems = (b->nwalkers + mcib::WAVE - 1) / mcib::WAVE;
-        b->nwaves = (int)std::max<int64_t>(1, std::min<int64_t>(nitems, (int64_t)sh.ncu * sh.waves_per_cu));
+        std::vector<int> idmax((size_t)nitems, 1);
+        for (int64_t g = 0; g < b->nwalkers; ++g) {
+            int &v = idmax[(size_t)(g / mcib::WAVE)];
+            v = std::max(v, b->d[wint[(size_t)g]]);
+        }
+        std::vector<int> items;
+        const DeviceShape sh = device_shape(b);
+        for (int c = 0; c < NCLASS; ++c) {
+            b->item_off[c] = (int)items.size();
+            b->item_dmax[c] = 1;
+            for (int64_t q = 0; q < nitems; ++q) {
+                if (dim_class(idmax[(size_t)q]) == c) {
+                    items.push_back((int)q);
+                    b->item_dmax[c] = std::max(b->item_dmax[c], idmax[(size_t)q]);
+                }
+            }
+            const int64_t n = (int64_t)items.size() - b->item_off[c];
+            int wpc = b->dev_ready ? mcib::batch_sample_waves_per_cu(b->item_dmax[c]) : 0;
+            if (wpc <= 0) { wpc = sh.waves_per_cu; }
+            b->nwaves[c] = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)sh.ncu * wpc));
+        }
+        b->item_off[NCLASS] = (int)items.size();
+        TRY(upload(b->d_items, items, b->stream));
         b->plan_up = true;
     }
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -330,10 +383,14 @@ mci_status enqueue(mci_batch *b, bool events)
 {
     const mcib::BatchParams P = params(b);
     if (events) { HIPCHK(hipEventRecord(b->ev[0], b->stream)); }
-    mcib::CalibArgs ca{P, b->nfind, b->target, b->d_step0.p, b->d_step.p, b->d_rate.p};
-    HIPCHK(mcib::launch_batch_calibrate(ca, b->dmax, b->stream));
+    for (int c = 0; c < NCLASS; ++c) {
+        const int n = b->int_off[c + 1] - b->int_off[c];
+        if (n == 0) { continue; }
+        mcib::CalibArgs ca{P, b->d_ilist.p + b->int_off[c], b->nfind, b->target, b->d_step0.p, b->d_step.p, b->d_rate.p};
+        HIPCHK(mcib::launch_batch_calibrate(ca, n, b->int_dmax[c], b->stream));
+    }
     if (events) { HIPCHK(hipEventRecord(b->ev[1], b->stream)); }
-    HIPCHK(mcib::launch_batch_zero(b->d_queue.p, b->stream));
+    HIPCHK(mcib::launch_batch_zero(b->d_queue.p, NCLASS, b->stream));
     mcib::SampleArgs sa;
     sa.P = P;
     sa.step = b->d_step.p;
@@ -342,11 +399,16 @@ mci_status enqueue(mci_batch *b, bool events)
     sa.nwalkers = b->nwalkers;
     sa.ndecorr = b->ndecorr;
     sa.nblocks = b->nblk;
-    sa.nitems = (b->nwalkers + mcib::WAVE - 1) / mcib::WAVE;
-    sa.queue = b->d_queue.p;
     sa.bmeans = b->d_bm.p;
     sa.acc = b->d_acc.p;
-    HIPCHK(mcib::launch_batch_sample(sa, b->dmax, b->nwaves, b->stream));
+    for (int c = 0; c < NCLASS; ++c) {
+        const int n = b->item_off[c + 1] - b->item_off[c];
+        if (n == 0) { continue; }
+        sa.items = b->d_items.p + b->item_off[c];
+        sa.nitems = n;
+        sa.queue = b->d_queue.p + c;
+        HIPCHK(mcib::launch_batch_sample(sa, b->item_dmax[c], b->nwaves[c], b->stream));
+    }
     if (events) { HIPCHK(hipEventRecord(b->ev[2], b->stream)); }
     mcid::EstimJob j;
     j.data = b->d_bm.p;

@@ -90,7 +90,7 @@ def test_planner_host_logic():
     R, T, off = b.plan()
     assert np.all(R * T >= nmc)
     assert np.all(T % 16 == 0) and np.all(((T // 16) & (T // 16 - 1)) == 0)
-    assert np.all((R >= 16) | (T == 16))
+    assert np.all((R >= 64) | (T == 16))
     # walker order: integrals sorted by (T, d, K) descending, as the oracle orders them
     R2, off2, W = orc_walker_order(ndim, np.array([len(x) for x in w]), nmc, T)
     assert np.array_equal(R, R2) and np.array_equal(off, off2) and W == b.nwalkers()
