@@ -265,6 +265,58 @@ mci_status mci_batch_get_block_means(mci_batch *b, double *data, int64_t n);
 /* [0] calibration, [1] sampling (persistent launch), [2] estimation+combine, [3] total (ms) */
 mci_status mci_batch_get_timings(mci_batch *b, double *ms, int n);
 
+/* ---- learned proposal ----
+ * A trial move whose proposal is learned (no single reference counterpart; it plugs into the
+ * reference's trial-move hook, whose return value multiplies the acceptance ratio,
+ * TrialMoveInterface.hpp:63-69 and src/MCIntegrator.cpp:343, as MultiStepMove.cpp:46 does for a
+ * non-symmetric move).  The network maps the walker's position x to a K-component diagonal
+ * Gaussian mixture q(x' | x); a learned move draws x' ~ q(. | x) and is accepted with
+ * min(1, p(x') q(x | x') / (p(x) q(x' | x))), so the target p stays exact whatever the network
+ * computes.  The network, d -> H -> H -> K (1 + 2 d) with SiLU, runs with BF16 weights and
+ * activations and FP32 accumulation on the matrix cores, forward and backward, and is trained
+ * with Adam on pairs of samples of the plain chain (same walker at two times, or two walkers)
+ * to maximise log q(x_b | x_a). */
+typedef struct mci_mlp mci_mlp;
+mci_status mci_mlp_create(int d, int hidden, int ncomp, uint64_t seed, mci_mlp **out);
+mci_status mci_mlp_destroy(mci_mlp *m);
+/* parameters, flat FP32: W1 [d][H], b1 [H], W2 [H][H], b2 [H], W3 [H][K(1+2d)], b3 [K(1+2d)] */
+mci_status mci_mlp_nparams(mci_mlp *m, int64_t *n);
+mci_status mci_mlp_get_params(mci_mlp *m, float *params, int64_t n);
+mci_status mci_mlp_set_params(mci_mlp *m, const float *params, int64_t n);
+/* raw outputs [n][K(1+2d)] for positions x [n][d]: K logits, K x d means / 4, K x d log-scale
+ * pre-activations (log-scale = 3 tanh(o / 3)); the input is x / 4 */
+mci_status mci_mlp_forward(mci_mlp *m, int64_t n, const float *x, float *out);
+/* mean of -log q(y_r | x_r) over n pairs and its gradient with respect to the FP32 parameters */
+mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *x, const float *y, double *loss, float *grad);
+
+/* the sampler: target p(x) = sum_k w_k N(x; mu_k, Sigma_k), d <= 32, K <= 8 (means [K][d],
+ * covariances [K][d][d]); nwalkers independent walkers start from exact draws of p.
+ * Observables: |x|^2 and 1[x_0 > 0]. */
+#define MCI_LP_PLAIN_FP64   0  /* random-walk Metropolis (single-coordinate uniform moves), FP64 */
+#define MCI_LP_PLAIN_FP32   1  /* the same chain in FP32 with Kahan block sums */
+#define MCI_LP_MIXED_FP32   2  /* each step is a learned move with probability pmix, else a random-walk move */
+typedef struct mci_lp mci_lp;
+mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *means, const double *covs, int64_t nwalkers,
+                         mci_lp **out);
+mci_status mci_lp_destroy(mci_lp *lp);
+mci_status mci_lp_set_seed(mci_lp *lp, uint64_t seed);
+mci_status mci_lp_set_network(mci_lp *lp, int hidden, int ncomp); /* default 128, 8 (resets the network) */
+/* exact start, step calibration (MCI::findMRT2Step rule, walkers as ranks), nsteps steps in
+ * nblocks blocks (a power of two <= 64), MJBlocker per walker.  avg/err [2]: mean over walkers
+ * and the standard error of the walker means. */
+mci_status mci_lp_run(mci_lp *lp, int mode, int64_t nsteps, int nblocks, double pmix, double *avg, double *err);
+/* training samples: nsnap snapshots of every walker of the plain FP32 chain, lag steps apart */
+mci_status mci_lp_collect(mci_lp *lp, int nsnap, int64_t lag);
+/* Adam on minibatches of training pairs; losses of the first and last minibatch */
+mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *loss_first, double *loss_last);
+/* last run: [0..1] avg, [2..3] standard error of walker means, [4..5] MPIMCI combine of the
+ * walkers' MJBlocker errors, [6] random-walk acceptance, [7] learned-move acceptance */
+mci_status mci_lp_get_results(mci_lp *lp, double *results, int n);
+/* [0] calibration, [1] sampling, [2] estimation, [3] training (ms, host clock around synchronised work) */
+mci_status mci_lp_get_timings(mci_lp *lp, double *ms, int n);
+mci_status mci_lp_get_walker_means(mci_lp *lp, double *means, int64_t n); /* [2][nwalkers] */
+mci_status mci_lp_get_network(mci_lp *lp, mci_mlp **m);                   /* owned by the sampler */
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,538 @@
+// mci_lp.hip — the learned-proposal sampler's kernels.
+//
+// Target: p(x) = sum_k w_k N(x; mu_k, Sigma_k) in d dimensions with full covariances, held as
+// precision matrices P_k and c_k = log w_k - log det(2 pi Sigma_k)/2, so that
+// log p(x) = LSE_k (c_k - q_k/2) with q_k = (x - mu_k)^T P_k (x - mu_k).
+//
+// Random-walk Metropolis (k_lp_rw): single-coordinate uniform moves (SRRDVecMove, veclen 1).  A
+// move x_j -> x_j + delta changes q_k by 2 delta (P_k (x - mu_k))_j + delta^2 (P_k)_jj, i.e. one row
+// of each P_k against x: 32 K fused multiply-adds at d = 32, all operands in LDS (the P_k rows
+// padded to d + 1 so that lanes at different j read different banks).  The test u <= p'/p is
+// evaluated as u S < exp(m' - m) S' with log p = m + log S (no logarithm).  Every block end
+// recomputes q_k from x.  The kernel runs in FP64 (the plain chain) or FP32.
+//
+// Learned move (FP32; k_lp_propose, the network forward, k_lp_accept): the network maps the
+// current x to a K-component diagonal Gaussian mixture q(. | x); propose x' ~ q(. | x), evaluate
+// the network at x' and accept with min(1, p(x') q(x | x') / (p(x) q(x' | x))), so the target stays
+// exact whatever the network computes.  The network's output at the current x is kept per
+// walker, so one forward pass (at x') is needed per learned move.
+//
+// Random words (Philox4x32-10, key = seed, counter (c0, walker, step low, step high)):
+//   c0 = 0            random-walk step: [0] coordinate (Lemire), [1] proposal, [2..3] u
+//   c0 = 0x80000000+k Lemire rejection words
+//   c0 = 0x10000000   learned move: [0] component, [1] u;  0x10000001 + n: polar-method normals
+//   c0 = 0x40000000+n exact start draw (step 0): call 0 the component, then normals
+#include "mci_lp.hpp"
+#include "mci_f32math.hpp"
+#include "mci_mlp.hpp"
+
+#include <float.h>
+
+namespace mcil
+{
+namespace
+{
+using mcif::blog;
+using mcif::sym24;
+
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t o[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = c2;
+    o[3] = c3;
+}
+
+__device__ __forceinline__ float rexp(float x) { return __expf(x); }
+__device__ __forceinline__ double rexp(double x) { return exp(x); }
+
+template <class R>
+__device__ __forceinline__ void kahan(R &s, R &c, R y)
+{
+    const R yy = y - c;
+    const R tt = s + yy;
+    c = (tt - s) - yy;
+    s = tt;
+}
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Marsaglia's polar method: the next standard normal of this lane's stream (c0base + call)
+struct Normals
+{
+    uint32_t o[4];
+    int avail;    // unused attempts (pairs) left in o
+    float spare;  // second normal of the last accepted pair
+    bool has_spare;
+    uint32_t call;
+};
+
+__device__ __forceinline__ float next_normal(Normals &N, uint32_t c0base, uint32_t id, uint32_t tl, uint32_t th, uint32_t k0, uint32_t k1)
+{
+    if (N.has_spare) {
+        N.has_spare = false;
+        return N.spare;
+    }
+    for (;;) {
+        if (N.avail == 0) {
+            philox(c0base + N.call, id, tl, th, k0, k1, N.o);
+            ++N.call;
+            N.avail = 2;
+        }
+        const int a = 2 - N.avail;
+        --N.avail;
+        const float v1 = sym24(N.o[2 * a]);
+        const float v2 = sym24(N.o[2 * a + 1]);
+        const float s = fmaf(v1, v1, v2 * v2);
+        if (s < 1.0f) {
+            const float f = sqrtf(-2.0f * blog(s) / s);
+            N.spare = v2 * f;
+            N.has_spare = true;
+            return v1 * f;
+        }
+    }
+}
+
+// log-density and state (q_k, m, S) of x (column w of xs with stride ldx) under the target
+template <class R>
+__device__ void target_eval(const Target<R> &T, const R *x, int64_t ldx, R q[KMAX], R &m, R &S)
+{
+    const int d = T.d;
+    m = -__builtin_huge_val();
+    R e[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const R *mu = T.mu + k * d;
+        const R *P = T.P + (int64_t)k * d * d;
+        R qk = 0;
+        for (int i = 0; i < d; ++i) {
+            const R di = x[i * ldx] - mu[i];
+            R row = 0;
+            for (int j = 0; j < d; ++j) { row = fma(P[i * d + j], x[j * ldx] - mu[j], row); }
+            qk = fma(di, row, qk);
+        }
+        q[k] = qk;
+        e[k] = T.c[k] - (R)0.5 * qk;
+        m = e[k] > m ? e[k] : m;
+    }
+    S = 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { S = S + rexp(e[k] - m); }
+}
+
+template <class R>
+__device__ __forceinline__ void accumulate(const State<R> &St, int64_t w, bool live, double sq, R x0, R &s0, R &c0, R &s1, R &c1, int &jb,
+                                           int &bi)
+{
+    kahan(s0, c0, (R)sq);
+    kahan(s1, c1, x0 > 0 ? (R)1 : (R)0);
+    if (++jb == St.B) {
+        if (live) {
+            St.bmeans[((int64_t)bi * 2 + 0) * St.W + w] = ((double)s0 - (double)c0) / (double)St.B;
+            St.bmeans[((int64_t)bi * 2 + 1) * St.W + w] = ((double)s1 - (double)c1) / (double)St.B;
+        }
+        s0 = c0 = s1 = c1 = 0;
+        jb = 0;
+        ++bi;
+    }
+}
+
+// exact start: component by cumulative weight, z ~ N(0, I) by the polar method, x = mu + L z
+template <class R>
+__global__ __launch_bounds__(WG) void k_lp_init(Target<R> T, State<R> St, uint32_t k0, uint32_t k1)
+{
+    extern __shared__ float zs[]; // [d][WG]
+    const int tid = threadIdx.x, d = T.d;
+    const int64_t w = (int64_t)blockIdx.x * WG + tid;
+    if (w >= St.W) { return; }
+    uint32_t o[4];
+    philox(0x40000000u, (uint32_t)w, 0u, 0u, k0, k1, o);
+    const float uc = (float)(o[0] >> 8) * 0x1p-24f;
+    int kc = 0;
+    for (int k = 0; k < T.K - 1; ++k) { kc += uc >= T.cw[k] ? 1 : 0; }
+    Normals N;
+    N.avail = 0;
+    N.has_spare = false;
+    N.call = 1;
+    for (int j = 0; j < d; ++j) { zs[j * WG + tid] = next_normal(N, 0x40000000u, (uint32_t)w, 0u, 0u, k0, k1); }
+    const float *Lk = T.L + (int64_t)kc * d * d;
+    double sq = 0.;
+    for (int i = 0; i < d; ++i) {
+        float xi = T.muf[kc * d + i];
+        for (int j = 0; j <= i; ++j) { xi = fmaf(Lk[i * d + j], zs[j * WG + tid], xi); }
+        St.x[(int64_t)i * St.W + w] = (R)xi;
+        sq = fma((double)xi, (double)xi, sq);
+    }
+    R q[KMAX], m, S;
+    target_eval(T, St.x + w, St.W, q, m, S);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * St.W + w] = q[k]; }
+    St.m[w] = m;
+    St.S[w] = S;
+    St.sq[w] = sq;
+    for (int o2 = 0; o2 < 2; ++o2) {
+        St.sb[o2 * St.W + w] = 0;
+        St.cb[o2 * St.W + w] = 0;
+    }
+    St.jb[w] = 0;
+    St.bi[w] = 0;
+    St.nacc[w] = 0;
+}
+
+template <class R>
+__device__ __forceinline__ void rw_draw(uint32_t o1, uint32_t o2, uint32_t o3, R step, R &delta, R &ua);
+template <>
+__device__ __forceinline__ void rw_draw<double>(uint32_t o1, uint32_t o2, uint32_t o3, double step, double &delta, double &ua)
+{
+    delta = step * (((double)o1 + 0.5) * 0x1p-31 - 1.0);
+    ua = ((double)(o2 >> 5) * 67108864.0 + (double)(o3 >> 6)) * 0x1p-53;
+}
+template <>
+__device__ __forceinline__ void rw_draw<float>(uint32_t o1, uint32_t o2, uint32_t, float step, float &delta, float &ua)
+{
+    delta = step * sym24(o1);
+    ua = (float)(o2 >> 8) * 0x1p-24f;
+}
+
+// nsteps random-walk steps t0 .. t0 + nsteps - 1 for 256 walkers per workgroup
+template <class R>
+__global__ __launch_bounds__(WG) void k_lp_rw(Target<R> T, State<R> St, R step, uint64_t t0, int64_t nsteps, uint32_t k0, uint32_t k1)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int d = T.d, dp = d + 1, tid = threadIdx.x;
+    R *Ps = reinterpret_cast<R *>(lds_raw); // [KMAX][d][d + 1]
+    R *vs = Ps + KMAX * d * dp;              // [KMAX][d]
+    R *xs = vs + KMAX * d;                   // [d][WG]
+    const int64_t w = (int64_t)blockIdx.x * WG + tid;
+    const bool live = w < St.W;
+    const int64_t wl = live ? w : St.W - 1;
+    for (int e = tid; e < KMAX * d * d; e += WG) {
+        const int k = e / (d * d), r = (e / d) % d, c = e % d;
+        Ps[(k * d + r) * dp + c] = T.P[e];
+    }
+    for (int e = tid; e < KMAX * d; e += WG) { vs[e] = T.v[e]; }
+    for (int i = 0; i < d; ++i) { xs[i * WG + tid] = St.x[(int64_t)i * St.W + wl]; }
+    __syncthreads();
+    R q[KMAX], c[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        q[k] = St.q[(int64_t)k * St.W + wl];
+        c[k] = T.c[k];
+    }
+    R m = St.m[wl], S = St.S[wl];
+    double sq = St.sq[wl];
+    R s0 = St.sb[wl], c0 = St.cb[wl], s1 = St.sb[St.W + wl], c1 = St.cb[St.W + wl];
+    int jb = St.jb[wl], bi = St.bi[wl];
+    int32_t nacc = 0;
+    const uint32_t thr = (uint32_t)(0u - (uint32_t)d) % (uint32_t)d;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const uint64_t t = t0 + (uint64_t)s;
+        const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
+        uint32_t o[4];
+        philox(0u, (uint32_t)wl, tl, th, k0, k1, o);
+        uint64_t prod = (uint64_t)o[0] * (uint32_t)d;
+        if (__builtin_expect(__any((uint32_t)prod < thr), 0)) {
+            uint32_t low = (uint32_t)prod, kx = 0;
+            while (low < thr) {
+                const uint32_t kk = 0x80000000u + (kx++);
+                uint32_t e[4];
+                philox(kk >> 2, (uint32_t)wl, tl, th, k0, k1, e);
+                prod = (uint64_t)e[kk & 3u] * (uint32_t)d;
+                low = (uint32_t)prod;
+            }
+        }
+        const int j = (int)(prod >> 32);
+        R delta, ua;
+        rw_draw<R>(o[1], o[2], o[3], step, delta, ua);
+        const R xo = xs[j * WG + tid];
+        const R xn = xo + delta;
+        R g[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { g[k] = -vs[k * d + j]; }
+        for (int i = 0; i < d; ++i) {
+            const R xi = xs[i * WG + tid];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) { g[k] = fma(Ps[(k * d + j) * dp + i], xi, g[k]); }
+        }
+        R qn[KMAX], en[KMAX];
+        R mn = -__builtin_huge_val();
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            const R pjj = Ps[(k * d + j) * dp + j];
+            qn[k] = q[k] + fma(delta + delta, g[k], delta * delta * pjj);
+            en[k] = c[k] - (R)0.5 * qn[k];
+            mn = en[k] > mn ? en[k] : mn;
+        }
+        R Sn = 0;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { Sn = Sn + rexp(en[k] - mn); }
+        const R E = rexp(mn - m);
+        const bool acc = ua * S < E * Sn;
+        if (acc) { xs[j * WG + tid] = xn; }
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { q[k] = acc ? qn[k] : q[k]; }
+        m = acc ? mn : m;
+        S = acc ? Sn : S;
+        sq = acc ? sq + fma((double)xn, (double)xn, -((double)xo * (double)xo)) : sq;
+        nacc += acc ? 1 : 0;
+        if (St.B > 0) {
+            const int bprev = bi;
+            accumulate(St, w, live, sq, xs[tid], s0, c0, s1, c1, jb, bi);
+            if (bi != bprev) { // block end: recompute q_k, m, S, |x|^2 from x
+                double q2 = 0.;
+                for (int i = 0; i < d; ++i) {
+                    const double v = (double)xs[i * WG + tid];
+                    q2 = fma(v, v, q2);
+                }
+                sq = q2;
+                m = -__builtin_huge_val();
+                R e2[KMAX];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    const R *mu = T.mu + k * d;
+                    R qk = 0;
+                    for (int i = 0; i < d; ++i) {
+                        R row = 0;
+                        for (int jj = 0; jj < d; ++jj) { row = fma(Ps[(k * d + i) * dp + jj], xs[jj * WG + tid] - mu[jj], row); }
+                        qk = fma(xs[i * WG + tid] - mu[i], row, qk);
+                    }
+                    q[k] = qk;
+                    e2[k] = c[k] - (R)0.5 * qk;
+                    m = e2[k] > m ? e2[k] : m;
+                }
+                S = 0;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) { S = S + rexp(e2[k] - m); }
+            }
+        }
+    }
+    if (!live) { return; }
+    for (int i = 0; i < d; ++i) { St.x[(int64_t)i * St.W + w] = xs[i * WG + tid]; }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * St.W + w] = q[k]; }
+    St.m[w] = m;
+    St.S[w] = S;
+    St.sq[w] = sq;
+    St.sb[w] = s0;
+    St.cb[w] = c0;
+    St.sb[St.W + w] = s1;
+    St.cb[St.W + w] = c1;
+    St.jb[w] = jb;
+    St.bi[w] = bi;
+    St.nacc[w] += nacc;
+}
+
+// log q(y | params) of the network's mixture, params in column w of out [rows][W]
+__device__ float mix_logq(const float *out, int64_t W, int64_t w, const float *y, int K, int d)
+{
+    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    float amax = -__builtin_huge_valf();
+    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, out[k * W + w]); }
+    float sa = 0.f;
+    for (int k = 0; k < K; ++k) { sa += __expf(out[k * W + w] - amax); }
+    const float lsa = amax + __logf(sa);
+    float comp[KMAX];
+    float cmax = -__builtin_huge_valf();
+    for (int k = 0; k < K; ++k) {
+        float c = out[k * W + w] - lsa;
+        for (int j = 0; j < d; ++j) {
+            const float mu = mcim::MU_SCALE * out[(int64_t)(K + k * d + j) * W + w];
+            const float ls = mcim::LS_MAX * tanhf(out[(int64_t)(K + K * d + k * d + j) * W + w] / mcim::LS_MAX);
+            const float z = (y[(int64_t)j * W + w] - mu) * __expf(-ls);
+            c += -ls - HALF_LOG_2PI - 0.5f * z * z;
+        }
+        comp[k] = c;
+        cmax = fmaxf(cmax, c);
+    }
+    float sc = 0.f;
+    for (int k = 0; k < K; ++k) { sc += __expf(comp[k] - cmax); }
+    return cmax + __logf(sc);
+}
+
+__global__ __launch_bounds__(WG) void k_lp_propose(Target<float> T, State<float> St, const float *out, int rows, int K, uint64_t t,
+                                                   uint32_t k0, uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f)
+{
+    const int64_t w = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (w >= St.W) { return; }
+    const int d = T.d;
+    const int64_t W = St.W;
+    const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
+    uint32_t o[4];
+    philox(0x10000000u, (uint32_t)w, tl, th, k0, k1, o);
+    float amax = -__builtin_huge_valf();
+    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, out[k * W + w]); }
+    float sa = 0.f;
+    for (int k = 0; k < K; ++k) { sa += __expf(out[k * W + w] - amax); }
+    const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
+    int kc = K - 1;
+    float cum = 0.f;
+    for (int k = 0; k < K - 1; ++k) {
+        cum += __expf(out[k * W + w] - amax);
+        if (uc < cum) { kc = k; break; }
+    }
+    Normals N;
+    N.avail = 0;
+    N.has_spare = false;
+    N.call = 1;
+    for (int j = 0; j < d; ++j) {
+        const float z = next_normal(N, 0x10000000u, (uint32_t)w, tl, th, k0, k1);
+        const float mu = mcim::MU_SCALE * out[(int64_t)(K + kc * d + j) * W + w];
+        const float ls = mcim::LS_MAX * tanhf(out[(int64_t)(K + K * d + kc * d + j) * W + w] / mcim::LS_MAX);
+        const float v = fmaf(__expf(ls), z, mu);
+        xp[(int64_t)j * W + w] = v;
+        xpb[(int64_t)j * W + w] = f2bf(v * in_scale);
+    }
+    logq_f[w] = mix_logq(out, W, w, xp, K, d);
+    (void)rows;
+}
+
+__global__ __launch_bounds__(WG) void k_lp_accept(Target<float> T, State<float> St, float *out, const float *outn, int rows, int K,
+                                                  uint64_t t, uint32_t k0, uint32_t k1, const float *xp, const float *logq_f,
+                                                  int32_t *nacc_ind)
+{
+    const int64_t w = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (w >= St.W) { return; }
+    const int d = T.d;
+    const int64_t W = St.W;
+    const float logq_r = mix_logq(outn, W, w, St.x, K, d);
+    float qn[KMAX], mn, Sn;
+    target_eval(T, xp + w, W, qn, mn, Sn);
+    const float logp_new = mn + __logf(Sn);
+    const float logp_old = St.m[w] + __logf(St.S[w]);
+    uint32_t o[4];
+    philox(0x10000000u, (uint32_t)w, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
+    const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
+    const bool acc = __logf(ua) < (logp_new - logp_old) + (logq_r - logq_f[w]);
+    if (acc) {
+        double sq = 0.;
+        for (int j = 0; j < d; ++j) {
+            const float v = xp[(int64_t)j * W + w];
+            St.x[(int64_t)j * W + w] = v;
+            sq = fma((double)v, (double)v, sq);
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * W + w] = qn[k]; }
+        St.m[w] = mn;
+        St.S[w] = Sn;
+        St.sq[w] = sq;
+        for (int r = 0; r < rows; ++r) { out[(int64_t)r * W + w] = outn[(int64_t)r * W + w]; }
+        nacc_ind[w] += 1;
+    }
+    if (St.B > 0) {
+        float s0 = St.sb[w], c0 = St.cb[w], s1 = St.sb[W + w], c1 = St.cb[W + w];
+        int jb = St.jb[w], bi = St.bi[w];
+        accumulate(St, w, true, St.sq[w], St.x[w], s0, c0, s1, c1, jb, bi);
+        St.sb[w] = s0;
+        St.cb[w] = c0;
+        St.sb[W + w] = s1;
+        St.cb[W + w] = c1;
+        St.jb[w] = jb;
+        St.bi[w] = bi;
+    }
+}
+
+template <class R>
+__global__ void k_lp_snapshot(State<R> St, int d, float *dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (int64_t)d * St.W) { dst[e] = (float)St.x[e]; }
+}
+
+__global__ void k_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float *ctx,
+                           float *tgt)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= B) { return; }
+    uint32_t o[4];
+    philox(0x30000000u, (uint32_t)p, (uint32_t)iter, (uint32_t)(iter >> 32), k0, k1, o);
+    const int s1 = (int)(o[0] % (uint32_t)nsnap);
+    const int64_t w1 = (int64_t)(o[1] % (uint64_t)W);
+    int s2 = (int)(o[2] % (uint32_t)nsnap);
+    const bool same = (o[3] & 1u) != 0;
+    const int64_t w2 = same ? w1 : (int64_t)((o[3] >> 1) % (uint64_t)W);
+    if (same && nsnap > 1 && s2 == s1) { s2 = (s1 + 1) % nsnap; }
+    const float *a = snap + (int64_t)s1 * d * W;
+    const float *b = snap + (int64_t)s2 * d * W;
+    for (int j = 0; j < d; ++j) {
+        ctx[(int64_t)j * B + p] = a[(int64_t)j * W + w1];
+        tgt[(int64_t)j * B + p] = b[(int64_t)j * W + w2];
+    }
+}
+} // namespace
+
+size_t rw_lds_bytes(int d, int esize) { return (size_t)esize * ((size_t)KMAX * d * (d + 1) + (size_t)KMAX * d + (size_t)d * WG); }
+
+template <class R>
+hipError_t launch_lp_init(const Target<R> &T, const State<R> &S, uint32_t k0, uint32_t k1, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lp_init<R>, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), sizeof(float) * (size_t)T.d * WG, s, T, S, k0, k1);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_lp_rw(const Target<R> &T, const State<R> &S, R step, uint64_t t0, int64_t nsteps, uint32_t k0, uint32_t k1, hipStream_t s)
+{
+    const size_t lds = rw_lds_bytes(T.d, sizeof(R));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_rw<R>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL(k_lp_rw<R>, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), rw_lds_bytes(T.d, sizeof(R)), s, T, S, step, t0, nsteps,
+                       k0, k1);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_propose(const Target<float> &T, const State<float> &S, const float *out, int rows, int K, uint64_t t, uint32_t k0,
+                             uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lp_propose, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), 0, s, T, S, out, rows, K, t, k0, k1, xp, xpb, in_scale,
+                       logq_f);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_accept(const Target<float> &T, const State<float> &S, float *out, const float *outn, int rows, int K, uint64_t t,
+                            uint32_t k0, uint32_t k1, const float *xp, const float *logq_f, int32_t *nacc_ind, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lp_accept, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), 0, s, T, S, out, outn, rows, K, t, k0, k1, xp, logq_f,
+                       nacc_ind);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t s)
+{
+    const int64_t n = (int64_t)d * S.W;
+    hipLaunchKernelGGL(k_lp_snapshot<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, d, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float *ctx,
+                           float *tgt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lp_pairs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, snap, nsnap, d, W, B, iter, k0, k1, ctx, tgt);
+    return hipGetLastError();
+}
+
+template hipError_t launch_lp_init<double>(const Target<double> &, const State<double> &, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_lp_init<float>(const Target<float> &, const State<float> &, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_lp_rw<double>(const Target<double> &, const State<double> &, double, uint64_t, int64_t, uint32_t, uint32_t,
+                                         hipStream_t);
+template hipError_t launch_lp_rw<float>(const Target<float> &, const State<float> &, float, uint64_t, int64_t, uint32_t, uint32_t,
+                                        hipStream_t);
+template hipError_t launch_lp_snapshot<double>(const State<double> &, int, float *, hipStream_t);
+template hipError_t launch_lp_snapshot<float>(const State<float> &, int, float *, hipStream_t);
+} // namespace mcil

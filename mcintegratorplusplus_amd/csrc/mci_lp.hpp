@@ -1,0 +1,64 @@
+// mci_lp.hpp — kernels of the learned-proposal sampler: a d-dimensional target that is a
+// mixture of K full-covariance Gaussians, random-walk Metropolis (single-coordinate uniform
+// moves, FP64 or FP32 chain) and the MH-corrected move drawn from the network's Gaussian
+// mixture q(x' | x) (FP32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mcil
+{
+constexpr int WG = 256;   // walkers per workgroup
+constexpr int KMAX = 8;   // target components (fewer are padded with inert ones)
+
+template <class R>
+struct Target
+{
+    int d;
+    const R *mu;        // [KMAX][d]
+    const R *P;         // [KMAX][d][d] precision matrices
+    const R *v;         // [KMAX][d] P_k mu_k
+    const R *c;         // [KMAX] log w_k - log det(2 pi Sigma_k)/2 (-inf for padding)
+    const float *L;     // [KMAX][d][d] Cholesky factor of Sigma_k (exact start)
+    const float *muf;   // [KMAX][d]
+    const float *cw;    // [KMAX] cumulative weights
+    int K;              // real components
+};
+
+// walker state (HBM, walker index fastest)
+template <class R>
+struct State
+{
+    int64_t W;
+    R *x;               // [d][W]
+    R *q;               // [KMAX][W] (x - mu_k)^T P_k (x - mu_k)
+    R *m, *S;           // [W] log-sum-exp shift and sum: log p(x) = m + log S
+    double *sq;         // [W] |x|^2
+    R *sb, *cb;         // [2][W] block sums and Kahan compensations of |x|^2 and 1[x_0 > 0]
+    int *jb, *bi;       // [W] steps in the current block, block index
+    double *bmeans;     // [nblocks][2][W]
+    int B;              // block length (0: no accumulation)
+    int32_t *nacc;      // [W] accepted moves of the current phase
+};
+
+size_t rw_lds_bytes(int d, int esize);
+template <class R>
+hipError_t launch_lp_init(const Target<R> &T, const State<R> &S, uint32_t k0, uint32_t k1, hipStream_t s);
+template <class R>
+hipError_t launch_lp_rw(const Target<R> &T, const State<R> &S, R step, uint64_t t0, int64_t nsteps, uint32_t k0, uint32_t k1,
+                        hipStream_t s);
+// learned move, FP32 chain.  out / outn: network outputs [rows][W] at x and at the proposal.
+hipError_t launch_lp_propose(const Target<float> &T, const State<float> &S, const float *out, int rows, int K, uint64_t t,
+                             uint32_t k0, uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f, hipStream_t s);
+hipError_t launch_lp_accept(const Target<float> &T, const State<float> &S, float *out, const float *outn, int rows, int K,
+                            uint64_t t, uint32_t k0, uint32_t k1, const float *xp, const float *logq_f, int32_t *nacc_ind,
+                            hipStream_t s);
+// x [d][W] -> dst [d][W] FP32 (snapshots)
+template <class R>
+hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t s);
+// training pairs: column p of ctx/tgt [d][B] from snapshots [nsnap][d][W]
+hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1,
+                           float *ctx, float *tgt, hipStream_t s);
+} // namespace mcil

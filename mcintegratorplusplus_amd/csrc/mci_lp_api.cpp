@@ -1,0 +1,876 @@
+// mci_lp_api.cpp — C-ABI of the learned-proposal path (include/mci_amd.h, mci_mlp_* and
+// mci_lp_*): the network (parameters, BF16 MFMA forward/backward, Adam) and the sampler that
+// runs the plain random-walk chain (FP64 or FP32), collects training pairs from it, trains the
+// network on the GPU and runs the chain that mixes random-walk and learned moves.
+#include "mci_host.hpp"
+#include "mci_kernels.hpp"
+#include "mci_lp.hpp"
+#include "mci_mlp.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using mcih::DBuf;
+using mcih::det_sum;
+using mcih::fail;
+using mcim::bf16_t;
+
+namespace
+{
+uint64_t splitmix(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+void philox_host(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t o[4])
+{
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = c2;
+    o[3] = c3;
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+} // namespace
+
+// ---- the network ----
+struct mci_mlp
+{
+    int d = 0, H = 0, K = 0, O = 0, Op = 0;
+    int64_t np = 0, npad = 0; // flat (unpadded) and padded parameter counts
+    int64_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0;
+    DBuf<float> p, m, v, g;
+    DBuf<bf16_t> pb;
+    int64_t cap = 0;
+    DBuf<bf16_t> Xb, H1b, H2b, dOb, dZ2b, dZ1b;
+    DBuf<float> X, Y, Z1, Z2, OUT, dO, dH2, dH1, loss;
+    int adam_t = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    ~mci_mlp()
+    {
+        if (own_stream && stream) { (void)hipStreamDestroy(stream); }
+    }
+};
+
+namespace
+{
+#define CHECKM(x)                                                                                        \
+    do {                                                                                                 \
+        if (!(x)) { return fail(MCI_E_INVALID_ARGUMENT, "null network"); }                               \
+    } while (0)
+
+// flat (unpadded) <-> padded parameter layouts: only W3 and b3 differ (O -> Op columns)
+void flat_to_padded(const mci_mlp *n, const float *f, std::vector<float> &pad)
+{
+    pad.assign((size_t)n->npad, 0.f);
+    const int d = n->d, H = n->H, O = n->O, Op = n->Op;
+    std::copy(f, f + (int64_t)d * H + H + (int64_t)H * H + H, pad.begin());
+    const float *w3 = f + (int64_t)d * H + H + (int64_t)H * H + H;
+    for (int i = 0; i < H; ++i) { std::copy(w3 + (int64_t)i * O, w3 + (int64_t)i * O + O, pad.begin() + n->oW3 + (int64_t)i * Op); }
+    std::copy(w3 + (int64_t)H * O, w3 + (int64_t)H * O + O, pad.begin() + n->ob3);
+}
+
+void padded_to_flat(const mci_mlp *n, const std::vector<float> &pad, float *f)
+{
+    const int d = n->d, H = n->H, O = n->O, Op = n->Op;
+    std::copy(pad.begin(), pad.begin() + (int64_t)d * H + H + (int64_t)H * H + H, f);
+    float *w3 = f + (int64_t)d * H + H + (int64_t)H * H + H;
+    for (int i = 0; i < H; ++i) { std::copy(pad.begin() + n->oW3 + (int64_t)i * Op, pad.begin() + n->oW3 + (int64_t)i * Op + O, w3 + (int64_t)i * O); }
+    std::copy(pad.begin() + n->ob3, pad.begin() + n->ob3 + O, w3 + (int64_t)H * O);
+}
+
+mci_status mlp_upload_params(mci_mlp *n, const std::vector<float> &pad)
+{
+    HIPCHK(hipMemcpy(n->p.p, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<bf16_t> b(pad.size());
+    for (size_t i = 0; i < pad.size(); ++i) {
+        uint32_t u;
+        std::memcpy(&u, &pad[i], 4);
+        b[i] = (bf16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+    HIPCHK(hipMemcpy(n->pb.p, b.data(), b.size() * sizeof(bf16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(n->m.p, 0, pad.size() * sizeof(float)));
+    HIPCHK(hipMemset(n->v.p, 0, pad.size() * sizeof(float)));
+    n->adam_t = 0;
+    return MCI_OK;
+}
+
+mci_status mlp_reserve(mci_mlp *n, int64_t cap)
+{
+    if (cap <= n->cap) { return MCI_OK; }
+    const int d = n->d, H = n->H, Op = n->Op;
+    HIPCHK(n->X.ensure((size_t)d * cap));
+    HIPCHK(n->Y.ensure((size_t)d * cap));
+    HIPCHK(n->Xb.ensure((size_t)d * cap));
+    HIPCHK(n->Z1.ensure((size_t)H * cap));
+    HIPCHK(n->H1b.ensure((size_t)H * cap));
+    HIPCHK(n->Z2.ensure((size_t)H * cap));
+    HIPCHK(n->H2b.ensure((size_t)H * cap));
+    HIPCHK(n->OUT.ensure((size_t)Op * cap));
+    HIPCHK(n->dO.ensure((size_t)Op * cap));
+    HIPCHK(n->dOb.ensure((size_t)Op * cap));
+    HIPCHK(n->dH2.ensure((size_t)H * cap));
+    HIPCHK(n->dZ2b.ensure((size_t)H * cap));
+    HIPCHK(n->dH1.ensure((size_t)H * cap));
+    HIPCHK(n->dZ1b.ensure((size_t)H * cap));
+    HIPCHK(n->loss.ensure((size_t)cap));
+    n->cap = cap;
+    return MCI_OK;
+}
+
+// forward of n samples from BF16 inputs xb [d][n] into out [Op][n] (feature-major)
+mci_status mlp_forward(mci_mlp *n, int64_t ns, const bf16_t *xb, float *out)
+{
+    const int d = n->d, H = n->H, Op = n->Op;
+    const bf16_t *W = n->pb.p;
+    const float *P = n->p.p;
+    hipStream_t s = n->stream;
+    HIPCHK(mcim::gemm_bf16(true, false, mcim::EPI_BIAS_SILU, H, (int)ns, d, W + n->oW1, xb, P + n->ob1, n->Z1.p, (int)ns, n->H1b.p,
+                           (int)ns, s));
+    HIPCHK(mcim::gemm_bf16(true, false, mcim::EPI_BIAS_SILU, H, (int)ns, H, W + n->oW2, n->H1b.p, P + n->ob2, n->Z2.p, (int)ns,
+                           n->H2b.p, (int)ns, s));
+    HIPCHK(mcim::gemm_bf16(true, false, mcim::EPI_BIAS, Op, (int)ns, H, W + n->oW3, n->H2b.p, P + n->ob3, out, (int)ns, nullptr, 0, s));
+    return MCI_OK;
+}
+
+// one forward + backward on (X, Y) [d][ns] already on the device; gradient of the mean loss in n->g
+mci_status mlp_fwd_bwd(mci_mlp *n, int64_t ns)
+{
+    const int d = n->d, H = n->H, K = n->K, Op = n->Op;
+    hipStream_t s = n->stream;
+    const int N = (int)ns;
+    HIPCHK(mcim::launch_to_bf16_scaled(n->X.p, n->Xb.p, (int64_t)d * ns, mcim::IN_SCALE, s));
+    TRY(mlp_forward(n, ns, n->Xb.p, n->OUT.p));
+    HIPCHK(mcim::launch_mixture_head(n->OUT.p, N, n->Y.p, N, d, K, Op, n->loss.p, n->dO.p, n->dOb.p, 1.0f / (float)ns, s));
+    float *g = n->g.p;
+    const bf16_t *W = n->pb.p;
+    HIPCHK(mcim::launch_rowsum(n->dO.p, Op, N, N, g + n->ob3, s));
+    HIPCHK(mcim::gemm_bf16(false, true, mcim::EPI_NONE, H, Op, N, n->H2b.p, n->dOb.p, nullptr, g + n->oW3, Op, nullptr, 0, s));
+    HIPCHK(mcim::gemm_bf16(false, false, mcim::EPI_NONE, H, N, Op, W + n->oW3, n->dOb.p, nullptr, n->dH2.p, N, nullptr, 0, s));
+    HIPCHK(mcim::launch_silu_back(n->dH2.p, n->Z2.p, n->dZ2b.p, (int64_t)H * ns, s));
+    HIPCHK(mcim::launch_rowsum(n->dH2.p, H, N, N, g + n->ob2, s));
+    HIPCHK(mcim::gemm_bf16(false, true, mcim::EPI_NONE, H, H, N, n->H1b.p, n->dZ2b.p, nullptr, g + n->oW2, H, nullptr, 0, s));
+    HIPCHK(mcim::gemm_bf16(false, false, mcim::EPI_NONE, H, N, H, W + n->oW2, n->dZ2b.p, nullptr, n->dH1.p, N, nullptr, 0, s));
+    HIPCHK(mcim::launch_silu_back(n->dH1.p, n->Z1.p, n->dZ1b.p, (int64_t)H * ns, s));
+    HIPCHK(mcim::launch_rowsum(n->dH1.p, H, N, N, g + n->ob1, s));
+    HIPCHK(mcim::gemm_bf16(false, true, mcim::EPI_NONE, d, H, N, n->Xb.p, n->dZ1b.p, nullptr, g + n->oW1, H, nullptr, 0, s));
+    return MCI_OK;
+}
+
+mci_status mlp_adam(mci_mlp *n, float lr)
+{
+    ++n->adam_t;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = 1.0f - std::pow(b1, (float)n->adam_t), bc2 = 1.0f - std::pow(b2, (float)n->adam_t);
+    HIPCHK(mcim::launch_adam(n->p.p, n->m.p, n->v.p, n->g.p, n->pb.p, n->npad, lr, b1, b2, 1e-8f, bc1, bc2, n->stream));
+    return MCI_OK;
+}
+
+mci_status mlp_make(int d, int H, int K, uint64_t seed, hipStream_t stream, mci_mlp **out)
+{
+    if (d < 1 || d > 256 || H < 16 || H > 1024 || K < 1 || K > 8) { return fail(MCI_E_INVALID_ARGUMENT, "network sizes out of range"); }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { return fail(MCI_E_DEVICE, "no HIP device available"); }
+    mci_mlp *n = new mci_mlp();
+    n->d = d;
+    n->H = H;
+    n->K = K;
+    n->O = K * (1 + 2 * d);
+    n->Op = round_up(n->O, 64);
+    n->oW1 = 0;
+    n->ob1 = n->oW1 + (int64_t)d * H;
+    n->oW2 = n->ob1 + H;
+    n->ob2 = n->oW2 + (int64_t)H * H;
+    n->oW3 = n->ob2 + H;
+    n->ob3 = n->oW3 + (int64_t)H * n->Op;
+    n->npad = n->ob3 + n->Op;
+    n->np = (int64_t)d * H + H + (int64_t)H * H + H + (int64_t)H * n->O + n->O;
+    if (stream) { n->stream = stream; }
+    else {
+        if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete n;
+            return fail(MCI_E_DEVICE, "stream creation failed");
+        }
+        n->own_stream = true;
+    }
+    if (n->p.ensure(n->npad) != hipSuccess || n->m.ensure(n->npad) != hipSuccess || n->v.ensure(n->npad) != hipSuccess ||
+        n->g.ensure(n->npad) != hipSuccess || n->pb.ensure(n->npad) != hipSuccess) {
+        delete n;
+        return fail(MCI_E_DEVICE, "allocation failed");
+    }
+    // Glorot-uniform weights (output layer scaled by 0.1), zero biases
+    std::vector<float> flat((size_t)n->np, 0.f);
+    uint64_t st = seed ^ 0x5DEECE66DULL;
+    auto fill = [&](int64_t off, int fin, int fout, float gain) {
+        const double a = gain * std::sqrt(6.0 / (fin + fout));
+        for (int64_t i = 0; i < (int64_t)fin * fout; ++i) {
+            const double u = (double)(splitmix(st) >> 11) * 0x1p-53;
+            flat[(size_t)(off + i)] = (float)((2.0 * u - 1.0) * a);
+        }
+    };
+    fill(0, d, H, 1.f);
+    fill((int64_t)d * H + H, H, H, 1.f);
+    fill((int64_t)d * H + H + (int64_t)H * H + H, H, n->O, 0.1f);
+    std::vector<float> pad;
+    flat_to_padded(n, flat.data(), pad);
+    const mci_status s = mlp_upload_params(n, pad);
+    if (s != MCI_OK) {
+        delete n;
+        return s;
+    }
+    *out = n;
+    return MCI_OK;
+}
+
+// host [n][d] -> device [d][n]
+mci_status upload_transposed(float *dst, const float *src, int64_t n, int d)
+{
+    std::vector<float> t((size_t)n * d);
+    for (int64_t r = 0; r < n; ++r) {
+        for (int j = 0; j < d; ++j) { t[(size_t)j * n + r] = src[(size_t)r * d + j]; }
+    }
+    HIPCHK(hipMemcpy(dst, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    return MCI_OK;
+}
+} // namespace
+
+// ---- the sampler ----
+struct mci_lp
+{
+    int d = 0, K = 0;
+    int64_t W = 0;
+    uint64_t seed = 1;
+    int H = 128, Kq = 8;
+    std::vector<double> mu, P, v, c, L; // padded to KMAX components
+    std::vector<double> cw;
+    bool dev = false;
+    hipStream_t stream = nullptr;
+    // target on the device
+    DBuf<double> mu64, P64, v64, c64;
+    DBuf<float> mu32, P32, v32, c32, Lf, muf, cwf;
+    // chain state (one precision at a time)
+    DBuf<double> x64, q64, m64, S64, sb64, cb64;
+    DBuf<float> x32, q32, m32, S32, sb32, cb32;
+    DBuf<double> sq, bmeans, wavg, werr;
+    DBuf<int> jb, bi;
+    DBuf<int32_t> nacc, nacc_ind;
+    // learned move
+    mci_mlp *net = nullptr;
+    DBuf<float> out, outn, xp, logq_f, snap;
+    DBuf<bf16_t> xpb;
+    int nsnap = 0;
+    uint64_t t_next = 0;
+    double step64 = 0., step32 = 0.;
+    double results[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double timings[4] = {0, 0, 0, 0};
+    double loss_first = 0., loss_last = 0.;
+    std::vector<double> h_wavg;
+    ~mci_lp()
+    {
+        delete net;
+        if (stream) { (void)hipStreamDestroy(stream); }
+    }
+};
+
+namespace
+{
+#define CHECKL(x)                                                                                        \
+    do {                                                                                                 \
+        if (!(x)) { return fail(MCI_E_INVALID_ARGUMENT, "null sampler"); }                               \
+    } while (0)
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+template <class T>
+mci_status up(DBuf<T> &dst, const std::vector<T> &src)
+{
+    HIPCHK(dst.ensure(src.size()));
+    HIPCHK(hipMemcpy(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MCI_OK;
+}
+
+template <class T>
+std::vector<float> to_f(const std::vector<T> &a)
+{
+    return std::vector<float>(a.begin(), a.end());
+}
+
+mci_status lp_device(mci_lp *lp)
+{
+    if (lp->dev) { return MCI_OK; }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) { return fail(MCI_E_DEVICE, "no HIP device available"); }
+    HIPCHK(hipStreamCreateWithFlags(&lp->stream, hipStreamNonBlocking));
+    TRY(up(lp->mu64, lp->mu));
+    TRY(up(lp->P64, lp->P));
+    TRY(up(lp->v64, lp->v));
+    TRY(up(lp->c64, lp->c));
+    std::vector<float> cf(lp->c.begin(), lp->c.end());
+    TRY(up(lp->mu32, to_f(lp->mu)));
+    TRY(up(lp->P32, to_f(lp->P)));
+    TRY(up(lp->v32, to_f(lp->v)));
+    TRY(up(lp->c32, cf));
+    TRY(up(lp->Lf, to_f(lp->L)));
+    TRY(up(lp->muf, to_f(lp->mu)));
+    TRY(up(lp->cwf, to_f(lp->cw)));
+    const size_t W = (size_t)lp->W;
+    HIPCHK(lp->sq.ensure(W));
+    HIPCHK(lp->jb.ensure(W));
+    HIPCHK(lp->bi.ensure(W));
+    HIPCHK(lp->nacc.ensure(W));
+    HIPCHK(lp->nacc_ind.ensure(W));
+    lp->dev = true;
+    return MCI_OK;
+}
+
+template <class R>
+mcil::Target<R> target_of(mci_lp *lp);
+template <>
+mcil::Target<double> target_of<double>(mci_lp *lp)
+{
+    return {lp->d, lp->mu64.p, lp->P64.p, lp->v64.p, lp->c64.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
+}
+template <>
+mcil::Target<float> target_of<float>(mci_lp *lp)
+{
+    return {lp->d, lp->mu32.p, lp->P32.p, lp->v32.p, lp->c32.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
+}
+
+template <class R>
+mci_status state_of(mci_lp *lp, int B, int nblocks, mcil::State<R> &S);
+template <>
+mci_status state_of<double>(mci_lp *lp, int B, int nblocks, mcil::State<double> &S)
+{
+    const size_t W = (size_t)lp->W;
+    HIPCHK(lp->x64.ensure(W * lp->d));
+    HIPCHK(lp->q64.ensure(W * mcil::KMAX));
+    HIPCHK(lp->m64.ensure(W));
+    HIPCHK(lp->S64.ensure(W));
+    HIPCHK(lp->sb64.ensure(2 * W));
+    HIPCHK(lp->cb64.ensure(2 * W));
+    HIPCHK(lp->bmeans.ensure((size_t)std::max(nblocks, 1) * 2 * W));
+    S = {lp->W, lp->x64.p, lp->q64.p, lp->m64.p, lp->S64.p, lp->sq.p, lp->sb64.p, lp->cb64.p, lp->jb.p, lp->bi.p, lp->bmeans.p, B, lp->nacc.p};
+    return MCI_OK;
+}
+template <>
+mci_status state_of<float>(mci_lp *lp, int B, int nblocks, mcil::State<float> &S)
+{
+    const size_t W = (size_t)lp->W;
+    HIPCHK(lp->x32.ensure(W * lp->d));
+    HIPCHK(lp->q32.ensure(W * mcil::KMAX));
+    HIPCHK(lp->m32.ensure(W));
+    HIPCHK(lp->S32.ensure(W));
+    HIPCHK(lp->sb32.ensure(2 * W));
+    HIPCHK(lp->cb32.ensure(2 * W));
+    HIPCHK(lp->bmeans.ensure((size_t)std::max(nblocks, 1) * 2 * W));
+    S = {lp->W, lp->x32.p, lp->q32.p, lp->m32.p, lp->S32.p, lp->sq.p, lp->sb32.p, lp->cb32.p, lp->jb.p, lp->bi.p, lp->bmeans.p, B, lp->nacc.p};
+    return MCI_OK;
+}
+
+// MCI::findMRT2Step (src/MCIntegrator.cpp:87-168) with all walkers as ranks: MIN_STAT 100 steps
+// per iteration (max(100, sqrt(40000 d)/W)), rate averaged over walkers, MIN_CONS 5, TOL 0.05,
+// factor in [0.5, 2]
+template <class R>
+mci_status calibrate(mci_lp *lp, mcil::State<R> S, double &step)
+{
+    const mcil::Target<R> T = target_of<R>(lp);
+    S.B = 0;
+    const int64_t min_stat = (int64_t)std::fmax(100., std::sqrt(40000. * lp->d) / (double)lp->W);
+    std::vector<int32_t> acc((size_t)lp->W);
+    std::vector<double> rates((size_t)lp->W);
+    int cons = 0, counter = 0;
+    const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
+    while (cons < 5) {
+        HIPCHK(hipMemsetAsync(lp->nacc.p, 0, sizeof(int32_t) * lp->W, lp->stream));
+        HIPCHK(mcil::launch_lp_rw<R>(T, S, (R)step, lp->t_next, min_stat, k0, k1, lp->stream));
+        lp->t_next += (uint64_t)min_stat;
+        HIPCHK(hipMemcpyAsync(acc.data(), lp->nacc.p, sizeof(int32_t) * lp->W, hipMemcpyDeviceToHost, lp->stream));
+        HIPCHK(hipStreamSynchronize(lp->stream));
+        for (int64_t w = 0; w < lp->W; ++w) {
+            rates[(size_t)w] = acc[(size_t)w] > 0 ? (double)acc[(size_t)w] / (double)min_stat : 0.;
+        }
+        const double rate = det_sum(rates.data(), lp->W) / (double)lp->W;
+        if (std::fabs(rate - 0.5) < 0.05) { ++cons; }
+        else { cons = 0; }
+        step = step * std::fmin(2., std::fmax(0.5, rate / 0.5));
+        step = std::fmin(std::fmax(step, (double)FLT_MIN), (double)FLT_MAX);
+        if (++counter >= 50) { break; }
+    }
+    return MCI_OK;
+}
+
+mci_status net_ready(mci_lp *lp)
+{
+    if (!lp->net) { TRY(mlp_make(lp->d, lp->H, lp->Kq, lp->seed, lp->stream, &lp->net)); }
+    return MCI_OK;
+}
+
+// per-walker MJBlocker over the block means, then avg, the standard error of the walker means
+// and the MPIMCI combine (src/MPIMCI.cpp:85-92), both observables
+mci_status estimate(mci_lp *lp, int nblocks)
+{
+    const size_t W = (size_t)lp->W;
+    HIPCHK(lp->wavg.ensure(2 * W));
+    HIPCHK(lp->werr.ensure(2 * W));
+    mcid::EstimJob j;
+    j.data = lp->bmeans.p;
+    j.scratch = nullptr;
+    j.n = nblocks;
+    j.W = lp->W;
+    j.nobs = 2;
+    j.off = 0;
+    j.estim = MCI_ESTIM_MJBLOCKER;
+    j.scale = 1.0;
+    j.active = nullptr;
+    j.avg = lp->wavg.p;
+    j.err = lp->werr.p;
+    HIPCHK(mcid::launch_estimate(j, lp->stream));
+    std::vector<double> a(2 * W), e(2 * W), tmp(W);
+    HIPCHK(hipMemcpyAsync(a.data(), lp->wavg.p, sizeof(double) * 2 * W, hipMemcpyDeviceToHost, lp->stream));
+    HIPCHK(hipMemcpyAsync(e.data(), lp->werr.p, sizeof(double) * 2 * W, hipMemcpyDeviceToHost, lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    for (int o = 0; o < 2; ++o) {
+        const double *ao = a.data() + o * W, *eo = e.data() + o * W;
+        const double avg = det_sum(ao, (int64_t)W) / (double)W;
+        for (size_t w = 0; w < W; ++w) { tmp[w] = (ao[w] - avg) * (ao[w] - avg); }
+        const double spread = std::sqrt(det_sum(tmp.data(), (int64_t)W) / ((double)W * (double)(W - 1)));
+        for (size_t w = 0; w < W; ++w) { tmp[w] = eo[w] * eo[w]; }
+        const double comb = std::sqrt(det_sum(tmp.data(), (int64_t)W)) / (double)W;
+        lp->results[o] = avg;
+        lp->results[2 + o] = spread;
+        lp->results[4 + o] = comb;
+    }
+    lp->h_wavg = a;
+    return MCI_OK;
+}
+
+template <class R>
+mci_status run_plain(mci_lp *lp, int64_t nsteps, int nblocks)
+{
+    mcil::State<R> S;
+    const int B = (int)(nsteps / nblocks);
+    TRY(state_of<R>(lp, B, nblocks, S));
+    const mcil::Target<R> T = target_of<R>(lp);
+    const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
+    auto t0 = clk::now();
+    HIPCHK(mcil::launch_lp_init<R>(T, S, k0, k1, lp->stream));
+    double &step = (sizeof(R) == 8) ? lp->step64 : lp->step32;
+    if (step <= 0.) { step = 1.0; }
+    TRY(calibrate<R>(lp, S, step));
+    lp->timings[0] = ms_since(t0);
+    HIPCHK(hipMemsetAsync(lp->nacc.p, 0, sizeof(int32_t) * lp->W, lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    t0 = clk::now();
+    HIPCHK(mcil::launch_lp_rw<R>(T, S, (R)step, lp->t_next, nsteps, k0, k1, lp->stream));
+    lp->t_next += (uint64_t)nsteps;
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    lp->timings[1] = ms_since(t0);
+    t0 = clk::now();
+    TRY(estimate(lp, nblocks));
+    lp->timings[2] = ms_since(t0);
+    std::vector<int32_t> acc((size_t)lp->W);
+    HIPCHK(hipMemcpy(acc.data(), lp->nacc.p, sizeof(int32_t) * lp->W, hipMemcpyDeviceToHost));
+    double a = 0.;
+    for (int32_t v : acc) { a += v; }
+    lp->results[6] = a / ((double)lp->W * (double)nsteps);
+    lp->results[7] = 0.;
+    return MCI_OK;
+}
+
+mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
+{
+    mcil::State<float> S;
+    const int B = (int)(nsteps / nblocks);
+    TRY(state_of<float>(lp, B, nblocks, S));
+    TRY(net_ready(lp));
+    mci_mlp *n = lp->net;
+    const mcil::Target<float> T = target_of<float>(lp);
+    const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
+    const size_t W = (size_t)lp->W;
+    HIPCHK(lp->out.ensure((size_t)n->Op * W));
+    HIPCHK(lp->outn.ensure((size_t)n->Op * W));
+    HIPCHK(lp->xp.ensure((size_t)lp->d * W));
+    HIPCHK(lp->xpb.ensure((size_t)lp->d * W));
+    HIPCHK(lp->logq_f.ensure(W));
+    TRY(mlp_reserve(n, lp->W));
+    auto t0 = clk::now();
+    HIPCHK(mcil::launch_lp_init<float>(T, S, k0, k1, lp->stream));
+    if (lp->step32 <= 0.) { lp->step32 = 1.0; }
+    TRY(calibrate<float>(lp, S, lp->step32));
+    lp->timings[0] = ms_since(t0);
+    HIPCHK(hipMemsetAsync(lp->nacc.p, 0, sizeof(int32_t) * W, lp->stream));
+    HIPCHK(hipMemsetAsync(lp->nacc_ind.p, 0, sizeof(int32_t) * W, lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    t0 = clk::now();
+    // the network's mixture at the current positions
+    HIPCHK(mcim::launch_to_bf16_scaled(S.x, lp->xpb.p, (int64_t)lp->d * lp->W, mcim::IN_SCALE, lp->stream));
+    TRY(mlp_forward(n, lp->W, lp->xpb.p, lp->out.p));
+    const uint64_t tb = lp->t_next;
+    int64_t nind = 0;
+    int64_t s = 0;
+    while (s < nsteps) {
+        // move type of step tb + s: the same for every walker
+        auto learned = [&](int64_t st) {
+            uint32_t o[4];
+            const uint64_t t = tb + (uint64_t)st;
+            philox_host(0x20000000u, 0xffffffffu, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
+            return (double)(o[0] >> 8) * 0x1p-24 < pmix;
+        };
+        if (!learned(s)) {
+            int64_t e = s + 1;
+            while (e < nsteps && !learned(e)) { ++e; }
+            HIPCHK(mcil::launch_lp_rw<float>(T, S, (float)lp->step32, tb + (uint64_t)s, e - s, k0, k1, lp->stream));
+            s = e;
+            continue;
+        }
+        const uint64_t t = tb + (uint64_t)s;
+        HIPCHK(mcil::launch_lp_propose(T, S, lp->out.p, n->Op, n->K, t, k0, k1, lp->xp.p, lp->xpb.p, mcim::IN_SCALE, lp->logq_f.p,
+                                       lp->stream));
+        TRY(mlp_forward(n, lp->W, lp->xpb.p, lp->outn.p));
+        HIPCHK(mcil::launch_lp_accept(T, S, lp->out.p, lp->outn.p, n->Op, n->K, t, k0, k1, lp->xp.p, lp->logq_f.p, lp->nacc_ind.p,
+                                      lp->stream));
+        ++nind;
+        ++s;
+    }
+    lp->t_next += (uint64_t)nsteps;
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    lp->timings[1] = ms_since(t0);
+    t0 = clk::now();
+    TRY(estimate(lp, nblocks));
+    lp->timings[2] = ms_since(t0);
+    std::vector<int32_t> acc(W), acci(W);
+    HIPCHK(hipMemcpy(acc.data(), lp->nacc.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(acci.data(), lp->nacc_ind.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
+    double a = 0., ai = 0.;
+    for (size_t w = 0; w < W; ++w) {
+        a += acc[w];
+        ai += acci[w];
+    }
+    const int64_t nrw = nsteps - nind;
+    lp->results[6] = nrw > 0 ? a / ((double)lp->W * (double)nrw) : 0.;
+    lp->results[7] = nind > 0 ? ai / ((double)lp->W * (double)nind) : 0.;
+    return MCI_OK;
+}
+} // namespace
+
+extern "C" {
+
+// ---- network ----
+mci_status mci_mlp_create(int d, int hidden, int ncomp, uint64_t seed, mci_mlp **out)
+{
+    if (!out) { return fail(MCI_E_INVALID_ARGUMENT, "null output pointer"); }
+    *out = nullptr;
+    return mlp_make(d, hidden, ncomp, seed, nullptr, out);
+}
+
+mci_status mci_mlp_destroy(mci_mlp *m)
+{
+    delete m;
+    return MCI_OK;
+}
+
+mci_status mci_mlp_nparams(mci_mlp *m, int64_t *n)
+{
+    CHECKM(m);
+    *n = m->np;
+    return MCI_OK;
+}
+
+mci_status mci_mlp_get_params(mci_mlp *m, float *p, int64_t n)
+{
+    CHECKM(m);
+    if (n != m->np || !p) { return fail(MCI_E_INVALID_ARGUMENT, "expected %lld parameters", (long long)m->np); }
+    std::vector<float> pad((size_t)m->npad);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(pad.data(), m->p.p, pad.size() * sizeof(float), hipMemcpyDeviceToHost));
+    padded_to_flat(m, pad, p);
+    return MCI_OK;
+}
+
+mci_status mci_mlp_set_params(mci_mlp *m, const float *p, int64_t n)
+{
+    CHECKM(m);
+    if (n != m->np || !p) { return fail(MCI_E_INVALID_ARGUMENT, "expected %lld parameters", (long long)m->np); }
+    std::vector<float> pad;
+    flat_to_padded(m, p, pad);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return mlp_upload_params(m, pad);
+}
+
+mci_status mci_mlp_forward(mci_mlp *m, int64_t n, const float *x, float *out)
+{
+    CHECKM(m);
+    if (n < 1 || !x || !out) { return fail(MCI_E_INVALID_ARGUMENT, "bad arguments"); }
+    TRY(mlp_reserve(m, n));
+    TRY(upload_transposed(m->X.p, x, n, m->d));
+    HIPCHK(mcim::launch_to_bf16_scaled(m->X.p, m->Xb.p, (int64_t)m->d * n, mcim::IN_SCALE, m->stream));
+    TRY(mlp_forward(m, n, m->Xb.p, m->OUT.p));
+    std::vector<float> t((size_t)m->Op * n);
+    HIPCHK(hipMemcpyAsync(t.data(), m->OUT.p, t.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (int64_t r = 0; r < n; ++r) {
+        for (int c = 0; c < m->O; ++c) { out[(size_t)r * m->O + c] = t[(size_t)c * n + r]; }
+    }
+    return MCI_OK;
+}
+
+mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *ctx, const float *y, double *loss, float *grad)
+{
+    CHECKM(m);
+    if (n < 1 || !ctx || !y) { return fail(MCI_E_INVALID_ARGUMENT, "bad arguments"); }
+    TRY(mlp_reserve(m, n));
+    TRY(upload_transposed(m->X.p, ctx, n, m->d));
+    TRY(upload_transposed(m->Y.p, y, n, m->d));
+    TRY(mlp_fwd_bwd(m, n));
+    std::vector<float> l((size_t)n), gp((size_t)m->npad);
+    HIPCHK(hipMemcpyAsync(l.data(), m->loss.p, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(gp.data(), m->g.p, sizeof(float) * m->npad, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (loss) {
+        double s = 0.;
+        for (float v : l) { s += v; }
+        *loss = s / (double)n;
+    }
+    if (grad) { padded_to_flat(m, gp, grad); }
+    return MCI_OK;
+}
+
+// ---- sampler ----
+mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *means, const double *covs, int64_t nwalkers, mci_lp **out)
+{
+    if (!out) { return fail(MCI_E_INVALID_ARGUMENT, "null output pointer"); }
+    *out = nullptr;
+    if (d < 1 || d > 32 || ncomp < 1 || ncomp > mcil::KMAX || nwalkers < 2 || nwalkers > (int64_t(1) << 30)) {
+        return fail(MCI_E_INVALID_ARGUMENT, "need 1 <= d <= 32, 1 <= ncomp <= %d, 2 <= nwalkers", mcil::KMAX);
+    }
+    if (!weights || !means || !covs) { return fail(MCI_E_INVALID_ARGUMENT, "null parameter array"); }
+    mci_lp *lp = new mci_lp();
+    lp->d = d;
+    lp->K = ncomp;
+    lp->W = nwalkers;
+    const int KM = mcil::KMAX;
+    lp->mu.assign((size_t)KM * d, 0.);
+    lp->P.assign((size_t)KM * d * d, 0.);
+    lp->v.assign((size_t)KM * d, 0.);
+    lp->L.assign((size_t)KM * d * d, 0.);
+    lp->c.assign(KM, -HUGE_VAL);
+    lp->cw.assign(KM, 1.);
+    double ws = 0.;
+    for (int k = 0; k < ncomp; ++k) { ws += weights[k]; }
+    if (!(ws > 0.)) {
+        delete lp;
+        return fail(MCI_E_INVALID_ARGUMENT, "weights must be >= 0 with a positive sum");
+    }
+    double cum = 0.;
+    for (int k = 0; k < ncomp; ++k) {
+        if (!(weights[k] >= 0.)) {
+            delete lp;
+            return fail(MCI_E_INVALID_ARGUMENT, "weights must be >= 0");
+        }
+        const double *S = covs + (size_t)k * d * d;
+        double *Lk = lp->L.data() + (size_t)k * d * d;
+        // Cholesky S = L L^T
+        for (int i = 0; i < d; ++i) {
+            for (int j = 0; j <= i; ++j) {
+                double s = S[i * d + j];
+                for (int q = 0; q < j; ++q) { s -= Lk[i * d + q] * Lk[j * d + q]; }
+                if (i == j) {
+                    if (!(s > 0.)) {
+                        delete lp;
+                        return fail(MCI_E_INVALID_ARGUMENT, "covariance %d is not positive definite", k);
+                    }
+                    Lk[i * d + i] = std::sqrt(s);
+                }
+                else { Lk[i * d + j] = s / Lk[j * d + j]; }
+            }
+        }
+        // P = S^-1 from the columns of L^-T L^-1
+        double *Pk = lp->P.data() + (size_t)k * d * d;
+        std::vector<double> col(d), y(d);
+        for (int e = 0; e < d; ++e) {
+            for (int i = 0; i < d; ++i) { // L y = e_e
+                double s = (i == e) ? 1. : 0.;
+                for (int q = 0; q < i; ++q) { s -= Lk[i * d + q] * y[q]; }
+                y[i] = s / Lk[i * d + i];
+            }
+            for (int i = d - 1; i >= 0; --i) { // L^T col = y
+                double s = y[i];
+                for (int q = i + 1; q < d; ++q) { s -= Lk[q * d + i] * col[q]; }
+                col[i] = s / Lk[i * d + i];
+            }
+            for (int i = 0; i < d; ++i) { Pk[i * d + e] = col[i]; }
+        }
+        for (int i = 0; i < d; ++i) { // exact symmetry
+            for (int j = 0; j < i; ++j) {
+                const double a = 0.5 * (Pk[i * d + j] + Pk[j * d + i]);
+                Pk[i * d + j] = a;
+                Pk[j * d + i] = a;
+            }
+        }
+        double logdet = 0.;
+        for (int i = 0; i < d; ++i) { logdet += 2.0 * std::log(Lk[i * d + i]); }
+        const double wn = weights[k] / ws;
+        lp->c[k] = std::log(wn) - 0.5 * (logdet + d * std::log(2.0 * M_PI));
+        for (int i = 0; i < d; ++i) { lp->mu[(size_t)k * d + i] = means[(size_t)k * d + i]; }
+        for (int i = 0; i < d; ++i) {
+            double s = 0.;
+            for (int j = 0; j < d; ++j) { s += Pk[i * d + j] * means[(size_t)k * d + j]; }
+            lp->v[(size_t)k * d + i] = s;
+        }
+        cum += wn;
+        lp->cw[k] = (k == ncomp - 1) ? 1. : cum;
+    }
+    *out = lp;
+    return MCI_OK;
+}
+
+mci_status mci_lp_destroy(mci_lp *lp)
+{
+    delete lp;
+    return MCI_OK;
+}
+
+mci_status mci_lp_set_seed(mci_lp *lp, uint64_t seed)
+{
+    CHECKL(lp);
+    lp->seed = seed;
+    return MCI_OK;
+}
+
+mci_status mci_lp_set_network(mci_lp *lp, int hidden, int ncomp)
+{
+    CHECKL(lp);
+    if (hidden < 16 || hidden > 1024 || ncomp < 1 || ncomp > 8) { return fail(MCI_E_INVALID_ARGUMENT, "network sizes out of range"); }
+    delete lp->net;
+    lp->net = nullptr;
+    lp->H = hidden;
+    lp->Kq = ncomp;
+    return MCI_OK;
+}
+
+mci_status mci_lp_run(mci_lp *lp, int mode, int64_t nsteps, int nblocks, double pmix, double *avg, double *err)
+{
+    CHECKL(lp);
+    if (nblocks < 2 || (nblocks & (nblocks - 1)) != 0 || nblocks > 64 || nsteps < nblocks || nsteps % nblocks != 0) {
+        return fail(MCI_E_INVALID_ARGUMENT, "nsteps must be a multiple of nblocks, a power of two in 2..64");
+    }
+    if (mode == MCI_LP_MIXED_FP32 && !(pmix >= 0. && pmix <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "mixing probability must be in [0, 1]"); }
+    TRY(lp_device(lp));
+    switch (mode) {
+    case MCI_LP_PLAIN_FP64: TRY(run_plain<double>(lp, nsteps, nblocks)); break;
+    case MCI_LP_PLAIN_FP32: TRY(run_plain<float>(lp, nsteps, nblocks)); break;
+    case MCI_LP_MIXED_FP32: TRY(run_mixed(lp, nsteps, nblocks, pmix)); break;
+    default: return fail(MCI_E_DOMAIN, "unknown mode %d", mode);
+    }
+    if (avg) { avg[0] = lp->results[0], avg[1] = lp->results[1]; }
+    if (err) { err[0] = lp->results[2], err[1] = lp->results[3]; }
+    return MCI_OK;
+}
+
+mci_status mci_lp_collect(mci_lp *lp, int nsnap, int64_t lag)
+{
+    CHECKL(lp);
+    if (nsnap < 2 || lag < 1) { return fail(MCI_E_INVALID_ARGUMENT, "need nsnap >= 2 and lag >= 1"); }
+    TRY(lp_device(lp));
+    mcil::State<float> S;
+    TRY(state_of<float>(lp, 0, 1, S));
+    const mcil::Target<float> T = target_of<float>(lp);
+    const uint32_t k0 = (uint32_t)lp->seed ^ 0x9E3779B9u, k1 = (uint32_t)(lp->seed >> 32) ^ 0x7F4A7C15u; // streams of their own
+    HIPCHK(lp->snap.ensure((size_t)nsnap * lp->d * lp->W));
+    HIPCHK(mcil::launch_lp_init<float>(T, S, k0, k1, lp->stream));
+    if (lp->step32 <= 0.) { lp->step32 = 1.0; }
+    TRY(calibrate<float>(lp, S, lp->step32));
+    for (int i = 0; i < nsnap; ++i) {
+        HIPCHK(mcil::launch_lp_rw<float>(T, S, (float)lp->step32, lp->t_next, lag, k0, k1, lp->stream));
+        lp->t_next += (uint64_t)lag;
+        HIPCHK(mcil::launch_lp_snapshot<float>(S, lp->d, lp->snap.p + (size_t)i * lp->d * lp->W, lp->stream));
+    }
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    lp->nsnap = nsnap;
+    return MCI_OK;
+}
+
+mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *loss_first, double *loss_last)
+{
+    CHECKL(lp);
+    if (lp->nsnap < 2) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect first"); }
+    if (iters < 1 || batch < 64 || batch > (1 << 20) || !(lr > 0.)) { return fail(MCI_E_INVALID_ARGUMENT, "bad training settings"); }
+    TRY(net_ready(lp));
+    mci_mlp *n = lp->net;
+    TRY(mlp_reserve(n, batch));
+    const uint32_t k0 = (uint32_t)lp->seed ^ 0x85EBCA6Bu, k1 = (uint32_t)(lp->seed >> 32) ^ 0xC2B2AE35u;
+    auto t0 = clk::now();
+    std::vector<float> l((size_t)batch);
+    auto mean_loss = [&](double &out) -> mci_status {
+        HIPCHK(hipMemcpyAsync(l.data(), n->loss.p, sizeof(float) * batch, hipMemcpyDeviceToHost, n->stream));
+        HIPCHK(hipStreamSynchronize(n->stream));
+        double s = 0.;
+        for (float v : l) { s += v; }
+        out = s / batch;
+        return MCI_OK;
+    };
+    for (int it = 0; it < iters; ++it) {
+        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, (uint64_t)it, k0, k1, n->X.p, n->Y.p, n->stream));
+        TRY(mlp_fwd_bwd(n, batch));
+        if (it == 0) { TRY(mean_loss(lp->loss_first)); }
+        TRY(mlp_adam(n, (float)lr));
+    }
+    TRY(mean_loss(lp->loss_last)); // loss of the last minibatch (before its update)
+    lp->timings[3] = ms_since(t0);
+    if (loss_first) { *loss_first = lp->loss_first; }
+    if (loss_last) { *loss_last = lp->loss_last; }
+    return MCI_OK;
+}
+
+mci_status mci_lp_get_results(mci_lp *lp, double *r, int n)
+{
+    CHECKL(lp);
+    for (int i = 0; i < n && i < 8; ++i) { r[i] = lp->results[i]; }
+    return MCI_OK;
+}
+
+mci_status mci_lp_get_timings(mci_lp *lp, double *ms, int n)
+{
+    CHECKL(lp);
+    for (int i = 0; i < n && i < 4; ++i) { ms[i] = lp->timings[i]; }
+    return MCI_OK;
+}
+
+mci_status mci_lp_get_walker_means(mci_lp *lp, double *m, int64_t n)
+{
+    CHECKL(lp);
+    if (n != 2 * lp->W || lp->h_wavg.size() != (size_t)n) { return fail(MCI_E_INVALID_ARGUMENT, "expected 2 x nwalkers values after a run"); }
+    std::copy(lp->h_wavg.begin(), lp->h_wavg.end(), m);
+    return MCI_OK;
+}
+
+mci_status mci_lp_get_network(mci_lp *lp, mci_mlp **m)
+{
+    CHECKL(lp);
+    TRY(lp_device(lp));
+    TRY(net_ready(lp));
+    *m = lp->net;
+    return MCI_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,251 @@
+// mci_mlp.hip — BF16 MFMA GEMM (FP32 accumulation) with fused bias/SiLU epilogue, the
+// Gaussian-mixture likelihood head, SiLU backward, row sums and Adam.
+//
+// GEMM tiling: a 256-thread workgroup computes a 64 x 64 tile of C as four 32 x 32 wave tiles
+// with v_mfma_f32_32x32x16_bf16.  Per 32-wide slice of the reduction dimension the A and B
+// tiles are staged in LDS as [row][k] and [col][k] (k contiguous), so each lane's MFMA operand
+// (8 consecutive k of one row or column) is one 16-byte LDS read.  Operand layout of the
+// instruction (checked on the device): lane l supplies row/column l % 32 and k = 8 (l / 32) .. +7;
+// accumulator element i of lane l is C[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
+// Loads are bounds-checked (zero fill) and stores guarded, so any M, N, K are valid.
+#include "mci_mlp.hpp"
+
+namespace mcim
+{
+namespace
+{
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int TILE = 64, KC = 32, KP = KC + 8; // LDS row stride: 40 bf16 = 80 bytes
+
+__device__ __forceinline__ bf16_t f2bf(float f)
+{
+    // round to nearest even (finite inputs)
+    const uint32_t u = __float_as_uint(f);
+    return (bf16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t *__restrict__ A, const bf16_t *__restrict__ B,
+                                              const float *__restrict__ bias, float *__restrict__ C, int ldc, bf16_t *__restrict__ H,
+                                              int ldh)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t As[TILE][KP];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[TILE][KP];
+    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid * 8 + q; // 0 .. 2047
+            int mm, kk;
+            if (TA) { kk = e / TILE; mm = e % TILE; }
+            else { mm = e / KC; kk = e % KC; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            bf16_t v = 0;
+            if (gm < M && gk < K) { v = TA ? A[(int64_t)gk * M + gm] : A[(int64_t)gm * K + gk]; }
+            As[mm][kk] = v;
+            int nn;
+            if (TB) { nn = e / KC; kk = e % KC; }
+            else { kk = e / TILE; nn = e % TILE; }
+            const int gn = n0 + nn, gk2 = k0 + kk;
+            bf16_t u = 0;
+            if (gn < N && gk2 < K) { u = TB ? B[(int64_t)gn * K + gk2] : B[(int64_t)gk2 * N + gn]; }
+            Bs[nn][kk] = u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 16) {
+            const int kb = ks + 8 * (lane >> 5);
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wm * 32 + (lane & 31)][kb]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[wn * 32 + (lane & 31)][kb]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int col = n0 + wn * 32 + (lane & 31);
+    if (col >= N) { return; }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = m0 + wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        if (row < M) {
+            const float v = acc[i] + ((EPI != EPI_NONE) ? bias[row] : 0.f);
+            C[(int64_t)row * ldc + col] = v;
+            if (EPI == EPI_BIAS_SILU) { H[(int64_t)row * ldh + col] = f2bf(silu(v)); }
+        }
+    }
+}
+
+__global__ void k_to_bf16(const float *x, bf16_t *xb, int64_t n, float scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { xb[i] = f2bf(x[i] * scale); }
+}
+
+// one sample per thread, feature-major layout (element c of sample r at out[c * ld + r]):
+// -log of the mixture density of y under the sample's parameters and the gradient of
+// (inv_n * that) with respect to the sample's raw outputs
+__global__ void k_mixture_head(const float *out, int ld, const float *y, int n, int d, int K, int rows, float *loss, float *dout,
+                               bf16_t *doutb, float inv_n)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) { return; }
+    const float *o = out + r;
+    const float *yr = y + r;
+    float *g = dout + r;
+    bf16_t *gb = doutb + r;
+    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    float amax = -__builtin_huge_valf();
+    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, o[(int64_t)k * ld]); }
+    float sa = 0.f;
+    for (int k = 0; k < K; ++k) { sa += __expf(o[(int64_t)k * ld] - amax); }
+    const float lsa = amax + __logf(sa);
+    float comp[8];
+    float cmax = -__builtin_huge_valf();
+    for (int k = 0; k < K; ++k) {
+        float c = o[(int64_t)k * ld] - lsa;
+        for (int j = 0; j < d; ++j) {
+            const float mu = MU_SCALE * o[(int64_t)(K + k * d + j) * ld];
+            const float ls = LS_MAX * tanhf(o[(int64_t)(K + K * d + k * d + j) * ld] / LS_MAX);
+            const float z = (yr[(int64_t)j * ld] - mu) * __expf(-ls);
+            c += -ls - HALF_LOG_2PI - 0.5f * z * z;
+        }
+        comp[k] = c;
+        cmax = fmaxf(cmax, c);
+    }
+    float sc = 0.f;
+    for (int k = 0; k < K; ++k) { sc += __expf(comp[k] - cmax); }
+    const float L = cmax + __logf(sc);
+    loss[r] = -L;
+    for (int k = 0; k < K; ++k) {
+        const float gam = __expf(comp[k] - L);
+        const float pik = __expf(o[(int64_t)k * ld] - lsa);
+        const float ga = inv_n * (pik - gam);
+        g[(int64_t)k * ld] = ga;
+        gb[(int64_t)k * ld] = f2bf(ga);
+        for (int j = 0; j < d; ++j) {
+            const int64_t im = (int64_t)(K + k * d + j) * ld, il = (int64_t)(K + K * d + k * d + j) * ld;
+            const float mu = MU_SCALE * o[im];
+            const float t = tanhf(o[il] / LS_MAX);
+            const float ls = LS_MAX * t;
+            const float isg = __expf(-ls);
+            const float z = (yr[(int64_t)j * ld] - mu) * isg;
+            const float gm = inv_n * (-gam * z * isg) * MU_SCALE;
+            const float gl = inv_n * gam * (1.0f - z * z) * (1.0f - t * t);
+            g[im] = gm;
+            g[il] = gl;
+            gb[im] = f2bf(gm);
+            gb[il] = f2bf(gl);
+        }
+    }
+    for (int c = K * (1 + 2 * d); c < rows; ++c) {
+        g[(int64_t)c * ld] = 0.f;
+        gb[(int64_t)c * ld] = 0;
+    }
+}
+
+__global__ void k_silu_back(const float *dh, const float *z, bf16_t *dzb, float *dz, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    const float zz = z[i];
+    const float s = 1.0f / (1.0f + __expf(-zz));
+    const float v = dh[i] * (s * (1.0f + zz * (1.0f - s)));
+    dz[i] = v;
+    dzb[i] = f2bf(v);
+}
+
+// out[f] = sum of row f of a [rows][cols] (one wave per row, fixed order: lane partial sums in
+// column order, then a tree over lanes)
+__global__ void k_rowsum(const float *a, int rows, int cols, int lda, float *out)
+{
+    const int f = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (f >= rows) { return; }
+    float s = 0.f;
+    for (int c = lane; c < cols; c += 64) { s += a[(int64_t)f * lda + c]; }
+    for (int o = 32; o > 0; o >>= 1) { s += __shfl_down(s, o); }
+    if (lane == 0) { out[f] = s; }
+}
+
+__global__ void k_adam(float *p, float *m, float *v, const float *g, bf16_t *copy, int64_t n, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    p[i] = pi;
+    if (copy) { copy[i] = f2bf(pi); }
+}
+
+dim3 grid1(int64_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+template <bool TA, bool TB>
+hipError_t gemm_epi(int epi, dim3 g, hipStream_t s, int M, int N, int K, const bf16_t *A, const bf16_t *B, const float *bias, float *C,
+                    int ldc, bf16_t *H, int ldh)
+{
+    switch (epi) {
+    case EPI_NONE: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_NONE>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh); break;
+    case EPI_BIAS: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh); break;
+    default: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS_SILU>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh); break;
+    }
+    return hipGetLastError();
+}
+} // namespace
+
+hipError_t gemm_bf16(bool ta, bool tb, int epi, int M, int N, int K, const bf16_t *A, const bf16_t *B, const float *bias, float *C,
+                     int ldc, bf16_t *H, int ldh, hipStream_t s)
+{
+    const dim3 g((unsigned)((N + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
+    if (ta && tb) { return gemm_epi<true, true>(epi, g, s, M, N, K, A, B, bias, C, ldc, H, ldh); }
+    if (ta) { return gemm_epi<true, false>(epi, g, s, M, N, K, A, B, bias, C, ldc, H, ldh); }
+    if (tb) { return gemm_epi<false, true>(epi, g, s, M, N, K, A, B, bias, C, ldc, H, ldh); }
+    return gemm_epi<false, false>(epi, g, s, M, N, K, A, B, bias, C, ldc, H, ldh);
+}
+
+hipError_t launch_to_bf16_scaled(const float *x, bf16_t *xb, int64_t n, float scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_bf16, grid1(n, 256), dim3(256), 0, s, x, xb, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float *x, bf16_t *xb, int64_t n, hipStream_t s) { return launch_to_bf16_scaled(x, xb, n, 1.0f, s); }
+
+hipError_t launch_mixture_head(const float *out, int ld, const float *y, int n, int d, int K, int rows, float *loss, float *dout,
+                               bf16_t *doutb, float inv_n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mixture_head, grid1(n, 64), dim3(64), 0, s, out, ld, y, n, d, K, rows, loss, dout, doutb, inv_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_silu_back(const float *dh, const float *z, bf16_t *dzb, int64_t n, hipStream_t s)
+{
+    // dz (FP32) overwrites dh in place
+    hipLaunchKernelGGL(k_silu_back, grid1(n, 256), dim3(256), 0, s, dh, z, dzb, const_cast<float *>(dh), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rowsum(const float *a, int rows, int cols, int lda, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned)rows), dim3(64), 0, s, a, rows, cols, lda, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, bf16_t *copy, int64_t n, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam, grid1(n, 256), dim3(256), 0, s, p, m, v, g, copy, n, lr, b1, b2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+} // namespace mcim

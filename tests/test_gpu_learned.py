@@ -1,0 +1,91 @@
+"""The learned proposal on the GPU: the BF16 MFMA network forward and gradient against the CPU
+oracle (oracle/mlp_oracle.py, same bfloat16 rounding points), and the sampler: plain chains and
+the mixed chain (untrained and trained network) unbiased against the analytic values, with the
+trained network's gain in error bar at equal steps."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from mcitest import ROOT
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.mark.parametrize("d,H,K,n", [(32, 128, 8, 512), (8, 64, 3, 300)])
+def test_network_forward_and_gradient_match_oracle(d, H, K, n):
+    from mcintegratorplusplus_amd.learned import Network
+    print("network", d, H, K, n, flush=True)
+    net = Network(d, H, K, seed=7)
+    p = net.params().astype(np.float64)
+    rng = np.random.default_rng(11)
+    x = rng.normal(0.0, 3.0, (n, d)).astype(np.float32)
+    y = (x + rng.normal(0.0, 1.0, (n, d))).astype(np.float32)
+    out = net.forward(x)
+    f, _ = M.forward(p, x, d, H, K, emulate=True)
+    assert _rel(out, f["OUT"]) < 2e-3
+    loss, g = net.lossGrad(x, y)
+    lo, go = M.loss_grad(p, x, y, d, H, K, emulate=True)
+    assert abs(loss - lo) < 1e-3 * abs(lo)
+    off = 0
+    for name, shp in M.sizes(d, H, K):
+        m = int(np.prod(shp))
+        assert _rel(g[off:off + m], go[off:off + m]) < 2e-2, name
+        off += m
+    # parameters round-trip
+    q = p.astype(np.float32) * 0.5
+    net.setParams(q)
+    assert np.array_equal(net.params(), q)
+
+
+def _small_target():
+    from mcintegratorplusplus_amd.learned import mixture_target
+    return mixture_target(d=8, ncomp=3, seed=3, mean_sd=2.0, eig=(0.2, 1.0))
+
+
+def _z(avg, err, exact):
+    return (np.asarray(avg) - np.asarray(exact)) / np.asarray(err)
+
+
+def test_sampler_plain_and_untrained_mixed_are_unbiased():
+    from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP32, PLAIN_FP64, LearnedProposalSampler
+    w, mu, cov, exact = _small_target()
+    s = LearnedProposalSampler(w, mu, cov, 8192)
+    s.setSeed(21)
+    s.setNetwork(64, 3)
+    for mode in (PLAIN_FP64, PLAIN_FP32, MIXED_FP32):
+        print("sampler mode", mode, flush=True)
+        avg, err = s.run(mode, 2048, 32, 0.5)
+        z = _z(avg, err, exact)
+        r = s.results()
+        print(mode, avg, err, z, r["acc_rw"], r["acc_learned"], s.timings(), flush=True)
+        assert np.all(np.abs(z) < 4.5), (mode, z)
+        assert 0.3 < r["acc_rw"] < 0.7
+
+
+def test_trained_network_reduces_the_error_bar():
+    from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP32, LearnedProposalSampler
+    w, mu, cov, exact = _small_target()
+    s = LearnedProposalSampler(w, mu, cov, 8192)
+    s.setSeed(5)
+    s.setNetwork(64, 3)
+    print("collect + train", flush=True)
+    s.collect(8, 256)
+    l0, l1 = s.train(400, 2048, 2e-3)
+    print("loss", l0, l1, s.timings(), flush=True)
+    assert l1 < l0
+    a_p, e_p = s.run(PLAIN_FP32, 2048, 32)
+    a_m, e_m = s.run(MIXED_FP32, 2048, 32, 0.5)
+    r = s.results()
+    print("plain", a_p, e_p, "mixed", a_m, e_m, "learned acceptance", r["acc_learned"], flush=True)
+    assert np.all(np.abs(_z(a_m, e_m, exact)) < 4.5)
+    assert r["acc_learned"] > 0.05
+    assert e_m[0] < 0.7 * e_p[0]
