@@ -295,6 +295,8 @@ mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *x, const float 
 #define MCI_LP_PLAIN_FP64   0  /* random-walk Metropolis (single-coordinate uniform moves), FP64 */
 #define MCI_LP_PLAIN_FP32   1  /* the same chain in FP32 with Kahan block sums */
 #define MCI_LP_MIXED_FP32   2  /* each step is a learned move with probability pmix, else a random-walk move */
+#define MCI_LP_INDEP_FP32   3  /* as MIXED, with the independence proposal q(x') = the network's mixture at one fixed
+                                  input (the mean of the last training snapshot), all steps in one launch */
 typedef struct mci_lp mci_lp;
 mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *means, const double *covs, int64_t nwalkers,
                          mci_lp **out);
@@ -307,6 +309,9 @@ mci_status mci_lp_set_network(mci_lp *lp, int hidden, int ncomp); /* default 128
 mci_status mci_lp_run(mci_lp *lp, int mode, int64_t nsteps, int nblocks, double pmix, double *avg, double *err);
 /* training samples: nsnap snapshots of every walker of the plain FP32 chain, lag steps apart */
 mci_status mci_lp_collect(mci_lp *lp, int nsnap, int64_t lag);
+/* training pairs (x_a, x_b): x_b is the same walker at another snapshot with probability same_fraction
+ * (default 0.5; teaches moves within a mode), else another walker (an independent sample of p) */
+mci_status mci_lp_set_training_pairs(mci_lp *lp, double same_fraction);
 /* Adam on minibatches of training pairs; losses of the first and last minibatch */
 mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *loss_first, double *loss_last);
 /* last run: [0..1] avg, [2..3] standard error of walker means, [4..5] MPIMCI combine of the

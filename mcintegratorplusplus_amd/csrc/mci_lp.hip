@@ -95,10 +95,10 @@ __device__ __forceinline__ float next_normal(Normals &N, uint32_t c0base, uint32
             ++N.call;
             N.avail = 2;
         }
-        const int a = 2 - N.avail;
+        const bool second = N.avail == 1;
         --N.avail;
-        const float v1 = sym24(N.o[2 * a]);
-        const float v2 = sym24(N.o[2 * a + 1]);
+        const float v1 = sym24(second ? N.o[2] : N.o[0]);
+        const float v2 = sym24(second ? N.o[3] : N.o[1]);
         const float s = fmaf(v1, v1, v2 * v2);
         if (s < 1.0f) {
             const float f = sqrtf(-2.0f * blog(s) / s);
@@ -109,23 +109,30 @@ __device__ __forceinline__ float next_normal(Normals &N, uint32_t c0base, uint32
     }
 }
 
-// log-density and state (q_k, m, S) of x (column w of xs with stride ldx) under the target
+// log-density state (q_k, m, S) of a position held in registers: q_k = |U_k x - U_k mu_k|^2 with
+// U_k the upper Cholesky factor of P_k (d (d + 1)/2 fused multiply-adds per component); U and
+// U mu are read at the same address by every lane
 template <class R>
-__device__ void target_eval(const Target<R> &T, const R *x, int64_t ldx, R q[KMAX], R &m, R &S)
+__device__ void target_eval_reg(const Target<R> &T, const R *U, const R xr[DMAX], R q[KMAX], R &m, R &S)
 {
     const int d = T.d;
     m = -__builtin_huge_val();
     R e[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-        const R *mu = T.mu + k * d;
-        const R *P = T.P + (int64_t)k * d * d;
+        const R *Uk = U + (int64_t)k * d * d;
+        const R *bk = T.Um + k * d;
         R qk = 0;
-        for (int i = 0; i < d; ++i) {
-            const R di = x[i * ldx] - mu[i];
-            R row = 0;
-            for (int j = 0; j < d; ++j) { row = fma(P[i * d + j], x[j * ldx] - mu[j], row); }
-            qk = fma(di, row, qk);
+#pragma unroll
+        for (int i = 0; i < DMAX; ++i) {
+            if (i < d) {
+                R r = -bk[i];
+#pragma unroll
+                for (int j = i; j < DMAX; ++j) {
+                    if (j < d) { r = fma(Uk[i * d + j], xr[j], r); }
+                }
+                qk = fma(r, r, qk);
+            }
         }
         q[k] = qk;
         e[k] = T.c[k] - (R)0.5 * qk;
@@ -134,6 +141,16 @@ __device__ void target_eval(const Target<R> &T, const R *x, int64_t ldx, R q[KMA
     S = 0;
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) { S = S + rexp(e[k] - m); }
+}
+
+// the same for column w of x [d][ldx] in global memory
+template <class R>
+__device__ void target_eval(const Target<R> &T, const R *x, int64_t ldx, R q[KMAX], R &m, R &S)
+{
+    R xr[DMAX];
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j) { xr[j] = (j < T.d) ? x[j * ldx] : (R)0; }
+    target_eval_reg(T, T.U, xr, q, m, S);
 }
 
 template <class R>
@@ -210,132 +227,326 @@ __device__ __forceinline__ void rw_draw<float>(uint32_t o1, uint32_t o2, uint32_
     ua = (float)(o2 >> 8) * 0x1p-24f;
 }
 
+// per-lane chain state held in registers by the multi-step kernels
+template <class R>
+struct Lane
+{
+    R q[KMAX], c[KMAX];
+    R m, S;
+    double sq;
+    R s0, c0, s1, c1;
+    int jb, bi;
+    int32_t nacc, nacc_ind;
+};
+
+template <class R>
+__device__ void lane_load(const Target<R> &T, const State<R> &St, int64_t wl, Lane<R> &L)
+{
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        L.q[k] = St.q[(int64_t)k * St.W + wl];
+        L.c[k] = T.c[k];
+    }
+    L.m = St.m[wl];
+    L.S = St.S[wl];
+    L.sq = St.sq[wl];
+    L.s0 = St.sb[wl];
+    L.c0 = St.cb[wl];
+    L.s1 = St.sb[St.W + wl];
+    L.c1 = St.cb[St.W + wl];
+    L.jb = St.jb[wl];
+    L.bi = St.bi[wl];
+    L.nacc = 0;
+    L.nacc_ind = 0;
+}
+
+template <class R, int WGS>
+__device__ void lane_store(const State<R> &St, int d, int64_t w, const R *xs, int tid, const Lane<R> &L)
+{
+    for (int i = 0; i < d; ++i) { St.x[(int64_t)i * St.W + w] = xs[i * WGS + tid]; }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * St.W + w] = L.q[k]; }
+    St.m[w] = L.m;
+    St.S[w] = L.S;
+    St.sq[w] = L.sq;
+    St.sb[w] = L.s0;
+    St.cb[w] = L.c0;
+    St.sb[St.W + w] = L.s1;
+    St.cb[St.W + w] = L.c1;
+    St.jb[w] = L.jb;
+    St.bi[w] = L.bi;
+    St.nacc[w] += L.nacc;
+}
+
+// LDS images of the target used by the multi-step kernels: P_k rows padded to d + 1 and P_k mu_k
+template <class R, int WGS>
+__device__ void load_target_lds(const Target<R> &T, R *Ps, R *vs)
+{
+    const int d = T.d, dp = d + 1;
+    for (int e = threadIdx.x; e < KMAX * d * d; e += WGS) {
+        const int k = e / (d * d), r = (e / d) % d, c = e % d;
+        Ps[(k * d + r) * dp + c] = T.P[e];
+    }
+    for (int e = threadIdx.x; e < KMAX * d; e += WGS) { vs[e] = T.v[e]; }
+}
+
+// one random-walk step at time t (single-coordinate uniform move)
+template <class R, int WGS>
+__device__ __forceinline__ bool rw_step(const Target<R> &T, const R *Ps, const R *vs, R *xs, int tid, uint32_t wid, Lane<R> &L, R step,
+                                        uint64_t t, uint32_t k0, uint32_t k1)
+{
+    const int d = T.d, dp = d + 1;
+    const uint32_t thr = (uint32_t)(0u - (uint32_t)d) % (uint32_t)d;
+    const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
+    uint32_t o[4];
+    philox(0u, wid, tl, th, k0, k1, o);
+    uint64_t prod = (uint64_t)o[0] * (uint32_t)d;
+    if (__builtin_expect(__any((uint32_t)prod < thr), 0)) {
+        uint32_t low = (uint32_t)prod, kx = 0;
+        while (low < thr) {
+            const uint32_t kk = 0x80000000u + (kx++);
+            uint32_t e[4];
+            philox(kk >> 2, wid, tl, th, k0, k1, e);
+            prod = (uint64_t)e[kk & 3u] * (uint32_t)d;
+            low = (uint32_t)prod;
+        }
+    }
+    const int j = (int)(prod >> 32);
+    R delta, ua;
+    rw_draw<R>(o[1], o[2], o[3], step, delta, ua);
+    const R xo = xs[j * WGS + tid];
+    const R xn = xo + delta;
+    R g[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { g[k] = -vs[k * d + j]; }
+    for (int i = 0; i < d; ++i) {
+        const R xi = xs[i * WGS + tid];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { g[k] = fma(Ps[(k * d + j) * dp + i], xi, g[k]); }
+    }
+    R qn[KMAX], en[KMAX];
+    R mn = -__builtin_huge_val();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const R pjj = Ps[(k * d + j) * dp + j];
+        qn[k] = L.q[k] + fma(delta + delta, g[k], delta * delta * pjj);
+        en[k] = L.c[k] - (R)0.5 * qn[k];
+        mn = en[k] > mn ? en[k] : mn;
+    }
+    R Sn = 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { Sn = Sn + rexp(en[k] - mn); }
+    const R E = rexp(mn - L.m);
+    const bool acc = ua * L.S < E * Sn;
+    if (acc) { xs[j * WGS + tid] = xn; }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { L.q[k] = acc ? qn[k] : L.q[k]; }
+    L.m = acc ? mn : L.m;
+    L.S = acc ? Sn : L.S;
+    L.sq = acc ? L.sq + fma((double)xn, (double)xn, -((double)xo * (double)xo)) : L.sq;
+    L.nacc += acc ? 1 : 0;
+    return acc;
+}
+
+// observables after a step; at a block end the block means are written and q_k, m, S, |x|^2
+// recomputed from x (bounding the drift of the incremental updates)
+template <class R, int WGS>
+__device__ __forceinline__ void after_step(const Target<R> &T, const State<R> &St, const R *Ps, const R *xs, int tid, int64_t w, bool live,
+                                          Lane<R> &L)
+{
+    if (St.B <= 0) { return; }
+    const int d = T.d, dp = d + 1;
+    const int bprev = L.bi;
+    accumulate(St, w, live, L.sq, xs[tid], L.s0, L.c0, L.s1, L.c1, L.jb, L.bi);
+    if (L.bi == bprev) { return; }
+    double q2 = 0.;
+    for (int i = 0; i < d; ++i) {
+        const double v = (double)xs[i * WGS + tid];
+        q2 = fma(v, v, q2);
+    }
+    L.sq = q2;
+    L.m = -__builtin_huge_val();
+    R e2[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const R *mu = T.mu + k * d;
+        R qk = 0;
+        for (int i = 0; i < d; ++i) {
+            R row = 0;
+            for (int jj = 0; jj < d; ++jj) { row = fma(Ps[(k * d + i) * dp + jj], xs[jj * WGS + tid] - mu[jj], row); }
+            qk = fma(xs[i * WGS + tid] - mu[i], row, qk);
+        }
+        L.q[k] = qk;
+        e2[k] = L.c[k] - (R)0.5 * qk;
+        L.m = e2[k] > L.m ? e2[k] : L.m;
+    }
+    L.S = 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { L.S = L.S + rexp(e2[k] - L.m); }
+}
+
 // nsteps random-walk steps t0 .. t0 + nsteps - 1 for 256 walkers per workgroup
 template <class R>
 __global__ __launch_bounds__(WG) void k_lp_rw(Target<R> T, State<R> St, R step, uint64_t t0, int64_t nsteps, uint32_t k0, uint32_t k1)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    const int d = T.d, dp = d + 1, tid = threadIdx.x;
+    const int d = T.d, tid = threadIdx.x;
     R *Ps = reinterpret_cast<R *>(lds_raw); // [KMAX][d][d + 1]
-    R *vs = Ps + KMAX * d * dp;              // [KMAX][d]
+    R *vs = Ps + KMAX * d * (d + 1);         // [KMAX][d]
     R *xs = vs + KMAX * d;                   // [d][WG]
     const int64_t w = (int64_t)blockIdx.x * WG + tid;
     const bool live = w < St.W;
     const int64_t wl = live ? w : St.W - 1;
-    for (int e = tid; e < KMAX * d * d; e += WG) {
-        const int k = e / (d * d), r = (e / d) % d, c = e % d;
-        Ps[(k * d + r) * dp + c] = T.P[e];
-    }
-    for (int e = tid; e < KMAX * d; e += WG) { vs[e] = T.v[e]; }
+    load_target_lds<R, WG>(T, Ps, vs);
     for (int i = 0; i < d; ++i) { xs[i * WG + tid] = St.x[(int64_t)i * St.W + wl]; }
     __syncthreads();
-    R q[KMAX], c[KMAX];
+    Lane<R> L;
+    lane_load(T, St, wl, L);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        (void)rw_step<R, WG>(T, Ps, vs, xs, tid, (uint32_t)wl, L, step, t0 + (uint64_t)s, k0, k1);
+        after_step<R, WG>(T, St, Ps, xs, tid, w, live, L);
+    }
+    if (live) { lane_store<R, WG>(St, d, w, xs, tid, L); }
+}
+
+// One launch for a whole run of the chain that mixes random-walk moves with the independence
+// proposal q(x') = sum_k pi_k N(x'; mu_k, diag sigma_k^2) given by the network at a fixed input.
+// Step t is a learned move for every walker when the step's common word says so (probability
+// pmix), so the branch is uniform.  A learned move: component k ~ pi, x' = mu_k + sigma_k z,
+// accept with min(1, p(x') q(x) / (p(x) q(x'))); q and the target's U_k live in LDS.
+constexpr int WGM = 512;
+
+__device__ __forceinline__ float indep_logq(const float *Qs, int Kq, int d, const float xr[DMAX])
+{
+    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    float comp[KMAX];
+    float cmax = -__builtin_huge_valf();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-        q[k] = St.q[(int64_t)k * St.W + wl];
-        c[k] = T.c[k];
+        if (k >= Kq) { comp[k] = -__builtin_huge_valf(); continue; }
+        const float *row = Qs + k * (1 + 3 * d); // [logc][mu d][sigma d][1/sigma d]
+        float acc = 0.f;
+#pragma unroll
+        for (int j = 0; j < DMAX; ++j) {
+            if (j < d) {
+                const float z = (xr[j] - row[1 + j]) * row[1 + 2 * d + j];
+                acc = fmaf(z, z, acc);
+            }
+        }
+        comp[k] = row[0] - 0.5f * acc - (float)d * HALF_LOG_2PI;
+        cmax = fmaxf(cmax, comp[k]);
     }
-    R m = St.m[wl], S = St.S[wl];
-    double sq = St.sq[wl];
-    R s0 = St.sb[wl], c0 = St.cb[wl], s1 = St.sb[St.W + wl], c1 = St.cb[St.W + wl];
-    int jb = St.jb[wl], bi = St.bi[wl];
-    int32_t nacc = 0;
-    const uint32_t thr = (uint32_t)(0u - (uint32_t)d) % (uint32_t)d;
+    float sc = 0.f;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { sc += (k < Kq) ? __expf(comp[k] - cmax) : 0.f; }
+    return cmax + __logf(sc);
+}
+
+// the same for a position in LDS (stride ld)
+__device__ __forceinline__ float indep_logq_lds(const float *Qs, int Kq, int d, const float *x, int ld)
+{
+    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    float comp[KMAX];
+    float cmax = -__builtin_huge_valf();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k >= Kq) { comp[k] = -__builtin_huge_valf(); continue; }
+        const float *row = Qs + k * (1 + 3 * d);
+        float acc = 0.f;
+        for (int j = 0; j < d; ++j) {
+            const float z = (x[j * ld] - row[1 + j]) * row[1 + 2 * d + j];
+            acc = fmaf(z, z, acc);
+        }
+        comp[k] = row[0] - 0.5f * acc - (float)d * HALF_LOG_2PI;
+        cmax = fmaxf(cmax, comp[k]);
+    }
+    float sc = 0.f;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { sc += (k < Kq) ? __expf(comp[k] - cmax) : 0.f; }
+    return cmax + __logf(sc);
+}
+
+__global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<float> St, const float *Q, int Kq, float step, uint64_t t0,
+                                                      int64_t nsteps, float pmix, uint32_t k0, uint32_t k1, int32_t *nacc_ind)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int d = T.d, tid = threadIdx.x;
+    float *Ps = reinterpret_cast<float *>(lds_raw); // [KMAX][d][d + 1]
+    float *vs = Ps + KMAX * d * (d + 1);             // [KMAX][d]
+    float *Us = vs + KMAX * d;                       // [KMAX][d][d]
+    float *Qs = Us + KMAX * d * d;                   // [Kq][1 + 3 d], then cumulative weights [Kq]
+    float *xs = Qs + Kq * (1 + 3 * d) + Kq;          // [d][WGM]
+    const int64_t w = (int64_t)blockIdx.x * WGM + tid;
+    const bool live = w < St.W;
+    const int64_t wl = live ? w : St.W - 1;
+    load_target_lds<float, WGM>(T, Ps, vs);
+    for (int e = tid; e < KMAX * d * d; e += WGM) { Us[e] = T.U[e]; }
+    for (int e = tid; e < Kq * (1 + 3 * d) + Kq; e += WGM) { Qs[e] = Q[e]; }
+    for (int i = 0; i < d; ++i) { xs[i * WGM + tid] = St.x[(int64_t)i * St.W + wl]; }
+    __syncthreads();
+    const float *cwq = Qs + Kq * (1 + 3 * d);
+    Target<float> TL = T;
+    TL.U = Us;
+    Lane<float> L;
+    lane_load(T, St, wl, L);
     for (int64_t s = 0; s < nsteps; ++s) {
         const uint64_t t = t0 + (uint64_t)s;
         const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
-        uint32_t o[4];
-        philox(0u, (uint32_t)wl, tl, th, k0, k1, o);
-        uint64_t prod = (uint64_t)o[0] * (uint32_t)d;
-        if (__builtin_expect(__any((uint32_t)prod < thr), 0)) {
-            uint32_t low = (uint32_t)prod, kx = 0;
-            while (low < thr) {
-                const uint32_t kk = 0x80000000u + (kx++);
-                uint32_t e[4];
-                philox(kk >> 2, (uint32_t)wl, tl, th, k0, k1, e);
-                prod = (uint64_t)e[kk & 3u] * (uint32_t)d;
-                low = (uint32_t)prod;
-            }
+        uint32_t oc[4];
+        philox(0x20000000u, 0xffffffffu, tl, th, k0, k1, oc);
+        const bool learned = (float)(oc[0] >> 8) * 0x1p-24f < pmix;
+        if (!learned) {
+            (void)rw_step<float, WGM>(T, Ps, vs, xs, tid, (uint32_t)wl, L, step, t, k0, k1);
         }
-        const int j = (int)(prod >> 32);
-        R delta, ua;
-        rw_draw<R>(o[1], o[2], o[3], step, delta, ua);
-        const R xo = xs[j * WG + tid];
-        const R xn = xo + delta;
-        R g[KMAX];
+        else {
+            uint32_t o[4];
+            philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
+            const float uc = (float)(o[0] >> 8) * 0x1p-24f;
+            int kc = 0;
+            for (int k = 0; k < Kq - 1; ++k) { kc += uc >= cwq[k] ? 1 : 0; }
+            const float *row = Qs + kc * (1 + 3 * d);
+            const float lq_old = indep_logq_lds(Qs, Kq, d, xs + tid, WGM);
+            Normals N;
+            N.avail = 0;
+            N.has_spare = false;
+            N.call = 1;
+            float xn[DMAX];
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) { g[k] = -vs[k * d + j]; }
-        for (int i = 0; i < d; ++i) {
-            const R xi = xs[i * WG + tid];
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) { g[k] = fma(Ps[(k * d + j) * dp + i], xi, g[k]); }
-        }
-        R qn[KMAX], en[KMAX];
-        R mn = -__builtin_huge_val();
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            const R pjj = Ps[(k * d + j) * dp + j];
-            qn[k] = q[k] + fma(delta + delta, g[k], delta * delta * pjj);
-            en[k] = c[k] - (R)0.5 * qn[k];
-            mn = en[k] > mn ? en[k] : mn;
-        }
-        R Sn = 0;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) { Sn = Sn + rexp(en[k] - mn); }
-        const R E = rexp(mn - m);
-        const bool acc = ua * S < E * Sn;
-        if (acc) { xs[j * WG + tid] = xn; }
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) { q[k] = acc ? qn[k] : q[k]; }
-        m = acc ? mn : m;
-        S = acc ? Sn : S;
-        sq = acc ? sq + fma((double)xn, (double)xn, -((double)xo * (double)xo)) : sq;
-        nacc += acc ? 1 : 0;
-        if (St.B > 0) {
-            const int bprev = bi;
-            accumulate(St, w, live, sq, xs[tid], s0, c0, s1, c1, jb, bi);
-            if (bi != bprev) { // block end: recompute q_k, m, S, |x|^2 from x
-                double q2 = 0.;
-                for (int i = 0; i < d; ++i) {
-                    const double v = (double)xs[i * WG + tid];
-                    q2 = fma(v, v, q2);
+            for (int j = 0; j < DMAX; ++j) {
+                xn[j] = 0.f;
+                if (j < d) {
+                    const float z = next_normal(N, 0x10000000u, (uint32_t)wl, tl, th, k0, k1);
+                    xn[j] = fmaf(row[1 + d + j], z, row[1 + j]);
                 }
-                sq = q2;
-                m = -__builtin_huge_val();
-                R e2[KMAX];
+            }
+            const float lq_new = indep_logq(Qs, Kq, d, xn);
+            float qn[KMAX], mn, Sn;
+            target_eval_reg(TL, Us, xn, qn, mn, Sn);
+            const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
+            const bool acc = __logf(ua) < (mn + __logf(Sn)) - (L.m + __logf(L.S)) + (lq_old - lq_new);
+            if (acc) {
+                double sq = 0.;
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k) {
-                    const R *mu = T.mu + k * d;
-                    R qk = 0;
-                    for (int i = 0; i < d; ++i) {
-                        R row = 0;
-                        for (int jj = 0; jj < d; ++jj) { row = fma(Ps[(k * d + i) * dp + jj], xs[jj * WG + tid] - mu[jj], row); }
-                        qk = fma(xs[i * WG + tid] - mu[i], row, qk);
+                for (int j = 0; j < DMAX; ++j) {
+                    if (j < d) {
+                        xs[j * WGM + tid] = xn[j];
+                        sq = fma((double)xn[j], (double)xn[j], sq);
                     }
-                    q[k] = qk;
-                    e2[k] = c[k] - (R)0.5 * qk;
-                    m = e2[k] > m ? e2[k] : m;
                 }
-                S = 0;
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k) { S = S + rexp(e2[k] - m); }
+                for (int k = 0; k < KMAX; ++k) { L.q[k] = qn[k]; }
+                L.m = mn;
+                L.S = Sn;
+                L.sq = sq;
             }
+            L.nacc_ind += acc ? 1 : 0;
         }
+        after_step<float, WGM>(T, St, Ps, xs, tid, w, live, L);
     }
-    if (!live) { return; }
-    for (int i = 0; i < d; ++i) { St.x[(int64_t)i * St.W + w] = xs[i * WG + tid]; }
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * St.W + w] = q[k]; }
-    St.m[w] = m;
-    St.S[w] = S;
-    St.sq[w] = sq;
-    St.sb[w] = s0;
-    St.cb[w] = c0;
-    St.sb[St.W + w] = s1;
-    St.cb[St.W + w] = c1;
-    St.jb[w] = jb;
-    St.bi[w] = bi;
-    St.nacc[w] += nacc;
+    if (live) {
+        lane_store<float, WGM>(St, d, w, xs, tid, L);
+        nacc_ind[w] += L.nacc_ind;
+    }
 }
 
 // log q(y | params) of the network's mixture, params in column w of out [rows][W]
@@ -343,13 +554,17 @@ __device__ float mix_logq(const float *out, int64_t W, int64_t w, const float *y
 {
     constexpr float HALF_LOG_2PI = 0.91893853320467274f;
     float amax = -__builtin_huge_valf();
-    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, out[k * W + w]); }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { amax = (k < K) ? fmaxf(amax, out[k * W + w]) : amax; }
     float sa = 0.f;
-    for (int k = 0; k < K; ++k) { sa += __expf(out[k * W + w] - amax); }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { sa += (k < K) ? __expf(out[k * W + w] - amax) : 0.f; }
     const float lsa = amax + __logf(sa);
     float comp[KMAX];
     float cmax = -__builtin_huge_valf();
-    for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k >= K) { comp[k] = -__builtin_huge_valf(); continue; }
         float c = out[k * W + w] - lsa;
         for (int j = 0; j < d; ++j) {
             const float mu = mcim::MU_SCALE * out[(int64_t)(K + k * d + j) * W + w];
@@ -361,7 +576,8 @@ __device__ float mix_logq(const float *out, int64_t W, int64_t w, const float *y
         cmax = fmaxf(cmax, c);
     }
     float sc = 0.f;
-    for (int k = 0; k < K; ++k) { sc += __expf(comp[k] - cmax); }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { sc += (k < K) ? __expf(comp[k] - cmax) : 0.f; }
     return cmax + __logf(sc);
 }
 
@@ -454,8 +670,8 @@ __global__ void k_lp_snapshot(State<R> St, int d, float *dst)
     if (e < (int64_t)d * St.W) { dst[e] = (float)St.x[e]; }
 }
 
-__global__ void k_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float *ctx,
-                           float *tgt)
+__global__ void k_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float same_frac,
+                           float *ctx, float *tgt)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= B) { return; }
@@ -464,8 +680,8 @@ __global__ void k_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B
     const int s1 = (int)(o[0] % (uint32_t)nsnap);
     const int64_t w1 = (int64_t)(o[1] % (uint64_t)W);
     int s2 = (int)(o[2] % (uint32_t)nsnap);
-    const bool same = (o[3] & 1u) != 0;
-    const int64_t w2 = same ? w1 : (int64_t)((o[3] >> 1) % (uint64_t)W);
+    const bool same = (float)(o[3] >> 24) * 0x1p-8f < same_frac;
+    const int64_t w2 = same ? w1 : (int64_t)((o[3] & 0xffffffu) % (uint64_t)W);
     if (same && nsnap > 1 && s2 == s1) { s2 = (s1 + 1) % nsnap; }
     const float *a = snap + (int64_t)s1 * d * W;
     const float *b = snap + (int64_t)s2 * d * W;
@@ -496,6 +712,23 @@ hipError_t launch_lp_rw(const Target<R> &T, const State<R> &S, R step, uint64_t 
     return hipGetLastError();
 }
 
+size_t mix_lds_bytes(int d, int Kq)
+{
+    return sizeof(float) * ((size_t)KMAX * d * (d + 1) + (size_t)KMAX * d + (size_t)KMAX * d * d + (size_t)Kq * (1 + 3 * d) + Kq +
+                            (size_t)d * WGM);
+}
+
+hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, const float *Q, int Kq, float step, uint64_t t0, int64_t nsteps,
+                               float pmix, uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
+{
+    const size_t lds = mix_lds_bytes(T.d, Kq);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_mix_indep), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL(k_lp_mix_indep, dim3((unsigned)((S.W + WGM - 1) / WGM)), dim3(WGM), lds, s, T, S, Q, Kq, step, t0, nsteps, pmix, k0, k1,
+                       nacc_ind);
+    return hipGetLastError();
+}
+
 hipError_t launch_lp_propose(const Target<float> &T, const State<float> &S, const float *out, int rows, int K, uint64_t t, uint32_t k0,
                              uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f, hipStream_t s)
 {
@@ -520,10 +753,10 @@ hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float *ctx,
-                           float *tgt, hipStream_t s)
+hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float same_frac,
+                           float *ctx, float *tgt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lp_pairs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, snap, nsnap, d, W, B, iter, k0, k1, ctx, tgt);
+    hipLaunchKernelGGL(k_lp_pairs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, snap, nsnap, d, W, B, iter, k0, k1, same_frac, ctx, tgt);
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@ namespace mcil
 {
 constexpr int WG = 256;   // walkers per workgroup
 constexpr int KMAX = 8;   // target components (fewer are padded with inert ones)
+constexpr int DMAX = 32;  // dimensions
 
 template <class R>
 struct Target
@@ -19,6 +20,8 @@ struct Target
     int d;
     const R *mu;        // [KMAX][d]
     const R *P;         // [KMAX][d][d] precision matrices
+    const R *U;         // [KMAX][d][d] upper Cholesky factors, P_k = U_k^T U_k
+    const R *Um;        // [KMAX][d] U_k mu_k
     const R *v;         // [KMAX][d] P_k mu_k
     const R *c;         // [KMAX] log w_k - log det(2 pi Sigma_k)/2 (-inf for padding)
     const float *L;     // [KMAX][d][d] Cholesky factor of Sigma_k (exact start)
@@ -55,10 +58,17 @@ hipError_t launch_lp_propose(const Target<float> &T, const State<float> &S, cons
 hipError_t launch_lp_accept(const Target<float> &T, const State<float> &S, float *out, const float *outn, int rows, int K,
                             uint64_t t, uint32_t k0, uint32_t k1, const float *xp, const float *logq_f, int32_t *nacc_ind,
                             hipStream_t s);
+// a whole run of the chain mixing random-walk moves and the independence proposal Q
+// ([Kq][1 + 3 d] rows: log pi_k - sum log sigma_k, mu_k, sigma_k, 1/sigma_k; then Kq cumulative
+// weights), FP32
+size_t mix_lds_bytes(int d, int Kq);
+hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, const float *Q, int Kq, float step, uint64_t t0, int64_t nsteps,
+                               float pmix, uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s);
 // x [d][W] -> dst [d][W] FP32 (snapshots)
 template <class R>
 hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t s);
-// training pairs: column p of ctx/tgt [d][B] from snapshots [nsnap][d][W]
+// training pairs: column p of ctx/tgt [d][B] from snapshots [nsnap][d][W]; with probability
+// same_frac the target is the same walker at another snapshot, else another walker (24-bit index)
 hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1,
-                           float *ctx, float *tgt, hipStream_t s);
+                           float same_frac, float *ctx, float *tgt, hipStream_t s);
 } // namespace mcil

@@ -260,13 +260,13 @@ struct mci_lp
     int64_t W = 0;
     uint64_t seed = 1;
     int H = 128, Kq = 8;
-    std::vector<double> mu, P, v, c, L; // padded to KMAX components
+    std::vector<double> mu, P, U, Um, v, c, L; // padded to KMAX components
     std::vector<double> cw;
     bool dev = false;
     hipStream_t stream = nullptr;
     // target on the device
-    DBuf<double> mu64, P64, v64, c64;
-    DBuf<float> mu32, P32, v32, c32, Lf, muf, cwf;
+    DBuf<double> mu64, P64, U64, Um64, v64, c64;
+    DBuf<float> mu32, P32, U32, Um32, v32, c32, Lf, muf, cwf;
     // chain state (one precision at a time)
     DBuf<double> x64, q64, m64, S64, sb64, cb64;
     DBuf<float> x32, q32, m32, S32, sb32, cb32;
@@ -278,6 +278,9 @@ struct mci_lp
     DBuf<float> out, outn, xp, logq_f, snap;
     DBuf<bf16_t> xpb;
     int nsnap = 0;
+    double same_frac = 0.5;
+    std::vector<float> ctx0;   // mean of the last snapshot: the network input of the independence proposal
+    DBuf<float> qpar;          // independence proposal [Kq][1 + 3 d] + cumulative weights [Kq]
     uint64_t t_next = 0;
     double step64 = 0., step32 = 0.;
     double results[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -323,6 +326,10 @@ mci_status lp_device(mci_lp *lp)
     HIPCHK(hipStreamCreateWithFlags(&lp->stream, hipStreamNonBlocking));
     TRY(up(lp->mu64, lp->mu));
     TRY(up(lp->P64, lp->P));
+    TRY(up(lp->U64, lp->U));
+    TRY(up(lp->U32, to_f(lp->U)));
+    TRY(up(lp->Um64, lp->Um));
+    TRY(up(lp->Um32, to_f(lp->Um)));
     TRY(up(lp->v64, lp->v));
     TRY(up(lp->c64, lp->c));
     std::vector<float> cf(lp->c.begin(), lp->c.end());
@@ -348,12 +355,12 @@ mcil::Target<R> target_of(mci_lp *lp);
 template <>
 mcil::Target<double> target_of<double>(mci_lp *lp)
 {
-    return {lp->d, lp->mu64.p, lp->P64.p, lp->v64.p, lp->c64.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
+    return {lp->d, lp->mu64.p, lp->P64.p, lp->U64.p, lp->Um64.p, lp->v64.p, lp->c64.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
 }
 template <>
 mcil::Target<float> target_of<float>(mci_lp *lp)
 {
-    return {lp->d, lp->mu32.p, lp->P32.p, lp->v32.p, lp->c32.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
+    return {lp->d, lp->mu32.p, lp->P32.p, lp->U32.p, lp->Um32.p, lp->v32.p, lp->c32.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
 }
 
 template <class R>
@@ -571,6 +578,100 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
     lp->results[7] = nind > 0 ? ai / ((double)lp->W * (double)nind) : 0.;
     return MCI_OK;
 }
+// the independence proposal: the network's mixture at the fixed input ctx0, as
+// [Kq][log pi_k - sum_j log sigma_kj, mu_k, sigma_k, 1/sigma_k] and cumulative weights
+mci_status indep_params(mci_lp *lp)
+{
+    if (lp->ctx0.empty()) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect and mci_lp_train first"); }
+    TRY(net_ready(lp));
+    mci_mlp *n = lp->net;
+    const int d = lp->d, K = n->K, ns = 64;
+    TRY(mlp_reserve(n, ns));
+    std::vector<float> x((size_t)d * ns);
+    for (int j = 0; j < d; ++j) {
+        for (int r = 0; r < ns; ++r) { x[(size_t)j * ns + r] = lp->ctx0[j]; }
+    }
+    HIPCHK(hipMemcpy(n->X.p, x.data(), x.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(mcim::launch_to_bf16_scaled(n->X.p, n->Xb.p, (int64_t)d * ns, mcim::IN_SCALE, n->stream));
+    TRY(mlp_forward(n, ns, n->Xb.p, n->OUT.p));
+    std::vector<float> o((size_t)n->Op * ns);
+    HIPCHK(hipMemcpyAsync(o.data(), n->OUT.p, o.size() * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    auto col = [&](int c) { return (double)o[(size_t)c * ns]; };
+    double amax = -HUGE_VAL;
+    for (int k = 0; k < K; ++k) { amax = std::fmax(amax, col(k)); }
+    double sa = 0.;
+    for (int k = 0; k < K; ++k) { sa += std::exp(col(k) - amax); }
+    const double lsa = amax + std::log(sa);
+    std::vector<float> q((size_t)K * (1 + 3 * d) + K);
+    double cum = 0.;
+    for (int k = 0; k < K; ++k) {
+        float *row = q.data() + (size_t)k * (1 + 3 * d);
+        double lsum = 0.;
+        for (int j = 0; j < d; ++j) {
+            const double mu = mcim::MU_SCALE * col(K + k * d + j);
+            const double ls = mcim::LS_MAX * std::tanh(col(K + K * d + k * d + j) / mcim::LS_MAX);
+            row[1 + j] = (float)mu;
+            row[1 + d + j] = (float)std::exp(ls);
+            row[1 + 2 * d + j] = (float)std::exp(-ls);
+            lsum += ls;
+        }
+        const double lp_k = col(k) - lsa;
+        row[0] = (float)(lp_k - lsum);
+        cum += std::exp(lp_k);
+        q[(size_t)K * (1 + 3 * d) + k] = (k == K - 1) ? 1.f : (float)cum;
+    }
+    TRY(up(lp->qpar, q));
+    return MCI_OK;
+}
+
+mci_status run_mixed_indep(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
+{
+    mcil::State<float> S;
+    const int B = (int)(nsteps / nblocks);
+    TRY(state_of<float>(lp, B, nblocks, S));
+    TRY(indep_params(lp));
+    const mcil::Target<float> T = target_of<float>(lp);
+    const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
+    const size_t W = (size_t)lp->W;
+    auto t0 = clk::now();
+    HIPCHK(mcil::launch_lp_init<float>(T, S, k0, k1, lp->stream));
+    if (lp->step32 <= 0.) { lp->step32 = 1.0; }
+    TRY(calibrate<float>(lp, S, lp->step32));
+    lp->timings[0] = ms_since(t0);
+    HIPCHK(hipMemsetAsync(lp->nacc.p, 0, sizeof(int32_t) * W, lp->stream));
+    HIPCHK(hipMemsetAsync(lp->nacc_ind.p, 0, sizeof(int32_t) * W, lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    t0 = clk::now();
+    HIPCHK(mcil::launch_lp_mix_indep(T, S, lp->qpar.p, lp->net->K, (float)lp->step32, lp->t_next, nsteps, (float)pmix, k0, k1, lp->nacc_ind.p,
+                                     lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    lp->timings[1] = ms_since(t0);
+    // learned steps of this run (the common word of each step)
+    int64_t nind = 0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        uint32_t o[4];
+        const uint64_t t = lp->t_next + (uint64_t)s;
+        philox_host(0x20000000u, 0xffffffffu, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
+        nind += ((float)(o[0] >> 8) * 0x1p-24f < (float)pmix) ? 1 : 0;
+    }
+    lp->t_next += (uint64_t)nsteps;
+    t0 = clk::now();
+    TRY(estimate(lp, nblocks));
+    lp->timings[2] = ms_since(t0);
+    std::vector<int32_t> acc(W), acci(W);
+    HIPCHK(hipMemcpy(acc.data(), lp->nacc.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(acci.data(), lp->nacc_ind.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
+    double a = 0., ai = 0.;
+    for (size_t w = 0; w < W; ++w) {
+        a += acc[w];
+        ai += acci[w];
+    }
+    const int64_t nrw = nsteps - nind;
+    lp->results[6] = nrw > 0 ? a / ((double)lp->W * (double)nrw) : 0.;
+    lp->results[7] = nind > 0 ? ai / ((double)lp->W * (double)nind) : 0.;
+    return MCI_OK;
+}
 } // namespace
 
 extern "C" {
@@ -671,6 +772,8 @@ mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *
     const int KM = mcil::KMAX;
     lp->mu.assign((size_t)KM * d, 0.);
     lp->P.assign((size_t)KM * d * d, 0.);
+    lp->U.assign((size_t)KM * d * d, 0.);
+    lp->Um.assign((size_t)KM * d, 0.);
     lp->v.assign((size_t)KM * d, 0.);
     lp->L.assign((size_t)KM * d * d, 0.);
     lp->c.assign(KM, -HUGE_VAL);
@@ -727,6 +830,27 @@ mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *
                 Pk[j * d + i] = a;
             }
         }
+        // U = (Cholesky factor of P)^T, upper: P = U^T U
+        double *Uk = lp->U.data() + (size_t)k * d * d;
+        {
+            std::vector<double> Lp((size_t)d * d, 0.);
+            for (int i = 0; i < d; ++i) {
+                for (int j = 0; j <= i; ++j) {
+                    double s2 = Pk[i * d + j];
+                    for (int q = 0; q < j; ++q) { s2 -= Lp[i * d + q] * Lp[j * d + q]; }
+                    if (i == j) { Lp[i * d + i] = std::sqrt(std::fmax(s2, 1e-300)); }
+                    else { Lp[i * d + j] = s2 / Lp[j * d + j]; }
+                }
+            }
+            for (int i = 0; i < d; ++i) {
+                for (int j = i; j < d; ++j) { Uk[i * d + j] = Lp[j * d + i]; }
+            }
+            for (int i = 0; i < d; ++i) {
+                double sm = 0.;
+                for (int j = i; j < d; ++j) { sm += Uk[i * d + j] * means[(size_t)k * d + j]; }
+                lp->Um[(size_t)k * d + i] = sm;
+            }
+        }
         double logdet = 0.;
         for (int i = 0; i < d; ++i) { logdet += 2.0 * std::log(Lk[i * d + i]); }
         const double wn = weights[k] / ws;
@@ -774,12 +898,13 @@ mci_status mci_lp_run(mci_lp *lp, int mode, int64_t nsteps, int nblocks, double 
     if (nblocks < 2 || (nblocks & (nblocks - 1)) != 0 || nblocks > 64 || nsteps < nblocks || nsteps % nblocks != 0) {
         return fail(MCI_E_INVALID_ARGUMENT, "nsteps must be a multiple of nblocks, a power of two in 2..64");
     }
-    if (mode == MCI_LP_MIXED_FP32 && !(pmix >= 0. && pmix <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "mixing probability must be in [0, 1]"); }
+    if ((mode == MCI_LP_MIXED_FP32 || mode == MCI_LP_INDEP_FP32) && !(pmix >= 0. && pmix <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "mixing probability must be in [0, 1]"); }
     TRY(lp_device(lp));
     switch (mode) {
     case MCI_LP_PLAIN_FP64: TRY(run_plain<double>(lp, nsteps, nblocks)); break;
     case MCI_LP_PLAIN_FP32: TRY(run_plain<float>(lp, nsteps, nblocks)); break;
     case MCI_LP_MIXED_FP32: TRY(run_mixed(lp, nsteps, nblocks, pmix)); break;
+    case MCI_LP_INDEP_FP32: TRY(run_mixed_indep(lp, nsteps, nblocks, pmix)); break;
     default: return fail(MCI_E_DOMAIN, "unknown mode %d", mode);
     }
     if (avg) { avg[0] = lp->results[0], avg[1] = lp->results[1]; }
@@ -807,6 +932,15 @@ mci_status mci_lp_collect(mci_lp *lp, int nsnap, int64_t lag)
     }
     HIPCHK(hipStreamSynchronize(lp->stream));
     lp->nsnap = nsnap;
+    // mean position of the last snapshot
+    std::vector<float> last((size_t)lp->d * lp->W);
+    HIPCHK(hipMemcpy(last.data(), lp->snap.p + (size_t)(nsnap - 1) * lp->d * lp->W, last.size() * sizeof(float), hipMemcpyDeviceToHost));
+    lp->ctx0.assign(lp->d, 0.f);
+    for (int j = 0; j < lp->d; ++j) {
+        double sm = 0.;
+        for (int64_t w = 0; w < lp->W; ++w) { sm += last[(size_t)j * lp->W + w]; }
+        lp->ctx0[j] = (float)(sm / (double)lp->W);
+    }
     return MCI_OK;
 }
 
@@ -830,7 +964,8 @@ mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *los
         return MCI_OK;
     };
     for (int it = 0; it < iters; ++it) {
-        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, (uint64_t)it, k0, k1, n->X.p, n->Y.p, n->stream));
+        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, (uint64_t)it, k0, k1, (float)lp->same_frac, n->X.p, n->Y.p,
+                                     n->stream));
         TRY(mlp_fwd_bwd(n, batch));
         if (it == 0) { TRY(mean_loss(lp->loss_first)); }
         TRY(mlp_adam(n, (float)lr));
@@ -839,6 +974,14 @@ mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *los
     lp->timings[3] = ms_since(t0);
     if (loss_first) { *loss_first = lp->loss_first; }
     if (loss_last) { *loss_last = lp->loss_last; }
+    return MCI_OK;
+}
+
+mci_status mci_lp_set_training_pairs(mci_lp *lp, double same_fraction)
+{
+    CHECKL(lp);
+    if (!(same_fraction >= 0. && same_fraction <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "fraction must be in [0, 1]"); }
+    lp->same_frac = same_fraction;
     return MCI_OK;
 }
 

@@ -15,7 +15,7 @@ from .mci import _check, _lib as _mci_lib
 _P, _I, _I64, _U64, _D = C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_double
 _DP, _FP, _IP = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
 
-PLAIN_FP64, PLAIN_FP32, MIXED_FP32 = 0, 1, 2
+PLAIN_FP64, PLAIN_FP32, MIXED_FP32, INDEP_FP32 = 0, 1, 2, 3
 
 _SIGS = {
     "mci_mlp_create": [_I, _I, _I, _U64, C.POINTER(_P)], "mci_mlp_destroy": [_P],
@@ -27,7 +27,7 @@ _SIGS = {
     "mci_lp_run": [_P, _I, _I64, _I, _D, _DP, _DP], "mci_lp_collect": [_P, _I, _I64],
     "mci_lp_train": [_P, _I, _I, _D, _DP, _DP], "mci_lp_get_results": [_P, _DP, _I],
     "mci_lp_get_timings": [_P, _DP, _I], "mci_lp_get_walker_means": [_P, _DP, _I64],
-    "mci_lp_get_network": [_P, C.POINTER(_P)],
+    "mci_lp_get_network": [_P, C.POINTER(_P)], "mci_lp_set_training_pairs": [_P, _D],
 }
 _bound = None
 
@@ -129,6 +129,9 @@ class LearnedProposalSampler:
 
     def collect(self, nsnap, lag):
         _check(_lib().mci_lp_collect(self._h, nsnap, lag))
+
+    def setTrainingPairs(self, same_fraction):
+        _check(_lib().mci_lp_set_training_pairs(self._h, same_fraction))
 
     def train(self, iters, batch=4096, lr=1e-3):
         l0, l1 = C.c_double(), C.c_double()

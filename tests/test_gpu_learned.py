@@ -72,7 +72,7 @@ def test_sampler_plain_and_untrained_mixed_are_unbiased():
 
 
 def test_trained_network_reduces_the_error_bar():
-    from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP32, LearnedProposalSampler
+    from mcintegratorplusplus_amd.learned import INDEP_FP32, MIXED_FP32, PLAIN_FP32, LearnedProposalSampler
     w, mu, cov, exact = _small_target()
     s = LearnedProposalSampler(w, mu, cov, 8192)
     s.setSeed(5)
@@ -89,3 +89,13 @@ def test_trained_network_reduces_the_error_bar():
     assert np.all(np.abs(_z(a_m, e_m, exact)) < 4.5)
     assert r["acc_learned"] > 0.05
     assert e_m[0] < 0.7 * e_p[0]
+    # independence proposal from the network trained on independent pairs
+    s.setTrainingPairs(0.0)
+    s.collect(8, 256)
+    s.train(400, 2048, 2e-3)
+    a_i, e_i = s.run(INDEP_FP32, 2048, 32, 0.5)
+    r = s.results()
+    print("independence", a_i, e_i, "learned acceptance", r["acc_learned"], s.timings(), flush=True)
+    assert np.all(np.abs(_z(a_i, e_i, exact)) < 4.5)
+    assert r["acc_learned"] > 0.005
+    assert e_i[0] < 0.7 * e_p[0]
