@@ -114,6 +114,101 @@ def profile_summary():
         return json.load(f)
 
 
+def run_precision_form():
+    """d = 64, 262,144 walkers x 10,240 steps (x_i^2 and exp(-|x|^2/64) in 64 blocks of 160,
+    MJBlocker, shared calibration, 1,000 decorrelation steps): the FP32 chain (Kahan block sums)
+    against the FP64 chain on the same Philox words and against the analytic values."""
+    from mcintegratorplusplus_amd import MCI, constants as K
+    with open(os.path.join(ROOT, "tests", "golden", "sigmas.json")) as f:
+        sig = np.array([float.fromhex(v) for v in json.load(f)["sigma64"]])
+    Wp, D64 = 262144, 64
+    exact = np.concatenate([sig ** 2, [np.prod((1 + 2 * sig ** 2 / D64) ** -0.5)]])
+    out = {}
+    for prec in (0, 1):
+        m = MCI(D64, Wp)
+        m.setSeed(4242)
+        m.setRandomMode(K.RNG_PHILOX)
+        m.setPrecision(prec)
+        m.setWalkerCoupling(K.COUPLING_SHARED)
+        m.setTrialMove(K.SRRD_UNIFORM, 1, 1)
+        m.setNdecorrelationSteps(NDECORR)
+        m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+        m.addObservable(K.OBS_X2, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+        m.addObservable(K.OBS_EXP_X2, (float(D64),), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+        m.integrate(NMC)
+        t0 = time.perf_counter()
+        avg, err = m.integrate(NMC)
+        ms = 1000 * (time.perf_counter() - t0)
+        out[prec] = (avg, err, ms, m.getTimings())
+    (a64, e64, ms64, kt64), (a32, e32, ms32, kt32) = out[0], out[1]
+    return {"workload": "d=64 anisotropic Gaussian, 262144 walkers x 10240 steps, x_i^2 + exp(-|x|^2/64)",
+            "fp64": {"ms": ms64, "walker_steps_per_s": Wp * NMC / (ms64 / 1000), "sampling_kernel_ms": kt64[2],
+                     "max_abs_z": float(np.max(np.abs(a64 - exact) / e64))},
+            "fp32": {"ms": ms32, "walker_steps_per_s": Wp * NMC / (ms32 / 1000), "sampling_kernel_ms": kt32[2],
+                     "max_abs_z": float(np.max(np.abs(a32 - exact) / e32))},
+            "fp32_vs_fp64_max_rel_diff": float(np.max(np.abs(a32 - a64) / np.abs(a64))),
+            "fp32_vs_fp64_max_diff_over_err": float(np.max(np.abs(a32 - a64) / e64))}
+
+
+def run_batched(reps=5):
+    """4,096 independent Gaussian-mixture integrals (d log-uniform in [1, 128], nmc log-uniform in
+    [1e3, 1e5], 1-4 isotropic components), |x|^2, FP32 walkers, one hipGraph replay per batch."""
+    from mcintegratorplusplus_amd import BatchIntegrals, mixture_workload
+    ndim, nmc, w, s, mu, exact = mixture_workload(n=4096, seed=4096)
+    b = BatchIntegrals(ndim, nmc, w, s, mu)
+    b.setSeed(1337)
+    b.autotune(3)
+    R, T, _ = b.plan()
+    b.setGraph(False)
+    b.run()
+    phases = b.timings()
+    b.setGraph(True)
+    b.run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        avg, err = b.run()
+    ms = 1000 * (time.perf_counter() - t0) / reps
+    steps = int(np.sum(R * T))
+    z = (avg - exact) / err
+    return {"workload": "4096 mixture integrals, d in [1,128], nmc in [1e3,1e5], |x|^2",
+            "ms_per_batch": ms, "walker_steps": steps, "walker_steps_per_s": steps / (ms / 1000),
+            "requested_steps": int(np.sum(nmc)), "walkers": int(np.sum(R)),
+            "phase_ms_eager": {"calibration": phases[0], "sampling": phases[1], "estimation": phases[2]},
+            "frac_abs_z_below_3": float(np.mean(np.abs(z) < 3)), "mean_z2": float(np.mean(z * z))}
+
+
+def run_learned():
+    """d = 32 mixture of 8 rotated anisotropic Gaussians, 65,536 walkers x 10,240 steps: the plain
+    FP64 random-walk chain against the FP32 chain mixing random-walk moves 50/50 with moves drawn
+    from the trained network (MH-corrected); ESS = Var_p(f) / err^2 with the exact variance."""
+    from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP64, LearnedProposalSampler, mixture_target
+    w, mu, cov, (ex2, pind) = mixture_target(d=32, ncomp=8, seed=5)
+    ex4 = float(np.sum(w * [(np.trace(c) + m @ m) ** 2 + 2 * np.trace(c @ c) + 4 * m @ c @ m for m, c in zip(mu, cov)]))
+    var = np.array([ex4 - ex2 ** 2, pind * (1 - pind)])
+    Wl = 65536
+    s = LearnedProposalSampler(w, mu, cov, Wl)
+    s.setSeed(1)
+    a_p, e_p = s.run(PLAIN_FP64, NMC, 64)
+    t_p = s.timings()
+    s.setTrainingPairs(0.0)
+    s.collect(16, 512)
+    l0, l1 = s.train(1500, 4096, 2e-3)
+    t_train = s.timings()["training"]
+    a_m, e_m = s.run(MIXED_FP32, NMC, 64, 0.5)
+    t_m = s.timings()
+    r = s.results()
+    ess_p = var / e_p ** 2 / ((t_p["sampling"] + t_p["calibration"]) / 1000)
+    ess_m = var / e_m ** 2 / ((t_m["sampling"] + t_m["calibration"]) / 1000)
+    exact = np.array([ex2, pind])
+    return {"workload": "d=32, 8 rotated anisotropic Gaussians, 65536 walkers x 10240 steps, |x|^2 and 1[x_0>0]",
+            "plain_fp64": {"sampling_ms": t_p["sampling"], "err": list(e_p), "ess_per_s": list(ess_p),
+                           "z": list((a_p - exact) / e_p)},
+            "learned_fp32": {"sampling_ms": t_m["sampling"], "err": list(e_m), "ess_per_s": list(ess_m),
+                             "z": list((a_m - exact) / e_m), "learned_acceptance": r["acc_learned"]},
+            "training_ms": t_train, "training_loss": [l0, l1],
+            "ess_per_s_gain": list(ess_m / ess_p), "ess_per_step_gain": list((e_p / e_m) ** 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +217,8 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
+    ap.add_argument("--workloads", default="all", choices=["all", "none"],
+                    help="also measure the precision form, the batched integrals and the learned proposal")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -192,6 +289,9 @@ def main():
                      "instructions_per_wave_step_rocprof": (prof.get("instructions_per_walker_step") or {}).get("SQ_INSTS"),
                      "hbm_gbps": (prof["bytes_per_launch"] / (kt[2] / 1000) / 1e9) if "bytes_per_launch" in prof else None},
     }
+    if args.workloads == "all" and args.precision == 0 and not args.generic:
+        line["other_workloads"] = {"precision_fp32": run_precision_form(), "batched": run_batched(),
+                                   "learned_proposal": run_learned()}
     if not args.no_cpu:
         cb = cpu_baseline()
         if cb is not None:
