@@ -283,8 +283,8 @@ mci_status mci_mlp_destroy(mci_mlp *m);
 mci_status mci_mlp_nparams(mci_mlp *m, int64_t *n);
 mci_status mci_mlp_get_params(mci_mlp *m, float *params, int64_t n);
 mci_status mci_mlp_set_params(mci_mlp *m, const float *params, int64_t n);
-/* raw outputs [n][K(1+2d)] for positions x [n][d]: K logits, K x d means / 4, K x d log-scale
- * pre-activations (log-scale = 3 tanh(o / 3)); the input is x / 4 */
+/* raw outputs [n][K(1+2d)] for positions x [n][d]: for each component k, d means / 4 then d
+ * log-scale pre-activations (log-scale = 3 tanh(o / 3)); then K logits.  The input is x / 4. */
 mci_status mci_mlp_forward(mci_mlp *m, int64_t n, const float *x, float *out);
 /* mean of -log q(y_r | x_r) over n pairs and its gradient with respect to the FP32 parameters */
 mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *x, const float *y, double *loss, float *grad);

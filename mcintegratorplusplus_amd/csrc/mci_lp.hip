@@ -549,117 +549,328 @@ __global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<flo
     }
 }
 
-// log q(y | params) of the network's mixture, params in column w of out [rows][W]
-__device__ float mix_logq(const float *out, int64_t W, int64_t w, const float *y, int K, int d)
+// ---- the learned move, fused: one launch per step for all walkers ----
+// A workgroup of 256 threads takes 64 walkers (column c = tid % 64, part = tid / 64).  The
+// network's mixture at the current x of every walker is kept in a BF16 cache [R][W] (rows per
+// component k: d means, d log-scales (already transformed), then K logits); the proposal q(. | x)
+// is defined by these rounded values.  A step:
+//  1. draw the component from the cached logits and x' = mu + exp(ls) z; log q(x' | x);
+//  2. the network at x' on the matrix cores (64 x 64 tiles of v_mfma_f32_32x32x16_bf16, weights
+//     read from L2 as [out][in] rows, activations walker-major in LDS), each 64-row chunk of the
+//     last layer consumed at once: rounded to BF16 into LDS and folded into log q(x | x');
+//  3. log p(x') (components split over the four parts), the MH test, and for accepted walkers the
+//     new position, target state and cached mixture.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int TW = 64;    // walkers per workgroup
+constexpr int TT = 256;   // threads per workgroup
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// rows r0 .. r0 + 63 of Wt [M][K] (BF16, row = output unit) times act [64 walkers][lda] (BF16, LDS):
+// wave (wm, wn) accumulates rows r0 + 32 wm .., walkers 32 wn ..
+__device__ __forceinline__ f32x16 mfma_chunk(const uint16_t *Wt, int K, int M, int r0, const uint16_t *act, int lda)
 {
-    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
-    float amax = -__builtin_huge_valf();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+    const int row = r0 + wm * 32 + (lane & 31);
+    // the weight fragments of the whole reduction first (K <= 128: at most 8), then the products
+    bf16x8 a[8];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) { amax = (k < K) ? fmaxf(amax, out[k * W + w]) : amax; }
-    float sa = 0.f;
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) { sa += (k < K) ? __expf(out[k * W + w] - amax) : 0.f; }
-    const float lsa = amax + __logf(sa);
-    float comp[KMAX];
-    float cmax = -__builtin_huge_valf();
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k >= K) { comp[k] = -__builtin_huge_valf(); continue; }
-        float c = out[k * W + w] - lsa;
-        for (int j = 0; j < d; ++j) {
-            const float mu = mcim::MU_SCALE * out[(int64_t)(K + k * d + j) * W + w];
-            const float ls = mcim::LS_MAX * tanhf(out[(int64_t)(K + K * d + k * d + j) * W + w] / mcim::LS_MAX);
-            const float z = (y[(int64_t)j * W + w] - mu) * __expf(-ls);
-            c += -ls - HALF_LOG_2PI - 0.5f * z * z;
-        }
-        comp[k] = c;
-        cmax = fmaxf(cmax, c);
+    for (int q = 0; q < 8; ++q) {
+        const int k = 16 * q + 8 * (lane >> 5);
+        a[q] = bf16x8{};
+        if (16 * q < K && row < M) { a[q] = *reinterpret_cast<const bf16x8 *>(Wt + (int64_t)row * K + k); }
     }
-    float sc = 0.f;
+    f32x16 acc = {};
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) { sc += (k < K) ? __expf(comp[k] - cmax) : 0.f; }
-    return cmax + __logf(sc);
+    for (int q = 0; q < 8; ++q) {
+        if (16 * q < K) {
+            const int k = 16 * q + 8 * (lane >> 5);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(act + (wn * 32 + (lane & 31)) * lda + k);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q], b, acc, 0, 0, 0);
+        }
+    }
+    return acc;
 }
 
-__global__ __launch_bounds__(WG) void k_lp_propose(Target<float> T, State<float> St, const float *out, int rows, int K, uint64_t t,
-                                                   uint32_t k0, uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f)
+// hidden layer: out[c][r] = bf16(silu(W act + b)) for rows 0 .. M-1
+__device__ void hidden_layer(const uint16_t *Wt, const float *bias, int K, int M, const uint16_t *act, int lda, uint16_t *out, int ldo)
 {
-    const int64_t w = (int64_t)blockIdx.x * WG + threadIdx.x;
-    if (w >= St.W) { return; }
-    const int d = T.d;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+    for (int r0 = 0; r0 < M; r0 += 64) {
+        const f32x16 acc = mfma_chunk(Wt, K, M, r0, act, lda);
+        const int col = wn * 32 + (lane & 31);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = r0 + wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+            if (row < M) {
+                const float v = acc[i] + bias[row];
+                out[col * ldo + row] = f2bf(v / (1.0f + __expf(-v)));
+            }
+        }
+    }
+}
+
+template <int MODE> // 0: fill the cache at the current positions, 1: one learned step
+__global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, State<float> St, uint16_t *cache, uint64_t t, uint32_t k0,
+                                                   uint32_t k1, int32_t *nacc_ind)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int d = N.d, K = N.K, H = N.H, R = N.O;
+    const int tid = threadIdx.x, col = tid & 63, part = tid >> 6;
     const int64_t W = St.W;
+    const int64_t w = (int64_t)blockIdx.x * TW + col;
+    const bool live = w < W;
+    const int64_t wl = live ? w : W - 1;
+    const int ldx = N.dp + 8, ldh = H + 8;
+    // LDS carve-up (byte offsets multiple of 16)
+    float *xs = reinterpret_cast<float *>(lds_raw);            // [DMAX][64] current x
+    float *xps = xs + DMAX * TW;                                // [DMAX][64] proposal
+    float *part_f = xps + DMAX * TW;                            // [KMAX][4][64] partial log-densities
+    float *part_r = part_f + KMAX * 4 * TW;                     // [KMAX][4][64]
+    float *qn = part_r + KMAX * 4 * TW;                         // [KMAX][64]
+    float *misc = qn + KMAX * TW;                               // [4][64]: log q(x'|x), lse of logits, ...
+    int *kc = reinterpret_cast<int *>(misc + 4 * TW);           // [64]
+    int *accf = kc + TW;                                        // [64]
+    uint16_t *Xb = reinterpret_cast<uint16_t *>(accf + TW);     // [64][ldx]
+    uint16_t *H1 = Xb + TW * ldx;                               // [64][ldh]
+    uint16_t *H2 = H1 + TW * ldh;                               // [64][ldh]  (before: normals; after: last-layer tile)
+    uint16_t *Pn = H2 + TW * ldh;                               // [R][64] mixture at x' (MODE 1)
+    float *zs = reinterpret_cast<float *>(H2);                  // [DMAX][64]
     const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
-    uint32_t o[4];
-    philox(0x10000000u, (uint32_t)w, tl, th, k0, k1, o);
-    float amax = -__builtin_huge_valf();
-    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, out[k * W + w]); }
-    float sa = 0.f;
-    for (int k = 0; k < K; ++k) { sa += __expf(out[k * W + w] - amax); }
-    const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
-    int kc = K - 1;
-    float cum = 0.f;
-    for (int k = 0; k < K - 1; ++k) {
-        cum += __expf(out[k * W + w] - amax);
-        if (uc < cum) { kc = k; break; }
-    }
-    Normals N;
-    N.avail = 0;
-    N.has_spare = false;
-    N.call = 1;
-    for (int j = 0; j < d; ++j) {
-        const float z = next_normal(N, 0x10000000u, (uint32_t)w, tl, th, k0, k1);
-        const float mu = mcim::MU_SCALE * out[(int64_t)(K + kc * d + j) * W + w];
-        const float ls = mcim::LS_MAX * tanhf(out[(int64_t)(K + K * d + kc * d + j) * W + w] / mcim::LS_MAX);
-        const float v = fmaf(__expf(ls), z, mu);
-        xp[(int64_t)j * W + w] = v;
-        xpb[(int64_t)j * W + w] = f2bf(v * in_scale);
-    }
-    logq_f[w] = mix_logq(out, W, w, xp, K, d);
-    (void)rows;
-}
+    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
 
-__global__ __launch_bounds__(WG) void k_lp_accept(Target<float> T, State<float> St, float *out, const float *outn, int rows, int K,
-                                                  uint64_t t, uint32_t k0, uint32_t k1, const float *xp, const float *logq_f,
-                                                  int32_t *nacc_ind)
-{
-    const int64_t w = (int64_t)blockIdx.x * WG + threadIdx.x;
-    if (w >= St.W) { return; }
-    const int d = T.d;
-    const int64_t W = St.W;
-    const float logq_r = mix_logq(outn, W, w, St.x, K, d);
-    float qn[KMAX], mn, Sn;
-    target_eval(T, xp + w, W, qn, mn, Sn);
-    const float logp_new = mn + __logf(Sn);
-    const float logp_old = St.m[w] + __logf(St.S[w]);
-    uint32_t o[4];
-    philox(0x10000000u, (uint32_t)w, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
-    const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
-    const bool acc = __logf(ua) < (logp_new - logp_old) + (logq_r - logq_f[w]);
-    if (acc) {
-        double sq = 0.;
-        for (int j = 0; j < d; ++j) {
-            const float v = xp[(int64_t)j * W + w];
-            St.x[(int64_t)j * W + w] = v;
-            sq = fma((double)v, (double)v, sq);
-        }
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * W + w] = qn[k]; }
-        St.m[w] = mn;
-        St.S[w] = Sn;
-        St.sq[w] = sq;
-        for (int r = 0; r < rows; ++r) { out[(int64_t)r * W + w] = outn[(int64_t)r * W + w]; }
-        nacc_ind[w] += 1;
+    // current positions
+    for (int j = part; j < DMAX; j += 4) { xs[j * TW + col] = (j < d) ? St.x[(int64_t)j * W + wl] : 0.f; }
+    if (MODE == 0) {
+        for (int j = part; j < N.dp; j += 4) { Xb[col * ldx + j] = f2bf(j < d ? xs[j * TW + col] * mcim::IN_SCALE : 0.f); }
     }
-    if (St.B > 0) {
-        float s0 = St.sb[w], c0 = St.cb[w], s1 = St.sb[W + w], c1 = St.cb[W + w];
-        int jb = St.jb[w], bi = St.bi[w];
-        accumulate(St, w, true, St.sq[w], St.x[w], s0, c0, s1, c1, jb, bi);
-        St.sb[w] = s0;
-        St.cb[w] = c0;
-        St.sb[W + w] = s1;
-        St.cb[W + w] = c1;
-        St.jb[w] = jb;
-        St.bi[w] = bi;
+    __syncthreads();
+    if (MODE == 1) {
+        // 1. component and proposal from the cached mixture at x
+        if (part == 0) {
+            float amax = -__builtin_huge_valf();
+            for (int k = 0; k < K; ++k) { amax = fmaxf(amax, bf2f(cache[(int64_t)(2 * d * K + k) * W + wl])); }
+            float sa = 0.f;
+            for (int k = 0; k < K; ++k) { sa += __expf(bf2f(cache[(int64_t)(2 * d * K + k) * W + wl]) - amax); }
+            uint32_t o[4];
+            philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
+            const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
+            int kk = K - 1;
+            float cum = 0.f;
+            for (int k = 0; k < K - 1; ++k) {
+                cum += __expf(bf2f(cache[(int64_t)(2 * d * K + k) * W + wl]) - amax);
+                if (uc < cum) { kk = k; break; }
+            }
+            kc[col] = kk;
+            misc[1 * TW + col] = amax + __logf(sa); // lse of the logits at x
+        }
+        {
+            Normals Nm;
+            Nm.avail = 0;
+            Nm.has_spare = false;
+            Nm.call = 1;
+            const uint32_t base = 0x10000000u + ((uint32_t)(part + 1) << 20);
+            for (int j = part; j < d; j += 4) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, tl, th, k0, k1); }
+        }
+        __syncthreads();
+        {
+            const int kk = kc[col];
+            for (int j = part; j < DMAX; j += 4) {
+                float v = 0.f;
+                if (j < d) {
+                    const float mu = bf2f(cache[(int64_t)(2 * d * kk + j) * W + wl]);
+                    const float ls = bf2f(cache[(int64_t)(2 * d * kk + d + j) * W + wl]);
+                    v = fmaf(__expf(ls), zs[j * TW + col], mu);
+                }
+                xps[j * TW + col] = v;
+                if (j < N.dp) { Xb[col * ldx + j] = f2bf(v * mcim::IN_SCALE); }
+            }
+        }
+        __syncthreads();
+        // log q(x' | x): part p sums coordinates j = p, p + 4, ...
+        for (int k = 0; k < K; ++k) {
+            float acc = 0.f;
+            for (int j = part; j < d; j += 4) {
+                const float mu = bf2f(cache[(int64_t)(2 * d * k + j) * W + wl]);
+                const float ls = bf2f(cache[(int64_t)(2 * d * k + d + j) * W + wl]);
+                const float z = (xps[j * TW + col] - mu) * __expf(-ls);
+                acc += -ls - 0.5f * z * z;
+            }
+            part_f[(k * 4 + part) * TW + col] = acc;
+        }
+        __syncthreads();
+        if (part == 0) {
+            const float lsa = misc[1 * TW + col];
+            float comp[KMAX];
+            float cmax = -__builtin_huge_valf();
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                comp[k] = -__builtin_huge_valf();
+                if (k < K) {
+                    const float sm = ((part_f[(k * 4 + 0) * TW + col] + part_f[(k * 4 + 1) * TW + col]) + part_f[(k * 4 + 2) * TW + col]) +
+                                     part_f[(k * 4 + 3) * TW + col];
+                    comp[k] = bf2f(cache[(int64_t)(2 * d * K + k) * W + wl]) - lsa + sm - (float)d * HALF_LOG_2PI;
+                    cmax = fmaxf(cmax, comp[k]);
+                }
+            }
+            float sc = 0.f;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) { sc += (k < K) ? __expf(comp[k] - cmax) : 0.f; }
+            misc[0 * TW + col] = cmax + __logf(sc); // log q(x' | x)
+        }
+    }
+    // 2. the network at Xb (x' for a step, x for a fill)
+    hidden_layer(N.W1t, N.b1, N.dp, H, Xb, ldx, H2, ldh);
+    __syncthreads();
+    hidden_layer(N.W2t, N.b2, H, H, H2, ldh, H1, ldh);
+    __syncthreads();
+    // H1 now holds layer 2; the last layer's tile goes to Cs, which overlaps H2
+    float *Ct = reinterpret_cast<float *>(H2);
+    for (int r0 = 0; r0 < R; r0 += 64) {
+        const f32x16 acc = mfma_chunk(N.W3t, H, N.Op, r0, H1, ldh);
+        {
+            const int lane = tid & 63, wv = tid >> 6, wm = wv >> 1, wn = wv & 1;
+            const int c = wn * 32 + (lane & 31);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int rr = wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                Ct[rr * (TW + 1) + c] = acc[i] + ((r0 + rr < N.Op) ? N.b3[r0 + rr] : 0.f);
+            }
+        }
+        __syncthreads();
+        // transform and round: means x MU_SCALE, log-scales LS_MAX tanh(o / LS_MAX), logits as they are
+        for (int rr = part; rr < 64; rr += 4) {
+            const int r = r0 + rr;
+            if (r >= R) { break; }
+            const float v = Ct[rr * (TW + 1) + col];
+            float val = v;
+            if (r < 2 * d * K) { val = ((r % (2 * d)) < d) ? mcim::MU_SCALE * v : mcim::LS_MAX * tanhf(v / mcim::LS_MAX); }
+            const uint16_t b = f2bf(val);
+            if (MODE == 0) {
+                if (live) { cache[(int64_t)r * W + w] = b; }
+            }
+            else { Pn[r * TW + col] = b; }
+        }
+        if (MODE == 1) {
+            __syncthreads();
+            // fold this chunk's components into log q(x | x'): part covers 8 coordinate pairs
+            // (component k occupies rows 2 d k .. 2 d k + 2 d - 1, so pair g = (component g / d,
+            // coordinate g % d) and a chunk of 64 rows holds 32 consecutive pairs; d is a multiple of 8)
+            const int g0 = (r0 >> 1) + part * 8;
+            const int k = g0 / d, j0 = g0 % d;
+            if (k < K) {
+                float acc = 0.f;
+                for (int q = 0; q < 8; ++q) {
+                    const int j = j0 + q;
+                    const float mu = bf2f(Pn[(2 * d * k + j) * TW + col]);
+                    const float ls = bf2f(Pn[(2 * d * k + d + j) * TW + col]);
+                    const float z = (xs[j * TW + col] - mu) * __expf(-ls);
+                    acc += -ls - 0.5f * z * z;
+                }
+                part_r[(k * 4 + (j0 >> 3)) * TW + col] = acc;
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE == 0) { return; }
+    // 3. log q(x | x') and the target at x'
+    if (part == 0) {
+        float amax = -__builtin_huge_valf();
+        for (int k = 0; k < K; ++k) { amax = fmaxf(amax, bf2f(Pn[(2 * d * K + k) * TW + col])); }
+        float sa = 0.f;
+        for (int k = 0; k < K; ++k) { sa += __expf(bf2f(Pn[(2 * d * K + k) * TW + col]) - amax); }
+        const float lsa = amax + __logf(sa);
+        const int nsub = d / 8;
+        float comp[KMAX];
+        float cmax = -__builtin_huge_valf();
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            comp[k] = -__builtin_huge_valf();
+            if (k < K) {
+                float sm = 0.f;
+                for (int sp = 0; sp < nsub; ++sp) { sm += part_r[(k * 4 + sp) * TW + col]; }
+                comp[k] = bf2f(Pn[(2 * d * K + k) * TW + col]) - lsa + sm - (float)d * HALF_LOG_2PI;
+                cmax = fmaxf(cmax, comp[k]);
+            }
+        }
+        float sc = 0.f;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { sc += (k < K) ? __expf(comp[k] - cmax) : 0.f; }
+        misc[2 * TW + col] = cmax + __logf(sc); // log q(x | x')
+    }
+    {
+        // components 2 part, 2 part + 1 of the target
+        float xr[DMAX];
+#pragma unroll
+        for (int j = 0; j < DMAX; ++j) { xr[j] = xps[j * TW + col]; }
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            const int k = 2 * part + kk;
+            const float *Uk = T.U + (int64_t)k * T.d * T.d;
+            const float *bk = T.Um + k * T.d;
+            float qk = 0.f;
+#pragma unroll
+            for (int i = 0; i < DMAX; ++i) {
+                if (i < T.d) {
+                    float r = -bk[i];
+#pragma unroll
+                    for (int j = i; j < DMAX; ++j) {
+                        if (j < T.d) { r = fmaf(Uk[i * T.d + j], xr[j], r); }
+                    }
+                    qk = fmaf(r, r, qk);
+                }
+            }
+            qn[k * TW + col] = qk;
+        }
+    }
+    __syncthreads();
+    if (part == 0) {
+        float mn = -__builtin_huge_valf();
+        float e[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            e[k] = T.c[k] - 0.5f * qn[k * TW + col];
+            mn = fmaxf(mn, e[k]);
+        }
+        float Sn = 0.f;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { Sn += __expf(e[k] - mn); }
+        uint32_t o[4];
+        philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
+        const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
+        const float logp_old = St.m[wl] + __logf(St.S[wl]);
+        const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - logp_old + (misc[2 * TW + col] - misc[0 * TW + col]));
+        accf[col] = acc ? 1 : 0;
+        if (acc) {
+            double sq = 0.;
+            for (int j = 0; j < d; ++j) {
+                const float v = xps[j * TW + col];
+                St.x[(int64_t)j * W + w] = v;
+                sq = fma((double)v, (double)v, sq);
+            }
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * W + w] = qn[k * TW + col]; }
+            St.m[w] = mn;
+            St.S[w] = Sn;
+            St.sq[w] = sq;
+            nacc_ind[w] += 1;
+        }
+        if (live && St.B > 0) {
+            float s0 = St.sb[w], c0 = St.cb[w], s1 = St.sb[W + w], c1 = St.cb[W + w];
+            int jb = St.jb[w], bi = St.bi[w];
+            accumulate(St, w, true, St.sq[w], acc ? xps[col] : xs[col], s0, c0, s1, c1, jb, bi);
+            St.sb[w] = s0;
+            St.cb[w] = c0;
+            St.sb[W + w] = s1;
+            St.cb[W + w] = c1;
+            St.jb[w] = jb;
+            St.bi[w] = bi;
+        }
+    }
+    __syncthreads();
+    if (accf[col]) {
+        for (int r = part; r < R; r += 4) { cache[(int64_t)r * W + w] = Pn[r * TW + col]; }
     }
 }
 
@@ -729,19 +940,28 @@ hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, co
     return hipGetLastError();
 }
 
-hipError_t launch_lp_propose(const Target<float> &T, const State<float> &S, const float *out, int rows, int K, uint64_t t, uint32_t k0,
-                             uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f, hipStream_t s)
+size_t learned_lds_bytes(const NetDev &N)
 {
-    hipLaunchKernelGGL(k_lp_propose, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), 0, s, T, S, out, rows, K, t, k0, k1, xp, xpb, in_scale,
-                       logq_f);
-    return hipGetLastError();
+    const size_t f = sizeof(float) * ((size_t)2 * DMAX * TW + (size_t)2 * KMAX * 4 * TW + (size_t)KMAX * TW + 4 * TW) + sizeof(int) * 2 * TW;
+    const size_t hb = sizeof(uint16_t) * ((size_t)TW * (N.dp + 8) + (size_t)2 * TW * (N.H + 8) + (size_t)N.O * TW);
+    return f + hb;
 }
 
-hipError_t launch_lp_accept(const Target<float> &T, const State<float> &S, float *out, const float *outn, int rows, int K, uint64_t t,
-                            uint32_t k0, uint32_t k1, const float *xp, const float *logq_f, int32_t *nacc_ind, hipStream_t s)
+hipError_t launch_lp_learned(int mode, const NetDev &N, const Target<float> &T, const State<float> &S, uint16_t *cache, uint64_t t,
+                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lp_accept, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), 0, s, T, S, out, outn, rows, K, t, k0, k1, xp, logq_f,
-                       nacc_ind);
+    const size_t lds = learned_lds_bytes(N);
+    const dim3 grid((unsigned)((S.W + TW - 1) / TW));
+    if (mode == 0) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_learned<0>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) { return e; }
+        hipLaunchKernelGGL(k_lp_learned<0>, grid, dim3(TT), lds, s, N, T, S, cache, t, k0, k1, nacc_ind);
+    }
+    else {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_learned<1>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) { return e; }
+        hipLaunchKernelGGL(k_lp_learned<1>, grid, dim3(TT), lds, s, N, T, S, cache, t, k0, k1, nacc_ind);
+    }
     return hipGetLastError();
 }
 

@@ -52,12 +52,19 @@ hipError_t launch_lp_init(const Target<R> &T, const State<R> &S, uint32_t k0, ui
 template <class R>
 hipError_t launch_lp_rw(const Target<R> &T, const State<R> &S, R step, uint64_t t0, int64_t nsteps, uint32_t k0, uint32_t k1,
                         hipStream_t s);
-// learned move, FP32 chain.  out / outn: network outputs [rows][W] at x and at the proposal.
-hipError_t launch_lp_propose(const Target<float> &T, const State<float> &S, const float *out, int rows, int K, uint64_t t,
-                             uint32_t k0, uint32_t k1, float *xp, uint16_t *xpb, float in_scale, float *logq_f, hipStream_t s);
-hipError_t launch_lp_accept(const Target<float> &T, const State<float> &S, float *out, const float *outn, int rows, int K,
-                            uint64_t t, uint32_t k0, uint32_t k1, const float *xp, const float *logq_f, int32_t *nacc_ind,
-                            hipStream_t s);
+// the network for the fused learned move: BF16 weights stored [out][in] (W1t [H][dp] with the
+// input padded to dp = max(16, d) columns, W2t [H][H], W3t [Op][H]), FP32 biases
+struct NetDev
+{
+    int d, dp, H, K, O, Op;
+    const uint16_t *W1t, *W2t, *W3t;
+    const float *b1, *b2, *b3;
+};
+size_t learned_lds_bytes(const NetDev &N);
+// mode 0: fill the BF16 mixture cache [O][W] at the current positions; mode 1: one learned move
+// at step t for every walker (FP32 chain)
+hipError_t launch_lp_learned(int mode, const NetDev &N, const Target<float> &T, const State<float> &S, uint16_t *cache, uint64_t t,
+                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s);
 // a whole run of the chain mixing random-walk moves and the independence proposal Q
 // ([Kq][1 + 3 d] rows: log pi_k - sum log sigma_k, mu_k, sigma_k, 1/sigma_k; then Kq cumulative
 // weights), FP32

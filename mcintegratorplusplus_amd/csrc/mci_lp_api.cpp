@@ -275,8 +275,8 @@ struct mci_lp
     DBuf<int32_t> nacc, nacc_ind;
     // learned move
     mci_mlp *net = nullptr;
-    DBuf<float> out, outn, xp, logq_f, snap;
-    DBuf<bf16_t> xpb;
+    DBuf<float> snap, nb1, nb2, nb3;
+    DBuf<uint16_t> nW1t, nW2t, nW3t, cache;
     int nsnap = 0;
     double same_frac = 0.5;
     std::vector<float> ctx0;   // mean of the last snapshot: the network input of the independence proposal
@@ -504,22 +504,52 @@ mci_status run_plain(mci_lp *lp, int64_t nsteps, int nblocks)
     return MCI_OK;
 }
 
+// the network in the layout of the fused learned move (BF16 [out][in] weights)
+mci_status net_dev(mci_lp *lp, mcil::NetDev &N)
+{
+    TRY(net_ready(lp));
+    mci_mlp *n = lp->net;
+    const int d = n->d, H = n->H, Op = n->Op, dp = std::max(16, d);
+    std::vector<float> p((size_t)n->npad);
+    HIPCHK(hipStreamSynchronize(n->stream));
+    HIPCHK(hipMemcpy(p.data(), n->p.p, p.size() * sizeof(float), hipMemcpyDeviceToHost));
+    auto bf = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    };
+    std::vector<uint16_t> w1((size_t)H * dp, 0), w2((size_t)H * H), w3((size_t)Op * H);
+    for (int h = 0; h < H; ++h) {
+        for (int i = 0; i < d; ++i) { w1[(size_t)h * dp + i] = bf(p[(size_t)(n->oW1 + (int64_t)i * H + h)]); }
+        for (int i = 0; i < H; ++i) { w2[(size_t)h * H + i] = bf(p[(size_t)(n->oW2 + (int64_t)i * H + h)]); }
+    }
+    for (int o = 0; o < Op; ++o) {
+        for (int h = 0; h < H; ++h) { w3[(size_t)o * H + h] = bf(p[(size_t)(n->oW3 + (int64_t)h * Op + o)]); }
+    }
+    TRY(up(lp->nW1t, w1));
+    TRY(up(lp->nW2t, w2));
+    TRY(up(lp->nW3t, w3));
+    TRY(up(lp->nb1, std::vector<float>(p.begin() + n->ob1, p.begin() + n->ob1 + H)));
+    TRY(up(lp->nb2, std::vector<float>(p.begin() + n->ob2, p.begin() + n->ob2 + H)));
+    TRY(up(lp->nb3, std::vector<float>(p.begin() + n->ob3, p.begin() + n->ob3 + Op)));
+    N = {d, dp, H, n->K, n->O, Op, lp->nW1t.p, lp->nW2t.p, lp->nW3t.p, lp->nb1.p, lp->nb2.p, lp->nb3.p};
+    return MCI_OK;
+}
+
+// the chain mixing random-walk moves (runs of them in one launch) with learned moves (one
+// fused launch each); the move type of step t is the same for every walker
 mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
 {
+    if (lp->d != 8 && lp->d != 16 && lp->d != 32) { return fail(MCI_E_INVALID_ARGUMENT, "the learned move needs d in {8, 16, 32}"); }
     mcil::State<float> S;
     const int B = (int)(nsteps / nblocks);
     TRY(state_of<float>(lp, B, nblocks, S));
-    TRY(net_ready(lp));
-    mci_mlp *n = lp->net;
+    mcil::NetDev N;
+    TRY(net_dev(lp, N));
     const mcil::Target<float> T = target_of<float>(lp);
     const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
     const size_t W = (size_t)lp->W;
-    HIPCHK(lp->out.ensure((size_t)n->Op * W));
-    HIPCHK(lp->outn.ensure((size_t)n->Op * W));
-    HIPCHK(lp->xp.ensure((size_t)lp->d * W));
-    HIPCHK(lp->xpb.ensure((size_t)lp->d * W));
-    HIPCHK(lp->logq_f.ensure(W));
-    TRY(mlp_reserve(n, lp->W));
+    HIPCHK(lp->cache.ensure((size_t)N.O * W));
     auto t0 = clk::now();
     HIPCHK(mcil::launch_lp_init<float>(T, S, k0, k1, lp->stream));
     if (lp->step32 <= 0.) { lp->step32 = 1.0; }
@@ -529,20 +559,16 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
     HIPCHK(hipMemsetAsync(lp->nacc_ind.p, 0, sizeof(int32_t) * W, lp->stream));
     HIPCHK(hipStreamSynchronize(lp->stream));
     t0 = clk::now();
-    // the network's mixture at the current positions
-    HIPCHK(mcim::launch_to_bf16_scaled(S.x, lp->xpb.p, (int64_t)lp->d * lp->W, mcim::IN_SCALE, lp->stream));
-    TRY(mlp_forward(n, lp->W, lp->xpb.p, lp->out.p));
+    HIPCHK(mcil::launch_lp_learned(0, N, T, S, lp->cache.p, 0, k0, k1, lp->nacc_ind.p, lp->stream));
     const uint64_t tb = lp->t_next;
-    int64_t nind = 0;
-    int64_t s = 0;
+    auto learned = [&](int64_t st) {
+        uint32_t o[4];
+        const uint64_t t = tb + (uint64_t)st;
+        philox_host(0x20000000u, 0xffffffffu, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
+        return (float)(o[0] >> 8) * 0x1p-24f < (float)pmix;
+    };
+    int64_t nind = 0, s = 0;
     while (s < nsteps) {
-        // move type of step tb + s: the same for every walker
-        auto learned = [&](int64_t st) {
-            uint32_t o[4];
-            const uint64_t t = tb + (uint64_t)st;
-            philox_host(0x20000000u, 0xffffffffu, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
-            return (double)(o[0] >> 8) * 0x1p-24 < pmix;
-        };
         if (!learned(s)) {
             int64_t e = s + 1;
             while (e < nsteps && !learned(e)) { ++e; }
@@ -550,12 +576,7 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
             s = e;
             continue;
         }
-        const uint64_t t = tb + (uint64_t)s;
-        HIPCHK(mcil::launch_lp_propose(T, S, lp->out.p, n->Op, n->K, t, k0, k1, lp->xp.p, lp->xpb.p, mcim::IN_SCALE, lp->logq_f.p,
-                                       lp->stream));
-        TRY(mlp_forward(n, lp->W, lp->xpb.p, lp->outn.p));
-        HIPCHK(mcil::launch_lp_accept(T, S, lp->out.p, lp->outn.p, n->Op, n->K, t, k0, k1, lp->xp.p, lp->logq_f.p, lp->nacc_ind.p,
-                                      lp->stream));
+        HIPCHK(mcil::launch_lp_learned(1, N, T, S, lp->cache.p, tb + (uint64_t)s, k0, k1, lp->nacc_ind.p, lp->stream));
         ++nind;
         ++s;
     }
@@ -578,6 +599,7 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
     lp->results[7] = nind > 0 ? ai / ((double)lp->W * (double)nind) : 0.;
     return MCI_OK;
 }
+
 // the independence proposal: the network's mixture at the fixed input ctx0, as
 // [Kq][log pi_k - sum_j log sigma_kj, mu_k, sigma_k, 1/sigma_k] and cumulative weights
 mci_status indep_params(mci_lp *lp)
@@ -598,10 +620,11 @@ mci_status indep_params(mci_lp *lp)
     HIPCHK(hipMemcpyAsync(o.data(), n->OUT.p, o.size() * sizeof(float), hipMemcpyDeviceToHost, n->stream));
     HIPCHK(hipStreamSynchronize(n->stream));
     auto col = [&](int c) { return (double)o[(size_t)c * ns]; };
+    const int L0 = 2 * d * K;
     double amax = -HUGE_VAL;
-    for (int k = 0; k < K; ++k) { amax = std::fmax(amax, col(k)); }
+    for (int k = 0; k < K; ++k) { amax = std::fmax(amax, col(L0 + k)); }
     double sa = 0.;
-    for (int k = 0; k < K; ++k) { sa += std::exp(col(k) - amax); }
+    for (int k = 0; k < K; ++k) { sa += std::exp(col(L0 + k) - amax); }
     const double lsa = amax + std::log(sa);
     std::vector<float> q((size_t)K * (1 + 3 * d) + K);
     double cum = 0.;
@@ -609,14 +632,14 @@ mci_status indep_params(mci_lp *lp)
         float *row = q.data() + (size_t)k * (1 + 3 * d);
         double lsum = 0.;
         for (int j = 0; j < d; ++j) {
-            const double mu = mcim::MU_SCALE * col(K + k * d + j);
-            const double ls = mcim::LS_MAX * std::tanh(col(K + K * d + k * d + j) / mcim::LS_MAX);
+            const double mu = mcim::MU_SCALE * col(2 * d * k + j);
+            const double ls = mcim::LS_MAX * std::tanh(col(2 * d * k + d + j) / mcim::LS_MAX);
             row[1 + j] = (float)mu;
             row[1 + d + j] = (float)std::exp(ls);
             row[1 + 2 * d + j] = (float)std::exp(-ls);
             lsum += ls;
         }
-        const double lp_k = col(k) - lsa;
+        const double lp_k = col(L0 + k) - lsa;
         row[0] = (float)(lp_k - lsum);
         cum += std::exp(lp_k);
         q[(size_t)K * (1 + 3 * d) + k] = (k == K - 1) ? 1.f : (float)cum;

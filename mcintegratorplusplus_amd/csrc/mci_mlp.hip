@@ -90,7 +90,8 @@ __global__ void k_to_bf16(const float *x, bf16_t *xb, int64_t n, float scale)
 
 // one sample per thread, feature-major layout (element c of sample r at out[c * ld + r]):
 // -log of the mixture density of y under the sample's parameters and the gradient of
-// (inv_n * that) with respect to the sample's raw outputs
+// (inv_n * that) with respect to the sample's raw outputs.  Output rows of one sample: for each
+// component k, d mean rows then d log-scale rows (rows 2 d k .. 2 d k + 2 d - 1); then K logits.
 __global__ void k_mixture_head(const float *out, int ld, const float *y, int n, int d, int K, int rows, float *loss, float *dout,
                                bf16_t *doutb, float inv_n)
 {
@@ -101,18 +102,19 @@ __global__ void k_mixture_head(const float *out, int ld, const float *y, int n, 
     float *g = dout + r;
     bf16_t *gb = doutb + r;
     constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    const int L0 = 2 * d * K; // first logit row
     float amax = -__builtin_huge_valf();
-    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, o[(int64_t)k * ld]); }
+    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, o[(int64_t)(L0 + k) * ld]); }
     float sa = 0.f;
-    for (int k = 0; k < K; ++k) { sa += __expf(o[(int64_t)k * ld] - amax); }
+    for (int k = 0; k < K; ++k) { sa += __expf(o[(int64_t)(L0 + k) * ld] - amax); }
     const float lsa = amax + __logf(sa);
     float comp[8];
     float cmax = -__builtin_huge_valf();
     for (int k = 0; k < K; ++k) {
-        float c = o[(int64_t)k * ld] - lsa;
+        float c = o[(int64_t)(L0 + k) * ld] - lsa;
         for (int j = 0; j < d; ++j) {
-            const float mu = MU_SCALE * o[(int64_t)(K + k * d + j) * ld];
-            const float ls = LS_MAX * tanhf(o[(int64_t)(K + K * d + k * d + j) * ld] / LS_MAX);
+            const float mu = MU_SCALE * o[(int64_t)(2 * d * k + j) * ld];
+            const float ls = LS_MAX * tanhf(o[(int64_t)(2 * d * k + d + j) * ld] / LS_MAX);
             const float z = (yr[(int64_t)j * ld] - mu) * __expf(-ls);
             c += -ls - HALF_LOG_2PI - 0.5f * z * z;
         }
@@ -125,12 +127,12 @@ __global__ void k_mixture_head(const float *out, int ld, const float *y, int n, 
     loss[r] = -L;
     for (int k = 0; k < K; ++k) {
         const float gam = __expf(comp[k] - L);
-        const float pik = __expf(o[(int64_t)k * ld] - lsa);
+        const float pik = __expf(o[(int64_t)(L0 + k) * ld] - lsa);
         const float ga = inv_n * (pik - gam);
-        g[(int64_t)k * ld] = ga;
-        gb[(int64_t)k * ld] = f2bf(ga);
+        g[(int64_t)(L0 + k) * ld] = ga;
+        gb[(int64_t)(L0 + k) * ld] = f2bf(ga);
         for (int j = 0; j < d; ++j) {
-            const int64_t im = (int64_t)(K + k * d + j) * ld, il = (int64_t)(K + K * d + k * d + j) * ld;
+            const int64_t im = (int64_t)(2 * d * k + j) * ld, il = (int64_t)(2 * d * k + d + j) * ld;
             const float mu = MU_SCALE * o[im];
             const float t = tanhf(o[il] / LS_MAX);
             const float ls = LS_MAX * t;

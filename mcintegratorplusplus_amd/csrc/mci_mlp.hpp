@@ -12,8 +12,8 @@ namespace mcim
 typedef uint16_t bf16_t; // bfloat16 bits
 
 // Activations are feature-major ([feature][sample]) so that one thread per sample reads and
-// writes coalesced.  Output features of one sample: [K logits][K x d means][K x d log-scale
-// pre-activations].
+// writes coalesced.  Output features of one sample, component-major: for each component k, d
+// mean rows then d log-scale pre-activation rows (rows 2 d k .. 2 d k + 2 d - 1), then K logits.
 constexpr float IN_SCALE = 0.25f;  // network input = x * IN_SCALE
 constexpr float MU_SCALE = 4.0f;   // mean = MU_SCALE * output
 constexpr float LS_MAX = 3.0f;     // log-scale = LS_MAX tanh(output / LS_MAX)

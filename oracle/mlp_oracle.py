@@ -3,7 +3,8 @@ checker, never by the product).
 
 Restates, in float64 numpy, what mcintegratorplusplus_amd/csrc/mci_mlp.hip and mci_lp_api.cpp
 compute: the MLP d -> H -> H -> K (1 + 2 d) with SiLU, the diagonal Gaussian-mixture head
-(K logits, means = 4 * output, log-scales = 3 tanh(output / 3), input = x / 4), the mean negative
+(outputs per component k: d means / 4 then d log-scale pre-activations, log-scale = 3 tanh(o / 3);
+then K logits; input = x / 4), the mean negative
 log-likelihood of targets y under q(. | x), and its gradient by backpropagation.  With
 emulate=True the operands of every matrix product are rounded to bfloat16 (round to nearest
 even) at the same points as on the GPU (input, weights, SiLU outputs, output gradient, SiLU
@@ -72,12 +73,23 @@ def _lse(a, axis):
     return (m + np.log(np.sum(np.exp(a - m), axis=axis, keepdims=True))).squeeze(axis)
 
 
+def _split(out, d, K):
+    """(logits [n][K], mean pre-activations [n][K][d], log-scale pre-activations [n][K][d])"""
+    n = out.shape[0]
+    comp = out[:, :2 * d * K].reshape(n, K, 2 * d)
+    return out[:, 2 * d * K:2 * d * K + K], comp[:, :, :d], comp[:, :, d:]
+
+
+def _join(g_a, g_mu, g_ls):
+    n, K, d = g_mu.shape
+    return np.concatenate([np.concatenate([g_mu, g_ls], axis=2).reshape(n, 2 * d * K), g_a], axis=1)
+
+
 def mixture_logq(out, y, d, K):
     """log q(y | parameters in out), rows of out [n][K (1 + 2 d)]"""
-    n = out.shape[0]
-    a = out[:, :K]
-    mu = MU_SCALE * out[:, K:K + K * d].reshape(n, K, d)
-    ls = LS_MAX * np.tanh(out[:, K + K * d:K + 2 * K * d].reshape(n, K, d) / LS_MAX)
+    a, om, ol = _split(out, d, K)
+    mu = MU_SCALE * om
+    ls = LS_MAX * np.tanh(ol / LS_MAX)
     z = (np.asarray(y, np.float64)[:, None, :] - mu) * np.exp(-ls)
     comp = a - _lse(a, 1)[:, None] + np.sum(-ls - HALF_LOG_2PI - 0.5 * z * z, axis=2)
     return _lse(comp, 1)
@@ -86,9 +98,9 @@ def mixture_logq(out, y, d, K):
 def head(out, y, d, K):
     """per-row loss -log q(y | out) and d(mean loss)/d(out)"""
     n = out.shape[0]
-    a = out[:, :K]
-    mu = MU_SCALE * out[:, K:K + K * d].reshape(n, K, d)
-    t = np.tanh(out[:, K + K * d:K + 2 * K * d].reshape(n, K, d) / LS_MAX)
+    a, om, ol = _split(out, d, K)
+    mu = MU_SCALE * om
+    t = np.tanh(ol / LS_MAX)
     ls = LS_MAX * t
     isg = np.exp(-ls)
     z = (np.asarray(y, np.float64)[:, None, :] - mu) * isg
@@ -100,8 +112,7 @@ def head(out, y, d, K):
     g_a = pi - gam
     g_mu = (-gam[:, :, None] * z * isg) * MU_SCALE
     g_ls = gam[:, :, None] * (1.0 - z * z) * (1.0 - t * t)
-    dout = np.concatenate([g_a, g_mu.reshape(n, K * d), g_ls.reshape(n, K * d)], axis=1) / n
-    return -L, dout
+    return -L, _join(g_a, g_mu, g_ls) / n
 
 
 def loss_grad(p, x, y, d, H, K, emulate=True):

@@ -42,11 +42,12 @@ def test_mixture_density_matches_scipy():
     out = rng.normal(size=(n, K * (1 + 2 * d)))
     y = rng.normal(size=(n, d)) * 3
     got = M.mixture_logq(out, y, d, K)
-    a = out[:, :K]
+    a = out[:, 2 * d * K:]
     pi = np.exp(a - a.max(1, keepdims=True))
     pi /= pi.sum(1, keepdims=True)
-    mu = 4.0 * out[:, K:K + K * d].reshape(n, K, d)
-    sig = np.exp(3.0 * np.tanh(out[:, K + K * d:].reshape(n, K, d) / 3.0))
+    comp = out[:, :2 * d * K].reshape(n, K, 2 * d)
+    mu = 4.0 * comp[:, :, :d]
+    sig = np.exp(3.0 * np.tanh(comp[:, :, d:] / 3.0))
     want = np.log(np.sum(pi * np.exp(np.sum(norm.logpdf(y[:, None, :], mu, sig), axis=2)), axis=1))
     assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
 
