@@ -284,7 +284,7 @@ mci_status mci_mlp_nparams(mci_mlp *m, int64_t *n);
 mci_status mci_mlp_get_params(mci_mlp *m, float *params, int64_t n);
 mci_status mci_mlp_set_params(mci_mlp *m, const float *params, int64_t n);
 /* raw outputs [n][K(1+2d)] for positions x [n][d]: for each component k, d means / 4 then d
- * log-scale pre-activations (log-scale = 3 tanh(o / 3)); then K logits.  The input is x / 4. */
+ * log-scale pre-activations (log-scale = o clamped to [-3, 3]); then K logits.  The input is x / 4. */
 mci_status mci_mlp_forward(mci_mlp *m, int64_t n, const float *x, float *out);
 /* mean of -log q(y_r | x_r) over n pairs and its gradient with respect to the FP32 parameters */
 mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *x, const float *y, double *loss, float *grad);
@@ -294,9 +294,11 @@ mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *x, const float 
  * Observables: |x|^2 and 1[x_0 > 0]. */
 #define MCI_LP_PLAIN_FP64   0  /* random-walk Metropolis (single-coordinate uniform moves), FP64 */
 #define MCI_LP_PLAIN_FP32   1  /* the same chain in FP32 with Kahan block sums */
-#define MCI_LP_MIXED_FP32   2  /* each step is a learned move with probability pmix, else a random-walk move */
-#define MCI_LP_INDEP_FP32   3  /* as MIXED, with the independence proposal q(x') = the network's mixture at one fixed
-                                  input (the mean of the last training snapshot), all steps in one launch */
+#define MCI_LP_MIXED_FP32   2  /* cycles of 32 steps: round(32 (1 - pmix)) random-walk moves, then round(32 pmix)
+                                  (at least 1) learned moves; pmix in (0, 1] */
+#define MCI_LP_INDEP_FP32   3  /* each step is, with probability pmix, an MH move from the independence proposal
+                                  q(x') = the network's mixture at one fixed input (the mean of the last training
+                                  snapshot), else a random-walk move; all steps in one launch */
 typedef struct mci_lp mci_lp;
 mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *means, const double *covs, int64_t nwalkers,
                          mci_lp **out);
@@ -321,6 +323,10 @@ mci_status mci_lp_get_results(mci_lp *lp, double *results, int n);
 mci_status mci_lp_get_timings(mci_lp *lp, double *ms, int n);
 mci_status mci_lp_get_walker_means(mci_lp *lp, double *means, int64_t n); /* [2][nwalkers] */
 mci_status mci_lp_get_network(mci_lp *lp, mci_mlp **m);                   /* owned by the sampler */
+/* the network as the learned move's kernel evaluates it (BF16 matrix cores, transformed outputs), at
+ * n positions x [d][n]: out [K(1+2d)][n] holds per component k the d means, then the d log-scales,
+ * then the K logits.  Needs d in {8, 16, 32} and hidden in {64, 128}. */
+mci_status mci_lp_eval_network(mci_lp *lp, int64_t n, const float *x, float *out);
 
 #ifdef __cplusplus
 }

@@ -11,11 +11,10 @@
 // evaluated as u S < exp(m' - m) S' with log p = m + log S (no logarithm).  Every block end
 // recomputes q_k from x.  The kernel runs in FP64 (the plain chain) or FP32.
 //
-// Learned move (FP32; k_lp_propose, the network forward, k_lp_accept): the network maps the
-// current x to a K-component diagonal Gaussian mixture q(. | x); propose x' ~ q(. | x), evaluate
-// the network at x' and accept with min(1, p(x') q(x | x') / (p(x) q(x' | x))), so the target stays
-// exact whatever the network computes.  The network's output at the current x is kept per
-// walker, so one forward pass (at x') is needed per learned move.
+// Learned move (FP32, k_lp_learned): the network maps the current x to a K-component diagonal
+// Gaussian mixture q(. | x); propose x' ~ q(. | x), evaluate the network at x' and accept with
+// min(1, p(x') q(x | x') / (p(x) q(x' | x))), so the target stays exact whatever the network
+// computes.  Both forward passes (at x and at x') run inside the move's one launch.
 //
 // Random words (Philox4x32-10, key = seed, counter (c0, walker, step low, step high)):
 //   c0 = 0            random-walk step: [0] coordinate (Lemire), [1] proposal, [2..3] u
@@ -549,317 +548,487 @@ __global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<flo
     }
 }
 
-// ---- the learned move, fused: one launch per step for all walkers ----
-// A workgroup of 256 threads takes 64 walkers (column c = tid % 64, part = tid / 64).  The
-// network's mixture at the current x of every walker is kept in a BF16 cache [R][W] (rows per
-// component k: d means, d log-scales (already transformed), then K logits); the proposal q(. | x)
-// is defined by these rounded values.  A step:
-//  1. draw the component from the cached logits and x' = mu + exp(ls) z; log q(x' | x);
-//  2. the network at x' on the matrix cores (64 x 64 tiles of v_mfma_f32_32x32x16_bf16, weights
-//     read from L2 as [out][in] rows, activations walker-major in LDS), each 64-row chunk of the
-//     last layer consumed at once: rounded to BF16 into LDS and folded into log q(x | x');
-//  3. log p(x') (components split over the four parts), the MH test, and for accepted walkers the
-//     new position, target state and cached mixture.
+// ---- the learned move, fused: one launch per run of consecutive learned steps ----
+// A workgroup of 512 threads (eight waves) takes 64 walkers for the whole run.  The network is
+// evaluated at the walkers' positions when the run starts (random-walk moves in between need no
+// refresh) and at every proposal; the mixture at the current position is carried in registers
+// from step to step.  Network: v_mfma_f32_32x32x16_bf16 tiles, weights read from L2 as [out][in]
+// rows, biases as the accumulators' start values, activations walker-major in LDS, one activation
+// buffer per workgroup (a layer's output stays in accumulators until every wave has read its
+// input).  The last layer stays in registers: wave w holds components 2 (w % 4), 2 (w % 4) + 1 for
+// the walker half w / 4, its rows ordered as (mean_j, log-scale_j) pairs so that each lane holds
+// whole pairs of one walker, each pair kept as (mean, 1 / sigma) in two halves; waves 0 and 4
+// also compute the K logits.  A step:
+//  1. the component from the logits at x, x' = mu + sigma z from the lanes holding that component,
+//     log q(x' | x) folded from the registers; log p(x') (one target component per wave);
+//  2. the network at x', log q(x | x') folded the same way;
+//  3. the MH test; accepted walkers take x', its target state, logits and mixture.
+// Mode 0 evaluates the network only and writes the mixture (means, log-scales, logits) for tests.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-constexpr int TW = 64;    // walkers per workgroup
-constexpr int TT = 256;   // threads per workgroup
+typedef __fp16 h2_t __attribute__((ext_vector_type(2)));
+#ifndef MCIL_NS
+#define MCIL_NS 2 // last-layer row blocks per pass
+#endif
+constexpr int TW = 64;   // walkers per workgroup
+constexpr int TT = 512;  // threads per workgroup
+constexpr int NWV = 8;   // waves per workgroup
 
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-
-// rows r0 .. r0 + 63 of Wt [M][K] (BF16, row = output unit) times act [64 walkers][lda] (BF16, LDS):
-// wave (wm, wn) accumulates rows r0 + 32 wm .., walkers 32 wn ..
-__device__ __forceinline__ f32x16 mfma_chunk(const uint16_t *Wt, int K, int M, int r0, const uint16_t *act, int lda)
+template <int D, int H>
+struct LCfg
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
-    const int row = r0 + wm * 32 + (lane & 31);
-    // the weight fragments of the whole reduction first (K <= 128: at most 8), then the products
-    bf16x8 a[8];
+    static constexpr int DP = D < 16 ? 16 : D;  // padded input width
+    static constexpr int LDX = DP + 8;          // LDS row pitches (BF16 elements)
+    static constexpr int LDH = H + 8;
+    static constexpr int NRB = D / 8;           // 32-row blocks of a wave's two components
+    static constexpr int LOFF = 2 * D * KMAX;   // first logit row
+    static constexpr int NB = 2 * H + LOFF + 32; // biases
+};
+
+__device__ __forceinline__ float silu(float v) { return v * __builtin_amdgcn_rcpf(1.0f + __expf(-v)); }
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// acc[r] = bias + rows row0 + 32 r .. + 31 of Wt [.][KD] (BF16, row = output unit) times walkers
+// 32 cb .. + 31 of act [64][lda] (BF16, LDS), r < NR; bias in LDS, indexed by row
+template <int KD, int NR>
+__device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, const float *bias, int row0, const uint16_t *act, int lda, int cb,
+                                          f32x16 acc[NR])
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int NQ = KD / 16;
+    const uint16_t *wr = Wt + (int64_t)(row0 + (lane & 31)) * KD + 8 * (lane >> 5);
+    const uint16_t *br = act + (32 * cb + (lane & 31)) * lda + 8 * (lane >> 5);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int k = 16 * q + 8 * (lane >> 5);
-        a[q] = bf16x8{};
-        if (16 * q < K && row < M) { a[q] = *reinterpret_cast<const bf16x8 *>(Wt + (int64_t)row * K + k); }
-    }
-    f32x16 acc = {};
+    for (int r = 0; r < NR; ++r) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        if (16 * q < K) {
-            const int k = 16 * q + 8 * (lane >> 5);
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(act + (wn * 32 + (lane & 31)) * lda + k);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q], b, acc, 0, 0, 0);
+        for (int g = 0; g < 4; ++g) {
+            const float4 b = *reinterpret_cast<const float4 *>(bias + row0 + 32 * r + 8 * g + 4 * (lane >> 5));
+            acc[r][4 * g + 0] = b.x;
+            acc[r][4 * g + 1] = b.y;
+            acc[r][4 * g + 2] = b.z;
+            acc[r][4 * g + 3] = b.w;
         }
     }
-    return acc;
+    // one reduction step of fragments in flight ahead of the products (bounds the live registers)
+    bf16x8 a[2][NR], b[2];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) { a[0][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * KD); }
+    b[0] = *reinterpret_cast<const bf16x8 *>(br);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int cu = q & 1, nx = cu ^ 1;
+        if (q + 1 < NQ) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) { a[nx][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * KD + 16 * (q + 1)); }
+            b[nx] = *reinterpret_cast<const bf16x8 *>(br + 16 * (q + 1));
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) { acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[cu][r], b[cu], acc[r], 0, 0, 0); }
+        __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
-// hidden layer: out[c][r] = bf16(silu(W act + b)) for rows 0 .. M-1
-__device__ void hidden_layer(const uint16_t *Wt, const float *bias, int K, int M, const uint16_t *act, int lda, uint16_t *out, int ldo)
+// hidden layer: Hb [64][LDH] = bf16(silu(W act + b)); the 2 H / 32 blocks of 32 rows x 32 walkers
+// go to waves 0 .. 2 H / 32 - 1 (row block w % (H / 32), walker half w / (H / 32)).  The caller
+// has synchronised so that act is complete; when act is Hb itself, the waves synchronise between
+// the products and the write-back.
+template <int KD, int H>
+__device__ __forceinline__ void hidden_layer(const uint16_t *Wt, const float *bias, const uint16_t *act, int lda, uint16_t *Hb, bool in_place)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
-    for (int r0 = 0; r0 < M; r0 += 64) {
-        const f32x16 acc = mfma_chunk(Wt, K, M, r0, act, lda);
-        const int col = wn * 32 + (lane & 31);
+    constexpr int NRH = H / 32, LDH = H + 8;
+    const int w = wave_id(), lane = threadIdx.x & 63;
+    const bool mine = w < 2 * NRH;
+    const int rb = w % NRH, cb = w / NRH;
+    f32x16 acc[1];
+    if (mine) { mfma_tile<KD, 1>(Wt, bias, 32 * rb, act, lda, cb, acc); }
+    if (in_place) { __syncthreads(); }
+    if (mine) {
+        const int col = 32 * cb + (lane & 31);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int r = 32 * rb + 8 * g + 4 * (lane >> 5);
+            uint32_t p[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const __bf16 lo = (__bf16)silu(acc[0][4 * g + 2 * h]);
+                const __bf16 hi = (__bf16)silu(acc[0][4 * g + 2 * h + 1]);
+                p[h] = (uint32_t)__builtin_bit_cast(uint16_t, lo) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+            }
+            *reinterpret_cast<uint2 *>(Hb + col * LDH + r) = make_uint2(p[0], p[1]);
+        }
+    }
+}
+
+// the network at the positions in Xb.  Wave w keeps the last layer of components 2 (w % 4),
+// 2 (w % 4) + 1 for walkers 32 (w / 4) + lane % 32 in registers: pair (rb 4 + g) 2 + h of mx holds
+// (mean, 1 / sigma) of one coordinate as two halves (rounded toward zero; the move samples and
+// evaluates with exactly these values), lpi[cl] the sum of log(1 / sigma) over the lane's pairs of
+// component 2 (w % 4) + cl.  The logits go to lg [KMAX][64]; mode 0 also writes the mixture to out
+// [K (1 + 2 D)][n] (walker w0 + column): means, log-scales -log(1 / sigma), logits.
+__device__ __forceinline__ constexpr int pair_index(int rb, int g, int h) { return (rb * 4 + g) * 2 + h; }
+
+template <int D, int H>
+__device__ __forceinline__ void network(const NetDev &N, const float *bs, const uint16_t *Xb, uint16_t *Hb, h2_t mx[D], float lpi[2], float *lg,
+                                        float *out, int64_t n, int64_t w0)
+{
+    using C = LCfg<D, H>;
+    const int w = wave_id(), lane = threadIdx.x & 63, K = N.K;
+    const int wc = w & 3, cb = w >> 2;
+    const float *b3 = bs + 2 * H;
+    hidden_layer<C::DP, H>(N.W1t, bs, Xb, C::LDX, Hb, false);
+    __syncthreads();
+    hidden_layer<H, H>(N.W2t, bs + H, Hb, C::LDH, Hb, true);
+    __syncthreads();
+    const int col = 32 * cb + (lane & 31);
+    if (wc == 0) {
+        f32x16 al[1];
+        mfma_tile<H, 1>(N.W3t, b3, C::LOFF, Hb, C::LDH, cb, al);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const int row = r0 + wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-            if (row < M) {
-                const float v = acc[i] + bias[row];
-                out[col * ldo + row] = f2bf(v / (1.0f + __expf(-v)));
+            const int k = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+            if (k < K) {
+                lg[k * TW + col] = al[0][i];
+                if (out != nullptr && w0 + col < n) { out[(int64_t)(2 * D * K + k) * n + w0 + col] = al[0][i]; }
             }
         }
+    }
+    if (2 * wc < K) {
+        constexpr int NS = C::NRB >= MCIL_NS ? MCIL_NS : 1; // row blocks per pass (bounds the live accumulators)
+        float pr[2] = {1.f, 1.f};
+#pragma unroll
+        for (int rb0 = 0; rb0 < C::NRB; rb0 += NS) {
+            f32x16 acc[NS];
+            mfma_tile<H, NS>(N.W3t, b3, 4 * D * wc + 32 * rb0, Hb, C::LDH, cb, acc);
+#pragma unroll
+            for (int rr = 0; rr < NS; ++rr) {
+                const int rb = rb0 + rr;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int cl = (32 * rb + 8 * g) / (2 * D);
+                    const int r = 32 * rb + 8 * g + 4 * (lane >> 5); // wave-local row of the first pair
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float mu = __builtin_amdgcn_fmed3f(mcim::MU_SCALE * acc[rr][4 * g + 2 * h], -6.0e4f, 6.0e4f);
+                        const float ls = __builtin_amdgcn_fmed3f(acc[rr][4 * g + 2 * h + 1], -mcim::LS_MAX, mcim::LS_MAX);
+                        const h2_t p = __builtin_amdgcn_cvt_pkrtz(mu, __expf(-ls));
+                        mx[pair_index(rb, g, h)] = p;
+                        pr[cl] *= (float)p[1];
+                        const int k = 2 * wc + cl;
+                        if (out != nullptr && k < K && w0 + col < n) {
+                            const int j = ((r % (2 * D)) >> 1) + h;
+                            out[(int64_t)(2 * D * k + j) * n + w0 + col] = (float)p[0];
+                            out[(int64_t)(2 * D * k + D + j) * n + w0 + col] = -__logf((float)p[1]);
+                        }
+                    }
+                }
+            }
+        }
+        lpi[0] = __logf(pr[0]);
+        lpi[1] = __logf(pr[1]);
     }
 }
 
-template <int MODE> // 0: fill the cache at the current positions, 1: one learned step
-__global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, State<float> St, uint16_t *cache, uint64_t t, uint32_t k0,
-                                                   uint32_t k1, int32_t *nacc_ind)
+// per component and walker: sum_j log(1 / sigma_j) - |(y - mu) / sigma|^2 / 2 over the wave's two
+// components, y [D][64] in LDS, into cf [KMAX][64]
+template <int D>
+__device__ __forceinline__ void fold(const h2_t mx[D], const float lpi[2], const float *y, float *cf, int K)
 {
+    const int w = wave_id(), lane = threadIdx.x & 63;
+    const int wc = w & 3, col = 32 * (w >> 2) + (lane & 31);
+    if (2 * wc >= K) { return; }
+    float s[2] = {0.f, 0.f};
+#pragma unroll
+    for (int rb = 0; rb < D / 8; ++rb) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int cl = (32 * rb + 8 * g) / (2 * D);
+            const int j = ((32 * rb + 8 * g + 4 * (lane >> 5)) % (2 * D)) >> 1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const h2_t p = mx[pair_index(rb, g, h)];
+                const float z = (y[(j + h) * TW + col] - (float)p[0]) * (float)p[1];
+                s[cl] = fmaf(z, z, s[cl]);
+            }
+        }
+    }
+#pragma unroll
+    for (int cl = 0; cl < 2; ++cl) {
+        float v = lpi[cl] - 0.5f * s[cl];
+        v += __shfl_xor(v, 32);
+        if (lane < 32 && 2 * wc + cl < K) { cf[(2 * wc + cl) * TW + col] = v; }
+    }
+}
+
+// q_k = |U_k y - U_k mu_k|^2 for target component k and the 64 walkers of y [D][64] (LDS) with
+// v_mfma_f32_32x32x2_f32 (FP32 products, FP32 sums): rows of U_k (zero-padded to 32) times y, the
+// accumulators starting at -U_k mu_k; lanes 0 .. 31 return q of walkers lane and 32 + lane in
+// out[0].x and out[1].x
+typedef float f32x16v __attribute__((ext_vector_type(16)));
+template <int D>
+__device__ __forceinline__ void target_q(const float *U, const float *Um, int k, const float *y, int lane, float2 out[2])
+{
+    const int row = lane & 31, kk = lane >> 5;
+    const float *Uk = U + (int64_t)k * D * D;
+    f32x16v acc[2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int r = 8 * (i >> 2) + 4 * kk + (i & 3);
+        const float v = r < D ? -Um[k * D + r] : 0.f;
+        acc[0][i] = v;
+        acc[1][i] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < D / 2; ++q) {
+        const int j = 2 * q + kk;
+        const float a = row < D ? Uk[row * D + j] : 0.f;
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, y[j * TW + row], acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, y[j * TW + 32 + row], acc[1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) { s = fmaf(acc[c][i], acc[c][i], s); }
+        s += __shfl_xor(s, 32);
+        out[c] = make_float2(s, 0.f);
+    }
+}
+
+// log q(y | the mixture whose logits are lg and component sums cf), for walker col
+__device__ __forceinline__ float mixture_logq(const float *lg, const float *cf, int K, int D, int col)
+{
+    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    float amax = -__builtin_huge_valf(), cmax = -__builtin_huge_valf();
+    float comp[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < K) {
+            const float a = lg[k * TW + col];
+            amax = fmaxf(amax, a);
+            comp[k] = a + cf[k * TW + col];
+            cmax = fmaxf(cmax, comp[k]);
+        }
+    }
+    float sa = 0.f, sc = 0.f;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < K) {
+            sa += __expf(lg[k * TW + col] - amax);
+            sc += __expf(comp[k] - cmax);
+        }
+    }
+    return (cmax + __logf(sc)) - (amax + __logf(sa)) - (float)D * HALF_LOG_2PI;
+}
+
+template <int D, int H>
+size_t learned_lds()
+{
+    using C = LCfg<D, H>;
+    return sizeof(float) * ((size_t)2 * D * TW + (size_t)5 * KMAX * TW + C::NB) + sizeof(int) * 3 * TW +
+           sizeof(uint16_t) * ((size_t)TW * C::LDX + (size_t)TW * C::LDH);
+}
+
+template <int MODE, int D, int H> // 0: the network at xin, written to out; 1: nl learned steps from step t0
+__global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, State<float> St, const float *xin, float *out, uint64_t t0, int nl,
+                                                   uint32_t k0, uint32_t k1, int32_t *nacc_ind)
+{
+    using C = LCfg<D, H>;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    const int d = N.d, K = N.K, H = N.H, R = N.O;
-    const int tid = threadIdx.x, col = tid & 63, part = tid >> 6;
+    const int K = N.K;
+    const int tid = threadIdx.x, col = tid & 63, part = tid >> 6, lane = tid & 63;
+    const int wv = wave_id(), wc = wv & 3, cbw = wv >> 2;
     const int64_t W = St.W;
-    const int64_t w = (int64_t)blockIdx.x * TW + col;
+    const int64_t w0 = (int64_t)blockIdx.x * TW;
+    const int64_t w = w0 + col;
     const bool live = w < W;
     const int64_t wl = live ? w : W - 1;
-    const int ldx = N.dp + 8, ldh = H + 8;
-    // LDS carve-up (byte offsets multiple of 16)
-    float *xs = reinterpret_cast<float *>(lds_raw);            // [DMAX][64] current x
-    float *xps = xs + DMAX * TW;                                // [DMAX][64] proposal
-    float *part_f = xps + DMAX * TW;                            // [KMAX][4][64] partial log-densities
-    float *part_r = part_f + KMAX * 4 * TW;                     // [KMAX][4][64]
-    float *qn = part_r + KMAX * 4 * TW;                         // [KMAX][64]
-    float *misc = qn + KMAX * TW;                               // [4][64]: log q(x'|x), lse of logits, ...
-    int *kc = reinterpret_cast<int *>(misc + 4 * TW);           // [64]
-    int *accf = kc + TW;                                        // [64]
-    uint16_t *Xb = reinterpret_cast<uint16_t *>(accf + TW);     // [64][ldx]
-    uint16_t *H1 = Xb + TW * ldx;                               // [64][ldh]
-    uint16_t *H2 = H1 + TW * ldh;                               // [64][ldh]  (before: normals; after: last-layer tile)
-    uint16_t *Pn = H2 + TW * ldh;                               // [R][64] mixture at x' (MODE 1)
-    float *zs = reinterpret_cast<float *>(H2);                  // [DMAX][64]
-    const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
-    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
+    float *xs = reinterpret_cast<float *>(lds_raw);      // [D][64] current x
+    float *xps = xs + D * TW;                             // [D][64] normals, then the proposal
+    float *lgx = xps + D * TW;                            // [KMAX][64] logits at x
+    float *lg = lgx + KMAX * TW;                          // [KMAX][64] logits at x'
+    float *cf = lg + KMAX * TW;                           // [KMAX][64] component sums of a fold
+    float *qn = cf + KMAX * TW;                           // [KMAX][64] target quadratic forms at x'
+    float *qx = qn + KMAX * TW;                           // [KMAX][64] the same at x (accepted moves)
+    float *bs = qx + KMAX * TW;                           // biases: b1 [H], b2 [H], b3 [LOFF + 32]
+    int *kst = reinterpret_cast<int *>(bs + C::NB);       // [64] chosen component
+    int *af = kst + TW;                                   // [64] accepted in this step
+    int *anyacc = af + TW;                                // [64] accepted in this run
+    uint16_t *Xb = reinterpret_cast<uint16_t *>(anyacc + TW); // [64][LDX]
+    uint16_t *Hb = Xb + TW * C::LDX;                      // [64][LDH]
+    const float *xsrc = MODE == 0 ? xin : St.x;
 
-    // current positions
-    for (int j = part; j < DMAX; j += 4) { xs[j * TW + col] = (j < d) ? St.x[(int64_t)j * W + wl] : 0.f; }
-    if (MODE == 0) {
-        for (int j = part; j < N.dp; j += 4) { Xb[col * ldx + j] = f2bf(j < d ? xs[j * TW + col] * mcim::IN_SCALE : 0.f); }
+    for (int i = tid; i < H; i += TT) {
+        bs[i] = N.b1[i];
+        bs[H + i] = N.b2[i];
     }
+    for (int i = tid; i < C::LOFF + 32; i += TT) { bs[2 * H + i] = N.b3[i]; }
+    for (int j = part; j < D; j += NWV) { xs[j * TW + col] = xsrc[(int64_t)j * W + wl]; }
+    for (int j = part; j < C::DP; j += NWV) { Xb[col * C::LDX + j] = f2bf(j < D ? xsrc[(int64_t)j * W + wl] * mcim::IN_SCALE : 0.f); }
     __syncthreads();
-    if (MODE == 1) {
-        // 1. component and proposal from the cached mixture at x
-        if (part == 0) {
-            float amax = -__builtin_huge_valf();
-            for (int k = 0; k < K; ++k) { amax = fmaxf(amax, bf2f(cache[(int64_t)(2 * d * K + k) * W + wl])); }
-            float sa = 0.f;
-            for (int k = 0; k < K; ++k) { sa += __expf(bf2f(cache[(int64_t)(2 * d * K + k) * W + wl]) - amax); }
-            uint32_t o[4];
-            philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
-            const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
-            int kk = K - 1;
-            float cum = 0.f;
-            for (int k = 0; k < K - 1; ++k) {
-                cum += __expf(bf2f(cache[(int64_t)(2 * d * K + k) * W + wl]) - amax);
-                if (uc < cum) { kk = k; break; }
-            }
-            kc[col] = kk;
-            misc[1 * TW + col] = amax + __logf(sa); // lse of the logits at x
+    h2_t mx[D];
+    float lpi[2] = {0.f, 0.f};
+    network<D, H>(N, bs, Xb, Hb, mx, lpi, lgx, MODE == 0 ? out : nullptr, W, w0);
+    if (MODE == 0) { return; }
+    // the chain state of walker col, held by wave 0
+    float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+    double sq = 0.;
+    int jb = 0, bi = 0, nacc = 0;
+    if (part == 0) {
+        m = St.m[wl];
+        S = St.S[wl];
+        sq = St.sq[wl];
+        if (St.B > 0) {
+            s0 = St.sb[wl];
+            c0 = St.cb[wl];
+            s1 = St.sb[W + wl];
+            c1 = St.cb[W + wl];
+            jb = St.jb[wl];
+            bi = St.bi[wl];
         }
+    }
+    for (int st = 0; st < nl; ++st) {
+        // the weights and target factors are read afresh each step: an empty asm makes their
+        // addresses opaque, so the compiler does not hoist those loads out of the loop (they would
+        // not fit the registers)
+        NetDev Ns = N;
+        const float *Ubase = T.U, *Umbase = T.Um;
+        asm volatile("" : "+s"(Ns.W1t), "+s"(Ns.W2t), "+s"(Ns.W3t), "+s"(Ubase), "+s"(Umbase));
+        const uint64_t t = t0 + (uint64_t)st;
+        const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
+        uint32_t o[4] = {0u, 0u, 0u, 0u};
         {
             Normals Nm;
             Nm.avail = 0;
             Nm.has_spare = false;
             Nm.call = 1;
             const uint32_t base = 0x10000000u + ((uint32_t)(part + 1) << 20);
-            for (int j = part; j < d; j += 4) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, tl, th, k0, k1); }
+            for (int j = part; j < D; j += NWV) { xps[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, tl, th, k0, k1); }
+            if (part == 0) { philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o); }
         }
         __syncthreads();
-        {
-            const int kk = kc[col];
-            for (int j = part; j < DMAX; j += 4) {
-                float v = 0.f;
-                if (j < d) {
-                    const float mu = bf2f(cache[(int64_t)(2 * d * kk + j) * W + wl]);
-                    const float ls = bf2f(cache[(int64_t)(2 * d * kk + d + j) * W + wl]);
-                    v = fmaf(__expf(ls), zs[j * TW + col], mu);
-                }
-                xps[j * TW + col] = v;
-                if (j < N.dp) { Xb[col * ldx + j] = f2bf(v * mcim::IN_SCALE); }
-            }
-        }
-        __syncthreads();
-        // log q(x' | x): part p sums coordinates j = p, p + 4, ...
-        for (int k = 0; k < K; ++k) {
-            float acc = 0.f;
-            for (int j = part; j < d; j += 4) {
-                const float mu = bf2f(cache[(int64_t)(2 * d * k + j) * W + wl]);
-                const float ls = bf2f(cache[(int64_t)(2 * d * k + d + j) * W + wl]);
-                const float z = (xps[j * TW + col] - mu) * __expf(-ls);
-                acc += -ls - 0.5f * z * z;
-            }
-            part_f[(k * 4 + part) * TW + col] = acc;
-        }
-        __syncthreads();
+        // the component from the logits at x
         if (part == 0) {
-            const float lsa = misc[1 * TW + col];
-            float comp[KMAX];
-            float cmax = -__builtin_huge_valf();
+            float amax = -__builtin_huge_valf();
+            for (int k = 0; k < K; ++k) { amax = fmaxf(amax, lgx[k * TW + col]); }
+            float e[KMAX], sa = 0.f;
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
-                comp[k] = -__builtin_huge_valf();
-                if (k < K) {
-                    const float sm = ((part_f[(k * 4 + 0) * TW + col] + part_f[(k * 4 + 1) * TW + col]) + part_f[(k * 4 + 2) * TW + col]) +
-                                     part_f[(k * 4 + 3) * TW + col];
-                    comp[k] = bf2f(cache[(int64_t)(2 * d * K + k) * W + wl]) - lsa + sm - (float)d * HALF_LOG_2PI;
-                    cmax = fmaxf(cmax, comp[k]);
-                }
+                e[k] = (k < K) ? __expf(lgx[k * TW + col] - amax) : 0.f;
+                sa += e[k];
             }
-            float sc = 0.f;
+            const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
+            int kk = K - 1;
+            float cum = 0.f;
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) { sc += (k < K) ? __expf(comp[k] - cmax) : 0.f; }
-            misc[0 * TW + col] = cmax + __logf(sc); // log q(x' | x)
-        }
-    }
-    // 2. the network at Xb (x' for a step, x for a fill)
-    hidden_layer(N.W1t, N.b1, N.dp, H, Xb, ldx, H2, ldh);
-    __syncthreads();
-    hidden_layer(N.W2t, N.b2, H, H, H2, ldh, H1, ldh);
-    __syncthreads();
-    // H1 now holds layer 2; the last layer's tile goes to Cs, which overlaps H2
-    float *Ct = reinterpret_cast<float *>(H2);
-    for (int r0 = 0; r0 < R; r0 += 64) {
-        const f32x16 acc = mfma_chunk(N.W3t, H, N.Op, r0, H1, ldh);
-        {
-            const int lane = tid & 63, wv = tid >> 6, wm = wv >> 1, wn = wv & 1;
-            const int c = wn * 32 + (lane & 31);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int rr = wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-                Ct[rr * (TW + 1) + c] = acc[i] + ((r0 + rr < N.Op) ? N.b3[r0 + rr] : 0.f);
+            for (int k = 0; k < KMAX - 1; ++k) {
+                cum += e[k];
+                if (k < K - 1 && kk == K - 1 && uc < cum) { kk = k; }
             }
+            kst[col] = kk;
         }
         __syncthreads();
-        // transform and round: means x MU_SCALE, log-scales LS_MAX tanh(o / LS_MAX), logits as they are
-        for (int rr = part; rr < 64; rr += 4) {
-            const int r = r0 + rr;
-            if (r >= R) { break; }
-            const float v = Ct[rr * (TW + 1) + col];
-            float val = v;
-            if (r < 2 * d * K) { val = ((r % (2 * d)) < d) ? mcim::MU_SCALE * v : mcim::LS_MAX * tanhf(v / mcim::LS_MAX); }
-            const uint16_t b = f2bf(val);
-            if (MODE == 0) {
-                if (live) { cache[(int64_t)r * W + w] = b; }
-            }
-            else { Pn[r * TW + col] = b; }
-        }
-        if (MODE == 1) {
-            __syncthreads();
-            // fold this chunk's components into log q(x | x'): part covers 8 coordinate pairs
-            // (component k occupies rows 2 d k .. 2 d k + 2 d - 1, so pair g = (component g / d,
-            // coordinate g % d) and a chunk of 64 rows holds 32 consecutive pairs; d is a multiple of 8)
-            const int g0 = (r0 >> 1) + part * 8;
-            const int k = g0 / d, j0 = g0 % d;
-            if (k < K) {
-                float acc = 0.f;
-                for (int q = 0; q < 8; ++q) {
-                    const int j = j0 + q;
-                    const float mu = bf2f(Pn[(2 * d * k + j) * TW + col]);
-                    const float ls = bf2f(Pn[(2 * d * k + d + j) * TW + col]);
-                    const float z = (xs[j * TW + col] - mu) * __expf(-ls);
-                    acc += -ls - 0.5f * z * z;
-                }
-                part_r[(k * 4 + (j0 >> 3)) * TW + col] = acc;
-            }
-        }
-        __syncthreads();
-    }
-    if (MODE == 0) { return; }
-    // 3. log q(x | x') and the target at x'
-    if (part == 0) {
-        float amax = -__builtin_huge_valf();
-        for (int k = 0; k < K; ++k) { amax = fmaxf(amax, bf2f(Pn[(2 * d * K + k) * TW + col])); }
-        float sa = 0.f;
-        for (int k = 0; k < K; ++k) { sa += __expf(bf2f(Pn[(2 * d * K + k) * TW + col]) - amax); }
-        const float lsa = amax + __logf(sa);
-        const int nsub = d / 8;
-        float comp[KMAX];
-        float cmax = -__builtin_huge_valf();
+        // x' = mu + sigma z from the lanes holding the chosen component
+        if (2 * wc < K) {
+            const int cc = 32 * cbw + (lane & 31);
+            const int kc = kst[cc] - 2 * wc;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            comp[k] = -__builtin_huge_valf();
-            if (k < K) {
-                float sm = 0.f;
-                for (int sp = 0; sp < nsub; ++sp) { sm += part_r[(k * 4 + sp) * TW + col]; }
-                comp[k] = bf2f(Pn[(2 * d * K + k) * TW + col]) - lsa + sm - (float)d * HALF_LOG_2PI;
-                cmax = fmaxf(cmax, comp[k]);
-            }
-        }
-        float sc = 0.f;
+            for (int rb = 0; rb < D / 8; ++rb) {
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) { sc += (k < K) ? __expf(comp[k] - cmax) : 0.f; }
-        misc[2 * TW + col] = cmax + __logf(sc); // log q(x | x')
-    }
-    {
-        // components 2 part, 2 part + 1 of the target
-        float xr[DMAX];
+                for (int g = 0; g < 4; ++g) {
+                    const int cl = (32 * rb + 8 * g) / (2 * D);
+                    const int j = ((32 * rb + 8 * g + 4 * (lane >> 5)) % (2 * D)) >> 1;
+                    if (kc == cl) {
 #pragma unroll
-        for (int j = 0; j < DMAX; ++j) { xr[j] = xps[j * TW + col]; }
-#pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            const int k = 2 * part + kk;
-            const float *Uk = T.U + (int64_t)k * T.d * T.d;
-            const float *bk = T.Um + k * T.d;
-            float qk = 0.f;
-#pragma unroll
-            for (int i = 0; i < DMAX; ++i) {
-                if (i < T.d) {
-                    float r = -bk[i];
-#pragma unroll
-                    for (int j = i; j < DMAX; ++j) {
-                        if (j < T.d) { r = fmaf(Uk[i * T.d + j], xr[j], r); }
+                        for (int h = 0; h < 2; ++h) {
+                            const h2_t p = mx[pair_index(rb, g, h)];
+                            float *z = xps + (j + h) * TW + cc;
+                            *z = fmaf(*z, __builtin_amdgcn_rcpf((float)p[1]), (float)p[0]);
+                        }
                     }
-                    qk = fmaf(r, r, qk);
                 }
             }
-            qn[k * TW + col] = qk;
+        }
+        __syncthreads();
+        fold<D>(mx, lpi, xps, cf, K);
+        for (int j = part; j < C::DP; j += NWV) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
+        {
+            // target component wv at x' on the matrix cores (FP32 products and sums):
+            // U_k x' - U_k mu_k for both walker halves, then |.|^2 over the 32 rows
+            static_assert(D == 32 || D == 16 || D == 8, "dimension");
+            float2 acc2[2];
+            target_q<D>(Ubase, Umbase, wv, xps, lane, acc2);
+            if (lane < 32) {
+                qn[wv * TW + lane] = acc2[0].x;
+                qn[wv * TW + 32 + lane] = acc2[1].x;
+            }
+        }
+        __syncthreads();
+        float logq_f = 0.f;
+        if (part == 0) { logq_f = mixture_logq(lgx, cf, K, D, col); }
+        __syncthreads();
+        // the network at x' and log q(x | x')
+        h2_t mx2[D];
+        float lpi2[2] = {0.f, 0.f};
+        network<D, H>(Ns, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
+        fold<D>(mx2, lpi2, xs, cf, K);
+        __syncthreads();
+        if (part == 0) {
+            const float logq_r = mixture_logq(lg, cf, K, D, col);
+            float mn = -__builtin_huge_valf();
+            float e[KMAX];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                e[k] = T.c[k] - 0.5f * qn[k * TW + col];
+                mn = fmaxf(mn, e[k]);
+            }
+            float Sn = 0.f;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) { Sn += __expf(e[k] - mn); }
+            const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
+            const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - (m + __logf(S)) + (logq_r - logq_f));
+            af[col] = acc ? 1 : 0;
+            if (acc) {
+                double s2 = 0.;
+                for (int j = 0; j < D; ++j) {
+                    const float v = xps[j * TW + col];
+                    s2 = fma((double)v, (double)v, s2);
+                }
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) { qx[k * TW + col] = qn[k * TW + col]; }
+                for (int k = 0; k < K; ++k) { lgx[k * TW + col] = lg[k * TW + col]; }
+                m = mn;
+                S = Sn;
+                sq = s2;
+                ++nacc;
+            }
+            if (live && St.B > 0) { accumulate(St, w, true, sq, acc ? xps[col] : xs[col], s0, c0, s1, c1, jb, bi); }
+        }
+        __syncthreads();
+        // accepted walkers: position and mixture at x take the proposal's
+        if (af[col]) {
+            for (int j = part; j < D; j += NWV) { xs[j * TW + col] = xps[j * TW + col]; }
+        }
+        {
+            const bool a = af[32 * cbw + (lane & 31)] != 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) { mx[i] = a ? mx2[i] : mx[i]; }
+            lpi[0] = a ? lpi2[0] : lpi[0];
+            lpi[1] = a ? lpi2[1] : lpi[1];
         }
     }
+    // the run's end: accepted walkers' position and quadratic forms, every walker's chain state
+    if (part == 0) { anyacc[col] = nacc; }
     __syncthreads();
-    if (part == 0) {
-        float mn = -__builtin_huge_valf();
-        float e[KMAX];
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            e[k] = T.c[k] - 0.5f * qn[k * TW + col];
-            mn = fmaxf(mn, e[k]);
-        }
-        float Sn = 0.f;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) { Sn += __expf(e[k] - mn); }
-        uint32_t o[4];
-        philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
-        const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
-        const float logp_old = St.m[wl] + __logf(St.S[wl]);
-        const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - logp_old + (misc[2 * TW + col] - misc[0 * TW + col]));
-        accf[col] = acc ? 1 : 0;
-        if (acc) {
-            double sq = 0.;
-            for (int j = 0; j < d; ++j) {
-                const float v = xps[j * TW + col];
-                St.x[(int64_t)j * W + w] = v;
-                sq = fma((double)v, (double)v, sq);
-            }
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * W + w] = qn[k * TW + col]; }
-            St.m[w] = mn;
-            St.S[w] = Sn;
-            St.sq[w] = sq;
-            nacc_ind[w] += 1;
-        }
-        if (live && St.B > 0) {
-            float s0 = St.sb[w], c0 = St.cb[w], s1 = St.sb[W + w], c1 = St.cb[W + w];
-            int jb = St.jb[w], bi = St.bi[w];
-            accumulate(St, w, true, St.sq[w], acc ? xps[col] : xs[col], s0, c0, s1, c1, jb, bi);
+    if (live && anyacc[col] > 0) {
+        for (int j = part; j < D; j += NWV) { St.x[(int64_t)j * W + w] = xs[j * TW + col]; }
+        St.q[(int64_t)part * W + w] = qx[part * TW + col];
+    }
+    if (part == 0 && live) {
+        St.m[w] = m;
+        St.S[w] = S;
+        St.sq[w] = sq;
+        if (St.B > 0) {
             St.sb[w] = s0;
             St.cb[w] = c0;
             St.sb[W + w] = s1;
@@ -867,10 +1036,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             St.jb[w] = jb;
             St.bi[w] = bi;
         }
-    }
-    __syncthreads();
-    if (accf[col]) {
-        for (int r = part; r < R; r += 4) { cache[(int64_t)r * W + w] = Pn[r * TW + col]; }
+        nacc_ind[w] += nacc;
     }
 }
 
@@ -940,29 +1106,50 @@ hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, co
     return hipGetLastError();
 }
 
-size_t learned_lds_bytes(const NetDev &N)
+namespace
 {
-    const size_t f = sizeof(float) * ((size_t)2 * DMAX * TW + (size_t)2 * KMAX * 4 * TW + (size_t)KMAX * TW + 4 * TW) + sizeof(int) * 2 * TW;
-    const size_t hb = sizeof(uint16_t) * ((size_t)TW * (N.dp + 8) + (size_t)2 * TW * (N.H + 8) + (size_t)N.O * TW);
-    return f + hb;
+template <int MODE, int D, int H>
+hipError_t launch_learned_t(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, uint64_t t,
+                            int nl, uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
+{
+    const size_t lds = learned_lds<D, H>();
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_learned<MODE, D, H>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL((k_lp_learned<MODE, D, H>), dim3((unsigned)((S.W + TW - 1) / TW)), dim3(TT), lds, s, N, T, S, xin, out, t, nl, k0,
+                       k1, nacc_ind);
+    return hipGetLastError();
 }
 
-hipError_t launch_lp_learned(int mode, const NetDev &N, const Target<float> &T, const State<float> &S, uint16_t *cache, uint64_t t,
-                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
+template <int MODE>
+hipError_t launch_learned_m(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, uint64_t t,
+                            int nl, uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
 {
-    const size_t lds = learned_lds_bytes(N);
-    const dim3 grid((unsigned)((S.W + TW - 1) / TW));
-    if (mode == 0) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_learned<0>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) { return e; }
-        hipLaunchKernelGGL(k_lp_learned<0>, grid, dim3(TT), lds, s, N, T, S, cache, t, k0, k1, nacc_ind);
-    }
-    else {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_learned<1>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) { return e; }
-        hipLaunchKernelGGL(k_lp_learned<1>, grid, dim3(TT), lds, s, N, T, S, cache, t, k0, k1, nacc_ind);
-    }
-    return hipGetLastError();
+#define MCIL_CASE(DD, HH)                                                                                                                  \
+    if (N.d == DD && N.H == HH) { return launch_learned_t<MODE, DD, HH>(N, T, S, xin, out, t, nl, k0, k1, nacc_ind, s); }
+    MCIL_CASE(8, 64)
+    MCIL_CASE(8, 128)
+    MCIL_CASE(16, 64)
+    MCIL_CASE(16, 128)
+    MCIL_CASE(32, 64)
+    MCIL_CASE(32, 128)
+#undef MCIL_CASE
+    return hipErrorInvalidValue;
+}
+} // namespace
+
+bool learned_supported(int d, int H, int K) { return (d == 8 || d == 16 || d == 32) && (H == 64 || H == 128) && K >= 1 && K <= KMAX; }
+
+hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                             int32_t *nacc_ind, hipStream_t s)
+{
+    if (!learned_supported(N.d, N.H, N.K) || T.d != N.d || nl < 1) { return hipErrorInvalidValue; }
+    return launch_learned_m<1>(N, T, S, nullptr, nullptr, t0, nl, k0, k1, nacc_ind, s);
+}
+
+hipError_t launch_lp_network(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, hipStream_t s)
+{
+    if (!learned_supported(N.d, N.H, N.K)) { return hipErrorInvalidValue; }
+    return launch_learned_m<0>(N, T, S, xin, out, 0, 0, 0, 0, nullptr, s);
 }
 
 template <class R>

@@ -52,19 +52,24 @@ hipError_t launch_lp_init(const Target<R> &T, const State<R> &S, uint32_t k0, ui
 template <class R>
 hipError_t launch_lp_rw(const Target<R> &T, const State<R> &S, R step, uint64_t t0, int64_t nsteps, uint32_t k0, uint32_t k1,
                         hipStream_t s);
-// the network for the fused learned move: BF16 weights stored [out][in] (W1t [H][dp] with the
-// input padded to dp = max(16, d) columns, W2t [H][H], W3t [Op][H]), FP32 biases
+// the network for the fused learned move: BF16 weights stored [out][in]: W1t [H][dp] with the input
+// padded to dp = max(16, d) columns, W2t [H][H], W3t [Op][H] with Op = 2 d KMAX + 32 rows: component
+// k at rows 2 d k + 2 j (mean of coordinate j) and 2 d k + 2 j + 1 (log-scale), zero rows for
+// components k >= K, the K logits from row 2 d KMAX; FP32 biases in the same row order
 struct NetDev
 {
     int d, dp, H, K, O, Op;
     const uint16_t *W1t, *W2t, *W3t;
     const float *b1, *b2, *b3;
 };
-size_t learned_lds_bytes(const NetDev &N);
-// mode 0: fill the BF16 mixture cache [O][W] at the current positions; mode 1: one learned move
-// at step t for every walker (FP32 chain)
-hipError_t launch_lp_learned(int mode, const NetDev &N, const Target<float> &T, const State<float> &S, uint16_t *cache, uint64_t t,
-                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s);
+// the fused kernel exists for d in {8, 16, 32}, H in {64, 128}, 1 <= K <= KMAX
+bool learned_supported(int d, int H, int K);
+// nl consecutive learned moves, steps t0 .. t0 + nl - 1, for every walker (FP32 chain)
+hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                             int32_t *nacc_ind, hipStream_t s);
+// the same kernel's network at positions xin [d][S.W], the mixture to out [K (1 + 2 d)][S.W]
+// (per component d means and d log-scales, transformed; then K logits)
+hipError_t launch_lp_network(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, hipStream_t s);
 // a whole run of the chain mixing random-walk moves and the independence proposal Q
 // ([Kq][1 + 3 d] rows: log pi_k - sum log sigma_k, mu_k, sigma_k, 1/sigma_k; then Kq cumulative
 // weights), FP32

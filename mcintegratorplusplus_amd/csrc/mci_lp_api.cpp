@@ -276,7 +276,8 @@ struct mci_lp
     // learned move
     mci_mlp *net = nullptr;
     DBuf<float> snap, nb1, nb2, nb3;
-    DBuf<uint16_t> nW1t, nW2t, nW3t, cache;
+    DBuf<uint16_t> nW1t, nW2t, nW3t;
+    uint64_t train_iter = 0; // minibatches drawn so far (each draws its own pairs)
     int nsnap = 0;
     double same_frac = 0.5;
     std::vector<float> ctx0;   // mean of the last snapshot: the network input of the independence proposal
@@ -509,7 +510,11 @@ mci_status net_dev(mci_lp *lp, mcil::NetDev &N)
 {
     TRY(net_ready(lp));
     mci_mlp *n = lp->net;
-    const int d = n->d, H = n->H, Op = n->Op, dp = std::max(16, d);
+    const int d = n->d, H = n->H, K = n->K, dp = std::max(16, d);
+    if (!mcil::learned_supported(d, H, K)) {
+        return fail(MCI_E_INVALID_ARGUMENT, "the learned move needs d in {8, 16, 32}, hidden in {64, 128} and 1 <= ncomp <= %d", mcil::KMAX);
+    }
+    const int Op = 2 * d * mcil::KMAX + 32, L0 = 2 * d * K;
     std::vector<float> p((size_t)n->npad);
     HIPCHK(hipStreamSynchronize(n->stream));
     HIPCHK(hipMemcpy(p.data(), n->p.p, p.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -518,38 +523,48 @@ mci_status net_dev(mci_lp *lp, mcil::NetDev &N)
         std::memcpy(&u, &f, 4);
         return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
     };
-    std::vector<uint16_t> w1((size_t)H * dp, 0), w2((size_t)H * H), w3((size_t)Op * H);
+    std::vector<uint16_t> w1((size_t)H * dp, 0), w2((size_t)H * H), w3((size_t)Op * H, 0);
+    std::vector<float> b3((size_t)Op, 0.f);
     for (int h = 0; h < H; ++h) {
         for (int i = 0; i < d; ++i) { w1[(size_t)h * dp + i] = bf(p[(size_t)(n->oW1 + (int64_t)i * H + h)]); }
         for (int i = 0; i < H; ++i) { w2[(size_t)h * H + i] = bf(p[(size_t)(n->oW2 + (int64_t)i * H + h)]); }
     }
-    for (int o = 0; o < Op; ++o) {
-        for (int h = 0; h < H; ++h) { w3[(size_t)o * H + h] = bf(p[(size_t)(n->oW3 + (int64_t)h * Op + o)]); }
+    // network output row o -> kernel row: (component k, mean j) -> 2 d k + 2 j, (k, log-scale j)
+    // -> 2 d k + 2 j + 1, logit k -> 2 d KMAX + k
+    auto dst = [&](int o) {
+        if (o >= L0) { return 2 * d * mcil::KMAX + (o - L0); }
+        const int k = o / (2 * d), r = o % (2 * d);
+        return 2 * d * k + (r < d ? 2 * r : 2 * (r - d) + 1);
+    };
+    for (int o = 0; o < n->O; ++o) {
+        const int r = dst(o);
+        for (int h = 0; h < H; ++h) { w3[(size_t)r * H + h] = bf(p[(size_t)(n->oW3 + (int64_t)h * n->Op + o)]); }
+        b3[(size_t)r] = p[(size_t)(n->ob3 + o)];
     }
     TRY(up(lp->nW1t, w1));
     TRY(up(lp->nW2t, w2));
     TRY(up(lp->nW3t, w3));
     TRY(up(lp->nb1, std::vector<float>(p.begin() + n->ob1, p.begin() + n->ob1 + H)));
     TRY(up(lp->nb2, std::vector<float>(p.begin() + n->ob2, p.begin() + n->ob2 + H)));
-    TRY(up(lp->nb3, std::vector<float>(p.begin() + n->ob3, p.begin() + n->ob3 + Op)));
-    N = {d, dp, H, n->K, n->O, Op, lp->nW1t.p, lp->nW2t.p, lp->nW3t.p, lp->nb1.p, lp->nb2.p, lp->nb3.p};
+    TRY(up(lp->nb3, b3));
+    N = {d, dp, H, K, n->O, Op, lp->nW1t.p, lp->nW2t.p, lp->nW3t.p, lp->nb1.p, lp->nb2.p, lp->nb3.p};
     return MCI_OK;
 }
 
-// the chain mixing random-walk moves (runs of them in one launch) with learned moves (one
-// fused launch each); the move type of step t is the same for every walker
+// the chain mixing random-walk moves with learned moves: cycles of LP_CYCLE steps, the first
+// LP_CYCLE - nl random-walk moves (one launch), then nl = round(pmix LP_CYCLE) learned moves (one
+// fused launch).  Each move leaves p invariant, so does the cycle.
+constexpr int LP_CYCLE = 32;
 mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
 {
-    if (lp->d != 8 && lp->d != 16 && lp->d != 32) { return fail(MCI_E_INVALID_ARGUMENT, "the learned move needs d in {8, 16, 32}"); }
+    mcil::NetDev N;
+    TRY(net_dev(lp, N));
     mcil::State<float> S;
     const int B = (int)(nsteps / nblocks);
     TRY(state_of<float>(lp, B, nblocks, S));
-    mcil::NetDev N;
-    TRY(net_dev(lp, N));
     const mcil::Target<float> T = target_of<float>(lp);
     const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
     const size_t W = (size_t)lp->W;
-    HIPCHK(lp->cache.ensure((size_t)N.O * W));
     auto t0 = clk::now();
     HIPCHK(mcil::launch_lp_init<float>(T, S, k0, k1, lp->stream));
     if (lp->step32 <= 0.) { lp->step32 = 1.0; }
@@ -559,26 +574,22 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
     HIPCHK(hipMemsetAsync(lp->nacc_ind.p, 0, sizeof(int32_t) * W, lp->stream));
     HIPCHK(hipStreamSynchronize(lp->stream));
     t0 = clk::now();
-    HIPCHK(mcil::launch_lp_learned(0, N, T, S, lp->cache.p, 0, k0, k1, lp->nacc_ind.p, lp->stream));
     const uint64_t tb = lp->t_next;
-    auto learned = [&](int64_t st) {
-        uint32_t o[4];
-        const uint64_t t = tb + (uint64_t)st;
-        philox_host(0x20000000u, 0xffffffffu, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
-        return (float)(o[0] >> 8) * 0x1p-24f < (float)pmix;
-    };
+    // cycles of LP_CYCLE steps: random-walk moves first, then learned moves (one launch each)
+    const int nl = std::min(LP_CYCLE, std::max(1, (int)std::lround(pmix * LP_CYCLE))), nr = LP_CYCLE - nl;
     int64_t nind = 0, s = 0;
     while (s < nsteps) {
-        if (!learned(s)) {
-            int64_t e = s + 1;
-            while (e < nsteps && !learned(e)) { ++e; }
-            HIPCHK(mcil::launch_lp_rw<float>(T, S, (float)lp->step32, tb + (uint64_t)s, e - s, k0, k1, lp->stream));
-            s = e;
-            continue;
+        const int64_t r = std::min<int64_t>(nr, nsteps - s);
+        if (r > 0) {
+            HIPCHK(mcil::launch_lp_rw<float>(T, S, (float)lp->step32, tb + (uint64_t)s, r, k0, k1, lp->stream));
+            s += r;
         }
-        HIPCHK(mcil::launch_lp_learned(1, N, T, S, lp->cache.p, tb + (uint64_t)s, k0, k1, lp->nacc_ind.p, lp->stream));
-        ++nind;
-        ++s;
+        const int64_t l = std::min<int64_t>(nl, nsteps - s);
+        if (l > 0) {
+            HIPCHK(mcil::launch_lp_learned(N, T, S, tb + (uint64_t)s, (int)l, k0, k1, lp->nacc_ind.p, lp->stream));
+            nind += l;
+            s += l;
+        }
     }
     lp->t_next += (uint64_t)nsteps;
     HIPCHK(hipStreamSynchronize(lp->stream));
@@ -633,7 +644,7 @@ mci_status indep_params(mci_lp *lp)
         double lsum = 0.;
         for (int j = 0; j < d; ++j) {
             const double mu = mcim::MU_SCALE * col(2 * d * k + j);
-            const double ls = mcim::LS_MAX * std::tanh(col(2 * d * k + d + j) / mcim::LS_MAX);
+            const double ls = std::fmin(std::fmax(col(2 * d * k + d + j), -(double)mcim::LS_MAX), (double)mcim::LS_MAX);
             row[1 + j] = (float)mu;
             row[1 + d + j] = (float)std::exp(ls);
             row[1 + 2 * d + j] = (float)std::exp(-ls);
@@ -987,8 +998,8 @@ mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *los
         return MCI_OK;
     };
     for (int it = 0; it < iters; ++it) {
-        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, (uint64_t)it, k0, k1, (float)lp->same_frac, n->X.p, n->Y.p,
-                                     n->stream));
+        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, lp->train_iter++, k0, k1, (float)lp->same_frac, n->X.p,
+                                     n->Y.p, n->stream));
         TRY(mlp_fwd_bwd(n, batch));
         if (it == 0) { TRY(mean_loss(lp->loss_first)); }
         TRY(mlp_adam(n, (float)lr));
@@ -1036,6 +1047,25 @@ mci_status mci_lp_get_network(mci_lp *lp, mci_mlp **m)
     TRY(lp_device(lp));
     TRY(net_ready(lp));
     *m = lp->net;
+    return MCI_OK;
+}
+
+mci_status mci_lp_eval_network(mci_lp *lp, int64_t n, const float *x, float *out)
+{
+    CHECKL(lp);
+    if (n < 1 || n > (int64_t)1 << 24 || !x || !out) { return fail(MCI_E_INVALID_ARGUMENT, "need 1 <= n <= 2^24 and x, out"); }
+    TRY(lp_device(lp));
+    mcil::NetDev N;
+    TRY(net_dev(lp, N));
+    DBuf<float> dx, dout;
+    HIPCHK(dx.ensure((size_t)lp->d * n));
+    HIPCHK(dout.ensure((size_t)N.O * n));
+    HIPCHK(hipMemcpy(dx.p, x, sizeof(float) * (size_t)lp->d * n, hipMemcpyHostToDevice));
+    mcil::State<float> S{};
+    S.W = n;
+    HIPCHK(mcil::launch_lp_network(N, target_of<float>(lp), S, dx.p, dout.p, lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    HIPCHK(hipMemcpy(out, dout.p, sizeof(float) * (size_t)N.O * n, hipMemcpyDeviceToHost));
     return MCI_OK;
 }
 

@@ -114,7 +114,7 @@ __global__ void k_mixture_head(const float *out, int ld, const float *y, int n, 
         float c = o[(int64_t)(L0 + k) * ld] - lsa;
         for (int j = 0; j < d; ++j) {
             const float mu = MU_SCALE * o[(int64_t)(2 * d * k + j) * ld];
-            const float ls = LS_MAX * tanhf(o[(int64_t)(2 * d * k + d + j) * ld] / LS_MAX);
+            const float ls = fminf(fmaxf(o[(int64_t)(2 * d * k + d + j) * ld], -LS_MAX), LS_MAX);
             const float z = (yr[(int64_t)j * ld] - mu) * __expf(-ls);
             c += -ls - HALF_LOG_2PI - 0.5f * z * z;
         }
@@ -134,12 +134,12 @@ __global__ void k_mixture_head(const float *out, int ld, const float *y, int n, 
         for (int j = 0; j < d; ++j) {
             const int64_t im = (int64_t)(2 * d * k + j) * ld, il = (int64_t)(2 * d * k + d + j) * ld;
             const float mu = MU_SCALE * o[im];
-            const float t = tanhf(o[il] / LS_MAX);
-            const float ls = LS_MAX * t;
+            const float ol = o[il];
+            const float ls = fminf(fmaxf(ol, -LS_MAX), LS_MAX);
             const float isg = __expf(-ls);
             const float z = (yr[(int64_t)j * ld] - mu) * isg;
             const float gm = inv_n * (-gam * z * isg) * MU_SCALE;
-            const float gl = inv_n * gam * (1.0f - z * z) * (1.0f - t * t);
+            const float gl = (fabsf(ol) < LS_MAX) ? inv_n * gam * (1.0f - z * z) : 0.0f;
             g[im] = gm;
             g[il] = gl;
             gb[im] = f2bf(gm);

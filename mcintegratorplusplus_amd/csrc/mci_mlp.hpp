@@ -16,7 +16,7 @@ typedef uint16_t bf16_t; // bfloat16 bits
 // mean rows then d log-scale pre-activation rows (rows 2 d k .. 2 d k + 2 d - 1), then K logits.
 constexpr float IN_SCALE = 0.25f;  // network input = x * IN_SCALE
 constexpr float MU_SCALE = 4.0f;   // mean = MU_SCALE * output
-constexpr float LS_MAX = 3.0f;     // log-scale = LS_MAX tanh(output / LS_MAX)
+constexpr float LS_MAX = 3.0f;     // log-scale = output clamped to [-LS_MAX, LS_MAX]
 
 enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2 };
 

@@ -28,6 +28,7 @@ _SIGS = {
     "mci_lp_train": [_P, _I, _I, _D, _DP, _DP], "mci_lp_get_results": [_P, _DP, _I],
     "mci_lp_get_timings": [_P, _DP, _I], "mci_lp_get_walker_means": [_P, _DP, _I64],
     "mci_lp_get_network": [_P, C.POINTER(_P)], "mci_lp_set_training_pairs": [_P, _D],
+    "mci_lp_eval_network": [_P, _I64, _FP, _FP],
 }
 _bound = None
 
@@ -105,6 +106,7 @@ class LearnedProposalSampler:
         self.weights, self.means, self.covs = _d(weights), _d(means), _d(covs)
         self.ncomp, self.d = self.means.shape
         self.nwalkers = nwalkers
+        self.hidden, self.net_ncomp = 128, 8
         h = _P()
         _check(_lib().mci_lp_create(self.d, self.ncomp, self.weights.ctypes.data_as(_DP), self.means.ctypes.data_as(_DP),
                                     self.covs.ctypes.data_as(_DP), nwalkers, C.byref(h)))
@@ -121,6 +123,7 @@ class LearnedProposalSampler:
 
     def setNetwork(self, hidden=128, ncomp=8):
         _check(_lib().mci_lp_set_network(self._h, hidden, ncomp))
+        self.hidden, self.net_ncomp = hidden, ncomp
 
     def run(self, mode, nsteps, nblocks=64, pmix=0.5):
         a, e = np.zeros(2), np.zeros(2)
@@ -153,10 +156,21 @@ class LearnedProposalSampler:
         _check(_lib().mci_lp_get_walker_means(self._h, m.ctypes.data_as(_DP), m.size))
         return m.reshape(2, self.nwalkers)
 
+    def evalNetwork(self, x):
+        """the mixture as the learned move computes it at positions x [n][d]: [n][K (1 + 2 d)]
+        (per component d means, d log-scales; then K logits)"""
+        x = _f(x)
+        n = x.shape[0]
+        net = self.network()
+        out = np.zeros((net.nout, n), np.float32)
+        xt = np.ascontiguousarray(x.T)
+        _check(_lib().mci_lp_eval_network(self._h, n, xt.ctypes.data_as(_FP), out.ctypes.data_as(_FP)))
+        return out.T.copy()
+
     def network(self):
         h = _P()
         _check(_lib().mci_lp_get_network(self._h, C.byref(h)))
-        return Network(self.d, _handle=h)
+        return Network(self.d, self.hidden, self.net_ncomp, _handle=h)
 
 
 def mixture_target(d=32, ncomp=8, seed=5, mean_sd=3.0, eig=(0.1, 1.0)):

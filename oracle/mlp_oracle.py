@@ -3,7 +3,7 @@ checker, never by the product).
 
 Restates, in float64 numpy, what mcintegratorplusplus_amd/csrc/mci_mlp.hip and mci_lp_api.cpp
 compute: the MLP d -> H -> H -> K (1 + 2 d) with SiLU, the diagonal Gaussian-mixture head
-(outputs per component k: d means / 4 then d log-scale pre-activations, log-scale = 3 tanh(o / 3);
+(outputs per component k: d means / 4 then d log-scale pre-activations, log-scale = o clamped to [-3, 3];
 then K logits; input = x / 4), the mean negative
 log-likelihood of targets y under q(. | x), and its gradient by backpropagation.  With
 emulate=True the operands of every matrix product are rounded to bfloat16 (round to nearest
@@ -89,7 +89,7 @@ def mixture_logq(out, y, d, K):
     """log q(y | parameters in out), rows of out [n][K (1 + 2 d)]"""
     a, om, ol = _split(out, d, K)
     mu = MU_SCALE * om
-    ls = LS_MAX * np.tanh(ol / LS_MAX)
+    ls = np.clip(ol, -LS_MAX, LS_MAX)
     z = (np.asarray(y, np.float64)[:, None, :] - mu) * np.exp(-ls)
     comp = a - _lse(a, 1)[:, None] + np.sum(-ls - HALF_LOG_2PI - 0.5 * z * z, axis=2)
     return _lse(comp, 1)
@@ -100,8 +100,8 @@ def head(out, y, d, K):
     n = out.shape[0]
     a, om, ol = _split(out, d, K)
     mu = MU_SCALE * om
-    t = np.tanh(ol / LS_MAX)
-    ls = LS_MAX * t
+    inside = np.abs(ol) < LS_MAX
+    ls = np.clip(ol, -LS_MAX, LS_MAX)
     isg = np.exp(-ls)
     z = (np.asarray(y, np.float64)[:, None, :] - mu) * isg
     lsa = _lse(a, 1)
@@ -111,7 +111,7 @@ def head(out, y, d, K):
     pi = np.exp(a - lsa[:, None])
     g_a = pi - gam
     g_mu = (-gam[:, :, None] * z * isg) * MU_SCALE
-    g_ls = gam[:, :, None] * (1.0 - z * z) * (1.0 - t * t)
+    g_ls = gam[:, :, None] * (1.0 - z * z) * inside
     return -L, _join(g_a, g_mu, g_ls) / n
 
 

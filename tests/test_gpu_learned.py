@@ -46,6 +46,35 @@ def test_network_forward_and_gradient_match_oracle(d, H, K, n):
     assert np.array_equal(net.params(), q)
 
 
+@pytest.mark.parametrize("d,H,K,n", [(32, 128, 8, 1000), (8, 64, 3, 200), (16, 128, 5, 130)])
+def test_fused_move_network_matches_oracle(d, H, K, n):
+    """the network as the learned move's fused kernel evaluates it (last layer in registers, rows
+    reordered as (mean, log-scale) pairs, logits from a separate block) against the oracle"""
+    from mcintegratorplusplus_amd.learned import LearnedProposalSampler, mixture_target
+    print("fused network", d, H, K, n, flush=True)
+    w, mu, cov, _ = mixture_target(d=d, ncomp=3, seed=2)
+    s = LearnedProposalSampler(w, mu, cov, 256)
+    s.setNetwork(H, K)
+    p = s.network().params().astype(np.float64)
+    rng = np.random.default_rng(d + K)
+    x = rng.normal(0.0, 3.0, (n, d)).astype(np.float32)
+    got = s.evalNetwork(x)
+    f, _ = M.forward(p, x, d, H, K, emulate=True)
+    out = f["OUT"]
+    assert got.shape == out.shape
+    assert np.all(np.isfinite(got))
+    # the move keeps (mean, 1 / sigma) as halves rounded toward zero (relative error < 2^-10), so
+    # the means agree to that and the log-scales to about 2^-10 absolute; the logits are FP32
+    for k in range(K):
+        b = 2 * d * k
+        mu_ref = M.MU_SCALE * out[:, b:b + d]
+        ls_ref = np.clip(out[:, b + d:b + 2 * d], -M.LS_MAX, M.LS_MAX)
+        mu, ls = got[:, b:b + d], got[:, b + d:b + 2 * d]
+        assert np.all(np.abs(mu - mu_ref) <= 2.0 ** -9 * np.abs(mu_ref) + 2e-3 * np.abs(mu_ref).max()), k
+        assert np.max(np.abs(ls - ls_ref)) < 3e-3, k
+    assert _rel(got[:, 2 * d * K:], out[:, 2 * d * K:]) < 2e-3
+
+
 def _small_target():
     from mcintegratorplusplus_amd.learned import mixture_target
     return mixture_target(d=8, ncomp=3, seed=3, mean_sd=2.0, eig=(0.2, 1.0))

@@ -5,6 +5,8 @@ import os
 import sys
 
 import numpy as np
+import pytest
+from scipy.special import logsumexp
 from scipy.stats import norm
 
 from mcitest import ROOT
@@ -19,11 +21,17 @@ def _params(d, H, K, seed):
     return rng.normal(0.0, 0.4, n)
 
 
-def test_backprop_matches_finite_differences():
+@pytest.mark.parametrize("scale", [1.0, 4.0])
+def test_backprop_matches_finite_differences(scale):
     d, H, K, n = 3, 8, 2, 6
     rng = np.random.default_rng(1)
     p = _params(d, H, K, 2)
+    p[-K * (1 + 2 * d):] *= scale  # scale 4: output biases that push log-scales beyond the clamp
     x, y = rng.normal(size=(n, d)) * 2, rng.normal(size=(n, d)) * 2
+    ol = M._split(M.forward(p, x, d, H, K, False)[0]["OUT"], d, K)[2]
+    assert np.min(np.abs(np.abs(ol) - M.LS_MAX)) > 1e-3  # no output at the clamp's kink
+    if scale > 1.0:
+        assert np.any(np.abs(ol) > M.LS_MAX) and np.any(np.abs(ol) < M.LS_MAX)
     loss, g = M.loss_grad(p, x, y, d, H, K, emulate=False)
     num = np.zeros_like(p)
     h = 1e-6
@@ -39,7 +47,7 @@ def test_backprop_matches_finite_differences():
 def test_mixture_density_matches_scipy():
     d, K, n = 4, 3, 5
     rng = np.random.default_rng(3)
-    out = rng.normal(size=(n, K * (1 + 2 * d)))
+    out = rng.normal(size=(n, K * (1 + 2 * d))) * 2.5  # some log-scales beyond the clamp
     y = rng.normal(size=(n, d)) * 3
     got = M.mixture_logq(out, y, d, K)
     a = out[:, 2 * d * K:]
@@ -47,8 +55,9 @@ def test_mixture_density_matches_scipy():
     pi /= pi.sum(1, keepdims=True)
     comp = out[:, :2 * d * K].reshape(n, K, 2 * d)
     mu = 4.0 * comp[:, :, :d]
-    sig = np.exp(3.0 * np.tanh(comp[:, :, d:] / 3.0))
-    want = np.log(np.sum(pi * np.exp(np.sum(norm.logpdf(y[:, None, :], mu, sig), axis=2)), axis=1))
+    assert np.any(np.abs(comp[:, :, d:]) > 3.0)
+    sig = np.exp(np.clip(comp[:, :, d:], -3.0, 3.0))
+    want = logsumexp(np.log(pi) + np.sum(norm.logpdf(y[:, None, :], mu, sig), axis=2), axis=1)
     assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
 
 
