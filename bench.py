@@ -177,36 +177,91 @@ def run_batched(reps=5):
             "frac_abs_z_below_3": float(np.mean(np.abs(z) < 3)), "mean_z2": float(np.mean(z * z))}
 
 
-def run_learned():
+def cpu_learned_target(w, mu, cov, var, nsteps, budget_s=10.0):
+    """The reference CPU path (oracle/_ref/mci_ref_fast) on the learned proposal's target: the
+    full-covariance mixture as a reference sampling-function plugin with incremental
+    single-coordinate updates, one MCI per walker (exact start, own step calibration), nsteps
+    steps; P processes.  ESS from the spread of the walker means, as on the GPU."""
+    binary = os.path.join(ROOT, "oracle", "_ref", "mci_ref_fast")
+    if not os.path.exists(binary):
+        return None
+    K, d = mu.shape
+    P = np.linalg.inv(cov)
+    c = np.log(w) - 0.5 * np.array([np.linalg.slogdet(2 * np.pi * cv)[1] for cv in cov])
+    L = np.linalg.cholesky(cov)
+    p = np.concatenate([[K], w, mu.ravel(), P.ravel(), c, L.ravel()])
+    a = " ".join(float(v).hex() for v in p)
+    bs = max(1, nsteps // 64)
+    script = f"create {d}\nmove 0 1 1\nndecorr 0\npdf 5 {len(p)} {a}\nobs 2 0 {bs} 1 1 4\nobs 7 1 0 {bs} 1 1 4\nendbench\n"
+    procs = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.time()
+    subprocess.run([binary], input=f"bench 4 {nsteps} 5001\n" + script, capture_output=True, text=True, check=True)
+    per_walker = max((time.time() - t0) / 4, 1e-4)
+    per_proc = int(min(20000, max(8, budget_s / per_walker)))
+    t0 = time.time()
+    ps = [subprocess.Popen([binary], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True) for _ in range(procs)]
+    for i, pr in enumerate(ps):
+        pr.stdin.write(f"bench {per_proc} {nsteps} {700000 + i * per_proc}\n" + script)
+        pr.stdin.close()
+    outs = [json.loads([l for l in pr.stdout.read().splitlines() if l.startswith("{")][-1]) for pr in ps]
+    for pr in ps:
+        pr.wait()
+    wall = time.time() - t0
+    nw = sum(o["walkers"] for o in outs)
+    s1 = np.sum([[float.fromhex(v) for v in o["avg_sum"]] for o in outs], axis=0)
+    s2 = np.sum([[float.fromhex(v) for v in o["avg2_sum"]] for o in outs], axis=0)
+    mean = s1 / nw
+    se = np.sqrt(np.maximum(s2 / nw - mean * mean, 0.0) / (nw - 1))
+    ess_s = var / se ** 2 / wall
+    return {"walkers": nw, "steps": nsteps, "processes": procs, "wall_s": wall, "err": list(se),
+            "walker_steps_per_s": nw * nsteps / wall, "ess_per_s": list(ess_s)}
+
+
+def run_learned(with_cpu=True):
     """d = 32 mixture of 8 rotated anisotropic Gaussians, 65,536 walkers x 10,240 steps: the plain
-    FP64 random-walk chain against the FP32 chain mixing random-walk moves 50/50 with moves drawn
-    from the trained network (MH-corrected); ESS = Var_p(f) / err^2 with the exact variance."""
+    FP64 random-walk chain against the FP32 chain that mixes random-walk moves with moves drawn
+    from the trained network (MH-corrected; cycles of 32 steps, a fraction pmix of them learned),
+    and the reference CPU path on the same target.  ESS = Var_p(f) / err^2 with the exact
+    variance and err the standard error of the walker means; time = calibration + sampling.  The
+    network is trained on pairs of independent snapshots of the plain FP32 chain (16 snapshots,
+    512 steps apart; 3 x 1,000 Adam steps of 4,096 pairs, learning rate 1e-2, 5e-3, 2.5e-3)."""
     from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP64, LearnedProposalSampler, mixture_target
     w, mu, cov, (ex2, pind) = mixture_target(d=32, ncomp=8, seed=5)
     ex4 = float(np.sum(w * [(np.trace(c) + m @ m) ** 2 + 2 * np.trace(c @ c) + 4 * m @ c @ m for m, c in zip(mu, cov)]))
     var = np.array([ex4 - ex2 ** 2, pind * (1 - pind)])
+    exact = np.array([ex2, pind])
     Wl = 65536
     s = LearnedProposalSampler(w, mu, cov, Wl)
     s.setSeed(1)
     a_p, e_p = s.run(PLAIN_FP64, NMC, 64)
     t_p = s.timings()
+    sec_p = (t_p["sampling"] + t_p["calibration"]) / 1000
+    ess_p = var / e_p ** 2 / sec_p
     s.setTrainingPairs(0.0)
     s.collect(16, 512)
-    l0, l1 = s.train(1500, 4096, 2e-3)
-    t_train = s.timings()["training"]
-    a_m, e_m = s.run(MIXED_FP32, NMC, 64, 0.5)
-    t_m = s.timings()
-    r = s.results()
-    ess_p = var / e_p ** 2 / ((t_p["sampling"] + t_p["calibration"]) / 1000)
-    ess_m = var / e_m ** 2 / ((t_m["sampling"] + t_m["calibration"]) / 1000)
-    exact = np.array([ex2, pind])
-    return {"workload": "d=32, 8 rotated anisotropic Gaussians, 65536 walkers x 10240 steps, |x|^2 and 1[x_0>0]",
-            "plain_fp64": {"sampling_ms": t_p["sampling"], "err": list(e_p), "ess_per_s": list(ess_p),
-                           "z": list((a_p - exact) / e_p)},
-            "learned_fp32": {"sampling_ms": t_m["sampling"], "err": list(e_m), "ess_per_s": list(ess_m),
-                             "z": list((a_m - exact) / e_m), "learned_acceptance": r["acc_learned"]},
-            "training_ms": t_train, "training_loss": [l0, l1],
-            "ess_per_s_gain": list(ess_m / ess_p), "ess_per_step_gain": list((e_p / e_m) ** 2)}
+    t_train, losses = 0.0, []
+    for lr in (1e-2, 5e-3, 2.5e-3):
+        l0, l1 = s.train(1000, 4096, lr)
+        t_train += s.timings()["training"]
+        losses.append(l1)
+    out = {"workload": "d=32, 8 rotated anisotropic Gaussians, 65536 walkers x 10240 steps, |x|^2 and 1[x_0>0]",
+           "plain_fp64": {"seconds": sec_p, "err": list(e_p), "ess_per_s": list(ess_p), "z": list((a_p - exact) / e_p)},
+           "training_ms": t_train, "training_loss_after_each_stage": losses}
+    for pm in (0.25, 0.5):
+        a_m, e_m = s.run(MIXED_FP32, NMC, 64, pm)
+        t_m = s.timings()
+        sec_m = (t_m["sampling"] + t_m["calibration"]) / 1000
+        ess_m = var / e_m ** 2 / sec_m
+        out[f"learned_fp32_pmix{pm}"] = {"seconds": sec_m, "err": list(e_m), "ess_per_s": list(ess_m),
+                                         "z": list((a_m - exact) / e_m), "learned_acceptance": s.results()["acc_learned"],
+                                         "ess_per_s_gain_vs_plain_fp64": list(ess_m / ess_p),
+                                         "ess_per_step_gain": list((e_p / e_m) ** 2)}
+    cpu = cpu_learned_target(w, mu, cov, var, NMC) if with_cpu else None
+    if cpu is not None:
+        out["reference_cpu"] = cpu
+        best = out["learned_fp32_pmix0.25"]["ess_per_s"]
+        out["ess_per_s_gain_vs_reference_cpu"] = list(np.array(best) / np.array(cpu["ess_per_s"]))
+    return out
 
 
 def main():
@@ -291,7 +346,7 @@ def main():
     }
     if args.workloads == "all" and args.precision == 0 and not args.generic:
         line["other_workloads"] = {"precision_fp32": run_precision_form(), "batched": run_batched(),
-                                   "learned_proposal": run_learned()}
+                                   "learned_proposal": run_learned(not args.no_cpu)}
     if not args.no_cpu:
         cb = cpu_baseline()
         if cb is not None:

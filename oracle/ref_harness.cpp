@@ -25,6 +25,7 @@
 #include <limits>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -32,7 +33,7 @@
 namespace
 {
 // functor kinds, as in include/mci_amd.h
-enum { PDF_GAUSS_ISO = 1, PDF_GAUSS_DIAG = 2, PDF_EXP_DIAG = 3, PDF_GMM_ISO = 4 };
+enum { PDF_GAUSS_ISO = 1, PDF_GAUSS_DIAG = 2, PDF_EXP_DIAG = 3, PDF_GMM_ISO = 4, PDF_GMM_FULL = 5 };
 enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7, OBS_GAUSS_X2 = 8 };
 
 std::string hx(double v)
@@ -151,6 +152,83 @@ public:
     }
 };
 
+// mixture of K full-covariance Gaussians held as precision matrices P_k and
+// c_k = log w_k - log det(2 pi Sigma_k) / 2; protovalues q_k = (x - mu_k)^T P_k (x - mu_k) and log p.
+// A move of the coordinates in C changes q_k by 2 sum_{a in C} delta_a (P_k (x - mu_k))_a
+// + sum_{a, b in C} delta_a delta_b (P_k)_ab.  L_k (Cholesky factor of Sigma_k) draws exact starts.
+class GmmFull final: public mci::SamplingFunctionInterface
+{
+    int _K;
+    std::vector<double> _w, _mu, _P, _c, _L;
+protected:
+    mci::SamplingFunctionInterface *_clone() const final { return new GmmFull(_ndim, _K, _w, _mu, _P, _c, _L); }
+public:
+    GmmFull(int nd, int K, std::vector<double> w, std::vector<double> mu, std::vector<double> P, std::vector<double> c, std::vector<double> L):
+            mci::SamplingFunctionInterface(nd, K + 1), _K(K), _w(std::move(w)), _mu(std::move(mu)), _P(std::move(P)), _c(std::move(c)),
+            _L(std::move(L)) {}
+    double logp(const double q[]) const
+    {
+        double m = -std::numeric_limits<double>::infinity();
+        for (int k = 0; k < _K; ++k) { const double t = _c[k] - 0.5*q[k]; if (t > m) { m = t; } }
+        double s = 0.;
+        for (int k = 0; k < _K; ++k) { s = s + exp((_c[k] - 0.5*q[k]) - m); }
+        return m + log(s);
+    }
+    void protoFunction(const double in[], double pv[]) final
+    {
+        for (int k = 0; k < _K; ++k) {
+            const double *P = &_P[(size_t)k*_ndim*_ndim], *mu = &_mu[(size_t)k*_ndim];
+            double q = 0.;
+            for (int i = 0; i < _ndim; ++i) {
+                double r = 0.;
+                for (int j = 0; j < _ndim; ++j) { r = r + P[i*_ndim + j]*(in[j] - mu[j]); }
+                q = q + (in[i] - mu[i])*r;
+            }
+            pv[k] = q;
+        }
+        pv[_K] = logp(pv);
+    }
+    double samplingFunction(const double pv[]) const final { return exp(pv[_K]); }
+    double acceptanceFunction(const double po[], const double pn[]) const final { return exp(pn[_K] - po[_K]); }
+    double updatedAcceptance(const mci::WalkerState &wlk, const double po[], double pn[]) final
+    {
+        for (int k = 0; k < _K; ++k) {
+            const double *P = &_P[(size_t)k*_ndim*_ndim], *mu = &_mu[(size_t)k*_ndim];
+            double dq = 0.;
+            for (int a = 0; a < wlk.nchanged; ++a) {
+                const int ja = wlk.changedIdx[a];
+                const double da = wlk.xnew[ja] - wlk.xold[ja];
+                double r = 0.;
+                for (int i = 0; i < _ndim; ++i) { r = r + P[ja*_ndim + i]*(wlk.xold[i] - mu[i]); }
+                dq = dq + 2.*da*r;
+                for (int b = 0; b < wlk.nchanged; ++b) {
+                    const int jb = wlk.changedIdx[b];
+                    dq = dq + da*(wlk.xnew[jb] - wlk.xold[jb])*P[ja*_ndim + jb];
+                }
+            }
+            pn[k] = po[k] + dq;
+        }
+        pn[_K] = logp(pn);
+        return exp(pn[_K] - po[_K]);
+    }
+    // an exact draw of p: component by weight, x = mu_k + L_k z
+    void draw(std::mt19937_64 &g, double x[]) const
+    {
+        std::uniform_real_distribution<double> u(0., 1.);
+        std::normal_distribution<double> n(0., 1.);
+        double c = u(g), cum = 0.;
+        int k = _K - 1;
+        for (int i = 0; i < _K - 1; ++i) { cum += _w[i]; if (c < cum) { k = i; break; } }
+        std::vector<double> z(_ndim);
+        for (int i = 0; i < _ndim; ++i) { z[i] = n(g); }
+        for (int i = 0; i < _ndim; ++i) {
+            double v = _mu[(size_t)k*_ndim + i];
+            for (int j = 0; j <= i; ++j) { v += _L[(size_t)k*_ndim*_ndim + i*_ndim + j]*z[j]; }
+            x[i] = v;
+        }
+    }
+};
+
 // ---- observables ----
 class CatObs final: public mci::ObservableFunctionInterface
 {
@@ -197,6 +275,18 @@ std::unique_ptr<mci::SamplingFunctionInterface> makePDF(int nd, int kind, const 
         std::vector<double> w(p.begin() + 1, p.begin() + 1 + K), s(p.begin() + 1 + K, p.begin() + 1 + 2*K);
         std::vector<double> mu(p.begin() + 1 + 2*K, p.end());
         return std::unique_ptr<mci::SamplingFunctionInterface>(new GmmIso(nd, K, w, s, mu));
+    }
+    case PDF_GMM_FULL: {
+        // K, w[K], mu[K][d], P[K][d][d], c[K], L[K][d][d]
+        const int K = static_cast<int>(p[0]);
+        const size_t dd = (size_t)nd*nd;
+        auto at = p.begin() + 1;
+        std::vector<double> w(at, at + K); at += K;
+        std::vector<double> mu(at, at + (size_t)K*nd); at += (size_t)K*nd;
+        std::vector<double> P(at, at + K*dd); at += K*dd;
+        std::vector<double> c(at, at + K); at += K;
+        std::vector<double> L(at, at + K*dd);
+        return std::unique_ptr<mci::SamplingFunctionInterface>(new GmmFull(nd, K, w, mu, P, c, L));
     }
     default: throw std::invalid_argument("unknown pdf kind");
     }
@@ -436,15 +526,21 @@ int main()
                 while (std::getline(std::cin, l2) && l2 != "endbench") { setup.push_back(l2); }
                 double tot = 0., avgsum = 0.;
                 int64_t steps_total = 0;
-                std::vector<double> asum, e2sum;
+                std::vector<double> asum, a2sum, e2sum;
                 for (int64_t w = 0; w < W; ++w) {
                     std::unique_ptr<mci::MCI> m;
+                    std::unique_ptr<GmmFull> start; // exact start when the target is GMM_FULL
                     int nd0 = 0;
                     for (auto &sl : setup) {
                         std::istringstream ss(sl);
                         std::string c2; ss >> c2;
                         if (c2 == "create") { ss >> nd0; m.reset(new mci::MCI(nd0)); m->setSeed(seed0 + static_cast<uint64_t>(w)); }
-                        else if (c2 == "pdf") { int kind, np; ss >> kind >> np; auto p = readDoubles(ss, np); m->addSamplingFunction(makePDF(nd0, kind, p)); }
+                        else if (c2 == "pdf") {
+                            int kind, np; ss >> kind >> np; auto p = readDoubles(ss, np);
+                            auto f = makePDF(nd0, kind, p);
+                            if (kind == PDF_GMM_FULL) { start.reset(static_cast<GmmFull *>(f->clone().release())); }
+                            m->addSamplingFunction(std::move(f));
+                        }
                         else if (c2 == "obs") {
                             int kind, np, bs, nskip, eq, est; ss >> kind >> np; auto p = readDoubles(ss, np); ss >> bs >> nskip >> eq >> est;
                             m->addObservable(std::unique_ptr<mci::ObservableFunctionInterface>(new CatObs(nd0, kind, p)), bs, nskip, eq != 0, estimOf(est));
@@ -454,6 +550,12 @@ int main()
                         else if (c2 == "ndecorr") { int64_t n; ss >> n; m->setNdecorrelationSteps(n); }
                     }
                     std::vector<double> a(m->getNObsDim()), e(m->getNObsDim());
+                    if (start) {
+                        std::mt19937_64 g(seed0 + static_cast<uint64_t>(w) + 0x5eed0000ull);
+                        std::vector<double> x0(nd0);
+                        start->draw(g, x0.data());
+                        m->setX(x0.data());
+                    }
                     auto t0 = std::chrono::steady_clock::now();
                     m->integrate(nmc, a.data(), e.data(), true, true);
                     auto t1 = std::chrono::steady_clock::now();
@@ -461,11 +563,13 @@ int main()
                     steps_total += nmc;
                     avgsum += a.empty() ? 0. : a[0];
                     asum.resize(a.size(), 0.);
+                    a2sum.resize(a.size(), 0.);
                     e2sum.resize(e.size(), 0.);
-                    for (size_t i = 0; i < a.size(); ++i) { asum[i] += a[i]; e2sum[i] += e[i]*e[i]; }
+                    for (size_t i = 0; i < a.size(); ++i) { asum[i] += a[i]; a2sum[i] += a[i]*a[i]; e2sum[i] += e[i]*e[i]; }
                 }
                 std::cout << "{\"cmd\":\"bench\",\"walkers\":" << W << ",\"seconds\":" << tot << ",\"walker_steps\":" << steps_total
                           << ",\"avg0_mean\":" << avgsum/W << ",\"avg_sum\":" << hxv(asum.data(), static_cast<int>(asum.size()))
+                          << ",\"avg2_sum\":" << hxv(a2sum.data(), static_cast<int>(a2sum.size()))
                           << ",\"err2_sum\":" << hxv(e2sum.data(), static_cast<int>(e2sum.size())) << "}" << std::endl;
             }
             else { std::cout << "{\"cmd\":\"error\",\"what\":\"unknown command " << cmd << "\"}" << std::endl; }
