@@ -272,8 +272,9 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
-    ap.add_argument("--workloads", default="all", choices=["all", "none"],
-                    help="also measure the precision form, the batched integrals and the learned proposal")
+    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned"],
+                    help="also measure the precision form, the batched integrals and the learned proposal (all), "
+                         "none of them, or one of them")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -344,9 +345,14 @@ def main():
                      "instructions_per_wave_step_rocprof": (prof.get("instructions_per_walker_step") or {}).get("SQ_INSTS"),
                      "hbm_gbps": (prof["bytes_per_launch"] / (kt[2] / 1000) / 1e9) if "bytes_per_launch" in prof else None},
     }
-    if args.workloads == "all" and args.precision == 0 and not args.generic:
-        line["other_workloads"] = {"precision_fp32": run_precision_form(), "batched": run_batched(),
-                                   "learned_proposal": run_learned(not args.no_cpu)}
+    if args.workloads != "none" and args.precision == 0 and not args.generic:
+        runs = {"precision_fp32": ("precision", run_precision_form), "batched": ("batched", run_batched),
+                "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu))}
+        line["other_workloads"] = {}
+        for k, (name, f) in runs.items():
+            if args.workloads in ("all", name):
+                print(f"bench: {k}", file=sys.stderr, flush=True)
+                line["other_workloads"][k] = f()
     if not args.no_cpu:
         cb = cpu_baseline()
         if cb is not None:

@@ -62,6 +62,7 @@ struct mci_mlp
     int64_t cap = 0;
     DBuf<bf16_t> Xb, H1b, H2b, dOb, dZ2b, dZ1b;
     DBuf<float> X, Y, Z1, Z2, OUT, dO, dH2, dH1, loss;
+    DBuf<float> ws; // split-reduction partials of the weight gradients
     int adam_t = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -133,6 +134,7 @@ mci_status mlp_reserve(mci_mlp *n, int64_t cap)
     HIPCHK(n->dH1.ensure((size_t)H * cap));
     HIPCHK(n->dZ1b.ensure((size_t)H * cap));
     HIPCHK(n->loss.ensure((size_t)cap));
+    HIPCHK(n->ws.ensure((size_t)32 * std::max({(size_t)H * Op, (size_t)H * H, (size_t)d * H})));
     n->cap = cap;
     return MCI_OK;
 }
@@ -164,15 +166,15 @@ mci_status mlp_fwd_bwd(mci_mlp *n, int64_t ns)
     float *g = n->g.p;
     const bf16_t *W = n->pb.p;
     HIPCHK(mcim::launch_rowsum(n->dO.p, Op, N, N, g + n->ob3, s));
-    HIPCHK(mcim::gemm_bf16(false, true, mcim::EPI_NONE, H, Op, N, n->H2b.p, n->dOb.p, nullptr, g + n->oW3, Op, nullptr, 0, s));
+    HIPCHK(mcim::gemm_bf16_splitk(false, true, H, Op, N, n->H2b.p, n->dOb.p, g + n->oW3, Op, n->ws.p, n->ws.n, s));
     HIPCHK(mcim::gemm_bf16(false, false, mcim::EPI_NONE, H, N, Op, W + n->oW3, n->dOb.p, nullptr, n->dH2.p, N, nullptr, 0, s));
     HIPCHK(mcim::launch_silu_back(n->dH2.p, n->Z2.p, n->dZ2b.p, (int64_t)H * ns, s));
     HIPCHK(mcim::launch_rowsum(n->dH2.p, H, N, N, g + n->ob2, s));
-    HIPCHK(mcim::gemm_bf16(false, true, mcim::EPI_NONE, H, H, N, n->H1b.p, n->dZ2b.p, nullptr, g + n->oW2, H, nullptr, 0, s));
+    HIPCHK(mcim::gemm_bf16_splitk(false, true, H, H, N, n->H1b.p, n->dZ2b.p, g + n->oW2, H, n->ws.p, n->ws.n, s));
     HIPCHK(mcim::gemm_bf16(false, false, mcim::EPI_NONE, H, N, H, W + n->oW2, n->dZ2b.p, nullptr, n->dH1.p, N, nullptr, 0, s));
     HIPCHK(mcim::launch_silu_back(n->dH1.p, n->Z1.p, n->dZ1b.p, (int64_t)H * ns, s));
     HIPCHK(mcim::launch_rowsum(n->dH1.p, H, N, N, g + n->ob1, s));
-    HIPCHK(mcim::gemm_bf16(false, true, mcim::EPI_NONE, d, H, N, n->Xb.p, n->dZ1b.p, nullptr, g + n->oW1, H, nullptr, 0, s));
+    HIPCHK(mcim::gemm_bf16_splitk(false, true, d, H, N, n->Xb.p, n->dZ1b.p, g + n->oW1, H, n->ws.p, n->ws.n, s));
     return MCI_OK;
 }
 

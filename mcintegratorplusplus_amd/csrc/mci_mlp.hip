@@ -30,17 +30,21 @@ __device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((uint32
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 
+// split z = blockIdx.z takes the reduction range [z kchunk, min(K, (z + 1) kchunk)) and writes
+// C + z zstride (kchunk a multiple of KC; one split: kchunk >= K, zstride 0)
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t *__restrict__ A, const bf16_t *__restrict__ B,
                                               const float *__restrict__ bias, float *__restrict__ C, int ldc, bf16_t *__restrict__ H,
-                                              int ldh)
+                                              int ldh, int kchunk, int64_t zstride)
 {
     __shared__ __attribute__((aligned(16))) bf16_t As[TILE][KP];
     __shared__ __attribute__((aligned(16))) bf16_t Bs[TILE][KP];
     const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
+    const int kb = (int)blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    C += (int64_t)blockIdx.z * zstride;
     f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += KC) {
+    for (int k0 = kb; k0 < ke; k0 += KC) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int e = tid * 8 + q; // 0 .. 2047
@@ -49,14 +53,14 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t 
             else { mm = e / KC; kk = e % KC; }
             const int gm = m0 + mm, gk = k0 + kk;
             bf16_t v = 0;
-            if (gm < M && gk < K) { v = TA ? A[(int64_t)gk * M + gm] : A[(int64_t)gm * K + gk]; }
+            if (gm < M && gk < ke) { v = TA ? A[(int64_t)gk * M + gm] : A[(int64_t)gm * K + gk]; }
             As[mm][kk] = v;
             int nn;
             if (TB) { nn = e / KC; kk = e % KC; }
             else { kk = e / TILE; nn = e % TILE; }
             const int gn = n0 + nn, gk2 = k0 + kk;
             bf16_t u = 0;
-            if (gn < N && gk2 < K) { u = TB ? B[(int64_t)gn * K + gk2] : B[(int64_t)gk2 * N + gn]; }
+            if (gn < N && gk2 < ke) { u = TB ? B[(int64_t)gn * K + gk2] : B[(int64_t)gk2 * N + gn]; }
             Bs[nn][kk] = u;
         }
         __syncthreads();
@@ -82,53 +86,74 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16_t 
     }
 }
 
+// C[r][c] = sum over the S splits of ws[z][r][c], in split order
+__global__ void k_sum_splits(const float *ws, int S, int M, int N, float *C, int ldc)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * N) { return; }
+    float v = 0.f;
+    for (int z = 0; z < S; ++z) { v += ws[(int64_t)z * M * N + e]; }
+    C[(e / N) * ldc + e % N] = v;
+}
+
 __global__ void k_to_bf16(const float *x, bf16_t *xb, int64_t n, float scale)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { xb[i] = f2bf(x[i] * scale); }
 }
 
-// one sample per thread, feature-major layout (element c of sample r at out[c * ld + r]):
-// -log of the mixture density of y under the sample's parameters and the gradient of
-// (inv_n * that) with respect to the sample's raw outputs.  Output rows of one sample: for each
-// component k, d mean rows then d log-scale rows (rows 2 d k .. 2 d k + 2 d - 1); then K logits.
-__global__ void k_mixture_head(const float *out, int ld, const float *y, int n, int d, int K, int rows, float *loss, float *dout,
-                               bf16_t *doutb, float inv_n)
+// eight threads per sample (thread c of the group takes component c), feature-major layout
+// (element c of sample r at out[c * ld + r]): -log of the mixture density of y under the sample's
+// parameters and the gradient of (inv_n * that) with respect to the sample's raw outputs.  Output
+// rows of one sample: for each component k, d mean rows then d log-scale rows (rows 2 d k ..
+// 2 d k + 2 d - 1); then K logits.
+__device__ __forceinline__ float group8_max(float v)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) { return; }
-    const float *o = out + r;
-    const float *yr = y + r;
-    float *g = dout + r;
-    bf16_t *gb = doutb + r;
+    v = fmaxf(v, __shfl_xor(v, 1));
+    v = fmaxf(v, __shfl_xor(v, 2));
+    return fmaxf(v, __shfl_xor(v, 4));
+}
+
+__device__ __forceinline__ float group8_sum(float v)
+{
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v + __shfl_xor(v, 4);
+}
+
+__global__ __launch_bounds__(256) void k_mixture_head(const float *out, int ld, const float *y, int n, int d, int K, int rows, float *loss,
+                                                      float *dout, bf16_t *doutb, float inv_n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t >> 3, k = t & 7;
+    const bool ok = r < n;                // whole groups of eight agree
+    const bool on = ok && k < K;
+    const int rr = ok ? r : n - 1;
+    const float *o = out + rr;
+    const float *yr = y + rr;
+    float *g = dout + rr;
+    bf16_t *gb = doutb + rr;
     constexpr float HALF_LOG_2PI = 0.91893853320467274f;
     const int L0 = 2 * d * K; // first logit row
-    float amax = -__builtin_huge_valf();
-    for (int k = 0; k < K; ++k) { amax = fmaxf(amax, o[(int64_t)(L0 + k) * ld]); }
-    float sa = 0.f;
-    for (int k = 0; k < K; ++k) { sa += __expf(o[(int64_t)(L0 + k) * ld] - amax); }
-    const float lsa = amax + __logf(sa);
-    float comp[8];
-    float cmax = -__builtin_huge_valf();
-    for (int k = 0; k < K; ++k) {
-        float c = o[(int64_t)(L0 + k) * ld] - lsa;
+    const float a = on ? o[(int64_t)(L0 + k) * ld] : -__builtin_huge_valf();
+    const float amax = group8_max(a);
+    const float lsa = amax + __logf(group8_sum(on ? __expf(a - amax) : 0.f));
+    float c = -__builtin_huge_valf();
+    if (on) {
+        c = a - lsa;
         for (int j = 0; j < d; ++j) {
             const float mu = MU_SCALE * o[(int64_t)(2 * d * k + j) * ld];
             const float ls = fminf(fmaxf(o[(int64_t)(2 * d * k + d + j) * ld], -LS_MAX), LS_MAX);
             const float z = (yr[(int64_t)j * ld] - mu) * __expf(-ls);
             c += -ls - HALF_LOG_2PI - 0.5f * z * z;
         }
-        comp[k] = c;
-        cmax = fmaxf(cmax, c);
     }
-    float sc = 0.f;
-    for (int k = 0; k < K; ++k) { sc += __expf(comp[k] - cmax); }
-    const float L = cmax + __logf(sc);
-    loss[r] = -L;
-    for (int k = 0; k < K; ++k) {
-        const float gam = __expf(comp[k] - L);
-        const float pik = __expf(o[(int64_t)(L0 + k) * ld] - lsa);
-        const float ga = inv_n * (pik - gam);
+    const float cmax = group8_max(c);
+    const float L = cmax + __logf(group8_sum(on ? __expf(c - cmax) : 0.f));
+    if (ok && k == 0) { loss[r] = -L; }
+    if (on) {
+        const float gam = __expf(c - L);
+        const float ga = inv_n * (__expf(a - lsa) - gam);
         g[(int64_t)(L0 + k) * ld] = ga;
         gb[(int64_t)(L0 + k) * ld] = f2bf(ga);
         for (int j = 0; j < d; ++j) {
@@ -146,9 +171,11 @@ __global__ void k_mixture_head(const float *out, int ld, const float *y, int n, 
             gb[il] = f2bf(gl);
         }
     }
-    for (int c = K * (1 + 2 * d); c < rows; ++c) {
-        g[(int64_t)c * ld] = 0.f;
-        gb[(int64_t)c * ld] = 0;
+    if (ok) {
+        for (int cc = K * (1 + 2 * d) + k; cc < rows; cc += 8) {
+            g[(int64_t)cc * ld] = 0.f;
+            gb[(int64_t)cc * ld] = 0;
+        }
     }
 }
 
@@ -197,10 +224,12 @@ template <bool TA, bool TB>
 hipError_t gemm_epi(int epi, dim3 g, hipStream_t s, int M, int N, int K, const bf16_t *A, const bf16_t *B, const float *bias, float *C,
                     int ldc, bf16_t *H, int ldh)
 {
+    const int kc = (int)g.z > 1 ? ((K + (int)g.z - 1) / (int)g.z + KC - 1) / KC * KC : K;
+    const int64_t zs = (int)g.z > 1 ? (int64_t)M * N : 0;
     switch (epi) {
-    case EPI_NONE: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_NONE>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh); break;
-    case EPI_BIAS: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh); break;
-    default: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS_SILU>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh); break;
+    case EPI_NONE: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_NONE>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh, kc, zs); break;
+    case EPI_BIAS: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh, kc, zs); break;
+    default: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS_SILU>), g, dim3(256), 0, s, M, N, K, A, B, bias, C, ldc, H, ldh, kc, zs); break;
     }
     return hipGetLastError();
 }
@@ -216,6 +245,24 @@ hipError_t gemm_bf16(bool ta, bool tb, int epi, int M, int N, int K, const bf16_
     return gemm_epi<false, false>(epi, g, s, M, N, K, A, B, bias, C, ldc, H, ldh);
 }
 
+hipError_t gemm_bf16_splitk(bool ta, bool tb, int M, int N, int K, const bf16_t *A, const bf16_t *B, float *C, int ldc, float *ws,
+                            size_t ws_elems, hipStream_t s)
+{
+    const int nt = (N + TILE - 1) / TILE, mt = (M + TILE - 1) / TILE;
+    int S = 1;
+    while (S < 32 && nt * mt * S < 512 && K / (2 * S) >= 4 * KC && (size_t)(2 * S) * M * N <= ws_elems) { S *= 2; }
+    if (S == 1) { return gemm_bf16(ta, tb, EPI_NONE, M, N, K, A, B, nullptr, C, ldc, nullptr, 0, s); }
+    const dim3 g((unsigned)nt, (unsigned)mt, (unsigned)S);
+    hipError_t e;
+    if (ta && tb) { e = gemm_epi<true, true>(EPI_NONE, g, s, M, N, K, A, B, nullptr, ws, N, nullptr, 0); }
+    else if (ta) { e = gemm_epi<true, false>(EPI_NONE, g, s, M, N, K, A, B, nullptr, ws, N, nullptr, 0); }
+    else if (tb) { e = gemm_epi<false, true>(EPI_NONE, g, s, M, N, K, A, B, nullptr, ws, N, nullptr, 0); }
+    else { e = gemm_epi<false, false>(EPI_NONE, g, s, M, N, K, A, B, nullptr, ws, N, nullptr, 0); }
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL(k_sum_splits, grid1((int64_t)M * N, 256), dim3(256), 0, s, ws, S, M, N, C, ldc);
+    return hipGetLastError();
+}
+
 hipError_t launch_to_bf16_scaled(const float *x, bf16_t *xb, int64_t n, float scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_to_bf16, grid1(n, 256), dim3(256), 0, s, x, xb, n, scale);
@@ -227,7 +274,7 @@ hipError_t launch_to_bf16(const float *x, bf16_t *xb, int64_t n, hipStream_t s) 
 hipError_t launch_mixture_head(const float *out, int ld, const float *y, int n, int d, int K, int rows, float *loss, float *dout,
                                bf16_t *doutb, float inv_n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mixture_head, grid1(n, 64), dim3(64), 0, s, out, ld, y, n, d, K, rows, loss, dout, doutb, inv_n);
+    hipLaunchKernelGGL(k_mixture_head, grid1((int64_t)n * 8, 256), dim3(256), 0, s, out, ld, y, n, d, K, rows, loss, dout, doutb, inv_n);
     return hipGetLastError();
 }
 

@@ -26,6 +26,11 @@ enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2 };
 //   feature-major); EPI_BIAS_SILU also writes H = bf16(silu(C)).
 hipError_t gemm_bf16(bool ta, bool tb, int epi, int M, int N, int K, const bf16_t *A, const bf16_t *B, const float *bias,
                      float *C, int ldc, bf16_t *H, int ldh, hipStream_t s);
+// the same without epilogue, the reduction split over up to 32 workgroups per output tile (for
+// long reductions over few tiles: weight gradients over a minibatch); partial products in ws
+// (ws_elems floats, S M N used), summed in split order (deterministic)
+hipError_t gemm_bf16_splitk(bool ta, bool tb, int M, int N, int K, const bf16_t *A, const bf16_t *B, float *C, int ldc, float *ws,
+                            size_t ws_elems, hipStream_t s);
 
 // x [n][d] FP32 -> bf16(x * IN_SCALE) [n][d]
 hipError_t launch_to_bf16_scaled(const float *x, bf16_t *xb, int64_t n, float scale, hipStream_t s);
