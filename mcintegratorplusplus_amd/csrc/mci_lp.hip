@@ -825,7 +825,7 @@ template <int D, int H>
 size_t learned_lds()
 {
     using C = LCfg<D, H>;
-    return sizeof(float) * ((size_t)2 * D * TW + (size_t)5 * KMAX * TW + C::NB) + sizeof(int) * 3 * TW +
+    return sizeof(float) * ((size_t)3 * D * TW + (size_t)6 * KMAX * TW + C::NB) + sizeof(int) * 3 * TW +
            sizeof(uint16_t) * ((size_t)TW * C::LDX + (size_t)TW * C::LDH);
 }
 
@@ -844,11 +844,13 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     const bool live = w < W;
     const int64_t wl = live ? w : W - 1;
     float *xs = reinterpret_cast<float *>(lds_raw);      // [D][64] current x
-    float *xps = xs + D * TW;                             // [D][64] normals, then the proposal
+    float *xps = xs + D * TW;                             // [D][64] the proposal
     float *lgx = xps + D * TW;                            // [KMAX][64] logits at x
     float *lg = lgx + KMAX * TW;                          // [KMAX][64] logits at x'
-    float *cf = lg + KMAX * TW;                           // [KMAX][64] component sums of a fold
-    float *qn = cf + KMAX * TW;                           // [KMAX][64] target quadratic forms at x'
+    float *cf = lg + KMAX * TW;                           // [KMAX][64] component sums of the fold at x'
+    float *cf2 = cf + KMAX * TW;                          // [KMAX][64] the same of the fold at x
+    float *zs = cf2 + KMAX * TW;                          // [D][64] normals of the next proposal
+    float *qn = zs + D * TW;                              // [KMAX][64] target quadratic forms at x'
     float *qx = qn + KMAX * TW;                           // [KMAX][64] the same at x (accepted moves)
     float *bs = qx + KMAX * TW;                           // biases: b1 [H], b2 [H], b3 [LOFF + 32]
     int *kst = reinterpret_cast<int *>(bs + C::NB);       // [64] chosen component
@@ -887,6 +889,19 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             bi = St.bi[wl];
         }
     }
+    // normals of step t go to zs, drawn by waves 1 .. 7 (generator g = wave - 1 takes coordinates
+    // g, g + 7, ...); they are drawn for the next step while wave 0 runs the MH test
+    auto draw_normals = [&](uint64_t t) {
+        if (part > 0) {
+            Normals Nm;
+            Nm.avail = 0;
+            Nm.has_spare = false;
+            Nm.call = 1;
+            const uint32_t base = 0x10000000u + ((uint32_t)part << 20);
+            for (int j = part - 1; j < D; j += NWV - 1) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1); }
+        }
+    };
+    draw_normals(t0);
     for (int st = 0; st < nl; ++st) {
         // the weights and target factors are read afresh each step: an empty asm makes their
         // addresses opaque, so the compiler does not hoist those loads out of the loop (they would
@@ -897,18 +912,9 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         const uint64_t t = t0 + (uint64_t)st;
         const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
         uint32_t o[4] = {0u, 0u, 0u, 0u};
-        {
-            Normals Nm;
-            Nm.avail = 0;
-            Nm.has_spare = false;
-            Nm.call = 1;
-            const uint32_t base = 0x10000000u + ((uint32_t)(part + 1) << 20);
-            for (int j = part; j < D; j += NWV) { xps[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, tl, th, k0, k1); }
-            if (part == 0) { philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o); }
-        }
-        __syncthreads();
         // the component from the logits at x
         if (part == 0) {
+            philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
             float amax = -__builtin_huge_valf();
             for (int k = 0; k < K; ++k) { amax = fmaxf(amax, lgx[k * TW + col]); }
             float e[KMAX], sa = 0.f;
@@ -942,8 +948,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             const h2_t p = mx[pair_index(rb, g, h)];
-                            float *z = xps + (j + h) * TW + cc;
-                            *z = fmaf(*z, __builtin_amdgcn_rcpf((float)p[1]), (float)p[0]);
+                            xps[(j + h) * TW + cc] = fmaf(zs[(j + h) * TW + cc], __builtin_amdgcn_rcpf((float)p[1]), (float)p[0]);
                         }
                     }
                 }
@@ -964,17 +969,15 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             }
         }
         __syncthreads();
-        float logq_f = 0.f;
-        if (part == 0) { logq_f = mixture_logq(lgx, cf, K, D, col); }
-        __syncthreads();
         // the network at x' and log q(x | x')
         h2_t mx2[D];
         float lpi2[2] = {0.f, 0.f};
         network<D, H>(Ns, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
-        fold<D>(mx2, lpi2, xs, cf, K);
+        fold<D>(mx2, lpi2, xs, cf2, K);
         __syncthreads();
         if (part == 0) {
-            const float logq_r = mixture_logq(lg, cf, K, D, col);
+            const float logq_f = mixture_logq(lgx, cf, K, D, col);
+            const float logq_r = mixture_logq(lg, cf2, K, D, col);
             float mn = -__builtin_huge_valf();
             float e[KMAX];
 #pragma unroll
@@ -1004,6 +1007,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             }
             if (live && St.B > 0) { accumulate(St, w, true, sq, acc ? xps[col] : xs[col], s0, c0, s1, c1, jb, bi); }
         }
+        else if (st + 1 < nl) { draw_normals(t + 1); }
         __syncthreads();
         // accepted walkers: position and mixture at x take the proposal's
         if (af[col]) {
