@@ -549,7 +549,7 @@ __global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<flo
 }
 
 // ---- the learned move, fused: one launch per run of consecutive learned steps ----
-// A workgroup of 512 threads (eight waves) takes 64 walkers for the whole run.  The network is
+// A workgroup takes TW walkers (32: four waves; 64: eight waves) for the whole run.  The network is
 // evaluated at the walkers' positions when the run starts (random-walk moves in between need no
 // refresh) and at every proposal; the mixture at the current position is carried in registers
 // from step to step.  Network: v_mfma_f32_32x32x16_bf16 tiles, weights read from L2 as [out][in]
@@ -557,8 +557,8 @@ __global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<flo
 // buffer per workgroup (a layer's output stays in accumulators until every wave has read its
 // input).  The last layer stays in registers: wave w holds components 2 (w % 4), 2 (w % 4) + 1 for
 // the walker half w / 4, its rows ordered as (mean_j, log-scale_j) pairs so that each lane holds
-// whole pairs of one walker, each pair kept as (mean, 1 / sigma) in two halves; waves 0 and 4
-// also compute the K logits.  A step:
+// whole pairs of one walker, each pair kept as (mean, 1 / sigma) in two halves; wave 0 (and
+// wave 4) also computes the K logits.  A step:
 //  1. the component from the logits at x, x' = mu + sigma z from the lanes holding that component,
 //     log q(x' | x) folded from the registers; log p(x') (one target component per wave);
 //  2. the network at x', log q(x | x') folded the same way;
@@ -570,9 +570,15 @@ typedef __fp16 h2_t __attribute__((ext_vector_type(2)));
 #ifndef MCIL_NS
 #define MCIL_NS 2 // last-layer row blocks per pass
 #endif
-constexpr int TW = 64;   // walkers per workgroup
-constexpr int TT = 512;  // threads per workgroup
-constexpr int NWV = 8;   // waves per workgroup
+#ifndef MCIL_TW
+#define MCIL_TW 64
+#endif
+constexpr int TW = MCIL_TW;    // walkers per workgroup (32 or 64)
+constexpr int NCB = TW / 32;   // walker blocks of 32 (matrix-core columns)
+constexpr int NWV = 4 * NCB;   // waves per workgroup: 4 component pairs x walker blocks
+constexpr int TT = 64 * NWV;   // threads per workgroup
+constexpr int NPART = TT / TW; // threads per walker in the per-walker loops (8)
+static_assert(NPART == KMAX, "one thread per walker and target component");
 
 template <int D, int H>
 struct LCfg
@@ -589,15 +595,15 @@ __device__ __forceinline__ float silu(float v) { return v * __builtin_amdgcn_rcp
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
-// acc[r] = bias + rows row0 + 32 r .. + 31 of Wt [.][KD] (BF16, row = output unit) times walkers
+// acc[r] = bias + rows row0 + 32 r .. + 31 of Wt [.][ldw] (BF16, row = output unit, KD used) times walkers
 // 32 cb .. + 31 of act [64][lda] (BF16, LDS), r < NR; bias in LDS, indexed by row
 template <int KD, int NR>
-__device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, const float *bias, int row0, const uint16_t *act, int lda, int cb,
-                                          f32x16 acc[NR])
+__device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, int ldw, const float *bias, int row0, const uint16_t *act, int lda,
+                                          int cb, f32x16 acc[NR])
 {
     const int lane = threadIdx.x & 63;
     constexpr int NQ = KD / 16;
-    const uint16_t *wr = Wt + (int64_t)(row0 + (lane & 31)) * KD + 8 * (lane >> 5);
+    const uint16_t *wr = Wt + (int64_t)(row0 + (lane & 31)) * ldw + 8 * (lane >> 5);
     const uint16_t *br = act + (32 * cb + (lane & 31)) * lda + 8 * (lane >> 5);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -613,14 +619,14 @@ __device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, const
     // one reduction step of fragments in flight ahead of the products (bounds the live registers)
     bf16x8 a[2][NR], b[2];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) { a[0][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * KD); }
+    for (int r = 0; r < NR; ++r) { a[0][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * ldw); }
     b[0] = *reinterpret_cast<const bf16x8 *>(br);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int cu = q & 1, nx = cu ^ 1;
         if (q + 1 < NQ) {
 #pragma unroll
-            for (int r = 0; r < NR; ++r) { a[nx][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * KD + 16 * (q + 1)); }
+            for (int r = 0; r < NR; ++r) { a[nx][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * ldw + 16 * (q + 1)); }
             b[nx] = *reinterpret_cast<const bf16x8 *>(br + 16 * (q + 1));
         }
 #pragma unroll
@@ -634,14 +640,15 @@ __device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, const
 // has synchronised so that act is complete; when act is Hb itself, the waves synchronise between
 // the products and the write-back.
 template <int KD, int H>
-__device__ __forceinline__ void hidden_layer(const uint16_t *Wt, const float *bias, const uint16_t *act, int lda, uint16_t *Hb, bool in_place)
+__device__ __forceinline__ void hidden_layer(const uint16_t *Wt, int ldw, const float *bias, const uint16_t *act, int lda, uint16_t *Hb,
+                                             bool in_place)
 {
     constexpr int NRH = H / 32, LDH = H + 8;
     const int w = wave_id(), lane = threadIdx.x & 63;
-    const bool mine = w < 2 * NRH;
+    const bool mine = w < NRH * NCB;
     const int rb = w % NRH, cb = w / NRH;
     f32x16 acc[1];
-    if (mine) { mfma_tile<KD, 1>(Wt, bias, 32 * rb, act, lda, cb, acc); }
+    if (mine) { mfma_tile<KD, 1>(Wt, ldw, bias, 32 * rb, act, lda, cb, acc); }
     if (in_place) { __syncthreads(); }
     if (mine) {
         const int col = 32 * cb + (lane & 31);
@@ -669,21 +676,21 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *Wt, const float *bi
 __device__ __forceinline__ constexpr int pair_index(int rb, int g, int h) { return (rb * 4 + g) * 2 + h; }
 
 template <int D, int H>
-__device__ __forceinline__ void network(const NetDev &N, const float *bs, const uint16_t *Xb, uint16_t *Hb, h2_t mx[D], float lpi[2], float *lg,
-                                        float *out, int64_t n, int64_t w0)
+__device__ __forceinline__ void network(const NetDev &N, const uint16_t *W1s, const uint16_t *W2s, const float *bs, const uint16_t *Xb,
+                                        uint16_t *Hb, h2_t mx[D], float lpi[2], float *lg, float *out, int64_t n, int64_t w0)
 {
     using C = LCfg<D, H>;
     const int w = wave_id(), lane = threadIdx.x & 63, K = N.K;
     const int wc = w & 3, cb = w >> 2;
     const float *b3 = bs + 2 * H;
-    hidden_layer<C::DP, H>(N.W1t, bs, Xb, C::LDX, Hb, false);
+    hidden_layer<C::DP, H>(W1s, C::LDX, bs, Xb, C::LDX, Hb, false);
     __syncthreads();
-    hidden_layer<H, H>(N.W2t, bs + H, Hb, C::LDH, Hb, true);
+    hidden_layer<H, H>(W2s, C::LDH, bs + H, Hb, C::LDH, Hb, true);
     __syncthreads();
     const int col = 32 * cb + (lane & 31);
     if (wc == 0) {
         f32x16 al[1];
-        mfma_tile<H, 1>(N.W3t, b3, C::LOFF, Hb, C::LDH, cb, al);
+        mfma_tile<H, 1>(N.W3t, H, b3, C::LOFF, Hb, C::LDH, cb, al);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int k = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
@@ -699,7 +706,7 @@ __device__ __forceinline__ void network(const NetDev &N, const float *bs, const 
 #pragma unroll
         for (int rb0 = 0; rb0 < C::NRB; rb0 += NS) {
             f32x16 acc[NS];
-            mfma_tile<H, NS>(N.W3t, b3, 4 * D * wc + 32 * rb0, Hb, C::LDH, cb, acc);
+            mfma_tile<H, NS>(N.W3t, H, b3, 4 * D * wc + 32 * rb0, Hb, C::LDH, cb, acc);
 #pragma unroll
             for (int rr = 0; rr < NS; ++rr) {
                 const int rb = rb0 + rr;
@@ -760,30 +767,31 @@ __device__ __forceinline__ void fold(const h2_t mx[D], const float lpi[2], const
     }
 }
 
-// q_k = |U_k y - U_k mu_k|^2 for target component k and the 64 walkers of y [D][64] (LDS) with
-// v_mfma_f32_32x32x2_f32 (FP32 products, FP32 sums): rows of U_k (zero-padded to 32) times y, the
-// accumulators starting at -U_k mu_k; lanes 0 .. 31 return q of walkers lane and 32 + lane in
-// out[0].x and out[1].x
+// q_k = |U_k y - U_k mu_k|^2 with v_mfma_f32_32x32x2_f32 (FP32 products, FP32 sums) for two
+// (component, walker block of 32) pairs: rows of U_k (zero-padded to 32) times y [D][TW] (LDS),
+// the accumulators starting at -U_k mu_k; lanes 0 .. 31 return q of walker 32 cb + lane
 typedef float f32x16v __attribute__((ext_vector_type(16)));
 template <int D>
-__device__ __forceinline__ void target_q(const float *U, const float *Um, int k, const float *y, int lane, float2 out[2])
+__device__ __forceinline__ void target_q(const float *U, const float *Um, const int k[2], const int cb[2], const float *y, int lane, float out[2])
 {
     const int row = lane & 31, kk = lane >> 5;
-    const float *Uk = U + (int64_t)k * D * D;
     f32x16v acc[2];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int r = 8 * (i >> 2) + 4 * kk + (i & 3);
-        const float v = r < D ? -Um[k * D + r] : 0.f;
-        acc[0][i] = v;
-        acc[1][i] = v;
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = 8 * (i >> 2) + 4 * kk + (i & 3);
+            acc[c][i] = r < D ? -Um[k[c] * D + r] : 0.f;
+        }
     }
 #pragma unroll
     for (int q = 0; q < D / 2; ++q) {
         const int j = 2 * q + kk;
-        const float a = row < D ? Uk[row * D + j] : 0.f;
-        acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, y[j * TW + row], acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, y[j * TW + 32 + row], acc[1], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float a = row < D ? U[((int64_t)k[c] * D + row) * D + j] : 0.f;
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, y[j * TW + 32 * cb[c] + row], acc[c], 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -791,7 +799,7 @@ __device__ __forceinline__ void target_q(const float *U, const float *Um, int k,
 #pragma unroll
         for (int i = 0; i < 16; ++i) { s = fmaf(acc[c][i], acc[c][i], s); }
         s += __shfl_xor(s, 32);
-        out[c] = make_float2(s, 0.f);
+        out[c] = s;
     }
 }
 
@@ -826,7 +834,7 @@ size_t learned_lds()
 {
     using C = LCfg<D, H>;
     return sizeof(float) * ((size_t)3 * D * TW + (size_t)6 * KMAX * TW + C::NB) + sizeof(int) * 3 * TW +
-           sizeof(uint16_t) * ((size_t)TW * C::LDX + (size_t)TW * C::LDH);
+           sizeof(uint16_t) * ((size_t)TW * C::LDX + (size_t)TW * C::LDH + (size_t)H * C::LDX + (size_t)H * C::LDH);
 }
 
 template <int MODE, int D, int H> // 0: the network at xin, written to out; 1: nl learned steps from step t0
@@ -836,7 +844,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     using C = LCfg<D, H>;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int K = N.K;
-    const int tid = threadIdx.x, col = tid & 63, part = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, col = tid % TW, part = tid / TW, lane = tid & 63;
     const int wv = wave_id(), wc = wv & 3, cbw = wv >> 2;
     const int64_t W = St.W;
     const int64_t w0 = (int64_t)blockIdx.x * TW;
@@ -858,19 +866,30 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     int *anyacc = af + TW;                                // [64] accepted in this run
     uint16_t *Xb = reinterpret_cast<uint16_t *>(anyacc + TW); // [64][LDX]
     uint16_t *Hb = Xb + TW * C::LDX;                      // [64][LDH]
+    uint16_t *W1s = Hb + TW * C::LDH;                     // [H][LDX] first-layer weights, read every step
+    uint16_t *W2s = W1s + H * C::LDX;                     // [H][LDH] second-layer weights
     const float *xsrc = MODE == 0 ? xin : St.x;
+
+    for (int e = tid; e < H * (C::DP / 8); e += TT) {
+        const int h = e / (C::DP / 8), c = 8 * (e % (C::DP / 8));
+        *reinterpret_cast<uint4 *>(W1s + h * C::LDX + c) = *reinterpret_cast<const uint4 *>(N.W1t + (int64_t)h * C::DP + c);
+    }
+    for (int e = tid; e < H * (H / 8); e += TT) {
+        const int h = e / (H / 8), c = 8 * (e % (H / 8));
+        *reinterpret_cast<uint4 *>(W2s + h * C::LDH + c) = *reinterpret_cast<const uint4 *>(N.W2t + (int64_t)h * H + c);
+    }
 
     for (int i = tid; i < H; i += TT) {
         bs[i] = N.b1[i];
         bs[H + i] = N.b2[i];
     }
     for (int i = tid; i < C::LOFF + 32; i += TT) { bs[2 * H + i] = N.b3[i]; }
-    for (int j = part; j < D; j += NWV) { xs[j * TW + col] = xsrc[(int64_t)j * W + wl]; }
-    for (int j = part; j < C::DP; j += NWV) { Xb[col * C::LDX + j] = f2bf(j < D ? xsrc[(int64_t)j * W + wl] * mcim::IN_SCALE : 0.f); }
+    for (int j = part; j < D; j += NPART) { xs[j * TW + col] = xsrc[(int64_t)j * W + wl]; }
+    for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xsrc[(int64_t)j * W + wl] * mcim::IN_SCALE : 0.f); }
     __syncthreads();
     h2_t mx[D];
     float lpi[2] = {0.f, 0.f};
-    network<D, H>(N, bs, Xb, Hb, mx, lpi, lgx, MODE == 0 ? out : nullptr, W, w0);
+    network<D, H>(N, W1s, W2s, bs, Xb, Hb, mx, lpi, lgx, MODE == 0 ? out : nullptr, W, w0);
     if (MODE == 0) { return; }
     // the chain state of walker col, held by wave 0
     float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
@@ -898,7 +917,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             Nm.has_spare = false;
             Nm.call = 1;
             const uint32_t base = 0x10000000u + ((uint32_t)part << 20);
-            for (int j = part - 1; j < D; j += NWV - 1) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1); }
+            for (int j = part - 1; j < D; j += NPART - 1) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1); }
         }
     };
     draw_normals(t0);
@@ -908,7 +927,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         // not fit the registers)
         NetDev Ns = N;
         const float *Ubase = T.U, *Umbase = T.Um;
-        asm volatile("" : "+s"(Ns.W1t), "+s"(Ns.W2t), "+s"(Ns.W3t), "+s"(Ubase), "+s"(Umbase));
+        asm volatile("" : "+s"(Ns.W3t), "+s"(Ubase), "+s"(Umbase));
         const uint64_t t = t0 + (uint64_t)st;
         const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
         uint32_t o[4] = {0u, 0u, 0u, 0u};
@@ -956,23 +975,26 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         }
         __syncthreads();
         fold<D>(mx, lpi, xps, cf, K);
-        for (int j = part; j < C::DP; j += NWV) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
+        for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
         {
             // target component wv at x' on the matrix cores (FP32 products and sums):
             // U_k x' - U_k mu_k for both walker halves, then |.|^2 over the 32 rows
             static_assert(D == 32 || D == 16 || D == 8, "dimension");
-            float2 acc2[2];
-            target_q<D>(Ubase, Umbase, wv, xps, lane, acc2);
+            // wave w: components w for both walker blocks (64 walkers) or 2 w, 2 w + 1 (32)
+            const int kq[2] = {NCB == 2 ? wv : 2 * wv, NCB == 2 ? wv : 2 * wv + 1};
+            const int cq[2] = {0, NCB == 2 ? 1 : 0};
+            float qv[2];
+            target_q<D>(Ubase, Umbase, kq, cq, xps, lane, qv);
             if (lane < 32) {
-                qn[wv * TW + lane] = acc2[0].x;
-                qn[wv * TW + 32 + lane] = acc2[1].x;
+                qn[kq[0] * TW + 32 * cq[0] + lane] = qv[0];
+                qn[kq[1] * TW + 32 * cq[1] + lane] = qv[1];
             }
         }
         __syncthreads();
         // the network at x' and log q(x | x')
         h2_t mx2[D];
         float lpi2[2] = {0.f, 0.f};
-        network<D, H>(Ns, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
+        network<D, H>(Ns, W1s, W2s, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
         fold<D>(mx2, lpi2, xs, cf2, K);
         __syncthreads();
         if (part == 0) {
@@ -1011,7 +1033,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         __syncthreads();
         // accepted walkers: position and mixture at x take the proposal's
         if (af[col]) {
-            for (int j = part; j < D; j += NWV) { xs[j * TW + col] = xps[j * TW + col]; }
+            for (int j = part; j < D; j += NPART) { xs[j * TW + col] = xps[j * TW + col]; }
         }
         {
             const bool a = af[32 * cbw + (lane & 31)] != 0;
@@ -1025,7 +1047,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     if (part == 0) { anyacc[col] = nacc; }
     __syncthreads();
     if (live && anyacc[col] > 0) {
-        for (int j = part; j < D; j += NWV) { St.x[(int64_t)j * W + w] = xs[j * TW + col]; }
+        for (int j = part; j < D; j += NPART) { St.x[(int64_t)j * W + w] = xs[j * TW + col]; }
         St.q[(int64_t)part * W + w] = qx[part * TW + col];
     }
     if (part == 0 && live) {
