@@ -570,6 +570,9 @@ typedef __fp16 h2_t __attribute__((ext_vector_type(2)));
 #ifndef MCIL_NS
 #define MCIL_NS 2 // last-layer row blocks per pass
 #endif
+#ifndef MCIL_PD
+#define MCIL_PD 3 // last-layer weight fragments in flight (reduction steps ahead, from L2)
+#endif
 #ifndef MCIL_TW
 #define MCIL_TW 64
 #endif
@@ -597,12 +600,12 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 
 // acc[r] = bias + rows row0 + 32 r .. + 31 of Wt [.][ldw] (BF16, row = output unit, KD used) times walkers
 // 32 cb .. + 31 of act [64][lda] (BF16, LDS), r < NR; bias in LDS, indexed by row
-template <int KD, int NR>
+template <int KD, int NR, int PD = 1>
 __device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, int ldw, const float *bias, int row0, const uint16_t *act, int lda,
                                           int cb, f32x16 acc[NR])
 {
     const int lane = threadIdx.x & 63;
-    constexpr int NQ = KD / 16;
+    constexpr int NQ = KD / 16, NB = PD + 1;
     const uint16_t *wr = Wt + (int64_t)(row0 + (lane & 31)) * ldw + 8 * (lane >> 5);
     const uint16_t *br = act + (32 * cb + (lane & 31)) * lda + 8 * (lane >> 5);
 #pragma unroll
@@ -616,18 +619,22 @@ __device__ __forceinline__ void mfma_tile(const uint16_t *__restrict__ Wt, int l
             acc[r][4 * g + 3] = b.w;
         }
     }
-    // one reduction step of fragments in flight ahead of the products (bounds the live registers)
-    bf16x8 a[2][NR], b[2];
+    // PD reduction steps of fragments in flight ahead of the products (a ring of PD + 1 sets;
+    // the depth bounds the live registers)
+    bf16x8 a[NB][NR], b[NB];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) { a[0][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * ldw); }
-    b[0] = *reinterpret_cast<const bf16x8 *>(br);
+    for (int q = 0; q < PD && q < NQ; ++q) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) { a[q][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * ldw + 16 * q); }
+        b[q] = *reinterpret_cast<const bf16x8 *>(br + 16 * q);
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int cu = q & 1, nx = cu ^ 1;
-        if (q + 1 < NQ) {
+        const int cu = q % NB, nx = (q + PD) % NB;
+        if (q + PD < NQ) {
 #pragma unroll
-            for (int r = 0; r < NR; ++r) { a[nx][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * ldw + 16 * (q + 1)); }
-            b[nx] = *reinterpret_cast<const bf16x8 *>(br + 16 * (q + 1));
+            for (int r = 0; r < NR; ++r) { a[nx][r] = *reinterpret_cast<const bf16x8 *>(wr + (int64_t)32 * r * ldw + 16 * (q + PD)); }
+            b[nx] = *reinterpret_cast<const bf16x8 *>(br + 16 * (q + PD));
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) { acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[cu][r], b[cu], acc[r], 0, 0, 0); }
@@ -690,7 +697,7 @@ __device__ __forceinline__ void network(const NetDev &N, const uint16_t *W1s, co
     const int col = 32 * cb + (lane & 31);
     if (wc == 0) {
         f32x16 al[1];
-        mfma_tile<H, 1>(N.W3t, H, b3, C::LOFF, Hb, C::LDH, cb, al);
+        mfma_tile<H, 1, MCIL_PD>(N.W3t, H, b3, C::LOFF, Hb, C::LDH, cb, al);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int k = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
@@ -706,7 +713,7 @@ __device__ __forceinline__ void network(const NetDev &N, const uint16_t *W1s, co
 #pragma unroll
         for (int rb0 = 0; rb0 < C::NRB; rb0 += NS) {
             f32x16 acc[NS];
-            mfma_tile<H, NS>(N.W3t, H, b3, 4 * D * wc + 32 * rb0, Hb, C::LDH, cb, acc);
+            mfma_tile<H, NS, MCIL_PD>(N.W3t, H, b3, 4 * D * wc + 32 * rb0, Hb, C::LDH, cb, acc);
 #pragma unroll
             for (int rr = 0; rr < NS; ++rr) {
                 const int rb = rb0 + rr;
@@ -920,7 +927,32 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             for (int j = part - 1; j < D; j += NPART - 1) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1); }
         }
     };
-    draw_normals(t0);
+    // the component of step t from the logits at x (wave 0; o holds the step's words: [0] the
+    // component, [1] the acceptance uniform)
+    uint32_t o[4] = {0u, 0u, 0u, 0u};
+    auto choose = [&](uint64_t t) {
+        philox(0x10000000u, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
+        float amax = -__builtin_huge_valf();
+        for (int k = 0; k < K; ++k) { amax = fmaxf(amax, lgx[k * TW + col]); }
+        float e[KMAX], sa = 0.f;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            e[k] = (k < K) ? __expf(lgx[k * TW + col] - amax) : 0.f;
+            sa += e[k];
+        }
+        const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
+        int kk = K - 1;
+        float cum = 0.f;
+#pragma unroll
+        for (int k = 0; k < KMAX - 1; ++k) {
+            cum += e[k];
+            if (k < K - 1 && kk == K - 1 && uc < cum) { kk = k; }
+        }
+        kst[col] = kk;
+    };
+    if (part == 0) { choose(t0); }
+    else { draw_normals(t0); }
+    __syncthreads();
     for (int st = 0; st < nl; ++st) {
         // the weights and target factors are read afresh each step: an empty asm makes their
         // addresses opaque, so the compiler does not hoist those loads out of the loop (they would
@@ -929,30 +961,6 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         const float *Ubase = T.U, *Umbase = T.Um;
         asm volatile("" : "+s"(Ns.W3t), "+s"(Ubase), "+s"(Umbase));
         const uint64_t t = t0 + (uint64_t)st;
-        const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
-        uint32_t o[4] = {0u, 0u, 0u, 0u};
-        // the component from the logits at x
-        if (part == 0) {
-            philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
-            float amax = -__builtin_huge_valf();
-            for (int k = 0; k < K; ++k) { amax = fmaxf(amax, lgx[k * TW + col]); }
-            float e[KMAX], sa = 0.f;
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                e[k] = (k < K) ? __expf(lgx[k * TW + col] - amax) : 0.f;
-                sa += e[k];
-            }
-            const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
-            int kk = K - 1;
-            float cum = 0.f;
-#pragma unroll
-            for (int k = 0; k < KMAX - 1; ++k) {
-                cum += e[k];
-                if (k < K - 1 && kk == K - 1 && uc < cum) { kk = k; }
-            }
-            kst[col] = kk;
-        }
-        __syncthreads();
         // x' = mu + sigma z from the lanes holding the chosen component
         if (2 * wc < K) {
             const int cc = 32 * cbw + (lane & 31);
@@ -997,6 +1005,8 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         network<D, H>(Ns, W1s, W2s, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
         fold<D>(mx2, lpi2, xs, cf2, K);
         __syncthreads();
+        // wave 0: the MH test, the accepted walkers' new state, then the next step's component;
+        // waves 1 .. 7 meanwhile draw the next step's normals
         if (part == 0) {
             const float logq_f = mixture_logq(lgx, cf, K, D, col);
             const float logq_r = mixture_logq(lg, cf2, K, D, col);
@@ -1013,10 +1023,12 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
             const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - (m + __logf(S)) + (logq_r - logq_f));
             af[col] = acc ? 1 : 0;
+            const float x0 = acc ? xps[col] : xs[col];
             if (acc) {
                 double s2 = 0.;
                 for (int j = 0; j < D; ++j) {
                     const float v = xps[j * TW + col];
+                    xs[j * TW + col] = v;
                     s2 = fma((double)v, (double)v, s2);
                 }
 #pragma unroll
@@ -1027,14 +1039,12 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
                 sq = s2;
                 ++nacc;
             }
-            if (live && St.B > 0) { accumulate(St, w, true, sq, acc ? xps[col] : xs[col], s0, c0, s1, c1, jb, bi); }
+            if (live && St.B > 0) { accumulate(St, w, true, sq, x0, s0, c0, s1, c1, jb, bi); }
+            if (st + 1 < nl) { choose(t + 1); }
         }
         else if (st + 1 < nl) { draw_normals(t + 1); }
         __syncthreads();
-        // accepted walkers: position and mixture at x take the proposal's
-        if (af[col]) {
-            for (int j = part; j < D; j += NPART) { xs[j * TW + col] = xps[j * TW + col]; }
-        }
+        // accepted walkers: the mixture at x takes the proposal's
         {
             const bool a = af[32 * cbw + (lane & 31)] != 0;
 #pragma unroll
