@@ -150,7 +150,54 @@ def run_precision_form():
             "fp32_vs_fp64_max_diff_over_err": float(np.max(np.abs(a32 - a64) / e64))}
 
 
-def run_batched(reps=5):
+def cpu_batched_sample(ndim, nmc, w, s, mu, budget_s=8.0):
+    """The reference CPU path on the batched workload: each integral as the reference computes one
+    (one MCI, the isotropic-mixture plugin of oracle/ref_harness.cpp, nmc single-coordinate steps
+    with its own findMRT2Step and automatic decorrelation, |x|^2 in a simple accumulator), on a
+    subsample of every k-th integral sized to about budget_s of CPU work, P processes."""
+    binary = os.path.join(ROOT, "oracle", "_ref", "mci_ref_fast")
+    if not os.path.exists(binary):
+        return None
+    n = len(ndim)
+
+    def block(i):
+        K = len(w[i])
+        p = np.concatenate([[K], w[i], s[i], np.asarray(mu[i]).ravel()])
+        a = " ".join(float(v).hex() for v in p)
+        return (f"bench 1 {int(nmc[i])} {9000 + i}\ncreate {int(ndim[i])}\nmove 0 1 1\npdf 4 {len(p)} {a}\n"
+                f"obs 2 0 0 1 0 0\nendbench\n")
+
+    # per-step cost from a small probe, then a subsample of every k-th integral
+    probe = list(range(0, n, max(1, n // 16)))
+    t0 = time.time()
+    subprocess.run([binary], input="".join(block(i) for i in probe), capture_output=True, text=True, check=True)
+    rate1 = sum(int(nmc[i]) for i in probe) / max(time.time() - t0, 1e-3)
+    procs = max(1, min(16, os.cpu_count() or 1))
+    k = max(1, int(np.ceil(float(np.sum(nmc)) / (rate1 * procs * budget_s))))
+    chosen = list(range(0, n, k))
+    shares = [chosen[r::procs] for r in range(procs)]
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        ins = []
+        for r, sh in enumerate(shares):
+            fn = os.path.join(tmp, f"in{r}.txt")
+            with open(fn, "w") as f:
+                f.write("".join(block(i) for i in sh))
+            ins.append(fn)
+        t0 = time.time()
+        ps = []
+        for r, fn in enumerate(ins):  # inputs and outputs through files: no pipe can fill up
+            ps.append(subprocess.Popen([binary], stdin=open(fn), stdout=open(os.path.join(tmp, f"out{r}.txt"), "w")))
+        for pr in ps:
+            pr.wait()
+        wall = time.time() - t0
+    steps = int(sum(int(nmc[i]) for i in chosen))
+    rate = steps / wall
+    return {"integrals": len(chosen), "of": n, "walker_steps": steps, "processes": procs, "wall_s": wall,
+            "walker_steps_per_s": rate, "seconds_for_all": float(np.sum(nmc)) / rate}
+
+
+def run_batched(reps=5, with_cpu=True):
     """4,096 independent Gaussian-mixture integrals (d log-uniform in [1, 128], nmc log-uniform in
     [1e3, 1e5], 1-4 isotropic components), |x|^2, FP32 walkers, one hipGraph replay per batch."""
     from mcintegratorplusplus_amd import BatchIntegrals, mixture_workload
@@ -170,11 +217,16 @@ def run_batched(reps=5):
     ms = 1000 * (time.perf_counter() - t0) / reps
     steps = int(np.sum(R * T))
     z = (avg - exact) / err
-    return {"workload": "4096 mixture integrals, d in [1,128], nmc in [1e3,1e5], |x|^2",
+    out = {"workload": "4096 mixture integrals, d in [1,128], nmc in [1e3,1e5], |x|^2",
             "ms_per_batch": ms, "walker_steps": steps, "walker_steps_per_s": steps / (ms / 1000),
             "requested_steps": int(np.sum(nmc)), "walkers": int(np.sum(R)),
             "phase_ms_eager": {"calibration": phases[0], "sampling": phases[1], "estimation": phases[2]},
             "frac_abs_z_below_3": float(np.mean(np.abs(z) < 3)), "mean_z2": float(np.mean(z * z))}
+    cpu = cpu_batched_sample(ndim, nmc, w, s, mu) if with_cpu else None
+    if cpu is not None:
+        out["reference_cpu"] = cpu
+        out["speedup_vs_reference_cpu"] = cpu["seconds_for_all"] / (ms / 1000)
+    return out
 
 
 def cpu_learned_target(w, mu, cov, var, nsteps, budget_s=10.0):
@@ -346,7 +398,7 @@ def main():
                      "hbm_gbps": (prof["bytes_per_launch"] / (kt[2] / 1000) / 1e9) if "bytes_per_launch" in prof else None},
     }
     if args.workloads != "none" and args.precision == 0 and not args.generic:
-        runs = {"precision_fp32": ("precision", run_precision_form), "batched": ("batched", run_batched),
+        runs = {"precision_fp32": ("precision", run_precision_form), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
                 "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu))}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
