@@ -282,11 +282,82 @@ __device__ inline void mjblocker(int64_t n, const Col &x, double *y, int64_t yst
     err = sqrt(var[k] / ldexp(1.0, npow - k));
 }
 
+// the same for n <= 64 (at most 6 blocking levels): identical operations in identical order,
+// with the per-level arrays in registers (fixed-size loops, runtime indices as selects) instead
+// of private memory
+template <int L>
+__device__ inline double pick(const double (&a)[L], int i)
+{
+    double v = a[0];
+#pragma unroll
+    for (int q = 1; q < L; ++q) { v = (i == q) ? a[q] : v; }
+    return v;
+}
+
+__device__ inline void mjblocker_small(int64_t n, const Col &x, double *y, int64_t ystride, double &avg, double &err)
+{
+    constexpr int LMAX = 6; // log2(MJ_LDS_MAX)
+    int npow = 0;
+    while (((int64_t)1 << npow) < n) { ++npow; }
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) { const double v = x[i]; y[i * ystride] = v; mean = mean + v; }
+    mean = mean / (double)n;
+    double var[LMAX], gam[LMAX];
+    int64_t nred = n;
+#pragma unroll
+    for (int k = 0; k < LMAX; ++k) {
+        var[k] = 0.0;
+        gam[k] = 0.0;
+        if (k < npow) {
+            double g0 = 0.0, g1 = 0.0;
+            double prev = y[0] - mean;
+            g0 = g0 + prev * prev;
+            for (int64_t i = 1; i < nred; ++i) {
+                const double cur = y[i * ystride] - mean;
+                g0 = g0 + cur * cur;
+                g1 = g1 + prev * cur;
+                prev = cur;
+            }
+            var[k] = g0 / (double)nred;
+            gam[k] = g1 / (double)nred;
+            for (int64_t i = 0; i < nred / 2; ++i) {
+                y[i * ystride] = 0.5 * (y[(2 * i) * ystride] + y[(2 * i + 1) * ystride]);
+            }
+            nred = nred / 2;
+        }
+    }
+    if (npow == 0) { avg = mean; err = 0.0; return; }
+    // M of level i goes to reversed position npow - 1 - i; cumulative sums in that order
+    double Ml[LMAX];
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i) {
+        const double v = var[i], g = gam[i];
+        const double r = g / v;
+        Ml[i] = (v == 0.0) ? 0.0 : (r * r) * ldexp(1.0, npow - i);
+    }
+    double ms[LMAX];
+    double msum = 0.0;
+#pragma unroll
+    for (int q = 0; q < LMAX; ++q) {
+        if (q < npow) { msum = msum + pick(Ml, npow - 1 - q); }
+        ms[q] = msum;
+    }
+    int kk = -1;
+#pragma unroll
+    for (int q = LMAX - 1; q >= 0; --q) {
+        if (kk < 0 && q < npow && ms[q] < c_chi2_q95[q]) { kk = q; }
+    }
+    int k = npow - (kk + 1);
+    if (k >= npow) { k = npow - 1; }
+    avg = mean;
+    err = sqrt(pick(var, k) / ldexp(1.0, npow - k));
+}
+
 // MJBlocker's working copy: this thread's LDS column when the series is short (block means),
 // else its column of the scratch copy in HBM
 __device__ inline void mj_any(const EstimJob &j, const Col &x, int comp, int64_t w, double *sh, double &a, double &e)
 {
-    if (j.n <= MJ_LDS_MAX) { mjblocker(j.n, x, sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
+    if (j.n <= MJ_LDS_MAX) { mjblocker_small(j.n, x, sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
     else { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
 }
 
