@@ -65,23 +65,23 @@ def ess_per_obs(err, walkers_steps):
     return s2 / err ** 2
 
 
-def cpu_baseline(budget_s=20.0):
+def cpu_reference_gauss(dim, sig, obs_script, budget_s):
     """The reference CPU path (oracle/_ref/mci_ref_fast: the reference library built from its
-    own sources with -O3 -flto) timed on this host: P processes, each integrating its share of
-    independent walkers of the same workload (per-process calibration, no MPI)."""
+    own sources with -O3 -flto) on a diagonal Gaussian in `dim` dimensions, timed on this host:
+    P processes, each integrating its share of independent walkers of the workload (one MCI per
+    walker with its own calibration and NDECORR decorrelation steps, no MPI).  Returns the
+    walkers, walker-steps, wall seconds and the sum of the walkers' squared errors."""
     binary = os.path.join(ROOT, "oracle", "_ref", "mci_ref_fast")
-    sig = sigmas()
-    a = " ".join(float(v).hex() for v in 0.5 / sig ** 2)
-    script = (f"create {D}\nmove 0 1 1\nndecorr {NDECORR}\npdf 2 {D} {a}\n"
-              f"obs 1 0 {BLOCK} 1 1 4\nobs 2 0 {BLOCK} 1 1 4\nendbench\n")
-    procs = max(1, min(16, os.cpu_count() or 1))
     if not os.path.exists(binary):
         return None
+    a = " ".join(float(v).hex() for v in 0.5 / sig ** 2)
+    script = f"create {dim}\nmove 0 1 1\nndecorr {NDECORR}\npdf 2 {dim} {a}\n{obs_script}endbench\n"
+    procs = max(1, min(16, os.cpu_count() or 1))
     # calibrate the per-walker cost with one process, then size the sample to the budget
     t0 = time.time()
-    r = subprocess.run([binary], input=f"bench 8 {NMC} 9001\n" + script, capture_output=True, text=True, check=True)
-    per_walker = max((time.time() - t0) / 8, 1e-4)
-    per_proc = int(min(100000, max(8, budget_s / (procs * per_walker))))  # ~budget_s of CPU work in total
+    subprocess.run([binary], input=f"bench 4 {NMC} 9001\n" + script, capture_output=True, text=True, check=True)
+    per_walker = max((time.time() - t0) / 4, 1e-4)
+    per_proc = int(min(100000, max(4, budget_s / (procs * per_walker))))  # ~budget_s of CPU work in total
     t0 = time.time()
     ps = [subprocess.Popen([binary], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
           for _ in range(procs)]
@@ -92,10 +92,17 @@ def cpu_baseline(budget_s=20.0):
     for p in ps:
         p.wait()
     wall = time.time() - t0
-    nw = sum(o["walkers"] for o in outs)
-    steps = sum(o["walker_steps"] for o in outs)
-    err2 = np.sum([[float.fromhex(v) for v in o["err2_sum"]] for o in outs], axis=0)
-    err = np.sqrt(err2) / nw
+    return dict(walkers=sum(o["walkers"] for o in outs), steps=sum(o["walker_steps"] for o in outs), wall=wall, procs=procs,
+                err2=np.sum([[float.fromhex(v) for v in o["err2_sum"]] for o in outs], axis=0))
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference CPU path on the benchmark workload (d = 16, x_i^2 and |x|^2)."""
+    r = cpu_reference_gauss(D, sigmas(), f"obs 1 0 {BLOCK} 1 1 4\nobs 2 0 {BLOCK} 1 1 4\n", budget_s)
+    if r is None:
+        return None
+    nw, steps, wall, procs = r["walkers"], r["steps"], r["wall"], r["procs"]
+    err = np.sqrt(r["err2"]) / nw
     ess = ess_per_obs(err, steps)
     return dict(value=steps / wall, unit="walker-steps/s", cores=procs, kind="reference",
                 ess_per_s=float(np.min(ess)) / wall,
@@ -114,7 +121,7 @@ def profile_summary():
         return json.load(f)
 
 
-def run_precision_form():
+def run_precision_form(with_cpu=True):
     """d = 64, 262,144 walkers x 10,240 steps (x_i^2 and exp(-|x|^2/64) in 64 blocks of 160,
     MJBlocker, shared calibration, 1,000 decorrelation steps): the FP32 chain (Kahan block sums)
     against the FP64 chain on the same Philox words and against the analytic values."""
@@ -147,7 +154,18 @@ def run_precision_form():
             "fp32": {"ms": ms32, "walker_steps_per_s": Wp * NMC / (ms32 / 1000), "sampling_kernel_ms": kt32[2],
                      "max_abs_z": float(np.max(np.abs(a32 - exact) / e32))},
             "fp32_vs_fp64_max_rel_diff": float(np.max(np.abs(a32 - a64) / np.abs(a64))),
-            "fp32_vs_fp64_max_diff_over_err": float(np.max(np.abs(a32 - a64) / e64))}
+            "fp32_vs_fp64_max_diff_over_err": float(np.max(np.abs(a32 - a64) / e64)),
+            "reference_cpu": cpu_precision_form(sig) if with_cpu else None}
+
+
+def cpu_precision_form(sig, budget_s=8.0):
+    """The reference CPU path on the precision-form workload (d = 64, x_i^2 and exp(-|x|^2/64)):
+    walker-steps/s of a bounded sample of independent walkers."""
+    r = cpu_reference_gauss(64, sig, f"obs 1 0 {BLOCK} 1 1 4\nobs 6 1 0x1p+6 {BLOCK} 1 1 4\n", budget_s)
+    if r is None:
+        return None
+    return {"walkers": r["walkers"], "steps_per_walker": NMC, "processes": r["procs"], "wall_s": r["wall"],
+            "walker_steps_per_s": r["steps"] / r["wall"]}
 
 
 def cpu_batched_sample(ndim, nmc, w, s, mu, budget_s=8.0):
@@ -398,7 +416,7 @@ def main():
                      "hbm_gbps": (prof["bytes_per_launch"] / (kt[2] / 1000) / 1e9) if "bytes_per_launch" in prof else None},
     }
     if args.workloads != "none" and args.precision == 0 and not args.generic:
-        runs = {"precision_fp32": ("precision", run_precision_form), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
+        runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
                 "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu))}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
