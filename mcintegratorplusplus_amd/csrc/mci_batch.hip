@@ -23,7 +23,7 @@
 //   c0 = 0x20000000 + k/4  Lemire rejection words k of that step
 //   c0 = 0x40000000 + n  (step 0) initial draw: call 0 picks the component, calls 1.. feed
 //                        Marsaglia's polar method two attempts per call
-// Sampling walkers use id = walker index; calibration lanes use id = 2^31 | (64 i + lane).
+// Sampling walkers use id = walker index; calibration lanes use id = 2^31 | (CALR i + lane).
 #include "mci_batch.hpp"
 #include "mci_f32math.hpp"
 #include "../../include/mci_amd.h"
@@ -281,14 +281,14 @@ __global__ void k_batch_zero(unsigned int *q, int n)
     if ((int)threadIdx.x < n) { q[threadIdx.x] = 0u; }
 }
 
-// MCI::findMRT2Step (src/MCIntegrator.cpp:87-168) for one integral with its 64 lanes as 64
-// ranks: MIN_STAT = max(100, sqrt(40000 d)/64), the rank-averaged acceptance rate (tree sum
-// over lanes), MIN_CONS 5, TOL 0.05, factor in [0.5, 2], step in [FLT_MIN, FLT_MAX]
+// MCI::findMRT2Step (src/MCIntegrator.cpp:87-168) for one integral with its CALR lanes as CALR
+// ranks: MIN_STAT = max(100, sqrt(40000 d)/CALR), the rank-averaged acceptance rate (tree sum
+// over the group's lanes), MIN_CONS 5, TOL 0.05, factor in [0.5, 2], step in [FLT_MIN, FLT_MAX]
 template <int KW>
 __device__ void calibrate_loop(const CalibArgs &A, BW &L, float *xs, int lane, const Keys &K, double &step, double &rate)
 {
     const int N = A.nfind;
-    const int64_t min_stat = (int64_t)fmax(100., sqrt(40000. * L.d) / (double)WAVE);
+    const int64_t min_stat = (int64_t)fmax(100., sqrt(40000. * L.d) / (double)CALR);
     int cons = 0, counter = 0;
     Pre cur;
     bw_fetch<KW>(L, L.t, cur, K);
@@ -297,9 +297,9 @@ __device__ void calibrate_loop(const CalibArgs &A, BW &L, float *xs, int lane, c
         int64_t acc = 0;
         for (int64_t s = 0; s < min_stat; ++s) { acc += bw_step<KW>(L, xs, lane, cur, K) ? 1 : 0; }
         double v = (acc > 0) ? (double)acc / ((double)acc + (double)(min_stat - acc)) : 0.;
-        for (int o = WAVE / 2; o > 0; o >>= 1) { v = v + __shfl_down(v, o); }
-        v = __shfl(v, 0);
-        rate = v / (double)WAVE;
+        for (int o = CALR / 2; o > 0; o >>= 1) { v = v + __shfl_down(v, o, CALR); }
+        v = __shfl(v, 0, CALR);
+        rate = v / (double)CALR;
         if (fabs(rate - A.target) < 0.05) { ++cons; }
         else { cons = 0; }
         const double fact = fmin(2., fmax(0.5, rate / A.target));
@@ -311,17 +311,22 @@ __device__ void calibrate_loop(const CalibArgs &A, BW &L, float *xs, int lane, c
     }
 }
 
-__global__ __launch_bounds__(WAVE) void k_batch_calibrate(CalibArgs A)
+// one integral per group of CALR lanes (WAVE / CALR integrals per wave; the host sorts each
+// launch's list by component count so that a wave's groups mostly take the same path)
+__global__ __launch_bounds__(WAVE) void k_batch_calibrate(CalibArgs A, int nlist)
 {
     extern __shared__ float cal_xs[];
-    const int i = A.ilist[blockIdx.x];
     const int lane = threadIdx.x;
+    const int gi = blockIdx.x * (WAVE / CALR) + lane / CALR;
+    if (gi >= nlist) { return; } // whole groups leave together
+    const int i = A.ilist[gi];
+    const int l = lane % CALR;
     double step = A.step0[i];
     double rate = 0.;
     if (A.nfind != 0) {
         const Keys K = make_keys(A.P.key0, A.P.key1);
         BW L;
-        bw_load(L, A.P, i, 0x80000000u | (uint32_t)(i * WAVE + lane));
+        bw_load(L, A.P, i, 0x80000000u | (uint32_t)(i * CALR + l));
         bw_draw(L, cal_xs, lane, A.P, K, i);
         bw_refresh(L, cal_xs, lane);
         switch (L.K) {
@@ -331,7 +336,7 @@ __global__ __launch_bounds__(WAVE) void k_batch_calibrate(CalibArgs A)
         default: calibrate_loop<4>(A, L, cal_xs, lane, K, step, rate); break;
         }
     }
-    if (lane == 0) {
+    if (l == 0) {
         A.step[i] = step;
         A.rate[i] = rate;
     }
@@ -473,7 +478,8 @@ hipError_t launch_batch_zero(unsigned int *q, int n, hipStream_t s)
 
 hipError_t launch_batch_calibrate(const CalibArgs &a, int nlist, int dmax, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_calibrate, dim3((unsigned)nlist), dim3(WAVE), batch_lds_bytes(dmax), s, a);
+    const unsigned g = (unsigned)((nlist + WAVE / CALR - 1) / (WAVE / CALR));
+    hipLaunchKernelGGL(k_batch_calibrate, dim3(g), dim3(WAVE), batch_lds_bytes(dmax), s, a, nlist);
     return hipGetLastError();
 }
 

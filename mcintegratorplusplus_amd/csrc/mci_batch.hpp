@@ -10,6 +10,8 @@ namespace mcib
 {
 constexpr int WAVE = 64;
 constexpr int MAXK = 4;
+constexpr int CALR = 64;  // calibration lanes (pooled ranks) per integral: one wave (16 lanes, four
+                          // integrals per wave, measured slower: the wave count, not the lane-steps, sets the time)
 
 // per-integral parameters, struct-of-arrays over integrals (integral index order)
 struct BatchParams
@@ -29,7 +31,7 @@ struct BatchParams
 struct CalibArgs
 {
     BatchParams P;
-    const int *ilist;         // integrals of this launch (one workgroup each)
+    const int *ilist;         // integrals of this launch (CALR lanes each)
     int nfind;
     double target;
     const double *step0;      // [nint] initial step

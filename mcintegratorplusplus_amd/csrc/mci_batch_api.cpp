@@ -297,7 +297,8 @@ mci_status ensure_uploaded(mci_batch *b)
         HIPCHK(b->d_err.ensure(b->nint));
         HIPCHK(b->d_accr.ensure(b->nint));
         HIPCHK(b->d_queue.ensure(NCLASS));
-        // integrals by dimension class for the calibration launches
+        // integrals by dimension class for the calibration launches, each class sorted by
+        // component count and dimension (the groups of one wave then mostly take one path)
         std::vector<int> il;
         for (int c = 0; c < NCLASS; ++c) {
             b->int_off[c] = (int)il.size();
@@ -308,6 +309,10 @@ mci_status ensure_uploaded(mci_batch *b)
                     b->int_dmax[c] = std::max(b->int_dmax[c], b->d[i]);
                 }
             }
+            std::stable_sort(il.begin() + b->int_off[c], il.end(), [&](int x, int y) {
+                if (b->K[x] != b->K[y]) { return b->K[x] < b->K[y]; }
+                return b->d[x] < b->d[y];
+            });
         }
         b->int_off[NCLASS] = (int)il.size();
         TRY(upload(b->d_ilist, il, b->stream));

@@ -29,6 +29,7 @@
 
 #define BK 4 /* components per integral, at most */
 #define BW 64
+#define CALR 64 /* calibration lanes (pooled ranks) per integral */
 
 static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -262,29 +263,29 @@ static void derive(int nint, const int *d, const int *K, const double *w, const 
     }
 }
 
-/* findMRT2Step over 64 pooled lanes (MCIntegrator.cpp:87-168 with R = 64 ranks) */
+/* findMRT2Step over CALR pooled lanes (MCIntegrator.cpp:87-168 with R = CALR ranks) */
 static void calibrate(const bparams *P, int i, int nfind, double target, double *step_out, double *rate_out)
 {
     double step = P->step0[i], rate = 0.;
     const int N = nfind;
     if (N != 0) {
-        bwalker *L = malloc(sizeof(bwalker) * BW);
-        for (int l = 0; l < BW; ++l) {
-            bw_load(&L[l], P, i, 0x80000000u | (uint32_t)(i * BW + l));
+        bwalker *L = malloc(sizeof(bwalker) * CALR);
+        for (int l = 0; l < CALR; ++l) {
+            bw_load(&L[l], P, i, 0x80000000u | (uint32_t)(i * CALR + l));
             bw_draw(&L[l], P, i);
             bw_refresh(&L[l]);
         }
-        const int64_t min_stat = (int64_t)fmax(100., sqrt(40000. * P->d[i]) / (double)BW);
+        const int64_t min_stat = (int64_t)fmax(100., sqrt(40000. * P->d[i]) / (double)CALR);
         int cons = 0, counter = 0;
-        double v[BW];
+        double v[CALR];
         while ((N < 0 && cons < 5) || counter < N) {
-            for (int l = 0; l < BW; ++l) {
+            for (int l = 0; l < CALR; ++l) {
                 L[l].step = (float)step;
                 int64_t acc = 0;
                 for (int64_t s = 0; s < min_stat; ++s) acc += bw_step(&L[l], P);
                 v[l] = (acc > 0) ? (double)acc / ((double)acc + (double)(min_stat - acc)) : 0.;
             }
-            rate = orc_det_sum(v, BW) / (double)BW;
+            rate = orc_det_sum(v, CALR) / (double)CALR;
             if (fabs(rate - target) < 0.05) ++cons;
             else cons = 0;
             const double fact = fmin(2., fmax(0.5, rate / target));
