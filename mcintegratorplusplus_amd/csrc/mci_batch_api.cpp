@@ -135,7 +135,9 @@ void derive(mci_batch *b)
             b->cw[i * MAXK + k] = (k == K - 1) ? 1.0f : (float)cum;
             st = st + wn * s[k];
         }
-        b->step0[i] = 2.0 * st;
+        // a uniform single-coordinate move of half-width 2.93 s is accepted half the time on a
+        // Gaussian of width s: start calibration there (with the weighted mean width)
+        b->step0[i] = 2.93 * st;
     }
 }
 

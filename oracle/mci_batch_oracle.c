@@ -257,7 +257,7 @@ static void derive(int nint, const int *d, const int *K, const double *w, const 
             P->cw[i * BK + k] = (k == K[i] - 1) ? 1.0f : (float)cum;
             st = st + wn * sk;
         }
-        P->step0[i] = 2.0 * st;
+        P->step0[i] = 2.93 * st; /* half acceptance on a Gaussian of width st */
         co += K[i];
         mo += (int64_t)K[i] * d[i];
     }
