@@ -295,7 +295,7 @@ def run_learned(with_cpu=True):
     variance and err the standard error of the walker means; time = calibration + sampling.  The
     network is trained on pairs of independent snapshots of the plain FP32 chain (16 snapshots,
     512 steps apart; 3 x 1,000 Adam steps of 4,096 pairs, learning rate 1e-2, 5e-3, 2.5e-3)."""
-    from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP64, LearnedProposalSampler, mixture_target
+    from mcintegratorplusplus_amd.learned import INDEP_FP32, MIXED_FP32, PLAIN_FP64, LearnedProposalSampler, mixture_target
     w, mu, cov, (ex2, pind) = mixture_target(d=32, ncomp=8, seed=5)
     ex4 = float(np.sum(w * [(np.trace(c) + m @ m) ** 2 + 2 * np.trace(c @ c) + 4 * m @ c @ m for m, c in zip(mu, cov)]))
     var = np.array([ex4 - ex2 ** 2, pind * (1 - pind)])
@@ -326,6 +326,20 @@ def run_learned(with_cpu=True):
                                          "z": list((a_m - exact) / e_m), "learned_acceptance": s.results()["acc_learned"],
                                          "ess_per_s_gain_vs_plain_fp64": list(ess_m / ess_p),
                                          "ess_per_step_gain": list((e_p / e_m) ** 2)}
+    # the independence form: a second network trained with one fixed input, its mixture there
+    # the proposal q(x') of every learned move, mixed 50/50 with random-walk moves
+    s.setNetwork(128, 8)
+    s.setFixedInput(True)
+    s.collect(16, 512)
+    for lr in (1e-2, 5e-3, 2.5e-3):
+        s.train(1000, 4096, lr)
+    a_i, e_i = s.run(INDEP_FP32, NMC, 64, 0.5)
+    t_i = s.timings()
+    sec_i = (t_i["sampling"] + t_i["calibration"]) / 1000
+    ess_i = var / e_i ** 2 / sec_i
+    out["independence_fp32_pmix0.5"] = {"seconds": sec_i, "err": list(e_i), "ess_per_s": list(ess_i), "z": list((a_i - exact) / e_i),
+                                         "learned_acceptance": s.results()["acc_learned"],
+                                         "ess_per_s_gain_vs_plain_fp64": list(ess_i / ess_p), "ess_per_step_gain": list((e_p / e_i) ** 2)}
     cpu = cpu_learned_target(w, mu, cov, var, NMC) if with_cpu else None
     if cpu is not None:
         out["reference_cpu"] = cpu

@@ -314,6 +314,10 @@ mci_status mci_lp_collect(mci_lp *lp, int nsnap, int64_t lag);
 /* training pairs (x_a, x_b): x_b is the same walker at another snapshot with probability same_fraction
  * (default 0.5; teaches moves within a mode), else another walker (an independent sample of p) */
 mci_status mci_lp_set_training_pairs(mci_lp *lp, double same_fraction);
+/* on: every training input is one fixed position (the mean of the last snapshot), so the network's
+ * mixture there is an independence proposal q(x'), fitted to the plain chain's samples; it is the
+ * proposal of MCI_LP_INDEP_FP32 (default off) */
+mci_status mci_lp_set_fixed_input(mci_lp *lp, int on);
 /* Adam on minibatches of training pairs; losses of the first and last minibatch */
 mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *loss_first, double *loss_last);
 /* last run: [0..1] avg, [2..3] standard error of walker means, [4..5] MPIMCI combine of the

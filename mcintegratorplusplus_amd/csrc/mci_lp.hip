@@ -1084,7 +1084,7 @@ __global__ void k_lp_snapshot(State<R> St, int d, float *dst)
 }
 
 __global__ void k_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float same_frac,
-                           float *ctx, float *tgt)
+                           const float *fixed_ctx, float *ctx, float *tgt)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= B) { return; }
@@ -1099,7 +1099,7 @@ __global__ void k_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B
     const float *a = snap + (int64_t)s1 * d * W;
     const float *b = snap + (int64_t)s2 * d * W;
     for (int j = 0; j < d; ++j) {
-        ctx[(int64_t)j * B + p] = a[(int64_t)j * W + w1];
+        ctx[(int64_t)j * B + p] = fixed_ctx ? fixed_ctx[j] : a[(int64_t)j * W + w1];
         tgt[(int64_t)j * B + p] = b[(int64_t)j * W + w2];
     }
 }
@@ -1197,9 +1197,9 @@ hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t 
 }
 
 hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1, float same_frac,
-                           float *ctx, float *tgt, hipStream_t s)
+                           const float *fixed_ctx, float *ctx, float *tgt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lp_pairs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, snap, nsnap, d, W, B, iter, k0, k1, same_frac, ctx, tgt);
+    hipLaunchKernelGGL(k_lp_pairs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, snap, nsnap, d, W, B, iter, k0, k1, same_frac, fixed_ctx, ctx, tgt);
     return hipGetLastError();
 }
 

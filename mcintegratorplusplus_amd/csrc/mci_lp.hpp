@@ -80,7 +80,8 @@ hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, co
 template <class R>
 hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t s);
 // training pairs: column p of ctx/tgt [d][B] from snapshots [nsnap][d][W]; with probability
-// same_frac the target is the same walker at another snapshot, else another walker (24-bit index)
+// same_frac the target is the same walker at another snapshot, else another walker (24-bit index);
+// with fixed_ctx (device [d]) every input is that position instead
 hipError_t launch_lp_pairs(const float *snap, int nsnap, int d, int64_t W, int B, uint64_t iter, uint32_t k0, uint32_t k1,
-                           float same_frac, float *ctx, float *tgt, hipStream_t s);
+                           float same_frac, const float *fixed_ctx, float *ctx, float *tgt, hipStream_t s);
 } // namespace mcil

@@ -280,6 +280,8 @@ struct mci_lp
     DBuf<float> snap, nb1, nb2, nb3;
     DBuf<uint16_t> nW1t, nW2t, nW3t;
     uint64_t train_iter = 0; // minibatches drawn so far (each draws its own pairs)
+    bool fixed_input = false; // train with the fixed input ctx0 (an independence proposal)
+    DBuf<float> dctx0;        // ctx0 on the device
     int nsnap = 0;
     double same_frac = 0.5;
     std::vector<float> ctx0;   // mean of the last snapshot: the network input of the independence proposal
@@ -988,6 +990,10 @@ mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *los
     TRY(net_ready(lp));
     mci_mlp *n = lp->net;
     TRY(mlp_reserve(n, batch));
+    if (lp->fixed_input) {
+        HIPCHK(lp->dctx0.ensure((size_t)lp->d));
+        HIPCHK(hipMemcpy(lp->dctx0.p, lp->ctx0.data(), sizeof(float) * (size_t)lp->d, hipMemcpyHostToDevice));
+    }
     const uint32_t k0 = (uint32_t)lp->seed ^ 0x85EBCA6Bu, k1 = (uint32_t)(lp->seed >> 32) ^ 0xC2B2AE35u;
     auto t0 = clk::now();
     std::vector<float> l((size_t)batch);
@@ -1000,8 +1006,8 @@ mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *los
         return MCI_OK;
     };
     for (int it = 0; it < iters; ++it) {
-        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, lp->train_iter++, k0, k1, (float)lp->same_frac, n->X.p,
-                                     n->Y.p, n->stream));
+        HIPCHK(mcil::launch_lp_pairs(lp->snap.p, lp->nsnap, lp->d, lp->W, batch, lp->train_iter++, k0, k1, (float)lp->same_frac,
+                                     lp->fixed_input ? lp->dctx0.p : nullptr, n->X.p, n->Y.p, n->stream));
         TRY(mlp_fwd_bwd(n, batch));
         if (it == 0) { TRY(mean_loss(lp->loss_first)); }
         TRY(mlp_adam(n, (float)lr));
@@ -1018,6 +1024,13 @@ mci_status mci_lp_set_training_pairs(mci_lp *lp, double same_fraction)
     CHECKL(lp);
     if (!(same_fraction >= 0. && same_fraction <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "fraction must be in [0, 1]"); }
     lp->same_frac = same_fraction;
+    return MCI_OK;
+}
+
+mci_status mci_lp_set_fixed_input(mci_lp *lp, int on)
+{
+    CHECKL(lp);
+    lp->fixed_input = on != 0;
     return MCI_OK;
 }
 

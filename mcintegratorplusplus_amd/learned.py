@@ -28,7 +28,7 @@ _SIGS = {
     "mci_lp_train": [_P, _I, _I, _D, _DP, _DP], "mci_lp_get_results": [_P, _DP, _I],
     "mci_lp_get_timings": [_P, _DP, _I], "mci_lp_get_walker_means": [_P, _DP, _I64],
     "mci_lp_get_network": [_P, C.POINTER(_P)], "mci_lp_set_training_pairs": [_P, _D],
-    "mci_lp_eval_network": [_P, _I64, _FP, _FP],
+    "mci_lp_eval_network": [_P, _I64, _FP, _FP], "mci_lp_set_fixed_input": [_P, _I],
 }
 _bound = None
 
@@ -135,6 +135,10 @@ class LearnedProposalSampler:
 
     def setTrainingPairs(self, same_fraction):
         _check(_lib().mci_lp_set_training_pairs(self._h, same_fraction))
+
+    def setFixedInput(self, on=True):
+        """train with one fixed input: the network's mixture there is an independence proposal"""
+        _check(_lib().mci_lp_set_fixed_input(self._h, 1 if on else 0))
 
     def train(self, iters, batch=4096, lr=1e-3):
         l0, l1 = C.c_double(), C.c_double()

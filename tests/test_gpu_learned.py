@@ -138,8 +138,9 @@ def test_trained_network_reduces_the_error_bar():
     assert np.all(np.abs(_z(a_m, e_m, exact)) < 4.5)
     assert r["acc_learned"] > 0.05
     assert e_m[0] < 0.7 * e_p[0]
-    # independence proposal from the network trained on independent pairs
+    # independence proposal: the network trained with one fixed input
     s.setTrainingPairs(0.0)
+    s.setFixedInput(True)
     s.collect(8, 256)
     s.train(400, 2048, 2e-3)
     a_i, e_i = s.run(INDEP_FP32, 2048, 32, 0.5)
