@@ -12,6 +12,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <random>
+#include <thread>
 #include <vector>
 
 using mcih::DBuf;
@@ -1024,6 +1026,195 @@ mci_status mci_lp_set_training_pairs(mci_lp *lp, double same_fraction)
     CHECKL(lp);
     if (!(same_fraction >= 0. && same_fraction <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "fraction must be in [0, 1]"); }
     lp->same_frac = same_fraction;
+    return MCI_OK;
+}
+
+// Diagonal Gaussian mixture fitted by EM to x [n][d] (row-major): greedy k-means++ seeds from the
+// stream of `seed`, every component starting at the pooled variance with weight 1 / K, then
+// em_iters EM iterations (responsibilities by log-sum-exp; variances = E[x^2] - mean^2 + 1e-6).
+// Host code only; outputs mu [K][d], var [K][d], pi [K] and the mean log-likelihood.
+mci_status mci_gmm_fit_diag(const double *xin, int64_t n, int d, int K, int em_iters, uint64_t seed, double *mu_out, double *var_out,
+                            double *pi_out, double *loglik)
+{
+    if (!xin || !mu_out || !var_out || !pi_out || n < 1 || d < 1 || d > 4096 || K < 1 || K > 64 || n < K) {
+        return fail(MCI_E_INVALID_ARGUMENT, "need x, outputs, 1 <= d <= 4096, 1 <= K <= 64 and n >= K");
+    }
+    if (em_iters < 0 || em_iters > 1000) { return fail(MCI_E_INVALID_ARGUMENT, "EM iterations must be in [0, 1000]"); }
+    const int64_t ns = n;
+    const double *x = xin;
+    // the sums over samples run in NCH fixed chunks on threads of their own and are added in chunk
+    // order, so that the fit does not depend on the machine's thread count
+    constexpr int NCH = 16;
+    auto chunks = [&](auto &&fn) {
+        std::vector<std::thread> th;
+        for (int c = 0; c < NCH; ++c) { th.emplace_back([&, c] { fn(c, ns * c / NCH, ns * (c + 1) / NCH); }); }
+        for (auto &t : th) { t.join(); }
+    };
+    std::mt19937_64 g(seed ^ 0x5DEECE66DULL);
+    std::uniform_real_distribution<double> U(0., 1.);
+    std::vector<double> mu((size_t)K * d), var((size_t)K * d), pi((size_t)K, 1.0 / K), d2((size_t)ns, HUGE_VAL);
+    double tot = 0.;
+    std::vector<double> mean(d, 0.);
+    for (int64_t r = 0; r < ns; ++r) {
+        for (int j = 0; j < d; ++j) { mean[j] += x[(size_t)r * d + j] / (double)ns; }
+    }
+    for (int64_t r = 0; r < ns; ++r) {
+        for (int j = 0; j < d; ++j) { const double t = x[(size_t)r * d + j] - mean[j]; tot += t * t; }
+    }
+    const double v0 = std::max(tot / ((double)ns * d), 1e-6);
+    // greedy k-means++: each further seed is the best of LTRIALS candidates drawn with probability
+    // proportional to the squared distance to the nearest seed (the one that lowers the summed
+    // squared distance most), so that well-separated clusters are not left without a seed
+    constexpr int LTRIALS = 16;
+    auto dist2 = [&](int64_t r, int64_t c) {
+        double e = 0.;
+        for (int j = 0; j < d; ++j) { const double t = x[(size_t)r * d + j] - x[(size_t)c * d + j]; e += t * t; }
+        return e;
+    };
+    int64_t pick = (int64_t)(U(g) * (double)ns) % ns;
+    for (int k = 0; k < K; ++k) {
+        for (int j = 0; j < d; ++j) { mu[(size_t)k * d + j] = x[(size_t)pick * d + j]; var[(size_t)k * d + j] = v0; }
+        double sum = 0.;
+        for (int64_t r = 0; r < ns; ++r) {
+            d2[(size_t)r] = std::min(d2[(size_t)r], dist2(r, pick));
+            sum += d2[(size_t)r];
+        }
+        if (k == K - 1) { break; }
+        double best = HUGE_VAL;
+        for (int l = 0; l < LTRIALS; ++l) {
+            const double u = U(g) * sum;
+            double c = 0.;
+            int64_t cand = ns - 1;
+            for (int64_t r = 0; r < ns; ++r) {
+                c += d2[(size_t)r];
+                if (u < c) { cand = r; break; }
+            }
+            double part[NCH];
+            chunks([&](int c, int64_t r0, int64_t r1) {
+                double s = 0.;
+                for (int64_t r = r0; r < r1; ++r) { s += std::min(d2[(size_t)r], dist2(r, cand)); }
+                part[c] = s;
+            });
+            double pot = 0.;
+            for (int c = 0; c < NCH; ++c) { pot += part[c]; }
+            if (pot < best) { best = pot; pick = cand; }
+        }
+    }
+    // EM
+    const size_t KS = (size_t)K * (1 + 2 * d); // per chunk: nk [K], s1 [K][d], s2 [K][d]
+    std::vector<double> acc(KS * NCH), nk((size_t)K), s1((size_t)K * d), s2((size_t)K * d), lc((size_t)K);
+    for (int it = 0; it < em_iters; ++it) {
+        for (int k = 0; k < K; ++k) {
+            double ld = 0.;
+            for (int j = 0; j < d; ++j) { ld += std::log(2. * M_PI * var[(size_t)k * d + j]); }
+            lc[(size_t)k] = std::log(pi[(size_t)k]) - 0.5 * ld;
+        }
+        chunks([&](int c, int64_t r0, int64_t r1) {
+            double *a = &acc[KS * c], *ank = a, *as1 = a + K, *as2 = a + K + (size_t)K * d;
+            std::fill(a, a + KS, 0.);
+            std::vector<double> resp((size_t)K);
+            for (int64_t r = r0; r < r1; ++r) {
+                const double *xr = &x[(size_t)r * d];
+                double mx = -HUGE_VAL;
+                for (int k = 0; k < K; ++k) {
+                    double q = 0.;
+                    for (int j = 0; j < d; ++j) { const double t = xr[j] - mu[(size_t)k * d + j]; q += t * t / var[(size_t)k * d + j]; }
+                    resp[(size_t)k] = lc[(size_t)k] - 0.5 * q;
+                    mx = std::max(mx, resp[(size_t)k]);
+                }
+                double z = 0.;
+                for (int k = 0; k < K; ++k) { resp[(size_t)k] = std::exp(resp[(size_t)k] - mx); z += resp[(size_t)k]; }
+                for (int k = 0; k < K; ++k) {
+                    const double rk = resp[(size_t)k] / z;
+                    ank[k] += rk;
+                    for (int j = 0; j < d; ++j) { as1[(size_t)k * d + j] += rk * xr[j]; as2[(size_t)k * d + j] += rk * xr[j] * xr[j]; }
+                }
+            }
+        });
+        std::fill(nk.begin(), nk.end(), 0.);
+        std::fill(s1.begin(), s1.end(), 0.);
+        std::fill(s2.begin(), s2.end(), 0.);
+        for (int c = 0; c < NCH; ++c) {
+            const double *a = &acc[KS * c];
+            for (int k = 0; k < K; ++k) { nk[(size_t)k] += a[k]; }
+            for (size_t e = 0; e < (size_t)K * d; ++e) { s1[e] += a[K + e]; s2[e] += a[K + (size_t)K * d + e]; }
+        }
+        for (int k = 0; k < K; ++k) {
+            const double nkk = nk[(size_t)k] + 1e-9;
+            pi[(size_t)k] = std::max(nkk / (double)ns, 1e-12);
+            for (int j = 0; j < d; ++j) {
+                const double m1 = s1[(size_t)k * d + j] / nkk;
+                mu[(size_t)k * d + j] = m1;
+                var[(size_t)k * d + j] = std::max(s2[(size_t)k * d + j] / nkk - m1 * m1, 0.) + 1e-6;
+            }
+        }
+    }
+    double ll = 0.;
+    {
+        std::vector<double> lc((size_t)K), q((size_t)K);
+        for (int k = 0; k < K; ++k) {
+            double ld = 0.;
+            for (int j = 0; j < d; ++j) { ld += std::log(2. * M_PI * var[(size_t)k * d + j]); }
+            lc[(size_t)k] = std::log(pi[(size_t)k]) - 0.5 * ld;
+        }
+        for (int64_t r = 0; r < ns; ++r) {
+            double mx = -HUGE_VAL;
+            for (int k = 0; k < K; ++k) {
+                double e = 0.;
+                for (int j = 0; j < d; ++j) { const double t = x[(size_t)r * d + j] - mu[(size_t)k * d + j]; e += t * t / var[(size_t)k * d + j]; }
+                q[(size_t)k] = lc[(size_t)k] - 0.5 * e;
+                mx = std::max(mx, q[(size_t)k]);
+            }
+            double z = 0.;
+            for (int k = 0; k < K; ++k) { z += std::exp(q[(size_t)k] - mx); }
+            ll += mx + std::log(z);
+        }
+    }
+    std::copy(mu.begin(), mu.end(), mu_out);
+    std::copy(var.begin(), var.end(), var_out);
+    std::copy(pi.begin(), pi.end(), pi_out);
+    if (loglik) { *loglik = ll / (double)ns; }
+    return MCI_OK;
+}
+
+// The network's last layer started at the diagonal mixture that EM fits (mci_gmm_fit_diag, seeded
+// from the sampler's seed) to the first 16,384 walkers of the last snapshot: biases = (means / 4,
+// log-scales = log sqrt(variances) clamped inside [-3, 3], log-weights), weights scaled by 0.02,
+// so that before training the network's mixture is about that fit at every input.  Training
+// then refines the whole network.
+mci_status mci_lp_init_network_from_samples(mci_lp *lp, int em_iters)
+{
+    CHECKL(lp);
+    if (lp->nsnap < 1) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect first"); }
+    if (em_iters < 1 || em_iters > 1000) { return fail(MCI_E_INVALID_ARGUMENT, "EM iterations must be in [1, 1000]"); }
+    TRY(net_ready(lp));
+    mci_mlp *n = lp->net;
+    const int d = lp->d, K = n->K, H = n->H, O = n->O;
+    const int64_t W = lp->W, ns = std::min<int64_t>(W, 16384);
+    if (ns < K) { return fail(MCI_E_INVALID_ARGUMENT, "fewer walkers than mixture components"); }
+    std::vector<float> last((size_t)d * W);
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    HIPCHK(hipMemcpy(last.data(), lp->snap.p + (size_t)(lp->nsnap - 1) * d * W, last.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<double> x((size_t)ns * d);
+    for (int64_t r = 0; r < ns; ++r) {
+        for (int j = 0; j < d; ++j) { x[(size_t)r * d + j] = last[(size_t)j * W + r]; }
+    }
+    std::vector<double> mu((size_t)K * d), var((size_t)K * d), pi((size_t)K);
+    TRY(mci_gmm_fit_diag(x.data(), ns, d, K, em_iters, lp->seed, mu.data(), var.data(), pi.data(), nullptr));
+    // into the last layer
+    std::vector<float> p((size_t)n->np);
+    TRY(mci_mlp_get_params(n, p.data(), n->np));
+    const int64_t oW3 = (int64_t)d * H + H + (int64_t)H * H + H, ob3 = oW3 + (int64_t)H * O;
+    for (int64_t e = oW3; e < ob3; ++e) { p[(size_t)e] *= 0.02f; }
+    for (int k = 0; k < K; ++k) {
+        for (int j = 0; j < d; ++j) {
+            p[(size_t)(ob3 + 2 * d * k + j)] = (float)(mu[(size_t)k * d + j] / mcim::MU_SCALE);
+            const double ls = 0.5 * std::log(var[(size_t)k * d + j]);
+            p[(size_t)(ob3 + 2 * d * k + d + j)] = (float)std::fmin(std::fmax(ls, -(double)mcim::LS_MAX + 1e-3), (double)mcim::LS_MAX - 1e-3);
+        }
+        p[(size_t)(ob3 + 2 * d * K + k)] = (float)std::log(pi[(size_t)k]);
+    }
+    TRY(mci_mlp_set_params(n, p.data(), n->np));
     return MCI_OK;
 }
 

@@ -29,6 +29,8 @@ _SIGS = {
     "mci_lp_get_timings": [_P, _DP, _I], "mci_lp_get_walker_means": [_P, _DP, _I64],
     "mci_lp_get_network": [_P, C.POINTER(_P)], "mci_lp_set_training_pairs": [_P, _D],
     "mci_lp_eval_network": [_P, _I64, _FP, _FP], "mci_lp_set_fixed_input": [_P, _I],
+    "mci_lp_init_network_from_samples": [_P, _I],
+    "mci_gmm_fit_diag": [_DP, _I64, _I, _I, _I, C.c_uint64, _DP, _DP, _DP, _DP],
 }
 _bound = None
 
@@ -136,6 +138,10 @@ class LearnedProposalSampler:
     def setTrainingPairs(self, same_fraction):
         _check(_lib().mci_lp_set_training_pairs(self._h, same_fraction))
 
+    def initFromSamples(self, em_iters=40):
+        """start the network's last layer at an EM fit of its mixture to the last snapshot"""
+        _check(_lib().mci_lp_init_network_from_samples(self._h, em_iters))
+
     def setFixedInput(self, on=True):
         """train with one fixed input: the network's mixture there is an independence proposal"""
         _check(_lib().mci_lp_set_fixed_input(self._h, 1 if on else 0))
@@ -175,6 +181,18 @@ class LearnedProposalSampler:
         h = _P()
         _check(_lib().mci_lp_get_network(self._h, C.byref(h)))
         return Network(self.d, self.hidden, self.net_ncomp, _handle=h)
+
+
+def gmm_fit_diag(x, ncomp, em_iters=60, seed=1):
+    """diagonal Gaussian mixture fitted to x [n][d] by EM from greedy k-means++ seeds (host code,
+    the fit the learned proposal's network starts from): (means [K][d], variances [K][d],
+    weights [K], mean log-likelihood)"""
+    x = _d(x)
+    n, d = x.shape
+    mu, var, pi, ll = np.zeros((ncomp, d)), np.zeros((ncomp, d)), np.zeros(ncomp), C.c_double()
+    _check(_lib().mci_gmm_fit_diag(x.ctypes.data_as(_DP), n, d, ncomp, em_iters, seed, mu.ctypes.data_as(_DP),
+                                   var.ctypes.data_as(_DP), pi.ctypes.data_as(_DP), C.byref(ll)))
+    return mu, var, pi, ll.value
 
 
 def mixture_target(d=32, ncomp=8, seed=5, mean_sd=3.0, eig=(0.1, 1.0)):

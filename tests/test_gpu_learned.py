@@ -120,6 +120,37 @@ def test_ragged_walker_count_and_small_dimension():
     assert m.shape == (2, 1000) and np.all(np.isfinite(m))
 
 
+def test_em_start_fits_the_target_mixture():
+    """The EM start of the network's last layer on samples of a 3-component target whose
+    components lie about 8 apart: before any training, the network's mixture at any input has,
+    up to the order of its components, the target's weights (1/3), means and per-coordinate
+    log-scales (0.5 log of the covariance diagonal), to within the sampling error of 2,048
+    walkers and the 0.02-scaled input dependence of the last layer."""
+    from mcintegratorplusplus_amd.learned import LearnedProposalSampler
+    w, mu, cov, exact = _small_target()
+    s = LearnedProposalSampler(w, mu, cov, 2048)
+    s.setSeed(7)
+    s.setNetwork(64, 3)
+    s.collect(2, 256)
+    s.initFromSamples(60)
+    d, K = 8, 3
+    x = np.random.default_rng(0).normal(0.0, 2.0, (16, d))
+    out = s.evalNetwork(x)
+    for r in range(x.shape[0]):
+        o = out[r]
+        means = np.array([o[2 * d * k: 2 * d * k + d] for k in range(K)])
+        ls = np.array([o[2 * d * k + d: 2 * d * k + 2 * d] for k in range(K)])
+        lg = o[2 * d * K: 2 * d * K + K]
+        pi = np.exp(lg - lg.max())
+        pi /= pi.sum()
+        order = [int(np.argmin(np.sum((means - m) ** 2, axis=1))) for m in mu]
+        assert sorted(order) == list(range(K)), order
+        for k, j in enumerate(order):
+            assert np.max(np.abs(means[j] - mu[k])) < 0.3, (r, k, means[j], mu[k])
+            assert np.max(np.abs(ls[j] - 0.5 * np.log(np.diag(cov[k])))) < 0.25, (r, k)
+            assert abs(pi[j] - w[k]) < 0.06, (r, k, pi)
+
+
 def test_trained_network_reduces_the_error_bar():
     from mcintegratorplusplus_amd.learned import INDEP_FP32, MIXED_FP32, PLAIN_FP32, LearnedProposalSampler
     w, mu, cov, exact = _small_target()
