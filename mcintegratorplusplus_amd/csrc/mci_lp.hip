@@ -108,6 +108,27 @@ __device__ __forceinline__ float next_normal(Normals &N, uint32_t c0base, uint32
     }
 }
 
+// four standard normals from one Philox block by the Box-Muller transform (no rejection, so no
+// divergence): u1 in (0, 1], u2 in [0, 1) from the top 24 bits of a word; r = sqrt(-2 ln u1),
+// (r cos 2 pi u2, r sin 2 pi u2), with the hardware's approximate log, square root, sine and cosine
+// (errors of a few units in the last place: the proposal they define differs from the normal
+// density by that relative amount).  The learned moves draw their normals with it.
+__device__ __forceinline__ void normal4(uint32_t c0, uint32_t id, uint32_t tl, uint32_t th, uint32_t k0, uint32_t k1, float z[4])
+{
+    uint32_t o[4];
+    philox(c0, id, tl, th, k0, k1, o);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const float u1 = (float)((o[2 * p] >> 8) + 1u) * 0x1p-24f;
+        const float u2 = (float)(o[2 * p + 1] >> 8) * 0x1p-24f;
+        const float r = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1));
+        float s, c;
+        __sincosf(6.28318530717958648f * u2, &s, &c);
+        z[2 * p] = r * c;
+        z[2 * p + 1] = r * s;
+    }
+}
+
 // log-density state (q_k, m, S) of a position held in registers: q_k = |U_k x - U_k mu_k|^2 with
 // U_k the upper Cholesky factor of P_k (d (d + 1)/2 fused multiply-adds per component); U and
 // U mu are read at the same address by every lane
@@ -408,146 +429,6 @@ __global__ __launch_bounds__(WG) void k_lp_rw(Target<R> T, State<R> St, R step, 
     if (live) { lane_store<R, WG>(St, d, w, xs, tid, L); }
 }
 
-// One launch for a whole run of the chain that mixes random-walk moves with the independence
-// proposal q(x') = sum_k pi_k N(x'; mu_k, diag sigma_k^2) given by the network at a fixed input.
-// Step t is a learned move for every walker when the step's common word says so (probability
-// pmix), so the branch is uniform.  A learned move: component k ~ pi, x' = mu_k + sigma_k z,
-// accept with min(1, p(x') q(x) / (p(x) q(x'))); q and the target's U_k live in LDS.
-constexpr int WGM = 512;
-
-__device__ __forceinline__ float indep_logq(const float *Qs, int Kq, int d, const float xr[DMAX])
-{
-    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
-    float comp[KMAX];
-    float cmax = -__builtin_huge_valf();
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k >= Kq) { comp[k] = -__builtin_huge_valf(); continue; }
-        const float *row = Qs + k * (1 + 3 * d); // [logc][mu d][sigma d][1/sigma d]
-        float acc = 0.f;
-#pragma unroll
-        for (int j = 0; j < DMAX; ++j) {
-            if (j < d) {
-                const float z = (xr[j] - row[1 + j]) * row[1 + 2 * d + j];
-                acc = fmaf(z, z, acc);
-            }
-        }
-        comp[k] = row[0] - 0.5f * acc - (float)d * HALF_LOG_2PI;
-        cmax = fmaxf(cmax, comp[k]);
-    }
-    float sc = 0.f;
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) { sc += (k < Kq) ? __expf(comp[k] - cmax) : 0.f; }
-    return cmax + __logf(sc);
-}
-
-// the same for a position in LDS (stride ld)
-__device__ __forceinline__ float indep_logq_lds(const float *Qs, int Kq, int d, const float *x, int ld)
-{
-    constexpr float HALF_LOG_2PI = 0.91893853320467274f;
-    float comp[KMAX];
-    float cmax = -__builtin_huge_valf();
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k >= Kq) { comp[k] = -__builtin_huge_valf(); continue; }
-        const float *row = Qs + k * (1 + 3 * d);
-        float acc = 0.f;
-        for (int j = 0; j < d; ++j) {
-            const float z = (x[j * ld] - row[1 + j]) * row[1 + 2 * d + j];
-            acc = fmaf(z, z, acc);
-        }
-        comp[k] = row[0] - 0.5f * acc - (float)d * HALF_LOG_2PI;
-        cmax = fmaxf(cmax, comp[k]);
-    }
-    float sc = 0.f;
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) { sc += (k < Kq) ? __expf(comp[k] - cmax) : 0.f; }
-    return cmax + __logf(sc);
-}
-
-__global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<float> St, const float *Q, int Kq, float step, uint64_t t0,
-                                                      int64_t nsteps, float pmix, uint32_t k0, uint32_t k1, int32_t *nacc_ind)
-{
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    const int d = T.d, tid = threadIdx.x;
-    float *Ps = reinterpret_cast<float *>(lds_raw); // [KMAX][d][d + 1]
-    float *vs = Ps + KMAX * d * (d + 1);             // [KMAX][d]
-    float *Us = vs + KMAX * d;                       // [KMAX][d][d]
-    float *Qs = Us + KMAX * d * d;                   // [Kq][1 + 3 d], then cumulative weights [Kq]
-    float *xs = Qs + Kq * (1 + 3 * d) + Kq;          // [d][WGM]
-    const int64_t w = (int64_t)blockIdx.x * WGM + tid;
-    const bool live = w < St.W;
-    const int64_t wl = live ? w : St.W - 1;
-    load_target_lds<float, WGM>(T, Ps, vs);
-    for (int e = tid; e < KMAX * d * d; e += WGM) { Us[e] = T.U[e]; }
-    for (int e = tid; e < Kq * (1 + 3 * d) + Kq; e += WGM) { Qs[e] = Q[e]; }
-    for (int i = 0; i < d; ++i) { xs[i * WGM + tid] = St.x[(int64_t)i * St.W + wl]; }
-    __syncthreads();
-    const float *cwq = Qs + Kq * (1 + 3 * d);
-    Target<float> TL = T;
-    TL.U = Us;
-    Lane<float> L;
-    lane_load(T, St, wl, L);
-    for (int64_t s = 0; s < nsteps; ++s) {
-        const uint64_t t = t0 + (uint64_t)s;
-        const uint32_t tl = (uint32_t)t, th = (uint32_t)(t >> 32);
-        uint32_t oc[4];
-        philox(0x20000000u, 0xffffffffu, tl, th, k0, k1, oc);
-        const bool learned = (float)(oc[0] >> 8) * 0x1p-24f < pmix;
-        if (!learned) {
-            (void)rw_step<float, WGM>(T, Ps, vs, xs, tid, (uint32_t)wl, L, step, t, k0, k1);
-        }
-        else {
-            uint32_t o[4];
-            philox(0x10000000u, (uint32_t)wl, tl, th, k0, k1, o);
-            const float uc = (float)(o[0] >> 8) * 0x1p-24f;
-            int kc = 0;
-            for (int k = 0; k < Kq - 1; ++k) { kc += uc >= cwq[k] ? 1 : 0; }
-            const float *row = Qs + kc * (1 + 3 * d);
-            const float lq_old = indep_logq_lds(Qs, Kq, d, xs + tid, WGM);
-            Normals N;
-            N.avail = 0;
-            N.has_spare = false;
-            N.call = 1;
-            float xn[DMAX];
-#pragma unroll
-            for (int j = 0; j < DMAX; ++j) {
-                xn[j] = 0.f;
-                if (j < d) {
-                    const float z = next_normal(N, 0x10000000u, (uint32_t)wl, tl, th, k0, k1);
-                    xn[j] = fmaf(row[1 + d + j], z, row[1 + j]);
-                }
-            }
-            const float lq_new = indep_logq(Qs, Kq, d, xn);
-            float qn[KMAX], mn, Sn;
-            target_eval_reg(TL, Us, xn, qn, mn, Sn);
-            const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
-            const bool acc = __logf(ua) < (mn + __logf(Sn)) - (L.m + __logf(L.S)) + (lq_old - lq_new);
-            if (acc) {
-                double sq = 0.;
-#pragma unroll
-                for (int j = 0; j < DMAX; ++j) {
-                    if (j < d) {
-                        xs[j * WGM + tid] = xn[j];
-                        sq = fma((double)xn[j], (double)xn[j], sq);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k) { L.q[k] = qn[k]; }
-                L.m = mn;
-                L.S = Sn;
-                L.sq = sq;
-            }
-            L.nacc_ind += acc ? 1 : 0;
-        }
-        after_step<float, WGM>(T, St, Ps, xs, tid, w, live, L);
-    }
-    if (live) {
-        lane_store<float, WGM>(St, d, w, xs, tid, L);
-        nacc_ind[w] += L.nacc_ind;
-    }
-}
-
 // ---- the learned move, fused: one launch per run of consecutive learned steps ----
 // A workgroup takes TW walkers (32: four waves; 64: eight waves) for the whole run.  The network is
 // evaluated at the walkers' positions when the run starts (random-walk moves in between need no
@@ -564,6 +445,10 @@ __global__ __launch_bounds__(WGM) void k_lp_mix_indep(Target<float> T, State<flo
 //  2. the network at x', log q(x | x') folded the same way;
 //  3. the MH test; accepted walkers take x', its target state, logits and mixture.
 // Mode 0 evaluates the network only and writes the mixture (means, log-scales, logits) for tests.
+// Mode 2 is the independence proposal: the network is evaluated once, at the fixed input xin [D]
+// for every walker, and its mixture q is the proposal of every step; a step is then step 1 (x' ~
+// q, log q(x') folded, log p(x')) and the MH test with the log q(x) carried from the step that
+// last moved the walker.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __fp16 h2_t __attribute__((ext_vector_type(2)));
@@ -776,19 +661,31 @@ __device__ __forceinline__ void fold(const h2_t mx[D], const float lpi[2], const
 
 // q_k = |U_k y - U_k mu_k|^2 with v_mfma_f32_32x32x2_f32 (FP32 products, FP32 sums) for two
 // (component, walker block of 32) pairs: rows of U_k (zero-padded to 32) times y [D][TW] (LDS),
-// the accumulators starting at -U_k mu_k; lanes 0 .. 31 return q of walker 32 cb + lane
+// the accumulators starting at -U_k mu_k; lanes 0 .. 31 return q of walker 32 cb + lane.  U comes
+// transposed, Ut [k][j][32 rows], so that a reduction step's operand is two 128-byte lines; all
+// of a pass's operands are loaded (global address space: the loads need not wait on LDS traffic)
+// before the products.  SAMEK: k[0] == k[1] (one set of operands).
 typedef float f32x16v __attribute__((ext_vector_type(16)));
-template <int D>
-__device__ __forceinline__ void target_q(const float *U, const float *Um, const int k[2], const int cb[2], const float *y, int lane, float out[2])
+typedef const float __attribute__((address_space(1))) *gfloat_p;
+template <int D, bool SAMEK>
+__device__ __forceinline__ void target_q(const float *Ut, const float *Um, const int k[2], const int cb[2], const float *y, int lane, float out[2])
 {
     const int row = lane & 31, kk = lane >> 5;
+    const gfloat_p Ug = (gfloat_p)Ut, Umg = (gfloat_p)Um;
+    constexpr int NA = SAMEK ? 1 : 2;
+    float a[NA][D / 2];
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+#pragma unroll
+        for (int q = 0; q < D / 2; ++q) { a[c][q] = Ug[((int64_t)k[c] * D + 2 * q + kk) * 32 + row]; }
+    }
     f32x16v acc[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int r = 8 * (i >> 2) + 4 * kk + (i & 3);
-            acc[c][i] = r < D ? -Um[k[c] * D + r] : 0.f;
+            acc[c][i] = r < D ? -Umg[k[c] * D + r] : 0.f;
         }
     }
 #pragma unroll
@@ -796,8 +693,7 @@ __device__ __forceinline__ void target_q(const float *U, const float *Um, const 
         const int j = 2 * q + kk;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const float a = row < D ? U[((int64_t)k[c] * D + row) * D + j] : 0.f;
-            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, y[j * TW + 32 * cb[c] + row], acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[SAMEK ? 0 : c][q], y[j * TW + 32 * cb[c] + row], acc[c], 0, 0, 0);
         }
     }
 #pragma unroll
@@ -844,7 +740,7 @@ size_t learned_lds()
            sizeof(uint16_t) * ((size_t)TW * C::LDX + (size_t)TW * C::LDH + (size_t)H * C::LDX + (size_t)H * C::LDH);
 }
 
-template <int MODE, int D, int H> // 0: the network at xin, written to out; 1: nl learned steps from step t0
+template <int MODE, int D, int H> // 0: the network at xin, written to out; 1: nl learned steps from step t0; 2: nl independence steps
 __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, State<float> St, const float *xin, float *out, uint64_t t0, int nl,
                                                    uint32_t k0, uint32_t k1, int32_t *nacc_ind)
 {
@@ -892,17 +788,25 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     }
     for (int i = tid; i < C::LOFF + 32; i += TT) { bs[2 * H + i] = N.b3[i]; }
     for (int j = part; j < D; j += NPART) { xs[j * TW + col] = xsrc[(int64_t)j * W + wl]; }
-    for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xsrc[(int64_t)j * W + wl] * mcim::IN_SCALE : 0.f); }
+    for (int j = part; j < C::DP; j += NPART) {
+        const float v = j >= D ? 0.f : MODE == 2 ? xin[j] : xsrc[(int64_t)j * W + wl];
+        Xb[col * C::LDX + j] = f2bf(v * mcim::IN_SCALE);
+    }
     __syncthreads();
     h2_t mx[D];
     float lpi[2] = {0.f, 0.f};
     network<D, H>(N, W1s, W2s, bs, Xb, Hb, mx, lpi, lgx, MODE == 0 ? out : nullptr, W, w0);
     if (MODE == 0) { return; }
+    // the independence proposal's log q at the walkers' positions (cf2 is free in mode 2)
+    if (MODE == 2) { fold<D>(mx, lpi, xs, cf2, K); }
+    // the logits of walkers 32 .. 63 come from wave 4
+    __syncthreads();
     // the chain state of walker col, held by wave 0
-    float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+    float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f, lqx = 0.f;
     double sq = 0.;
     int jb = 0, bi = 0, nacc = 0;
     if (part == 0) {
+        if (MODE == 2) { lqx = mixture_logq(lgx, cf2, K, D, col); }
         m = St.m[wl];
         S = St.S[wl];
         sq = St.sq[wl];
@@ -919,12 +823,16 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     // g, g + 7, ...); they are drawn for the next step while wave 0 runs the MH test
     auto draw_normals = [&](uint64_t t) {
         if (part > 0) {
-            Normals Nm;
-            Nm.avail = 0;
-            Nm.has_spare = false;
-            Nm.call = 1;
+            constexpr int NZ = (D + NPART - 2) / (NPART - 1); // normals per generator at most
+            float zq[4 * ((NZ + 3) / 4)];
             const uint32_t base = 0x10000000u + ((uint32_t)part << 20);
-            for (int j = part - 1; j < D; j += NPART - 1) { zs[j * TW + col] = next_normal(Nm, base, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1); }
+#pragma unroll
+            for (int c = 0; c < (NZ + 3) / 4; ++c) { normal4(base + 1u + (uint32_t)c, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, zq + 4 * c); }
+#pragma unroll
+            for (int i = 0; i < NZ; ++i) {
+                const int j = part - 1 + (NPART - 1) * i;
+                if (j < D) { zs[j * TW + col] = zq[i]; }
+            }
         }
     };
     // the component of step t from the logits at x (wave 0; o holds the step's words: [0] the
@@ -958,13 +866,24 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         // addresses opaque, so the compiler does not hoist those loads out of the loop (they would
         // not fit the registers)
         NetDev Ns = N;
-        const float *Ubase = T.U, *Umbase = T.Um;
+        const float *Ubase = T.Ut, *Umbase = T.Um;
         asm volatile("" : "+s"(Ns.W3t), "+s"(Ubase), "+s"(Umbase));
         const uint64_t t = t0 + (uint64_t)st;
         // x' = mu + sigma z from the lanes holding the chosen component
         if (2 * wc < K) {
             const int cc = 32 * cbw + (lane & 31);
             const int kc = kst[cc] - 2 * wc;
+            // the lane's normals first (one wait), then x'
+            float zv[D];
+#pragma unroll
+            for (int rb = 0; rb < D / 8; ++rb) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int j = ((32 * rb + 8 * g + 4 * (lane >> 5)) % (2 * D)) >> 1;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) { zv[pair_index(rb, g, h)] = zs[(j + h) * TW + cc]; }
+                }
+            }
 #pragma unroll
             for (int rb = 0; rb < D / 8; ++rb) {
 #pragma unroll
@@ -975,7 +894,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             const h2_t p = mx[pair_index(rb, g, h)];
-                            xps[(j + h) * TW + cc] = fmaf(zs[(j + h) * TW + cc], __builtin_amdgcn_rcpf((float)p[1]), (float)p[0]);
+                            xps[(j + h) * TW + cc] = fmaf(zv[pair_index(rb, g, h)], __builtin_amdgcn_rcpf((float)p[1]), (float)p[0]);
                         }
                     }
                 }
@@ -983,7 +902,9 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         }
         __syncthreads();
         fold<D>(mx, lpi, xps, cf, K);
-        for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
+        if (MODE == 1) {
+            for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
+        }
         {
             // target component wv at x' on the matrix cores (FP32 products and sums):
             // U_k x' - U_k mu_k for both walker halves, then |.|^2 over the 32 rows
@@ -992,7 +913,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             const int kq[2] = {NCB == 2 ? wv : 2 * wv, NCB == 2 ? wv : 2 * wv + 1};
             const int cq[2] = {0, NCB == 2 ? 1 : 0};
             float qv[2];
-            target_q<D>(Ubase, Umbase, kq, cq, xps, lane, qv);
+            target_q<D, NCB == 2>(Ubase, Umbase, kq, cq, xps, lane, qv);
             if (lane < 32) {
                 qn[kq[0] * TW + 32 * cq[0] + lane] = qv[0];
                 qn[kq[1] * TW + 32 * cq[1] + lane] = qv[1];
@@ -1002,14 +923,16 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         // the network at x' and log q(x | x')
         h2_t mx2[D];
         float lpi2[2] = {0.f, 0.f};
-        network<D, H>(Ns, W1s, W2s, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
-        fold<D>(mx2, lpi2, xs, cf2, K);
-        __syncthreads();
+        if (MODE == 1) {
+            network<D, H>(Ns, W1s, W2s, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
+            fold<D>(mx2, lpi2, xs, cf2, K);
+            __syncthreads();
+        }
         // wave 0: the MH test, the accepted walkers' new state, then the next step's component;
         // waves 1 .. 7 meanwhile draw the next step's normals
         if (part == 0) {
             const float logq_f = mixture_logq(lgx, cf, K, D, col);
-            const float logq_r = mixture_logq(lg, cf2, K, D, col);
+            const float logq_r = MODE == 2 ? lqx : mixture_logq(lg, cf2, K, D, col);
             float mn = -__builtin_huge_valf();
             float e[KMAX];
 #pragma unroll
@@ -1026,14 +949,23 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             const float x0 = acc ? xps[col] : xs[col];
             if (acc) {
                 double s2 = 0.;
-                for (int j = 0; j < D; ++j) {
-                    const float v = xps[j * TW + col];
-                    xs[j * TW + col] = v;
-                    s2 = fma((double)v, (double)v, s2);
+#pragma unroll
+                for (int j0 = 0; j0 < D; j0 += 8) {
+                    float v[8]; // eight reads in flight, then the writes
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) { v[j] = xps[(j0 + j) * TW + col]; }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        xs[(j0 + j) * TW + col] = v[j];
+                        s2 = fma((double)v[j], (double)v[j], s2);
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k) { qx[k * TW + col] = qn[k * TW + col]; }
-                for (int k = 0; k < K; ++k) { lgx[k * TW + col] = lg[k * TW + col]; }
+                if (MODE == 1) {
+                    for (int k = 0; k < K; ++k) { lgx[k * TW + col] = lg[k * TW + col]; }
+                }
+                lqx = logq_f;
                 m = mn;
                 S = Sn;
                 sq = s2;
@@ -1045,7 +977,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         else if (st + 1 < nl) { draw_normals(t + 1); }
         __syncthreads();
         // accepted walkers: the mixture at x takes the proposal's
-        {
+        if (MODE == 1) {
             const bool a = af[32 * cbw + (lane & 31)] != 0;
 #pragma unroll
             for (int i = 0; i < D; ++i) { mx[i] = a ? mx2[i] : mx[i]; }
@@ -1125,23 +1057,6 @@ hipError_t launch_lp_rw(const Target<R> &T, const State<R> &S, R step, uint64_t 
     return hipGetLastError();
 }
 
-size_t mix_lds_bytes(int d, int Kq)
-{
-    return sizeof(float) * ((size_t)KMAX * d * (d + 1) + (size_t)KMAX * d + (size_t)KMAX * d * d + (size_t)Kq * (1 + 3 * d) + Kq +
-                            (size_t)d * WGM);
-}
-
-hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, const float *Q, int Kq, float step, uint64_t t0, int64_t nsteps,
-                               float pmix, uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
-{
-    const size_t lds = mix_lds_bytes(T.d, Kq);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_mix_indep), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) { return e; }
-    hipLaunchKernelGGL(k_lp_mix_indep, dim3((unsigned)((S.W + WGM - 1) / WGM)), dim3(WGM), lds, s, T, S, Q, Kq, step, t0, nsteps, pmix, k0, k1,
-                       nacc_ind);
-    return hipGetLastError();
-}
-
 namespace
 {
 template <int MODE, int D, int H>
@@ -1175,10 +1090,11 @@ hipError_t launch_learned_m(const NetDev &N, const Target<float> &T, const State
 
 bool learned_supported(int d, int H, int K) { return (d == 8 || d == 16 || d == 32) && (H == 64 || H == 128) && K >= 1 && K <= KMAX; }
 
-hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
-                             int32_t *nacc_ind, hipStream_t s)
+hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, const float *fixed_input, uint64_t t0, int nl,
+                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
 {
     if (!learned_supported(N.d, N.H, N.K) || T.d != N.d || nl < 1) { return hipErrorInvalidValue; }
+    if (fixed_input != nullptr) { return launch_learned_m<2>(N, T, S, fixed_input, nullptr, t0, nl, k0, k1, nacc_ind, s); }
     return launch_learned_m<1>(N, T, S, nullptr, nullptr, t0, nl, k0, k1, nacc_ind, s);
 }
 

@@ -28,6 +28,7 @@ struct Target
     const float *muf;   // [KMAX][d]
     const float *cw;    // [KMAX] cumulative weights
     int K;              // real components
+    const float *Ut;    // [KMAX][d][32] U_k transposed, rows zero-padded to 32 (FP32; matrix-core operand)
 };
 
 // walker state (HBM, walker index fastest)
@@ -64,18 +65,14 @@ struct NetDev
 };
 // the fused kernel exists for d in {8, 16, 32}, H in {64, 128}, 1 <= K <= KMAX
 bool learned_supported(int d, int H, int K);
-// nl consecutive learned moves, steps t0 .. t0 + nl - 1, for every walker (FP32 chain)
-hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
-                             int32_t *nacc_ind, hipStream_t s);
+// nl consecutive learned moves, steps t0 .. t0 + nl - 1, for every walker (FP32 chain): with
+// fixed_input null the conditional proposal q(x' | x) (the network at x), else the independence
+// proposal q(x') (the network at fixed_input, device [d])
+hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, const float *fixed_input, uint64_t t0, int nl,
+                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s);
 // the same kernel's network at positions xin [d][S.W], the mixture to out [K (1 + 2 d)][S.W]
 // (per component d means and d log-scales, transformed; then K logits)
 hipError_t launch_lp_network(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, hipStream_t s);
-// a whole run of the chain mixing random-walk moves and the independence proposal Q
-// ([Kq][1 + 3 d] rows: log pi_k - sum log sigma_k, mu_k, sigma_k, 1/sigma_k; then Kq cumulative
-// weights), FP32
-size_t mix_lds_bytes(int d, int Kq);
-hipError_t launch_lp_mix_indep(const Target<float> &T, const State<float> &S, const float *Q, int Kq, float step, uint64_t t0, int64_t nsteps,
-                               float pmix, uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s);
 // x [d][W] -> dst [d][W] FP32 (snapshots)
 template <class R>
 hipError_t launch_lp_snapshot(const State<R> &S, int d, float *dst, hipStream_t s);

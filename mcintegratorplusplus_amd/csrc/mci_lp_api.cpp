@@ -270,7 +270,7 @@ struct mci_lp
     hipStream_t stream = nullptr;
     // target on the device
     DBuf<double> mu64, P64, U64, Um64, v64, c64;
-    DBuf<float> mu32, P32, U32, Um32, v32, c32, Lf, muf, cwf;
+    DBuf<float> mu32, P32, U32, Um32, v32, c32, Lf, muf, cwf, Ut32;
     // chain state (one precision at a time)
     DBuf<double> x64, q64, m64, S64, sb64, cb64;
     DBuf<float> x32, q32, m32, S32, sb32, cb32;
@@ -287,7 +287,6 @@ struct mci_lp
     int nsnap = 0;
     double same_frac = 0.5;
     std::vector<float> ctx0;   // mean of the last snapshot: the network input of the independence proposal
-    DBuf<float> qpar;          // independence proposal [Kq][1 + 3 d] + cumulative weights [Kq]
     uint64_t t_next = 0;
     double step64 = 0., step32 = 0.;
     double results[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -335,6 +334,16 @@ mci_status lp_device(mci_lp *lp)
     TRY(up(lp->P64, lp->P));
     TRY(up(lp->U64, lp->U));
     TRY(up(lp->U32, to_f(lp->U)));
+    {
+        const int d = lp->d;
+        std::vector<float> ut((size_t)mcil::KMAX * d * 32, 0.f);
+        for (int k = 0; k < mcil::KMAX; ++k) {
+            for (int j = 0; j < d; ++j) {
+                for (int r = 0; r < d; ++r) { ut[((size_t)k * d + j) * 32 + r] = (float)lp->U[((size_t)k * d + r) * d + j]; }
+            }
+        }
+        TRY(up(lp->Ut32, ut));
+    }
     TRY(up(lp->Um64, lp->Um));
     TRY(up(lp->Um32, to_f(lp->Um)));
     TRY(up(lp->v64, lp->v));
@@ -362,12 +371,12 @@ mcil::Target<R> target_of(mci_lp *lp);
 template <>
 mcil::Target<double> target_of<double>(mci_lp *lp)
 {
-    return {lp->d, lp->mu64.p, lp->P64.p, lp->U64.p, lp->Um64.p, lp->v64.p, lp->c64.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
+    return {lp->d, lp->mu64.p, lp->P64.p, lp->U64.p, lp->Um64.p, lp->v64.p, lp->c64.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K, lp->Ut32.p};
 }
 template <>
 mcil::Target<float> target_of<float>(mci_lp *lp)
 {
-    return {lp->d, lp->mu32.p, lp->P32.p, lp->U32.p, lp->Um32.p, lp->v32.p, lp->c32.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K};
+    return {lp->d, lp->mu32.p, lp->P32.p, lp->U32.p, lp->Um32.p, lp->v32.p, lp->c32.p, lp->Lf.p, lp->muf.p, lp->cwf.p, lp->K, lp->Ut32.p};
 }
 
 template <class R>
@@ -559,12 +568,21 @@ mci_status net_dev(mci_lp *lp, mcil::NetDev &N)
 
 // the chain mixing random-walk moves with learned moves: cycles of LP_CYCLE steps, the first
 // LP_CYCLE - nl random-walk moves (one launch), then nl = round(pmix LP_CYCLE) learned moves (one
-// fused launch).  Each move leaves p invariant, so does the cycle.
+// fused launch).  Each move leaves p invariant, so does the cycle.  The learned moves use the
+// conditional proposal q(x' | x), or with indep the independence proposal q(x'): the network's
+// mixture at ctx0, the mean of the last snapshot.
 constexpr int LP_CYCLE = 32;
-mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
+mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix, bool indep)
 {
     mcil::NetDev N;
     TRY(net_dev(lp, N));
+    const float *fixed = nullptr;
+    if (indep) {
+        if (lp->ctx0.empty()) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect and mci_lp_train first"); }
+        HIPCHK(lp->dctx0.ensure((size_t)lp->d));
+        HIPCHK(hipMemcpy(lp->dctx0.p, lp->ctx0.data(), sizeof(float) * (size_t)lp->d, hipMemcpyHostToDevice));
+        fixed = lp->dctx0.p;
+    }
     mcil::State<float> S;
     const int B = (int)(nsteps / nblocks);
     TRY(state_of<float>(lp, B, nblocks, S));
@@ -592,7 +610,7 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
         }
         const int64_t l = std::min<int64_t>(nl, nsteps - s);
         if (l > 0) {
-            HIPCHK(mcil::launch_lp_learned(N, T, S, tb + (uint64_t)s, (int)l, k0, k1, lp->nacc_ind.p, lp->stream));
+            HIPCHK(mcil::launch_lp_learned(N, T, S, fixed, tb + (uint64_t)s, (int)l, k0, k1, lp->nacc_ind.p, lp->stream));
             nind += l;
             s += l;
         }
@@ -617,101 +635,6 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
     return MCI_OK;
 }
 
-// the independence proposal: the network's mixture at the fixed input ctx0, as
-// [Kq][log pi_k - sum_j log sigma_kj, mu_k, sigma_k, 1/sigma_k] and cumulative weights
-mci_status indep_params(mci_lp *lp)
-{
-    if (lp->ctx0.empty()) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect and mci_lp_train first"); }
-    TRY(net_ready(lp));
-    mci_mlp *n = lp->net;
-    const int d = lp->d, K = n->K, ns = 64;
-    TRY(mlp_reserve(n, ns));
-    std::vector<float> x((size_t)d * ns);
-    for (int j = 0; j < d; ++j) {
-        for (int r = 0; r < ns; ++r) { x[(size_t)j * ns + r] = lp->ctx0[j]; }
-    }
-    HIPCHK(hipMemcpy(n->X.p, x.data(), x.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(mcim::launch_to_bf16_scaled(n->X.p, n->Xb.p, (int64_t)d * ns, mcim::IN_SCALE, n->stream));
-    TRY(mlp_forward(n, ns, n->Xb.p, n->OUT.p));
-    std::vector<float> o((size_t)n->Op * ns);
-    HIPCHK(hipMemcpyAsync(o.data(), n->OUT.p, o.size() * sizeof(float), hipMemcpyDeviceToHost, n->stream));
-    HIPCHK(hipStreamSynchronize(n->stream));
-    auto col = [&](int c) { return (double)o[(size_t)c * ns]; };
-    const int L0 = 2 * d * K;
-    double amax = -HUGE_VAL;
-    for (int k = 0; k < K; ++k) { amax = std::fmax(amax, col(L0 + k)); }
-    double sa = 0.;
-    for (int k = 0; k < K; ++k) { sa += std::exp(col(L0 + k) - amax); }
-    const double lsa = amax + std::log(sa);
-    std::vector<float> q((size_t)K * (1 + 3 * d) + K);
-    double cum = 0.;
-    for (int k = 0; k < K; ++k) {
-        float *row = q.data() + (size_t)k * (1 + 3 * d);
-        double lsum = 0.;
-        for (int j = 0; j < d; ++j) {
-            const double mu = mcim::MU_SCALE * col(2 * d * k + j);
-            const double ls = std::fmin(std::fmax(col(2 * d * k + d + j), -(double)mcim::LS_MAX), (double)mcim::LS_MAX);
-            row[1 + j] = (float)mu;
-            row[1 + d + j] = (float)std::exp(ls);
-            row[1 + 2 * d + j] = (float)std::exp(-ls);
-            lsum += ls;
-        }
-        const double lp_k = col(L0 + k) - lsa;
-        row[0] = (float)(lp_k - lsum);
-        cum += std::exp(lp_k);
-        q[(size_t)K * (1 + 3 * d) + k] = (k == K - 1) ? 1.f : (float)cum;
-    }
-    TRY(up(lp->qpar, q));
-    return MCI_OK;
-}
-
-mci_status run_mixed_indep(mci_lp *lp, int64_t nsteps, int nblocks, double pmix)
-{
-    mcil::State<float> S;
-    const int B = (int)(nsteps / nblocks);
-    TRY(state_of<float>(lp, B, nblocks, S));
-    TRY(indep_params(lp));
-    const mcil::Target<float> T = target_of<float>(lp);
-    const uint32_t k0 = (uint32_t)lp->seed, k1 = (uint32_t)(lp->seed >> 32);
-    const size_t W = (size_t)lp->W;
-    auto t0 = clk::now();
-    HIPCHK(mcil::launch_lp_init<float>(T, S, k0, k1, lp->stream));
-    if (lp->step32 <= 0.) { lp->step32 = 1.0; }
-    TRY(calibrate<float>(lp, S, lp->step32));
-    lp->timings[0] = ms_since(t0);
-    HIPCHK(hipMemsetAsync(lp->nacc.p, 0, sizeof(int32_t) * W, lp->stream));
-    HIPCHK(hipMemsetAsync(lp->nacc_ind.p, 0, sizeof(int32_t) * W, lp->stream));
-    HIPCHK(hipStreamSynchronize(lp->stream));
-    t0 = clk::now();
-    HIPCHK(mcil::launch_lp_mix_indep(T, S, lp->qpar.p, lp->net->K, (float)lp->step32, lp->t_next, nsteps, (float)pmix, k0, k1, lp->nacc_ind.p,
-                                     lp->stream));
-    HIPCHK(hipStreamSynchronize(lp->stream));
-    lp->timings[1] = ms_since(t0);
-    // learned steps of this run (the common word of each step)
-    int64_t nind = 0;
-    for (int64_t s = 0; s < nsteps; ++s) {
-        uint32_t o[4];
-        const uint64_t t = lp->t_next + (uint64_t)s;
-        philox_host(0x20000000u, 0xffffffffu, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
-        nind += ((float)(o[0] >> 8) * 0x1p-24f < (float)pmix) ? 1 : 0;
-    }
-    lp->t_next += (uint64_t)nsteps;
-    t0 = clk::now();
-    TRY(estimate(lp, nblocks));
-    lp->timings[2] = ms_since(t0);
-    std::vector<int32_t> acc(W), acci(W);
-    HIPCHK(hipMemcpy(acc.data(), lp->nacc.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(acci.data(), lp->nacc_ind.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
-    double a = 0., ai = 0.;
-    for (size_t w = 0; w < W; ++w) {
-        a += acc[w];
-        ai += acci[w];
-    }
-    const int64_t nrw = nsteps - nind;
-    lp->results[6] = nrw > 0 ? a / ((double)lp->W * (double)nrw) : 0.;
-    lp->results[7] = nind > 0 ? ai / ((double)lp->W * (double)nind) : 0.;
-    return MCI_OK;
-}
 } // namespace
 
 extern "C" {
@@ -943,8 +866,8 @@ mci_status mci_lp_run(mci_lp *lp, int mode, int64_t nsteps, int nblocks, double 
     switch (mode) {
     case MCI_LP_PLAIN_FP64: TRY(run_plain<double>(lp, nsteps, nblocks)); break;
     case MCI_LP_PLAIN_FP32: TRY(run_plain<float>(lp, nsteps, nblocks)); break;
-    case MCI_LP_MIXED_FP32: TRY(run_mixed(lp, nsteps, nblocks, pmix)); break;
-    case MCI_LP_INDEP_FP32: TRY(run_mixed_indep(lp, nsteps, nblocks, pmix)); break;
+    case MCI_LP_MIXED_FP32: TRY(run_mixed(lp, nsteps, nblocks, pmix, false)); break;
+    case MCI_LP_INDEP_FP32: TRY(run_mixed(lp, nsteps, nblocks, pmix, true)); break;
     default: return fail(MCI_E_DOMAIN, "unknown mode %d", mode);
     }
     if (avg) { avg[0] = lp->results[0], avg[1] = lp->results[1]; }
