@@ -445,10 +445,6 @@ __global__ __launch_bounds__(WG) void k_lp_rw(Target<R> T, State<R> St, R step, 
 //  2. the network at x', log q(x | x') folded the same way;
 //  3. the MH test; accepted walkers take x', its target state, logits and mixture.
 // Mode 0 evaluates the network only and writes the mixture (means, log-scales, logits) for tests.
-// Mode 2 is the independence proposal: the network is evaluated once, at the fixed input xin [D]
-// for every walker, and its mixture q is the proposal of every step; a step is then step 1 (x' ~
-// q, log q(x') folded, log p(x')) and the MH test with the log q(x) carried from the step that
-// last moved the walker.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __fp16 h2_t __attribute__((ext_vector_type(2)));
@@ -740,7 +736,7 @@ size_t learned_lds()
            sizeof(uint16_t) * ((size_t)TW * C::LDX + (size_t)TW * C::LDH + (size_t)H * C::LDX + (size_t)H * C::LDH);
 }
 
-template <int MODE, int D, int H> // 0: the network at xin, written to out; 1: nl learned steps from step t0; 2: nl independence steps
+template <int MODE, int D, int H> // 0: the network at xin, written to out; 1: nl learned steps from step t0
 __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, State<float> St, const float *xin, float *out, uint64_t t0, int nl,
                                                    uint32_t k0, uint32_t k1, int32_t *nacc_ind)
 {
@@ -789,7 +785,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     for (int i = tid; i < C::LOFF + 32; i += TT) { bs[2 * H + i] = N.b3[i]; }
     for (int j = part; j < D; j += NPART) { xs[j * TW + col] = xsrc[(int64_t)j * W + wl]; }
     for (int j = part; j < C::DP; j += NPART) {
-        const float v = j >= D ? 0.f : MODE == 2 ? xin[j] : xsrc[(int64_t)j * W + wl];
+        const float v = j < D ? xsrc[(int64_t)j * W + wl] : 0.f;
         Xb[col * C::LDX + j] = f2bf(v * mcim::IN_SCALE);
     }
     __syncthreads();
@@ -797,16 +793,13 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
     float lpi[2] = {0.f, 0.f};
     network<D, H>(N, W1s, W2s, bs, Xb, Hb, mx, lpi, lgx, MODE == 0 ? out : nullptr, W, w0);
     if (MODE == 0) { return; }
-    // the independence proposal's log q at the walkers' positions (cf2 is free in mode 2)
-    if (MODE == 2) { fold<D>(mx, lpi, xs, cf2, K); }
     // the logits of walkers 32 .. 63 come from wave 4
     __syncthreads();
     // the chain state of walker col, held by wave 0
-    float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f, lqx = 0.f;
+    float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
     double sq = 0.;
     int jb = 0, bi = 0, nacc = 0;
     if (part == 0) {
-        if (MODE == 2) { lqx = mixture_logq(lgx, cf2, K, D, col); }
         m = St.m[wl];
         S = St.S[wl];
         sq = St.sq[wl];
@@ -902,9 +895,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         }
         __syncthreads();
         fold<D>(mx, lpi, xps, cf, K);
-        if (MODE == 1) {
-            for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
-        }
+        for (int j = part; j < C::DP; j += NPART) { Xb[col * C::LDX + j] = f2bf(j < D ? xps[j * TW + col] * mcim::IN_SCALE : 0.f); }
         {
             // target component wv at x' on the matrix cores (FP32 products and sums):
             // U_k x' - U_k mu_k for both walker halves, then |.|^2 over the 32 rows
@@ -923,16 +914,14 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         // the network at x' and log q(x | x')
         h2_t mx2[D];
         float lpi2[2] = {0.f, 0.f};
-        if (MODE == 1) {
-            network<D, H>(Ns, W1s, W2s, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
-            fold<D>(mx2, lpi2, xs, cf2, K);
-            __syncthreads();
-        }
+        network<D, H>(Ns, W1s, W2s, bs, Xb, Hb, mx2, lpi2, lg, nullptr, W, w0);
+        fold<D>(mx2, lpi2, xs, cf2, K);
+        __syncthreads();
         // wave 0: the MH test, the accepted walkers' new state, then the next step's component;
         // waves 1 .. 7 meanwhile draw the next step's normals
         if (part == 0) {
             const float logq_f = mixture_logq(lgx, cf, K, D, col);
-            const float logq_r = MODE == 2 ? lqx : mixture_logq(lg, cf2, K, D, col);
+            const float logq_r = mixture_logq(lg, cf2, K, D, col);
             float mn = -__builtin_huge_valf();
             float e[KMAX];
 #pragma unroll
@@ -962,10 +951,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
                 }
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k) { qx[k * TW + col] = qn[k * TW + col]; }
-                if (MODE == 1) {
-                    for (int k = 0; k < K; ++k) { lgx[k * TW + col] = lg[k * TW + col]; }
-                }
-                lqx = logq_f;
+                for (int k = 0; k < K; ++k) { lgx[k * TW + col] = lg[k * TW + col]; }
                 m = mn;
                 S = Sn;
                 sq = s2;
@@ -977,7 +963,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         else if (st + 1 < nl) { draw_normals(t + 1); }
         __syncthreads();
         // accepted walkers: the mixture at x takes the proposal's
-        if (MODE == 1) {
+        {
             const bool a = af[32 * cbw + (lane & 31)] != 0;
 #pragma unroll
             for (int i = 0; i < D; ++i) { mx[i] = a ? mx2[i] : mx[i]; }
@@ -1006,6 +992,230 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         }
         nacc_ind[w] += nacc;
     }
+}
+
+// ---- the independence proposal: q(x') = sum_k pi_k N(x'; mu_k, diag sigma_k^2), one mixture for
+// every walker ----
+// A workgroup takes TW = 64 walkers for a run of nl consecutive learned steps with eight waves.  The
+// mixture lives in LDS, read by a whole wave at one address (wave k folds component k) or with a
+// padded pitch (the x' build, where lanes hold different components), so that no lane keeps it in
+// registers and two workgroups share a compute unit (the phases of one overlap the other's).  A step:
+//  1. every thread builds D / 8 coordinates of x' = mu_k + sigma_k z (k and z were drawn at the end
+//     of the previous step);
+//  2. wave k: the fold of x' under component k, lc_k - |(x' - mu_k) / sigma_k|^2 / 2, and the
+//     target's q_k at x' on the matrix cores;
+//  3. wave 0: log q(x') by log-sum-exp over the folds, the MH test against the log q(x) kept from the
+//     step that last moved the walker, the accepted walkers' new state, the next component; waves
+//     1 .. 7 meanwhile draw the next step's normals.
+// The random words are those of the learned move (component and acceptance uniform from counter
+// 0x10000000, normals from 0x10000000 + (generator << 20) + call).
+template <int D>
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(4))) void k_lp_indep(IndepMix M, Target<float> T, State<float> St, uint64_t t0,
+                                                                                        int nl, uint32_t k0, uint32_t k1, int32_t *nacc_ind)
+{
+    static_assert(TW == 64 && NPART == KMAX, "64 walkers, one wave per component");
+    constexpr int DP = D + 1; // mixture pitch
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int K = M.K;
+    const int tid = threadIdx.x, col = tid % TW, part = tid / TW, lane = tid & 63;
+    const int wv = wave_id();
+    const int64_t W = St.W;
+    const int64_t w = (int64_t)blockIdx.x * TW + col;
+    const bool live = w < W;
+    const int64_t wl = live ? w : W - 1;
+    float *xs = reinterpret_cast<float *>(lds_raw); // [D][64] current x
+    float *xps = xs + D * TW;                        // [D][64] the proposal
+    float *zs = xps + D * TW;                        // [D][64] normals of the next proposal
+    float *cf = zs + D * TW;                         // [KMAX][64] folds (at x', at the start at x)
+    float *qn = cf + KMAX * TW;                      // [KMAX][64] target quadratic forms at x'
+    float *qx = qn + KMAX * TW;                      // [KMAX][64] the same at x (accepted moves)
+    float *mus = qx + KMAX * TW;                     // [KMAX][DP]
+    float *isg = mus + KMAX * DP;                    // [KMAX][DP] 1 / sigma
+    float *sgs = isg + KMAX * DP;                    // [KMAX][DP] sigma
+    float *lcs = sgs + KMAX * DP;                    // [KMAX]
+    float *cws = lcs + KMAX;                         // [KMAX]
+    int *kst = reinterpret_cast<int *>(cws + KMAX);  // [64] chosen component
+    int *af = kst + TW;                              // [64] accepted in this step
+    int *anyacc = af + TW;                           // [64] accepted in this run
+    for (int e = tid; e < KMAX * D; e += TT) {
+        const int k = e / D, j = e % D;
+        mus[k * DP + j] = M.mu[e];
+        isg[k * DP + j] = M.isig[e];
+        sgs[k * DP + j] = M.sig[e];
+    }
+    if (tid < KMAX) {
+        lcs[tid] = M.lc[tid];
+        cws[tid] = M.cw[tid];
+    }
+    for (int j = part; j < D; j += NPART) { xs[j * TW + col] = St.x[(int64_t)j * W + wl]; }
+    __syncthreads();
+    // wave k: the fold of y [D][64] under component k into cf
+    auto fold_k = [&](const float *y) {
+        if (wv < K) {
+            float s = 0.f;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const float z = (y[j * TW + lane] - mus[wv * DP + j]) * isg[wv * DP + j];
+                s = fmaf(z, z, s);
+            }
+            cf[wv * TW + lane] = lcs[wv] - 0.5f * s;
+        }
+    };
+    // log q at walker col from the folds in cf (wave 0)
+    auto logq = [&]() {
+        float v[KMAX], mx = -__builtin_huge_valf();
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            v[k] = k < K ? cf[k * TW + col] : -__builtin_huge_valf();
+            mx = fmaxf(mx, v[k]);
+        }
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) { s += k < K ? __expf(v[k] - mx) : 0.f; }
+        return mx + __logf(s);
+    };
+    fold_k(xs);
+    __syncthreads();
+    float m = 0.f, S = 1.f, s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f, lqx = 0.f;
+    double sq = 0.;
+    int jb = 0, bi = 0, nacc = 0;
+    uint32_t o[4] = {0u, 0u, 0u, 0u};
+    auto choose = [&](uint64_t t) {
+        philox(0x10000000u, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, o);
+        const float u = (float)(o[0] >> 8) * 0x1p-24f;
+        int kk = K - 1;
+#pragma unroll
+        for (int k = KMAX - 2; k >= 0; --k) { kk = (k < K - 1 && u < cws[k]) ? k : kk; }
+        kst[col] = kk;
+    };
+    auto draw_normals = [&](uint64_t t) {
+        constexpr int NZ = (D + NPART - 2) / (NPART - 1);
+        float zq[4 * ((NZ + 3) / 4)];
+        const uint32_t base = 0x10000000u + ((uint32_t)part << 20);
+#pragma unroll
+        for (int c = 0; c < (NZ + 3) / 4; ++c) { normal4(base + 1u + (uint32_t)c, (uint32_t)wl, (uint32_t)t, (uint32_t)(t >> 32), k0, k1, zq + 4 * c); }
+#pragma unroll
+        for (int i = 0; i < NZ; ++i) {
+            const int j = part - 1 + (NPART - 1) * i;
+            if (j < D) { zs[j * TW + col] = zq[i]; }
+        }
+    };
+    if (part == 0) {
+        lqx = logq();
+        m = St.m[wl];
+        S = St.S[wl];
+        sq = St.sq[wl];
+        if (St.B > 0) {
+            s0 = St.sb[wl];
+            c0 = St.cb[wl];
+            s1 = St.sb[W + wl];
+            c1 = St.cb[W + wl];
+            jb = St.jb[wl];
+            bi = St.bi[wl];
+        }
+        choose(t0);
+    }
+    else { draw_normals(t0); }
+    __syncthreads();
+    for (int st = 0; st < nl; ++st) {
+        const float *Ubase = T.Ut, *Umbase = T.Um;
+        asm volatile("" : "+s"(Ubase), "+s"(Umbase));
+        const uint64_t t = t0 + (uint64_t)st;
+        {
+            const int k = kst[col];
+            float zv[(D + NPART - 1) / NPART];
+#pragma unroll
+            for (int i = 0; i < (D + NPART - 1) / NPART; ++i) {
+                const int j = part + NPART * i;
+                zv[i] = j < D ? zs[j * TW + col] : 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < (D + NPART - 1) / NPART; ++i) {
+                const int j = part + NPART * i;
+                if (j < D) { xps[j * TW + col] = fmaf(sgs[k * DP + j], zv[i], mus[k * DP + j]); }
+            }
+        }
+        __syncthreads();
+        fold_k(xps);
+        {
+            const int kq[2] = {wv, wv};
+            const int cq[2] = {0, 1};
+            float qv[2];
+            target_q<D, true>(Ubase, Umbase, kq, cq, xps, lane, qv);
+            if (lane < 32) {
+                qn[wv * TW + lane] = qv[0];
+                qn[wv * TW + 32 + lane] = qv[1];
+            }
+        }
+        __syncthreads();
+        if (part == 0) {
+            const float lqn = logq();
+            float mn = -__builtin_huge_valf();
+            float e[KMAX];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                e[k] = T.c[k] - 0.5f * qn[k * TW + col];
+                mn = fmaxf(mn, e[k]);
+            }
+            float Sn = 0.f;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) { Sn += __expf(e[k] - mn); }
+            const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
+            const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - (m + __logf(S)) + (lqx - lqn));
+            const float x0 = acc ? xps[col] : xs[col];
+            if (acc) {
+                double s2 = 0.;
+#pragma unroll
+                for (int j0 = 0; j0 < D; j0 += 8) {
+                    float v[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) { v[j] = xps[(j0 + j) * TW + col]; }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        xs[(j0 + j) * TW + col] = v[j];
+                        s2 = fma((double)v[j], (double)v[j], s2);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) { qx[k * TW + col] = qn[k * TW + col]; }
+                m = mn;
+                S = Sn;
+                sq = s2;
+                lqx = lqn;
+                ++nacc;
+            }
+            if (live && St.B > 0) { accumulate(St, w, true, sq, x0, s0, c0, s1, c1, jb, bi); }
+            if (st + 1 < nl) { choose(t + 1); }
+        }
+        else if (st + 1 < nl) { draw_normals(t + 1); }
+        __syncthreads();
+    }
+    if (part == 0) { anyacc[col] = nacc; }
+    __syncthreads();
+    if (live && anyacc[col] > 0) {
+        for (int j = part; j < D; j += NPART) { St.x[(int64_t)j * W + w] = xs[j * TW + col]; }
+        St.q[(int64_t)part * W + w] = qx[part * TW + col];
+    }
+    if (part == 0 && live) {
+        St.m[w] = m;
+        St.S[w] = S;
+        St.sq[w] = sq;
+        if (St.B > 0) {
+            St.sb[w] = s0;
+            St.cb[w] = c0;
+            St.sb[W + w] = s1;
+            St.cb[W + w] = c1;
+            St.jb[w] = jb;
+            St.bi[w] = bi;
+        }
+        nacc_ind[w] += nacc;
+    }
+}
+
+template <int D>
+size_t indep_lds()
+{
+    return sizeof(float) * ((size_t)3 * D * TW + (size_t)3 * KMAX * TW + (size_t)3 * KMAX * (D + 1) + 2 * KMAX) + sizeof(int) * 3 * TW;
 }
 
 template <class R>
@@ -1090,12 +1300,35 @@ hipError_t launch_learned_m(const NetDev &N, const Target<float> &T, const State
 
 bool learned_supported(int d, int H, int K) { return (d == 8 || d == 16 || d == 32) && (H == 64 || H == 128) && K >= 1 && K <= KMAX; }
 
-hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, const float *fixed_input, uint64_t t0, int nl,
-                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s)
+hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                             int32_t *nacc_ind, hipStream_t s)
 {
     if (!learned_supported(N.d, N.H, N.K) || T.d != N.d || nl < 1) { return hipErrorInvalidValue; }
-    if (fixed_input != nullptr) { return launch_learned_m<2>(N, T, S, fixed_input, nullptr, t0, nl, k0, k1, nacc_ind, s); }
     return launch_learned_m<1>(N, T, S, nullptr, nullptr, t0, nl, k0, k1, nacc_ind, s);
+}
+
+namespace
+{
+template <int D>
+hipError_t launch_indep_t(const IndepMix &M, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                          int32_t *nacc_ind, hipStream_t s)
+{
+    const size_t lds = indep_lds<D>();
+    hipLaunchKernelGGL(k_lp_indep<D>, dim3((unsigned)((S.W + TW - 1) / TW)), dim3(TT), lds, s, M, T, S, t0, nl, k0, k1, nacc_ind);
+    return hipGetLastError();
+}
+} // namespace
+
+hipError_t launch_lp_indep(const IndepMix &M, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                           int32_t *nacc_ind, hipStream_t s)
+{
+    if (M.K < 1 || M.K > KMAX || nl < 1 || S.W < 1) { return hipErrorInvalidValue; }
+    switch (T.d) {
+    case 8: return launch_indep_t<8>(M, T, S, t0, nl, k0, k1, nacc_ind, s);
+    case 16: return launch_indep_t<16>(M, T, S, t0, nl, k0, k1, nacc_ind, s);
+    case 32: return launch_indep_t<32>(M, T, S, t0, nl, k0, k1, nacc_ind, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_lp_network(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, hipStream_t s)

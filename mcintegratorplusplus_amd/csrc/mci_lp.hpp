@@ -65,11 +65,24 @@ struct NetDev
 };
 // the fused kernel exists for d in {8, 16, 32}, H in {64, 128}, 1 <= K <= KMAX
 bool learned_supported(int d, int H, int K);
-// nl consecutive learned moves, steps t0 .. t0 + nl - 1, for every walker (FP32 chain): with
-// fixed_input null the conditional proposal q(x' | x) (the network at x), else the independence
-// proposal q(x') (the network at fixed_input, device [d])
-hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, const float *fixed_input, uint64_t t0, int nl,
-                             uint32_t k0, uint32_t k1, int32_t *nacc_ind, hipStream_t s);
+// nl consecutive learned moves with the conditional proposal q(x' | x) (the network at x), steps
+// t0 .. t0 + nl - 1, for every walker (FP32 chain)
+hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                             int32_t *nacc_ind, hipStream_t s);
+// the independence proposal's mixture (device arrays): q(x) = sum_k exp(lc_k - |(x - mu_k) isig_k|^2 / 2)
+struct IndepMix
+{
+    const float *mu;   // [KMAX][d] means
+    const float *isig; // [KMAX][d] 1 / sigma
+    const float *sig;  // [KMAX][d] sigma (the move's scale: x' = mu + sigma z)
+    const float *lc;   // [KMAX] log pi_k + sum_j log(1 / sigma_kj) - d log(2 pi) / 2 (-inf past K)
+    const float *cw;   // [KMAX] cumulative weights (1 from component K - 1 on)
+    int K;
+};
+// nl consecutive independence moves, steps t0 .. t0 + nl - 1, for every walker (FP32 chain; d in
+// {8, 16, 32})
+hipError_t launch_lp_indep(const IndepMix &M, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
+                           int32_t *nacc_ind, hipStream_t s);
 // the same kernel's network at positions xin [d][S.W], the mixture to out [K (1 + 2 d)][S.W]
 // (per component d means and d log-scales, transformed; then K logits)
 hipError_t launch_lp_network(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, hipStream_t s);

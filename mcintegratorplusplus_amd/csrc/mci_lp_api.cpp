@@ -284,6 +284,7 @@ struct mci_lp
     uint64_t train_iter = 0; // minibatches drawn so far (each draws its own pairs)
     bool fixed_input = false; // train with the fixed input ctx0 (an independence proposal)
     DBuf<float> dctx0;        // ctx0 on the device
+    DBuf<float> imix;         // the independence proposal's mixture (mu, 1 / sigma, sigma, lc, cumulative weights)
     int nsnap = 0;
     double same_frac = 0.5;
     std::vector<float> ctx0;   // mean of the last snapshot: the network input of the independence proposal
@@ -566,6 +567,56 @@ mci_status net_dev(mci_lp *lp, mcil::NetDev &N)
     return MCI_OK;
 }
 
+// the independence proposal: the network's mixture at ctx0 (the mean of the last snapshot), as the
+// fused kernel evaluates it (means and 1 / sigma rounded to halves), into lp->imix: mu, 1 / sigma,
+// sigma [KMAX][d], lc [KMAX] (log weight + sum log(1 / sigma) - d log(2 pi) / 2, -inf past K),
+// cumulative weights [KMAX]
+mci_status indep_mixture(mci_lp *lp, const mcil::NetDev &N, mcil::IndepMix &M)
+{
+    if (lp->ctx0.empty()) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect and mci_lp_train first"); }
+    const int d = lp->d, K = N.K, KM = mcil::KMAX;
+    HIPCHK(lp->dctx0.ensure((size_t)d));
+    HIPCHK(hipMemcpy(lp->dctx0.p, lp->ctx0.data(), sizeof(float) * (size_t)d, hipMemcpyHostToDevice));
+    DBuf<float> dout;
+    HIPCHK(dout.ensure((size_t)N.O));
+    mcil::State<float> S1{};
+    S1.W = 1;
+    HIPCHK(mcil::launch_lp_network(N, target_of<float>(lp), S1, lp->dctx0.p, dout.p, lp->stream));
+    std::vector<float> o((size_t)N.O);
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    HIPCHK(hipMemcpy(o.data(), dout.p, sizeof(float) * o.size(), hipMemcpyDeviceToHost));
+    const size_t KD = (size_t)KM * d;
+    std::vector<float> h(3 * KD + 2 * KM, 0.f);
+    float *mu = h.data(), *isig = mu + KD, *sig = isig + KD, *lc = sig + KD, *cw = lc + KM;
+    double amax = -HUGE_VAL, sa = 0., cum = 0.;
+    for (int k = 0; k < K; ++k) { amax = std::fmax(amax, (double)o[(size_t)2 * d * K + k]); }
+    for (int k = 0; k < K; ++k) { sa += std::exp((double)o[(size_t)2 * d * K + k] - amax); }
+    for (int k = 0; k < KM; ++k) {
+        if (k >= K) {
+            for (int j = 0; j < d; ++j) { isig[(size_t)k * d + j] = sig[(size_t)k * d + j] = 1.f; }
+            lc[k] = -HUGE_VALF;
+            cw[k] = 1.f;
+            continue;
+        }
+        const double lw = (double)o[(size_t)2 * d * K + k] - amax - std::log(sa);
+        double ls = 0.;
+        for (int j = 0; j < d; ++j) {
+            mu[(size_t)k * d + j] = o[(size_t)2 * d * k + j];
+            const float is = (float)std::exp(-(double)o[(size_t)2 * d * k + d + j]);
+            isig[(size_t)k * d + j] = is;
+            sig[(size_t)k * d + j] = (float)(1.0 / (double)is);
+            ls += std::log((double)is);
+        }
+        lc[k] = (float)(lw + ls - 0.5 * d * std::log(2.0 * M_PI));
+        cum += std::exp(lw);
+        cw[k] = k == K - 1 ? 1.f : (float)cum;
+    }
+    TRY(up(lp->imix, h));
+    const float *b = lp->imix.p;
+    M = {b, b + KD, b + 2 * KD, b + 3 * KD, b + 3 * KD + KM, K};
+    return MCI_OK;
+}
+
 // the chain mixing random-walk moves with learned moves: cycles of LP_CYCLE steps, the first
 // LP_CYCLE - nl random-walk moves (one launch), then nl = round(pmix LP_CYCLE) learned moves (one
 // fused launch).  Each move leaves p invariant, so does the cycle.  The learned moves use the
@@ -576,13 +627,8 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix, bool 
 {
     mcil::NetDev N;
     TRY(net_dev(lp, N));
-    const float *fixed = nullptr;
-    if (indep) {
-        if (lp->ctx0.empty()) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect and mci_lp_train first"); }
-        HIPCHK(lp->dctx0.ensure((size_t)lp->d));
-        HIPCHK(hipMemcpy(lp->dctx0.p, lp->ctx0.data(), sizeof(float) * (size_t)lp->d, hipMemcpyHostToDevice));
-        fixed = lp->dctx0.p;
-    }
+    mcil::IndepMix M{};
+    if (indep) { TRY(indep_mixture(lp, N, M)); }
     mcil::State<float> S;
     const int B = (int)(nsteps / nblocks);
     TRY(state_of<float>(lp, B, nblocks, S));
@@ -610,7 +656,8 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix, bool 
         }
         const int64_t l = std::min<int64_t>(nl, nsteps - s);
         if (l > 0) {
-            HIPCHK(mcil::launch_lp_learned(N, T, S, fixed, tb + (uint64_t)s, (int)l, k0, k1, lp->nacc_ind.p, lp->stream));
+            if (indep) { HIPCHK(mcil::launch_lp_indep(M, T, S, tb + (uint64_t)s, (int)l, k0, k1, lp->nacc_ind.p, lp->stream)); }
+            else { HIPCHK(mcil::launch_lp_learned(N, T, S, tb + (uint64_t)s, (int)l, k0, k1, lp->nacc_ind.p, lp->stream)); }
             nind += l;
             s += l;
         }
