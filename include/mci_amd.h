@@ -296,9 +296,9 @@ mci_status mci_mlp_loss_grad(mci_mlp *m, int64_t n, const float *x, const float 
 #define MCI_LP_PLAIN_FP32   1  /* the same chain in FP32 with Kahan block sums */
 #define MCI_LP_MIXED_FP32   2  /* cycles of 32 steps: round(32 (1 - pmix)) random-walk moves, then round(32 pmix)
                                   (at least 1) learned moves; pmix in (0, 1] */
-#define MCI_LP_INDEP_FP32   3  /* each step is, with probability pmix, an MH move from the independence proposal
-                                  q(x') = the network's mixture at one fixed input (the mean of the last training
-                                  snapshot), else a random-walk move; all steps in one launch */
+#define MCI_LP_INDEP_FP32   3  /* the same cycles, the learned moves from the independence proposal q(x') = the
+                                  network's mixture at one fixed input (the mean of the last training snapshot),
+                                  evaluated once per run */
 typedef struct mci_lp mci_lp;
 mci_status mci_lp_create(int d, int ncomp, const double *weights, const double *means, const double *covs, int64_t nwalkers,
                          mci_lp **out);

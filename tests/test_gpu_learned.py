@@ -101,16 +101,17 @@ def test_sampler_plain_and_untrained_mixed_are_unbiased():
 
 
 def test_ragged_walker_count_and_small_dimension():
-    """1,000 walkers (the last workgroup of the learned move holds 40 live walkers of 64) and
-    d = 16 with 5 network components: every chain stays unbiased."""
-    from mcintegratorplusplus_amd.learned import MIXED_FP32, PLAIN_FP32, mixture_target, LearnedProposalSampler
+    """1,000 walkers (the last workgroup of the learned moves holds 40 live walkers of 64) and
+    d = 16 with 5 network components: every chain stays unbiased, the independence chain (the
+    network's mixture at the snapshot mean) included."""
+    from mcintegratorplusplus_amd.learned import INDEP_FP32, MIXED_FP32, PLAIN_FP32, mixture_target, LearnedProposalSampler
     w, mu, cov, exact = mixture_target(d=16, ncomp=4, seed=9, mean_sd=2.0, eig=(0.2, 1.0))
     s = LearnedProposalSampler(w, mu, cov, 1000)
     s.setSeed(3)
     s.setNetwork(128, 5)
     s.collect(4, 128)
     s.train(100, 1024, 2e-3)
-    for mode, pm in ((PLAIN_FP32, 0.5), (MIXED_FP32, 0.5), (MIXED_FP32, 1.0)):
+    for mode, pm in ((PLAIN_FP32, 0.5), (MIXED_FP32, 0.5), (MIXED_FP32, 1.0), (INDEP_FP32, 0.5), (INDEP_FP32, 1.0)):
         avg, err = s.run(mode, 4096, 32, pm)
         r = s.results()
         print("ragged", mode, pm, avg, err, r["acc_learned"], flush=True)
