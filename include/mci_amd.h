@@ -52,9 +52,20 @@ typedef int mci_status;
 #define MCI_MOVE_ALL              0
 #define MCI_MOVE_VEC              1
 
-/* SRRDType, Factories.hpp:119-133 (same order).  Uniform is implemented. */
+/* SRRDType, Factories.hpp:119-133 (same order), each with the default parameters of createSymRRD
+ * (TrialMoveInterface.hpp:107-183): uniform on (-1, 1), normal(0, 1), Student-t(1), Cauchy(0, 1),
+ * then +/- (fair sign) exponential(1), gamma(1, 1), Weibull(1, 1), lognormal(0, 1), chi-squared(1),
+ * Fisher-F(1, 1).  The reference-stream mode draws them as libstdc++ (GCC 11) does. */
 #define MCI_SRRD_UNIFORM          0
 #define MCI_SRRD_GAUSSIAN         1
+#define MCI_SRRD_STUDENT          2
+#define MCI_SRRD_CAUCHY           3
+#define MCI_SRRD_EXPONENTIAL      4
+#define MCI_SRRD_GAMMA            5
+#define MCI_SRRD_WEIBULL          6
+#define MCI_SRRD_LOGNORMAL        7
+#define MCI_SRRD_CHISQ            8
+#define MCI_SRRD_FISHER           9
 
 /* ---- device functor catalogue (sampling functions) ----
  * The reference's SamplingFunctionInterface subclasses become device functors

@@ -105,6 +105,8 @@ struct mci_ctx
     bool walker_results_host = false; // h_wavg/h_werr hold the device's per-walker results
     DBuf<uint64_t> mt, ctr, seeds;
     DBuf<int32_t> mtp;
+    DBuf<double> nsv;     // [2][W] saved normals of the move's distribution
+    DBuf<int32_t> nsf;    // [W] their availability bits
     DBuf<int64_t> acc, rej;
     DBuf<uint8_t> active;
     std::vector<DBuf<double>> accdata, scratch;
@@ -199,6 +201,9 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     ch.movetype = c->movetype;
     ch.veclen = c->veclen;
     ch.nvecs = c->nvecs;
+    ch.srrd = c->srrd;
+    ch.nsv = c->nsv.p;
+    ch.nsf = c->nsf.p;
     ch.periodic = c->periodic ? 1 : 0;
     ch.lb = c->lbd.p;
     ch.ub = c->ubd.p;
@@ -283,7 +288,7 @@ float elapsed(mci_ctx *c, int a, int b)
 // the specialised kernels apply to this chain (observables are checked separately)
 bool fast_chain(const mci_ctx *c)
 {
-    return c->kernel_path == MCI_KERNELS_AUTO && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
+    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
                          c->pdfs.empty() ? 0 : c->pdfs[0].kind) &&
            !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
 }
@@ -736,8 +741,7 @@ mci_status mci_domain_is_finite(mci_ctx *c, int *f) { CHECKCTX(c); *f = c->perio
 mci_status mci_set_trial_move(mci_ctx *c, int srrd, int veclen, int ntypes, const int *te)
 {
     CHECKCTX(c);
-    if (srrd < MCI_SRRD_UNIFORM || srrd > 9) { return fail(MCI_E_DOMAIN, "[createSRRDAllMove] Unhandled SRRDType enumerator."); }
-    if (srrd != MCI_SRRD_UNIFORM) { return fail(MCI_E_DOMAIN, "only the uniform SRRD move is implemented on the device"); }
+    if (srrd < MCI_SRRD_UNIFORM || srrd > MCI_SRRD_FISHER) { return fail(MCI_E_DOMAIN, "[createSRRDAllMove] Unhandled SRRDType enumerator."); }
     if (ntypes < 1) { ntypes = 1; }
     if (ntypes > MAX_TYPES) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d step-size types", MAX_TYPES); }
     if (ntypes > 1 && !te) { return fail(MCI_E_INVALID_ARGUMENT, "[checkTrialMoveSanity] ntypes>1 requires passed typeEnds array."); }
@@ -764,6 +768,10 @@ mci_status mci_set_trial_move(mci_ctx *c, int srrd, int veclen, int ntypes, cons
     c->veclen = veclen > 0 ? veclen : 0;
     c->nvecs = veclen > 0 ? c->ndim / veclen : 0;
     HIPCHK(c->steps.ensure((size_t)ntypes * c->W));
+    // a new move has fresh distribution objects: no saved normals
+    HIPCHK(c->nsv.ensure((size_t)2 * c->W));
+    HIPCHK(c->nsf.ensure((size_t)c->W));
+    HIPCHK(hipMemsetAsync(c->nsf.p, 0, sizeof(int32_t) * (size_t)c->W, c->stream));
     return fill_rows(c, c->steps.p, 0, ntypes, 0.05); // DEFAULT_MRT2STEP (Factories.hpp:145)
 }
 

@@ -61,6 +61,9 @@ struct Chain
     int ntypes;
     int typeEnds[MAX_TYPES];
     int movetype, veclen, nvecs;
+    int srrd;                   // the move's distribution (MCI_SRRD_*)
+    double *nsv;                // [2][W] saved second normals of the move's distribution
+    int32_t *nsf;               // [W] bit s: nsv row s available
     int periodic;
     const double *lb, *ub;      // [ndim]
     int npdf;
@@ -86,6 +89,8 @@ struct Lane
     int cfirst;
     bool accepted;
     int64_t acc, rej;
+    double sv[2];      // saved normals of the move's distribution (nsv)
+    int sf;            // their availability bits (nsf)
 };
 
 // mt19937_64 with the twist done lazily, one element per output: element k of a round
@@ -204,6 +209,87 @@ __device__ __forceinline__ int draw_index(const Chain &c, Lane &L, int n)
         }
     }
     return (int)(prod >> 32);
+}
+
+// --------------------------------------------------------------------------
+// the move distributions (SRRDType, Factories.hpp:118-131) with createSymRRD's default parameters
+// (TrialMoveInterface.hpp:107-183), drawn as libstdc++ (GCC 11) draws them: every uniform is one
+// canonical double (draw_unit), the normal is Marsaglia's polar method with the pair's second
+// value kept per walker (two generators at most: slots 0 and 1), the gamma Marsaglia-Tsang
+// boosted by u^(1/alpha) for alpha < 1, the one-sided ones signed by one fair bernoulli draw after
+// the value (SymmetrizedPRRD, TrialMoveInterface.hpp:80-98).  With the reference stream the
+// draws are the reference's up to the last place of the device's log, tan, pow and exp.
+// --------------------------------------------------------------------------
+__device__ inline double nd_draw(const Chain &c, Lane &L, int s)
+{
+    double ret;
+    if (L.sf & (1 << s)) {
+        L.sf &= ~(1 << s);
+        ret = L.sv[s];
+    }
+    else {
+        double x, y, r2;
+        do {
+            x = 2.0 * draw_unit(c, L) - 1.0;
+            y = 2.0 * draw_unit(c, L) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double mult = sqrt(-2 * log(r2) / r2);
+        L.sv[s] = x * mult;
+        L.sf |= 1 << s;
+        ret = y * mult;
+    }
+    return ret * 1.0 + 0.0;
+}
+
+__device__ inline double gamma_draw(const Chain &c, Lane &L, int s, double alpha, double beta)
+{
+    const double malpha = alpha < 1.0 ? alpha + 1.0 : alpha;
+    const double a1 = malpha - 1.0 / 3.0;
+    const double a2 = 1.0 / sqrt(9.0 * a1);
+    double u, v, n;
+    do {
+        do {
+            n = nd_draw(c, L, s);
+            v = 1.0 + a2 * n;
+        } while (v <= 0.0);
+        v = v * v * v;
+        u = draw_unit(c, L);
+    } while (u > 1.0 - 0.0331 * n * n * n * n && (log(u) > (0.5 * n * n + a1 * (1.0 - v + log(v)))));
+    if (alpha == malpha) { return a1 * v * beta; }
+    do { u = draw_unit(c, L); } while (u == 0.0);
+    return pow(u, 1.0 / alpha) * a1 * v * beta;
+}
+
+__device__ inline double srrd_draw(const Chain &c, Lane &L)
+{
+    double val;
+    switch (c.srrd) {
+    case MCI_SRRD_UNIFORM: return draw_sym(c, L);
+    case MCI_SRRD_GAUSSIAN: return nd_draw(c, L, 0);
+    case MCI_SRRD_STUDENT: {
+        const double z = nd_draw(c, L, 0);
+        return z * sqrt(1.0 / gamma_draw(c, L, 1, 0.5, 2.0));
+    }
+    case MCI_SRRD_CAUCHY: {
+        double u;
+        do { u = draw_unit(c, L); } while (u == 0.5);
+        return 0.0 + 1.0 * tan(3.141592653589793 * u);
+    }
+    case MCI_SRRD_EXPONENTIAL: val = -log(1.0 - draw_unit(c, L)) / 1.0; break;
+    case MCI_SRRD_GAMMA: val = gamma_draw(c, L, 0, 1.0, 1.0); break;
+    case MCI_SRRD_WEIBULL: val = 1.0 * pow(-log(1.0 - draw_unit(c, L)), 1.0 / 1.0); break;
+    case MCI_SRRD_LOGNORMAL: val = exp(1.0 * nd_draw(c, L, 0) + 0.0); break;
+    case MCI_SRRD_CHISQ: val = 2 * gamma_draw(c, L, 0, 0.5, 1.0); break;
+    case MCI_SRRD_FISHER: {
+        const double gx = gamma_draw(c, L, 0, 0.5, 1.0);
+        const double gy = gamma_draw(c, L, 1, 0.5, 1.0);
+        val = (gx * 1.0) / (gy * 1.0);
+        break;
+    }
+    default: return __builtin_nan("");
+    }
+    return ((draw_unit(c, L) - 0.0) < 0.5 * (1.0 - 0.0)) ? val : -val;
 }
 
 // --------------------------------------------------------------------------
@@ -343,7 +429,7 @@ __device__ inline double trial_move(const Chain &c, Lane &L)
             const double st = c.steps[(int64_t)t * W + w];
             while (xidx < c.typeEnds[t]) {
                 double &xn = c.xnew[(int64_t)xidx * W + w];
-                xn = xn + st * draw_sym(c, L);
+                xn = xn + st * srrd_draw(c, L);
                 ++xidx;
             }
         }
@@ -361,7 +447,7 @@ __device__ inline double trial_move(const Chain &c, Lane &L)
     const double st = c.steps[(int64_t)t * W + w];
     for (int i = 0; i < c.veclen; ++i) {
         double &xn = c.xnew[(int64_t)(xidx + i) * W + w];
-        xn = xn + st * draw_sym(c, L);
+        xn = xn + st * srrd_draw(c, L);
     }
     L.nchanged = c.veclen;
     L.cfirst = xidx;
@@ -544,12 +630,24 @@ __device__ inline void load_lane(const Chain &c, Lane &L, int64_t w)
     L.accepted = true;
     L.acc = 0;
     L.rej = 0;
+    L.sf = 0;
+    L.sv[0] = L.sv[1] = 0.0;
+    if (c.nsf != nullptr) {
+        L.sf = c.nsf[w];
+        L.sv[0] = c.nsv[w];
+        L.sv[1] = c.nsv[c.W + w];
+    }
 }
 
 __device__ inline void store_lane(const Chain &c, const Lane &L)
 {
     if (c.rng == MCI_RNG_MT19937_64) { c.mtp[L.w] = L.mtp; }
     else { c.ctr[L.w] = L.t; }
+    if (c.nsf != nullptr) {
+        c.nsf[L.w] = L.sf;
+        c.nsv[L.w] = L.sv[0];
+        c.nsv[c.W + L.w] = L.sv[1];
+    }
     c.acc[L.w] = L.acc;
     c.rej[L.w] = L.rej;
 }

@@ -464,6 +464,11 @@ __global__ void k_move_x(Chain c)
         c.xold[(int64_t)j * W + w] = x;
     }
     if (c.rng == MCI_RNG_MT19937_64) { c.mtp[w] = L.mtp; } else { c.ctr[w] = L.t; }
+    if (c.nsf != nullptr) { // the move's distribution keeps its saved normals
+        c.nsf[w] = L.sf;
+        c.nsv[w] = L.sv[0];
+        c.nsv[W + w] = L.sv[1];
+    }
 }
 
 // MCI::newRandomX (MCIntegrator.cpp:614-619)

@@ -232,6 +232,8 @@ typedef struct {
     uint64_t t;            /* Philox: steps taken */
     uint32_t k, kx;        /* Philox: next word within the step / extra words */
     uint32_t wid;
+    double nsv[2];         /* saved second normals of the move's distribution (two normal generators at most) */
+    int nsf;               /* bit s: nsv[s] available */
 } walker;
 
 #define MAX_PDF 16
@@ -341,6 +343,98 @@ static double draw_unit(const struct orc_ctx *c, walker *w)
     uint64_t a = ph_next(c, w), b = ph_next(c, w);
     uint64_t v = (a << 32) | b;
     return (double)(v >> 11) * 0x1p-53;
+}
+
+/* ------------------------------------------------------------------------- */
+/* the move distributions (SRRDType, Factories.hpp:118-131) with createSymRRD's default parameters
+ * (TrialMoveInterface.hpp:107-183).  The reference draws them with libstdc++'s distributions (GCC
+ * 11.4, the system's; bits/random.h and bits/random.tcc), restated here: every uniform is
+ * generate_canonical<double, 53> on the walker's engine (one word, as draw_unit), the normal is
+ * Marsaglia's polar method keeping the pair's second value in the distribution object, the gamma is
+ * Marsaglia-Tsang (boosted by u^(1/alpha) for alpha < 1), and the one-sided ones take a fair sign
+ * from bernoulli_distribution(0.5) after the value (SymmetrizedPRRD, TrialMoveInterface.hpp:80-98).
+ * Each distribution object of the reference is per walker (one MCI per walker), so is its saved
+ * normal: slot 0 and slot 1 below.  The native (Philox) mode draws the same way on its own uniforms. */
+/* ------------------------------------------------------------------------- */
+static double canon(const struct orc_ctx *c, walker *w) { return draw_unit(c, w); }
+
+/* normal_distribution<double>(0, 1)::operator() with its saved value in slot s */
+static double nd_draw(const struct orc_ctx *c, walker *w, int s)
+{
+    double ret;
+    if (w->nsf & (1 << s)) {
+        w->nsf &= ~(1 << s);
+        ret = w->nsv[s];
+    }
+    else {
+        double x, y, r2;
+        do {
+            x = 2.0 * canon(c, w) - 1.0;
+            y = 2.0 * canon(c, w) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double mult = sqrt(-2 * log(r2) / r2);
+        w->nsv[s] = x * mult;
+        w->nsf |= 1 << s;
+        ret = y * mult;
+    }
+    return ret * 1.0 + 0.0;
+}
+
+/* gamma_distribution<double>(alpha, beta)::operator() with its normal in slot s */
+static double gamma_draw(const struct orc_ctx *c, walker *w, int s, double alpha, double beta)
+{
+    const double malpha = alpha < 1.0 ? alpha + 1.0 : alpha;
+    const double a1 = malpha - 1.0 / 3.0;
+    const double a2 = 1.0 / sqrt(9.0 * a1);
+    double u, v, n;
+    do {
+        do {
+            n = nd_draw(c, w, s);
+            v = 1.0 + a2 * n;
+        } while (v <= 0.0);
+        v = v * v * v;
+        u = canon(c, w);
+    } while (u > 1.0 - 0.0331 * n * n * n * n && (log(u) > (0.5 * n * n + a1 * (1.0 - v + log(v)))));
+    if (alpha == malpha) return a1 * v * beta;
+    do u = canon(c, w);
+    while (u == 0.0);
+    return pow(u, 1.0 / alpha) * a1 * v * beta;
+}
+
+/* one draw of the move's distribution */
+static double srrd_draw(const struct orc_ctx *c, walker *w)
+{
+    double val;
+    switch (c->srrd) {
+    case MCI_SRRD_UNIFORM: return draw_sym(c, w);
+    case MCI_SRRD_GAUSSIAN: return nd_draw(c, w, 0);
+    case MCI_SRRD_STUDENT: { /* student_t_distribution(1): nd * sqrt(n / gd), gd = gamma(n/2, 2) */
+        const double z = nd_draw(c, w, 0);
+        return z * sqrt(1.0 / gamma_draw(c, w, 1, 0.5, 2.0));
+    }
+    case MCI_SRRD_CAUCHY: { /* cauchy_distribution(0, 1) */
+        double u;
+        do u = canon(c, w);
+        while (u == 0.5);
+        const double pi = 3.1415926535897932384626433832795029;
+        return 0.0 + 1.0 * tan(pi * u);
+    }
+    case MCI_SRRD_EXPONENTIAL: val = -log(1.0 - canon(c, w)) / 1.0; break;
+    case MCI_SRRD_GAMMA: val = gamma_draw(c, w, 0, 1.0, 1.0); break;
+    case MCI_SRRD_WEIBULL: val = 1.0 * pow(-log(1.0 - canon(c, w)), 1.0 / 1.0); break;
+    case MCI_SRRD_LOGNORMAL: val = exp(1.0 * nd_draw(c, w, 0) + 0.0); break;
+    case MCI_SRRD_CHISQ: val = 2 * gamma_draw(c, w, 0, 0.5, 1.0); break;
+    case MCI_SRRD_FISHER: { /* fisher_f_distribution(1, 1): (gd_x * n) / (gd_y * m), gd_x = gamma(m/2), gd_y = gamma(n/2) */
+        const double gx = gamma_draw(c, w, 0, 0.5, 1.0);
+        const double gy = gamma_draw(c, w, 1, 0.5, 1.0);
+        val = (gx * 1.0) / (gy * 1.0);
+        break;
+    }
+    default: return NAN;
+    }
+    /* SymmetrizedPRRD: bernoulli_distribution(0.5) on one canonical draw */
+    return ((canon(c, w) - 0.0) < 0.5 * (1.0 - 0.0)) ? val : -val;
 }
 
 /* uniform integer in [0, n): MT mode = uniform_int_distribution<int>(0, n-1) (SRRDVecMove.hpp:27,78);
@@ -995,7 +1089,7 @@ static void seed_walkers(orc_ctx *c)
  * TypedMoveInterface ctor (TypedMoveInterface.hpp:27-39), SRRDVecMove ctor checks (SRRDVecMove.hpp:39-50) */
 static mci_status set_move(orc_ctx *c, int srrd, int veclen, int ntypes, const int *typeEnds, double init)
 {
-    if (srrd != MCI_SRRD_UNIFORM) return fail(MCI_E_DOMAIN, "[oracle] only the uniform SRRD move is restated");
+    if (srrd < MCI_SRRD_UNIFORM || srrd > MCI_SRRD_FISHER) return fail(MCI_E_DOMAIN, "[createSRRDAllMove] Unhandled SRRDType enumerator.");
     if (ntypes < 1) ntypes = 1;
     if (ntypes > 1 && typeEnds == NULL) return fail(MCI_E_INVALID_ARGUMENT, "[checkTrialMoveSanity] ntypes>1 requires passed typeEnds array.");
     if (veclen > 0 && c->ndim % veclen != 0)
@@ -1014,8 +1108,10 @@ static mci_status set_move(orc_ctx *c, int srrd, int veclen, int ntypes, const i
     c->movetype = (veclen > 0) ? MCI_MOVE_VEC : MCI_MOVE_ALL;
     c->veclen = (veclen > 0) ? veclen : 0;
     c->nvecs = (veclen > 0) ? c->ndim / veclen : 0;
-    for (int64_t k = 0; k < c->W; ++k)
+    for (int64_t k = 0; k < c->W; ++k) {
         for (int i = 0; i < ntypes; ++i) c->w[k].steps[i] = init;
+        c->w[k].nsf = 0; /* a new move: fresh distribution objects */
+    }
     return MCI_OK;
 }
 
@@ -1435,7 +1531,7 @@ static double trial_move(const orc_ctx *c, walker *w)
         int xidx = 0;
         for (int t = 0; t < c->ntypes; ++t)
             while (xidx < c->typeEnds[t]) {
-                w->xnew[xidx] = w->xnew[xidx] + w->steps[t] * draw_sym(c, w);
+                w->xnew[xidx] = w->xnew[xidx] + w->steps[t] * srrd_draw(c, w);
                 ++xidx;
             }
         w->nchanged = c->ndim;
@@ -1449,7 +1545,7 @@ static double trial_move(const orc_ctx *c, walker *w)
         ++t;
     }
     for (int i = 0; i < c->veclen; ++i) {
-        w->xnew[xidx + i] = w->xnew[xidx + i] + w->steps[t] * draw_sym(c, w);
+        w->xnew[xidx + i] = w->xnew[xidx + i] + w->steps[t] * srrd_draw(c, w);
         w->changedIdx[i] = xidx + i;
     }
     w->nchanged = c->veclen;

@@ -110,7 +110,21 @@ def scenarios():
     S.append(dict(name="ut2_nopdf_gauss_x2", cfg=dict(ndim=3, seed=5649871, irange=([-5.] * 3, [5.] * 3),
                                                       obs=[(OBS_GAUSS_X2, [], 1, 1, 0, ESTIM_UNCORR)]),
                   calls=[dict(nmc=10000, find=0, decorr=0)]))
+    # every move distribution (SRRDType) with createSymRRD's defaults, as an all-coordinate move
+    # and as a single-coordinate move with two step types; the second call continues with the
+    # distributions' saved normals
+    for srrd, nm in enumerate(SRRD_NAMES):
+        if srrd == 0:
+            continue
+        for veclen, ntypes, te in ((0, 1, None), (1, 2, [1, 3])):
+            S.append(dict(name=f"move_{nm}_{'all' if veclen == 0 else 'vec'}",
+                          cfg=dict(ndim=3, seed=1000 + srrd, ndecorr=200, move=(srrd, veclen, ntypes, te),
+                                   pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0])], obs=[(OBS_X2, [], 16, 1, 0, ESTIM_UNCORR)]),
+                          calls=[dict(nmc=2048, find=1, decorr=1), dict(nmc=1024, find=0, decorr=0)]))
     return S
+
+
+SRRD_NAMES = ["uniform", "gaussian", "student", "cauchy", "exponential", "gamma", "weibull", "lognormal", "chisq", "fisher"]
 
 
 def run_scenario(s):
