@@ -51,6 +51,7 @@ typedef int mci_status;
 /* MoveType, Factories.hpp:108-113 */
 #define MCI_MOVE_ALL              0
 #define MCI_MOVE_VEC              1
+#define MCI_MOVE_MULTISTEP        2  /* MultiStepMove(ndim): ndim single-coordinate uniform steps, no own sampling function */
 
 /* SRRDType, Factories.hpp:119-133 (same order), each with the default parameters of createSymRRD
  * (TrialMoveInterface.hpp:107-183): uniform on (-1, 1), normal(0, 1), Student-t(1), Cauchy(0, 1),
@@ -170,6 +171,17 @@ mci_status mci_domain_is_finite(mci_ctx *ctx, int *finite);
  *      veclen > 0 = single-vector move (SRRDVecMove.hpp:75-96). typeEnds may be NULL if ntypes == 1. */
 mci_status mci_set_trial_move(mci_ctx *ctx, int srrd, int veclen, int ntypes, const int *type_ends);
 mci_status mci_set_trial_move_type(mci_ctx *ctx, int move_type);
+/* MultiStepMove(ndim, nsteps) (MultiStepMove.hpp:40-46, MultiStepMove.cpp:6-47): every trial move
+ * is nsteps Metropolis steps of the contained SRRD move (built from srrd, veclen, ntypes,
+ * type_ends as mci_set_trial_move does) under the move's own sampling functions, with acceptance
+ * factor oldPDF / newPDF of those; without own sampling functions every inner step is accepted.
+ * The step sizes are the contained move's.  The generic kernels run it. */
+mci_status mci_set_trial_move_multistep(mci_ctx *ctx, int nsteps, int srrd, int veclen, int ntypes, const int *type_ends);
+/* MultiStepMove::addSamplingFunction / clearSamplingFunctions: the move's own sampling
+ * functions, from the same catalogue as mci_add_sampling_function (MCI_E_RUNTIME if the trial
+ * move is not a MultiStepMove) */
+mci_status mci_add_multistep_sampling_function(mci_ctx *ctx, int kind, int ndim, const double *params, int nparams);
+mci_status mci_clear_multistep_sampling_functions(mci_ctx *ctx);
 mci_status mci_get_change_rate(mci_ctx *ctx, double *rate); /* TrialMoveInterface::getChangeRate */
 
 /* ---- sampling functions: MCI::addSamplingFunction src/MCIntegrator.cpp:486-492,

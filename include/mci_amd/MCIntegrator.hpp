@@ -27,7 +27,7 @@ namespace mci
 {
 // --- enums of include/mci/Factories.hpp (same order and names) ---
 enum class EstimatorType { Noop, Uncorrelated, Correlated, FCBlocker, MJBlocker };
-enum class MoveType { All, Vec };
+enum class MoveType { All, Vec, MultiStep };
 enum class SRRDType { Uniform, Gaussian, Student, Cauchy, Exponential, Gamma, Weibull, Lognormal, Chisq, Fisher };
 enum class RandomMode { MT19937_64 = MCI_RNG_MT19937_64, Philox = MCI_RNG_PHILOX };
 enum class Precision { FP64 = MCI_PREC_FP64, FP32 = MCI_PREC_FP32 };
@@ -64,6 +64,36 @@ struct SamplingFunction
     int ndim;
     std::vector<double> params;
     int getNDim() const { return ndim; }
+};
+
+// MultiStepMove (MultiStepMove.hpp): nsteps inner Metropolis steps of a contained SRRD move under
+// the move's own sampling functions, then one outer acceptance; handed to MCI::setTrialMove
+struct MultiStepMove
+{
+    int ndim, nsteps;
+    SRRDType srrd = SRRDType::Uniform; // contained move: default UniformVecMove(ndim, 1, 0.05)
+    int veclen = 1, ntypes = 1;
+    std::vector<int> typeEnds;
+    std::vector<SamplingFunction> pdfs;
+    MultiStepMove(int nd, int ns) : ndim(nd), nsteps(ns) {}
+    explicit MultiStepMove(int nd) : MultiStepMove(nd, nd) {}
+    void setNSteps(int n) { nsteps = n; }
+    int getNSteps() const { return nsteps; }
+    void setTrialMove(SRRDType s, int vlen = 0, int nt = 1, const int te[] = nullptr)
+    {
+        srrd = s;
+        veclen = vlen;
+        ntypes = nt;
+        typeEnds.assign(te, te ? te + nt : te);
+    }
+    void addSamplingFunction(const SamplingFunction &pdf)
+    {
+        if (pdf.getNDim() != ndim) {
+            throw std::invalid_argument("[MultiStepMove::addSamplingFunction] Passed sampling function's number of inputs is not equal to number of walkers.");
+        }
+        pdfs.push_back(pdf);
+    }
+    void clearSamplingFunctions() { pdfs.clear(); }
 };
 
 struct ObservableFunction
@@ -179,6 +209,12 @@ public:
     void setTrialMove(SRRDType srrd, int veclen = 0, int ntypes = 1, int typeEnds[] = nullptr)
     {
         check(mci_set_trial_move(_ctx, static_cast<int>(srrd), veclen, ntypes, typeEnds));
+    }
+    void setTrialMove(const MultiStepMove &m)
+    {
+        if (m.ndim != _ndim) { throw std::invalid_argument("[MCI::setTrialMove] Passed trial move's number of inputs is not equal to MCI's number of walkers."); }
+        check(mci_set_trial_move_multistep(_ctx, m.nsteps, static_cast<int>(m.srrd), m.veclen, m.ntypes, m.typeEnds.empty() ? nullptr : m.typeEnds.data()));
+        for (const auto &p : m.pdfs) { check(mci_add_multistep_sampling_function(_ctx, p.kind, p.ndim, p.params.data(), (int)p.params.size())); }
     }
 
     // --- observables (MCIntegrator.hpp:154-169) ---

@@ -92,6 +92,8 @@ struct mci_ctx
     std::vector<double> lb, ub;
     int movetype = MCI_MOVE_ALL, srrd = MCI_SRRD_UNIFORM, veclen = 0, nvecs = 0, ntypes = 1;
     std::vector<int> typeEnds;
+    int ms_on = 0, ms_nsteps = 0;     // MultiStepMove around the move above
+    std::vector<PdfHost> ms_pdfs;     // its own sampling functions
     std::vector<PdfHost> pdfs;
     std::vector<ObsHost> obs;
 
@@ -106,6 +108,7 @@ struct mci_ctx
     DBuf<uint64_t> mt, ctr, seeds;
     DBuf<int32_t> mtp;
     DBuf<double> nsv;     // [2][W] saved normals of the move's distribution
+    DBuf<double> ms_pold, ms_pnew, ms_x0; // MultiStepMove: proto values of its own sampling functions, start position
     DBuf<int32_t> nsf;    // [W] their availability bits
     DBuf<int64_t> acc, rej;
     DBuf<uint8_t> active;
@@ -204,6 +207,32 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     ch.srrd = c->srrd;
     ch.nsv = c->nsv.p;
     ch.nsf = c->nsf.p;
+    ch.ms_on = c->ms_on;
+    ch.ms_nsteps = c->ms_nsteps;
+    ch.ms_npdf = (int)c->ms_pdfs.size();
+    ch.ms_pold = c->ms_pold.p;
+    ch.ms_pnew = c->ms_pnew.p;
+    ch.ms_x0 = c->ms_x0.p;
+    {
+        int moff = 0;
+        for (size_t i = 0; i < c->ms_pdfs.size(); ++i) {
+            const PdfHost &p = c->ms_pdfs[i];
+            PdfDev &d = ch.ms_pdf[i];
+            d.kind = p.kind;
+            d.nproto = p.nproto;
+            d.off = moff;
+            d.K = p.K;
+            moff += p.nproto;
+            if (p.kind == MCI_PDF_GMM_ISO) {
+                d.a = nullptr;
+                d.gc = p.dev.p;
+                d.gb = p.dev.p + p.K;
+                d.gmu = p.dev.p + 2 * p.K;
+            }
+            else { d.a = p.params.empty() ? nullptr : p.dev.p; }
+        }
+        ch.ms_nproto = moff;
+    }
     ch.periodic = c->periodic ? 1 : 0;
     ch.lb = c->lbd.p;
     ch.ub = c->ubd.p;
@@ -288,7 +317,7 @@ float elapsed(mci_ctx *c, int a, int b)
 // the specialised kernels apply to this chain (observables are checked separately)
 bool fast_chain(const mci_ctx *c)
 {
-    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
+    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
                          c->pdfs.empty() ? 0 : c->pdfs[0].kind) &&
            !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
 }
@@ -772,6 +801,9 @@ mci_status mci_set_trial_move(mci_ctx *c, int srrd, int veclen, int ntypes, cons
     HIPCHK(c->nsv.ensure((size_t)2 * c->W));
     HIPCHK(c->nsf.ensure((size_t)c->W));
     HIPCHK(hipMemsetAsync(c->nsf.p, 0, sizeof(int32_t) * (size_t)c->W, c->stream));
+    c->ms_on = 0;
+    c->ms_nsteps = 0;
+    c->ms_pdfs.clear();
     return fill_rows(c, c->steps.p, 0, ntypes, 0.05); // DEFAULT_MRT2STEP (Factories.hpp:145)
 }
 
@@ -780,6 +812,7 @@ mci_status mci_set_trial_move_type(mci_ctx *c, int t)
     CHECKCTX(c);
     if (t == MCI_MOVE_ALL) { return mci_set_trial_move(c, MCI_SRRD_UNIFORM, 0, 1, nullptr); }
     if (t == MCI_MOVE_VEC) { return mci_set_trial_move(c, MCI_SRRD_UNIFORM, 1, 1, nullptr); }
+    if (t == MCI_MOVE_MULTISTEP) { return mci_set_trial_move_multistep(c, c->ndim, MCI_SRRD_UNIFORM, 1, 1, nullptr); } // MultiStepMove(ndim)
     return fail(MCI_E_DOMAIN, "[createMoveDefault] Unhandled MoveType enumerator.");
 }
 
@@ -787,19 +820,18 @@ mci_status mci_get_change_rate(mci_ctx *c, double *r)
 {
     CHECKCTX(c);
     *r = (c->movetype == MCI_MOVE_ALL) ? 1. : 1. / c->nvecs;
+    if (c->ms_on) { *r = std::fmin(1., *r * c->ms_nsteps); } // MultiStepMove.hpp:67
     return MCI_OK;
 }
 
-mci_status mci_add_sampling_function(mci_ctx *c, int kind, int ndim, const double *params, int nparams)
+} // extern "C"
+
+namespace
 {
-    CHECKCTX(c);
-    if (ndim != c->ndim) {
-        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
-    }
-    if ((int)c->pdfs.size() >= MAX_PDF) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d sampling functions", MAX_PDF); }
+// a sampling function of the catalogue: proto count and device parameter blob
+mci_status build_pdf(int kind, int ndim, const double *params, int nparams, PdfHost &p)
+{
     if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
-    TRY(set_current(c));
-    PdfHost p;
     p.kind = kind;
     p.params.assign(params, params + nparams);
     std::vector<double> blob;
@@ -836,8 +868,73 @@ mci_status mci_add_sampling_function(mci_ctx *c, int kind, int ndim, const doubl
         HIPCHK(p.dev.ensure(blob.size()));
         HIPCHK(hipMemcpy(p.dev.p, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    return MCI_OK;
+}
+
+mci_status ensure_ms_protos(mci_ctx *c)
+{
+    int n = 0;
+    for (const auto &p : c->ms_pdfs) { n += p.nproto; }
+    const size_t sz = (size_t)std::max(n, 1) * c->W;
+    HIPCHK(c->ms_pold.ensure(sz));
+    HIPCHK(c->ms_pnew.ensure(sz));
+    return MCI_OK;
+}
+} // namespace
+
+extern "C" {
+
+mci_status mci_add_sampling_function(mci_ctx *c, int kind, int ndim, const double *params, int nparams)
+{
+    CHECKCTX(c);
+    if (ndim != c->ndim) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
+    }
+    if ((int)c->pdfs.size() >= MAX_PDF) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d sampling functions", MAX_PDF); }
+    TRY(set_current(c));
+    PdfHost p;
+    TRY(build_pdf(kind, ndim, params, nparams, p));
     c->pdfs.push_back(std::move(p));
     return ensure_protos(c);
+}
+
+// MultiStepMove(ndim, nsteps) (MultiStepMove.hpp:40-46) whose contained move is the SRRD move
+// that mci_set_trial_move builds from the same arguments (MultiStepMove::setTrialMove,
+// MultiStepMove.cpp:51-58); it has no sampling function of its own yet
+mci_status mci_set_trial_move_multistep(mci_ctx *c, int nsteps, int srrd, int veclen, int ntypes, const int *te)
+{
+    CHECKCTX(c);
+    if (nsteps < 0) { return fail(MCI_E_INVALID_ARGUMENT, "the number of steps must not be negative"); }
+    TRY(mci_set_trial_move(c, srrd, veclen, ntypes, te));
+    HIPCHK(c->ms_x0.ensure((size_t)c->ndim * c->W));
+    c->ms_on = 1;
+    c->ms_nsteps = nsteps;
+    return ensure_ms_protos(c);
+}
+
+// MultiStepMove::addSamplingFunction (MultiStepMove.cpp:69-75)
+mci_status mci_add_multistep_sampling_function(mci_ctx *c, int kind, int ndim, const double *params, int nparams)
+{
+    CHECKCTX(c);
+    if (!c->ms_on) { return fail(MCI_E_RUNTIME, "the trial move is not a MultiStepMove"); }
+    if (ndim != c->ndim) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MultiStepMove::addSamplingFunction] Passed sampling function's number of inputs is not equal to number of walkers.");
+    }
+    if ((int)c->ms_pdfs.size() >= MAX_PDF) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d sampling functions", MAX_PDF); }
+    TRY(set_current(c));
+    PdfHost p;
+    TRY(build_pdf(kind, ndim, params, nparams, p));
+    c->ms_pdfs.push_back(std::move(p));
+    return ensure_ms_protos(c);
+}
+
+// MultiStepMove::clearSamplingFunctions (MultiStepMove.cpp:64-67)
+mci_status mci_clear_multistep_sampling_functions(mci_ctx *c)
+{
+    CHECKCTX(c);
+    if (!c->ms_on) { return fail(MCI_E_RUNTIME, "the trial move is not a MultiStepMove"); }
+    c->ms_pdfs.clear();
+    return MCI_OK;
 }
 
 mci_status mci_pop_sampling_function(mci_ctx *c)

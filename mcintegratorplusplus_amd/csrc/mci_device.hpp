@@ -64,6 +64,12 @@ struct Chain
     int srrd;                   // the move's distribution (MCI_SRRD_*)
     double *nsv;                // [2][W] saved second normals of the move's distribution
     int32_t *nsf;               // [W] bit s: nsv row s available
+    int ms_nsteps;              // MultiStepMove around the move above: inner steps (0: none)
+    int ms_on;
+    int ms_npdf, ms_nproto;     // its own sampling functions and their proto values
+    PdfDev ms_pdf[MAX_PDF];
+    double *ms_pold, *ms_pnew;  // [ms_nproto][W]
+    double *ms_x0;              // [ndim][W] the position the move starts from
     int periodic;
     const double *lb, *ub;      // [ndim]
     int npdf;
@@ -338,11 +344,11 @@ __device__ inline void pdf_proto(const Chain &c, const PdfDev &p, const double *
     }
 }
 
-__device__ inline double pdf_acceptance(const Chain &c, const PdfDev &p, int64_t w)
+__device__ inline double pdf_acceptance(const Chain &c, const PdfDev &p, const double *pold, const double *pnew, int64_t w)
 {
     const int64_t W = c.W;
-    const double *po = c.pold + (int64_t)p.off * W;
-    const double *pn = c.pnew + (int64_t)p.off * W;
+    const double *po = pold + (int64_t)p.off * W;
+    const double *pn = pnew + (int64_t)p.off * W;
     switch (p.kind) {
     case MCI_PDF_GAUSS_ISO:
         return exp(-pn[w] + po[w]);
@@ -362,12 +368,12 @@ __device__ inline double pdf_acceptance(const Chain &c, const PdfDev &p, int64_t
 }
 
 // SamplingFunctionInterface::computeAcceptance (+ the updatedAcceptance of each kind)
-__device__ inline double pdf_compute_acceptance(const Chain &c, const PdfDev &p, const Lane &L)
+__device__ inline double pdf_compute_acceptance(const Chain &c, const PdfDev &p, const Lane &L, double *pold, double *pnew)
 {
     const int64_t W = c.W, w = L.w;
     if (L.nchanged < c.ndim) {
-        double *po = c.pold + (int64_t)p.off * W;
-        double *pn = c.pnew + (int64_t)p.off * W;
+        double *po = pold + (int64_t)p.off * W;
+        double *pn = pnew + (int64_t)p.off * W;
         if (p.kind == MCI_PDF_GAUSS_DIAG || p.kind == MCI_PDF_EXP_DIAG) {
             double e = 0.0;
             for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) {
@@ -392,8 +398,26 @@ __device__ inline double pdf_compute_acceptance(const Chain &c, const PdfDev &p,
             return exp(pn[(int64_t)p.K * W + w] - po[(int64_t)p.K * W + w]);
         }
     }
-    pdf_proto(c, p, c.xnew, c.pnew, w);
-    return pdf_acceptance(c, p, w);
+    pdf_proto(c, p, c.xnew, pnew, w);
+    return pdf_acceptance(c, p, pold, pnew, w);
+}
+
+// samplingFunction(proto values) of each kind (exp of minus the proto sum, or of the mixture's log p)
+__device__ inline double pdf_value(const Chain &c, const PdfDev &p, const double *pv, int64_t w)
+{
+    const int64_t W = c.W;
+    const double *q = pv + (int64_t)p.off * W;
+    switch (p.kind) {
+    case MCI_PDF_GAUSS_ISO: return exp(-q[w]);
+    case MCI_PDF_GAUSS_DIAG:
+    case MCI_PDF_EXP_DIAG: {
+        double s = 0.0;
+        for (int i = 0; i < p.nproto; ++i) { s = s + q[(int64_t)i * W + w]; }
+        return exp(-s);
+    }
+    case MCI_PDF_GMM_ISO: return exp(q[(int64_t)p.K * W + w]);
+    }
+    return 0.0;
 }
 
 // --------------------------------------------------------------------------
@@ -420,7 +444,7 @@ __device__ __forceinline__ void wrap(const Chain &c, int i, double &x)
 // the Metropolis step
 // --------------------------------------------------------------------------
 // SRRDAllMove / SRRDVecMove trial moves (uniform SRRD)
-__device__ inline double trial_move(const Chain &c, Lane &L)
+__device__ inline double srrd_move(const Chain &c, Lane &L)
 {
     const int64_t W = c.W, w = L.w;
     if (c.movetype == MCI_MOVE_ALL) {
@@ -454,6 +478,37 @@ __device__ inline double trial_move(const Chain &c, Lane &L)
     return 1.;
 }
 
+// MultiStepMove::trialMove (MultiStepMove.cpp:6-47): ms_nsteps Metropolis steps of the contained
+// move under the move's own sampling functions (none: every step accepted), from xold; xnew ends at
+// the last position, xold is restored, every coordinate counts as changed, and the factor
+// oldPDF / newPDF of the own sampling functions is the move's acceptance factor
+__device__ inline double trial_move(const Chain &c, Lane &L)
+{
+    if (!c.ms_on) { return srrd_move(c, L); }
+    const int64_t W = c.W, w = L.w;
+    for (int i = 0; i < c.ndim; ++i) { c.ms_x0[(int64_t)i * W + w] = c.xold[(int64_t)i * W + w]; }
+    for (int p = 0; p < c.ms_npdf; ++p) { pdf_proto(c, c.ms_pdf[p], c.xold, c.ms_pold, w); }
+    for (int i = 0; i < c.ms_nproto; ++i) { c.ms_pnew[(int64_t)i * W + w] = c.ms_pold[(int64_t)i * W + w]; }
+    double oldPDF = 1.;
+    for (int p = 0; p < c.ms_npdf; ++p) { oldPDF *= pdf_value(c, c.ms_pdf[p], c.ms_pold, w); }
+    for (int s = 0; s < c.ms_nsteps; ++s) {
+        const double moveAcc = srrd_move(c, L);
+        double pdfAcc = 1.;
+        for (int p = 0; p < c.ms_npdf; ++p) { pdfAcc *= pdf_compute_acceptance(c, c.ms_pdf[p], L, c.ms_pold, c.ms_pnew); }
+        const bool acc = (draw_unit(c, L) <= pdfAcc * moveAcc);
+        double *pf = acc ? c.ms_pnew : c.ms_pold, *pt = acc ? c.ms_pold : c.ms_pnew;
+        double *xf = acc ? c.xnew : c.xold, *xt = acc ? c.xold : c.xnew;
+        for (int i = 0; i < c.ms_nproto; ++i) { pt[(int64_t)i * W + w] = pf[(int64_t)i * W + w]; }
+        for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) { xt[(int64_t)j * W + w] = xf[(int64_t)j * W + w]; }
+    }
+    double newPDF = 1.;
+    for (int p = 0; p < c.ms_npdf; ++p) { newPDF *= pdf_value(c, c.ms_pdf[p], c.ms_pold, w); }
+    for (int i = 0; i < c.ndim; ++i) { c.xold[(int64_t)i * W + w] = c.ms_x0[(int64_t)i * W + w]; }
+    L.nchanged = c.ndim;
+    L.cfirst = 0;
+    return oldPDF / newPDF;
+}
+
 // MCI::initializeSampling (MCIntegrator.cpp:252-271)
 __device__ inline void init_sampling(const Chain &c, Lane &L)
 {
@@ -478,7 +533,7 @@ __device__ inline void step_mrt2(const Chain &c, Lane &L)
         for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) { wrap(c, j, c.xnew[(int64_t)j * W + w]); }
     }
     double pdfAcc = 1.;
-    for (int p = 0; p < c.npdf; ++p) { pdfAcc = pdfAcc * pdf_compute_acceptance(c, c.pdf[p], L); }
+    for (int p = 0; p < c.npdf; ++p) { pdfAcc = pdfAcc * pdf_compute_acceptance(c, c.pdf[p], L, c.pold, c.pnew); }
     const double u = draw_unit(c, L);
     L.accepted = (u <= pdfAcc * moveAcc);
     if (L.accepted) { ++L.acc; } else { ++L.rej; }

@@ -234,6 +234,8 @@ typedef struct {
     uint32_t wid;
     double nsv[2];         /* saved second normals of the move's distribution (two normal generators at most) */
     int nsf;               /* bit s: nsv[s] available */
+    double *ms_pold, *ms_pnew; /* MultiStepMove: proto values of its own sampling functions */
+    double *ms_x0;         /* MultiStepMove: the position the move starts from */
 } walker;
 
 #define MAX_PDF 16
@@ -251,9 +253,12 @@ struct orc_ctx {
     /* domain */
     int periodic;
     double *lb, *ub;
-    /* trial move */
+    /* trial move (with a MultiStepMove: its contained move) */
     int movetype, srrd, veclen, nvecs, ntypes;
     int *typeEnds;
+    int ms_on, ms_nsteps;   /* MultiStepMove(ndim, nsteps) around the move above */
+    int ms_npdf, ms_nproto;
+    pdf_desc ms_pdf[MAX_PDF];
     /* functors */
     int npdf;
     pdf_desc pdf[MAX_PDF];
@@ -1060,6 +1065,22 @@ static void walker_alloc(orc_ctx *c, walker *w)
     w->steps = calloc(64, sizeof(double));
     w->pold = NULL;
     w->pnew = NULL;
+    w->ms_x0 = calloc((size_t)c->ndim, sizeof(double));
+    w->ms_pold = NULL;
+    w->ms_pnew = NULL;
+}
+
+static void rebuild_ms_protos(orc_ctx *c)
+{
+    int np = 0;
+    for (int i = 0; i < c->ms_npdf; ++i) np += c->ms_pdf[i].nproto;
+    c->ms_nproto = np;
+    for (int64_t k = 0; k < c->W; ++k) {
+        walker *w = &c->w[k];
+        free(w->ms_pold); free(w->ms_pnew);
+        w->ms_pold = calloc((size_t)np + 1, sizeof(double));
+        w->ms_pnew = calloc((size_t)np + 1, sizeof(double));
+    }
 }
 
 static void rebuild_protos(orc_ctx *c)
@@ -1084,6 +1105,8 @@ static void seed_walkers(orc_ctx *c)
         c->w[k].wid = (uint32_t)k;
     }
 }
+
+static void pdf_free(pdf_desc *p);
 
 /* trial move construction: createSRRDAllMove / createSRRDVecMove (Factories.hpp:160-227),
  * TypedMoveInterface ctor (TypedMoveInterface.hpp:27-39), SRRDVecMove ctor checks (SRRDVecMove.hpp:39-50) */
@@ -1112,6 +1135,9 @@ static mci_status set_move(orc_ctx *c, int srrd, int veclen, int ntypes, const i
         for (int i = 0; i < ntypes; ++i) c->w[k].steps[i] = init;
         c->w[k].nsf = 0; /* a new move: fresh distribution objects */
     }
+    c->ms_on = 0;
+    c->ms_nsteps = 0;
+    while (c->ms_npdf > 0) pdf_free(&c->ms_pdf[--c->ms_npdf]);
     return MCI_OK;
 }
 
@@ -1391,23 +1417,22 @@ mci_status orc_set_trial_move_type(orc_ctx *c, int mt)
 {
     if (mt == MCI_MOVE_ALL) return set_move(c, MCI_SRRD_UNIFORM, 0, 1, NULL, 0.05);
     if (mt == MCI_MOVE_VEC) return set_move(c, MCI_SRRD_UNIFORM, 1, 1, NULL, 0.05);
+    if (mt == MCI_MOVE_MULTISTEP) return orc_set_trial_move_multistep(c, c->ndim, MCI_SRRD_UNIFORM, 1, 1, NULL); /* MultiStepMove(ndim) */
     return fail(MCI_E_DOMAIN, "[createMoveDefault] Unhandled MoveType enumerator.");
 }
 
 mci_status orc_get_change_rate(orc_ctx *c, double *r)
 {
     *r = (c->movetype == MCI_MOVE_ALL) ? 1. : 1. / c->nvecs;
+    if (c->ms_on) *r = fmin(1., *r * c->ms_nsteps); /* MultiStepMove.hpp:67 */
     return MCI_OK;
 }
 
 /* ------------------------------------------------------------------------- */
 /* functor registration                                                       */
 /* ------------------------------------------------------------------------- */
-mci_status orc_add_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams)
+static mci_status make_pdf(int kind, int ndim, const double *params, int nparams, pdf_desc *out)
 {
-    if (ndim != c->ndim)
-        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
-    if (c->npdf >= MAX_PDF) return fail(MCI_E_INVALID_ARGUMENT, "too many sampling functions");
     pdf_desc p;
     memset(&p, 0, sizeof p);
     p.kind = kind;
@@ -1447,8 +1472,55 @@ mci_status orc_add_sampling_function(orc_ctx *c, int kind, int ndim, const doubl
         free(p.params);
         return fail(MCI_E_DOMAIN, "unknown sampling-function kind %d", kind);
     }
-    c->pdf[c->npdf++] = p;
+    *out = p;
+    return MCI_OK;
+}
+
+mci_status orc_add_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams)
+{
+    if (ndim != c->ndim)
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
+    if (c->npdf >= MAX_PDF) return fail(MCI_E_INVALID_ARGUMENT, "too many sampling functions");
+    mci_status st = make_pdf(kind, ndim, params, nparams, &c->pdf[c->npdf]);
+    if (st != MCI_OK) return st;
+    c->npdf++;
     rebuild_protos(c);
+    return MCI_OK;
+}
+
+/* MultiStepMove(ndim, nsteps) (MultiStepMove.hpp:40-46) with setTrialMove(the SRRD move built as
+ * mci_set_trial_move does) (MultiStepMove.cpp:51-58); no own sampling functions yet */
+mci_status orc_set_trial_move_multistep(orc_ctx *c, int nsteps, int srrd, int veclen, int ntypes, const int *te)
+{
+    if (nsteps < 0) return fail(MCI_E_INVALID_ARGUMENT, "the number of steps must not be negative");
+    mci_status st = set_move(c, srrd, veclen, ntypes, te, 0.05);
+    if (st != MCI_OK) return st;
+    c->ms_on = 1;
+    c->ms_nsteps = nsteps;
+    rebuild_ms_protos(c);
+    return MCI_OK;
+}
+
+/* MultiStepMove::addSamplingFunction (MultiStepMove.cpp:69-75) */
+mci_status orc_add_multistep_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams)
+{
+    if (!c->ms_on) return fail(MCI_E_RUNTIME, "the trial move is not a MultiStepMove");
+    if (ndim != c->ndim)
+        return fail(MCI_E_INVALID_ARGUMENT, "[MultiStepMove::addSamplingFunction] Passed sampling function's number of inputs is not equal to number of walkers.");
+    if (c->ms_npdf >= MAX_PDF) return fail(MCI_E_INVALID_ARGUMENT, "too many sampling functions");
+    mci_status st = make_pdf(kind, ndim, params, nparams, &c->ms_pdf[c->ms_npdf]);
+    if (st != MCI_OK) return st;
+    c->ms_npdf++;
+    rebuild_ms_protos(c);
+    return MCI_OK;
+}
+
+/* MultiStepMove::clearSamplingFunctions (MultiStepMove.cpp:64-67) */
+mci_status orc_clear_multistep_sampling_functions(orc_ctx *c)
+{
+    if (!c->ms_on) return fail(MCI_E_RUNTIME, "the trial move is not a MultiStepMove");
+    while (c->ms_npdf > 0) pdf_free(&c->ms_pdf[--c->ms_npdf]);
+    rebuild_ms_protos(c);
     return MCI_OK;
 }
 
@@ -1525,7 +1597,7 @@ mci_status orc_get_nobsdim(orc_ctx *c, int *n)
 /* Metropolis step                                                            */
 /* ------------------------------------------------------------------------- */
 /* SRRDAllMove::trialMove SRRDAllMove.hpp:67-80, SRRDVecMove::trialMove SRRDVecMove.hpp:75-96 */
-static double trial_move(const orc_ctx *c, walker *w)
+static double srrd_move(const orc_ctx *c, walker *w)
 {
     if (c->movetype == MCI_MOVE_ALL) {
         int xidx = 0;
@@ -1550,6 +1622,73 @@ static double trial_move(const orc_ctx *c, walker *w)
     }
     w->nchanged = c->veclen;
     return 1.;
+}
+
+/* samplingFunction(protovalues) of each kind (the forms of oracle/ref_harness.cpp's classes) */
+static double pdf_value(const pdf_desc *p, const double *pv)
+{
+    switch (p->kind) {
+    case MCI_PDF_GAUSS_ISO: return exp(-pv[0]);
+    case MCI_PDF_GAUSS_DIAG:
+    case MCI_PDF_EXP_DIAG: {
+        double s = 0.0;
+        for (int i = 0; i < p->nproto; ++i) s = s + pv[i];
+        return exp(-s);
+    }
+    case MCI_PDF_GMM_ISO: return exp(pv[p->K]);
+    }
+    return 0.0;
+}
+
+/* MultiStepMove::trialMove (MultiStepMove.cpp:6-47): nsteps Metropolis steps of the contained
+ * move under the move's own sampling functions (none: every step accepted), from xold; xnew ends
+ * at the last position, xold is restored, all coordinates count as changed, and the factor
+ * oldPDF / newPDF of the own sampling functions is returned */
+static double trial_move(const orc_ctx *c, walker *w)
+{
+    if (!c->ms_on) return srrd_move(c, w);
+    const int nd = c->ndim;
+    memcpy(w->ms_x0, w->xold, sizeof(double) * (size_t)nd);
+    int off = 0;
+    for (int p = 0; p < c->ms_npdf; ++p) { /* initializeProtoValues(xold) */
+        pdf_proto(&c->ms_pdf[p], w->xold, w->ms_pnew + off);
+        memcpy(w->ms_pold + off, w->ms_pnew + off, sizeof(double) * (size_t)c->ms_pdf[p].nproto);
+        off += c->ms_pdf[p].nproto;
+    }
+    double oldPDF = 1.;
+    off = 0;
+    for (int p = 0; p < c->ms_npdf; ++p) {
+        oldPDF *= pdf_value(&c->ms_pdf[p], w->ms_pold + off);
+        off += c->ms_pdf[p].nproto;
+    }
+    for (int s = 0; s < c->ms_nsteps; ++s) {
+        const double moveAcc = srrd_move(c, w);
+        double pdfAcc = 1.;
+        off = 0;
+        for (int p = 0; p < c->ms_npdf; ++p) {
+            pdfAcc *= pdf_compute_acceptance(&c->ms_pdf[p], w, w->ms_pold + off, w->ms_pnew + off);
+            off += c->ms_pdf[p].nproto;
+        }
+        const int acc = (draw_unit(c, w) <= pdfAcc * moveAcc);
+        if (acc) {
+            memcpy(w->ms_pold, w->ms_pnew, sizeof(double) * (size_t)c->ms_nproto);
+            memcpy(w->xold, w->xnew, sizeof(double) * (size_t)nd);
+        }
+        else {
+            memcpy(w->ms_pnew, w->ms_pold, sizeof(double) * (size_t)c->ms_nproto);
+            memcpy(w->xnew, w->xold, sizeof(double) * (size_t)nd);
+        }
+    }
+    double newPDF = 1.;
+    off = 0;
+    for (int p = 0; p < c->ms_npdf; ++p) {
+        newPDF *= pdf_value(&c->ms_pdf[p], w->ms_pold + off);
+        off += c->ms_pdf[p].nproto;
+    }
+    memcpy(w->xold, w->ms_x0, sizeof(double) * (size_t)nd);
+    w->nchanged = nd;
+    w->accepted = 0;
+    return oldPDF / newPDF;
 }
 
 /* MCI::initializeSampling MCIntegrator.cpp:252-271 (+ WalkerState::initialize WalkerState.hpp:41-49,

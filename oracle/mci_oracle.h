@@ -56,6 +56,9 @@ mci_status orc_domain_is_finite(orc_ctx *c, int *finite);
 
 mci_status orc_set_trial_move(orc_ctx *c, int srrd, int veclen, int ntypes, const int *type_ends);
 mci_status orc_set_trial_move_type(orc_ctx *c, int move_type);
+mci_status orc_set_trial_move_multistep(orc_ctx *c, int nsteps, int srrd, int veclen, int ntypes, const int *type_ends);
+mci_status orc_add_multistep_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams);
+mci_status orc_clear_multistep_sampling_functions(orc_ctx *c);
 mci_status orc_get_change_rate(orc_ctx *c, double *rate);
 
 mci_status orc_add_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams);

@@ -11,6 +11,8 @@
 // Input: a line-oriented script on stdin.  Output: one JSON object per result line;
 // doubles are printed as C99 hex-floats ("%a") so they round-trip bit-exactly.
 #include "mci/MCIntegrator.hpp"
+#include "mci/Factories.hpp"
+#include "mci/MultiStepMove.hpp"
 #include "mci/Estimators.hpp"
 #include "mci/BlockAccumulator.hpp"
 #include "mci/FullAccumulator.hpp"
@@ -364,6 +366,7 @@ struct Trace
 int main()
 {
     std::unique_ptr<mci::MCI> mci;
+    std::unique_ptr<mci::MultiStepMove> msm; // a MultiStepMove being assembled (msmove, mspdf, msapply)
     int ndim = 0;
     Trace tr;
     std::string line;
@@ -393,6 +396,20 @@ int main()
                 mci->setTrialMove(static_cast<mci::SRRDType>(srrd), veclen, ntypes, ntypes > 1 ? te.data() : nullptr);
             }
             else if (cmd == "movetype") { int t; is >> t; mci->setTrialMove(static_cast<mci::MoveType>(t)); }
+            else if (cmd == "msmove") {
+                // MultiStepMove(ndim, nsteps) around an SRRD move built as MCI::setTrialMove(SRRDType, ...) does
+                int nsteps, srrd, veclen, ntypes;
+                is >> nsteps >> srrd >> veclen >> ntypes;
+                std::vector<int> te(ntypes > 0 ? ntypes : 1);
+                for (int i = 0; i < ntypes; ++i) { is >> te[i]; }
+                const auto st = static_cast<mci::SRRDType>(srrd);
+                int *tep = ntypes > 1 ? te.data() : nullptr;
+                auto sub = veclen > 0 ? mci::createSRRDVecMove(st, ndim / veclen, veclen, ntypes, tep) : mci::createSRRDAllMove(st, ndim, ntypes, tep);
+                msm.reset(new mci::MultiStepMove(ndim, nsteps));
+                msm->setTrialMove(*sub);
+            }
+            else if (cmd == "mspdf") { int kind, np; is >> kind >> np; auto p = readDoubles(is, np); msm->addSamplingFunction(*makePDF(ndim, kind, p)); }
+            else if (cmd == "msapply") { mci->setTrialMove(std::unique_ptr<mci::TrialMoveInterface>(msm.release())); }
             else if (cmd == "pdf") { int kind, np; is >> kind >> np; auto p = readDoubles(is, np); mci->addSamplingFunction(makePDF(ndim, kind, p)); }
             else if (cmd == "poppdf") { mci->popSamplingFunction(); }
             else if (cmd == "obs") {

@@ -207,6 +207,37 @@ static void walkers()
     std::printf("walkers ok: <x0^2> = %.6f +- %.6f over %lld walkers\n", average[0], error[0], (long long)W);
 }
 
+static void multistep()
+{
+    // a MultiStepMove of 4 Gaussian single-coordinate steps under a cheap own Gaussian, around
+    // exp(-sum x^2): unbiased, and the change rate is the contained move's times the steps
+    const int64_t W = 128;
+    MCI mci(3, W);
+    mci.setSeed(77);
+    mci.addSamplingFunction(Gauss(3));
+    MultiStepMove ms(3, 4);
+    ms.setTrialMove(SRRDType::Gaussian, 1);
+    ms.addSamplingFunction(AnisotropicGaussPDF({1.5, 1.5, 1.5}));
+    mci.setTrialMove(ms);
+    double rate = 0.;
+    check(mci_get_change_rate(mci.handle(), &rate));
+    CHECK(std::fabs(rate - 1.) < 1e-15); // min(1, 4 / 3)
+    mci.addObservable(X2(3), 16, 1, false, EstimatorType::Uncorrelated);
+    double average[3], error[3];
+    mci.integrate(16 * 128, average, error);
+    for (int i = 0; i < 3; ++i) { CHECK(std::fabs(average[i] - 0.5) < 4. * error[i]); }
+    bool threw = false;
+    try {
+        MultiStepMove bad(3, 2);
+        bad.addSamplingFunction(Gauss(2));
+    }
+    catch (const std::invalid_argument &) {
+        threw = true;
+    }
+    CHECK(threw);
+    std::printf("multistep ok: <x0^2> = %.6f +- %.6f\n", average[0], error[0]);
+}
+
 int main()
 {
     int ndev = 0;
@@ -222,6 +253,7 @@ int main()
     ut4();
     exceptions();
     walkers();
+    multistep();
     std::printf("all %d checks passed\n", g_checks);
     return 0;
 }

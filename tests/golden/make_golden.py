@@ -121,6 +121,23 @@ def scenarios():
                           cfg=dict(ndim=3, seed=1000 + srrd, ndecorr=200, move=(srrd, veclen, ntypes, te),
                                    pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0])], obs=[(OBS_X2, [], 16, 1, 0, ESTIM_UNCORR)]),
                           calls=[dict(nmc=2048, find=1, decorr=1), dict(nmc=1024, find=0, decorr=0)]))
+    # MultiStepMove: the default one (ndim uniform single-coordinate steps, no own sampling function),
+    # one with a cheap Gaussian of its own around a Gaussian-mixture target, and one around a
+    # Gaussian move of all coordinates with two own sampling functions in a periodic box
+    S.append(dict(name="multistep_default", cfg=dict(ndim=3, seed=4711, ndecorr=100, multistep=(3, 0, 1, 1, None, []),
+                                                      pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0])], obs=[(OBS_X2, [], 16, 1, 0, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=1024, find=1, decorr=1), dict(nmc=512, find=0, decorr=0)]))
+    # (a fixed step: the calibration of a multi-step move whose own density rejects every inner
+    # step sees acceptance 1 and grows the step without end, in the reference as here)
+    S.append(dict(name="multistep_own_pdf", cfg=dict(ndim=2, seed=815, ndecorr=100, step=0.8,
+                                                      multistep=(4, 0, 1, 2, [1, 2], [(PDF_GAUSS_DIAG, [0.3, 0.3])]),
+                                                      pdfs=[(PDF_GMM_ISO, [2, 0.3, 0.7, 0.5, 1.2, 0.2, 0.1, -0.5, 1.0])],
+                                                      obs=[(OBS_X2, [], 16, 1, 0, ESTIM_UNCORR), (OBS_INDICATOR, [0], 16, 1, 0, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=2048, find=0, decorr=1), dict(nmc=512, find=0, decorr=0)]))
+    S.append(dict(name="multistep_gauss_two_pdfs_pbc", cfg=dict(ndim=3, seed=99, ndecorr=100, irange=([-2.] * 3, [2.] * 3),
+                                                                 multistep=(2, 1, 0, 1, None, [(PDF_GAUSS_ISO, []), (PDF_EXP_DIAG, [0.5, 0.5, 0.5])]),
+                                                                 pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0])], obs=[(OBS_X2_SUM, [3.0], 16, 1, 0, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=1024, find=1, decorr=1)]))
     return S
 
 

@@ -66,6 +66,9 @@ SIGS = {
     "domain_is_finite": [_IP],
     "set_trial_move": [_I, _I, _I, _IP],
     "set_trial_move_type": [_I],
+    "set_trial_move_multistep": [_I, _I, _I, _I, _IP],
+    "add_multistep_sampling_function": [_I, _I, _DP, _I],
+    "clear_multistep_sampling_functions": [],
     "get_change_rate": [_DP],
     "add_sampling_function": [_I, _I, _DP, _I],
     "pop_sampling_function": [],
@@ -229,6 +232,17 @@ class Chain:
 
     def setTrialMoveType(self, t): self.call("set_trial_move_type", t)
 
+    def setTrialMoveMultistep(self, nsteps, srrd=SRRD_UNIFORM, veclen=1, ntypes=1, typeEnds=None):
+        te = None
+        if typeEnds is not None:
+            arr = (C.c_int * len(typeEnds))(*typeEnds)
+            te = C.cast(arr, _IP)
+        self.call("set_trial_move_multistep", nsteps, srrd, veclen, ntypes, te)
+
+    def addMultistepSamplingFunction(self, kind, params=()):
+        p = f64(params)
+        self.call("add_multistep_sampling_function", kind, self.ndim, dptr(p) if p.size else None, p.size)
+
     def getChangeRate(self):
         v = _D()
         self.call("get_change_rate", C.byref(v))
@@ -340,6 +354,12 @@ def script_setup(cfg):
     if "move" in cfg:
         srrd, veclen, ntypes, te = cfg["move"]
         L.append(f"move {srrd} {veclen} {ntypes} " + " ".join(str(t) for t in (te or [])))
+    if "multistep" in cfg:
+        nsteps, srrd, veclen, ntypes, te, mpdfs = cfg["multistep"]
+        L.append(f"msmove {nsteps} {srrd} {veclen} {ntypes} " + " ".join(str(t) for t in (te or [])))
+        for kind, params in mpdfs:
+            L.append(f"mspdf {kind} {len(params)} " + " ".join(fmt(p) for p in params))
+        L.append("msapply")
     if "irange" in cfg:
         lb, ub = cfg["irange"]
         L.append("irange " + " ".join(fmt(v) for v in lb) + " " + " ".join(fmt(v) for v in ub))
@@ -369,6 +389,11 @@ def chain_setup(prefix, cfg, nwalkers=1):
     if "move" in cfg:
         srrd, veclen, ntypes, te = cfg["move"]
         c.setTrialMove(srrd, veclen, ntypes, te)
+    if "multistep" in cfg:
+        nsteps, srrd, veclen, ntypes, te, mpdfs = cfg["multistep"]
+        c.setTrialMoveMultistep(nsteps, srrd, veclen, ntypes, te)
+        for kind, params in mpdfs:
+            c.addMultistepSamplingFunction(kind, params)
     if "irange" in cfg:
         c.setIRange(*cfg["irange"])
     for kind, params in cfg.get("pdfs", []):
