@@ -182,6 +182,15 @@ mci_status mci_set_trial_move_multistep(mci_ctx *ctx, int nsteps, int srrd, int 
  * move is not a MultiStepMove) */
 mci_status mci_add_multistep_sampling_function(mci_ctx *ctx, int kind, int ndim, const double *params, int nparams);
 mci_status mci_clear_multistep_sampling_functions(mci_ctx *ctx);
+/* MCI::storeObservablesOnFile / storeWalkerPositionsOnFile and their clear calls
+ * (MCIntegrator.cpp:495-542): during the main sampling of each integrate, every freq steps (the
+ * step index first, then the last observable values, or the position), in the reference's stream
+ * format.  One file per walker: the path itself for one walker, path.<walker> for several.  The
+ * values are recorded on the device (at most 16 GB) and written when the sampling ends; FP64 only. */
+mci_status mci_store_observables_on_file(mci_ctx *ctx, const char *path, int freq);
+mci_status mci_clear_observable_file(mci_ctx *ctx);
+mci_status mci_store_walker_positions_on_file(mci_ctx *ctx, const char *path, int freq);
+mci_status mci_clear_walker_file(mci_ctx *ctx);
 mci_status mci_get_change_rate(mci_ctx *ctx, double *rate); /* TrialMoveInterface::getChangeRate */
 
 /* ---- sampling functions: MCI::addSamplingFunction src/MCIntegrator.cpp:486-492,

@@ -210,6 +210,12 @@ public:
     {
         check(mci_set_trial_move(_ctx, static_cast<int>(srrd), veclen, ntypes, typeEnds));
     }
+    // --- file output (MCIntegrator.hpp:192-196) ---
+    void storeObservablesOnFile(const std::string &filepath, int freq) { check(mci_store_observables_on_file(_ctx, filepath.c_str(), freq)); }
+    void clearObservableFile() { check(mci_clear_observable_file(_ctx)); }
+    void storeWalkerPositionsOnFile(const std::string &filepath, int freq) { check(mci_store_walker_positions_on_file(_ctx, filepath.c_str(), freq)); }
+    void clearWalkerFile() { check(mci_clear_walker_file(_ctx)); }
+
     void setTrialMove(const MultiStepMove &m)
     {
         if (m.ndim != _ndim) { throw std::invalid_argument("[MCI::setTrialMove] Passed trial move's number of inputs is not equal to MCI's number of walkers."); }

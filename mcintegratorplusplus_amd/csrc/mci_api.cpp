@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -93,6 +94,8 @@ struct mci_ctx
     int movetype = MCI_MOVE_ALL, srrd = MCI_SRRD_UNIFORM, veclen = 0, nvecs = 0, ntypes = 1;
     std::vector<int> typeEnds;
     int ms_on = 0, ms_nsteps = 0;     // MultiStepMove around the move above
+    std::string obsfile, wlkfile;     // file output of the main sampling (empty: off)
+    int64_t obsfreq = 0, wlkfreq = 0;
     std::vector<PdfHost> ms_pdfs;     // its own sampling functions
     std::vector<PdfHost> pdfs;
     std::vector<ObsHost> obs;
@@ -109,6 +112,7 @@ struct mci_ctx
     DBuf<int32_t> mtp;
     DBuf<double> nsv;     // [2][W] saved normals of the move's distribution
     DBuf<double> ms_pold, ms_pnew, ms_x0; // MultiStepMove: proto values of its own sampling functions, start position
+    DBuf<double> rec_obs, rec_x;          // file output: recorded observable values and positions
     DBuf<int32_t> nsf;    // [W] their availability bits
     DBuf<int64_t> acc, rej;
     DBuf<uint8_t> active;
@@ -871,6 +875,44 @@ mci_status build_pdf(int kind, int ndim, const double *params, int nparams, PdfH
     return MCI_OK;
 }
 
+// the files of MCI::storeObservables / storeWalkerPositions (MCIntegrator.cpp:509-516, 533-542),
+// one per walker (the file name itself for one walker, name.<walker> for several), written with the
+// same stream formatting from the values the sampling kernel recorded
+mci_status write_files(mci_ctx *c, int64_t nmc, int nd, bool rec_obs, bool rec_x)
+{
+    const int64_t W = c->W;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto name = [&](const std::string &p, int64_t w) { return W == 1 ? p : p + "." + std::to_string(w); };
+    if (rec_obs) {
+        const int64_t no = (nmc - 1) / c->obsfreq + 1, nc = std::max(nd, 1);
+        std::vector<double> h((size_t)no * nc * W);
+        HIPCHK(hipMemcpy(h.data(), c->rec_obs.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t w = 0; w < W; ++w) {
+            std::ofstream f(name(c->obsfile, w));
+            for (int64_t r = 0; r < no; ++r) {
+                f << r * c->obsfreq;
+                for (int j = 0; j < nd; ++j) { f << " " << h[((size_t)r * nc + j) * W + w]; }
+                f << " ";
+                f << std::endl;
+            }
+        }
+    }
+    if (rec_x) {
+        const int64_t nx = (nmc - 1) / c->wlkfreq + 1;
+        std::vector<double> h((size_t)nx * c->ndim * W);
+        HIPCHK(hipMemcpy(h.data(), c->rec_x.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t w = 0; w < W; ++w) {
+            std::ofstream f(name(c->wlkfile, w));
+            for (int64_t r = 0; r < nx; ++r) {
+                f << r * c->wlkfreq;
+                for (int j = 0; j < c->ndim; ++j) { f << "   " << h[((size_t)r * c->ndim + j) * W + w]; }
+                f << std::endl;
+            }
+        }
+    }
+    return MCI_OK;
+}
+
 mci_status ensure_ms_protos(mci_ctx *c)
 {
     int n = 0;
@@ -910,6 +952,42 @@ mci_status mci_set_trial_move_multistep(mci_ctx *c, int nsteps, int srrd, int ve
     c->ms_on = 1;
     c->ms_nsteps = nsteps;
     return ensure_ms_protos(c);
+}
+
+// MCI::storeObservablesOnFile / clearObservableFile / storeWalkerPositionsOnFile / clearWalkerFile
+// (MCIntegrator.cpp:495-531)
+mci_status mci_store_observables_on_file(mci_ctx *c, const char *path, int freq)
+{
+    CHECKCTX(c);
+    if (!path || freq < 1) { return fail(MCI_E_INVALID_ARGUMENT, "need a file path and a frequency of at least 1"); }
+    c->obsfile = path;
+    c->obsfreq = freq;
+    return MCI_OK;
+}
+
+mci_status mci_clear_observable_file(mci_ctx *c)
+{
+    CHECKCTX(c);
+    c->obsfile.clear();
+    c->obsfreq = 0;
+    return MCI_OK;
+}
+
+mci_status mci_store_walker_positions_on_file(mci_ctx *c, const char *path, int freq)
+{
+    CHECKCTX(c);
+    if (!path || freq < 1) { return fail(MCI_E_INVALID_ARGUMENT, "need a file path and a frequency of at least 1"); }
+    c->wlkfile = path;
+    c->wlkfreq = freq;
+    return MCI_OK;
+}
+
+mci_status mci_clear_walker_file(mci_ctx *c)
+{
+    CHECKCTX(c);
+    c->wlkfile.clear();
+    c->wlkfreq = 0;
+    return MCI_OK;
 }
 
 // MultiStepMove::addSamplingFunction (MultiStepMove.cpp:69-75)
@@ -1057,6 +1135,21 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
             return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
         }
     }
+    // ---- file output of the main sampling (MCIntegrator.cpp:61-65, 495-542): the generic kernel
+    //      records the values every freq steps, the host writes the files afterwards ----
+    const bool rec_obs = nmc > 0 && !c->obsfile.empty(), rec_x = nmc > 0 && !c->wlkfile.empty();
+    if (rec_obs || rec_x) {
+        if (c->prec == MCI_PREC_FP32) { return fail(MCI_E_INVALID_ARGUMENT, "file output is written from the FP64 chain"); }
+        fast_main = false;
+        const int64_t no = rec_obs ? (nmc - 1) / c->obsfreq + 1 : 0, nx = rec_x ? (nmc - 1) / c->wlkfreq + 1 : 0;
+        const double bytes = 8.0 * (double)W * ((double)no * std::max(nd, 1) + (double)nx * c->ndim);
+        if (bytes > 16e9) {
+            return fail(MCI_E_INVALID_ARGUMENT, "file output would record %.3g GB on the device (at most 16 GB): raise the frequency or use fewer walkers",
+                        bytes / 1e9);
+        }
+        if (rec_obs) { HIPCHK(c->rec_obs.ensure((size_t)no * std::max(nd, 1) * W)); }
+        if (rec_x) { HIPCHK(c->rec_x.ensure((size_t)nx * c->ndim * W)); }
+    }
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     // ---- calibration (MCIntegrator.cpp:49-51) ----
     if (!c->pdfs.empty() && do_find && c->ntypes > 0) {
@@ -1086,9 +1179,19 @@ mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error
     // ---- sample (MCIntegrator.cpp:58-62) ----
     for (auto &d : od) { c->nstore_last.push_back(d.nstore); }
     Chain ch = make_chain(c, &od);
+    if (rec_obs) {
+        ch.rec_obs = c->rec_obs.p;
+        ch.rec_fobs = c->obsfreq;
+        ch.rec_nobsdim = nd;
+    }
+    if (rec_x) {
+        ch.rec_x = c->rec_x.p;
+        ch.rec_fx = c->wlkfreq;
+    }
     if (fast_main) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
     else { HIPCHK(launch_sample(ch, nmc, 1, nullptr, c->stream)); }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    if (rec_obs || rec_x) { TRY(write_files(c, nmc, nd, rec_obs, rec_x)); }
     // ---- estimate (ObservableContainer.cpp:104-111), volume scaling (:71-77), combine ----
     double vol = 1.0;
     if (c->pdfs.empty()) {

@@ -70,6 +70,10 @@ struct Chain
     PdfDev ms_pdf[MAX_PDF];
     double *ms_pold, *ms_pnew;  // [ms_nproto][W]
     double *ms_x0;              // [ndim][W] the position the move starts from
+    double *rec_obs;            // [nrec][rec_nobsdim][W] observable values every rec_fobs steps (file output) or null
+    double *rec_x;              // [nrec][ndim][W] positions every rec_fx steps or null
+    int64_t rec_fobs, rec_fx;
+    int rec_nobsdim;
     int periodic;
     const double *lb, *ub;      // [ndim]
     int npdf;

@@ -42,7 +42,19 @@ __global__ void k_sample(Chain c, int64_t nsteps, int accumulate_obs, const uint
     for (int64_t s = 0; s < nsteps; ++s) {
         if (c.npdf > 0) { step_mrt2(c, L); }
         else { step_random(c, L); }
-        if (accumulate_obs) { accumulate(c, L, s, dirty); }
+        if (accumulate_obs) {
+            accumulate(c, L, s, dirty);
+            // file output (MCIntegrator.cpp:315-316, 509-516, 533-542): the last observable values
+            // and the position after steps 0, f, 2f, ...
+            if (c.rec_obs != nullptr && s % c.rec_fobs == 0) {
+                const int64_t r = s / c.rec_fobs;
+                for (int j = 0; j < c.rec_nobsdim; ++j) { c.rec_obs[(r * c.rec_nobsdim + j) * c.W + w] = c.obsval[(int64_t)j * c.W + w]; }
+            }
+            if (c.rec_x != nullptr && s % c.rec_fx == 0) {
+                const int64_t r = s / c.rec_fx;
+                for (int i = 0; i < c.ndim; ++i) { c.rec_x[(r * c.ndim + i) * c.W + w] = c.xold[(int64_t)i * c.W + w]; }
+            }
+        }
     }
     if (accumulate_obs) { finalize_accu(c, w, nsteps); }
     store_lane(c, L);

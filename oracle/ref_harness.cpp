@@ -409,6 +409,8 @@ int main()
                 msm->setTrialMove(*sub);
             }
             else if (cmd == "mspdf") { int kind, np; is >> kind >> np; auto p = readDoubles(is, np); msm->addSamplingFunction(*makePDF(ndim, kind, p)); }
+            else if (cmd == "obsfile") { std::string path; int freq; is >> path >> freq; mci->storeObservablesOnFile(path, freq); }
+            else if (cmd == "wlkfile") { std::string path; int freq; is >> path >> freq; mci->storeWalkerPositionsOnFile(path, freq); }
             else if (cmd == "msapply") { mci->setTrialMove(std::unique_ptr<mci::TrialMoveInterface>(msm.release())); }
             else if (cmd == "pdf") { int kind, np; is >> kind >> np; auto p = readDoubles(is, np); mci->addSamplingFunction(makePDF(ndim, kind, p)); }
             else if (cmd == "poppdf") { mci->popSamplingFunction(); }

@@ -159,6 +159,26 @@ def run_scenario(s):
     return recs
 
 
+def file_scenarios():
+    """The observable and walker files of the main sampling (MCIntegrator.cpp:495-542) as the
+    reference writes them (calibration and decorrelation on: they write nothing)."""
+    import tempfile
+    F = []
+    for name, cfg, nmc, fo, fw in [
+        ("files_gauss3", dict(ndim=3, seed=1337, pdfs=[(PDF_GAUSS_ISO, [])],
+                              obs=[(OBS_X2, [], 1, 1, 0, ESTIM_UNCORR), (OBS_X2_SUM, [3.0], 4, 2, 0, ESTIM_UNCORR)]), 400, 7, 5),
+        ("files_vec_gmm", dict(ndim=2, seed=99, ndecorr=100, move=(0, 1, 1, None),
+                               pdfs=[(PDF_GMM_ISO, [2, 0.3, 0.7, 0.5, 1.2, 0.2, 0.1, -0.5, 1.0])],
+                               obs=[(OBS_X, [], 1, 3, 0, ESTIM_UNCORR), (OBS_EXP_X2, [], 1, 1, 0, ESTIM_UNCORR)]), 300, 1, 3),
+    ]:
+        with tempfile.TemporaryDirectory() as d:
+            L = script_setup(cfg) + [f"obsfile {d}/obs.txt {fo}", f"wlkfile {d}/wlk.txt {fw}", f"integrate {nmc} 1 1"]
+            ref_run(L)
+            with open(f"{d}/obs.txt") as fobs, open(f"{d}/wlk.txt") as fwlk:
+                F.append(dict(name=name, cfg=cfg, nmc=nmc, freq_obs=fo, freq_wlk=fw, obs=fobs.read(), wlk=fwlk.read()))
+    return F
+
+
 def trace_scenarios():
     """Step-by-step traces of the main sampling (no calibration, no decorrelation)."""
     T = []
@@ -227,6 +247,8 @@ def main():
         json.dump(dict(meta=meta, scenarios=chains), f, indent=0)
     with open(os.path.join(HERE, "traces.json"), "w") as f:
         json.dump(dict(meta=meta, traces=trace_scenarios()), f, indent=0)
+    with open(os.path.join(HERE, "files.json"), "w") as f:
+        json.dump(dict(meta=meta, files=file_scenarios()), f, indent=0)
     with open(os.path.join(HERE, "walkers.json"), "w") as f:
         json.dump(dict(meta=meta, **walker_scenarios()), f, indent=0)
     with open(os.path.join(HERE, "estimators.json"), "w") as f:

@@ -93,6 +93,10 @@ MCI_ONLY = {
     "set_kernel_path": [_I],
     "get_timings": [_DP, _I],
     "get_observable_data": [_I, _DP, _I64, _I64P],
+    "store_observables_on_file": [C.c_char_p, _I],
+    "clear_observable_file": [],
+    "store_walker_positions_on_file": [C.c_char_p, _I],
+    "clear_walker_file": [],
 }
 
 
@@ -238,6 +242,9 @@ class Chain:
             arr = (C.c_int * len(typeEnds))(*typeEnds)
             te = C.cast(arr, _IP)
         self.call("set_trial_move_multistep", nsteps, srrd, veclen, ntypes, te)
+
+    def storeObservablesOnFile(self, path, freq): self.call("store_observables_on_file", str(path).encode(), freq)
+    def storeWalkerPositionsOnFile(self, path, freq): self.call("store_walker_positions_on_file", str(path).encode(), freq)
 
     def addMultistepSamplingFunction(self, kind, params=()):
         p = f64(params)

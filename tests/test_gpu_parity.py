@@ -96,6 +96,36 @@ def test_reference_step_traces_bit_exact(tr):
     assert np.array_equal(xs[-1], G.hexv(tr["result"]["x"]))
 
 
+FILES = G.load("files.json")["files"]
+
+
+@pytest.mark.parametrize("fs", FILES, ids=[f["name"] for f in FILES])
+def test_reference_observable_and_walker_files(fs, tmp_path):
+    """storeObservablesOnFile / storeWalkerPositionsOnFile: the files equal the reference's text;
+    with three walkers, walker 0 (the context seed) writes the reference's files and the others
+    files of the same shape."""
+    print("files", fs["name"], flush=True)
+    cfg = G.cfg_of(fs["cfg"])
+    for W in (1, 3):
+        c = chain_setup("mci", cfg, nwalkers=W)
+        c.storeObservablesOnFile(tmp_path / "obs.txt", fs["freq_obs"])
+        c.storeWalkerPositionsOnFile(tmp_path / "wlk.txt", fs["freq_wlk"])
+        c.integrate(fs["nmc"], True, True)
+        if W == 1:
+            assert (tmp_path / "obs.txt").read_text() == fs["obs"]
+            assert (tmp_path / "wlk.txt").read_text() == fs["wlk"]
+            continue
+        assert (tmp_path / "obs.txt.0").read_text() == fs["obs"]
+        assert (tmp_path / "wlk.txt.0").read_text() == fs["wlk"]
+        for w in (1, 2):
+            for nm, ref in (("obs.txt", fs["obs"]), ("wlk.txt", fs["wlk"])):
+                got = (tmp_path / f"{nm}.{w}").read_text().splitlines()
+                want = ref.splitlines()
+                assert len(got) == len(want)
+                assert [g.split()[0] for g in got] == [r.split()[0] for r in want]
+                assert [len(g.split()) for g in got] == [len(r.split()) for r in want]
+
+
 def test_reference_walkers_bit_exact():
     d = G.load("walkers.json")
     W = len(d["seeds"])
