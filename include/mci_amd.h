@@ -188,6 +188,23 @@ mci_status mci_clear_multistep_sampling_functions(mci_ctx *ctx);
  * format.  One file per walker: the path itself for one walker, path.<walker> for several.  The
  * values are recorded on the device (at most 16 GB) and written when the sampling ends; FP64 only. */
 mci_status mci_store_observables_on_file(mci_ctx *ctx, const char *path, int freq);
+/* MCI::setCallback / clearCallback (MCIntegrator.hpp:189-190; calls at MCIntegrator.cpp:265,
+ * 346-347, 373): the callback sees, at the start of every sample() call of calibration,
+ * decorrelation and sampling and at every step after the decision, walker 0's committed
+ * position and its accepted / rejected counters.  The device records up to `capacity` such
+ * states during integrate and the calls are made in order before mci_integrate returns; more
+ * calls than the capacity give MCI_E_RUNTIME after the first `capacity`.  The generic kernels run. */
+typedef struct mci_callback_view
+{
+    int64_t call;      /* 0, 1, ... within the integrate */
+    int ndim;
+    const double *x;   /* [ndim], valid during the call */
+    int64_t acc, rej;
+    double rate;       /* acc / (acc + rej), 0 while acc is 0 (MCI::getAcceptanceRate) */
+} mci_callback_view;
+typedef void (*mci_callback_fn)(void *user, const mci_callback_view *view);
+mci_status mci_set_callback(mci_ctx *ctx, mci_callback_fn fn, void *user, int64_t capacity);
+mci_status mci_clear_callback(mci_ctx *ctx);
 mci_status mci_clear_observable_file(mci_ctx *ctx);
 mci_status mci_store_walker_positions_on_file(mci_ctx *ctx, const char *path, int freq);
 mci_status mci_clear_walker_file(mci_ctx *ctx);

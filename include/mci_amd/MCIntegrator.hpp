@@ -19,6 +19,8 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <exception>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -159,6 +161,23 @@ class MCI
     int64_t _nwalkers;
     std::vector<SamplingFunction> _pdfs;
     std::vector<ObservableFunction> _obs;
+    std::function<void(const MCI &)> _cback;   // MCI::setCallback
+    const mci_callback_view *_view = nullptr;  // the recorded state while a callback call runs
+    std::exception_ptr _cback_error;
+
+    static void trampoline(void *user, const mci_callback_view *v)
+    {
+        MCI *m = static_cast<MCI *>(user);
+        if (m->_cback_error) { return; }
+        m->_view = v;
+        try {
+            m->_cback(*m);
+        }
+        catch (...) {
+            m->_cback_error = std::current_exception();
+        }
+        m->_view = nullptr;
+    }
 
 public:
     explicit MCI(int ndim, int64_t nwalkers = 1): _ndim(ndim), _nwalkers(nwalkers) { check(mci_create(ndim, nwalkers, &_ctx)); }
@@ -279,7 +298,7 @@ public:
         check(mci_get_x(_ctx, all.data(), (int64_t)all.size()));
         return std::vector<double>(all.begin() + walker * _ndim, all.begin() + (walker + 1) * _ndim);
     }
-    double getX(int i) const { return getXv(0)[i]; }
+    double getX(int i) const { return _view ? _view->x[i] : getXv(0)[i]; }
     double getMRT2Step(int i) const
     {
         double s = 0.;
@@ -306,6 +325,7 @@ public:
     }
     double getAcceptanceRate() const
     {
+        if (_view) { return _view->rate; }
         int64_t a = 0, r = 0;
         double rate = 0.;
         check(mci_get_acceptance(_ctx, &a, &r, &rate));
@@ -332,7 +352,24 @@ public:
     // --- integrate (MCIntegrator.hpp:225) ---
     void integrate(int64_t Nmc, double average[], double error[], bool doFindMRT2step = true, bool doDecorrelation = true)
     {
-        check(mci_integrate(_ctx, Nmc, average, error, doFindMRT2step ? 1 : 0, doDecorrelation ? 1 : 0));
+        _cback_error = nullptr;
+        const mci_status st = mci_integrate(_ctx, Nmc, average, error, doFindMRT2step ? 1 : 0, doDecorrelation ? 1 : 0);
+        if (_cback_error) { std::rethrow_exception(_cback_error); }
+        check(st);
+    }
+
+    // --- callback (MCIntegrator.hpp:189-190): called in order after the device work of each
+    //     integrate, with getX and getAcceptanceRate returning what the reference's callback
+    //     would see at that call (walker 0); capacity bounds the recorded calls per integrate ---
+    void setCallback(const std::function<void(const MCI &)> &cback, int64_t capacity = (int64_t)1 << 20)
+    {
+        check(mci_set_callback(_ctx, &MCI::trampoline, this, capacity));
+        _cback = cback;
+    }
+    void clearCallback()
+    {
+        check(mci_clear_callback(_ctx));
+        _cback = nullptr;
     }
     // per-walker estimates of the last integrate, [walker][nobsdim]
     void getWalkerResults(std::vector<double> &average, std::vector<double> &error) const

@@ -95,6 +95,9 @@ struct mci_ctx
     std::vector<int> typeEnds;
     int ms_on = 0, ms_nsteps = 0;     // MultiStepMove around the move above
     std::string obsfile, wlkfile;     // file output of the main sampling (empty: off)
+    mci_callback_fn cb = nullptr;     // MCI::setCallback: replayed from walker 0's record
+    void *cb_user = nullptr;
+    int64_t cb_cap = 0;
     int64_t obsfreq = 0, wlkfreq = 0;
     std::vector<PdfHost> ms_pdfs;     // its own sampling functions
     std::vector<PdfHost> pdfs;
@@ -113,6 +116,8 @@ struct mci_ctx
     DBuf<double> nsv;     // [2][W] saved normals of the move's distribution
     DBuf<double> ms_pold, ms_pnew, ms_x0; // MultiStepMove: proto values of its own sampling functions, start position
     DBuf<double> rec_obs, rec_x;          // file output: recorded observable values and positions
+    DBuf<double> cb_buf;                  // callback record [cb_cap][ndim + 2]
+    DBuf<int64_t> cb_n;
     DBuf<int32_t> nsf;    // [W] their availability bits
     DBuf<int64_t> acc, rej;
     DBuf<uint8_t> active;
@@ -211,6 +216,11 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     ch.srrd = c->srrd;
     ch.nsv = c->nsv.p;
     ch.nsf = c->nsf.p;
+    if (c->cb) {
+        ch.cb_buf = c->cb_buf.p;
+        ch.cb_n = c->cb_n.p;
+        ch.cb_cap = c->cb_cap;
+    }
     ch.ms_on = c->ms_on;
     ch.ms_nsteps = c->ms_nsteps;
     ch.ms_npdf = (int)c->ms_pdfs.size();
@@ -321,7 +331,7 @@ float elapsed(mci_ctx *c, int a, int b)
 // the specialised kernels apply to this chain (observables are checked separately)
 bool fast_chain(const mci_ctx *c)
 {
-    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
+    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && !c->cb && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
                          c->pdfs.empty() ? 0 : c->pdfs[0].kind) &&
            !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
 }
@@ -1070,9 +1080,72 @@ mci_status mci_clear_observables(mci_ctx *c) { CHECKCTX(c); c->obs.clear(); retu
 mci_status mci_get_nobs(mci_ctx *c, int *n) { CHECKCTX(c); *n = (int)c->obs.size(); return MCI_OK; }
 mci_status mci_get_nobsdim(mci_ctx *c, int *n) { CHECKCTX(c); *n = nobsdim(c); return MCI_OK; }
 
+mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *error, int do_find, int do_decorr);
+
+// the callback calls of the last integrate, in order, from walker 0's record
+mci_status replay_callback(mci_ctx *c)
+{
+    int64_t n = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&n, c->cb_n.p, sizeof n, hipMemcpyDeviceToHost));
+    const int64_t m = std::min(n, c->cb_cap), rw = c->ndim + 2;
+    std::vector<double> h((size_t)std::max<int64_t>(m, 1) * rw);
+    if (m > 0) { HIPCHK(hipMemcpy(h.data(), c->cb_buf.p, sizeof(double) * (size_t)(m * rw), hipMemcpyDeviceToHost)); }
+    for (int64_t k = 0; k < m; ++k) {
+        const double *r = &h[(size_t)(k * rw)];
+        mci_callback_view v;
+        v.call = k;
+        v.ndim = c->ndim;
+        v.x = r;
+        v.acc = (int64_t)r[c->ndim];
+        v.rej = (int64_t)r[c->ndim + 1];
+        v.rate = (v.acc > 0) ? static_cast<double>(v.acc) / (static_cast<double>(v.acc) + v.rej) : 0.; // MCI::getAcceptanceRate (MCIntegrator.cpp:587-592)
+        c->cb(c->cb_user, &v);
+    }
+    if (n > c->cb_cap) {
+        return fail(MCI_E_RUNTIME, "the callback was due %lld times, its record holds %lld: the first %lld calls were made", (long long)n,
+                    (long long)c->cb_cap, (long long)m);
+    }
+    return MCI_OK;
+}
+
 mci_status mci_integrate(mci_ctx *c, int64_t nmc, double *average, double *error, int do_find, int do_decorr)
 {
     CHECKCTX(c);
+    if (c->cb) {
+        TRY(set_current(c));
+        HIPCHK(hipMemsetAsync(c->cb_n.p, 0, sizeof(int64_t), c->stream));
+    }
+    TRY(integrate_phases(c, nmc, average, error, do_find, do_decorr));
+    return c->cb ? replay_callback(c) : MCI_OK;
+}
+
+// MCI::setCallback / clearCallback (MCIntegrator.hpp:189-190)
+mci_status mci_set_callback(mci_ctx *c, mci_callback_fn fn, void *user, int64_t capacity)
+{
+    CHECKCTX(c);
+    if (!fn) { return fail(MCI_E_INVALID_ARGUMENT, "null callback (use mci_clear_callback)"); }
+    if (capacity < 1 || (double)capacity * (c->ndim + 2) * 8.0 > 4e9) { return fail(MCI_E_INVALID_ARGUMENT, "callback record capacity must be in [1, 4 GB]"); }
+    TRY(set_current(c));
+    HIPCHK(c->cb_buf.ensure((size_t)capacity * (c->ndim + 2)));
+    HIPCHK(c->cb_n.ensure(1));
+    c->cb = fn;
+    c->cb_user = user;
+    c->cb_cap = capacity;
+    return MCI_OK;
+}
+
+mci_status mci_clear_callback(mci_ctx *c)
+{
+    CHECKCTX(c);
+    c->cb = nullptr;
+    c->cb_user = nullptr;
+    c->cb_cap = 0;
+    return MCI_OK;
+}
+
+mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *error, int do_find, int do_decorr)
+{
     if (c->pdfs.empty() && !c->periodic) {
         return fail(MCI_E_DOMAIN, "[MCI::integrate] Integrating over an infinite domain requires a sampling function.");
     }

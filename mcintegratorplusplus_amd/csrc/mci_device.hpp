@@ -74,6 +74,9 @@ struct Chain
     double *rec_x;              // [nrec][ndim][W] positions every rec_fx steps or null
     int64_t rec_fobs, rec_fx;
     int rec_nobsdim;
+    double *cb_buf;             // callback record of walker 0: [cb_cap][ndim + 2] (position, accepted, rejected) or null
+    int64_t *cb_n;              // [1] calls so far (also past cb_cap)
+    int64_t cb_cap;
     int periodic;
     const double *lb, *ub;      // [ndim]
     int npdf;
@@ -482,6 +485,21 @@ __device__ inline double srrd_move(const Chain &c, Lane &L)
     return 1.;
 }
 
+// the state the reference's callback sees (MCIntegrator.cpp:265, 346-347, 373): walker 0's committed
+// position and its counters, recorded for the host to replay the calls in order
+__device__ inline void cb_record(const Chain &c, const Lane &L)
+{
+    if (c.cb_buf == nullptr || L.w != 0) { return; }
+    const int64_t n = c.cb_n[0];
+    if (n < c.cb_cap) {
+        double *r = c.cb_buf + n * (c.ndim + 2);
+        for (int i = 0; i < c.ndim; ++i) { r[i] = c.xold[(int64_t)i * c.W]; }
+        r[c.ndim] = (double)L.acc;
+        r[c.ndim + 1] = (double)L.rej;
+    }
+    c.cb_n[0] = n + 1;
+}
+
 // MultiStepMove::trialMove (MultiStepMove.cpp:6-47): ms_nsteps Metropolis steps of the contained
 // move under the move's own sampling functions (none: every step accepted), from xold; xnew ends at
 // the last position, xold is restored, every coordinate counts as changed, and the factor
@@ -525,6 +543,7 @@ __device__ inline void init_sampling(const Chain &c, Lane &L)
     L.accepted = true;
     for (int p = 0; p < c.npdf; ++p) { pdf_proto(c, c.pdf[p], c.xold, c.pold, w); }
     for (int i = 0; i < c.nproto; ++i) { c.pnew[(int64_t)i * W + w] = c.pold[(int64_t)i * W + w]; }
+    cb_record(c, L);
 }
 
 // MCI::doStepMRT2 (MCIntegrator.cpp:326-360)
@@ -541,6 +560,7 @@ __device__ inline void step_mrt2(const Chain &c, Lane &L)
     const double u = draw_unit(c, L);
     L.accepted = (u <= pdfAcc * moveAcc);
     if (L.accepted) { ++L.acc; } else { ++L.rej; }
+    cb_record(c, L);
     step_end(c, L);
     // commit or roll back the coordinates and proto values that can differ
     if (L.accepted) {
@@ -564,12 +584,13 @@ __device__ inline void step_random(const Chain &c, Lane &L)
         double &x = c.xnew[(int64_t)i * W + w];
         if (c.periodic) { x = c.lb[i] + x * (c.ub[i] - c.lb[i]); }
         else { x = -(double)3.402823466e+38f + x * ((double)3.402823466e+38f + (double)3.402823466e+38f); }
-        c.xold[(int64_t)i * W + w] = x;
     }
     L.nchanged = c.ndim;
     L.cfirst = 0;
     L.accepted = true;
     ++L.acc;
+    cb_record(c, L);
+    for (int i = 0; i < c.ndim; ++i) { c.xold[(int64_t)i * W + w] = c.xnew[(int64_t)i * W + w]; }
 }
 
 // --------------------------------------------------------------------------

@@ -238,6 +238,41 @@ static void multistep()
     std::printf("multistep ok: <x0^2> = %.6f +- %.6f\n", average[0], error[0]);
 }
 
+static void callback()
+{
+    // the callback sees the start of the sample() call and every step: n + 1 calls without
+    // calibration and decorrelation, rate 0 at the start, positions as getX returns them
+    MCI mci(2);
+    mci.setSeed(9);
+    mci.addSamplingFunction(Gauss(2));
+    mci.addObservable(X2(2), 1, 1, false, EstimatorType::Uncorrelated);
+    std::vector<double> rates, xs;
+    mci.setCallback([&](const MCI &m) {
+        rates.push_back(m.getAcceptanceRate());
+        xs.push_back(m.getX(0));
+    });
+    double average[2], error[2];
+    mci.integrate(100, average, error, false, false);
+    CHECK(rates.size() == 101);
+    CHECK(rates[0] == 0.);
+    bool in01 = true;
+    for (double r : rates) { in01 = in01 && r >= 0. && r <= 1.; }
+    CHECK(in01);
+    mci.clearCallback();
+    mci.integrate(100, average, error, false, false);
+    CHECK(rates.size() == 101);
+    bool threw = false;
+    mci.setCallback([](const MCI &) { throw std::runtime_error("from the callback"); });
+    try {
+        mci.integrate(10, average, error, false, false);
+    }
+    catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw);
+    std::printf("callback ok: %zu calls\n", rates.size());
+}
+
 int main()
 {
     int ndev = 0;
@@ -254,6 +289,7 @@ int main()
     exceptions();
     walkers();
     multistep();
+    callback();
     std::printf("all %d checks passed\n", g_checks);
     return 0;
 }

@@ -197,6 +197,25 @@ def trace_scenarios():
     return T
 
 
+def phase_trace_scenarios():
+    """What the callback sees through calibration, decorrelation and sampling (MCIntegrator.cpp:265,
+    346-347, 373): fixed and automatic calibration and decorrelation, and random sampling without a
+    sampling function."""
+    T = []
+    for name, cfg, nmc, find, decorr in [
+        ("phases_fixed", dict(ndim=3, seed=4242, nfind=3, ndecorr=50, pdfs=[(PDF_GAUSS_ISO, [])], obs=[(OBS_X2, [], 1, 1, 0, ESTIM_UNCORR)],
+                              x0=[1.0, -0.5, 0.25]), 64, 1, 1),
+        ("phases_auto", dict(ndim=2, seed=99, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, [0.5, 2.0])],
+                             obs=[(OBS_X2, [], 1, 1, 1, ESTIM_CORR)], x0=[3.0, -2.0]), 64, 1, 1),
+        ("phases_nopdf", dict(ndim=2, seed=5, irange=([-1.0, -1.0], [2.0, 1.0]), obs=[(OBS_X2_SUM, [], 1, 1, 0, ESTIM_UNCORR)]), 32, 1, 1),
+    ]:
+        L = script_setup(cfg) + ["trace", f"integrate {nmc} {find} {decorr}"]
+        r = ref_run(L)[-1]
+        r.pop("seconds", None)
+        T.append(dict(name=name, cfg=cfg, nmc=nmc, find=find, decorr=decorr, result=r))
+    return T
+
+
 def walker_scenarios():
     """Several independent reference MCIs with their own seeds (one per GPU walker)."""
     seeds = [1337, 11, 12345, 987654321, 2 ** 40 + 3, 5649871, 42, 7]
@@ -246,7 +265,7 @@ def main():
     with open(os.path.join(HERE, "chains.json"), "w") as f:
         json.dump(dict(meta=meta, scenarios=chains), f, indent=0)
     with open(os.path.join(HERE, "traces.json"), "w") as f:
-        json.dump(dict(meta=meta, traces=trace_scenarios()), f, indent=0)
+        json.dump(dict(meta=meta, traces=trace_scenarios(), phase_traces=phase_trace_scenarios()), f, indent=0)
     with open(os.path.join(HERE, "files.json"), "w") as f:
         json.dump(dict(meta=meta, files=file_scenarios()), f, indent=0)
     with open(os.path.join(HERE, "walkers.json"), "w") as f:

@@ -93,6 +93,8 @@ MCI_ONLY = {
     "set_kernel_path": [_I],
     "get_timings": [_DP, _I],
     "get_observable_data": [_I, _DP, _I64, _I64P],
+    "set_callback": [C.c_void_p, C.c_void_p, _I64],
+    "clear_callback": [],
     "store_observables_on_file": [C.c_char_p, _I],
     "clear_observable_file": [],
     "store_walker_positions_on_file": [C.c_char_p, _I],
@@ -242,6 +244,19 @@ class Chain:
             arr = (C.c_int * len(typeEnds))(*typeEnds)
             te = C.cast(arr, _IP)
         self.call("set_trial_move_multistep", nsteps, srrd, veclen, ntypes, te)
+
+    def setCallback(self, fn, capacity=1 << 20):
+        """fn(x [ndim] array, acc, rej, rate) for every callback call of an integrate"""
+        class View(C.Structure):
+            _fields_ = [("call", C.c_int64), ("ndim", C.c_int), ("x", C.POINTER(C.c_double)),
+                        ("acc", C.c_int64), ("rej", C.c_int64), ("rate", C.c_double)]
+        proto = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(View))
+
+        def tramp(user, v):
+            v = v.contents
+            fn(np.ctypeslib.as_array(v.x, (v.ndim,)).copy(), v.acc, v.rej, v.rate)
+        self._cb = proto(tramp)  # kept alive with the chain
+        self.call("set_callback", C.cast(self._cb, C.c_void_p), None, capacity)
 
     def storeObservablesOnFile(self, path, freq): self.call("store_observables_on_file", str(path).encode(), freq)
     def storeWalkerPositionsOnFile(self, path, freq): self.call("store_walker_positions_on_file", str(path).encode(), freq)

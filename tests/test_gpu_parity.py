@@ -96,6 +96,27 @@ def test_reference_step_traces_bit_exact(tr):
     assert np.array_equal(xs[-1], G.hexv(tr["result"]["x"]))
 
 
+PHASES = G.load("traces.json")["phase_traces"]
+
+
+@pytest.mark.parametrize("tr", PHASES, ids=[t["name"] for t in PHASES])
+def test_reference_callback_through_all_phases(tr):
+    """setCallback: the calls and what each sees (position, acceptance rate) through calibration,
+    decorrelation and sampling equal the reference's, bit for bit, in the same order."""
+    print("callback", tr["name"], flush=True)
+    cfg = G.cfg_of(tr["cfg"])
+    nd = cfg["ndim"]
+    seen_x, seen_rate = [], []
+    c = chain_setup("mci", cfg)
+    c.setCallback(lambda x, acc, rej, rate: (seen_x.append(x), seen_rate.append(rate)))
+    c.integrate(tr["nmc"], bool(tr["find"]), bool(tr["decorr"]))
+    ref_x = G.hexv(tr["result"]["trace_x"]).reshape(-1, nd)
+    ref_rate = G.hexv(tr["result"]["trace_rate"])
+    assert len(seen_x) == len(ref_x)
+    assert np.array_equal(np.array(seen_x), ref_x)
+    assert np.array_equal(np.array(seen_rate), ref_rate)
+
+
 FILES = G.load("files.json")["files"]
 
 
