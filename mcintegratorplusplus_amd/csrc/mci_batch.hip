@@ -36,6 +36,7 @@ namespace
 {
 using mcif::bexp;
 using mcif::blog;
+using mcif::xor3;
 using mcif::sym24;
 
 // Philox4x32-10 with the round keys precomputed (they are the same for every lane and step)
@@ -61,8 +62,8 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ K.k0[r];
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ K.k1[r];
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, K.k0[r]);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, K.k1[r]);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;

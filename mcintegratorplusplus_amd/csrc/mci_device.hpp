@@ -15,9 +15,11 @@
 #include <stdint.h>
 
 #include "../../include/mci_amd.h"
+#include "mci_f32math.hpp"
 
 namespace mcid
 {
+using mcif::xor3;
 constexpr int MAX_PDF = 4;
 constexpr int MAX_OBS = 16;
 constexpr int MAX_TYPES = 16;
@@ -140,8 +142,8 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;

@@ -9,6 +9,10 @@
 
 namespace mcif
 {
+// a ^ b ^ c in one instruction (v_bitop3_b32 with the truth table of a three-input xor): the
+// Philox rounds' xors of a product half, a counter word and a key word
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+
 // e^x: Cody–Waite reduction by ln 2 (two-part constant, fused multiply-add), the degree-7
 // Taylor polynomial of e^r on |r| <= ln2/2 (Horner, fused), exact scaling by 2^n.
 // 0 below -87 (and for NaN), +inf above 88; at most 0.92 ulp from e^x on [-87, 88].
