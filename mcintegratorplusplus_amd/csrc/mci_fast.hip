@@ -52,6 +52,7 @@ struct FastArgs
     double exp_div;        // EXP_X2 divisor
     int nvecs;
     uint32_t lemire_thr;   // 2^32 mod nvecs
+    int vshift;            // nvecs = 2^(32 - vshift) >= 2: the index is w0 >> vshift; else 0
     const int *skip;       // nonzero *skip: return at once (finished calibration loop)
 };
 
@@ -75,10 +76,14 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
 
 // Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution):
 // low = (w0 * range) mod 2^32 < thr = 2^32 mod range rejects and draws from the extra-word stream.
-// thr is computed on the host (0 for powers of two); a wave vote skips the rare path.
-__device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t thr, uint32_t wid, uint64_t t, uint32_t k0,
+// thr is computed on the host (0 for powers of two); a wave vote skips the rare path.  For a
+// power-of-two range 2^m the product's high word is w0 >> (32 - m) and nothing is rejected, so the
+// 64-bit multiply (a quarter-rate instruction) is skipped.
+template <bool P2>
+__device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t thr, int vshift, uint32_t wid, uint64_t t, uint32_t k0,
                                          uint32_t k1)
 {
+    if (P2) { return (int)(w0 >> vshift); }
     uint64_t prod = (uint64_t)w0 * range;
     if (__builtin_expect(__any((uint32_t)prod < thr), 0)) {
         uint32_t low = (uint32_t)prod;
@@ -95,7 +100,7 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t t
 }
 
 // observable set of one launch (template flags: no per-step branches on them)
-enum { F_X2 = 1, F_SUM = 2, F_EXP = 4 };
+enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8 }; // F_P2: the index range is a power of two
 
 // FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
 // 1,024 waves), so a step costs about 4 cycles per instruction of any kind: the step is
@@ -105,7 +110,7 @@ enum { F_X2 = 1, F_SUM = 2, F_EXP = 4 };
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
-    constexpr bool ACC = FL != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool ACC = (FL & 7) != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
     extern __shared__ double lds[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
     double mult = (double)B;  // B - j: steps of the block the new value counts for
     for (int64_t s = 0; s < A.nsteps; ++s) {
-        const int v = draw_vidx(w0, range, thr, wid, t, A.key0, A.key1);
+        const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
         const int slot = v * WAVE + lane;
         const double xo = xs[slot];
         const double av = as[v];
@@ -234,7 +239,7 @@ __device__ __forceinline__ void kahan_if(float &s, float &c, float y, bool on)
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 {
-    constexpr bool ACC = FL != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool ACC = (FL & 7) != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
     extern __shared__ float lds32[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
     float mult = (float)B;
     for (int64_t s = 0; s < A.nsteps; ++s) {
-        const int v = draw_vidx(w0, range, thr, wid, t, A.key0, A.key1);
+        const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
         const int slot = v * WAVE + lane;
         const float xo = xs[slot];
         const float asv = as[v];
@@ -405,6 +410,10 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     A.rej = c.rej;
     A.nvecs = c.nvecs;
     A.lemire_thr = (0u - (uint32_t)c.nvecs) % (uint32_t)c.nvecs;
+    A.vshift = 0;
+    for (int m = 1; m < 32; ++m) {
+        if ((uint32_t)c.nvecs == (1u << m)) { A.vshift = 32 - m; }
+    }
     A.blocksize = 1;
     A.nblocks = 0;
     A.d_x2 = A.d_x2sum = A.d_exp = nullptr;
@@ -422,6 +431,7 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     if (!A.a) { return MCI_E_INVALID_ARGUMENT; } // the fast path needs explicit weights
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
     const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0);
+    if (A.vshift != 0) { fl |= F_P2; }
     switch (fl) {
     case 0: launch_fl<0>(prec, grid, sh, s, A); break;
     case 1: launch_fl<1>(prec, grid, sh, s, A); break;
@@ -430,7 +440,15 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     case 4: launch_fl<4>(prec, grid, sh, s, A); break;
     case 5: launch_fl<5>(prec, grid, sh, s, A); break;
     case 6: launch_fl<6>(prec, grid, sh, s, A); break;
-    default: launch_fl<7>(prec, grid, sh, s, A); break;
+    case 7: launch_fl<7>(prec, grid, sh, s, A); break;
+    case 8: launch_fl<8>(prec, grid, sh, s, A); break;
+    case 9: launch_fl<9>(prec, grid, sh, s, A); break;
+    case 10: launch_fl<10>(prec, grid, sh, s, A); break;
+    case 11: launch_fl<11>(prec, grid, sh, s, A); break;
+    case 12: launch_fl<12>(prec, grid, sh, s, A); break;
+    case 13: launch_fl<13>(prec, grid, sh, s, A); break;
+    case 14: launch_fl<14>(prec, grid, sh, s, A); break;
+    default: launch_fl<15>(prec, grid, sh, s, A); break;
     }
     return hipGetLastError() == hipSuccess ? MCI_OK : MCI_E_DEVICE;
 }
