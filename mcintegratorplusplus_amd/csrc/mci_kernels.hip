@@ -312,7 +312,19 @@ __device__ inline void mjblocker_small(int64_t n, const Col &x, double *y, int64
     int npow = 0;
     while (((int64_t)1 << npow) < n) { ++npow; }
     double mean = 0.0;
-    for (int64_t i = 0; i < n; ++i) { const double v = x[i]; y[i * ystride] = v; mean = mean + v; }
+    // sixteen loads in flight, then the copies and the sum in order
+    for (int64_t i0 = 0; i0 < n; i0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) { v[q] = (i0 + q < n) ? x[i0 + q] : 0.0; }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (i0 + q < n) {
+                y[(i0 + q) * ystride] = v[q];
+                mean = mean + v[q];
+            }
+        }
+    }
     mean = mean / (double)n;
     double var[LMAX], gam[LMAX];
     int64_t nred = n;
@@ -365,14 +377,11 @@ __device__ inline void mjblocker_small(int64_t n, const Col &x, double *y, int64
     err = sqrt(pick(var, k) / ldexp(1.0, npow - k));
 }
 
-// MJBlocker's working copy: this thread's LDS column when the series is short (block means),
-// else its column of the scratch copy in HBM
-__device__ inline void mj_any(const EstimJob &j, const Col &x, int comp, int64_t w, double *sh, double &a, double &e)
-{
-    if (j.n <= MJ_LDS_MAX) { mjblocker_small(j.n, x, sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
-    else { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
-}
-
+// one instantiation per estimator (E: MCI_ESTIM_NOOP, _UNCORRELATED, _FCBLOCKER, _MJBLOCKER, or
+// EST_MJ_SMALL for MJBlocker on at most 64 values; the host resolves CORRELATED by n), so that the
+// private arrays of FCBlocker and of the general MJBlocker (scratch memory) weigh on no other
+constexpr int EST_MJ_SMALL = 5;
+template <int E>
 __global__ void k_estimate(EstimJob j)
 {
     extern __shared__ double est_sh[];
@@ -384,16 +393,11 @@ __global__ void k_estimate(EstimJob j)
     const Col x{j.data + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W};
     const bool onedim = (j.nobs == 1);
     double a = 0., e = 0.;
-    switch (j.estim) {
-    case MCI_ESTIM_NOOP: a = x[0]; e = 0.; break;
-    case MCI_ESTIM_UNCORRELATED: uncorr(j.n, x, onedim, a, e); break;
-    case MCI_ESTIM_FCBLOCKER: fcblocker(j.n, x, onedim, a, e); break;
-    case MCI_ESTIM_MJBLOCKER: mj_any(j, x, comp, w, est_sh, a, e); break;
-    case MCI_ESTIM_CORRELATED:
-        if ((j.n & (j.n - 1)) != 0) { fcblocker(j.n, x, onedim, a, e); }
-        else { mj_any(j, x, comp, w, est_sh, a, e); }
-        break;
-    }
+    if (E == MCI_ESTIM_NOOP) { a = x[0]; e = 0.; }
+    if (E == MCI_ESTIM_UNCORRELATED) { uncorr(j.n, x, onedim, a, e); }
+    if (E == MCI_ESTIM_FCBLOCKER) { fcblocker(j.n, x, onedim, a, e); }
+    if (E == MCI_ESTIM_MJBLOCKER) { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
+    if (E == EST_MJ_SMALL) { mjblocker_small(j.n, x, est_sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
     j.avg[(int64_t)(j.off + comp) * j.W + w] = a * j.scale;
     j.err[(int64_t)(j.off + comp) * j.W + w] = e * j.scale;
 }
@@ -557,7 +561,12 @@ hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
 {
     const bool mj = j.estim == MCI_ESTIM_MJBLOCKER || (j.estim == MCI_ESTIM_CORRELATED && (j.n & (j.n - 1)) == 0);
     const size_t sh = (mj && j.n <= MJ_LDS_MAX) ? sizeof(double) * 64 * (size_t)j.n : 0;
-    hipLaunchKernelGGL(k_estimate, grid1(j.W * j.nobs, 64), dim3(64), sh, s, j);
+    const dim3 g = grid1(j.W * j.nobs, 64);
+    if (mj && j.n <= MJ_LDS_MAX) { hipLaunchKernelGGL(k_estimate<EST_MJ_SMALL>, g, dim3(64), sh, s, j); }
+    else if (mj) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_MJBLOCKER>, g, dim3(64), 0, s, j); }
+    else if (j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_FCBLOCKER>, g, dim3(64), 0, s, j); }
+    else if (j.estim == MCI_ESTIM_UNCORRELATED) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_UNCORRELATED>, g, dim3(64), 0, s, j); }
+    else { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_NOOP>, g, dim3(64), 0, s, j); }
     return hipGetLastError();
 }
 
