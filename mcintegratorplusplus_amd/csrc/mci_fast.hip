@@ -74,6 +74,66 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
+// The same Philox4x32-10 block for the counter (0, wid, tlo, thi) when tlo and thi are the same on
+// every lane of the wave: the rounds are those of philox() with the values that do not depend on
+// wid kept apart, so that the compiler holds them in scalar registers.  Round 1 multiplies only
+// tlo (c0 = 0), round 2 one lane value, round 3 one; from round 4 on both products depend on wid.
+__device__ __forceinline__ void philox_uniform_ctr(uint32_t wid, uint32_t tlo, uint32_t thi, uint32_t k0, uint32_t k1,
+                                                   uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3)
+{
+    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    // round 1 (c0 = 0: its product is 0)
+    const uint64_t q1 = (uint64_t)M1 * tlo;
+    uint32_t c0 = ((uint32_t)(q1 >> 32) ^ k0) ^ wid;
+    uint32_t c1 = (uint32_t)q1;
+    uint32_t c2 = thi ^ k1;
+    uint32_t c3;
+    // round 2: c1, c2 uniform, c3 = 0
+    k0 += W0; k1 += W1;
+    {
+        const uint64_t p0 = (uint64_t)M0 * c0;
+        const uint64_t q2 = (uint64_t)M1 * c2;
+        const uint32_t n0 = (uint32_t)(q2 >> 32) ^ c1 ^ k0;
+        c2 = (uint32_t)(p0 >> 32) ^ k1;
+        c1 = (uint32_t)q2;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+    }
+    // round 3: c0, c1 uniform
+    k0 += W0; k1 += W1;
+    {
+        const uint64_t q3 = (uint64_t)M0 * c0;
+        const uint64_t p1 = (uint64_t)M1 * c2;
+        c0 = (uint32_t)(p1 >> 32) ^ (c1 ^ k0);
+        c2 = c3 ^ ((uint32_t)(q3 >> 32) ^ k1);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)q3;
+    }
+    // round 4: c3 uniform
+    k0 += W0; k1 += W1;
+    {
+        const uint64_t p0 = (uint64_t)M0 * c0;
+        const uint64_t p1 = (uint64_t)M1 * c2;
+        c0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        c2 = (uint32_t)(p0 >> 32) ^ (c3 ^ k1);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+    }
+#pragma unroll
+    for (int r = 4; r < 10; ++r) {
+        k0 += W0; k1 += W1;
+        const uint64_t p0 = (uint64_t)M0 * c0;
+        const uint64_t p1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
 // Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution):
 // low = (w0 * range) mod 2^32 < thr = 2^32 mod range rejects and draws from the extra-word stream.
 // thr is computed on the host (0 for powers of two); a wave vote skips the rare path.  For a
@@ -102,117 +162,178 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t t
 // observable set of one launch (template flags: no per-step branches on them)
 enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8 }; // F_P2: the index range is a power of two
 
+// compile-time flag passed to a generic lambda
+template <bool V>
+struct Flag
+{
+    static constexpr bool value = V;
+};
+
 // FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
-// 1,024 waves), so a step costs about 4 cycles per instruction of any kind: the step is
-// written to issue few instructions — no per-step branch on the observable set (template),
-// a select-formed decision screened in FP32 with the high 32 bits of u, and the exact FP64
-// test (53-bit u, FP64 exp) only for lanes inside the screening band.
+// 1,024 waves), so a step costs about 4 cycles per instruction of any kind and 16 per
+// v_mad_u64_u32 (quarter rate; the Philox rounds hold up to 19 of them).  The step is written
+// to issue few instructions:
+//  - no per-step branch on the observable set (template), the steps of one block as an inner
+//    loop with a 32-bit accept count (one scalar loop test per step);
+//  - with X2 the coordinate and its block sum are one 16-byte LDS record (one ds_read_b128 and
+//    one ds_write_b128 per step);
+//  - the proposal (2 w1 + 1 - 2^32) 2^-32 as one exact fused multiply-add;
+//  - the decision screened in FP32 in units of 2^-32 (F = 2^(32 - ex log2 e) against the high
+//    word of u), the exact FP64 test (53-bit u, FP64 exp) only for lanes inside the band;
+//  - when the wave's step counters agree (shared calibration), the counter lives in scalar
+//    registers: the first three Philox rounds then hold only three vector multiplies, the rest
+//    run on the scalar unit.
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
     constexpr bool ACC = (FL & 7) != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool P2 = (FL & F_P2) != 0;
     extern __shared__ double lds[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
     if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
-    double *xs = lds;                             // [d][64]
-    double *Ss = lds + d * WAVE;                  // [d][64] block sums of x_i^2 (X2)
-    double *as = lds + (HX2 ? 2 : 1) * d * WAVE;  // [d]
+    const int B = A.blocksize;
+    double2 *rec = reinterpret_cast<double2 *>(lds);  // X2: [d][64] {x_i, block sum of x_i^2}
+    double *xs = lds;                                  // otherwise [d][64] x_i
+    double *as = lds + (HX2 ? 2 : 1) * d * WAVE;       // [d]
     for (int i = lane; i < d; i += WAVE) { as[i] = A.a[i]; }
-    double r = 0.0;                               // |x|^2
+    double r = 0.0;                                    // |x|^2
     if (live) {
         for (int i = 0; i < d; ++i) {
             const double v = A.x64[(int64_t)i * A.W + w];
-            xs[i * WAVE + lane] = v;
+            if (HX2) { rec[i * WAVE + lane] = make_double2(v, (double)B * (v * v)); }
+            else { xs[i * WAVE + lane] = v; }
             r = r + v * v;
         }
     }
     __syncthreads();
     if (!live) { return; }
     const double step = A.steps[w];
-    uint64_t t = A.ctr[w];
+    const uint64_t t_in = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
-    int64_t nacc = 0;
-    const int B = A.blocksize;
-    double Sr = 0.0, Se = 0.0, e = 0.0;
-    int j = 0;
-    int64_t b = 0;
-    if (HX2) { for (int i = 0; i < d; ++i) { const double v = xs[i * WAVE + lane]; Ss[i * WAVE + lane] = (double)B * (v * v); } }
-    if (ACC) { Sr = (double)B * r; }
-    if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-    // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
-    // path has no divergent branch: the decision is a select and the LDS state is always
-    // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
-    // number equals the branched form).
-    uint32_t w0, w1, w2, w3;
-    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-    double mult = (double)B;  // B - j: steps of the block the new value counts for
-    for (int64_t s = 0; s < A.nsteps; ++s) {
-        const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
-        const int slot = v * WAVE + lane;
-        const double xo = xs[slot];
-        const double av = as[v];
-        const double So = HX2 ? Ss[slot] : 0.0;
-        // proposal (2 w1 + 1 - 2^32) 2^-32, exact
-        const double rr = ((double)(int32_t)(w1 ^ 0x80000000u) + 0.5) * 0x1p-31;
-        const uint32_t u_hi = w2, u_lo = w3;
-        ++t;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-        const double xn = xo + step * rr;
-        const double xn2 = xn * xn, xo2 = xo * xo;
-        const double ex = xn2 * av - xo2 * av;
-        // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  With the
-        // FP32 exp f (relative error far below 4e-5 for 0 < ex < 88), u <= exp(-ex) is certain when
-        // uf (1 + 4e-5) + 2^-32 < f (1 - 4e-5) and false when uf (1 - 4e-5) > f (1 + 4e-5).
-        const float f = __expf(-(float)ex);
-        const float uf = (float)u_hi * 0x1p-32f;
-        bool acc = (ex <= 0.0) || (fmaf(uf, 1.00004f, 0x1p-32f) < f * 0.99996f);
-        const bool band = !acc && !(uf * 0.99996f > f * 1.00004f);
-        if (__builtin_expect(__any(band), 0)) {
-            if (band) {
-                const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
-                acc = u <= exp(-ex);
+    const uint32_t tlo = (uint32_t)t_in, thi = (uint32_t)(t_in >> 32);
+    const bool uniform_t = __builtin_amdgcn_ballot_w64(tlo != __builtin_amdgcn_readfirstlane(tlo) ||
+                                                       thi != __builtin_amdgcn_readfirstlane(thi)) == 0;
+
+    auto run = [&](auto uniform_tag) {
+        constexpr bool UT = decltype(uniform_tag)::value;
+        uint64_t t = t_in;
+        if (UT) { t = ((uint64_t)__builtin_amdgcn_readfirstlane(thi) << 32) | __builtin_amdgcn_readfirstlane(tlo); }
+        int64_t nacc = 0;
+        double Sr = 0.0, Se = 0.0, e = 0.0;
+        int64_t b = 0;
+        if (ACC) { Sr = (double)B * r; }
+        if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+        // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
+        // path has no divergent branch: the decision is a select and the LDS state is always
+        // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
+        // number equals the branched form).
+        auto next_words = [&](uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3) {
+            if (UT) { philox_uniform_ctr(wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, o0, o1, o2, o3); }
+            else { philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, o0, o1, o2, o3); }
+        };
+        uint32_t w0, w1, w2, w3;
+        next_words(w0, w1, w2, w3);
+        const int64_t chunk = ACC ? (int64_t)B : ((int64_t)1 << 30);  // blocks start at step 0
+        for (int64_t s = 0; s < A.nsteps;) {
+            const int len = (int)(A.nsteps - s < chunk ? A.nsteps - s : chunk);
+            double mult = (double)B;  // B - j: steps of the block the new value counts for
+            uint32_t na = 0;
+            for (int q = 0; q < len; ++q) {
+                const int v = draw_vidx<P2>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
+                const int slot = v * WAVE + lane;
+                double xo, So = 0.0;
+                if (HX2) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
+                else { xo = xs[slot]; }
+                const double av = as[v];
+                // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
+                const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
+                const uint32_t u_hi = w2, u_lo = w3;
+                ++t;
+                next_words(w0, w1, w2, w3);
+                const double xn = xo + step * rr;
+                const double xn2 = xn * xn, xo2 = xo * xo;
+                const double ex = xn2 * av - xo2 * av;
+                // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  With
+                // F = 2^(32 - ex log2 e) in FP32 (relative error below 2e-5 wherever F >= 1, i.e.
+                // ex < 22.2) and U = u_hi rounded to FP32: U < 0.9999 F - 2 makes u <= exp(-ex)
+                // certain, U > 1.0001 F makes it false; between the two the FP64 test decides.
+                const float U = (float)u_hi;
+                const float F = __builtin_amdgcn_exp2f(fmaf((float)ex, -0x1.715476p+0f, 32.0f));
+                const bool acc = U < fmaf(F, 0.9999f, -2.0f);
+                const bool band = !acc && U <= F * 1.0001f;
+                na += acc ? 1u : 0u;
+                const double xsel = acc ? xn : xo;
+                if (ACC) {
+                    const double dx2 = xsel * xsel - xo2;  // xn2 - xo2 if accepted, else +0
+                    if (HX2) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
+                    else { xs[slot] = xsel; }
+                    r = r + dx2;
+                    Sr = Sr + dx2 * mult;
+                    if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
+                }
+                else { xs[slot] = xsel; }
+                // the next step's words are needed here, so the compiler keeps their computation in
+                // this block (instead of sinking it below the branch, where nothing overlaps it)
+                asm volatile("" ::"v"(w0), "v"(w1), "v"(w2), "v"(w3));
+                // Lanes inside the band went through the update above as rejections (which leaves
+                // every number as it was: +0.0 added to non-negative sums).  When the wave holds
+                // any, the exact test runs and the lanes it accepts are updated as accepted steps,
+                // from the values read at the start of the step.  The common path thus stays one
+                // basic block, in which the next step's Philox words fill the latency of this
+                // step's dependent chain.
+                // (two ballots of plain compares: each is the compare's own lane mask)
+                if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
+                    const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
+                    if (band && u <= exp(-ex)) {
+                        na += 1u;
+                        if (ACC) {
+                            const double dx2 = xn2 - xo2;
+                            if (HX2) { rec[slot] = make_double2(xn, So + dx2 * mult); }
+                            else { xs[slot] = xn; }
+                            r = r + dx2;
+                            Sr = Sr + dx2 * mult;
+                            if (HEXP) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
+                        }
+                        else { xs[slot] = xn; }
+                    }
+                }
+                if (ACC) { mult = mult - 1.0; }
             }
-        }
-        nacc += acc ? 1 : 0;
-        xs[slot] = acc ? xn : xo;
-        if (ACC) {
-            const double dx2 = acc ? (xn2 - xo2) : 0.0;
-            if (HX2) { Ss[slot] = So + dx2 * mult; }
-            r = r + dx2;
-            Sr = Sr + dx2 * mult;
-            if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
-            mult = mult - 1.0;
-            if (++j == B) {
+            nacc += na;
+            s += len;
+            if (ACC && len == B) {
                 const double norm = 1. / B;
                 double rx = 0.0;
                 for (int i = 0; i < d; ++i) {
-                    const double v2 = xs[i * WAVE + lane];
-                    const double q = v2 * v2;
-                    rx = rx + q;
+                    double v2;
                     if (HX2) {
-                        A.d_x2[(b * d + i) * A.W + w] = Ss[i * WAVE + lane] * norm;
-                        Ss[i * WAVE + lane] = (double)B * q;
+                        const double2 rv = rec[i * WAVE + lane];
+                        v2 = rv.x;
+                        A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
+                        rec[i * WAVE + lane] = make_double2(v2, (double)B * (v2 * v2));
                     }
+                    else { v2 = xs[i * WAVE + lane]; }
+                    rx = rx + v2 * v2;
                 }
                 if (HSUM) { A.d_x2sum[b * A.W + w] = (Sr * norm) / A.x2sum_div; }
                 if (HEXP) { A.d_exp[b * A.W + w] = Se * norm; }
                 r = rx;  // refresh |x|^2 exactly at every block boundary
                 Sr = (double)B * r;
                 if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-                j = 0;
-                mult = (double)B;
                 ++b;
             }
         }
-    }
-    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = xs[i * WAVE + lane]; }
-    A.ctr[w] = t;
-    A.acc[w] = nacc;
-    A.rej[w] = A.nsteps - nacc;
+        for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; }
+        A.ctr[w] = t;
+        A.acc[w] = nacc;
+        A.rej[w] = A.nsteps - nacc;
+    };
+    if (uniform_t) { run(Flag<true>{}); }
+    else { run(Flag<false>{}); }
 }
 
 // Kahan-compensated FP32 accumulation: s + y with compensation c
