@@ -377,12 +377,69 @@ __device__ inline void mjblocker_small(int64_t n, const Col &x, double *y, int64
     err = sqrt(pick(var, k) / ldexp(1.0, npow - k));
 }
 
+// the same for n = 2^P <= 64: the series in registers, every loop unrolled at compile time, so
+// that all n loads are in flight at once and no LDS round trip sits in the sums' dependency
+// chains (identical operations in identical order)
+template <int P>
+__device__ inline void mjblocker_reg(const Col &x, double &avg, double &err)
+{
+    constexpr int N = 1 << P;
+    double y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) { y[i] = x[i]; }
+    double mean = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) { mean = mean + y[i]; }
+    mean = mean / (double)N;
+    if (P == 0) { avg = mean; err = 0.0; return; }
+    double var[P > 0 ? P : 1], gam[P > 0 ? P : 1];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int nred = N >> k;
+        double g0 = 0.0, g1 = 0.0;
+        double prev = y[0] - mean;
+        g0 = g0 + prev * prev;
+#pragma unroll
+        for (int i = 1; i < nred; ++i) {
+            const double cur = y[i] - mean;
+            g0 = g0 + cur * cur;
+            g1 = g1 + prev * cur;
+            prev = cur;
+        }
+        var[k] = g0 / (double)nred;
+        gam[k] = g1 / (double)nred;
+#pragma unroll
+        for (int i = 0; i < nred / 2; ++i) { y[i] = 0.5 * (y[2 * i] + y[2 * i + 1]); }
+    }
+    // M of level i at reversed position P - 1 - i; cumulative sums in that order
+    double ms[P > 0 ? P : 1];
+    double msum = 0.0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int i = P - 1 - q;
+        const double v = var[i], g = gam[i];
+        const double r = g / v;
+        msum = msum + ((v == 0.0) ? 0.0 : (r * r) * ldexp(1.0, P - i));
+        ms[q] = msum;
+    }
+    int kk = -1;
+#pragma unroll
+    for (int q = P - 1; q >= 0; --q) {
+        if (kk < 0 && ms[q] < c_chi2_q95[q]) { kk = q; }
+    }
+    int k = P - (kk + 1);
+    if (k >= P) { k = P - 1; }
+    avg = mean;
+    err = sqrt(pick(var, k) / ldexp(1.0, P - k));
+}
+
 // one instantiation per estimator (E: MCI_ESTIM_NOOP, _UNCORRELATED, _FCBLOCKER, _MJBLOCKER, or
 // EST_MJ_SMALL for MJBlocker on at most 64 values; the host resolves CORRELATED by n), so that the
 // private arrays of FCBlocker and of the general MJBlocker (scratch memory) weigh on no other
 constexpr int EST_MJ_SMALL = 5;
-template <int E>
-__global__ void k_estimate(EstimJob j)
+constexpr int EST_MJ_REG = 6;  // MJBlocker on 2^P <= 64 values in registers
+template <int E, int P = 0>
+__global__ __launch_bounds__(256) void k_estimate(EstimJob j)
 {
     extern __shared__ double est_sh[];
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -398,6 +455,7 @@ __global__ void k_estimate(EstimJob j)
     if (E == MCI_ESTIM_FCBLOCKER) { fcblocker(j.n, x, onedim, a, e); }
     if (E == MCI_ESTIM_MJBLOCKER) { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
     if (E == EST_MJ_SMALL) { mjblocker_small(j.n, x, est_sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
+    if (E == EST_MJ_REG) { mjblocker_reg<P>(x, a, e); }
     j.avg[(int64_t)(j.off + comp) * j.W + w] = a * j.scale;
     j.err[(int64_t)(j.off + comp) * j.W + w] = e * j.scale;
 }
@@ -562,7 +620,20 @@ hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
     const bool mj = j.estim == MCI_ESTIM_MJBLOCKER || (j.estim == MCI_ESTIM_CORRELATED && (j.n & (j.n - 1)) == 0);
     const size_t sh = (mj && j.n <= MJ_LDS_MAX) ? sizeof(double) * 64 * (size_t)j.n : 0;
     const dim3 g = grid1(j.W * j.nobs, 64);
-    if (mj && j.n <= MJ_LDS_MAX) { hipLaunchKernelGGL(k_estimate<EST_MJ_SMALL>, g, dim3(64), sh, s, j); }
+    const bool pow2 = j.n >= 1 && (j.n & (j.n - 1)) == 0;
+    if (mj && pow2 && j.n <= 64) {
+        const dim3 g256 = grid1(j.W * j.nobs, 256);
+        switch (j.n) {
+        case 1: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 0>), g256, dim3(256), 0, s, j); break;
+        case 2: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 1>), g256, dim3(256), 0, s, j); break;
+        case 4: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 2>), g256, dim3(256), 0, s, j); break;
+        case 8: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 3>), g256, dim3(256), 0, s, j); break;
+        case 16: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 4>), g256, dim3(256), 0, s, j); break;
+        case 32: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 5>), g256, dim3(256), 0, s, j); break;
+        default: hipLaunchKernelGGL((k_estimate<EST_MJ_REG, 6>), g256, dim3(256), 0, s, j); break;
+        }
+    }
+    else if (mj && j.n <= MJ_LDS_MAX) { hipLaunchKernelGGL(k_estimate<EST_MJ_SMALL>, g, dim3(64), sh, s, j); }
     else if (mj) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_MJBLOCKER>, g, dim3(64), 0, s, j); }
     else if (j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_FCBLOCKER>, g, dim3(64), 0, s, j); }
     else if (j.estim == MCI_ESTIM_UNCORRELATED) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_UNCORRELATED>, g, dim3(64), 0, s, j); }

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import golden_io as G
-from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, KERNELS_AUTO, KERNELS_GENERIC,
+from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_CORR, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, KERNELS_AUTO, KERNELS_GENERIC,
                      OBS_EXP_X2, OBS_X, OBS_X2, OBS_X2_SUM, PDF_GAUSS_DIAG, PREC_FP32, RNG_MT, RNG_PHILOX, Chain,
                      chain_setup, estimate)
 
@@ -171,6 +171,23 @@ def test_device_estimators_bit_exact(s):
         a, e = estimate("mci", int(est), x)
         assert np.array_equal(a, G.hexv(r["avg"])), (est, a)
         assert np.array_equal(e, G.hexv(r["err"])), (est, e)
+
+
+@pytest.mark.parametrize("n", [2, 8, 32, 64])
+def test_device_mjblocker_power_of_two_sizes_match_oracle(n):
+    """MJBlocker, and Correlated (MJBlocker at these sizes), on 2^k <= 64 values: the device's
+    register-resident form equals the oracle bit for bit, here at the sizes the golden sets leave
+    open (8, 32) besides 2 and 64, on an AR(1) series in 5 components.  (n = 1 is left out: the
+    reference's error formula then reads before its level arrays.)"""
+    rng = np.random.default_rng(100 + n)
+    x = np.zeros((n, 5))
+    e = rng.normal(size=(n, 5))
+    for i in range(1, n):
+        x[i] = 0.8 * x[i - 1] + e[i]
+    for est in (ESTIM_MJ, ESTIM_CORR):
+        a, er = estimate("mci", est, x)
+        b, eb = estimate("orc", est, x)
+        assert np.array_equal(a, b) and np.array_equal(er, eb), (n, est)
 
 
 def sigma_a(n):
