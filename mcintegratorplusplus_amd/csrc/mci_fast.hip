@@ -74,66 +74,6 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
-// The same Philox4x32-10 block for the counter (0, wid, tlo, thi) when tlo and thi are the same on
-// every lane of the wave: the rounds are those of philox() with the values that do not depend on
-// wid kept apart, so that the compiler holds them in scalar registers.  Round 1 multiplies only
-// tlo (c0 = 0), round 2 one lane value, round 3 one; from round 4 on both products depend on wid.
-__device__ __forceinline__ void philox_uniform_ctr(uint32_t wid, uint32_t tlo, uint32_t thi, uint32_t k0, uint32_t k1,
-                                                   uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3)
-{
-    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-    // round 1 (c0 = 0: its product is 0)
-    const uint64_t q1 = (uint64_t)M1 * tlo;
-    uint32_t c0 = ((uint32_t)(q1 >> 32) ^ k0) ^ wid;
-    uint32_t c1 = (uint32_t)q1;
-    uint32_t c2 = thi ^ k1;
-    uint32_t c3;
-    // round 2: c1, c2 uniform, c3 = 0
-    k0 += W0; k1 += W1;
-    {
-        const uint64_t p0 = (uint64_t)M0 * c0;
-        const uint64_t q2 = (uint64_t)M1 * c2;
-        const uint32_t n0 = (uint32_t)(q2 >> 32) ^ c1 ^ k0;
-        c2 = (uint32_t)(p0 >> 32) ^ k1;
-        c1 = (uint32_t)q2;
-        c3 = (uint32_t)p0;
-        c0 = n0;
-    }
-    // round 3: c0, c1 uniform
-    k0 += W0; k1 += W1;
-    {
-        const uint64_t q3 = (uint64_t)M0 * c0;
-        const uint64_t p1 = (uint64_t)M1 * c2;
-        c0 = (uint32_t)(p1 >> 32) ^ (c1 ^ k0);
-        c2 = c3 ^ ((uint32_t)(q3 >> 32) ^ k1);
-        c1 = (uint32_t)p1;
-        c3 = (uint32_t)q3;
-    }
-    // round 4: c3 uniform
-    k0 += W0; k1 += W1;
-    {
-        const uint64_t p0 = (uint64_t)M0 * c0;
-        const uint64_t p1 = (uint64_t)M1 * c2;
-        c0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        c2 = (uint32_t)(p0 >> 32) ^ (c3 ^ k1);
-        c1 = (uint32_t)p1;
-        c3 = (uint32_t)p0;
-    }
-#pragma unroll
-    for (int r = 4; r < 10; ++r) {
-        k0 += W0; k1 += W1;
-        const uint64_t p0 = (uint64_t)M0 * c0;
-        const uint64_t p1 = (uint64_t)M1 * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
-        c1 = (uint32_t)p1;
-        c3 = (uint32_t)p0;
-        c0 = n0;
-        c2 = n2;
-    }
-    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
-}
-
 // Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution):
 // low = (w0 * range) mod 2^32 < thr = 2^32 mod range rejects and draws from the extra-word stream.
 // thr is computed on the host (0 for powers of two); a wave vote skips the rare path.  For a
@@ -162,13 +102,6 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t t
 // observable set of one launch (template flags: no per-step branches on them)
 enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8 }; // F_P2: the index range is a power of two
 
-// compile-time flag passed to a generic lambda
-template <bool V>
-struct Flag
-{
-    static constexpr bool value = V;
-};
-
 // FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
 // 1,024 waves), so a step costs about 4 cycles per instruction of any kind and 16 per
 // v_mad_u64_u32 (quarter rate; the Philox rounds hold up to 19 of them).  The step is written
@@ -180,9 +113,11 @@ struct Flag
 //  - the proposal (2 w1 + 1 - 2^32) 2^-32 as one exact fused multiply-add;
 //  - the decision screened in FP32 in units of 2^-32 (F = 2^(32 - ex log2 e) against the high
 //    word of u), the exact FP64 test (53-bit u, FP64 exp) only for lanes inside the band;
-//  - when the wave's step counters agree (shared calibration), the counter lives in scalar
-//    registers: the first three Philox rounds then hold only three vector multiplies, the rest
-//    run on the scalar unit.
+//  - with X2 and X2_SUM both present, the block sum of |x|^2 is the sum of the coordinates' block
+//    sums, taken at the block's end (no per-step update).
+// Scalar instructions cost an issue slot as well: keeping the counter in scalar registers when a
+// wave's counters agree (three fewer vector multiplies, about twenty more scalar instructions per
+// step) measured 11 % slower.
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
@@ -214,14 +149,8 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const uint64_t t_in = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
-    const uint32_t tlo = (uint32_t)t_in, thi = (uint32_t)(t_in >> 32);
-    const bool uniform_t = __builtin_amdgcn_ballot_w64(tlo != __builtin_amdgcn_readfirstlane(tlo) ||
-                                                       thi != __builtin_amdgcn_readfirstlane(thi)) == 0;
-
-    auto run = [&](auto uniform_tag) {
-        constexpr bool UT = decltype(uniform_tag)::value;
+    {
         uint64_t t = t_in;
-        if (UT) { t = ((uint64_t)__builtin_amdgcn_readfirstlane(thi) << 32) | __builtin_amdgcn_readfirstlane(tlo); }
         int64_t nacc = 0;
         double Sr = 0.0, Se = 0.0, e = 0.0;
         int64_t b = 0;
@@ -231,12 +160,8 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         // path has no divergent branch: the decision is a select and the LDS state is always
         // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
         // number equals the branched form).
-        auto next_words = [&](uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3) {
-            if (UT) { philox_uniform_ctr(wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, o0, o1, o2, o3); }
-            else { philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, o0, o1, o2, o3); }
-        };
         uint32_t w0, w1, w2, w3;
-        next_words(w0, w1, w2, w3);
+        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
         const int64_t chunk = ACC ? (int64_t)B : ((int64_t)1 << 30);  // blocks start at step 0
         for (int64_t s = 0; s < A.nsteps;) {
             const int len = (int)(A.nsteps - s < chunk ? A.nsteps - s : chunk);
@@ -253,7 +178,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
                 const uint32_t u_hi = w2, u_lo = w3;
                 ++t;
-                next_words(w0, w1, w2, w3);
+                philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
                 const double xn = xo + step * rr;
                 const double xn2 = xn * xn, xo2 = xo * xo;
                 const double ex = xn2 * av - xo2 * av;
@@ -272,7 +197,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                     if (HX2) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
                     else { xs[slot] = xsel; }
                     r = r + dx2;
-                    Sr = Sr + dx2 * mult;
+                    if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
                     if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
                 }
                 else { xs[slot] = xsel; }
@@ -295,7 +220,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                             if (HX2) { rec[slot] = make_double2(xn, So + dx2 * mult); }
                             else { xs[slot] = xn; }
                             r = r + dx2;
-                            Sr = Sr + dx2 * mult;
+                            if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
                             if (HEXP) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
                         }
                         else { xs[slot] = xn; }
@@ -307,19 +232,20 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             s += len;
             if (ACC && len == B) {
                 const double norm = 1. / B;
-                double rx = 0.0;
+                double rx = 0.0, sS = 0.0;
                 for (int i = 0; i < d; ++i) {
                     double v2;
                     if (HX2) {
                         const double2 rv = rec[i * WAVE + lane];
                         v2 = rv.x;
+                        sS = sS + rv.y;
                         A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
                         rec[i * WAVE + lane] = make_double2(v2, (double)B * (v2 * v2));
                     }
                     else { v2 = xs[i * WAVE + lane]; }
                     rx = rx + v2 * v2;
                 }
-                if (HSUM) { A.d_x2sum[b * A.W + w] = (Sr * norm) / A.x2sum_div; }
+                if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : Sr) * norm) / A.x2sum_div; }
                 if (HEXP) { A.d_exp[b * A.W + w] = Se * norm; }
                 r = rx;  // refresh |x|^2 exactly at every block boundary
                 Sr = (double)B * r;
@@ -331,9 +257,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         A.ctr[w] = t;
         A.acc[w] = nacc;
         A.rej[w] = A.nsteps - nacc;
-    };
-    if (uniform_t) { run(Flag<true>{}); }
-    else { run(Flag<false>{}); }
+    }
 }
 
 // Kahan-compensated FP32 accumulation: s + y with compensation c
