@@ -132,8 +132,12 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const int B = A.blocksize;
     double2 *rec = reinterpret_cast<double2 *>(lds);  // X2: [d][64] {x_i, block sum of x_i^2}
     double *xs = lds;                                  // otherwise [d][64] x_i
-    double *as = lds + (HX2 ? 2 : 1) * d * WAVE;       // [d]
-    for (int i = lane; i < d; i += WAVE) { as[i] = A.a[i]; }
+    double *as = lds + (HX2 ? 2 : 1) * d * WAVE;       // [d] a_i
+    float *cs = reinterpret_cast<float *>(as + d);     // [d] -a_i log2 e in FP32 (screening exponent)
+    for (int i = lane; i < d; i += WAVE) {
+        as[i] = A.a[i];
+        cs[i] = (float)(-A.a[i] * 0x1.71547652b82fep+0);
+    }
     double r = 0.0;                                    // |x|^2
     if (live) {
         for (int i = 0; i < d; ++i) {
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 double xo, So = 0.0;
                 if (HX2) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
                 else { xo = xs[slot]; }
-                const double av = as[v];
+                const float cv = cs[v];
                 // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
                 const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
                 const uint32_t u_hi = w2, u_lo = w3;
@@ -181,19 +185,21 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
                 const double xn = xo + step * rr;
                 const double xn2 = xn * xn, xo2 = xo * xo;
-                const double ex = xn2 * av - xo2 * av;
-                // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  With
-                // F = 2^(32 - ex log2 e) in FP32 (relative error below 2e-5 wherever F >= 1, i.e.
-                // ex < 22.2) and U = u_hi rounded to FP32: U < 0.9999 F - 2 makes u <= exp(-ex)
-                // certain, U > 1.0001 F makes it false; between the two the FP64 test decides.
+                const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
+                // The FP64 test is u <= exp(-ex) with ex = a_v xn^2 - a_v xo^2 (pdf_acceptance's form).
+                // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  The
+                // screen takes F = 2^(32 - a_v log2 e (xn^2 - xo^2)) in FP32: relative error below 2e-5
+                // wherever F >= 1 (ex < 22.2; the FP64 rounding of ex is far smaller), and U = u_hi
+                // rounded to FP32.  U < 0.9999 F - 2 makes the FP64 test certain to accept,
+                // U > 1.0001 F certain to reject; between the two the FP64 test decides.
                 const float U = (float)u_hi;
-                const float F = __builtin_amdgcn_exp2f(fmaf((float)ex, -0x1.715476p+0f, 32.0f));
+                const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, 32.0f));
                 const bool acc = U < fmaf(F, 0.9999f, -2.0f);
                 const bool band = !acc && U <= F * 1.0001f;
-                na += acc ? 1u : 0u;
+                if (acc) { ++na; }
                 const double xsel = acc ? xn : xo;
                 if (ACC) {
-                    const double dx2 = xsel * xsel - xo2;  // xn2 - xo2 if accepted, else +0
+                    const double dx2 = acc ? dx2n : 0.0;
                     if (HX2) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
                     else { xs[slot] = xsel; }
                     r = r + dx2;
@@ -213,10 +219,12 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 // (two ballots of plain compares: each is the compare's own lane mask)
                 if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
                     const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
+                    const double av = as[v];
+                    const double ex = xn2 * av - xo2 * av;
                     if (band && u <= exp(-ex)) {
                         na += 1u;
                         if (ACC) {
-                            const double dx2 = xn2 - xo2;
+                            const double dx2 = dx2n;
                             if (HX2) { rec[slot] = make_double2(xn, So + dx2 * mult); }
                             else { xs[slot] = xn; }
                             r = r + dx2;
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 
 size_t lds_bytes(int prec, int ndim, bool x2)
 {
-    if (prec == MCI_PREC_FP64) { return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + ndim); }
+    if (prec == MCI_PREC_FP64) { return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + ndim) + sizeof(float) * ndim; }
     return sizeof(float) * ((size_t)(x2 ? 3 : 1) * ndim * WAVE + ndim);
 }
 
