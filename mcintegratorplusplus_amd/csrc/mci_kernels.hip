@@ -478,19 +478,35 @@ __global__ void k_chunk_sum(const double *v, int64_t W, int ncomp, int square, d
     if (threadIdx.x == 0) { chunks[(int64_t)comp * nchunks + chunk] = buf[0]; }
 }
 
-// MPIMCI::integrate combine (MPIMCI.cpp:85-92): avg = sum/W, err = sqrt(sum err^2)/W
+// MPIMCI::integrate combine (MPIMCI.cpp:85-92): avg = sum/W, err = sqrt(sum err^2)/W.  One
+// workgroup per component: the chunk sums are staged through LDS (coalesced), then added in
+// order by lane 0 (one dependent add per chunk, no memory round trip inside the chain)
 __global__ void k_combine(const double *csum_avg, const double *csum_err2, int64_t nchunks, int ncomp, int64_t W,
                           double *avg, double *err)
 {
-    const int comp = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sa[1024], se[1024];
+    const int comp = blockIdx.x;
     if (comp >= ncomp) { return; }
-    double sa = csum_avg[(int64_t)comp * nchunks], se = csum_err2[(int64_t)comp * nchunks];
-    for (int64_t k = 1; k < nchunks; ++k) {
-        sa = sa + csum_avg[(int64_t)comp * nchunks + k];
-        se = se + csum_err2[(int64_t)comp * nchunks + k];
+    double ta = 0.0, te = 0.0;
+    for (int64_t base = 0; base < nchunks; base += 1024) {
+        const int64_t m = (nchunks - base < 1024) ? nchunks - base : 1024;
+        for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+            sa[i] = csum_avg[(int64_t)comp * nchunks + base + i];
+            se[i] = csum_err2[(int64_t)comp * nchunks + base + i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int64_t i = 0; i < m; ++i) {
+                if (base == 0 && i == 0) { ta = sa[0]; te = se[0]; }
+                else { ta = ta + sa[i]; te = te + se[i]; }
+            }
+        }
+        __syncthreads();
     }
-    avg[comp] = sa / (double)W;
-    err[comp] = sqrt(se) / (double)W;
+    if (threadIdx.x == 0) {
+        avg[comp] = ta / (double)W;
+        err[comp] = sqrt(te) / (double)W;
+    }
 }
 
 // plain ordered sum of the chunk sums (rates), result / W
@@ -651,7 +667,8 @@ hipError_t launch_chunk_sum(const double *v, int64_t W, int ncomp, int square, d
 hipError_t launch_combine(const double *ca, const double *ce, int64_t W, int ncomp, double *avg, double *err, hipStream_t s)
 {
     const int64_t nch = (W + 255) / 256;
-    hipLaunchKernelGGL(k_combine, grid1(ncomp, 64), dim3(64), 0, s, ca, ce, nch, ncomp, W, avg, err);
+    if (ncomp <= 0) { return hipSuccess; }
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)ncomp), dim3(256), 0, s, ca, ce, nch, ncomp, W, avg, err);
     return hipGetLastError();
 }
 
