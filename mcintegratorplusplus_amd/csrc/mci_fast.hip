@@ -74,6 +74,38 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
+// philox() split for the step loop: rounds 0..8 and the p1 half of round 9 give words 0 and 1;
+// s0 and s3 carry what the p0 half needs.  The p0 half (words 2 and 3) is philox_tail9().
+__device__ __forceinline__ void philox_head9(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                             uint32_t &o0, uint32_t &o1, uint32_t &s0, uint32_t &s3)
+{
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    k0 += 0x9E3779B9u;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    o0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+    o1 = (uint32_t)p1;
+    s0 = c0;
+    s3 = c3;
+}
+
+__device__ __forceinline__ void philox_tail9(uint32_t s0, uint32_t s3, uint32_t k1, uint32_t &o2, uint32_t &o3)
+{
+    const uint64_t p0 = (uint64_t)0xD2511F53u * s0;
+    o2 = xor3((uint32_t)(p0 >> 32), s3, k1 + 9u * 0xBB67AE85u);
+    o3 = (uint32_t)p0;
+}
+
 // Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution):
 // low = (w0 * range) mod 2^32 < thr = 2^32 mod range rejects and draws from the extra-word stream.
 // thr is computed on the host (0 for powers of two); a wave vote skips the rare path.  For a
@@ -182,7 +214,8 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
                 const uint32_t u_hi = w2, u_lo = w3;
                 ++t;
-                philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+                uint32_t s0, s3;
+                philox_head9(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, s0, s3);
                 const double xn = xo + step * rr;
                 const double xn2 = xn * xn, xo2 = xo * xo;
                 const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
@@ -207,9 +240,10 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                     if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
                 }
                 else { xs[slot] = xsel; }
-                // the next step's words are needed here, so the compiler keeps their computation in
-                // this block (instead of sinking it below the branch, where nothing overlaps it)
-                asm volatile("" ::"v"(w0), "v"(w1), "v"(w2), "v"(w3));
+                // the next step's words 0 and 1 (and the state for 2 and 3) are needed here, so the
+                // compiler keeps their computation in this block (instead of sinking it below the
+                // branch, where nothing overlaps it)
+                asm volatile("" ::"v"(w0), "v"(w1), "v"(s0), "v"(s3));
                 // Lanes inside the band went through the update above as rejections (which leaves
                 // every number as it was: +0.0 added to non-negative sums).  When the wave holds
                 // any, the exact test runs and the lanes it accepts are updated as accepted steps,
@@ -235,6 +269,9 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                     }
                 }
                 if (ACC) { mult = mult - 1.0; }
+                // words 2 and 3 after the exact path, which still reads this step's u_hi and u_lo:
+                // they then take over its registers (no copies at the loop's back edge)
+                philox_tail9(s0, s3, A.key1, w2, w3);
             }
             nacc += na;
             s += len;
