@@ -110,6 +110,7 @@ struct mci_ctx
     DBuf<double> x, xn, pold, pnew, steps, lbd, ubd, obsval, dimsizes, wavg, werr, chunks_a, chunks_e, res, rates, fact;
     DBuf<double> oa, oe;  // decorrelation: previous estimates (independent walkers)
     DBuf<CalibCtrl> ctrl;
+    DBuf<unsigned> arrive; // k_calib_round's arrival counter (zero between launches)
     bool walker_results_host = false; // h_wavg/h_werr hold the device's per-walker results
     DBuf<uint64_t> mt, ctr, seeds;
     DBuf<int32_t> mtp;
@@ -349,6 +350,10 @@ mci_status find_shared(mci_ctx *c)
     HIPCHK(c->rates.ensure(c->W));
     HIPCHK(c->chunks_a.ensure(nch));
     HIPCHK(c->ctrl.ensure(1));
+    if (!c->arrive.p) {
+        HIPCHK(c->arrive.ensure(1));
+        HIPCHK(hipMemsetAsync(c->arrive.p, 0, sizeof(unsigned), c->stream));
+    }
     CalibCtrl init;
     std::memset(&init, 0, sizeof init);
     HIPCHK(hipMemcpyAsync(c->ctrl.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
@@ -358,18 +363,27 @@ mci_status find_shared(mci_ctx *c)
     for (int it = 0; it < maxit;) {
         for (int b = 0; b < batch && it < maxit; ++b, ++it) {
             // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
-            if (fast) { TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, &c->ctrl.p->done)); }
-            else { HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream, c->ctrl.p)); }
-            HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
-            HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
-            HIPCHK(launch_calib_control(c->chunks_a.p, c->W, c->target, N, c->ctrl.p, c->stream));
-            HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream));
+            if (fast) {
+                // two launches per iteration: the sampling launch first applies the previous
+                // iteration's factor; rates, sums and control are one launch
+                TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, c->ctrl.p, c->dimsizes.p));
+                HIPCHK(launch_calib_round(c->acc.p, c->rej.p, c->W, c->chunks_a.p, c->target, N, c->ctrl.p, c->arrive.p, c->stream));
+            }
+            else {
+                HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream, c->ctrl.p));
+                HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
+                HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
+                HIPCHK(launch_calib_control(c->chunks_a.p, c->W, c->target, N, c->ctrl.p, c->stream));
+                HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream));
+            }
         }
         int done = 0;
         HIPCHK(hipMemcpyAsync(&done, &c->ctrl.p->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         if (done) { break; }
     }
+    // the fast path's last factor is still pending when no sampling launch followed its control
+    if (fast) { HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream)); }
     return MCI_OK;
 }
 

@@ -23,6 +23,7 @@
 // acceptance uniform on a 2^-24 grid, block sums as Kahan-compensated FP32 pairs, flushed to
 // FP64 for the FP64 estimators.
 #include "mci_fast.hpp"
+#include "mci_kernels.hpp"
 
 #include <float.h>
 
@@ -41,7 +42,7 @@ struct FastArgs
     uint64_t *ctr;         // [W]
     double *x64;           // [ndim][W] (FP64 state, also FP32 mode's persistent copy)
     const double *a;       // [ndim] p_i = a_i x_i^2
-    const double *steps;   // [W] (one type)
+    double *steps;         // [W] (one type)
     int64_t *acc, *rej;    // [W]
     int blocksize;         // B (Simple: nsteps)
     int64_t nblocks;
@@ -53,7 +54,8 @@ struct FastArgs
     int nvecs;
     uint32_t lemire_thr;   // 2^32 mod nvecs
     int vshift;            // nvecs = 2^(32 - vshift) >= 2: the index is w0 >> vshift; else 0
-    const int *skip;       // nonzero *skip: return at once (finished calibration loop)
+    const CalibCtrl *pend; // shared calibration: pending step factor, loop finished (FP64 kernel only)
+    const double *dimsizes;
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
@@ -159,8 +161,20 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
-    if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
+    if (A.pend) {
+        // the previous calibration iteration's factor (k_scale_steps with one step type), then
+        // nothing more once the loop has finished
+        if (A.pend->apply && live) {
+            double st = A.steps[w] * A.pend->fact;
+            for (int i = 0; i < d; ++i) {
+                if (st > 0.5 * A.dimsizes[i]) { st = 0.5 * A.dimsizes[i]; }
+            }
+            if (st < (double)FLT_MIN) { st = (double)FLT_MIN; }
+            A.steps[w] = st;
+        }
+        if (A.pend->done) { return; }
+    }
     const int B = A.blocksize;
     double2 *rec = reinterpret_cast<double2 *>(lds);  // X2: [d][64] {x_i, block sum of x_i^2}
     double *xs = lds;                                  // otherwise [d][64] x_i
@@ -334,7 +348,6 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
-    if (A.skip && *A.skip) { return; }
     const bool live = w < A.W;
     float *xs = lds32;                              // [d][64]
     float *Ss = lds32 + d * WAVE;                   // [d][64] Kahan sums of x_i^2
@@ -482,11 +495,14 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
     return lds_bytes(prec, ndim, x2) <= lds_limit();
 }
 
-int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s, const int *skip)
+int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
+                const CalibCtrl *pend, const double *dimsizes)
 {
     (void)fs;
+    if (pend && (prec != MCI_PREC_FP64 || !dimsizes)) { return MCI_E_INVALID_ARGUMENT; }
     FastArgs A;
-    A.skip = skip;
+    A.pend = pend;
+    A.dimsizes = dimsizes;
     A.W = c.W;
     A.ndim = c.ndim;
     A.nsteps = nsteps;

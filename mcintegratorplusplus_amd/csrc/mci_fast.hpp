@@ -18,8 +18,12 @@ struct FastState
 bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind);
 // the observables of the main sampling fit the specialised kernel (and its LDS)
 bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec);
-// nsteps Metropolis steps; obs == nullptr: no accumulation (decorrelation)
+struct CalibCtrl;
+// nsteps Metropolis steps; obs == nullptr: no accumulation (decorrelation).  pend (FP64 only):
+// the shared calibration's control state; the launch first applies its pending step factor
+// (pend->apply, clamped by dimsizes as k_scale_steps does) and then does nothing more if
+// pend->done.
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
-                const int *skip = nullptr);
+                const CalibCtrl *pend = nullptr, const double *dimsizes = nullptr);
 void fast_release(FastState &fs);
 } // namespace mcid

@@ -150,6 +150,52 @@ __global__ void k_calib_control(const double *chunks, int64_t nch, int64_t W, do
     ctrl->done = (stop_cap || !again) ? 1 : 0;
 }
 
+// k_walker_rates + k_chunk_sum + k_calib_control in one launch: each workgroup sums the rates of
+// its chunk of 256 walkers (the same tree), and the workgroup that arrives last sums the chunks in
+// order and updates the control state.  The arrival counter is a device-scope atomic between
+// device-scope fences (the chunk sums may sit in another XCD's L2); the last workgroup resets it.
+__global__ __launch_bounds__(256) void k_calib_round(const int64_t *acc, const int64_t *rej, int64_t W, double *chunks,
+                                                     int64_t nch, double target, int nfind, CalibCtrl *ctrl, unsigned *arrive)
+{
+    __shared__ double buf[256];
+    __shared__ double sh[1024];
+    __shared__ int last;
+    if (ctrl->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl->apply = 0; }
+        return;
+    }
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double x = 0.0;
+    if (i < W) { x = (acc[i] > 0) ? (double)acc[i] / ((double)acc[i] + (double)rej[i]) : 0.; }
+    buf[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { buf[threadIdx.x] = buf[threadIdx.x] + buf[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        chunks[blockIdx.x] = buf[0];
+        __threadfence();
+        last = (atomicAdd(arrive, 1u) == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!last) { return; }
+    __threadfence();
+    const double sum = ordered_chunk_sum(chunks, nch, sh);
+    if (threadIdx.x != 0) { return; }
+    *arrive = 0u;
+    const double rate = sum / (double)W;
+    ctrl->rate = rate;
+    if (fabs(rate - target) < 0.05) { ++ctrl->cons; }
+    else { ctrl->cons = 0; }
+    ctrl->fact = fmin(2., fmax(0.5, rate / target));
+    ctrl->apply = 1;
+    ++ctrl->counter;
+    const bool stop_cap = (nfind < 0 && ctrl->counter >= -nfind);
+    const bool again = (nfind < 0 && ctrl->cons < 5) || ctrl->counter < nfind;
+    ctrl->done = (stop_cap || !again) ? 1 : 0;
+}
+
 __global__ void k_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -615,6 +661,14 @@ hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int6
 hipError_t launch_scale_steps(const Chain &c, const CalibCtrl *ctrl, const double *dimSizes, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scale_steps, grid1(c.W, 256), dim3(256), 0, s, c, ctrl, dimSizes);
+    return hipGetLastError();
+}
+
+hipError_t launch_calib_round(const int64_t *acc, const int64_t *rej, int64_t W, double *chunks, double target, int nfind,
+                              CalibCtrl *ctrl, unsigned *arrive, hipStream_t s)
+{
+    const int64_t nch = (W + 255) / 256;
+    hipLaunchKernelGGL(k_calib_round, dim3((unsigned)nch), dim3(256), 0, s, acc, rej, W, chunks, nch, target, nfind, ctrl, arrive);
     return hipGetLastError();
 }
 
