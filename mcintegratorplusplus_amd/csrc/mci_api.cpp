@@ -111,6 +111,7 @@ struct mci_ctx
     DBuf<double> oa, oe;  // decorrelation: previous estimates (independent walkers)
     DBuf<CalibCtrl> ctrl;
     DBuf<unsigned> arrive; // k_calib_round's arrival counter (zero between launches)
+    int calib_hint = 0;    // iterations the last shared calibration took: the next one's first batch
     bool walker_results_host = false; // h_wavg/h_werr hold the device's per-walker results
     DBuf<uint64_t> mt, ctr, seeds;
     DBuf<int32_t> mtp;
@@ -359,7 +360,9 @@ mci_status find_shared(mci_ctx *c)
     HIPCHK(hipMemcpyAsync(c->ctrl.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
     const bool fast = fast_chain(c);
     const int maxit = std::abs(N);
-    const int batch = 4;
+    // the first batch is as long as the last calibration of this context took (iterations after
+    // the exit are no-ops, so the batch length changes the time, never the result)
+    int batch = (c->calib_hint > 0) ? c->calib_hint : 4;
     for (int it = 0; it < maxit;) {
         for (int b = 0; b < batch && it < maxit; ++b, ++it) {
             // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
@@ -377,10 +380,14 @@ mci_status find_shared(mci_ctx *c)
                 HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream));
             }
         }
-        int done = 0;
-        HIPCHK(hipMemcpyAsync(&done, &c->ctrl.p->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        CalibCtrl h;
+        HIPCHK(hipMemcpyAsync(&h, c->ctrl.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (done) { break; }
+        if (h.done) {
+            c->calib_hint = std::min(std::max(h.counter, 1), 64);
+            break;
+        }
+        batch = 4;
     }
     // the fast path's last factor is still pending when no sampling launch followed its control
     if (fast) { HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream)); }
