@@ -320,6 +320,30 @@ def test_specialised_kernel_matches_generic_and_oracle(blocksize, coupling):
         np.testing.assert_allclose(f[k], g[k], rtol=RTOL_BLOCKS, atol=1e-300)
 
 
+def test_specialised_shared_calibration_ragged_chunks_and_repeat():
+    """Shared calibration on the specialised path with 1,000 walkers (four chunks of the
+    deterministic walker sum, the last one partial: the fused rates/sums/control launch and its
+    last-arriving workgroup), then a second integrate on the same context (its first calibration
+    batch is sized from the first call): positions, step sizes and acceptance counters equal the
+    oracle's bit for bit after each call, block means and estimates to 1e-12."""
+    W, nmc = 1000, 64 * 20
+    out = {}
+    for name in ("fast", "orc"):
+        c = chain_setup("orc" if name == "orc" else "mci", cfg2(blocksize=20), nwalkers=W)
+        c.setRng(RNG_PHILOX)
+        c.setCoupling(COUPLING_SHARED)
+        calls = []
+        for _ in range(2):
+            a, e = c.integrate(nmc)
+            calls.append((a, e, *c.walkerResults(), c.getWalkerSteps(), c.getX(), *c.walkerAcceptance()))
+        out[name] = calls
+    for f, o in zip(out["fast"], out["orc"]):
+        for k in (4, 5, 6, 7):
+            assert np.array_equal(f[k], o[k]), k
+        for k in (0, 1, 2, 3):
+            np.testing.assert_allclose(f[k], o[k], rtol=RTOL_BLOCKS, atol=1e-300)
+
+
 def test_full_size_configuration2_walker_subset():
     """65,536 walkers x 10,240 steps (d=16): independent walkers 0..63 equal an oracle run of 64
     walkers (walker w's Philox stream depends only on (seed, w)): positions and counters bit for
