@@ -42,7 +42,7 @@ struct FastArgs
     uint64_t *ctr;         // [W]
     double *x64;           // [ndim][W] (FP64 state, also FP32 mode's persistent copy)
     const double *a;       // [ndim] p_i = a_i x_i^2
-    double *steps;         // [W] (one type)
+    double *steps;         // [W] (one type; rescaled in place by a launch with a pending factor)
     int64_t *acc, *rej;    // [W]
     int blocksize;         // B (Simple: nsteps)
     int64_t nblocks;
@@ -196,127 +196,124 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     __syncthreads();
     if (!live) { return; }
     const double step = A.steps[w];
-    const uint64_t t_in = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
-    {
-        uint64_t t = t_in;
-        int64_t nacc = 0;
-        double Sr = 0.0, Se = 0.0, e = 0.0;
-        int64_t b = 0;
-        if (ACC) { Sr = (double)B * r; }
-        if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-        // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
-        // path has no divergent branch: the decision is a select and the LDS state is always
-        // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
-        // number equals the branched form).
-        uint32_t w0, w1, w2, w3;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-        const int64_t chunk = ACC ? (int64_t)B : ((int64_t)1 << 30);  // blocks start at step 0
-        for (int64_t s = 0; s < A.nsteps;) {
-            const int len = (int)(A.nsteps - s < chunk ? A.nsteps - s : chunk);
-            double mult = (double)B;  // B - j: steps of the block the new value counts for
-            uint32_t na = 0;
-            for (int q = 0; q < len; ++q) {
-                const int v = draw_vidx<P2>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
-                const int slot = v * WAVE + lane;
-                double xo, So = 0.0;
-                if (HX2) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
-                else { xo = xs[slot]; }
-                const float cv = cs[v];
-                // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
-                const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
-                const uint32_t u_hi = w2, u_lo = w3;
-                ++t;
-                uint32_t s0, s3;
-                philox_head9(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, s0, s3);
-                const double xn = xo + step * rr;
-                const double xn2 = xn * xn, xo2 = xo * xo;
-                const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
-                // The FP64 test is u <= exp(-ex) with ex = a_v xn^2 - a_v xo^2 (pdf_acceptance's form).
-                // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  The
-                // screen takes F = 2^(32 - a_v log2 e (xn^2 - xo^2)) in FP32: relative error below 2e-5
-                // wherever F >= 1 (ex < 22.2; the FP64 rounding of ex is far smaller), and U = u_hi
-                // rounded to FP32.  U < 0.9999 F - 2 makes the FP64 test certain to accept,
-                // U > 1.0001 F certain to reject; between the two the FP64 test decides.
-                const float U = (float)u_hi;
-                const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, 32.0f));
-                const bool acc = U < fmaf(F, 0.9999f, -2.0f);
-                const bool band = !acc && U <= F * 1.0001f;
-                if (acc) { ++na; }
-                const double xsel = acc ? xn : xo;
-                if (ACC) {
-                    const double dx2 = acc ? dx2n : 0.0;
-                    if (HX2) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
-                    else { xs[slot] = xsel; }
-                    r = r + dx2;
-                    if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
-                    if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
-                }
+    uint64_t t = A.ctr[w];
+    int64_t nacc = 0;
+    double Sr = 0.0, Se = 0.0, e = 0.0;
+    int64_t b = 0;
+    if (ACC) { Sr = (double)B * r; }
+    if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+    // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
+    // path has no divergent branch: the decision is a select and the LDS state is always
+    // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
+    // number equals the branched form).
+    uint32_t w0, w1, w2, w3;
+    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+    const int64_t chunk = ACC ? (int64_t)B : ((int64_t)1 << 30);  // blocks start at step 0
+    for (int64_t s = 0; s < A.nsteps;) {
+        const int len = (int)(A.nsteps - s < chunk ? A.nsteps - s : chunk);
+        double mult = (double)B;  // B - j: steps of the block the new value counts for
+        uint32_t na = 0;
+        for (int q = 0; q < len; ++q) {
+            const int v = draw_vidx<P2>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
+            const int slot = v * WAVE + lane;
+            double xo, So = 0.0;
+            if (HX2) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
+            else { xo = xs[slot]; }
+            const float cv = cs[v];
+            // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
+            const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
+            const uint32_t u_hi = w2, u_lo = w3;
+            ++t;
+            uint32_t s0, s3;
+            philox_head9(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, s0, s3);
+            const double xn = xo + step * rr;
+            const double xn2 = xn * xn, xo2 = xo * xo;
+            const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
+            // The FP64 test is u <= exp(-ex) with ex = a_v xn^2 - a_v xo^2 (pdf_acceptance's form).
+            // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  The
+            // screen takes F = 2^(32 - a_v log2 e (xn^2 - xo^2)) in FP32: relative error below 2e-5
+            // wherever F >= 1 (ex < 22.2; the FP64 rounding of ex is far smaller), and U = u_hi
+            // rounded to FP32.  U < 0.9999 F - 2 makes the FP64 test certain to accept,
+            // U > 1.0001 F certain to reject; between the two the FP64 test decides.
+            const float U = (float)u_hi;
+            const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, 32.0f));
+            const bool acc = U < fmaf(F, 0.9999f, -2.0f);
+            const bool band = !acc && U <= F * 1.0001f;
+            if (acc) { ++na; }
+            const double xsel = acc ? xn : xo;
+            if (ACC) {
+                const double dx2 = acc ? dx2n : 0.0;
+                if (HX2) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
                 else { xs[slot] = xsel; }
-                // the next step's words 0 and 1 (and the state for 2 and 3) are needed here, so the
-                // compiler keeps their computation in this block (instead of sinking it below the
-                // branch, where nothing overlaps it)
-                asm volatile("" ::"v"(w0), "v"(w1), "v"(s0), "v"(s3));
-                // Lanes inside the band went through the update above as rejections (which leaves
-                // every number as it was: +0.0 added to non-negative sums).  When the wave holds
-                // any, the exact test runs and the lanes it accepts are updated as accepted steps,
-                // from the values read at the start of the step.  The common path thus stays one
-                // basic block, in which the next step's Philox words fill the latency of this
-                // step's dependent chain.
-                // (two ballots of plain compares: each is the compare's own lane mask)
-                if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
-                    const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
-                    const double av = as[v];
-                    const double ex = xn2 * av - xo2 * av;
-                    if (band && u <= exp(-ex)) {
-                        na += 1u;
-                        if (ACC) {
-                            const double dx2 = dx2n;
-                            if (HX2) { rec[slot] = make_double2(xn, So + dx2 * mult); }
-                            else { xs[slot] = xn; }
-                            r = r + dx2;
-                            if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
-                            if (HEXP) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
-                        }
+                r = r + dx2;
+                if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
+                if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
+            }
+            else { xs[slot] = xsel; }
+            // the next step's words 0 and 1 (and the state for 2 and 3) are needed here, so the
+            // compiler keeps their computation in this block (instead of sinking it below the
+            // branch, where nothing overlaps it)
+            asm volatile("" ::"v"(w0), "v"(w1), "v"(s0), "v"(s3));
+            // Lanes inside the band went through the update above as rejections (which leaves
+            // every number as it was: +0.0 added to non-negative sums).  When the wave holds
+            // any, the exact test runs and the lanes it accepts are updated as accepted steps,
+            // from the values read at the start of the step.  The common path thus stays one
+            // basic block, in which the next step's Philox words fill the latency of this
+            // step's dependent chain.
+            // (two ballots of plain compares: each is the compare's own lane mask)
+            if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
+                const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
+                const double av = as[v];
+                const double ex = xn2 * av - xo2 * av;
+                if (band && u <= exp(-ex)) {
+                    na += 1u;
+                    if (ACC) {
+                        const double dx2 = dx2n;
+                        if (HX2) { rec[slot] = make_double2(xn, So + dx2 * mult); }
                         else { xs[slot] = xn; }
+                        r = r + dx2;
+                        if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
+                        if (HEXP) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
                     }
+                    else { xs[slot] = xn; }
                 }
-                if (ACC) { mult = mult - 1.0; }
-                // words 2 and 3 after the exact path, which still reads this step's u_hi and u_lo:
-                // they then take over its registers (no copies at the loop's back edge)
-                philox_tail9(s0, s3, A.key1, w2, w3);
             }
-            nacc += na;
-            s += len;
-            if (ACC && len == B) {
-                const double norm = 1. / B;
-                double rx = 0.0, sS = 0.0;
-                for (int i = 0; i < d; ++i) {
-                    double v2;
-                    if (HX2) {
-                        const double2 rv = rec[i * WAVE + lane];
-                        v2 = rv.x;
-                        sS = sS + rv.y;
-                        A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
-                        rec[i * WAVE + lane] = make_double2(v2, (double)B * (v2 * v2));
-                    }
-                    else { v2 = xs[i * WAVE + lane]; }
-                    rx = rx + v2 * v2;
-                }
-                if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : Sr) * norm) / A.x2sum_div; }
-                if (HEXP) { A.d_exp[b * A.W + w] = Se * norm; }
-                r = rx;  // refresh |x|^2 exactly at every block boundary
-                Sr = (double)B * r;
-                if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-                ++b;
-            }
+            if (ACC) { mult = mult - 1.0; }
+            // words 2 and 3 after the exact path, which still reads this step's u_hi and u_lo:
+            // they then take over its registers (no copies at the loop's back edge)
+            philox_tail9(s0, s3, A.key1, w2, w3);
         }
-        for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; }
-        A.ctr[w] = t;
-        A.acc[w] = nacc;
-        A.rej[w] = A.nsteps - nacc;
+        nacc += na;
+        s += len;
+        if (ACC && len == B) {
+            const double norm = 1. / B;
+            double rx = 0.0, sS = 0.0;
+            for (int i = 0; i < d; ++i) {
+                double v2;
+                if (HX2) {
+                    const double2 rv = rec[i * WAVE + lane];
+                    v2 = rv.x;
+                    sS = sS + rv.y;
+                    A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
+                    rec[i * WAVE + lane] = make_double2(v2, (double)B * (v2 * v2));
+                }
+                else { v2 = xs[i * WAVE + lane]; }
+                rx = rx + v2 * v2;
+            }
+            if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : Sr) * norm) / A.x2sum_div; }
+            if (HEXP) { A.d_exp[b * A.W + w] = Se * norm; }
+            r = rx;  // refresh |x|^2 exactly at every block boundary
+            Sr = (double)B * r;
+            if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+            ++b;
+        }
     }
+    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; }
+    A.ctr[w] = t;
+    A.acc[w] = nacc;
+    A.rej[w] = A.nsteps - nacc;
 }
 
 // Kahan-compensated FP32 accumulation: s + y with compensation c
