@@ -438,6 +438,10 @@ def main():
                      "kernel_ms": kt[2], "flops_per_walker_step": FLOPS_PER_STEP,
                      "valu_busy_rocprof": prof.get("valu_busy"),
                      "instructions_per_wave_step_rocprof": (prof.get("instructions_per_walker_step") or {}).get("SQ_INSTS"),
+                     # one wave per SIMD: each instruction of the wave takes an issue slot of 4 cycles;
+                     # the share of the step's cycles those slots fill (rocprof counts)
+                     "issue_slots_frac": (4 * prof["instructions_per_walker_step"]["SQ_INSTS"] / prof["cycles_per_wave_step"])
+                     if "cycles_per_wave_step" in prof else None,
                      "hbm_gbps": (prof["bytes_per_launch"] / (kt[2] / 1000) / 1e9) if "bytes_per_launch" in prof else None},
     }
     if args.workloads != "none" and args.precision == 0 and not args.generic:
