@@ -344,6 +344,29 @@ def test_specialised_shared_calibration_ragged_chunks_and_repeat():
             np.testing.assert_allclose(f[k], o[k], rtol=RTOL_BLOCKS, atol=1e-300)
 
 
+@pytest.mark.parametrize("nfind", [1, 3, 6])
+def test_specialised_shared_calibration_fixed_iterations(nfind):
+    """A fixed number of shared calibration iterations on the specialised path (the loop ends on
+    the counter, its last factor applied either by a following sampling launch or by the step
+    scaling after the loop, depending on where the batch ends): three integrates in a row equal
+    the oracle's positions, step sizes and counters bit for bit."""
+    W, nmc = 320, 64 * 10
+    out = {}
+    for name in ("fast", "orc"):
+        c = chain_setup("orc" if name == "orc" else "mci", cfg2(blocksize=10), nwalkers=W)
+        c.setRng(RNG_PHILOX)
+        c.setCoupling(COUPLING_SHARED)
+        c.setNfindMRT2Iterations(nfind)
+        calls = []
+        for _ in range(3):
+            c.integrate(nmc)
+            calls.append((c.getWalkerSteps(), c.getX(), *c.walkerAcceptance()))
+        out[name] = calls
+    for f, o in zip(out["fast"], out["orc"]):
+        for k in range(4):
+            assert np.array_equal(f[k], o[k]), (nfind, k)
+
+
 def test_full_size_configuration2_walker_subset():
     """65,536 walkers x 10,240 steps (d=16): independent walkers 0..63 equal an oracle run of 64
     walkers (walker w's Philox stream depends only on (seed, w)): positions and counters bit for
