@@ -74,6 +74,9 @@ struct mci_batch
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
+    // graph form: the dimension classes' launches on side branches (fork/join events per phase)
+    hipStream_t side[NCLASS - 1] = {};
+    hipEvent_t fork_ev[2] = {}, join_ev[2][NCLASS - 1] = {};
     DBuf<int> d_nd, d_nc, d_wint;
     DBuf<float> d_lw, d_h, d_sig, d_cw, d_mu;
     DBuf<int64_t> d_moff, d_T, d_woff, d_R;
@@ -98,6 +101,11 @@ struct mci_batch
     {
         drop_graph();
         for (auto &e : ev) { if (e) { (void)hipEventDestroy(e); } }
+        for (auto &e : fork_ev) { if (e) { (void)hipEventDestroy(e); } }
+        for (auto &row : join_ev) {
+            for (auto &e : row) { if (e) { (void)hipEventDestroy(e); } }
+        }
+        for (auto &q : side) { if (q) { (void)hipStreamDestroy(q); } }
         if (stream) { (void)hipStreamDestroy(stream); }
     }
 };
@@ -265,6 +273,11 @@ mci_status ensure_device(mci_batch *b)
     HIPCHK(hipGetDevice(&b->device));
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto &e : b->ev) { HIPCHK(hipEventCreate(&e)); }
+    for (auto &q : b->side) { HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking)); }
+    for (auto &e : b->fork_ev) { HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    for (auto &row : b->join_ev) {
+        for (auto &e : row) { HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    }
     b->dev_ready = true;
     return MCI_OK;
 }
@@ -385,19 +398,43 @@ mcib::BatchParams params(const mci_batch *b)
     return P;
 }
 
-// the phases of one submission on b->stream; events only when not capturing
+// The phases of one submission.  Timed (events, eager): every launch on b->stream in order.
+// Captured (graph): the dimension classes' calibration launches run as parallel branches, and
+// so do their sampling launches, so that the classes' tails overlap.  A sampling item's class
+// is that of the largest d among its walkers, so it may hold walkers of lower classes: all
+// calibrations join before any sampling starts.  Every class writes its own integrals' steps
+// and its own items' results, so the order changes no number.
 mci_status enqueue(mci_batch *b, bool events)
 {
     const mcib::BatchParams P = params(b);
+    const bool branches = !events;
+    auto lane_stream = [&](int c) { return (branches && c > 0) ? b->side[c - 1] : b->stream; };
+    auto fork = [&](int k) -> mci_status {
+        if (!branches) { return MCI_OK; }
+        HIPCHK(hipEventRecord(b->fork_ev[k], b->stream));
+        for (auto &q : b->side) { HIPCHK(hipStreamWaitEvent(q, b->fork_ev[k], 0)); }
+        return MCI_OK;
+    };
+    auto join = [&](int k) -> mci_status {
+        if (!branches) { return MCI_OK; }
+        for (int c = 0; c < NCLASS - 1; ++c) {
+            HIPCHK(hipEventRecord(b->join_ev[k][c], b->side[c]));
+            HIPCHK(hipStreamWaitEvent(b->stream, b->join_ev[k][c], 0));
+        }
+        return MCI_OK;
+    };
     if (events) { HIPCHK(hipEventRecord(b->ev[0], b->stream)); }
+    HIPCHK(mcib::launch_batch_zero(b->d_queue.p, NCLASS, b->stream));
+    TRY(fork(0));
     for (int c = 0; c < NCLASS; ++c) {
         const int n = b->int_off[c + 1] - b->int_off[c];
         if (n == 0) { continue; }
         mcib::CalibArgs ca{P, b->d_ilist.p + b->int_off[c], b->nfind, b->target, b->d_step0.p, b->d_step.p, b->d_rate.p};
-        HIPCHK(mcib::launch_batch_calibrate(ca, n, b->int_dmax[c], b->stream));
+        HIPCHK(mcib::launch_batch_calibrate(ca, n, b->int_dmax[c], lane_stream(c)));
     }
+    TRY(join(0));
     if (events) { HIPCHK(hipEventRecord(b->ev[1], b->stream)); }
-    HIPCHK(mcib::launch_batch_zero(b->d_queue.p, NCLASS, b->stream));
+    TRY(fork(1));
     mcib::SampleArgs sa;
     sa.P = P;
     sa.step = b->d_step.p;
@@ -414,8 +451,9 @@ mci_status enqueue(mci_batch *b, bool events)
         sa.items = b->d_items.p + b->item_off[c];
         sa.nitems = n;
         sa.queue = b->d_queue.p + c;
-        HIPCHK(mcib::launch_batch_sample(sa, b->item_dmax[c], b->nwaves[c], b->stream));
+        HIPCHK(mcib::launch_batch_sample(sa, b->item_dmax[c], b->nwaves[c], lane_stream(c)));
     }
+    TRY(join(1));
     if (events) { HIPCHK(hipEventRecord(b->ev[2], b->stream)); }
     mcid::EstimJob j;
     j.data = b->d_bm.p;

@@ -42,16 +42,24 @@ def test_batch_bit_exact_with_oracle(mode, ndecorr, nblocks):
     assert np.all(np.abs(z) < 5), z
 
 
-def test_batch_graph_replays_equal_eager():
-    ndim, nmc, w, s, mu, _ = small_mixtures(n=32, seed=21, dmax=40, nmc=(1000, 8000))
-    print("graph replay: 32 integrals", flush=True)
+@pytest.mark.parametrize("dmax", [40, 200])
+def test_batch_graph_replays_equal_eager(dmax):
+    """Graph replays equal the eager run bit for bit.  With dmax 200 the integrals fall in all
+    three dimension classes, whose launches the graph runs as parallel branches."""
+    ndim, nmc, w, s, mu, _ = small_mixtures(n=32, seed=21, dmax=dmax, nmc=(1000, 8000))
+    print("graph replay: 32 integrals, dmax", dmax, flush=True)
     eager = _gpu(ndim, nmc, w, s, mu, 5, setGraph=False)
     a0, e0 = eager.run()
+    st0 = eager.steps()
+    bm0 = eager.blockMeans(16)
     g = _gpu(ndim, nmc, w, s, mu, 5, setGraph=True)
     other = _gpu(ndim[:8], nmc[:8], w[:8], s[:8], mu[:8], 6, setGraph=False)
     for _ in range(4):
         a, e = g.run()
         assert np.array_equal(a, a0) and np.array_equal(e, e0)
+        st = g.steps()
+        assert np.array_equal(st[0], st0[0]) and np.array_equal(st[1], st0[1])
+        assert np.array_equal(g.blockMeans(16), bm0)
         other.run()  # unrelated eager work between replays
     assert g.timings()[3] > 0
 
