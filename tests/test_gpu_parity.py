@@ -289,12 +289,13 @@ def test_auto_decorrelation_and_calibration_parity():
 RTOL_BLOCKS = 1e-12  # dwell-time block sums vs step-by-step sums (same FP64 values, other rounding)
 
 
-@pytest.mark.parametrize("blocksize,coupling", [(20, COUPLING_INDEPENDENT), (20, COUPLING_SHARED), (0, COUPLING_INDEPENDENT)])
-def test_specialised_kernel_matches_generic_and_oracle(blocksize, coupling):
+@pytest.mark.parametrize("blocksize,coupling,W", [(20, COUPLING_INDEPENDENT, 256), (20, COUPLING_SHARED, 256),
+                                                 (0, COUPLING_INDEPENDENT, 256), (20, COUPLING_INDEPENDENT, 200)])
+def test_specialised_kernel_matches_generic_and_oracle(blocksize, coupling, W):
     """The LDS/dwell-time kernel makes the same decisions as the generic kernel and the oracle
     (positions, step sizes and acceptance counters bit-identical); block means and estimates
-    agree to 1e-12 relative."""
-    W, nmc = 256, 64 * 20
+    agree to 1e-12 relative.  (200 walkers: the last wave has 8 live lanes.)"""
+    nmc = 64 * 20
     cfg = cfg2(blocksize=blocksize)
     if blocksize == 0:
         cfg["obs"] = [(OBS_X2, [], 0, 1, 0, ESTIM_NOOP), (OBS_X2_SUM, [], 0, 1, 0, ESTIM_NOOP), (OBS_EXP_X2, [16.0], 0, 1, 0, ESTIM_NOOP)]
