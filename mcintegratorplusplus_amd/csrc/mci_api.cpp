@@ -110,7 +110,6 @@ struct mci_ctx
     DBuf<double> x, xn, pold, pnew, steps, lbd, ubd, obsval, dimsizes, wavg, werr, chunks_a, chunks_e, res, rates, fact;
     DBuf<double> oa, oe;  // decorrelation: previous estimates (independent walkers)
     DBuf<CalibCtrl> ctrl;
-    DBuf<unsigned> arrive; // k_calib_round's arrival counter (zero between launches)
     int calib_hint = 0;    // iterations the last shared calibration took: the next one's first batch
     bool walker_results_host = false; // h_wavg/h_werr hold the device's per-walker results
     DBuf<uint64_t> mt, ctr, seeds;
@@ -351,10 +350,6 @@ mci_status find_shared(mci_ctx *c)
     HIPCHK(c->rates.ensure(c->W));
     HIPCHK(c->chunks_a.ensure(nch));
     HIPCHK(c->ctrl.ensure(1));
-    if (!c->arrive.p) {
-        HIPCHK(c->arrive.ensure(1));
-        HIPCHK(hipMemsetAsync(c->arrive.p, 0, sizeof(unsigned), c->stream));
-    }
     CalibCtrl init;
     std::memset(&init, 0, sizeof init);
     HIPCHK(hipMemcpyAsync(c->ctrl.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
@@ -370,7 +365,7 @@ mci_status find_shared(mci_ctx *c)
                 // two launches per iteration: the sampling launch first applies the previous
                 // iteration's factor; rates, sums and control are one launch
                 TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, c->ctrl.p, c->dimsizes.p));
-                HIPCHK(launch_calib_round(c->acc.p, c->rej.p, c->W, c->chunks_a.p, c->target, N, c->ctrl.p, c->arrive.p, c->stream));
+                HIPCHK(launch_calib_round(c->acc.p, c->rej.p, c->W, c->chunks_a.p, c->target, N, c->ctrl.p, c->stream));
             }
             else {
                 HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream, c->ctrl.p));

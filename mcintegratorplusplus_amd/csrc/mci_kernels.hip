@@ -152,10 +152,11 @@ __global__ void k_calib_control(const double *chunks, int64_t nch, int64_t W, do
 
 // k_walker_rates + k_chunk_sum + k_calib_control in one launch: each workgroup sums the rates of
 // its chunk of 256 walkers (the same tree), and the workgroup that arrives last sums the chunks in
-// order and updates the control state.  The arrival counter is a device-scope atomic between
-// device-scope fences (the chunk sums may sit in another XCD's L2); the last workgroup resets it.
+// order and updates the control state.  The arrival counter (ctrl->arrive, zeroed with the
+// control state at the start of each calibration) is a device-scope atomic between device-scope
+// fences (the chunk sums may sit in another XCD's L2); the last workgroup resets it.
 __global__ __launch_bounds__(256) void k_calib_round(const int64_t *acc, const int64_t *rej, int64_t W, double *chunks,
-                                                     int64_t nch, double target, int nfind, CalibCtrl *ctrl, unsigned *arrive)
+                                                     int64_t nch, double target, int nfind, CalibCtrl *ctrl)
 {
     __shared__ double buf[256];
     __shared__ double sh[1024];
@@ -176,14 +177,14 @@ __global__ __launch_bounds__(256) void k_calib_round(const int64_t *acc, const i
     if (threadIdx.x == 0) {
         chunks[blockIdx.x] = buf[0];
         __threadfence();
-        last = (atomicAdd(arrive, 1u) == gridDim.x - 1) ? 1 : 0;
+        last = (atomicAdd(&ctrl->arrive, 1u) == gridDim.x - 1) ? 1 : 0;
     }
     __syncthreads();
     if (!last) { return; }
     __threadfence();
     const double sum = ordered_chunk_sum(chunks, nch, sh);
     if (threadIdx.x != 0) { return; }
-    *arrive = 0u;
+    ctrl->arrive = 0u;
     const double rate = sum / (double)W;
     ctrl->rate = rate;
     if (fabs(rate - target) < 0.05) { ++ctrl->cons; }
@@ -665,10 +666,10 @@ hipError_t launch_scale_steps(const Chain &c, const CalibCtrl *ctrl, const doubl
 }
 
 hipError_t launch_calib_round(const int64_t *acc, const int64_t *rej, int64_t W, double *chunks, double target, int nfind,
-                              CalibCtrl *ctrl, unsigned *arrive, hipStream_t s)
+                              CalibCtrl *ctrl, hipStream_t s)
 {
     const int64_t nch = (W + 255) / 256;
-    hipLaunchKernelGGL(k_calib_round, dim3((unsigned)nch), dim3(256), 0, s, acc, rej, W, chunks, nch, target, nfind, ctrl, arrive);
+    hipLaunchKernelGGL(k_calib_round, dim3((unsigned)nch), dim3(256), 0, s, acc, rej, W, chunks, nch, target, nfind, ctrl);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ struct CalibCtrl
 {
     double rate, fact;
     int cons, counter, done, apply;
+    unsigned arrive;  // k_calib_round's arrival counter (zeroed with the rest at each calibration)
 };
 
 // one estimator pass over one observable's accumulator data [n][nobs][W]
@@ -34,9 +35,9 @@ hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t 
 hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int64_t min_stat, const double *dimSizes, hipStream_t s);
 hipError_t launch_scale_steps(const Chain &c, const CalibCtrl *ctrl, const double *dimSizes, hipStream_t s);
 hipError_t launch_calib_control(const double *chunks, int64_t W, double target, int nfind, CalibCtrl *ctrl, hipStream_t s);
-// rates + chunk sums + control of one calibration iteration in one launch (arrive: a zeroed counter)
+// rates + chunk sums + control of one calibration iteration in one launch
 hipError_t launch_calib_round(const int64_t *acc, const int64_t *rej, int64_t W, double *chunks, double target, int nfind,
-                              CalibCtrl *ctrl, unsigned *arrive, hipStream_t s);
+                              CalibCtrl *ctrl, hipStream_t s);
 hipError_t launch_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W, hipStream_t s);
 hipError_t launch_estimate(const EstimJob &j, hipStream_t s);
 hipError_t launch_chunk_sum(const double *v, int64_t W, int ncomp, int square, double *chunks, hipStream_t s);
