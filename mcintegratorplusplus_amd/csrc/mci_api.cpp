@@ -1223,6 +1223,11 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
         if (c->prec == MCI_PREC_FP32 && !fast_main) {
             return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
         }
+        if (c->prec == MCI_PREC_FP32 && nmc > INT32_MAX) {
+            for (const ObsDev &d : od) {
+                if (d.acc == ACC_SIMPLE) { return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision accumulates at most 2^31 - 1 steps per block (Simple accumulator: per run)"); }
+            }
+        }
     }
     // ---- file output of the main sampling (MCIntegrator.cpp:61-65, 495-542): the generic kernel
     //      records the values every freq steps, the host writes the files afterwards ----

@@ -404,19 +404,40 @@ mcib::BatchParams params(const mci_batch *b)
 // is that of the largest d among its walkers, so it may hold walkers of lower classes: all
 // calibrations join before any sampling starts.  Every class writes its own integrals' steps
 // and its own items' results, so the order changes no number.
+// Joins the side streams of a fork back into b->stream if the enqueue returns between fork and
+// join (a failed launch): during capture, side streams that never join leave the capture
+// unterminated.  Errors of the join itself are ignored; the failure being returned is reported.
+struct ForkGuard
+{
+    mci_batch *b = nullptr;
+    int k = -1;
+    ~ForkGuard()
+    {
+        if (k < 0) { return; }
+        for (int c = 0; c < NCLASS - 1; ++c) {
+            (void)hipEventRecord(b->join_ev[k][c], b->side[c]);
+            (void)hipStreamWaitEvent(b->stream, b->join_ev[k][c], 0);
+        }
+    }
+};
+
 mci_status enqueue(mci_batch *b, bool events)
 {
     const mcib::BatchParams P = params(b);
     const bool branches = !events;
+    ForkGuard open;
     auto lane_stream = [&](int c) { return (branches && c > 0) ? b->side[c - 1] : b->stream; };
     auto fork = [&](int k) -> mci_status {
         if (!branches) { return MCI_OK; }
         HIPCHK(hipEventRecord(b->fork_ev[k], b->stream));
+        open.b = b;
+        open.k = k;
         for (auto &q : b->side) { HIPCHK(hipStreamWaitEvent(q, b->fork_ev[k], 0)); }
         return MCI_OK;
     };
     auto join = [&](int k) -> mci_status {
         if (!branches) { return MCI_OK; }
+        open.k = -1;
         for (int c = 0; c < NCLASS - 1; ++c) {
             HIPCHK(hipEventRecord(b->join_ev[k][c], b->side[c]));
             HIPCHK(hipStreamWaitEvent(b->stream, b->join_ev[k][c], 0));
@@ -491,9 +512,15 @@ mci_status run_once(mci_batch *b)
             const mci_status st = enqueue(b, false);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
-            if (st != MCI_OK) {
+            if (st != MCI_OK || e != hipSuccess) {
                 if (g) { (void)hipGraphDestroy(g); }
-                return st;
+                // a failed capture can leave the side streams unusable: replace them
+                for (auto &q : b->side) {
+                    (void)hipStreamDestroy(q);
+                    q = nullptr;
+                    HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+                }
+                if (st != MCI_OK) { return st; }
             }
             HIPCHK(e);
             b->graph = g;

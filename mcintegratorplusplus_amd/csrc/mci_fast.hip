@@ -21,8 +21,11 @@
 //
 // FP32 precision: coordinates, proposal (24-bit symmetric grid), log-ratio and exp in FP32,
 // acceptance uniform on a 2^-24 grid, block sums as Kahan-compensated FP32 pairs, flushed to
-// FP64 for the FP64 estimators.
+// FP64 for the FP64 estimators.  Every FP32 operation is correctly rounded and exp is the
+// library's own bexp (mci_f32math.hpp), so the CPU oracle reproduces the chain bit for bit.
+#include "mci_f32math.hpp"
 #include "mci_fast.hpp"
+#include "mci_fast_plan.hpp"
 #include "mci_kernels.hpp"
 
 #include <float.h>
@@ -44,7 +47,7 @@ struct FastArgs
     const double *a;       // [ndim] p_i = a_i x_i^2
     double *steps;         // [W] (one type; rescaled in place by a launch with a pending factor)
     int64_t *acc, *rej;    // [W]
-    int blocksize;         // B (Simple: nsteps)
+    int64_t blocksize;     // B (Simple: nsteps)
     int64_t nblocks;
     double *d_x2;          // X2 data [nblocks][ndim][W] or nullptr
     double *d_x2sum;       // X2_SUM data [nblocks][1][W] or nullptr
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         }
         if (A.pend->done) { return; }
     }
-    const int B = A.blocksize;
+    const int64_t B = A.blocksize;
     double2 *rec = reinterpret_cast<double2 *>(lds);  // X2: [d][64] {x_i, block sum of x_i^2}
     double *xs = lds;                                  // otherwise [d][64] x_i
     double *as = lds + (HX2 ? 2 : 1) * d * WAVE;       // [d] a_i
@@ -210,10 +213,13 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     // number equals the branched form).
     uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-    const int64_t chunk = ACC ? (int64_t)B : ((int64_t)1 << 30);  // blocks start at step 0
+    // Blocks start at step 0.  The steps run in chunks of at most 2^30 that never cross a block
+    // end, so the inner loop's counters are 32-bit whatever the block length (a Simple
+    // accumulator's block is the whole run).
+    double mult = (double)B;  // B - j: steps of the block the new value counts for
+    int64_t inblk = 0;        // steps done in the current block
     for (int64_t s = 0; s < A.nsteps;) {
-        const int len = (int)(A.nsteps - s < chunk ? A.nsteps - s : chunk);
-        double mult = (double)B;  // B - j: steps of the block the new value counts for
+        const int len = fast_chunk_len(A.nsteps, s, B, inblk, ACC);
         uint32_t na = 0;
         for (int q = 0; q < len; ++q) {
             const int v = draw_vidx<P2>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
@@ -287,8 +293,9 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         }
         nacc += na;
         s += len;
-        if (ACC && len == B) {
-            const double norm = 1. / B;
+        inblk += len;
+        if (ACC && inblk == B) {
+            const double norm = 1. / (double)B;
             double rx = 0.0, sS = 0.0;
             for (int i = 0; i < d; ++i) {
                 double v2;
@@ -307,6 +314,8 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             r = rx;  // refresh |x|^2 exactly at every block boundary
             Sr = (double)B * r;
             if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+            mult = (double)B;
+            inblk = 0;
             ++b;
         }
     }
@@ -336,7 +345,40 @@ __device__ __forceinline__ void kahan_if(float &s, float &c, float y, bool on)
     c = on ? cc : c;
 }
 
-// FP32 chain (same structure)
+// Kahan step adding the exact product a b: its rounding error (fused multiply-add, exact) goes
+// into the compensation first.  Dwell weights b reach the block length, so an uncompensated
+// FP32 product would lose about 2^-24 b a per accepted change.
+__device__ __forceinline__ void kahan_prod(float &s, float &c, float a, float b)
+{
+    const float p = a * b;
+    const float pe = fmaf(a, b, -p);
+    const float yy = p - (c - pe);
+    const float tt = s + yy;
+    c = (tt - s) - yy;
+    s = tt;
+}
+
+__device__ __forceinline__ void kahan_prod_if(float &s, float &c, float a, float b, bool on)
+{
+    const float p = a * b;
+    const float pe = fmaf(a, b, -p);
+    const float yy = p - (c - pe);
+    const float tt = s + yy;
+    const float cc = (tt - s) - yy;
+    s = on ? tt : s;
+    c = on ? cc : c;
+}
+
+
+// FP32 chain.  Block sums are sums of positive terms: a value (x_i^2, |x|^2 or exp(-|x|^2/c))
+// enters its block sum once, when it is replaced or the block ends, times the steps it was held
+// (its dwell).  Each such term is rounded once and enters a Kahan pair exactly (the product's
+// rounding error, by fused multiply-add, goes into the compensation), so the block mean's
+// relative error stays about 1e-7 whatever the block length.  (Adding the changes x_new^2 -
+// x_old^2 times the remaining steps instead would sum terms much larger than the block sum, of
+// both signs, and lose precision with the block length.)  With X2 a coordinate, its block sum,
+// the sum's compensation and the step of its last change form one 16-byte LDS record
+// [coordinate][lane]: one ds_read_b128 and one ds_write_b128 per step.
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 {
@@ -346,16 +388,16 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
     const bool live = w < A.W;
-    float *xs = lds32;                              // [d][64]
-    float *Ss = lds32 + d * WAVE;                   // [d][64] Kahan sums of x_i^2
-    float *Cs = lds32 + 2 * d * WAVE;               // [d][64] their compensations
-    float *as = lds32 + (HX2 ? 3 : 1) * d * WAVE;   // [d]
+    float4 *rec = reinterpret_cast<float4 *>(lds32); // X2: [d][64] {x_i, block sum of x_i^2, its compensation, step of the last change}
+    float *xs = lds32;                               // otherwise [d][64] x_i
+    float *as = lds32 + (HX2 ? 4 : 1) * d * WAVE;    // [d]
     for (int i = lane; i < d; i += WAVE) { as[i] = (float)A.a[i]; }
-    float r = 0.f, rc = 0.f;                        // |x|^2, Kahan-compensated
+    float r = 0.f, rc = 0.f;                         // |x|^2, Kahan-compensated
     if (live) {
         for (int i = 0; i < d; ++i) {
             const float v = (float)A.x64[(int64_t)i * A.W + w];
-            xs[i * WAVE + lane] = v;
+            if (HX2) { rec[i * WAVE + lane] = make_float4(v, 0.f, 0.f, __int_as_float(0)); }
+            else { xs[i * WAVE + lane] = v; }
             kahan(r, rc, v * v);
         }
     }
@@ -366,74 +408,116 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const uint32_t wid = (uint32_t)w;
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
     int64_t nacc = 0;
-    const int B = A.blocksize;
+    const int B = (int)A.blocksize;  // at most 2^31 - 1 (checked by the host)
     const float inv_ediv = (float)(1.0 / A.exp_div);
+    // block sums of |x|^2 and exp(-|x|^2/c), their compensations, the steps their current values
+    // were set; j is the step within the block
     float Sr = 0.f, Cr = 0.f, Se = 0.f, Ce = 0.f, e = 0.f;
-    int j = 0;
+    int jr = 0, je = 0, j = 0;
     int64_t b = 0;
-    if (HX2) {
-        for (int i = 0; i < d; ++i) { const float v = xs[i * WAVE + lane]; Ss[i * WAVE + lane] = (float)B * (v * v); Cs[i * WAVE + lane] = 0.f; }
-    }
-    if (ACC) { Sr = (float)B * r; }
-    if (HEXP) { e = __expf(-r * inv_ediv); Se = (float)B * e; }
+    if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
     uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-    float mult = (float)B;
-    for (int64_t s = 0; s < A.nsteps; ++s) {
-        const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
-        const int slot = v * WAVE + lane;
-        const float xo = xs[slot];
-        const float asv = as[v];
-        float So = 0.f, Co = 0.f;
-        if (HX2) { So = Ss[slot]; Co = Cs[slot]; }
-        // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24
-        const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
-        const float u = (float)(w2 >> 8) * 0x1p-24f;
-        ++t;
-        philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-        const float xn = fmaf(step, rr, xo);
-        const float dx2 = (xn - xo) * (xn + xo);
-        const float ex = asv * dx2;
-        const bool acc = u <= __expf(-ex);
-        nacc += acc ? 1 : 0;
-        xs[slot] = acc ? xn : xo;
-        if (ACC) {
-            if (HX2) {
-                kahan_if(So, Co, dx2 * mult, acc);
-                Ss[slot] = So;
-                Cs[slot] = Co;
+    // |x|^2 is updated by the accepted changes, whose FP32 roundings add up: it is recomputed
+    // from the position at every block end and at least every FAST32_REFRESH steps
+    for (int64_t s = 0; s < A.nsteps;) {
+        const int len = fast32_chunk_len(A.nsteps, s, B, j, ACC);
+        for (int q = 0; q < len; ++q) {
+            const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
+            const int slot = v * WAVE + lane;
+            float xo, So = 0.f, Co = 0.f;
+            int jl = 0;
+            if (HX2) { const float4 rv = rec[slot]; xo = rv.x; So = rv.y; Co = rv.z; jl = __float_as_int(rv.w); }
+            else { xo = xs[slot]; }
+            const float asv = as[v];
+            // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24
+            const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
+            const float u = (float)(w2 >> 8) * 0x1p-24f;
+            ++t;
+            philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+            const float xn = fmaf(step, rr, xo);
+            const float dx2 = (xn - xo) * (xn + xo);
+            const float ex = asv * dx2;
+            // u <= bexp(-ex), this library's own exp (the CPU evaluates the same operations).  The
+            // hardware exp2 screens it: F = 2^(-ex log2 e) is within 1e-5 relative of bexp(-ex)
+            // wherever bexp is neither 0 nor inf (the rounding of ex log2 e contributes at most
+            // 127 * 9e-8 to the exponent), so u < 0.9998 F accepts and u > 1.0002 F rejects with
+            // certainty; lanes between the two (and F == 0, flushed) take bexp under a wave vote.
+            const float F = __builtin_amdgcn_exp2f(ex * -0x1.715476p+0f);
+            bool acc = u < F * 0.9998f;
+            const bool band = !acc && !(u > F * 1.0002f);
+            if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0, 0)) {
+                if (band) { acc = u <= mcif::bexp(-ex); }
             }
-            kahan_if(r, rc, dx2, acc);
-            kahan_if(Sr, Cr, dx2 * mult, acc);
-            if (HEXP && acc) { const float en = __expf(-r * inv_ediv); kahan(Se, Ce, (en - e) * mult); e = en; }
-            mult = mult - 1.f;
-            if (++j == B) {
+            nacc += acc ? 1 : 0;
+            if (ACC) {
+                if (HX2) {
+                    // the old x_v^2 was held for steps jl .. j-1 of this block
+                    kahan_prod_if(So, Co, xo * xo, (float)(j - jl), acc);
+                    rec[slot] = make_float4(acc ? xn : xo, So, Co, __int_as_float(acc ? j : jl));
+                }
+                else { xs[slot] = acc ? xn : xo; }
+                if (HSUM) {
+                    kahan_prod_if(Sr, Cr, r, (float)(j - jr), acc);
+                    jr = acc ? j : jr;
+                }
+                kahan_if(r, rc, dx2, acc);
+                if (HEXP && acc) { kahan_prod(Se, Ce, e, (float)(j - je)); je = j; e = mcif::bexp(-r * inv_ediv); }
+                ++j;
+            }
+            else { xs[slot] = acc ? xn : xo; }
+        }
+        s += len;
+        if (ACC) {
+            if (j == B) {
+                // the values held at the block end count until it; flush, restart from |x|^2 of
+                // the position
                 const double norm = 1. / B;
                 float rx = 0.f, rxc = 0.f;
                 for (int i = 0; i < d; ++i) {
-                    const float v2 = xs[i * WAVE + lane];
-                    const float q = v2 * v2;
-                    kahan(rx, rxc, q);
+                    float v2;
                     if (HX2) {
-                        A.d_x2[(b * d + i) * A.W + w] = ((double)Ss[i * WAVE + lane] - (double)Cs[i * WAVE + lane]) * norm;
-                        Ss[i * WAVE + lane] = (float)B * q;
-                        Cs[i * WAVE + lane] = 0.f;
+                        const float4 rv = rec[i * WAVE + lane];
+                        v2 = rv.x;
+                        float Si = rv.y, Ci = rv.z;
+                        kahan_prod(Si, Ci, v2 * v2, (float)(B - __float_as_int(rv.w)));
+                        A.d_x2[(b * d + i) * A.W + w] = ((double)Si - (double)Ci) * norm;
+                        rec[i * WAVE + lane] = make_float4(v2, 0.f, 0.f, __int_as_float(0));
                     }
+                    else { v2 = xs[i * WAVE + lane]; }
+                    kahan(rx, rxc, v2 * v2);
                 }
-                if (HSUM) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
-                if (HEXP) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
+                if (HSUM) {
+                    kahan_prod(Sr, Cr, r, (float)(B - jr));
+                    A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div;
+                }
+                if (HEXP) {
+                    kahan_prod(Se, Ce, e, (float)(B - je));
+                    A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm;
+                }
                 r = rx;
                 rc = rxc;
-                Sr = (float)B * r;
-                Cr = 0.f;
-                if (HEXP) { e = __expf(-r * inv_ediv); Se = (float)B * e; Ce = 0.f; }
-                j = 0;
-                mult = (float)B;
+                Sr = Cr = Se = Ce = 0.f;
+                jr = je = j = 0;
+                if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
                 ++b;
+            }
+            else {
+                // refresh |x|^2 inside a long block: the values it replaces end their dwell here
+                float rx = 0.f, rxc = 0.f;
+                for (int i = 0; i < d; ++i) {
+                    const float v2 = HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane];
+                    kahan(rx, rxc, v2 * v2);
+                }
+                if (HSUM) { kahan_prod(Sr, Cr, r, (float)(j - jr)); jr = j; }
+                if (HEXP) { kahan_prod(Se, Ce, e, (float)(j - je)); je = j; }
+                r = rx;
+                rc = rxc;
+                if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
             }
         }
     }
-    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)xs[i * WAVE + lane]; }
+    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)(HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]); }
     A.ctr[w] = t;
     A.acc[w] = nacc;
     A.rej[w] = A.nsteps - nacc;
@@ -442,7 +526,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 size_t lds_bytes(int prec, int ndim, bool x2)
 {
     if (prec == MCI_PREC_FP64) { return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + ndim) + sizeof(float) * ndim; }
-    return sizeof(float) * ((size_t)(x2 ? 3 : 1) * ndim * WAVE + ndim);
+    return sizeof(float) * ((size_t)(x2 ? 4 : 1) * ndim * WAVE + ndim);
 }
 
 size_t lds_limit()
@@ -525,13 +609,14 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     int fl = 0;
     for (int i = 0; i < nobs && obs; ++i) {
         const ObsDev &o = obs[i];
-        A.blocksize = (o.acc == ACC_SIMPLE) ? (int)nsteps : o.blocksize;
+        A.blocksize = (o.acc == ACC_SIMPLE) ? nsteps : (int64_t)o.blocksize;
         A.nblocks = o.nstore;
         if (o.kind == MCI_OBS_X2) { A.d_x2 = o.data; fl |= F_X2; }
         if (o.kind == MCI_OBS_X2_SUM) { A.d_x2sum = o.data; A.x2sum_div = o.p0; fl |= F_SUM; }
         if (o.kind == MCI_OBS_EXP_X2) { A.d_exp = o.data; A.exp_div = o.p0; fl |= F_EXP; }
     }
     if (!A.a) { return MCI_E_INVALID_ARGUMENT; } // the fast path needs explicit weights
+    if (prec != MCI_PREC_FP64 && A.blocksize > INT32_MAX) { return MCI_E_INVALID_ARGUMENT; }
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
     const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0);
     if (A.vshift != 0) { fl |= F_P2; }

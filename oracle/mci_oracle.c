@@ -215,6 +215,7 @@ typedef struct {
     int64_t nsteps, stepidx;
     int skipidx, flag_final;
     double *data;
+    double *shadow;   /* FP32 chain only: the same block means summed step by step in FP64 */
     int64_t nstore;   /* Simple: 1 row; Full: NAccu; Block: nblocks */
     int bidx;
     int64_t storeidx;
@@ -245,6 +246,7 @@ struct orc_ctx {
     int ndim;
     int64_t W;
     int rng, prec, coupling;
+    int fp32_shadow;   /* FP32 chain: also sum the block means step by step in FP64 */
     uint64_t seed;
     uint64_t *wseeds;
     double target;
@@ -680,7 +682,7 @@ static void accu_reset(accu *a) /* AccumulatorInterface.cpp:123-127 + child _res
 
 static void accu_free(accu *a)
 {
-    free(a->obs_values); free(a->flags_xchanged); free(a->data);
+    free(a->obs_values); free(a->flags_xchanged); free(a->data); free(a->shadow);
     memset(a, 0, sizeof *a);
 }
 
@@ -1222,8 +1224,14 @@ mci_status orc_set_rng(orc_ctx *c, int k)
 
 mci_status orc_set_precision(orc_ctx *c, int p)
 {
-    if (p != MCI_PREC_FP64) return fail(MCI_E_INVALID_ARGUMENT, "the oracle restates the FP64 chain only");
+    if (p != MCI_PREC_FP64 && p != MCI_PREC_FP32) return fail(MCI_E_DOMAIN, "unknown precision %d", p);
     c->prec = p;
+    return MCI_OK;
+}
+
+mci_status orc_set_fp32_shadow(orc_ctx *c, int on)
+{
+    c->fp32_shadow = (on != 0);
     return MCI_OK;
 }
 
@@ -1776,6 +1784,220 @@ static mci_status sample_obs(orc_ctx *c, walker *w, int64_t n, accu *accs, const
     return MCI_OK;
 }
 
+/* ------------------------------------------------------------------------- */
+/* FP32 precision form (MCI_PREC_FP32).  The reference computes in FP64 only; this is the chain
+ * of MCI::sample / doStepMRT2 (MCIntegrator.cpp:290-360) with SRRDVecMove's single-coordinate
+ * uniform move (SRRDVecMove.hpp:75-96), one Gauss-form sampling function (p_i = a_i x_i^2,
+ * TestMCIFunctions.hpp:151-190) and the X2 / X2_SUM / EXP_X2 observables into Block or Simple
+ * accumulators (BlockAccumulator.cpp:20-41, SimpleAccumulator.cpp:14-28), carried out in FP32 as
+ * the library's FP32 kernel does it:
+ *  - per step one Philox block (t, walker): word 0 the index (Lemire, extra words on rejection),
+ *    word 1 the proposal on the symmetric 24-bit grid, the high 24 bits of word 2 the uniform
+ *    u on the 2^-24 grid; x' = fmaf(step, r, x); accept iff u <= bexp(-a_v (x' - x)(x' + x));
+ *  - block sums by dwell time, as sums of positive terms: a value (x_i^2, |x|^2 or
+ *    exp(-|x|^2/c)) enters its block sum once, when it is replaced or the block ends, times the
+ *    steps it was held; the sums are Kahan-compensated FP32 pairs and each product enters
+ *    exactly (its rounding error, by fused multiply-add, goes into the compensation); |x|^2 is
+ *    itself a Kahan pair updated by the accepted changes; at a block end the sums flush to FP64
+ *    block means ((S - C) / B) and |x|^2 restarts from the position.
+ * Calibration and the automatic decorrelation stay on the FP64 chain.  With c->fp32_shadow the
+ * accumulators also get the block means of the same FP32 trajectory summed step by step in FP64
+ * (the reference's BlockAccumulator applied to the FP32 chain), which bounds the FP32 sums' error. */
+/* ------------------------------------------------------------------------- */
+static void kahan32(float *s, float *cc, float y)
+{
+    const float yy = y - *cc;
+    const float tt = *s + yy;
+    *cc = (tt - *s) - yy;
+    *s = tt;
+}
+
+static void kahan32_if(float *s, float *cc, float y, int on)
+{
+    const float yy = y - *cc;
+    const float tt = *s + yy;
+    const float c2 = (tt - *s) - yy;
+    if (on) { *s = tt; *cc = c2; }
+}
+
+/* Kahan step adding the exact product a b (its rounding error, exact by fused multiply-add,
+ * enters the compensation first) */
+static void kahan32_prod(float *s, float *cc, float a, float b, int on)
+{
+    const float p = a * b;
+    const float pe = fmaf(a, b, -p);
+    const float yy = p - (*cc - pe);
+    const float tt = *s + yy;
+    const float c2 = (tt - *s) - yy;
+    if (on) { *s = tt; *cc = c2; }
+}
+
+static float sym24f(uint32_t wd) { return (float)(2 * (int32_t)(wd >> 8) + 1 - 16777216) * 0x1p-24f; }
+
+/* the configuration the FP32 chain supports (the library refuses the others alike) */
+static mci_status fp32_supported(const orc_ctx *c, const obs_desc *obs, int nobs)
+{
+    if (c->rng != MCI_RNG_PHILOX || c->movetype != MCI_MOVE_VEC || c->veclen != 1 || c->ntypes != 1 || c->periodic ||
+        c->npdf != 1 || c->pdf[0].kind != MCI_PDF_GAUSS_DIAG || c->pdf[0].nparams < c->ndim || c->ms_on || c->srrd != MCI_SRRD_UNIFORM)
+        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports Philox streams with single-coordinate uniform moves on one diagonal "
+                                            "Gaussian (explicit weights) in an unbounded domain");
+    int seen = 0, B = -1;
+    for (int i = 0; i < nobs; ++i) {
+        const obs_desc *o = &obs[i];
+        const int bs = o->blocksize == 0 ? -2 : o->blocksize;
+        if (o->nskip != 1 || o->blocksize == 1 || (o->kind != MCI_OBS_X2 && o->kind != MCI_OBS_X2_SUM && o->kind != MCI_OBS_EXP_X2) ||
+            (seen & (1 << o->kind)) || (B != -1 && B != bs))
+            return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
+        seen |= 1 << o->kind;
+        B = bs;
+    }
+    return MCI_OK;
+}
+
+/* n FP32 steps of walker w; accs/obs (nobs of them) may be NULL/0 (no accumulation) */
+static mci_status sample_fp32(orc_ctx *c, walker *w, int64_t n, accu *accs, const obs_desc *obs, int nobs)
+{
+    TRY(fp32_supported(c, obs, nobs));
+    const int d = c->ndim;
+    const int acc_on = (nobs > 0);
+    int ix2 = -1, isum = -1, iexp = -1;
+    int64_t B = n;
+    double x2sum_div = 1.0, exp_div = (double)d;
+    for (int i = 0; i < nobs; ++i) {
+        if (obs[i].kind == MCI_OBS_X2) ix2 = i;
+        if (obs[i].kind == MCI_OBS_X2_SUM) { isum = i; x2sum_div = obs_param(&obs[i], 0, 1.0); }
+        if (obs[i].kind == MCI_OBS_EXP_X2) { iexp = i; exp_div = obs_param(&obs[i], 0, (double)d); }
+        B = (obs[i].blocksize == 0) ? n : obs[i].blocksize;
+    }
+    if (acc_on && B > INT32_MAX)
+        return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision accumulates at most 2^31 - 1 steps per block (Simple accumulator: per run)");
+    const int Bi = (int)B;
+    const int64_t nblk = acc_on ? n / B : 0;
+    for (int i = 0; i < nobs; ++i) {
+        accu *a = &accs[i];
+        if (a->nstore != nblk) return fail(MCI_E_RUNTIME, "FP32 chain: accumulator rows %lld, blocks %lld", (long long)a->nstore, (long long)nblk);
+        memset(a->data, 0, sizeof(double) * (size_t)(a->nstore * a->nobs));
+        free(a->shadow);
+        a->shadow = c->fp32_shadow ? calloc((size_t)(a->nstore * a->nobs), sizeof(double)) : NULL;
+    }
+    float *xs = malloc(sizeof(float) * (size_t)d), *as = malloc(sizeof(float) * (size_t)d);
+    float *S = calloc((size_t)d, sizeof(float)), *Cm = calloc((size_t)d, sizeof(float));
+    int *jl = calloc((size_t)d, sizeof(int));           /* step of the coordinate's last change */
+    double *shx = calloc((size_t)d + 2, sizeof(double)); /* shadow sums: x_i^2 [d], |x|^2, exp */
+    for (int i = 0; i < d; ++i) { xs[i] = (float)w->xold[i]; as[i] = (float)pdf_a(&c->pdf[0], i); }
+    const float step = (float)w->steps[0];
+    const float inv_ediv = (float)(1.0 / exp_div);
+    float r = 0.f, rc = 0.f;
+    for (int i = 0; i < d; ++i) kahan32(&r, &rc, xs[i] * xs[i]);
+    float Sr = 0.f, Cr = 0.f, Se = 0.f, Ce = 0.f, e = 0.f;
+    int jr = 0, je = 0, j = 0;
+    if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
+    const uint32_t range = (uint32_t)c->nvecs;
+    const uint32_t key[2] = { (uint32_t)c->seed, (uint32_t)(c->seed >> 32) };
+    int64_t b = 0, nacc = 0, chunk_start = 0;
+    const double norm = 1. / (double)Bi;
+    for (int64_t s = 0; s < n; ++s) {
+        uint32_t ctr[4] = { 0u, w->wid, (uint32_t)w->t, (uint32_t)(w->t >> 32) }, wd[4];
+        orc_philox4x32_10(ctr, key, wd);
+        /* index: Lemire on word 0, rejected draws from the extra-word blocks (draw_index) */
+        uint64_t prod = (uint64_t)wd[0] * range;
+        uint32_t low = (uint32_t)prod;
+        if (low < range) {
+            const uint32_t thr = (0u - range) % range;
+            w->kx = 0;
+            while (low < thr) {
+                prod = (uint64_t)ph_extra(c, w) * range;
+                low = (uint32_t)prod;
+            }
+        }
+        const int v = (int)(prod >> 32);
+        const float xo = xs[v];
+        const float rr = sym24f(wd[1]);
+        const float u = (float)(wd[2] >> 8) * 0x1p-24f;
+        ++w->t;
+        const float xn = fmaf(step, rr, xo);
+        const float dx2 = (xn - xo) * (xn + xo);
+        const float ex = as[v] * dx2;
+        const int acc = (u <= orc_bexp(-ex));
+        nacc += acc;
+        if (acc_on) {
+            if (ix2 >= 0) { /* the old x_v^2 was held for steps jl .. j-1 */
+                kahan32_prod(&S[v], &Cm[v], xo * xo, (float)(j - jl[v]), acc);
+                if (acc) jl[v] = j;
+            }
+            if (isum >= 0) {
+                kahan32_prod(&Sr, &Cr, r, (float)(j - jr), acc);
+                if (acc) jr = j;
+            }
+            kahan32_if(&r, &rc, dx2, acc);
+            if (iexp >= 0 && acc) { kahan32_prod(&Se, &Ce, e, (float)(j - je), 1); je = j; e = orc_bexp(-r * inv_ediv); }
+        }
+        if (acc) xs[v] = xn;
+        if (acc_on) {
+            if (c->fp32_shadow) { /* FP64, step by step, over the same positions */
+                double q = 0.0;
+                for (int i = 0; i < d; ++i) { const double xi = (double)xs[i]; shx[i] = shx[i] + xi * xi; q = q + xi * xi; }
+                shx[d] = shx[d] + q;
+                shx[d + 1] = shx[d + 1] + exp(-(q / exp_div));
+            }
+            ++j;
+            /* |x|^2 is recomputed from the position at every block end and at least every
+             * 1024 steps (the kernel's chunks, FAST32_REFRESH in mci_fast_plan.hpp) */
+            const int chunk_end = (s + 1 == n) || (j == Bi) || ((s + 1 - chunk_start) == 1024);
+            if (chunk_end && j != Bi) {
+                float rx = 0.f, rxc = 0.f;
+                for (int i = 0; i < d; ++i) kahan32(&rx, &rxc, xs[i] * xs[i]);
+                if (isum >= 0) { kahan32_prod(&Sr, &Cr, r, (float)(j - jr), 1); jr = j; }
+                if (iexp >= 0) { kahan32_prod(&Se, &Ce, e, (float)(j - je), 1); je = j; }
+                r = rx;
+                rc = rxc;
+                if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
+            }
+            if (chunk_end) chunk_start = s + 1;
+            if (j == Bi) {
+                float rx = 0.f, rxc = 0.f;
+                for (int i = 0; i < d; ++i) {
+                    const float q = xs[i] * xs[i];
+                    if (ix2 >= 0) {
+                        kahan32_prod(&S[i], &Cm[i], q, (float)(Bi - jl[i]), 1);
+                        accs[ix2].data[b * d + i] = ((double)S[i] - (double)Cm[i]) * norm;
+                        S[i] = Cm[i] = 0.f;
+                        jl[i] = 0;
+                    }
+                    kahan32(&rx, &rxc, q);
+                }
+                if (isum >= 0) {
+                    kahan32_prod(&Sr, &Cr, r, (float)(Bi - jr), 1);
+                    accs[isum].data[b] = (((double)Sr - (double)Cr) * norm) / x2sum_div;
+                }
+                if (iexp >= 0) {
+                    kahan32_prod(&Se, &Ce, e, (float)(Bi - je), 1);
+                    accs[iexp].data[b] = ((double)Se - (double)Ce) * norm;
+                }
+                if (c->fp32_shadow) {
+                    if (ix2 >= 0) for (int i = 0; i < d; ++i) accs[ix2].shadow[b * d + i] = shx[i] * norm;
+                    if (isum >= 0) accs[isum].shadow[b] = (shx[d] * norm) / x2sum_div;
+                    if (iexp >= 0) accs[iexp].shadow[b] = shx[d + 1] * norm;
+                    memset(shx, 0, sizeof(double) * ((size_t)d + 2));
+                }
+                r = rx;
+                rc = rxc;
+                Sr = Cr = Se = Ce = 0.f;
+                jr = je = j = 0;
+                if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
+                ++b;
+            }
+        }
+    }
+    for (int i = 0; i < d; ++i) { w->xold[i] = (double)xs[i]; w->xnew[i] = w->xold[i]; }
+    w->acc = nacc;
+    w->rej = n - nacc;
+    w->accepted = 0;
+    for (int i = 0; i < nobs; ++i) { accs[i].stepidx = accs[i].nsteps; accs[i].flag_final = 1; }
+    free(xs); free(as); free(S); free(Cm); free(jl); free(shx);
+    return MCI_OK;
+}
+
 static double walker_rate(const walker *w) /* MCI::getAcceptanceRate MCIntegrator.cpp:587-592 */
 {
     return (w->acc > 0) ? (double)w->acc / ((double)w->acc + (double)w->rej) : 0.;
@@ -1843,7 +2065,10 @@ static void reduce_avg_err(int nd, int64_t R, const double *wa, const double *we
 static mci_status decorrelate(orc_ctx *c, int64_t w0, int64_t w1, int64_t nranks)
 {
     if (c->ndecorr > 0) {
-        for (int64_t k = w0; k < w1; ++k) sample_plain(c, &c->w[k], c->ndecorr);
+        for (int64_t k = w0; k < w1; ++k) {
+            if (c->prec == MCI_PREC_FP32) TRY(sample_fp32(c, &c->w[k], c->ndecorr, NULL, NULL, 0));
+            else sample_plain(c, &c->w[k], c->ndecorr);
+        }
         return MCI_OK;
     }
     if (c->ndecorr == 0) return MCI_OK;
@@ -1964,8 +2189,10 @@ mci_status orc_integrate(orc_ctx *c, int64_t nmc, double *average, double *error
     c->werr = calloc((size_t)W * (size_t)(nd > 0 ? nd : 1), sizeof(double));
     c->nobsdim_last = nd;
     c->trace_n = 0;
-    for (int64_t k = 0; k < W; ++k)
-        TRY(sample_obs(c, &c->w[k], nmc, &c->acc_last[k * c->nobs], c->obs, c->nobs, k == 0));
+    for (int64_t k = 0; k < W; ++k) {
+        if (c->prec == MCI_PREC_FP32) TRY(sample_fp32(c, &c->w[k], nmc, &c->acc_last[k * c->nobs], c->obs, c->nobs));
+        else TRY(sample_obs(c, &c->w[k], nmc, &c->acc_last[k * c->nobs], c->obs, c->nobs, k == 0));
+    }
     /* ObservableContainer::estimate ObservableContainer.cpp:104-111 */
     for (int64_t k = 0; k < W; ++k) {
         int off = 0;
@@ -2016,6 +2243,19 @@ mci_status orc_get_walker_acceptance(orc_ctx *c, int64_t *acc, int64_t *rej, int
 }
 
 mci_status orc_get_warnings(orc_ctx *c, int64_t *n) { *n = c->nwarn; return MCI_OK; }
+
+mci_status orc_get_shadow_data(orc_ctx *c, int obs, int64_t walker, double *data, int64_t n, int64_t *nstore)
+{
+    if (!c->acc_last || obs < 0 || obs >= c->acc_last_nobs || walker < 0 || walker >= c->W)
+        return fail(MCI_E_INVALID_ARGUMENT, "no such accumulator");
+    accu *a = &c->acc_last[walker * c->acc_last_nobs + obs];
+    if (!a->shadow) return fail(MCI_E_RUNTIME, "no FP64 shadow: the last integrate did not run the FP32 chain with the shadow on");
+    *nstore = a->nstore;
+    int64_t m = a->nstore * a->nobs;
+    if (n < m) return fail(MCI_E_INVALID_ARGUMENT, "output too small");
+    memcpy(data, a->shadow, sizeof(double) * (size_t)m);
+    return MCI_OK;
+}
 
 mci_status orc_get_accu_data(orc_ctx *c, int obs, int64_t walker, double *data, int64_t n, int64_t *nstore)
 {

@@ -84,6 +84,10 @@ mci_status orc_get_warnings(orc_ctx *c, int64_t *nwarn);
 /* per-step trace of walker 0 during the next integrate's main sampling:
  * x after each step ([nsteps][ndim]) and the accepted flag; capacity in steps */
 mci_status orc_trace_enable(orc_ctx *c, double *xbuf, uint8_t *accbuf, int64_t capacity);
+/* FP32 chain (MCI_PREC_FP32): also accumulate the block means of the same FP32 trajectory step by
+ * step in FP64 (the reference's BlockAccumulator on the FP32 chain), read by orc_get_shadow_data */
+mci_status orc_set_fp32_shadow(orc_ctx *c, int on);
+mci_status orc_get_shadow_data(orc_ctx *c, int obs, int64_t walker, double *data, int64_t n, int64_t *nstore);
 /* the raw (not yet finalised) accumulator data of observable i, walker w after the last integrate */
 mci_status orc_get_accu_data(orc_ctx *c, int obs, int64_t walker, double *data, int64_t n, int64_t *nstore);
 

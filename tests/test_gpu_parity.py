@@ -407,34 +407,3 @@ def test_full_size_shared_coupling_physics():
     sig, _ = sigma_a(16)
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
     assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
-
-
-def cfg3(prec_walkers=262144):
-    sig, a = sigma_a(64)
-    return dict(ndim=64, seed=4242, ndecorr=1000, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, a)],
-                obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_EXP_X2, [64.0], 160, 1, 1, ESTIM_MJ)])
-
-
-def test_fp32_precision_form_full_size():
-    """Configuration 3: d=64, 262,144 walkers x 10,240 steps, FP32 state/proposal/exp with Kahan
-    FP32 block sums, FP64 calibration, estimators and combine.  The FP32 chain and the FP64
-    chain driven by the same Philox words stay coupled; their estimates differ by far less than
-    the error bar, and both agree with the analytic values."""
-    W, nmc = 262144, 10240
-    sig, _ = sigma_a(64)
-    exact = np.concatenate([sig ** 2, [np.prod((1 + 2 * sig ** 2 / 64) ** -0.5)]])
-    res = {}
-    for prec in (0, 1):
-        c = chain_setup("mci", cfg3(), nwalkers=W)
-        c.setRng(RNG_PHILOX)
-        c.setCoupling(COUPLING_SHARED)
-        c.setPrecision(prec)
-        res[prec] = c.integrate(nmc)
-    a64, e64 = res[0]
-    a32, e32 = res[1]
-    for a, e in ((a64, e64), (a32, e32)):
-        assert np.all(np.abs(a - exact) < 4 * e), np.max(np.abs(a - exact) / e)
-    rel = np.abs(a32 - a64) / np.abs(a64)
-    print("FP32 vs FP64 (coupled chains): max relative difference", rel.max(), "median", np.median(rel),
-          "; max |diff|/err", np.max(np.abs(a32 - a64) / e64), flush=True)
-    assert np.all(np.abs(a32 - a64) < 0.25 * e64)
