@@ -370,15 +370,26 @@ __device__ __forceinline__ void kahan_prod_if(float &s, float &c, float a, float
 }
 
 
+// Compensations of the FP32 block sums are stored in BF16 (round to nearest even): a Kahan
+// compensation is below half an ulp of its sum, so its top 8 bits keep the pair exact to about
+// 2^-33 of the sum, and the 16 bits left over hold the chunk step of the coordinate's last change.
+__device__ __forceinline__ uint32_t bf16_bits(float v)
+{
+    const uint32_t u = __float_as_uint(v);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
 // FP32 chain.  Block sums are sums of positive terms: a value (x_i^2, |x|^2 or exp(-|x|^2/c))
-// enters its block sum once, when it is replaced or the block ends, times the steps it was held
+// enters its block sum once, when it is replaced or its chunk ends, times the steps it was held
 // (its dwell).  Each such term is rounded once and enters a Kahan pair exactly (the product's
 // rounding error, by fused multiply-add, goes into the compensation), so the block mean's
 // relative error stays about 1e-7 whatever the block length.  (Adding the changes x_new^2 -
 // x_old^2 times the remaining steps instead would sum terms much larger than the block sum, of
-// both signs, and lose precision with the block length.)  With X2 a coordinate, its block sum,
-// the sum's compensation and the step of its last change form one 16-byte LDS record
-// [coordinate][lane]: one ds_read_b128 and one ds_write_b128 per step.
+// both signs, and lose precision with the block length.)  A run is cut into chunks of at most
+// FAST32_REFRESH steps that never cross a block end; every chunk end closes all dwells and
+// recomputes |x|^2 from the position.  With X2 the per-coordinate state is 12 bytes in three
+// [coordinate][lane] arrays: x_i, the block sum of x_i^2, and (BF16 compensation, step of the last
+// change in the chunk) -- 48 KiB at d = 64, so three workgroups share a CU.
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 {
@@ -388,16 +399,17 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
     const bool live = w < A.W;
-    float4 *rec = reinterpret_cast<float4 *>(lds32); // X2: [d][64] {x_i, block sum of x_i^2, its compensation, step of the last change}
-    float *xs = lds32;                               // otherwise [d][64] x_i
-    float *as = lds32 + (HX2 ? 4 : 1) * d * WAVE;    // [d]
+    float *xs = lds32;                                                // [d][64] x_i
+    float *Ss = lds32 + d * WAVE;                                     // X2: [d][64] block sums of x_i^2
+    uint32_t *CJ = reinterpret_cast<uint32_t *>(lds32 + 2 * d * WAVE); // X2: [d][64] BF16 compensation << 16 | last change
+    float *as = lds32 + (HX2 ? 3 : 1) * d * WAVE;                     // [d]
     for (int i = lane; i < d; i += WAVE) { as[i] = (float)A.a[i]; }
-    float r = 0.f, rc = 0.f;                         // |x|^2, Kahan-compensated
+    float r = 0.f, rc = 0.f;                                          // |x|^2, Kahan-compensated
     if (live) {
         for (int i = 0; i < d; ++i) {
             const float v = (float)A.x64[(int64_t)i * A.W + w];
-            if (HX2) { rec[i * WAVE + lane] = make_float4(v, 0.f, 0.f, __int_as_float(0)); }
-            else { xs[i * WAVE + lane] = v; }
+            xs[i * WAVE + lane] = v;
+            if (HX2) { Ss[i * WAVE + lane] = 0.f; CJ[i * WAVE + lane] = 0u; }
             kahan(r, rc, v * v);
         }
     }
@@ -410,25 +422,23 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     int64_t nacc = 0;
     const int B = (int)A.blocksize;  // at most 2^31 - 1 (checked by the host)
     const float inv_ediv = (float)(1.0 / A.exp_div);
-    // block sums of |x|^2 and exp(-|x|^2/c), their compensations, the steps their current values
-    // were set; j is the step within the block
+    // block sums of |x|^2 and exp(-|x|^2/c) with their compensations; jr, je: the chunk steps
+    // at which their current values were set; j: steps done in the block
     float Sr = 0.f, Cr = 0.f, Se = 0.f, Ce = 0.f, e = 0.f;
     int jr = 0, je = 0, j = 0;
     int64_t b = 0;
     if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
     uint32_t w0, w1, w2, w3;
     philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
-    // |x|^2 is updated by the accepted changes, whose FP32 roundings add up: it is recomputed
-    // from the position at every block end and at least every FAST32_REFRESH steps
     for (int64_t s = 0; s < A.nsteps;) {
         const int len = fast32_chunk_len(A.nsteps, s, B, j, ACC);
         for (int q = 0; q < len; ++q) {
             const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
             const int slot = v * WAVE + lane;
-            float xo, So = 0.f, Co = 0.f;
-            int jl = 0;
-            if (HX2) { const float4 rv = rec[slot]; xo = rv.x; So = rv.y; Co = rv.z; jl = __float_as_int(rv.w); }
-            else { xo = xs[slot]; }
+            const float xo = xs[slot];
+            float So = 0.f, Co = 0.f;
+            uint32_t cj = 0u;
+            if (HX2) { So = Ss[slot]; cj = CJ[slot]; Co = __uint_as_float(cj & 0xFFFF0000u); }
             const float asv = as[v];
             // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24
             const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
@@ -450,74 +460,66 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
                 if (band) { acc = u <= mcif::bexp(-ex); }
             }
             nacc += acc ? 1 : 0;
+            xs[slot] = acc ? xn : xo;
             if (ACC) {
                 if (HX2) {
-                    // the old x_v^2 was held for steps jl .. j-1 of this block
-                    kahan_prod_if(So, Co, xo * xo, (float)(j - jl), acc);
-                    rec[slot] = make_float4(acc ? xn : xo, So, Co, __int_as_float(acc ? j : jl));
+                    // the old x_v^2 was held since the chunk step in the low half of cj
+                    const int jl = (int)(cj & 0xFFFFu);
+                    kahan_prod_if(So, Co, xo * xo, (float)(q - jl), acc);
+                    Ss[slot] = So;
+                    CJ[slot] = (bf16_bits(Co) << 16) | (acc ? (uint32_t)q : (uint32_t)jl);
                 }
-                else { xs[slot] = acc ? xn : xo; }
                 if (HSUM) {
-                    kahan_prod_if(Sr, Cr, r, (float)(j - jr), acc);
-                    jr = acc ? j : jr;
+                    kahan_prod_if(Sr, Cr, r, (float)(q - jr), acc);
+                    jr = acc ? q : jr;
                 }
                 kahan_if(r, rc, dx2, acc);
-                if (HEXP && acc) { kahan_prod(Se, Ce, e, (float)(j - je)); je = j; e = mcif::bexp(-r * inv_ediv); }
-                ++j;
+                if (HEXP && acc) { kahan_prod(Se, Ce, e, (float)(q - je)); je = q; e = mcif::bexp(-r * inv_ediv); }
             }
-            else { xs[slot] = acc ? xn : xo; }
         }
         s += len;
         if (ACC) {
-            if (j == B) {
-                // the values held at the block end count until it; flush, restart from |x|^2 of
-                // the position
-                const double norm = 1. / B;
-                float rx = 0.f, rxc = 0.f;
-                for (int i = 0; i < d; ++i) {
-                    float v2;
-                    if (HX2) {
-                        const float4 rv = rec[i * WAVE + lane];
-                        v2 = rv.x;
-                        float Si = rv.y, Ci = rv.z;
-                        kahan_prod(Si, Ci, v2 * v2, (float)(B - __float_as_int(rv.w)));
+            // the chunk ends: every value held counts until here; at a block end the sums flush
+            // to FP64 block means, else they carry on into the next chunk
+            j += len;
+            const bool bend = (j == B);
+            const double norm = 1. / B;
+            float rx = 0.f, rxc = 0.f;
+            for (int i = 0; i < d; ++i) {
+                const float v2 = xs[i * WAVE + lane];
+                if (HX2) {
+                    float Si = Ss[i * WAVE + lane];
+                    const uint32_t c2 = CJ[i * WAVE + lane];
+                    float Ci = __uint_as_float(c2 & 0xFFFF0000u);
+                    kahan_prod(Si, Ci, v2 * v2, (float)(len - (int)(c2 & 0xFFFFu)));
+                    if (bend) {
                         A.d_x2[(b * d + i) * A.W + w] = ((double)Si - (double)Ci) * norm;
-                        rec[i * WAVE + lane] = make_float4(v2, 0.f, 0.f, __int_as_float(0));
+                        Ss[i * WAVE + lane] = 0.f;
+                        CJ[i * WAVE + lane] = 0u;
                     }
-                    else { v2 = xs[i * WAVE + lane]; }
-                    kahan(rx, rxc, v2 * v2);
+                    else {
+                        Ss[i * WAVE + lane] = Si;
+                        CJ[i * WAVE + lane] = bf16_bits(Ci) << 16;
+                    }
                 }
-                if (HSUM) {
-                    kahan_prod(Sr, Cr, r, (float)(B - jr));
-                    A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div;
-                }
-                if (HEXP) {
-                    kahan_prod(Se, Ce, e, (float)(B - je));
-                    A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm;
-                }
-                r = rx;
-                rc = rxc;
+                kahan(rx, rxc, v2 * v2);
+            }
+            if (HSUM) { kahan_prod(Sr, Cr, r, (float)(len - jr)); }
+            if (HEXP) { kahan_prod(Se, Ce, e, (float)(len - je)); }
+            if (bend) {
+                if (HSUM) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
+                if (HEXP) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
                 Sr = Cr = Se = Ce = 0.f;
-                jr = je = j = 0;
-                if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
+                j = 0;
                 ++b;
             }
-            else {
-                // refresh |x|^2 inside a long block: the values it replaces end their dwell here
-                float rx = 0.f, rxc = 0.f;
-                for (int i = 0; i < d; ++i) {
-                    const float v2 = HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane];
-                    kahan(rx, rxc, v2 * v2);
-                }
-                if (HSUM) { kahan_prod(Sr, Cr, r, (float)(j - jr)); jr = j; }
-                if (HEXP) { kahan_prod(Se, Ce, e, (float)(j - je)); je = j; }
-                r = rx;
-                rc = rxc;
-                if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
-            }
+            jr = je = 0;
+            r = rx;
+            rc = rxc;
+            if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
         }
     }
-    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)(HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]); }
+    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)xs[i * WAVE + lane]; }
     A.ctr[w] = t;
     A.acc[w] = nacc;
     A.rej[w] = A.nsteps - nacc;
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 size_t lds_bytes(int prec, int ndim, bool x2)
 {
     if (prec == MCI_PREC_FP64) { return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + ndim) + sizeof(float) * ndim; }
-    return sizeof(float) * ((size_t)(x2 ? 4 : 1) * ndim * WAVE + ndim);
+    return sizeof(float) * ((size_t)(x2 ? 3 : 1) * ndim * WAVE + ndim);
 }
 
 size_t lds_limit()

@@ -1795,11 +1795,13 @@ static mci_status sample_obs(orc_ctx *c, walker *w, int64_t n, accu *accs, const
  *    word 1 the proposal on the symmetric 24-bit grid, the high 24 bits of word 2 the uniform
  *    u on the 2^-24 grid; x' = fmaf(step, r, x); accept iff u <= bexp(-a_v (x' - x)(x' + x));
  *  - block sums by dwell time, as sums of positive terms: a value (x_i^2, |x|^2 or
- *    exp(-|x|^2/c)) enters its block sum once, when it is replaced or the block ends, times the
- *    steps it was held; the sums are Kahan-compensated FP32 pairs and each product enters
- *    exactly (its rounding error, by fused multiply-add, goes into the compensation); |x|^2 is
- *    itself a Kahan pair updated by the accepted changes; at a block end the sums flush to FP64
- *    block means ((S - C) / B) and |x|^2 restarts from the position.
+ *    exp(-|x|^2/c)) enters its block sum once, when it is replaced or its chunk ends, times the
+ *    steps it was held; the sums are Kahan-compensated FP32 pairs (the per-coordinate
+ *    compensations stored in BF16) and each product enters exactly (its rounding error, by fused
+ *    multiply-add, goes into the compensation); |x|^2 is itself a Kahan pair updated by the
+ *    accepted changes; chunks of at most 1024 steps end at every block end, where the sums flush
+ *    to FP64 block means ((S - C) / B); every chunk end closes all dwells and recomputes |x|^2
+ *    from the position.
  * Calibration and the automatic decorrelation stay on the FP64 chain.  With c->fp32_shadow the
  * accumulators also get the block means of the same FP32 trajectory summed step by step in FP64
  * (the reference's BlockAccumulator applied to the FP32 chain), which bounds the FP32 sums' error. */
@@ -1820,6 +1822,9 @@ static void kahan32_if(float *s, float *cc, float y, int on)
     if (on) { *s = tt; *cc = c2; }
 }
 
+static uint32_t f2u_(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f_(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
 /* Kahan step adding the exact product a b (its rounding error, exact by fused multiply-add,
  * enters the compensation first) */
 static void kahan32_prod(float *s, float *cc, float a, float b, int on)
@@ -1830,6 +1835,15 @@ static void kahan32_prod(float *s, float *cc, float a, float b, int on)
     const float tt = *s + yy;
     const float c2 = (tt - *s) - yy;
     if (on) { *s = tt; *cc = c2; }
+}
+
+/* v rounded to BF16 (nearest, ties to even), as an FP32 value: the stored form of the per-coordinate
+ * Kahan compensations (a compensation is below half an ulp of its sum, so 8 bits of it keep the pair
+ * exact to about 2^-33 of the sum) */
+static float bf16r(float v)
+{
+    const uint32_t u = f2u_(v);
+    return u2f_(((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16) << 16);
 }
 
 static float sym24f(uint32_t wd) { return (float)(2 * (int32_t)(wd >> 8) + 1 - 16777216) * 0x1p-24f; }
@@ -1882,7 +1896,7 @@ static mci_status sample_fp32(orc_ctx *c, walker *w, int64_t n, accu *accs, cons
     }
     float *xs = malloc(sizeof(float) * (size_t)d), *as = malloc(sizeof(float) * (size_t)d);
     float *S = calloc((size_t)d, sizeof(float)), *Cm = calloc((size_t)d, sizeof(float));
-    int *jl = calloc((size_t)d, sizeof(int));           /* step of the coordinate's last change */
+    int *jl = calloc((size_t)d, sizeof(int));           /* chunk step of the coordinate's last change */
     double *shx = calloc((size_t)d + 2, sizeof(double)); /* shadow sums: x_i^2 [d], |x|^2, exp */
     for (int i = 0; i < d; ++i) { xs[i] = (float)w->xold[i]; as[i] = (float)pdf_a(&c->pdf[0], i); }
     const float step = (float)w->steps[0];
@@ -1894,99 +1908,96 @@ static mci_status sample_fp32(orc_ctx *c, walker *w, int64_t n, accu *accs, cons
     if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
     const uint32_t range = (uint32_t)c->nvecs;
     const uint32_t key[2] = { (uint32_t)c->seed, (uint32_t)(c->seed >> 32) };
-    int64_t b = 0, nacc = 0, chunk_start = 0;
+    int64_t b = 0, nacc = 0;
     const double norm = 1. / (double)Bi;
-    for (int64_t s = 0; s < n; ++s) {
-        uint32_t ctr[4] = { 0u, w->wid, (uint32_t)w->t, (uint32_t)(w->t >> 32) }, wd[4];
-        orc_philox4x32_10(ctr, key, wd);
-        /* index: Lemire on word 0, rejected draws from the extra-word blocks (draw_index) */
-        uint64_t prod = (uint64_t)wd[0] * range;
-        uint32_t low = (uint32_t)prod;
-        if (low < range) {
-            const uint32_t thr = (0u - range) % range;
-            w->kx = 0;
-            while (low < thr) {
-                prod = (uint64_t)ph_extra(c, w) * range;
-                low = (uint32_t)prod;
+    for (int64_t s = 0; s < n;) {
+        /* a chunk: at most 1024 steps (FAST32_REFRESH in mci_fast_plan.hpp), never past a block end */
+        int64_t lim = n - s;
+        if (acc_on && (int64_t)(Bi - j) < lim) lim = Bi - j;
+        const int len = (int)(lim < 1024 ? lim : 1024);
+        for (int q = 0; q < len; ++q) {
+            uint32_t ctr[4] = { 0u, w->wid, (uint32_t)w->t, (uint32_t)(w->t >> 32) }, wd[4];
+            orc_philox4x32_10(ctr, key, wd);
+            /* index: Lemire on word 0, rejected draws from the extra-word blocks (draw_index) */
+            uint64_t prod = (uint64_t)wd[0] * range;
+            uint32_t low = (uint32_t)prod;
+            if (low < range) {
+                const uint32_t thr = (0u - range) % range;
+                w->kx = 0;
+                while (low < thr) {
+                    prod = (uint64_t)ph_extra(c, w) * range;
+                    low = (uint32_t)prod;
+                }
             }
-        }
-        const int v = (int)(prod >> 32);
-        const float xo = xs[v];
-        const float rr = sym24f(wd[1]);
-        const float u = (float)(wd[2] >> 8) * 0x1p-24f;
-        ++w->t;
-        const float xn = fmaf(step, rr, xo);
-        const float dx2 = (xn - xo) * (xn + xo);
-        const float ex = as[v] * dx2;
-        const int acc = (u <= orc_bexp(-ex));
-        nacc += acc;
-        if (acc_on) {
-            if (ix2 >= 0) { /* the old x_v^2 was held for steps jl .. j-1 */
-                kahan32_prod(&S[v], &Cm[v], xo * xo, (float)(j - jl[v]), acc);
-                if (acc) jl[v] = j;
-            }
-            if (isum >= 0) {
-                kahan32_prod(&Sr, &Cr, r, (float)(j - jr), acc);
-                if (acc) jr = j;
-            }
-            kahan32_if(&r, &rc, dx2, acc);
-            if (iexp >= 0 && acc) { kahan32_prod(&Se, &Ce, e, (float)(j - je), 1); je = j; e = orc_bexp(-r * inv_ediv); }
-        }
-        if (acc) xs[v] = xn;
-        if (acc_on) {
-            if (c->fp32_shadow) { /* FP64, step by step, over the same positions */
-                double q = 0.0;
-                for (int i = 0; i < d; ++i) { const double xi = (double)xs[i]; shx[i] = shx[i] + xi * xi; q = q + xi * xi; }
-                shx[d] = shx[d] + q;
-                shx[d + 1] = shx[d + 1] + exp(-(q / exp_div));
-            }
-            ++j;
-            /* |x|^2 is recomputed from the position at every block end and at least every
-             * 1024 steps (the kernel's chunks, FAST32_REFRESH in mci_fast_plan.hpp) */
-            const int chunk_end = (s + 1 == n) || (j == Bi) || ((s + 1 - chunk_start) == 1024);
-            if (chunk_end && j != Bi) {
-                float rx = 0.f, rxc = 0.f;
-                for (int i = 0; i < d; ++i) kahan32(&rx, &rxc, xs[i] * xs[i]);
-                if (isum >= 0) { kahan32_prod(&Sr, &Cr, r, (float)(j - jr), 1); jr = j; }
-                if (iexp >= 0) { kahan32_prod(&Se, &Ce, e, (float)(j - je), 1); je = j; }
-                r = rx;
-                rc = rxc;
-                if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
-            }
-            if (chunk_end) chunk_start = s + 1;
-            if (j == Bi) {
-                float rx = 0.f, rxc = 0.f;
-                for (int i = 0; i < d; ++i) {
-                    const float q = xs[i] * xs[i];
-                    if (ix2 >= 0) {
-                        kahan32_prod(&S[i], &Cm[i], q, (float)(Bi - jl[i]), 1);
-                        accs[ix2].data[b * d + i] = ((double)S[i] - (double)Cm[i]) * norm;
-                        S[i] = Cm[i] = 0.f;
-                        jl[i] = 0;
-                    }
-                    kahan32(&rx, &rxc, q);
+            const int v = (int)(prod >> 32);
+            const float xo = xs[v];
+            const float rr = sym24f(wd[1]);
+            const float u = (float)(wd[2] >> 8) * 0x1p-24f;
+            ++w->t;
+            const float xn = fmaf(step, rr, xo);
+            const float dx2 = (xn - xo) * (xn + xo);
+            const float ex = as[v] * dx2;
+            const int acc = (u <= orc_bexp(-ex));
+            nacc += acc;
+            if (acc) xs[v] = xn;
+            if (acc_on) {
+                if (ix2 >= 0) { /* the old x_v^2 was held since chunk step jl */
+                    kahan32_prod(&S[v], &Cm[v], xo * xo, (float)(q - jl[v]), acc);
+                    Cm[v] = bf16r(Cm[v]);
+                    if (acc) jl[v] = q;
                 }
                 if (isum >= 0) {
-                    kahan32_prod(&Sr, &Cr, r, (float)(Bi - jr), 1);
-                    accs[isum].data[b] = (((double)Sr - (double)Cr) * norm) / x2sum_div;
+                    kahan32_prod(&Sr, &Cr, r, (float)(q - jr), acc);
+                    if (acc) jr = q;
                 }
-                if (iexp >= 0) {
-                    kahan32_prod(&Se, &Ce, e, (float)(Bi - je), 1);
-                    accs[iexp].data[b] = ((double)Se - (double)Ce) * norm;
+                kahan32_if(&r, &rc, dx2, acc);
+                if (iexp >= 0 && acc) { kahan32_prod(&Se, &Ce, e, (float)(q - je), 1); je = q; e = orc_bexp(-r * inv_ediv); }
+                if (c->fp32_shadow) { /* FP64, step by step, over the same positions */
+                    double qq = 0.0;
+                    for (int i = 0; i < d; ++i) { const double xi = (double)xs[i]; shx[i] = shx[i] + xi * xi; qq = qq + xi * xi; }
+                    shx[d] = shx[d] + qq;
+                    shx[d + 1] = shx[d + 1] + exp(-(qq / exp_div));
                 }
+            }
+        }
+        s += len;
+        if (acc_on) {
+            /* the chunk ends: every value held counts until here; a block end flushes */
+            j += len;
+            const int bend = (j == Bi);
+            float rx = 0.f, rxc = 0.f;
+            for (int i = 0; i < d; ++i) {
+                const float q2 = xs[i] * xs[i];
+                if (ix2 >= 0) {
+                    kahan32_prod(&S[i], &Cm[i], q2, (float)(len - jl[i]), 1);
+                    if (bend) {
+                        accs[ix2].data[b * d + i] = ((double)S[i] - (double)Cm[i]) * norm;
+                        S[i] = Cm[i] = 0.f;
+                    }
+                    else Cm[i] = bf16r(Cm[i]);
+                    jl[i] = 0;
+                }
+                kahan32(&rx, &rxc, q2);
+            }
+            if (isum >= 0) kahan32_prod(&Sr, &Cr, r, (float)(len - jr), 1);
+            if (iexp >= 0) kahan32_prod(&Se, &Ce, e, (float)(len - je), 1);
+            if (bend) {
+                if (isum >= 0) accs[isum].data[b] = (((double)Sr - (double)Cr) * norm) / x2sum_div;
+                if (iexp >= 0) accs[iexp].data[b] = ((double)Se - (double)Ce) * norm;
                 if (c->fp32_shadow) {
                     if (ix2 >= 0) for (int i = 0; i < d; ++i) accs[ix2].shadow[b * d + i] = shx[i] * norm;
                     if (isum >= 0) accs[isum].shadow[b] = (shx[d] * norm) / x2sum_div;
                     if (iexp >= 0) accs[iexp].shadow[b] = shx[d + 1] * norm;
                     memset(shx, 0, sizeof(double) * ((size_t)d + 2));
                 }
-                r = rx;
-                rc = rxc;
                 Sr = Cr = Se = Ce = 0.f;
-                jr = je = j = 0;
-                if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
+                j = 0;
                 ++b;
             }
+            jr = je = 0;
+            r = rx;
+            rc = rxc;
+            if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
         }
     }
     for (int i = 0; i < d; ++i) { w->xold[i] = (double)xs[i]; w->xnew[i] = w->xold[i]; }
