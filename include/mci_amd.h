@@ -249,6 +249,17 @@ mci_status mci_get_timings(mci_ctx *ctx, double *ms, int n);
 /* ---- standalone estimators on host arrays (x has layout [n][ndim] like the reference),
  *      run on the device: Estimators.cpp:125-155,191-246,271-286, MJBlocker.cpp:127-154 ---- */
 mci_status mci_estimate(int estimator, int64_t n, int ndim, const double *x, double *average, double *error);
+/* The same estimators without any allocation.  No counterpart in the reference: this entry point is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  x [n][ndim], average [ndim] and error [ndim] are DEVICE pointers the caller owns; the work
+ * is enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call returns without
+ * waiting.  The only other memory is a fixed array of MCI_ESTIMATE_DEVICE_MAX doubles per device,
+ * held until the process ends: n * ndim must not exceed it (MCI_E_INVALID_ARGUMENT otherwise).  All
+ * calls on one device share that array, so calls that overlap on one device (different streams,
+ * or different host threads) race; serialise them.  The first call on a device looks up the
+ * array's address (hipGetSymbolAddress): make it outside a graph capture. */
+#define MCI_ESTIMATE_DEVICE_MAX 2097152
+mci_status mci_estimate_device(int estimator, int64_t n, int ndim, const double *x, double *average, double *error, void *stream);
 
 /* ---- batched integrals ----
  * Many independent integrals in one submission.  Each integral i is what one reference MCI

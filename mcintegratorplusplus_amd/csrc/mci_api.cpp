@@ -20,6 +20,7 @@
 
 using namespace mcid;
 
+using mcih::check_estimator;
 using mcih::DBuf;
 using mcih::det_sum;
 using mcih::fail;
@@ -289,27 +290,6 @@ double obs_p0(const ObsHost &o, int ndim)
     case MCI_OBS_GAUSS_X2: return o.params.empty() ? 1. / sqrt(M_PI * M_PI * M_PI) / 3. : o.params[0];
     }
     return 0.0;
-}
-
-// the estimator's own size requirements (Estimators.cpp:39,85,127,197; MJBlocker.cpp:25-30),
-// checked before sampling
-mci_status check_estimator(int estim, int64_t n)
-{
-    switch (estim) {
-    case MCI_ESTIM_UNCORRELATED:
-        if (n < 2) { return fail(MCI_E_INVALID_ARGUMENT, "[UncorrelatedEstimator] n must be larger than 1"); }
-        break;
-    case MCI_ESTIM_FCBLOCKER:
-        if (n < 50) { return fail(MCI_E_INVALID_ARGUMENT, "[FCBlockerEstimator] n must be >= 50"); }
-        break;
-    case MCI_ESTIM_MJBLOCKER:
-        if (!is_pow2(n)) { return fail(MCI_E_INVALID_ARGUMENT, "[MJBlocker] ndata must be a power of two."); }
-        break;
-    case MCI_ESTIM_CORRELATED:
-        if (!is_pow2(n) && n < 50) { return fail(MCI_E_INVALID_ARGUMENT, "[FCBlockerEstimator] n must be >= 50"); }
-        break;
-    }
-    return MCI_OK;
 }
 
 int64_t nstore_of(int blocksize, int nskip, int64_t nmc)

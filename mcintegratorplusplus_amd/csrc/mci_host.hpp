@@ -49,6 +49,29 @@ struct DBuf
     }
 };
 
+inline bool is_pow2(int64_t n) { return n > 0 && (((uint64_t)n & ((uint64_t)n - 1)) == 0); }
+
+// the estimator's own size requirements (Estimators.cpp:39,85,127,197; MJBlocker.cpp:25-30),
+// checked before any device work
+inline mci_status check_estimator(int estim, int64_t n)
+{
+    switch (estim) {
+    case MCI_ESTIM_UNCORRELATED:
+        if (n < 2) { return fail(MCI_E_INVALID_ARGUMENT, "[UncorrelatedEstimator] n must be larger than 1"); }
+        break;
+    case MCI_ESTIM_FCBLOCKER:
+        if (n < 50) { return fail(MCI_E_INVALID_ARGUMENT, "[FCBlockerEstimator] n must be >= 50"); }
+        break;
+    case MCI_ESTIM_MJBLOCKER:
+        if (!is_pow2(n)) { return fail(MCI_E_INVALID_ARGUMENT, "[MJBlocker] ndata must be a power of two."); }
+        break;
+    case MCI_ESTIM_CORRELATED:
+        if (!is_pow2(n) && n < 50) { return fail(MCI_E_INVALID_ARGUMENT, "[FCBlockerEstimator] n must be >= 50"); }
+        break;
+    }
+    return MCI_OK;
+}
+
 // the deterministic walker sum used on the device (chunks of 256, pairwise tree, chunks in order)
 inline double det_sum(const double *v, int64_t n)
 {
