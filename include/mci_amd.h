@@ -94,6 +94,24 @@ typedef int mci_status;
 #define MCI_OBS_INDICATOR   7 /* nobs 1: out = x_j > 0 ? 1 : 0. params: [j] (default 0) */
 #define MCI_OBS_GAUSS_X2    8 /* nobs 1: out = exp(-s)*s*c with s = x0*x0 + x1*x1 + ... (GaussXSquared,
                                  TestMCIFunctions.hpp:278-295). params: [c] (default 1./sqrt(pi*pi*pi)/3.) */
+/* Dependent observables (DependentObservableInterface, include/mci/DependentObservableInterface.hpp:24-54):
+ * they read what a sampling function computed or what an earlier observable last computed.  The
+ * reference hands a PDF-dependent observable the sampling function's proto values of the last
+ * accepted state through SamplingFunctionInterface::observationCallback (SamplingFunctionInterface.hpp:
+ * 100-104), called by MCI::sample (MCIntegrator.cpp:294-305) whenever the state changed since the last
+ * call and the step is observed (every gcd of the PDF-dependent observables' nskip steps,
+ * ObservableContainer.cpp:16-35); at every step such an observable is evaluated it therefore sees the
+ * current state's proto values, which is what the device reads directly. */
+#define MCI_OBS_PDF_LOG     9 /* nobs 1: log of sampling function j's value from its proto values p of the
+                                 accepted state: -(0 + p_0 + p_1 + ...) for GAUSS_ISO, GAUSS_DIAG and EXP_DIAG,
+                                 log p for GMM_ISO.  params: [j] (default 0) */
+#define MCI_OBS_PDF_PROTO  10 /* nobs ndim: the proto values p_i of sampling function j (GAUSS_DIAG or EXP_DIAG,
+                                 nproto == ndim), e.g. a_i x_i^2.  params: [j] (default 0) */
+#define MCI_OBS_DEP_SQUARE 11 /* nobs of observable k: the squares of observable k's last values.  k must be an
+                                 earlier observable whose nskip divides this one's (the two rules of
+                                 DependentObservableInterface::isObsDepValid, DependentObservableInterface.hpp:36-44);
+                                 in the automatic decorrelation k must be an equilibration observable too.
+                                 params: [k] */
 
 /* ---- random-number modes ---- */
 #define MCI_RNG_MT19937_64  0 /* reference streams: one std::mt19937_64 per walker, libstdc++ distributions

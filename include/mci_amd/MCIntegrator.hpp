@@ -103,8 +103,13 @@ struct ObservableFunction
     int kind;
     int ndim;
     std::vector<double> params;
+    int nobs_dep = 0; // MCI_OBS_DEP_SQUARE: the observable it squares has this many components
     int getNDim() const { return ndim; }
-    int getNObs() const { return (kind == MCI_OBS_X2 || kind == MCI_OBS_X) ? ndim : 1; }
+    int getNObs() const
+    {
+        if (kind == MCI_OBS_DEP_SQUARE) { return nobs_dep; }
+        return (kind == MCI_OBS_X2 || kind == MCI_OBS_X || kind == MCI_OBS_PDF_PROTO) ? ndim : 1;
+    }
 };
 
 // sampling functions (TestMCIFunctions.hpp:123-256 and the benchmark targets)
@@ -142,6 +147,15 @@ inline ObservableFunction Constval(int nd, double c = 1.3) { return {MCI_OBS_CON
 inline ObservableFunction ExpX2(int nd, double c) { return {MCI_OBS_EXP_X2, nd, {c}}; }  // exp(-|x|^2/c)
 inline ObservableFunction Indicator(int nd, int j) { return {MCI_OBS_INDICATOR, nd, {(double)j}}; } // 1[x_j > 0]
 inline ObservableFunction GaussXSquared() { return {MCI_OBS_GAUSS_X2, 3, {}}; }        // exp(-|x|^2)|x|^2/(3 pi^1.5)
+// dependent observables (DependentObservableInterface.hpp): what sampling function j computed at the
+// accepted state, or the squares of the values observable k (added earlier, nskip dividing this one's)
+// last computed
+inline ObservableFunction PdfLogValue(int nd, int j = 0) { return {MCI_OBS_PDF_LOG, nd, {(double)j}}; }     // log of pdf j
+inline ObservableFunction PdfProtoValues(int nd, int j = 0) { return {MCI_OBS_PDF_PROTO, nd, {(double)j}}; } // its proto values
+inline ObservableFunction SquaredObservable(const ObservableFunction &dep, int k)
+{
+    return {MCI_OBS_DEP_SQUARE, dep.ndim, {(double)k}, dep.getNObs()};
+}
 
 // estimator functions on host data, run on the device (Estimators.hpp)
 inline void estimate(EstimatorType est, int64_t n, int ndim, const double x[], double average[], double error[])

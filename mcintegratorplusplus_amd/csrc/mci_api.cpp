@@ -53,7 +53,8 @@ uint64_t walker_seed(uint64_t seed, int64_t w)
 int obs_nobs(int kind, int ndim)
 {
     switch (kind) {
-    case MCI_OBS_X2: case MCI_OBS_X: return ndim;
+    case MCI_OBS_X2: case MCI_OBS_X: case MCI_OBS_PDF_PROTO: return ndim;
+    case MCI_OBS_PDF_LOG: return 1;
     case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR:
     case MCI_OBS_GAUSS_X2: return 1;
     }
@@ -288,6 +289,7 @@ double obs_p0(const ObsHost &o, int ndim)
     case MCI_OBS_EXP_X2: return o.params.empty() ? (double)ndim : o.params[0];
     case MCI_OBS_INDICATOR: return o.params.empty() ? 0.0 : o.params[0];
     case MCI_OBS_GAUSS_X2: return o.params.empty() ? 1. / sqrt(M_PI * M_PI * M_PI) / 3. : o.params[0];
+    case MCI_OBS_PDF_LOG: case MCI_OBS_PDF_PROTO: case MCI_OBS_DEP_SQUARE: return o.params.empty() ? 0.0 : o.params[0];
     }
     return 0.0;
 }
@@ -369,6 +371,23 @@ mci_status find_shared(mci_ctx *c)
     return MCI_OK;
 }
 
+// what the dependent observables depend on exists (checked when integrate starts: sampling
+// functions may be added after the observables)
+mci_status check_dependencies(const mci_ctx *c)
+{
+    for (const ObsHost &o : c->obs) {
+        if (o.kind != MCI_OBS_PDF_LOG && o.kind != MCI_OBS_PDF_PROTO) { continue; }
+        const int j = (int)obs_p0(o, c->ndim);
+        if (j >= (int)c->pdfs.size()) {
+            return fail(MCI_E_INVALID_ARGUMENT, "an observable depends on sampling function %d, but there are %d", j, (int)c->pdfs.size());
+        }
+        if (o.kind == MCI_OBS_PDF_PROTO && c->pdfs[j].nproto != c->ndim) {
+            return fail(MCI_E_INVALID_ARGUMENT, "the proto-value observable needs a sampling function with one proto value per dimension");
+        }
+    }
+    return MCI_OK;
+}
+
 // MCI::initialDecorrelation, automatic branch (MCIntegrator.cpp:173-243): equilibration
 // observables with Full accumulators and the Correlated estimator; walkers either stop
 // individually (independent) or together on the rank-combined estimates (shared)
@@ -380,9 +399,12 @@ mci_status decorrelate_auto(mci_ctx *c)
     const int64_t W = c->W;
     std::vector<ObsDev> od;
     std::vector<DBuf<double>> data, scr;
+    std::vector<int> eq_index(c->obs.size(), -1); // an observable's index in the equilibration container
     int nd = 0;
-    for (auto &o : c->obs) {
+    for (size_t i = 0; i < c->obs.size(); ++i) {
+        const ObsHost &o = c->obs[i];
         if (!o.equil) { continue; }
+        eq_index[i] = (int)od.size();
         ObsDev d;
         std::memset(&d, 0, sizeof d);
         d.kind = o.kind;
@@ -392,6 +414,13 @@ mci_status decorrelate_auto(mci_ctx *c)
         d.acc = ACC_FULL;
         d.blocksize = 1;
         d.p0 = obs_p0(o, c->ndim);
+        if (o.kind == MCI_OBS_DEP_SQUARE) {
+            const int k = eq_index[(size_t)d.p0];
+            if (k < 0) {
+                return fail(MCI_E_INVALID_ARGUMENT, "an equilibration observable depends on observable %d, which is not one", (int)d.p0);
+            }
+            d.p0 = k;
+        }
         d.nstore = min_nmc;
         data.emplace_back();
         HIPCHK(data.back().ensure((size_t)min_nmc * o.nobs * W));
@@ -1045,10 +1074,25 @@ mci_status mci_add_observable(mci_ctx *c, int kind, int ndim, const double *para
         return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Requested automatic observable equilibration requires estimator with error calculation.");
     }
     if (estim < MCI_ESTIM_NOOP || estim > MCI_ESTIM_MJBLOCKER) { return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator."); }
-    const int nobs = obs_nobs(kind, ndim);
+    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
+    int nobs = obs_nobs(kind, ndim);
+    if (kind == MCI_OBS_DEP_SQUARE) {
+        // DependentObservableInterface::isObsDepValid (DependentObservableInterface.hpp:36-44): an earlier
+        // observable whose nskip divides this one's
+        const int k = nparams > 0 ? (int)params[0] : -1;
+        if (nparams < 1 || params[0] != (double)k || k < 0 || k >= (int)c->obs.size()) {
+            return fail(MCI_E_INVALID_ARGUMENT, "a squared observable needs the index of an earlier observable");
+        }
+        if (nskip % c->obs[k].nskip != 0) {
+            return fail(MCI_E_INVALID_ARGUMENT, "the nskip of a dependent observable must be a multiple of its dependency's");
+        }
+        nobs = c->obs[k].nobs;
+    }
+    if ((kind == MCI_OBS_PDF_LOG || kind == MCI_OBS_PDF_PROTO) && nparams > 0 && (params[0] < 0 || params[0] >= MAX_PDF || params[0] != (int)params[0])) {
+        return fail(MCI_E_INVALID_ARGUMENT, "sampling-function index out of range");
+    }
     if (nobs < 0) { return fail(MCI_E_DOMAIN, "unknown observable kind %d", kind); }
     if ((int)c->obs.size() >= MAX_OBS) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d observables", MAX_OBS); }
-    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
     if (kind == MCI_OBS_INDICATOR && nparams > 0 && (params[0] < 0 || params[0] >= ndim)) {
         return fail(MCI_E_INVALID_ARGUMENT, "indicator index out of range");
     }
@@ -1147,6 +1191,7 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
     }
     const int nd = nobsdim(c);
     if (nmc > 0 && nd > 0 && (!average || !error)) { return fail(MCI_E_INVALID_ARGUMENT, "null output arrays"); }
+    TRY(check_dependencies(c));
     // accumulator and estimator requirements (BlockAccumulator.cpp:8-13 and the estimators)
     if (nmc > 0) {
         for (auto &o : c->obs) {

@@ -640,6 +640,27 @@ __device__ inline void obs_eval(const Chain &c, const ObsDev &o, int64_t w)
         out[w] = exp(-s) * s * o.p0;
         break;
     }
+    case MCI_OBS_PDF_LOG: { // the committed proto values of sampling function p0
+        const PdfDev &p = c.pdf[(int)o.p0];
+        const double *pv = c.pold + (int64_t)p.off * W + w;
+        if (p.kind == MCI_PDF_GMM_ISO) { out[w] = pv[(int64_t)p.K * W]; }
+        else {
+            double s = 0.0;
+            for (int j = 0; j < p.nproto; ++j) { s = s + pv[(int64_t)j * W]; }
+            out[w] = -s;
+        }
+        break;
+    }
+    case MCI_OBS_PDF_PROTO: {
+        const PdfDev &p = c.pdf[(int)o.p0];
+        for (int j = 0; j < o.nobs; ++j) { out[(int64_t)j * W + w] = c.pold[(int64_t)(p.off + j) * W + w]; }
+        break;
+    }
+    case MCI_OBS_DEP_SQUARE: { // the last values of an earlier observable (evaluated before this one)
+        const double *v = c.obsval + (int64_t)c.obs[(int)o.p0].voff * W;
+        for (int j = 0; j < o.nobs; ++j) { const double y = v[(int64_t)j * W + w]; out[(int64_t)j * W + w] = y * y; }
+        break;
+    }
     }
 }
 

@@ -201,6 +201,7 @@ typedef struct {
     int kind, ndim, nparams, nobs, updateable;
     double *params;
     int blocksize, nskip, flag_equil, estim;
+    int dep;          /* MCI_OBS_DEP_SQUARE: index of the observable it squares, in its container */
 } obs_desc;
 
 /* accumulator types, Factories.hpp:29-43 */
@@ -592,7 +593,18 @@ static double pdf_compute_acceptance(const pdf_desc *p, const walker *w, const d
 /* ------------------------------------------------------------------------- */
 static double obs_param(const obs_desc *o, int i, double dflt) { return (o->nparams > i) ? o->params[i] : dflt; }
 
-static void obs_eval(const obs_desc *o, const double *x, double *out)
+/* what dependent observables read (DependentObservableInterface::registerDeps,
+ * DependentObservableInterface.hpp:49-51): the sampling functions of the context with the walker's
+ * committed proto values (what observationCallback receives, SamplingFunctionInterface.hpp:59-62,
+ * 100-104), and the accumulators of the observable container being filled (their last values,
+ * AccumulatorInterface.hpp:86) */
+typedef struct {
+    const struct orc_ctx *c;
+    const walker *w;
+    const accu *accs;
+} obs_env;
+
+static void obs_eval(const obs_desc *o, const double *x, double *out, const obs_env *env)
 {
     int n = o->ndim;
     switch (o->kind) {
@@ -634,26 +646,48 @@ static void obs_eval(const obs_desc *o, const double *x, double *out)
         out[0] = exp(-s) * s * obs_param(o, 0, 1. / sqrt(M_PI * M_PI * M_PI) / 3.);
         break;
     }
+    case MCI_OBS_PDF_LOG: case MCI_OBS_PDF_PROTO: { /* the proto values of sampling function j at the accepted state */
+        const int j = (int)obs_param(o, 0, 0.0);
+        int off = 0;
+        for (int p = 0; p < j; ++p) off += env->c->pdf[p].nproto;
+        const pdf_desc *pd = &env->c->pdf[j];
+        const double *pv = env->w->pold + off;
+        if (o->kind == MCI_OBS_PDF_PROTO) {
+            for (int i = 0; i < o->nobs; ++i) out[i] = pv[i];
+        }
+        else if (pd->kind == MCI_PDF_GMM_ISO) out[0] = pv[pd->K];
+        else {
+            double s = 0.0;
+            for (int i = 0; i < pd->nproto; ++i) s = s + pv[i];
+            out[0] = -s;
+        }
+        break;
+    }
+    case MCI_OBS_DEP_SQUARE: { /* the last values of an earlier observable of the container */
+        const double *v = env->accs[o->dep].obs_values;
+        for (int i = 0; i < o->nobs; ++i) out[i] = v[i] * v[i];
+        break;
+    }
     }
 }
 
 /* updatedObservable of the updateable kinds (X2, TestMCIFunctions.hpp:453-460) */
-static void obs_update(const obs_desc *o, const double *x, const uint8_t *flags, double *out)
+static void obs_update(const obs_desc *o, const double *x, const uint8_t *flags, double *out, const obs_env *env)
 {
     if (o->kind == MCI_OBS_X2) {
         for (int i = 0; i < o->ndim; ++i)
             if (flags[i]) out[i] = x[i] * x[i];
         return;
     }
-    obs_eval(o, x, out);
+    obs_eval(o, x, out, env);
 }
 
 static int obs_nobs(int kind, int ndim)
 {
     switch (kind) {
-    case MCI_OBS_X2: case MCI_OBS_X: return ndim;
+    case MCI_OBS_X2: case MCI_OBS_X: case MCI_OBS_PDF_PROTO: return ndim;
     case MCI_OBS_X2_SUM: case MCI_OBS_X_SUM: case MCI_OBS_CONST: case MCI_OBS_EXP_X2: case MCI_OBS_INDICATOR:
-    case MCI_OBS_GAUSS_X2: return 1;
+    case MCI_OBS_GAUSS_X2: case MCI_OBS_PDF_LOG: return 1;
     }
     return -1;
 }
@@ -743,7 +777,7 @@ static void accu_store(accu *a) /* child _accumulate */
 }
 
 /* accumulate (AccumulatorInterface.cpp:99-114, _processOld/_processFull/_processSelective :34-85) */
-static void accu_accumulate(accu *a, const obs_desc *o, const walker *w)
+static void accu_accumulate(accu *a, const obs_desc *o, const walker *w, const obs_env *env)
 {
     if (w->accepted || a->nchanged > 0) {
         if (a->updateable) {
@@ -759,8 +793,8 @@ static void accu_accumulate(accu *a, const obs_desc *o, const walker *w)
             }
             if (++a->skipidx == a->nskip) {
                 a->skipidx = 0;
-                if (a->nchanged < a->xndim) obs_update(o, w->xnew, a->flags_xchanged, a->obs_values);
-                else obs_eval(o, w->xnew, a->obs_values);
+                if (a->nchanged < a->xndim) obs_update(o, w->xnew, a->flags_xchanged, a->obs_values, env);
+                else obs_eval(o, w->xnew, a->obs_values, env);
                 memset(a->flags_xchanged, 0, (size_t)a->xndim);
                 a->nchanged = 0;
                 accu_store(a);
@@ -769,7 +803,7 @@ static void accu_accumulate(accu *a, const obs_desc *o, const walker *w)
             a->nchanged = a->xndim;
             if (++a->skipidx == a->nskip) {
                 a->skipidx = 0;
-                obs_eval(o, w->xnew, a->obs_values);
+                obs_eval(o, w->xnew, a->obs_values, env);
                 a->nchanged = 0;
                 accu_store(a);
             }
@@ -1566,10 +1600,22 @@ mci_status orc_add_observable(orc_ctx *c, int kind, int ndim, const double *para
         return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Requested automatic observable equilibration requires estimator with error calculation.");
     if (estim < MCI_ESTIM_NOOP || estim > MCI_ESTIM_MJBLOCKER) return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator.");
     int nobs = obs_nobs(kind, ndim);
+    int dep = -1;
+    if (kind == MCI_OBS_DEP_SQUARE) { /* DependentObservableInterface::isObsDepValid (DependentObservableInterface.hpp:36-44) */
+        dep = (nparams > 0) ? (int)params[0] : -1;
+        if (nparams < 1 || params[0] != (double)dep || dep < 0 || dep >= c->nobs)
+            return fail(MCI_E_INVALID_ARGUMENT, "a squared observable needs the index of an earlier observable");
+        if (nskip % c->obs[dep].nskip != 0)
+            return fail(MCI_E_INVALID_ARGUMENT, "the nskip of a dependent observable must be a multiple of its dependency's");
+        nobs = c->obs[dep].nobs;
+    }
+    if ((kind == MCI_OBS_PDF_LOG || kind == MCI_OBS_PDF_PROTO) && nparams > 0 && (params[0] < 0 || params[0] >= MAX_PDF || params[0] != (int)params[0]))
+        return fail(MCI_E_INVALID_ARGUMENT, "sampling-function index out of range");
     if (nobs < 0) return fail(MCI_E_DOMAIN, "unknown observable kind %d", kind);
     if (c->nobs >= MAX_OBS) return fail(MCI_E_INVALID_ARGUMENT, "too many observables");
     obs_desc o;
     memset(&o, 0, sizeof o);
+    o.dep = dep;
     o.kind = kind; o.ndim = ndim; o.nparams = nparams; o.nobs = nobs;
     o.updateable = (kind == MCI_OBS_X2);
     if (nparams > 0) { o.params = malloc(sizeof(double) * (size_t)nparams); memcpy(o.params, params, sizeof(double) * (size_t)nparams); }
@@ -1773,7 +1819,8 @@ static mci_status sample_obs(orc_ctx *c, walker *w, int64_t n, accu *accs, const
     for (int64_t s = 0; s < n; ++s) {
         if (c->npdf > 0) do_step_mrt2(c, w);
         else do_step_random(c, w);
-        for (int i = 0; i < nobs; ++i) accu_accumulate(&accs[i], &obs[i], w);
+        const obs_env env = { c, w, accs };
+        for (int i = 0; i < nobs; ++i) accu_accumulate(&accs[i], &obs[i], w, &env);
         if (record && c->tracing && c->trace_n < c->trace_cap) {
             memcpy(c->trace_x + c->trace_n * c->ndim, w->xold, sizeof(double) * (size_t)c->ndim);
             c->trace_acc[c->trace_n] = (uint8_t)w->accepted;
@@ -2085,14 +2132,23 @@ static mci_status decorrelate(orc_ctx *c, int64_t w0, int64_t w1, int64_t nranks
     if (c->ndecorr == 0) return MCI_OK;
     /* equilibration container: clones of flag_equil observables, Full accumulator, Correlated estimator */
     obs_desc eo[MAX_OBS];
+    int eq_index[MAX_OBS];
     int neo = 0, nd = 0;
-    for (int i = 0; i < c->nobs; ++i)
+    for (int i = 0; i < c->nobs; ++i) {
+        eq_index[i] = -1;
         if (c->obs[i].flag_equil) {
+            eq_index[i] = neo;
             eo[neo] = c->obs[i];
             eo[neo].blocksize = 1; eo[neo].nskip = 1; eo[neo].estim = MCI_ESTIM_CORRELATED;
+            if (eo[neo].kind == MCI_OBS_DEP_SQUARE) { /* its dependency's index in this container */
+                eo[neo].dep = eq_index[c->obs[i].dep];
+                if (eo[neo].dep < 0)
+                    return fail(MCI_E_INVALID_ARGUMENT, "an equilibration observable depends on observable %d, which is not one", c->obs[i].dep);
+            }
             nd += eo[neo].nobs;
             ++neo;
         }
+    }
     const int64_t MIN_NMC = (int64_t)fmax(100., sqrt(40000. * c->ndim) / (double)nranks);
     const int64_t R = w1 - w0;
     accu *acc = calloc((size_t)(R * (neo > 0 ? neo : 1)), sizeof(accu));
@@ -2170,6 +2226,14 @@ mci_status orc_integrate(orc_ctx *c, int64_t nmc, double *average, double *error
     int finite = dom_volume(c) != 0.;
     if (c->npdf == 0 && !finite)
         return fail(MCI_E_DOMAIN, "[MCI::integrate] Integrating over an infinite domain requires a sampling function.");
+    for (int i = 0; i < c->nobs; ++i) { /* what the dependent observables depend on exists */
+        const obs_desc *o = &c->obs[i];
+        if (o->kind != MCI_OBS_PDF_LOG && o->kind != MCI_OBS_PDF_PROTO) continue;
+        const int j = (int)obs_param(o, 0, 0.0);
+        if (j >= c->npdf) return fail(MCI_E_INVALID_ARGUMENT, "an observable depends on sampling function %d, but there are %d", j, c->npdf);
+        if (o->kind == MCI_OBS_PDF_PROTO && c->pdf[j].nproto != c->ndim)
+            return fail(MCI_E_INVALID_ARGUMENT, "the proto-value observable needs a sampling function with one proto value per dimension");
+    }
     c->nwarn = 0;
     const int64_t W = c->W;
     const int shared = (c->coupling == MCI_COUPLING_SHARED);

@@ -11,6 +11,7 @@
 // Input: a line-oriented script on stdin.  Output: one JSON object per result line;
 // doubles are printed as C99 hex-floats ("%a") so they round-trip bit-exactly.
 #include "mci/MCIntegrator.hpp"
+#include "mci/DependentObservableInterface.hpp"
 #include "mci/Factories.hpp"
 #include "mci/MultiStepMove.hpp"
 #include "mci/Estimators.hpp"
@@ -36,7 +37,8 @@ namespace
 {
 // functor kinds, as in include/mci_amd.h
 enum { PDF_GAUSS_ISO = 1, PDF_GAUSS_DIAG = 2, PDF_EXP_DIAG = 3, PDF_GMM_ISO = 4, PDF_GMM_FULL = 5 };
-enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7, OBS_GAUSS_X2 = 8 };
+enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7, OBS_GAUSS_X2 = 8,
+       OBS_PDF_LOG = 9, OBS_PDF_PROTO = 10, OBS_DEP_SQUARE = 11 };
 
 std::string hx(double v)
 {
@@ -53,12 +55,30 @@ std::string hxv(const double *v, int n)
 }
 
 // ---- sampling functions ----
-class GaussIso final: public mci::SamplingFunctionInterface
+// Every sampling function of the harness keeps what its observation callback last received
+// (SamplingFunctionInterface::observationCallback, SamplingFunctionInterface.hpp:100-104) for the
+// dependent observables below, and the log of its value in the form of its samplingFunction.
+class ObservedPDF: public mci::SamplingFunctionInterface
+{
+public:
+    std::vector<double> seen_pv;  // proto values at the last callback
+    int64_t ncallbacks = 0;
+    ObservedPDF(int nd, int np): mci::SamplingFunctionInterface(nd, np) {}
+    void observationCallback(const double /*x*/[], const double pv[]) override
+    {
+        seen_pv.assign(pv, pv + _nproto);
+        ++ncallbacks;
+    }
+    virtual double logValue(const double pv[]) const = 0;
+};
+
+class GaussIso final: public ObservedPDF
 {
 protected:
     mci::SamplingFunctionInterface *_clone() const final { return new GaussIso(_ndim); }
 public:
-    explicit GaussIso(int nd): mci::SamplingFunctionInterface(nd, 1) {}
+    explicit GaussIso(int nd): ObservedPDF(nd, 1) {}
+    double logValue(const double pv[]) const final { double s = 0.; s = s + pv[0]; return -s; }
     void protoFunction(const double in[], double pv[]) final
     {
         double s = in[0]*in[0];
@@ -69,17 +89,18 @@ public:
     double acceptanceFunction(const double po[], const double pn[]) const final { return exp(-pn[0] + po[0]); }
 };
 
-class DiagPDF final: public mci::SamplingFunctionInterface
+class DiagPDF final: public ObservedPDF
 {
     bool _abs;
     std::vector<double> _a;
 protected:
     mci::SamplingFunctionInterface *_clone() const final { return new DiagPDF(_ndim, _abs, _a); }
 public:
-    DiagPDF(int nd, bool absform, std::vector<double> a): mci::SamplingFunctionInterface(nd, nd), _abs(absform), _a(std::move(a))
+    DiagPDF(int nd, bool absform, std::vector<double> a): ObservedPDF(nd, nd), _abs(absform), _a(std::move(a))
     {
         if (_a.empty()) { _a.assign(nd, 1.); }
     }
+    double logValue(const double pv[]) const final { return -std::accumulate(pv, pv + _nproto, 0.); }
     double f(double x, int i) const { return _abs ? fabs(x)*_a[i] : (x*x)*_a[i]; }
     void protoFunction(const double in[], double pv[]) final
     {
@@ -104,15 +125,16 @@ public:
     }
 };
 
-class GmmIso final: public mci::SamplingFunctionInterface
+class GmmIso final: public ObservedPDF
 {
     int _K;
     std::vector<double> _w, _s, _mu, _c, _b;
 protected:
     mci::SamplingFunctionInterface *_clone() const final { return new GmmIso(_ndim, _K, _w, _s, _mu); }
 public:
+    double logValue(const double pv[]) const final { return pv[_K]; }
     GmmIso(int nd, int K, std::vector<double> w, std::vector<double> s, std::vector<double> mu):
-            mci::SamplingFunctionInterface(nd, K + 1), _K(K), _w(std::move(w)), _s(std::move(s)), _mu(std::move(mu))
+            ObservedPDF(nd, K + 1), _K(K), _w(std::move(w)), _s(std::move(s)), _mu(std::move(mu))
     {
         for (int k = 0; k < _K; ++k) {
             _c.push_back(log(_w[k]) - static_cast<double>(_ndim)*log(_s[k]));
@@ -158,7 +180,7 @@ public:
 // c_k = log w_k - log det(2 pi Sigma_k) / 2; protovalues q_k = (x - mu_k)^T P_k (x - mu_k) and log p.
 // A move of the coordinates in C changes q_k by 2 sum_{a in C} delta_a (P_k (x - mu_k))_a
 // + sum_{a, b in C} delta_a delta_b (P_k)_ab.  L_k (Cholesky factor of Sigma_k) draws exact starts.
-class GmmFull final: public mci::SamplingFunctionInterface
+class GmmFull final: public ObservedPDF
 {
     int _K;
     std::vector<double> _w, _mu, _P, _c, _L;
@@ -166,8 +188,9 @@ protected:
     mci::SamplingFunctionInterface *_clone() const final { return new GmmFull(_ndim, _K, _w, _mu, _P, _c, _L); }
 public:
     GmmFull(int nd, int K, std::vector<double> w, std::vector<double> mu, std::vector<double> P, std::vector<double> c, std::vector<double> L):
-            mci::SamplingFunctionInterface(nd, K + 1), _K(K), _w(std::move(w)), _mu(std::move(mu)), _P(std::move(P)), _c(std::move(c)),
+            ObservedPDF(nd, K + 1), _K(K), _w(std::move(w)), _mu(std::move(mu)), _P(std::move(P)), _c(std::move(c)),
             _L(std::move(L)) {}
+    double logValue(const double pv[]) const final { return pv[_K]; }
     double logp(const double q[]) const
     {
         double m = -std::numeric_limits<double>::infinity();
@@ -263,6 +286,52 @@ public:
             for (int i = 0; i < _ndim; ++i) { if (flags[i]) { out[i] = in[i]*in[i]; } }
         }
         else { observableFunction(in, out); }
+    }
+};
+
+// dependent observables (DependentObservableInterface.hpp:24-54): the log value or the proto values of
+// sampling function j as its observation callback last saw them, and the squares of the last values
+// of an earlier observable k (read through its accumulator)
+class PdfObs final: public mci::ObservableFunctionInterface, public mci::DependentObservableInterface
+{
+    int _kind, _j;
+    const ObservedPDF *_pdf = nullptr;
+protected:
+    mci::ObservableFunctionInterface *_clone() const final { return new PdfObs(_ndim, _kind, _j); }
+public:
+    PdfObs(int nd, int kind, int j):
+            mci::ObservableFunctionInterface(nd, kind == OBS_PDF_PROTO ? nd : 1, false), mci::DependentObservableInterface(true), _kind(kind), _j(j) {}
+    void registerDeps(const mci::SamplingFunctionContainer &pdfcont, const std::vector<mci::AccumulatorInterface *> & /*accuvec*/, int /*selfIdx*/) final
+    {
+        _pdf = dynamic_cast<const ObservedPDF *>(&pdfcont.getSamplingFunction(_j));
+        if (_pdf == nullptr) { throw std::invalid_argument("PdfObs: not a harness sampling function"); }
+    }
+    void deregisterDeps() final { _pdf = nullptr; }
+    void observableFunction(const double /*in*/[], double out[]) final
+    {
+        if (_pdf == nullptr || (int)_pdf->seen_pv.size() != _pdf->getNProto()) { throw std::runtime_error("PdfObs: no observation yet"); }
+        if (_kind == OBS_PDF_PROTO) { for (int i = 0; i < _nobs; ++i) { out[i] = _pdf->seen_pv[i]; } }
+        else { out[0] = _pdf->logValue(_pdf->seen_pv.data()); }
+    }
+};
+
+class DepSquareObs final: public mci::ObservableFunctionInterface, public mci::DependentObservableInterface
+{
+    int _k;
+    const double *_v = nullptr;
+protected:
+    mci::ObservableFunctionInterface *_clone() const final { return new DepSquareObs(_ndim, _nobs, _k); }
+public:
+    DepSquareObs(int nd, int nobs, int k): mci::ObservableFunctionInterface(nd, nobs, false), mci::DependentObservableInterface(false), _k(k) {}
+    void registerDeps(const mci::SamplingFunctionContainer & /*pdfcont*/, const std::vector<mci::AccumulatorInterface *> &accuvec, int selfIdx) final
+    {
+        if (_k >= (int)accuvec.size() || !isObsDepValid(accuvec, selfIdx, _k)) { throw std::invalid_argument("DepSquareObs: invalid dependency"); }
+        _v = accuvec[_k]->getObsValues();
+    }
+    void deregisterDeps() final { _v = nullptr; }
+    void observableFunction(const double /*in*/[], double out[]) final
+    {
+        for (int i = 0; i < _nobs; ++i) { out[i] = _v[i] * _v[i]; }
     }
 };
 
@@ -368,6 +437,7 @@ int main()
     std::unique_ptr<mci::MCI> mci;
     std::unique_ptr<mci::MultiStepMove> msm; // a MultiStepMove being assembled (msmove, mspdf, msapply)
     int ndim = 0;
+    std::vector<int> obs_nobs; // components of each observable added, in order
     Trace tr;
     std::string line;
     while (std::getline(std::cin, line)) {
@@ -378,6 +448,7 @@ int main()
             if (cmd == "create") {
                 is >> ndim;
                 mci.reset(new mci::MCI(ndim));
+                obs_nobs.clear();
             }
             else if (cmd == "seed") { uint64_t s; is >> s; mci->setSeed(s); }
             else if (cmd == "target") { mci->setTargetAcceptanceRate(readDoubles(is, 1)[0]); }
@@ -419,10 +490,18 @@ int main()
                 is >> kind >> np;
                 auto p = readDoubles(is, np);
                 is >> bs >> nskip >> eq >> est;
-                mci->addObservable(std::unique_ptr<mci::ObservableFunctionInterface>(new CatObs(ndim, kind, p)), bs, nskip, eq != 0, estimOf(est));
+                std::unique_ptr<mci::ObservableFunctionInterface> o;
+                if (kind == OBS_PDF_LOG || kind == OBS_PDF_PROTO) { o.reset(new PdfObs(ndim, kind, p.empty() ? 0 : static_cast<int>(p[0]))); }
+                else if (kind == OBS_DEP_SQUARE) {
+                    const int k = static_cast<int>(p.at(0));
+                    o.reset(new DepSquareObs(ndim, obs_nobs.at(k), k));
+                }
+                else { o.reset(new CatObs(ndim, kind, p)); }
+                obs_nobs.push_back(o->getNObs());
+                mci->addObservable(std::move(o), bs, nskip, eq != 0, estimOf(est));
             }
-            else if (cmd == "popobs") { mci->popObservable(); }
-            else if (cmd == "clearobs") { mci->clearObservables(); }
+            else if (cmd == "popobs") { mci->popObservable(); obs_nobs.pop_back(); }
+            else if (cmd == "clearobs") { mci->clearObservables(); obs_nobs.clear(); }
             else if (cmd == "movex") { mci->moveX(); }
             else if (cmd == "newrandomx") { mci->newRandomX(); }
             else if (cmd == "centerx") { mci->centerX(); }

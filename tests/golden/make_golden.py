@@ -19,8 +19,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from mcitest import (ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, OBS_CONST, OBS_EXP_X2, OBS_GAUSS_X2,  # noqa: E402
-                     OBS_INDICATOR, OBS_X, OBS_X2, OBS_X2_SUM, OBS_X_SUM, PDF_EXP_DIAG, PDF_GAUSS_DIAG,
+from mcitest import (ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, OBS_CONST, OBS_DEP_SQUARE, OBS_EXP_X2,  # noqa: E402
+                     OBS_GAUSS_X2, OBS_INDICATOR, OBS_PDF_LOG, OBS_PDF_PROTO, OBS_X, OBS_X2, OBS_X2_SUM, OBS_X_SUM, PDF_EXP_DIAG, PDF_GAUSS_DIAG,
                      PDF_GAUSS_ISO, PDF_GMM_ISO, REF_BIN, ROOT, fmt, ref_run, script_setup)
 
 FLAGS = "g++ 11.4 -std=c++14 -O2 -ffp-contract=off -include stdexcept (oracle/Makefile target _ref/mci_ref)"
@@ -138,6 +138,24 @@ def scenarios():
                                                                  multistep=(2, 1, 0, 1, None, [(PDF_GAUSS_ISO, []), (PDF_EXP_DIAG, [0.5, 0.5, 0.5])]),
                                                                  pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0])], obs=[(OBS_X2_SUM, [3.0], 16, 1, 0, ESTIM_UNCORR)]),
                   calls=[dict(nmc=1024, find=1, decorr=1)]))
+    # dependent observables (DependentObservableInterface.hpp): a sampling function's log value and
+    # proto values as its observation callback sees them, with different nskip (the callback runs
+    # every gcd of them, MCIntegrator.cpp:294-305), and the squares of an earlier observable's last
+    # values; automatic decorrelation takes the equilibration observables among them
+    S.append(dict(name="depobs_diag_vec", cfg=dict(ndim=4, seed=2718, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0, 0.25])],
+                                                   obs=[(OBS_X2, [], 16, 2, 1, ESTIM_MJ), (OBS_PDF_PROTO, [0], 16, 4, 1, ESTIM_MJ),
+                                                        (OBS_PDF_LOG, [0], 1, 6, 0, ESTIM_CORR), (OBS_DEP_SQUARE, [0], 8, 4, 1, ESTIM_UNCORR)],
+                                                   x0=[2.0, -1.0, 0.5, 3.0]),
+                  calls=[dict(nmc=2 ** 15, find=1, decorr=1), dict(nmc=2 ** 14, find=0, decorr=0)]))
+    S.append(dict(name="depobs_two_pdfs_all", cfg=dict(ndim=3, seed=31415, pdfs=[(PDF_GAUSS_ISO, []), (PDF_EXP_DIAG, [0.5, 1.0, 1.5])],
+                                                       obs=[(OBS_X2_SUM, [], 32, 1, 1, ESTIM_MJ), (OBS_PDF_LOG, [1], 32, 2, 1, ESTIM_UNCORR),
+                                                            (OBS_PDF_PROTO, [1], 0, 3, 0, ESTIM_NOOP), (OBS_PDF_LOG, [0], 32, 4, 1, ESTIM_MJ),
+                                                            (OBS_DEP_SQUARE, [0], 32, 4, 1, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=4096, find=1, decorr=1), dict(nmc=8192, find=0, decorr=0)]))
+    S.append(dict(name="depobs_gmm_pbc", cfg=dict(ndim=2, seed=99, irange=([-1., -2.], [1., 3.]), move=(0, 1, 1, None),
+                                                  pdfs=[(PDF_GMM_ISO, [2, 0.3, 0.7, 0.5, 1.2, 0.2, 0.1, -0.5, 1.0])],
+                                                  obs=[(OBS_PDF_LOG, [0], 64, 1, 1, ESTIM_MJ), (OBS_DEP_SQUARE, [0], 64, 2, 1, ESTIM_UNCORR)]),
+                  calls=[dict(nmc=64 * 128, find=1, decorr=1)]))
     return S
 
 

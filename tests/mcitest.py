@@ -25,6 +25,7 @@ MOVE_ALL, MOVE_VEC = 0, 1
 SRRD_UNIFORM = 0
 PDF_GAUSS_ISO, PDF_GAUSS_DIAG, PDF_EXP_DIAG, PDF_GMM_ISO = 1, 2, 3, 4
 OBS_X2, OBS_X2_SUM, OBS_X, OBS_X_SUM, OBS_CONST, OBS_EXP_X2, OBS_INDICATOR, OBS_GAUSS_X2 = 1, 2, 3, 4, 5, 6, 7, 8
+OBS_PDF_LOG, OBS_PDF_PROTO, OBS_DEP_SQUARE = 9, 10, 11
 RNG_MT, RNG_PHILOX = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 COUPLING_INDEPENDENT, COUPLING_SHARED = 0, 1

@@ -19,14 +19,22 @@ pytestmark = pytest.mark.gpu
 
 
 def exact_components(cfg):
-    """Observable components that must be bit-identical: all but those whose observable calls
-    exp (device ocml exp and glibc exp may differ by one ulp on a sample; the decisions, the
-    positions and every other component stay bit-exact)."""
-    mask = []
+    """Observable components that must be bit-identical: all but those whose value passes through
+    exp or log (device ocml exp/log and glibc's may differ by one ulp on a sample; the decisions,
+    the positions and every other component stay bit-exact): EXP_X2, the log value of a mixture
+    density, and the squares of such observables."""
+    from mcitest import OBS_DEP_SQUARE, OBS_PDF_LOG, OBS_PDF_PROTO, PDF_GMM_ISO
+    masks = []
     for kind, params, *_ in cfg.get("obs", []):
-        n = cfg["ndim"] if kind in (OBS_X2, OBS_X) else 1
-        mask += [kind != OBS_EXP_X2] * n
-    return np.array(mask, dtype=bool)
+        if kind == OBS_DEP_SQUARE:
+            masks.append(masks[int(params[0])])
+            continue
+        n = cfg["ndim"] if kind in (OBS_X2, OBS_X, OBS_PDF_PROTO) else 1
+        exact = kind != OBS_EXP_X2
+        if kind == OBS_PDF_LOG:
+            exact = cfg["pdfs"][int(params[0]) if params else 0][0] != PDF_GMM_ISO
+        masks.append([exact] * n)
+    return np.array([m for ms in masks for m in ms], dtype=bool)
 
 
 def check_components(got, ref, exact):
@@ -407,3 +415,43 @@ def test_full_size_shared_coupling_physics():
     sig, _ = sigma_a(16)
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
     assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
+
+
+@pytest.mark.parametrize("rng", [RNG_MT, RNG_PHILOX])
+def test_dependent_observables_many_walkers(rng):
+    """Dependent observables (a sampling function's log value and proto values, the squares of an
+    earlier observable) on 100 walkers with automatic calibration and decorrelation: bit-exact with
+    the oracle, which matches the reference's own chains with them (depobs_* in chains.json)."""
+    from mcitest import OBS_DEP_SQUARE, OBS_PDF_LOG, OBS_PDF_PROTO
+    a = np.array([0.5, 1.0, 2.0, 0.25])
+    cfg = dict(ndim=4, seed=2718, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, a)],
+               obs=[(OBS_X2, [], 16, 2, 1, ESTIM_MJ), (OBS_PDF_PROTO, [0], 16, 4, 1, ESTIM_MJ),
+                    (OBS_PDF_LOG, [0], 1, 6, 0, ESTIM_CORR), (OBS_DEP_SQUARE, [0], 8, 4, 1, ESTIM_UNCORR)])
+    W, nmc = 100, 2 ** 12
+    res = {}
+    for prefix in ("orc", "mci"):
+        c = chain_setup(prefix, cfg, nwalkers=W)
+        c.setRng(rng)
+        a_, e_ = c.integrate(nmc)
+        res[prefix] = (a_, e_, *c.walkerResults(), c.getWalkerSteps(), c.getX(), *c.walkerAcceptance())
+    for k in range(len(res["orc"])):
+        assert np.array_equal(res["orc"][k], res["mci"][k]), k
+
+
+def test_dependent_observable_rules_on_the_device():
+    from mcitest import E_INVALID, MciError, OBS_DEP_SQUARE, OBS_PDF_LOG, OBS_PDF_PROTO, PDF_GAUSS_ISO
+    c = Chain("mci", 3, 64)
+    c.addSamplingFunction(PDF_GAUSS_ISO)
+    c.addObservable(OBS_X, [], 4, 2, 0, ESTIM_UNCORR)
+    for params, nskip in (([1], 2), ([0], 3), ([], 2)):
+        with pytest.raises(MciError) as ei:
+            c.addObservable(OBS_DEP_SQUARE, params, 4, nskip, 0, ESTIM_UNCORR)
+        assert ei.value.code == E_INVALID
+    c.addObservable(OBS_PDF_LOG, [1], 4, 1, 0, ESTIM_UNCORR)
+    with pytest.raises(MciError):
+        c.integrate(64, False, False)
+    c.addSamplingFunction(PDF_GAUSS_ISO)
+    c.integrate(64, False, False)
+    c.addObservable(OBS_PDF_PROTO, [0], 4, 1, 0, ESTIM_UNCORR)
+    with pytest.raises(MciError):
+        c.integrate(64, False, False)
