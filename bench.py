@@ -36,6 +36,15 @@ PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector
 SEED = 1337
 
 
+def cpu_share():
+    """CPU processes for the reference CPU path: the cores this process may run on, capped by the
+    host's CPU share for one GPU (OMP_NUM_THREADS on the GPU host, 16: worker pools there are
+    sized to it whatever the machine shows).  Returns (processes, cores visible)."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(visible, share)), visible
+
+
 def sigmas():
     with open(os.path.join(ROOT, "tests", "golden", "sigmas.json")) as f:
         return np.array([float.fromhex(v) for v in json.load(f)["sigma16"]])
@@ -59,7 +68,11 @@ def make_mci(walkers, precision=0):
 
 def ess_per_obs(err, walkers_steps):
     """ESS_j = s_j^2 / err_j^2 with the per-step variance s^2 of x_i^2 (2 sigma_i^4) and of |x|^2
-    (sum 2 sigma_i^4) under the Gaussian target."""
+    (sum 2 sigma_i^4) under the Gaussian target.  err is the standard error of the mean of the
+    independent walkers' means (calibrated: the deviations from the analytic values are N(0, 1) in
+    units of it); the MPIMCI combine of the walkers' MJBlocker errors on 64 blocks of 160 steps is
+    about 14 % smaller on this workload (its blocking test has little power on 64 blocks), and
+    would overstate ESS by about 1.34x."""
     sig = sigmas()
     s2 = np.concatenate([2 * sig ** 4, [np.sum(2 * sig ** 4)]])
     return s2 / err ** 2
@@ -76,7 +89,7 @@ def cpu_reference_gauss(dim, sig, obs_script, budget_s):
         return None
     a = " ".join(float(v).hex() for v in 0.5 / sig ** 2)
     script = f"create {dim}\nmove 0 1 1\nndecorr {NDECORR}\npdf 2 {dim} {a}\n{obs_script}endbench\n"
-    procs = max(1, min(16, os.cpu_count() or 1))
+    procs, _ = cpu_share()
     # calibrate the per-walker cost with one process, then size the sample to the budget
     t0 = time.time()
     subprocess.run([binary], input=f"bench 4 {NMC} 9001\n" + script, capture_output=True, text=True, check=True)
@@ -92,8 +105,12 @@ def cpu_reference_gauss(dim, sig, obs_script, budget_s):
     for p in ps:
         p.wait()
     wall = time.time() - t0
-    return dict(walkers=sum(o["walkers"] for o in outs), steps=sum(o["walker_steps"] for o in outs), wall=wall, procs=procs,
-                err2=np.sum([[float.fromhex(v) for v in o["err2_sum"]] for o in outs], axis=0))
+    nw = sum(o["walkers"] for o in outs)
+    s1 = np.sum([[float.fromhex(v) for v in o["avg_sum"]] for o in outs], axis=0)
+    s2 = np.sum([[float.fromhex(v) for v in o["avg2_sum"]] for o in outs], axis=0)
+    spread = np.sqrt(np.maximum(s2 / nw - (s1 / nw) ** 2, 0.0) * nw / (nw - 1) / nw)  # standard error of the walker means
+    return dict(walkers=nw, steps=sum(o["walker_steps"] for o in outs), wall=wall, procs=procs,
+                err2=np.sum([[float.fromhex(v) for v in o["err2_sum"]] for o in outs], axis=0), spread_err=spread)
 
 
 def cpu_baseline(budget_s=20.0):
@@ -102,13 +119,68 @@ def cpu_baseline(budget_s=20.0):
     if r is None:
         return None
     nw, steps, wall, procs = r["walkers"], r["steps"], r["wall"], r["procs"]
-    err = np.sqrt(r["err2"]) / nw
-    ess = ess_per_obs(err, steps)
+    ess = ess_per_obs(r["spread_err"], steps)
     return dict(value=steps / wall, unit="walker-steps/s", cores=procs, kind="reference",
                 ess_per_s=float(np.min(ess)) / wall,
                 sample=f"{nw} walkers x {NMC} steps (+calibration, {NDECORR} decorrelation steps each) in {procs} "
                        f"processes, {wall:.1f} s wall; reference built from its sources with -O3 -flto",
                 seconds=wall)
+
+
+def cpu_single_core_cfg1():
+    """The reference's own CPU case, timed on one core of this host (the harness model of
+    benchmark/common/MCIBenchmarks.hpp:34-45): d = 3, p ~ exp(-|x|^2), x_i^2 in 16 blocks of 62,500
+    with MJBlocker, default all-coordinate uniform move with automatic calibration and automatic
+    decorrelation, 1 walker, 10^6 steps, seed 1337.  steps/s over the whole integrate and over the
+    sampling alone (a second integrate without calibration and decorrelation); ESS = Var(x_i^2) /
+    err^2 with the exact per-step variance 1/2."""
+    binary = os.path.join(ROOT, "oracle", "_ref", "mci_ref_fast")
+    if not os.path.exists(binary):
+        return None
+    nmc = 1000000
+    script = (f"create 3\nseed 1337\npdf 1 0\nobs 1 0 62500 1 1 4\nintegrate {nmc} 1 1\nintegrate {nmc} 0 0\n")
+    out = subprocess.run([binary], input=script, capture_output=True, text=True, check=True).stdout
+    recs = [json.loads(l) for l in out.splitlines() if l.startswith("{")]
+    full, samp = recs[0], recs[1]
+    err = np.array([float.fromhex(v) for v in samp["err"]])
+    ess = 0.5 / err ** 2
+    return {"workload": "d=3 exp(-|x|^2), x_i^2 in 16 blocks of 62500, MJBlocker, all-move, 1 walker x 1e6 steps, seed 1337",
+            "cores": 1, "integrate_s": full["seconds"], "steps_per_s_integrate": nmc / full["seconds"],
+            "sampling_s": samp["seconds"], "steps_per_s_sampling": nmc / samp["seconds"],
+            "ess_per_s_sampling": float(np.min(ess)) / samp["seconds"],
+            "avg": [float.fromhex(v) for v in samp["avg"]], "err": list(err)}
+
+
+def run_full_traces(nmc=8192):
+    """The benchmark workload with Full accumulators (blocksize 1, the reference's default
+    addObservable(obs)): 65,536 walkers x 8,192 steps, every step's x_i^2 (16) and |x|^2 stored,
+    Correlated estimator (MJBlocker on 8,192 values per walker and component): 73 GB of traces in
+    HBM.  Reports the trace kernel's HBM write rate (algorithmic bytes / kernel time) beside the
+    8 TB/s peak, and the whole integrate."""
+    from mcintegratorplusplus_amd import MCI, constants as K
+    sig = sigmas()
+    m = MCI(D, W)
+    m.setSeed(SEED)
+    m.setRandomMode(K.RNG_PHILOX)
+    m.setWalkerCoupling(K.COUPLING_SHARED)
+    m.setTrialMove(K.SRRD_UNIFORM, 1, 1)
+    m.setNdecorrelationSteps(NDECORR)
+    m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+    m.addObservable(K.OBS_X2, (), 1, 1, True, estimator=K.ESTIM_CORRELATED)
+    m.addObservable(K.OBS_X2_SUM, (), 1, 1, True, estimator=K.ESTIM_CORRELATED)
+    m.integrate(nmc)
+    t0 = time.perf_counter()
+    avg, err = m.integrate(nmc)
+    ms = 1000 * (time.perf_counter() - t0)
+    kt = m.getTimings()
+    trace_bytes = W * nmc * (D + 1) * 8
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    z = (avg - exact) / err
+    return {"workload": f"{W} walkers x {nmc} steps, d={D}, x_i^2 and |x|^2 as Full traces (blocksize 1), Correlated estimator",
+            "trace_bytes": trace_bytes, "trace_kernel_ms": kt[2], "hbm_gbps": trace_bytes / (kt[2] / 1000) / 1e9,
+            "hbm_frac_of_8tbs": trace_bytes / (kt[2] / 1000) / 8e12, "integrate_ms": ms,
+            "phase_ms": {"calibration": kt[0], "decorrelation": kt[1], "sampling_kernel": kt[2], "estimation": kt[3]},
+            "walker_steps_per_s": W * nmc / (ms / 1000), "max_abs_z": float(np.max(np.abs(z))), "mean_z2": float(np.mean(z * z))}
 
 
 def profile_summary():
@@ -190,7 +262,7 @@ def cpu_batched_sample(ndim, nmc, w, s, mu, budget_s=8.0):
     t0 = time.time()
     subprocess.run([binary], input="".join(block(i) for i in probe), capture_output=True, text=True, check=True)
     rate1 = sum(int(nmc[i]) for i in probe) / max(time.time() - t0, 1e-3)
-    procs = max(1, min(16, os.cpu_count() or 1))
+    procs, _ = cpu_share()
     k = max(1, int(np.ceil(float(np.sum(nmc)) / (rate1 * procs * budget_s))))
     chosen = list(range(0, n, k))
     shares = [chosen[r::procs] for r in range(procs)]
@@ -263,7 +335,7 @@ def cpu_learned_target(w, mu, cov, var, nsteps, budget_s=10.0):
     a = " ".join(float(v).hex() for v in p)
     bs = max(1, nsteps // 64)
     script = f"create {d}\nmove 0 1 1\nndecorr 0\npdf 5 {len(p)} {a}\nobs 2 0 {bs} 1 1 4\nobs 7 1 0 {bs} 1 1 4\nendbench\n"
-    procs = max(1, min(16, os.cpu_count() or 1))
+    procs, _ = cpu_share()
     t0 = time.time()
     subprocess.run([binary], input=f"bench 4 {nsteps} 5001\n" + script, capture_output=True, text=True, check=True)
     per_walker = max((time.time() - t0) / 4, 1e-4)
@@ -367,7 +439,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
-    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned"],
+    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full"],
                     help="also measure the precision form, the batched integrals and the learned proposal (all), "
                          "none of them, or one of them")
     ap.add_argument("--copy-probe", action="store_true",
@@ -402,11 +474,14 @@ def main():
         ktimes.append(m.getTimings())
         results = (avg, err)
     total = time.perf_counter() - t_all
+    wavg, _ = m.walkerResults()  # after the timed region: the walker means of the last integrate
+    err_spread = wavg.std(axis=0, ddof=1) / np.sqrt(W)
     ms_per_step = 1000 * total / args.steps
     kt = np.mean(ktimes, axis=0)  # calibration, decorrelation, sampling kernel, estimation, total (ms)
     value = W * NMC / (ms_per_step / 1000)
     avg, err = results
-    ess = ess_per_obs(err, W * NMC)
+    ess = ess_per_obs(err_spread, W * NMC)
+    ess_mj = ess_per_obs(err, W * NMC)
     sig = sigmas()
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
     achieved = FLOPS_PER_STEP * W * NMC / (kt[2] / 1000) / 1e12
@@ -430,8 +505,10 @@ def main():
                    "walkers": W, "nmc": NMC, "ndim": D, "rng": "philox4x32-10",
                    "parallelism": "one walker per lane, 1 GPU"},
         "ess_per_s": float(np.min(ess)) / (ms_per_step / 1000),
+        "ess_error_bar": "standard error of the 65536 walker means (the MJBlocker combine is 14 % smaller here)",
+        "ess_per_s_with_mjblocker_errors": float(np.min(ess_mj)) / (ms_per_step / 1000),
         "phase_ms": {"calibration": kt[0], "decorrelation": kt[1], "sampling_kernel": kt[2], "estimation": kt[3]},
-        "max_abs_z_vs_analytic": float(np.max(np.abs(avg - exact) / err)),
+        "max_abs_z_vs_analytic": float(np.max(np.abs(avg - exact) / err_spread)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": prof.get("bytes_per_launch"),
                      "kernel": "k_fast64 sampling kernel (10240 steps x 65536 walkers)",
@@ -446,7 +523,7 @@ def main():
     }
     if args.workloads != "none" and args.precision == 0 and not args.generic:
         runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
-                "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu))}
+                "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces)}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
             if args.workloads in ("all", name):
@@ -457,6 +534,12 @@ def main():
         if cb is not None:
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
             line["cpu_baseline"]["ess_per_s"] = cb["ess_per_s"]
+            line["cpu_baseline"]["host_cpus_visible"] = cpu_share()[1]
+            line["cpu_baseline"]["cores_note"] = ("processes = the host's CPU share for one GPU (OMP_NUM_THREADS), "
+                                                  "at most the CPUs this process may use")
+            c1 = cpu_single_core_cfg1()
+            if c1 is not None:
+                line["cpu_baseline"]["cfg1_single_core"] = c1
             line["speedup_vs_cpu_steps"] = value / cb["value"]
             line["speedup_vs_cpu_ess"] = line["ess_per_s"] / cb["ess_per_s"]
     print(json.dumps(line))

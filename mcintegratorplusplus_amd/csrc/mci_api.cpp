@@ -426,7 +426,7 @@ mci_status decorrelate_auto(mci_ctx *c)
         HIPCHK(data.back().ensure((size_t)min_nmc * o.nobs * W));
         d.data = data.back().p;
         scr.emplace_back();
-        if (is_pow2(min_nmc)) { HIPCHK(scr.back().ensure((size_t)min_nmc * o.nobs * W)); }
+        if (is_pow2(min_nmc)) { HIPCHK(scr.back().ensure((size_t)mj_scratch_rows(min_nmc) * o.nobs * W)); }
         od.push_back(d);
         nd += o.nobs;
     }
@@ -1240,13 +1240,13 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
             d.nstore = nstore_of(o.blocksize, o.nskip, nmc);
             HIPCHK(c->accdata[i].ensure((size_t)d.nstore * o.nobs * W));
             d.data = c->accdata[i].p;
-            if (needs_scratch(o.estim, d.nstore)) { HIPCHK(c->scratch[i].ensure((size_t)d.nstore * o.nobs * W)); }
+            if (needs_scratch(o.estim, d.nstore)) { HIPCHK(c->scratch[i].ensure((size_t)mj_scratch_rows(d.nstore) * o.nobs * W)); }
             od.push_back(d);
             voff += o.nobs;
         }
         fast_main = fast && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec);
         if (c->prec == MCI_PREC_FP32 && !fast_main) {
-            return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
+            return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, Block or Simple accumulators, nskip 1)");
         }
         if (c->prec == MCI_PREC_FP32 && nmc > INT32_MAX) {
             for (const ObsDev &d : od) {
@@ -1456,7 +1456,7 @@ mci_status mci_estimate(int estim, int64_t n, int ndim, const double *x, double 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { return fail(MCI_E_DEVICE, "no HIP device available"); }
     DBuf<double> d, s, a, e;
     HIPCHK(d.ensure((size_t)n * ndim));
-    HIPCHK(s.ensure((size_t)n * ndim));
+    HIPCHK(s.ensure((size_t)mj_scratch_rows(n) * ndim));
     HIPCHK(a.ensure(ndim));
     HIPCHK(e.ensure(ndim));
     HIPCHK(hipMemcpy(d.p, x, sizeof(double) * n * ndim, hipMemcpyHostToDevice));

@@ -137,7 +137,8 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t t
 }
 
 // observable set of one launch (template flags: no per-step branches on them)
-enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8 }; // F_P2: the index range is a power of two
+// F_P2: the index range is a power of two; F_FULL: Full accumulators (every step's values stored)
+enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8, F_FULL = 16 };
 
 // FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
 // 1,024 waves), so a step costs about 4 cycles per instruction of any kind and 16 per
@@ -155,10 +156,21 @@ enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8 }; // F_P2: the index range is a
 // Scalar instructions cost an issue slot as well: keeping the counter in scalar registers when a
 // wave's counters agree (three fewer vector multiplies, about twenty more scalar instructions per
 // step) measured 11 % slower.
+//
+// With Full accumulators (F_FULL, FullAccumulator.cpp:14-18: the reference's default for
+// addObservable(obs)) every step stores the whole observable vector: row s of X2 is x_i^2 for all
+// i, of X2_SUM the ordered sum (0 + x_0^2 + x_1^2 + ...)/c and of EXP_X2 exp(-that/c), the
+// expressions of the generic kernel, so the traces are bit-identical to it.  Each row of a wave is
+// one 512-byte store per component; the coordinates are read from LDS, the exp is recomputed only
+// when the walker moved.  These launches are bound by HBM writes (136 bytes per walker-step at
+// d = 16 with X2 and X2_SUM).
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
-    constexpr bool ACC = (FL & 7) != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool FULL = (FL & F_FULL) != 0;
+    constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool ACC = (FL & 7) != 0 && !FULL;  // dwell-time block sums
+    constexpr bool REC = HX2 && !FULL;            // 16-byte {x_i, block sum} records
     constexpr bool P2 = (FL & F_P2) != 0;
     extern __shared__ double lds[];
     const int d = A.ndim;
@@ -179,9 +191,9 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         if (A.pend->done) { return; }
     }
     const int64_t B = A.blocksize;
-    double2 *rec = reinterpret_cast<double2 *>(lds);  // X2: [d][64] {x_i, block sum of x_i^2}
+    double2 *rec = reinterpret_cast<double2 *>(lds);  // X2 blocks: [d][64] {x_i, block sum of x_i^2}
     double *xs = lds;                                  // otherwise [d][64] x_i
-    double *as = lds + (HX2 ? 2 : 1) * d * WAVE;       // [d] a_i
+    double *as = lds + (REC ? 2 : 1) * d * WAVE;       // [d] a_i
     float *cs = reinterpret_cast<float *>(as + d);     // [d] -a_i log2 e in FP32 (screening exponent)
     for (int i = lane; i < d; i += WAVE) {
         as[i] = A.a[i];
@@ -191,7 +203,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     if (live) {
         for (int i = 0; i < d; ++i) {
             const double v = A.x64[(int64_t)i * A.W + w];
-            if (HX2) { rec[i * WAVE + lane] = make_double2(v, (double)B * (v * v)); }
+            if (REC) { rec[i * WAVE + lane] = make_double2(v, (double)B * (v * v)); }
             else { xs[i * WAVE + lane] = v; }
             r = r + v * v;
         }
@@ -206,7 +218,12 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     double Sr = 0.0, Se = 0.0, e = 0.0;
     int64_t b = 0;
     if (ACC) { Sr = (double)B * r; }
-    if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+    if (HEXP && ACC) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+    if (HEXP && FULL) {
+        double q0 = 0.0;
+        for (int i = 0; i < d; ++i) { const double v = xs[i * WAVE + lane]; q0 = q0 + v * v; }
+        e = exp(-(q0 / A.exp_div));
+    }
     // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
     // path has no divergent branch: the decision is a select and the LDS state is always
     // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
@@ -225,7 +242,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             const int v = draw_vidx<P2>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
             const int slot = v * WAVE + lane;
             double xo, So = 0.0;
-            if (HX2) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
+            if (REC) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
             else { xo = xs[slot]; }
             const float cv = cs[v];
             // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
@@ -248,10 +265,11 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             const bool acc = U < fmaf(F, 0.9999f, -2.0f);
             const bool band = !acc && U <= F * 1.0001f;
             if (acc) { ++na; }
+            bool moved = acc;
             const double xsel = acc ? xn : xo;
             if (ACC) {
                 const double dx2 = acc ? dx2n : 0.0;
-                if (HX2) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
+                if (REC) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
                 else { xs[slot] = xsel; }
                 r = r + dx2;
                 if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
@@ -275,9 +293,10 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 const double ex = xn2 * av - xo2 * av;
                 if (band && u <= exp(-ex)) {
                     na += 1u;
+                    moved = true;
                     if (ACC) {
                         const double dx2 = dx2n;
-                        if (HX2) { rec[slot] = make_double2(xn, So + dx2 * mult); }
+                        if (REC) { rec[slot] = make_double2(xn, So + dx2 * mult); }
                         else { xs[slot] = xn; }
                         r = r + dx2;
                         if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
@@ -287,6 +306,22 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 }
             }
             if (ACC) { mult = mult - 1.0; }
+            if (FULL) {
+                // this step's row of every trace (the observable values after the step)
+                const int64_t row = s + q;
+                double q2s = 0.0;
+                for (int i = 0; i < d; ++i) {
+                    const double v = xs[i * WAVE + lane];
+                    const double q2 = v * v;
+                    if (HX2) { A.d_x2[(row * d + i) * A.W + w] = q2; }
+                    q2s = q2s + q2;
+                }
+                if (HSUM) { A.d_x2sum[row * A.W + w] = q2s / A.x2sum_div; }
+                if (HEXP) {
+                    if (moved) { e = exp(-(q2s / A.exp_div)); }
+                    A.d_exp[row * A.W + w] = e;
+                }
+            }
             // words 2 and 3 after the exact path, which still reads this step's u_hi and u_lo:
             // they then take over its registers (no copies at the loop's back edge)
             philox_tail9(s0, s3, A.key1, w2, w3);
@@ -299,7 +334,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             double rx = 0.0, sS = 0.0;
             for (int i = 0; i < d; ++i) {
                 double v2;
-                if (HX2) {
+                if (REC) {
                     const double2 rv = rec[i * WAVE + lane];
                     v2 = rv.x;
                     sS = sS + rv.y;
@@ -319,7 +354,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
             ++b;
         }
     }
-    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = HX2 ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; }
+    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = REC ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; }
     A.ctr[w] = t;
     A.acc[w] = nacc;
     A.rej[w] = A.nsteps - nacc;
@@ -547,7 +582,7 @@ template <int FL>
 void launch_fl(int prec, dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
 {
     if (prec == MCI_PREC_FP64) { hipLaunchKernelGGL(k_fast64<FL>, grid, dim3(WAVE), sh, s, A); }
-    else { hipLaunchKernelGGL(k_fast32<FL>, grid, dim3(WAVE), sh, s, A); }
+    else if constexpr ((FL & F_FULL) == 0) { hipLaunchKernelGGL(k_fast32<FL>, grid, dim3(WAVE), sh, s, A); }
 }
 } // namespace
 
@@ -565,7 +600,7 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
     bool x2 = false;
     for (int i = 0; i < nobs; ++i) {
         const ObsDev &o = obs[i];
-        if (o.nskip != 1 || o.acc == ACC_FULL) { return false; }
+        if (o.nskip != 1 || (o.acc == ACC_FULL && prec != MCI_PREC_FP64)) { return false; }
         if (o.kind != MCI_OBS_X2 && o.kind != MCI_OBS_X2_SUM && o.kind != MCI_OBS_EXP_X2) { return false; }
         const int bit = 1 << o.kind;
         if (seen & bit) { return false; }
@@ -575,7 +610,7 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
         if (B == -1) { B = bs; }
         else if (B != bs) { return false; }
     }
-    return lds_bytes(prec, ndim, x2) <= lds_limit();
+    return lds_bytes(prec, ndim, x2 && B != 1) <= lds_limit();  // B == 1: Full accumulators, no block-sum records
 }
 
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
@@ -611,6 +646,7 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     int fl = 0;
     for (int i = 0; i < nobs && obs; ++i) {
         const ObsDev &o = obs[i];
+        if (o.acc == ACC_FULL) { fl |= F_FULL; }
         A.blocksize = (o.acc == ACC_SIMPLE) ? nsteps : (int64_t)o.blocksize;
         A.nblocks = o.nstore;
         if (o.kind == MCI_OBS_X2) { A.d_x2 = o.data; fl |= F_X2; }
@@ -620,7 +656,7 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     if (!A.a) { return MCI_E_INVALID_ARGUMENT; } // the fast path needs explicit weights
     if (prec != MCI_PREC_FP64 && A.blocksize > INT32_MAX) { return MCI_E_INVALID_ARGUMENT; }
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
-    const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0);
+    const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0 && !(fl & F_FULL));
     if (A.vshift != 0) { fl |= F_P2; }
     switch (fl) {
     case 0: launch_fl<0>(prec, grid, sh, s, A); break;
@@ -638,7 +674,23 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     case 12: launch_fl<12>(prec, grid, sh, s, A); break;
     case 13: launch_fl<13>(prec, grid, sh, s, A); break;
     case 14: launch_fl<14>(prec, grid, sh, s, A); break;
-    default: launch_fl<15>(prec, grid, sh, s, A); break;
+    case 15: launch_fl<15>(prec, grid, sh, s, A); break;
+    // Full accumulators (FP64 only)
+    case 17: launch_fl<17>(prec, grid, sh, s, A); break;
+    case 18: launch_fl<18>(prec, grid, sh, s, A); break;
+    case 19: launch_fl<19>(prec, grid, sh, s, A); break;
+    case 20: launch_fl<20>(prec, grid, sh, s, A); break;
+    case 21: launch_fl<21>(prec, grid, sh, s, A); break;
+    case 22: launch_fl<22>(prec, grid, sh, s, A); break;
+    case 23: launch_fl<23>(prec, grid, sh, s, A); break;
+    case 25: launch_fl<25>(prec, grid, sh, s, A); break;
+    case 26: launch_fl<26>(prec, grid, sh, s, A); break;
+    case 27: launch_fl<27>(prec, grid, sh, s, A); break;
+    case 28: launch_fl<28>(prec, grid, sh, s, A); break;
+    case 29: launch_fl<29>(prec, grid, sh, s, A); break;
+    case 30: launch_fl<30>(prec, grid, sh, s, A); break;
+    case 31: launch_fl<31>(prec, grid, sh, s, A); break;
+    default: return MCI_E_INVALID_ARGUMENT;
     }
     return hipGetLastError() == hipSuccess ? MCI_OK : MCI_E_DEVICE;
 }

@@ -299,26 +299,35 @@ __device__ inline void mjblocker(int64_t n, const Col &x, double *y, int64_t yst
     int npow = 0;
     while (((int64_t)1 << npow) < n) { ++npow; }
     double mean = 0.0;
-    for (int64_t i = 0; i < n; ++i) { const double v = x[i]; y[i * ystride] = v; mean = mean + v; }
+    for (int64_t i = 0; i < n; ++i) { mean = mean + x[i]; }
     mean = mean / (double)n;
     double var[64], gam[64];
     int64_t nred = n;
-    for (int k = 0; k < npow; ++k) {
+    // One pass per level: the sums of the level and the pair means of the next one, written in
+    // place (index i/2 is written after i has been read).  Level 0 reads the data itself, so the
+    // copy holds at most n/2 values and every level is read once: about 4n values of traffic
+    // where a copy followed by separate sum and halving passes moves 7n.
+    auto level = [&](auto src, int k) {
         double g0 = 0.0, g1 = 0.0;
-        double prev = y[0] - mean;
+        double even = src(0);
+        double prev = even - mean;
         g0 = g0 + prev * prev;
         for (int64_t i = 1; i < nred; ++i) {
-            const double cur = y[i * ystride] - mean;
+            const double raw = src(i);
+            const double cur = raw - mean;
             g0 = g0 + cur * cur;
             g1 = g1 + prev * cur;
             prev = cur;
+            if (i & 1) { y[(i >> 1) * ystride] = 0.5 * (even + raw); }
+            else { even = raw; }
         }
         var[k] = g0 / (double)nred;
         gam[k] = g1 / (double)nred;
-        for (int64_t i = 0; i < nred / 2; ++i) {
-            y[i * ystride] = 0.5 * (y[(2 * i) * ystride] + y[(2 * i + 1) * ystride]);
-        }
         nred = nred / 2;
+    };
+    for (int k = 0; k < npow; ++k) {
+        if (k == 0) { level([&](int64_t i) { return x[i]; }, k); }
+        else { level([&](int64_t i) { return y[i * ystride]; }, k); }
     }
     // M (reversed) and its cumulative sum; largest k with Msum_k < chi^2_{0.95}(k+1)
     double M[64];

@@ -18,10 +18,14 @@ struct CalibCtrl
 };
 
 // one estimator pass over one observable's accumulator data [n][nobs][W]
+// rows of the general MJBlocker's working copy for a series of n values: the blocked levels,
+// n/2 values at most (the data itself is level 0)
+inline int64_t mj_scratch_rows(int64_t n) { return n / 2 > 0 ? n / 2 : 1; }
+
 struct EstimJob
 {
     const double *data;
-    double *scratch;      // same layout as data (MJBlocker's in-place blocking)
+    double *scratch;      // [mj_scratch_rows(n)] rows of the data's layout (MJBlocker's in-place blocking)
     int64_t n, W;
     int nobs, off, estim;
     double scale;         // volume factor (no sampling function), else 1

@@ -1908,7 +1908,7 @@ static mci_status fp32_supported(const orc_ctx *c, const obs_desc *obs, int nobs
         const int bs = o->blocksize == 0 ? -2 : o->blocksize;
         if (o->nskip != 1 || o->blocksize == 1 || (o->kind != MCI_OBS_X2 && o->kind != MCI_OBS_X2_SUM && o->kind != MCI_OBS_EXP_X2) ||
             (seen & (1 << o->kind)) || (B != -1 && B != bs))
-            return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, nskip 1)");
+            return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, Block or Simple accumulators, nskip 1)");
         seen |= 1 << o->kind;
         B = bs;
     }

@@ -455,3 +455,85 @@ def test_dependent_observable_rules_on_the_device():
     c.addObservable(OBS_PDF_PROTO, [0], 4, 1, 0, ESTIM_UNCORR)
     with pytest.raises(MciError):
         c.integrate(64, False, False)
+
+
+@pytest.mark.parametrize("d,W,coupling", [(16, 200, COUPLING_SHARED), (5, 130, COUPLING_INDEPENDENT)])
+def test_full_traces_on_the_specialised_kernel(d, W, coupling):
+    """FullAccumulator traces (the reference's default addObservable(obs): blocksize 1, Correlated)
+    written by the specialised kernel: every step's x_i^2, |x|^2 and exp(-|x|^2/d) of every walker
+    bit-identical to the generic kernel's traces, and to the oracle's (exp to 1e-12); estimates
+    equal the generic kernel's bit for bit.  d = 5 takes the Lemire index with rejections."""
+    print("full traces", d, W, coupling, flush=True)
+    sig = np.random.default_rng(d).uniform(0.5, 2.0, d) if d != 16 else sigma_a(16)[0]
+    cfg = dict(ndim=d, seed=99, ndecorr=500, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, 0.5 / sig ** 2)],
+               obs=[(OBS_X2, [], 1, 1, 1, ESTIM_CORR), (OBS_X2_SUM, [], 1, 1, 1, ESTIM_CORR), (OBS_EXP_X2, [float(d)], 1, 1, 1, ESTIM_CORR)])
+    nmc = 1024
+    out = {}
+    for name in ("fast", "generic", "orc"):
+        c = chain_setup("orc" if name == "orc" else "mci", cfg, nwalkers=W)
+        c.setRng(RNG_PHILOX)
+        c.setCoupling(coupling)
+        if name == "generic":
+            c.setKernelPath(KERNELS_GENERIC)
+        a, e = c.integrate(nmc)
+        out[name] = (c, a, e, c.getX(), *c.walkerAcceptance())
+    f, g, o = out["fast"], out["generic"], out["orc"]
+    for k in (1, 2, 3, 4, 5):
+        assert np.array_equal(f[k], g[k]), k
+    for k in (3, 4, 5):
+        assert np.array_equal(f[k], o[k]), k
+    for i, nobs in enumerate((d, 1, 1)):
+        tf, tg = f[0].observableData(i, nobs), g[0].observableData(i, nobs)
+        assert tf.shape == (W, nmc, nobs)
+        assert np.array_equal(tf, tg), i
+    # the oracle's traces of the first walkers
+    import ctypes as C
+    from mcitest import load
+    lib = load("orc")
+    lib.orc_get_accu_data.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
+    for i, nobs in enumerate((d, 1, 1)):
+        tf = f[0].observableData(i, nobs)
+        for w in (0, 1, W - 1):
+            buf = np.zeros(nmc * nobs)
+            ns = C.c_int64()
+            assert lib.orc_get_accu_data(o[0].ctx, i, w, buf.ctypes.data_as(C.POINTER(C.c_double)), buf.size, C.byref(ns)) == 0
+            ref = buf.reshape(nmc, nobs)
+            if i < 2:
+                assert np.array_equal(tf[w], ref), (i, w)
+            else:
+                np.testing.assert_allclose(tf[w], ref, rtol=1e-12, atol=0)
+
+
+def test_error_bars_calibrated_on_the_benchmark_workload():
+    """The benchmark workload (65,536 walkers x 10,240 steps, d = 16, 64 MJBlocker blocks per walker,
+    MPIMCI combine) integrated 12 times in a row; the 12 x 17 deviations from the analytic values in
+    units of an error bar.  With the standard error of the 65,536 independent walker means the
+    deviations are distributed as N(0, 1) (mean z^2 near 1, Kolmogorov-Smirnov test).  The MPIMCI
+    combine of the walkers' MJBlocker errors is the reference's estimator on 64 blocks of 160 steps
+    per walker, whose blocking test has little power on so few blocks: it is reported beside the
+    walker spread, and must stay within 40 % of it.  (The |x|^2 component is the sum of the
+    others, so the components of one run are not independent; the runs are.)"""
+    from scipy import stats
+    sig, a = sigma_a(16)
+    cfg = cfg2()
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    W = 65536
+    g = chain_setup("mci", cfg, nwalkers=W)
+    g.setRng(RNG_PHILOX)
+    g.setCoupling(COUPLING_SHARED)
+    z_mj, z_sp, ratio = [], [], []
+    for _ in range(12):
+        avg, err = g.integrate(64 * 160)
+        wa, _ = g.walkerResults()
+        err_sp = wa.std(axis=0, ddof=1) / np.sqrt(W)
+        z_mj.append((avg - exact) / err)
+        z_sp.append((avg - exact) / err_sp)
+        ratio.append(err_sp / err)
+    z_mj, z_sp, ratio = np.concatenate(z_mj), np.concatenate(z_sp), np.concatenate(ratio)
+    ks = stats.kstest(z_sp, "norm")
+    print(f"benchmark workload: walker-spread error: mean z^2 {np.mean(z_sp * z_sp):.3f}, max |z| {np.max(np.abs(z_sp)):.2f}, "
+          f"KS p {ks.pvalue:.3f}; MJBlocker combine: mean z^2 {np.mean(z_mj * z_mj):.3f}, max |z| {np.max(np.abs(z_mj)):.2f}; "
+          f"spread / MJBlocker error {np.mean(ratio):.3f} (min {ratio.min():.3f}, max {ratio.max():.3f})", flush=True)
+    assert 0.7 < np.mean(z_sp * z_sp) < 1.4
+    assert ks.pvalue > 0.01
+    assert np.all((0.6 < ratio) & (ratio < 1.4))
