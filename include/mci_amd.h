@@ -260,6 +260,19 @@ mci_status mci_get_observable_data(mci_ctx *ctx, int obs, double *data, int64_t 
 mci_status mci_get_acceptance(mci_ctx *ctx, int64_t *accepted, int64_t *rejected, double *rate);
 mci_status mci_get_walker_acceptance(mci_ctx *ctx, int64_t *accepted, int64_t *rejected, int64_t n);
 
+/* ---- checkpoint and resume of the walkers ----
+ * The reference keeps a chain's state in the MCI object between integrate calls (positions, step
+ * sizes, the private engine _rgen, MCIntegrator.hpp:33, and the move's distribution objects); a
+ * second integrate continues it (test/ut2/main.cpp:74-76).  These calls save and restore that state
+ * of every walker, so that a run can continue in another context or process with the same
+ * configuration: positions, step sizes, the random streams (mt19937_64 states, or the Philox key and
+ * counters) and the saved normals of the move's distribution.  The blob is plain bytes (a header
+ * with the shape, then the arrays); restoring needs the same ndim, walkers, step-size count and
+ * random mode (MCI_E_INVALID_ARGUMENT otherwise). */
+mci_status mci_get_state_size(mci_ctx *ctx, int64_t *nbytes);
+mci_status mci_get_state(mci_ctx *ctx, void *buf, int64_t nbytes);
+mci_status mci_set_state(mci_ctx *ctx, const void *buf, int64_t nbytes);
+
 /* ---- timing of the last integrate (HIP events on the context's stream, milliseconds) ----
  * phases: [0] calibration, [1] decorrelation, [2] sampling kernel, [3] estimation+combine, [4] total */
 mci_status mci_get_timings(mci_ctx *ctx, double *ms, int n);

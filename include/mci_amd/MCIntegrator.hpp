@@ -201,6 +201,16 @@ public:
 
     // --- setters (MCIntegrator.hpp:93-123) ---
     void setSeed(uint_fast64_t seed) { check(mci_set_seed(_ctx, seed)); }
+    // checkpoint and resume: every walker's position, step sizes, random stream and saved normals
+    std::vector<char> getWalkerState() const
+    {
+        int64_t n = 0;
+        check(mci_get_state_size(_ctx, &n));
+        std::vector<char> buf((size_t)n);
+        check(mci_get_state(_ctx, buf.data(), n));
+        return buf;
+    }
+    void setWalkerState(const std::vector<char> &state) { check(mci_set_state(_ctx, state.data(), (int64_t)state.size())); }
     void setWalkerSeeds(const std::vector<uint64_t> &seeds) { check(mci_set_walker_seeds(_ctx, seeds.data(), (int64_t)seeds.size())); }
     void setRandomMode(RandomMode m) { check(mci_set_rng(_ctx, static_cast<int>(m))); }
     void setPrecision(Precision p) { check(mci_set_precision(_ctx, static_cast<int>(p))); }

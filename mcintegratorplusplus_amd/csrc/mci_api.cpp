@@ -1439,6 +1439,98 @@ mci_status mci_get_walker_acceptance(mci_ctx *c, int64_t *acc, int64_t *rej, int
     return MCI_OK;
 }
 
+// ---- checkpoint of the walkers: positions, step sizes, random streams, saved normals ----
+namespace
+{
+struct StateHeader
+{
+    uint32_t magic, version;
+    int32_t ndim, ntypes;
+    int64_t W;
+    int32_t rng, reserved;
+    uint64_t seed;
+};
+constexpr uint32_t STATE_MAGIC = 0x5349434Du; // "MCIS"
+
+int64_t state_bytes(const mci_ctx *c)
+{
+    const int64_t W = c->W;
+    int64_t n = (int64_t)sizeof(StateHeader) + 8 * W * (c->ndim + c->ntypes) + 8 * 2 * W + 4 * W;
+    n += (c->rng == MCI_RNG_MT19937_64) ? 8 * (int64_t)MT_N * W + 4 * W : 8 * W;
+    return n;
+}
+
+// the sections of the state after the header, in order: device buffer, bytes
+std::vector<std::pair<void *, size_t>> state_sections(mci_ctx *c)
+{
+    const size_t W = (size_t)c->W;
+    std::vector<std::pair<void *, size_t>> v = {{c->x.p, 8 * W * c->ndim}, {c->steps.p, 8 * W * c->ntypes}};
+    if (c->rng == MCI_RNG_MT19937_64) {
+        v.push_back({c->mt.p, 8 * W * MT_N});
+        v.push_back({c->mtp.p, 4 * W});
+    }
+    else { v.push_back({c->ctr.p, 8 * W}); }
+    v.push_back({c->nsv.p, 8 * 2 * W});
+    v.push_back({c->nsf.p, 4 * W});
+    return v;
+}
+} // namespace
+
+mci_status mci_get_state_size(mci_ctx *c, int64_t *nbytes)
+{
+    CHECKCTX(c);
+    if (!nbytes) { return fail(MCI_E_INVALID_ARGUMENT, "null output"); }
+    *nbytes = state_bytes(c);
+    return MCI_OK;
+}
+
+mci_status mci_get_state(mci_ctx *c, void *buf, int64_t nbytes)
+{
+    CHECKCTX(c);
+    if (!buf || nbytes < state_bytes(c)) { return fail(MCI_E_INVALID_ARGUMENT, "state buffer too small (%lld bytes needed)", (long long)state_bytes(c)); }
+    TRY(set_current(c));
+    TRY(ensure_rng(c));
+    StateHeader h{STATE_MAGIC, 1u, c->ndim, c->ntypes, c->W, c->rng, 0, c->seed};
+    char *p = static_cast<char *>(buf);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    for (auto &sec : state_sections(c)) {
+        HIPCHK(hipMemcpyAsync(p, sec.first, sec.second, hipMemcpyDeviceToHost, c->stream));
+        p += sec.second;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MCI_OK;
+}
+
+mci_status mci_set_state(mci_ctx *c, const void *buf, int64_t nbytes)
+{
+    CHECKCTX(c);
+    StateHeader h;
+    if (!buf || nbytes < (int64_t)sizeof h) { return fail(MCI_E_INVALID_ARGUMENT, "not a walker state"); }
+    std::memcpy(&h, buf, sizeof h);
+    if (h.magic != STATE_MAGIC || h.version != 1u) { return fail(MCI_E_INVALID_ARGUMENT, "not a walker state of this library"); }
+    if (h.ndim != c->ndim || h.W != c->W || h.ntypes != c->ntypes || h.rng != c->rng) {
+        return fail(MCI_E_INVALID_ARGUMENT, "the state holds %d dimensions, %lld walkers, %d step sizes and random mode %d; the context %d, %lld, %d, %d",
+                    h.ndim, (long long)h.W, h.ntypes, h.rng, c->ndim, (long long)c->W, c->ntypes, c->rng);
+    }
+    if (nbytes < state_bytes(c)) { return fail(MCI_E_INVALID_ARGUMENT, "truncated walker state"); }
+    TRY(set_current(c));
+    c->seed = h.seed;  // the Philox key
+    if (c->rng == MCI_RNG_MT19937_64) {
+        HIPCHK(c->mt.ensure((size_t)MT_N * c->W));
+        HIPCHK(c->mtp.ensure(c->W));
+    }
+    else { HIPCHK(c->ctr.ensure(c->W)); }
+    const char *p = static_cast<const char *>(buf) + sizeof h;
+    for (auto &sec : state_sections(c)) {
+        HIPCHK(hipMemcpyAsync(sec.first, p, sec.second, hipMemcpyHostToDevice, c->stream));
+        p += sec.second;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->reseed = false;  // the streams continue from the state
+    return MCI_OK;
+}
+
 mci_status mci_get_timings(mci_ctx *c, double *ms, int n)
 {
     CHECKCTX(c);

@@ -30,6 +30,7 @@ _SIGS = {
     "mci_get_observable_data": [_P, _I, _DP, _I64, _I64P],
     "mci_get_acceptance": [_P, _I64P, _I64P, _DP], "mci_get_walker_acceptance": [_P, _I64P, _I64P, _I64],
     "mci_get_timings": [_P, _DP, _I], "mci_estimate": [_I, _I64, _I, _DP, _DP, _DP], "mci_device_count": [_IP],
+    "mci_get_state_size": [_P, _I64P], "mci_get_state": [_P, C.c_void_p, _I64], "mci_set_state": [_P, C.c_void_p, _I64],
 }
 _bound = None
 
@@ -196,6 +197,19 @@ class MCI:
         a, r, rate = _I64(), _I64(), _D()
         self._c("mci_get_acceptance", C.byref(a), C.byref(r), C.byref(rate))
         return a.value, r.value
+
+    # --- checkpoint and resume of the walkers ---
+    def getState(self):
+        """positions, step sizes, random streams and saved normals of every walker, as bytes"""
+        n = _I64()
+        self._c("mci_get_state_size", C.byref(n))
+        buf = (C.c_char * n.value)()
+        self._c("mci_get_state", buf, n.value)
+        return bytes(buf)
+
+    def setState(self, state):
+        buf = C.create_string_buffer(bytes(state), len(state))
+        self._c("mci_set_state", buf, len(state))
 
     def getTimings(self):
         """[calibration, decorrelation, sampling, estimation, total] of the last integrate, ms"""

@@ -91,6 +91,9 @@ ORACLE_ONLY = {
     "get_accu_data": [_I, _I64, _DP, _I64, _I64P],
 }
 MCI_ONLY = {
+    "get_state_size": [_I64P],
+    "get_state": [C.c_void_p, _I64],
+    "set_state": [C.c_void_p, _I64],
     "set_kernel_path": [_I],
     "get_timings": [_DP, _I],
     "get_observable_data": [_I, _DP, _I64, _I64P],
@@ -320,6 +323,17 @@ class Chain:
         buf = np.zeros(self.W * ns.value * nobs)
         self.call("get_observable_data", i, dptr(buf), buf.size, C.byref(ns))
         return buf.reshape(self.W, ns.value, nobs)
+
+    def getState(self):
+        n = _I64()
+        self.call("get_state_size", C.byref(n))
+        buf = (C.c_char * n.value)()
+        self.call("get_state", buf, n.value)
+        return bytes(buf)
+
+    def setState(self, state):
+        buf = C.create_string_buffer(bytes(state), len(state))
+        self.call("set_state", buf, len(state))
 
     def walkerAcceptance(self):
         a = np.zeros(self.W, dtype=np.int64)
