@@ -383,7 +383,9 @@ def run_learned(with_cpu=True):
     sec_p = (t_p["sampling"] + t_p["calibration"]) / 1000
     ess_p = var / e_p ** 2 / sec_p
     s.setTrainingPairs(0.0)
+    t_c0 = time.perf_counter()
     s.collect(16, 512)
+    t_col = time.perf_counter() - t_c0
     t0 = time.perf_counter()
     s.initFromSamples(60)
     t_em = 1000 * (time.perf_counter() - t0)
@@ -400,28 +402,50 @@ def run_learned(with_cpu=True):
         t_m = s.timings()
         sec_m = (t_m["sampling"] + t_m["calibration"]) / 1000
         ess_m = var / e_m ** 2 / sec_m
+        oneoff = t_col + t_em / 1000 + t_train / 1000
         out[f"learned_fp32_pmix{pm}"] = {"seconds": sec_m, "err": list(e_m), "ess_per_s": list(ess_m),
                                          "z": list((a_m - exact) / e_m), "learned_acceptance": s.results()["acc_learned"],
                                          "ess_per_s_gain_vs_plain_fp64": list(ess_m / ess_p),
-                                         "ess_per_step_gain": list((e_p / e_m) ** 2)}
+                                         "ess_per_step_gain": list((e_p / e_m) ** 2),
+                                         "oneoff_s": oneoff,
+                                         "ess_per_s_gain_vs_plain_fp64_with_oneoff": list(var / e_m ** 2 / (sec_m + oneoff) / ess_p)}
     # the independence form: a second network trained with one fixed input, its mixture there the
-    # proposal q(x') of every learned move (a fraction pmix of each cycle of 32 steps)
-    s.setNetwork(128, 8)
-    s.setFixedInput(True)
-    s.collect(16, 512)
-    s.initFromSamples(60)
-    for lr in (1e-3, 5e-4):
-        s.train(1000, 4096, lr)
-    for pm in (0.5, 0.75):
-        a_i, e_i = s.run(INDEP_FP32, NMC, 64, pm)
-        t_i = s.timings()
-        sec_i = (t_i["sampling"] + t_i["calibration"]) / 1000
-        ess_i = var / e_i ** 2 / sec_i
-        out[f"independence_fp32_pmix{pm}"] = {"seconds": sec_i, "err": list(e_i), "ess_per_s": list(ess_i), "z": list((a_i - exact) / e_i),
-                                              "learned_acceptance": s.results()["acc_learned"],
-                                              "ess_per_s_gain_vs_plain_fp64": list(ess_i / ess_p), "ess_per_step_gain": list((e_p / e_i) ** 2)}
+    # proposal q(x') of every learned move (a fraction pmix of each cycle of 32 steps).  Three ways to
+    # set the network: the EM fit written into its last layer with no training ("em_only"), Adam from
+    # the network's random start with no EM fit ("adam_only", 4 x 1,000 steps), and both ("em_adam",
+    # the default); the one-off cost of each (collection, EM start, training: host clock around the
+    # synchronous calls) is reported and counted in the *_with_oneoff gains.
+    def indep_row(key, use_em, stages, pms):
+        s.setNetwork(128, 8)
+        s.setFixedInput(True)
+        t0 = time.perf_counter()
+        s.collect(16, 512)
+        t_col = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        if use_em:
+            s.initFromSamples(60)
+        t_em = time.perf_counter() - t1
+        t2 = time.perf_counter()
+        ls = [s.train(1000, 4096, lr)[1] for lr in stages]
+        t_tr = time.perf_counter() - t2
+        oneoff = t_col + t_em + t_tr
+        for pm in pms:
+            a_i, e_i = s.run(INDEP_FP32, NMC, 64, pm)
+            t_i = s.timings()
+            sec_i = (t_i["sampling"] + t_i["calibration"]) / 1000
+            ess_i = var / e_i ** 2 / sec_i
+            ess_all = var / e_i ** 2 / (sec_i + oneoff)
+            out[f"independence_{key}_fp32_pmix{pm}"] = {
+                "seconds": sec_i, "err": list(e_i), "ess_per_s": list(ess_i), "z": list((a_i - exact) / e_i),
+                "learned_acceptance": s.results()["acc_learned"], "ess_per_s_gain_vs_plain_fp64": list(ess_i / ess_p),
+                "ess_per_step_gain": list((e_p / e_i) ** 2),
+                "oneoff_s": {"collection": t_col, "em_start": t_em, "training": t_tr},
+                "ess_per_s_gain_vs_plain_fp64_with_oneoff": list(ess_all / ess_p), "training_loss_after_each_stage": ls}
+    indep_row("em_only", True, (), (0.5,))
+    indep_row("adam_only", False, (1e-3, 1e-3, 5e-4, 5e-4), (0.5,))
+    indep_row("em_adam", True, (1e-3, 5e-4), (0.5, 0.75))
     # the best learned configuration by the smaller of its two ESS/s gains
-    runs = [k for k in out if k.startswith(("learned_fp32", "independence_fp32"))]
+    runs = [k for k in out if k.startswith(("learned_fp32", "independence_"))]
     best = max(runs, key=lambda k: min(out[k]["ess_per_s_gain_vs_plain_fp64"]))
     out["best"] = best
     cpu = cpu_learned_target(w, mu, cov, var, NMC) if with_cpu else None

@@ -199,6 +199,18 @@ mci_status mci_set_trial_move_multistep(mci_ctx *ctx, int nsteps, int srrd, int 
  * functions, from the same catalogue as mci_add_sampling_function (MCI_E_RUNTIME if the trial
  * move is not a MultiStepMove) */
 mci_status mci_add_multistep_sampling_function(mci_ctx *ctx, int kind, int ndim, const double *params, int nparams);
+/* A mixed trial move with a learned half (the reference's non-symmetric trial-move hook: the move's
+ * factor multiplies the acceptance ratio, TrialMoveInterface.hpp:63-69, MCIntegrator.cpp:343, as
+ * MultiStepMove.cpp:46 does): at every step, with probability pmix, an independence move x' ~ q from
+ * the diagonal Gaussian mixture q = sum_k pi_k N(mu_k, diag sigma_k^2) (K <= 8 components: weights
+ * [K] normalised here, means [K][ndim], sigmas [K][ndim]) with factor q(x) / q(x'); otherwise the move
+ * set by mci_set_trial_move / _type / _multistep before (a later call of those replaces the whole
+ * mixed move, as setTrialMove replaces a move; set the learned half again after it).  The target stays exact for
+ * any mixture; a mixture fitted to the target (e.g. the learned proposal's, mci_lp_get_mixture)
+ * moves walkers between far-apart modes.  It composes with every sampling function, observable and
+ * domain of the catalogue; the generic kernels run it. */
+mci_status mci_set_learned_move(mci_ctx *ctx, int K, const double *weights, const double *means, const double *sigmas, double pmix);
+mci_status mci_clear_learned_move(mci_ctx *ctx);
 mci_status mci_clear_multistep_sampling_functions(mci_ctx *ctx);
 /* MCI::storeObservablesOnFile / storeWalkerPositionsOnFile and their clear calls
  * (MCIntegrator.cpp:495-542): during the main sampling of each integrate, every freq steps (the
@@ -428,6 +440,20 @@ mci_status mci_lp_get_results(mci_lp *lp, double *results, int n);
 mci_status mci_lp_get_timings(mci_lp *lp, double *ms, int n);
 mci_status mci_lp_get_walker_means(mci_lp *lp, double *means, int64_t n); /* [2][nwalkers] */
 mci_status mci_lp_get_network(mci_lp *lp, mci_mlp **m);                   /* owned by the sampler */
+/* The trained independence proposal (the network's mixture at the fixed input, as MCI_LP_INDEP_FP32
+ * uses it: means and widths rounded to FP32) in the form mci_set_learned_move takes, so that a context
+ * of the catalogue path can mix it into its own trial move: *K components, weights [K] (normalised),
+ * means [K][d], sigmas [K][d]; the arrays hold kcap components. */
+mci_status mci_lp_get_mixture(mci_lp *lp, int kcap, int *K, double *weights, double *means, double *sigmas);
+/* The independence learned move alone (MCI_LP_INDEP_FP32's learned steps, TrialMoveInterface's
+ * non-symmetric hook, TrialMoveInterface.hpp:63-69): n walkers at the FP32 positions x [d][n] take nl
+ * consecutive moves of the current independence proposal, steps t0 .. t0 + nl - 1 of their Philox
+ * streams (walker w uses stream w); x is updated in place, accepted [n] receives each walker's
+ * accepted moves.  mix (optional, 3 * 8 * d + 16 floats) receives the proposal as the move uses it:
+ * means [8][d], 1 / sigma [8][d], sigma [8][d], log normalisers [8] and cumulative weights [8]
+ * (components past the network's K are inert).  For callers that compose the move with moves of
+ * their own, and for checking it. */
+mci_status mci_lp_indep_move(mci_lp *lp, int64_t n, float *x, int nl, uint64_t t0, int32_t *accepted, float *mix);
 /* the network as the learned move's kernel evaluates it (BF16 matrix cores, transformed outputs), at
  * n positions x [d][n]: out [K(1+2d)][n] holds per component k the d means, then the d log-scales,
  * then the K logits.  Needs d in {8, 16, 32} and hidden in {64, 128}. */

@@ -98,6 +98,35 @@ struct MultiStepMove
     void clearSamplingFunctions() { pdfs.clear(); }
 };
 
+// A mixed trial move (no reference counterpart; it plugs into the non-symmetric trial-move hook,
+// TrialMoveInterface.hpp:63-69): with probability pmix an independence draw from a diagonal Gaussian
+// mixture with factor q(x)/q(x'), e.g. the trained mixture of a learned proposal
+// (mci_lp_get_mixture), else the contained SRRD move; handed to MCI::setTrialMove
+struct LearnedMixtureMove
+{
+    int ndim;
+    double pmix;
+    std::vector<double> weights, means, sigmas; // [K], [K][ndim], [K][ndim]
+    SRRDType srrd = SRRDType::Uniform;          // contained move, as MCI::setTrialMove(SRRDType, ...)
+    int veclen = 0, ntypes = 1;
+    std::vector<int> typeEnds;
+    LearnedMixtureMove(int nd, double p) : ndim(nd), pmix(p) {}
+    void addComponent(double w, const std::vector<double> &mu, const std::vector<double> &sigma)
+    {
+        if ((int)mu.size() != ndim || (int)sigma.size() != ndim) { throw std::invalid_argument("[LearnedMixtureMove] component sizes must equal ndim"); }
+        weights.push_back(w);
+        means.insert(means.end(), mu.begin(), mu.end());
+        sigmas.insert(sigmas.end(), sigma.begin(), sigma.end());
+    }
+    void setTrialMove(SRRDType s, int vlen = 0, int nt = 1, const int te[] = nullptr)
+    {
+        srrd = s;
+        veclen = vlen;
+        ntypes = nt;
+        typeEnds.assign(te, te ? te + nt : te);
+    }
+};
+
 struct ObservableFunction
 {
     int kind;
@@ -264,6 +293,12 @@ public:
         if (m.ndim != _ndim) { throw std::invalid_argument("[MCI::setTrialMove] Passed trial move's number of inputs is not equal to MCI's number of walkers."); }
         check(mci_set_trial_move_multistep(_ctx, m.nsteps, static_cast<int>(m.srrd), m.veclen, m.ntypes, m.typeEnds.empty() ? nullptr : m.typeEnds.data()));
         for (const auto &p : m.pdfs) { check(mci_add_multistep_sampling_function(_ctx, p.kind, p.ndim, p.params.data(), (int)p.params.size())); }
+    }
+    void setTrialMove(const LearnedMixtureMove &m)
+    {
+        if (m.ndim != _ndim) { throw std::invalid_argument("[MCI::setTrialMove] Passed trial move's number of inputs is not equal to MCI's number of walkers."); }
+        check(mci_set_trial_move(_ctx, static_cast<int>(m.srrd), m.veclen, m.ntypes, m.typeEnds.empty() ? nullptr : const_cast<int *>(m.typeEnds.data())));
+        check(mci_set_learned_move(_ctx, (int)m.weights.size(), m.weights.data(), m.means.data(), m.sigmas.data(), m.pmix));
     }
 
     // --- observables (MCIntegrator.hpp:154-169) ---

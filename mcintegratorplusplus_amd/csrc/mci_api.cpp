@@ -96,6 +96,9 @@ struct mci_ctx
     int movetype = MCI_MOVE_ALL, srrd = MCI_SRRD_UNIFORM, veclen = 0, nvecs = 0, ntypes = 1;
     std::vector<int> typeEnds;
     int ms_on = 0, ms_nsteps = 0;     // MultiStepMove around the move above
+    int lm_K = 0;                     // the independence half of a mixed move: mixture components (0: off)
+    double lm_pmix = 0.;
+    DBuf<double> lm;                  // mu [K][ndim], 1 / sigma [K][ndim], sigma [K][ndim], lc [K], cw [K]
     std::string obsfile, wlkfile;     // file output of the main sampling (empty: off)
     mci_callback_fn cb = nullptr;     // MCI::setCallback: replayed from walker 0's record
     void *cb_user = nullptr;
@@ -226,6 +229,17 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     }
     ch.ms_on = c->ms_on;
     ch.ms_nsteps = c->ms_nsteps;
+    ch.lm_on = c->lm_K > 0 && c->lm_pmix > 0. ? 1 : 0;
+    ch.lm_K = c->lm_K;
+    ch.lm_pmix = c->lm_pmix;
+    if (c->lm_K > 0) {
+        const size_t KD = (size_t)c->lm_K * c->ndim;
+        ch.lm_mu = c->lm.p;
+        ch.lm_isig = c->lm.p + KD;
+        ch.lm_sig = c->lm.p + 2 * KD;
+        ch.lm_lc = c->lm.p + 3 * KD;
+        ch.lm_cw = c->lm.p + 3 * KD + c->lm_K;
+    }
     ch.ms_npdf = (int)c->ms_pdfs.size();
     ch.ms_pold = c->ms_pold.p;
     ch.ms_pnew = c->ms_pnew.p;
@@ -314,7 +328,7 @@ float elapsed(mci_ctx *c, int a, int b)
 // the specialised kernels apply to this chain (observables are checked separately)
 bool fast_chain(const mci_ctx *c)
 {
-    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && !c->cb && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
+    return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && c->lm_K == 0 && !c->cb && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
                          c->pdfs.empty() ? 0 : c->pdfs[0].kind) &&
            !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
 }
@@ -843,6 +857,8 @@ mci_status mci_set_trial_move(mci_ctx *c, int srrd, int veclen, int ntypes, cons
     c->ms_on = 0;
     c->ms_nsteps = 0;
     c->ms_pdfs.clear();
+    c->lm_K = 0;  // a new trial move replaces a mixed one too
+    c->lm_pmix = 0.;
     return fill_rows(c, c->steps.p, 0, ntypes, 0.05); // DEFAULT_MRT2STEP (Factories.hpp:145)
 }
 
@@ -1039,6 +1055,55 @@ mci_status mci_add_multistep_sampling_function(mci_ctx *c, int kind, int ndim, c
     TRY(build_pdf(kind, ndim, params, nparams, p));
     c->ms_pdfs.push_back(std::move(p));
     return ensure_ms_protos(c);
+}
+
+// A mixed trial move: with probability pmix an independence draw from a diagonal Gaussian mixture
+// (the learned proposal's), else the move set before. No reference counterpart; it enters through the
+// reference's trial-move hook, whose factor multiplies the acceptance (MCIntegrator.cpp:343).
+mci_status mci_set_learned_move(mci_ctx *c, int K, const double *weights, const double *means, const double *sigmas, double pmix)
+{
+    CHECKCTX(c);
+    if (K < 1 || K > 8 || !weights || !means || !sigmas) { return fail(MCI_E_INVALID_ARGUMENT, "need 1 to 8 mixture components"); }
+    if (!(pmix >= 0. && pmix <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "mixing probability must be in [0, 1]"); }
+    const int d = c->ndim;
+    double wsum = 0.;
+    for (int k = 0; k < K; ++k) {
+        if (!(weights[k] > 0.)) { return fail(MCI_E_INVALID_ARGUMENT, "mixture weights must be positive"); }
+        wsum += weights[k];
+    }
+    for (int e = 0; e < K * d; ++e) {
+        if (!(sigmas[e] > 0.) || !std::isfinite(means[e])) { return fail(MCI_E_INVALID_ARGUMENT, "mixture widths must be positive, means finite"); }
+    }
+    const size_t KD = (size_t)K * d;
+    std::vector<double> h(3 * KD + 2 * (size_t)K);
+    double cum = 0.;
+    for (int k = 0; k < K; ++k) {
+        const double pi = weights[k] / wsum;
+        double ls = 0.;
+        for (int j = 0; j < d; ++j) {
+            h[(size_t)k * d + j] = means[(size_t)k * d + j];
+            h[KD + (size_t)k * d + j] = 1. / sigmas[(size_t)k * d + j];
+            h[2 * KD + (size_t)k * d + j] = sigmas[(size_t)k * d + j];
+            ls = ls + log(1. / sigmas[(size_t)k * d + j]);
+        }
+        h[3 * KD + k] = log(pi) + ls - 0.5 * (double)d * log(2. * M_PI);
+        cum = cum + pi;
+        h[3 * KD + K + k] = (k == K - 1) ? 1. : cum;
+    }
+    TRY(set_current(c));
+    HIPCHK(c->lm.ensure(h.size()));
+    HIPCHK(hipMemcpy(c->lm.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->lm_K = K;
+    c->lm_pmix = pmix;
+    return MCI_OK;
+}
+
+mci_status mci_clear_learned_move(mci_ctx *c)
+{
+    CHECKCTX(c);
+    c->lm_K = 0;
+    c->lm_pmix = 0.;
+    return MCI_OK;
 }
 
 // MultiStepMove::clearSamplingFunctions (MultiStepMove.cpp:64-67)

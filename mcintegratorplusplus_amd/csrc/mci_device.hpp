@@ -72,6 +72,10 @@ struct Chain
     PdfDev ms_pdf[MAX_PDF];
     double *ms_pold, *ms_pnew;  // [ms_nproto][W]
     double *ms_x0;              // [ndim][W] the position the move starts from
+    int lm_on, lm_K;            // the independence half of a mixed move (0: off), mixture components
+    double lm_pmix;             // its probability per step
+    const double *lm_mu, *lm_isig, *lm_sig; // [K][ndim] means, 1 / sigma, sigma
+    const double *lm_lc, *lm_cw;             // [K] log pi_k + sum_j log(1 / sigma_kj) - ndim log(2 pi) / 2, cumulative weights
     double *rec_obs;            // [nrec][rec_nobsdim][W] observable values every rec_fobs steps (file output) or null
     double *rec_x;              // [nrec][ndim][W] positions every rec_fx steps or null
     int64_t rec_fobs, rec_fx;
@@ -502,12 +506,63 @@ __device__ inline void cb_record(const Chain &c, const Lane &L)
     c.cb_n[0] = n + 1;
 }
 
+// log q(y) of the independence mixture at column w of y [ndim][W]: the log-sum-exp over components of
+// lc_k - |(y - mu_k) / sigma_k|^2 / 2
+__device__ inline double lm_logq(const Chain &c, const double *y, int64_t w)
+{
+    const int64_t W = c.W;
+    double e[8], mx = -__builtin_huge_val();
+    for (int k = 0; k < c.lm_K; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < c.ndim; ++j) {
+            const double z = (y[(int64_t)j * W + w] - c.lm_mu[k * c.ndim + j]) * c.lm_isig[k * c.ndim + j];
+            s = s + z * z;
+        }
+        e[k] = c.lm_lc[k] - 0.5 * s;
+        mx = e[k] > mx ? e[k] : mx;
+    }
+    double s = 0.0;
+    for (int k = 0; k < c.lm_K; ++k) { s = s + exp(e[k] - mx); }
+    return mx + log(s);
+}
+
+// The independence move of a mixed trial move: a component by its cumulative weight (one uniform),
+// standard normals by Marsaglia's polar method (pairs, nothing kept between steps), x' = mu_k +
+// sigma_k z for every coordinate; the move's acceptance factor q(x) / q(x') (the reference's
+// trial-move hook, TrialMoveInterface.hpp:63-69, MCIntegrator.cpp:343), so that the chain keeps the
+// target exact whatever the mixture is.
+__device__ inline double lm_move(const Chain &c, Lane &L)
+{
+    const int64_t W = c.W, w = L.w;
+    const double uc = draw_unit(c, L);
+    int k = c.lm_K - 1;
+    for (int kk = 0; kk < c.lm_K - 1; ++kk) {
+        if (uc < c.lm_cw[kk]) { k = kk; break; }
+    }
+    for (int j = 0; j < c.ndim; j += 2) {
+        double x, y, r2;
+        do {
+            x = 2.0 * draw_unit(c, L) - 1.0;
+            y = 2.0 * draw_unit(c, L) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double mult = sqrt(-2 * log(r2) / r2);
+        c.xnew[(int64_t)j * W + w] = c.lm_mu[k * c.ndim + j] + c.lm_sig[k * c.ndim + j] * (y * mult);
+        if (j + 1 < c.ndim) { c.xnew[(int64_t)(j + 1) * W + w] = c.lm_mu[k * c.ndim + j + 1] + c.lm_sig[k * c.ndim + j + 1] * (x * mult); }
+    }
+    L.nchanged = c.ndim;
+    L.cfirst = 0;
+    return exp(lm_logq(c, c.xold, w) - lm_logq(c, c.xnew, w));
+}
+
 // MultiStepMove::trialMove (MultiStepMove.cpp:6-47): ms_nsteps Metropolis steps of the contained
 // move under the move's own sampling functions (none: every step accepted), from xold; xnew ends at
 // the last position, xold is restored, every coordinate counts as changed, and the factor
 // oldPDF / newPDF of the own sampling functions is the move's acceptance factor
 __device__ inline double trial_move(const Chain &c, Lane &L)
 {
+    // a mixed move: first the uniform that picks the independence move with probability lm_pmix
+    if (c.lm_on && draw_unit(c, L) < c.lm_pmix) { return lm_move(c, L); }
     if (!c.ms_on) { return srrd_move(c, L); }
     const int64_t W = c.W, w = L.w;
     for (int i = 0; i < c.ndim; ++i) { c.ms_x0[(int64_t)i * W + w] = c.xold[(int64_t)i * W + w]; }

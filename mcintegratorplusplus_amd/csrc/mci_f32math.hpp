@@ -56,6 +56,35 @@ __device__ __forceinline__ float blog(float x)
     return fmaf(fe, 0x1.62e43p-1f, fmaf(fe, -0x1.05c61p-29f, lm));
 }
 
+// sin and cos of 2 pi u for u in [0, 1): the quadrant from 4u (exact), the angle within it
+// t = (4u - quadrant) pi/2 in [0, pi/2), then sin t and cos t by their Taylor series to t^11 and
+// t^12 (Horner in t^2, fused; truncation below 6e-8 on the quadrant), signs and roles swapped by
+// quadrant.  A few ulp from the true values.
+__device__ __forceinline__ void bsincos2pi(float u, float &s, float &c)
+{
+    const float t4 = 4.0f * u;
+    const float q = floorf(t4);
+    const float th = (t4 - q) * 1.57079632679f;
+    const float z = th * th;
+    float ps = -2.50521084e-8f;
+    ps = fmaf(ps, z, 2.75573192e-6f);
+    ps = fmaf(ps, z, -1.98412698e-4f);
+    ps = fmaf(ps, z, 8.33333333e-3f);
+    ps = fmaf(ps, z, -1.66666667e-1f);
+    ps = fmaf(ps, z, 1.0f);
+    const float sn = th * ps;
+    float pc = 2.08767570e-9f;
+    pc = fmaf(pc, z, -2.75573192e-7f);
+    pc = fmaf(pc, z, 2.48015873e-5f);
+    pc = fmaf(pc, z, -1.38888889e-3f);
+    pc = fmaf(pc, z, 4.16666667e-2f);
+    pc = fmaf(pc, z, -0.5f);
+    pc = fmaf(pc, z, 1.0f);
+    const int qi = (int)q;
+    s = (qi == 0) ? sn : (qi == 1) ? pc : (qi == 2) ? -sn : -pc;
+    c = (qi == 0) ? pc : (qi == 1) ? -sn : (qi == 2) ? -pc : sn;
+}
+
 // symmetric grid (2m + 1 - 2^24) 2^-24 in (-1, 1) from the high 24 bits of a word; never 0
 __device__ __forceinline__ float sym24(uint32_t w) { return (float)(2 * (int32_t)(w >> 8) + 1 - 16777216) * 0x1p-24f; }
 

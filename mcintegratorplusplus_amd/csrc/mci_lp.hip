@@ -31,7 +31,9 @@ namespace mcil
 {
 namespace
 {
+using mcif::bexp;
 using mcif::blog;
+using mcif::bsincos2pi;
 using mcif::xor3;
 using mcif::sym24;
 
@@ -111,9 +113,10 @@ __device__ __forceinline__ float next_normal(Normals &N, uint32_t c0base, uint32
 
 // four standard normals from one Philox block by the Box-Muller transform (no rejection, so no
 // divergence): u1 in (0, 1], u2 in [0, 1) from the top 24 bits of a word; r = sqrt(-2 ln u1),
-// (r cos 2 pi u2, r sin 2 pi u2), with the hardware's approximate log, square root, sine and cosine
-// (errors of a few units in the last place: the proposal they define differs from the normal
-// density by that relative amount).  The learned moves draw their normals with it.
+// (r cos 2 pi u2, r sin 2 pi u2), with this library's own log and sine/cosine (mci_f32math.hpp)
+// and the correctly rounded square root, so that the CPU reproduces every normal bit for bit; each
+// is within a few ulp of the exact transform of its two words.  The learned moves draw their
+// normals with it.
 __device__ __forceinline__ void normal4(uint32_t c0, uint32_t id, uint32_t tl, uint32_t th, uint32_t k0, uint32_t k1, float z[4])
 {
     uint32_t o[4];
@@ -122,9 +125,9 @@ __device__ __forceinline__ void normal4(uint32_t c0, uint32_t id, uint32_t tl, u
     for (int p = 0; p < 2; ++p) {
         const float u1 = (float)((o[2 * p] >> 8) + 1u) * 0x1p-24f;
         const float u2 = (float)(o[2 * p + 1] >> 8) * 0x1p-24f;
-        const float r = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1));
+        const float r = sqrtf(-2.0f * blog(u1));
         float s, c;
-        __sincosf(6.28318530717958648f * u2, &s, &c);
+        bsincos2pi(u2, s, c);
         z[2 * p] = r * c;
         z[2 * p + 1] = r * s;
     }
@@ -219,6 +222,31 @@ __global__ __launch_bounds__(WG) void k_lp_init(Target<R> T, State<R> St, uint32
     }
     R q[KMAX], m, S;
     target_eval(T, St.x + w, St.W, q, m, S);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * St.W + w] = q[k]; }
+    St.m[w] = m;
+    St.S[w] = S;
+    St.sq[w] = sq;
+    for (int o2 = 0; o2 < 2; ++o2) {
+        St.sb[o2 * St.W + w] = 0;
+        St.cb[o2 * St.W + w] = 0;
+    }
+    St.jb[w] = 0;
+    St.bi[w] = 0;
+    St.nacc[w] = 0;
+}
+
+// the target's state at given positions (no draw): q_k, m, S and |x|^2 of every walker, empty
+// block sums and counters
+template <class R>
+__global__ __launch_bounds__(WG) void k_lp_state(Target<R> T, State<R> St)
+{
+    const int64_t w = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (w >= St.W) { return; }
+    R q[KMAX], m, S;
+    target_eval(T, St.x + w, St.W, q, m, S);
+    double sq = 0.;
+    for (int i = 0; i < T.d; ++i) { const double v = (double)St.x[(int64_t)i * St.W + w]; sq = fma(v, v, sq); }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) { St.q[(int64_t)k * St.W + w] = q[k]; }
     St.m[w] = m;
@@ -607,21 +635,21 @@ __device__ __forceinline__ void network(const NetDev &N, const uint16_t *W1s, co
                     for (int h = 0; h < 2; ++h) {
                         const float mu = __builtin_amdgcn_fmed3f(mcim::MU_SCALE * acc[rr][4 * g + 2 * h], -6.0e4f, 6.0e4f);
                         const float ls = __builtin_amdgcn_fmed3f(acc[rr][4 * g + 2 * h + 1], -mcim::LS_MAX, mcim::LS_MAX);
-                        const h2_t p = __builtin_amdgcn_cvt_pkrtz(mu, __expf(-ls));
+                        const h2_t p = __builtin_amdgcn_cvt_pkrtz(mu, bexp(-ls));
                         mx[pair_index(rb, g, h)] = p;
                         pr[cl] *= (float)p[1];
                         const int k = 2 * wc + cl;
                         if (out != nullptr && k < K && w0 + col < n) {
                             const int j = ((r % (2 * D)) >> 1) + h;
                             out[(int64_t)(2 * D * k + j) * n + w0 + col] = (float)p[0];
-                            out[(int64_t)(2 * D * k + D + j) * n + w0 + col] = -__logf((float)p[1]);
+                            out[(int64_t)(2 * D * k + D + j) * n + w0 + col] = -blog((float)p[1]);
                         }
                     }
                 }
             }
         }
-        lpi[0] = __logf(pr[0]);
-        lpi[1] = __logf(pr[1]);
+        lpi[0] = blog(pr[0]);
+        lpi[1] = blog(pr[1]);
     }
 }
 
@@ -722,11 +750,11 @@ __device__ __forceinline__ float mixture_logq(const float *lg, const float *cf, 
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < K) {
-            sa += __expf(lg[k * TW + col] - amax);
-            sc += __expf(comp[k] - cmax);
+            sa += bexp(lg[k * TW + col] - amax);
+            sc += bexp(comp[k] - cmax);
         }
     }
-    return (cmax + __logf(sc)) - (amax + __logf(sa)) - (float)D * HALF_LOG_2PI;
+    return (cmax + blog(sc)) - (amax + blog(sa)) - (float)D * HALF_LOG_2PI;
 }
 
 template <int D, int H>
@@ -839,7 +867,7 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
         float e[KMAX], sa = 0.f;
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-            e[k] = (k < K) ? __expf(lgx[k * TW + col] - amax) : 0.f;
+            e[k] = (k < K) ? bexp(lgx[k * TW + col] - amax) : 0.f;
             sa += e[k];
         }
         const float uc = (float)(o[0] >> 8) * 0x1p-24f * sa;
@@ -932,9 +960,9 @@ __global__ __launch_bounds__(TT) void k_lp_learned(NetDev N, Target<float> T, St
             }
             float Sn = 0.f;
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) { Sn += __expf(e[k] - mn); }
+            for (int k = 0; k < KMAX; ++k) { Sn += bexp(e[k] - mn); }
             const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
-            const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - (m + __logf(S)) + (logq_r - logq_f));
+            const bool acc = live && (blog(ua) < (mn + blog(Sn)) - (m + blog(S)) + (logq_r - logq_f));
             af[col] = acc ? 1 : 0;
             const float x0 = acc ? xps[col] : xs[col];
             if (acc) {
@@ -1072,8 +1100,8 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         float s = 0.f;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) { s += k < K ? __expf(v[k] - mx) : 0.f; }
-        return mx + __logf(s);
+        for (int k = 0; k < KMAX; ++k) { s += k < K ? bexp(v[k] - mx) : 0.f; }
+        return mx + blog(s);
     };
     fold_k(xs);
     __syncthreads();
@@ -1160,9 +1188,9 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             float Sn = 0.f;
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) { Sn += __expf(e[k] - mn); }
+            for (int k = 0; k < KMAX; ++k) { Sn += bexp(e[k] - mn); }
             const float ua = ((float)(o[1] >> 8) + 0.5f) * 0x1p-24f;
-            const bool acc = live && (__logf(ua) < (mn + __logf(Sn)) - (m + __logf(S)) + (lqx - lqn));
+            const bool acc = live && (blog(ua) < (mn + blog(Sn)) - (m + blog(S)) + (lqx - lqn));
             const float x0 = acc ? xps[col] : xs[col];
             if (acc) {
                 double s2 = 0.;
@@ -1306,6 +1334,12 @@ hipError_t launch_lp_learned(const NetDev &N, const Target<float> &T, const Stat
 {
     if (!learned_supported(N.d, N.H, N.K) || T.d != N.d || nl < 1) { return hipErrorInvalidValue; }
     return launch_learned_m<1>(N, T, S, nullptr, nullptr, t0, nl, k0, k1, nacc_ind, s);
+}
+
+hipError_t launch_lp_state(const Target<float> &T, const State<float> &S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lp_state<float>, dim3((unsigned)((S.W + WG - 1) / WG)), dim3(WG), 0, s, T, S);
+    return hipGetLastError();
 }
 
 namespace

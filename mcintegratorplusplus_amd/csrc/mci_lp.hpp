@@ -83,6 +83,8 @@ struct IndepMix
 // {8, 16, 32})
 hipError_t launch_lp_indep(const IndepMix &M, const Target<float> &T, const State<float> &S, uint64_t t0, int nl, uint32_t k0, uint32_t k1,
                            int32_t *nacc_ind, hipStream_t s);
+// the target's state (q, m, S, |x|^2) at the positions S.x, empty block sums (FP32 chain)
+hipError_t launch_lp_state(const Target<float> &T, const State<float> &S, hipStream_t s);
 // the same kernel's network at positions xin [d][S.W], the mixture to out [K (1 + 2 d)][S.W]
 // (per component d means and d log-scales, transformed; then K logits)
 hipError_t launch_lp_network(const NetDev &N, const Target<float> &T, const State<float> &S, const float *xin, float *out, hipStream_t s);

@@ -570,8 +570,8 @@ mci_status net_dev(mci_lp *lp, mcil::NetDev &N)
 // the independence proposal: the network's mixture at ctx0 (the mean of the last snapshot), as the
 // fused kernel evaluates it (means and 1 / sigma rounded to halves), into lp->imix: mu, 1 / sigma,
 // sigma [KMAX][d], lc [KMAX] (log weight + sum log(1 / sigma) - d log(2 pi) / 2, -inf past K),
-// cumulative weights [KMAX]
-mci_status indep_mixture(mci_lp *lp, const mcil::NetDev &N, mcil::IndepMix &M)
+// cumulative weights [KMAX]; with wts, also the normalised weights [K] in FP64
+mci_status indep_mixture(mci_lp *lp, const mcil::NetDev &N, mcil::IndepMix &M, std::vector<double> *wts = nullptr)
 {
     if (lp->ctx0.empty()) { return fail(MCI_E_RUNTIME, "no training samples: call mci_lp_collect and mci_lp_train first"); }
     const int d = lp->d, K = N.K, KM = mcil::KMAX;
@@ -609,6 +609,7 @@ mci_status indep_mixture(mci_lp *lp, const mcil::NetDev &N, mcil::IndepMix &M)
         }
         lc[k] = (float)(lw + ls - 0.5 * d * std::log(2.0 * M_PI));
         cum += std::exp(lw);
+        if (wts) { wts->push_back(std::exp(lw)); }
         cw[k] = k == K - 1 ? 1.f : (float)cum;
     }
     TRY(up(lp->imix, h));
@@ -1223,6 +1224,77 @@ mci_status mci_lp_get_network(mci_lp *lp, mci_mlp **m)
     TRY(lp_device(lp));
     TRY(net_ready(lp));
     *m = lp->net;
+    return MCI_OK;
+}
+
+mci_status mci_lp_get_mixture(mci_lp *lp, int kcap, int *K, double *weights, double *means, double *sigmas)
+{
+    CHECKL(lp);
+    if (!K || !weights || !means || !sigmas) { return fail(MCI_E_INVALID_ARGUMENT, "bad arguments"); }
+    TRY(lp_device(lp));
+    TRY(net_ready(lp));
+    mcil::NetDev N;
+    TRY(net_dev(lp, N));
+    if (kcap < N.K) { return fail(MCI_E_INVALID_ARGUMENT, "room for %d components needed", N.K); }
+    mcil::IndepMix M{};
+    std::vector<double> w;
+    TRY(indep_mixture(lp, N, M, &w));
+    const int d = lp->d;
+    const size_t KD = (size_t)mcil::KMAX * d;
+    std::vector<float> h(3 * KD);
+    HIPCHK(hipMemcpy(h.data(), lp->imix.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < N.K; ++k) {
+        weights[k] = w[(size_t)k];
+        for (int j = 0; j < d; ++j) {
+            means[(size_t)k * d + j] = (double)h[(size_t)k * d + j];
+            sigmas[(size_t)k * d + j] = (double)h[2 * KD + (size_t)k * d + j];
+        }
+    }
+    *K = N.K;
+    return MCI_OK;
+}
+
+mci_status mci_lp_indep_move(mci_lp *lp, int64_t n, float *x, int nl, uint64_t t0, int32_t *accepted, float *mix)
+{
+    CHECKL(lp);
+    if (n < 1 || !x || !accepted || nl < 1) { return fail(MCI_E_INVALID_ARGUMENT, "bad arguments"); }
+    TRY(lp_device(lp));
+    TRY(net_ready(lp));
+    mcil::NetDev N;
+    TRY(net_dev(lp, N));
+    mcil::IndepMix M{};
+    TRY(indep_mixture(lp, N, M));
+    const int d = lp->d;
+    const size_t W = (size_t)n;
+    DBuf<float> xs, q, m, S, sb, cb;
+    DBuf<double> sq, bm;
+    DBuf<int> jb, bi;
+    DBuf<int32_t> na, nai;
+    HIPCHK(xs.ensure(W * d));
+    HIPCHK(q.ensure(W * mcil::KMAX));
+    HIPCHK(m.ensure(W));
+    HIPCHK(S.ensure(W));
+    HIPCHK(sb.ensure(2 * W));
+    HIPCHK(cb.ensure(2 * W));
+    HIPCHK(sq.ensure(W));
+    HIPCHK(bm.ensure(2 * W));
+    HIPCHK(jb.ensure(W));
+    HIPCHK(bi.ensure(W));
+    HIPCHK(na.ensure(W));
+    HIPCHK(nai.ensure(W));
+    HIPCHK(hipMemcpy(xs.p, x, sizeof(float) * W * d, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(nai.p, 0, sizeof(int32_t) * W, lp->stream));
+    mcil::State<float> St{n, xs.p, q.p, m.p, S.p, sq.p, sb.p, cb.p, jb.p, bi.p, bm.p, 0, na.p};
+    const mcil::Target<float> T = target_of<float>(lp);
+    HIPCHK(mcil::launch_lp_state(T, St, lp->stream));
+    HIPCHK(mcil::launch_lp_indep(M, T, St, t0, nl, (uint32_t)lp->seed, (uint32_t)(lp->seed >> 32), nai.p, lp->stream));
+    HIPCHK(hipStreamSynchronize(lp->stream));
+    HIPCHK(hipMemcpy(x, xs.p, sizeof(float) * W * d, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(accepted, nai.p, sizeof(int32_t) * W, hipMemcpyDeviceToHost));
+    if (mix) {
+        const size_t KD = (size_t)mcil::KMAX * d;
+        HIPCHK(hipMemcpy(mix, lp->imix.p, sizeof(float) * (3 * KD + 2 * mcil::KMAX), hipMemcpyDeviceToHost));
+    }
     return MCI_OK;
 }
 

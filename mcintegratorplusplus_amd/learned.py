@@ -30,6 +30,8 @@ _SIGS = {
     "mci_lp_get_network": [_P, C.POINTER(_P)], "mci_lp_set_training_pairs": [_P, _D],
     "mci_lp_eval_network": [_P, _I64, _FP, _FP], "mci_lp_set_fixed_input": [_P, _I],
     "mci_lp_init_network_from_samples": [_P, _I],
+    "mci_lp_indep_move": [_P, _I64, _FP, _I, _U64, C.POINTER(C.c_int32), _FP],
+    "mci_lp_get_mixture": [_P, _I, C.POINTER(C.c_int), _DP, _DP, _DP],
     "mci_gmm_fit_diag": [_DP, _I64, _I, _I, _I, C.c_uint64, _DP, _DP, _DP, _DP],
 }
 _bound = None
@@ -176,6 +178,29 @@ class LearnedProposalSampler:
         xt = np.ascontiguousarray(x.T)
         _check(_lib().mci_lp_eval_network(self._h, n, xt.ctypes.data_as(_FP), out.ctypes.data_as(_FP)))
         return out.T.copy()
+
+    def indepMove(self, x, nl, t0=0):
+        """nl independence moves of walkers at positions x [n][d] (FP32), steps t0 .. t0 + nl - 1 of
+        walker streams 0 .. n - 1: (new positions [n][d], accepted moves [n], the proposal as the move
+        uses it: means, 1/sigma, sigma [8][d], log normalisers [8], cumulative weights [8])"""
+        x = _f(x)
+        n, d = x.shape
+        xt = np.ascontiguousarray(x.T)
+        acc = np.zeros(n, np.int32)
+        mix = np.zeros(3 * 8 * d + 16, np.float32)
+        _check(_lib().mci_lp_indep_move(self._h, n, xt.ctypes.data_as(_FP), nl, t0, acc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        mix.ctypes.data_as(_FP)))
+        return xt.T.copy(), acc, mix
+
+    def mixture(self):
+        """the trained independence proposal (weights [K], means [K][d], sigmas [K][d]), the form
+        MCI.setLearnedMove takes"""
+        kc = self.net_ncomp
+        w, m, s = np.zeros(kc), np.zeros((kc, self.d)), np.zeros((kc, self.d))
+        k = C.c_int()
+        _check(_lib().mci_lp_get_mixture(self._h, kc, C.byref(k), w.ctypes.data_as(_DP), m.ctypes.data_as(_DP),
+                                         s.ctypes.data_as(_DP)))
+        return w[:k.value].copy(), m[:k.value].copy(), s[:k.value].copy()
 
     def network(self):
         h = _P()

@@ -22,6 +22,7 @@ _SIGS = {
     "mci_get_mrt2step": [_P, _I, _DP], "mci_get_walker_mrt2steps": [_P, _DP, _I64], "mci_get_n_step_sizes": [_P, _IP],
     "mci_reset_domain": [_P], "mci_set_irange": [_P, _DP, _DP], "mci_domain_is_finite": [_P, _IP],
     "mci_set_trial_move": [_P, _I, _I, _I, _IP], "mci_set_trial_move_type": [_P, _I], "mci_get_change_rate": [_P, _DP],
+    "mci_set_learned_move": [_P, _I, _DP, _DP, _DP, _D], "mci_clear_learned_move": [_P],
     "mci_add_sampling_function": [_P, _I, _I, _DP, _I], "mci_pop_sampling_function": [_P],
     "mci_clear_sampling_functions": [_P], "mci_get_npdf": [_P, _IP],
     "mci_add_observable": [_P, _I, _I, _DP, _I, _I, _I, _I, _I], "mci_pop_observable": [_P],
@@ -143,6 +144,19 @@ class MCI:
         if typeEnds is not None:
             te = C.cast((C.c_int * len(typeEnds))(*typeEnds), _IP)
         self._c("mci_set_trial_move", move, veclen, ntypes, te)
+
+    def setLearnedMove(self, weights, means, sigmas, pmix):
+        """Mix an independence move from the diagonal Gaussian mixture (weights [K], means [K][ndim],
+        sigmas [K][ndim]) into the trial move with probability pmix, e.g. the trained mixture of
+        LearnedProposalSampler.mixture(); a later setTrialMove replaces the mixed move."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        m = np.ascontiguousarray(means, dtype=np.float64)
+        s = np.ascontiguousarray(sigmas, dtype=np.float64)
+        if m.shape != (w.size, self.ndim) or s.shape != m.shape:
+            raise ValueError("means and sigmas must be [K][ndim] with K = len(weights)")
+        self._c("mci_set_learned_move", w.size, _dp(w), _dp(m), _dp(s), float(pmix))
+
+    def clearLearnedMove(self): self._c("mci_clear_learned_move")
 
     def addSamplingFunction(self, kind, params=()):
         p = np.ascontiguousarray(params, dtype=np.float64)

@@ -260,6 +260,9 @@ struct orc_ctx {
     int movetype, srrd, veclen, nvecs, ntypes;
     int *typeEnds;
     int ms_on, ms_nsteps;   /* MultiStepMove(ndim, nsteps) around the move above */
+    int lm_K;               /* the independence half of a mixed move: mixture components (0: off) */
+    double lm_pmix;
+    double *lm;             /* mu [K][ndim], 1 / sigma [K][ndim], sigma [K][ndim], lc [K], cw [K] */
     int ms_npdf, ms_nproto;
     pdf_desc ms_pdf[MAX_PDF];
     /* functors */
@@ -1174,6 +1177,8 @@ static mci_status set_move(orc_ctx *c, int srrd, int veclen, int ntypes, const i
     c->ms_on = 0;
     c->ms_nsteps = 0;
     while (c->ms_npdf > 0) pdf_free(&c->ms_pdf[--c->ms_npdf]);
+    c->lm_K = 0; /* a new trial move replaces a mixed one too */
+    c->lm_pmix = 0.;
     return MCI_OK;
 }
 
@@ -1225,6 +1230,7 @@ mci_status orc_destroy(orc_ctx *c)
     }
     for (int i = 0; i < c->nobs; ++i) free(c->obs[i].params);
     free(c->w); free(c->lb); free(c->ub); free(c->typeEnds); free(c->wseeds); free(c->wavg); free(c->werr);
+    free(c->lm);
     free(c);
     return MCI_OK;
 }
@@ -1698,8 +1704,99 @@ static double pdf_value(const pdf_desc *p, const double *pv)
  * move under the move's own sampling functions (none: every step accepted), from xold; xnew ends
  * at the last position, xold is restored, all coordinates count as changed, and the factor
  * oldPDF / newPDF of the own sampling functions is returned */
+/* log q(y) of the independence mixture: log-sum-exp over components of lc_k - |(y - mu_k)/sigma_k|^2/2 */
+static double lm_logq(const orc_ctx *c, const double *y)
+{
+    const int d = c->ndim, K = c->lm_K;
+    const size_t KD = (size_t)K * d;
+    const double *mu = c->lm, *isg = c->lm + KD, *lc = c->lm + 3 * KD;
+    double e[8], mx = -HUGE_VAL;
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < d; ++j) {
+            const double z = (y[j] - mu[k * d + j]) * isg[k * d + j];
+            s = s + z * z;
+        }
+        e[k] = lc[k] - 0.5 * s;
+        if (e[k] > mx) mx = e[k];
+    }
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s = s + exp(e[k] - mx);
+    return mx + log(s);
+}
+
+/* the independence move of a mixed trial move: a component by cumulative weight, normals by the polar
+ * method in pairs (nothing kept between steps), x' = mu_k + sigma_k z, factor q(x) / q(x') (the
+ * reference's trial-move hook, TrialMoveInterface.hpp:63-69, MCIntegrator.cpp:343) */
+static double lm_move(const orc_ctx *c, walker *w)
+{
+    const int d = c->ndim, K = c->lm_K;
+    const size_t KD = (size_t)K * d;
+    const double *mu = c->lm, *sg = c->lm + 2 * KD, *cw = c->lm + 3 * KD + K;
+    const double uc = draw_unit(c, w);
+    int k = K - 1;
+    for (int kk = 0; kk < K - 1; ++kk)
+        if (uc < cw[kk]) { k = kk; break; }
+    for (int j = 0; j < d; j += 2) {
+        double x, y, r2;
+        do {
+            x = 2.0 * draw_unit(c, w) - 1.0;
+            y = 2.0 * draw_unit(c, w) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double mult = sqrt(-2 * log(r2) / r2);
+        w->xnew[j] = mu[k * d + j] + sg[k * d + j] * (y * mult);
+        if (j + 1 < d) w->xnew[j + 1] = mu[k * d + j + 1] + sg[k * d + j + 1] * (x * mult);
+    }
+    w->nchanged = d;
+    return exp(lm_logq(c, w->xold) - lm_logq(c, w->xnew));
+}
+
+mci_status orc_set_learned_move(orc_ctx *c, int K, const double *weights, const double *means, const double *sigmas, double pmix)
+{
+    if (K < 1 || K > 8 || !weights || !means || !sigmas) return fail(MCI_E_INVALID_ARGUMENT, "need 1 to 8 mixture components");
+    if (!(pmix >= 0. && pmix <= 1.)) return fail(MCI_E_INVALID_ARGUMENT, "mixing probability must be in [0, 1]");
+    const int d = c->ndim;
+    double wsum = 0.;
+    for (int k = 0; k < K; ++k) {
+        if (!(weights[k] > 0.)) return fail(MCI_E_INVALID_ARGUMENT, "mixture weights must be positive");
+        wsum += weights[k];
+    }
+    for (int e = 0; e < K * d; ++e)
+        if (!(sigmas[e] > 0.) || !isfinite(means[e])) return fail(MCI_E_INVALID_ARGUMENT, "mixture widths must be positive, means finite");
+    const size_t KD = (size_t)K * d;
+    free(c->lm);
+    c->lm = malloc(sizeof(double) * (3 * KD + 2 * (size_t)K));
+    double cum = 0.;
+    for (int k = 0; k < K; ++k) {
+        const double pi = weights[k] / wsum;
+        double ls = 0.;
+        for (int j = 0; j < d; ++j) {
+            c->lm[(size_t)k * d + j] = means[(size_t)k * d + j];
+            c->lm[KD + (size_t)k * d + j] = 1. / sigmas[(size_t)k * d + j];
+            c->lm[2 * KD + (size_t)k * d + j] = sigmas[(size_t)k * d + j];
+            ls = ls + log(1. / sigmas[(size_t)k * d + j]);
+        }
+        c->lm[3 * KD + k] = log(pi) + ls - 0.5 * (double)d * log(2. * M_PI);
+        cum = cum + pi;
+        c->lm[3 * KD + K + k] = (k == K - 1) ? 1. : cum;
+    }
+    c->lm_K = K;
+    c->lm_pmix = pmix;
+    return MCI_OK;
+}
+
+mci_status orc_clear_learned_move(orc_ctx *c)
+{
+    c->lm_K = 0;
+    c->lm_pmix = 0.;
+    return MCI_OK;
+}
+
 static double trial_move(const orc_ctx *c, walker *w)
 {
+    /* a mixed move: first the uniform that picks the independence move with probability lm_pmix */
+    if (c->lm_K > 0 && c->lm_pmix > 0. && draw_unit(c, w) < c->lm_pmix) return lm_move(c, w);
     if (!c->ms_on) return srrd_move(c, w);
     const int nd = c->ndim;
     memcpy(w->ms_x0, w->xold, sizeof(double) * (size_t)nd);
@@ -1899,7 +1996,7 @@ static float sym24f(uint32_t wd) { return (float)(2 * (int32_t)(wd >> 8) + 1 - 1
 static mci_status fp32_supported(const orc_ctx *c, const obs_desc *obs, int nobs)
 {
     if (c->rng != MCI_RNG_PHILOX || c->movetype != MCI_MOVE_VEC || c->veclen != 1 || c->ntypes != 1 || c->periodic ||
-        c->npdf != 1 || c->pdf[0].kind != MCI_PDF_GAUSS_DIAG || c->pdf[0].nparams < c->ndim || c->ms_on || c->srrd != MCI_SRRD_UNIFORM)
+        c->npdf != 1 || c->pdf[0].kind != MCI_PDF_GAUSS_DIAG || c->pdf[0].nparams < c->ndim || c->ms_on || c->lm_K > 0 || c->srrd != MCI_SRRD_UNIFORM)
         return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports Philox streams with single-coordinate uniform moves on one diagonal "
                                             "Gaussian (explicit weights) in an unbounded domain");
     int seen = 0, B = -1;

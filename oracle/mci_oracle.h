@@ -57,6 +57,8 @@ mci_status orc_domain_is_finite(orc_ctx *c, int *finite);
 mci_status orc_set_trial_move(orc_ctx *c, int srrd, int veclen, int ntypes, const int *type_ends);
 mci_status orc_set_trial_move_type(orc_ctx *c, int move_type);
 mci_status orc_set_trial_move_multistep(orc_ctx *c, int nsteps, int srrd, int veclen, int ntypes, const int *type_ends);
+mci_status orc_set_learned_move(orc_ctx *c, int K, const double *weights, const double *means, const double *sigmas, double pmix);
+mci_status orc_clear_learned_move(orc_ctx *c);
 mci_status orc_add_multistep_sampling_function(orc_ctx *c, int kind, int ndim, const double *params, int nparams);
 mci_status orc_clear_multistep_sampling_functions(orc_ctx *c);
 mci_status orc_get_change_rate(orc_ctx *c, double *rate);
@@ -118,6 +120,17 @@ mci_status orc_batch_run(int nint, const int *ndim, const int64_t *nmc, const in
                          int64_t ndecorr, int mode, const int64_t *T, double *avg, double *err, double *steps,
                          double *calib_rate, double *accrate, double *wavg, double *werr, double *bmeans,
                          double *bmeans_fp64);
+
+/* the learned independence move (mci_lp_oracle.c): the library's sine/cosine of 2 pi u, its
+ * Box-Muller normals of one Philox block, and nl moves of n walkers (positions x [d][n], FP32) with
+ * the proposal mix in the layout mci_lp_indep_move returns and the target given by FP64 means
+ * [KT][d], precision matrices [KT][d][d] and log normalisers c_k = log w_k - log det(2 pi Sigma_k)/2;
+ * decisions [n][nl] and MH margins [n][nl] may be NULL */
+void orc_bsincos2pi(float u, float *s, float *c);
+void orc_normal4(uint32_t c0, uint32_t id, uint32_t tl, uint32_t th, uint32_t k0, uint32_t k1, float z[4]);
+mci_status orc_lp_indep_move(int d, int K, const float *mix, int KT, const double *tmu, const double *tprec, const double *tc,
+                             uint64_t seed, int64_t n, float *x, int nl, uint64_t t0, int32_t *accepted, uint8_t *decisions,
+                             double *margins);
 
 #ifdef __cplusplus
 }

@@ -273,6 +273,32 @@ static void callback()
     std::printf("callback ok: %zu calls\n", rates.size());
 }
 
+static void learned_mixture_move()
+{
+    // a bimodal isotropic mixture (weights 0.3 / 0.7 at x_0 = -8 / +8) whose modes plain moves do not
+    // cross; every walker starts in the light mode; the mixed move (an independence draw from a
+    // mixture with the weights reversed, with probability 0.25) gives P(x_0 > 0) = 0.7
+    const int64_t W = 256;
+    MCI mci(2, W);
+    mci.setSeed(99);
+    mci.setRandomMode(RandomMode::Philox);
+    mci.addSamplingFunction(GaussianMixturePDF(2, {0.3, 0.7}, {0.8, 1.3}, {-8., 0., 8., 0.}));
+    LearnedMixtureMove lm(2, 0.25);
+    lm.addComponent(0.75, {-7.7, 0.2}, {1.1, 1.0});
+    lm.addComponent(0.25, {8.3, -0.1}, {1.4, 1.5});
+    lm.setTrialMove(SRRDType::Uniform, 1);
+    mci.setTrialMove(lm);
+    std::vector<double> x0(2 * W, 0.);
+    for (int64_t w = 0; w < W; ++w) { x0[2 * w] = -8.; }
+    check(mci_set_x(mci.handle(), x0.data(), (int64_t)x0.size()));
+    mci.addObservable(Indicator(2, 0), 64, 1, false, EstimatorType::Uncorrelated);
+    double average = 0., error = 0.;
+    mci.integrate(64 * 64, &average, &error);
+    CHECK(std::fabs(average - 0.7) < 5. * error + 0.02);
+    CHECK(throws<std::invalid_argument>([] { LearnedMixtureMove bad(2, 0.5); bad.addComponent(1., {0.}, {1., 1.}); }));
+    std::printf("learned mixture move ok: P(x0 > 0) = %.4f +- %.4f\n", average, error);
+}
+
 int main()
 {
     int ndev = 0;
@@ -290,6 +316,7 @@ int main()
     walkers();
     multistep();
     callback();
+    learned_mixture_move();
     std::printf("all %d checks passed\n", g_checks);
     return 0;
 }

@@ -70,6 +70,8 @@ SIGS = {
     "set_trial_move_multistep": [_I, _I, _I, _I, _IP],
     "add_multistep_sampling_function": [_I, _I, _DP, _I],
     "clear_multistep_sampling_functions": [],
+    "set_learned_move": [_I, _DP, _DP, _DP, _D],
+    "clear_learned_move": [],
     "get_change_rate": [_DP],
     "add_sampling_function": [_I, _I, _DP, _I],
     "pop_sampling_function": [],
@@ -248,6 +250,12 @@ class Chain:
             arr = (C.c_int * len(typeEnds))(*typeEnds)
             te = C.cast(arr, _IP)
         self.call("set_trial_move_multistep", nsteps, srrd, veclen, ntypes, te)
+
+    def setLearnedMove(self, weights, means, sigmas, pmix):
+        w, m, s = f64(weights), f64(means), f64(sigmas)
+        self.call("set_learned_move", len(w), dptr(w), dptr(m), dptr(s), pmix)
+
+    def clearLearnedMove(self): self.call("clear_learned_move")
 
     def setCallback(self, fn, capacity=1 << 20):
         """fn(x [ndim] array, acc, rej, rate) for every callback call of an integrate"""
@@ -431,6 +439,8 @@ def chain_setup(prefix, cfg, nwalkers=1):
         c.setTrialMoveMultistep(nsteps, srrd, veclen, ntypes, te)
         for kind, params in mpdfs:
             c.addMultistepSamplingFunction(kind, params)
+    if "learned" in cfg:
+        c.setLearnedMove(*cfg["learned"])
     if "irange" in cfg:
         c.setIRange(*cfg["irange"])
     for kind, params in cfg.get("pdfs", []):

@@ -181,3 +181,100 @@ def test_trained_network_reduces_the_error_bar():
     assert np.all(np.abs(_z(a_i, e_i, exact)) < 4.5)
     assert r["acc_learned"] > 0.005
     assert e_i[0] < 0.7 * e_p[0]
+
+
+@pytest.mark.parametrize("d,n", [(32, 300), (16, 200)])
+def test_independence_move_matches_oracle(d, n):
+    """The independence learned move (k_lp_indep) against its CPU restatement (oracle/mci_lp_oracle.c),
+    fed the same positions, proposal and Philox words: the same component, normals (the library's
+    own log, square root and sine/cosine), proposals and proposal densities bit for bit, hence the
+    same decisions wherever the MH margin exceeds the FP32 error of the device's target density
+    (2e-3); on every walker without such a near tie the positions and accepted counts are identical."""
+    import ctypes as C
+    from mcintegratorplusplus_amd.learned import LearnedProposalSampler, mixture_target
+    from mcitest import load
+    print("indep move vs oracle", d, n, flush=True)
+    w, mu, cov, _ = mixture_target(d=d, ncomp=8 if d == 32 else 4, seed=5)
+    s = LearnedProposalSampler(w, mu, cov, 4096)
+    s.setSeed(11)
+    s.setNetwork(128, 8)
+    s.setFixedInput(True)
+    s.setTrainingPairs(0.0)
+    s.collect(4, 256)
+    s.initFromSamples(30)
+    rng = np.random.default_rng(d)
+    k = rng.choice(len(w), n, p=w)
+    x0 = (mu[k] + np.einsum("nij,nj->ni", np.linalg.cholesky(cov)[k], rng.normal(size=(n, d)))).astype(np.float32)
+    nl, t0 = 32, 1000
+    xd, accd, mix = s.indepMove(x0, nl, t0)
+    L = load("orc")
+    _FP, _DP = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    L.orc_lp_indep_move.argtypes = [C.c_int, C.c_int, _FP, C.c_int, _DP, _DP, _DP, C.c_uint64, C.c_int64, _FP, C.c_int,
+                                    C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_uint8), _DP]
+    prec = np.ascontiguousarray(np.linalg.inv(cov))
+    c = np.log(w) - 0.5 * np.array([np.linalg.slogdet(2 * np.pi * cv)[1] for cv in cov])
+    xo = np.ascontiguousarray(x0.T)
+    acco = np.zeros(n, np.int32)
+    dec = np.zeros(n * nl, np.uint8)
+    mar = np.zeros(n * nl)
+    assert L.orc_lp_indep_move(d, 8, mix.ctypes.data_as(_FP), len(w), np.ascontiguousarray(mu).ctypes.data_as(_DP), prec.ctypes.data_as(_DP),
+                               np.ascontiguousarray(c).ctypes.data_as(_DP), 11, n, xo.ctypes.data_as(_FP), nl, t0,
+                               acco.ctypes.data_as(C.POINTER(C.c_int32)), dec.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               mar.ctypes.data_as(_DP)) == 0
+    near = np.any(np.abs(mar.reshape(n, nl)) < 2e-3, axis=1)
+    ok = ~near
+    print(f"accepted {acco.sum()} of {n * nl} moves; walkers with a near tie: {near.sum()}", flush=True)
+    assert acco.sum() > 0.02 * n * nl
+    assert near.mean() < 0.05
+    assert np.array_equal(accd[ok], acco[ok])
+    assert np.array_equal(xd.T[:, ok], xo[:, ok])
+
+
+def test_trained_mixture_as_the_trial_move_of_a_context():
+    """The learned proposal trained on an isotropic three-mode target in d = 8, its mixture handed to
+    an MCI context over the same target (the catalogue's isotropic mixture density) as the learned
+    half of a mixed trial move: from walkers that all start in one mode, P(x_0 > 0) and E|x|^2 land
+    within 4 standard errors of the walker means; without the learned half the walkers stay in that
+    mode and miss P(x_0 > 0)."""
+    from scipy.stats import norm
+
+    from mcintegratorplusplus_amd import constants as K
+    from mcintegratorplusplus_amd.learned import LearnedProposalSampler
+    from mcintegratorplusplus_amd.mci import MCI
+    d, W = 8, 16384
+    w = np.array([0.2, 0.3, 0.5])
+    s = np.array([0.6, 0.9, 1.2])
+    mu = np.random.default_rng(3).normal(0.0, 4.0, (3, d))
+    mu[:, 0] = [-6.0, 5.0, 7.0]
+    exact = np.array([np.sum(w * norm.cdf(mu[:, 0] / s)), np.sum(w * (np.sum(mu * mu, axis=1) + d * s * s))])
+    lp = LearnedProposalSampler(w, mu, np.array([sk * sk * np.eye(d) for sk in s]), W)
+    lp.setSeed(5)
+    lp.setNetwork(64, 4)
+    lp.setFixedInput(True)
+    lp.collect(8, 256)
+    lp.initFromSamples(60)
+    lp.train(400, 4096, 1e-3)
+    lw, lm, ls = lp.mixture()
+    print("learned mixture weights", np.round(lw, 3), flush=True)
+    assert lm.shape == (4, d) and ls.shape == (4, d) and abs(lw.sum() - 1) < 1e-6
+
+    def run(learned):
+        c = MCI(d, W)
+        c.setSeed(8)
+        c.setRandomMode(K.RNG_PHILOX)
+        c.addSamplingFunction(K.PDF_GMM_ISO, [3, *w, *s, *mu.ravel()])
+        c.setTrialMove(K.SRRD_UNIFORM, 1)
+        if learned:
+            c.setLearnedMove(lw, lm, ls, 0.25)
+        c.addObservable(K.OBS_INDICATOR, [0], 64, 1, False, estimator=K.ESTIM_UNCORRELATED)
+        c.addObservable(K.OBS_X2_SUM, [], 64, 1, False, estimator=K.ESTIM_UNCORRELATED)
+        c.setX(np.tile(mu[0], (W, 1)))
+        c.integrate(64 * 64)
+        wa, _ = c.walkerResults()
+        return (wa.mean(axis=0) - exact) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
+
+    z = run(True)
+    z0 = run(False)
+    print("z with the learned half", z, "without", z0, flush=True)
+    assert np.all(np.abs(z) < 4), z
+    assert abs(z0[0]) > 20, z0

@@ -537,3 +537,60 @@ def test_error_bars_calibrated_on_the_benchmark_workload():
     assert 0.7 < np.mean(z_sp * z_sp) < 1.4
     assert ks.pvalue > 0.01
     assert np.all((0.6 < ratio) & (ratio < 1.4))
+
+
+@pytest.mark.parametrize("rng", [RNG_MT, RNG_PHILOX])
+def test_learned_mixed_move_many_walkers(rng):
+    """The mixed trial move (a learned diagonal-mixture independence draw with probability 0.3, else
+    the Gaussian single-coordinate move) on a bimodal isotropic mixture, 96 walkers with automatic
+    calibration and decorrelation: identical decisions, calibrated steps and acceptance with the
+    oracle, positions and estimates within 1e-12 relative (errors 1e-10)."""
+    from mcitest import OBS_INDICATOR, PDF_GMM_ISO
+    print("learned mixed move", rng, flush=True)
+    d, W, nmc = 4, 96, 16 * 64
+    mu = np.array([[-8.0, 0, 0, 0], [8.0, 0, 0, 0]])
+    learned = (np.array([0.75, 0.25]), mu + 0.3, np.array([[1.2, 1.0, 0.9, 1.1], [1.5, 1.4, 1.6, 1.2]]), 0.3)
+    cfg = dict(ndim=d, seed=404, ndecorr=100, move=(1, 1, 1, None), learned=learned,
+               pdfs=[(PDF_GMM_ISO, [2, 0.3, 0.7, 0.8, 1.3, *mu.ravel()])],
+               obs=[(OBS_INDICATOR, [0], 16, 1, 0, ESTIM_UNCORR), (OBS_X2_SUM, [], 16, 1, 0, ESTIM_UNCORR)])
+    res = {}
+    for prefix in ("orc", "mci"):
+        c = chain_setup(prefix, cfg, nwalkers=W)
+        c.setRng(rng)
+        x = np.zeros((W, d))
+        x[:, 0] = 8.0
+        c.setX(x)
+        a, e = c.integrate(nmc)
+        res[prefix] = (a, e, *c.walkerResults(), c.getX(), c.getWalkerSteps(), c.walkerAcceptance())
+    o, g = res["orc"], res["mci"]
+    assert np.array_equal(o[6][0], g[6][0]) and np.array_equal(o[6][1], g[6][1]), "acceptance"
+    assert np.array_equal(o[5], g[5]), "steps"
+    for k in (0, 2, 4):
+        check_close(g[k], o[k], 1e-12)
+    for k in (1, 3):
+        check_close(g[k], o[k], 1e-10)
+    # the mode weights from walkers that all started in the heavy mode
+    assert abs(g[0][0] - 0.7) < 5 * g[1][0] + 0.02, g[0]
+
+
+def test_learned_mixed_move_physics_at_size():
+    """The mixed move on 65,536 walkers (device only): P(x_0 > 0) = 0.7 and E|x|^2 = 64 + 4 (0.3 0.64
+    + 0.7 1.69) within 4 standard errors of the walker means, from walkers all in the light mode."""
+    from mcitest import OBS_INDICATOR, PDF_GMM_ISO
+    d, W = 4, 65536
+    mu = np.array([[-8.0, 0, 0, 0], [8.0, 0, 0, 0]])
+    learned = (np.array([0.5, 0.5]), mu, np.ones((2, d)), 0.2)
+    cfg = dict(ndim=d, seed=405, ndecorr=200, move=(0, 1, 1, None), learned=learned,
+               pdfs=[(PDF_GMM_ISO, [2, 0.3, 0.7, 0.8, 1.3, *mu.ravel()])],
+               obs=[(OBS_INDICATOR, [0], 64, 1, 0, ESTIM_UNCORR), (OBS_X2_SUM, [], 64, 1, 0, ESTIM_UNCORR)])
+    c = chain_setup("mci", cfg, nwalkers=W)
+    c.setRng(RNG_PHILOX)
+    x = np.zeros((W, d))
+    x[:, 0] = -8.0
+    c.setX(x)
+    c.integrate(64 * 64)
+    wa, _ = c.walkerResults()
+    exact = np.array([0.7, 64 + 4 * (0.3 * 0.64 + 0.7 * 1.69)])
+    z = (wa.mean(axis=0) - exact) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
+    print("z", z, flush=True)
+    assert np.all(np.abs(z) < 4), z
