@@ -4,6 +4,7 @@
 // stream; the host only steers the loops whose exit depends on a statistic.
 #include "mci_kernels.hpp"
 #include "mci_fast.hpp"
+#include "mci_fast_plan.hpp"
 #include "mci_host.hpp"
 
 #include <hip/hip_runtime.h>
@@ -219,6 +220,7 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     ch.movetype = c->movetype;
     ch.veclen = c->veclen;
     ch.nvecs = c->nvecs;
+    ch.ibits = shared_index_bits(c->rng == MCI_RNG_PHILOX, c->movetype == MCI_MOVE_VEC, c->ms_on != 0, c->nvecs);
     ch.srrd = c->srrd;
     ch.nsv = c->nsv.p;
     ch.nsf = c->nsf.p;

@@ -63,6 +63,7 @@ struct Chain
     int ntypes;
     int typeEnds[MAX_TYPES];
     int movetype, veclen, nvecs;
+    int ibits;                  // index bits in the native shared word (shared_index_bits), -1: none
     int srrd;                   // the move's distribution (MCI_SRRD_*)
     double *nsv;                // [2][W] saved second normals of the move's distribution
     int32_t *nsf;               // [W] bit s: nsv row s available
@@ -156,15 +157,25 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// word k of the current step: counter = (k/4, walker, step_lo, step_hi), key = seed
-__device__ __forceinline__ uint32_t ph_word(const Chain &c, const Lane &L, uint32_t k)
+// The native stream (key = seed).  Steps go in pairs: block j of step pair t / 2 is Philox(counter =
+// (j, walker, t / 2)), and main word k of step t is word 2 (t mod 2) + (k mod 2) of block k / 2, so
+// one block holds words 0 and 1 of two consecutive steps.  Main word 0 is the step's shared word
+// (draw_accept); the move's draws start at word 1.  Side streams keyed by the step itself: Lemire
+// rejections (block 0x20000000 + kx / 4, word kx mod 4) and the acceptance uniform's low word
+// (block 0x30000000, word 0).
+__device__ __forceinline__ uint32_t ph_block_word(const Chain &c, const Lane &L, uint32_t blk, uint64_t ct, uint32_t word)
 {
     uint32_t o[4];
-    philox4x32_10(k >> 2, (uint32_t)L.w, (uint32_t)L.t, (uint32_t)(L.t >> 32), c.key0, c.key1, o);
-    return o[k & 3u];
+    philox4x32_10(blk, (uint32_t)L.w, (uint32_t)ct, (uint32_t)(ct >> 32), c.key0, c.key1, o);
+    return o[word];
 }
 
-__device__ __forceinline__ void step_begin(Lane &L) { L.k = 0; L.kx = 0; }
+__device__ __forceinline__ uint32_t ph_word(const Chain &c, const Lane &L, uint32_t k)
+{
+    return ph_block_word(c, L, k >> 1, L.t >> 1, 2u * (uint32_t)(L.t & 1u) + (k & 1u));
+}
+
+__device__ __forceinline__ void step_begin(Lane &L) { L.k = 1; L.kx = 0; }
 __device__ __forceinline__ void step_end(const Chain &c, Lane &L) { if (c.rng == MCI_RNG_PHILOX) { ++L.t; } }
 
 // generate_canonical<double,53>(mt19937_64): double(word)/2^64, clamped below 1
@@ -198,6 +209,18 @@ __device__ __forceinline__ double draw_unit(const Chain &c, Lane &L)
     return (double)(((a << 32) | b) >> 11) * 0x1p-53;
 }
 
+// the acceptance uniform of a Metropolis step (MCIntegrator.cpp:343).  Native: with a single-vector
+// move over 2^m vectors (c.ibits = m <= 11) the shared word's top m bits are the move's index and
+// u = ((shared word : side-stream word) << m) >> 11, times 2^-53 (the m zero bits shifted in fall
+// below the 53 kept); otherwise m = 0.
+__device__ __forceinline__ double draw_accept(const Chain &c, Lane &L)
+{
+    if (c.rng == MCI_RNG_MT19937_64) { return draw_unit(c, L); }
+    const uint32_t m = c.ibits > 0 ? (uint32_t)c.ibits : 0u;
+    const uint64_t hi = ph_word(c, L, 0), lo = ph_block_word(c, L, 0x30000000u, L.t, 0);
+    return (double)((((hi << 32) | lo) << m) >> 11) * 0x1p-53;
+}
+
 // uniform integer on [0, n) (SRRDVecMove.hpp:78): Lemire's method as in libstdc++
 // (64-bit engine, 128-bit product) or on 32-bit Philox words
 __device__ __forceinline__ int draw_index(const Chain &c, Lane &L, int n)
@@ -223,7 +246,8 @@ __device__ __forceinline__ int draw_index(const Chain &c, Lane &L, int n)
     if (low < range) {
         const uint32_t thr = (0u - range) % range;
         while (low < thr) {
-            prod = (uint64_t)ph_word(c, L, 0x80000000u + (L.kx++)) * range;
+            const uint32_t kx = L.kx++;
+            prod = (uint64_t)ph_block_word(c, L, 0x20000000u + (kx >> 2), L.t, kx & 3u) * range;
             low = (uint32_t)prod;
         }
     }
@@ -457,7 +481,7 @@ __device__ __forceinline__ void wrap(const Chain &c, int i, double &x)
 // the Metropolis step
 // --------------------------------------------------------------------------
 // SRRDAllMove / SRRDVecMove trial moves (uniform SRRD)
-__device__ inline double srrd_move(const Chain &c, Lane &L)
+__device__ inline double srrd_move(const Chain &c, Lane &L, bool outer)
 {
     const int64_t W = c.W, w = L.w;
     if (c.movetype == MCI_MOVE_ALL) {
@@ -474,7 +498,9 @@ __device__ inline double srrd_move(const Chain &c, Lane &L)
         L.cfirst = 0;
         return 1.;
     }
-    const int vidx = draw_index(c, L, c.nvecs);
+    // the chain's own move takes a power-of-two index from the shared word's top bits
+    const int ib = outer ? c.ibits : -1;
+    const int vidx = ib > 0 ? (int)(ph_word(c, L, 0) >> (32 - ib)) : ib == 0 ? 0 : draw_index(c, L, c.nvecs);
     const int xidx = vidx * c.veclen;
     int t = 0;
     while (t < c.ntypes) {
@@ -563,7 +589,7 @@ __device__ inline double trial_move(const Chain &c, Lane &L)
 {
     // a mixed move: first the uniform that picks the independence move with probability lm_pmix
     if (c.lm_on && draw_unit(c, L) < c.lm_pmix) { return lm_move(c, L); }
-    if (!c.ms_on) { return srrd_move(c, L); }
+    if (!c.ms_on) { return srrd_move(c, L, true); }
     const int64_t W = c.W, w = L.w;
     for (int i = 0; i < c.ndim; ++i) { c.ms_x0[(int64_t)i * W + w] = c.xold[(int64_t)i * W + w]; }
     for (int p = 0; p < c.ms_npdf; ++p) { pdf_proto(c, c.ms_pdf[p], c.xold, c.ms_pold, w); }
@@ -571,7 +597,7 @@ __device__ inline double trial_move(const Chain &c, Lane &L)
     double oldPDF = 1.;
     for (int p = 0; p < c.ms_npdf; ++p) { oldPDF *= pdf_value(c, c.ms_pdf[p], c.ms_pold, w); }
     for (int s = 0; s < c.ms_nsteps; ++s) {
-        const double moveAcc = srrd_move(c, L);
+        const double moveAcc = srrd_move(c, L, false);
         double pdfAcc = 1.;
         for (int p = 0; p < c.ms_npdf; ++p) { pdfAcc *= pdf_compute_acceptance(c, c.ms_pdf[p], L, c.ms_pold, c.ms_pnew); }
         const bool acc = (draw_unit(c, L) <= pdfAcc * moveAcc);
@@ -614,7 +640,7 @@ __device__ inline void step_mrt2(const Chain &c, Lane &L)
     }
     double pdfAcc = 1.;
     for (int p = 0; p < c.npdf; ++p) { pdfAcc = pdfAcc * pdf_compute_acceptance(c, c.pdf[p], L, c.pold, c.pnew); }
-    const double u = draw_unit(c, L);
+    const double u = draw_accept(c, L);
     L.accepted = (u <= pdfAcc * moveAcc);
     if (L.accepted) { ++L.acc; } else { ++L.rej; }
     cb_record(c, L);

@@ -56,7 +56,11 @@ struct FastArgs
     double exp_div;        // EXP_X2 divisor
     int nvecs;
     uint32_t lemire_thr;   // 2^32 mod nvecs
-    int vshift;            // nvecs = 2^(32 - vshift) >= 2: the index is w0 >> vshift; else 0
+    int ibits;             // index bits m in the shared word (nvecs = 2^m), -1: Lemire index
+    int vshift;            // 32 - m: the index is the shared word's bits [vshift, 32) (none when m = 0)
+    uint32_t ubits;        // max(m, 0): the acceptance uniform's bits start below the index
+    uint32_t umask;        // FP64: the shared word's n = 32 - ubits acceptance bits
+    float uexp;            // FP64: n, the screening exponent's offset
     const CalibCtrl *pend; // shared calibration: pending step factor, loop finished (FP64 kernel only)
     const double *dimsizes;
 };
@@ -79,48 +83,12 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
-// philox() split for the step loop: rounds 0..8 and the p1 half of round 9 give words 0 and 1;
-// s0 and s3 carry what the p0 half needs.  The p0 half (words 2 and 3) is philox_tail9().
-__device__ __forceinline__ void philox_head9(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
-                                             uint32_t &o0, uint32_t &o1, uint32_t &s0, uint32_t &s3)
+// Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution), for
+// a range whose index the shared word does not hold: low = (w0 * range) mod 2^32 < thr = 2^32 mod
+// range rejects and draws from the extra-word stream of step t.  thr is computed on the host; a wave
+// vote skips the rare path.
+__device__ __forceinline__ int draw_lemire(uint32_t w0, uint32_t range, uint32_t thr, uint32_t wid, uint64_t t, uint32_t k0, uint32_t k1)
 {
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
-        c1 = (uint32_t)p1;
-        c3 = (uint32_t)p0;
-        c0 = n0;
-        c2 = n2;
-    }
-    k0 += 0x9E3779B9u;
-    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    o0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-    o1 = (uint32_t)p1;
-    s0 = c0;
-    s3 = c3;
-}
-
-__device__ __forceinline__ void philox_tail9(uint32_t s0, uint32_t s3, uint32_t k1, uint32_t &o2, uint32_t &o3)
-{
-    const uint64_t p0 = (uint64_t)0xD2511F53u * s0;
-    o2 = xor3((uint32_t)(p0 >> 32), s3, k1 + 9u * 0xBB67AE85u);
-    o3 = (uint32_t)p0;
-}
-
-// Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution):
-// low = (w0 * range) mod 2^32 < thr = 2^32 mod range rejects and draws from the extra-word stream.
-// thr is computed on the host (0 for powers of two); a wave vote skips the rare path.  For a
-// power-of-two range 2^m the product's high word is w0 >> (32 - m) and nothing is rejected, so the
-// 64-bit multiply (a quarter-rate instruction) is skipped.
-template <bool P2>
-__device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t thr, int vshift, uint32_t wid, uint64_t t, uint32_t k0,
-                                         uint32_t k1)
-{
-    if (P2) { return (int)(w0 >> vshift); }
     uint64_t prod = (uint64_t)w0 * range;
     if (__builtin_expect(__any((uint32_t)prod < thr), 0)) {
         uint32_t low = (uint32_t)prod;
@@ -137,25 +105,29 @@ __device__ __forceinline__ int draw_vidx(uint32_t w0, uint32_t range, uint32_t t
 }
 
 // observable set of one launch (template flags: no per-step branches on them)
-// F_P2: the index range is a power of two; F_FULL: Full accumulators (every step's values stored)
+// F_P2: the index is the shared word's top bits (a power-of-two range); F_FULL: Full accumulators (every
+// step's values stored)
 enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8, F_FULL = 16 };
 
 // FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
 // 1,024 waves), so a step costs about 4 cycles per instruction of any kind and 16 per
-// v_mad_u64_u32 (quarter rate; the Philox rounds hold up to 19 of them).  The step is written
-// to issue few instructions:
+// v_mad_u64_u32 (quarter rate; a Philox call holds 20 of them).  The step is written to issue few
+// instructions:
+//  - steps go in pairs, as the native stream does: one Philox block holds the shared word
+//    (index bits, then the acceptance uniform's top bits) and the proposal word of two
+//    consecutive steps, so a pair body reads both steps' coordinates from LDS at once, computes
+//    the next pair's block while the reads are in flight, then runs the two steps (the second
+//    reads what the first wrote when both picked the same coordinate);
 //  - no per-step branch on the observable set (template), the steps of one block as an inner
-//    loop with a 32-bit accept count (one scalar loop test per step);
+//    loop with a 32-bit accept count;
 //  - with X2 the coordinate and its block sum are one 16-byte LDS record (one ds_read_b128 and
 //    one ds_write_b128 per step);
 //  - the proposal (2 w1 + 1 - 2^32) 2^-32 as one exact fused multiply-add;
-//  - the decision screened in FP32 in units of 2^-32 (F = 2^(32 - ex log2 e) against the high
-//    word of u), the exact FP64 test (53-bit u, FP64 exp) only for lanes inside the band;
+//  - the decision screened in FP32 (F = 2^(n - ex log2 e) against the n acceptance bits of the
+//    shared word), the exact FP64 test (53-bit u from the side-stream word, FP64 exp) only for
+//    lanes inside the band, under a wave vote;
 //  - with X2 and X2_SUM both present, the block sum of |x|^2 is the sum of the coordinates' block
 //    sums, taken at the block's end (no per-step update).
-// Scalar instructions cost an issue slot as well: keeping the counter in scalar registers when a
-// wave's counters agree (three fewer vector multiplies, about twenty more scalar instructions per
-// step) measured 11 % slower.
 //
 // With Full accumulators (F_FULL, FullAccumulator.cpp:14-18: the reference's default for
 // addObservable(obs)) every step stores the whole observable vector: row s of X2 is x_i^2 for all
@@ -164,6 +136,148 @@ enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8, F_FULL = 16 };
 // one 512-byte store per component; the coordinates are read from LDS, the exp is recomputed only
 // when the walker moved.  These launches are bound by HBM writes (136 bytes per walker-step at
 // d = 16 with X2 and X2_SUM).
+
+// the running sums of one walker in k_fast64
+struct Run64
+{
+    double r;     // |x|^2
+    double Sr;    // block sum of |x|^2 (dwell-weighted)
+    double Se, e; // block sum of exp(-|x|^2/c), its current value
+    double mult;  // steps of the block a value set now counts for (B - j)
+    uint32_t na;  // accepted steps of the chunk
+};
+
+// One step on coordinate v (LDS slot `slot`) from its value xo and block sum So, which the caller
+// read and which hold the values after the step on return.  w0 is the step's shared word, w1 its
+// proposal word, t its counter (the exact test's side-stream word is keyed by it), row its index
+// in the run (Full traces).
+template <int FL>
+__device__ __forceinline__ void step64(const FastArgs &A, double2 *rec, double *xs, const double *as, Run64 &R, int v, int slot,
+                                       double &xo, double &So, float cv, uint32_t w0, uint32_t w1, uint64_t t, int64_t row,
+                                       uint32_t wid, double step, int lane, int64_t w)
+{
+    constexpr bool FULL = (FL & F_FULL) != 0;
+    constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool ACC = (FL & 7) != 0 && !FULL;
+    constexpr bool REC = HX2 && !FULL;
+    // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
+    const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
+    const double xn = xo + step * rr;
+    const double xn2 = xn * xn, xo2 = xo * xo;
+    const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
+    // The FP64 test is u <= exp(-ex) with ex = a_v xn^2 - a_v xo^2 (pdf_acceptance's form).  The
+    // shared word's low n = 32 - m bits are u's top bits: u lies in [U 2^-n, (U + 1) 2^-n).  The
+    // screen takes F = 2^(n - a_v log2 e (xn^2 - xo^2)) in FP32: relative error below 2e-5 wherever
+    // F >= 1 (the FP64 rounding of ex is far smaller), and U rounded to FP32.  U < 0.9999 F - 2
+    // makes the FP64 test certain to accept, U > 1.0001 F certain to reject; between the two the
+    // FP64 test decides.
+    const float U = (float)(w0 & A.umask);
+    const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, A.uexp));
+    const bool acc = U < fmaf(F, 0.9999f, -2.0f);
+    const bool band = !acc && U <= F * 1.0001f;
+    if (acc) { ++R.na; }
+    bool moved = acc;
+    double xsel = acc ? xn : xo, Snew = So;
+    if (ACC) {
+        // the new value counts for the block's remaining steps (one fused multiply-add: the block
+        // means agree with the per-step sums of the generic kernel to about 1e-15 relative)
+        const double dx2 = acc ? dx2n : 0.0;
+        Snew = fma(dx2, R.mult, So);
+        if (REC) { rec[slot] = make_double2(xsel, Snew); }
+        else { xs[slot] = xsel; }
+        R.r = R.r + dx2;
+        if (!(HX2 && HSUM)) { R.Sr = fma(dx2, R.mult, R.Sr); }
+        if (HEXP && acc) { const double en = exp(-(R.r / A.exp_div)); R.Se = R.Se + (en - R.e) * R.mult; R.e = en; }
+    }
+    else { xs[slot] = xsel; }
+    // Lanes inside the band went through the update above as rejections (which leaves every number
+    // as it was: +0.0 added to non-negative sums).  When the wave holds any, the exact test runs and
+    // the lanes it accepts are updated as accepted steps, from the values read at the start of the
+    // step.  (two ballots of plain compares: each is the compare's own lane mask)
+    if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
+        uint32_t l0, l1, l2, l3;
+        philox(0x30000000u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, l0, l1, l2, l3);
+        const uint64_t uw = (((uint64_t)w0 << 32) | l0) << A.ubits;
+        const double u = (double)(uw >> 11) * 0x1p-53;
+        const double av = as[v];
+        const double ex = xn2 * av - xo2 * av;
+        if (band && u <= exp(-ex)) {
+            R.na += 1u;
+            moved = true;
+            xsel = xn;
+            if (ACC) {
+                Snew = fma(dx2n, R.mult, So);
+                if (REC) { rec[slot] = make_double2(xn, Snew); }
+                else { xs[slot] = xn; }
+                R.r = R.r + dx2n;
+                if (!(HX2 && HSUM)) { R.Sr = fma(dx2n, R.mult, R.Sr); }
+                if (HEXP) { const double en = exp(-(R.r / A.exp_div)); R.Se = R.Se + (en - R.e) * R.mult; R.e = en; }
+            }
+            else { xs[slot] = xn; }
+        }
+    }
+    if (ACC) { R.mult = R.mult - 1.0; }
+    if (FULL) {
+        // this step's row of every trace (the observable values after the step)
+        double q2s = 0.0;
+        for (int i = 0; i < A.ndim; ++i) {
+            const double xi = xs[i * WAVE + lane];
+            const double q2 = xi * xi;
+            if (HX2) { A.d_x2[(row * A.ndim + i) * A.W + w] = q2; }
+            q2s = q2s + q2;
+        }
+        if (HSUM) { A.d_x2sum[row * A.W + w] = q2s / A.x2sum_div; }
+        if (HEXP) {
+            if (moved) { R.e = exp(-(q2s / A.exp_div)); }
+            A.d_exp[row * A.W + w] = R.e;
+        }
+    }
+    xo = xsel;
+    So = Snew;
+}
+
+// Steps t and t + 1 (t even) of k_fast64: c = blocks 0 (and 1 without a shared index) of their
+// pair, n receives the next pair's.  Both steps' records are read at once, the next blocks are
+// computed while the reads are in flight, then the two steps run; the second reads what the first
+// wrote when both picked the same coordinate.
+template <int FL>
+__device__ __forceinline__ void pair64(const FastArgs &A, double2 *rec, double *xs, const double *as, const float *cs, Run64 &R,
+                                       const uint32_t (&c)[6], uint32_t (&n)[6], uint64_t t, int64_t row, uint32_t wid, double step,
+                                       int lane, int64_t w)
+{
+    constexpr bool REC = (FL & F_X2) != 0 && (FL & F_FULL) == 0;
+    constexpr bool P2 = (FL & F_P2) != 0;
+    // c[0..3]: block 0 (shared and next word of step t, then of step t + 1); c[4], c[5]: words 0
+    // and 2 of block 1 (the proposal words when the index is drawn by Lemire's method)
+    int vA, vB;
+    if (P2) {
+        vA = (int)__builtin_amdgcn_ubfe(c[0], A.vshift, A.ibits);
+        vB = (int)__builtin_amdgcn_ubfe(c[2], A.vshift, A.ibits);
+    }
+    else {
+        vA = draw_lemire(c[1], (uint32_t)A.nvecs, A.lemire_thr, wid, t, A.key0, A.key1);
+        vB = draw_lemire(c[3], (uint32_t)A.nvecs, A.lemire_thr, wid, t + 1, A.key0, A.key1);
+    }
+    const int sA = vA * WAVE + lane, sB = vB * WAVE + lane;
+    double xA, SA = 0.0, xB, SB = 0.0;
+    if (REC) {
+        const double2 ra = rec[sA], rb = rec[sB];
+        xA = ra.x; SA = ra.y; xB = rb.x; SB = rb.y;
+    }
+    else { xA = xs[sA]; xB = xs[sB]; }
+    const float cA = cs[vA], cB = cs[vB];
+    const uint64_t tn = (t + 2) >> 1;
+    uint32_t m1, m3;
+    philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, n[0], n[1], n[2], n[3]);
+    if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, n[4], m1, n[5], m3); }
+    // the next blocks are needed here, so that their computation fills the reads' latency (instead
+    // of sinking below the first step's branch)
+    asm volatile("" ::"v"(n[0]), "v"(n[1]), "v"(n[2]), "v"(n[3]), "v"(n[4]), "v"(n[5]));
+    step64<FL>(A, rec, xs, as, R, vA, sA, xA, SA, cA, c[0], P2 ? c[1] : c[4], t, row, wid, step, lane, w);
+    if (vB == vA) { xB = xA; SB = SA; }  // the second step reads what the first wrote
+    step64<FL>(A, rec, xs, as, R, vB, sB, xB, SB, cB, c[2], P2 ? c[3] : c[5], t + 1, row + 1, wid, step, lane, w);
+}
+
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
@@ -171,7 +285,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
     constexpr bool ACC = (FL & 7) != 0 && !FULL;  // dwell-time block sums
     constexpr bool REC = HX2 && !FULL;            // 16-byte {x_i, block sum} records
-    constexpr bool P2 = (FL & F_P2) != 0;
+    constexpr bool P2 = (FL & F_P2) != 0;         // the index is the shared word's top bits
     extern __shared__ double lds[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
@@ -199,13 +313,13 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         as[i] = A.a[i];
         cs[i] = (float)(-A.a[i] * 0x1.71547652b82fep+0);
     }
-    double r = 0.0;                                    // |x|^2
+    Run64 R{0.0, 0.0, 0.0, 0.0, (double)B, 0u};
     if (live) {
         for (int i = 0; i < d; ++i) {
             const double v = A.x64[(int64_t)i * A.W + w];
             if (REC) { rec[i * WAVE + lane] = make_double2(v, (double)B * (v * v)); }
             else { xs[i * WAVE + lane] = v; }
-            r = r + v * v;
+            R.r = R.r + v * v;
         }
     }
     __syncthreads();
@@ -215,118 +329,68 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
     uint64_t t = A.ctr[w];
     int64_t nacc = 0;
-    double Sr = 0.0, Se = 0.0, e = 0.0;
     int64_t b = 0;
-    if (ACC) { Sr = (double)B * r; }
-    if (HEXP && ACC) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
+    if (ACC) { R.Sr = (double)B * R.r; }
+    if (HEXP && ACC) { R.e = exp(-(R.r / A.exp_div)); R.Se = (double)B * R.e; }
     if (HEXP && FULL) {
         double q0 = 0.0;
         for (int i = 0; i < d; ++i) { const double v = xs[i * WAVE + lane]; q0 = q0 + v * v; }
-        e = exp(-(q0 / A.exp_div));
+        R.e = exp(-(q0 / A.exp_div));
     }
-    // The words of step t+1 are generated while step t's LDS reads are in flight.  The common
-    // path has no divergent branch: the decision is a select and the LDS state is always
-    // written back (old value on rejection; adding +0.0 to a sum is exact, so every stored
-    // number equals the branched form).
-    uint32_t w0, w1, w2, w3;
-    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+    // Blocks 0 and (without a shared index) 1 of the current step pair t / 2, as pair64 takes them:
+    // p[0], p[1] are step t's shared and next word when t is even, p[2], p[3] when t is odd; p[4] /
+    // p[5] the word after (block 1's words 0 / 2).
+    uint32_t p[6] = {0u, 0u, 0u, 0u, 0u, 0u}, nx[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    uint32_t g1, g3;
+    philox(0u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[0], p[1], p[2], p[3]);
+    if (!P2) { philox(1u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[4], g1, p[5], g3); }
+    // The lanes of a wave step in pairs when their counters agree in parity (always, unless the
+    // counters were set so); otherwise every step runs alone, each lane on its own parity.
+    const uint64_t podd = __builtin_amdgcn_ballot_w64(((uint32_t)t & 1u) != 0);
+    const bool uni = podd == 0 || podd == __builtin_amdgcn_ballot_w64(true);
     // Blocks start at step 0.  The steps run in chunks of at most 2^30 that never cross a block
     // end, so the inner loop's counters are 32-bit whatever the block length (a Simple
     // accumulator's block is the whole run).
-    double mult = (double)B;  // B - j: steps of the block the new value counts for
-    int64_t inblk = 0;        // steps done in the current block
+    int64_t inblk = 0;  // steps done in the current block
     for (int64_t s = 0; s < A.nsteps;) {
         const int len = fast_chunk_len(A.nsteps, s, B, inblk, ACC);
-        uint32_t na = 0;
-        for (int q = 0; q < len; ++q) {
-            const int v = draw_vidx<P2>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
-            const int slot = v * WAVE + lane;
+        R.na = 0;
+        // one step alone, on the lane's own parity
+        auto single = [&](int q) {
+            const bool odd = ((uint32_t)t & 1u) != 0;
+            const uint32_t ws = odd ? p[2] : p[0], wn = odd ? p[3] : p[1];
+            int v;
+            uint32_t wp;
+            if (P2) { v = (int)__builtin_amdgcn_ubfe(ws, A.vshift, A.ibits); wp = wn; }
+            else {
+                v = draw_lemire(wn, range, thr, wid, t, A.key0, A.key1);
+                wp = odd ? p[5] : p[4];
+            }
+            const int sl = v * WAVE + lane;
             double xo, So = 0.0;
-            if (REC) { const double2 rv = rec[slot]; xo = rv.x; So = rv.y; }
-            else { xo = xs[slot]; }
-            const float cv = cs[v];
-            // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
-            const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
-            const uint32_t u_hi = w2, u_lo = w3;
+            if (REC) { const double2 rv = rec[sl]; xo = rv.x; So = rv.y; }
+            else { xo = xs[sl]; }
+            step64<FL>(A, rec, xs, as, R, v, sl, xo, So, cs[v], ws, wp, t, s + q, wid, step, lane, w);
             ++t;
-            uint32_t s0, s3;
-            philox_head9(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, s0, s3);
-            const double xn = xo + step * rr;
-            const double xn2 = xn * xn, xo2 = xo * xo;
-            const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
-            // The FP64 test is u <= exp(-ex) with ex = a_v xn^2 - a_v xo^2 (pdf_acceptance's form).
-            // u = top 53 bits of (u_hi:u_lo) 2^-53 lies in [u_hi 2^-32, (u_hi + 1) 2^-32).  The
-            // screen takes F = 2^(32 - a_v log2 e (xn^2 - xo^2)) in FP32: relative error below 2e-5
-            // wherever F >= 1 (ex < 22.2; the FP64 rounding of ex is far smaller), and U = u_hi
-            // rounded to FP32.  U < 0.9999 F - 2 makes the FP64 test certain to accept,
-            // U > 1.0001 F certain to reject; between the two the FP64 test decides.
-            const float U = (float)u_hi;
-            const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, 32.0f));
-            const bool acc = U < fmaf(F, 0.9999f, -2.0f);
-            const bool band = !acc && U <= F * 1.0001f;
-            if (acc) { ++na; }
-            bool moved = acc;
-            const double xsel = acc ? xn : xo;
-            if (ACC) {
-                const double dx2 = acc ? dx2n : 0.0;
-                if (REC) { rec[slot] = make_double2(xsel, So + dx2 * mult); }
-                else { xs[slot] = xsel; }
-                r = r + dx2;
-                if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
-                if (HEXP && acc) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
+            if (odd) {
+                const uint64_t tn = t >> 1;
+                philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[0], p[1], p[2], p[3]);
+                if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[4], g1, p[5], g3); }
             }
-            else { xs[slot] = xsel; }
-            // the next step's words 0 and 1 (and the state for 2 and 3) are needed here, so the
-            // compiler keeps their computation in this block (instead of sinking it below the
-            // branch, where nothing overlaps it)
-            asm volatile("" ::"v"(w0), "v"(w1), "v"(s0), "v"(s3));
-            // Lanes inside the band went through the update above as rejections (which leaves
-            // every number as it was: +0.0 added to non-negative sums).  When the wave holds
-            // any, the exact test runs and the lanes it accepts are updated as accepted steps,
-            // from the values read at the start of the step.  The common path thus stays one
-            // basic block, in which the next step's Philox words fill the latency of this
-            // step's dependent chain.
-            // (two ballots of plain compares: each is the compare's own lane mask)
-            if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
-                const double u = (double)u_hi * 0x1p-32 + (double)(u_lo >> 11) * 0x1p-53;
-                const double av = as[v];
-                const double ex = xn2 * av - xo2 * av;
-                if (band && u <= exp(-ex)) {
-                    na += 1u;
-                    moved = true;
-                    if (ACC) {
-                        const double dx2 = dx2n;
-                        if (REC) { rec[slot] = make_double2(xn, So + dx2 * mult); }
-                        else { xs[slot] = xn; }
-                        r = r + dx2;
-                        if (!(HX2 && HSUM)) { Sr = Sr + dx2 * mult; }
-                        if (HEXP) { const double en = exp(-(r / A.exp_div)); Se = Se + (en - e) * mult; e = en; }
-                    }
-                    else { xs[slot] = xn; }
-                }
+        };
+        int q = 0;
+        if (uni) {
+            // an odd counter first takes one step; then groups of four steps (two pairs, the blocks
+            // alternating between p and nx), then the rest alone
+            if (((uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t) & 1u) != 0u && q < len) { single(q); ++q; }
+            for (; q + 4 <= len; q += 4) {
+                pair64<FL>(A, rec, xs, as, cs, R, p, nx, t, s + q, wid, step, lane, w);
+                pair64<FL>(A, rec, xs, as, cs, R, nx, p, t + 2, s + q + 2, wid, step, lane, w);
+                t += 4;
             }
-            if (ACC) { mult = mult - 1.0; }
-            if (FULL) {
-                // this step's row of every trace (the observable values after the step)
-                const int64_t row = s + q;
-                double q2s = 0.0;
-                for (int i = 0; i < d; ++i) {
-                    const double v = xs[i * WAVE + lane];
-                    const double q2 = v * v;
-                    if (HX2) { A.d_x2[(row * d + i) * A.W + w] = q2; }
-                    q2s = q2s + q2;
-                }
-                if (HSUM) { A.d_x2sum[row * A.W + w] = q2s / A.x2sum_div; }
-                if (HEXP) {
-                    if (moved) { e = exp(-(q2s / A.exp_div)); }
-                    A.d_exp[row * A.W + w] = e;
-                }
-            }
-            // words 2 and 3 after the exact path, which still reads this step's u_hi and u_lo:
-            // they then take over its registers (no copies at the loop's back edge)
-            philox_tail9(s0, s3, A.key1, w2, w3);
         }
-        nacc += na;
+        for (; q < len; ++q) { single(q); }
+        nacc += R.na;
         s += len;
         inblk += len;
         if (ACC && inblk == B) {
@@ -344,12 +408,12 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
                 else { v2 = xs[i * WAVE + lane]; }
                 rx = rx + v2 * v2;
             }
-            if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : Sr) * norm) / A.x2sum_div; }
-            if (HEXP) { A.d_exp[b * A.W + w] = Se * norm; }
-            r = rx;  // refresh |x|^2 exactly at every block boundary
-            Sr = (double)B * r;
-            if (HEXP) { e = exp(-(r / A.exp_div)); Se = (double)B * e; }
-            mult = (double)B;
+            if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : R.Sr) * norm) / A.x2sum_div; }
+            if (HEXP) { A.d_exp[b * A.W + w] = R.Se * norm; }
+            R.r = rx;  // refresh |x|^2 exactly at every block boundary
+            R.Sr = (double)B * R.r;
+            if (HEXP) { R.e = exp(-(R.r / A.exp_div)); R.Se = (double)B * R.e; }
+            R.mult = (double)B;
             inblk = 0;
             ++b;
         }
@@ -463,12 +527,32 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     int jr = 0, je = 0, j = 0;
     int64_t b = 0;
     if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
-    uint32_t w0, w1, w2, w3;
-    philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+    // Blocks 0 and (without a shared index) 1 of the current step pair t / 2 (the FP64 chain's
+    // layout): step t's shared and next word are p0, p1 when t is even, p2, p3 when odd; g0 / g2
+    // the word after.  A new pair's blocks are computed after its odd step.
+    constexpr bool P2 = (FL & F_P2) != 0;
+    uint32_t p0, p1, p2, p3, g0 = 0u, g1 = 0u, g2 = 0u, g3 = 0u;
+    philox(0u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p0, p1, p2, p3);
+    if (!P2) { philox(1u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, g0, g1, g2, g3); }
+    (void)g1;
+    (void)g3;
     for (int64_t s = 0; s < A.nsteps;) {
         const int len = fast32_chunk_len(A.nsteps, s, B, j, ACC);
         for (int q = 0; q < len; ++q) {
-            const int v = draw_vidx<(FL & F_P2) != 0>(w0, range, thr, A.vshift, wid, t, A.key0, A.key1);
+            const bool odd = ((uint32_t)t & 1u) != 0;
+            const uint32_t ws = odd ? p2 : p0, wn = odd ? p3 : p1;
+            int v;
+            uint32_t wp, ub;
+            if (P2) {
+                v = (int)__builtin_amdgcn_ubfe(ws, A.vshift, A.ibits);
+                wp = wn;
+                ub = (ws << A.ubits) >> 8;
+            }
+            else {
+                v = draw_lemire(wn, range, thr, wid, t, A.key0, A.key1);
+                wp = odd ? g2 : g0;
+                ub = ws >> 8;
+            }
             const int slot = v * WAVE + lane;
             const float xo = xs[slot];
             float So = 0.f, Co = 0.f;
@@ -476,10 +560,15 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
             if (HX2) { So = Ss[slot]; cj = CJ[slot]; Co = __uint_as_float(cj & 0xFFFF0000u); }
             const float asv = as[v];
             // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24
-            const float rr = (float)(2 * (int32_t)(w1 >> 8) + 1 - 16777216) * 0x1p-24f;
-            const float u = (float)(w2 >> 8) * 0x1p-24f;
+            // (the 24 bits of the shared word below the index)
+            const float rr = (float)(2 * (int32_t)(wp >> 8) + 1 - 16777216) * 0x1p-24f;
+            const float u = (float)ub * 0x1p-24f;
             ++t;
-            philox(0u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, w0, w1, w2, w3);
+            if (odd) {
+                const uint64_t tn = t >> 1;
+                philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p0, p1, p2, p3);
+                if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, g0, g1, g2, g3); }
+            }
             const float xn = fmaf(step, rr, xo);
             const float dx2 = (xn - xo) * (xn + xo);
             const float ex = asv * dx2;
@@ -634,10 +723,12 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     A.rej = c.rej;
     A.nvecs = c.nvecs;
     A.lemire_thr = (0u - (uint32_t)c.nvecs) % (uint32_t)c.nvecs;
-    A.vshift = 0;
-    for (int m = 1; m < 32; ++m) {
-        if ((uint32_t)c.nvecs == (1u << m)) { A.vshift = 32 - m; }
-    }
+    // the native stream's shared word: the index's bits (FP32: at most 8), then the acceptance bits
+    A.ibits = prec == MCI_PREC_FP64 ? c.ibits : shared_index_bits32(c.ibits);
+    A.vshift = A.ibits >= 0 ? 32 - A.ibits : 0;
+    A.ubits = A.ibits > 0 ? (uint32_t)A.ibits : 0u;
+    A.umask = 0xFFFFFFFFu >> A.ubits;
+    A.uexp = (float)(32 - (int)A.ubits);
     A.blocksize = 1;
     A.nblocks = 0;
     A.d_x2 = A.d_x2sum = A.d_exp = nullptr;
@@ -657,7 +748,7 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     if (prec != MCI_PREC_FP64 && A.blocksize > INT32_MAX) { return MCI_E_INVALID_ARGUMENT; }
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
     const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0 && !(fl & F_FULL));
-    if (A.vshift != 0) { fl |= F_P2; }
+    if (A.ibits >= 0) { fl |= F_P2; }
     switch (fl) {
     case 0: launch_fl<0>(prec, grid, sh, s, A); break;
     case 1: launch_fl<1>(prec, grid, sh, s, A); break;

@@ -20,6 +20,21 @@ constexpr int fast_chunk_len(int64_t nsteps, int64_t s, int64_t B, int64_t inblk
     if (acc && B - inblk < lim) { lim = B - inblk; }
     return (int)(lim < FAST_MAX_CHUNK ? lim : FAST_MAX_CHUNK);
 }
+// Bits of a native step's shared word (main word 0) that the index of a single-vector move over
+// nvecs vectors takes: m when nvecs = 2^m with m <= 11 -- the FP64 acceptance uniform keeps the 53
+// bits below them, the FP32 one (m <= 8) 24 -- else -1, and the index is drawn by Lemire's method
+// from a word of its own.  Only the chain's own move shares (not one inside a MultiStepMove), and
+// only in the native stream.
+constexpr int shared_index_bits(bool philox, bool vec_move, bool multistep, int64_t nvecs)
+{
+    if (!philox || !vec_move || multistep || nvecs < 1 || (nvecs & (nvecs - 1)) != 0) { return -1; }
+    int m = 0;
+    while (((int64_t)1 << m) < nvecs) { ++m; }
+    return m <= 11 ? m : -1;
+}
+// The FP32 chain shares only up to 8 bits (its 24-bit acceptance uniform sits below them).
+constexpr int shared_index_bits32(int ibits) { return ibits <= 8 ? ibits : -1; }
+
 // Steps between recomputations of |x|^2 from the position in the FP32 kernel.
 constexpr int FAST32_REFRESH = 1024;
 

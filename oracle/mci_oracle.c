@@ -307,22 +307,37 @@ static uint64_t walker_seed(uint64_t seed, int64_t w)
     return splitmix64(&s);
 }
 
-static uint32_t ph_word(const struct orc_ctx *c, walker *w, uint32_t k)
+/* The native stream (Philox4x32-10, key = seed).  Steps go in pairs: block j of step pair
+ * t >> 1 is Philox(counter = (j, walker, t >> 1)); main word k of step t is word
+ * 2 (t & 1) + (k & 1) of block k >> 1, so one block holds words 0 and 1 of two consecutive steps.
+ * Main word 0 of a step is its shared word (see draw_accept); the draws of the move start at
+ * word 1.  Two side streams keyed by the step itself: the Lemire rejections (block 0x20000000 +
+ * (kx >> 2), word kx & 3) and the low word of the acceptance uniform (block 0x30000000, word 0). */
+static uint32_t ph_block_word(const struct orc_ctx *c, const walker *w, uint32_t blk, uint64_t ct, uint32_t word)
 {
-    uint32_t ctr[4] = { k >> 2, w->wid, (uint32_t)w->t, (uint32_t)(w->t >> 32) };
+    uint32_t ctr[4] = { blk, w->wid, (uint32_t)ct, (uint32_t)(ct >> 32) };
     uint32_t key[2] = { (uint32_t)c->seed, (uint32_t)(c->seed >> 32) };
     uint32_t out[4];
     orc_philox4x32_10(ctr, key, out);
-    return out[k & 3u];
+    return out[word];
+}
+
+static uint32_t ph_word(const struct orc_ctx *c, walker *w, uint32_t k)
+{
+    return ph_block_word(c, w, k >> 1, w->t >> 1, 2u * (uint32_t)(w->t & 1u) + (k & 1u));
 }
 
 static uint32_t ph_next(const struct orc_ctx *c, walker *w) { return ph_word(c, w, w->k++); }
-static uint32_t ph_extra(const struct orc_ctx *c, walker *w) { return ph_word(c, w, 0x80000000u + (w->kx++)); }
+static uint32_t ph_extra(const struct orc_ctx *c, walker *w)
+{
+    const uint32_t kx = w->kx++;
+    return ph_block_word(c, w, 0x20000000u + (kx >> 2), w->t, kx & 3u);
+}
 
 static void step_begin(const struct orc_ctx *c, walker *w)
 {
     (void)c;
-    w->k = 0;
+    w->k = 1; /* main word 0 is the step's shared word */
     w->kx = 0;
 }
 
@@ -446,6 +461,33 @@ static double srrd_draw(const struct orc_ctx *c, walker *w)
     }
     /* SymmetrizedPRRD: bernoulli_distribution(0.5) on one canonical draw */
     return ((canon(c, w) - 0.0) < 0.5 * (1.0 - 0.0)) ? val : -val;
+}
+
+/* The shared word (main word 0 of a native step).  With a single-vector move over n = 2^m vectors
+ * (m <= 11, not inside a MultiStepMove) its top m bits are the vector index; the acceptance
+ * uniform is the 53 bits below them: u = ((word0 : low word) << m) >> 11, times 2^-53, the low word
+ * from its side stream (the m <= 11 zero bits shifted in fall below the 53).  Otherwise m = 0 and
+ * the index is drawn by draw_index.  Returns m, or -1 when the index is not shared. */
+int orc_shared_index_bits(const orc_ctx *c)
+{
+    if (c->rng != MCI_RNG_PHILOX || c->movetype != MCI_MOVE_VEC || c->ms_on || c->nvecs < 1) return -1;
+    const uint32_t n = (uint32_t)c->nvecs;
+    if (n & (n - 1u)) return -1;
+    int m = 0;
+    while ((1u << m) < n) ++m;
+    return m <= 11 ? m : -1;
+}
+
+/* the acceptance uniform of a Metropolis step: MT mode = uniform_real_distribution<double>(0,1)
+ * (MCIntegrator.cpp:343); native = the shared word below its index bits, then the side-stream word */
+static double draw_accept(const struct orc_ctx *c, walker *w)
+{
+    if (c->rng == MCI_RNG_MT19937_64) return draw_unit(c, w);
+    const int ib = orc_shared_index_bits(c);
+    const unsigned m = ib > 0 ? (unsigned)ib : 0u;
+    const uint64_t hi = ph_word(c, w, 0), lo = ph_block_word(c, w, 0x30000000u, w->t, 0);
+    const uint64_t v = ((hi << 32) | lo) << m;
+    return (double)(v >> 11) * 0x1p-53;
 }
 
 /* uniform integer in [0, n): MT mode = uniform_int_distribution<int>(0, n-1) (SRRDVecMove.hpp:27,78);
@@ -1657,7 +1699,7 @@ mci_status orc_get_nobsdim(orc_ctx *c, int *n)
 /* Metropolis step                                                            */
 /* ------------------------------------------------------------------------- */
 /* SRRDAllMove::trialMove SRRDAllMove.hpp:67-80, SRRDVecMove::trialMove SRRDVecMove.hpp:75-96 */
-static double srrd_move(const orc_ctx *c, walker *w)
+static double srrd_move(const orc_ctx *c, walker *w, int outer)
 {
     if (c->movetype == MCI_MOVE_ALL) {
         int xidx = 0;
@@ -1669,7 +1711,8 @@ static double srrd_move(const orc_ctx *c, walker *w)
         w->nchanged = c->ndim;
         return 1.;
     }
-    const int vidx = draw_index(c, w, c->nvecs);
+    const int ib = outer ? orc_shared_index_bits(c) : -1;
+    const int vidx = ib > 0 ? (int)(ph_word(c, w, 0) >> (32 - ib)) : ib == 0 ? 0 : draw_index(c, w, c->nvecs);
     const int xidx = vidx * c->veclen;
     int t = 0;
     while (t < c->ntypes) {
@@ -1797,7 +1840,7 @@ static double trial_move(const orc_ctx *c, walker *w)
 {
     /* a mixed move: first the uniform that picks the independence move with probability lm_pmix */
     if (c->lm_K > 0 && c->lm_pmix > 0. && draw_unit(c, w) < c->lm_pmix) return lm_move(c, w);
-    if (!c->ms_on) return srrd_move(c, w);
+    if (!c->ms_on) return srrd_move(c, w, 1);
     const int nd = c->ndim;
     memcpy(w->ms_x0, w->xold, sizeof(double) * (size_t)nd);
     int off = 0;
@@ -1813,7 +1856,7 @@ static double trial_move(const orc_ctx *c, walker *w)
         off += c->ms_pdf[p].nproto;
     }
     for (int s = 0; s < c->ms_nsteps; ++s) {
-        const double moveAcc = srrd_move(c, w);
+        const double moveAcc = srrd_move(c, w, 0);
         double pdfAcc = 1.;
         off = 0;
         for (int p = 0; p < c->ms_npdf; ++p) {
@@ -1874,7 +1917,7 @@ static void do_step_mrt2(orc_ctx *c, walker *w)
         pdfAcc = pdfAcc * pdf_compute_acceptance(&c->pdf[p], w, w->pold + off, w->pnew + off);
         off += c->pdf[p].nproto;
     }
-    const double u = draw_unit(c, w);
+    const double u = draw_accept(c, w);
     w->accepted = (u <= pdfAcc * moveAcc);
     if (w->accepted) ++w->acc;
     else ++w->rej;
@@ -2051,7 +2094,8 @@ static mci_status sample_fp32(orc_ctx *c, walker *w, int64_t n, accu *accs, cons
     int jr = 0, je = 0, j = 0;
     if (iexp >= 0) e = orc_bexp(-r * inv_ediv);
     const uint32_t range = (uint32_t)c->nvecs;
-    const uint32_t key[2] = { (uint32_t)c->seed, (uint32_t)(c->seed >> 32) };
+    /* the index shares the step's word 0 with the 24-bit acceptance uniform when it takes at most 8 bits */
+    const int ib0 = orc_shared_index_bits(c), ib = ib0 <= 8 ? ib0 : -1;
     int64_t b = 0, nacc = 0;
     const double norm = 1. / (double)Bi;
     for (int64_t s = 0; s < n;) {
@@ -2060,23 +2104,35 @@ static mci_status sample_fp32(orc_ctx *c, walker *w, int64_t n, accu *accs, cons
         if (acc_on && (int64_t)(Bi - j) < lim) lim = Bi - j;
         const int len = (int)(lim < 1024 ? lim : 1024);
         for (int q = 0; q < len; ++q) {
-            uint32_t ctr[4] = { 0u, w->wid, (uint32_t)w->t, (uint32_t)(w->t >> 32) }, wd[4];
-            orc_philox4x32_10(ctr, key, wd);
-            /* index: Lemire on word 0, rejected draws from the extra-word blocks (draw_index) */
-            uint64_t prod = (uint64_t)wd[0] * range;
-            uint32_t low = (uint32_t)prod;
-            if (low < range) {
-                const uint32_t thr = (0u - range) % range;
-                w->kx = 0;
-                while (low < thr) {
-                    prod = (uint64_t)ph_extra(c, w) * range;
-                    low = (uint32_t)prod;
-                }
+            /* the draws in the FP64 chain's order: main word 0 is shared (index bits, then the
+             * acceptance uniform), the move's index (when not shared) and proposal follow */
+            const uint32_t wd0 = ph_word(c, w, 0);
+            int v;
+            uint32_t ubits, wp;
+            if (ib >= 0) {
+                v = ib > 0 ? (int)(wd0 >> (32 - ib)) : 0;
+                ubits = (wd0 << ib) >> 8;
+                wp = ph_word(c, w, 1);
             }
-            const int v = (int)(prod >> 32);
+            else {
+                /* index: Lemire on main word 1, rejected draws from the extra-word blocks (draw_index) */
+                uint64_t prod = (uint64_t)ph_word(c, w, 1) * range;
+                uint32_t low = (uint32_t)prod;
+                if (low < range) {
+                    const uint32_t thr = (0u - range) % range;
+                    w->kx = 0;
+                    while (low < thr) {
+                        prod = (uint64_t)ph_extra(c, w) * range;
+                        low = (uint32_t)prod;
+                    }
+                }
+                v = (int)(prod >> 32);
+                ubits = wd0 >> 8;
+                wp = ph_word(c, w, 2);
+            }
             const float xo = xs[v];
-            const float rr = sym24f(wd[1]);
-            const float u = (float)(wd[2] >> 8) * 0x1p-24f;
+            const float rr = sym24f(wp);
+            const float u = (float)ubits * 0x1p-24f;
             ++w->t;
             const float xn = fmaf(step, rr, xo);
             const float dx2 = (xn - xo) * (xn + xo);

@@ -65,3 +65,46 @@ def test_chunks_stay_32bit_and_flush_at_block_ends(plan, nsteps, B, acc):
     if acc:
         assert flushes == nsteps // B
     assert chunks >= -(-nsteps // 2 ** 30)
+
+
+BITS = r"""
+#include "mci_fast_plan.hpp"
+#include <cstdio>
+int main()
+{
+    for (long long n = 0; n <= 5000; ++n)
+        std::printf("%lld %d %d %d %d\n", n, mcid::shared_index_bits(true, true, false, n), mcid::shared_index_bits(false, true, false, n),
+                    mcid::shared_index_bits(true, false, false, n), mcid::shared_index_bits(true, true, true, n));
+    return 0;
+}
+"""
+
+
+def test_shared_index_bits_match_the_oracle(tmp_path):
+    """The index bits the library takes from a native step's shared word (mci_fast_plan.hpp) and the
+    oracle's rule (orc_shared_index_bits) agree: m for 2^m vectors up to 2^11, else -1 (Lemire);
+    only native streams, vector moves, outside a MultiStepMove."""
+    import ctypes as C
+
+    from mcitest import Chain
+    src = tmp_path / "bits.cpp"
+    src.write_text(BITS)
+    exe = tmp_path / "bits"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", CSRC, str(src), "-o", str(exe)], check=True)
+    rows = [list(map(int, ln.split())) for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n") if ln]
+    for n, m, mt, allmove, ms in rows:
+        exp = -1
+        if n >= 1 and n & (n - 1) == 0 and n.bit_length() - 1 <= 11:
+            exp = n.bit_length() - 1
+        assert (m, mt, allmove, ms) == (exp, -1, -1, -1), n
+    # the oracle, through contexts of 2^k coordinates with single-coordinate moves
+    from mcitest import RNG_PHILOX, load
+    lib = load("orc")
+    lib.orc_shared_index_bits.argtypes = [C.c_void_p]
+    for d in (1, 2, 3, 16, 24, 2048, 4096):
+        c = Chain("orc", d, 1)
+        c.setRng(RNG_PHILOX)
+        c.setTrialMove(0, 1, 1, None)
+        assert lib.orc_shared_index_bits(c.ctx) == rows[d][1], d
+        c.setTrialMoveMultistep(2, 0, 1, 1, None)
+        assert lib.orc_shared_index_bits(c.ctx) == -1

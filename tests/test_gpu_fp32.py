@@ -132,7 +132,11 @@ def test_fp32_precision_form_full_size():
     error bars of their analytic values (sigma_i^2 and prod (1 + 2 sigma_i^2 / 64)^(-1/2); with 65
     independent components one beyond 3 sigma has probability 0.16) and none beyond 3.5, the error
     bars are calibrated (mean z^2 of the 65 components near 1), and FP32 and FP64 agree to a small
-    fraction of an error bar."""
+    fraction of an error bar.  The error bar is the standard error of the 262,144 walker means (the
+    walkers sample independently once the shared calibration is done); the MPIMCI combine of the
+    walkers' MJBlocker errors, which the integrate returns, runs low here (64 blocks of 160 steps
+    per walker are too few for the blocking to reach the chain's correlation time at d = 64): its
+    ratio to the spread is printed and bounded."""
     W, nmc = 262144, 10240
     sig = sigmas(64)
     exact = np.concatenate([sig ** 2, [np.prod((1 + 2 * sig ** 2 / 64) ** -0.5)]])
@@ -142,17 +146,20 @@ def test_fp32_precision_form_full_size():
         c.setRng(RNG_PHILOX)
         c.setCoupling(COUPLING_SHARED)
         c.setPrecision(prec)
-        res[prec] = c.integrate(nmc)
-    zs = {}
-    for prec, (a, e) in res.items():
-        z = (a - exact) / e
-        zs[prec] = z
-        print(f"prec {prec}: max |z| {np.max(np.abs(z)):.2f}, mean z^2 {np.mean(z * z):.2f}", flush=True)
+        a, e = c.integrate(nmc)
+        wa, _ = c.walkerResults()
+        res[prec] = (a, e, wa.std(axis=0, ddof=1) / np.sqrt(W))
+        assert np.allclose(wa.mean(axis=0), a, rtol=1e-12, atol=0)
+    for prec, (a, e, es) in res.items():
+        z = (a - exact) / es
+        print(f"prec {prec}: max |z| {np.max(np.abs(z)):.2f}, mean z^2 {np.mean(z * z):.2f}; "
+              f"MJBlocker combine / walker spread: median {np.median(e / es):.3f}", flush=True)
         assert np.all(np.abs(z) < 3.5), np.max(np.abs(z))
         assert np.mean(np.abs(z) < 3) >= 64 / 65
-        assert 0.4 < np.mean(z * z) < 1.8
-    a64, e64 = res[PREC_FP64]
-    a32, _ = res[PREC_FP32]
+        assert 0.5 < np.mean(z * z) < 1.6
+        assert 0.5 < np.median(e / es) < 1.1
+    a64, _, e64 = res[PREC_FP64]
+    a32 = res[PREC_FP32][0]
     print("FP32 vs FP64 (chains coupled through the same words): max |diff|/err",
           np.max(np.abs(a32 - a64) / e64), "max rel", np.max(np.abs(a32 - a64) / np.abs(a64)), flush=True)
     assert np.all(np.abs(a32 - a64) < 0.25 * e64)
