@@ -489,6 +489,108 @@ __device__ __forceinline__ uint32_t bf16_bits(float v)
 // recomputes |x|^2 from the position.  With X2 the per-coordinate state is 12 bytes in three
 // [coordinate][lane] arrays: x_i, the block sum of x_i^2, and (BF16 compensation, step of the last
 // change in the chunk) -- 48 KiB at d = 64, so three workgroups share a CU.
+// the running sums of one walker in k_fast32 (chunk steps jr, je: when the current |x|^2 and exp
+// values were set)
+struct Run32
+{
+    float r, rc;   // |x|^2, Kahan-compensated
+    float Sr, Cr;  // block sum of |x|^2
+    float Se, Ce;  // block sum of exp(-|x|^2/c)
+    float e;
+    int jr, je;
+    uint32_t na;   // accepted steps of the chunk
+};
+
+// One FP32 step at chunk step q on coordinate v (LDS slot `slot`) from its value xo, block sum So
+// and packed compensation / last change cj, which the caller read and which hold the values after
+// the step on return.  ub: the 24 acceptance bits, wp: the proposal word.
+template <int FL>
+__device__ __forceinline__ void step32(const FastArgs &A, float *xs, float *Ss, uint32_t *CJ, const float *as, Run32 &R, int v, int slot,
+                                       float &xo, float &So, uint32_t &cj, uint32_t ub, uint32_t wp, int q, float step, float inv_ediv)
+{
+    constexpr bool ACC = (FL & 7) != 0, HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    const float asv = as[v];
+    // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24 (the 24
+    // bits of the shared word below the index)
+    const float rr = (float)(2 * (int32_t)(wp >> 8) + 1 - 16777216) * 0x1p-24f;
+    const float u = (float)ub * 0x1p-24f;
+    const float xn = fmaf(step, rr, xo);
+    const float dx2 = (xn - xo) * (xn + xo);
+    const float ex = asv * dx2;
+    // u <= bexp(-ex), this library's own exp (the CPU evaluates the same operations).  The hardware
+    // exp2 screens it: F = 2^(-ex log2 e) is within 1e-5 relative of bexp(-ex) wherever bexp is
+    // neither 0 nor inf (the rounding of ex log2 e contributes at most 127 * 9e-8 to the exponent),
+    // so u < 0.9998 F accepts and u > 1.0002 F rejects with certainty; lanes between the two (and
+    // F == 0, flushed) take bexp under a wave vote.
+    const float F = __builtin_amdgcn_exp2f(ex * -0x1.715476p+0f);
+    bool acc = u < F * 0.9998f;
+    const bool band = !acc && !(u > F * 1.0002f);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0, 0)) {
+        if (band) { acc = u <= mcif::bexp(-ex); }
+    }
+    R.na += acc ? 1u : 0u;
+    const float xsel = acc ? xn : xo;
+    xs[slot] = xsel;
+    if (ACC) {
+        if (HX2) {
+            // the old x_v^2 was held since the chunk step in the low half of cj
+            const int jl = (int)(cj & 0xFFFFu);
+            float Co = __uint_as_float(cj & 0xFFFF0000u);
+            kahan_prod_if(So, Co, xo * xo, (float)(q - jl), acc);
+            Ss[slot] = So;
+            cj = (bf16_bits(Co) << 16) | (acc ? (uint32_t)q : (uint32_t)jl);
+            CJ[slot] = cj;
+        }
+        if (HSUM) {
+            kahan_prod_if(R.Sr, R.Cr, R.r, (float)(q - R.jr), acc);
+            R.jr = acc ? q : R.jr;
+        }
+        kahan_if(R.r, R.rc, dx2, acc);
+        if (HEXP && acc) { kahan_prod(R.Se, R.Ce, R.e, (float)(q - R.je)); R.je = q; R.e = mcif::bexp(-R.r * inv_ediv); }
+    }
+    xo = xsel;
+}
+
+// Steps t and t + 1 (t even) of k_fast32 at chunk steps q, q + 1, as pair64 does for the FP64 chain
+template <int FL>
+__device__ __forceinline__ void pair32(const FastArgs &A, float *xs, float *Ss, uint32_t *CJ, const float *as, Run32 &R,
+                                       const uint32_t (&c)[6], uint32_t (&n)[6], uint64_t t, int q, uint32_t wid, float step,
+                                       float inv_ediv, int lane)
+{
+    constexpr bool HX2 = (FL & F_X2) != 0;
+    constexpr bool P2 = (FL & F_P2) != 0;
+    int vA, vB;
+    uint32_t uA, uB, wA, wB;
+    if (P2) {
+        vA = (int)__builtin_amdgcn_ubfe(c[0], A.vshift, A.ibits);
+        vB = (int)__builtin_amdgcn_ubfe(c[2], A.vshift, A.ibits);
+        uA = (c[0] << A.ubits) >> 8;
+        uB = (c[2] << A.ubits) >> 8;
+        wA = c[1];
+        wB = c[3];
+    }
+    else {
+        vA = draw_lemire(c[1], (uint32_t)A.nvecs, A.lemire_thr, wid, t, A.key0, A.key1);
+        vB = draw_lemire(c[3], (uint32_t)A.nvecs, A.lemire_thr, wid, t + 1, A.key0, A.key1);
+        uA = c[0] >> 8;
+        uB = c[2] >> 8;
+        wA = c[4];
+        wB = c[5];
+    }
+    const int sA = vA * WAVE + lane, sB = vB * WAVE + lane;
+    float xA = xs[sA], xB = xs[sB], SA = 0.f, SB = 0.f;
+    uint32_t jA = 0u, jB = 0u;
+    if (HX2) { SA = Ss[sA]; jA = CJ[sA]; SB = Ss[sB]; jB = CJ[sB]; }
+    const uint64_t tn = (t + 2) >> 1;
+    uint32_t m1, m3;
+    philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, n[0], n[1], n[2], n[3]);
+    if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, n[4], m1, n[5], m3); }
+    asm volatile("" ::"v"(n[0]), "v"(n[1]), "v"(n[2]), "v"(n[3]), "v"(n[4]), "v"(n[5]));
+    step32<FL>(A, xs, Ss, CJ, as, R, vA, sA, xA, SA, jA, uA, wA, q, step, inv_ediv);
+    if (vB == vA) { xB = xA; SB = SA; jB = jA; }  // the second step reads what the first wrote
+    step32<FL>(A, xs, Ss, CJ, as, R, vB, sB, xB, SB, jB, uB, wB, q + 1, step, inv_ediv);
+}
+
 template <int FL>
 __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
 {
@@ -518,29 +620,27 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     uint64_t t = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
-    int64_t nacc = 0;
     const int B = (int)A.blocksize;  // at most 2^31 - 1 (checked by the host)
     const float inv_ediv = (float)(1.0 / A.exp_div);
-    // block sums of |x|^2 and exp(-|x|^2/c) with their compensations; jr, je: the chunk steps
-    // at which their current values were set; j: steps done in the block
-    float Sr = 0.f, Cr = 0.f, Se = 0.f, Ce = 0.f, e = 0.f;
-    int jr = 0, je = 0, j = 0;
+    Run32 R{r, rc, 0.f, 0.f, 0.f, 0.f, 0.f, 0, 0, 0u};
+    int64_t nacc = 0;
+    int j = 0;  // steps done in the block
     int64_t b = 0;
-    if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
-    // Blocks 0 and (without a shared index) 1 of the current step pair t / 2 (the FP64 chain's
-    // layout): step t's shared and next word are p0, p1 when t is even, p2, p3 when odd; g0 / g2
-    // the word after.  A new pair's blocks are computed after its odd step.
+    if (HEXP) { R.e = mcif::bexp(-R.r * inv_ediv); }
+    // Blocks 0 and (without a shared index) 1 of the current step pair t / 2, laid out as for the
+    // FP64 kernel (pair64)
     constexpr bool P2 = (FL & F_P2) != 0;
-    uint32_t p0, p1, p2, p3, g0 = 0u, g1 = 0u, g2 = 0u, g3 = 0u;
-    philox(0u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p0, p1, p2, p3);
-    if (!P2) { philox(1u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, g0, g1, g2, g3); }
-    (void)g1;
-    (void)g3;
+    uint32_t p[6] = {0u, 0u, 0u, 0u, 0u, 0u}, nx[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    uint32_t g1, g3;
+    philox(0u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[0], p[1], p[2], p[3]);
+    if (!P2) { philox(1u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[4], g1, p[5], g3); }
+    const uint64_t podd = __builtin_amdgcn_ballot_w64(((uint32_t)t & 1u) != 0);
+    const bool uni = podd == 0 || podd == __builtin_amdgcn_ballot_w64(true);
     for (int64_t s = 0; s < A.nsteps;) {
         const int len = fast32_chunk_len(A.nsteps, s, B, j, ACC);
-        for (int q = 0; q < len; ++q) {
+        auto single = [&](int q) {
             const bool odd = ((uint32_t)t & 1u) != 0;
-            const uint32_t ws = odd ? p2 : p0, wn = odd ? p3 : p1;
+            const uint32_t ws = odd ? p[2] : p[0], wn = odd ? p[3] : p[1];
             int v;
             uint32_t wp, ub;
             if (P2) {
@@ -550,57 +650,33 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
             }
             else {
                 v = draw_lemire(wn, range, thr, wid, t, A.key0, A.key1);
-                wp = odd ? g2 : g0;
+                wp = odd ? p[5] : p[4];
                 ub = ws >> 8;
             }
             const int slot = v * WAVE + lane;
-            const float xo = xs[slot];
-            float So = 0.f, Co = 0.f;
+            float xo = xs[slot], So = 0.f;
             uint32_t cj = 0u;
-            if (HX2) { So = Ss[slot]; cj = CJ[slot]; Co = __uint_as_float(cj & 0xFFFF0000u); }
-            const float asv = as[v];
-            // proposal on the symmetric 24-bit grid (2m + 1 - 2^24) 2^-24, acceptance u on 2^-24
-            // (the 24 bits of the shared word below the index)
-            const float rr = (float)(2 * (int32_t)(wp >> 8) + 1 - 16777216) * 0x1p-24f;
-            const float u = (float)ub * 0x1p-24f;
+            if (HX2) { So = Ss[slot]; cj = CJ[slot]; }
+            step32<FL>(A, xs, Ss, CJ, as, R, v, slot, xo, So, cj, ub, wp, q, step, inv_ediv);
             ++t;
             if (odd) {
                 const uint64_t tn = t >> 1;
-                philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p0, p1, p2, p3);
-                if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, g0, g1, g2, g3); }
+                philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[0], p[1], p[2], p[3]);
+                if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[4], g1, p[5], g3); }
             }
-            const float xn = fmaf(step, rr, xo);
-            const float dx2 = (xn - xo) * (xn + xo);
-            const float ex = asv * dx2;
-            // u <= bexp(-ex), this library's own exp (the CPU evaluates the same operations).  The
-            // hardware exp2 screens it: F = 2^(-ex log2 e) is within 1e-5 relative of bexp(-ex)
-            // wherever bexp is neither 0 nor inf (the rounding of ex log2 e contributes at most
-            // 127 * 9e-8 to the exponent), so u < 0.9998 F accepts and u > 1.0002 F rejects with
-            // certainty; lanes between the two (and F == 0, flushed) take bexp under a wave vote.
-            const float F = __builtin_amdgcn_exp2f(ex * -0x1.715476p+0f);
-            bool acc = u < F * 0.9998f;
-            const bool band = !acc && !(u > F * 1.0002f);
-            if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0, 0)) {
-                if (band) { acc = u <= mcif::bexp(-ex); }
-            }
-            nacc += acc ? 1 : 0;
-            xs[slot] = acc ? xn : xo;
-            if (ACC) {
-                if (HX2) {
-                    // the old x_v^2 was held since the chunk step in the low half of cj
-                    const int jl = (int)(cj & 0xFFFFu);
-                    kahan_prod_if(So, Co, xo * xo, (float)(q - jl), acc);
-                    Ss[slot] = So;
-                    CJ[slot] = (bf16_bits(Co) << 16) | (acc ? (uint32_t)q : (uint32_t)jl);
-                }
-                if (HSUM) {
-                    kahan_prod_if(Sr, Cr, r, (float)(q - jr), acc);
-                    jr = acc ? q : jr;
-                }
-                kahan_if(r, rc, dx2, acc);
-                if (HEXP && acc) { kahan_prod(Se, Ce, e, (float)(q - je)); je = q; e = mcif::bexp(-r * inv_ediv); }
+        };
+        int q = 0;
+        if (uni) {
+            if (((uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t) & 1u) != 0u && q < len) { single(q); ++q; }
+            for (; q + 4 <= len; q += 4) {
+                pair32<FL>(A, xs, Ss, CJ, as, R, p, nx, t, q, wid, step, inv_ediv, lane);
+                pair32<FL>(A, xs, Ss, CJ, as, R, nx, p, t + 2, q + 2, wid, step, inv_ediv, lane);
+                t += 4;
             }
         }
+        for (; q < len; ++q) { single(q); }
+        nacc += R.na;  // chunks hold at most FAST32_REFRESH steps
+        R.na = 0;
         s += len;
         if (ACC) {
             // the chunk ends: every value held counts until here; at a block end the sums flush
@@ -628,19 +704,19 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
                 }
                 kahan(rx, rxc, v2 * v2);
             }
-            if (HSUM) { kahan_prod(Sr, Cr, r, (float)(len - jr)); }
-            if (HEXP) { kahan_prod(Se, Ce, e, (float)(len - je)); }
+            if (HSUM) { kahan_prod(R.Sr, R.Cr, R.r, (float)(len - R.jr)); }
+            if (HEXP) { kahan_prod(R.Se, R.Ce, R.e, (float)(len - R.je)); }
             if (bend) {
-                if (HSUM) { A.d_x2sum[b * A.W + w] = (((double)Sr - (double)Cr) * norm) / A.x2sum_div; }
-                if (HEXP) { A.d_exp[b * A.W + w] = ((double)Se - (double)Ce) * norm; }
-                Sr = Cr = Se = Ce = 0.f;
+                if (HSUM) { A.d_x2sum[b * A.W + w] = (((double)R.Sr - (double)R.Cr) * norm) / A.x2sum_div; }
+                if (HEXP) { A.d_exp[b * A.W + w] = ((double)R.Se - (double)R.Ce) * norm; }
+                R.Sr = R.Cr = R.Se = R.Ce = 0.f;
                 j = 0;
                 ++b;
             }
-            jr = je = 0;
-            r = rx;
-            rc = rxc;
-            if (HEXP) { e = mcif::bexp(-r * inv_ediv); }
+            R.jr = R.je = 0;
+            R.r = rx;
+            R.rc = rxc;
+            if (HEXP) { R.e = mcif::bexp(-R.r * inv_ediv); }
         }
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = (double)xs[i * WAVE + lane]; }
