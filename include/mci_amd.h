@@ -422,15 +422,22 @@ mci_status mci_lp_set_training_pairs(mci_lp *lp, double same_fraction);
  * proposal of MCI_LP_INDEP_FP32 (default off) */
 mci_status mci_lp_set_fixed_input(mci_lp *lp, int on);
 /* before training: fit a diagonal Gaussian mixture (the network's K components) to the first
- * 16,384 walkers of the last snapshot (mci_gmm_fit_diag, em_iters iterations) and write it into
- * the network's last layer (biases = the fit, weights scaled by 0.02), so that training starts
- * from that mixture */
+ * 16,384 walkers of the last snapshot (mci_gmm_fit_diag_device on the sampler's device, em_iters
+ * iterations) and write it into the network's last layer (biases = the fit, weights scaled by
+ * 0.02), so that training starts from that mixture */
 mci_status mci_lp_init_network_from_samples(mci_lp *lp, int em_iters);
 /* host only (no device): diagonal Gaussian mixture of K components fitted to x [n][d] by em_iters
  * EM iterations from greedy k-means++ seeds (16 candidates per seed, stream of `seed`); writes
  * mu [K][d], var [K][d], pi [K] and, if loglik is not null, the mean log-likelihood of x */
 mci_status mci_gmm_fit_diag(const double *x, int64_t n, int d, int K, int em_iters, uint64_t seed, double *mu, double *var,
                             double *pi, double *loglik);
+/* the same fit on the current device (d <= 128, K <= 16; x and the outputs in host memory): the
+ * seeding draws the same candidates (their squared distances are bit-identical to the host fit's,
+ * the potentials that pick among them are sums in another order), the EM iterations run in FP64
+ * device kernels with every sum over samples in a fixed order, so the result is deterministic and
+ * agrees with mci_gmm_fit_diag to rounding */
+mci_status mci_gmm_fit_diag_device(const double *x, int64_t n, int d, int K, int em_iters, uint64_t seed, double *mu, double *var,
+                                   double *pi, double *loglik);
 /* Adam on minibatches of training pairs; losses of the first and last minibatch */
 mci_status mci_lp_train(mci_lp *lp, int iters, int batch, double lr, double *loss_first, double *loss_last);
 /* last run: [0..1] avg, [2..3] standard error of walker means, [4..5] MPIMCI combine of the

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT, "libmci_amd.so")
-SOURCES = ["mci_kernels.hip", "mci_fast.hip", "mci_batch.hip", "mci_mlp.hip", "mci_lp.hip", "mci_noalloc.hip", "mci_api.cpp",
+SOURCES = ["mci_kernels.hip", "mci_fast.hip", "mci_batch.hip", "mci_mlp.hip", "mci_lp.hip", "mci_noalloc.hip", "mci_em.hip", "mci_api.cpp",
            "mci_batch_api.cpp", "mci_lp_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCI_OFFLOAD_ARCH", "gfx950")

@@ -3,6 +3,7 @@
 // runs the plain random-walk chain (FP64 or FP32), collects training pairs from it, trains the
 // network on the GPU and runs the chain that mixes random-walk and learned moves.
 #include "mci_host.hpp"
+#include "mci_em.hpp"
 #include "mci_kernels.hpp"
 #include "mci_lp.hpp"
 #include "mci_mlp.hpp"
@@ -683,6 +684,105 @@ mci_status run_mixed(mci_lp *lp, int64_t nsteps, int nblocks, double pmix, bool 
     return MCI_OK;
 }
 
+// mci_gmm_fit_diag's fit with its sample sums on the device (stream s, current device): the same
+// greedy k-means++ seeding -- the host draws the candidates from the same engine, the device
+// computes the squared distances (bit-identical to the host fit's) and the candidates' potentials
+// -- then the EM iterations on the device (mci_em.hip).  x: host [n][d].
+mci_status gmm_fit_device(const double *x, int64_t n, int d, int K, int em_iters, uint64_t seed, hipStream_t s, double *mu_out,
+                          double *var_out, double *pi_out, double *loglik)
+{
+    using namespace mcie;
+    if (!x || !mu_out || !var_out || !pi_out || n < 1 || d < 1 || d > EM_DMAX || K < 1 || K > EM_KMAX || n < K) {
+        return fail(MCI_E_INVALID_ARGUMENT, "need x, outputs, 1 <= d <= %d, 1 <= K <= %d and n >= K", EM_DMAX, EM_KMAX);
+    }
+    if (em_iters < 0 || em_iters > 1000) { return fail(MCI_E_INVALID_ARGUMENT, "EM iterations must be in [0, 1000]"); }
+    const int64_t nb = em_nblocks(n);
+    const size_t ne = (size_t)K * (1 + 2 * d);
+    DBuf<double> dx, d2, part, dpi, dmu, dvar, dlc;
+    DBuf<int64_t> dcand;
+    HIPCHK(dx.ensure((size_t)n * d));
+    HIPCHK(d2.ensure((size_t)n));
+    HIPCHK(part.ensure((size_t)nb * std::max<size_t>(EM_NCAND, ne)));
+    HIPCHK(dpi.ensure((size_t)K));
+    HIPCHK(dmu.ensure((size_t)K * d));
+    HIPCHK(dvar.ensure((size_t)K * d));
+    HIPCHK(dlc.ensure((size_t)K));
+    HIPCHK(dcand.ensure(EM_NCAND));
+    HIPCHK(hipMemcpyAsync(dx.p, x, sizeof(double) * (size_t)n * d, hipMemcpyHostToDevice, s));
+    // the initial width: the samples' mean squared deviation (the host fit's loops)
+    std::vector<double> mean(d, 0.);
+    double tot = 0.;
+    for (int64_t r = 0; r < n; ++r) {
+        for (int j = 0; j < d; ++j) { mean[j] += x[(size_t)r * d + j] / (double)n; }
+    }
+    for (int64_t r = 0; r < n; ++r) {
+        for (int j = 0; j < d; ++j) { const double t = x[(size_t)r * d + j] - mean[j]; tot += t * t; }
+    }
+    const double v0 = std::max(tot / ((double)n * d), 1e-6);
+    std::mt19937_64 g(seed ^ 0x5DEECE66DULL);
+    std::uniform_real_distribution<double> U(0., 1.);
+    std::vector<double> mu((size_t)K * d), var((size_t)K * d, v0), pi((size_t)K, 1.0 / K), d2h((size_t)n), ph((size_t)nb * EM_NCAND);
+    int64_t pick = (int64_t)(U(g) * (double)n) % n;
+    for (int k = 0; k < K; ++k) {
+        for (int j = 0; j < d; ++j) { mu[(size_t)k * d + j] = x[(size_t)pick * d + j]; }
+        HIPCHK(launch_em_dist(dx.p, n, d, pick, d2.p, k == 0, s));
+        if (k == K - 1) { break; }
+        HIPCHK(hipMemcpyAsync(d2h.data(), d2.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double sum = 0.;
+        for (int64_t r = 0; r < n; ++r) { sum += d2h[(size_t)r]; }
+        // the candidates: the first sample whose running sum of d2 exceeds u sum, for EM_NCAND draws
+        // (one pass over the samples with the draws in increasing order)
+        double u[EM_NCAND];
+        int64_t cand[EM_NCAND];
+        int ord[EM_NCAND];
+        for (int l = 0; l < EM_NCAND; ++l) {
+            u[l] = U(g) * sum;
+            cand[l] = n - 1;
+            ord[l] = l;
+        }
+        std::sort(ord, ord + EM_NCAND, [&](int a, int b) { return u[a] < u[b]; });
+        double c = 0.;
+        int nx = 0;
+        for (int64_t r = 0; r < n && nx < EM_NCAND; ++r) {
+            c += d2h[(size_t)r];
+            while (nx < EM_NCAND && u[ord[nx]] < c) { cand[ord[nx++]] = r; }
+        }
+        HIPCHK(hipMemcpyAsync(dcand.p, cand, sizeof(cand), hipMemcpyHostToDevice, s));
+        HIPCHK(launch_em_potential(dx.p, n, d, d2.p, dcand.p, EM_NCAND, part.p, s));
+        HIPCHK(hipMemcpyAsync(ph.data(), part.p, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double best = HUGE_VAL;
+        for (int l = 0; l < EM_NCAND; ++l) {
+            double pot = 0.;
+            for (int64_t b = 0; b < nb; ++b) { pot += ph[(size_t)b * EM_NCAND + l]; }
+            if (pot < best) {
+                best = pot;
+                pick = cand[l];
+            }
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dmu.p, mu.data(), sizeof(double) * mu.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dvar.p, var.data(), sizeof(double) * var.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dpi.p, pi.data(), sizeof(double) * pi.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_em_lc(dpi.p, dvar.p, K, d, dlc.p, s));
+    for (int it = 0; it < em_iters; ++it) { HIPCHK(launch_em_iteration(dx.p, n, d, K, dpi.p, dmu.p, dvar.p, dlc.p, part.p, s)); }
+    if (loglik) {
+        HIPCHK(launch_em_loglik(dx.p, n, d, K, dmu.p, dvar.p, dlc.p, part.p, s));
+        HIPCHK(hipMemcpyAsync(ph.data(), part.p, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipMemcpyAsync(mu_out, dmu.p, sizeof(double) * mu.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(var_out, dvar.p, sizeof(double) * var.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(pi_out, dpi.p, sizeof(double) * pi.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (loglik) {
+        double ll = 0.;
+        for (int64_t b = 0; b < nb; ++b) { ll += ph[(size_t)b]; }
+        *loglik = ll / (double)n;
+    }
+    return MCI_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -1153,6 +1253,18 @@ mci_status mci_gmm_fit_diag(const double *xin, int64_t n, int d, int K, int em_i
 // log-scales = log sqrt(variances) clamped inside [-3, 3], log-weights), weights scaled by 0.02,
 // so that before training the network's mixture is about that fit at every input.  Training
 // then refines the whole network.
+mci_status mci_gmm_fit_diag_device(const double *x, int64_t n, int d, int K, int em_iters, uint64_t seed, double *mu, double *var,
+                                   double *pi, double *loglik)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { return fail(MCI_E_DEVICE, "no GPU visible"); }
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const mci_status st = gmm_fit_device(x, n, d, K, em_iters, seed, s, mu, var, pi, loglik);
+    (void)hipStreamDestroy(s);
+    return st;
+}
+
 mci_status mci_lp_init_network_from_samples(mci_lp *lp, int em_iters)
 {
     CHECKL(lp);
@@ -1171,7 +1283,7 @@ mci_status mci_lp_init_network_from_samples(mci_lp *lp, int em_iters)
         for (int j = 0; j < d; ++j) { x[(size_t)r * d + j] = last[(size_t)j * W + r]; }
     }
     std::vector<double> mu((size_t)K * d), var((size_t)K * d), pi((size_t)K);
-    TRY(mci_gmm_fit_diag(x.data(), ns, d, K, em_iters, lp->seed, mu.data(), var.data(), pi.data(), nullptr));
+    TRY(gmm_fit_device(x.data(), ns, d, K, em_iters, lp->seed, lp->stream, mu.data(), var.data(), pi.data(), nullptr));
     // into the last layer
     std::vector<float> p((size_t)n->np);
     TRY(mci_mlp_get_params(n, p.data(), n->np));

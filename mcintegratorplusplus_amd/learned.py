@@ -33,6 +33,7 @@ _SIGS = {
     "mci_lp_indep_move": [_P, _I64, _FP, _I, _U64, C.POINTER(C.c_int32), _FP],
     "mci_lp_get_mixture": [_P, _I, C.POINTER(C.c_int), _DP, _DP, _DP],
     "mci_gmm_fit_diag": [_DP, _I64, _I, _I, _I, C.c_uint64, _DP, _DP, _DP, _DP],
+    "mci_gmm_fit_diag_device": [_DP, _I64, _I, _I, _I, C.c_uint64, _DP, _DP, _DP, _DP],
 }
 _bound = None
 
@@ -208,15 +209,16 @@ class LearnedProposalSampler:
         return Network(self.d, self.hidden, self.net_ncomp, _handle=h)
 
 
-def gmm_fit_diag(x, ncomp, em_iters=60, seed=1):
-    """diagonal Gaussian mixture fitted to x [n][d] by EM from greedy k-means++ seeds (host code,
-    the fit the learned proposal's network starts from): (means [K][d], variances [K][d],
-    weights [K], mean log-likelihood)"""
+def gmm_fit_diag(x, ncomp, em_iters=60, seed=1, device=False):
+    """diagonal Gaussian mixture fitted to x [n][d] by EM from greedy k-means++ seeds: (means [K][d],
+    variances [K][d], weights [K], mean log-likelihood).  device=False: host code; device=True: the
+    same fit with its sample sums on the GPU (the one the learned proposal's network starts from)"""
     x = _d(x)
     n, d = x.shape
     mu, var, pi, ll = np.zeros((ncomp, d)), np.zeros((ncomp, d)), np.zeros(ncomp), C.c_double()
-    _check(_lib().mci_gmm_fit_diag(x.ctypes.data_as(_DP), n, d, ncomp, em_iters, seed, mu.ctypes.data_as(_DP),
-                                   var.ctypes.data_as(_DP), pi.ctypes.data_as(_DP), C.byref(ll)))
+    fn = _lib().mci_gmm_fit_diag_device if device else _lib().mci_gmm_fit_diag
+    _check(fn(x.ctypes.data_as(_DP), n, d, ncomp, em_iters, seed, mu.ctypes.data_as(_DP), var.ctypes.data_as(_DP),
+              pi.ctypes.data_as(_DP), C.byref(ll)))
     return mu, var, pi, ll.value
 
 

@@ -278,3 +278,31 @@ def test_trained_mixture_as_the_trial_move_of_a_context():
     print("z with the learned half", z, "without", z0, flush=True)
     assert np.all(np.abs(z) < 4), z
     assert abs(z0[0]) > 20, z0
+
+
+@pytest.mark.parametrize("n,d,K,iters", [(16384, 32, 8, 60), (5000, 8, 3, 40), (999, 5, 1, 1), (300, 3, 3, 0)])
+def test_device_em_fit_matches_the_host_fit(n, d, K, iters):
+    """The mixture fit the network starts from, on the device (seeding distances and potentials, EM
+    in FP64 kernels): the same seeds as the host fit (the squared distances are bit-identical, so
+    the candidates are the same draws) and, after the EM iterations, the same mixture to rounding
+    (sums over samples in another order); deterministic from run to run."""
+    from mcintegratorplusplus_amd.learned import gmm_fit_diag, mixture_target
+    print("device EM", n, d, K, iters, flush=True)
+    if d == 32:
+        w, mu, cov, _ = mixture_target(d=32, ncomp=8, seed=5)
+        rng = np.random.default_rng(1)
+        k = rng.choice(len(w), n, p=w)
+        L = np.linalg.cholesky(cov)
+        x = mu[k] + np.einsum("nij,nj->ni", L[k], rng.normal(size=(n, d)))
+    else:
+        x = np.random.default_rng(d).normal(size=(n, d)) * np.arange(1, d + 1) + np.repeat(np.arange(3), -(-n // 3))[:n, None] * 6.0
+    host = gmm_fit_diag(x, K, iters, seed=3)
+    dev = gmm_fit_diag(x, K, iters, seed=3, device=True)
+    dev2 = gmm_fit_diag(x, K, iters, seed=3, device=True)
+    for a, b in zip(dev[:3], dev2[:3]):
+        assert np.array_equal(a, b)
+    if iters == 0:  # the seeds themselves: samples, equal bit for bit
+        assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
+    for a, b, tol in zip(host[:3], dev[:3], (1e-9, 1e-8, 1e-9)):
+        assert np.allclose(b, a, rtol=tol, atol=tol * np.max(np.abs(a))), np.max(np.abs(b - a))
+    assert dev[3] == pytest.approx(host[3], rel=1e-11)
