@@ -184,6 +184,45 @@ def run_full_traces(nmc=8192):
             "walker_steps_per_s": W * nmc / (ms / 1000), "max_abs_z": float(np.max(np.abs(z))), "mean_z2": float(np.mean(z * z))}
 
 
+def run_reference_streams():
+    """The benchmark workload on the reference's random streams (one mt19937_64 per walker, the
+    library's default, which reproduces the reference's chains bit for bit), on the specialised
+    kernel and, for comparison, on the generic one: whole integrate and sampling kernel.  The
+    reference-stream kernel's traffic is the walkers' generator states: three outputs per step,
+    each reading two state words and writing one (24 B)."""
+    from mcintegratorplusplus_amd import MCI, constants as K
+    sig = sigmas()
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    out = {"workload": f"{W} walkers x {NMC} steps, d={D}, as the benchmark, mt19937_64 streams (MCI_RNG_MT19937_64)"}
+    for name, path in (("specialised", K.KERNELS_AUTO), ("generic", K.KERNELS_GENERIC)):
+        m = MCI(D, W)
+        m.setSeed(SEED)
+        m.setRandomMode(K.RNG_MT19937_64)
+        m.setKernelPath(path)
+        m.setWalkerCoupling(K.COUPLING_SHARED)
+        m.setTrialMove(K.SRRD_UNIFORM, 1, 1)
+        m.setNdecorrelationSteps(NDECORR)
+        m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+        m.addObservable(K.OBS_X2, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+        m.addObservable(K.OBS_X2_SUM, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+        if name == "specialised":
+            m.integrate(NMC)
+        t0 = time.perf_counter()
+        avg, err = m.integrate(NMC)
+        ms = 1000 * (time.perf_counter() - t0)
+        kt = m.getTimings()
+        wa, _ = m.walkerResults()
+        z = (avg - exact) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
+        out[name] = {"integrate_ms": ms, "walker_steps_per_s": W * NMC / (ms / 1000), "sampling_kernel_ms": kt[2],
+                     "max_abs_z": float(np.max(np.abs(z)))}
+        if name == "specialised":
+            out[name]["generator_state_gbps"] = 3 * 24 * W * NMC / (kt[2] / 1000) / 1e9
+            out[name]["avg"] = avg
+    out["speedup_specialised_vs_generic"] = out["generic"]["integrate_ms"] / out["specialised"]["integrate_ms"]
+    out["specialised"].pop("avg")
+    return out
+
+
 def profile_summary():
     """HBM bytes per launch of the sampling kernel (FETCH_SIZE + WRITE_SIZE, each scaled by a
     1 GiB copy probe) and its VALU-busy fraction, from the committed rocprofv3 summary."""
@@ -477,7 +516,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
-    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full"],
+    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt"],
                     help="also measure the precision form, the batched integrals and the learned proposal (all), "
                          "none of them, or one of them")
     ap.add_argument("--copy-probe", action="store_true",
@@ -561,7 +600,8 @@ def main():
     }
     if args.workloads != "none" and args.precision == 0 and not args.generic:
         runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
-                "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces)}
+                "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces),
+                "reference_streams": ("mt", run_reference_streams)}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
             if args.workloads in ("all", name):

@@ -331,7 +331,7 @@ float elapsed(mci_ctx *c, int a, int b)
 bool fast_chain(const mci_ctx *c)
 {
     return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && c->lm_K == 0 && !c->cb && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
-                         c->pdfs.empty() ? 0 : c->pdfs[0].kind) &&
+                         c->pdfs.empty() ? 0 : c->pdfs[0].kind, c->nvecs) &&
            !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
 }
 

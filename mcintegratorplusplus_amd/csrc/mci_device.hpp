@@ -117,6 +117,22 @@ struct Lane
 // uses x[k+1] (not yet twisted this round) and x[(k+156) % 312] (twisted iff k >= 156),
 // exactly the values the batch twist of libstdc++ (bits/random.tcc) uses, so the
 // output stream is identical.  State index 0 after seeding = batch index 312.
+// the new value of element k from x[k], x[k+1] and x[(k+156) % 312] as the twist reads them
+__device__ __forceinline__ uint64_t mt_twist(uint64_t xk, uint64_t xk1, uint64_t xkm)
+{
+    const uint64_t y = (xk & 0xffffffff80000000ULL) | (xk1 & 0x7fffffffULL);
+    return xkm ^ (y >> 1) ^ ((y & 1ULL) ? 0xb5026f5aa96619e9ULL : 0ULL);
+}
+
+__device__ __forceinline__ uint64_t mt_temper(uint64_t z)
+{
+    z ^= (z >> 29) & 0x5555555555555555ULL;
+    z ^= (z << 17) & 0x71d67fffeda60000ULL;
+    z ^= (z << 37) & 0xfff7eee000000000ULL;
+    z ^= (z >> 43);
+    return z;
+}
+
 __device__ __forceinline__ uint64_t mt_next(const Chain &c, Lane &L)
 {
     const int64_t W = c.W;
@@ -126,16 +142,10 @@ __device__ __forceinline__ uint64_t mt_next(const Chain &c, Lane &L)
     const uint64_t xk = c.mt[(int64_t)k * W + L.w];
     const uint64_t xk1 = c.mt[(int64_t)k1 * W + L.w];
     const uint64_t xkm = c.mt[(int64_t)km * W + L.w];
-    const uint64_t y = (xk & 0xffffffff80000000ULL) | (xk1 & 0x7fffffffULL);
-    const uint64_t nv = xkm ^ (y >> 1) ^ ((y & 1ULL) ? 0xb5026f5aa96619e9ULL : 0ULL);
+    const uint64_t nv = mt_twist(xk, xk1, xkm);
     c.mt[(int64_t)k * W + L.w] = nv;
     L.mtp = k1;
-    uint64_t z = nv;
-    z ^= (z >> 29) & 0x5555555555555555ULL;
-    z ^= (z << 17) & 0x71d67fffeda60000ULL;
-    z ^= (z << 37) & 0xfff7eee000000000ULL;
-    z ^= (z >> 43);
-    return z;
+    return mt_temper(nv);
 }
 
 // Philox4x32-10 (Salmon et al., SC'11)

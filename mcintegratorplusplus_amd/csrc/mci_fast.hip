@@ -63,6 +63,8 @@ struct FastArgs
     float uexp;            // FP64: n, the screening exponent's offset
     const CalibCtrl *pend; // shared calibration: pending step factor, loop finished (FP64 kernel only)
     const double *dimsizes;
+    uint64_t *mt;          // reference streams: [312][W] mt19937_64 states (F_MT launches)
+    int32_t *mtp;          // [W] next state index
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
@@ -82,6 +84,10 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     }
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
+
+// a or b by a lane's flag as arithmetic on the values: a select of two elements of a register array
+// can otherwise become a select of their addresses, which puts the array in scratch memory
+__device__ __forceinline__ uint32_t pick(bool on, uint32_t a, uint32_t b) { return b ^ ((a ^ b) & (0u - (uint32_t)on)); }
 
 // Lemire index on [0, range) from word w0 (SRRDVecMove.hpp:78 through uniform_int_distribution), for
 // a range whose index the shared word does not hold: low = (w0 * range) mod 2^32 < thr = 2^32 mod
@@ -107,7 +113,8 @@ __device__ __forceinline__ int draw_lemire(uint32_t w0, uint32_t range, uint32_t
 // observable set of one launch (template flags: no per-step branches on them)
 // F_P2: the index is the shared word's top bits (a power-of-two range); F_FULL: Full accumulators (every
 // step's values stored)
-enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8, F_FULL = 16 };
+// F_MT: the reference streams (mt19937_64 per walker) instead of the native ones
+enum { F_X2 = 1, F_SUM = 2, F_EXP = 4, F_P2 = 8, F_FULL = 16, F_MT = 32 };
 
 // FP64 chain.  One wave runs alone on its SIMD at the benchmark size (65,536 walkers =
 // 1,024 waves), so a step costs about 4 cycles per instruction of any kind and 16 per
@@ -147,21 +154,33 @@ struct Run64
     uint32_t na;  // accepted steps of the chunk
 };
 
+// canonical53 of the reference streams (generate_canonical<double, 53> on mt19937_64): w / 2^64,
+// clamped below 1 (as in mci_device.hpp)
+__device__ __forceinline__ double mt_canonical(uint64_t w)
+{
+    double r = (double)w / 18446744073709551616.0;
+    if (r >= 1.0) { r = 0x1.fffffffffffffp-1; }
+    return r;
+}
+
 // One step on coordinate v (LDS slot `slot`) from its value xo and block sum So, which the caller
-// read and which hold the values after the step on return.  w0 is the step's shared word, w1 its
-// proposal word, t its counter (the exact test's side-stream word is keyed by it), row its index
-// in the run (Full traces).
+// read and which hold the values after the step on return.  Native streams: w0 is the step's
+// shared word, w1 its proposal word, t its counter (the exact test's side-stream word is keyed by
+// it).  Reference streams (F_MT): w1 is the proposal's output and w0 the acceptance uniform's.  row:
+// the step's index in the run (Full traces).
 template <int FL>
-__device__ __forceinline__ void step64(const FastArgs &A, double2 *rec, double *xs, const double *as, Run64 &R, int v, int slot,
-                                       double &xo, double &So, float cv, uint32_t w0, uint32_t w1, uint64_t t, int64_t row,
+__device__ __forceinline__ bool step64(const FastArgs &A, double2 *rec, double *xs, const double *as, Run64 &R, int v, int slot,
+                                       double &xo, double &So, float cv, uint64_t w0, uint64_t w1, uint64_t t, int64_t row,
                                        uint32_t wid, double step, int lane, int64_t w)
 {
     constexpr bool FULL = (FL & F_FULL) != 0;
     constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
-    constexpr bool ACC = (FL & 7) != 0 && !FULL;
+    constexpr bool MT = (FL & F_MT) != 0;
+    constexpr bool ACC = (FL & 7) != 0 && !FULL && !MT;  // dwell-time sums (the reference streams sum per step)
     constexpr bool REC = HX2 && !FULL;
-    // proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32): exact
-    const double rr = fma((double)w1, 0x1p-31, -(1.0 - 0x1p-32));
+    // native proposal (2 w1 + 1 - 2^32) 2^-32 = w1 2^-31 - (1 - 2^-32), one exact fused
+    // multiply-add; reference streams: uniform_real_distribution(-1, 1) as u 2 + (-1)
+    const double rr = MT ? (mt_canonical(w1) * 2.0) + (-1.0) : fma((double)(uint32_t)w1, 0x1p-31, -(1.0 - 0x1p-32));
     const double xn = xo + step * rr;
     const double xn2 = xn * xn, xo2 = xo * xo;
     const double dx2n = xn2 - xo2;  // the change of x_v^2 if accepted
@@ -171,10 +190,14 @@ __device__ __forceinline__ void step64(const FastArgs &A, double2 *rec, double *
     // F >= 1 (the FP64 rounding of ex is far smaller), and U rounded to FP32.  U < 0.9999 F - 2
     // makes the FP64 test certain to accept, U > 1.0001 F certain to reject; between the two the
     // FP64 test decides.
-    const float U = (float)(w0 & A.umask);
-    const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, A.uexp));
-    const bool acc = U < fmaf(F, 0.9999f, -2.0f);
-    const bool band = !acc && U <= F * 1.0001f;
+    // Reference streams: u = w0 / 2^64 rounded to 53 bits lies in [(U - 1) 2^-32, (U + 2) 2^-32)
+    // for U the top word of w0, hence the margins 3 and 1.
+    const float U = MT ? (float)(uint32_t)(w0 >> 32) : (float)((uint32_t)w0 & A.umask);
+    const float F = __builtin_amdgcn_exp2f(fmaf((float)dx2n, cv, MT ? 32.0f : A.uexp));
+    const float Facc = MT ? fmaf(F, 0.9999f, -3.0f) : fmaf(F, 0.9999f, -2.0f);
+    const float Frej = MT ? fmaf(F, 1.0001f, 1.0f) : F * 1.0001f;
+    const bool acc = U < Facc;
+    const bool band = !acc && U <= Frej;
     if (acc) { ++R.na; }
     bool moved = acc;
     double xsel = acc ? xn : xo, Snew = So;
@@ -189,16 +212,21 @@ __device__ __forceinline__ void step64(const FastArgs &A, double2 *rec, double *
         if (!(HX2 && HSUM)) { R.Sr = fma(dx2, R.mult, R.Sr); }
         if (HEXP && acc) { const double en = exp(-(R.r / A.exp_div)); R.Se = R.Se + (en - R.e) * R.mult; R.e = en; }
     }
+    else if (REC) { rec[slot] = make_double2(xsel, So); }  // the reference streams' per-step sums: the caller adds
     else { xs[slot] = xsel; }
     // Lanes inside the band went through the update above as rejections (which leaves every number
     // as it was: +0.0 added to non-negative sums).  When the wave holds any, the exact test runs and
     // the lanes it accepts are updated as accepted steps, from the values read at the start of the
     // step.  (two ballots of plain compares: each is the compare's own lane mask)
-    if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= F * 1.0001f) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
-        uint32_t l0, l1, l2, l3;
-        philox(0x30000000u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, l0, l1, l2, l3);
-        const uint64_t uw = (((uint64_t)w0 << 32) | l0) << A.ubits;
-        const double u = (double)(uw >> 11) * 0x1p-53;
+    if (__builtin_expect((__builtin_amdgcn_ballot_w64(U <= Frej) & ~__builtin_amdgcn_ballot_w64(acc)) != 0, 0)) {
+        double u;
+        if (MT) { u = (mt_canonical(w0) * 1.0) + 0.0; }
+        else {
+            uint32_t l0, l1, l2, l3;
+            philox(0x30000000u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, l0, l1, l2, l3);
+            const uint64_t uw = (((uint64_t)(uint32_t)w0 << 32) | l0) << A.ubits;
+            u = (double)(uw >> 11) * 0x1p-53;
+        }
         const double av = as[v];
         const double ex = xn2 * av - xo2 * av;
         if (band && u <= exp(-ex)) {
@@ -213,6 +241,7 @@ __device__ __forceinline__ void step64(const FastArgs &A, double2 *rec, double *
                 if (!(HX2 && HSUM)) { R.Sr = fma(dx2n, R.mult, R.Sr); }
                 if (HEXP) { const double en = exp(-(R.r / A.exp_div)); R.Se = R.Se + (en - R.e) * R.mult; R.e = en; }
             }
+            else if (REC) { rec[slot] = make_double2(xn, So); }
             else { xs[slot] = xn; }
         }
     }
@@ -234,6 +263,7 @@ __device__ __forceinline__ void step64(const FastArgs &A, double2 *rec, double *
     }
     xo = xsel;
     So = Snew;
+    return moved;
 }
 
 // Steps t and t + 1 (t even) of k_fast64: c = blocks 0 (and 1 without a shared index) of their
@@ -283,9 +313,10 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
 {
     constexpr bool FULL = (FL & F_FULL) != 0;
     constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
-    constexpr bool ACC = (FL & 7) != 0 && !FULL;  // dwell-time block sums
+    constexpr bool ACC = (FL & 7) != 0 && !FULL;  // block sums (dwell time; per step with MT)
     constexpr bool REC = HX2 && !FULL;            // 16-byte {x_i, block sum} records
     constexpr bool P2 = (FL & F_P2) != 0;         // the index is the shared word's top bits
+    constexpr bool MT = (FL & F_MT) != 0;         // reference streams
     extern __shared__ double lds[];
     const int d = A.ndim;
     const int lane = threadIdx.x;
@@ -317,7 +348,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     if (live) {
         for (int i = 0; i < d; ++i) {
             const double v = A.x64[(int64_t)i * A.W + w];
-            if (REC) { rec[i * WAVE + lane] = make_double2(v, (double)B * (v * v)); }
+            if (REC) { rec[i * WAVE + lane] = make_double2(v, MT ? 0.0 : (double)B * (v * v)); }
             else { xs[i * WAVE + lane] = v; }
             R.r = R.r + v * v;
         }
@@ -325,101 +356,201 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     __syncthreads();
     if (!live) { return; }
     const double step = A.steps[w];
-    const uint32_t wid = (uint32_t)w;
-    const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
-    uint64_t t = A.ctr[w];
+    [[maybe_unused]] const uint32_t wid = (uint32_t)w;
+    [[maybe_unused]] const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
+    [[maybe_unused]] uint64_t t = MT ? 0 : A.ctr[w];
     int64_t nacc = 0;
     int64_t b = 0;
-    if (ACC) { R.Sr = (double)B * R.r; }
-    if (HEXP && ACC) { R.e = exp(-(R.r / A.exp_div)); R.Se = (double)B * R.e; }
+    if (ACC && !MT) { R.Sr = (double)B * R.r; }
+    if (HEXP && ACC) { R.e = exp(-(R.r / A.exp_div)); R.Se = MT ? 0.0 : (double)B * R.e; }
     if (HEXP && FULL) {
         double q0 = 0.0;
         for (int i = 0; i < d; ++i) { const double v = xs[i * WAVE + lane]; q0 = q0 + v * v; }
         R.e = exp(-(q0 / A.exp_div));
     }
-    // Blocks 0 and (without a shared index) 1 of the current step pair t / 2, as pair64 takes them:
-    // p[0], p[1] are step t's shared and next word when t is even, p[2], p[3] when t is odd; p[4] /
-    // p[5] the word after (block 1's words 0 / 2).
-    uint32_t p[6] = {0u, 0u, 0u, 0u, 0u, 0u}, nx[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-    uint32_t g1, g3;
-    philox(0u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[0], p[1], p[2], p[3]);
-    if (!P2) { philox(1u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[4], g1, p[5], g3); }
-    // The lanes of a wave step in pairs when their counters agree in parity (always, unless the
-    // counters were set so); otherwise every step runs alone, each lane on its own parity.
-    const uint64_t podd = __builtin_amdgcn_ballot_w64(((uint32_t)t & 1u) != 0);
-    const bool uni = podd == 0 || podd == __builtin_amdgcn_ballot_w64(true);
-    // Blocks start at step 0.  The steps run in chunks of at most 2^30 that never cross a block
-    // end, so the inner loop's counters are 32-bit whatever the block length (a Simple
-    // accumulator's block is the whole run).
     int64_t inblk = 0;  // steps done in the current block
-    for (int64_t s = 0; s < A.nsteps;) {
-        const int len = fast_chunk_len(A.nsteps, s, B, inblk, ACC);
-        R.na = 0;
-        // one step alone, on the lane's own parity
-        auto single = [&](int q) {
-            const bool odd = ((uint32_t)t & 1u) != 0;
-            const uint32_t ws = odd ? p[2] : p[0], wn = odd ? p[3] : p[1];
-            int v;
-            uint32_t wp;
-            if (P2) { v = (int)__builtin_amdgcn_ubfe(ws, A.vshift, A.ibits); wp = wn; }
-            else {
-                v = draw_lemire(wn, range, thr, wid, t, A.key0, A.key1);
-                wp = odd ? p[5] : p[4];
+    // a block's end: its means out, |x|^2 recomputed exactly, the sums restarted
+    auto flush_block = [&]() {
+        const double norm = 1. / (double)B;
+        double rx = 0.0, sS = 0.0;
+        for (int i = 0; i < d; ++i) {
+            double v2;
+            if (REC) {
+                const double2 rv = rec[i * WAVE + lane];
+                v2 = rv.x;
+                sS = sS + rv.y;
+                A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
+                rec[i * WAVE + lane] = make_double2(v2, (double)B * (v2 * v2));
             }
-            const int sl = v * WAVE + lane;
-            double xo, So = 0.0;
-            if (REC) { const double2 rv = rec[sl]; xo = rv.x; So = rv.y; }
-            else { xo = xs[sl]; }
-            step64<FL>(A, rec, xs, as, R, v, sl, xo, So, cs[v], ws, wp, t, s + q, wid, step, lane, w);
-            ++t;
-            if (odd) {
-                const uint64_t tn = t >> 1;
-                philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[0], p[1], p[2], p[3]);
-                if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[4], g1, p[5], g3); }
+            else { v2 = xs[i * WAVE + lane]; }
+            rx = rx + v2 * v2;
+        }
+        if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : R.Sr) * norm) / A.x2sum_div; }
+        if (HEXP) { A.d_exp[b * A.W + w] = R.Se * norm; }
+        R.r = rx;  // refresh |x|^2 exactly at every block boundary
+        R.Sr = (double)B * R.r;
+        if (HEXP) { R.e = exp(-(R.r / A.exp_div)); R.Se = (double)B * R.e; }
+        R.mult = (double)B;
+        inblk = 0;
+        ++b;
+    };
+    if constexpr (MT) {
+        // Reference streams: three outputs of the walker's mt19937_64 per step (the index, the
+        // proposal, the acceptance uniform; a power-of-two index never rejects).  The lazy twist
+        // of mci_device.hpp's mt_next, in groups of MT_G steps: the loads of a group's state words
+        // are issued a group ahead, so they are in flight while the group before runs.
+        constexpr int MT_G = 4, NO = 3 * MT_G;
+        const int64_t W = A.W;
+        uint64_t *mt = A.mt;
+        int k = A.mtp[w];
+        uint64_t xk = mt[(int64_t)k * W + w];   // the current value of element k
+        uint64_t x1[NO], xm[NO], o[NO];
+        auto issue = [&](int kk) {
+#pragma unroll
+            for (int j = 0; j < NO; ++j) {
+                const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
+                const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
+                x1[j] = mt[(int64_t)k1 * W + w];
+                xm[j] = mt[(int64_t)km * W + w];
+                kk = k1;
             }
         };
-        int q = 0;
-        if (uni) {
-            // an odd counter first takes one step; then groups of four steps (two pairs, the blocks
-            // alternating between p and nx), then the rest alone
-            if (((uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t) & 1u) != 0u && q < len) { single(q); ++q; }
-            for (; q + 4 <= len; q += 4) {
-                pair64<FL>(A, rec, xs, as, cs, R, p, nx, t, s + q, wid, step, lane, w);
-                pair64<FL>(A, rec, xs, as, cs, R, nx, p, t + 2, s + q + 2, wid, step, lane, w);
-                t += 4;
-            }
-        }
-        for (; q < len; ++q) { single(q); }
-        nacc += R.na;
-        s += len;
-        inblk += len;
-        if (ACC && inblk == B) {
-            const double norm = 1. / (double)B;
-            double rx = 0.0, sS = 0.0;
-            for (int i = 0; i < d; ++i) {
-                double v2;
-                if (REC) {
-                    const double2 rv = rec[i * WAVE + lane];
-                    v2 = rv.x;
-                    sS = sS + rv.y;
-                    A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
-                    rec[i * WAVE + lane] = make_double2(v2, (double)B * (v2 * v2));
+        issue(k);
+        for (int64_t s = 0; s < A.nsteps; s += MT_G) {
+            const int ng = (int)(A.nsteps - s < MT_G ? A.nsteps - s : MT_G);
+            // this group's outputs, exactly the 3 ng it consumes (the state advances by those)
+#pragma unroll
+            for (int j = 0; j < NO; ++j) {
+                if (j < 3 * ng) {
+                    const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
+                    mt[(int64_t)k * W + w] = nv;
+                    o[j] = mt_temper(nv);
+                    k = (k + 1 == MT_N) ? 0 : k + 1;
                 }
-                else { v2 = xs[i * WAVE + lane]; }
-                rx = rx + v2 * v2;
             }
-            if (HSUM) { A.d_x2sum[b * A.W + w] = ((HX2 ? sS : R.Sr) * norm) / A.x2sum_div; }
-            if (HEXP) { A.d_exp[b * A.W + w] = R.Se * norm; }
-            R.r = rx;  // refresh |x|^2 exactly at every block boundary
-            R.Sr = (double)B * R.r;
-            if (HEXP) { R.e = exp(-(R.r / A.exp_div)); R.Se = (double)B * R.e; }
-            R.mult = (double)B;
-            inblk = 0;
-            ++b;
+#pragma unroll
+            for (int j = 0; j < NO; ++j) {
+                if (j == 3 * ng - 1) { xk = x1[j]; }
+            }
+            if (s + MT_G < A.nsteps) { issue(k); }
+            R.na = 0;
+#pragma unroll
+            for (int j = 0; j < MT_G; ++j) {
+                if (j < ng) {
+                    const int v = A.ibits > 0 ? (int)(o[3 * j] >> (64 - A.ibits)) : 0;
+                    const int sl = v * WAVE + lane;
+                    double xo, So = 0.0;
+                    if (REC) { const double2 rv = rec[sl]; xo = rv.x; So = rv.y; }
+                    else { xo = xs[sl]; }
+                    const bool moved = step64<FL>(A, rec, xs, as, R, v, sl, xo, So, cs[v], o[3 * j + 2], o[3 * j + 1], 0, s + j, 0u,
+                                                  step, lane, w);
+                    if (ACC) {
+                        // the reference's per-step sums (BlockAccumulator / SimpleAccumulator): every
+                        // component's value after the step added in step order, so the block means
+                        // are bit-identical to the reference's
+                        double q = 0.0;
+                        for (int i = 0; i < d; ++i) {
+                            if (REC) {
+                                double2 rv = rec[i * WAVE + lane];
+                                const double q2 = rv.x * rv.x;
+                                q = q + q2;
+                                rv.y = rv.y + q2;
+                                rec[i * WAVE + lane] = rv;
+                            }
+                            else {
+                                const double xv = xs[i * WAVE + lane];
+                                q = q + xv * xv;
+                            }
+                        }
+                        if (HSUM) { R.Sr = R.Sr + q / A.x2sum_div; }
+                        if (HEXP) {
+                            if (moved) { R.e = exp(-(q / A.exp_div)); }
+                            R.Se = R.Se + R.e;
+                        }
+                        if (++inblk == B) {
+                            const double norm = 1. / (double)B;
+                            if (REC) {
+                                for (int i = 0; i < d; ++i) {
+                                    const double2 rv = rec[i * WAVE + lane];
+                                    A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
+                                    rec[i * WAVE + lane] = make_double2(rv.x, 0.0);
+                                }
+                            }
+                            if (HSUM) { A.d_x2sum[b * A.W + w] = R.Sr * norm; }
+                            if (HEXP) { A.d_exp[b * A.W + w] = R.Se * norm; }
+                            R.Sr = 0.0;
+                            R.Se = 0.0;
+                            inblk = 0;
+                            ++b;
+                        }
+                    }
+                }
+            }
+            nacc += R.na;
         }
+        A.mtp[w] = k;
+    }
+    else {
+        // Blocks 0 and (without a shared index) 1 of the current step pair t / 2, as pair64 takes them:
+        // p[0], p[1] are step t's shared and next word when t is even, p[2], p[3] when t is odd; p[4] /
+        // p[5] the word after (block 1's words 0 / 2).
+        uint32_t p[6] = {0u, 0u, 0u, 0u, 0u, 0u}, nx[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+        uint32_t g1, g3;
+        philox(0u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[0], p[1], p[2], p[3]);
+        if (!P2) { philox(1u, wid, (uint32_t)(t >> 1), (uint32_t)(t >> 33), A.key0, A.key1, p[4], g1, p[5], g3); }
+        // The lanes of a wave step in pairs when their counters agree in parity (always, unless the
+        // counters were set so); otherwise every step runs alone, each lane on its own parity.
+        const uint64_t podd = __builtin_amdgcn_ballot_w64(((uint32_t)t & 1u) != 0);
+        const bool uni = podd == 0 || podd == __builtin_amdgcn_ballot_w64(true);
+        // Blocks start at step 0.  The steps run in chunks of at most 2^30 that never cross a block
+        // end, so the inner loop's counters are 32-bit whatever the block length (a Simple
+        // accumulator's block is the whole run).
+        for (int64_t s = 0; s < A.nsteps;) {
+            const int len = fast_chunk_len(A.nsteps, s, B, inblk, ACC);
+            R.na = 0;
+            // one step alone, on the lane's own parity
+            auto single = [&](int q) {
+                const bool odd = ((uint32_t)t & 1u) != 0;
+                const uint32_t ws = pick(odd, p[2], p[0]), wn = pick(odd, p[3], p[1]);
+                int v;
+                uint32_t wp;
+                if (P2) { v = (int)__builtin_amdgcn_ubfe(ws, A.vshift, A.ibits); wp = wn; }
+                else {
+                    v = draw_lemire(wn, range, thr, wid, t, A.key0, A.key1);
+                    wp = pick(odd, p[5], p[4]);
+                }
+                const int sl = v * WAVE + lane;
+                double xo, So = 0.0;
+                if (REC) { const double2 rv = rec[sl]; xo = rv.x; So = rv.y; }
+                else { xo = xs[sl]; }
+                step64<FL>(A, rec, xs, as, R, v, sl, xo, So, cs[v], ws, wp, t, s + q, wid, step, lane, w);
+                ++t;
+                if (odd) {
+                    const uint64_t tn = t >> 1;
+                    philox(0u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[0], p[1], p[2], p[3]);
+                    if (!P2) { philox(1u, wid, (uint32_t)tn, (uint32_t)(tn >> 32), A.key0, A.key1, p[4], g1, p[5], g3); }
+                }
+            };
+            int q = 0;
+            if (uni) {
+                // an odd counter first takes one step; then groups of four steps (two pairs, the blocks
+                // alternating between p and nx), then the rest alone
+                if (((uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t) & 1u) != 0u && q < len) { single(q); ++q; }
+                for (; q + 4 <= len; q += 4) {
+                    pair64<FL>(A, rec, xs, as, cs, R, p, nx, t, s + q, wid, step, lane, w);
+                    pair64<FL>(A, rec, xs, as, cs, R, nx, p, t + 2, s + q + 2, wid, step, lane, w);
+                    t += 4;
+                }
+            }
+            for (; q < len; ++q) { single(q); }
+            nacc += R.na;
+            s += len;
+            inblk += len;
+            if (ACC && inblk == B) { flush_block(); }
+        }
+        A.ctr[w] = t;
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * A.W + w] = REC ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; }
-    A.ctr[w] = t;
     A.acc[w] = nacc;
     A.rej[w] = A.nsteps - nacc;
 }
@@ -640,7 +771,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
         const int len = fast32_chunk_len(A.nsteps, s, B, j, ACC);
         auto single = [&](int q) {
             const bool odd = ((uint32_t)t & 1u) != 0;
-            const uint32_t ws = odd ? p[2] : p[0], wn = odd ? p[3] : p[1];
+            const uint32_t ws = pick(odd, p[2], p[0]), wn = pick(odd, p[3], p[1]);
             int v;
             uint32_t wp, ub;
             if (P2) {
@@ -650,7 +781,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
             }
             else {
                 v = draw_lemire(wn, range, thr, wid, t, A.key0, A.key1);
-                wp = odd ? p[5] : p[4];
+                wp = pick(odd, p[5], p[4]);
                 ub = ws >> 8;
             }
             const int slot = v * WAVE + lane;
@@ -747,15 +878,16 @@ template <int FL>
 void launch_fl(int prec, dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
 {
     if (prec == MCI_PREC_FP64) { hipLaunchKernelGGL(k_fast64<FL>, grid, dim3(WAVE), sh, s, A); }
-    else if constexpr ((FL & F_FULL) == 0) { hipLaunchKernelGGL(k_fast32<FL>, grid, dim3(WAVE), sh, s, A); }
+    else if constexpr ((FL & (F_FULL | F_MT)) == 0) { hipLaunchKernelGGL(k_fast32<FL>, grid, dim3(WAVE), sh, s, A); }
 }
 } // namespace
 
-bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind)
+bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind, int nvecs)
 {
-    (void)prec;
-    return rng == MCI_RNG_PHILOX && movetype == MCI_MOVE_VEC && veclen == 1 && ntypes == 1 && !periodic && npdf == 1 &&
-           pdfkind == MCI_PDF_GAUSS_DIAG;
+    // the reference streams: the FP64 chain, and a power-of-two coordinate count, whose index draw
+    // (uniform_int_distribution, Lemire on 64-bit words) never rejects, so a step takes three outputs
+    const bool streams = rng == MCI_RNG_PHILOX || (rng == MCI_RNG_MT19937_64 && prec == MCI_PREC_FP64 && nvecs >= 1 && (nvecs & (nvecs - 1)) == 0);
+    return streams && movetype == MCI_MOVE_VEC && veclen == 1 && ntypes == 1 && !periodic && npdf == 1 && pdfkind == MCI_PDF_GAUSS_DIAG;
 }
 
 bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
@@ -799,8 +931,16 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     A.rej = c.rej;
     A.nvecs = c.nvecs;
     A.lemire_thr = (0u - (uint32_t)c.nvecs) % (uint32_t)c.nvecs;
-    // the native stream's shared word: the index's bits (FP32: at most 8), then the acceptance bits
+    // the native stream's shared word: the index's bits (FP32: at most 8), then the acceptance bits;
+    // the reference streams take the index from the top log2(nvecs) bits of its own output
+    const bool mt = c.rng == MCI_RNG_MT19937_64;
     A.ibits = prec == MCI_PREC_FP64 ? c.ibits : shared_index_bits32(c.ibits);
+    if (mt) {
+        A.ibits = 0;
+        while ((1 << A.ibits) < c.nvecs) { ++A.ibits; }
+    }
+    A.mt = c.mt;
+    A.mtp = c.mtp;
     A.vshift = A.ibits >= 0 ? 32 - A.ibits : 0;
     A.ubits = A.ibits > 0 ? (uint32_t)A.ibits : 0u;
     A.umask = 0xFFFFFFFFu >> A.ubits;
@@ -825,6 +965,10 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
     const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0 && !(fl & F_FULL));
     if (A.ibits >= 0) { fl |= F_P2; }
+    if (mt) {
+        if (prec != MCI_PREC_FP64 || (c.nvecs & (c.nvecs - 1)) != 0 || !c.mt || !c.mtp) { return MCI_E_INVALID_ARGUMENT; }
+        fl |= F_MT;
+    }
     switch (fl) {
     case 0: launch_fl<0>(prec, grid, sh, s, A); break;
     case 1: launch_fl<1>(prec, grid, sh, s, A); break;
@@ -857,6 +1001,22 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     case 29: launch_fl<29>(prec, grid, sh, s, A); break;
     case 30: launch_fl<30>(prec, grid, sh, s, A); break;
     case 31: launch_fl<31>(prec, grid, sh, s, A); break;
+    // reference streams (FP64; the index is always a shift)
+    case 40: launch_fl<40>(prec, grid, sh, s, A); break;
+    case 41: launch_fl<41>(prec, grid, sh, s, A); break;
+    case 42: launch_fl<42>(prec, grid, sh, s, A); break;
+    case 43: launch_fl<43>(prec, grid, sh, s, A); break;
+    case 44: launch_fl<44>(prec, grid, sh, s, A); break;
+    case 45: launch_fl<45>(prec, grid, sh, s, A); break;
+    case 46: launch_fl<46>(prec, grid, sh, s, A); break;
+    case 47: launch_fl<47>(prec, grid, sh, s, A); break;
+    case 57: launch_fl<57>(prec, grid, sh, s, A); break;
+    case 58: launch_fl<58>(prec, grid, sh, s, A); break;
+    case 59: launch_fl<59>(prec, grid, sh, s, A); break;
+    case 60: launch_fl<60>(prec, grid, sh, s, A); break;
+    case 61: launch_fl<61>(prec, grid, sh, s, A); break;
+    case 62: launch_fl<62>(prec, grid, sh, s, A); break;
+    case 63: launch_fl<63>(prec, grid, sh, s, A); break;
     default: return MCI_E_INVALID_ARGUMENT;
     }
     return hipGetLastError() == hipSuccess ? MCI_OK : MCI_E_DEVICE;

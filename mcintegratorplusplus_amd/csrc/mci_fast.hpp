@@ -15,7 +15,7 @@ struct FastState
     size_t nx32 = 0;
 };
 
-bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind);
+bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind, int nvecs);
 // the observables of the main sampling fit the specialised kernel (and its LDS)
 bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec);
 struct CalibCtrl;

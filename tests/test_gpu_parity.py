@@ -594,3 +594,37 @@ def test_learned_mixed_move_physics_at_size():
     z = (wa.mean(axis=0) - exact) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
     print("z", z, flush=True)
     assert np.all(np.abs(z) < 4), z
+
+
+@pytest.mark.parametrize("d,blocksize,coupling,W", [(16, 20, COUPLING_INDEPENDENT, 256), (16, 20, COUPLING_SHARED, 256),
+                                                     (4, 0, COUPLING_INDEPENDENT, 200), (8, 1, COUPLING_INDEPENDENT, 130)])
+def test_reference_streams_on_the_specialised_kernel(d, blocksize, coupling, W):
+    """The reference streams (one mt19937_64 per walker) on the specialised kernel, against the
+    generic kernel on the same streams: with a power-of-two coordinate count a step takes three
+    outputs, and the block sums are the reference's per-step sums, so everything is bit-identical
+    -- positions, step sizes, counters, every block mean or trace value and the estimates -- with
+    Block, Simple (blocksize 0) and Full (blocksize 1) accumulators, X2, X2_SUM and EXP_X2, both
+    couplings and a ragged last wave.  (The golden reference chains of this family, e.g. cfg2_walker
+    and the eight walkers of walkers.json, run through this kernel too.)"""
+    from mcitest import OBS_EXP_X2
+    print("reference streams, specialised vs generic", d, blocksize, coupling, W, flush=True)
+    sig = np.random.default_rng(d).uniform(0.5, 2.0, d)
+    est = ESTIM_MJ if blocksize > 1 else (ESTIM_NOOP if blocksize == 0 else ESTIM_MJ)
+    cfg = dict(ndim=d, seed=271 + d, ndecorr=300, move=(0, 1, 1, None), pdfs=[(PDF_GAUSS_DIAG, 0.5 / sig ** 2)],
+               obs=[(OBS_X2, [], blocksize, 1, 1 if blocksize > 0 else 0, est), (OBS_X2_SUM, [2.0], blocksize, 1, 1 if blocksize > 0 else 0, est),
+                    (OBS_EXP_X2, [float(d)], blocksize, 1, 1 if blocksize > 0 else 0, est)])
+    nmc = 1024 if blocksize == 1 else 64 * 20  # the MJBlocker on Full traces takes 2^k values
+    out = {}
+    for name in ("fast", "generic"):
+        c = chain_setup("mci", cfg, nwalkers=W)
+        c.setRng(RNG_MT)
+        c.setCoupling(coupling)
+        if name == "generic":
+            c.setKernelPath(KERNELS_GENERIC)
+        a, e = c.integrate(nmc)
+        a2, e2 = c.integrate(nmc // 2, False, False)
+        data = [c.observableData(i, n) for i, n in enumerate((d, 1, 1))]
+        out[name] = (a, e, a2, e2, *c.walkerResults(), c.getWalkerSteps(), c.getX(), *c.walkerAcceptance(), *data)
+    f, g = out["fast"], out["generic"]
+    for k in range(len(f)):
+        assert np.array_equal(f[k], g[k]), k
