@@ -223,6 +223,44 @@ def run_reference_streams():
     return out
 
 
+def run_all_coordinate_moves():
+    """The benchmark's target, observables and walkers with the reference's default move (every
+    coordinate at once, one step size, SRRDAllMove) instead of the single-coordinate move, on both
+    random-stream modes, on the specialised all-coordinate kernel and, for comparison, on the
+    generic one: whole integrate and sampling kernel.  A step reads and writes the walker's 16
+    coordinates in LDS; the native streams' Philox blocks (nine per step at d = 16) bound it."""
+    from mcintegratorplusplus_amd import MCI, constants as K
+    sig = sigmas()
+    exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
+    out = {"workload": f"{W} walkers x {NMC} steps, d={D}, as the benchmark with all-coordinate moves (veclen 0)"}
+    for rname, rng in (("native", K.RNG_PHILOX), ("reference_streams", K.RNG_MT19937_64)):
+        r = {}
+        for name, path in (("specialised", K.KERNELS_AUTO), ("generic", K.KERNELS_GENERIC)):
+            m = MCI(D, W)
+            m.setSeed(SEED)
+            m.setRandomMode(rng)
+            m.setKernelPath(path)
+            m.setWalkerCoupling(K.COUPLING_SHARED)
+            m.setTrialMove(K.SRRD_UNIFORM, 0, 1)
+            m.setNdecorrelationSteps(NDECORR)
+            m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+            m.addObservable(K.OBS_X2, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+            m.addObservable(K.OBS_X2_SUM, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+            if name == "specialised":
+                m.integrate(NMC)
+            t0 = time.perf_counter()
+            avg, err = m.integrate(NMC)
+            ms = 1000 * (time.perf_counter() - t0)
+            kt = m.getTimings()
+            wa, _ = m.walkerResults()
+            z = (avg - exact) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
+            r[name] = {"integrate_ms": ms, "walker_steps_per_s": W * NMC / (ms / 1000), "sampling_kernel_ms": kt[2],
+                       "max_abs_z": float(np.max(np.abs(z))), "acceptance": float(m.getAcceptanceRate())}
+        r["speedup_specialised_vs_generic"] = r["generic"]["integrate_ms"] / r["specialised"]["integrate_ms"]
+        out[rname] = r
+    return out
+
+
 def profile_summary():
     """HBM bytes per launch of the sampling kernel (FETCH_SIZE + WRITE_SIZE, each scaled by a
     1 GiB copy probe) and its VALU-busy fraction, from the committed rocprofv3 summary."""
@@ -516,7 +554,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
-    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt"],
+    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt", "allmove"],
                     help="also measure the precision form, the batched integrals and the learned proposal (all), "
                          "none of them, or one of them")
     ap.add_argument("--copy-probe", action="store_true",
@@ -601,7 +639,7 @@ def main():
     if args.workloads != "none" and args.precision == 0 and not args.generic:
         runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
                 "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces),
-                "reference_streams": ("mt", run_reference_streams)}
+                "reference_streams": ("mt", run_reference_streams), "all_coordinate_moves": ("allmove", run_all_coordinate_moves)}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
             if args.workloads in ("all", name):

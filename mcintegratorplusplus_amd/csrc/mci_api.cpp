@@ -332,7 +332,7 @@ bool fast_chain(const mci_ctx *c)
 {
     return c->kernel_path == MCI_KERNELS_AUTO && c->srrd == MCI_SRRD_UNIFORM && !c->ms_on && c->lm_K == 0 && !c->cb && fast_eligible(c->prec, c->rng, c->movetype, c->veclen, c->ntypes, c->periodic, (int)c->pdfs.size(),
                          c->pdfs.empty() ? 0 : c->pdfs[0].kind, c->nvecs) &&
-           !c->pdfs[0].params.empty() && fast_obs_eligible(nullptr, 0, c->ndim, c->prec);
+           (c->movetype == MCI_MOVE_ALL || !c->pdfs[0].params.empty()) && fast_obs_eligible(nullptr, 0, c->ndim, c->prec, c->movetype, c->rng);
 }
 
 // MCI::findMRT2Step for walkers that share the rank-averaged rate (MCIntegrator.cpp:87-168, USE_MPI).
@@ -1311,7 +1311,7 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
             od.push_back(d);
             voff += o.nobs;
         }
-        fast_main = fast && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec);
+        fast_main = fast && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec, c->movetype, c->rng);
         if (c->prec == MCI_PREC_FP32 && !fast_main) {
             return fail(MCI_E_INVALID_ARGUMENT, "FP32 precision supports X2, X2_SUM and EXP_X2 observables (one each, one block size, Block or Simple accumulators, nskip 1)");
         }

@@ -555,6 +555,484 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     A.rej[w] = A.nsteps - nacc;
 }
 
+// All-coordinate moves (SRRDAllMove with one step size: MoveType::All, the reference's default
+// move), FP64, both stream modes.  Every step proposes every coordinate, so the state is read and
+// written whole: [d][64] positions (with X2 16-byte {x_i, block sum} records), the proposal in
+// [d][64] beside them, the proto sum of the accepted state in a register.  The step follows the
+// generic kernel's arithmetic: x'_i = x_i + step r_i in coordinate order, p'_i = (x'_i x'_i) a_i,
+// e = (0 + sum p_i) - (0 + sum p'_i), accept u <= exp(e); block sums are the reference's per-step
+// sums, so every number is bit-identical to the generic kernel.  Random draws: native streams --
+// the shared word 0 carries the acceptance uniform's top 32 bits (no index shares it), words 1 .. d
+// the proposals; reference streams -- d proposal outputs, then the acceptance uniform.  The screen
+// of k_fast64 decides all but a band of lanes, which take the exact FP64 test.
+template <int FL>
+__global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
+{
+    constexpr bool FULL = (FL & F_FULL) != 0;
+    constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool ACC = (FL & 7) != 0 && !FULL;  // per-step block sums
+    constexpr bool REC = HX2 && !FULL;
+    constexpr bool MT = (FL & F_MT) != 0;
+    extern __shared__ double lds[];
+    const int d = A.ndim;
+    const int lane = threadIdx.x;
+    const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
+    const bool live = w < A.W;
+    if (A.pend) {
+        if (A.pend->apply && live) {
+            double st = A.steps[w] * A.pend->fact;
+            for (int i = 0; i < d; ++i) {
+                if (st > 0.5 * A.dimsizes[i]) { st = 0.5 * A.dimsizes[i]; }
+            }
+            if (st < (double)FLT_MIN) { st = (double)FLT_MIN; }
+            A.steps[w] = st;
+        }
+        if (A.pend->done) { return; }
+    }
+    const int64_t B = A.blocksize;
+    double2 *rec = reinterpret_cast<double2 *>(lds);  // X2: [d][64] {x_i, block sum of x_i^2}
+    double *xs = lds;                                  // otherwise [d][64] x_i
+    double *xn = lds + (REC ? 2 : 1) * d * WAVE;       // [d][64] the proposal
+    double *as = xn + d * WAVE;                        // [d] a_i, then the reference streams' ring
+    for (int i = lane; i < d; i += WAVE) { as[i] = A.a ? A.a[i] : 1.0; }
+    auto xget = [&](int i) -> double { return REC ? rec[i * WAVE + lane].x : xs[i * WAVE + lane]; };
+    double Sold = 0.0, e = 0.0, Sr = 0.0, Se = 0.0;
+    if (live) {
+        for (int i = 0; i < d; ++i) {
+            const double v = A.x64[(int64_t)i * A.W + w];
+            if (REC) { rec[i * WAVE + lane] = make_double2(v, 0.0); }
+            else { xs[i * WAVE + lane] = v; }
+        }
+    }
+    __syncthreads();
+    if (!live) { return; }
+    double q0 = 0.0;
+    for (int i = 0; i < d; ++i) {
+        const double v = xget(i);
+        Sold = Sold + (v * v) * as[i];
+        q0 = q0 + v * v;
+    }
+    if (HEXP) { e = exp(-(q0 / A.exp_div)); }
+    const double step = A.steps[w];
+    const uint32_t wid = (uint32_t)w;
+    const int64_t W = A.W;
+    [[maybe_unused]] uint64_t t = MT ? 0 : A.ctr[w];
+    // Reference streams: the walker's mt19937_64 outputs in chunks of MT_C into a per-lane LDS ring
+    // (a step takes d + 1 of them, in order).  The lazy twist of mci_device.hpp's mt_next, with the
+    // state words of a chunk loaded a chunk ahead; the chunks produce exactly the outputs the launch
+    // consumes, so the state ends where the generic kernel's does.
+    constexpr int MT_C = 16;
+    [[maybe_unused]] uint64_t *ring = reinterpret_cast<uint64_t *>(as + d);  // [MT_C][64]
+    [[maybe_unused]] int k = 0, pos = MT_C;
+    [[maybe_unused]] uint64_t xk = 0;
+    [[maybe_unused]] int64_t need = 0;
+    [[maybe_unused]] uint64_t x1[MT_C], xm[MT_C];
+    auto issue = [&](int kk) {
+#pragma unroll
+        for (int j = 0; j < MT_C; ++j) {
+            const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
+            const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
+            x1[j] = A.mt[(int64_t)k1 * W + w];
+            xm[j] = A.mt[(int64_t)km * W + w];
+            kk = k1;
+        }
+    };
+    auto mt_out = [&]() -> uint64_t {
+        if (pos == MT_C) {
+            const int64_t c = need < MT_C ? need : MT_C;
+#pragma unroll
+            for (int j = 0; j < MT_C; ++j) {
+                if (j < c) {
+                    const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
+                    A.mt[(int64_t)k * W + w] = nv;
+                    ring[j * WAVE + lane] = mt_temper(nv);
+                    k = (k + 1 == MT_N) ? 0 : k + 1;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < MT_C; ++j) {
+                if (j == c - 1) { xk = x1[j]; }
+            }
+            need -= c;
+            if (need > 0) { issue(k); }
+            pos = 0;
+        }
+        return ring[(pos++) * WAVE + lane];
+    };
+    if constexpr (MT) {
+        k = A.mtp[w];
+        xk = A.mt[(int64_t)k * W + w];
+        need = A.nsteps * (int64_t)(d + 1);
+        if (need > 0) { issue(k); }
+    }
+    int64_t nacc = 0, inblk = 0, b = 0;
+    for (int64_t s = 0; s < A.nsteps; ++s) {
+        // proposals in coordinate order, the proto sum of the proposal
+        double Snew = 0.0;
+        uint64_t uword = 0;  // native: the shared word; reference: the acceptance output
+        if constexpr (MT) {
+            for (int i = 0; i < d; ++i) {
+                const double r = (mt_canonical(mt_out()) * 2.0) + (-1.0);
+                const double v = xget(i) + step * r;
+                xn[i * WAVE + lane] = v;
+                Snew = Snew + (v * v) * as[i];
+            }
+            uword = mt_out();
+        }
+        else {
+            // main word j of step t: word 2 (t mod 2) + (j mod 2) of block j / 2 of pair t / 2
+            const uint32_t par = 2u * (uint32_t)(t & 1u);
+            const uint64_t tp = t >> 1;
+            for (int j = 0; j <= d; j += 2) {
+                uint32_t o0, o1, o2, o3;
+                philox((uint32_t)(j >> 1), wid, (uint32_t)tp, (uint32_t)(tp >> 32), A.key0, A.key1, o0, o1, o2, o3);
+                const uint32_t wa = par ? o2 : o0, wb = par ? o3 : o1;
+                if (j == 0) { uword = wa; }
+                else {
+                    const double v = xget(j - 1) + step * fma((double)wa, 0x1p-31, -(1.0 - 0x1p-32));
+                    xn[(j - 1) * WAVE + lane] = v;
+                    Snew = Snew + (v * v) * as[j - 1];
+                }
+                if (j + 1 <= d) {
+                    const double v = xget(j) + step * fma((double)wb, 0x1p-31, -(1.0 - 0x1p-32));
+                    xn[j * WAVE + lane] = v;
+                    Snew = Snew + (v * v) * as[j];
+                }
+            }
+        }
+        const double ee = Sold - Snew;  // log of the acceptance ratio, as the generic kernel forms it
+        const float U = MT ? (float)(uint32_t)(uword >> 32) : (float)(uint32_t)uword;
+        const float F = __builtin_amdgcn_exp2f(fmaf((float)ee, 0x1.715476p+0f, 32.0f));
+        const float Facc = MT ? fmaf(F, 0.9999f, -3.0f) : fmaf(F, 0.9999f, -2.0f);
+        const float Frej = MT ? fmaf(F, 1.0001f, 1.0f) : F * 1.0001f;
+        bool acc = U < Facc;
+        const bool band = !acc && U <= Frej;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0, 0)) {
+            double u;
+            if (MT) { u = (mt_canonical(uword) * 1.0) + 0.0; }
+            else {
+                uint32_t l0, l1, l2, l3;
+                philox(0x30000000u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, l0, l1, l2, l3);
+                u = (double)((((uint64_t)(uint32_t)uword << 32) | l0) >> 11) * 0x1p-53;
+            }
+            if (band) { acc = u <= exp(ee); }
+        }
+        if (!MT) { ++t; }
+        if (acc) {
+            ++nacc;
+            Sold = Snew;
+            for (int i = 0; i < d; ++i) {
+                const double v = xn[i * WAVE + lane];
+                if (REC) { rec[i * WAVE + lane].x = v; }
+                else { xs[i * WAVE + lane] = v; }
+            }
+        }
+        // the observables after the step: the reference's per-step sums, or the trace row
+        if (ACC || FULL) {
+            double q = 0.0;
+            for (int i = 0; i < d; ++i) {
+                if (REC) {
+                    double2 rv = rec[i * WAVE + lane];
+                    const double q2 = rv.x * rv.x;
+                    q = q + q2;
+                    rv.y = rv.y + q2;
+                    rec[i * WAVE + lane] = rv;
+                }
+                else {
+                    const double xv = xs[i * WAVE + lane];
+                    const double q2 = xv * xv;
+                    if (FULL && HX2) { A.d_x2[(s * d + i) * W + w] = q2; }
+                    q = q + q2;
+                }
+            }
+            if (HEXP && acc) { e = exp(-(q / A.exp_div)); }
+            if (FULL) {
+                if (HSUM) { A.d_x2sum[s * W + w] = q / A.x2sum_div; }
+                if (HEXP) { A.d_exp[s * W + w] = e; }
+            }
+            else {
+                if (HSUM) { Sr = Sr + q / A.x2sum_div; }
+                if (HEXP) { Se = Se + e; }
+                if (++inblk == B) {
+                    const double norm = 1. / (double)B;
+                    if (REC) {
+                        for (int i = 0; i < d; ++i) {
+                            const double2 rv = rec[i * WAVE + lane];
+                            A.d_x2[(b * d + i) * W + w] = rv.y * norm;
+                            rec[i * WAVE + lane] = make_double2(rv.x, 0.0);
+                        }
+                    }
+                    if (HSUM) { A.d_x2sum[b * W + w] = Sr * norm; }
+                    if (HEXP) { A.d_exp[b * W + w] = Se * norm; }
+                    Sr = 0.0;
+                    Se = 0.0;
+                    inblk = 0;
+                    ++b;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * W + w] = xget(i); }
+    if constexpr (MT) { A.mtp[w] = k; }
+    else { A.ctr[w] = t; }
+    A.acc[w] = nacc;
+    A.rej[w] = A.nsteps - nacc;
+}
+
+// The all-coordinate kernel with the walker in registers, for d <= DM: positions, the proposal and
+// the block sums are DM-entry register arrays, every loop over the coordinates unrolled to DM with a
+// wave-uniform d guard, so a step touches no LDS.  The arithmetic, the draws and the per-step sums
+// are those of k_fast64_all (bit-identical to the generic kernel).  Native streams: steps go in
+// pairs, one Philox block per two words of each of the two steps (the first step keeps words 2 and
+// 3 of every block for the second), so a pair computes ceil((d + 1) / 2) blocks.  Reference streams:
+// the walker's mt19937_64 outputs are made 32 at a time (their state words loaded a chunk ahead)
+// into a per-lane LDS ring of 64; a step reads its d + 1 outputs from the ring.
+template <int FL, int DM>
+__global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
+{
+    constexpr bool FULL = (FL & F_FULL) != 0;
+    constexpr bool HX2 = (FL & F_X2) != 0, HSUM = (FL & F_SUM) != 0, HEXP = (FL & F_EXP) != 0;
+    constexpr bool ACC = (FL & 7) != 0 && !FULL;  // per-step block sums
+    constexpr bool REC = HX2 && !FULL;             // per-coordinate block sums
+    constexpr bool MT = (FL & F_MT) != 0;
+    constexpr int NB = DM / 2 + 1;                 // Philox blocks of a step pair at most
+    extern __shared__ double lds[];
+    const int d = A.ndim;
+    const int lane = threadIdx.x;
+    const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
+    const bool live = w < A.W;
+    if (A.pend) {
+        if (A.pend->apply && live) {
+            double st = A.steps[w] * A.pend->fact;
+            for (int i = 0; i < d; ++i) {
+                if (st > 0.5 * A.dimsizes[i]) { st = 0.5 * A.dimsizes[i]; }
+            }
+            if (st < (double)FLT_MIN) { st = (double)FLT_MIN; }
+            A.steps[w] = st;
+        }
+        if (A.pend->done) { return; }
+    }
+    if (!live) { return; }
+    const int64_t W = A.W, B = A.blocksize;
+    double x[DM], xn[DM], sm[DM], a[DM];
+    double Sold = 0.0, q = 0.0, e = 0.0, Sr = 0.0, Se = 0.0;
+#pragma unroll
+    for (int i = 0; i < DM; ++i) {
+        x[i] = 0.0;
+        xn[i] = 0.0;
+        sm[i] = 0.0;
+        a[i] = 0.0;
+        if (i < d) {
+            x[i] = A.x64[(int64_t)i * W + w];
+            a[i] = A.a ? A.a[i] : 1.0;
+            Sold = Sold + (x[i] * x[i]) * a[i];
+            q = q + x[i] * x[i];
+        }
+    }
+    if (HEXP) { e = exp(-(q / A.exp_div)); }
+    const double step = A.steps[w];
+    const uint32_t wid = (uint32_t)w;
+    [[maybe_unused]] uint64_t t = MT ? 0 : A.ctr[w];
+    int64_t nacc = 0, inblk = 0, b = 0, s = 0;
+
+    // the decision of a step from the proto sum Snew and |x'|^2 qn of its proposal (xn), then the
+    // observables after it; uw: the native shared word or the reference acceptance output
+    auto finish = [&](double Snew, double qn, uint64_t uw) {
+        const double ee = Sold - Snew;
+        const float U = MT ? (float)(uint32_t)(uw >> 32) : (float)(uint32_t)uw;
+        const float F = __builtin_amdgcn_exp2f(fmaf((float)ee, 0x1.715476p+0f, 32.0f));
+        const float Facc = MT ? fmaf(F, 0.9999f, -3.0f) : fmaf(F, 0.9999f, -2.0f);
+        const float Frej = MT ? fmaf(F, 1.0001f, 1.0f) : F * 1.0001f;
+        bool acc = U < Facc;
+        const bool band = !acc && U <= Frej;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(band) != 0, 0)) {
+            double u;
+            if (MT) { u = (mt_canonical(uw) * 1.0) + 0.0; }
+            else {
+                uint32_t l0, l1, l2, l3;
+                philox(0x30000000u, wid, (uint32_t)t, (uint32_t)(t >> 32), A.key0, A.key1, l0, l1, l2, l3);
+                u = (double)((((uint64_t)(uint32_t)uw << 32) | l0) >> 11) * 0x1p-53;
+            }
+            if (band) { acc = u <= exp(ee); }
+        }
+        if (!MT) { ++t; }
+        nacc += acc ? 1 : 0;
+        Sold = acc ? Snew : Sold;
+        q = acc ? qn : q;
+#pragma unroll
+        for (int i = 0; i < DM; ++i) {
+            if (i < d) { x[i] = acc ? xn[i] : x[i]; }
+        }
+        if (HEXP && acc) { e = exp(-(q / A.exp_div)); }
+        if (FULL) {
+            if (HX2) {
+#pragma unroll
+                for (int i = 0; i < DM; ++i) {
+                    if (i < d) { A.d_x2[(s * d + i) * W + w] = x[i] * x[i]; }
+                }
+            }
+            if (HSUM) { A.d_x2sum[s * W + w] = q / A.x2sum_div; }
+            if (HEXP) { A.d_exp[s * W + w] = e; }
+        }
+        if (ACC) {
+            if (REC) {
+#pragma unroll
+                for (int i = 0; i < DM; ++i) {
+                    if (i < d) { sm[i] = sm[i] + x[i] * x[i]; }
+                }
+            }
+            if (HSUM) { Sr = Sr + q / A.x2sum_div; }
+            if (HEXP) { Se = Se + e; }
+            if (++inblk == B) {
+                const double norm = 1. / (double)B;
+                if (REC) {
+#pragma unroll
+                    for (int i = 0; i < DM; ++i) {
+                        if (i < d) {
+                            A.d_x2[(b * d + i) * W + w] = sm[i] * norm;
+                            sm[i] = 0.0;
+                        }
+                    }
+                }
+                if (HSUM) { A.d_x2sum[b * W + w] = Sr * norm; }
+                if (HEXP) { A.d_exp[b * W + w] = Se * norm; }
+                Sr = 0.0;
+                Se = 0.0;
+                inblk = 0;
+                ++b;
+            }
+        }
+        ++s;
+    };
+    // coordinate i of the proposal from the random value r (the generic kernel's xn + st * r, then
+    // the proto (xn xn) a_i and |x'|^2 summed in coordinate order)
+    auto propose = [&](int i, double r, double &Sn, double &qn) {
+        const double v = x[i] + step * r;
+        xn[i] = v;
+        Sn = Sn + (v * v) * a[i];
+        qn = qn + v * v;
+    };
+
+    if constexpr (MT) {
+        constexpr int MT_C = 32, RING = 64;
+        uint64_t *ring = reinterpret_cast<uint64_t *>(lds);  // [RING][64]
+        uint64_t *mt = A.mt;
+        int k = A.mtp[w];
+        uint64_t xk = mt[(int64_t)k * W + w];
+        uint64_t x1[MT_C], xm[MT_C];
+        int64_t need = A.nsteps * (int64_t)(d + 1);
+        auto issue = [&](int kk) {
+#pragma unroll
+            for (int j = 0; j < MT_C; ++j) {
+                const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
+                const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
+                x1[j] = mt[(int64_t)k1 * W + w];
+                xm[j] = mt[(int64_t)km * W + w];
+                kk = k1;
+            }
+        };
+        if (need > 0) { issue(k); }
+        int head = 0, tail = 0, avail = 0;
+        for (int64_t st = 0; st < A.nsteps; ++st) {
+            while (avail < d + 1) {
+                // the next chunk: exactly the outputs still to be consumed, at most MT_C
+                const int c = (int)(need < MT_C ? need : MT_C);
+#pragma unroll
+                for (int j = 0; j < MT_C; ++j) {
+                    if (j < c) {
+                        const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
+                        mt[(int64_t)k * W + w] = nv;
+                        ring[((head + j) & (RING - 1)) * WAVE + lane] = mt_temper(nv);
+                        k = (k + 1 == MT_N) ? 0 : k + 1;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < MT_C; ++j) {
+                    if (j == c - 1) { xk = x1[j]; }
+                }
+                head = (head + c) & (RING - 1);
+                avail += c;
+                need -= c;
+                if (need > 0) { issue(k); }
+            }
+            double Sn = 0.0, qn = 0.0;
+#pragma unroll
+            for (int i = 0; i < DM; ++i) {
+                if (i < d) { propose(i, (mt_canonical(ring[((tail + i) & (RING - 1)) * WAVE + lane]) * 2.0) + (-1.0), Sn, qn); }
+            }
+            const uint64_t uw = ring[((tail + d) & (RING - 1)) * WAVE + lane];
+            tail = (tail + d + 1) & (RING - 1);
+            avail -= d + 1;
+            finish(Sn, qn, uw);
+        }
+        A.mtp[w] = k;
+    }
+    else {
+        // main word j of step t: word 2 (t mod 2) + (j mod 2) of block j / 2 of pair t / 2; the
+        // native proposal (2 w + 1 - 2^32) 2^-32 as one exact fused multiply-add
+        auto rnd = [](uint32_t v) { return fma((double)v, 0x1p-31, -(1.0 - 0x1p-32)); };
+        const int nb = (d + 2) >> 1;
+        uint32_t sw[2 * NB];  // the second step's words of a pair
+        // one step on its own (either parity)
+        auto single = [&]() {
+            const uint32_t par = (uint32_t)(t & 1u);
+            const uint64_t tp = t >> 1;
+            double Sn = 0.0, qn = 0.0;
+            uint32_t uw = 0;
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi) {
+                if (bi < nb) {
+                    uint32_t o0, o1, o2, o3;
+                    philox((uint32_t)bi, wid, (uint32_t)tp, (uint32_t)(tp >> 32), A.key0, A.key1, o0, o1, o2, o3);
+                    const uint32_t wa = par ? o2 : o0, wb = par ? o3 : o1;
+                    if (bi == 0) { uw = wa; }
+                    else { propose(2 * bi - 1, rnd(wa), Sn, qn); }
+                    if (2 * bi < d) { propose(2 * bi, rnd(wb), Sn, qn); }
+                }
+            }
+            finish(Sn, qn, uw);
+        };
+        if ((t & 1u) && s < A.nsteps) { single(); }
+        while (s + 2 <= A.nsteps) {
+            const uint64_t tp = t >> 1;
+            double Sn = 0.0, qn = 0.0;
+            uint32_t uw = 0;
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi) {
+                if (bi < nb) {
+                    uint32_t o0, o1, o2, o3;
+                    philox((uint32_t)bi, wid, (uint32_t)tp, (uint32_t)(tp >> 32), A.key0, A.key1, o0, o1, o2, o3);
+                    sw[2 * bi] = o2;
+                    sw[2 * bi + 1] = o3;
+                    if (bi == 0) { uw = o0; }
+                    else { propose(2 * bi - 1, rnd(o0), Sn, qn); }
+                    if (2 * bi < d) { propose(2 * bi, rnd(o1), Sn, qn); }
+                }
+            }
+            finish(Sn, qn, uw);
+            Sn = 0.0;
+            qn = 0.0;
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi) {
+                if (bi < nb) {
+                    if (bi == 0) { uw = sw[0]; }
+                    else { propose(2 * bi - 1, rnd(sw[2 * bi]), Sn, qn); }
+                    if (2 * bi < d) { propose(2 * bi, rnd(sw[2 * bi + 1]), Sn, qn); }
+                }
+            }
+            finish(Sn, qn, uw);
+        }
+        if (s < A.nsteps) { single(); }
+        A.ctr[w] = t;
+    }
+#pragma unroll
+    for (int i = 0; i < DM; ++i) {
+        if (i < d) { A.x64[(int64_t)i * W + w] = x[i]; }
+    }
+    A.acc[w] = nacc;
+    A.rej[w] = A.nsteps - nacc;
+}
+
 // Kahan-compensated FP32 accumulation: s + y with compensation c
 __device__ __forceinline__ void kahan(float &s, float &c, float y)
 {
@@ -862,6 +1340,15 @@ size_t lds_bytes(int prec, int ndim, bool x2)
     return sizeof(float) * ((size_t)(x2 ? 3 : 1) * ndim * WAVE + ndim);
 }
 
+// the all-coordinate kernels: registers up to ALL_DM coordinates (LDS: the reference streams' ring);
+// beyond, positions (X2 blocks: records), proposals, a_i and the ring in LDS
+constexpr int ALL_DM = 16;
+size_t lds_bytes_all(int ndim, bool x2, bool mt)
+{
+    if (ndim <= ALL_DM) { return mt ? sizeof(uint64_t) * 64 * WAVE : 0; }
+    return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + (size_t)ndim * WAVE + ndim) + (mt ? sizeof(uint64_t) * 16 * WAVE : 0);
+}
+
 size_t lds_limit()
 {
     static size_t lim = 0;
@@ -880,6 +1367,13 @@ void launch_fl(int prec, dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
     if (prec == MCI_PREC_FP64) { hipLaunchKernelGGL(k_fast64<FL>, grid, dim3(WAVE), sh, s, A); }
     else if constexpr ((FL & (F_FULL | F_MT)) == 0) { hipLaunchKernelGGL(k_fast32<FL>, grid, dim3(WAVE), sh, s, A); }
 }
+
+template <int FL>
+void launch_all(dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
+{
+    if (A.ndim <= ALL_DM) { hipLaunchKernelGGL((k_fast64_allr<FL, ALL_DM>), grid, dim3(WAVE), sh, s, A); }
+    else { hipLaunchKernelGGL(k_fast64_all<FL>, grid, dim3(WAVE), sh, s, A); }
+}
 } // namespace
 
 bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind, int nvecs)
@@ -887,10 +1381,17 @@ bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int 
     // the reference streams: the FP64 chain, and a power-of-two coordinate count, whose index draw
     // (uniform_int_distribution, Lemire on 64-bit words) never rejects, so a step takes three outputs
     const bool streams = rng == MCI_RNG_PHILOX || (rng == MCI_RNG_MT19937_64 && prec == MCI_PREC_FP64 && nvecs >= 1 && (nvecs & (nvecs - 1)) == 0);
-    return streams && movetype == MCI_MOVE_VEC && veclen == 1 && ntypes == 1 && !periodic && npdf == 1 && pdfkind == MCI_PDF_GAUSS_DIAG;
+    const bool target = ntypes == 1 && !periodic && npdf == 1 && pdfkind == MCI_PDF_GAUSS_DIAG;
+    // all-coordinate moves (one step size): FP64, either stream
+    // (exp(-|x|^2) is the diagonal form with a_i = 1: the same protos, sums and ratio bit for bit)
+    if (movetype == MCI_MOVE_ALL) {
+        return ntypes == 1 && !periodic && npdf == 1 && (pdfkind == MCI_PDF_GAUSS_DIAG || pdfkind == MCI_PDF_GAUSS_ISO) &&
+               prec == MCI_PREC_FP64 && (rng == MCI_RNG_PHILOX || rng == MCI_RNG_MT19937_64);
+    }
+    return streams && movetype == MCI_MOVE_VEC && veclen == 1 && target;
 }
 
-bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
+bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec, int movetype, int rng)
 {
     int B = -1;
     int seen = 0;
@@ -907,7 +1408,9 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec)
         if (B == -1) { B = bs; }
         else if (B != bs) { return false; }
     }
-    return lds_bytes(prec, ndim, x2 && B != 1) <= lds_limit();  // B == 1: Full accumulators, no block-sum records
+    // B == 1: Full accumulators, no block-sum records
+    if (movetype == MCI_MOVE_ALL) { return lds_bytes_all(ndim, x2 && B != 1, rng == MCI_RNG_MT19937_64) <= lds_limit(); }
+    return lds_bytes(prec, ndim, x2 && B != 1) <= lds_limit();
 }
 
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
@@ -925,12 +1428,12 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
     A.key1 = c.key1;
     A.ctr = c.ctr;
     A.x64 = c.xold;
-    A.a = c.pdf[0].a;
+    A.a = c.pdf[0].kind == MCI_PDF_GAUSS_ISO ? nullptr : c.pdf[0].a;  // exp(-|x|^2): a_i = 1
     A.steps = c.steps;
     A.acc = c.acc;
     A.rej = c.rej;
     A.nvecs = c.nvecs;
-    A.lemire_thr = (0u - (uint32_t)c.nvecs) % (uint32_t)c.nvecs;
+    A.lemire_thr = c.nvecs > 0 ? (0u - (uint32_t)c.nvecs) % (uint32_t)c.nvecs : 0u;  // all-coordinate moves: no index
     // the native stream's shared word: the index's bits (FP32: at most 8), then the acceptance bits;
     // the reference streams take the index from the top log2(nvecs) bits of its own output
     const bool mt = c.rng == MCI_RNG_MT19937_64;
@@ -960,9 +1463,48 @@ int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const O
         if (o.kind == MCI_OBS_X2_SUM) { A.d_x2sum = o.data; A.x2sum_div = o.p0; fl |= F_SUM; }
         if (o.kind == MCI_OBS_EXP_X2) { A.d_exp = o.data; A.exp_div = o.p0; fl |= F_EXP; }
     }
-    if (!A.a) { return MCI_E_INVALID_ARGUMENT; } // the fast path needs explicit weights
+    if (!A.a && c.movetype != MCI_MOVE_ALL) { return MCI_E_INVALID_ARGUMENT; } // single-coordinate moves: explicit weights
     if (prec != MCI_PREC_FP64 && A.blocksize > INT32_MAX) { return MCI_E_INVALID_ARGUMENT; }
     const dim3 grid((unsigned)((c.W + WAVE - 1) / WAVE));
+    if (c.movetype == MCI_MOVE_ALL) {
+        if (prec != MCI_PREC_FP64 || (mt && (!c.mt || !c.mtp))) { return MCI_E_INVALID_ARGUMENT; }
+        if (mt) { fl |= F_MT; }
+        const size_t sha = lds_bytes_all(c.ndim, (fl & F_X2) != 0 && !(fl & F_FULL), mt);
+        switch (fl) {
+        case 0: launch_all<0>(grid, sha, s, A); break;
+        case 1: launch_all<1>(grid, sha, s, A); break;
+        case 2: launch_all<2>(grid, sha, s, A); break;
+        case 3: launch_all<3>(grid, sha, s, A); break;
+        case 4: launch_all<4>(grid, sha, s, A); break;
+        case 5: launch_all<5>(grid, sha, s, A); break;
+        case 6: launch_all<6>(grid, sha, s, A); break;
+        case 7: launch_all<7>(grid, sha, s, A); break;
+        case 17: launch_all<17>(grid, sha, s, A); break;
+        case 18: launch_all<18>(grid, sha, s, A); break;
+        case 19: launch_all<19>(grid, sha, s, A); break;
+        case 20: launch_all<20>(grid, sha, s, A); break;
+        case 21: launch_all<21>(grid, sha, s, A); break;
+        case 22: launch_all<22>(grid, sha, s, A); break;
+        case 23: launch_all<23>(grid, sha, s, A); break;
+        case 32: launch_all<32>(grid, sha, s, A); break;
+        case 33: launch_all<33>(grid, sha, s, A); break;
+        case 34: launch_all<34>(grid, sha, s, A); break;
+        case 35: launch_all<35>(grid, sha, s, A); break;
+        case 36: launch_all<36>(grid, sha, s, A); break;
+        case 37: launch_all<37>(grid, sha, s, A); break;
+        case 38: launch_all<38>(grid, sha, s, A); break;
+        case 39: launch_all<39>(grid, sha, s, A); break;
+        case 49: launch_all<49>(grid, sha, s, A); break;
+        case 50: launch_all<50>(grid, sha, s, A); break;
+        case 51: launch_all<51>(grid, sha, s, A); break;
+        case 52: launch_all<52>(grid, sha, s, A); break;
+        case 53: launch_all<53>(grid, sha, s, A); break;
+        case 54: launch_all<54>(grid, sha, s, A); break;
+        case 55: launch_all<55>(grid, sha, s, A); break;
+        default: return MCI_E_INVALID_ARGUMENT;
+        }
+        return hipGetLastError() == hipSuccess ? MCI_OK : MCI_E_DEVICE;
+    }
     const size_t sh = lds_bytes(prec, c.ndim, (fl & F_X2) != 0 && !(fl & F_FULL));
     if (A.ibits >= 0) { fl |= F_P2; }
     if (mt) {

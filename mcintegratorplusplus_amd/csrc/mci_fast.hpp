@@ -17,7 +17,7 @@ struct FastState
 
 bool fast_eligible(int prec, int rng, int movetype, int veclen, int ntypes, int periodic, int npdf, int pdfkind, int nvecs);
 // the observables of the main sampling fit the specialised kernel (and its LDS)
-bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec);
+bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec, int movetype, int rng);
 struct CalibCtrl;
 // nsteps Metropolis steps; obs == nullptr: no accumulation (decorrelation).  pend (FP64 only):
 // the shared calibration's control state; the launch first applies its pending step factor

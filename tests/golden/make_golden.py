@@ -89,6 +89,19 @@ def scenarios():
                                                pdfs=[(PDF_GAUSS_DIAG, cfg2_a(64))],
                                                obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_EXP_X2, [64.0], 160, 1, 1, ESTIM_MJ)]),
                   calls=[dict(nmc=10240, find=1, decorr=1)]))
+    # the reference's default move (all coordinates, one step size) on configuration 2's target:
+    # Block accumulators, then a continuation; d = 5 with Full traces; d = 7 with a Simple accumulator
+    S.append(dict(name="cfg2_allmove_walker", cfg=dict(ndim=16, seed=1337, ndecorr=1000, pdfs=[(PDF_GAUSS_DIAG, cfg2_a(16))],
+                                                       obs=[(OBS_X2, [], 160, 1, 1, ESTIM_MJ), (OBS_X2_SUM, [], 160, 1, 1, ESTIM_MJ),
+                                                            (OBS_EXP_X2, [16.0], 160, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=10240, find=1, decorr=1), dict(nmc=5120, find=0, decorr=0)]))
+    S.append(dict(name="allmove_full", cfg=dict(ndim=5, seed=77, pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0, 0.25, 0.75])],
+                                                obs=[(OBS_X2, [], 1, 1, 1, ESTIM_CORR), (OBS_X2_SUM, [5.0], 1, 1, 1, ESTIM_CORR),
+                                                     (OBS_EXP_X2, [5.0], 1, 1, 1, ESTIM_CORR)]),
+                  calls=[dict(nmc=4096, find=1, decorr=1)]))
+    S.append(dict(name="allmove_simple", cfg=dict(ndim=7, seed=4711, ndecorr=500, pdfs=[(PDF_GAUSS_DIAG, cfg2_a(7))],
+                                                  obs=[(OBS_X2, [], 0, 1, 0, ESTIM_NOOP), (OBS_X2_SUM, [7.0], 0, 1, 0, ESTIM_NOOP)]),
+                  calls=[dict(nmc=3000, find=1, decorr=1)]))
     # remaining catalogue kinds, periodic domain, mixture target, Simple/Noop, FCBlocker, Uncorrelated
     S.append(dict(name="catalogue_exp", cfg=dict(ndim=4, seed=7, nfind=20, ndecorr=2000, move=(0, 2, 2, [2, 4]),
                                                  pdfs=[(PDF_EXP_DIAG, [1., 2., 0.5, 1.])],

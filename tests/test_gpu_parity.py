@@ -628,3 +628,49 @@ def test_reference_streams_on_the_specialised_kernel(d, blocksize, coupling, W):
     f, g = out["fast"], out["generic"]
     for k in range(len(f)):
         assert np.array_equal(f[k], g[k]), k
+
+
+@pytest.mark.parametrize("rng", [RNG_PHILOX, RNG_MT])
+@pytest.mark.parametrize("d,blocksize,coupling,W,iso", [(16, 20, COUPLING_INDEPENDENT, 256, False), (16, 20, COUPLING_SHARED, 200, False),
+                                                         (5, 0, COUPLING_INDEPENDENT, 130, False), (7, 1, COUPLING_INDEPENDENT, 100, False),
+                                                         (3, 1, COUPLING_SHARED, 70, False), (3, 20, COUPLING_INDEPENDENT, 90, True),
+                                                         (20, 20, COUPLING_INDEPENDENT, 100, False), (23, 1, COUPLING_SHARED, 70, False)])
+def test_all_coordinate_moves_on_the_specialised_kernel(rng, d, blocksize, coupling, W, iso):
+    """The reference's default move (every coordinate at once, one step size) on the specialised
+    all-coordinate kernels (walker in registers up to d = 16, in LDS beyond), against the generic
+    kernel on the same streams: positions, step sizes, counters, every block mean or trace value and
+    the estimates bit-identical, with Block, Simple (blocksize 0) and Full (blocksize 1)
+    accumulators, X2, X2_SUM and EXP_X2, both couplings, both stream modes, ragged last waves, the
+    diagonal Gaussian and exp(-|x|^2); the native streams also against the oracle's walkers.  (The
+    golden reference chains cfg1, ut2_sequence, ut3_sequence, cfg2_allmove_walker, allmove_full and
+    allmove_simple run through these kernels as well.)"""
+    from mcitest import PDF_GAUSS_ISO
+    print("all-coordinate moves, specialised vs generic", rng, d, blocksize, coupling, W, iso, flush=True)
+    sig = np.random.default_rng(d).uniform(0.5, 2.0, d)
+    est = ESTIM_NOOP if blocksize == 0 else ESTIM_MJ
+    eq = 1 if blocksize > 0 else 0
+    pdf = (PDF_GAUSS_ISO, []) if iso else (PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+    cfg = dict(ndim=d, seed=913 + d, ndecorr=300, move=(0, 0, 1, None), pdfs=[pdf],
+               obs=[(OBS_X2, [], blocksize, 1, eq, est), (OBS_X2_SUM, [2.0], blocksize, 1, eq, est),
+                    (OBS_EXP_X2, [float(d)], blocksize, 1, eq, est)])
+    nmc = 1024 if blocksize == 1 else 64 * 20
+    out = {}
+    for name in ("fast", "generic", "orc"):
+        if name == "orc" and rng != RNG_PHILOX:
+            continue
+        c = chain_setup("orc" if name == "orc" else "mci", cfg, nwalkers=W)
+        c.setRng(rng)
+        c.setCoupling(coupling)
+        if name == "generic":
+            c.setKernelPath(KERNELS_GENERIC)
+        a, e = c.integrate(nmc)
+        a2, e2 = c.integrate(nmc // 2, False, False)
+        data = [] if name == "orc" else [c.observableData(i, n) for i, n in enumerate((d, 1, 1))]
+        out[name] = (c.getX(), c.getWalkerSteps(), *c.walkerAcceptance(), a, e, a2, e2, *c.walkerResults(), *data)
+    f, g = out["fast"], out["generic"]
+    for k in range(len(f)):
+        assert np.array_equal(f[k], g[k]), k
+    if "orc" in out:
+        o = out["orc"]
+        for k in range(4):
+            assert np.array_equal(f[k], o[k]), k
