@@ -29,6 +29,7 @@
 #include "mci_kernels.hpp"
 
 #include <float.h>
+#include <type_traits>
 
 namespace mcid
 {
@@ -787,7 +788,8 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
 // 3 of every block for the second), so a pair computes ceil((d + 1) / 2) blocks.  Reference streams:
 // the walker's mt19937_64 outputs are made 32 at a time (their state words loaded a chunk ahead)
 // into a per-lane LDS ring of 64; a step reads its d + 1 outputs from the ring.
-template <int FL, int DM>
+// EX: d == DM exactly (every coordinate guard folds away, the step is straight-line code).
+template <int FL, int DM, bool EX>
 __global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
 {
     constexpr bool FULL = (FL & F_FULL) != 0;
@@ -797,7 +799,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
     constexpr bool MT = (FL & F_MT) != 0;
     constexpr int NB = DM / 2 + 1;                 // Philox blocks of a step pair at most
     extern __shared__ double lds[];
-    const int d = A.ndim;
+    const int d = EX ? DM : A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
     const bool live = w < A.W;
@@ -916,56 +918,71 @@ __global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
     if constexpr (MT) {
         constexpr int MT_C = 32, RING = 64;
         uint64_t *ring = reinterpret_cast<uint64_t *>(lds);  // [RING][64]
-        uint64_t *mt = A.mt;
-        int k = A.mtp[w];
-        uint64_t xk = mt[(int64_t)k * W + w];
-        uint64_t x1[MT_C], xm[MT_C];
-        int64_t need = A.nsteps * (int64_t)(d + 1);
-        auto issue = [&](int kk) {
-#pragma unroll
-            for (int j = 0; j < MT_C; ++j) {
-                const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
-                const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
-                x1[j] = mt[(int64_t)k1 * W + w];
-                xm[j] = mt[(int64_t)km * W + w];
-                kk = k1;
-            }
-        };
-        if (need > 0) { issue(k); }
-        int head = 0, tail = 0, avail = 0;
-        for (int64_t st = 0; st < A.nsteps; ++st) {
-            while (avail < d + 1) {
-                // the next chunk: exactly the outputs still to be consumed, at most MT_C
-                const int c = (int)(need < MT_C ? need : MT_C);
+        const int kw = A.mtp[w];
+        // The walkers of a wave are usually at the same state position (every step of this kernel
+        // takes d + 1 outputs; seeding starts all at 0); then the position is wave-uniform and every
+        // state access is a uniform row pointer plus the lane (scalar address arithmetic).
+        const int kf = __builtin_amdgcn_readfirstlane(kw);
+        const bool uniform = __builtin_amdgcn_ballot_w64(kw != kf) == 0;
+        const int64_t wbase = (int64_t)blockIdx.x * WAVE;
+        auto run = [&](auto uni_c) {
+            constexpr bool UNI = decltype(uni_c)::value;
+            int k = UNI ? kf : kw;
+            auto at = [&](int kk) -> uint64_t * {
+                if constexpr (UNI) { return (A.mt + ((int64_t)kk * W + wbase)) + lane; }
+                else { return A.mt + ((int64_t)kk * W + w); }
+            };
+            uint64_t xk = *at(k);
+            uint64_t x1[MT_C], xm[MT_C];
+            int64_t need = A.nsteps * (int64_t)(d + 1);
+            auto issue = [&](int kk) {
 #pragma unroll
                 for (int j = 0; j < MT_C; ++j) {
-                    if (j < c) {
-                        const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
-                        mt[(int64_t)k * W + w] = nv;
-                        ring[((head + j) & (RING - 1)) * WAVE + lane] = mt_temper(nv);
-                        k = (k + 1 == MT_N) ? 0 : k + 1;
+                    const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
+                    const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
+                    x1[j] = *at(k1);
+                    xm[j] = *at(km);
+                    kk = k1;
+                }
+            };
+            if (need > 0) { issue(k); }
+            int head = 0, tail = 0, avail = 0;
+            for (int64_t st = 0; st < A.nsteps; ++st) {
+                while (avail < d + 1) {
+                    // the next chunk: exactly the outputs still to be consumed, at most MT_C
+                    const int c = (int)(need < MT_C ? need : MT_C);
+#pragma unroll
+                    for (int j = 0; j < MT_C; ++j) {
+                        if (j < c) {
+                            const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
+                            *at(k) = nv;
+                            ring[((head + j) & (RING - 1)) * WAVE + lane] = mt_temper(nv);
+                            k = (k + 1 == MT_N) ? 0 : k + 1;
+                        }
                     }
-                }
 #pragma unroll
-                for (int j = 0; j < MT_C; ++j) {
-                    if (j == c - 1) { xk = x1[j]; }
+                    for (int j = 0; j < MT_C; ++j) {
+                        if (j == c - 1) { xk = x1[j]; }
+                    }
+                    head = (head + c) & (RING - 1);
+                    avail += c;
+                    need -= c;
+                    if (need > 0) { issue(k); }
                 }
-                head = (head + c) & (RING - 1);
-                avail += c;
-                need -= c;
-                if (need > 0) { issue(k); }
-            }
-            double Sn = 0.0, qn = 0.0;
+                double Sn = 0.0, qn = 0.0;
 #pragma unroll
-            for (int i = 0; i < DM; ++i) {
-                if (i < d) { propose(i, (mt_canonical(ring[((tail + i) & (RING - 1)) * WAVE + lane]) * 2.0) + (-1.0), Sn, qn); }
+                for (int i = 0; i < DM; ++i) {
+                    if (i < d) { propose(i, (mt_canonical(ring[((tail + i) & (RING - 1)) * WAVE + lane]) * 2.0) + (-1.0), Sn, qn); }
+                }
+                const uint64_t uw = ring[((tail + d) & (RING - 1)) * WAVE + lane];
+                tail = (tail + d + 1) & (RING - 1);
+                avail -= d + 1;
+                finish(Sn, qn, uw);
             }
-            const uint64_t uw = ring[((tail + d) & (RING - 1)) * WAVE + lane];
-            tail = (tail + d + 1) & (RING - 1);
-            avail -= d + 1;
-            finish(Sn, qn, uw);
-        }
-        A.mtp[w] = k;
+            A.mtp[w] = k;
+        };
+        if (uniform) { run(std::integral_constant<bool, true>{}); }
+        else { run(std::integral_constant<bool, false>{}); }
     }
     else {
         // main word j of step t: word 2 (t mod 2) + (j mod 2) of block j / 2 of pair t / 2; the
@@ -1371,7 +1388,13 @@ void launch_fl(int prec, dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
 template <int FL>
 void launch_all(dim3 grid, size_t sh, hipStream_t s, const FastArgs &A)
 {
-    if (A.ndim <= ALL_DM) { hipLaunchKernelGGL((k_fast64_allr<FL, ALL_DM>), grid, dim3(WAVE), sh, s, A); }
+    if constexpr ((FL & F_MT) == 0) {
+        if (A.ndim == ALL_DM) {
+            hipLaunchKernelGGL((k_fast64_allr<FL, ALL_DM, true>), grid, dim3(WAVE), sh, s, A);
+            return;
+        }
+    }
+    if (A.ndim <= ALL_DM) { hipLaunchKernelGGL((k_fast64_allr<FL, ALL_DM, false>), grid, dim3(WAVE), sh, s, A); }
     else { hipLaunchKernelGGL(k_fast64_all<FL>, grid, dim3(WAVE), sh, s, A); }
 }
 } // namespace

@@ -23,6 +23,7 @@ OK, E_INVALID, E_DOMAIN, E_RUNTIME, E_DEVICE = 0, 1, 2, 3, 4
 ESTIM_NOOP, ESTIM_UNCORR, ESTIM_CORR, ESTIM_FC, ESTIM_MJ = 0, 1, 2, 3, 4
 MOVE_ALL, MOVE_VEC = 0, 1
 SRRD_UNIFORM = 0
+SRRD_GAUSSIAN = 1
 PDF_GAUSS_ISO, PDF_GAUSS_DIAG, PDF_EXP_DIAG, PDF_GMM_ISO = 1, 2, 3, 4
 OBS_X2, OBS_X2_SUM, OBS_X, OBS_X_SUM, OBS_CONST, OBS_EXP_X2, OBS_INDICATOR, OBS_GAUSS_X2 = 1, 2, 3, 4, 5, 6, 7, 8
 OBS_PDF_LOG, OBS_PDF_PROTO, OBS_DEP_SQUARE = 9, 10, 11
