@@ -572,7 +572,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the walkers of one GPU are the unit of "
-                                   "work (multi-GPU sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+                                   "work, and sharding walkers across GPUs is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     m = make_mci(W, args.precision)

@@ -151,8 +151,9 @@ mci_status mci_set_walker_seeds(mci_ctx *ctx, const uint64_t *seeds, int64_t nse
 mci_status mci_set_rng(mci_ctx *ctx, int rng_kind);
 mci_status mci_set_precision(mci_ctx *ctx, int precision);
 mci_status mci_set_coupling(mci_ctx *ctx, int coupling);
-/* MCI_KERNELS_AUTO / MCI_KERNELS_GENERIC.  Both give identical accept/reject decisions and
- * positions; block means differ only by rounding. */
+/* MCI_KERNELS_AUTO / MCI_KERNELS_GENERIC.  Both give identical accept/reject decisions,
+ * positions and results, except the block means of single-coordinate moves on the native
+ * streams, which the specialised kernel sums by dwell time (within 1e-15 relative). */
 mci_status mci_set_kernel_path(mci_ctx *ctx, int path);
 /* MCI::setTargetAcceptanceRate src/MCIntegrator.cpp:551-554 */
 mci_status mci_set_target_acceptance_rate(mci_ctx *ctx, double rate);
