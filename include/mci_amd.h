@@ -83,6 +83,7 @@ typedef int mci_status;
                                  params: a[ndim] (optional) */
 #define MCI_PDF_GMM_ISO     4 /* isotropic Gaussian mixture, log-density via log-sum-exp.
                                  params: [K, w_0..w_{K-1}, s_0..s_{K-1}, mu_0[ndim] .. mu_{K-1}[ndim]] */
+#define MCI_PDF_DEVICE    100 /* the caller's own sampling function as device code (mci_add_sampling_function_device) */
 
 /* ---- device functor catalogue (observables), ObservableFunctionInterface.hpp:29-64 ---- */
 #define MCI_OBS_X2          1 /* nobs ndim: out_i = x_i*x_i (X2 / XYZSquared) */
@@ -112,6 +113,7 @@ typedef int mci_status;
                                  DependentObservableInterface::isObsDepValid, DependentObservableInterface.hpp:36-44);
                                  in the automatic decorrelation k must be an equilibration observable too.
                                  params: [k] */
+#define MCI_OBS_DEVICE    100 /* the caller's own observable as device code (mci_add_observable_device) */
 
 /* ---- random-number modes ---- */
 #define MCI_RNG_MT19937_64  0 /* reference streams: one std::mt19937_64 per walker, libstdc++ distributions
@@ -256,6 +258,29 @@ mci_status mci_pop_observable(mci_ctx *ctx);
 mci_status mci_clear_observables(mci_ctx *ctx);
 mci_status mci_get_nobs(mci_ctx *ctx, int *nobs);
 mci_status mci_get_nobsdim(mci_ctx *ctx, int *nobsdim);
+
+/* ---- the caller's own plugins as device code: a user subclass of SamplingFunctionInterface
+ *      (SamplingFunctionInterface.hpp:74-97 with ProtoFunctionInterface.hpp:24-65) or of
+ *      ObservableFunctionInterface (ObservableFunctionInterface.hpp:48-61) added with
+ *      MCI::addSamplingFunction / MCI::addObservable (src/MCIntegrator.cpp:455-492).  `source` is
+ *      HIP C++ defining, for the C identifier `name` (vectors as MciVec, mci_amd/device_functor.hpp),
+ *        sampling function: NAME_proto (protoFunction), NAME_acceptance (acceptanceFunction),
+ *                           NAME_sampling (samplingFunction), over nproto proto values;
+ *        observable:        NAME_observable (observableFunction), nobs values;
+ *      params are passed to every call.  The chain's generic kernels are compiled with the sources
+ *      at run time (hiprtc) for the context's GPU, once per set of functors and process, at the
+ *      first integrate or at mci_build_device_functors, whose error carries the compiler's messages.
+ *      The full proto recomputation is the reference's default updatedAcceptance
+ *      (SamplingFunctionInterface.hpp:85-90); observables are evaluated in full
+ *      (flag_updateable false). ---- */
+mci_status mci_add_sampling_function_device(mci_ctx *ctx, const char *name, const char *source, int ndim, int nproto,
+                                            const double *params, int nparams);
+mci_status mci_add_observable_device(mci_ctx *ctx, const char *name, const char *source, int ndim, int nobs,
+                                     const double *params, int nparams, int blocksize, int nskip, int flag_equil, int estimator);
+mci_status mci_build_device_functors(mci_ctx *ctx);
+/* compiles one functor with the chain's kernels for gfx950 without a GPU; log (capacity logcap)
+ * receives the compiler's messages.  No counterpart in the reference: a check before a GPU run. */
+mci_status mci_check_device_functor(const char *name, const char *source, int is_observable, char *log, int64_t logcap);
 
 /* ---- integrate: MCI::integrate src/MCIntegrator.cpp:43-82 (combine over walkers:
  *      src/MPIMCI.cpp:85-92).  avg/err have nobsdim entries. ---- */

@@ -65,8 +65,23 @@ struct SamplingFunction
     int kind;
     int ndim;
     std::vector<double> params;
+    std::string name{}, source{}; // MCI_PDF_DEVICE: the functor's name and HIP source
+    int nproto = 0;               // MCI_PDF_DEVICE: its proto values
     int getNDim() const { return ndim; }
 };
+
+// a user subclass of SamplingFunctionInterface (SamplingFunctionInterface.hpp:74-97) as device code:
+// `source` defines NAME_proto, NAME_acceptance and NAME_sampling (mci_amd/device_functor.hpp), with
+// nproto proto values and the given parameters
+inline SamplingFunction DeviceSamplingFunction(const std::string &name, const std::string &source, int ndim, int nproto,
+                                               const std::vector<double> &params = {})
+{
+    SamplingFunction f{MCI_PDF_DEVICE, ndim, params};
+    f.name = name;
+    f.source = source;
+    f.nproto = nproto;
+    return f;
+}
 
 // MultiStepMove (MultiStepMove.hpp): nsteps inner Metropolis steps of a contained SRRD move under
 // the move's own sampling functions, then one outer acceptance; handed to MCI::setTrialMove
@@ -133,13 +148,26 @@ struct ObservableFunction
     int ndim;
     std::vector<double> params;
     int nobs_dep = 0; // MCI_OBS_DEP_SQUARE: the observable it squares has this many components
+                      // MCI_OBS_DEVICE: the functor's number of values
+    std::string name{}, source{}; // MCI_OBS_DEVICE: the functor's name and HIP source
     int getNDim() const { return ndim; }
     int getNObs() const
     {
-        if (kind == MCI_OBS_DEP_SQUARE) { return nobs_dep; }
+        if (kind == MCI_OBS_DEP_SQUARE || kind == MCI_OBS_DEVICE) { return nobs_dep; }
         return (kind == MCI_OBS_X2 || kind == MCI_OBS_X || kind == MCI_OBS_PDF_PROTO) ? ndim : 1;
     }
 };
+
+// a user subclass of ObservableFunctionInterface (ObservableFunctionInterface.hpp:48-61) as device
+// code: `source` defines NAME_observable (mci_amd/device_functor.hpp) with nobs values
+inline ObservableFunction DeviceObservable(const std::string &name, const std::string &source, int ndim, int nobs,
+                                           const std::vector<double> &params = {})
+{
+    ObservableFunction o{MCI_OBS_DEVICE, ndim, params, nobs};
+    o.name = name;
+    o.source = source;
+    return o;
+}
 
 // sampling functions (TestMCIFunctions.hpp:123-256 and the benchmark targets)
 inline SamplingFunction ThreeDimGaussianPDF() { return {MCI_PDF_GAUSS_ISO, 3, {}}; }       // exp(-|x|^2), d = 3
@@ -304,8 +332,14 @@ public:
     // --- observables (MCIntegrator.hpp:154-169) ---
     void addObservable(const ObservableFunction &obs, int blocksize, int nskip, bool flag_equil, EstimatorType estimType)
     {
-        check(mci_add_observable(_ctx, obs.kind, obs.ndim, obs.params.data(), (int)obs.params.size(), blocksize, nskip,
-                                 flag_equil ? 1 : 0, static_cast<int>(estimType)));
+        if (obs.kind == MCI_OBS_DEVICE) {
+            check(mci_add_observable_device(_ctx, obs.name.c_str(), obs.source.c_str(), obs.ndim, obs.nobs_dep, obs.params.data(),
+                                            (int)obs.params.size(), blocksize, nskip, flag_equil ? 1 : 0, static_cast<int>(estimType)));
+        }
+        else {
+            check(mci_add_observable(_ctx, obs.kind, obs.ndim, obs.params.data(), (int)obs.params.size(), blocksize, nskip,
+                                     flag_equil ? 1 : 0, static_cast<int>(estimType)));
+        }
         _obs.push_back(obs);
     }
     void addObservable(const ObservableFunction &obs, int blocksize, int nskip, bool flag_equil, bool flag_correlated)
@@ -332,7 +366,11 @@ public:
     // --- sampling functions (MCIntegrator.hpp:172-175) ---
     void addSamplingFunction(const SamplingFunction &pdf)
     {
-        check(mci_add_sampling_function(_ctx, pdf.kind, pdf.ndim, pdf.params.data(), (int)pdf.params.size()));
+        if (pdf.kind == MCI_PDF_DEVICE) {
+            check(mci_add_sampling_function_device(_ctx, pdf.name.c_str(), pdf.source.c_str(), pdf.ndim, pdf.nproto, pdf.params.data(),
+                                                   (int)pdf.params.size()));
+        }
+        else { check(mci_add_sampling_function(_ctx, pdf.kind, pdf.ndim, pdf.params.data(), (int)pdf.params.size())); }
         _pdfs.push_back(pdf);
     }
     SamplingFunction popSamplingFunction()

@@ -1,0 +1,32 @@
+/*
+ * device_functor.hpp — what the source of a caller's device functor sees.
+ *
+ * The reference's user plugins are C++ subclasses whose virtual methods take the walker as plain
+ * arrays (SamplingFunctionInterface.hpp:74-97, ProtoFunctionInterface.hpp:24-65,
+ * ObservableFunctionInterface.hpp:48-61).  Here the walkers are structure-of-arrays in device
+ * memory, one walker per lane, so a functor receives each vector as an MciVec: element i of this
+ * walker's vector, read or written as v[i].  A functor source (HIP C++) is handed to
+ * mci_add_sampling_function_device / mci_add_observable_device with a name NAME and defines
+ *
+ *   __device__ void   NAME_proto(int ndim, MciVec x, const double *par, MciVec proto);
+ *   __device__ double NAME_acceptance(int nproto, MciVec protoold, MciVec protonew, const double *par);
+ *   __device__ double NAME_sampling(int nproto, MciVec proto, const double *par);
+ *
+ * for a sampling function (protoFunction, acceptanceFunction, samplingFunction), or
+ *
+ *   __device__ void   NAME_observable(int ndim, MciVec x, const double *par, MciVec out);
+ *
+ * for an observable (observableFunction), par being the parameters given with it.  The library
+ * compiles the sources together with its generic kernels for the context's GPU when the chain
+ * first needs them.  This header is included ahead of the sources; it needs nothing else.
+ */
+#pragma once
+
+#include <stdint.h>
+
+struct MciVec
+{
+    double *p;  // element 0 of this walker's vector
+    int64_t s;  // distance between consecutive elements (the walker count)
+    __device__ double &operator[](int i) const { return p[(int64_t)i * s]; }
+};

@@ -6,6 +6,7 @@
 #include "mci_fast.hpp"
 #include "mci_fast_plan.hpp"
 #include "mci_host.hpp"
+#include "mci_rtc.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -68,13 +69,16 @@ struct PdfHost
 {
     int kind = 0, nproto = 0, K = 0;
     std::vector<double> params;
-    DBuf<double> dev; // a[ndim] | gc[K], gb[K], gmu[K*ndim]
+    DBuf<double> dev; // a[ndim] | gc[K], gb[K], gmu[K*ndim] | a device functor's parameters
+    std::string fname, fsrc; // MCI_PDF_DEVICE: the functor's name and source
 };
 
 struct ObsHost
 {
     int kind = 0, nobs = 0, blocksize = 1, nskip = 1, equil = 0, estim = 0;
     std::vector<double> params;
+    DBuf<double> dev;        // MCI_OBS_DEVICE: its parameters on the device
+    std::string fname, fsrc; // MCI_OBS_DEVICE: the functor's name and source
 };
 
 const char *kVersion = "mcintegratorplusplus_amd 0.1 (gfx950)";
@@ -108,6 +112,8 @@ struct mci_ctx
     std::vector<PdfHost> ms_pdfs;     // its own sampling functions
     std::vector<PdfHost> pdfs;
     std::vector<ObsHost> obs;
+    RtcKernels rtc;                   // the step kernels built with the device functors (none: null)
+    std::string rtc_text;             // their program text
 
     int device = 0;
     hipStream_t stream = nullptr;
@@ -255,6 +261,7 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
             d.nproto = p.nproto;
             d.off = moff;
             d.K = p.K;
+            d.uid = 0;
             moff += p.nproto;
             if (p.kind == MCI_PDF_GMM_ISO) {
                 d.a = nullptr;
@@ -270,7 +277,7 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     ch.lb = c->lbd.p;
     ch.ub = c->ubd.p;
     ch.npdf = (int)c->pdfs.size();
-    int off = 0;
+    int off = 0, uid = 0;
     for (size_t i = 0; i < c->pdfs.size(); ++i) {
         const PdfHost &p = c->pdfs[i];
         PdfDev &d = ch.pdf[i];
@@ -278,6 +285,7 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
         d.nproto = p.nproto;
         d.off = off;
         d.K = p.K;
+        d.uid = p.kind == MCI_PDF_DEVICE ? uid++ : 0;
         off += p.nproto;
         if (p.kind == MCI_PDF_GMM_ISO) {
             d.a = nullptr;
@@ -295,6 +303,56 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
         for (size_t i = 0; i < obsd->size(); ++i) { ch.obs[i] = (*obsd)[i]; }
     }
     return ch;
+}
+
+// a device observable's index among the context's device observables (the module's dispatch)
+int obs_uid(const mci_ctx *c, size_t i)
+{
+    int u = 0;
+    for (size_t j = 0; j < i; ++j) { u += c->obs[j].kind == MCI_OBS_DEVICE ? 1 : 0; }
+    return c->obs[i].kind == MCI_OBS_DEVICE ? u : 0;
+}
+
+// the step kernels for the context's device functors: built (or taken from the process's cache)
+// when the set of functors changed; none: the library's own kernels
+mci_status ensure_rtc(mci_ctx *c)
+{
+    std::vector<RtcFunctor> fp, fo;
+    for (const PdfHost &p : c->pdfs) {
+        if (p.kind == MCI_PDF_DEVICE) { fp.push_back({p.fname, p.fsrc}); }
+    }
+    for (const ObsHost &o : c->obs) {
+        if (o.kind == MCI_OBS_DEVICE) { fo.push_back({o.fname, o.fsrc}); }
+    }
+    if (fp.empty() && fo.empty()) {
+        c->rtc = RtcKernels{};
+        c->rtc_text.clear();
+        return MCI_OK;
+    }
+    std::string text, msg;
+    if (!rtc_program(fp, fo, text, msg)) { return fail(MCI_E_INVALID_ARGUMENT, "%s", msg.c_str()); }
+    if (text == c->rtc_text && c->rtc.sample) { return MCI_OK; }
+    RtcKernels k;
+    std::string log;
+    const int st = rtc_get(text, k, log);
+    if (st != MCI_OK) { return fail(st, "building the device functors failed:\n%s", log.c_str()); }
+    c->rtc = k;
+    c->rtc_text = text;
+    return MCI_OK;
+}
+
+hipError_t sample_launch(const mci_ctx *c, const Chain &ch, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s,
+                         const CalibCtrl *skip = nullptr)
+{
+    if (c->rtc.sample) { return rtc_launch_sample(c->rtc, ch, nsteps, acc, active, s, skip); }
+    return launch_sample(ch, nsteps, acc, active, s, skip);
+}
+
+hipError_t calibrate_indep_launch(const mci_ctx *c, const Chain &ch, int nfind, double target, int64_t min_stat, const double *dimSizes,
+                                  hipStream_t s)
+{
+    if (c->rtc.calibrate) { return rtc_launch_calibrate_indep(c->rtc, ch, nfind, target, min_stat, dimSizes, s); }
+    return launch_calibrate_indep(ch, nfind, target, min_stat, dimSizes, s);
 }
 
 double obs_p0(const ObsHost &o, int ndim)
@@ -366,7 +424,7 @@ mci_status find_shared(mci_ctx *c)
                 HIPCHK(launch_calib_round(c->acc.p, c->rej.p, c->W, c->chunks_a.p, c->target, N, c->ctrl.p, c->stream));
             }
             else {
-                HIPCHK(launch_sample(ch, min_stat, 0, nullptr, c->stream, c->ctrl.p));
+                HIPCHK(sample_launch(c, ch, min_stat, 0, nullptr, c->stream, c->ctrl.p));
                 HIPCHK(launch_walker_rates(c->acc.p, c->rej.p, c->rates.p, c->W, c->stream));
                 HIPCHK(launch_chunk_sum(c->rates.p, c->W, 1, 0, c->chunks_a.p, c->stream));
                 HIPCHK(launch_calib_control(c->chunks_a.p, c->W, c->target, N, c->ctrl.p, c->stream));
@@ -430,6 +488,8 @@ mci_status decorrelate_auto(mci_ctx *c)
         d.acc = ACC_FULL;
         d.blocksize = 1;
         d.p0 = obs_p0(o, c->ndim);
+        d.uid = obs_uid(c, i);
+        d.par = o.dev.p;
         if (o.kind == MCI_OBS_DEP_SQUARE) {
             const int k = eq_index[(size_t)d.p0];
             if (k < 0) {
@@ -461,7 +521,7 @@ mci_status decorrelate_auto(mci_ctx *c)
     Chain ch = make_chain(c, &od);
 
     auto sample_estimate = [&](const uint8_t *act) -> mci_status {
-        HIPCHK(launch_sample(ch, min_nmc, 1, act, c->stream));
+        HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
         for (size_t i = 0; i < od.size(); ++i) {
             EstimJob j;
             j.data = od[i].data;
@@ -503,7 +563,7 @@ mci_status decorrelate_auto(mci_ctx *c)
     std::vector<uint8_t> hact(W);
     while (true) {
         const uint8_t *act = shared ? nullptr : c->active.p;
-        HIPCHK(launch_sample(ch, min_nmc, 1, act, c->stream));
+        HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
         count += min_nmc;
         if (count >= cap) {
             std::printf("Warning [MCI::initialDecorrelation]: Max number of MC steps reached without equilibration.\n");
@@ -1171,7 +1231,7 @@ mci_status mci_add_observable(mci_ctx *c, int kind, int ndim, const double *para
     o.equil = flag_equil ? 1 : 0;
     o.estim = estim;
     o.params.assign(params, params + nparams);
-    c->obs.push_back(o);
+    c->obs.push_back(std::move(o));
     return MCI_OK;
 }
 
@@ -1184,6 +1244,93 @@ mci_status mci_pop_observable(mci_ctx *c)
 }
 
 mci_status mci_clear_observables(mci_ctx *c) { CHECKCTX(c); c->obs.clear(); return MCI_OK; }
+
+mci_status mci_add_sampling_function_device(mci_ctx *c, const char *name, const char *source, int ndim, int nproto, const double *params,
+                                            int nparams)
+{
+    CHECKCTX(c);
+    if (ndim != c->ndim) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addSamplingFunction] Passed sampling function's number of inputs is not equal to MCI's number of walkers.");
+    }
+    if (!name || !source || !rtc_valid_name(name)) { return fail(MCI_E_INVALID_ARGUMENT, "a device functor needs a source and a C identifier as its name"); }
+    if (nproto < 1) { return fail(MCI_E_INVALID_ARGUMENT, "a sampling function needs at least one proto value"); }
+    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
+    if ((int)c->pdfs.size() >= MAX_PDF) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d sampling functions", MAX_PDF); }
+    TRY(set_current(c));
+    PdfHost p;
+    p.kind = MCI_PDF_DEVICE;
+    p.nproto = nproto;
+    p.params.assign(params, params + nparams);
+    p.fname = name;
+    p.fsrc = source;
+    if (nparams > 0) {
+        HIPCHK(p.dev.ensure((size_t)nparams));
+        HIPCHK(hipMemcpy(p.dev.p, params, sizeof(double) * (size_t)nparams, hipMemcpyHostToDevice));
+    }
+    c->pdfs.push_back(std::move(p));
+    return ensure_protos(c);
+}
+
+mci_status mci_add_observable_device(mci_ctx *c, const char *name, const char *source, int ndim, int nobs, const double *params, int nparams,
+                                     int blocksize, int nskip, int flag_equil, int estim)
+{
+    CHECKCTX(c);
+    blocksize = std::max(0, blocksize);
+    nskip = std::max(1, nskip);
+    if (ndim != c->ndim) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Passed observable function's number of inputs is not equal to MCI's number of walkers.");
+    }
+    if (flag_equil && estim == MCI_ESTIM_NOOP) {
+        return fail(MCI_E_INVALID_ARGUMENT, "[MCI::addObservable] Requested automatic observable equilibration requires estimator with error calculation.");
+    }
+    if (estim < MCI_ESTIM_NOOP || estim > MCI_ESTIM_MJBLOCKER) { return fail(MCI_E_DOMAIN, "[createEstimator] Unhandled estimator enumerator."); }
+    if (!name || !source || !rtc_valid_name(name)) { return fail(MCI_E_INVALID_ARGUMENT, "a device functor needs a source and a C identifier as its name"); }
+    if (nobs < 1) { return fail(MCI_E_INVALID_ARGUMENT, "an observable needs at least one value"); }
+    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
+    if ((int)c->obs.size() >= MAX_OBS) { return fail(MCI_E_INVALID_ARGUMENT, "at most %d observables", MAX_OBS); }
+    TRY(set_current(c));
+    ObsHost o;
+    o.kind = MCI_OBS_DEVICE;
+    o.nobs = nobs;
+    o.blocksize = blocksize;
+    o.nskip = nskip;
+    o.equil = flag_equil ? 1 : 0;
+    o.estim = estim;
+    o.params.assign(params, params + nparams);
+    o.fname = name;
+    o.fsrc = source;
+    if (nparams > 0) {
+        HIPCHK(o.dev.ensure((size_t)nparams));
+        HIPCHK(hipMemcpy(o.dev.p, params, sizeof(double) * (size_t)nparams, hipMemcpyHostToDevice));
+    }
+    c->obs.push_back(std::move(o));
+    return MCI_OK;
+}
+
+mci_status mci_build_device_functors(mci_ctx *c)
+{
+    CHECKCTX(c);
+    TRY(set_current(c));
+    return ensure_rtc(c);
+}
+
+mci_status mci_check_device_functor(const char *name, const char *source, int is_observable, char *log, int64_t logcap)
+{
+    if (log && logcap > 0) { log[0] = '\0'; }
+    if (!name || !source || !rtc_valid_name(name)) { return fail(MCI_E_INVALID_ARGUMENT, "a device functor needs a source and a C identifier as its name"); }
+    std::vector<RtcFunctor> fp, fo;
+    (is_observable ? fo : fp).push_back({name, source});
+    std::string text, msg, lg;
+    if (!rtc_program(fp, fo, text, msg)) { return fail(MCI_E_INVALID_ARGUMENT, "%s", msg.c_str()); }
+    const int st = rtc_check(text, "gfx950", lg);
+    if (log && logcap > 0) {
+        const size_t n = std::min<size_t>(lg.size(), (size_t)logcap - 1);
+        std::memcpy(log, lg.data(), n);
+        log[n] = '\0';
+    }
+    if (st != MCI_OK) { return fail(st, "the device functor does not compile:\n%s", lg.c_str()); }
+    return MCI_OK;
+}
 mci_status mci_get_nobs(mci_ctx *c, int *n) { CHECKCTX(c); *n = (int)c->obs.size(); return MCI_OK; }
 mci_status mci_get_nobsdim(mci_ctx *c, int *n) { CHECKCTX(c); *n = nobsdim(c); return MCI_OK; }
 
@@ -1277,6 +1424,7 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
     TRY(set_current(c));
     TRY(ensure_protos(c));
     TRY(ensure_rng(c));
+    TRY(ensure_rtc(c));
     const int64_t W = c->W;
     HIPCHK(c->obsval.ensure((size_t)std::max(nd, 1) * W));
     for (double &t : c->timings) { t = 0.; }
@@ -1304,6 +1452,8 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
             d.acc = (o.blocksize == 0) ? ACC_SIMPLE : (o.blocksize == 1 ? ACC_FULL : ACC_BLOCK);
             d.blocksize = std::max(o.blocksize, 1);
             d.p0 = obs_p0(o, c->ndim);
+            d.uid = obs_uid(c, i);
+            d.par = o.dev.p;
             d.nstore = nstore_of(o.blocksize, o.nskip, nmc);
             HIPCHK(c->accdata[i].ensure((size_t)d.nstore * o.nobs * W));
             d.data = c->accdata[i].p;
@@ -1342,7 +1492,7 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
         if (c->coupling == MCI_COUPLING_SHARED && W > 1) { TRY(find_shared(c)); }
         else {
             const int64_t min_stat = (int64_t)std::fmax(100., std::sqrt(40000. * c->ndim));
-            HIPCHK(launch_calibrate_indep(make_chain(c, nullptr), c->nfind, c->target, min_stat, c->dimsizes.p, c->stream));
+            HIPCHK(calibrate_indep_launch(c, make_chain(c, nullptr), c->nfind, c->target, min_stat, c->dimsizes.p, c->stream));
         }
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
@@ -1350,7 +1500,7 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
     if (!c->pdfs.empty() && do_decorr) {
         if (c->ndecorr > 0) {
             if (fast) { TRY(fast_sample(c->fast, make_chain(c, nullptr), c->prec, c->ndecorr, nullptr, 0, c->stream)); }
-            else { HIPCHK(launch_sample(make_chain(c, nullptr), c->ndecorr, 0, nullptr, c->stream)); }
+            else { HIPCHK(sample_launch(c, make_chain(c, nullptr), c->ndecorr, 0, nullptr, c->stream)); }
         }
         else if (c->ndecorr < 0) { TRY(decorrelate_auto(c)); }
     }
@@ -1375,7 +1525,7 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
         ch.rec_fx = c->wlkfreq;
     }
     if (fast_main) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
-    else { HIPCHK(launch_sample(ch, nmc, 1, nullptr, c->stream)); }
+    else { HIPCHK(sample_launch(c, ch, nmc, 1, nullptr, c->stream)); }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     if (rec_obs || rec_x) { TRY(write_files(c, nmc, nd, rec_obs, rec_x)); }
     // ---- estimate (ObservableContainer.cpp:104-111), volume scaling (:71-77), combine ----

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "../../include/mci_amd.h"
+#include "../../include/mci_amd/device_functor.hpp"
 #include "mci_f32math.hpp"
 
 namespace mcid
@@ -32,7 +33,8 @@ enum { ACC_SIMPLE = 0, ACC_FULL = 1, ACC_BLOCK = 2 };
 struct PdfDev
 {
     int kind, nproto, off, K;
-    const double *a;   // per-dimension weights (GAUSS_DIAG, EXP_DIAG) or nullptr (= 1)
+    int uid;           // MCI_PDF_DEVICE: the functor's index in the module built with it
+    const double *a;   // per-dimension weights (GAUSS_DIAG, EXP_DIAG), the functor's parameters (DEVICE), or nullptr (= 1)
     const double *gc;  // GMM_ISO: log w_k - ndim log s_k
     const double *gb;  // GMM_ISO: 1/(2 s_k^2)
     const double *gmu; // GMM_ISO: means [K][ndim]
@@ -41,6 +43,8 @@ struct PdfDev
 struct ObsDev
 {
     int kind, nobs, voff, nskip, acc, blocksize;
+    int uid;          // MCI_OBS_DEVICE: the functor's index in the module built with it
+    const double *par; // MCI_OBS_DEVICE: its parameters
     double p0;        // first parameter (divisor, constant or index)
     int64_t nstore;   // rows of data: 1 (simple), NAccu (full), nblocks (block)
     double *data;     // [nstore][nobs][W]
@@ -352,7 +356,7 @@ __device__ __forceinline__ double pdf_a(const PdfDev &p, int i) { return p.a ? p
 
 __device__ inline double gmm_logp(const Chain &c, const PdfDev &p, const double *pv, int64_t W, int64_t w)
 {
-    double m = -INFINITY;
+    double m = -__builtin_huge_val();
     for (int k = 0; k < p.K; ++k) {
         const double t = p.gc[k] - p.gb[k] * pv[(int64_t)k * W + w];
         if (t > m) { m = t; }
@@ -388,6 +392,11 @@ __device__ inline void pdf_proto(const Chain &c, const PdfDev &p, const double *
         }
         q[(int64_t)p.K * W + w] = gmm_logp(c, p, q, W, w);
         break;
+#ifdef MCI_RTC_USER
+    case MCI_PDF_DEVICE: // the caller's protoFunction (built at run time with the functor's source)
+        mci_user_pdf_proto(p.uid, c.ndim, MciVec{const_cast<double *>(x) + w, W}, p.a, MciVec{q + w, W});
+        break;
+#endif
     }
 }
 
@@ -410,6 +419,10 @@ __device__ inline double pdf_acceptance(const Chain &c, const PdfDev &p, const d
     }
     case MCI_PDF_GMM_ISO:
         return exp(pn[(int64_t)p.K * W + w] - po[(int64_t)p.K * W + w]);
+#ifdef MCI_RTC_USER
+    case MCI_PDF_DEVICE: // the caller's acceptanceFunction
+        return mci_user_pdf_acceptance(p.uid, p.nproto, MciVec{const_cast<double *>(po) + w, W}, MciVec{const_cast<double *>(pn) + w, W}, p.a);
+#endif
     }
     return 0.0;
 }
@@ -463,6 +476,9 @@ __device__ inline double pdf_value(const Chain &c, const PdfDev &p, const double
         return exp(-s);
     }
     case MCI_PDF_GMM_ISO: return exp(q[(int64_t)p.K * W + w]);
+#ifdef MCI_RTC_USER
+    case MCI_PDF_DEVICE: return mci_user_pdf_sampling(p.uid, p.nproto, MciVec{const_cast<double *>(q) + w, W}, p.a);
+#endif
     }
     return 0.0;
 }
@@ -735,6 +751,9 @@ __device__ inline void obs_eval(const Chain &c, const ObsDev &o, int64_t w)
         const PdfDev &p = c.pdf[(int)o.p0];
         const double *pv = c.pold + (int64_t)p.off * W + w;
         if (p.kind == MCI_PDF_GMM_ISO) { out[w] = pv[(int64_t)p.K * W]; }
+#ifdef MCI_RTC_USER
+        else if (p.kind == MCI_PDF_DEVICE) { out[w] = log(mci_user_pdf_sampling(p.uid, p.nproto, MciVec{const_cast<double *>(pv), W}, p.a)); }
+#endif
         else {
             double s = 0.0;
             for (int j = 0; j < p.nproto; ++j) { s = s + pv[(int64_t)j * W]; }
@@ -752,6 +771,11 @@ __device__ inline void obs_eval(const Chain &c, const ObsDev &o, int64_t w)
         for (int j = 0; j < o.nobs; ++j) { const double y = v[(int64_t)j * W + w]; out[(int64_t)j * W + w] = y * y; }
         break;
     }
+#ifdef MCI_RTC_USER
+    case MCI_OBS_DEVICE: // the caller's observableFunction (built at run time with the functor's source)
+        mci_user_observable(o.uid, c.ndim, MciVec{const_cast<double *>(x) + w, W}, o.par, MciVec{out + w, W});
+        break;
+#endif
     }
 }
 

@@ -400,96 +400,111 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         // proposal, the acceptance uniform; a power-of-two index never rejects).  The lazy twist
         // of mci_device.hpp's mt_next, in groups of MT_G steps: the loads of a group's state words
         // are issued a group ahead, so they are in flight while the group before runs.
-        constexpr int MT_G = 4, NO = 3 * MT_G;
+        constexpr int MT_G = 8, NO = 3 * MT_G;
         const int64_t W = A.W;
-        uint64_t *mt = A.mt;
-        int k = A.mtp[w];
-        uint64_t xk = mt[(int64_t)k * W + w];   // the current value of element k
-        uint64_t x1[NO], xm[NO], o[NO];
-        auto issue = [&](int kk) {
+        // The walkers of a wave are usually at the same state position (every step takes three
+        // outputs; seeding starts all at 0); then the position is wave-uniform and every state
+        // access is a uniform row pointer plus the lane (scalar address arithmetic).
+        const int kw = A.mtp[w];
+        const int kf = __builtin_amdgcn_readfirstlane(kw);
+        const bool uniform = __builtin_amdgcn_ballot_w64(kw != kf) == 0;
+        const int64_t wbase = (int64_t)blockIdx.x * WAVE;
+        auto run = [&](auto uni_c) {
+            constexpr bool UNI = decltype(uni_c)::value;
+            auto at = [&](int kk) -> uint64_t * {
+                if constexpr (UNI) { return (A.mt + ((int64_t)kk * W + wbase)) + lane; }
+                else { return A.mt + ((int64_t)kk * W + w); }
+            };
+            int k = UNI ? kf : kw;
+            uint64_t xk = *at(k);   // the current value of element k
+            uint64_t x1[NO], xm[NO], o[NO];
+            auto issue = [&](int kk) {
 #pragma unroll
-            for (int j = 0; j < NO; ++j) {
-                const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
-                const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
-                x1[j] = mt[(int64_t)k1 * W + w];
-                xm[j] = mt[(int64_t)km * W + w];
-                kk = k1;
-            }
-        };
-        issue(k);
-        for (int64_t s = 0; s < A.nsteps; s += MT_G) {
-            const int ng = (int)(A.nsteps - s < MT_G ? A.nsteps - s : MT_G);
-            // this group's outputs, exactly the 3 ng it consumes (the state advances by those)
-#pragma unroll
-            for (int j = 0; j < NO; ++j) {
-                if (j < 3 * ng) {
-                    const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
-                    mt[(int64_t)k * W + w] = nv;
-                    o[j] = mt_temper(nv);
-                    k = (k + 1 == MT_N) ? 0 : k + 1;
+                for (int j = 0; j < NO; ++j) {
+                    const int k1 = (kk + 1 == MT_N) ? 0 : kk + 1;
+                    const int km = (kk < MT_N - MT_M) ? kk + MT_M : kk + MT_M - MT_N;
+                    x1[j] = *at(k1);
+                    xm[j] = *at(km);
+                    kk = k1;
                 }
-            }
+            };
+            issue(k);
+            for (int64_t s = 0; s < A.nsteps; s += MT_G) {
+                const int ng = (int)(A.nsteps - s < MT_G ? A.nsteps - s : MT_G);
+                // this group's outputs, exactly the 3 ng it consumes (the state advances by those)
 #pragma unroll
-            for (int j = 0; j < NO; ++j) {
-                if (j == 3 * ng - 1) { xk = x1[j]; }
-            }
-            if (s + MT_G < A.nsteps) { issue(k); }
-            R.na = 0;
+                for (int j = 0; j < NO; ++j) {
+                    if (j < 3 * ng) {
+                        const uint64_t nv = mt_twist(j == 0 ? xk : x1[j - 1], x1[j], xm[j]);
+                        *at(k) = nv;
+                        o[j] = mt_temper(nv);
+                        k = (k + 1 == MT_N) ? 0 : k + 1;
+                    }
+                }
 #pragma unroll
-            for (int j = 0; j < MT_G; ++j) {
-                if (j < ng) {
-                    const int v = A.ibits > 0 ? (int)(o[3 * j] >> (64 - A.ibits)) : 0;
-                    const int sl = v * WAVE + lane;
-                    double xo, So = 0.0;
-                    if (REC) { const double2 rv = rec[sl]; xo = rv.x; So = rv.y; }
-                    else { xo = xs[sl]; }
-                    const bool moved = step64<FL>(A, rec, xs, as, R, v, sl, xo, So, cs[v], o[3 * j + 2], o[3 * j + 1], 0, s + j, 0u,
-                                                  step, lane, w);
-                    if (ACC) {
-                        // the reference's per-step sums (BlockAccumulator / SimpleAccumulator): every
-                        // component's value after the step added in step order, so the block means
-                        // are bit-identical to the reference's
-                        double q = 0.0;
-                        for (int i = 0; i < d; ++i) {
-                            if (REC) {
-                                double2 rv = rec[i * WAVE + lane];
-                                const double q2 = rv.x * rv.x;
-                                q = q + q2;
-                                rv.y = rv.y + q2;
-                                rec[i * WAVE + lane] = rv;
-                            }
-                            else {
-                                const double xv = xs[i * WAVE + lane];
-                                q = q + xv * xv;
-                            }
-                        }
-                        if (HSUM) { R.Sr = R.Sr + q / A.x2sum_div; }
-                        if (HEXP) {
-                            if (moved) { R.e = exp(-(q / A.exp_div)); }
-                            R.Se = R.Se + R.e;
-                        }
-                        if (++inblk == B) {
-                            const double norm = 1. / (double)B;
-                            if (REC) {
-                                for (int i = 0; i < d; ++i) {
-                                    const double2 rv = rec[i * WAVE + lane];
-                                    A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
-                                    rec[i * WAVE + lane] = make_double2(rv.x, 0.0);
+                for (int j = 0; j < NO; ++j) {
+                    if (j == 3 * ng - 1) { xk = x1[j]; }
+                }
+                if (s + MT_G < A.nsteps) { issue(k); }
+                R.na = 0;
+#pragma unroll
+                for (int j = 0; j < MT_G; ++j) {
+                    if (j < ng) {
+                        const int v = A.ibits > 0 ? (int)(o[3 * j] >> (64 - A.ibits)) : 0;
+                        const int sl = v * WAVE + lane;
+                        double xo, So = 0.0;
+                        if (REC) { const double2 rv = rec[sl]; xo = rv.x; So = rv.y; }
+                        else { xo = xs[sl]; }
+                        const bool moved = step64<FL>(A, rec, xs, as, R, v, sl, xo, So, cs[v], o[3 * j + 2], o[3 * j + 1], 0, s + j, 0u,
+                                                      step, lane, w);
+                        if (ACC) {
+                            // the reference's per-step sums (BlockAccumulator / SimpleAccumulator): every
+                            // component's value after the step added in step order, so the block means
+                            // are bit-identical to the reference's
+                            double q = 0.0;
+                            for (int i = 0; i < d; ++i) {
+                                if (REC) {
+                                    double2 rv = rec[i * WAVE + lane];
+                                    const double q2 = rv.x * rv.x;
+                                    q = q + q2;
+                                    rv.y = rv.y + q2;
+                                    rec[i * WAVE + lane] = rv;
+                                }
+                                else {
+                                    const double xv = xs[i * WAVE + lane];
+                                    q = q + xv * xv;
                                 }
                             }
-                            if (HSUM) { A.d_x2sum[b * A.W + w] = R.Sr * norm; }
-                            if (HEXP) { A.d_exp[b * A.W + w] = R.Se * norm; }
-                            R.Sr = 0.0;
-                            R.Se = 0.0;
-                            inblk = 0;
-                            ++b;
+                            if (HSUM) { R.Sr = R.Sr + q / A.x2sum_div; }
+                            if (HEXP) {
+                                if (moved) { R.e = exp(-(q / A.exp_div)); }
+                                R.Se = R.Se + R.e;
+                            }
+                            if (++inblk == B) {
+                                const double norm = 1. / (double)B;
+                                if (REC) {
+                                    for (int i = 0; i < d; ++i) {
+                                        const double2 rv = rec[i * WAVE + lane];
+                                        A.d_x2[(b * d + i) * A.W + w] = rv.y * norm;
+                                        rec[i * WAVE + lane] = make_double2(rv.x, 0.0);
+                                    }
+                                }
+                                if (HSUM) { A.d_x2sum[b * A.W + w] = R.Sr * norm; }
+                                if (HEXP) { A.d_exp[b * A.W + w] = R.Se * norm; }
+                                R.Sr = 0.0;
+                                R.Se = 0.0;
+                                inblk = 0;
+                                ++b;
+                            }
                         }
                     }
                 }
+                nacc += R.na;
             }
-            nacc += R.na;
-        }
-        A.mtp[w] = k;
+            A.mtp[w] = k;
+        };
+        if (uniform) { run(std::integral_constant<bool, true>{}); }
+        else { run(std::integral_constant<bool, false>{}); }
     }
     else {
         // Blocks 0 and (without a shared index) 1 of the current step pair t / 2, as pair64 takes them:

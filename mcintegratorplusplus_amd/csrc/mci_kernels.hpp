@@ -33,6 +33,7 @@ struct EstimJob
     double *avg, *err;    // [nobsdim][W]
 };
 
+#ifndef MCI_RTC_USER // the host's launch interface (not part of the run-time build of the step kernels)
 hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s);
 hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s,
                          const CalibCtrl *skip = nullptr);
@@ -52,4 +53,5 @@ hipError_t launch_decorr_compare(double *oa, double *oe, const double *na, const
 hipError_t launch_move_x(const Chain &c, hipStream_t s);
 hipError_t launch_new_random_x(const Chain &c, hipStream_t s);
 hipError_t launch_apply_domain(const Chain &c, hipStream_t s);
+#endif
 } // namespace mcid

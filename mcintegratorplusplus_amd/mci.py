@@ -32,6 +32,9 @@ _SIGS = {
     "mci_get_acceptance": [_P, _I64P, _I64P, _DP], "mci_get_walker_acceptance": [_P, _I64P, _I64P, _I64],
     "mci_get_timings": [_P, _DP, _I], "mci_estimate": [_I, _I64, _I, _DP, _DP, _DP], "mci_device_count": [_IP],
     "mci_get_state_size": [_P, _I64P], "mci_get_state": [_P, C.c_void_p, _I64], "mci_set_state": [_P, C.c_void_p, _I64],
+    "mci_add_sampling_function_device": [_P, C.c_char_p, C.c_char_p, _I, _I, _DP, _I],
+    "mci_add_observable_device": [_P, C.c_char_p, C.c_char_p, _I, _I, _DP, _I, _I, _I, _I, _I],
+    "mci_build_device_functors": [_P], "mci_check_device_functor": [C.c_char_p, C.c_char_p, _I, C.c_char_p, _I64],
 }
 _bound = None
 
@@ -78,6 +81,14 @@ def estimate(estimator, x):
     a, e = np.zeros(x.shape[1]), np.zeros(x.shape[1])
     _check(_lib().mci_estimate(estimator, x.shape[0], x.shape[1], _dp(x), _dp(a), _dp(e)))
     return a, e
+
+
+def check_device_functor(name, source, is_observable=False):
+    """Compile a device functor with the chain's kernels for gfx950 (no GPU needed); raises
+    MciError with the compiler's messages if it does not build."""
+    log = C.create_string_buffer(1 << 16)
+    _check(_lib().mci_check_device_functor(name.encode(), source.encode(), int(bool(is_observable)), log, len(log)))
+    return log.value.decode()
 
 
 class MCI:
@@ -162,6 +173,15 @@ class MCI:
         p = np.ascontiguousarray(params, dtype=np.float64)
         self._c("mci_add_sampling_function", kind, self.ndim, _dp(p) if p.size else None, p.size)
 
+    def addSamplingFunctionDevice(self, name, source, nproto, params=()):
+        """A sampling function of the caller's as device code (SamplingFunctionInterface's protoFunction,
+        acceptanceFunction and samplingFunction as NAME_proto, NAME_acceptance, NAME_sampling over MciVec
+        views, include/mci_amd/device_functor.hpp), compiled with the chain's kernels at the first integrate."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._c("mci_add_sampling_function_device", name.encode(), source.encode(), self.ndim, nproto, _dp(p) if p.size else None, p.size)
+
+    def buildDeviceFunctors(self): self._c("mci_build_device_functors")
+
     def popSamplingFunction(self): self._c("mci_pop_sampling_function")
     def clearSamplingFunctions(self): self._c("mci_clear_sampling_functions")
 
@@ -180,6 +200,22 @@ class MCI:
         p = np.ascontiguousarray(params, dtype=np.float64)
         self._c("mci_add_observable", kind, self.ndim, _dp(p) if p.size else None, p.size, blocksize, nskip,
                 int(bool(flag_equil)), estimator)
+
+    def addObservableDevice(self, name, source, nobs, params=(), blocksize=1, nskip=1, flag_equil=None, flag_correlated=None, estimator=None):
+        """An observable of the caller's as device code (observableFunction as NAME_observable), with
+        addObservable's accumulator and estimator arguments."""
+        if flag_equil is None:
+            flag_equil = blocksize > 0
+        if estimator is None:
+            if flag_correlated is None:
+                flag_correlated = blocksize == 1
+            flag_error = blocksize > 0
+            if flag_correlated and not flag_error:
+                raise MciError(K.E_INVALID_ARGUMENT, "[selectEstimatorType] Error calculation is set off, but correlated error estimation is set on.")
+            estimator = K.ESTIM_CORRELATED if flag_correlated else (K.ESTIM_UNCORRELATED if flag_error else K.ESTIM_NOOP)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._c("mci_add_observable_device", name.encode(), source.encode(), self.ndim, nobs, _dp(p) if p.size else None, p.size,
+                blocksize, nskip, int(bool(flag_equil)), estimator)
 
     def popObservable(self): self._c("mci_pop_observable")
     def clearObservables(self): self._c("mci_clear_observables")

@@ -39,6 +39,9 @@ namespace
 enum { PDF_GAUSS_ISO = 1, PDF_GAUSS_DIAG = 2, PDF_EXP_DIAG = 3, PDF_GMM_ISO = 4, PDF_GMM_FULL = 5 };
 enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_EXP_X2 = 6, OBS_INDICATOR = 7, OBS_GAUSS_X2 = 8,
        OBS_PDF_LOG = 9, OBS_PDF_PROTO = 10, OBS_DEP_SQUARE = 11 };
+// plugins outside the library's catalogue, written as a reference user writes them (the fixtures of
+// the library's device functors, which take them as source)
+enum { PDF_USER_QUARTIC = 101, OBS_USER_MOMENTS = 101 };
 
 std::string hx(double v)
 {
@@ -122,6 +125,32 @@ public:
             e += pn[j] - po[j];
         }
         return exp(-e);
+    }
+};
+
+// p(x) ~ exp(-sum_i c_i x_i^4), one proto value per coordinate, no updatedAcceptance of its own
+// (the interface's default: full recomputation, SamplingFunctionInterface.hpp:85-90)
+class QuarticPDF final: public ObservedPDF
+{
+    std::vector<double> _c;
+protected:
+    mci::SamplingFunctionInterface *_clone() const final { return new QuarticPDF(_ndim, _c); }
+public:
+    QuarticPDF(int nd, std::vector<double> c): ObservedPDF(nd, nd), _c(std::move(c)) {}
+    double logValue(const double pv[]) const final { return -std::accumulate(pv, pv + _nproto, 0.); }
+    void protoFunction(const double in[], double pv[]) final
+    {
+        for (int i = 0; i < _ndim; ++i) {
+            const double q = in[i]*in[i];
+            pv[i] = (q*q)*_c[i];
+        }
+    }
+    double samplingFunction(const double pv[]) const final { return exp(-std::accumulate(pv, pv + _nproto, 0.)); }
+    double acceptanceFunction(const double po[], const double pn[]) const final
+    {
+        double e = std::accumulate(po, po + _nproto, 0.);
+        e -= std::accumulate(pn, pn + _nproto, 0.);
+        return exp(e);
     }
 };
 
@@ -289,6 +318,26 @@ public:
     }
 };
 
+// sum_i |x_i|^3 and max_i |x_i|, not updateable
+class MomentsObs final: public mci::ObservableFunctionInterface
+{
+protected:
+    mci::ObservableFunctionInterface *_clone() const final { return new MomentsObs(_ndim); }
+public:
+    explicit MomentsObs(int nd): mci::ObservableFunctionInterface(nd, 2, false) {}
+    void observableFunction(const double in[], double out[]) final
+    {
+        double s = 0., m = 0.;
+        for (int i = 0; i < _ndim; ++i) {
+            const double a = fabs(in[i]);
+            s = s + (a*a)*a;
+            m = (a > m) ? a : m;
+        }
+        out[0] = s;
+        out[1] = m;
+    }
+};
+
 // dependent observables (DependentObservableInterface.hpp:24-54): the log value or the proto values of
 // sampling function j as its observation callback last saw them, and the squares of the last values
 // of an earlier observable k (read through its accumulator)
@@ -341,6 +390,7 @@ std::unique_ptr<mci::SamplingFunctionInterface> makePDF(int nd, int kind, const 
     case PDF_GAUSS_ISO: return std::unique_ptr<mci::SamplingFunctionInterface>(new GaussIso(nd));
     case PDF_GAUSS_DIAG: return std::unique_ptr<mci::SamplingFunctionInterface>(new DiagPDF(nd, false, p));
     case PDF_EXP_DIAG: return std::unique_ptr<mci::SamplingFunctionInterface>(new DiagPDF(nd, true, p));
+    case PDF_USER_QUARTIC: return std::unique_ptr<mci::SamplingFunctionInterface>(new QuarticPDF(nd, p));
     case PDF_GMM_ISO: {
         const int K = static_cast<int>(p[0]);
         std::vector<double> w(p.begin() + 1, p.begin() + 1 + K), s(p.begin() + 1 + K, p.begin() + 1 + 2*K);
@@ -496,6 +546,7 @@ int main()
                     const int k = static_cast<int>(p.at(0));
                     o.reset(new DepSquareObs(ndim, obs_nobs.at(k), k));
                 }
+                else if (kind == OBS_USER_MOMENTS) { o.reset(new MomentsObs(ndim)); }
                 else { o.reset(new CatObs(ndim, kind, p)); }
                 obs_nobs.push_back(o->getNObs());
                 mci->addObservable(std::move(o), bs, nskip, eq != 0, estimOf(est));

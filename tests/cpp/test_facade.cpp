@@ -299,6 +299,53 @@ static void learned_mixture_move()
     std::printf("learned mixture move ok: P(x0 > 0) = %.4f +- %.4f\n", average, error);
 }
 
+// a reference user's own plugins written as device code: exp(-|x|^2) in ThreeDimGaussianPDF's
+// arithmetic and x_i^2 in XYZSquared's give the catalogue's chain bit for bit
+static void device_functors()
+{
+    const char *pdf = R"(
+__device__ void gauss3_proto(int ndim, MciVec x, const double *par, MciVec proto)
+{
+    double s = x[0] * x[0];
+    for (int i = 1; i < ndim; ++i) { s = s + x[i] * x[i]; }
+    proto[0] = s;
+}
+__device__ double gauss3_acceptance(int nproto, MciVec po, MciVec pn, const double *par) { return exp(-pn[0] + po[0]); }
+__device__ double gauss3_sampling(int nproto, MciVec pv, const double *par) { return exp(-pv[0]); }
+)";
+    const char *obs = R"(
+__device__ void sq_observable(int ndim, MciVec x, const double *par, MciVec out)
+{
+    for (int i = 0; i < ndim; ++i) { out[i] = x[i] * x[i]; }
+}
+)";
+    double a1[3], e1[3], a2[3], e2[3];
+    {
+        MCI mci(3);
+        mci.setSeed(5649871);
+        mci.addSamplingFunction(DeviceSamplingFunction("gauss3", pdf, 3, 1));
+        mci.addObservable(DeviceObservable("sq", obs, 3, 3), 16, 1, true, EstimatorType::MJBlocker);
+        mci.integrate(16 * 1024, a1, e1);
+    }
+    {
+        MCI mci(3);
+        mci.setSeed(5649871);
+        mci.addSamplingFunction(ThreeDimGaussianPDF());
+        mci.addObservable(XYZSquared(), 16, 1, true, EstimatorType::MJBlocker);
+        mci.integrate(16 * 1024, a2, e2);
+    }
+    for (int i = 0; i < 3; ++i) {
+        CHECK(a1[i] == a2[i] && e1[i] == e2[i]);
+        CHECK(std::fabs(a1[i] - 0.5) < 4. * e1[i]);
+    }
+    MCI bad(2);
+    bad.addSamplingFunction(DeviceSamplingFunction("nope", "__device__ void nope_proto(int n, MciVec x, const double *p, MciVec q) { q[0] = z; }", 2, 1));
+    bad.addObservable(DeviceObservable("sq", obs, 2, 2), 16, 1, false, EstimatorType::Uncorrelated);
+    double a[2], e[2];
+    CHECK(throws<std::invalid_argument>([&] { bad.integrate(256, a, e); }));
+    std::printf("device functors ok: <x_i^2> = %.4f %.4f %.4f\n", a1[0], a1[1], a1[2]);
+}
+
 int main()
 {
     int ndev = 0;
@@ -317,6 +364,7 @@ int main()
     multistep();
     callback();
     learned_mixture_move();
+    device_functors();
     std::printf("all %d checks passed\n", g_checks);
     return 0;
 }

@@ -1,0 +1,80 @@
+"""Device functor sources of the tests, written as a caller writes them for
+mci_add_sampling_function_device / mci_add_observable_device (include/mci_amd/device_functor.hpp):
+
+- the reference harness's plugins outside the library's catalogue (oracle/ref_harness.cpp
+  QuarticPDF, MomentsObs), whose chains the reference wrote into tests/golden/device_functors.json;
+- restatements of the catalogue's diagonal Gaussian and x_i^2, which must give the catalogue's
+  chains bit for bit (all-coordinate moves recompute every proto value in both)."""
+
+PDF_USER_QUARTIC = 101   # the harness's kind codes
+OBS_USER_MOMENTS = 101
+
+QUARTIC = r"""
+__device__ void quartic_proto(int ndim, MciVec x, const double *par, MciVec proto)
+{
+    for (int i = 0; i < ndim; ++i) {
+        const double q = x[i] * x[i];
+        proto[i] = (q * q) * par[i];
+    }
+}
+__device__ double quartic_acceptance(int nproto, MciVec po, MciVec pn, const double *par)
+{
+    double e = 0.0;
+    for (int i = 0; i < nproto; ++i) { e = e + po[i]; }
+    double s = 0.0;
+    for (int i = 0; i < nproto; ++i) { s = s + pn[i]; }
+    return exp(e - s);
+}
+__device__ double quartic_sampling(int nproto, MciVec pv, const double *par)
+{
+    double s = 0.0;
+    for (int i = 0; i < nproto; ++i) { s = s + pv[i]; }
+    return exp(-s);
+}
+"""
+
+MOMENTS = r"""
+__device__ void moments_observable(int ndim, MciVec x, const double *par, MciVec out)
+{
+    double s = 0.0, m = 0.0;
+    for (int i = 0; i < ndim; ++i) {
+        const double a = fabs(x[i]);
+        s = s + (a * a) * a;
+        m = (a > m) ? a : m;
+    }
+    out[0] = s;
+    out[1] = m;
+}
+"""
+
+DIAG = r"""
+__device__ void diag_proto(int ndim, MciVec x, const double *par, MciVec proto)
+{
+    for (int i = 0; i < ndim; ++i) { const double v = x[i]; proto[i] = (v * v) * par[i]; }
+}
+__device__ double diag_acceptance(int nproto, MciVec po, MciVec pn, const double *par)
+{
+    double e = 0.0;
+    for (int i = 0; i < nproto; ++i) { e = e + po[i]; }
+    double s = 0.0;
+    for (int i = 0; i < nproto; ++i) { s = s + pn[i]; }
+    return exp(e - s);
+}
+__device__ double diag_sampling(int nproto, MciVec pv, const double *par)
+{
+    double s = 0.0;
+    for (int i = 0; i < nproto; ++i) { s = s + pv[i]; }
+    return exp(-s);
+}
+"""
+
+X2 = r"""
+__device__ void xsq_observable(int ndim, MciVec x, const double *par, MciVec out)
+{
+    for (int i = 0; i < ndim; ++i) { const double v = x[i]; out[i] = v * v; }
+}
+"""
+
+# kind code -> (name, source, number of proto values or observable values for ndim)
+PDFS = {PDF_USER_QUARTIC: ("quartic", QUARTIC, lambda d: d)}
+OBS = {OBS_USER_MOMENTS: ("moments", MOMENTS, lambda d: 2)}

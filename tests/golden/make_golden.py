@@ -190,6 +190,26 @@ def run_scenario(s):
     return recs
 
 
+def device_functor_scenarios():
+    """Chains of plugins outside the library's catalogue, written as a reference user writes them
+    (oracle/ref_harness.cpp QuarticPDF: p ~ exp(-sum c_i x_i^4) without an updatedAcceptance of its
+    own; MomentsObs: sum |x_i|^3 and max |x_i|, not updateable).  The library runs them as device
+    functors compiled from source (tests/device_functors.py)."""
+    from device_functors import OBS_USER_MOMENTS, PDF_USER_QUARTIC
+    S = []
+    S.append(dict(name="quartic_allmove", cfg=dict(ndim=4, seed=2024, ndecorr=500, pdfs=[(PDF_USER_QUARTIC, [0.5, 1.0, 2.0, 0.25])],
+                                                   obs=[(OBS_USER_MOMENTS, [], 16, 1, 1, ESTIM_MJ), (OBS_X2, [], 16, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=16 * 64, find=1, decorr=1), dict(nmc=16 * 32, find=0, decorr=0)]))
+    S.append(dict(name="quartic_vecmove_full", cfg=dict(ndim=3, seed=77, ndecorr=300, move=(0, 1, 1, None),
+                                                        pdfs=[(PDF_USER_QUARTIC, [1.0, 1.0, 0.5])],
+                                                        obs=[(OBS_USER_MOMENTS, [], 1, 1, 1, ESTIM_CORR)]),
+                  calls=[dict(nmc=1024, find=1, decorr=1)]))
+    S.append(dict(name="quartic_auto_decorrelation", cfg=dict(ndim=2, seed=5649871, pdfs=[(PDF_USER_QUARTIC, [1.0, 2.0])],
+                                                              obs=[(OBS_USER_MOMENTS, [], 8, 1, 1, ESTIM_MJ)], x0=[2.0, -1.5]),
+                  calls=[dict(nmc=8 * 128, find=1, decorr=1)]))
+    return [dict(name=s["name"], cfg=s["cfg"], calls=s["calls"], results=run_scenario(s)) for s in S]
+
+
 def file_scenarios():
     """The observable and walker files of the main sampling (MCIntegrator.cpp:495-542) as the
     reference writes them (calibration and decorrelation on: they write nothing)."""
@@ -297,6 +317,8 @@ def main():
         json.dump(dict(meta=meta, scenarios=chains), f, indent=0)
     with open(os.path.join(HERE, "traces.json"), "w") as f:
         json.dump(dict(meta=meta, traces=trace_scenarios(), phase_traces=phase_trace_scenarios()), f, indent=0)
+    with open(os.path.join(HERE, "device_functors.json"), "w") as f:
+        json.dump(dict(meta=meta, scenarios=device_functor_scenarios()), f, indent=0)
     with open(os.path.join(HERE, "files.json"), "w") as f:
         json.dump(dict(meta=meta, files=file_scenarios()), f, indent=0)
     with open(os.path.join(HERE, "walkers.json"), "w") as f:

@@ -106,6 +106,9 @@ MCI_ONLY = {
     "clear_observable_file": [],
     "store_walker_positions_on_file": [C.c_char_p, _I],
     "clear_walker_file": [],
+    "add_sampling_function_device": [C.c_char_p, C.c_char_p, _I, _I, _DP, _I],
+    "add_observable_device": [C.c_char_p, C.c_char_p, _I, _I, _DP, _I, _I, _I, _I, _I],
+    "build_device_functors": [],
 }
 
 
@@ -287,6 +290,19 @@ class Chain:
         p = f64(params)
         self.call("add_sampling_function", kind, self.ndim, dptr(p) if p.size else None, p.size)
 
+    def addSamplingFunctionDevice(self, name, source, nproto, params=()):
+        p = f64(params)
+        self.call("add_sampling_function_device", name.encode(), source.encode(), self.ndim, nproto, dptr(p) if p.size else None, p.size)
+
+    def addObservableDevice(self, name, source, nobs, params=(), blocksize=1, nskip=1, flag_equil=None, estim=None):
+        if flag_equil is None:
+            flag_equil = blocksize > 0
+        if estim is None:
+            estim = ESTIM_CORR if blocksize == 1 else (ESTIM_UNCORR if blocksize > 0 else ESTIM_NOOP)
+        p = f64(params)
+        self.call("add_observable_device", name.encode(), source.encode(), self.ndim, nobs, dptr(p) if p.size else None, p.size,
+                  blocksize, nskip, int(bool(flag_equil)), estim)
+
     def popSamplingFunction(self): self.call("pop_sampling_function")
     def clearSamplingFunctions(self): self.call("clear_sampling_functions")
 
@@ -445,10 +461,20 @@ def chain_setup(prefix, cfg, nwalkers=1):
     if "irange" in cfg:
         c.setIRange(*cfg["irange"])
     for kind, params in cfg.get("pdfs", []):
-        c.addSamplingFunction(kind, params)
+        if kind >= 100:  # a plugin outside the catalogue: its device functor source
+            import device_functors as DF
+            name, src, nproto = DF.PDFS[kind]
+            c.addSamplingFunctionDevice(name, src, nproto(cfg["ndim"]), params)
+        else:
+            c.addSamplingFunction(kind, params)
     for o in cfg.get("obs", []):
         kind, params, bs, nskip, eq, est = o
-        c.addObservable(kind, params, bs, nskip, eq, est)
+        if kind >= 100:
+            import device_functors as DF
+            name, src, nobs = DF.OBS[kind]
+            c.addObservableDevice(name, src, nobs(cfg["ndim"]), params, bs, nskip, eq, est)
+        else:
+            c.addObservable(kind, params, bs, nskip, eq, est)
     if "x0" in cfg:
         c.setX(cfg["x0"])
     if "step" in cfg:

@@ -676,13 +676,15 @@ def test_all_coordinate_moves_on_the_specialised_kernel(rng, d, blocksize, coupl
             assert np.array_equal(f[k], o[k]), k
 
 
-def test_all_coordinate_moves_after_generator_positions_diverge():
+@pytest.mark.parametrize("veclen,d", [(0, 6), (1, 8)])
+def test_specialised_moves_after_generator_positions_diverge(veclen, d):
     """Reference streams whose walkers sit at different generator positions (Gaussian moves take a
-    varying number of outputs per step), then uniform all-coordinate moves on the specialised kernel:
-    its per-lane-position path, bit-identical to the generic kernel on the same sequence."""
+    varying number of outputs per step), then uniform all-coordinate (veclen 0) or single-coordinate
+    (veclen 1) moves on the specialised kernels: their per-lane-position paths, bit-identical to the
+    generic kernel on the same sequence."""
     from mcitest import SRRD_GAUSSIAN
-    d, W = 6, 150
-    print("all-coordinate moves after diverged generator positions", d, W, flush=True)
+    W = 150
+    print("specialised moves after diverged generator positions", veclen, d, W, flush=True)
     sig = np.random.default_rng(d).uniform(0.5, 2.0, d)
     cfg = dict(ndim=d, seed=4242, ndecorr=100, move=(SRRD_GAUSSIAN, 0, 1, None), pdfs=[(PDF_GAUSS_DIAG, 0.5 / sig ** 2)],
                obs=[(OBS_X2, [], 20, 1, 1, ESTIM_MJ), (OBS_X2_SUM, [], 20, 1, 1, ESTIM_MJ)])
@@ -693,7 +695,7 @@ def test_all_coordinate_moves_after_generator_positions_diverge():
         if name == "generic":
             c.setKernelPath(KERNELS_GENERIC)
         a0, e0 = c.integrate(64 * 20)
-        c.setTrialMove(0, 0, 1, None)
+        c.setTrialMove(0, veclen, 1, None)
         a, e = c.integrate(64 * 20)
         a2, e2 = c.integrate(32 * 20, False, False)
         out[name] = (a0, e0, a, e, a2, e2, *c.walkerResults(), c.getWalkerSteps(), c.getX(), *c.walkerAcceptance(),
