@@ -12,7 +12,14 @@
  *   __device__ double NAME_acceptance(int nproto, MciVec protoold, MciVec protonew, const double *par);
  *   __device__ double NAME_sampling(int nproto, MciVec proto, const double *par);
  *
- * for a sampling function (protoFunction, acceptanceFunction, samplingFunction), or
+ * for a sampling function (protoFunction, acceptanceFunction, samplingFunction) and, if it
+ * overrides updatedAcceptance (SamplingFunctionInterface.hpp:74-91: the acceptance after a move of
+ * some coordinates, updating only the proto values those change),
+ *
+ *   __device__ double NAME_updated_acceptance(int ndim, int nproto, MciVec xnew, MciChanged changed,
+ *                                             MciVec protoold, MciVec protonew, const double *par);
+ *
+ * (without it the proto values are recomputed in full, the interface's default), or
  *
  *   __device__ void   NAME_observable(int ndim, MciVec x, const double *par, MciVec out);
  *
@@ -29,4 +36,12 @@ struct MciVec
     double *p;  // element 0 of this walker's vector
     int64_t s;  // distance between consecutive elements (the walker count)
     __device__ double &operator[](int i) const { return p[(int64_t)i * s]; }
+};
+
+// the coordinates a move changed, in ascending order (WalkerState::changedIdx / nchanged):
+// changed[i] for i < changed.n
+struct MciChanged
+{
+    int first, n;
+    __device__ int operator[](int i) const { return first + i; }
 };

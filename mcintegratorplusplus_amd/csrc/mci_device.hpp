@@ -431,6 +431,15 @@ __device__ inline double pdf_acceptance(const Chain &c, const PdfDev &p, const d
 __device__ inline double pdf_compute_acceptance(const Chain &c, const PdfDev &p, const Lane &L, double *pold, double *pnew)
 {
     const int64_t W = c.W, w = L.w;
+#ifdef MCI_RTC_USER
+    // the caller's updatedAcceptance when it defines one (else its protoFunction and
+    // acceptanceFunction, the interface's default), for the changed coordinates
+    if (p.kind == MCI_PDF_DEVICE && L.nchanged < c.ndim) {
+        double *po = pold + (int64_t)p.off * W + w, *pn = pnew + (int64_t)p.off * W + w;
+        return mci_user_pdf_updated(p.uid, c.ndim, p.nproto, MciVec{c.xnew + w, W}, MciChanged{L.cfirst, L.nchanged}, MciVec{po, W},
+                                    MciVec{pn, W}, p.a);
+    }
+#endif
     if (L.nchanged < c.ndim) {
         double *po = pold + (int64_t)p.off * W;
         double *pn = pnew + (int64_t)p.off * W;

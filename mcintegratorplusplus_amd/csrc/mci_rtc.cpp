@@ -132,6 +132,17 @@ bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunct
              "_sampling");
     dispatch(text, "void mci_user_observable(int uid, int ndim, MciVec x, const double *par, MciVec out)", "(ndim, x, par, out)", "", obs,
              "_observable");
+    // updatedAcceptance: the caller's overload when the source defines one (a non-template function
+    // is preferred to this template of the same signature), else the interface's default
+    std::set<std::string> upd;
+    for (const RtcFunctor &f : pdfs) {
+        if (!upd.insert(f.name).second) { continue; }
+        text += "template <int = 0>\n__device__ inline double " + f.name +
+                "_updated_acceptance(int ndim, int nproto, MciVec xnew, MciChanged, MciVec po, MciVec pn, const double *par)\n{\n    " +
+                f.name + "_proto(ndim, xnew, par, pn);\n    return " + f.name + "_acceptance(nproto, po, pn, par);\n}\n";
+    }
+    dispatch(text, "double mci_user_pdf_updated(int uid, int ndim, int nproto, MciVec x, MciChanged ch, MciVec po, MciVec pn, const double *par)",
+             "(ndim, nproto, x, ch, po, pn, par)", "0.0", pdfs, "_updated_acceptance");
     text += "#include \"mci_step_kernels.hpp\"\n";
     return true;
 }

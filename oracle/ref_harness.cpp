@@ -41,7 +41,7 @@ enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_
        OBS_PDF_LOG = 9, OBS_PDF_PROTO = 10, OBS_DEP_SQUARE = 11 };
 // plugins outside the library's catalogue, written as a reference user writes them (the fixtures of
 // the library's device functors, which take them as source)
-enum { PDF_USER_QUARTIC = 101, OBS_USER_MOMENTS = 101 };
+enum { PDF_USER_QUARTIC = 101, PDF_USER_QUARTIC_UPD = 102, OBS_USER_MOMENTS = 101 };
 
 std::string hx(double v)
 {
@@ -128,13 +128,14 @@ public:
     }
 };
 
-// p(x) ~ exp(-sum_i c_i x_i^4), one proto value per coordinate, no updatedAcceptance of its own
-// (the interface's default: full recomputation, SamplingFunctionInterface.hpp:85-90)
-class QuarticPDF final: public ObservedPDF
+// p(x) ~ exp(-sum_i c_i x_i^4), one proto value per coordinate; without an updatedAcceptance of its
+// own (the interface's default: full recomputation, SamplingFunctionInterface.hpp:85-90) or, as
+// QuarticUpdPDF, with one that updates the changed coordinates' proto values
+class QuarticPDF: public ObservedPDF
 {
-    std::vector<double> _c;
 protected:
-    mci::SamplingFunctionInterface *_clone() const final { return new QuarticPDF(_ndim, _c); }
+    std::vector<double> _c;
+    mci::SamplingFunctionInterface *_clone() const override { return new QuarticPDF(_ndim, _c); }
 public:
     QuarticPDF(int nd, std::vector<double> c): ObservedPDF(nd, nd), _c(std::move(c)) {}
     double logValue(const double pv[]) const final { return -std::accumulate(pv, pv + _nproto, 0.); }
@@ -151,6 +152,25 @@ public:
         double e = std::accumulate(po, po + _nproto, 0.);
         e -= std::accumulate(pn, pn + _nproto, 0.);
         return exp(e);
+    }
+};
+
+class QuarticUpdPDF final: public QuarticPDF
+{
+protected:
+    mci::SamplingFunctionInterface *_clone() const final { return new QuarticUpdPDF(_ndim, _c); }
+public:
+    QuarticUpdPDF(int nd, std::vector<double> c): QuarticPDF(nd, std::move(c)) {}
+    double updatedAcceptance(const mci::WalkerState &wlk, const double po[], double pn[]) final
+    {
+        double e = 0.;
+        for (int i = 0; i < wlk.nchanged; ++i) {
+            const int j = wlk.changedIdx[i];
+            const double q = wlk.xnew[j]*wlk.xnew[j];
+            pn[j] = (q*q)*_c[j];
+            e += pn[j] - po[j];
+        }
+        return exp(-e);
     }
 };
 
@@ -391,6 +411,7 @@ std::unique_ptr<mci::SamplingFunctionInterface> makePDF(int nd, int kind, const 
     case PDF_GAUSS_DIAG: return std::unique_ptr<mci::SamplingFunctionInterface>(new DiagPDF(nd, false, p));
     case PDF_EXP_DIAG: return std::unique_ptr<mci::SamplingFunctionInterface>(new DiagPDF(nd, true, p));
     case PDF_USER_QUARTIC: return std::unique_ptr<mci::SamplingFunctionInterface>(new QuarticPDF(nd, p));
+    case PDF_USER_QUARTIC_UPD: return std::unique_ptr<mci::SamplingFunctionInterface>(new QuarticUpdPDF(nd, p));
     case PDF_GMM_ISO: {
         const int K = static_cast<int>(p[0]);
         std::vector<double> w(p.begin() + 1, p.begin() + 1 + K), s(p.begin() + 1 + K, p.begin() + 1 + 2*K);

@@ -7,6 +7,7 @@ mci_add_sampling_function_device / mci_add_observable_device (include/mci_amd/de
   chains bit for bit (all-coordinate moves recompute every proto value in both)."""
 
 PDF_USER_QUARTIC = 101   # the harness's kind codes
+PDF_USER_QUARTIC_UPD = 102
 OBS_USER_MOMENTS = 101
 
 QUARTIC = r"""
@@ -30,6 +31,22 @@ __device__ double quartic_sampling(int nproto, MciVec pv, const double *par)
     double s = 0.0;
     for (int i = 0; i < nproto; ++i) { s = s + pv[i]; }
     return exp(-s);
+}
+"""
+
+# QuarticUpdPDF: the same with an updatedAcceptance of its own (the changed coordinates only)
+QUARTIC_UPD = QUARTIC.replace("quartic_", "quarticupd_") + r"""
+__device__ double quarticupd_updated_acceptance(int ndim, int nproto, MciVec xnew, MciChanged changed, MciVec po, MciVec pn,
+                                                const double *par)
+{
+    double e = 0.0;
+    for (int i = 0; i < changed.n; ++i) {
+        const int j = changed[i];
+        const double q = xnew[j] * xnew[j];
+        pn[j] = (q * q) * par[j];
+        e += pn[j] - po[j];
+    }
+    return exp(-e);
 }
 """
 
@@ -76,5 +93,5 @@ __device__ void xsq_observable(int ndim, MciVec x, const double *par, MciVec out
 """
 
 # kind code -> (name, source, number of proto values or observable values for ndim)
-PDFS = {PDF_USER_QUARTIC: ("quartic", QUARTIC, lambda d: d)}
+PDFS = {PDF_USER_QUARTIC: ("quartic", QUARTIC, lambda d: d), PDF_USER_QUARTIC_UPD: ("quarticupd", QUARTIC_UPD, lambda d: d)}
 OBS = {OBS_USER_MOMENTS: ("moments", MOMENTS, lambda d: 2)}

@@ -195,7 +195,7 @@ def device_functor_scenarios():
     (oracle/ref_harness.cpp QuarticPDF: p ~ exp(-sum c_i x_i^4) without an updatedAcceptance of its
     own; MomentsObs: sum |x_i|^3 and max |x_i|, not updateable).  The library runs them as device
     functors compiled from source (tests/device_functors.py)."""
-    from device_functors import OBS_USER_MOMENTS, PDF_USER_QUARTIC
+    from device_functors import OBS_USER_MOMENTS, PDF_USER_QUARTIC, PDF_USER_QUARTIC_UPD
     S = []
     S.append(dict(name="quartic_allmove", cfg=dict(ndim=4, seed=2024, ndecorr=500, pdfs=[(PDF_USER_QUARTIC, [0.5, 1.0, 2.0, 0.25])],
                                                    obs=[(OBS_USER_MOMENTS, [], 16, 1, 1, ESTIM_MJ), (OBS_X2, [], 16, 1, 1, ESTIM_MJ)]),
@@ -207,6 +207,13 @@ def device_functor_scenarios():
     S.append(dict(name="quartic_auto_decorrelation", cfg=dict(ndim=2, seed=5649871, pdfs=[(PDF_USER_QUARTIC, [1.0, 2.0])],
                                                               obs=[(OBS_USER_MOMENTS, [], 8, 1, 1, ESTIM_MJ)], x0=[2.0, -1.5]),
                   calls=[dict(nmc=8 * 128, find=1, decorr=1)]))
+    # with updatedAcceptance (QuarticUpdPDF): single-coordinate and two-coordinate moves update the
+    # changed proto values only
+    for veclen in (1, 2):
+        S.append(dict(name=f"quartic_updated_vec{veclen}", cfg=dict(ndim=4, seed=31 + veclen, ndecorr=400, move=(0, veclen, 1, None),
+                                                                   pdfs=[(PDF_USER_QUARTIC_UPD, [0.5, 1.0, 2.0, 0.25])],
+                                                                   obs=[(OBS_USER_MOMENTS, [], 16, 1, 1, ESTIM_MJ)]),
+                      calls=[dict(nmc=16 * 64, find=1, decorr=1)]))
     return [dict(name=s["name"], cfg=s["cfg"], calls=s["calls"], results=run_scenario(s)) for s in S]
 
 

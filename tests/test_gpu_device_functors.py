@@ -79,3 +79,27 @@ def test_device_functor_errors():
     with pytest.raises(MciError) as e:
         c2.integrate(64)
     assert "different sources" in str(e.value)
+
+
+def test_updated_acceptance_override_is_used_for_partial_moves_only():
+    """A functor whose updatedAcceptance accepts everything: single-coordinate moves (the override)
+    are all accepted, all-coordinate moves (protoFunction and acceptanceFunction) are not."""
+    src = DF.QUARTIC.replace("quartic_", "always_") + r"""
+__device__ double always_updated_acceptance(int ndim, int nproto, MciVec xnew, MciChanged changed, MciVec po, MciVec pn,
+                                            const double *par)
+{
+    for (int i = 0; i < changed.n; ++i) { const int j = changed[i]; const double q = xnew[j] * xnew[j]; pn[j] = (q * q) * par[j]; }
+    return 1.0;
+}
+"""
+    rates = {}
+    for veclen in (1, 0):
+        c = Chain("mci", 3, 64)
+        c.setSeed(7)
+        c.setTrialMove(0, veclen, 1, None)
+        c.setMRT2Step(2.0)
+        c.addSamplingFunctionDevice("always", src, 3, [1.0, 1.0, 1.0])
+        c.addObservableDevice("moments", DF.MOMENTS, 2, (), 16, 1, False, ESTIM_MJ)
+        c.integrate(16 * 16, False, False)
+        rates[veclen] = c.acceptance()[2]
+    assert rates[1] == 1.0 and rates[0] < 0.9, rates
