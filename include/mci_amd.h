@@ -52,6 +52,7 @@ typedef int mci_status;
 #define MCI_MOVE_ALL              0
 #define MCI_MOVE_VEC              1
 #define MCI_MOVE_MULTISTEP        2  /* MultiStepMove(ndim): ndim single-coordinate uniform steps, no own sampling function */
+#define MCI_MOVE_DEVICE           3  /* the caller's own trial move as device code (mci_set_trial_move_device) */
 
 /* SRRDType, Factories.hpp:119-133 (same order), each with the default parameters of createSymRRD
  * (TrialMoveInterface.hpp:107-183): uniform on (-1, 1), normal(0, 1), Student-t(1), Cauchy(0, 1),
@@ -277,10 +278,21 @@ mci_status mci_add_sampling_function_device(mci_ctx *ctx, const char *name, cons
                                             const double *params, int nparams);
 mci_status mci_add_observable_device(mci_ctx *ctx, const char *name, const char *source, int ndim, int nobs,
                                      const double *params, int nparams, int blocksize, int nskip, int flag_equil, int estimator);
+/* MCI::setTrialMove(std::unique_ptr<TrialMoveInterface>) (MCIntegrator.hpp:140, MCIntegrator.cpp:399-424)
+ * with the caller's own TrialMoveInterface subclass (TrialMoveInterface.hpp:16-70) as device code:
+ * `source` defines NAME_move (trialMove) and, with nproto > 0 proto values, NAME_proto
+ * (protoFunction).  nsteps step sizes (getNStepSizes), typeEnds[nsteps] their coordinate ranges as
+ * for mci_set_trial_move (getStepSizeIndex; may be NULL for one), change_rate its getChangeRate,
+ * steps[nsteps] the initial step sizes (NULL: 0.05).  The move's draws come from the chain's stream
+ * (mci_uniform); its proto values follow accept and reject (newToOld / oldToNew,
+ * MCIntegrator.cpp:350-359).  Replaced by mci_set_trial_move and the other move setters. */
+mci_status mci_set_trial_move_device(mci_ctx *ctx, const char *name, const char *source, int nproto, int nsteps, const int *typeEnds,
+                                     double change_rate, const double *steps, const double *params, int nparams);
 mci_status mci_build_device_functors(mci_ctx *ctx);
-/* compiles one functor with the chain's kernels for gfx950 without a GPU; log (capacity logcap)
- * receives the compiler's messages.  No counterpart in the reference: a check before a GPU run. */
-mci_status mci_check_device_functor(const char *name, const char *source, int is_observable, char *log, int64_t logcap);
+/* compiles one functor (what: 0 sampling function, 1 observable, 2 trial move) with the chain's
+ * kernels for gfx950 without a GPU; log (capacity logcap) receives the compiler's messages.  No
+ * counterpart in the reference: a check before a GPU run. */
+mci_status mci_check_device_functor(const char *name, const char *source, int what, char *log, int64_t logcap);
 
 /* ---- integrate: MCI::integrate src/MCIntegrator.cpp:43-82 (combine over walkers:
  *      src/MPIMCI.cpp:85-92).  avg/err have nobsdim entries. ---- */

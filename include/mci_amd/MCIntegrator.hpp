@@ -169,6 +169,19 @@ inline ObservableFunction DeviceObservable(const std::string &name, const std::s
     return o;
 }
 
+// a user subclass of TrialMoveInterface (TrialMoveInterface.hpp:16-70) as device code: `source`
+// defines NAME_move and, with proto values, NAME_proto (mci_amd/device_functor.hpp); its step sizes
+// (initial values) over typeEnds as for SRRD moves, and its change rate; handed to MCI::setTrialMove
+struct DeviceTrialMove
+{
+    std::string name, source;
+    int nproto = 0;
+    std::vector<double> steps{0.05};
+    std::vector<int> typeEnds{};
+    double changeRate = 1.;
+    std::vector<double> params{};
+};
+
 // sampling functions (TestMCIFunctions.hpp:123-256 and the benchmark targets)
 inline SamplingFunction ThreeDimGaussianPDF() { return {MCI_PDF_GAUSS_ISO, 3, {}}; }       // exp(-|x|^2), d = 3
 inline SamplingFunction GaussianPDF(int nd) { return {MCI_PDF_GAUSS_ISO, nd, {}}; }         // exp(-|x|^2), one proto value
@@ -327,6 +340,12 @@ public:
         if (m.ndim != _ndim) { throw std::invalid_argument("[MCI::setTrialMove] Passed trial move's number of inputs is not equal to MCI's number of walkers."); }
         check(mci_set_trial_move(_ctx, static_cast<int>(m.srrd), m.veclen, m.ntypes, m.typeEnds.empty() ? nullptr : const_cast<int *>(m.typeEnds.data())));
         check(mci_set_learned_move(_ctx, (int)m.weights.size(), m.weights.data(), m.means.data(), m.sigmas.data(), m.pmix));
+    }
+    void setTrialMove(const DeviceTrialMove &m)
+    {
+        check(mci_set_trial_move_device(_ctx, m.name.c_str(), m.source.c_str(), m.nproto, (int)m.steps.size(),
+                                        m.typeEnds.empty() ? nullptr : m.typeEnds.data(), m.changeRate, m.steps.data(), m.params.data(),
+                                        (int)m.params.size()));
     }
 
     // --- observables (MCIntegrator.hpp:154-169) ---

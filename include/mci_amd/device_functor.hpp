@@ -23,7 +23,16 @@
  *
  *   __device__ void   NAME_observable(int ndim, MciVec x, const double *par, MciVec out);
  *
- * for an observable (observableFunction), par being the parameters given with it.  The library
+ * for an observable (observableFunction), or
+ *
+ *   __device__ double NAME_move(int ndim, MciVec xnew, MciMoved &moved, MciVec protoold,
+ *                               MciVec protonew, MciVec steps, MciRng &rng, const double *par);
+ *   __device__ void   NAME_proto(int ndim, MciVec x, const double *par, MciVec proto);  (with proto values)
+ *
+ * for a trial move (TrialMoveInterface.hpp:16-70: trialMove and protoFunction; xnew holds the
+ * current position on entry, the move changes the range moved.first .. moved.first + moved.n - 1
+ * (all coordinates unless it says otherwise) and returns its acceptance factor; steps[i] are its
+ * step sizes, draws come from mci_uniform(rng)), par being the parameters given with it.  The library
  * compiles the sources together with its generic kernels for the context's GPU when the chain
  * first needs them.  This header is included ahead of the sources; it needs nothing else.
  */
@@ -45,3 +54,19 @@ struct MciChanged
     int first, n;
     __device__ int operator[](int i) const { return first + i; }
 };
+
+// what a trial move changed: coordinates first .. first + n - 1 (WalkerState::nchanged / changedIdx
+// for one ascending range; unchanged coordinates inside it are harmless)
+struct MciMoved
+{
+    int first, n;
+};
+
+// the walker's random stream; mci_uniform(rng) is a uniform on [0, 1), the draw of
+// std::uniform_real_distribution<double>(0, 1) on MCI's generator in the reference-stream mode
+struct MciRng
+{
+    const void *c;
+    void *L;
+};
+__device__ double mci_uniform(MciRng &rng);

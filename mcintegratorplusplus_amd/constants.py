@@ -4,7 +4,7 @@ OK, E_INVALID_ARGUMENT, E_DOMAIN, E_RUNTIME, E_DEVICE = 0, 1, 2, 3, 4
 # EstimatorType (Factories.hpp:52-59)
 ESTIM_NOOP, ESTIM_UNCORRELATED, ESTIM_CORRELATED, ESTIM_FCBLOCKER, ESTIM_MJBLOCKER = 0, 1, 2, 3, 4
 # MoveType (Factories.hpp:108-113) and SRRDType (Factories.hpp:119-133)
-MOVE_ALL, MOVE_VEC = 0, 1
+MOVE_ALL, MOVE_VEC, MOVE_MULTISTEP, MOVE_DEVICE = 0, 1, 2, 3
 (SRRD_UNIFORM, SRRD_GAUSSIAN, SRRD_STUDENT, SRRD_CAUCHY, SRRD_EXPONENTIAL, SRRD_GAMMA, SRRD_WEIBULL, SRRD_LOGNORMAL,
  SRRD_CHISQ, SRRD_FISHER) = range(10)
 # device functor catalogue

@@ -114,6 +114,10 @@ struct mci_ctx
     std::vector<ObsHost> obs;
     RtcKernels rtc;                   // the step kernels built with the device functors (none: null)
     std::string rtc_text;             // their program text
+    std::string mv_name, mv_src;      // MCI_MOVE_DEVICE: the caller's trial move
+    int mv_nproto = 0;
+    double mv_rate = 1.;              // its getChangeRate
+    DBuf<double> mv_par, mv_pold, mv_pnew;
 
     int device = 0;
     hipStream_t stream = nullptr;
@@ -250,6 +254,10 @@ Chain make_chain(mci_ctx *c, const std::vector<ObsDev> *obsd)
     }
     ch.ms_npdf = (int)c->ms_pdfs.size();
     ch.ms_pold = c->ms_pold.p;
+    ch.mv_par = c->mv_par.p;
+    ch.mv_nproto = c->movetype == MCI_MOVE_DEVICE ? c->mv_nproto : 0;
+    ch.mv_pold = c->mv_pold.p;
+    ch.mv_pnew = c->mv_pnew.p;
     ch.ms_pnew = c->ms_pnew.p;
     ch.ms_x0 = c->ms_x0.p;
     {
@@ -324,13 +332,15 @@ mci_status ensure_rtc(mci_ctx *c)
     for (const ObsHost &o : c->obs) {
         if (o.kind == MCI_OBS_DEVICE) { fo.push_back({o.fname, o.fsrc}); }
     }
-    if (fp.empty() && fo.empty()) {
+    const bool dmove = c->movetype == MCI_MOVE_DEVICE;
+    if (fp.empty() && fo.empty() && !dmove) {
         c->rtc = RtcKernels{};
         c->rtc_text.clear();
         return MCI_OK;
     }
+    const RtcFunctor mv{c->mv_name, c->mv_src};
     std::string text, msg;
-    if (!rtc_program(fp, fo, text, msg)) { return fail(MCI_E_INVALID_ARGUMENT, "%s", msg.c_str()); }
+    if (!rtc_program(fp, fo, dmove ? &mv : nullptr, text, msg)) { return fail(MCI_E_INVALID_ARGUMENT, "%s", msg.c_str()); }
     if (text == c->rtc_text && c->rtc.sample) { return MCI_OK; }
     RtcKernels k;
     std::string log;
@@ -936,7 +946,7 @@ mci_status mci_set_trial_move_type(mci_ctx *c, int t)
 mci_status mci_get_change_rate(mci_ctx *c, double *r)
 {
     CHECKCTX(c);
-    *r = (c->movetype == MCI_MOVE_ALL) ? 1. : 1. / c->nvecs;
+    *r = (c->movetype == MCI_MOVE_ALL) ? 1. : (c->movetype == MCI_MOVE_DEVICE ? c->mv_rate : 1. / c->nvecs);
     if (c->ms_on) { *r = std::fmin(1., *r * c->ms_nsteps); } // MultiStepMove.hpp:67
     return MCI_OK;
 }
@@ -1307,6 +1317,50 @@ mci_status mci_add_observable_device(mci_ctx *c, const char *name, const char *s
     return MCI_OK;
 }
 
+mci_status mci_set_trial_move_device(mci_ctx *c, const char *name, const char *source, int nproto, int nsteps, const int *te, double change_rate,
+                                     const double *steps, const double *params, int nparams)
+{
+    CHECKCTX(c);
+    if (!name || !source || !rtc_valid_name(name)) { return fail(MCI_E_INVALID_ARGUMENT, "a device functor needs a source and a C identifier as its name"); }
+    if (nproto < 0) { return fail(MCI_E_INVALID_ARGUMENT, "nproto must not be negative"); }
+    if (nsteps < 1 || nsteps > MAX_TYPES) { return fail(MCI_E_INVALID_ARGUMENT, "a trial move needs 1 to %d step sizes", MAX_TYPES); }
+    if (nsteps > 1 && !te) { return fail(MCI_E_INVALID_ARGUMENT, "[checkTrialMoveSanity] ntypes>1 requires passed typeEnds array."); }
+    if (nsteps > 1) {
+        for (int i = 0; i < nsteps; ++i) {
+            if (te[i] < 1 || te[i] > c->ndim || (i > 0 && te[i] <= te[i - 1])) { return fail(MCI_E_INVALID_ARGUMENT, "typeEnds must increase within [1, ndim]"); }
+        }
+        if (te[nsteps - 1] != c->ndim) { return fail(MCI_E_INVALID_ARGUMENT, "the last type end must equal ndim"); }
+    }
+    if (!(change_rate > 0. && change_rate <= 1.)) { return fail(MCI_E_INVALID_ARGUMENT, "the change rate must lie in (0, 1]"); }
+    if (nparams < 0 || (nparams > 0 && !params)) { return fail(MCI_E_INVALID_ARGUMENT, "bad parameter array"); }
+    TRY(set_current(c));
+    c->typeEnds.assign(nsteps, c->ndim);
+    if (nsteps > 1) { c->typeEnds.assign(te, te + nsteps); }
+    c->ntypes = nsteps;
+    c->srrd = MCI_SRRD_UNIFORM;
+    c->movetype = MCI_MOVE_DEVICE;
+    c->veclen = 0;
+    c->nvecs = 0;
+    c->ms_on = 0;
+    c->ms_nsteps = 0;
+    c->ms_pdfs.clear();
+    c->lm_K = 0;
+    c->lm_pmix = 0.;
+    c->mv_name = name;
+    c->mv_src = source;
+    c->mv_nproto = nproto;
+    c->mv_rate = change_rate;
+    if (nparams > 0) {
+        HIPCHK(c->mv_par.ensure((size_t)nparams));
+        HIPCHK(hipMemcpy(c->mv_par.p, params, sizeof(double) * (size_t)nparams, hipMemcpyHostToDevice));
+    }
+    HIPCHK(c->mv_pold.ensure((size_t)std::max(nproto, 1) * c->W));
+    HIPCHK(c->mv_pnew.ensure((size_t)std::max(nproto, 1) * c->W));
+    HIPCHK(c->steps.ensure((size_t)nsteps * c->W));
+    for (int t = 0; t < nsteps; ++t) { TRY(fill_rows(c, c->steps.p, t, 1, steps ? steps[t] : 0.05)); }
+    return MCI_OK;
+}
+
 mci_status mci_build_device_functors(mci_ctx *c)
 {
     CHECKCTX(c);
@@ -1314,14 +1368,17 @@ mci_status mci_build_device_functors(mci_ctx *c)
     return ensure_rtc(c);
 }
 
-mci_status mci_check_device_functor(const char *name, const char *source, int is_observable, char *log, int64_t logcap)
+mci_status mci_check_device_functor(const char *name, const char *source, int what, char *log, int64_t logcap)
 {
     if (log && logcap > 0) { log[0] = '\0'; }
     if (!name || !source || !rtc_valid_name(name)) { return fail(MCI_E_INVALID_ARGUMENT, "a device functor needs a source and a C identifier as its name"); }
+    if (what < 0 || what > 2) { return fail(MCI_E_DOMAIN, "what: 0 sampling function, 1 observable, 2 trial move"); }
     std::vector<RtcFunctor> fp, fo;
-    (is_observable ? fo : fp).push_back({name, source});
+    const RtcFunctor mv{name, source};
+    if (what == 0) { fp.push_back(mv); }
+    if (what == 1) { fo.push_back(mv); }
     std::string text, msg, lg;
-    if (!rtc_program(fp, fo, text, msg)) { return fail(MCI_E_INVALID_ARGUMENT, "%s", msg.c_str()); }
+    if (!rtc_program(fp, fo, what == 2 ? &mv : nullptr, text, msg)) { return fail(MCI_E_INVALID_ARGUMENT, "%s", msg.c_str()); }
     const int st = rtc_check(text, "gfx950", lg);
     if (log && logcap > 0) {
         const size_t n = std::min<size_t>(lg.size(), (size_t)logcap - 1);

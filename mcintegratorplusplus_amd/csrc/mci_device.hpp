@@ -76,6 +76,10 @@ struct Chain
     int ms_npdf, ms_nproto;     // its own sampling functions and their proto values
     PdfDev ms_pdf[MAX_PDF];
     double *ms_pold, *ms_pnew;  // [ms_nproto][W]
+    // the caller's own trial move (movetype MCI_MOVE_DEVICE): its parameters and proto values
+    const double *mv_par;
+    int mv_nproto;
+    double *mv_pold, *mv_pnew;  // [mv_nproto][W]
     double *ms_x0;              // [ndim][W] the position the move starts from
     int lm_on, lm_K;            // the independence half of a mixed move (0: off), mixture components
     double lm_pmix;             // its probability per step
@@ -624,6 +628,20 @@ __device__ inline double trial_move(const Chain &c, Lane &L)
 {
     // a mixed move: first the uniform that picks the independence move with probability lm_pmix
     if (c.lm_on && draw_unit(c, L) < c.lm_pmix) { return lm_move(c, L); }
+#ifdef MCI_RTC_USER
+    if (c.movetype == MCI_MOVE_DEVICE) {
+        // the caller's trialMove: it moves xnew (which holds xold), names the changed coordinates
+        // (one ascending range) and returns its acceptance factor
+        const int64_t W = c.W, w = L.w;
+        MciRng rng{&c, &L};
+        MciMoved mv{0, c.ndim};
+        const double f = mci_user_move(c.ndim, MciVec{c.xnew + w, W}, mv, MciVec{c.mv_pold + w, W}, MciVec{c.mv_pnew + w, W},
+                                       MciVec{c.steps + w, W}, rng, c.mv_par);
+        L.cfirst = mv.first;
+        L.nchanged = mv.n;
+        return f;
+    }
+#endif
     if (!c.ms_on) { return srrd_move(c, L, true); }
     const int64_t W = c.W, w = L.w;
     for (int i = 0; i < c.ndim; ++i) { c.ms_x0[(int64_t)i * W + w] = c.xold[(int64_t)i * W + w]; }
@@ -661,6 +679,12 @@ __device__ inline void init_sampling(const Chain &c, Lane &L)
     L.accepted = true;
     for (int p = 0; p < c.npdf; ++p) { pdf_proto(c, c.pdf[p], c.xold, c.pold, w); }
     for (int i = 0; i < c.nproto; ++i) { c.pnew[(int64_t)i * W + w] = c.pold[(int64_t)i * W + w]; }
+#ifdef MCI_RTC_USER
+    if (c.movetype == MCI_MOVE_DEVICE && c.mv_nproto > 0) { // the trial move's initializeProtoValues
+        mci_user_move_proto(c.ndim, MciVec{c.xold + w, W}, c.mv_par, MciVec{c.mv_pold + w, W});
+        for (int i = 0; i < c.mv_nproto; ++i) { c.mv_pnew[(int64_t)i * W + w] = c.mv_pold[(int64_t)i * W + w]; }
+    }
+#endif
     cb_record(c, L);
 }
 
@@ -689,6 +713,12 @@ __device__ inline void step_mrt2(const Chain &c, Lane &L)
         for (int j = L.cfirst; j < L.cfirst + L.nchanged; ++j) { c.xnew[(int64_t)j * W + w] = c.xold[(int64_t)j * W + w]; }
         for (int i = 0; i < c.nproto; ++i) { c.pnew[(int64_t)i * W + w] = c.pold[(int64_t)i * W + w]; }
     }
+#ifdef MCI_RTC_USER
+    if (c.movetype == MCI_MOVE_DEVICE) { // the trial move's newToOld / oldToNew
+        double *mf = L.accepted ? c.mv_pnew : c.mv_pold, *mt = L.accepted ? c.mv_pold : c.mv_pnew;
+        for (int i = 0; i < c.mv_nproto; ++i) { mt[(int64_t)i * W + w] = mf[(int64_t)i * W + w]; }
+    }
+#endif
 }
 
 // MCI::doStepRandom (MCIntegrator.cpp:362-376): no sampling function
@@ -879,3 +909,12 @@ __device__ inline void store_lane(const Chain &c, const Lane &L)
     c.rej[L.w] = L.rej;
 }
 } // namespace mcid
+
+#ifdef MCI_RTC_USER
+// a uniform on [0, 1) from the chain's stream, as MCI's generator through
+// uniform_real_distribution<double>(0, 1) gives it (reference streams) or the native stream's draw
+__device__ inline double mci_uniform(MciRng &rng)
+{
+    return mcid::draw_unit(*static_cast<const mcid::Chain *>(rng.c), *static_cast<mcid::Lane *>(rng.L));
+}
+#endif

@@ -102,10 +102,12 @@ bool rtc_valid_name(const std::string &n)
     return true;
 }
 
-bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunctor> &obs, std::string &text, std::string &msg)
+bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunctor> &obs, const RtcFunctor *move, std::string &text,
+                 std::string &msg)
 {
     std::map<std::string, const std::string *> src;
-    for (const auto *v : {&pdfs, &obs}) {
+    const std::vector<RtcFunctor> mv = move ? std::vector<RtcFunctor>{*move} : std::vector<RtcFunctor>{};
+    for (const auto *v : {&pdfs, &obs, &mv}) {
         for (const RtcFunctor &f : *v) {
             auto it = src.find(f.name);
             if (it != src.end() && *it->second != f.source) {
@@ -117,7 +119,7 @@ bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunct
     }
     text = "#include \"../../include/mci_amd/device_functor.hpp\"\n";
     std::set<std::string> done;
-    for (const auto *v : {&pdfs, &obs}) {
+    for (const auto *v : {&pdfs, &obs, &mv}) {
         for (const RtcFunctor &f : *v) {
             if (!done.insert(f.name).second) { continue; }
             text += "#line 1 \"" + f.name + "\"\n" + f.source + "\n";
@@ -143,6 +145,16 @@ bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunct
     }
     dispatch(text, "double mci_user_pdf_updated(int uid, int ndim, int nproto, MciVec x, MciChanged ch, MciVec po, MciVec pn, const double *par)",
              "(ndim, nproto, x, ch, po, pn, par)", "0.0", pdfs, "_updated_acceptance");
+    // the trial move (trialMove, and protoFunction when it has proto values: without one of its own a
+    // template that computes nothing is taken)
+    text += "__device__ inline double mci_user_move(int ndim, MciVec x, MciMoved &moved, MciVec po, MciVec pn, MciVec steps, MciRng &rng, "
+            "const double *par)\n{\n";
+    text += move ? "    return " + move->name + "_move(ndim, x, moved, po, pn, steps, rng, par);\n}\n" : "    return 1.0;\n}\n";
+    if (move && upd.count(move->name) == 0) {
+        text += "template <int = 0>\n__device__ inline void " + move->name + "_proto(int, MciVec, const double *, MciVec) {}\n";
+    }
+    text += "__device__ inline void mci_user_move_proto(int ndim, MciVec x, const double *par, MciVec proto)\n{\n";
+    text += move ? "    " + move->name + "_proto(ndim, x, par, proto);\n}\n" : "}\n";
     text += "#include \"mci_step_kernels.hpp\"\n";
     return true;
 }

@@ -24,9 +24,11 @@ struct RtcKernels
 // true when name is a C identifier
 bool rtc_valid_name(const std::string &name);
 // the program text: the functors' sources (one copy per name), their dispatch by index (pdfs[i] is
-// sampling function index i, obs[i] observable index i), then the step kernels.  Returns false with
-// a message when two functors share a name but not a source.
-bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunctor> &obs, std::string &text, std::string &msg);
+// sampling function index i, obs[i] observable index i; the move, if any), then the step kernels.
+// Returns false with a message when two functors share a name but not a source.
+// move: the caller's trial move (or null)
+bool rtc_program(const std::vector<RtcFunctor> &pdfs, const std::vector<RtcFunctor> &obs, const RtcFunctor *move, std::string &text,
+                 std::string &msg);
 // compiles text for arch (e.g. "gfx950") without loading it; log: the compiler's messages
 int rtc_check(const std::string &text, const char *arch, std::string &log);
 // the kernels of text on the current device: compiled and loaded once per text and device

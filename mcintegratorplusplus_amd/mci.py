@@ -35,6 +35,7 @@ _SIGS = {
     "mci_add_sampling_function_device": [_P, C.c_char_p, C.c_char_p, _I, _I, _DP, _I],
     "mci_add_observable_device": [_P, C.c_char_p, C.c_char_p, _I, _I, _DP, _I, _I, _I, _I, _I],
     "mci_build_device_functors": [_P], "mci_check_device_functor": [C.c_char_p, C.c_char_p, _I, C.c_char_p, _I64],
+    "mci_set_trial_move_device": [_P, C.c_char_p, C.c_char_p, _I, _I, _IP, _D, _DP, _DP, _I],
 }
 _bound = None
 
@@ -83,11 +84,11 @@ def estimate(estimator, x):
     return a, e
 
 
-def check_device_functor(name, source, is_observable=False):
-    """Compile a device functor with the chain's kernels for gfx950 (no GPU needed); raises
-    MciError with the compiler's messages if it does not build."""
+def check_device_functor(name, source, what="sampling"):
+    """Compile a device functor ("sampling", "observable" or "move") with the chain's kernels for
+    gfx950 (no GPU needed); raises MciError with the compiler's messages if it does not build."""
     log = C.create_string_buffer(1 << 16)
-    _check(_lib().mci_check_device_functor(name.encode(), source.encode(), int(bool(is_observable)), log, len(log)))
+    _check(_lib().mci_check_device_functor(name.encode(), source.encode(), {"sampling": 0, "observable": 1, "move": 2}[what], log, len(log)))
     return log.value.decode()
 
 
@@ -179,6 +180,17 @@ class MCI:
         views, include/mci_amd/device_functor.hpp), compiled with the chain's kernels at the first integrate."""
         p = np.ascontiguousarray(params, dtype=np.float64)
         self._c("mci_add_sampling_function_device", name.encode(), source.encode(), self.ndim, nproto, _dp(p) if p.size else None, p.size)
+
+    def setTrialMoveDevice(self, name, source, nproto=0, steps=(0.05,), typeEnds=None, change_rate=1.0, params=()):
+        """MCI::setTrialMove with a TrialMoveInterface subclass of the caller's as device code
+        (NAME_move, and NAME_proto with nproto > 0); len(steps) step sizes over typeEnds."""
+        st = np.ascontiguousarray(steps, dtype=np.float64)
+        te = None
+        if typeEnds is not None:
+            te = (C.c_int * len(typeEnds))(*typeEnds)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._c("mci_set_trial_move_device", name.encode(), source.encode(), nproto, st.size, te, float(change_rate), _dp(st),
+                _dp(p) if p.size else None, p.size)
 
     def buildDeviceFunctors(self): self._c("mci_build_device_functors")
 

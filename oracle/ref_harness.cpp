@@ -42,6 +42,7 @@ enum { OBS_X2 = 1, OBS_X2_SUM = 2, OBS_X = 3, OBS_X_SUM = 4, OBS_CONST = 5, OBS_
 // plugins outside the library's catalogue, written as a reference user writes them (the fixtures of
 // the library's device functors, which take them as source)
 enum { PDF_USER_QUARTIC = 101, PDF_USER_QUARTIC_UPD = 102, OBS_USER_MOMENTS = 101 };
+enum { MOVE_USER_PAIR = 201, MOVE_USER_SCALE = 202 };
 
 std::string hx(double v)
 {
@@ -338,6 +339,75 @@ public:
     }
 };
 
+// trial moves outside the SRRD family, written as a reference user writes them (TrialMoveInterface,
+// drawing from MCI's generator through a uniform_real_distribution(0, 1)).
+// PairMove: a uniformly chosen coordinate i and, if it exists, i + 1, each moved by its step type's
+// size times a uniform on (-1, 1); step type 0 covers [0, ndim / 2), type 1 the rest; factor 1.
+class PairMove final: public mci::TrialMoveInterface
+{
+    std::vector<double> _s;
+    std::uniform_real_distribution<double> _rd{0., 1.};
+protected:
+    mci::TrialMoveInterface *_clone() const final { return new PairMove(_ndim, _s); }
+public:
+    PairMove(int nd, std::vector<double> s): mci::TrialMoveInterface(nd, 0), _s(std::move(s)) {}
+    int getNStepSizes() const final { return 2; }
+    double getStepSize(int i) const final { return _s[i]; }
+    void setStepSize(int i, double v) final { _s[i] = v; }
+    double getChangeRate() const final { return 2. / _ndim; }
+    int getStepSizeIndex(int xidx) const final { return xidx < _ndim / 2 ? 0 : 1; }
+    void protoFunction(const double /*in*/[], double /*out*/[]) final {}
+    double trialMove(mci::WalkerState &wlk, const double /*po*/[], double /*pn*/[]) final
+    {
+        int i = static_cast<int>(_rd(*_rgen) * _ndim);
+        if (i > _ndim - 1) { i = _ndim - 1; }
+        const int n = (i + 1 < _ndim) ? 2 : 1;
+        for (int k = 0; k < n; ++k) {
+            const int j = i + k;
+            const double r = 2. * _rd(*_rgen) - 1.;
+            wlk.xnew[j] = wlk.xnew[j] + _s[getStepSizeIndex(j)] * r;
+            wlk.changedIdx[k] = j;
+        }
+        wlk.nchanged = n;
+        return 1.;
+    }
+};
+
+// ScaleMove: every coordinate times s = 1 + step (2u - 1), with one proto value |x|^2 (updated as
+// s^2 |x|^2) and the move factor (protonew / protoold)^(ndim / 2) from the proto values: a
+// non-symmetric move whose factor depends on the move's own proto bookkeeping
+class ScaleMove final: public mci::TrialMoveInterface
+{
+    double _s;
+    std::uniform_real_distribution<double> _rd{0., 1.};
+protected:
+    mci::TrialMoveInterface *_clone() const final { return new ScaleMove(_ndim, _s); }
+public:
+    ScaleMove(int nd, double s): mci::TrialMoveInterface(nd, 1), _s(s) {}
+    int getNStepSizes() const final { return 1; }
+    double getStepSize(int /*i*/) const final { return _s; }
+    void setStepSize(int /*i*/, double v) final { _s = v; }
+    double getChangeRate() const final { return 1.; }
+    int getStepSizeIndex(int /*xidx*/) const final { return 0; }
+    void protoFunction(const double in[], double out[]) final
+    {
+        double q = 0.;
+        for (int i = 0; i < _ndim; ++i) { q = q + in[i]*in[i]; }
+        out[0] = q;
+    }
+    double trialMove(mci::WalkerState &wlk, const double po[], double pn[]) final
+    {
+        const double s = 1. + _s * (2. * _rd(*_rgen) - 1.);
+        for (int i = 0; i < _ndim; ++i) {
+            wlk.xnew[i] = wlk.xnew[i] * s;
+            wlk.changedIdx[i] = i;
+        }
+        wlk.nchanged = _ndim;
+        pn[0] = (s*s)*po[0];
+        return pow(pn[0]/po[0], 0.5*_ndim);
+    }
+};
+
 // sum_i |x_i|^3 and max_i |x_i|, not updateable
 class MomentsObs final: public mci::ObservableFunctionInterface
 {
@@ -538,6 +608,14 @@ int main()
                 mci->setTrialMove(static_cast<mci::SRRDType>(srrd), veclen, ntypes, ntypes > 1 ? te.data() : nullptr);
             }
             else if (cmd == "movetype") { int t; is >> t; mci->setTrialMove(static_cast<mci::MoveType>(t)); }
+            else if (cmd == "umove") {
+                int kind, np;
+                is >> kind >> np;
+                auto p = readDoubles(is, np);
+                if (kind == MOVE_USER_PAIR) { mci->setTrialMove(std::unique_ptr<mci::TrialMoveInterface>(new PairMove(ndim, p))); }
+                else if (kind == MOVE_USER_SCALE) { mci->setTrialMove(std::unique_ptr<mci::TrialMoveInterface>(new ScaleMove(ndim, p.at(0)))); }
+                else { throw std::invalid_argument("unknown user move kind"); }
+            }
             else if (cmd == "msmove") {
                 // MultiStepMove(ndim, nsteps) around an SRRD move built as MCI::setTrialMove(SRRDType, ...) does
                 int nsteps, srrd, veclen, ntypes;

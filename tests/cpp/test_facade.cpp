@@ -344,6 +344,34 @@ __device__ void sq_observable(int ndim, MciVec x, const double *par, MciVec out)
     double a[2], e[2];
     CHECK(throws<std::invalid_argument>([&] { bad.integrate(256, a, e); }));
     std::printf("device functors ok: <x_i^2> = %.4f %.4f %.4f\n", a1[0], a1[1], a1[2]);
+
+    // a trial move of the caller's (two adjacent coordinates, two step types) on exp(-|x|^2): <x_i^2> = 1/2
+    DeviceTrialMove pm;
+    pm.name = "pair";
+    pm.source = R"(
+__device__ double pair_move(int ndim, MciVec x, MciMoved &moved, MciVec po, MciVec pn, MciVec steps, MciRng &rng, const double *par)
+{
+    int i = (int)(mci_uniform(rng) * ndim);
+    if (i > ndim - 1) { i = ndim - 1; }
+    const int n = (i + 1 < ndim) ? 2 : 1;
+    for (int k = 0; k < n; ++k) { const int j = i + k; x[j] = x[j] + steps[j < ndim / 2 ? 0 : 1] * (2.0 * mci_uniform(rng) - 1.0); }
+    moved.first = i;
+    moved.n = n;
+    return 1.0;
+}
+)";
+    pm.steps = {0.3, 0.6};
+    pm.typeEnds = {2, 4};
+    pm.changeRate = 0.5;
+    MCI mm(4, 256);
+    mm.setSeed(11);
+    mm.setTrialMove(pm);
+    mm.addSamplingFunction(Gauss(4));
+    mm.addObservable(X2(4), 32, 1, true, EstimatorType::MJBlocker);
+    double a4[4], e4[4];
+    mm.integrate(32 * 64, a4, e4);
+    for (int i = 0; i < 4; ++i) { CHECK(std::fabs(a4[i] - 0.5) < 4. * e4[i]); }
+    std::printf("device trial move ok: <x_i^2> = %.4f %.4f %.4f %.4f\n", a4[0], a4[1], a4[2], a4[3]);
 }
 
 int main()

@@ -92,6 +92,48 @@ __device__ void xsq_observable(int ndim, MciVec x, const double *par, MciVec out
 }
 """
 
+# trial moves outside the SRRD family (oracle/ref_harness.cpp PairMove, ScaleMove)
+MOVE_USER_PAIR = 201
+MOVE_USER_SCALE = 202
+
+PAIR = r"""
+__device__ double pair_move(int ndim, MciVec x, MciMoved &moved, MciVec po, MciVec pn, MciVec steps, MciRng &rng, const double *par)
+{
+    int i = (int)(mci_uniform(rng) * ndim);
+    if (i > ndim - 1) { i = ndim - 1; }
+    const int n = (i + 1 < ndim) ? 2 : 1;
+    for (int k = 0; k < n; ++k) {
+        const int j = i + k;
+        const double r = 2.0 * mci_uniform(rng) - 1.0;
+        x[j] = x[j] + steps[j < ndim / 2 ? 0 : 1] * r;
+    }
+    moved.first = i;
+    moved.n = n;
+    return 1.0;
+}
+"""
+
+SCALE = r"""
+__device__ void scale_proto(int ndim, MciVec x, const double *par, MciVec proto)
+{
+    double q = 0.0;
+    for (int i = 0; i < ndim; ++i) { q = q + x[i] * x[i]; }
+    proto[0] = q;
+}
+__device__ double scale_move(int ndim, MciVec x, MciMoved &moved, MciVec po, MciVec pn, MciVec steps, MciRng &rng, const double *par)
+{
+    const double s = 1.0 + steps[0] * (2.0 * mci_uniform(rng) - 1.0);
+    for (int i = 0; i < ndim; ++i) { x[i] = x[i] * s; }
+    moved.first = 0;
+    moved.n = ndim;
+    pn[0] = (s * s) * po[0];
+    return pow(pn[0] / po[0], 0.5 * ndim);
+}
+"""
+
 # kind code -> (name, source, number of proto values or observable values for ndim)
 PDFS = {PDF_USER_QUARTIC: ("quartic", QUARTIC, lambda d: d), PDF_USER_QUARTIC_UPD: ("quarticupd", QUARTIC_UPD, lambda d: d)}
 OBS = {OBS_USER_MOMENTS: ("moments", MOMENTS, lambda d: 2)}
+# kind code -> (name, source, nproto, typeEnds for ndim, change rate for ndim, initial steps from the params)
+MOVES = {MOVE_USER_PAIR: ("pair", PAIR, 0, lambda d: [d // 2, d], lambda d: 2.0 / d, lambda p: list(p)),
+         MOVE_USER_SCALE: ("scale", SCALE, 1, lambda d: None, lambda d: 1.0, lambda p: [p[0]])}

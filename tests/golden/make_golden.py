@@ -195,7 +195,7 @@ def device_functor_scenarios():
     (oracle/ref_harness.cpp QuarticPDF: p ~ exp(-sum c_i x_i^4) without an updatedAcceptance of its
     own; MomentsObs: sum |x_i|^3 and max |x_i|, not updateable).  The library runs them as device
     functors compiled from source (tests/device_functors.py)."""
-    from device_functors import OBS_USER_MOMENTS, PDF_USER_QUARTIC, PDF_USER_QUARTIC_UPD
+    from device_functors import MOVE_USER_PAIR, MOVE_USER_SCALE, OBS_USER_MOMENTS, PDF_USER_QUARTIC, PDF_USER_QUARTIC_UPD
     S = []
     S.append(dict(name="quartic_allmove", cfg=dict(ndim=4, seed=2024, ndecorr=500, pdfs=[(PDF_USER_QUARTIC, [0.5, 1.0, 2.0, 0.25])],
                                                    obs=[(OBS_USER_MOMENTS, [], 16, 1, 1, ESTIM_MJ), (OBS_X2, [], 16, 1, 1, ESTIM_MJ)]),
@@ -214,6 +214,17 @@ def device_functor_scenarios():
                                                                    pdfs=[(PDF_USER_QUARTIC_UPD, [0.5, 1.0, 2.0, 0.25])],
                                                                    obs=[(OBS_USER_MOMENTS, [], 16, 1, 1, ESTIM_MJ)]),
                       calls=[dict(nmc=16 * 64, find=1, decorr=1)]))
+    # trial moves outside the SRRD family (PairMove: two step types, a two-coordinate range, the
+    # catalogue's diagonal Gaussian with its single-index update; ScaleMove: the move's own proto
+    # value and factor, on the quartic sampling function)
+    S.append(dict(name="pairmove_diag", cfg=dict(ndim=4, seed=606, ndecorr=300, umove=(MOVE_USER_PAIR, [0.3, 0.6]),
+                                                 pdfs=[(PDF_GAUSS_DIAG, [0.5, 1.0, 2.0, 0.25])],
+                                                 obs=[(OBS_X2, [], 16, 1, 1, ESTIM_MJ), (OBS_X2_SUM, [], 16, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=16 * 64, find=1, decorr=1), dict(nmc=16 * 16, find=0, decorr=0)]))
+    S.append(dict(name="scalemove_quartic", cfg=dict(ndim=3, seed=808, ndecorr=200, umove=(MOVE_USER_SCALE, [0.2]),
+                                                     pdfs=[(PDF_USER_QUARTIC, [1.0, 0.5, 2.0])], x0=[0.5, -0.4, 0.3],
+                                                     obs=[(OBS_USER_MOMENTS, [], 16, 1, 1, ESTIM_MJ)]),
+                  calls=[dict(nmc=16 * 64, find=1, decorr=1)]))
     return [dict(name=s["name"], cfg=s["cfg"], calls=s["calls"], results=run_scenario(s)) for s in S]
 
 

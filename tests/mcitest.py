@@ -109,6 +109,7 @@ MCI_ONLY = {
     "add_sampling_function_device": [C.c_char_p, C.c_char_p, _I, _I, _DP, _I],
     "add_observable_device": [C.c_char_p, C.c_char_p, _I, _I, _DP, _I, _I, _I, _I, _I],
     "build_device_functors": [],
+    "set_trial_move_device": [C.c_char_p, C.c_char_p, _I, _I, _IP, _D, _DP, _DP, _I],
 }
 
 
@@ -303,6 +304,16 @@ class Chain:
         self.call("add_observable_device", name.encode(), source.encode(), self.ndim, nobs, dptr(p) if p.size else None, p.size,
                   blocksize, nskip, int(bool(flag_equil)), estim)
 
+    def setTrialMoveDevice(self, name, source, nproto, steps, typeEnds=None, change_rate=1.0, params=()):
+        st = f64(steps)
+        te = None
+        if typeEnds is not None:
+            arr = (C.c_int * len(typeEnds))(*typeEnds)
+            te = C.cast(arr, _IP)
+        p = f64(params)
+        self.call("set_trial_move_device", name.encode(), source.encode(), nproto, st.size, te, change_rate, dptr(st),
+                  dptr(p) if p.size else None, p.size)
+
     def popSamplingFunction(self): self.call("pop_sampling_function")
     def clearSamplingFunctions(self): self.call("clear_sampling_functions")
 
@@ -416,6 +427,9 @@ def script_setup(cfg):
     if "move" in cfg:
         srrd, veclen, ntypes, te = cfg["move"]
         L.append(f"move {srrd} {veclen} {ntypes} " + " ".join(str(t) for t in (te or [])))
+    if "umove" in cfg:
+        kind, params = cfg["umove"]
+        L.append(f"umove {kind} {len(params)} " + " ".join(fmt(p) for p in params))
     if "multistep" in cfg:
         nsteps, srrd, veclen, ntypes, te, mpdfs = cfg["multistep"]
         L.append(f"msmove {nsteps} {srrd} {veclen} {ntypes} " + " ".join(str(t) for t in (te or [])))
@@ -451,6 +465,12 @@ def chain_setup(prefix, cfg, nwalkers=1):
     if "move" in cfg:
         srrd, veclen, ntypes, te = cfg["move"]
         c.setTrialMove(srrd, veclen, ntypes, te)
+    if "umove" in cfg:  # a trial move outside the SRRD family: its device functor source
+        import device_functors as DF
+        kind, params = cfg["umove"]
+        name, src, nproto, te, rate, steps = DF.MOVES[kind]
+        d = cfg["ndim"]
+        c.setTrialMoveDevice(name, src, nproto, steps(params), te(d), rate(d))
     if "multistep" in cfg:
         nsteps, srrd, veclen, ntypes, te, mpdfs = cfg["multistep"]
         c.setTrialMoveMultistep(nsteps, srrd, veclen, ntypes, te)
