@@ -400,7 +400,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         // proposal, the acceptance uniform; a power-of-two index never rejects).  The lazy twist
         // of mci_device.hpp's mt_next, in groups of MT_G steps: the loads of a group's state words
         // are issued a group ahead, so they are in flight while the group before runs.
-        constexpr int MT_G = 8, NO = 3 * MT_G;
+        constexpr int MT_G = 4, NO = 3 * MT_G;
         const int64_t W = A.W;
         // The walkers of a wave are usually at the same state position (every step takes three
         // outputs; seeding starts all at 0); then the position is wave-uniform and every state
