@@ -258,6 +258,94 @@ def run_all_coordinate_moves():
                        "max_abs_z": float(np.max(np.abs(z))), "acceptance": float(m.getAcceptanceRate())}
         r["speedup_specialised_vs_generic"] = r["generic"]["integrate_ms"] / r["specialised"]["integrate_ms"]
         out[rname] = r
+    # beyond the register width: d = 64 (the walker in LDS), native streams, the specialised kernel
+    d64 = 64
+    s64 = np.linspace(0.5, 2.0, d64)
+    m = MCI(d64, W)
+    m.setSeed(SEED)
+    m.setRandomMode(K.RNG_PHILOX)
+    m.setWalkerCoupling(K.COUPLING_SHARED)
+    m.setTrialMove(K.SRRD_UNIFORM, 0, 1)
+    m.setNdecorrelationSteps(NDECORR)
+    m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / s64 ** 2)
+    m.addObservable(K.OBS_X2, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+    m.integrate(NMC)
+    t0 = time.perf_counter()
+    avg, err = m.integrate(NMC)
+    ms = 1000 * (time.perf_counter() - t0)
+    wa, _ = m.walkerResults()
+    z = (avg - s64 ** 2) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
+    out["native_d64"] = {"workload": f"{W} walkers x {NMC} steps, d=64, sigma_i = linspace(0.5, 2, 64), x_i^2 in 64 blocks of 160",
+                         "integrate_ms": ms, "sampling_kernel_ms": m.getTimings()[2], "walker_steps_per_s": W * NMC / (ms / 1000),
+                         "max_abs_z": float(np.max(np.abs(z)))}
+    return out
+
+
+DIAG_SRC = r"""
+__device__ void diag_proto(int ndim, MciVec x, const double *par, MciVec proto)
+{
+    for (int i = 0; i < ndim; ++i) { const double v = x[i]; proto[i] = (v * v) * par[i]; }
+}
+__device__ double diag_acceptance(int nproto, MciVec po, MciVec pn, const double *par)
+{
+    double e = 0.0;
+    for (int i = 0; i < nproto; ++i) { e = e + po[i]; }
+    double s = 0.0;
+    for (int i = 0; i < nproto; ++i) { s = s + pn[i]; }
+    return exp(e - s);
+}
+__device__ double diag_sampling(int nproto, MciVec pv, const double *par)
+{
+    double s = 0.0;
+    for (int i = 0; i < nproto; ++i) { s = s + pv[i]; }
+    return exp(-s);
+}
+"""
+XSQ_SRC = r"""
+__device__ void xsq_observable(int ndim, MciVec x, const double *par, MciVec out)
+{
+    for (int i = 0; i < ndim; ++i) { const double v = x[i]; out[i] = v * v; }
+}
+"""
+
+
+def run_device_functors():
+    """The benchmark's target and x_i^2 written by the caller as device functors (compiled at run
+    time), against the same catalogue entries on the generic kernel and on the specialised
+    all-coordinate kernel: 65,536 walkers x 2,560 steps, d = 16, all-coordinate moves, 16 blocks of
+    160, shared calibration, 1,000 decorrelation steps."""
+    from mcintegratorplusplus_amd import MCI, constants as K
+    sig = sigmas()
+    a = 0.5 / sig ** 2
+    nmc = 2560
+    out = {"workload": f"{W} walkers x {nmc} steps, d={D}, all-coordinate moves, x_i^2 in 16 blocks of 160"}
+    res = {}
+    for name in ("device_functors", "catalogue_generic", "catalogue_specialised"):
+        m = MCI(D, W)
+        m.setSeed(SEED)
+        m.setRandomMode(K.RNG_PHILOX)
+        m.setWalkerCoupling(K.COUPLING_SHARED)
+        m.setTrialMove(K.SRRD_UNIFORM, 0, 1)
+        m.setNdecorrelationSteps(NDECORR)
+        if name == "catalogue_generic":
+            m.setKernelPath(K.KERNELS_GENERIC)
+        if name == "device_functors":
+            m.addSamplingFunctionDevice("diag", DIAG_SRC, D, a)
+            m.addObservableDevice("xsq", XSQ_SRC, D, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+            t0 = time.perf_counter()
+            m.buildDeviceFunctors()
+            out["build_s"] = time.perf_counter() - t0
+        else:
+            m.addSamplingFunction(K.PDF_GAUSS_DIAG, a)
+            m.addObservable(K.OBS_X2, (), BLOCK, 1, True, estimator=K.ESTIM_MJBLOCKER)
+        m.integrate(nmc)
+        t0 = time.perf_counter()
+        avg, err = m.integrate(nmc)
+        ms = 1000 * (time.perf_counter() - t0)
+        res[name] = avg
+        out[name] = {"integrate_ms": ms, "sampling_kernel_ms": m.getTimings()[2], "walker_steps_per_s": W * nmc / (ms / 1000)}
+    out["device_equals_catalogue_generic"] = bool(np.array_equal(res["device_functors"], res["catalogue_generic"]))
+    out["device_equals_catalogue_specialised"] = bool(np.array_equal(res["device_functors"], res["catalogue_specialised"]))
     return out
 
 
@@ -554,7 +642,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
-    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt", "allmove"],
+    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt", "allmove", "functors"],
                     help="also measure the precision form, the batched integrals and the learned proposal (all), "
                          "none of them, or one of them")
     ap.add_argument("--copy-probe", action="store_true",
@@ -639,7 +727,8 @@ def main():
     if args.workloads != "none" and args.precision == 0 and not args.generic:
         runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
                 "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces),
-                "reference_streams": ("mt", run_reference_streams), "all_coordinate_moves": ("allmove", run_all_coordinate_moves)}
+                "reference_streams": ("mt", run_reference_streams), "all_coordinate_moves": ("allmove", run_all_coordinate_moves),
+                "device_functors": ("functors", run_device_functors)}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
             if args.workloads in ("all", name):

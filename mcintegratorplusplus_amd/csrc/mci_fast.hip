@@ -681,41 +681,10 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
         need = A.nsteps * (int64_t)(d + 1);
         if (need > 0) { issue(k); }
     }
-    int64_t nacc = 0, inblk = 0, b = 0;
-    for (int64_t s = 0; s < A.nsteps; ++s) {
-        // proposals in coordinate order, the proto sum of the proposal
-        double Snew = 0.0;
-        uint64_t uword = 0;  // native: the shared word; reference: the acceptance output
-        if constexpr (MT) {
-            for (int i = 0; i < d; ++i) {
-                const double r = (mt_canonical(mt_out()) * 2.0) + (-1.0);
-                const double v = xget(i) + step * r;
-                xn[i * WAVE + lane] = v;
-                Snew = Snew + (v * v) * as[i];
-            }
-            uword = mt_out();
-        }
-        else {
-            // main word j of step t: word 2 (t mod 2) + (j mod 2) of block j / 2 of pair t / 2
-            const uint32_t par = 2u * (uint32_t)(t & 1u);
-            const uint64_t tp = t >> 1;
-            for (int j = 0; j <= d; j += 2) {
-                uint32_t o0, o1, o2, o3;
-                philox((uint32_t)(j >> 1), wid, (uint32_t)tp, (uint32_t)(tp >> 32), A.key0, A.key1, o0, o1, o2, o3);
-                const uint32_t wa = par ? o2 : o0, wb = par ? o3 : o1;
-                if (j == 0) { uword = wa; }
-                else {
-                    const double v = xget(j - 1) + step * fma((double)wa, 0x1p-31, -(1.0 - 0x1p-32));
-                    xn[(j - 1) * WAVE + lane] = v;
-                    Snew = Snew + (v * v) * as[j - 1];
-                }
-                if (j + 1 <= d) {
-                    const double v = xget(j) + step * fma((double)wb, 0x1p-31, -(1.0 - 0x1p-32));
-                    xn[j * WAVE + lane] = v;
-                    Snew = Snew + (v * v) * as[j];
-                }
-            }
-        }
+    int64_t nacc = 0, inblk = 0, b = 0, s = 0;
+    // a step's decision from the proto sum of its proposal (in xn) and its acceptance word (native:
+    // the shared word; reference streams: the acceptance output), then the observables after it
+    auto tail = [&](double Snew, uint64_t uword) {
         const double ee = Sold - Snew;  // log of the acceptance ratio, as the generic kernel forms it
         const float U = MT ? (float)(uint32_t)(uword >> 32) : (float)(uint32_t)uword;
         const float F = __builtin_amdgcn_exp2f(fmaf((float)ee, 0x1.715476p+0f, 32.0f));
@@ -737,6 +706,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
         if (acc) {
             ++nacc;
             Sold = Snew;
+#pragma unroll 4
             for (int i = 0; i < d; ++i) {
                 const double v = xn[i * WAVE + lane];
                 if (REC) { rec[i * WAVE + lane].x = v; }
@@ -746,6 +716,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
         // the observables after the step: the reference's per-step sums, or the trace row
         if (ACC || FULL) {
             double q = 0.0;
+#pragma unroll 4
             for (int i = 0; i < d; ++i) {
                 if (REC) {
                     double2 rv = rec[i * WAVE + lane];
@@ -787,6 +758,93 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
                 }
             }
         }
+        ++s;
+    };
+    // coordinate i of the proposal from the random value r, its proto added to Sn in coordinate order
+    auto prop = [&](int i, double r, double &Sn) {
+        const double v = xget(i) + step * r;
+        xn[i * WAVE + lane] = v;
+        Sn = Sn + (v * v) * as[i];
+    };
+    if constexpr (MT) {
+        while (s < A.nsteps) {
+            double Snew = 0.0;
+            for (int i = 0; i < d; ++i) { prop(i, (mt_canonical(mt_out()) * 2.0) + (-1.0), Snew); }
+            tail(Snew, mt_out());
+        }
+    }
+    else {
+        // main word j of step t: word 2 (t mod 2) + (j mod 2) of block j / 2 of pair t / 2.  Steps go
+        // in pairs: the first computes the blocks and keeps words 2 and 3 of each for the second.
+        uint32_t *wb = reinterpret_cast<uint32_t *>(as + d);  // [d + 2][64] the second step's words
+        auto rnd = [](uint32_t v) { return fma((double)v, 0x1p-31, -(1.0 - 0x1p-32)); };
+        // MODE 0: a step alone (either parity), 1: the first of a pair, 2: the second of a pair
+        // the words j and j + 1 of the step (block j / 2)
+        auto words = [&](auto mode_c, int j, uint32_t par, uint64_t tp, uint32_t &wa, uint32_t &wv) {
+            constexpr int MODE = decltype(mode_c)::value;
+            if constexpr (MODE == 2) {
+                wa = wb[j * WAVE + lane];
+                wv = wb[(j + 1) * WAVE + lane];
+            }
+            else {
+                uint32_t o0, o1, o2, o3;
+                philox((uint32_t)(j >> 1), wid, (uint32_t)tp, (uint32_t)(tp >> 32), A.key0, A.key1, o0, o1, o2, o3);
+                if constexpr (MODE == 1) {
+                    wa = o0;
+                    wv = o1;
+                    wb[j * WAVE + lane] = o2;
+                    wb[(j + 1) * WAVE + lane] = o3;
+                }
+                else {
+                    wa = par ? o2 : o0;
+                    wv = par ? o3 : o1;
+                }
+            }
+        };
+        auto step_words = [&](auto mode_c) {
+            const uint32_t par = (uint32_t)(t & 1u);
+            const uint64_t tp = t >> 1;
+            double Sn = 0.0;
+            uint32_t uw, wa, wv;
+            // block 0: the acceptance word and coordinate 0
+            words(mode_c, 0, par, tp, uw, wv);
+            prop(0, rnd(wv), Sn);
+            // then two coordinates a block, their LDS reads issued a block ahead (the next block's
+            // coordinates, clamped into range, are read while this block's Philox rounds run)
+            int j = 2;
+            double x0n = xget(1 < d ? 1 : d - 1), x1n = xget(2 < d ? 2 : d - 1), a0n = as[1 < d ? 1 : d - 1], a1n = as[2 < d ? 2 : d - 1];
+            for (; j + 1 <= d; j += 2) {
+                const double x0 = x0n, x1 = x1n, a0 = a0n, a1 = a1n;
+                const int n0 = j + 1 < d ? j + 1 : d - 1, n1 = j + 2 < d ? j + 2 : d - 1;
+                x0n = xget(n0);
+                x1n = xget(n1);
+                a0n = as[n0];
+                a1n = as[n1];
+                words(mode_c, j, par, tp, wa, wv);
+                const double v0 = x0 + step * rnd(wa);
+                xn[(j - 1) * WAVE + lane] = v0;
+                Sn = Sn + (v0 * v0) * a0;
+                const double v1 = x1 + step * rnd(wv);
+                xn[j * WAVE + lane] = v1;
+                Sn = Sn + (v1 * v1) * a1;
+            }
+            if (j == d) { // d even: word d alone (coordinate d - 1)
+                words(mode_c, j, par, tp, wa, wv);
+                prop(d - 1, rnd(wa), Sn);
+            }
+            tail(Sn, uw);
+        };
+        // pairs need the wave's counters to agree in parity (always, unless a restored state set them apart)
+        const uint64_t podd = __builtin_amdgcn_ballot_w64((t & 1u) != 0);
+        const bool pairs = podd == 0 || podd == __builtin_amdgcn_ballot_w64(true);
+        if (pairs) {
+            if ((t & 1u) && s < A.nsteps) { step_words(std::integral_constant<int, 0>{}); }
+            while (s + 2 <= A.nsteps) {
+                step_words(std::integral_constant<int, 1>{});
+                step_words(std::integral_constant<int, 2>{});
+            }
+        }
+        while (s < A.nsteps) { step_words(std::integral_constant<int, 0>{}); }
     }
     for (int i = 0; i < d; ++i) { A.x64[(int64_t)i * W + w] = xget(i); }
     if constexpr (MT) { A.mtp[w] = k; }
@@ -1378,7 +1436,8 @@ constexpr int ALL_DM = 16;
 size_t lds_bytes_all(int ndim, bool x2, bool mt)
 {
     if (ndim <= ALL_DM) { return mt ? sizeof(uint64_t) * 64 * WAVE : 0; }
-    return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + (size_t)ndim * WAVE + ndim) + (mt ? sizeof(uint64_t) * 16 * WAVE : 0);
+    return sizeof(double) * ((size_t)(x2 ? 2 : 1) * ndim * WAVE + (size_t)ndim * WAVE + ndim) +
+           (mt ? sizeof(uint64_t) * 16 * WAVE : sizeof(uint32_t) * (size_t)(ndim + 2) * WAVE);
 }
 
 size_t lds_limit()
