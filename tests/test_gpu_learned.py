@@ -96,7 +96,7 @@ def test_sampler_plain_and_untrained_mixed_are_unbiased():
         z = _z(avg, err, exact)
         r = s.results()
         print(mode, avg, err, z, r["acc_rw"], r["acc_learned"], s.timings(), flush=True)
-        assert np.all(np.abs(z) < 4.5), (mode, z)
+        assert np.all(np.abs(z) < 3), (mode, z)
         assert 0.3 < r["acc_rw"] < 0.7
 
 
@@ -114,9 +114,9 @@ def test_ragged_walker_count_and_small_dimension():
     for mode, pm in ((PLAIN_FP32, 0.5), (MIXED_FP32, 0.5), (MIXED_FP32, 1.0), (INDEP_FP32, 0.5), (INDEP_FP32, 1.0)):
         avg, err = s.run(mode, 4096, 32, pm)
         r = s.results()
-        print("ragged", mode, pm, avg, err, r["acc_learned"], flush=True)
+        print("ragged", mode, pm, avg, err, r["acc_learned"], "z", _z(avg, err, exact), flush=True)
         assert np.all(np.isfinite(avg)) and np.all(err > 0)
-        assert np.all(np.abs(_z(avg, err, exact)) < 4.5), (mode, pm)
+        assert np.all(np.abs(_z(avg, err, exact)) < 3), (mode, pm)
     m = s.walkerMeans()
     assert m.shape == (2, 1000) and np.all(np.isfinite(m))
 
@@ -166,8 +166,8 @@ def test_trained_network_reduces_the_error_bar():
     a_p, e_p = s.run(PLAIN_FP32, 2048, 32)
     a_m, e_m = s.run(MIXED_FP32, 2048, 32, 0.5)
     r = s.results()
-    print("plain", a_p, e_p, "mixed", a_m, e_m, "learned acceptance", r["acc_learned"], flush=True)
-    assert np.all(np.abs(_z(a_m, e_m, exact)) < 4.5)
+    print("plain", a_p, e_p, "mixed", a_m, e_m, "learned acceptance", r["acc_learned"], "z", _z(a_m, e_m, exact), flush=True)
+    assert np.all(np.abs(_z(a_m, e_m, exact)) < 3)
     assert r["acc_learned"] > 0.05
     assert e_m[0] < 0.7 * e_p[0]
     # independence proposal: the network trained with one fixed input
@@ -177,8 +177,8 @@ def test_trained_network_reduces_the_error_bar():
     s.train(400, 2048, 2e-3)
     a_i, e_i = s.run(INDEP_FP32, 2048, 32, 0.5)
     r = s.results()
-    print("independence", a_i, e_i, "learned acceptance", r["acc_learned"], s.timings(), flush=True)
-    assert np.all(np.abs(_z(a_i, e_i, exact)) < 4.5)
+    print("independence", a_i, e_i, "learned acceptance", r["acc_learned"], s.timings(), "z", _z(a_i, e_i, exact), flush=True)
+    assert np.all(np.abs(_z(a_i, e_i, exact)) < 3)
     assert r["acc_learned"] > 0.005
     assert e_i[0] < 0.7 * e_p[0]
 
@@ -234,7 +234,7 @@ def test_trained_mixture_as_the_trial_move_of_a_context():
     """The learned proposal trained on an isotropic three-mode target in d = 8, its mixture handed to
     an MCI context over the same target (the catalogue's isotropic mixture density) as the learned
     half of a mixed trial move: from walkers that all start in one mode, P(x_0 > 0) and E|x|^2 land
-    within 4 standard errors of the walker means; without the learned half the walkers stay in that
+    within 3 standard errors of the walker means; without the learned half the walkers stay in that
     mode and miss P(x_0 > 0)."""
     from scipy.stats import norm
 
@@ -277,7 +277,7 @@ def test_trained_mixture_as_the_trial_move_of_a_context():
     z = run(True)
     z0 = run(False)
     print("z with the learned half", z, "without", z0, flush=True)
-    assert np.all(np.abs(z) < 4), z
+    assert np.all(np.abs(z) < 3), z
     assert abs(z0[0]) > 20, z0
 
 

@@ -5,7 +5,7 @@
   positions, acceptance rates and every step of the traced chains;
 - native streams (Philox): the oracle restates the same draws; walkers and combined results
   must match it bit for bit (FP64), at full configuration size through a walker subset;
-- physics: estimates within 4 sigma of the analytic values.
+- physics: estimates within 3 sigma of the analytic values.
 """
 import numpy as np
 import pytest
@@ -380,7 +380,7 @@ def test_full_size_configuration2_walker_subset():
     """65,536 walkers x 10,240 steps (d=16): independent walkers 0..63 equal an oracle run of 64
     walkers (walker w's Philox stream depends only on (seed, w)): positions and counters bit for
     bit, estimates to 1e-12; the combined estimate agrees with sigma_i^2 and sum sigma_i^2
-    within 4 error bars."""
+    within 3 error bars."""
     W, nmc = 65536, 10240
     g = chain_setup("mci", cfg2(), nwalkers=W)
     g.setRng(RNG_PHILOX)
@@ -396,7 +396,8 @@ def test_full_size_configuration2_walker_subset():
     np.testing.assert_allclose(we[:64], oe, rtol=1e-10)
     sig, _ = sigma_a(16)
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
-    assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
+    print("z", (a - exact) / e, flush=True)
+    assert np.all(np.abs(a - exact) < 3 * e), (a - exact) / e
     # MPIMCI combine (MPIMCI.cpp:85-92) restated on the host
     assert np.allclose(a, wa.mean(axis=0), rtol=1e-13)
     assert np.allclose(e, np.sqrt((we ** 2).sum(axis=0)) / W, rtol=1e-13)
@@ -414,7 +415,8 @@ def test_full_size_shared_coupling_physics():
     assert acc + rej == W * nmc and abs(rate - 0.5) < 0.06
     sig, _ = sigma_a(16)
     exact = np.concatenate([sig ** 2, [np.sum(sig ** 2)]])
-    assert np.all(np.abs(a - exact) < 4 * e), (a - exact) / e
+    print("z", (a - exact) / e, flush=True)
+    assert np.all(np.abs(a - exact) < 3 * e), (a - exact) / e
 
 
 @pytest.mark.parametrize("rng", [RNG_MT, RNG_PHILOX])
@@ -575,7 +577,7 @@ def test_learned_mixed_move_many_walkers(rng):
 
 def test_learned_mixed_move_physics_at_size():
     """The mixed move on 65,536 walkers (device only): P(x_0 > 0) = 0.7 and E|x|^2 = 64 + 4 (0.3 0.64
-    + 0.7 1.69) within 4 standard errors of the walker means, from walkers all in the light mode."""
+    + 0.7 1.69) within 3 standard errors of the walker means, from walkers all in the light mode."""
     from mcitest import OBS_INDICATOR, PDF_GMM_ISO
     d, W = 4, 65536
     mu = np.array([[-8.0, 0, 0, 0], [8.0, 0, 0, 0]])
@@ -593,7 +595,7 @@ def test_learned_mixed_move_physics_at_size():
     exact = np.array([0.7, 64 + 4 * (0.3 * 0.64 + 0.7 * 1.69)])
     z = (wa.mean(axis=0) - exact) / (wa.std(axis=0, ddof=1) / np.sqrt(W))
     print("z", z, flush=True)
-    assert np.all(np.abs(z) < 4), z
+    assert np.all(np.abs(z) < 3), z
 
 
 @pytest.mark.parametrize("d,blocksize,coupling,W", [(16, 20, COUPLING_INDEPENDENT, 256), (16, 20, COUPLING_SHARED, 256),

@@ -63,7 +63,7 @@ def test_proposal_equal_to_the_target_is_always_accepted(rng):
     # the walkers then draw exactly from the target
     n = wa.shape[0]
     z = (wa.mean(axis=0) - exact()) / (wa.std(axis=0, ddof=1) / np.sqrt(n))
-    assert np.all(np.abs(z) < 4), z
+    assert np.all(np.abs(z) < 3), z
 
 
 @pytest.mark.parametrize("rng", [RNG_PHILOX, RNG_MT])
@@ -73,7 +73,7 @@ def test_bimodal_target_with_reversed_proposal_weights(rng):
     psig = np.array([[1.2, 1.0, 0.9, 1.1], [1.5, 1.4, 1.6, 1.2]])
     ch, a, wa = run(cfg(12, (pw, pmu, psig, 0.3)), W=64, nmc=8192, rng=rng, x0_mode=1)
     z = (wa.mean(axis=0) - exact()) / (wa.std(axis=0, ddof=1) / np.sqrt(len(wa)))
-    assert np.all(np.abs(z) < 4), (z, wa.mean(axis=0), exact())
+    assert np.all(np.abs(z) < 3), (z, wa.mean(axis=0), exact())
     assert abs(a[0] - exact()[0]) < 0.03
 
 

@@ -95,7 +95,7 @@ def test_gaussian_physics_within_error_bars():
                obs=[(OBS_X2, [], 64, 1, 1, ESTIM_MJ)])
     c = chain_setup("orc", cfg, nwalkers=16)
     avg, err = c.integrate(64 * 256)
-    assert np.all(np.abs(avg - 0.5 / a_i) < 4 * err)
+    assert np.all(np.abs(avg - 0.5 / a_i) < 3 * err)
 
 
 def test_uniform_sampling_volume_integral():
@@ -104,7 +104,7 @@ def test_uniform_sampling_volume_integral():
     c = chain_setup("orc", dict(ndim=3, seed=17, irange=([-5.] * 3, [5.] * 3),
                                 obs=[(OBS_GAUSS_X2, [], 1, 1, 0, ESTIM_UNCORR)]), nwalkers=16)
     avg, err = c.integrate(20000, False, False)
-    assert abs(avg[0] - 0.5) < 4 * err[0] and err[0] < 0.01
+    assert abs(avg[0] - 0.5) < 3 * err[0] and err[0] < 0.01
 
 
 def test_errors_map_to_reference_exception_types():
