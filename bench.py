@@ -349,6 +349,34 @@ def run_device_functors():
     return out
 
 
+def run_reference_defaults():
+    """A drop-in user's integrate with the reference's defaults: MCI(16) with 65,536 walkers, the
+    default move (all coordinates, one step size), the default random streams (mt19937_64 per
+    walker), automatic step calibration and automatic decorrelation, and addObservable(obs) with
+    its defaults (a Full accumulator, the Correlated estimator) for x_i^2, on the benchmark's target;
+    8,192 steps (MJBlocker on the traces).  Phases by HIP events."""
+    from mcintegratorplusplus_amd import MCI, constants as K
+    sig = sigmas()
+    nmc = 8192
+    out = {"workload": f"{W} walkers x {nmc} steps, d={D}, reference defaults (all-coordinate move, mt19937_64, "
+                       "automatic calibration and decorrelation, Full accumulator + Correlated estimator)"}
+    m = MCI(D, W)
+    m.setSeed(SEED)
+    m.setWalkerCoupling(K.COUPLING_SHARED)
+    m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
+    m.addObservable(K.OBS_X2)
+    m.integrate(nmc)
+    t0 = time.perf_counter()
+    avg, err = m.integrate(nmc)
+    ms = 1000 * (time.perf_counter() - t0)
+    kt = m.getTimings()
+    z = (avg - sig ** 2) / err
+    out.update({"integrate_ms": ms, "walker_steps_per_s": W * nmc / (ms / 1000),
+                "phase_ms": {"calibration": kt[0], "decorrelation": kt[1], "sampling": kt[2], "estimation": kt[3]},
+                "max_abs_z": float(np.max(np.abs(z)))})
+    return out
+
+
 def profile_summary():
     """HBM bytes per launch of the sampling kernel (FETCH_SIZE + WRITE_SIZE, each scaled by a
     1 GiB copy probe) and its VALU-busy fraction, from the committed rocprofv3 summary."""
@@ -642,7 +670,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 = FP64 (the benchmark), 1 = FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--generic", action="store_true", help="force the generic kernels (comparison)")
-    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt", "allmove", "functors"],
+    ap.add_argument("--workloads", default="all", choices=["all", "none", "precision", "batched", "learned", "full", "mt", "allmove", "functors", "defaults"],
                     help="also measure the precision form, the batched integrals and the learned proposal (all), "
                          "none of them, or one of them")
     ap.add_argument("--copy-probe", action="store_true",
@@ -728,7 +756,7 @@ def main():
         runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
                 "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces),
                 "reference_streams": ("mt", run_reference_streams), "all_coordinate_moves": ("allmove", run_all_coordinate_moves),
-                "device_functors": ("functors", run_device_functors)}
+                "device_functors": ("functors", run_device_functors), "reference_defaults": ("defaults", run_reference_defaults)}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
             if args.workloads in ("all", name):

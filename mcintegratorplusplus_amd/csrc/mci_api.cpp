@@ -529,9 +529,18 @@ mci_status decorrelate_auto(mci_ctx *c)
     HIPCHK(c->res.ensure(2 * (size_t)std::max(nd, 1)));
     HIPCHK(hipMemsetAsync(c->active.p, 1, W, c->stream));
     Chain ch = make_chain(c, &od);
+    // the equilibration traces on the specialised kernels where they apply (Full accumulators of the
+    // X2 family; every walker stepping: the shared loop, or the first round of independent ones) --
+    // the same traces, bit for bit, as the generic kernel's
+    const bool fast_dec = fast_chain(c) && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec, c->movetype, c->rng);
+    auto dec_launch = [&](const uint8_t *act) -> mci_status {
+        if (fast_dec && act == nullptr) { return fast_sample(c->fast, ch, c->prec, min_nmc, od.data(), (int)od.size(), c->stream); }
+        HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
+        return MCI_OK;
+    };
 
     auto sample_estimate = [&](const uint8_t *act) -> mci_status {
-        HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
+        TRY(dec_launch(act));
         for (size_t i = 0; i < od.size(); ++i) {
             EstimJob j;
             j.data = od[i].data;
@@ -573,7 +582,7 @@ mci_status decorrelate_auto(mci_ctx *c)
     std::vector<uint8_t> hact(W);
     while (true) {
         const uint8_t *act = shared ? nullptr : c->active.p;
-        HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
+        TRY(dec_launch(act));
         count += min_nmc;
         if (count >= cap) {
             std::printf("Warning [MCI::initialDecorrelation]: Max number of MC steps reached without equilibration.\n");
