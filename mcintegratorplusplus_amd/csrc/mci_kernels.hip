@@ -165,19 +165,6 @@ struct Arr
     __device__ double operator[](int64_t i) const { return p[i]; }
 };
 
-// OneDim/MultiDimBlockEstimator (Estimators.cpp:60-77 / 158-185) into av[nblocks]
-__device__ inline void block_est(int64_t n, const Col &x, int64_t nblocks, bool onedim, double *av, double &avg, double &err)
-{
-    const int64_t nper = n / nblocks;
-    const double norm = 1. / (double)nper;
-    for (int64_t b = 0; b < nblocks; ++b) {
-        double s = 0.0;
-        for (int64_t i = b * nper; i < (b + 1) * nper; ++i) { s = s + x[i]; }
-        av[b] = s * norm;
-    }
-    uncorr(nblocks, Arr{av}, onedim, avg, err);
-}
-
 __device__ inline double err_delta(int mode, const double *e)
 {
     const int im = 4;
@@ -192,24 +179,193 @@ __device__ inline double err_delta(int mode, const double *e)
     return 0.;
 }
 
-// FCBlocker (Estimators.cpp:83-122, 191-246): 45 block estimates (6..50 blocks), plateau by
-// the smallest summed central difference, 5-point average around it
-__device__ inline void fcblocker(int64_t n, const Col &x, bool onedim, double &avg, double &err)
+// FCBlocker (Estimators.cpp:83-122, 191-246): 45 block estimates (OneDim/MultiDimBlockEstimator,
+// Estimators.cpp:60-77 / 158-185, with k = 6..50 blocks of p = n / k values), the plateau by the
+// smallest summed central difference, a 5-point average around it.
+//
+// The estimate with k blocks sums each block from 0 in index order and then sums the block means
+// (and their squares) from 0 in block order.  Both are prefix computations: every k with the same
+// block length p sees the same block means, and its sums over the first k of them are the running
+// sums after k blocks.  So each distinct p is swept once, and every k of its range (kmin..kmax) is
+// finished as the sweep passes it -- the same operations in the same order as 45 separate
+// estimates, with about n values read per distinct p instead of per k (13 sweeps instead of 45 at
+// n = 100).  The estimates arrive in increasing k, so the plateau search runs as they arrive on a
+// window of the last nine: each central-difference sum and its 5-point averages are formed in the
+// reference's order of operations, and no array is indexed at run time.
+constexpr int FC_NB = 45;   // block counts 6..50
+__constant__ double c_fc_rk[FC_NB] = {1. / 6., 1. / 7., 1. / 8., 1. / 9., 1. / 10., 1. / 11., 1. / 12., 1. / 13., 1. / 14., 1. / 15., 1. / 16., 1. / 17., 1. / 18., 1. / 19., 1. / 20., 1. / 21., 1. / 22., 1. / 23., 1. / 24., 1. / 25., 1. / 26., 1. / 27., 1. / 28., 1. / 29., 1. / 30., 1. / 31., 1. / 32., 1. / 33., 1. / 34., 1. / 35., 1. / 36., 1. / 37., 1. / 38., 1. / 39., 1. / 40., 1. / 41., 1. / 42., 1. / 43., 1. / 44., 1. / 45., 1. / 46., 1. / 47., 1. / 48., 1. / 49., 1. / 50.};
+__constant__ double c_fc_rk1[FC_NB] = {1. / 5., 1. / 6., 1. / 7., 1. / 8., 1. / 9., 1. / 10., 1. / 11., 1. / 12., 1. / 13., 1. / 14., 1. / 15., 1. / 16., 1. / 17., 1. / 18., 1. / 19., 1. / 20., 1. / 21., 1. / 22., 1. / 23., 1. / 24., 1. / 25., 1. / 26., 1. / 27., 1. / 28., 1. / 29., 1. / 30., 1. / 31., 1. / 32., 1. / 33., 1. / 34., 1. / 35., 1. / 36., 1. / 37., 1. / 38., 1. / 39., 1. / 40., 1. / 41., 1. / 42., 1. / 43., 1. / 44., 1. / 45., 1. / 46., 1. / 47., 1. / 48., 1. / 49.};
+
+struct FcState
 {
-    double av[45], er[45], tmp[50], acd[37];
-    for (int i = 0; i < 45; ++i) { block_est(n, x, i + 6, onedim, tmp, av[i], er[i]); }
-    for (int i = 0; i < 37; ++i) { acd[i] = 0.; }
-    for (int i2 = 4; i2 < 41; ++i2) {
-        for (int i1 = 1; i1 <= 4; ++i1) { acd[i2 - 4] = acd[i2 - 4] + err_delta(i1, er + i2 - 4); }
+    double wa[9], we[9];    // the last nine estimates (averages, errors)
+    double best, bav, ber;  // the smallest |central-difference sum| so far and its 5-point averages
+};
+
+// finishes the estimate with k blocks (running sums a, e of its k block means: the tail of
+// uncorr()) and passes it to the plateau search; k is uniform
+__device__ inline void fc_emit(FcState &st, int64_t k, double a, double e, bool onedim)
+{
+    const double norm = c_fc_rk[k - 6];
+    const double aa = a * norm;
+    double ee = e * norm - aa * aa;
+    if (ee > 1.e-300) { ee = onedim ? sqrt(ee / ((double)k - 1.)) : sqrt(ee * c_fc_rk1[k - 6]); }
+    else { ee = 0.; }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        st.wa[t] = st.wa[t + 1];
+        st.we[t] = st.we[t + 1];
     }
-    int im = 0;
-    for (int i2 = 1; i2 < 37; ++i2) {
-        if (fabs(acd[i2]) < fabs(acd[im])) { im = i2; }
+    st.wa[8] = aa;
+    st.we[8] = ee;
+    if (k >= 14) {  // the window holds the estimates k-8..k: the central differences around k-4
+        double acd = 0.;
+#pragma unroll
+        for (int i1 = 1; i1 <= 4; ++i1) { acd = acd + err_delta(i1, st.we); }
+        const double fa = fabs(acd);
+        const double sa = 0.2 * (st.wa[2] + st.wa[3] + st.wa[4] + st.wa[5] + st.wa[6]);
+        const double se = 0.2 * (st.we[2] + st.we[3] + st.we[4] + st.we[5] + st.we[6]);
+        if (k == 14 || fa < st.best) {
+            st.best = fa;
+            st.bav = sa;
+            st.ber = se;
+        }
     }
-    im += 4;
-    avg = 0.2 * (av[im - 2] + av[im - 1] + av[im] + av[im + 1] + av[im + 2]);
-    err = 0.2 * (er[im - 2] + er[im - 1] + er[im] + er[im + 1] + er[im + 2]);
 }
+
+// The sweep of blocks 0..kmax-1 of length p (values 0..kmax*p-1), finishing k = kmin..kmax.  The
+// values are read in batches of eight, one batch ahead of the sums (X::load8 reads any batch start
+// below n + 16 in bounds); a uniform countdown closes the blocks.  Values past the sweep's end
+// (the rest of its last batches) go into blocks that are never finished.
+template <class X>
+__device__ inline void fc_sweep(const X &x, int64_t p, int64_t kmin, int64_t kmax, bool onedim, FcState &st)
+{
+    const double nrm = 1. / (double)p;
+    const int64_t iend = kmax * p;
+    double a = 0., e = 0., s = 0.0;
+    using I = typename X::index;  // block counters: 32 bits where n fits
+    I left = (I)p, b = 0;
+    double v0[8], v1[8];
+    x.load8(0, v0);
+    x.load8(8, v1);
+    auto batch = [&](double (&v)[8], int64_t inext) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            s = s + v[t];
+            if (--left == 0) {
+                const double m = s * nrm;
+                a = a + m;
+                e = e + m * m;
+                s = 0.0;
+                left = (I)p;
+                ++b;
+                if (b >= kmin && b <= kmax) { fc_emit(st, b, a, e, onedim); }
+            }
+        }
+        x.load8(inext, v);
+    };
+    for (int64_t i = 0; i < iend; i += 16) {
+        batch(v0, i + 16);
+        if (i + 8 < iend) { batch(v1, i + 24); }
+    }
+}
+
+template <class X>
+__device__ inline void fcblocker(int64_t n, const X &x, bool onedim, double &avg, double &err)
+{
+    FcState st;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) { st.wa[t] = 0.; st.we[t] = 0.; }
+    st.best = 0.;
+    st.bav = 0.;
+    st.ber = 0.;
+    for (int64_t kmin = 6; kmin <= 50;) {
+        const int64_t p = n / kmin;
+        const int64_t kmax = (n / p < 50) ? n / p : 50;  // the largest k with n / k == p
+        fc_sweep(x, p, kmin, kmax, onedim, st);
+        kmin = kmax + 1;
+    }
+    avg = st.bav;
+    err = st.ber;
+}
+
+// The same for a compile-time n (FcReg<N, 0, 0, ONEDIM>::run): the series in registers and every sweep
+// unrolled, so that the 45 estimates are straight-line code -- the sweeps of different block
+// lengths interleave freely, the window's shifts are register renames and no scalar bookkeeping is
+// left.  n = 100 is the series length of the automatic decorrelation with many walkers.
+template <int N, int Q, int BDONE, bool ONEDIM>
+struct FcReg
+{
+    static constexpr int K = Q + 6;
+    static constexpr int P = N / K;
+    static constexpr bool NEWP = (Q == 0) || (N / (K - 1) != P);  // the first k of its block length
+    static constexpr int B0 = NEWP ? 0 : BDONE;
+    __device__ static void run(const double (&y)[N], double a, double e, FcState &st)
+    {
+        if constexpr (NEWP) {
+            a = 0.;
+            e = 0.;
+        }
+        const double nrm = 1. / (double)P;
+#pragma unroll
+        for (int b = B0; b < K; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < P; ++t) { s = s + y[b * P + t]; }
+            const double m = s * nrm;
+            a = a + m;
+            e = e + m * m;
+        }
+        fc_emit(st, K, a, e, ONEDIM);
+        if constexpr (Q + 1 < FC_NB) { FcReg<N, Q + 1, K, ONEDIM>::run(y, a, e, st); }
+    }
+};
+
+template <int N, bool ONEDIM>
+__device__ inline void fcblocker_reg(const Col &x, double &avg, double &err)
+{
+    static_assert(N >= 50, "FCBlocker needs at least 50 values");
+    double y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) { y[i] = x[i]; }
+    FcState st;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) { st.wa[t] = 0.; st.we[t] = 0.; }
+    st.best = 0.;
+    st.bav = 0.;
+    st.ber = 0.;
+    FcReg<N, 0, 0, ONEDIM>::run(y, 0., 0., st);
+    avg = st.bav;
+    err = st.ber;
+}
+
+// a thread's series staged in LDS, one column per lane of a 64-thread workgroup, rows allocated to a
+// multiple of 8 (>= n): a batch that starts at or past the allocation reads rows 0..7 instead
+struct LdsCol
+{
+    using index = int;
+    const double *p;
+    int64_t rows;
+    __device__ void load8(int64_t i0, double (&v)[8]) const
+    {
+        const double *q = p + ((i0 < rows) ? i0 : 0) * 64;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) { v[t] = q[t * 64]; }
+    }
+};
+
+// a series read from memory: indices clamped to the last value
+struct MemCol
+{
+    using index = int64_t;
+    const double *p;
+    int64_t stride, n;
+    __device__ void load8(int64_t i0, double (&v)[8]) const
+    {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) { v[t] = p[((i0 + t < n) ? i0 + t : n - 1) * stride]; }
+    }
+};
 
 __constant__ double c_chi2_q95[64] = {
     3.841459, 5.991465, 7.814728, 9.487729, 11.070498, 12.591587, 14.067140, 15.507313,
@@ -421,10 +577,15 @@ __device__ inline void mjblocker_reg(const Col &x, double &avg, double &err)
 }
 
 // one instantiation per estimator (E: MCI_ESTIM_NOOP, _UNCORRELATED, _FCBLOCKER, _MJBLOCKER, or
-// EST_MJ_SMALL for MJBlocker on at most 64 values; the host resolves CORRELATED by n), so that the
-// private arrays of FCBlocker and of the general MJBlocker (scratch memory) weigh on no other
+// the staged forms below; the host resolves CORRELATED by n), so that the private arrays of the
+// general MJBlocker (scratch memory) and FCBlocker's 90 registers of estimates weigh on no other
 constexpr int EST_MJ_SMALL = 5;
 constexpr int EST_MJ_REG = 6;  // MJBlocker on 2^P <= 64 values in registers
+constexpr int EST_FC_LDS = 7;  // FCBlocker on at most FC_LDS_MAX values staged in LDS
+constexpr int EST_FC_REG = 8;  // FCBlocker on FC_REG_N values in registers
+constexpr int64_t FC_REG_N = 100;
+constexpr int64_t FC_LDS_MAX = 128;
+__host__ __device__ inline int64_t fc_rows(int64_t n) { return (n + 7) / 8 * 8; }
 template <int E, int P = 0>
 __global__ __launch_bounds__(256) void k_estimate(EstimJob j)
 {
@@ -439,7 +600,25 @@ __global__ __launch_bounds__(256) void k_estimate(EstimJob j)
     double a = 0., e = 0.;
     if (E == MCI_ESTIM_NOOP) { a = x[0]; e = 0.; }
     if (E == MCI_ESTIM_UNCORRELATED) { uncorr(j.n, x, onedim, a, e); }
-    if (E == MCI_ESTIM_FCBLOCKER) { fcblocker(j.n, x, onedim, a, e); }
+    if (E == MCI_ESTIM_FCBLOCKER) { fcblocker(j.n, MemCol{x.p, x.stride, j.n}, onedim, a, e); }
+    if (E == EST_FC_REG) { fcblocker_reg<FC_REG_N, P == 1>(x, a, e); }  // P: one component
+    if (E == EST_FC_LDS) {
+        // each lane writes and reads only its own column: no barrier.  64 loads in flight per lane
+        // (the indices past the series clamped to its last value, the rows past the allocation not
+        // written), so that a series of up to 128 values is staged in two memory round trips
+        double *col = est_sh + threadIdx.x;
+        const int64_t rows = fc_rows(j.n);
+        for (int64_t c0 = 0; c0 < rows; c0 += 64) {
+            double v[64];
+#pragma unroll
+            for (int t = 0; t < 64; ++t) { v[t] = x[(c0 + t < j.n) ? c0 + t : j.n - 1]; }
+#pragma unroll
+            for (int t = 0; t < 64; ++t) {
+                if (c0 + t < rows) { col[(c0 + t) * 64] = v[t]; }
+            }
+        }
+        fcblocker(j.n, LdsCol{col, rows}, onedim, a, e);
+    }
     if (E == MCI_ESTIM_MJBLOCKER) { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
     if (E == EST_MJ_SMALL) { mjblocker_small(j.n, x, est_sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
     if (E == EST_MJ_REG) { mjblocker_reg<P>(x, a, e); }
@@ -646,6 +825,13 @@ hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
     }
     else if (mj && j.n <= MJ_LDS_MAX) { hipLaunchKernelGGL(k_estimate<EST_MJ_SMALL>, g, dim3(64), sh, s, j); }
     else if (mj) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_MJBLOCKER>, g, dim3(64), 0, s, j); }
+    else if ((j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) && j.n == FC_REG_N) {
+        if (j.nobs == 1) { hipLaunchKernelGGL((k_estimate<EST_FC_REG, 1>), g, dim3(64), 0, s, j); }
+        else { hipLaunchKernelGGL((k_estimate<EST_FC_REG, 0>), g, dim3(64), 0, s, j); }
+    }
+    else if ((j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) && j.n <= FC_LDS_MAX) {
+        hipLaunchKernelGGL(k_estimate<EST_FC_LDS>, g, dim3(64), sizeof(double) * 64 * (size_t)fc_rows(j.n), s, j);
+    }
     else if (j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_FCBLOCKER>, g, dim3(64), 0, s, j); }
     else if (j.estim == MCI_ESTIM_UNCORRELATED) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_UNCORRELATED>, g, dim3(64), 0, s, j); }
     else { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_NOOP>, g, dim3(64), 0, s, j); }

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import golden_io as G
-from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_CORR, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, KERNELS_AUTO, KERNELS_GENERIC,
+from mcitest import (COUPLING_INDEPENDENT, COUPLING_SHARED, ESTIM_CORR, ESTIM_FC, ESTIM_MJ, ESTIM_NOOP, ESTIM_UNCORR, KERNELS_AUTO, KERNELS_GENERIC,
                      OBS_EXP_X2, OBS_X, OBS_X2, OBS_X2_SUM, PDF_GAUSS_DIAG, PREC_FP32, RNG_MT, RNG_PHILOX, Chain,
                      chain_setup, estimate)
 
@@ -196,6 +196,26 @@ def test_device_mjblocker_power_of_two_sizes_match_oracle(n):
         a, er = estimate("mci", est, x)
         b, eb = estimate("orc", est, x)
         assert np.array_equal(a, b) and np.array_equal(er, eb), (n, est)
+
+
+@pytest.mark.parametrize("n", [50, 51, 63, 99, 100, 101, 137, 160, 161, 250, 800, 1000, 4097])
+def test_device_fcblocker_matches_oracle(n):
+    """FCBlocker (and Correlated, FCBlocker off the powers of 2) against the oracle bit for bit over
+    the block lengths the 45 block counts take: n = 50 (every count its own length, down to 1),
+    n <= 160 (series staged in LDS, counts sharing a length), longer series (read from memory), in
+    one and in five components (the two error normalisations), with a constant component (zero
+    variance) among AR(1) series."""
+    rng = np.random.default_rng(700 + n)
+    x = np.zeros((n, 5))
+    e = rng.normal(size=(n, 5))
+    for i in range(1, n):
+        x[i] = np.array([0.3, 0.8, 0.95, 0.0, -0.5]) * x[i - 1] + e[i]
+    x[:, 3] = 1.75
+    for data in (x, x[:, 1:2]):
+        for est in (ESTIM_FC, ESTIM_CORR):
+            a, er = estimate("mci", est, data)
+            b, eb = estimate("orc", est, data)
+            assert np.array_equal(a, b) and np.array_equal(er, eb), (n, data.shape, est, a - b, er - eb)
 
 
 def sigma_a(n):
