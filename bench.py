@@ -349,20 +349,23 @@ def run_device_functors():
     return out
 
 
-def run_reference_defaults():
+def run_reference_defaults(shared=True):
     """A drop-in user's integrate with the reference's defaults: MCI(16) with 65,536 walkers, the
     default move (all coordinates, one step size), the default random streams (mt19937_64 per
     walker), automatic step calibration and automatic decorrelation, and addObservable(obs) with
     its defaults (a Full accumulator, the Correlated estimator) for x_i^2, on the benchmark's target;
-    8,192 steps (MJBlocker on the traces).  Phases by HIP events."""
+    8,192 steps (MJBlocker on the traces).  Walkers as ranks sharing calibration and decorrelation
+    (shared), or as independent MCI objects (the default coupling: each walker calibrates and
+    decorrelates by itself, on 800-step rounds).  Phases by HIP events."""
     from mcintegratorplusplus_amd import MCI, constants as K
     sig = sigmas()
     nmc = 8192
     out = {"workload": f"{W} walkers x {nmc} steps, d={D}, reference defaults (all-coordinate move, mt19937_64, "
-                       "automatic calibration and decorrelation, Full accumulator + Correlated estimator)"}
+                       "automatic calibration and decorrelation, Full accumulator + Correlated estimator), "
+                       + ("shared coupling" if shared else "independent walkers")}
     m = MCI(D, W)
     m.setSeed(SEED)
-    m.setWalkerCoupling(K.COUPLING_SHARED)
+    m.setWalkerCoupling(K.COUPLING_SHARED if shared else K.COUPLING_INDEPENDENT)
     m.addSamplingFunction(K.PDF_GAUSS_DIAG, 0.5 / sig ** 2)
     m.addObservable(K.OBS_X2)
     m.integrate(nmc)
@@ -756,7 +759,8 @@ def main():
         runs = {"precision_fp32": ("precision", lambda: run_precision_form(not args.no_cpu)), "batched": ("batched", lambda: run_batched(with_cpu=not args.no_cpu)),
                 "learned_proposal": ("learned", lambda: run_learned(not args.no_cpu)), "full_traces": ("full", run_full_traces),
                 "reference_streams": ("mt", run_reference_streams), "all_coordinate_moves": ("allmove", run_all_coordinate_moves),
-                "device_functors": ("functors", run_device_functors), "reference_defaults": ("defaults", run_reference_defaults)}
+                "device_functors": ("functors", run_device_functors), "reference_defaults": ("defaults", run_reference_defaults),
+                "reference_defaults_independent": ("defaults", lambda: run_reference_defaults(False))}
         line["other_workloads"] = {}
         for k, (name, f) in runs.items():
             if args.workloads in ("all", name):

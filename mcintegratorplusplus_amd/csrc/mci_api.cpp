@@ -530,11 +530,11 @@ mci_status decorrelate_auto(mci_ctx *c)
     HIPCHK(hipMemsetAsync(c->active.p, 1, W, c->stream));
     Chain ch = make_chain(c, &od);
     // the equilibration traces on the specialised kernels where they apply (Full accumulators of the
-    // X2 family; every walker stepping: the shared loop, or the first round of independent ones) --
+    // X2 family; the independent rounds skip the finished walkers as the generic kernel does) --
     // the same traces, bit for bit, as the generic kernel's
     const bool fast_dec = fast_chain(c) && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec, c->movetype, c->rng);
     auto dec_launch = [&](const uint8_t *act) -> mci_status {
-        if (fast_dec && act == nullptr) { return fast_sample(c->fast, ch, c->prec, min_nmc, od.data(), (int)od.size(), c->stream); }
+        if (fast_dec) { return fast_sample(c->fast, ch, c->prec, min_nmc, od.data(), (int)od.size(), c->stream, nullptr, nullptr, act); }
         HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
         return MCI_OK;
     };

@@ -66,6 +66,8 @@ struct FastArgs
     const double *dimsizes;
     uint64_t *mt;          // reference streams: [312][W] mt19937_64 states (F_MT launches)
     int32_t *mtp;          // [W] next state index
+    const uint8_t *active; // [W] or nullptr: walkers whose flag is 0 do nothing (the independent
+                           // decorrelation rounds of walkers that have finished theirs)
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
         }
     }
     __syncthreads();
-    if (!live) { return; }
+    if (!live || (A.active && !A.active[w])) { return; }
     const double step = A.steps[w];
     [[maybe_unused]] const uint32_t wid = (uint32_t)w;
     [[maybe_unused]] const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
@@ -621,7 +623,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
         }
     }
     __syncthreads();
-    if (!live) { return; }
+    if (!live || (A.active && !A.active[w])) { return; }
     double q0 = 0.0;
     for (int i = 0; i < d; ++i) {
         const double v = xget(i);
@@ -887,7 +889,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
         }
         if (A.pend->done) { return; }
     }
-    if (!live) { return; }
+    if (!live || (A.active && !A.active[w])) { return; }
     const int64_t W = A.W, B = A.blocksize;
     double x[DM], xn[DM], sm[DM], a[DM];
     double Sold = 0.0, q = 0.0, e = 0.0, Sr = 0.0, Se = 0.0;
@@ -1314,7 +1316,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
         }
     }
     __syncthreads();
-    if (!live) { return; }
+    if (!live || (A.active && !A.active[w])) { return; }
     const float step = (float)A.steps[w];
     uint64_t t = A.ctr[w];
     const uint32_t wid = (uint32_t)w;
@@ -1511,11 +1513,12 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec, int move
 }
 
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
-                const CalibCtrl *pend, const double *dimsizes)
+                const CalibCtrl *pend, const double *dimsizes, const uint8_t *active)
 {
     (void)fs;
-    if (pend && (prec != MCI_PREC_FP64 || !dimsizes)) { return MCI_E_INVALID_ARGUMENT; }
+    if (pend && (prec != MCI_PREC_FP64 || !dimsizes || active)) { return MCI_E_INVALID_ARGUMENT; }
     FastArgs A;
+    A.active = active;
     A.pend = pend;
     A.dimsizes = dimsizes;
     A.W = c.W;

@@ -22,8 +22,9 @@ struct CalibCtrl;
 // nsteps Metropolis steps; obs == nullptr: no accumulation (decorrelation).  pend (FP64 only):
 // the shared calibration's control state; the launch first applies its pending step factor
 // (pend->apply, clamped by dimsizes as k_scale_steps does) and then does nothing more if
-// pend->done.
+// pend->done.  active (not with pend): per-walker flags, walkers whose flag is 0 do nothing, as in
+// the generic kernel.
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
-                const CalibCtrl *pend = nullptr, const double *dimsizes = nullptr);
+                const CalibCtrl *pend = nullptr, const double *dimsizes = nullptr, const uint8_t *active = nullptr);
 void fast_release(FastState &fs);
 } // namespace mcid
