@@ -138,6 +138,8 @@ struct mci_ctx
     DBuf<int32_t> nsf;    // [W] their availability bits
     DBuf<int64_t> acc, rej;
     DBuf<uint8_t> active;
+    DBuf<int32_t> cal_cons, cal_count, cal_any; // independent calibration on the specialised kernels
+    int calib_indep_hint = 0; // iterations the last such calibration took: the next one's first batch
     std::vector<DBuf<double>> accdata, scratch;
     std::vector<double> h_wavg, h_werr;
     int nobsdim_last = 0;
@@ -452,6 +454,46 @@ mci_status find_shared(mci_ctx *c)
     }
     // the fast path's last factor is still pending when no sampling launch followed its control
     if (fast) { HIPCHK(launch_scale_steps(ch, c->ctrl.p, c->dimsizes.p, c->stream)); }
+    return MCI_OK;
+}
+
+// MCI::findMRT2Step for independent walkers (MCIntegrator.cpp:87-168, one loop per walker) on the
+// specialised kernels: an iteration is a sampling launch of the walkers still in their loop and
+// k_calib_indep_round's bookkeeping; the host issues batches of iterations and reads, after each
+// batch, whether any walker is left (iterations with none left are no-ops).  The same walker
+// states, counters and step sizes, bit for bit, as k_calibrate_indep.
+mci_status find_indep_fast(mci_ctx *c, int64_t min_stat)
+{
+    const int N = c->nfind;
+    if (N == 0) { return MCI_OK; }
+    const int64_t W = c->W;
+    const int maxit = std::abs(N);
+    HIPCHK(c->active.ensure(W));
+    HIPCHK(c->cal_cons.ensure(W));
+    HIPCHK(c->cal_count.ensure(W));
+    HIPCHK(c->cal_any.ensure(maxit));
+    HIPCHK(hipMemsetAsync(c->active.p, 1, W, c->stream));
+    HIPCHK(hipMemsetAsync(c->cal_cons.p, 0, sizeof(int32_t) * W, c->stream));
+    HIPCHK(hipMemsetAsync(c->cal_count.p, 0, sizeof(int32_t) * W, c->stream));
+    HIPCHK(hipMemsetAsync(c->cal_any.p, 0, sizeof(int32_t) * maxit, c->stream));
+    Chain ch = make_chain(c, nullptr);
+    int batch = (c->calib_indep_hint > 0) ? c->calib_indep_hint : 4;
+    for (int it = 0; it < maxit;) {
+        for (int b = 0; b < batch && it < maxit; ++b, ++it) {
+            // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
+            TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, nullptr, nullptr, c->active.p));
+            HIPCHK(launch_calib_indep_round(ch, c->active.p, c->cal_cons.p, c->cal_count.p, N, c->target, c->dimsizes.p,
+                                            c->cal_any.p + it, c->stream));
+        }
+        int32_t left = 0;
+        HIPCHK(hipMemcpyAsync(&left, c->cal_any.p + (it - 1), sizeof left, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!left) {
+            c->calib_indep_hint = std::min(std::max(it, 1), 64);
+            break;
+        }
+        batch = 4;
+    }
     return MCI_OK;
 }
 
@@ -1558,7 +1600,8 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
         if (c->coupling == MCI_COUPLING_SHARED && W > 1) { TRY(find_shared(c)); }
         else {
             const int64_t min_stat = (int64_t)std::fmax(100., std::sqrt(40000. * c->ndim));
-            HIPCHK(calibrate_indep_launch(c, make_chain(c, nullptr), c->nfind, c->target, min_stat, c->dimsizes.p, c->stream));
+            if (fast_chain(c)) { TRY(find_indep_fast(c, min_stat)); }
+            else { HIPCHK(calibrate_indep_launch(c, make_chain(c, nullptr), c->nfind, c->target, min_stat, c->dimsizes.p, c->stream)); }
         }
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));

@@ -29,6 +29,31 @@ __global__ void k_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int
     mtp[w] = 0;
 }
 
+// the bookkeeping of one findMRT2Step iteration for independent walkers (MCIntegrator.cpp:125-166,
+// the body of k_calibrate_indep's loop after its MIN_STAT steps): the walker's rate from the
+// counters of the sampling launch that just ran, the consecutive count, the factor and the clamps,
+// then the loop's exit test; a walker that leaves the loop is marked inactive, and any[0] is set
+// while some walker stays in it.  cons / count start at 0 and active at 1.
+__global__ void k_calib_indep_round(Chain c, uint8_t *active, int32_t *cons, int32_t *count, int nfind, double target,
+                                    const double *dimSizes, int32_t *any)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W || !active[w]) { return; }
+    const int64_t acc = c.acc[w], rej = c.rej[w];
+    const double rate = (acc > 0) ? (double)acc / ((double)acc + (double)rej) : 0.;
+    int cs = cons[w], ct = count[w];
+    if (fabs(rate - target) < 0.05) { ++cs; }
+    else { cs = 0; }
+    const double fact = fmin(2., fmax(0.5, rate / target));
+    scale_and_clamp(c, w, fact, dimSizes);
+    ++ct;
+    const bool stay = !(nfind < 0 && ct >= -nfind) && ((nfind < 0 && cs < 5) || ct < nfind);
+    cons[w] = cs;
+    count[w] = ct;
+    if (stay) { any[0] = 1; }
+    else { active[w] = 0; }
+}
+
 __global__ void k_scale_steps(Chain c, const CalibCtrl *ctrl, const double *dimSizes)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -781,6 +806,13 @@ hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int6
 hipError_t launch_scale_steps(const Chain &c, const CalibCtrl *ctrl, const double *dimSizes, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scale_steps, grid1(c.W, 256), dim3(256), 0, s, c, ctrl, dimSizes);
+    return hipGetLastError();
+}
+
+hipError_t launch_calib_indep_round(const Chain &c, uint8_t *active, int32_t *cons, int32_t *count, int nfind, double target,
+                                    const double *dimSizes, int32_t *any, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_calib_indep_round, grid1(c.W, 256), dim3(256), 0, s, c, active, cons, count, nfind, target, dimSizes, any);
     return hipGetLastError();
 }
 

@@ -44,6 +44,9 @@ hipError_t launch_calib_control(const double *chunks, int64_t W, double target, 
 hipError_t launch_calib_round(const int64_t *acc, const int64_t *rej, int64_t W, double *chunks, double target, int nfind,
                               CalibCtrl *ctrl, hipStream_t s);
 hipError_t launch_walker_rates(const int64_t *acc, const int64_t *rej, double *rate, int64_t W, hipStream_t s);
+// one independent calibration iteration's bookkeeping after its sampling launch (active walkers)
+hipError_t launch_calib_indep_round(const Chain &c, uint8_t *active, int32_t *cons, int32_t *count, int nfind, double target,
+                                    const double *dimSizes, int32_t *any, hipStream_t s);
 hipError_t launch_estimate(const EstimJob &j, hipStream_t s);
 hipError_t launch_chunk_sum(const double *v, int64_t W, int ncomp, int square, double *chunks, hipStream_t s);
 hipError_t launch_combine(const double *ca, const double *ce, int64_t W, int ncomp, double *avg, double *err, hipStream_t s);
