@@ -229,6 +229,8 @@ struct FcState
 
 // finishes the estimate with k blocks (running sums a, e of its k block means: the tail of
 // uncorr()) and passes it to the plateau search; k is uniform
+__device__ inline void fc_push(FcState &st, int64_t k, double aa, double ee);
+
 __device__ inline void fc_emit(FcState &st, int64_t k, double a, double e, bool onedim)
 {
     const double norm = c_fc_rk[k - 6];
@@ -236,6 +238,12 @@ __device__ inline void fc_emit(FcState &st, int64_t k, double a, double e, bool 
     double ee = e * norm - aa * aa;
     if (ee > 1.e-300) { ee = onedim ? sqrt(ee / ((double)k - 1.)) : sqrt(ee * c_fc_rk1[k - 6]); }
     else { ee = 0.; }
+    fc_push(st, k, aa, ee);
+}
+
+// the estimate with k blocks (average aa, error ee) into the window and the plateau search
+__device__ inline void fc_push(FcState &st, int64_t k, double aa, double ee)
+{
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         st.wa[t] = st.wa[t + 1];
@@ -362,6 +370,169 @@ __device__ inline void fcblocker_reg(const Col &x, double &avg, double &err)
     FcReg<N, 0, 0, ONEDIM>::run(y, 0., 0., st);
     avg = st.bav;
     err = st.ber;
+}
+
+// The same for longer series, streaming each value past running block sums.  Every block count k
+// keeps its own running block sum and every value is added to each of them (independent chains);
+// the block ends come from a table built per workgroup in LDS: for each position, the mask of the
+// block counts that close a block there and the mask of those that close their last block there.
+// Each block count tests its bit (groups of eight without a close are skipped together), so the
+// sums stay in registers and no per-value bookkeeping runs on the scalar unit.  Per block count the operations and their order
+// are those of OneDim/MultiDimBlockEstimator: each block summed from 0 in index order, its mean
+// added to the running sums of means and of their squares.  After a block count's last block its
+// running sum takes the rest of the series, which no close reads.  The 45 block counts take three
+// passes over the series (15 each: their three running sums each fit the registers); each pass
+// hands its finished estimates to the windowed plateau search in increasing k.  The values reach LDS in batches of eight through a two-batch ring per lane, the
+// loads two batches ahead; each value and its masks are read one position ahead.
+constexpr int64_t FC_STREAM_MAX = 2048;
+constexpr int FC_STREAM_TPB = 256;
+constexpr int FC_SB = 16;
+__host__ __device__ inline size_t fc_stream_lds(int64_t n)
+{
+    return 16 * (size_t)n + 48 * 8 + (FC_STREAM_TPB / 64) * (2 * FC_SB) * 64 * 8;
+}
+
+template <bool ONEDIM>
+__device__ inline void fc_finish(int k, double &a, double &e)
+{
+    const double norm = c_fc_rk[k - 6];
+    const double aa = a * norm;
+    double ee = e * norm - aa * aa;
+    if (ee > 1.e-300) { ee = ONEDIM ? sqrt(ee / ((double)k - 1.)) : sqrt(ee * c_fc_rk1[k - 6]); }
+    else { ee = 0.; }
+    a = aa;
+    e = ee;
+}
+
+__device__ inline uint64_t uniform64(uint64_t v)
+{
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// rows i0 .. i0+FC_SB-1 of a series (indices past the end clamped to its last value)
+__device__ inline void fc_load_batch(const double *xp, int64_t stride, int n, int i0, double (&v)[FC_SB])
+{
+    if (i0 + FC_SB <= n) {
+        const double *q = xp + (int64_t)i0 * stride;
+#pragma unroll
+        for (int t = 0; t < FC_SB; ++t) {
+            v[t] = *q;
+            q += stride;
+        }
+    }
+    else {
+#pragma unroll
+        for (int t = 0; t < FC_SB; ++t) { v[t] = xp[(int64_t)((i0 + t < n) ? i0 + t : n - 1) * stride]; }
+    }
+}
+
+// block counts k = Q0+6 .. Q1+5 over the whole series
+template <bool ONEDIM, int Q0, int Q1>
+__device__ inline void fc_stream_pass(const double *xp, int64_t stride, int n, const uint64_t *cmask, const uint64_t *fmask,
+                                      const double *nrm, double *ring, FcState &st)
+{
+    constexpr int NQ = Q1 - Q0;
+    constexpr uint64_t RANGE = ((1ull << NQ) - 1) << Q0;
+    double s[NQ], a[NQ], e[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        s[q] = 0.0;
+        a[q] = 0.;
+        e[q] = 0.;
+    }
+    double v[FC_SB];
+    fc_load_batch(xp, stride, n, 0, v);
+#pragma unroll
+    for (int t = 0; t < FC_SB; ++t) { ring[t * 64] = v[t]; }
+    fc_load_batch(xp, stride, n, FC_SB, v);
+    double x = ring[0];
+    uint64_t cm = cmask[0], fm = fmask[0];
+    for (int i0 = 0; i0 < n; i0 += FC_SB) {
+        // batch j + 1 into the ring's other half, batch j + 2's loads in flight
+        const int half = ((i0 / FC_SB + 1) & 1) * FC_SB;
+#pragma unroll
+        for (int t = 0; t < FC_SB; ++t) { ring[(half + t) * 64] = v[t]; }
+        fc_load_batch(xp, stride, n, i0 + 2 * FC_SB, v);
+        const int iend = (i0 + FC_SB < n) ? i0 + FC_SB : n;
+#pragma unroll 1
+        for (int i = i0; i < iend; ++i) {
+            const double xc = x;
+            const uint64_t cc = cm, ff = fm;
+            if (i + 1 < n) {
+                x = ring[((i + 1) & (2 * FC_SB - 1)) * 64];
+                cm = cmask[i + 1];
+                fm = fmask[i + 1];
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) { s[q] = s[q] + xc; }
+            const uint64_t c = uniform64(cc) & RANGE;
+            if (c == 0) { continue; }
+            const uint64_t f = uniform64(ff);
+            // each block count that closes here updates its own registers (no dispatch, no merge);
+            // groups of eight block counts with no close skipped at once
+#pragma unroll
+            for (int g = Q0; g < Q1; g += 8) {
+                if ((c >> g) & 0xFFull) {
+#pragma unroll
+                    for (int q = g; q < g + 8 && q < Q1; ++q) {
+                        if ((c >> q) & 1u) {
+                            const double m = s[q - Q0] * nrm[q];
+                            a[q - Q0] = a[q - Q0] + m;
+                            e[q - Q0] = e[q - Q0] + m * m;
+                            s[q - Q0] = 0.0;
+                            if ((f >> q) & 1u) { fc_finish<ONEDIM>(q + 6, a[q - Q0], e[q - Q0]); }
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) { fc_push(st, Q0 + q + 6, a[q], e[q]); }
+}
+
+template <bool ONEDIM>
+__global__ __launch_bounds__(FC_STREAM_TPB) void k_fc_stream(EstimJob j)
+{
+    extern __shared__ double fs_sh[];
+    const int n = (int)j.n;
+    uint64_t *cmask = reinterpret_cast<uint64_t *>(fs_sh);
+    uint64_t *fmask = cmask + n;
+    double *nrm = reinterpret_cast<double *>(fmask + n);
+    double *ring = nrm + 48 + (threadIdx.x >> 6) * (2 * FC_SB * 64) + (threadIdx.x & 63);  // [16][64] per wave
+    for (int i = threadIdx.x; i < n; i += FC_STREAM_TPB) {
+        cmask[i] = 0;
+        fmask[i] = 0;
+    }
+    if (threadIdx.x < FC_NB) { nrm[threadIdx.x] = 1. / (double)(n / ((int)threadIdx.x + 6)); }
+    __syncthreads();
+    for (int r = threadIdx.x; r < FC_NB * 50; r += FC_STREAM_TPB) {
+        const int q = r / 50, b = r % 50, k = q + 6;
+        if (b < k) {
+            const int pos = (b + 1) * (n / k) - 1;
+            atomicOr((unsigned long long *)&cmask[pos], 1ull << q);
+            if (b == k - 1) { atomicOr((unsigned long long *)&fmask[pos], 1ull << q); }
+        }
+    }
+    __syncthreads();
+    const int64_t tid = (int64_t)blockIdx.x * FC_STREAM_TPB + threadIdx.x;
+    if (tid >= j.W * j.nobs) { return; }
+    const int64_t w = tid % j.W;
+    const int comp = (int)(tid / j.W);
+    if (j.active && !j.active[w]) { return; }
+    const double *xp = j.data + (int64_t)comp * j.W + w;
+    const int64_t stride = (int64_t)j.nobs * j.W;
+    FcState st;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) { st.wa[t] = 0.; st.we[t] = 0.; }
+    st.best = 0.;
+    st.bav = 0.;
+    st.ber = 0.;
+    fc_stream_pass<ONEDIM, 0, 15>(xp, stride, n, cmask, fmask, nrm, ring, st);
+    fc_stream_pass<ONEDIM, 15, 30>(xp, stride, n, cmask, fmask, nrm, ring, st);
+    fc_stream_pass<ONEDIM, 30, FC_NB>(xp, stride, n, cmask, fmask, nrm, ring, st);
+    j.avg[(int64_t)(j.off + comp) * j.W + w] = st.bav * j.scale;
+    j.err[(int64_t)(j.off + comp) * j.W + w] = st.ber * j.scale;
 }
 
 // a thread's series staged in LDS, one column per lane of a 64-thread workgroup, rows allocated to a
@@ -863,6 +1034,17 @@ hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
     }
     else if ((j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) && j.n <= FC_LDS_MAX) {
         hipLaunchKernelGGL(k_estimate<EST_FC_LDS>, g, dim3(64), sizeof(double) * 64 * (size_t)fc_rows(j.n), s, j);
+    }
+    else if ((j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) && j.n <= FC_STREAM_MAX) {
+        const dim3 gs = grid1(j.W * j.nobs, FC_STREAM_TPB);
+        const size_t lds = fc_stream_lds(j.n);
+        const void *kf = (j.nobs == 1) ? reinterpret_cast<const void *>(k_fc_stream<true>) : reinterpret_cast<const void *>(k_fc_stream<false>);
+        if (lds > 65536) {
+            const hipError_t e = hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) { return e; }
+        }
+        if (j.nobs == 1) { hipLaunchKernelGGL(k_fc_stream<true>, gs, dim3(FC_STREAM_TPB), lds, s, j); }
+        else { hipLaunchKernelGGL(k_fc_stream<false>, gs, dim3(FC_STREAM_TPB), lds, s, j); }
     }
     else if (j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_FCBLOCKER>, g, dim3(64), 0, s, j); }
     else if (j.estim == MCI_ESTIM_UNCORRELATED) { hipLaunchKernelGGL(k_estimate<MCI_ESTIM_UNCORRELATED>, g, dim3(64), 0, s, j); }
