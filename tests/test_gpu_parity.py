@@ -198,13 +198,14 @@ def test_device_mjblocker_power_of_two_sizes_match_oracle(n):
         assert np.array_equal(a, b) and np.array_equal(er, eb), (n, est)
 
 
-@pytest.mark.parametrize("n", [50, 51, 63, 99, 100, 101, 137, 160, 161, 250, 800, 1000, 4097])
+@pytest.mark.parametrize("n", [50, 51, 63, 99, 100, 101, 128, 129, 137, 160, 161, 250, 800, 1000, 2047, 2049, 4097])
 def test_device_fcblocker_matches_oracle(n):
     """FCBlocker (and Correlated, FCBlocker off the powers of 2) against the oracle bit for bit over
-    the block lengths the 45 block counts take: n = 50 (every count its own length, down to 1),
-    n <= 160 (series staged in LDS, counts sharing a length), longer series (read from memory), in
-    one and in five components (the two error normalisations), with a constant component (zero
-    variance) among AR(1) series."""
+    the block lengths the 45 block counts take and the device's forms: n = 50 (every count its own
+    length, down to 1), n = 100 (in registers), n <= 128 (staged in LDS, counts sharing a length),
+    129..2048 (streamed past running block sums; 2047 with more than 64 KB of LDS), longer series
+    (swept from memory), in one and in five components (the two error normalisations), with a
+    constant component (zero variance) among AR(1) series."""
     rng = np.random.default_rng(700 + n)
     x = np.zeros((n, 5))
     e = rng.normal(size=(n, 5))
