@@ -130,6 +130,9 @@ struct mci_ctx
     bool walker_results_host = false; // h_wavg/h_werr hold the device's per-walker results
     DBuf<uint64_t> mt, ctr, seeds;
     DBuf<int32_t> mtp;
+    // the walkers' mt19937_64 next indices may differ (per-walker loops, generic-kernel launches,
+    // a restored state): realigned before the next specialised launch of all walkers
+    mutable bool mt_diverged = false;
     DBuf<double> nsv;     // [2][W] saved normals of the move's distribution
     DBuf<double> ms_pold, ms_pnew, ms_x0; // MultiStepMove: proto values of its own sampling functions, start position
     DBuf<double> rec_obs, rec_x;          // file output: recorded observable values and positions
@@ -198,6 +201,7 @@ mci_status ensure_rng(mci_ctx *c)
         HIPCHK(hipMemcpyAsync(c->seeds.p, s.data(), s.size() * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
         HIPCHK(launch_seed_mt(c->mt.p, c->mtp.p, c->seeds.p, c->W, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        c->mt_diverged = false;
     }
     else {
         HIPCHK(c->ctr.ensure(c->W));
@@ -356,6 +360,7 @@ mci_status ensure_rtc(mci_ctx *c)
 hipError_t sample_launch(const mci_ctx *c, const Chain &ch, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s,
                          const CalibCtrl *skip = nullptr)
 {
+    c->mt_diverged = true;
     if (c->rtc.sample) { return rtc_launch_sample(c->rtc, ch, nsteps, acc, active, s, skip); }
     return launch_sample(ch, nsteps, acc, active, s, skip);
 }
@@ -363,8 +368,20 @@ hipError_t sample_launch(const mci_ctx *c, const Chain &ch, int64_t nsteps, int 
 hipError_t calibrate_indep_launch(const mci_ctx *c, const Chain &ch, int nfind, double target, int64_t min_stat, const double *dimSizes,
                                   hipStream_t s)
 {
+    c->mt_diverged = true;
     if (c->rtc.calibrate) { return rtc_launch_calibrate_indep(c->rtc, ch, nfind, target, min_stat, dimSizes, s); }
     return launch_calibrate_indep(ch, nfind, target, min_stat, dimSizes, s);
+}
+
+// Before a specialised launch in which every walker steps: if the walkers' generator next indices
+// may differ, rotate each state to next index 0 (the same generator), so that every state access
+// of a wave is one row instead of a gather over 64 rows.
+mci_status mt_align_if_needed(mci_ctx *c)
+{
+    if (c->rng != MCI_RNG_MT19937_64 || !c->mt_diverged) { return MCI_OK; }
+    HIPCHK(launch_mt_align(c->mt.p, c->mtp.p, c->W, c->stream));
+    c->mt_diverged = false;
+    return MCI_OK;
 }
 
 double obs_p0(const ObsHost &o, int ndim)
@@ -422,6 +439,7 @@ mci_status find_shared(mci_ctx *c)
     std::memset(&init, 0, sizeof init);
     HIPCHK(hipMemcpyAsync(c->ctrl.p, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
     const bool fast = fast_chain(c);
+    if (fast) { TRY(mt_align_if_needed(c)); }
     const int maxit = std::abs(N);
     // the first batch is as long as the last calibration of this context took (iterations after
     // the exit are no-ops, so the batch length changes the time, never the result)
@@ -476,12 +494,14 @@ mci_status find_indep_fast(mci_ctx *c, int64_t min_stat)
     HIPCHK(hipMemsetAsync(c->cal_cons.p, 0, sizeof(int32_t) * W, c->stream));
     HIPCHK(hipMemsetAsync(c->cal_count.p, 0, sizeof(int32_t) * W, c->stream));
     HIPCHK(hipMemsetAsync(c->cal_any.p, 0, sizeof(int32_t) * maxit, c->stream));
+    TRY(mt_align_if_needed(c));
     Chain ch = make_chain(c, nullptr);
     int batch = (c->calib_indep_hint > 0) ? c->calib_indep_hint : 4;
     for (int it = 0; it < maxit;) {
         for (int b = 0; b < batch && it < maxit; ++b, ++it) {
             // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
             TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, nullptr, nullptr, c->active.p));
+            c->mt_diverged = true;
             HIPCHK(launch_calib_indep_round(ch, c->active.p, c->cal_cons.p, c->cal_count.p, N, c->target, c->dimsizes.p,
                                             c->cal_any.p + it, c->stream));
         }
@@ -575,7 +595,9 @@ mci_status decorrelate_auto(mci_ctx *c)
     // X2 family; the independent rounds skip the finished walkers as the generic kernel does) --
     // the same traces, bit for bit, as the generic kernel's
     const bool fast_dec = fast_chain(c) && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec, c->movetype, c->rng);
+    if (fast_dec) { TRY(mt_align_if_needed(c)); }
     auto dec_launch = [&](const uint8_t *act) -> mci_status {
+        if (act) { c->mt_diverged = true; }
         if (fast_dec) { return fast_sample(c->fast, ch, c->prec, min_nmc, od.data(), (int)od.size(), c->stream, nullptr, nullptr, act); }
         HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
         return MCI_OK;
@@ -846,6 +868,7 @@ mci_status mci_move_x(mci_ctx *c)
     TRY(set_current(c));
     TRY(ensure_protos(c));
     TRY(ensure_rng(c));
+    c->mt_diverged = true;
     HIPCHK(launch_move_x(make_chain(c, nullptr), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return MCI_OK;
@@ -856,6 +879,7 @@ mci_status mci_new_random_x(mci_ctx *c)
     CHECKCTX(c);
     TRY(set_current(c));
     TRY(ensure_rng(c));
+    c->mt_diverged = true;
     HIPCHK(launch_new_random_x(make_chain(c, nullptr), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return MCI_OK;
@@ -1608,7 +1632,10 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
     // ---- initial decorrelation (MCIntegrator.cpp:171-247) ----
     if (!c->pdfs.empty() && do_decorr) {
         if (c->ndecorr > 0) {
-            if (fast) { TRY(fast_sample(c->fast, make_chain(c, nullptr), c->prec, c->ndecorr, nullptr, 0, c->stream)); }
+            if (fast) {
+                TRY(mt_align_if_needed(c));
+                TRY(fast_sample(c->fast, make_chain(c, nullptr), c->prec, c->ndecorr, nullptr, 0, c->stream));
+            }
             else { HIPCHK(sample_launch(c, make_chain(c, nullptr), c->ndecorr, 0, nullptr, c->stream)); }
         }
         else if (c->ndecorr < 0) { TRY(decorrelate_auto(c)); }
@@ -1633,7 +1660,10 @@ mci_status integrate_phases(mci_ctx *c, int64_t nmc, double *average, double *er
         ch.rec_x = c->rec_x.p;
         ch.rec_fx = c->wlkfreq;
     }
-    if (fast_main) { TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream)); }
+    if (fast_main) {
+        TRY(mt_align_if_needed(c));
+        TRY(fast_sample(c->fast, ch, c->prec, nmc, od.data(), (int)od.size(), c->stream));
+    }
     else { HIPCHK(sample_launch(c, ch, nmc, 1, nullptr, c->stream)); }
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     if (rec_obs || rec_x) { TRY(write_files(c, nmc, nd, rec_obs, rec_x)); }
@@ -1853,6 +1883,7 @@ mci_status mci_set_state(mci_ctx *c, const void *buf, int64_t nbytes)
         p += sec.second;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->mt_diverged = true;
     c->reseed = false;  // the streams continue from the state
     return MCI_OK;
 }

@@ -54,6 +54,31 @@ __global__ void k_calib_indep_round(Chain c, uint8_t *active, int32_t *cons, int
     else { active[w] = 0; }
 }
 
+// Rotates each walker's mt19937_64 state so that its next index is 0: element j of the new state is
+// element (j + p) mod 312 of the old one, p the old next index.  The lazy twist regenerates the
+// element it outputs from the elements 1 and 156 places after it (mod 312), so a rotated state with
+// its index rotated alike produces the same outputs.  Walkers that went through per-walker loops
+// (independent calibration and decorrelation) stand at different indices; once realigned, every
+// state access of a wave is one row again.  Each lane rotates its own column through LDS.
+constexpr int MT_ALIGN_TPB = 64;
+__global__ __launch_bounds__(MT_ALIGN_TPB) void k_mt_align(uint64_t *mt, int32_t *mtp, int64_t W)
+{
+    extern __shared__ uint64_t mt_col[];  // [MT_N][MT_ALIGN_TPB]
+    const int64_t w = (int64_t)blockIdx.x * MT_ALIGN_TPB + threadIdx.x;
+    if (w >= W) { return; }
+    const int p = mtp[w];
+    if (p == 0) { return; }
+    uint64_t *col = mt_col + threadIdx.x;
+#pragma unroll 8
+    for (int j = 0; j < MT_N; ++j) {
+        const int src = (j + p < MT_N) ? j + p : j + p - MT_N;
+        col[j * MT_ALIGN_TPB] = mt[(int64_t)src * W + w];
+    }
+#pragma unroll 8
+    for (int j = 0; j < MT_N; ++j) { mt[(int64_t)j * W + w] = col[j * MT_ALIGN_TPB]; }
+    mtp[w] = 0;
+}
+
 __global__ void k_scale_steps(Chain c, const CalibCtrl *ctrl, const double *dimSizes)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -959,6 +984,15 @@ static inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1
 hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s)
 {
     hipLaunchKernelGGL(k_seed_mt, grid1(W, 256), dim3(256), 0, s, mt, mtp, seeds, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_mt_align(uint64_t *mt, int32_t *mtp, int64_t W, hipStream_t s)
+{
+    const size_t lds = sizeof(uint64_t) * MT_N * MT_ALIGN_TPB;
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_mt_align), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { return e; }
+    hipLaunchKernelGGL(k_mt_align, dim3((unsigned)((W + MT_ALIGN_TPB - 1) / MT_ALIGN_TPB)), dim3(MT_ALIGN_TPB), lds, s, mt, mtp, W);
     return hipGetLastError();
 }
 

@@ -35,6 +35,8 @@ struct EstimJob
 
 #ifndef MCI_RTC_USER // the host's launch interface (not part of the run-time build of the step kernels)
 hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s);
+// every walker's mt19937_64 state rotated to next index 0 (the same generator)
+hipError_t launch_mt_align(uint64_t *mt, int32_t *mtp, int64_t W, hipStream_t s);
 hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s,
                          const CalibCtrl *skip = nullptr);
 hipError_t launch_calibrate_indep(const Chain &c, int nfind, double target, int64_t min_stat, const double *dimSizes, hipStream_t s);
