@@ -142,6 +142,11 @@ struct mci_ctx
     DBuf<int64_t> acc, rej;
     DBuf<uint8_t> active;
     DBuf<int32_t> cal_cons, cal_count, cal_any; // independent calibration on the specialised kernels
+    // the walkers still calibrating, compacted: their sampling columns and the list of who they are
+    DBuf<double> cmp_x, cmp_steps;
+    DBuf<int64_t> cmp_acc, cmp_rej;
+    DBuf<uint64_t> cmp_mt, cmp_ctr;
+    DBuf<int32_t> cmp_mtp, cmp_list, cmp_count;
     int calib_indep_hint = 0; // iterations the last such calibration took: the next one's first batch
     std::vector<DBuf<double>> accdata, scratch;
     std::vector<double> h_wavg, h_werr;
@@ -478,8 +483,10 @@ mci_status find_shared(mci_ctx *c)
 // MCI::findMRT2Step for independent walkers (MCIntegrator.cpp:87-168, one loop per walker) on the
 // specialised kernels: an iteration is a sampling launch of the walkers still in their loop and
 // k_calib_indep_round's bookkeeping; the host issues batches of iterations and reads, after each
-// batch, whether any walker is left (iterations with none left are no-ops).  The same walker
-// states, counters and step sizes, bit for bit, as k_calibrate_indep.
+// batch, whether any walker is left (iterations with none left are no-ops).  The walkers still in
+// their loop are compacted into dense columns for the sampling launch (most have left after a few
+// iterations, and a wave with one of them would otherwise run in full), then moved back.  The same
+// walker states, counters and step sizes, bit for bit, as k_calibrate_indep.
 mci_status find_indep_fast(mci_ctx *c, int64_t min_stat)
 {
     const int N = c->nfind;
@@ -496,11 +503,36 @@ mci_status find_indep_fast(mci_ctx *c, int64_t min_stat)
     HIPCHK(hipMemsetAsync(c->cal_any.p, 0, sizeof(int32_t) * maxit, c->stream));
     TRY(mt_align_if_needed(c));
     Chain ch = make_chain(c, nullptr);
+    const bool mt = c->rng == MCI_RNG_MT19937_64;
+    HIPCHK(c->cmp_x.ensure((size_t)c->ndim * W));
+    HIPCHK(c->cmp_steps.ensure(W));
+    HIPCHK(c->cmp_acc.ensure(W));
+    HIPCHK(c->cmp_rej.ensure(W));
+    if (mt) {
+        HIPCHK(c->cmp_mt.ensure((size_t)MT_N * W));
+        HIPCHK(c->cmp_mtp.ensure(W));
+    }
+    else { HIPCHK(c->cmp_ctr.ensure(W)); }
+    HIPCHK(c->cmp_list.ensure(W));
+    HIPCHK(c->cmp_count.ensure(1));
+    Chain dense = ch;
+    dense.xold = c->cmp_x.p;
+    dense.steps = c->cmp_steps.p;
+    dense.acc = c->cmp_acc.p;
+    dense.rej = c->cmp_rej.p;
+    dense.mt = mt ? c->cmp_mt.p : nullptr;
+    dense.mtp = mt ? c->cmp_mtp.p : nullptr;
+    dense.ctr = mt ? nullptr : c->cmp_ctr.p;
     int batch = (c->calib_indep_hint > 0) ? c->calib_indep_hint : 4;
     for (int it = 0; it < maxit;) {
         for (int b = 0; b < batch && it < maxit; ++b, ++it) {
             // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
-            TRY(fast_sample(c->fast, ch, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, nullptr, nullptr, c->active.p));
+            HIPCHK(hipMemsetAsync(c->cmp_count.p, 0, sizeof(int32_t), c->stream));
+            HIPCHK(launch_compact_active(c->active.p, W, c->cmp_list.p, c->cmp_count.p, c->stream));
+            HIPCHK(launch_walker_columns(ch, dense, c->cmp_list.p, c->cmp_count.p, 1, c->stream));
+            TRY(fast_sample(c->fast, dense, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, nullptr, nullptr, nullptr, c->cmp_list.p,
+                            c->cmp_count.p));
+            HIPCHK(launch_walker_columns(ch, dense, c->cmp_list.p, c->cmp_count.p, 0, c->stream));
             c->mt_diverged = true;
             HIPCHK(launch_calib_indep_round(ch, c->active.p, c->cal_cons.p, c->cal_count.p, N, c->target, c->dimsizes.p,
                                             c->cal_any.p + it, c->stream));

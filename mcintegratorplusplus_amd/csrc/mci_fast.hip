@@ -68,6 +68,8 @@ struct FastArgs
     int32_t *mtp;          // [W] next state index
     const uint8_t *active; // [W] or nullptr: walkers whose flag is 0 do nothing (the independent
                            // decorrelation rounds of walkers that have finished theirs)
+    const int32_t *wmap;   // or nullptr: the walker each lane's columns belong to (its Philox stream),
+    const int32_t *wcount; // and the number of lanes in use, when the walkers are compacted
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
-    const bool live = w < A.W;
+    const bool live = w < A.W && (!A.wcount || w < *A.wcount);
     if (A.pend) {
         // the previous calibration iteration's factor (k_scale_steps with one step type), then
         // nothing more once the loop has finished
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64(FastArgs A)
     __syncthreads();
     if (!live || (A.active && !A.active[w])) { return; }
     const double step = A.steps[w];
-    [[maybe_unused]] const uint32_t wid = (uint32_t)w;
+    [[maybe_unused]] const uint32_t wid = A.wmap ? (uint32_t)A.wmap[w] : (uint32_t)w;
     [[maybe_unused]] const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
     [[maybe_unused]] uint64_t t = MT ? 0 : A.ctr[w];
     int64_t nacc = 0;
@@ -595,7 +597,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
-    const bool live = w < A.W;
+    const bool live = w < A.W && (!A.wcount || w < *A.wcount);
     if (A.pend) {
         if (A.pend->apply && live) {
             double st = A.steps[w] * A.pend->fact;
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_all(FastArgs A)
     }
     if (HEXP) { e = exp(-(q0 / A.exp_div)); }
     const double step = A.steps[w];
-    const uint32_t wid = (uint32_t)w;
+    const uint32_t wid = A.wmap ? (uint32_t)A.wmap[w] : (uint32_t)w;
     const int64_t W = A.W;
     [[maybe_unused]] uint64_t t = MT ? 0 : A.ctr[w];
     // Reference streams: the walker's mt19937_64 outputs in chunks of MT_C into a per-lane LDS ring
@@ -877,7 +879,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
     const int d = EX ? DM : A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
-    const bool live = w < A.W;
+    const bool live = w < A.W && (!A.wcount || w < *A.wcount);
     if (A.pend) {
         if (A.pend->apply && live) {
             double st = A.steps[w] * A.pend->fact;
@@ -908,7 +910,7 @@ __global__ __launch_bounds__(WAVE) void k_fast64_allr(FastArgs A)
     }
     if (HEXP) { e = exp(-(q / A.exp_div)); }
     const double step = A.steps[w];
-    const uint32_t wid = (uint32_t)w;
+    const uint32_t wid = A.wmap ? (uint32_t)A.wmap[w] : (uint32_t)w;
     [[maybe_unused]] uint64_t t = MT ? 0 : A.ctr[w];
     int64_t nacc = 0, inblk = 0, b = 0, s = 0;
 
@@ -1300,7 +1302,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     const int d = A.ndim;
     const int lane = threadIdx.x;
     const int64_t w = (int64_t)blockIdx.x * WAVE + lane;
-    const bool live = w < A.W;
+    const bool live = w < A.W && (!A.wcount || w < *A.wcount);
     float *xs = lds32;                                                // [d][64] x_i
     float *Ss = lds32 + d * WAVE;                                     // X2: [d][64] block sums of x_i^2
     uint32_t *CJ = reinterpret_cast<uint32_t *>(lds32 + 2 * d * WAVE); // X2: [d][64] BF16 compensation << 16 | last change
@@ -1319,7 +1321,7 @@ __global__ __launch_bounds__(WAVE) void k_fast32(FastArgs A)
     if (!live || (A.active && !A.active[w])) { return; }
     const float step = (float)A.steps[w];
     uint64_t t = A.ctr[w];
-    const uint32_t wid = (uint32_t)w;
+    const uint32_t wid = A.wmap ? (uint32_t)A.wmap[w] : (uint32_t)w;
     const uint32_t range = (uint32_t)A.nvecs, thr = A.lemire_thr;
     const int B = (int)A.blocksize;  // at most 2^31 - 1 (checked by the host)
     const float inv_ediv = (float)(1.0 / A.exp_div);
@@ -1513,12 +1515,15 @@ bool fast_obs_eligible(const ObsDev *obs, int nobs, int ndim, int prec, int move
 }
 
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
-                const CalibCtrl *pend, const double *dimsizes, const uint8_t *active)
+                const CalibCtrl *pend, const double *dimsizes, const uint8_t *active, const int32_t *wmap,
+                const int32_t *wcount)
 {
     (void)fs;
     if (pend && (prec != MCI_PREC_FP64 || !dimsizes || active)) { return MCI_E_INVALID_ARGUMENT; }
     FastArgs A;
     A.active = active;
+    A.wmap = wmap;
+    A.wcount = wcount;
     A.pend = pend;
     A.dimsizes = dimsizes;
     A.W = c.W;

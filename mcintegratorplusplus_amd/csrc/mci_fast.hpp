@@ -23,8 +23,10 @@ struct CalibCtrl;
 // the shared calibration's control state; the launch first applies its pending step factor
 // (pend->apply, clamped by dimsizes as k_scale_steps does) and then does nothing more if
 // pend->done.  active (not with pend): per-walker flags, walkers whose flag is 0 do nothing, as in
-// the generic kernel.
+// the generic kernel.  wmap / wcount (device): the chain's per-walker columns hold a compacted set of
+// walkers -- lane i < *wcount is walker wmap[i] (its Philox stream); the lanes past *wcount do nothing.
 int fast_sample(FastState &fs, const Chain &c, int prec, int64_t nsteps, const ObsDev *obs, int nobs, hipStream_t s,
-                const CalibCtrl *pend = nullptr, const double *dimsizes = nullptr, const uint8_t *active = nullptr);
+                const CalibCtrl *pend = nullptr, const double *dimsizes = nullptr, const uint8_t *active = nullptr,
+                const int32_t *wmap = nullptr, const int32_t *wcount = nullptr);
 void fast_release(FastState &fs);
 } // namespace mcid

@@ -79,6 +79,40 @@ __global__ __launch_bounds__(MT_ALIGN_TPB) void k_mt_align(uint64_t *mt, int32_t
     mtp[w] = 0;
 }
 
+// The walkers whose flag is set, listed in the first *count entries of list (*count starts at 0; the
+// order follows the atomic's, and no result depends on it: each walker's columns move with it).
+__global__ void k_compact_active(const uint8_t *active, int64_t W, int32_t *list, int32_t *count)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W || !active[w]) { return; }
+    list[atomicAdd(count, 1)] = (int32_t)w;
+}
+
+// The per-walker columns a specialised sampling launch reads and writes, moved between a chain's
+// arrays and compacted ones (stride W in both): gather (to_dense) moves walker list[i] to column i
+// (positions, the step size, the generator state), scatter moves column i back to walker list[i]
+// (positions, the generator state, the acceptance counters).
+__global__ void k_walker_columns(Chain c, Chain d, const int32_t *list, const int32_t *count, int to_dense)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) { return; }
+    const int64_t W = c.W, wl = list[i];
+    const int64_t a = to_dense ? wl : i, b = to_dense ? i : wl;  // source and destination columns
+    const Chain &src = to_dense ? c : d;
+    const Chain &dst = to_dense ? d : c;
+    for (int r = 0; r < c.ndim; ++r) { dst.xold[(int64_t)r * W + b] = src.xold[(int64_t)r * W + a]; }
+    if (to_dense) { dst.steps[b] = src.steps[a]; }
+    else {
+        dst.acc[b] = src.acc[a];
+        dst.rej[b] = src.rej[a];
+    }
+    if (c.rng == MCI_RNG_MT19937_64) {
+        for (int r = 0; r < MT_N; ++r) { dst.mt[(int64_t)r * W + b] = src.mt[(int64_t)r * W + a]; }
+        dst.mtp[b] = src.mtp[a];
+    }
+    else { dst.ctr[b] = src.ctr[a]; }
+}
+
 __global__ void k_scale_steps(Chain c, const CalibCtrl *ctrl, const double *dimSizes)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -984,6 +1018,19 @@ static inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1
 hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s)
 {
     hipLaunchKernelGGL(k_seed_mt, grid1(W, 256), dim3(256), 0, s, mt, mtp, seeds, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_active(const uint8_t *active, int64_t W, int32_t *list, int32_t *count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_compact_active, grid1(W, 256), dim3(256), 0, s, active, W, list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_walker_columns(const Chain &c, const Chain &dense, const int32_t *list, const int32_t *count, int to_dense,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_walker_columns, grid1(c.W, 256), dim3(256), 0, s, c, dense, list, count, to_dense);
     return hipGetLastError();
 }
 

@@ -35,6 +35,11 @@ struct EstimJob
 
 #ifndef MCI_RTC_USER // the host's launch interface (not part of the run-time build of the step kernels)
 hipError_t launch_seed_mt(uint64_t *mt, int32_t *mtp, const uint64_t *seeds, int64_t W, hipStream_t s);
+// the walkers with a set flag listed (count starts at 0); their sampling columns moved to / from
+// compacted arrays (dense: the chain with the compacted arrays)
+hipError_t launch_compact_active(const uint8_t *active, int64_t W, int32_t *list, int32_t *count, hipStream_t s);
+hipError_t launch_walker_columns(const Chain &c, const Chain &dense, const int32_t *list, const int32_t *count, int to_dense,
+                                 hipStream_t s);
 // every walker's mt19937_64 state rotated to next index 0 (the same generator)
 hipError_t launch_mt_align(uint64_t *mt, int32_t *mtp, int64_t W, hipStream_t s);
 hipError_t launch_sample(const Chain &c, int64_t nsteps, int acc, const uint8_t *active, hipStream_t s,
