@@ -480,6 +480,49 @@ mci_status find_shared(mci_ctx *c)
     return MCI_OK;
 }
 
+// The compacted copies of a chain's sampling columns (positions, step size, generator state,
+// acceptance counters), for the walkers still in a per-walker loop.
+mci_status dense_chain(mci_ctx *c, const Chain &ch, Chain &dense)
+{
+    const int64_t W = c->W;
+    const bool mt = c->rng == MCI_RNG_MT19937_64;
+    HIPCHK(c->cmp_x.ensure((size_t)c->ndim * W));
+    HIPCHK(c->cmp_steps.ensure(W));
+    HIPCHK(c->cmp_acc.ensure(W));
+    HIPCHK(c->cmp_rej.ensure(W));
+    if (mt) {
+        HIPCHK(c->cmp_mt.ensure((size_t)MT_N * W));
+        HIPCHK(c->cmp_mtp.ensure(W));
+    }
+    else { HIPCHK(c->cmp_ctr.ensure(W)); }
+    HIPCHK(c->cmp_list.ensure(W));
+    HIPCHK(c->cmp_count.ensure(1));
+    dense = ch;
+    dense.xold = c->cmp_x.p;
+    dense.steps = c->cmp_steps.p;
+    dense.acc = c->cmp_acc.p;
+    dense.rej = c->cmp_rej.p;
+    dense.mt = mt ? c->cmp_mt.p : nullptr;
+    dense.mtp = mt ? c->cmp_mtp.p : nullptr;
+    dense.ctr = mt ? nullptr : c->cmp_ctr.p;
+    return MCI_OK;
+}
+
+// One specialised sampling launch of the walkers whose flag (c->active) is set, compacted: the
+// flagged walkers listed (c->cmp_list, c->cmp_count), their columns gathered into the dense chain,
+// the launch over the list (a wave with one flagged walker would otherwise run in full), the columns
+// scattered back.  Full traces (obs) are written to the dense columns.
+mci_status compacted_sample(mci_ctx *c, const Chain &ch, const Chain &dense, int prec, int64_t nsteps, const ObsDev *obs, int nobs)
+{
+    HIPCHK(hipMemsetAsync(c->cmp_count.p, 0, sizeof(int32_t), c->stream));
+    HIPCHK(launch_compact_active(c->active.p, c->W, c->cmp_list.p, c->cmp_count.p, c->stream));
+    HIPCHK(launch_walker_columns(ch, dense, c->cmp_list.p, c->cmp_count.p, 1, c->stream));
+    TRY(fast_sample(c->fast, dense, prec, nsteps, obs, nobs, c->stream, nullptr, nullptr, nullptr, c->cmp_list.p, c->cmp_count.p));
+    HIPCHK(launch_walker_columns(ch, dense, c->cmp_list.p, c->cmp_count.p, 0, c->stream));
+    c->mt_diverged = true;
+    return MCI_OK;
+}
+
 // MCI::findMRT2Step for independent walkers (MCIntegrator.cpp:87-168, one loop per walker) on the
 // specialised kernels: an iteration is a sampling launch of the walkers still in their loop and
 // k_calib_indep_round's bookkeeping; the host issues batches of iterations and reads, after each
@@ -502,38 +545,13 @@ mci_status find_indep_fast(mci_ctx *c, int64_t min_stat)
     HIPCHK(hipMemsetAsync(c->cal_count.p, 0, sizeof(int32_t) * W, c->stream));
     HIPCHK(hipMemsetAsync(c->cal_any.p, 0, sizeof(int32_t) * maxit, c->stream));
     TRY(mt_align_if_needed(c));
-    Chain ch = make_chain(c, nullptr);
-    const bool mt = c->rng == MCI_RNG_MT19937_64;
-    HIPCHK(c->cmp_x.ensure((size_t)c->ndim * W));
-    HIPCHK(c->cmp_steps.ensure(W));
-    HIPCHK(c->cmp_acc.ensure(W));
-    HIPCHK(c->cmp_rej.ensure(W));
-    if (mt) {
-        HIPCHK(c->cmp_mt.ensure((size_t)MT_N * W));
-        HIPCHK(c->cmp_mtp.ensure(W));
-    }
-    else { HIPCHK(c->cmp_ctr.ensure(W)); }
-    HIPCHK(c->cmp_list.ensure(W));
-    HIPCHK(c->cmp_count.ensure(1));
-    Chain dense = ch;
-    dense.xold = c->cmp_x.p;
-    dense.steps = c->cmp_steps.p;
-    dense.acc = c->cmp_acc.p;
-    dense.rej = c->cmp_rej.p;
-    dense.mt = mt ? c->cmp_mt.p : nullptr;
-    dense.mtp = mt ? c->cmp_mtp.p : nullptr;
-    dense.ctr = mt ? nullptr : c->cmp_ctr.p;
+    Chain ch = make_chain(c, nullptr), dense;
+    TRY(dense_chain(c, ch, dense));
     int batch = (c->calib_indep_hint > 0) ? c->calib_indep_hint : 4;
     for (int it = 0; it < maxit;) {
         for (int b = 0; b < batch && it < maxit; ++b, ++it) {
             // the calibration runs the FP64 chain in both precisions (step sizes stay FP64 quantities)
-            HIPCHK(hipMemsetAsync(c->cmp_count.p, 0, sizeof(int32_t), c->stream));
-            HIPCHK(launch_compact_active(c->active.p, W, c->cmp_list.p, c->cmp_count.p, c->stream));
-            HIPCHK(launch_walker_columns(ch, dense, c->cmp_list.p, c->cmp_count.p, 1, c->stream));
-            TRY(fast_sample(c->fast, dense, MCI_PREC_FP64, min_stat, nullptr, 0, c->stream, nullptr, nullptr, nullptr, c->cmp_list.p,
-                            c->cmp_count.p));
-            HIPCHK(launch_walker_columns(ch, dense, c->cmp_list.p, c->cmp_count.p, 0, c->stream));
-            c->mt_diverged = true;
+            TRY(compacted_sample(c, ch, dense, MCI_PREC_FP64, min_stat, nullptr, 0));
             HIPCHK(launch_calib_indep_round(ch, c->active.p, c->cal_cons.p, c->cal_count.p, N, c->target, c->dimsizes.p,
                                             c->cal_any.p + it, c->stream));
         }
@@ -627,10 +645,19 @@ mci_status decorrelate_auto(mci_ctx *c)
     // X2 family; the independent rounds skip the finished walkers as the generic kernel does) --
     // the same traces, bit for bit, as the generic kernel's
     const bool fast_dec = fast_chain(c) && fast_obs_eligible(od.data(), (int)od.size(), c->ndim, c->prec, c->movetype, c->rng);
-    if (fast_dec) { TRY(mt_align_if_needed(c)); }
+    // the independent rounds after the first step the walkers still decorrelating, compacted (their
+    // traces in the dense columns, which the estimators map back to the walkers)
+    Chain dense;
+    if (fast_dec) {
+        TRY(mt_align_if_needed(c));
+        if (!shared) { TRY(dense_chain(c, ch, dense)); }
+    }
+    bool compacted = false;
     auto dec_launch = [&](const uint8_t *act) -> mci_status {
+        compacted = fast_dec && act != nullptr;
+        if (compacted) { return compacted_sample(c, ch, dense, c->prec, min_nmc, od.data(), (int)od.size()); }
         if (act) { c->mt_diverged = true; }
-        if (fast_dec) { return fast_sample(c->fast, ch, c->prec, min_nmc, od.data(), (int)od.size(), c->stream, nullptr, nullptr, act); }
+        if (fast_dec) { return fast_sample(c->fast, ch, c->prec, min_nmc, od.data(), (int)od.size(), c->stream); }
         HIPCHK(sample_launch(c, ch, min_nmc, 1, act, c->stream));
         return MCI_OK;
     };
@@ -650,6 +677,10 @@ mci_status decorrelate_auto(mci_ctx *c)
             j.active = act;
             j.avg = c->wavg.p;
             j.err = c->werr.p;
+            if (compacted) {
+                j.wmap = c->cmp_list.p;
+                j.wcount = c->cmp_count.p;
+            }
             HIPCHK(launch_estimate(j, c->stream));
         }
         return MCI_OK;
@@ -698,6 +729,10 @@ mci_status decorrelate_auto(mci_ctx *c)
             j.active = act;
             j.avg = c->wavg.p;
             j.err = c->werr.p;
+            if (compacted) {
+                j.wmap = c->cmp_list.p;
+                j.wcount = c->cmp_count.p;
+            }
             HIPCHK(launch_estimate(j, c->stream));
         }
         if (shared) {

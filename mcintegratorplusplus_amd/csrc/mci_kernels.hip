@@ -578,7 +578,9 @@ __global__ __launch_bounds__(FC_STREAM_TPB) void k_fc_stream(EstimJob j)
     if (tid >= j.W * j.nobs) { return; }
     const int64_t w = tid % j.W;
     const int comp = (int)(tid / j.W);
-    if (j.active && !j.active[w]) { return; }
+    if (j.wcount && w >= *j.wcount) { return; }
+    const int64_t wo = j.wmap ? j.wmap[w] : w;  // the walker of this column
+    if (j.active && !j.active[wo]) { return; }
     const double *xp = j.data + (int64_t)comp * j.W + w;
     const int64_t stride = (int64_t)j.nobs * j.W;
     FcState st;
@@ -590,8 +592,8 @@ __global__ __launch_bounds__(FC_STREAM_TPB) void k_fc_stream(EstimJob j)
     fc_stream_pass<ONEDIM, 0, 15>(xp, stride, n, cmask, fmask, nrm, ring, st);
     fc_stream_pass<ONEDIM, 15, 30>(xp, stride, n, cmask, fmask, nrm, ring, st);
     fc_stream_pass<ONEDIM, 30, FC_NB>(xp, stride, n, cmask, fmask, nrm, ring, st);
-    j.avg[(int64_t)(j.off + comp) * j.W + w] = st.bav * j.scale;
-    j.err[(int64_t)(j.off + comp) * j.W + w] = st.ber * j.scale;
+    j.avg[(int64_t)(j.off + comp) * j.W + wo] = st.bav * j.scale;
+    j.err[(int64_t)(j.off + comp) * j.W + wo] = st.ber * j.scale;
 }
 
 // a thread's series staged in LDS, one column per lane of a 64-thread workgroup, rows allocated to a
@@ -849,7 +851,9 @@ __global__ __launch_bounds__(256) void k_estimate(EstimJob j)
     if (tid >= j.W * j.nobs) { return; }
     const int64_t w = tid % j.W;
     const int comp = (int)(tid / j.W);
-    if (j.active && !j.active[w]) { return; }
+    if (j.wcount && w >= *j.wcount) { return; }
+    const int64_t wo = j.wmap ? j.wmap[w] : w;  // the walker of this column
+    if (j.active && !j.active[wo]) { return; }
     const Col x{j.data + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W};
     const bool onedim = (j.nobs == 1);
     double a = 0., e = 0.;
@@ -877,8 +881,8 @@ __global__ __launch_bounds__(256) void k_estimate(EstimJob j)
     if (E == MCI_ESTIM_MJBLOCKER) { mjblocker(j.n, x, j.scratch + (int64_t)comp * j.W + w, (int64_t)j.nobs * j.W, a, e); }
     if (E == EST_MJ_SMALL) { mjblocker_small(j.n, x, est_sh + threadIdx.x, (int64_t)blockDim.x, a, e); }
     if (E == EST_MJ_REG) { mjblocker_reg<P>(x, a, e); }
-    j.avg[(int64_t)(j.off + comp) * j.W + w] = a * j.scale;
-    j.err[(int64_t)(j.off + comp) * j.W + w] = e * j.scale;
+    j.avg[(int64_t)(j.off + comp) * j.W + wo] = a * j.scale;
+    j.err[(int64_t)(j.off + comp) * j.W + wo] = e * j.scale;
 }
 
 // ---- deterministic cross-walker sums: chunks of 256 (pairwise tree), chunks in order ----

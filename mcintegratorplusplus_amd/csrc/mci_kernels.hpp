@@ -31,6 +31,9 @@ struct EstimJob
     double scale;         // volume factor (no sampling function), else 1
     const uint8_t *active;
     double *avg, *err;    // [nobsdim][W]
+    // or nullptr: the data's columns hold a compacted set of walkers -- column i < *wcount belongs to
+    // walker wmap[i], whose results go to column wmap[i]; the other columns are not read
+    const int32_t *wmap = nullptr, *wcount = nullptr;
 };
 
 #ifndef MCI_RTC_USER // the host's launch interface (not part of the run-time build of the step kernels)
