@@ -634,6 +634,38 @@ __constant__ double c_chi2_q95[64] = {
     66.338649, 67.504807, 68.669294, 69.832160, 70.993453, 72.153216, 73.311493, 74.468324,
     75.623748, 76.777803, 77.930524, 79.081944, 80.232098, 81.381015, 82.528727, 83.675261};
 
+// The first MJ_LB blocking levels of a block of 2^MJ_LB values, in registers: each level's terms
+// added to its sums in index order (FIRST: the series' first block, whose first value has no
+// predecessor), then the block halved in place; v[0] ends as the block's value at level MJ_LB.
+constexpr int MJ_LB = 4, MJ_BS = 1 << MJ_LB;
+template <bool FIRST>
+__device__ inline void mj_block_levels(double (&v)[MJ_BS], double mean, double (&g0)[MJ_LB], double (&g1)[MJ_LB], double (&pv)[MJ_LB])
+{
+#pragma unroll
+    for (int k = 0; k < MJ_LB; ++k) {
+        const int len = MJ_BS >> k;
+#pragma unroll
+        for (int t = 0; t < MJ_BS; ++t) {
+            if (t < len) {
+                const double c = v[t] - mean;
+                g0[k] = g0[k] + c * c;
+                if (!(FIRST && t == 0)) { g1[k] = g1[k] + pv[k] * c; }
+                pv[k] = c;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < MJ_BS / 2; ++t) {
+            if (t < len / 2) { v[t] = 0.5 * (v[2 * t] + v[2 * t + 1]); }
+        }
+    }
+}
+
+__device__ inline void mj_load_block(const Col &x, int64_t i0, double (&v)[MJ_BS])
+{
+#pragma unroll
+    for (int t = 0; t < MJ_BS; ++t) { v[t] = x[i0 + t]; }
+}
+
 // Automatic blocking of Jonsson (PRE 98, 043304 (2018)) as organised in MJBlocker.cpp:127-154.
 // The blocking transform runs in place on this thread's column of the scratch copy `y`.
 // A level whose variance is exactly zero contributes M = 0 (the reference reads out of
@@ -643,10 +675,52 @@ __device__ inline void mjblocker(int64_t n, const Col &x, double *y, int64_t yst
     int npow = 0;
     while (((int64_t)1 << npow) < n) { ++npow; }
     double mean = 0.0;
-    for (int64_t i = 0; i < n; ++i) { mean = mean + x[i]; }
+    int64_t i = 0;
+    for (; i + MJ_BS <= n; i += MJ_BS) {  // batches of loads, the sum in index order
+        double v[MJ_BS];
+        mj_load_block(x, i, v);
+#pragma unroll
+        for (int t = 0; t < MJ_BS; ++t) { mean = mean + v[t]; }
+    }
+    for (; i < n; ++i) { mean = mean + x[i]; }
     mean = mean / (double)n;
     double var[64], gam[64];
     int64_t nred = n;
+    int k0 = 0;
+    if (n >= 2 * MJ_BS && (n & (n - 1)) == 0) {
+        // Levels 0 .. MJ_LB-1 in one pass over the data: blocks of MJ_BS values (every pair of
+        // every one of these levels lies inside one block) in registers; each level's sums run
+        // over its values in index order as in the per-level passes below, and only the
+        // level-MJ_LB values (n / MJ_BS) go to the scratch copy.  About 2n values of traffic
+        // instead of 4n; few registers, so that many waves keep loads in flight.
+        double g0[MJ_LB], g1[MJ_LB], pv[MJ_LB];
+#pragma unroll
+        for (int k = 0; k < MJ_LB; ++k) {
+            g0[k] = 0.0;
+            g1[k] = 0.0;
+            pv[k] = 0.0;
+        }
+        const int64_t nb = n / MJ_BS;
+        {
+            double v[MJ_BS];
+            mj_load_block(x, 0, v);
+            mj_block_levels<true>(v, mean, g0, g1, pv);
+            y[0] = v[0];
+        }
+        for (int64_t b = 1; b < nb; ++b) {
+            double v[MJ_BS];
+            mj_load_block(x, b * MJ_BS, v);
+            mj_block_levels<false>(v, mean, g0, g1, pv);
+            y[b * ystride] = v[0];
+        }
+#pragma unroll
+        for (int k = 0; k < MJ_LB; ++k) {
+            var[k] = g0[k] / (double)(n >> k);
+            gam[k] = g1[k] / (double)(n >> k);
+        }
+        nred = n >> MJ_LB;
+        k0 = MJ_LB;
+    }
     // One pass per level: the sums of the level and the pair means of the next one, written in
     // place (index i/2 is written after i has been read).  Level 0 reads the data itself, so the
     // copy holds at most n/2 values and every level is read once: about 4n values of traffic
@@ -669,7 +743,7 @@ __device__ inline void mjblocker(int64_t n, const Col &x, double *y, int64_t yst
         gam[k] = g1 / (double)nred;
         nred = nred / 2;
     };
-    for (int k = 0; k < npow; ++k) {
+    for (int k = k0; k < npow; ++k) {
         if (k == 0) { level([&](int64_t i) { return x[i]; }, k); }
         else { level([&](int64_t i) { return y[i * ystride]; }, k); }
     }

@@ -181,12 +181,13 @@ def test_device_estimators_bit_exact(s):
         assert np.array_equal(e, G.hexv(r["err"])), (est, e)
 
 
-@pytest.mark.parametrize("n", [2, 8, 32, 64])
+@pytest.mark.parametrize("n", [2, 8, 32, 64, 128, 256, 2048, 16384])
 def test_device_mjblocker_power_of_two_sizes_match_oracle(n):
-    """MJBlocker, and Correlated (MJBlocker at these sizes), on 2^k <= 64 values: the device's
-    register-resident form equals the oracle bit for bit, here at the sizes the golden sets leave
-    open (8, 32) besides 2 and 64, on an AR(1) series in 5 components.  (n = 1 is left out: the
-    reference's error formula then reads before its level arrays.)"""
+    """MJBlocker, and Correlated (MJBlocker at these sizes), on 2^k values: the device's
+    register-resident form (n <= 64) and its streaming form with the first four levels computed
+    on blocks of 16 in registers (n >= 128) equal the oracle bit for bit, on an AR(1) series in 5
+    components.  (n = 1 is left out: the reference's error formula then reads before its level
+    arrays.)"""
     rng = np.random.default_rng(100 + n)
     x = np.zeros((n, 5))
     e = rng.normal(size=(n, 5))
