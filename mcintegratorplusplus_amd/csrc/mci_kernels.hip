@@ -1197,10 +1197,16 @@ hipError_t launch_estimate(const EstimJob &j, hipStream_t s)
     else if ((j.estim == MCI_ESTIM_CORRELATED || j.estim == MCI_ESTIM_FCBLOCKER) && j.n <= FC_STREAM_MAX) {
         const dim3 gs = grid1(j.W * j.nobs, FC_STREAM_TPB);
         const size_t lds = fc_stream_lds(j.n);
-        const void *kf = (j.nobs == 1) ? reinterpret_cast<const void *>(k_fc_stream<true>) : reinterpret_cast<const void *>(k_fc_stream<false>);
         if (lds > 65536) {
-            const hipError_t e = hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) { return e; }
+            // the largest table once per process (the first such call; later calls, under a graph
+            // capture too, make no runtime call besides the launch)
+            static const hipError_t attr = [] {
+                const int mx = (int)fc_stream_lds(FC_STREAM_MAX);
+                hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_fc_stream<true>), hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+                if (e == hipSuccess) { e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_fc_stream<false>), hipFuncAttributeMaxDynamicSharedMemorySize, mx); }
+                return e;
+            }();
+            if (attr != hipSuccess) { return attr; }
         }
         if (j.nobs == 1) { hipLaunchKernelGGL(k_fc_stream<true>, gs, dim3(FC_STREAM_TPB), lds, s, j); }
         else { hipLaunchKernelGGL(k_fc_stream<false>, gs, dim3(FC_STREAM_TPB), lds, s, j); }
