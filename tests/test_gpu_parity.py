@@ -323,9 +323,10 @@ def test_auto_decorrelation_and_calibration_parity(rng):
             assert np.array_equal(res["generic"][k], res["mci"][k]), (coupling, k)
 
 
+@pytest.mark.parametrize("variant", ["all16", "single16", "all20"])
 @pytest.mark.parametrize("rng", [RNG_MT, RNG_PHILOX])
 @pytest.mark.parametrize("coupling", [COUPLING_INDEPENDENT, COUPLING_SHARED])
-def test_reference_defaults_on_the_specialised_kernels(coupling, rng):
+def test_reference_defaults_on_the_specialised_kernels(coupling, rng, variant):
     """The reference's defaults at d = 16 on 130 walkers (two full waves and two lanes): the
     all-coordinate move, mt19937_64 streams, automatic calibration (independent: 800-step rounds,
     the walkers still calibrating compacted for each launch), automatic decorrelation (independent:
@@ -334,11 +335,16 @@ def test_reference_defaults_on_the_specialised_kernels(coupling, rng):
     Correlated estimator on 1,024 steps (MJBlocker with its first levels on register blocks).
     Averages, errors, per-walker results, steps and positions equal the generic kernel's and the
     oracle's bit for bit.  On the native streams as well (there a compacted lane draws from its
-    walker's Philox stream through the lane-to-walker map)."""
-    sig, a = sigma_a(16)
+    walker's Philox stream through the lane-to-walker map), and on the other two specialised
+    kernels: single-coordinate moves at d = 16 and all-coordinate moves at d = 20 (held in LDS)."""
+    d = 20 if variant == "all20" else 16
+    sig, a = sigma_a(16 if d == 16 else 64)
+    a = a[:d]
     res = {}
     for name in ("orc", "mci", "generic"):
-        cfg = dict(ndim=16, seed=2024, pdfs=[(PDF_GAUSS_DIAG, a)], obs=[(OBS_X2, [], 1, 1, 1, ESTIM_CORR)])
+        cfg = dict(ndim=d, seed=2024, pdfs=[(PDF_GAUSS_DIAG, a)], obs=[(OBS_X2, [], 1, 1, 1, ESTIM_CORR)])
+        if variant == "single16":
+            cfg["move"] = (0, 1, 1, None)
         c = chain_setup("orc" if name == "orc" else "mci", cfg, nwalkers=130)
         c.setRng(rng)
         c.setCoupling(coupling)
@@ -347,8 +353,8 @@ def test_reference_defaults_on_the_specialised_kernels(coupling, rng):
         a_, e_ = c.integrate(1024)
         res[name] = (a_, e_, *c.walkerResults(), c.getWalkerSteps(), c.getX())
     for k in range(6):
-        assert np.array_equal(res["orc"][k], res["mci"][k]), (coupling, rng, k)
-        assert np.array_equal(res["generic"][k], res["mci"][k]), (coupling, rng, k)
+        assert np.array_equal(res["orc"][k], res["mci"][k]), (coupling, rng, variant, k)
+        assert np.array_equal(res["generic"][k], res["mci"][k]), (coupling, rng, variant, k)
 
 
 RTOL_BLOCKS = 1e-12  # dwell-time block sums vs step-by-step sums (same FP64 values, other rounding)
